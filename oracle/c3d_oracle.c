@@ -1,0 +1,2010 @@
+/*
+ * c3d_oracle.c -- FP64 CPU restatement of the compact3d per-timestep hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  Loaded by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg, never by the
+ * product library.
+ *
+ * Each function follows one subroutine of compact3d.f (file:line at its
+ * definition) operation by operation: same loop structure per element, same
+ * accumulation order into RUH/HH, same Thomas / bordered-LU sequences.  The
+ * loops over independent lines run under OpenMP; reductions are summed in a
+ * fixed order so that results do not depend on the thread count.
+ *
+ * Deliberate, value-preserving differences (each is exact):
+ *  - CTDMA11/CTDMA33 factorisations depend only on (b, n); they are computed
+ *    once with the reference's own sequence (compact3d.f:8959-8977) and reused.
+ *  - The 36 SGS line-set solves of RHS1-3 only run when FILES != 0.  With
+ *    FILES = 0 the reference feeds them SGSVIS == 0 (GETUP zeroes it,
+ *    compact3d.f:908-912) and they add exact zeros.
+ *  - Where the reference reads one element past a local work array and
+ *    multiplies it by an exact zero flag (e.g. SGSVIS1(I,JP) at J=N2M,
+ *    compact3d.f:6581), the element is taken as 0.
+ *  - FFTW's 2-D DFT (GETDP, compact3d.f:9315-9333) is replaced by the
+ *    textbook DFT definition evaluated with a mixed-radix FFT (c3o_dft2d).
+ */
+#include "c3d_oracle.h"
+
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef struct {
+  int n;
+  double b;
+  double *beta, *gama, *p, *q; /* 1-based, length n+1 */
+} cycfac;
+
+struct c3o {
+  int n1, n2, n3, n1m, n2m, n3m, n3mh;
+  double re, alx, aly, alz, vol, dt, cks, files, wmodel;
+  int ifwall;
+  double dx1, dx1q, dx3, dx3q;
+  double *y, *dy, *h, *hm, *hc, *hp, *dym, *dyc, *dyp, *yc; /* 0..n2 */
+  int *ipa, *ima, *kpa, *kma;
+  double *fjpa, *fjmu, *fjmv;
+  double *ak1, *ak3, *pmj, *pcj, *ppj;
+  double *ubc;        /* UBC(0:M1,0:M3,3,2) */
+  double *tauw, *taub; /* (0:M1,0:M3,2) */
+  double wsm[3];
+  double taow;
+  cycfac fx[3], fz[3]; /* b = 6, 10, 22 along x (n1m) and z (n3m) */
+};
+
+/* ---------------------------------------------------------------- layout */
+#define NI (o->n1 + 1)
+#define NJ (o->n2 + 1)
+#define NK (o->n3 + 1)
+#define NU ((size_t)NI * NJ * NK)
+#define IX(i, j, k) ((((size_t)(k)) * NJ + (size_t)(j)) * NI + (size_t)(i))
+#define PX(i, j, k) \
+  ((((size_t)(k) - 1) * o->n2 + ((size_t)(j) - 1)) * o->n1 + ((size_t)(i) - 1))
+#define UBCX(i, k, n, s) \
+  ((size_t)(i) + (size_t)NI * ((size_t)(k) + (size_t)NK * ((size_t)(n) - 1 + 3 * ((size_t)(s) - 1))))
+#define TAUX(i, k, s) ((size_t)(i) + (size_t)NI * ((size_t)(k) + (size_t)NK * ((size_t)(s) - 1)))
+
+static double *dalloc(size_t n) { return (double *)calloc(n ? n : 1, sizeof(double)); }
+
+/* ------------------------------------------------- periodic bordered LU */
+/* CTDMA11 / CTDMA33 (compact3d.f:8952-9046) with A=C=1, B=b: factorisation */
+static void cyc_factor(cycfac *f, int n, double b) {
+  const double A = 1.0, B = b, C = 1.0;
+  f->n = n;
+  f->b = b;
+  f->beta = dalloc(n + 1);
+  f->gama = dalloc(n + 1);
+  f->p = dalloc(n + 1);
+  f->q = dalloc(n + 1);
+  double *BETA = f->beta, *GAMA = f->gama, *P = f->p, *Q = f->q;
+  BETA[1] = B;
+  GAMA[1] = C / BETA[1];
+  for (int i = 2; i <= n - 1; i++) {
+    BETA[i] = B - A * GAMA[i - 1];
+    GAMA[i] = C / BETA[i];
+  }
+  P[1] = C;
+  Q[1] = A / BETA[1];
+  for (int i = 2; i <= n - 2; i++) {
+    P[i] = -P[i - 1] * GAMA[i - 1];
+    Q[i] = -(A * Q[i - 1] / BETA[i]);
+  }
+  P[n - 1] = A - P[n - 2] * GAMA[n - 2];
+  Q[n - 1] = (C - A * Q[n - 2]) / BETA[n - 1];
+  P[n] = B;
+  for (int i = 1; i <= n - 1; i++) P[n] = P[n] - P[i] * Q[i];
+}
+
+static void cyc_free(cycfac *f) {
+  free(f->beta);
+  free(f->gama);
+  free(f->p);
+  free(f->q);
+}
+
+/* CTDMA11 forward/backward substitution (compact3d.f:8979-8995); d,x 1-based */
+static void cyc_solve(const cycfac *f, const double *d, double *x) {
+  const int n = f->n;
+  const double A = 1.0;
+  const double *BETA = f->beta, *GAMA = f->gama, *P = f->p, *Q = f->q;
+  double z[n + 1];
+  z[1] = d[1] / BETA[1];
+  for (int i = 2; i <= n - 1; i++) z[i] = (d[i] - A * z[i - 1]) / BETA[i];
+  z[n] = d[n];
+  for (int i = 1; i <= n - 1; i++) z[n] = z[n] - P[i] * z[i];
+  z[n] = z[n] / P[n];
+  x[n] = z[n];
+  x[n - 1] = z[n - 1] - Q[n - 1] * x[n];
+  for (int i = n - 2; i >= 1; i--) x[i] = z[i] - GAMA[i] * x[i + 1] - Q[i] * x[n];
+}
+
+void c3o_ctdma(int n, double b, const double *d0, double *x0) {
+  cycfac f;
+  cyc_factor(&f, n, b);
+  double d[n + 1], x[n + 1];
+  for (int i = 1; i <= n; i++) d[i] = d0[i - 1];
+  cyc_solve(&f, d, x);
+  for (int i = 1; i <= n; i++) x0[i - 1] = x[i];
+  cyc_free(&f);
+}
+
+/* CTDMA1J / CTDMA3I single-line sequence (compact3d.f:8721-8750, 8766-8807);
+ * a,b,c,d,x 1-based; x may alias d */
+static void cyc_solve_general(int n, const double *a, const double *b,
+                              const double *c, const double *d, double *x) {
+  double gam[n + 1], p[n + 1], q[n + 1];
+  double bet = 1.0 / b[1];
+  gam[1] = c[1] * bet;
+  x[1] = d[1] * bet;
+  p[1] = c[n];
+  q[1] = a[1] * bet;
+  for (int k = 2; k <= n - 1; k++) {
+    bet = 1.0 / (b[k] - a[k] * gam[k - 1]);
+    gam[k] = c[k] * bet;
+    p[k] = -p[k - 1] * gam[k - 1];
+    q[k] = -(a[k] * bet * q[k - 1]);
+    x[k] = (d[k] - a[k] * x[k - 1]) * bet;
+  }
+  bet = 1.0 / (b[n - 1] - a[n - 1] * gam[n - 2]);
+  p[n - 1] = a[n] - p[n - 2] * gam[n - 2];
+  q[n - 1] = (c[n - 1] - a[n - 1] * q[n - 2]) * bet;
+  x[n] = d[n];
+  p[n] = b[n];
+  for (int k = 1; k <= n - 1; k++) {
+    x[n] = x[n] - p[k] * x[k];
+    p[n] = p[n] - p[k] * q[k];
+  }
+  x[n] = x[n] / p[n];
+  gam[n - 1] = 0.0;
+  for (int k = n - 1; k >= 1; k--) x[k] = x[k] - gam[k] * x[k + 1] - q[k] * x[n];
+}
+
+void c3o_ctdma_ref_general(int n, const double *a0, const double *b0,
+                           const double *c0, const double *d0, double *x0) {
+  double a[n + 1], b[n + 1], c[n + 1], x[n + 1];
+  for (int i = 1; i <= n; i++) {
+    a[i] = a0[i - 1];
+    b[i] = b0[i - 1];
+    c[i] = c0[i - 1];
+    x[i] = d0[i - 1];
+  }
+  cyc_solve_general(n, a, b, c, x, x);
+  for (int i = 1; i <= n; i++) x0[i - 1] = x[i];
+}
+
+/* index of the cached factor for b in {6,10,22} */
+static const cycfac *fxb(const c3o *o, double b) {
+  return b == 6.0 ? &o->fx[0] : (b == 10.0 ? &o->fx[1] : &o->fx[2]);
+}
+static const cycfac *fzb(const c3o *o, double b) {
+  return b == 6.0 ? &o->fz[0] : (b == 10.0 ? &o->fz[1] : &o->fz[2]);
+}
+
+/* ------------------------------------------------------------------ setup */
+void c3o_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+/* MESH (compact3d.f:302-443), INDICES (:446-494), INIWAVE (:9182-9228),
+ * METRICPOISSON (:9239-9263).  cks replaces the hard-coded CKS=2.5 (:338). */
+c3o *c3o_new(int n1, int n2, int n3, double re, double alx, double aly,
+             double alz, double dt, double files, double wmodel, double cks) {
+  c3o *o = (c3o *)calloc(1, sizeof(c3o));
+  o->n1 = n1;
+  o->n2 = n2;
+  o->n3 = n3;
+  o->n1m = n1 - 1;
+  o->n2m = n2 - 1;
+  o->n3m = n3 - 1;
+  o->n3mh = o->n3m / 2 + 1;
+  o->re = re;
+  o->alx = alx;
+  o->aly = aly;
+  o->alz = alz;
+  o->dt = dt;
+  o->files = files;
+  o->wmodel = wmodel;
+  o->cks = cks;
+  o->ifwall = 0;
+  const int N1M = o->n1m, N2M = o->n2m, N3M = o->n3m, N2 = n2;
+  o->y = dalloc(n2 + 2);
+  o->dy = dalloc(n2 + 2);
+  o->h = dalloc(n2 + 2);
+  o->hm = dalloc(n2 + 2);
+  o->hc = dalloc(n2 + 2);
+  o->hp = dalloc(n2 + 2);
+  o->dym = dalloc(n2 + 2);
+  o->dyc = dalloc(n2 + 2);
+  o->dyp = dalloc(n2 + 2);
+  o->yc = dalloc(n2 + 2);
+  o->fjpa = dalloc(n2 + 2);
+  o->fjmu = dalloc(n2 + 2);
+  o->fjmv = dalloc(n2 + 2);
+  o->pmj = dalloc(n2 + 2);
+  o->pcj = dalloc(n2 + 2);
+  o->ppj = dalloc(n2 + 2);
+  o->ipa = (int *)calloc(n1 + 2, sizeof(int));
+  o->ima = (int *)calloc(n1 + 2, sizeof(int));
+  o->kpa = (int *)calloc(n3 + 2, sizeof(int));
+  o->kma = (int *)calloc(n3 + 2, sizeof(int));
+  o->ak1 = dalloc(n1 + 2);
+  o->ak3 = dalloc(n3 + 2);
+  o->ubc = dalloc((size_t)(n1 + 1) * (n3 + 1) * 6);
+  o->tauw = dalloc((size_t)(n1 + 1) * (n3 + 1) * 2);
+  o->taub = dalloc((size_t)(n1 + 1) * (n3 + 1) * 2);
+
+  /* MESH: y grid (:338-361) */
+  double *Y = o->y, *DY = o->dy, *H = o->h;
+  const double CKS = cks, Y0 = 0.0;
+  Y[0] = (wmodel == 0.0) ? 0.0 : Y0;
+  for (int j = 1; j <= N2; j++) {
+    if (wmodel == 0.0) {
+      if (aly == 2.0)
+        Y[j] = 1 - tanh(CKS * (1 - aly * (j - 1) / (N2 - 1))) / tanh(CKS);
+      else
+        Y[j] = 1 - tanh(CKS * (1 - 2. * aly * (j - 1) / (2 * N2 - 1))) / tanh(CKS);
+    } else {
+      Y[j] = (aly - Y0) * (j - 1) / (N2 - 1) + Y0;
+    }
+  }
+  o->aly = Y[N2];
+  for (int j = 1; j <= N2M; j++) o->yc[j] = 0.5 * (Y[j + 1] + Y[j]);
+  o->yc[N2] = Y[N2];
+  o->vol = o->alx * o->aly * o->alz;
+  o->dx1 = (double)N1M / alx;
+  o->dx3 = (double)N3M / alz;
+  o->dx1q = o->dx1 * o->dx1;
+  o->dx3q = o->dx3 * o->dx3;
+  DY[1] = Y[2];
+  for (int j = 2; j <= N2M; j++) {
+    DY[j] = Y[j + 1] - Y[j];
+    H[j] = 0.5 * (DY[j] + DY[j - 1]);
+  }
+  H[1] = 0.5 * DY[1];
+  H[N2] = 0.5 * DY[N2M];
+  double *HP = o->hp, *HC = o->hc, *HM = o->hm;
+  for (int j = 2; j <= N2M - 1; j++) {
+    HP[j] = 1.0 / H[j + 1] / DY[j];
+    HC[j] = (H[j + 1] + H[j]) / H[j + 1] / H[j] / DY[j];
+    HM[j] = 1.0 / H[j] / DY[j];
+  }
+  HP[1] = 4.0 * H[1] / H[2] / (H[1] + H[2]) / DY[1];
+  HC[1] = 4.0 / H[2] / DY[1];
+  HM[1] = 4.0 / (H[1] + H[2]) / DY[1];
+  HP[N2M] = 4.0 / (H[N2M] + H[N2]) / DY[N2M];
+  HC[N2M] = 4.0 / H[N2M] / DY[N2M];
+  HM[N2M] = 4.0 * H[N2] / H[N2M] / (H[N2M] + H[N2]) / DY[N2M];
+  if (wmodel != 0.0) {
+    HP[1] = 1.0 / H[2] / DY[1];
+    HC[1] = 1.0 / H[2] / DY[1];
+    HM[1] = 0.0;
+    HP[N2M] = 0.0;
+    HC[N2M] = 1.0 / H[N2M] / DY[N2M];
+    HM[N2M] = 1.0 / H[N2M] / DY[N2M];
+  }
+  for (int j = 2; j <= N2M; j++) {
+    o->dyp[j] = 1.0 / H[j] / DY[j];
+    o->dyc[j] = 1.0 / H[j] * (1.0 / DY[j] + 1.0 / DY[j - 1]);
+    o->dym[j] = 1.0 / H[j] / DY[j - 1];
+  }
+
+  /* INDICES (:454-481) */
+  for (int i = 1; i <= N1M; i++) {
+    o->ipa[i] = i + 1;
+    o->ima[i] = i - 1;
+  }
+  o->ipa[N1M] = 1;
+  o->ima[1] = N1M;
+  for (int k = 1; k <= N3M; k++) {
+    o->kpa[k] = k + 1;
+    o->kma[k] = k - 1;
+  }
+  o->kpa[N3M] = 1;
+  o->kma[1] = N3M;
+  {
+    int jpa[n2 + 2], jmu[n2 + 2], jmv[n2 + 2];
+    for (int j = 1; j <= N2M; j++) {
+      jpa[j] = j + 1;
+      jmu[j] = j - 1;
+      jmv[j] = j - 1;
+    }
+    jpa[N2M] = N2M;
+    jmu[1] = 1;
+    jmv[2] = 2;
+    for (int j = 1; j <= N2M; j++) {
+      o->fjpa[j] = jpa[j] - j;
+      o->fjmu[j] = j - jmu[j];
+      o->fjmv[j] = j - jmv[j];
+    }
+  }
+
+  /* INIWAVE (:9197-9222) */
+  {
+    const double PI = acos(-1.0);
+    for (int i = 1; i <= N1M; i++) {
+      int ii = i;
+      if (i > N1M / 2 + 1) ii = i - N1M;
+      double ani = (ii - 1) * 2.0 * PI;
+      double c = cos(ani / N1M);
+      double den = 11. + c;
+      o->ak1[i] = 288. * o->dx1q * (1. - c) / (den * den);
+    }
+    for (int k = 1; k <= o->n3mh; k++) {
+      double ank = (k - 1) * 2. * PI;
+      double c = cos(ank / N3M);
+      double den = 11. + c;
+      o->ak3[k] = 288. * o->dx3q * (1. - c) / (den * den);
+    }
+  }
+  /* METRICPOISSON (:9249-9260) */
+  for (int jc = 1; jc <= N2M; jc++) {
+    int jp = jc + 1;
+    double fjum = o->fjmu[jc], fjup = o->fjpa[jc];
+    o->pmj[jc] = fjum * fjup * (1.0 / DY[jc] / H[jc]) +
+                 (1. - fjup) * (1.0 / DY[N2M] / H[N2M]);
+    o->pcj[jc] = fjum * fjup * (-1.0 / DY[jc] / H[jc] - 1.0 / DY[jc] / H[jp]) +
+                 (1. - fjup) * (-1.0 / DY[N2M] / H[N2M]) +
+                 (1. - fjum) * (-1.0 / DY[1] / H[2]);
+    o->ppj[jc] = fjum * fjup * (1.0 / DY[jc] / H[jp]) +
+                 (1. - fjum) * (1.0 / DY[1] / H[2]);
+  }
+  const double bs[3] = {6.0, 10.0, 22.0};
+  for (int t = 0; t < 3; t++) {
+    cyc_factor(&o->fx[t], N1M, bs[t]);
+    cyc_factor(&o->fz[t], N3M, bs[t]);
+  }
+  return o;
+}
+
+void c3o_free(c3o *o) {
+  if (!o) return;
+  double *ps[] = {o->y,   o->dy,  o->h,   o->hm,  o->hc,   o->hp,  o->dym,
+                  o->dyc, o->dyp, o->yc,  o->fjpa, o->fjmu, o->fjmv, o->pmj,
+                  o->pcj, o->ppj, o->ak1, o->ak3, o->ubc,  o->tauw, o->taub};
+  for (size_t t = 0; t < sizeof(ps) / sizeof(ps[0]); t++) free(ps[t]);
+  free(o->ipa);
+  free(o->ima);
+  free(o->kpa);
+  free(o->kma);
+  for (int t = 0; t < 3; t++) {
+    cyc_free(&o->fx[t]);
+    cyc_free(&o->fz[t]);
+  }
+  free(o);
+}
+
+void c3o_set_dt(c3o *o, double dt) { o->dt = dt; }
+void c3o_set_ifwall(c3o *o, int ifwall) { o->ifwall = ifwall; }
+
+int c3o_get_vec(const c3o *o, int which, double *out) {
+  const double *src = NULL;
+  int n = o->n2 + 1;
+  switch (which) {
+    case 0: src = o->y; break;
+    case 1: src = o->dy; break;
+    case 2: src = o->h; break;
+    case 3: src = o->hm; break;
+    case 4: src = o->hc; break;
+    case 5: src = o->hp; break;
+    case 6: src = o->dym; break;
+    case 7: src = o->dyc; break;
+    case 8: src = o->dyp; break;
+    case 9: src = o->ak1; n = o->n1 + 1; break;
+    case 10: src = o->ak3; n = o->n3mh + 1; break;
+    case 11: src = o->pmj; break;
+    case 12: src = o->pcj; break;
+    case 13: src = o->ppj; break;
+    case 14: src = o->fjpa; break;
+    case 15: src = o->fjmu; break;
+    case 16: src = o->fjmv; break;
+    default: return -1;
+  }
+  memcpy(out, src, sizeof(double) * n);
+  return n;
+}
+
+double c3o_get_scalar(const c3o *o, int which) {
+  switch (which) {
+    case 100: return o->dx1;
+    case 101: return o->dx1q;
+    case 102: return o->dx3;
+    case 103: return o->dx3q;
+    case 104: return o->aly;
+    case 105: return o->vol;
+    case 106: return o->taow;
+    default: return NAN;
+  }
+}
+
+void c3o_get_tau(const c3o *o, double *tauw, double *taub) {
+  size_t n = (size_t)(o->n1 + 1) * (o->n3 + 1) * 2;
+  if (tauw) memcpy(tauw, o->tauw, n * sizeof(double));
+  if (taub) memcpy(taub, o->taub, n * sizeof(double));
+}
+void c3o_set_tau(c3o *o, const double *tauw, const double *taub) {
+  size_t n = (size_t)(o->n1 + 1) * (o->n3 + 1) * 2;
+  if (tauw) memcpy(o->tauw, tauw, n * sizeof(double));
+  if (taub) memcpy(o->taub, taub, n * sizeof(double));
+}
+void c3o_get_wsm(const c3o *o, double *w) { memcpy(w, o->wsm, 3 * sizeof(double)); }
+void c3o_set_wsm(c3o *o, const double *w) { memcpy(o->wsm, w, 3 * sizeof(double)); }
+
+/* ------------------------------------------------------------------ BCOND */
+/* compact3d.f:963-1001 */
+void c3o_bcond(c3o *o, double *U, double *SGSVIS) {
+  const int N1M = o->n1m, N3M = o->n3m, N2 = o->n2, N2M = o->n2m;
+  const double ALY = o->aly;
+  for (int k = 1; k <= N3M; k++)
+    for (int i = 1; i <= N1M; i++) {
+      o->ubc[UBCX(i, k, 1, 1)] = 0.0;
+      o->ubc[UBCX(i, k, 2, 1)] = 0.0;
+      o->ubc[UBCX(i, k, 3, 1)] = 0.0;
+      U[IX(i, 0, k)] = o->ubc[UBCX(i, k, 1, 1)];
+      U[NU + IX(i, 0, k)] = o->ubc[UBCX(i, k, 2, 1)];
+      U[2 * NU + IX(i, 0, k)] = o->ubc[UBCX(i, k, 3, 1)];
+      o->ubc[UBCX(i, k, 1, 2)] = U[IX(i, N2M, k)] * (2.0 - ALY);
+      o->ubc[UBCX(i, k, 2, 2)] = 0.0;
+      o->ubc[UBCX(i, k, 3, 2)] = U[2 * NU + IX(i, N2M, k)] * (2.0 - ALY);
+      U[IX(i, N2, k)] = o->ubc[UBCX(i, k, 1, 2)];
+      U[NU + IX(i, N2, k)] = o->ubc[UBCX(i, k, 2, 2)];
+      U[2 * NU + IX(i, N2, k)] = o->ubc[UBCX(i, k, 3, 2)];
+      if (o->files != 0.0 && SGSVIS) {
+        SGSVIS[IX(i, 0, k)] = 0.0;
+        SGSVIS[IX(i, N2, k)] = SGSVIS[IX(i, N2M, k)] * (2.0 - ALY);
+      }
+    }
+}
+
+/* helpers for periodic line RHS builders; all arrays 1-based */
+#define WRAPM(i, n) ((i) == 1 ? (n) : (i) - 1)
+#define WRAPP(i, n) ((i) == (n) ? 1 : (i) + 1)
+
+/* ------------------------------------------------------------------- RHS1 */
+/* compact3d.f:6063-6782 */
+void c3o_rhs1(c3o *o, const double *U, const double *P, double *HH,
+              const double *HH2, double *RUH1, double presg,
+              const double *SGSVIS) {
+  const int N1M = o->n1m, N2M = o->n2m, N3M = o->n3m, N2 = o->n2;
+  const double RE = o->re, DX1 = o->dx1, DX1Q = o->dx1q, DX3 = o->dx3,
+               DX3Q = o->dx3q, ALY = o->aly;
+  const double *DY = o->dy, *H = o->h;
+  const int IFWALL = o->ifwall;
+  const double *V = U + NU, *W = U + 2 * NU;
+  double *H1 = HH;
+  const double *H21 = HH2;
+
+  /* loop 10: viscous y (:6104-6129) */
+#pragma omp parallel for
+  for (int k = 1; k <= N3M; k++)
+    for (int j = 1; j <= N2M; j++) {
+      int jp = j + 1, jm = j - 1;
+      for (int i = 1; i <= N1M; i++) {
+        double viscos = (1. / RE) * (o->hp[j] * U[IX(i, jp, k)] -
+                                     o->hc[j] * U[IX(i, j, k)] +
+                                     o->hm[j] * U[IX(i, jm, k)]);
+        RUH1[IX(i, j, k)] = viscos;
+      }
+    }
+  /* loop 20: D2/Dx2 (:6142-6157) */
+#pragma omp parallel for
+  for (int k = 1; k <= N3M; k++)
+    for (int j = 1; j <= N2M; j++) {
+      double d[N1M + 1], x[N1M + 1];
+      for (int i = 1; i <= N1M; i++) {
+        int im = WRAPM(i, N1M), ip = WRAPP(i, N1M);
+        d[i] = 12. * DX1Q * (U[IX(im, j, k)] - 2. * U[IX(i, j, k)] + U[IX(ip, j, k)]);
+      }
+      cyc_solve(fxb(o, 10.0), d, x);
+      for (int i = 1; i <= N1M; i++) RUH1[IX(i, j, k)] = RUH1[IX(i, j, k)] + (1. / RE) * x[i];
+    }
+  /* loop 30: D2/Dz2 (:6170-6185) */
+#pragma omp parallel for
+  for (int i = 1; i <= N1M; i++)
+    for (int j = 1; j <= N2M; j++) {
+      double d[N3M + 1], x[N3M + 1];
+      for (int k = 1; k <= N3M; k++) {
+        int km = WRAPM(k, N3M), kp = WRAPP(k, N3M);
+        d[k] = 12. * DX3Q * (U[IX(i, j, km)] - 2. * U[IX(i, j, k)] + U[IX(i, j, kp)]);
+      }
+      cyc_solve(fzb(o, 10.0), d, x);
+      for (int k = 1; k <= N3M; k++) RUH1[IX(i, j, k)] = RUH1[IX(i, j, k)] + (1. / RE) * x[k];
+    }
+  /* loop 40: pressure gradient (:6196-6210) */
+#pragma omp parallel for
+  for (int k = 1; k <= N3M; k++)
+    for (int j = 1; j <= N2M; j++) {
+      double d[N1M + 1], x[N1M + 1];
+      for (int i = 1; i <= N1M; i++)
+        d[i] = 24. * DX1 * (-P[PX(WRAPM(i, N1M), j, k)] + P[PX(i, j, k)]);
+      cyc_solve(fxb(o, 22.0), d, x);
+      for (int i = 1; i <= N1M; i++)
+        RUH1[IX(i, j, k)] = RUH1[IX(i, j, k)] - x[i] - presg;
+    }
+  /* D(UV)/DY (:6224-6276) */
+#pragma omp parallel for
+  for (int k = 1; k <= N3M; k++) {
+    double *VV1 = dalloc((size_t)(N1M + 1) * (N2M + 2)); /* VV1(I,J), J up to N2M+1 */
+    for (int j = 1; j <= N2M; j++) {
+      double d[N1M + 1], x[N1M + 1];
+      for (int i = 1; i <= N1M; i++)
+        d[i] = 4.0 * (V[IX(WRAPM(i, N1M), j, k)] + V[IX(i, j, k)]);
+      cyc_solve(fxb(o, 6.0), d, x);
+      for (int i = 1; i <= N1M; i++) VV1[(size_t)j * (N1M + 1) + i] = x[i];
+    }
+    for (int j = 1; j <= N2M; j++) {
+      int jp = j + 1, jm = j - 1;
+      double fjum = o->fjmu[j], fjup = o->fjpa[j];
+      for (int i = 1; i <= N1M; i++) {
+        double u2 = fjup * 1.0 / H[jp] * (DY[j] / 2. * U[IX(i, jp, k)] + DY[jp] / 2. * U[IX(i, j, k)]) +
+                    (1. - fjup) * U[IX(i, N2, k)];
+        double u1 = fjum * 1.0 / H[j] * (DY[jm] / 2. * U[IX(i, j, k)] + DY[j] / 2. * U[IX(i, jm, k)]) +
+                    (1. - fjum) * U[IX(i, 0, k)];
+        double v2 = VV1[(size_t)jp * (N1M + 1) + i];
+        double v1 = VV1[(size_t)j * (N1M + 1) + i];
+        if (j == N2M) v2 = 0.0;
+        if (j == 1) v1 = 0.0;
+        H1[IX(i, j, k)] = H1[IX(i, j, k)] + 1. / DY[j] * (v2 * u2 - v1 * u1);
+      }
+    }
+    free(VV1);
+  }
+  /* D(UU)/DX (:6295-6334) */
+#pragma omp parallel for
+  for (int k = 1; k <= N3M; k++)
+    for (int j = 1; j <= N2M; j++) {
+      double d[N1M + 1], uu1[N1M + 1], x[N1M + 1];
+      for (int i = 1; i <= N1M; i++)
+        d[i] = 4.0 * (U[IX(i, j, k)] + U[IX(WRAPP(i, N1M), j, k)]);
+      cyc_solve(fxb(o, 6.0), d, uu1);
+      for (int i = 1; i <= N1M; i++) {
+        int im = WRAPM(i, N1M);
+        d[i] = 24. * DX1 * (uu1[i] * uu1[i] - uu1[im] * uu1[im]);
+      }
+      cyc_solve(fxb(o, 22.0), d, x);
+      for (int i = 1; i <= N1M; i++) H1[IX(i, j, k)] = H1[IX(i, j, k)] + x[i];
+    }
+  /* D(UW)/DZ (:6360-6433) */
+#pragma omp parallel for
+  for (int j = 1; j <= N2M; j++) {
+    double *UU3 = dalloc((size_t)(N1M + 1) * (N3M + 1));
+    double *WW1 = dalloc((size_t)(N1M + 1) * (N3M + 1));
+#define A2(a, i, k) a[(size_t)(k) * (N1M + 1) + (i)]
+    for (int i = 1; i <= N1M; i++) {
+      double d[N3M + 1], x[N3M + 1];
+      for (int k = 1; k <= N3M; k++) d[k] = 4. * (U[IX(i, j, WRAPM(k, N3M))] + U[IX(i, j, k)]);
+      cyc_solve(fzb(o, 6.0), d, x);
+      for (int k = 1; k <= N3M; k++) A2(UU3, i, k) = x[k];
+    }
+    for (int k = 1; k <= N3M; k++) {
+      double d[N1M + 1], x[N1M + 1];
+      for (int i = 1; i <= N1M; i++) d[i] = 4. * (W[IX(WRAPM(i, N1M), j, k)] + W[IX(i, j, k)]);
+      cyc_solve(fxb(o, 6.0), d, x);
+      for (int i = 1; i <= N1M; i++) A2(WW1, i, k) = x[i];
+    }
+    for (int i = 1; i <= N1M; i++) {
+      double d[N3M + 1], x[N3M + 1];
+      for (int k = 1; k <= N3M; k++) {
+        int kp = WRAPP(k, N3M);
+        d[k] = 24. * DX3 * (-A2(UU3, i, k) * A2(WW1, i, k) + A2(UU3, i, kp) * A2(WW1, i, kp));
+      }
+      cyc_solve(fzb(o, 22.0), d, x);
+      for (int k = 1; k <= N3M; k++) H1[IX(i, j, k)] = H1[IX(i, j, k)] + x[k];
+    }
+    free(UU3);
+    free(WW1);
+  }
+
+  if (o->files != 0.0) {
+    /* D(SGS*(DU/DX))/DX (:6445-6466) */
+#pragma omp parallel for
+    for (int k = 1; k <= N3M; k++)
+      for (int j = 1; j <= N2M; j++) {
+        double d[N1M + 1], x[N1M + 1];
+        for (int i = 1; i <= N1M; i++)
+          d[i] = 24. * DX1 * (U[IX(WRAPP(i, N1M), j, k)] - U[IX(i, j, k)]);
+        cyc_solve(fxb(o, 22.0), d, x);
+        for (int i = 1; i <= N1M; i++) {
+          int im = WRAPM(i, N1M);
+          d[i] = 24. * DX1 * (SGSVIS[IX(i, j, k)] * x[i] - SGSVIS[IX(im, j, k)] * x[im]);
+        }
+        double x2[N1M + 1];
+        cyc_solve(fxb(o, 22.0), d, x2);
+        for (int i = 1; i <= N1M; i++) H1[IX(i, j, k)] = H1[IX(i, j, k)] - 2. * x2[i];
+      }
+    /* D(SGS*(DU/DZ))/DZ (:6491-6540) */
+#pragma omp parallel for
+    for (int j = 1; j <= N2M; j++) {
+      double *S2 = dalloc((size_t)(N1M + 1) * (N3M + 1));
+      for (int k = 1; k <= N3M; k++) {
+        double d[N1M + 1], x[N1M + 1];
+        for (int i = 1; i <= N1M; i++)
+          d[i] = 4. * (SGSVIS[IX(i, j, k)] + SGSVIS[IX(WRAPM(i, N1M), j, k)]);
+        cyc_solve(fxb(o, 6.0), d, x);
+        for (int i = 1; i <= N1M; i++) A2(S2, i, k) = x[i];
+      }
+      for (int i = 1; i <= N1M; i++) {
+        double d[N3M + 1], x3[N3M + 1], x33[N3M + 1], x4[N3M + 1];
+        for (int k = 1; k <= N3M; k++) d[k] = 4. * (A2(S2, i, k) + A2(S2, i, WRAPM(k, N3M)));
+        cyc_solve(fzb(o, 6.0), d, x3);
+        for (int k = 1; k <= N3M; k++)
+          d[k] = 24. * DX3 * (U[IX(i, j, k)] - U[IX(i, j, WRAPM(k, N3M))]);
+        cyc_solve(fzb(o, 22.0), d, x33);
+        for (int k = 1; k <= N3M; k++) {
+          int kp = WRAPP(k, N3M);
+          d[k] = 24. * DX3 * (x33[kp] * x3[kp] - x33[k] * x3[k]);
+        }
+        cyc_solve(fzb(o, 22.0), d, x4);
+        for (int k = 1; k <= N3M; k++) H1[IX(i, j, k)] = H1[IX(i, j, k)] - x4[k];
+      }
+      free(S2);
+    }
+    /* D(SGS*(DU/DY))/DY (:6551-6604) */
+#pragma omp parallel for
+    for (int k = 1; k <= N3M; k++) {
+      double *S1 = dalloc((size_t)(N1M + 1) * (N2M + 2)); /* rows 0..N2M+1, 0 and N2M+1 stay 0 */
+#define B2(a, i, j) a[(size_t)(j) * (N1M + 1) + (i)]
+      for (int j = 1; j <= N2M; j++) {
+        double d[N1M + 1], x[N1M + 1];
+        for (int i = 1; i <= N1M; i++)
+          d[i] = 4. * (SGSVIS[IX(i, j, k)] + SGSVIS[IX(WRAPM(i, N1M), j, k)]);
+        cyc_solve(fxb(o, 6.0), d, x);
+        for (int i = 1; i <= N1M; i++) B2(S1, i, j) = x[i];
+      }
+      for (int i = 1; i <= N1M; i++) {
+        int im = o->ima[i];
+        for (int j = 1; j <= N2M; j++) {
+          int jp = j + 1, jm = j - 1;
+          double fjum = o->fjmu[j], fjup = o->fjpa[j];
+          double sgsj2 = fjup * 1. / H[jp] * (DY[j] / 2. * B2(S1, i, jp) + DY[jp] / 2. * B2(S1, i, j)) +
+                         (1. - fjup) * 0.5 * (SGSVIS[IX(i, N2, k)] + SGSVIS[IX(im, N2, k)]);
+          double sgsj1 = fjum * 1. / H[j] * (DY[jm] / 2. * B2(S1, i, j) + DY[j] / 2. * B2(S1, i, jm)) +
+                         (1. - fjum) * 0.5 * (SGSVIS[IX(i, 0, k)] + SGSVIS[IX(im, 0, k)]);
+          double uu22 = (U[IX(i, jp, k)] - U[IX(i, j, k)]) / H[jp];
+          double uu11 = (U[IX(i, j, k)] - U[IX(i, jm, k)]) / H[j];
+          double sgsduy = 1. / DY[j] * ((1. - IFWALL * (1. - fjup) * (ALY - 1.)) * sgsj2 * uu22 -
+                                        (1. - IFWALL * (1. - fjum)) * sgsj1 * uu11);
+          H1[IX(i, j, k)] = H1[IX(i, j, k)] - sgsduy;
+        }
+      }
+      free(S1);
+    }
+    /* D(SGS*(DV/DX))/DY (:6622-6680) */
+#pragma omp parallel for
+    for (int k = 1; k <= N3M; k++) {
+      double *UU12 = dalloc((size_t)(N1M + 1) * (N2M + 2));
+      double *S12 = dalloc((size_t)(N1M + 1) * (N2M + 2));
+      for (int j = 1; j <= N2M; j++) {
+        double d[N1M + 1], x[N1M + 1];
+        for (int i = 1; i <= N1M; i++)
+          d[i] = 24. * DX1 * (V[IX(i, j, k)] - V[IX(WRAPM(i, N1M), j, k)]);
+        cyc_solve(fxb(o, 22.0), d, x);
+        for (int i = 1; i <= N1M; i++) B2(UU12, i, j) = x[i];
+        for (int i = 1; i <= N1M; i++)
+          d[i] = 4. * (SGSVIS[IX(i, j, k)] + SGSVIS[IX(WRAPM(i, N1M), j, k)]);
+        cyc_solve(fxb(o, 6.0), d, x);
+        for (int i = 1; i <= N1M; i++) B2(S12, i, j) = x[i];
+      }
+      for (int i = 1; i <= N1M; i++) {
+        int im = o->ima[i];
+        for (int j = 1; j <= N2M; j++) {
+          int jp = j + 1, jm = j - 1;
+          double fjum = o->fjmu[j], fjup = o->fjpa[j];
+          double sgsj2 = fjup * 1.0 / H[jp] * (DY[j] / 2. * B2(S12, i, jp) + DY[jp] / 2. * B2(S12, i, j)) +
+                         (1. - fjup) * 0.5 * (SGSVIS[IX(i, N2, k)] + SGSVIS[IX(im, N2, k)]);
+          double sgsj1 = fjum * 1.0 / H[j] * (DY[jm] / 2. * B2(S12, i, j) + DY[j] / 2. * B2(S12, i, jm)) +
+                         (1. - fjum) * 0.5 * (SGSVIS[IX(i, 0, k)] + SGSVIS[IX(im, 0, k)]);
+          double sgsduxy = 1.0 / DY[j] * ((1. - IFWALL * (1. - fjup) * (ALY - 1.)) * sgsj2 * B2(UU12, i, jp) -
+                                          (1. - IFWALL * (1. - fjum)) * sgsj1 * B2(UU12, i, j));
+          H1[IX(i, j, k)] = H1[IX(i, j, k)] - sgsduxy;
+        }
+      }
+      free(UU12);
+      free(S12);
+    }
+    /* D(SGS*(DW/DX))/DZ (:6713-6764) */
+#pragma omp parallel for
+    for (int j = 1; j <= N2M; j++) {
+      double *S13 = dalloc((size_t)(N1M + 1) * (N3M + 1));
+      double *WW13 = dalloc((size_t)(N1M + 1) * (N3M + 1));
+      for (int k = 1; k <= N3M; k++) {
+        double d[N1M + 1], x[N1M + 1];
+        for (int i = 1; i <= N1M; i++)
+          d[i] = 4. * (SGSVIS[IX(i, j, k)] + SGSVIS[IX(WRAPM(i, N1M), j, k)]);
+        cyc_solve(fxb(o, 6.0), d, x);
+        for (int i = 1; i <= N1M; i++) A2(S13, i, k) = x[i];
+        for (int i = 1; i <= N1M; i++)
+          d[i] = 24. * DX1 * (W[IX(i, j, k)] - W[IX(WRAPM(i, N1M), j, k)]);
+        cyc_solve(fxb(o, 22.0), d, x);
+        for (int i = 1; i <= N1M; i++) A2(WW13, i, k) = x[i];
+      }
+      for (int i = 1; i <= N1M; i++) {
+        double d[N3M + 1], x3[N3M + 1], x33[N3M + 1];
+        for (int k = 1; k <= N3M; k++) d[k] = 4. * (A2(S13, i, k) + A2(S13, i, WRAPM(k, N3M)));
+        cyc_solve(fzb(o, 6.0), d, x3);
+        for (int k = 1; k <= N3M; k++) {
+          int kp = WRAPP(k, N3M);
+          d[k] = 24. * DX3 * (A2(WW13, i, kp) * x3[kp] - A2(WW13, i, k) * x3[k]);
+        }
+        cyc_solve(fzb(o, 22.0), d, x33);
+        for (int k = 1; k <= N3M; k++) H1[IX(i, j, k)] = H1[IX(i, j, k)] - x33[k];
+      }
+      free(S13);
+      free(WW13);
+    }
+  }
+  /* AB2 + wall-stress BC (:6765-6775) */
+#pragma omp parallel for
+  for (int k = 1; k <= N3M; k++)
+    for (int j = 1; j <= N2M; j++) {
+      double fjum = o->fjmu[j], fjup = o->fjpa[j];
+      for (int i = 1; i <= N1M; i++) {
+        double bc_down = -o->tauw[TAUX(i, k, 1)] / DY[1] * IFWALL;
+        double bc_up = -o->tauw[TAUX(i, k, 2)] / DY[N2M] * IFWALL * (ALY - 1.);
+        double bc = (1. - fjum) * bc_down + (1. - fjup) * bc_up;
+        RUH1[IX(i, j, k)] = RUH1[IX(i, j, k)] - 3. / 2. * H1[IX(i, j, k)] +
+                            1. / 2. * H21[IX(i, j, k)] + bc;
+      }
+    }
+}
+
+/* ------------------------------------------------------------------- RHS2 */
+/* compact3d.f:6784-7396 */
+void c3o_rhs2(c3o *o, const double *U, const double *P, double *HH,
+              const double *HH2, double *RUH2, const double *SGSVIS) {
+  const int N1M = o->n1m, N2M = o->n2m, N3M = o->n3m;
+  const double RE = o->re, DX1 = o->dx1, DX1Q = o->dx1q, DX3 = o->dx3, DX3Q = o->dx3q;
+  const double *DY = o->dy, *H = o->h;
+  const double *Uu = U, *V = U + NU, *W = U + 2 * NU;
+  double *H2c = HH + NU;
+  const double *H22 = HH2 + NU;
+
+  /* loop 10 (:6825-6865) */
+#pragma omp parallel for
+  for (int k = 1; k <= N3M; k++)
+    for (int j = 2; j <= N2M; j++) {
+      int jp = j + 1, jm = j - 1;
+      for (int i = 1; i <= N1M; i++) {
+        double viscos = (1. / RE) * (o->dyp[j] * V[IX(i, jp, k)] - o->dyc[j] * V[IX(i, j, k)] +
+                                     o->dym[j] * V[IX(i, jm, k)]);
+        double pressg2 = (P[PX(i, j, k)] - P[PX(i, jm, k)]) / H[j];
+        RUH2[IX(i, j, k)] = -pressg2 + viscos;
+        double v2 = 0.5 * (V[IX(i, jp, k)] + V[IX(i, j, k)]);
+        double v1 = 0.5 * (V[IX(i, j, k)] + V[IX(i, jm, k)]);
+        H2c[IX(i, j, k)] = H2c[IX(i, j, k)] + 1. / H[j] * (v2 * v2 - v1 * v1);
+        double sgsj2 = SGSVIS[IX(i, j, k)];
+        double sgsj1 = SGSVIS[IX(i, jm, k)];
+        v2 = 1. / DY[j] * (V[IX(i, jp, k)] - V[IX(i, j, k)]);
+        v1 = 1. / DY[jm] * (V[IX(i, j, k)] - V[IX(i, jm, k)]);
+        double vis22 = 1.0 / H[j] * (sgsj2 * v2 - sgsj1 * v1);
+        H2c[IX(i, j, k)] = H2c[IX(i, j, k)] - 2. * vis22;
+      }
+    }
+  /* loop 20: D2/Dx2 (:6879-6895) */
+#pragma omp parallel for
+  for (int k = 1; k <= N3M; k++)
+    for (int j = 2; j <= N2M; j++) {
+      double d[N1M + 1], x[N1M + 1];
+      for (int i = 1; i <= N1M; i++) {
+        int im = WRAPM(i, N1M), ip = WRAPP(i, N1M);
+        d[i] = 12. * DX1Q * (V[IX(im, j, k)] - 2. * V[IX(i, j, k)] + V[IX(ip, j, k)]);
+      }
+      cyc_solve(fxb(o, 10.0), d, x);
+      for (int i = 1; i <= N1M; i++) RUH2[IX(i, j, k)] = RUH2[IX(i, j, k)] + (1. / RE) * x[i];
+    }
+  /* loop 30: D2/Dz2 (:6907-6923) */
+#pragma omp parallel for
+  for (int i = 1; i <= N1M; i++)
+    for (int j = 2; j <= N2M; j++) {
+      double d[N3M + 1], x[N3M + 1];
+      for (int k = 1; k <= N3M; k++) {
+        int km = WRAPM(k, N3M), kp = WRAPP(k, N3M);
+        d[k] = 12. * DX3Q * (V[IX(i, j, km)] - 2. * V[IX(i, j, k)] + V[IX(i, j, kp)]);
+      }
+      cyc_solve(fzb(o, 10.0), d, x);
+      for (int k = 1; k <= N3M; k++) RUH2[IX(i, j, k)] = RUH2[IX(i, j, k)] + (1. / RE) * x[k];
+    }
+  /* D(UV)/DX (:6942-7006) */
+#pragma omp parallel for
+  for (int k = 1; k <= N3M; k++)
+    for (int j = 2; j <= N2M; j++) {
+      double d[N1M + 1], vv1[N1M + 1], uu2[N1M + 1], x[N1M + 1];
+      for (int i = 1; i <= N1M; i++) d[i] = 4. * (V[IX(WRAPM(i, N1M), j, k)] + V[IX(i, j, k)]);
+      cyc_solve(fxb(o, 6.0), d, vv1);
+      for (int i = 1; i <= N1M; i++)
+        uu2[i] = 1. / H[j] * (DY[j] / 2. * Uu[IX(i, j - 1, k)] + DY[j - 1] / 2. * Uu[IX(i, j, k)]);
+      for (int i = 1; i <= N1M; i++) {
+        int ip = WRAPP(i, N1M);
+        d[i] = 24. * DX1 * (-uu2[i] * vv1[i] + uu2[ip] * vv1[ip]);
+      }
+      cyc_solve(fxb(o, 22.0), d, x);
+      for (int i = 1; i <= N1M; i++) H2c[IX(i, j, k)] = H2c[IX(i, j, k)] + x[i];
+    }
+  /* D(VW)/DZ (:7023-7084) */
+#pragma omp parallel for
+  for (int i = 1; i <= N1M; i++)
+    for (int j = 2; j <= N2M; j++) {
+      double d[N3M + 1], vv3[N3M + 1], ww2[N3M + 1], x[N3M + 1];
+      for (int k = 1; k <= N3M; k++) d[k] = 4. * (V[IX(i, j, WRAPM(k, N3M))] + V[IX(i, j, k)]);
+      cyc_solve(fzb(o, 6.0), d, vv3);
+      for (int k = 1; k <= N3M; k++)
+        ww2[k] = 1. / H[j] * (DY[j] / 2. * W[IX(i, j - 1, k)] + DY[j - 1] / 2. * W[IX(i, j, k)]);
+      for (int k = 1; k <= N3M; k++) {
+        int kp = WRAPP(k, N3M);
+        d[k] = 24. * DX3 * (-vv3[k] * ww2[k] + vv3[kp] * ww2[kp]);
+      }
+      cyc_solve(fzb(o, 22.0), d, x);
+      for (int k = 1; k <= N3M; k++) H2c[IX(i, j, k)] = H2c[IX(i, j, k)] + x[k];
+    }
+
+  if (o->files != 0.0) {
+    /* D(SGS*(DU/DY))/DX (:7104-7158) */
+#pragma omp parallel for
+    for (int k = 1; k <= N3M; k++) {
+      double *SV1 = dalloc((size_t)(N1M + 1) * (N2M + 2));
+      for (int j = 1; j <= N2M; j++) {
+        double d[N1M + 1], x[N1M + 1];
+        for (int i = 1; i <= N1M; i++)
+          d[i] = 4. * (SGSVIS[IX(i, j, k)] + SGSVIS[IX(WRAPM(i, N1M), j, k)]);
+        cyc_solve(fxb(o, 6.0), d, x);
+        for (int i = 1; i <= N1M; i++) B2(SV1, i, j) = x[i];
+      }
+      for (int j = 2; j <= N2M; j++) {
+        double sv21[N1M + 1], vv21[N1M + 1], d[N1M + 1], x[N1M + 1];
+        for (int i = 1; i <= N1M; i++) {
+          sv21[i] = 1. / H[j] * (DY[j - 1] / 2. * B2(SV1, i, j) + DY[j] / 2. * B2(SV1, i, j - 1));
+          vv21[i] = 1. / H[j] * (Uu[IX(i, j, k)] - Uu[IX(i, j - 1, k)]);
+        }
+        for (int i = 1; i <= N1M; i++) {
+          int ip = WRAPP(i, N1M);
+          d[i] = 24. * DX1 * (sv21[ip] * vv21[ip] - sv21[i] * vv21[i]);
+        }
+        cyc_solve(fxb(o, 22.0), d, x);
+        for (int i = 1; i <= N1M; i++) H2c[IX(i, j, k)] = H2c[IX(i, j, k)] - x[i];
+      }
+      free(SV1);
+    }
+    /* D(SGS*(DW/DY))/DZ (:7177-7226) */
+#pragma omp parallel for
+    for (int i = 1; i <= N1M; i++) {
+      double *S3 = dalloc((size_t)(N2M + 2) * (N3M + 1));
+#define C2(a, j, k) a[(size_t)(k) * (N2M + 2) + (j)]
+      for (int j = 1; j <= N2M; j++) {
+        double d[N3M + 1], x[N3M + 1];
+        for (int k = 1; k <= N3M; k++)
+          d[k] = 4. * (SGSVIS[IX(i, j, WRAPM(k, N3M))] + SGSVIS[IX(i, j, k)]);
+        cyc_solve(fzb(o, 6.0), d, x);
+        for (int k = 1; k <= N3M; k++) C2(S3, j, k) = x[k];
+      }
+      for (int j = 2; j <= N2M; j++) {
+        double s23[N3M + 1], vv32[N3M + 1], d[N3M + 1], x[N3M + 1];
+        for (int k = 1; k <= N3M; k++) {
+          s23[k] = 1. / H[j] * (DY[j] / 2. * C2(S3, j - 1, k) + DY[j - 1] / 2. * C2(S3, j, k));
+          vv32[k] = 1. / H[j] * (W[IX(i, j, k)] - W[IX(i, j - 1, k)]);
+        }
+        for (int k = 1; k <= N3M; k++) {
+          int kp = WRAPP(k, N3M);
+          d[k] = 24. * DX3 * (s23[kp] * vv32[kp] - s23[k] * vv32[k]);
+        }
+        cyc_solve(fzb(o, 22.0), d, x);
+        for (int k = 1; k <= N3M; k++) H2c[IX(i, j, k)] = H2c[IX(i, j, k)] - x[k];
+      }
+      free(S3);
+    }
+    /* D(SGS*(DV/DZ))/DZ (:7245-7304) */
+#pragma omp parallel for
+    for (int i = 1; i <= N1M; i++) {
+      double *SV3 = dalloc((size_t)(N2M + 2) * (N3M + 1));
+      double *VV23 = dalloc((size_t)(N2M + 2) * (N3M + 1));
+      for (int j = 1; j <= N2M; j++) {
+        double d[N3M + 1], x[N3M + 1];
+        for (int k = 1; k <= N3M; k++)
+          d[k] = 4. * (SGSVIS[IX(i, j, WRAPM(k, N3M))] + SGSVIS[IX(i, j, k)]);
+        cyc_solve(fzb(o, 6.0), d, x);
+        for (int k = 1; k <= N3M; k++) C2(SV3, j, k) = x[k];
+        for (int k = 1; k <= N3M; k++)
+          d[k] = 24. * DX3 * (V[IX(i, j, k)] - V[IX(i, j, WRAPM(k, N3M))]);
+        cyc_solve(fzb(o, 22.0), d, x);
+        for (int k = 1; k <= N3M; k++) C2(VV23, j, k) = x[k];
+      }
+      for (int j = 2; j <= N2M; j++) {
+        double s32[N3M + 1], d[N3M + 1], x[N3M + 1];
+        for (int k = 1; k <= N3M; k++)
+          s32[k] = 1. / H[j] * (DY[j] / 2. * C2(SV3, j - 1, k) + DY[j - 1] / 2. * C2(SV3, j, k));
+        for (int k = 1; k <= N3M; k++) {
+          int kp = WRAPP(k, N3M);
+          d[k] = 24. * DX3 * (s32[kp] * C2(VV23, j, kp) - s32[k] * C2(VV23, j, k));
+        }
+        cyc_solve(fzb(o, 22.0), d, x);
+        for (int k = 1; k <= N3M; k++) H2c[IX(i, j, k)] = H2c[IX(i, j, k)] - x[k];
+      }
+      free(SV3);
+      free(VV23);
+    }
+    /* D(SGS*(DV/DX))/DX (:7323-7384) */
+#pragma omp parallel for
+    for (int k = 1; k <= N3M; k++) {
+      double *S12 = dalloc((size_t)(N1M + 1) * (N2M + 2));
+      for (int j = 1; j <= N2M; j++) {
+        double d[N1M + 1], x[N1M + 1];
+        for (int i = 1; i <= N1M; i++)
+          d[i] = 4. * (SGSVIS[IX(i, j, k)] + SGSVIS[IX(WRAPM(i, N1M), j, k)]);
+        cyc_solve(fxb(o, 6.0), d, x);
+        for (int i = 1; i <= N1M; i++) B2(S12, i, j) = x[i];
+      }
+      for (int j = 2; j <= N2M; j++) {
+        double vv12[N1M + 1], s22[N1M + 1], d[N1M + 1], x[N1M + 1];
+        for (int i = 1; i <= N1M; i++)
+          d[i] = 24. * DX1 * (V[IX(i, j, k)] - V[IX(WRAPM(i, N1M), j, k)]);
+        cyc_solve(fxb(o, 22.0), d, vv12);
+        for (int i = 1; i <= N1M; i++)
+          s22[i] = 1.0 / H[j] * (DY[j - 1] / 2. * B2(S12, i, j) + DY[j] / 2. * B2(S12, i, j - 1));
+        for (int i = 1; i <= N1M; i++) {
+          int ip = WRAPP(i, N1M);
+          d[i] = 24. * DX1 * (s22[ip] * vv12[ip] - s22[i] * vv12[i]);
+        }
+        cyc_solve(fxb(o, 22.0), d, x);
+        for (int i = 1; i <= N1M; i++) H2c[IX(i, j, k)] = H2c[IX(i, j, k)] - x[i];
+      }
+      free(S12);
+    }
+  }
+  /* AB2 (:7386-7389) */
+#pragma omp parallel for
+  for (int k = 1; k <= N3M; k++)
+    for (int j = 2; j <= N2M; j++)
+      for (int i = 1; i <= N1M; i++)
+        RUH2[IX(i, j, k)] = RUH2[IX(i, j, k)] - 3. / 2. * H2c[IX(i, j, k)] + 1. / 2. * H22[IX(i, j, k)];
+}
+
+/* ------------------------------------------------------------------- RHS3 */
+/* compact3d.f:7399-8114 */
+void c3o_rhs3(c3o *o, const double *U, const double *P, double *HH,
+              const double *HH2, double *RUH3, double presg3,
+              const double *SGSVIS) {
+  const int N1M = o->n1m, N2M = o->n2m, N3M = o->n3m, N2 = o->n2;
+  const double RE = o->re, DX1 = o->dx1, DX1Q = o->dx1q, DX3 = o->dx3,
+               DX3Q = o->dx3q, ALY = o->aly;
+  const double *DY = o->dy, *H = o->h;
+  const int IFWALL = o->ifwall;
+  const double *Uu = U, *V = U + NU, *W = U + 2 * NU;
+  double *H3 = HH + 2 * NU;
+  const double *H23 = HH2 + 2 * NU;
+
+  /* loop 10 (:7442-7469) */
+#pragma omp parallel for
+  for (int k = 1; k <= N3M; k++)
+    for (int j = 1; j <= N2M; j++) {
+      int jp = j + 1, jm = j - 1;
+      for (int i = 1; i <= N1M; i++)
+        RUH3[IX(i, j, k)] = (1. / RE) * (o->hp[j] * W[IX(i, jp, k)] - o->hc[j] * W[IX(i, j, k)] +
+                                         o->hm[j] * W[IX(i, jm, k)]);
+    }
+  /* loop 20 (:7483-7498) */
+#pragma omp parallel for
+  for (int k = 1; k <= N3M; k++)
+    for (int j = 1; j <= N2M; j++) {
+      double d[N1M + 1], x[N1M + 1];
+      for (int i = 1; i <= N1M; i++) {
+        int im = WRAPM(i, N1M), ip = WRAPP(i, N1M);
+        d[i] = 12. * DX1Q * (W[IX(im, j, k)] - 2. * W[IX(i, j, k)] + W[IX(ip, j, k)]);
+      }
+      cyc_solve(fxb(o, 10.0), d, x);
+      for (int i = 1; i <= N1M; i++) RUH3[IX(i, j, k)] = RUH3[IX(i, j, k)] + (1. / RE) * x[i];
+    }
+  /* loop 30 (:7510-7525) */
+#pragma omp parallel for
+  for (int i = 1; i <= N1M; i++)
+    for (int j = 1; j <= N2M; j++) {
+      double d[N3M + 1], x[N3M + 1];
+      for (int k = 1; k <= N3M; k++) {
+        int km = WRAPM(k, N3M), kp = WRAPP(k, N3M);
+        d[k] = 12. * DX3Q * (W[IX(i, j, km)] - 2. * W[IX(i, j, k)] + W[IX(i, j, kp)]);
+      }
+      cyc_solve(fzb(o, 10.0), d, x);
+      for (int k = 1; k <= N3M; k++) RUH3[IX(i, j, k)] = RUH3[IX(i, j, k)] + (1. / RE) * x[k];
+    }
+  /* loop 40: pressure gradient (:7536-7550) */
+#pragma omp parallel for
+  for (int i = 1; i <= N1M; i++)
+    for (int j = 1; j <= N2M; j++) {
+      double d[N3M + 1], x[N3M + 1];
+      for (int k = 1; k <= N3M; k++)
+        d[k] = 24. * DX3 * (-P[PX(i, j, WRAPM(k, N3M))] + P[PX(i, j, k)]);
+      cyc_solve(fzb(o, 22.0), d, x);
+      for (int k = 1; k <= N3M; k++) RUH3[IX(i, j, k)] = RUH3[IX(i, j, k)] - x[k] - presg3;
+    }
+  /* D(WV)/DY (:7563-7613) */
+#pragma omp parallel for
+  for (int i = 1; i <= N1M; i++) {
+    double *VV3 = dalloc((size_t)(N2M + 2) * (N3M + 1));
+    for (int j = 1; j <= N2M; j++) {
+      double d[N3M + 1], x[N3M + 1];
+      for (int k = 1; k <= N3M; k++) d[k] = 4.0 * (V[IX(i, j, WRAPM(k, N3M))] + V[IX(i, j, k)]);
+      cyc_solve(fzb(o, 6.0), d, x);
+      for (int k = 1; k <= N3M; k++) C2(VV3, j, k) = x[k];
+    }
+    for (int k = 1; k <= N3M; k++)
+      for (int j = 1; j <= N2M; j++) {
+        int jp = j + 1, jm = j - 1;
+        double fjum = o->fjmu[j], fjup = o->fjpa[j];
+        double w2 = fjup * 1.0 / H[jp] * (DY[jp] / 2. * W[IX(i, j, k)] + DY[j] / 2. * W[IX(i, jp, k)]) +
+                    (1. - fjup) * W[IX(i, N2, k)];
+        double w1 = fjum * 1.0 / H[j] * (DY[j] / 2. * W[IX(i, jm, k)] + DY[jm] / 2. * W[IX(i, j, k)]) +
+                    (1. - fjum) * W[IX(i, 0, k)];
+        double v2 = C2(VV3, jp, k), v1 = C2(VV3, j, k);
+        if (j == N2M) v2 = 0.0;
+        if (j == 1) v1 = 0.0;
+        H3[IX(i, j, k)] = H3[IX(i, j, k)] + 1. / DY[j] * (v2 * w2 - v1 * w1);
+      }
+    free(VV3);
+  }
+  /* D(WW)/DZ (:7633-7672) */
+#pragma omp parallel for
+  for (int i = 1; i <= N1M; i++)
+    for (int j = 1; j <= N2M; j++) {
+      double d[N3M + 1], ww3[N3M + 1], x[N3M + 1];
+      for (int k = 1; k <= N3M; k++) d[k] = 4. * (W[IX(i, j, k)] + W[IX(i, j, WRAPP(k, N3M))]);
+      cyc_solve(fzb(o, 6.0), d, ww3);
+      for (int k = 1; k <= N3M; k++) {
+        int km = WRAPM(k, N3M);
+        d[k] = 24. * DX3 * (ww3[k] * ww3[k] - ww3[km] * ww3[km]);
+      }
+      cyc_solve(fzb(o, 22.0), d, x);
+      for (int k = 1; k <= N3M; k++) H3[IX(i, j, k)] = H3[IX(i, j, k)] + x[k];
+    }
+  /* D(UW)/DX (:7698-7771) */
+#pragma omp parallel for
+  for (int j = 1; j <= N2M; j++) {
+    double *UU3 = dalloc((size_t)(N1M + 1) * (N3M + 1));
+    double *WW1 = dalloc((size_t)(N1M + 1) * (N3M + 1));
+    for (int i = 1; i <= N1M; i++) {
+      double d[N3M + 1], x[N3M + 1];
+      for (int k = 1; k <= N3M; k++) d[k] = 4. * (Uu[IX(i, j, WRAPM(k, N3M))] + Uu[IX(i, j, k)]);
+      cyc_solve(fzb(o, 6.0), d, x);
+      for (int k = 1; k <= N3M; k++) A2(UU3, i, k) = x[k];
+    }
+    for (int k = 1; k <= N3M; k++) {
+      double d[N1M + 1], x[N1M + 1];
+      for (int i = 1; i <= N1M; i++) d[i] = 4. * (W[IX(WRAPM(i, N1M), j, k)] + W[IX(i, j, k)]);
+      cyc_solve(fxb(o, 6.0), d, x);
+      for (int i = 1; i <= N1M; i++) A2(WW1, i, k) = x[i];
+    }
+    for (int k = 1; k <= N3M; k++) {
+      double d[N1M + 1], x[N1M + 1];
+      for (int i = 1; i <= N1M; i++) {
+        int ip = WRAPP(i, N1M);
+        d[i] = 24. * DX1 * (-A2(UU3, i, k) * A2(WW1, i, k) + A2(UU3, ip, k) * A2(WW1, ip, k));
+      }
+      cyc_solve(fxb(o, 22.0), d, x);
+      for (int i = 1; i <= N1M; i++) H3[IX(i, j, k)] = H3[IX(i, j, k)] + x[i];
+    }
+    free(UU3);
+    free(WW1);
+  }
+
+  if (o->files != 0.0) {
+    /* D(SGS*(DW/DX))/DX (:7797-7847) */
+#pragma omp parallel for
+    for (int j = 1; j <= N2M; j++) {
+      double *SW3 = dalloc((size_t)(N1M + 1) * (N3M + 1));
+      for (int i = 1; i <= N1M; i++) {
+        double d[N3M + 1], x[N3M + 1];
+        for (int k = 1; k <= N3M; k++)
+          d[k] = 4. * (SGSVIS[IX(i, j, WRAPM(k, N3M))] + SGSVIS[IX(i, j, k)]);
+        cyc_solve(fzb(o, 6.0), d, x);
+        for (int k = 1; k <= N3M; k++) A2(SW3, i, k) = x[k];
+      }
+      for (int k = 1; k <= N3M; k++) {
+        double d[N1M + 1], x1[N1M + 1], x11[N1M + 1], x2[N1M + 1];
+        for (int i = 1; i <= N1M; i++) d[i] = 4. * (A2(SW3, WRAPM(i, N1M), k) + A2(SW3, i, k));
+        cyc_solve(fxb(o, 6.0), d, x1);
+        for (int i = 1; i <= N1M; i++) d[i] = 24. * DX1 * (W[IX(i, j, k)] - W[IX(WRAPM(i, N1M), j, k)]);
+        cyc_solve(fxb(o, 22.0), d, x11);
+        for (int i = 1; i <= N1M; i++) {
+          int ip = WRAPP(i, N1M);
+          d[i] = 24. * DX1 * (x1[ip] * x11[ip] - x1[i] * x11[i]);
+        }
+        cyc_solve(fxb(o, 22.0), d, x2);
+        for (int i = 1; i <= N1M; i++) H3[IX(i, j, k)] = H3[IX(i, j, k)] - x2[i];
+      }
+      free(SW3);
+    }
+    /* D(SGS*(DW/DY))/DY (:7858-7908) */
+#pragma omp parallel for
+    for (int i = 1; i <= N1M; i++) {
+      double *SW2 = dalloc((size_t)(N2M + 2) * (N3M + 1));
+      for (int j = 1; j <= N2M; j++) {
+        double d[N3M + 1], x[N3M + 1];
+        for (int k = 1; k <= N3M; k++)
+          d[k] = 4. * (SGSVIS[IX(i, j, k)] + SGSVIS[IX(i, j, WRAPM(k, N3M))]);
+        cyc_solve(fzb(o, 6.0), d, x);
+        for (int k = 1; k <= N3M; k++) C2(SW2, j, k) = x[k];
+      }
+      for (int k = 1; k <= N3M; k++) {
+        int km = o->kma[k];
+        for (int j = 1; j <= N2M; j++) {
+          int jp = j + 1, jm = j - 1;
+          double fjum = o->fjmu[j], fjup = o->fjpa[j];
+          double ww22 = 1. / H[jp] * (W[IX(i, jp, k)] - W[IX(i, j, k)]);
+          double ww11 = 1. / H[j] * (W[IX(i, j, k)] - W[IX(i, jm, k)]);
+          double sgsj1 = fjum * 1. / H[j] * (DY[jm] / 2. * C2(SW2, j, k) + DY[j] / 2. * C2(SW2, jm, k)) +
+                         (1. - fjum) * 0.5 * (SGSVIS[IX(i, 0, k)] + SGSVIS[IX(i, 0, km)]);
+          double sgsj2 = fjup * 1.0 / H[jp] * (DY[j] / 2. * C2(SW2, jp, k) + DY[jp] / 2. * C2(SW2, j, k)) +
+                         (1. - fjup) * 0.5 * (SGSVIS[IX(i, N2, k)] + SGSVIS[IX(i, N2, km)]);
+          double sgsdwy = 1. / DY[j] * ((1. - IFWALL * (1. - fjup) * (ALY - 1.)) * sgsj2 * ww22 -
+                                        (1. - IFWALL * (1. - fjum)) * sgsj1 * ww11);
+          H3[IX(i, j, k)] = H3[IX(i, j, k)] - sgsdwy;
+        }
+      }
+      free(SW2);
+    }
+    /* D(SGS*(DW/DZ))/DZ (:7919-7940) */
+#pragma omp parallel for
+    for (int i = 1; i <= N1M; i++)
+      for (int j = 1; j <= N2M; j++) {
+        double d[N3M + 1], x33[N3M + 1], x2[N3M + 1];
+        for (int k = 1; k <= N3M; k++)
+          d[k] = 24. * DX3 * (W[IX(i, j, WRAPP(k, N3M))] - W[IX(i, j, k)]);
+        cyc_solve(fzb(o, 22.0), d, x33);
+        for (int k = 1; k <= N3M; k++) {
+          int km = WRAPM(k, N3M);
+          d[k] = 24. * DX3 * (x33[k] * SGSVIS[IX(i, j, k)] - x33[km] * SGSVIS[IX(i, j, km)]);
+        }
+        cyc_solve(fzb(o, 22.0), d, x2);
+        for (int k = 1; k <= N3M; k++) H3[IX(i, j, k)] = H3[IX(i, j, k)] - 2. * x2[k];
+      }
+    /* D(SGS*(DU/DZ))/DX (:7972-8024) */
+#pragma omp parallel for
+    for (int j = 1; j <= N2M; j++) {
+      double *UU31 = dalloc((size_t)(N1M + 1) * (N3M + 1));
+      double *S31 = dalloc((size_t)(N1M + 1) * (N3M + 1));
+      for (int i = 1; i <= N1M; i++) {
+        double d[N3M + 1], x[N3M + 1];
+        for (int k = 1; k <= N3M; k++)
+          d[k] = 24. * DX3 * (Uu[IX(i, j, k)] - Uu[IX(i, j, WRAPM(k, N3M))]);
+        cyc_solve(fzb(o, 22.0), d, x);
+        for (int k = 1; k <= N3M; k++) A2(UU31, i, k) = x[k];
+        for (int k = 1; k <= N3M; k++)
+          d[k] = 4. * (SGSVIS[IX(i, j, k)] + SGSVIS[IX(i, j, WRAPM(k, N3M))]);
+        cyc_solve(fzb(o, 6.0), d, x);
+        for (int k = 1; k <= N3M; k++) A2(S31, i, k) = x[k];
+      }
+      for (int k = 1; k <= N3M; k++) {
+        double d[N1M + 1], x1[N1M + 1], x[N1M + 1];
+        for (int i = 1; i <= N1M; i++) d[i] = 4. * (A2(S31, i, k) + A2(S31, WRAPM(i, N1M), k));
+        cyc_solve(fxb(o, 6.0), d, x1);
+        for (int i = 1; i <= N1M; i++) {
+          int ip = WRAPP(i, N1M);
+          d[i] = 24. * DX1 * (x1[ip] * A2(UU31, ip, k) - x1[i] * A2(UU31, i, k));
+        }
+        cyc_solve(fxb(o, 22.0), d, x);
+        for (int i = 1; i <= N1M; i++) H3[IX(i, j, k)] = H3[IX(i, j, k)] - x[i];
+      }
+      free(UU31);
+      free(S31);
+    }
+    /* D(SGS*(DV/DZ))/DY, part 1 (:8042-8094) */
+#pragma omp parallel for
+    for (int i = 1; i <= N1M; i++) {
+      double *VV32 = dalloc((size_t)(N2M + 2) * (N3M + 1));
+      double *S32 = dalloc((size_t)(N2M + 2) * (N3M + 1));
+      for (int j = 1; j <= N2M; j++) {
+        double d[N3M + 1], x[N3M + 1];
+        for (int k = 1; k <= N3M; k++)
+          d[k] = 24. * DX3 * (V[IX(i, j, k)] - V[IX(i, j, WRAPM(k, N3M))]);
+        cyc_solve(fzb(o, 22.0), d, x);
+        for (int k = 1; k <= N3M; k++) C2(VV32, j, k) = x[k];
+        for (int k = 1; k <= N3M; k++)
+          d[k] = 4. * (SGSVIS[IX(i, j, k)] + SGSVIS[IX(i, j, WRAPM(k, N3M))]);
+        cyc_solve(fzb(o, 6.0), d, x);
+        for (int k = 1; k <= N3M; k++) C2(S32, j, k) = x[k];
+      }
+      for (int k = 1; k <= N3M; k++) {
+        int km = o->kma[k];
+        for (int j = 1; j <= N2M; j++) {
+          int jp = j + 1, jm = j - 1;
+          double fjum = o->fjmu[j], fjup = o->fjpa[j];
+          double sgsj2 = fjup * 1.0 / H[jp] * (DY[j] / 2. * C2(S32, jp, k) + DY[jp] / 2. * C2(S32, j, k)) +
+                         (1. - fjup) * 0.5 * (SGSVIS[IX(i, N2, k)] + SGSVIS[IX(i, N2, km)]);
+          double sgsj1 = fjum * 1.0 / H[j] * (DY[jm] / 2. * C2(S32, j, k) + DY[j] / 2. * C2(S32, jm, k)) +
+                         (1. - fjum) * 0.5 * (SGSVIS[IX(i, 0, k)] + SGSVIS[IX(i, 0, km)]);
+          double sgsdvy = 1.0 / DY[j] * ((1. - IFWALL * (1. - fjup) * (ALY - 1.)) * sgsj2 * C2(VV32, jp, k) -
+                                         (1. - IFWALL * (1. - fjum)) * sgsj1 * C2(VV32, j, k));
+          H3[IX(i, j, k)] = H3[IX(i, j, k)] - sgsdvy;
+        }
+      }
+      free(VV32);
+      free(S32);
+    }
+  }
+  /* wall-stress BC + AB2 (:8096-8103) */
+#pragma omp parallel for
+  for (int k = 1; k <= N3M; k++)
+    for (int j = 1; j <= N2M; j++) {
+      double fjum = o->fjmu[j], fjup = o->fjpa[j];
+      for (int i = 1; i <= N1M; i++) {
+        double bc_down = -o->taub[TAUX(i, k, 1)] / DY[1] * IFWALL;
+        double bc_up = -o->taub[TAUX(i, k, 2)] / DY[N2M] * IFWALL * (ALY - 1.);
+        double bc = (1. - fjum) * bc_down + (1. - fjup) * bc_up;
+        RUH3[IX(i, j, k)] = RUH3[IX(i, j, k)] - 3. / 2. * H3[IX(i, j, k)] +
+                            1. / 2. * H23[IX(i, j, k)] + bc;
+      }
+    }
+}
+
+/* ------------------------------------------------------------ GETUH1/2/3 */
+/* y-direction Thomas TDMAI (compact3d.f:8630-8659) on one line j=ns..nf;
+ * a,b,c,d indexed by j; x may alias d */
+static void tdmai_line(int ns, int nf, const double *a, const double *b,
+                       const double *c, const double *d, double *x) {
+  double gam[nf + 2];
+  double bet = 1.0 / b[ns];
+  gam[ns] = c[ns] * bet;
+  x[ns] = d[ns] * bet;
+  for (int j = ns + 1; j <= nf; j++) {
+    bet = 1.0 / (b[j] - a[j] * gam[j - 1]);
+    gam[j] = c[j] * bet;
+    x[j] = (d[j] - a[j] * x[j - 1]) * bet;
+  }
+  for (int j = nf - 1; j >= ns; j--) x[j] = x[j] - x[j + 1] * gam[j];
+}
+
+/* y-sweep of GETUH1/3 (:8249-8282, :8508-8543) and GETUH2 (:8374-8411) */
+static void getuh_y(c3o *o, double *UHc, const double *RUHc, int comp) {
+  const int N1M = o->n1m, N2M = o->n2m, N3M = o->n3m;
+  const double RE = o->re, DT = o->dt;
+  const int ns = (comp == 2) ? 2 : 1;
+  double am[N2M + 2], ac[N2M + 2], ap[N2M + 2];
+  for (int j = ns; j <= N2M; j++) {
+    double fjup = o->fjpa[j];
+    if (comp == 2) {
+      double fjum = o->fjmv[j];
+      ap[j] = fjup * (-(0.5 * o->dyp[j] * (1. / RE))) * DT;
+      ac[j] = 1.0 + (+0.5 * o->dyc[j] * (1. / RE)) * DT;
+      am[j] = fjum * (-(0.5 * o->dym[j] * (1. / RE))) * DT;
+    } else {
+      double fjum = o->fjmu[j];
+      ap[j] = fjup * (-(0.5 * o->hp[j] * (1. / RE))) * DT;
+      ac[j] = 1.0 + (0.5 * o->hc[j] * (1. / RE)) * DT;
+      am[j] = fjum * (-(0.5 * o->hm[j] * (1. / RE))) * DT;
+    }
+  }
+#pragma omp parallel for
+  for (int k = 1; k <= N3M; k++)
+    for (int i = 1; i <= N1M; i++) {
+      double r[N2M + 2];
+      for (int j = ns; j <= N2M; j++)
+        r[j] = (comp == 2) ? DT * RUHc[IX(i, j, k)] : RUHc[IX(i, j, k)] * DT;
+      tdmai_line(ns, N2M, am, ac, ap, r, r);
+      for (int j = ns; j <= N2M; j++) UHc[IX(i, j, k)] = r[j];
+    }
+}
+
+/* x-sweep (CTDMA1J, :8285-8311) and z-sweep (CTDMA3I, :8314-8338) */
+static void getuh_x(c3o *o, double *UHc, int js) {
+  const int N1M = o->n1m, N2M = o->n2m, N3M = o->n3m;
+  const double RE = o->re, DT = o->dt, DX1Q = o->dx1q;
+  double ai[N1M + 1], bi[N1M + 1];
+  for (int i = 1; i <= N1M; i++) {
+    ai[i] = 1.0 - DT / 2.0 * (1. / RE) * 12.0 * DX1Q * (1.0);
+    bi[i] = 10.0 - DT / 2.0 * (1. / RE) * 12.0 * DX1Q * (-2.0);
+  }
+#pragma omp parallel for
+  for (int k = 1; k <= N3M; k++)
+    for (int j = js; j <= N2M; j++) {
+      double r[N1M + 1];
+      for (int i = 1; i <= N1M; i++) {
+        int im = WRAPM(i, N1M), ip = WRAPP(i, N1M);
+        r[i] = UHc[IX(im, j, k)] + 10.0 * UHc[IX(i, j, k)] + UHc[IX(ip, j, k)];
+      }
+      cyc_solve_general(N1M, ai, bi, ai, r, r);
+      for (int i = 1; i <= N1M; i++) UHc[IX(i, j, k)] = r[i];
+    }
+}
+
+static void getuh_z(c3o *o, double *UHc, int js) {
+  const int N1M = o->n1m, N2M = o->n2m, N3M = o->n3m;
+  const double RE = o->re, DT = o->dt, DX3Q = o->dx3q;
+  double ak[N3M + 1], bk[N3M + 1];
+  for (int k = 1; k <= N3M; k++) {
+    ak[k] = 1.0 - DT / 2.0 * (1. / RE) * 12.0 * DX3Q * (1.0);
+    bk[k] = 10.0 - DT / 2.0 * (1. / RE) * 12.0 * DX3Q * (-2.0);
+  }
+#pragma omp parallel for
+  for (int j = js; j <= N2M; j++)
+    for (int i = 1; i <= N1M; i++) {
+      double r[N3M + 1];
+      for (int k = 1; k <= N3M; k++) {
+        int km = WRAPM(k, N3M), kp = WRAPP(k, N3M);
+        r[k] = UHc[IX(i, j, km)] + 10.0 * UHc[IX(i, j, k)] + UHc[IX(i, j, kp)];
+      }
+      cyc_solve_general(N3M, ak, bk, ak, r, r);
+      for (int k = 1; k <= N3M; k++) UHc[IX(i, j, k)] = r[k];
+    }
+}
+
+/* GETUH1 (compact3d.f:8218-8341): y, x, z; leaves delta in UH(.,1) */
+void c3o_getuh1(c3o *o, const double *U, double *UH, const double *RUH1) {
+  (void)U;
+  getuh_y(o, UH, RUH1, 1);
+  getuh_x(o, UH, 1);
+  getuh_z(o, UH, 1);
+}
+/* GETUH2 (compact3d.f:8344-8475): y, x, z on J=2..N2M */
+void c3o_getuh2(c3o *o, const double *U, double *UH, const double *RUH2) {
+  (void)U;
+  getuh_y(o, UH + NU, RUH2, 2);
+  getuh_x(o, UH + NU, 2);
+  getuh_z(o, UH + NU, 2);
+}
+/* GETUH3 (compact3d.f:8478-8627): y, z, x, then UH = U + delta for all three */
+void c3o_getuh3(c3o *o, const double *U, double *UH, const double *RUH3) {
+  const int N1M = o->n1m, N2M = o->n2m, N3M = o->n3m;
+  getuh_y(o, UH + 2 * NU, RUH3, 3);
+  getuh_z(o, UH + 2 * NU, 1);
+  getuh_x(o, UH + 2 * NU, 1);
+#pragma omp parallel for
+  for (int k = 1; k <= N3M; k++)
+    for (int j = 1; j <= N2M; j++)
+      for (int i = 1; i <= N1M; i++) {
+        UH[IX(i, j, k)] = U[IX(i, j, k)] + UH[IX(i, j, k)];
+        UH[2 * NU + IX(i, j, k)] = U[2 * NU + IX(i, j, k)] + UH[2 * NU + IX(i, j, k)];
+      }
+#pragma omp parallel for
+  for (int k = 1; k <= N3M; k++)
+    for (int j = 2; j <= N2M; j++)
+      for (int i = 1; i <= N1M; i++)
+        UH[NU + IX(i, j, k)] = U[NU + IX(i, j, k)] + UH[NU + IX(i, j, k)];
+}
+
+/* ------------------------------------------------------------------ RHSDP */
+/* compact3d.f:8117-8214 */
+void c3o_rhsdp(c3o *o, double *RDP, const double *UH) {
+  const int N1M = o->n1m, N2M = o->n2m, N3M = o->n3m;
+  const double DX1 = o->dx1, DX3 = o->dx3, DT = o->dt;
+  const double *DY = o->dy;
+#pragma omp parallel for
+  for (int k = 1; k <= N3M; k++)
+    for (int j = 1; j <= N2M; j++) {
+      int jp = j + 1;
+      double fjum = o->fjmu[j], fjup = o->fjpa[j];
+      for (int i = 1; i <= N1M; i++) {
+        double divuh = (fjup * UH[NU + IX(i, jp, k)] - fjum * UH[NU + IX(i, j, k)]) / DY[j];
+        double cbc = (1. - fjum) * o->ubc[UBCX(i, k, 2, 1)] / DY[j] -
+                     (1. - fjup) * o->ubc[UBCX(i, k, 2, 2)] / DY[j];
+        RDP[PX(i, j, k)] = (divuh - cbc) / DT;
+      }
+    }
+#pragma omp parallel for
+  for (int k = 1; k <= N3M; k++)
+    for (int j = 1; j <= N2M; j++) {
+      double d[N1M + 1], x[N1M + 1];
+      for (int i = 1; i <= N1M; i++)
+        d[i] = 24. * DX1 * (UH[IX(WRAPP(i, N1M), j, k)] - UH[IX(i, j, k)]);
+      cyc_solve(fxb(o, 22.0), d, x);
+      for (int i = 1; i <= N1M; i++) RDP[PX(i, j, k)] = RDP[PX(i, j, k)] + x[i] / DT;
+    }
+#pragma omp parallel for
+  for (int i = 1; i <= N1M; i++)
+    for (int j = 1; j <= N2M; j++) {
+      double d[N3M + 1], x[N3M + 1];
+      for (int k = 1; k <= N3M; k++)
+        d[k] = 24. * DX3 * (UH[2 * NU + IX(i, j, WRAPP(k, N3M))] - UH[2 * NU + IX(i, j, k)]);
+      cyc_solve(fzb(o, 22.0), d, x);
+      for (int k = 1; k <= N3M; k++) RDP[PX(i, j, k)] = RDP[PX(i, j, k)] + x[k] / DT;
+    }
+}
+
+/* ------------------------------------------------------------ DFT (FFTW) */
+typedef double complex cplx;
+
+static int small_factor(int n) {
+  if (n % 4 == 0) return 4;
+  for (int r = 2; r * r <= n; r++)
+    if (n % r == 0) return r;
+  return n;
+}
+
+/* y[k] = sum_j x[j*xs] * exp(sign*2 pi i j k / n); tw[m] = exp(sign*2 pi i m / N),
+ * n = N / s */
+static void fft_rec(int n, int s, const cplx *x, int xs, cplx *y,
+                    const cplx *tw, int N) {
+  if (n == 1) {
+    y[0] = x[0];
+    return;
+  }
+  int r = small_factor(n);
+  int m = n / r;
+  for (int q = 0; q < r; q++) fft_rec(m, s * r, x + (size_t)q * xs, xs * r, y + (size_t)q * m, tw, N);
+  cplx t[r];
+  for (int k = 0; k < m; k++) {
+    for (int q = 0; q < r; q++) t[q] = y[(size_t)q * m + k] * tw[((long)q * k * s) % N];
+    for (int p = 0; p < r; p++) {
+      cplx acc = 0;
+      for (int q = 0; q < r; q++) acc += t[q] * tw[((long)q * p * m * s) % N];
+      y[(size_t)k + (size_t)p * m] = acc;
+    }
+  }
+}
+
+static void fft1d(int n, cplx *data, int stride, int sign, const cplx *tw) {
+  cplx in[n], out[n];
+  for (int j = 0; j < n; j++) in[j] = data[(size_t)j * stride];
+  fft_rec(n, 1, in, 1, out, tw, n);
+  for (int j = 0; j < n; j++) data[(size_t)j * stride] = out[j];
+  (void)sign;
+}
+
+static cplx *make_tw(int n, int sign) {
+  cplx *tw = (cplx *)malloc(sizeof(cplx) * n);
+  const long double twopi = 6.283185307179586476925286766559L;
+  for (int m = 0; m < n; m++) {
+    long double a = sign * twopi * (long double)m / (long double)n;
+    tw[m] = (double)cosl(a) + I * (double)sinl(a);
+  }
+  return tw;
+}
+
+void c3o_dft2d(int n1, int n3, double *ri, int sign) {
+  cplx *c = (cplx *)ri;
+  cplx *tw1 = make_tw(n1, sign), *tw3 = make_tw(n3, sign);
+  for (int k = 0; k < n3; k++) fft1d(n1, c + (size_t)k * n1, 1, sign, tw1);
+  for (int i = 0; i < n1; i++) fft1d(n3, c + i, n1, sign, tw3);
+  free(tw1);
+  free(tw3);
+}
+
+/* TDMAP (compact3d.f:9444-9459) */
+static void tdmap(const double *a, const double *b, const double *c,
+                  const double *r, int n, double *x) {
+  double gam[n + 2];
+  double bet = b[1];
+  x[1] = r[1] / bet;
+  for (int j = 2; j <= n; j++) {
+    gam[j] = c[j - 1] / bet;
+    bet = b[j] - a[j] * gam[j];
+    x[j] = (r[j] - a[j] * x[j - 1]) / bet;
+  }
+  for (int j = n - 1; j >= 1; j--) x[j] = x[j] - gam[j + 1] * x[j + 1];
+}
+
+/* ------------------------------------------------------------------ GETDP */
+/* compact3d.f:9271-9441 */
+void c3o_getdp(c3o *o, double *DP, const double *RDP) {
+  const int N1M = o->n1m, N2M = o->n2m, N3M = o->n3m, N3MH = o->n3mh;
+  const int N1MH = N1M / 2 + 1;
+  cplx *coef = (cplx *)malloc(sizeof(cplx) * (size_t)N1M * N3M * N2M);
+  cplx *F = (cplx *)malloc(sizeof(cplx) * (size_t)N1M * N3MH * N2M); /* F(i,k,j) */
+  cplx *tw1f = make_tw(N1M, -1), *tw3f = make_tw(N3M, -1);
+  cplx *tw1b = make_tw(N1M, +1), *tw3b = make_tw(N3M, +1);
+#define CO(i, k, j) coef[((size_t)(j - 1) * N3M + (k - 1)) * N1M + (i - 1)]
+#define FF(i, k, j) F[((size_t)(j - 1) * N3MH + (k - 1)) * N1M + (i - 1)]
+#pragma omp parallel for
+  for (int j = 1; j <= N2M; j++) {
+    for (int k = 1; k <= N3M; k++)
+      for (int i = 1; i <= N1M; i++) CO(i, k, j) = RDP[PX(i, j, k)];
+    cplx *pl = &CO(1, 1, j);
+    for (int k = 0; k < N3M; k++) fft1d(N1M, pl + (size_t)k * N1M, 1, -1, tw1f);
+    for (int i = 0; i < N1M; i++) fft1d(N3M, pl + i, N1M, -1, tw3f);
+    for (int k = 1; k <= N3MH; k++)
+      for (int i = 1; i <= N1M; i++) FF(i, k, j) = CO(i, k, j);
+  }
+  /* real parts (:9353-9377) and imaginary parts (:9381-9406) */
+#pragma omp parallel for
+  for (int k = 1; k <= N3MH; k++)
+    for (int i = 1; i <= N1M; i++) {
+      double cm[N2M + 2], cc[N2M + 2], cp[N2M + 2], cf[N2M + 2];
+      for (int j = 1; j <= N2M; j++) {
+        cm[j] = o->pmj[j];
+        cc[j] = o->pcj[j] - o->ak3[k] - o->ak1[i];
+        cp[j] = o->ppj[j];
+        cf[j] = creal(FF(i, k, j));
+        if (i == 1 && k == 1) {
+          cm[1] = 0.0;
+          cc[1] = 1.0;
+          cp[1] = 0.0;
+          cf[1] = 0.0;
+        }
+      }
+      tdmap(cm, cc, cp, cf, N2M, cf);
+      double fr[N2M + 2];
+      for (int j = 1; j <= N2M; j++) fr[j] = cf[j];
+      for (int j = 1; j <= N2M; j++) {
+        cm[j] = o->pmj[j];
+        cc[j] = o->pcj[j] - o->ak3[k] - o->ak1[i];
+        cp[j] = o->ppj[j];
+        cf[j] = cimag(FF(i, k, j));
+        if ((i == 1 && k == 1) || (i == 1 && k == N3MH) || (i == N1MH && k == 1) ||
+            (i == N1MH && k == N3MH)) {
+          cm[j] = 0.0;
+          cc[j] = 1.0;
+          cp[j] = 0.0;
+          cf[j] = 0.0;
+        }
+      }
+      tdmap(cm, cc, cp, cf, N2M, cf);
+      for (int j = 1; j <= N2M; j++) FF(i, k, j) = fr[j] + I * cf[j];
+    }
+  /* inverse (:9410-9435) */
+#pragma omp parallel for
+  for (int j = 1; j <= N2M; j++) {
+    for (int k = 1; k <= N3MH; k++)
+      for (int i = 1; i <= N1M; i++) CO(i, k, j) = FF(i, k, j);
+    for (int k = N3MH + 1; k <= N3M; k++) {
+      int kk = N3M - k + 2;
+      for (int i = 1; i <= N1M; i++) {
+        int ii = N1M - i + 2;
+        if (i == 1) ii = 1;
+        CO(i, k, j) = conj(CO(ii, kk, j));
+      }
+    }
+    cplx *pl = &CO(1, 1, j);
+    for (int k = 0; k < N3M; k++) fft1d(N1M, pl + (size_t)k * N1M, 1, +1, tw1b);
+    for (int i = 0; i < N1M; i++) fft1d(N3M, pl + i, N1M, +1, tw3b);
+    for (int k = 1; k <= N3M; k++)
+      for (int i = 1; i <= N1M; i++) DP[PX(i, j, k)] = creal(CO(i, k, j)) / (double)(N1M * N3M);
+  }
+  free(coef);
+  free(F);
+  free(tw1f);
+  free(tw3f);
+  free(tw1b);
+  free(tw3b);
+#undef CO
+#undef FF
+}
+
+/* ----------------------------------------------------------------- UPCALC */
+/* compact3d.f:1052-1238 */
+void c3o_upcalc(c3o *o, double *U, double *P, const double *UH,
+                const double *DP, double *presg, double *presg3) {
+  const int N1M = o->n1m, N2M = o->n2m, N3M = o->n3m, N2 = o->n2;
+  const double DX1 = o->dx1, DX3 = o->dx3, DT = o->dt, VOL = o->vol;
+  const double *DY = o->dy, *H = o->h;
+  /* the sums run in the reference's serial loop order (k, j, i) */
+  double flow1 = 0.0, dmpresg = 0.0;
+  for (int k = 1; k <= N3M; k++)
+    for (int j = 1; j <= N2M; j++)
+      for (int i = 1; i <= N1M; i++) {
+        int im = o->ima[i];
+        flow1 = flow1 + U[IX(i, j, k)] * DY[j] / DX3 / DX1;
+        dmpresg = dmpresg + UH[IX(i, j, k)] * DY[j] / DX3 / DX1 -
+                  DT * (DP[PX(i, j, k)] - DP[PX(im, j, k)]) * DY[j] / DX3;
+      }
+  dmpresg = (dmpresg - flow1) / VOL / DT;
+#pragma omp parallel for
+  for (int k = 1; k <= N3M; k++)
+    for (int j = 1; j <= N2M; j++) {
+      double d[N1M + 1], x[N1M + 1];
+      for (int i = 1; i <= N1M; i++)
+        d[i] = 24. * DX1 * (-DP[PX(WRAPM(i, N1M), j, k)] + DP[PX(i, j, k)]);
+      cyc_solve(fxb(o, 22.0), d, x);
+      for (int i = 1; i <= N1M; i++) U[IX(i, j, k)] = UH[IX(i, j, k)] - DT * x[i] - DT * dmpresg;
+    }
+  for (int k = 1; k <= N3M; k++)
+    for (int i = 1; i <= N1M; i++) {
+      U[IX(i, 0, k)] = o->ubc[UBCX(i, k, 1, 1)];
+      U[IX(i, N2, k)] = o->ubc[UBCX(i, k, 1, 2)];
+    }
+#pragma omp parallel for
+  for (int k = 1; k <= N3M; k++)
+    for (int j = 2; j <= N2M; j++)
+      for (int i = 1; i <= N1M; i++)
+        U[NU + IX(i, j, k)] = UH[NU + IX(i, j, k)] - DT * (DP[PX(i, j, k)] - DP[PX(i, j - 1, k)]) / H[j];
+  for (int k = 1; k <= N3M; k++)
+    for (int i = 1; i <= N1M; i++) {
+      U[NU + IX(i, 1, k)] = o->ubc[UBCX(i, k, 2, 1)];
+      U[NU + IX(i, N2, k)] = o->ubc[UBCX(i, k, 2, 2)];
+    }
+  double flow3 = 0.0, dmpresg3 = 0.0;
+  for (int k = 1; k <= N3M; k++) {
+    int km = o->kma[k];
+    for (int j = 1; j <= N2M; j++)
+      for (int i = 1; i <= N1M; i++) {
+        flow3 = flow3 + U[2 * NU + IX(i, j, k)] * DY[j] / DX1 / DX3;
+        dmpresg3 = dmpresg3 + UH[2 * NU + IX(i, j, k)] * DY[j] / DX3 / DX1 -
+                   DT * (DP[PX(i, j, k)] - DP[PX(i, j, km)]) * DY[j] / DX1;
+      }
+  }
+  dmpresg3 = (dmpresg3 - flow3) / VOL / DT;
+#pragma omp parallel for
+  for (int i = 1; i <= N1M; i++)
+    for (int j = 1; j <= N2M; j++) {
+      double d[N3M + 1], x[N3M + 1];
+      for (int k = 1; k <= N3M; k++)
+        d[k] = 24. * DX3 * (-DP[PX(i, j, WRAPM(k, N3M))] + DP[PX(i, j, k)]);
+      cyc_solve(fzb(o, 22.0), d, x);
+      for (int k = 1; k <= N3M; k++)
+        U[2 * NU + IX(i, j, k)] = UH[2 * NU + IX(i, j, k)] - DT * x[k] - DT * dmpresg3;
+    }
+  for (int k = 1; k <= N3M; k++)
+    for (int i = 1; i <= N1M; i++) {
+      U[2 * NU + IX(i, 0, k)] = o->ubc[UBCX(i, k, 3, 1)];
+      U[2 * NU + IX(i, N2, k)] = o->ubc[UBCX(i, k, 3, 2)];
+    }
+#pragma omp parallel for
+  for (int k = 1; k <= N3M; k++)
+    for (int j = 1; j <= N2M; j++)
+      for (int i = 1; i <= N1M; i++) P[PX(i, j, k)] = P[PX(i, j, k)] + DP[PX(i, j, k)];
+  *presg = *presg + dmpresg;
+  *presg3 = *presg3 + dmpresg3;
+}
+
+/* --------------------------------------------------------------- DIVCHECK */
+/* compact3d.f:9049-9131; DIV (1..N1M,1..N2M,1..N3M) optional, P-like layout
+ * with extents (N1M,N2M,N3M) */
+double c3o_divcheck(c3o *o, const double *U, double *DIVo) {
+  const int N1M = o->n1m, N2M = o->n2m, N3M = o->n3m;
+  const double DX1 = o->dx1, DX3 = o->dx3;
+  const double *DY = o->dy;
+  double *DIV = dalloc((size_t)N1M * N2M * N3M);
+#define DV(i, j, k) DIV[(((size_t)(k) - 1) * N2M + ((j) - 1)) * N1M + ((i) - 1)]
+#pragma omp parallel for
+  for (int k = 1; k <= N3M; k++)
+    for (int j = 1; j <= N2M; j++)
+      for (int i = 1; i <= N1M; i++)
+        DV(i, j, k) = (U[NU + IX(i, j + 1, k)] - U[NU + IX(i, j, k)]) / DY[j];
+#pragma omp parallel for
+  for (int k = 1; k <= N3M; k++)
+    for (int j = 1; j <= N2M; j++) {
+      double d[N1M + 1], x[N1M + 1];
+      for (int i = 1; i <= N1M; i++) d[i] = 24. * DX1 * (U[IX(WRAPP(i, N1M), j, k)] - U[IX(i, j, k)]);
+      cyc_solve(fxb(o, 22.0), d, x);
+      for (int i = 1; i <= N1M; i++) DV(i, j, k) = DV(i, j, k) + x[i];
+    }
+  double mx[N1M + 1];
+#pragma omp parallel for
+  for (int i = 1; i <= N1M; i++) {
+    double m = 0.0;
+    for (int j = 1; j <= N2M; j++) {
+      double d[N3M + 1], x[N3M + 1];
+      for (int k = 1; k <= N3M; k++)
+        d[k] = 24. * DX3 * (U[2 * NU + IX(i, j, WRAPP(k, N3M))] - U[2 * NU + IX(i, j, k)]);
+      cyc_solve(fzb(o, 22.0), d, x);
+      for (int k = 1; k <= N3M; k++) {
+        DV(i, j, k) = DV(i, j, k) + x[k];
+        m = fmax(fabs(DV(i, j, k)), m);
+      }
+    }
+    mx[i] = m;
+  }
+  double divmax = 0.0;
+  for (int i = 1; i <= N1M; i++) divmax = fmax(mx[i], divmax);
+  if (DIVo) memcpy(DIVo, DIV, sizeof(double) * (size_t)N1M * N2M * N3M);
+  free(DIV);
+#undef DV
+  return divmax;
+}
+
+/* -------------------------------------------------------------------- CFL */
+/* compact3d.f:9138-9169 */
+double c3o_cfl(c3o *o, const double *U) {
+  const int N1M = o->n1m, N2M = o->n2m, N3M = o->n3m;
+  const double DX1 = o->dx1, DX3 = o->dx3;
+  const double *DY = o->dy;
+  double mk[N3M + 1];
+#pragma omp parallel for
+  for (int k = 1; k <= N3M; k++) {
+    int kp = o->kpa[k];
+    double m = 0.0;
+    for (int j = 1; j <= N2M; j++)
+      for (int i = 1; i <= N1M; i++) {
+        int ip = o->ipa[i];
+        double cfll = fabs(U[IX(i, j, k)] + U[IX(ip, j, k)]) * 0.5 * DX1 +
+                      fabs(U[NU + IX(i, j, k)] + U[NU + IX(i, j + 1, k)]) * 0.5 / DY[j] +
+                      fabs(U[2 * NU + IX(i, j, k)] + U[2 * NU + IX(i, j, kp)]) * 0.5 * DX3;
+        m = fmax(m, cfll);
+      }
+    mk[k] = m;
+  }
+  double cflm = 0.0;
+  for (int k = 1; k <= N3M; k++) cflm = fmax(cflm, mk[k]);
+  return cflm;
+}
+
+/* ------------------------------------------------------------------ CHKMF */
+/* compact3d.f:9463-9503, including its use of U(1,J,K,1) (:9484) and
+ * U(I,J,1,3) (:9495) */
+void c3o_chkmf(c3o *o, const double *U, double *flow1, double *flow3) {
+  const int N1M = o->n1m, N2M = o->n2m, N3M = o->n3m;
+  const double DX1 = o->dx1, DX3 = o->dx3;
+  const double *DY = o->dy;
+  double f1 = 0.0, f3 = 0.0;
+  for (int k = 1; k <= N3M; k++)
+    for (int j = 1; j <= N2M; j++)
+      for (int i = 1; i <= N1M; i++) f1 = f1 + U[IX(1, j, k)] * DY[j] / DX3 / DX1;
+  f1 = f1 / o->alx / o->alz;
+  for (int k = 1; k <= N3M; k++)
+    for (int j = 1; j <= N2M; j++)
+      for (int i = 1; i <= N1M; i++) f3 = f3 + U[2 * NU + IX(i, j, 1)] * DY[j] / DX1 / DX3;
+  f3 = f3 / o->alx / o->alz;
+  *flow1 = f1;
+  *flow3 = f3;
+}
+
+/* ----------------------------------------------------------------- WALLSS */
+/* compact3d.f:11077-11143 */
+void c3o_wallss(c3o *o, const double *U) {
+  const int N1M = o->n1m, N2M = o->n2m, N3M = o->n3m, N2 = o->n2;
+  const double *DY = o->dy;
+  const double ALY = o->aly;
+  double nxz = 1.0 / (double)(N1M * N3M);
+  double w1 = 0.0, w2 = 0.0, w3 = 0.0;
+  for (int k = 1; k <= N3M; k++) {
+    int kp = o->kpa[k];
+    for (int i = 1; i <= N1M; i++) {
+      int ip = o->ipa[i];
+      const double *Vv = U + NU, *Ww = U + 2 * NU;
+      double uhc = 0.5 * (U[IX(i, 1, k)] + U[IX(ip, 1, k)]);
+      double uhw = 0.5 * (U[IX(i, 0, k)] + U[IX(ip, 0, k)]);
+      double vhc = Vv[IX(i, 2, k)], vhw = Vv[IX(i, 1, k)];
+      double whc = 0.5 * (Ww[IX(i, 1, k)] + Ww[IX(i, 1, kp)]);
+      double whw = 0.5 * (Ww[IX(i, 0, k)] + Ww[IX(i, 0, kp)]);
+      double wsb1 = (uhc - uhw) / (DY[1] * 0.5);
+      double wsb2 = (vhc - vhw) / DY[1];
+      double wsb3 = (whc - whw) / (DY[1] * 0.5);
+      uhc = 0.5 * (U[IX(i, N2M, k)] + U[IX(ip, N2M, k)]);
+      uhw = 0.5 * (U[IX(i, N2, k)] + U[IX(ip, N2, k)]);
+      double wst1 = (uhc - uhw) / (DY[N2M] * 0.5);
+      w1 = w1 + wsb1 + wst1 * (ALY - 1.0);
+      w2 = w2 + wsb2 + wst1 * (ALY - 1.0);
+      w3 = w3 + wsb3 + wst1 * (ALY - 1.0);
+    }
+  }
+  o->wsm[0] = w1 * nxz / o->re / ALY;
+  o->wsm[1] = w2 * nxz / o->re / ALY;
+  o->wsm[2] = w3 * nxz / o->re / ALY;
+}
+
+/* ----------------------------------------------------------------- STRAIN */
+/* compact3d.f:5558-5689; SR(1:N1,1:N3,6) column-major */
+void c3o_strain(c3o *o, const double *U, double *SR, int j) {
+  const int N1M = o->n1m, N3M = o->n3m, N2 = o->n2, N2M = o->n2m;
+  const double DX1 = o->dx1, DX3 = o->dx3, WM = o->wmodel;
+  const double *DY = o->dy, *H = o->h;
+  const double *V = U + NU, *W = U + 2 * NU;
+  const int jp = j + 1, jm = j - 1;
+  const int fjum = (int)o->fjmu[j], fjup = (int)o->fjpa[j];
+#define SRX(i, k, n) SR[((size_t)(n) - 1) * o->n1 * o->n3 + ((size_t)(k) - 1) * o->n1 + ((i) - 1)]
+  for (int k = 1; k <= N3M; k++) {
+    int kp = o->kpa[k], km = o->kma[k];
+    for (int i = 1; i <= N1M; i++) {
+      int ip = o->ipa[i], im = o->ima[i];
+      double u2 = U[IX(ip, j, k)], u1 = U[IX(i, j, k)];
+      SRX(i, k, 1) = DX1 * (u2 - u1);
+      double ujp = 0.5 * (U[IX(i, jp, k)] + U[IX(ip, jp, k)]);
+      double ujc = 0.5 * (U[IX(i, j, k)] + U[IX(ip, j, k)]);
+      double ujm = 0.5 * (U[IX(i, jm, k)] + U[IX(ip, jm, k)]);
+      if (WM == 0.) {
+        u2 = fjup * 1.0 / H[jp] * (DY[j] / 2.0 * ujp + DY[jp] / 2.0 * ujc) +
+             (1. - fjup) * 0.5 * (U[IX(i, N2, k)] + U[IX(ip, N2, k)]);
+        u1 = fjum * 1.0 / H[j] * (DY[j] / 2.0 * ujm + DY[jm] / 2.0 * ujc) +
+             (1. - fjum) * 0.5 * (U[IX(i, 0, k)] + U[IX(ip, 0, k)]);
+      } else {
+        u2 = fjup * 1.0 / H[jp] * (DY[j] / 2.0 * ujp + DY[jp] / 2.0 * ujc) + (1. - fjup) * ujc;
+        u1 = fjum * 1.0 / H[j] * (DY[j] / 2.0 * ujm + DY[jm] / 2.0 * ujc) + (1. - fjum) * ujc;
+      }
+      double vip = 0.5 * (V[IX(ip, j, k)] + V[IX(ip, jp, k)]);
+      double vic = 0.5 * (V[IX(i, j, k)] + V[IX(i, jp, k)]);
+      double vim = 0.5 * (V[IX(im, j, k)] + V[IX(im, jp, k)]);
+      double v2 = 0.5 * (vip + vic), v1 = 0.5 * (vim + vic);
+      if (WM == 0.) {
+        SRX(i, k, 2) = 0.5 * (DX1 * (v2 - v1) + 1.0 / DY[j] * (u2 - u1));
+      } else {
+        if (j == 1 || j == N2M)
+          SRX(i, k, 2) = 0.5 * (DX1 * (v2 - v1) + 2.0 / DY[j] * (u2 - u1));
+        else
+          SRX(i, k, 2) = 0.5 * (DX1 * (v2 - v1) + 1.0 / DY[j] * (u2 - u1));
+      }
+      double ukp = 0.5 * (U[IX(i, j, kp)] + U[IX(ip, j, kp)]);
+      double ukc = 0.5 * (U[IX(i, j, k)] + U[IX(ip, j, k)]);
+      double ukm = 0.5 * (U[IX(i, j, km)] + U[IX(ip, j, km)]);
+      u2 = 0.5 * (ukp + ukc);
+      u1 = 0.5 * (ukm + ukc);
+      double wip = 0.5 * (W[IX(ip, j, k)] + W[IX(ip, j, kp)]);
+      double wic = 0.5 * (W[IX(i, j, k)] + W[IX(i, j, kp)]);
+      double wim = 0.5 * (W[IX(im, j, k)] + W[IX(im, j, kp)]);
+      double w2 = 0.5 * (wip + wic), w1 = 0.5 * (wim + wic);
+      SRX(i, k, 3) = 0.5 * (DX1 * (w2 - w1) + DX3 * (u2 - u1));
+      v2 = V[IX(i, jp, k)];
+      v1 = V[IX(i, j, k)];
+      SRX(i, k, 4) = 1.0 / DY[j] * (v2 - v1);
+      double wjp = 0.5 * (W[IX(i, jp, k)] + W[IX(i, jp, kp)]);
+      double wjc = 0.5 * (W[IX(i, j, k)] + W[IX(i, j, kp)]);
+      double wjm = 0.5 * (W[IX(i, jm, k)] + W[IX(i, jm, kp)]);
+      if (WM == 0.) {
+        w2 = fjup * 1.0 / H[jp] * (DY[j] / 2.0 * wjp + DY[jp] / 2.0 * wjc) +
+             (1. - fjup) * 0.5 * (W[IX(i, N2, k)] + W[IX(i, N2, kp)]);
+        w1 = fjum * 1.0 / H[j] * (DY[j] / 2.0 * wjm + DY[jm] / 2.0 * wjc) +
+             (1. - fjum) * 0.5 * (W[IX(i, 0, k)] + W[IX(i, 0, kp)]);
+      } else {
+        w2 = fjup * 1.0 / H[jp] * (DY[j] / 2.0 * wjp + DY[jp] / 2.0 * wjc) + (1. - fjup) * wjc;
+        w1 = fjum * 1.0 / H[j] * (DY[j] / 2.0 * wjm + DY[jm] / 2.0 * wjc) + (1. - fjum) * wjc;
+      }
+      double vkp = 0.5 * (V[IX(i, j, kp)] + V[IX(i, jp, kp)]);
+      double vkc = 0.5 * (V[IX(i, j, k)] + V[IX(i, jp, k)]);
+      double vkm = 0.5 * (V[IX(i, j, km)] + V[IX(i, jp, km)]);
+      v2 = 0.5 * (vkp + vkc);
+      v1 = 0.5 * (vkm + vkc);
+      if (WM == 0.) {
+        SRX(i, k, 5) = 0.5 * (1.0 / DY[j] * (w2 - w1) + DX3 * (v2 - v1));
+      } else {
+        if (j == 1 || j == N2M)
+          SRX(i, k, 5) = 0.5 * (2.0 / DY[j] * (w2 - w1) + DX3 * (v2 - v1));
+        else
+          SRX(i, k, 5) = 0.5 * (1.0 / DY[j] * (w2 - w1) + DX3 * (v2 - v1));
+      }
+      w2 = W[IX(i, j, kp)];
+      w1 = W[IX(i, j, k)];
+      SRX(i, k, 6) = DX3 * (w2 - w1);
+    }
+  }
+#undef SRX
+}
+
+/* ------------------------------------------------------------------ SGSCS */
+/* compact3d.f:1413-1488 */
+void c3o_sgscs(c3o *o, const double *U, double *SGSVIS) {
+  const int N1M = o->n1m, N2M = o->n2m, N3M = o->n3m;
+  const double RE = o->re, DX1 = o->dx1, DX3 = o->dx3;
+  const double *DY = o->dy, *Y = o->y;
+  const double reninv = 1. / (double)RE;
+  const double utau = sqrt(sqrt(o->wsm[0] * o->wsm[0] + o->wsm[2] * o->wsm[2]));
+#pragma omp parallel for
+  for (int j = 1; j <= N2M; j++) {
+    double *SR = dalloc((size_t)o->n1 * o->n3 * 6);
+#define SRX(i, k, n) SR[((size_t)(n) - 1) * o->n1 * o->n3 + ((size_t)(k) - 1) * o->n1 + ((i) - 1)]
+    double dxyz = 1.0 / DX1 * DY[j] / DX3;
+    c3o_strain(o, U, SR, j);
+    for (int k = 1; k <= N3M; k++)
+      for (int i = 1; i <= N1M; i++) {
+        double s1 = SRX(i, k, 1), s2 = SRX(i, k, 2), s3 = SRX(i, k, 3);
+        double s4 = SRX(i, k, 4), s5 = SRX(i, k, 5), s6 = SRX(i, k, 6);
+        double sumsr = 2. * (s2 * s2) + s1 * s1 + 2. * (s3 * s3) + s4 * s4 + 2. * (s5 * s5) + s6 * s6;
+        SGSVIS[IX(i, j, k)] = sqrt(2. * sumsr);
+      }
+    double cs = 0.1, a_plus = 26.0;
+    double y_plus = (Y[j] + DY[j] * 0.5) * utau * RE;
+    cs = cs * (1 - exp(-y_plus / a_plus));
+    double smagc = cs * cs;
+    double p23 = pow(dxyz, 2. / 3.);
+    for (int k = 1; k <= N3M; k++)
+      for (int i = 1; i <= N1M; i++) {
+        double turbv = smagc * p23 * SGSVIS[IX(i, j, k)];
+        SGSVIS[IX(i, j, k)] = fmax(turbv, -reninv);
+      }
+    free(SR);
+#undef SRX
+  }
+}
+
+/* ------------------------------------------------------ WALLMODEL_INSTANT */
+/* compact3d.f:2319-2493 */
+void c3o_wallmodel_instant(c3o *o, const double *U) {
+  const int N1M = o->n1m, N3M = o->n3m;
+  const double RE = o->re;
+  const double *DY = o->dy;
+  const double *W = U + 2 * NU;
+  const double KAPA = 0.41, B = 5.2;
+  const double deltap = DY[1] + DY[2] + DY[3] / 2.0;
+  double wsm1 = 0.0;
+  for (int k = 1; k <= N3M; k++)
+    for (int i = 1; i <= N1M; i++) {
+      double u3 = U[IX(i, 3, k)], w3 = W[IX(i, 3, k)];
+      double u_ave = sqrt(u3 * u3 + w3 * w3);
+      double ut1 = 1.0, ut2 = 0.000001, f1, f2, f3, lam = 1.0;
+      while (fabs(ut2 - ut1) > 1.0E-6) {
+        ut1 = ut2;
+        f1 = u_ave / ut1 - log(ut1 * deltap * RE) / KAPA - B;
+        f2 = -u_ave / (ut1 * ut1) - 1. / KAPA / ut1;
+        ut2 = ut1 - lam * f1 / f2;
+        f3 = u_ave / ut2 - log(ut2 * deltap * RE) / KAPA - B;
+        if (fabs(f3) > fabs(f1)) lam = 0.5 * lam;
+      }
+      double tauw1 = ut1 * ut1;
+      wsm1 = wsm1 + tauw1;
+      o->tauw[TAUX(i, k, 1)] = tauw1 * u3 / u_ave;
+      o->taub[TAUX(i, k, 1)] = tauw1 * w3 / u_ave;
+    }
+  o->wsm[0] = wsm1 / (double)(N1M * N3M);
+  o->taow = o->wsm[0];
+}
+
+/* ------------------------------------------------------------ FILTER(4) */
+/* FILTER (compact3d.f:5190-5228): FILV(1:N1,1:N3,n) */
+void c3o_filter(c3o *o, double *FILV, int n) {
+  const int N1M = o->n1m, N3M = o->n3m, M1 = o->n1, M3 = o->n3;
+  double *T = dalloc((size_t)M1 * M3);
+#define FV(i, k, l) FILV[((size_t)(l) - 1) * M1 * M3 + ((size_t)(k) - 1) * M1 + ((i) - 1)]
+#define TT(i, k) T[((size_t)(k) - 1) * M1 + ((i) - 1)]
+  for (int l = 1; l <= n; l++) {
+    for (int k = 1; k <= N3M; k++)
+      for (int i = 1; i <= N1M; i++)
+        TT(i, k) = (FV(o->ima[i], k, l) + 4.0 * FV(i, k, l) + FV(o->ipa[i], k, l)) / 6.0;
+    for (int k = 1; k <= N3M; k++)
+      for (int i = 1; i <= N1M; i++)
+        FV(i, k, l) = (TT(i, o->kma[k]) + 4.0 * TT(i, k) + TT(i, o->kpa[k])) / 6.0;
+  }
+  free(T);
+}
+
+/* FILTER4 (compact3d.f:5481-5546) */
+void c3o_filter4(c3o *o, double *FILV, int n) {
+  const int N1M = o->n1m, N3M = o->n3m, M1 = o->n1, M3 = o->n3;
+  double *T = dalloc((size_t)M1 * M3);
+  int ip2[N1M + 1], im2[N1M + 1], kp2[N3M + 1], km2[N3M + 1];
+  for (int i = 1; i <= N1M; i++) {
+    ip2[i] = i + 2;
+    im2[i] = i - 2;
+  }
+  ip2[N1M] = 2;
+  ip2[N1M - 1] = 1;
+  im2[1] = N1M - 1;
+  im2[2] = N1M;
+  for (int k = 1; k <= N3M; k++) {
+    kp2[k] = k + 2;
+    km2[k] = k - 2;
+  }
+  kp2[N3M] = 2;
+  kp2[N3M - 1] = 1;
+  km2[1] = N3M - 1;
+  km2[2] = N3M;
+  for (int l = 1; l <= n; l++) {
+    for (int k = 1; k <= N3M; k++)
+      for (int i = 1; i <= N1M; i++)
+        TT(i, k) = (FV(im2[i], k, l) + 4.0 * FV(o->ima[i], k, l) + 2.0 * FV(i, k, l) +
+                    4.0 * FV(o->ipa[i], k, l) + FV(ip2[i], k, l)) / 12.0;
+    for (int k = 1; k <= N3M; k++)
+      for (int i = 1; i <= N1M; i++)
+        FV(i, k, l) = (TT(i, km2[k]) + 4.0 * TT(i, o->kma[k]) + 2.0 * TT(i, k) +
+                       4.0 * TT(i, o->kpa[k]) + TT(i, kp2[k])) / 12.0;
+  }
+  free(T);
+#undef FV
+#undef TT
+}
+
+/* ------------------------------------------------------------------ GETUP */
+/* compact3d.f:879-960 */
+void c3o_getup(c3o *o, double *U, double *P, double *HH, double *SGSVIS,
+               double *presg, double *presg3, int ntime) {
+  (void)ntime;
+  const size_t nu = NU, np = (size_t)o->n1 * o->n2 * o->n3;
+  double *UH = dalloc(3 * nu), *HH2 = dalloc(3 * nu), *RUH = dalloc(3 * nu);
+  double *DP = dalloc(np), *RDP = dalloc(np);
+  for (size_t t = 0; t < nu; t++) SGSVIS[t] = 0.0;
+  memcpy(HH2, HH, sizeof(double) * 3 * nu);
+  memset(HH, 0, sizeof(double) * 3 * nu);
+  if (o->files == 2.0) c3o_sgscs(o, U, SGSVIS);
+  if (o->files != 0.0) {
+    if (o->wmodel == 0.0) {
+      o->ifwall = 0;
+      c3o_wallss(o, U);
+    } else if (o->wmodel == 1.0) {
+      c3o_wallmodel_instant(o, U);
+      o->ifwall = 1;
+    }
+  }
+  c3o_bcond(o, U, SGSVIS);
+  c3o_rhs1(o, U, P, HH, HH2, RUH, *presg, SGSVIS);
+  c3o_rhs2(o, U, P, HH, HH2, RUH + nu, SGSVIS);
+  c3o_rhs3(o, U, P, HH, HH2, RUH + 2 * nu, *presg3, SGSVIS);
+  c3o_getuh1(o, U, UH, RUH);
+  c3o_getuh2(o, U, UH, RUH + nu);
+  c3o_getuh3(o, U, UH, RUH + 2 * nu);
+  c3o_rhsdp(o, RDP, UH);
+  c3o_getdp(o, DP, RDP);
+  c3o_upcalc(o, U, P, UH, DP, presg, presg3);
+  free(UH);
+  free(HH2);
+  free(RUH);
+  free(DP);
+  free(RDP);
+}

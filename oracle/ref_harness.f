@@ -1,0 +1,153 @@
+C     ref_harness.f -- project-written accessors for the reference build.
+C
+C     TEST INFRASTRUCTURE ONLY.  Linked with the unmodified hot-path
+C     subroutines of compact3d.f (compiled from their own line ranges by
+C     oracle/Makefile) into oracle/_ref/libc3dref.so.  These routines only
+C     store into and read from the reference's COMMON blocks so that a
+C     test can call RHS1, GETUH1, UPCALC, ... directly.  The COMMON
+C     member lists are the reference's interface (COMMON1.FI and the
+C     inline declarations, e.g. compact3d.f:196-201, :1413-1434) and use
+C     the same implicit typing, so the storage layout is identical.
+C
+      SUBROUTINE REFH_SETUP(IN1,IN2,IN3,XRE,XALX,XALY,XALZ,XDT,
+     >                      XFILES,XWMODEL)
+      INCLUDE 'PARAM.H'
+      COMMON/DIM/N1,N2,N3,N1M,N2M,N3M
+      COMMON/PARA/RE
+      COMMON/SIZE/ALX,ALY,ALZ,VOL
+      COMMON/TSTEP/NTST,DT,CFLMAX
+      COMMON/FINOUT/INCODE,IDTOPT,NWRITE,NREAD,IAVG,NPRN,INSF,NINS,FILES
+      COMMON/WALLMODEL/WMODEL,IFWALL,TAUW1,TAUB1,IS
+      COMMON/ROUGHNESS/Y0
+      N1=IN1
+      N2=IN2
+      N3=IN3
+      N1M=N1-1
+      N2M=N2-1
+      N3M=N3-1
+      RE=XRE
+      ALX=XALX
+      ALY=XALY
+      ALZ=XALZ
+      DT=XDT
+      FILES=XFILES
+      WMODEL=XWMODEL
+      IFWALL=0
+      Y0=0.0
+      CALL MESH
+      CALL INDICES
+      CALL INIWAVE
+      RETURN
+      END
+
+      SUBROUTINE REFH_SETDT(XDT)
+      COMMON/TSTEP/NTST,DT,CFLMAX
+      DT=XDT
+      RETURN
+      END
+
+      SUBROUTINE REFH_SETIFWALL(IVAL)
+      COMMON/WALLMODEL/WMODEL,IFWALL,TAUW1,TAUB1,IS
+      IFWALL=IVAL
+      RETURN
+      END
+
+      SUBROUTINE REFH_GETMESH(XY,XDY,XH,XHM,XHC,XHP,XDYM,XDYC,XDYP,
+     >                        XAK1,XAK3,XPM,XPC,XPP,SC)
+      INCLUDE 'PARAM.H'
+      PARAMETER (M3M=M3-1,M3MH=M3M/2+1)
+      COMMON/DIM/N1,N2,N3,N1M,N2M,N3M
+      COMMON/MESH1/DX1,DX1Q,DX3,DX3Q
+      COMMON/MESH2/Y(0:M2)
+      COMMON/MESH3/DY(0:M2),H(M2),HM(M2),HC(M2),HP(M2)
+      COMMON/MESH4/DYM(M2),DYC(M2),DYP(M2)
+      COMMON/SIZE/ALX,ALY,ALZ,VOL
+      COMMON/WAVK13/AK1(M1),AK3(M3MH)
+      COMMON/METPOI/PMJ(M2),PCJ(M2),PPJ(M2)
+      REAL XY(0:M2),XDY(0:M2),XH(M2),XHM(M2),XHC(M2),XHP(M2)
+      REAL XDYM(M2),XDYC(M2),XDYP(M2),XAK1(M1),XAK3(M3MH)
+      REAL XPM(M2),XPC(M2),XPP(M2),SC(6)
+      DO J=0,M2
+      XY(J)=Y(J)
+      XDY(J)=DY(J)
+      ENDDO
+      DO J=1,M2
+      XH(J)=H(J)
+      XHM(J)=HM(J)
+      XHC(J)=HC(J)
+      XHP(J)=HP(J)
+      XDYM(J)=DYM(J)
+      XDYC(J)=DYC(J)
+      XDYP(J)=DYP(J)
+      XPM(J)=PMJ(J)
+      XPC(J)=PCJ(J)
+      XPP(J)=PPJ(J)
+      ENDDO
+      DO I=1,M1
+      XAK1(I)=AK1(I)
+      ENDDO
+      DO K=1,M3MH
+      XAK3(K)=AK3(K)
+      ENDDO
+      SC(1)=DX1
+      SC(2)=DX1Q
+      SC(3)=DX3
+      SC(4)=DX3Q
+      SC(5)=ALY
+      SC(6)=VOL
+      RETURN
+      END
+
+      SUBROUTINE REFH_SETTAU(TW,TB)
+      INCLUDE 'PARAM.H'
+      COMMON/TAUXYZ/TAUW(0:M1,0:M3,2),TAUB(0:M1,0:M3,2)
+      REAL TW(0:M1,0:M3,2),TB(0:M1,0:M3,2)
+      TAUW=TW
+      TAUB=TB
+      RETURN
+      END
+
+      SUBROUTINE REFH_GETTAU(TW,TB)
+      INCLUDE 'PARAM.H'
+      COMMON/TAUXYZ/TAUW(0:M1,0:M3,2),TAUB(0:M1,0:M3,2)
+      REAL TW(0:M1,0:M3,2),TB(0:M1,0:M3,2)
+      TW=TAUW
+      TB=TAUB
+      RETURN
+      END
+
+      SUBROUTINE REFH_SETWSM(W)
+      COMMON/WSMEM/WSM(3),WSMO(3)
+      REAL W(3)
+      WSM(1)=W(1)
+      WSM(2)=W(2)
+      WSM(3)=W(3)
+      RETURN
+      END
+
+      SUBROUTINE REFH_GETWSM(W)
+      COMMON/WSMEM/WSM(3),WSMO(3)
+      COMMON/WALLTAO/TAOW
+      REAL W(4)
+      W(1)=WSM(1)
+      W(2)=WSM(2)
+      W(3)=WSM(3)
+      W(4)=TAOW
+      RETURN
+      END
+
+      SUBROUTINE REFH_GETUBC(XUBC)
+      INCLUDE 'PARAM.H'
+      COMMON/BCON/UBC(0:M1,0:M3,3,2)
+      REAL XUBC(0:M1,0:M3,3,2)
+      XUBC=UBC
+      RETURN
+      END
+
+      SUBROUTINE REFH_GETFLOW(F)
+      COMMON/FLOWRATE/FLOW1,FLOW3
+      REAL F(2)
+      F(1)=FLOW1
+      F(2)=FLOW3
+      RETURN
+      END
