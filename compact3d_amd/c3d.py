@@ -1,0 +1,233 @@
+"""Python binding of libc3d.so (the C-ABI in include/c3d.h).
+
+`Channel` mirrors the reference's per-step subroutine interface
+(compact3d.f: GETUP :879, BCOND :963, RHS1/2/3 :6063/:6784/:7399,
+GETUH1/2/3 :8218/:8344/:8478, DPCALC :1037, UPCALC :1052, DIVCHECK :9049,
+CFL :9138, CHKMF :9463) on a state that stays resident in HBM.  Host arrays
+use the reference's Fortran layouts (U (N1+1,N2+1,N3+1,3), P (N1,N2,N3),
+order='F').
+
+There is no CPU fallback: constructing a Channel without the built library or
+without a GPU raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libc3d.so")
+
+FP64, MIXED = 0, 1
+F_RUH1, F_RUH2, F_RUH3, F_UH1, F_UH2, F_UH3, F_RDP, F_DP, F_HH1, F_HH2, F_HH3, F_TAUW, F_TAUB = range(1, 14)
+S_PRESG, S_PRESG3, S_DMPRESG, S_DMPRESG3, S_FLOW1, S_FLOW3, S_WSM1, S_WSM2, S_WSM3 = range(9)
+
+_dp = C.POINTER(C.c_double)
+
+
+class C3DError(RuntimeError):
+    pass
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("n1", C.c_int), ("n2", C.c_int), ("n3", C.c_int),
+        ("re", C.c_double), ("alx", C.c_double), ("aly", C.c_double), ("alz", C.c_double),
+        ("dt", C.c_double), ("files", C.c_double), ("wmodel", C.c_double), ("cks", C.c_double),
+        ("y", _dp), ("precision", C.c_int), ("device", C.c_int), ("stream", C.c_void_p),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load libc3d.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise C3DError(f"{LIB_PATH} not built; run __graft_entry__.build() or make -C compact3d_amd/csrc")
+        L = C.CDLL(LIB_PATH)
+        L.c3d_create.argtypes = [C.POINTER(Config), C.POINTER(C.c_void_p)]
+        L.c3d_last_error.restype = C.c_char_p
+        L.c3d_last_error.argtypes = [C.c_void_p]
+        L.c3d_footprint_bytes.restype = C.c_longlong
+        L.c3d_footprint_bytes.argtypes = [C.POINTER(Config)]
+        vp = [C.c_void_p]
+        for name in ("c3d_destroy", "c3d_bcond", "c3d_rhs", "c3d_getuh", "c3d_dpcalc", "c3d_upcalc",
+                     "c3d_sgs", "c3d_wall_model"):
+            getattr(L, name).argtypes = vp
+        L.c3d_getup.argtypes = [C.c_void_p, C.c_int]
+        L.c3d_step.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        L.c3d_upload_state.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, C.c_double, C.c_double]
+        L.c3d_download_state.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.c3d_divcheck.argtypes = [C.c_void_p, _dp]
+        L.c3d_cfl.argtypes = [C.c_void_p, _dp]
+        L.c3d_chkmf.argtypes = [C.c_void_p, _dp, _dp]
+        L.c3d_flowrate.argtypes = [C.c_void_p, _dp, _dp]
+        L.c3d_get_field.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.c3d_get_scalar.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.c3d_set_scalar.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.c3d_set_dt.argtypes = [C.c_void_p, C.c_double]
+        L.c3d_apply_op.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    if a.dtype != np.float64 or not (a.flags.f_contiguous or a.ndim == 1):
+        raise C3DError("arrays must be float64 in Fortran order")
+    return a.ctypes.data_as(_dp)
+
+
+def make_config(n1, n2, n3, re, alx, aly, alz, dt, files=0.0, wmodel=0.0, cks=2.5, y=None,
+                precision=FP64, device=0):
+    cfg = Config()
+    cfg.n1, cfg.n2, cfg.n3 = n1, n2, n3
+    cfg.re, cfg.alx, cfg.aly, cfg.alz, cfg.dt = re, alx, aly, alz, dt
+    cfg.files, cfg.wmodel, cfg.cks = files, wmodel, cks
+    cfg._y = None
+    if y is not None:
+        cfg._y = np.ascontiguousarray(y, dtype=np.float64)
+        cfg.y = cfg._y.ctypes.data_as(_dp)
+    cfg.precision, cfg.device, cfg.stream = precision, device, None
+    return cfg
+
+
+def footprint_bytes(**kw) -> int:
+    return int(lib().c3d_footprint_bytes(C.byref(make_config(**kw))))
+
+
+class Channel:
+    """One channel instance resident on a GPU."""
+
+    def __init__(self, n1, n2, n3, re, alx, aly, alz, dt, files=0.0, wmodel=0.0, cks=2.5, y=None,
+                 precision=FP64, device=0):
+        self.L = lib()
+        self.cfg = make_config(n1, n2, n3, re, alx, aly, alz, dt, files, wmodel, cks, y, precision, device)
+        h = C.c_void_p()
+        rc = self.L.c3d_create(C.byref(self.cfg), C.byref(h))
+        if rc != 0:
+            raise C3DError("c3d_create: " + self.L.c3d_last_error(None).decode())
+        self.h = h
+        self.n1, self.n2, self.n3 = n1, n2, n3
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.c3d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, rc, what):
+        if rc != 0:
+            raise C3DError(f"{what}: " + self.L.c3d_last_error(self.h).decode())
+
+    # ---- shapes
+    def ushape(self):
+        return (self.n1 + 1, self.n2 + 1, self.n3 + 1)
+
+    def pshape(self):
+        return (self.n1, self.n2, self.n3)
+
+    # ---- state
+    def upload(self, U, P, HH=None, SGSVIS=None, presg=0.0, presg3=0.0):
+        self._ck(self.L.c3d_upload_state(self.h, _p(U), _p(P), _p(HH), _p(SGSVIS), presg, presg3), "upload")
+
+    def download(self):
+        U = np.zeros(self.ushape() + (3,), order="F")
+        P = np.zeros(self.pshape(), order="F")
+        HH = np.zeros(self.ushape() + (3,), order="F")
+        S = np.zeros(self.ushape(), order="F")
+        a, b = C.c_double(), C.c_double()
+        self._ck(self.L.c3d_download_state(self.h, _p(U), _p(P), _p(HH), _p(S), C.byref(a), C.byref(b)),
+                 "download")
+        return dict(U=U, P=P, HH=HH, SGSVIS=S, PRESG=a.value, PRESG3=b.value)
+
+    # ---- reference-named operations
+    def getup(self, ntime=1):
+        self._ck(self.L.c3d_getup(self.h, ntime), "getup")
+
+    def step(self, ntime=1, checks=True):
+        if checks:
+            d, c = C.c_double(), C.c_double()
+            self._ck(self.L.c3d_step(self.h, ntime, C.byref(d), C.byref(c)), "step")
+            return d.value, c.value
+        self._ck(self.L.c3d_step(self.h, ntime, None, None), "step")
+        return None
+
+    def bcond(self):
+        self._ck(self.L.c3d_bcond(self.h), "bcond")
+
+    def rhs(self):
+        self._ck(self.L.c3d_rhs(self.h), "rhs")
+
+    def getuh(self):
+        self._ck(self.L.c3d_getuh(self.h), "getuh")
+
+    def dpcalc(self):
+        self._ck(self.L.c3d_dpcalc(self.h), "dpcalc")
+
+    def upcalc(self):
+        self._ck(self.L.c3d_upcalc(self.h), "upcalc")
+
+    def sgscs(self):
+        self._ck(self.L.c3d_sgs(self.h), "sgs")
+
+    def wall_model(self):
+        self._ck(self.L.c3d_wall_model(self.h), "wall_model")
+
+    def divcheck(self):
+        v = C.c_double()
+        self._ck(self.L.c3d_divcheck(self.h, C.byref(v)), "divcheck")
+        return v.value
+
+    def cfl(self):
+        v = C.c_double()
+        self._ck(self.L.c3d_cfl(self.h, C.byref(v)), "cfl")
+        return v.value
+
+    def chkmf(self):
+        a, b = C.c_double(), C.c_double()
+        self._ck(self.L.c3d_chkmf(self.h, C.byref(a), C.byref(b)), "chkmf")
+        return a.value, b.value
+
+    def flowrate(self):
+        a, b = C.c_double(), C.c_double()
+        self._ck(self.L.c3d_flowrate(self.h, C.byref(a), C.byref(b)), "flowrate")
+        return a.value, b.value
+
+    def field(self, which):
+        if which in (F_RDP, F_DP):
+            out = np.zeros(self.pshape(), order="F")
+        elif which in (F_TAUW, F_TAUB):
+            out = np.zeros((self.n1 + 1, self.n3 + 1, 2), order="F")
+        else:
+            out = np.zeros(self.ushape(), order="F")
+        self._ck(self.L.c3d_get_field(self.h, which, _p(out)), "get_field")
+        return out
+
+    def scalar(self, which):
+        v = C.c_double()
+        self._ck(self.L.c3d_get_scalar(self.h, which, C.byref(v)), "get_scalar")
+        return v.value
+
+    def set_scalar(self, which, value):
+        self._ck(self.L.c3d_set_scalar(self.h, which, float(value)), "set_scalar")
+
+    def set_dt(self, dt):
+        self._ck(self.L.c3d_set_dt(self.h, float(dt)), "set_dt")
+
+    def apply_op(self, op, direction, field):
+        inp = np.asfortranarray(field, dtype=np.float64)
+        out = np.zeros(self.pshape(), order="F")
+        self._ck(self.L.c3d_apply_op(self.h, op, direction, _p(inp), _p(out)), "apply_op")
+        return out

@@ -1,0 +1,741 @@
+// c3d_api.cpp -- host side of libc3d: the C-ABI declared in include/c3d.h.
+//
+// Setup reproduces SETUP/MESH/INDICES/INIWAVE/METRICPOISSON
+// (compact3d.f:192-494, 9182-9263) on the host and uploads the resulting
+// vectors; the step runs entirely on the device from then on.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/c3d.h"
+#include "c3d_impl.h"
+
+struct c3d_handle {
+  c3d_config cfg;
+  Ctx c;
+  bool own_stream = false;
+  std::string err;
+  std::vector<void*> allocs;
+  int hh_cur = 0;               // which of the two HH sets is "current"
+  double* hhbuf[2][3] = {{nullptr}};
+  // host copies of the 1-D vectors (reference J indexing)
+  std::vector<double> Y, DY, H, HM, HC, HP, DYM, DYC, DYP, FJPA, FJMU, FJMV, PMJ, PCJ, PPJ, AK1, AK3;
+  double* dvec = nullptr;       // device block holding all vectors
+  double* dtmp = nullptr;       // scratch field for apply_op
+  double* dtmp2 = nullptr;
+  std::vector<double> stage;    // host staging buffer
+};
+
+static std::string g_create_err;
+
+#define HIPCHK(x)                                                                         \
+  do {                                                                                    \
+    hipError_t e_ = (x);                                                                  \
+    if (e_ != hipSuccess) {                                                               \
+      h->err = std::string(#x) + ": " + hipGetErrorString(e_);                            \
+      return -1;                                                                          \
+    }                                                                                     \
+  } while (0)
+
+static int fail(c3d_handle* h, const std::string& m) {
+  h->err = m;
+  return -1;
+}
+
+// ------------------------------------------------------------ line solves
+static LineSolve make_solve(double a, double b) {
+  // a*x[p-1] + b*x[p] + a*x[p+1] = d[p]  ->  alpha, scale, warm-up (c3d_dev.h)
+  LineSolve s;
+  if (a == 0.0) {
+    s.alpha = 0.0;
+    s.scale = 1.0 / b;
+    s.w = 0;
+    return s;
+  }
+  long double beta = (long double)b / (long double)a;
+  long double sq = sqrtl(beta * beta - 4.0L);
+  long double al = 2.0L / (beta + (beta > 0 ? sq : -sq));
+  s.alpha = (double)al;
+  s.scale = (double)(al / (long double)a);
+  long double aa = fabsl(al);
+  int w = (aa > 0) ? (int)ceill(logl(0x1p-60L) / logl(aa)) + 2 : 0;
+  s.w = w;
+  return s;
+}
+
+static TileShape make_tile(int n, int ntiles) {
+  TileShape t;
+  t.n = n;
+  int g = (n + 7) / 8;  // segments of ~8 points
+  if (g > C3D_NT) g = C3D_NT;
+  int l = C3D_NT / g;
+  int lmax = (int)(65536 / (sizeof(double) * (size_t)n * ntiles)) - 1;
+  if (l > lmax) l = lmax;
+  if (l < 1) l = 1;
+  // n*l must fit MAXE*C3D_NT = 4096 elements
+  while (n * l > 4096 && l > 1) l--;
+  t.nl = l;
+  t.ls = l + 1;
+  t.g = g;
+  t.seg = (n + g - 1) / g;
+  return t;
+}
+
+static bool factor_fft(int n, int* rad, int& nr) {
+  nr = 0;
+  while (n % 4 == 0 && n > 0) {
+    rad[nr++] = 4;
+    n /= 4;
+  }
+  const int ps[3] = {2, 3, 5};
+  for (int q = 0; q < 3; q++)
+    while (n % ps[q] == 0) {
+      rad[nr++] = ps[q];
+      n /= ps[q];
+    }
+  return n == 1 && nr <= 12;
+}
+
+static std::vector<double> twiddles(int n, int sign) {
+  std::vector<double> t(2 * (size_t)n);
+  const long double tp = 6.283185307179586476925286766559L;
+  for (int m = 0; m < n; m++) {
+    long double a = sign * tp * (long double)m / (long double)n;
+    t[2 * m] = (double)cosl(a);
+    t[2 * m + 1] = (double)sinl(a);
+  }
+  return t;
+}
+
+static int pow2_floor(int x) {
+  int p = 1;
+  while (p * 2 <= x) p *= 2;
+  return p;
+}
+
+// ------------------------------------------------------------------ mesh
+static void build_mesh(c3d_handle* h) {
+  const c3d_config& cf = h->cfg;
+  const int N2 = cf.n2, N2M = N2 - 1, N1M = cf.n1 - 1, N3M = cf.n3 - 1;
+  auto z = [&](std::vector<double>& v, int n) { v.assign(n, 0.0); };
+  z(h->Y, N2 + 2); z(h->DY, N2 + 2); z(h->H, N2 + 2); z(h->HM, N2 + 2); z(h->HC, N2 + 2);
+  z(h->HP, N2 + 2); z(h->DYM, N2 + 2); z(h->DYC, N2 + 2); z(h->DYP, N2 + 2); z(h->FJPA, N2 + 2);
+  z(h->FJMU, N2 + 2); z(h->FJMV, N2 + 2); z(h->PMJ, N2 + 2); z(h->PCJ, N2 + 2); z(h->PPJ, N2 + 2);
+  z(h->AK1, N1M + 2); z(h->AK3, N3M / 2 + 3);
+  std::vector<double>&Y = h->Y, &DY = h->DY, &H = h->H;
+  double aly = cf.aly;
+  const double CKS = cf.cks, Y0 = 0.0;
+  // MESH (:338-361)
+  if (cf.y) {
+    for (int j = 0; j <= N2; j++) Y[j] = cf.y[j];
+  } else {
+    Y[0] = (cf.wmodel == 0.0) ? 0.0 : Y0;
+    for (int j = 1; j <= N2; j++) {
+      if (cf.wmodel == 0.0) {
+        if (aly == 2.0)
+          Y[j] = 1 - tanh(CKS * (1 - aly * (j - 1) / (N2 - 1))) / tanh(CKS);
+        else
+          Y[j] = 1 - tanh(CKS * (1 - 2. * aly * (j - 1) / (2 * N2 - 1))) / tanh(CKS);
+      } else {
+        Y[j] = (aly - Y0) * (j - 1) / (N2 - 1) + Y0;
+      }
+    }
+  }
+  aly = Y[N2];
+  Scal& s = h->c.s;
+  s.aly = aly;
+  s.re = cf.re;
+  s.rinv = 1. / cf.re;
+  s.dt = cf.dt;
+  s.dx1 = (double)N1M / cf.alx;
+  s.dx3 = (double)N3M / cf.alz;
+  s.dx1q = s.dx1 * s.dx1;
+  s.dx3q = s.dx3 * s.dx3;
+  s.vol = cf.alx * aly * cf.alz;
+  s.ifwall = 0;
+  s.files = cf.files != 0.0;
+  DY[1] = Y[2];
+  for (int j = 2; j <= N2M; j++) {
+    DY[j] = Y[j + 1] - Y[j];
+    H[j] = 0.5 * (DY[j] + DY[j - 1]);
+  }
+  H[1] = 0.5 * DY[1];
+  H[N2] = 0.5 * DY[N2M];
+  std::vector<double>&HP = h->HP, &HC = h->HC, &HM = h->HM;
+  for (int j = 2; j <= N2M - 1; j++) {
+    HP[j] = 1.0 / H[j + 1] / DY[j];
+    HC[j] = (H[j + 1] + H[j]) / H[j + 1] / H[j] / DY[j];
+    HM[j] = 1.0 / H[j] / DY[j];
+  }
+  HP[1] = 4.0 * H[1] / H[2] / (H[1] + H[2]) / DY[1];
+  HC[1] = 4.0 / H[2] / DY[1];
+  HM[1] = 4.0 / (H[1] + H[2]) / DY[1];
+  HP[N2M] = 4.0 / (H[N2M] + H[N2]) / DY[N2M];
+  HC[N2M] = 4.0 / H[N2M] / DY[N2M];
+  HM[N2M] = 4.0 * H[N2] / H[N2M] / (H[N2M] + H[N2]) / DY[N2M];
+  if (cf.wmodel != 0.0) {
+    HP[1] = 1.0 / H[2] / DY[1];
+    HC[1] = 1.0 / H[2] / DY[1];
+    HM[1] = 0.0;
+    HP[N2M] = 0.0;
+    HC[N2M] = 1.0 / H[N2M] / DY[N2M];
+    HM[N2M] = 1.0 / H[N2M] / DY[N2M];
+  }
+  for (int j = 2; j <= N2M; j++) {
+    h->DYP[j] = 1.0 / H[j] / DY[j];
+    h->DYC[j] = 1.0 / H[j] * (1.0 / DY[j] + 1.0 / DY[j - 1]);
+    h->DYM[j] = 1.0 / H[j] / DY[j - 1];
+  }
+  // INDICES (:468-481): wall flags
+  for (int j = 1; j <= N2M; j++) {
+    int jpa = (j == N2M) ? N2M : j + 1;
+    int jmu = (j == 1) ? 1 : j - 1;
+    int jmv = (j == 2) ? 2 : j - 1;
+    h->FJPA[j] = jpa - j;
+    h->FJMU[j] = j - jmu;
+    h->FJMV[j] = j - jmv;
+  }
+  // INIWAVE (:9197-9222)
+  const double PI = acos(-1.0);
+  for (int i = 1; i <= N1M; i++) {
+    int ii = i;
+    if (i > N1M / 2 + 1) ii = i - N1M;
+    double c = cos(((ii - 1) * 2.0 * PI) / N1M);
+    double den = 11. + c;
+    h->AK1[i] = 288. * s.dx1q * (1. - c) / (den * den);
+  }
+  for (int k = 1; k <= N3M / 2 + 1; k++) {
+    double c = cos(((k - 1) * 2. * PI) / N3M);
+    double den = 11. + c;
+    h->AK3[k] = 288. * s.dx3q * (1. - c) / (den * den);
+  }
+  // METRICPOISSON (:9249-9260)
+  for (int jc = 1; jc <= N2M; jc++) {
+    int jp = jc + 1;
+    double fjum = h->FJMU[jc], fjup = h->FJPA[jc];
+    h->PMJ[jc] = fjum * fjup * (1.0 / DY[jc] / H[jc]) + (1. - fjup) * (1.0 / DY[N2M] / H[N2M]);
+    h->PCJ[jc] = fjum * fjup * (-1.0 / DY[jc] / H[jc] - 1.0 / DY[jc] / H[jp]) +
+                 (1. - fjup) * (-1.0 / DY[N2M] / H[N2M]) + (1. - fjum) * (-1.0 / DY[1] / H[2]);
+    h->PPJ[jc] = fjum * fjup * (1.0 / DY[jc] / H[jp]) + (1. - fjum) * (1.0 / DY[1] / H[2]);
+  }
+}
+
+// ADI y chains (GETUH1/2/3 coefficient loops :8261-8267, :8389-8397,
+// :8524-8529 and TDMAI :8637-8650), all J-only; plus the x/z ADI factors
+// (:8294-8296, :8324-8326).  Depend on DT.
+static void build_dt_terms(c3d_handle* h, std::vector<double>& ych) {
+  const int N2 = h->cfg.n2, N2M = N2 - 1;
+  const double RE = h->c.s.re, DT = h->c.s.dt;
+  ych.assign(9 * (size_t)(N2 + 2), 0.0);
+  for (int comp = 0; comp < 3; comp++) {
+    double* A = &ych[(3 * comp + 0) * (size_t)(N2 + 2)];
+    double* BET = &ych[(3 * comp + 1) * (size_t)(N2 + 2)];
+    double* GAM = &ych[(3 * comp + 2) * (size_t)(N2 + 2)];
+    const int ns = (comp == 1) ? 2 : 1;
+    std::vector<double> ap(N2 + 2), ac(N2 + 2), am(N2 + 2);
+    for (int j = ns; j <= N2M; j++) {
+      double fjup = h->FJPA[j];
+      if (comp == 1) {
+        double fjum = h->FJMV[j];
+        ap[j] = fjup * (-(0.5 * h->DYP[j] * (1. / RE))) * DT;
+        ac[j] = 1.0 + (+0.5 * h->DYC[j] * (1. / RE)) * DT;
+        am[j] = fjum * (-(0.5 * h->DYM[j] * (1. / RE))) * DT;
+      } else {
+        double fjum = h->FJMU[j];
+        ap[j] = fjup * (-(0.5 * h->HP[j] * (1. / RE))) * DT;
+        ac[j] = 1.0 + (0.5 * h->HC[j] * (1. / RE)) * DT;
+        am[j] = fjum * (-(0.5 * h->HM[j] * (1. / RE))) * DT;
+      }
+    }
+    double bet = 1.0 / ac[ns];
+    BET[ns] = bet;
+    GAM[ns] = ap[ns] * bet;
+    A[ns] = am[ns];
+    for (int j = ns + 1; j <= N2M; j++) {
+      bet = 1.0 / (ac[j] - am[j] * GAM[j - 1]);
+      BET[j] = bet;
+      GAM[j] = ap[j] * bet;
+      A[j] = am[j];
+    }
+  }
+  Ops& o = h->c.o;
+  const double dx1q = h->c.s.dx1q, dx3q = h->c.s.dx3q;
+  o.adix_a = 1.0 - DT / 2.0 * (1. / RE) * 12.0 * dx1q * (1.0);
+  o.adix_b = 10.0 - DT / 2.0 * (1. / RE) * 12.0 * dx1q * (-2.0);
+  o.adiz_a = 1.0 - DT / 2.0 * (1. / RE) * 12.0 * dx3q * (1.0);
+  o.adiz_b = 10.0 - DT / 2.0 * (1. / RE) * 12.0 * dx3q * (-2.0);
+  o.adix = make_solve(o.adix_a, o.adix_b);
+  o.adiz = make_solve(o.adiz_a, o.adiz_b);
+}
+
+static int upload_dt_terms(c3d_handle* h) {
+  std::vector<double> ych;
+  build_dt_terms(h, ych);
+  const int N2 = h->cfg.n2;
+  double* base = (double*)h->c.m.ya[0];
+  HIPCHK(hipMemcpy(base, ych.data(), ych.size() * sizeof(double), hipMemcpyHostToDevice));
+  (void)N2;
+  return 0;
+}
+
+template <typename T>
+static int dalloc(c3d_handle* h, T** p, size_t n) {
+  void* q = nullptr;
+  HIPCHK(hipMalloc(&q, n * sizeof(T) + 64));
+  HIPCHK(hipMemset(q, 0, n * sizeof(T) + 64));
+  h->allocs.push_back(q);
+  *p = (T*)q;
+  return 0;
+}
+
+long long c3d_footprint_bytes(const c3d_config* cfg) {
+  if (!cfg) return -1;
+  long long nx = cfg->n1 - 1, nz = cfg->n3 - 1, nj = cfg->n2 + 1, n2m = cfg->n2 - 1;
+  long long fsz = nx * nj * nz;
+  long long fields = 26;
+  long long spec = (nx / 2 + 1) * nz * n2m * 16;
+  return fields * fsz * 8 + spec + 4 * nx * nz * 8 + (1 << 20);
+}
+
+extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
+  g_create_err.clear();
+  if (!cfg || !out) {
+    g_create_err = "null argument";
+    return -1;
+  }
+  *out = nullptr;
+  c3d_handle* h = new c3d_handle();
+  h->cfg = *cfg;
+  auto bad = [&](const std::string& m) {
+    g_create_err = m;
+    for (void* p : h->allocs) (void)hipFree(p);
+    delete h;
+    return -1;
+  };
+  const c3d_config& cf = h->cfg;
+  if (cf.n1 < 9 || cf.n2 < 5 || cf.n3 < 9) return bad("grid too small (need N1,N3 >= 9, N2 >= 5)");
+  if (cf.precision != C3D_FP64) return bad("only C3D_FP64 is available in this build");
+  if (cf.files != 0.0 && cf.files != 2.0) return bad("FILES must be 0 (DNS) or 2 (Smagorinsky)");
+  if (cf.wmodel != 0.0 && cf.wmodel != 1.0) return bad("WMODEL must be 0 or 1");
+  if (cf.files != 0.0) return bad("LES (FILES=2) is not available in this build yet");
+  if (cf.re <= 0 || cf.dt <= 0 || cf.alx <= 0 || cf.alz <= 0) return bad("RE, DT, ALX, ALZ must be > 0");
+  if (hipSetDevice(cf.device) != hipSuccess) return bad("hipSetDevice failed");
+  Ctx& c = h->c;
+  Geo& g = c.g;
+  g.nx = cf.n1 - 1;
+  g.nz = cf.n3 - 1;
+  g.n2 = cf.n2;
+  g.n2m = cf.n2 - 1;
+  g.nj = cf.n2 + 1;
+  g.nxh = g.nx / 2 + 1;
+  g.plane = (long long)g.nj * g.nx;
+  g.fsz = g.plane * g.nz;
+  if (g.nx % 2 || g.nz % 2) return bad("N1-1 and N3-1 must be even");
+  FFTPlan& f = c.f;
+  f.nx = g.nx;
+  f.nz = g.nz;
+  if (!factor_fft(g.nx, f.radx, f.nradx) || !factor_fft(g.nz, f.radz, f.nradz))
+    return bad("N1-1 and N3-1 must factor into 2, 3 and 5");
+  f.tlx = std::max(1, 2048 / g.nx);
+  f.tlz = std::max(1, 2048 / g.nz);
+  f.tly = std::max(1, std::min(32, pow2_floor((int)(65536 / (24 * (size_t)g.n2m)))));
+  if (g.n2m > 1300) return bad("N2 too large for the y tiles");
+  build_mesh(h);
+  Ops& o = c.o;
+  o.s6 = make_solve(1.0, 6.0);
+  o.s10 = make_solve(1.0, 10.0);
+  o.s22 = make_solve(1.0, 22.0);
+  o.tx = make_tile(g.nx, 2);
+  o.tz = make_tile(g.nz, 2);
+
+  if (cf.stream) {
+    c.st = (hipStream_t)cf.stream;
+  } else {
+    if (hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking) != hipSuccess) return bad("stream create failed");
+    h->own_stream = true;
+  }
+  // device vectors: 15 J-vectors + AK1 + AK3 + 9 ADI chains
+  const size_t nv = (size_t)(g.n2 + 2);
+  const size_t nvec = 15 * nv + (g.nx + 2) + (g.nz / 2 + 3) + 9 * nv;
+  if (dalloc(h, &h->dvec, nvec)) return bad(h->err);
+  {
+    std::vector<double> hv(nvec, 0.0);
+    std::vector<double>* src[15] = {&h->Y,   &h->DY,  &h->H,    &h->HP,   &h->HC,
+                                    &h->HM,  &h->DYP, &h->DYC,  &h->DYM,  &h->FJPA,
+                                    &h->FJMU, &h->FJMV, &h->PMJ, &h->PCJ, &h->PPJ};
+    for (int q = 0; q < 15; q++) std::copy(src[q]->begin(), src[q]->end(), hv.begin() + q * nv);
+    std::copy(h->AK1.begin(), h->AK1.end(), hv.begin() + 15 * nv);
+    std::copy(h->AK3.begin(), h->AK3.end(), hv.begin() + 15 * nv + (g.nx + 2));
+    if (hipMemcpy(h->dvec, hv.data(), nvec * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      return bad("vector upload failed");
+    const double* d = h->dvec;
+    Metrics& m = c.m;
+    m.y = d + 0 * nv;
+    m.dy = d + 1 * nv;
+    m.h = d + 2 * nv;
+    m.hp = d + 3 * nv;
+    m.hc = d + 4 * nv;
+    m.hm = d + 5 * nv;
+    m.dyp = d + 6 * nv;
+    m.dyc = d + 7 * nv;
+    m.dym = d + 8 * nv;
+    m.fjpa = d + 9 * nv;
+    m.fjmu = d + 10 * nv;
+    m.fjmv = d + 11 * nv;
+    m.pmj = d + 12 * nv;
+    m.pcj = d + 13 * nv;
+    m.ppj = d + 14 * nv;
+    m.ak1 = d + 15 * nv;
+    m.ak3 = d + 15 * nv + (g.nx + 2);
+    const double* yc = d + 15 * nv + (g.nx + 2) + (g.nz / 2 + 3);
+    for (int q = 0; q < 3; q++) {
+      m.ya[q] = yc + (3 * q + 0) * nv;
+      m.ybet[q] = yc + (3 * q + 1) * nv;
+      m.ygam[q] = yc + (3 * q + 2) * nv;
+    }
+  }
+  if (upload_dt_terms(h)) return bad(h->err);
+  // FFT twiddles
+  {
+    std::vector<double> a = twiddles(g.nx, -1), b = twiddles(g.nx, +1), cz = twiddles(g.nz, -1),
+                        dz = twiddles(g.nz, +1);
+    double2 *p1, *p2, *p3, *p4;
+    if (dalloc(h, &p1, g.nx) || dalloc(h, &p2, g.nx) || dalloc(h, &p3, g.nz) || dalloc(h, &p4, g.nz))
+      return bad(h->err);
+    if (hipMemcpy(p1, a.data(), a.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p2, b.data(), b.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p3, cz.data(), cz.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p4, dz.data(), dz.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+      return bad("twiddle upload failed");
+    f.twxf = p1;
+    f.twxb = p2;
+    f.twzf = p3;
+    f.twzb = p4;
+  }
+  // fields
+  Fields& F = c.d;
+  const size_t fs = (size_t)g.fsz;
+  double** fl[] = {&F.u,      &F.v,      &F.w,      &F.p,      &F.ruh[0], &F.ruh[1], &F.ruh[2],
+                   &F.uh[0],  &F.uh[1],  &F.uh[2],  &F.vv1,    &F.ww1,    &F.uw,     &F.vv3,
+                   &F.rdp,    &F.dp,     &F.sgs,    &F.div,    &h->dtmp,  &h->dtmp2};
+  for (auto pp : fl)
+    if (dalloc(h, pp, fs)) return bad(h->err);
+  for (int s2 = 0; s2 < 2; s2++)
+    for (int q = 0; q < 3; q++)
+      if (dalloc(h, &h->hhbuf[s2][q], fs)) return bad(h->err);
+  for (int q = 0; q < 3; q++) {
+    F.hh[q] = h->hhbuf[0][q];
+    F.hhp[q] = h->hhbuf[1][q];
+  }
+  const size_t n2d = (size_t)g.nx * g.nz;
+  if (dalloc(h, &F.ubc_u, n2d) || dalloc(h, &F.ubc_w, n2d) || dalloc(h, &F.tauw, n2d) ||
+      dalloc(h, &F.taub, n2d))
+    return bad(h->err);
+  if (dalloc(h, &F.spec, (size_t)g.nxh * g.nz * g.n2m)) return bad(h->err);
+  F.pgam = nullptr;
+  if (dalloc(h, &F.sc, SC_COUNT)) return bad(h->err);
+  F.nred = 1024;
+  if (dalloc(h, &F.red, 4 * (size_t)F.nred)) return bad(h->err);
+  *out = h;
+  return 0;
+}
+
+extern "C" int c3d_destroy(c3d_handle* h) {
+  if (!h) return 0;
+  (void)hipStreamSynchronize(h->c.st);
+  for (void* p : h->allocs) (void)hipFree(p);
+  if (h->own_stream) (void)hipStreamDestroy(h->c.st);
+  delete h;
+  return 0;
+}
+
+extern "C" const char* c3d_last_error(const c3d_handle* h) {
+  if (!h) return g_create_err.c_str();
+  return h->err.c_str();
+}
+
+// ------------------------------------------------------- layout transfers
+// Fortran U-like (0:N1,0:N2,0:N3) index
+static inline size_t fu(const c3d_handle* h, int I, int J, int K) {
+  return (size_t)I + (size_t)(h->cfg.n1 + 1) * ((size_t)J + (size_t)(h->cfg.n2 + 1) * (size_t)K);
+}
+static inline size_t fp(const c3d_handle* h, int I, int J, int K) {
+  return (size_t)(I - 1) + (size_t)h->cfg.n1 * ((size_t)(J - 1) + (size_t)h->cfg.n2 * (size_t)(K - 1));
+}
+
+enum { LAY_U = 0, LAY_P = 1 };
+
+static int to_dev(c3d_handle* h, double* dst, const double* src, int lay) {
+  const Geo& g = h->c.g;
+  h->stage.assign((size_t)g.fsz, 0.0);
+  double* s = h->stage.data();
+  if (src) {
+    for (int k = 0; k < g.nz; k++)
+      for (int j = 0; j <= g.n2; j++) {
+        if (lay == LAY_P && (j < 1 || j > g.n2m)) continue;
+        double* row = s + ((size_t)k * g.nj + j) * g.nx;
+        if (lay == LAY_U)
+          for (int i = 0; i < g.nx; i++) row[i] = src[fu(h, i + 1, j, k + 1)];
+        else
+          for (int i = 0; i < g.nx; i++) row[i] = src[fp(h, i + 1, j, k + 1)];
+      }
+  }
+  HIPCHK(hipMemcpyAsync(dst, s, (size_t)g.fsz * 8, hipMemcpyHostToDevice, h->c.st));
+  HIPCHK(hipStreamSynchronize(h->c.st));
+  return 0;
+}
+
+static int from_dev(c3d_handle* h, double* dst, const double* src, int lay) {
+  const Geo& g = h->c.g;
+  h->stage.assign((size_t)g.fsz, 0.0);
+  double* s = h->stage.data();
+  HIPCHK(hipMemcpyAsync(s, src, (size_t)g.fsz * 8, hipMemcpyDeviceToHost, h->c.st));
+  HIPCHK(hipStreamSynchronize(h->c.st));
+  for (int k = 0; k < g.nz; k++)
+    for (int j = 0; j <= g.n2; j++) {
+      if (lay == LAY_P && (j < 1 || j > g.n2m)) continue;
+      const double* row = s + ((size_t)k * g.nj + j) * g.nx;
+      if (lay == LAY_U)
+        for (int i = 0; i < g.nx; i++) dst[fu(h, i + 1, j, k + 1)] = row[i];
+      else
+        for (int i = 0; i < g.nx; i++) dst[fp(h, i + 1, j, k + 1)] = row[i];
+    }
+  return 0;
+}
+
+extern "C" int c3d_upload_state(c3d_handle* h, const double* U, const double* P, const double* HH,
+                                const double* SGSVIS, double presg, double presg3) {
+  if (!h) return -1;
+  if (!U || !P) return fail(h, "U and P are required");
+  const size_t nu = (size_t)(h->cfg.n1 + 1) * (h->cfg.n2 + 1) * (h->cfg.n3 + 1);
+  Fields& F = h->c.d;
+  if (to_dev(h, F.u, U, LAY_U) || to_dev(h, F.v, U + nu, LAY_U) || to_dev(h, F.w, U + 2 * nu, LAY_U))
+    return -1;
+  if (to_dev(h, F.p, P, LAY_P)) return -1;
+  for (int q = 0; q < 3; q++)
+    if (to_dev(h, F.hh[q], HH ? HH + q * nu : nullptr, LAY_U)) return -1;
+  if (to_dev(h, F.sgs, SGSVIS, LAY_U)) return -1;
+  double sc[2] = {presg, presg3};
+  HIPCHK(hipMemcpy(F.sc + SC_PRESG, sc, 2 * sizeof(double), hipMemcpyHostToDevice));
+  return 0;
+}
+
+extern "C" int c3d_download_state(c3d_handle* h, double* U, double* P, double* HH, double* SGSVIS,
+                                  double* presg, double* presg3) {
+  if (!h) return -1;
+  const size_t nu = (size_t)(h->cfg.n1 + 1) * (h->cfg.n2 + 1) * (h->cfg.n3 + 1);
+  Fields& F = h->c.d;
+  if (U && (from_dev(h, U, F.u, LAY_U) || from_dev(h, U + nu, F.v, LAY_U) ||
+            from_dev(h, U + 2 * nu, F.w, LAY_U)))
+    return -1;
+  if (P && from_dev(h, P, F.p, LAY_P)) return -1;
+  if (HH)
+    for (int q = 0; q < 3; q++)
+      if (from_dev(h, HH + q * nu, F.hh[q], LAY_U)) return -1;
+  if (SGSVIS && from_dev(h, SGSVIS, F.sgs, LAY_U)) return -1;
+  double sc[2];
+  HIPCHK(hipMemcpy(sc, F.sc + SC_PRESG, 2 * sizeof(double), hipMemcpyDeviceToHost));
+  if (presg) *presg = sc[0];
+  if (presg3) *presg3 = sc[1];
+  return 0;
+}
+
+// ------------------------------------------------------------------ phases
+static int launched(c3d_handle* h) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(h, std::string("kernel launch: ") + hipGetErrorString(e));
+  return 0;
+}
+
+// GETUP prologue (:915-923): the current HH becomes HH2, the new HH is
+// written from scratch by the RHS passes
+static void swap_hh(c3d_handle* h) {
+  h->hh_cur ^= 1;
+  for (int q = 0; q < 3; q++) {
+    h->c.d.hh[q] = h->hhbuf[h->hh_cur][q];
+    h->c.d.hhp[q] = h->hhbuf[h->hh_cur ^ 1][q];
+  }
+}
+
+extern "C" int c3d_bcond(c3d_handle* h) {
+  if (!h) return -1;
+  c3dk::bcond(h->c);
+  return launched(h);
+}
+
+extern "C" int c3d_rhs(c3d_handle* h) {
+  if (!h) return -1;
+  swap_hh(h);
+  c3dk::rhs(h->c);
+  return launched(h);
+}
+
+extern "C" int c3d_getuh(c3d_handle* h) {
+  if (!h) return -1;
+  c3dk::getuh(h->c);
+  return launched(h);
+}
+
+extern "C" int c3d_dpcalc(c3d_handle* h) {
+  if (!h) return -1;
+  c3dk::rhsdp(h->c);
+  c3dk::getdp(h->c);
+  return launched(h);
+}
+
+extern "C" int c3d_upcalc(c3d_handle* h) {
+  if (!h) return -1;
+  c3dk::upcalc(h->c, false);
+  return launched(h);
+}
+
+extern "C" int c3d_sgs(c3d_handle* h) {
+  if (!h) return -1;
+  return fail(h, "SGSCS is not available in this build yet");
+}
+
+extern "C" int c3d_wall_model(c3d_handle* h) {
+  if (!h) return -1;
+  return fail(h, "the wall model is not available in this build yet");
+}
+
+static int step_impl(c3d_handle* h, bool chk) {
+  swap_hh(h);
+  c3dk::bcond(h->c);
+  c3dk::rhs(h->c);
+  c3dk::getuh(h->c);
+  c3dk::rhsdp(h->c);
+  c3dk::getdp(h->c);
+  c3dk::upcalc(h->c, chk);
+  return launched(h);
+}
+
+extern "C" int c3d_getup(c3d_handle* h, int ntime) {
+  (void)ntime;
+  if (!h) return -1;
+  return step_impl(h, false);
+}
+
+static int read_sc(c3d_handle* h, int slot, double* v) {
+  HIPCHK(hipMemcpyAsync(v, h->c.d.sc + slot, sizeof(double), hipMemcpyDeviceToHost, h->c.st));
+  HIPCHK(hipStreamSynchronize(h->c.st));
+  return 0;
+}
+
+static double bits_to_double(double v) { return v; }
+
+extern "C" int c3d_step(c3d_handle* h, int ntime, double* divmax, double* cflm) {
+  (void)ntime;
+  if (!h) return -1;
+  if (step_impl(h, true)) return -1;
+  if (divmax && read_sc(h, SC_DIVMAX, divmax)) return -1;
+  if (cflm && read_sc(h, SC_CFLM, cflm)) return -1;
+  if (divmax) *divmax = bits_to_double(*divmax);
+  return 0;
+}
+
+extern "C" int c3d_divcheck(c3d_handle* h, double* divmax) {
+  if (!h) return -1;
+  c3dk::divcheck(h->c);
+  if (launched(h)) return -1;
+  if (divmax) return read_sc(h, SC_DIVMAX, divmax);
+  return 0;
+}
+
+extern "C" int c3d_cfl(c3d_handle* h, double* cflm) {
+  if (!h) return -1;
+  c3dk::cfl(h->c);
+  if (launched(h)) return -1;
+  if (cflm) return read_sc(h, SC_CFLM, cflm);
+  return 0;
+}
+
+extern "C" int c3d_chkmf(c3d_handle* h, double* flow1, double* flow3) {
+  if (!h) return -1;
+  c3dk::chkmf(h->c, true);
+  if (launched(h)) return -1;
+  if (flow1 && read_sc(h, SC_CHK1, flow1)) return -1;
+  if (flow3 && read_sc(h, SC_CHK3, flow3)) return -1;
+  return 0;
+}
+
+extern "C" int c3d_flowrate(c3d_handle* h, double* flow1, double* flow3) {
+  if (!h) return -1;
+  c3dk::chkmf(h->c, false);
+  if (launched(h)) return -1;
+  if (flow1 && read_sc(h, SC_CHK1, flow1)) return -1;
+  if (flow3 && read_sc(h, SC_CHK3, flow3)) return -1;
+  return 0;
+}
+
+extern "C" int c3d_get_field(c3d_handle* h, int which, double* out) {
+  if (!h || !out) return -1;
+  Fields& F = h->c.d;
+  double* src = nullptr;
+  int lay = LAY_U;
+  switch (which) {
+    case C3D_F_RUH1: src = F.ruh[0]; break;
+    case C3D_F_RUH2: src = F.ruh[1]; break;
+    case C3D_F_RUH3: src = F.ruh[2]; break;
+    case C3D_F_UH1: src = F.uh[0]; break;
+    case C3D_F_UH2: src = F.uh[1]; break;
+    case C3D_F_UH3: src = F.uh[2]; break;
+    case C3D_F_RDP: src = F.rdp; lay = LAY_P; break;
+    case C3D_F_DP: src = F.dp; lay = LAY_P; break;
+    case C3D_F_HH1: src = F.hh[0]; break;
+    case C3D_F_HH2: src = F.hh[1]; break;
+    case C3D_F_HH3: src = F.hh[2]; break;
+    case C3D_F_TAUW:
+    case C3D_F_TAUB: {
+      const Geo& g = h->c.g;
+      std::vector<double> t((size_t)g.nx * g.nz);
+      HIPCHK(hipMemcpy(t.data(), which == C3D_F_TAUW ? F.tauw : F.taub, t.size() * 8, hipMemcpyDeviceToHost));
+      const int n1 = h->cfg.n1, n3 = h->cfg.n3;
+      for (size_t q = 0; q < (size_t)(n1 + 1) * (n3 + 1) * 2; q++) out[q] = 0.0;
+      for (int k = 0; k < g.nz; k++)
+        for (int i = 0; i < g.nx; i++) out[(size_t)(i + 1) + (size_t)(n1 + 1) * (k + 1)] = t[(size_t)k * g.nx + i];
+      return 0;
+    }
+    default: return fail(h, "unknown field");
+  }
+  return from_dev(h, out, src, lay);
+}
+
+extern "C" int c3d_get_scalar(c3d_handle* h, int which, double* out) {
+  if (!h || !out) return -1;
+  const int map[9] = {SC_PRESG, SC_PRESG3, SC_DMPRESG, SC_DMPRESG3, SC_FLOW1, SC_FLOW3, SC_WSM1, SC_WSM2, SC_WSM3};
+  if (which < 0 || which > 8) return fail(h, "unknown scalar");
+  return read_sc(h, map[which], out);
+}
+
+extern "C" int c3d_set_scalar(c3d_handle* h, int which, double v) {
+  if (!h) return -1;
+  const int map[9] = {SC_PRESG, SC_PRESG3, SC_DMPRESG, SC_DMPRESG3, SC_FLOW1, SC_FLOW3, SC_WSM1, SC_WSM2, SC_WSM3};
+  if (which < 0 || which > 8) return fail(h, "unknown scalar");
+  HIPCHK(hipMemcpyAsync(h->c.d.sc + map[which], &v, sizeof(double), hipMemcpyHostToDevice, h->c.st));
+  HIPCHK(hipStreamSynchronize(h->c.st));
+  return 0;
+}
+
+extern "C" int c3d_set_dt(c3d_handle* h, double dt) {
+  if (!h) return -1;
+  if (!(dt > 0)) return fail(h, "dt must be > 0");
+  HIPCHK(hipStreamSynchronize(h->c.st));
+  h->c.s.dt = dt;
+  h->cfg.dt = dt;
+  return upload_dt_terms(h);
+}
+
+extern "C" int c3d_apply_op(c3d_handle* h, int op, int dir, const double* in, double* out) {
+  if (!h || !in || !out) return -1;
+  if (op < 0 || op > 4 || (dir != 1 && dir != 3)) return fail(h, "bad op/dir");
+  if (to_dev(h, h->dtmp, in, LAY_P)) return -1;
+  c3dk::apply_op(h->c, op, dir, h->dtmp, h->dtmp2);
+  if (launched(h)) return -1;
+  return from_dev(h, out, h->dtmp2, LAY_P);
+}

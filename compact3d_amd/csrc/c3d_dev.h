@@ -1,0 +1,199 @@
+// c3d_dev.h -- device-side layout, parameters and the LDS line engines.
+//
+// Field layout in HBM (one array per field): f[(k*NJ + j)*NX + i]
+//   i = I-1 in [0,NX), NX = N1M      (x, periodic, contiguous)
+//   j = J   in [0,N2], NJ = N2+1     (y, wall-normal, ghost planes 0 and N2)
+//   k = K-1 in [0,NZ), NZ = N3M      (z, periodic)
+// x-lines are contiguous rows; z-lines have stride NJ*NX; y-lines stride NX.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#ifndef C3D_NT
+#define C3D_NT 256  // threads per workgroup of the line kernels (4 wavefronts)
+#endif
+
+struct Geo {
+  int nx, nz, n2, n2m, nj, nxh;  // nxh = NX/2+1 (half spectrum)
+  long long plane;               // NJ*NX
+  long long fsz;                 // NZ*NJ*NX
+};
+
+__device__ __forceinline__ long long gidx(const Geo& g, int i, int j, int k) {
+  return ((long long)k * g.nj + j) * g.nx + i;
+}
+
+// Compact-operator line solve.  Every periodic system of the path has the
+// constant symmetric form a*x[p-1] + b*x[p] + a*x[p+1] = d[p] (CTDMA11/CTDMA33
+// with (1,b,1), CTDMA1J/CTDMA3I with the ADI factors).  Writing
+// beta = b/a and alpha = the root of alpha^2 - beta*alpha + 1 = 0 with
+// |alpha| < 1, the circulant factors exactly as
+//   a*(S^-1 + beta + S) = (a/alpha) (1 + alpha S^-1)(1 + alpha S),
+// so x = (alpha/a) * B^-1 F^-1 d with the first-order recursions
+//   F: y[p] = d[p] - alpha*y[p-1]      (forward)
+//   B: z[p] = y[p] - alpha*z[p+1]      (backward).
+// Each recursion forgets its start value as |alpha|^W; starting W points
+// early (wrapping periodically) with zero state gives the periodic solution
+// to |alpha|^W <= 2^-60.  That lets G threads share one line: each owns a
+// segment and warms up from its neighbours' data.
+struct LineSolve {
+  double alpha;  // decay factor
+  double scale;  // alpha / a
+  int w;         // warm-up length
+};
+
+// Work tile in LDS: element (p, l) at buf[p*ls + l]; p along the line, l the
+// line within the tile.  Threads t = l + L*g, g = segment.
+template <typename T>
+__device__ __forceinline__ void tile_solve(T* buf, int n, int nl, int ls, int g_cnt, int seg,
+                                           const LineSolve op) {
+  const int t = threadIdx.x;
+  const int l = t % nl;
+  const int g = t / nl;
+  const bool act = (g < g_cnt);
+  const int p0 = g * seg;
+  const int p1 = min(p0 + seg, n);
+  const T a = (T)op.alpha;
+  const int w = op.w;
+  T y = (T)0;
+  if (act && p0 < n) {
+    int q = p0 - w;
+    q %= n;
+    if (q < 0) q += n;
+    for (int s = 0; s < w; s++) {
+      y = buf[q * ls + l] - a * y;
+      q = (q + 1 == n) ? 0 : q + 1;
+    }
+  }
+  __syncthreads();
+  if (act) {
+    for (int p = p0; p < p1; p++) {
+      y = buf[p * ls + l] - a * y;
+      buf[p * ls + l] = y;
+    }
+  }
+  __syncthreads();
+  T z = (T)0;
+  if (act && p0 < n) {
+    int q = (p1 + w - 1) % n;
+    for (int s = 0; s < w; s++) {
+      z = buf[q * ls + l] - a * z;
+      q = (q == 0) ? n - 1 : q - 1;
+    }
+  }
+  __syncthreads();
+  if (act) {
+    const T sc = (T)op.scale;
+    for (int p = p1 - 1; p >= p0; p--) {
+      z = buf[p * ls + l] - a * z;
+      buf[p * ls + l] = sc * z;
+    }
+  }
+  __syncthreads();
+}
+
+// Tile shape: L lines x N points, G segments per line, L*G <= C3D_NT.
+struct TileShape {
+  int n;    // line length
+  int nl;   // lines per tile (L)
+  int ls;   // LDS line stride (L, padded)
+  int g;    // segments per line
+  int seg;  // segment length
+};
+
+// ---------------------------------------------------------------- FFT
+struct cpx {
+  double x, y;
+};
+template <typename T>
+struct cplx_t {
+  T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cplx_t<T> cmul(cplx_t<T> a, cplx_t<T> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+
+// small DFT of size R in place, sign = -1 forward / +1 backward
+template <typename T>
+__device__ __forceinline__ void dft_small(cplx_t<T>* v, int R, int sign) {
+  if (R == 2) {
+    cplx_t<T> a = v[0], b = v[1];
+    v[0] = {a.x + b.x, a.y + b.y};
+    v[1] = {a.x - b.x, a.y - b.y};
+  } else if (R == 4) {
+    cplx_t<T> a0 = {v[0].x + v[2].x, v[0].y + v[2].y};
+    cplx_t<T> a1 = {v[0].x - v[2].x, v[0].y - v[2].y};
+    cplx_t<T> a2 = {v[1].x + v[3].x, v[1].y + v[3].y};
+    cplx_t<T> a3 = {v[1].x - v[3].x, v[1].y - v[3].y};
+    // multiply a3 by sign*i
+    cplx_t<T> a3r = (sign < 0) ? cplx_t<T>{a3.y, -a3.x} : cplx_t<T>{-a3.y, a3.x};
+    v[0] = {a0.x + a2.x, a0.y + a2.y};
+    v[2] = {a0.x - a2.x, a0.y - a2.y};
+    v[1] = {a1.x + a3r.x, a1.y + a3r.y};
+    v[3] = {a1.x - a3r.x, a1.y - a3r.y};
+  } else if (R == 3) {
+    const T c1 = (T)-0.5;
+    const T s1 = (T)(sign < 0 ? -0.86602540378443864676 : 0.86602540378443864676);
+    cplx_t<T> a = v[0], b = v[1], c = v[2];
+    cplx_t<T> t1 = {b.x + c.x, b.y + c.y};
+    cplx_t<T> t2 = {a.x + c1 * t1.x, a.y + c1 * t1.y};
+    cplx_t<T> t3 = {s1 * (b.x - c.x), s1 * (b.y - c.y)};
+    v[0] = {a.x + t1.x, a.y + t1.y};
+    v[1] = {t2.x - t3.y, t2.y + t3.x};
+    v[2] = {t2.x + t3.y, t2.y - t3.x};
+  } else {  // R == 5, direct
+    const T cs[5] = {(T)1.0, (T)0.30901699437494742410, (T)-0.80901699437494742410,
+                     (T)-0.80901699437494742410, (T)0.30901699437494742410};
+    const T sn0[5] = {(T)0.0, (T)0.95105651629515357212, (T)0.58778525229247312917,
+                      (T)-0.58778525229247312917, (T)-0.95105651629515357212};
+    cplx_t<T> in[5];
+    for (int q = 0; q < 5; q++) in[q] = v[q];
+    for (int p = 0; p < 5; p++) {
+      T ax = 0, ay = 0;
+      for (int q = 0; q < 5; q++) {
+        int m = (p * q) % 5;
+        T c = cs[m], s = (sign < 0 ? -sn0[m] : sn0[m]);
+        ax += in[q].x * c - in[q].y * s;
+        ay += in[q].x * s + in[q].y * c;
+      }
+      v[p] = {ax, ay};
+    }
+  }
+}
+
+// Mixed-radix Stockham FFT of nl lines of length n held in LDS as
+// a[p*ls + l] (complex).  b is a second buffer of the same shape.  Radices
+// (4,2,3,5) are listed in rad[0..nrad).  tw[m] = exp(sign*2*pi*i*m/n) for the
+// sign of this transform.  Returns the buffer holding the result.
+template <typename T>
+__device__ cplx_t<T>* tile_fft(cplx_t<T>* a, cplx_t<T>* b, int n, int nl, int ls,
+                               const int* rad, int nrad, const cplx_t<T>* tw, int sign) {
+  int ns = 1;
+  for (int s = 0; s < nrad; s++) {
+    const int R = rad[s];
+    const int nb = n / R;  // butterflies per line
+    const int tot = nb * nl;
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+      const int l = e % nl;
+      const int jb = e / nl;
+      const int k = jb % ns;
+      const int tstep = k * (n / (ns * R));
+      cplx_t<T> v[5];
+      for (int r = 0; r < R; r++) {
+        cplx_t<T> x = a[(jb + r * nb) * ls + l];
+        v[r] = (r == 0) ? x : cmul(x, tw[(tstep * r) % n]);
+      }
+      dft_small(v, R, sign);
+      const int d = (jb / ns) * ns * R + k;
+      for (int r = 0; r < R; r++) b[(d + r * ns) * ls + l] = v[r];
+    }
+    __syncthreads();
+    cplx_t<T>* t = a;
+    a = b;
+    b = t;
+    ns *= R;
+  }
+  return a;
+}

@@ -1,0 +1,90 @@
+// c3d_impl.h -- host/device shared structures of libc3d.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "c3d_dev.h"
+
+// device scalar slots (double unless noted)
+enum {
+  SC_PRESG = 0, SC_PRESG3, SC_DMPRESG, SC_DMPRESG3, SC_FLOW1, SC_FLOW3,
+  SC_WSM1, SC_WSM2, SC_WSM3, SC_TAOW,
+  SC_DIVMAX,  // stored as uint64 bit pattern (atomicMax on non-negative doubles)
+  SC_CFLM,    // same
+  SC_CHK1, SC_CHK3,
+  SC_COUNT = 16
+};
+
+// metric arrays on the device, indexed by the reference's J (0..N2+1)
+struct Metrics {
+  const double *y, *dy, *h, *hp, *hc, *hm, *dyp, *dyc, *dym, *fjpa, *fjmu, *fjmv;
+  const double *pmj, *pcj, *ppj;
+  const double *ak1;  // AK1(I), I = 1..N1M  (index I)
+  const double *ak3;  // AK3(K), K = 1..N3M/2+1
+  // TDMAI chains of the ADI y-sweeps (component c = 0,1,2): A(J), BET(J), GAM(J)
+  const double *ya[3], *ybet[3], *ygam[3];
+};
+
+struct Scal {
+  double re, rinv, dt, dx1, dx1q, dx3, dx3q, aly, vol;
+  int ifwall;
+  int files;  // FILES != 0
+};
+
+struct Ops {
+  LineSolve s6, s10, s22, adix, adiz;
+  TileShape tx, tz;  // x-line and z-line tile shapes
+  double adix_a, adix_b, adiz_a, adiz_b;
+};
+
+struct FFTPlan {
+  int nx, nz;
+  int radx[12], nradx, radz[12], nradz;
+  const double2 *twxf, *twxb, *twzf, *twzb;  // exp(-+2 pi i m/n)
+  int tlx;  // lines per x-FFT tile
+  int tlz;  // kx columns per z-FFT tile
+  int tly;  // modes per y-solve tile
+};
+
+struct Fields {
+  double *u, *v, *w, *p;
+  double *hh[3], *hhp[3];
+  double *ruh[3];
+  double *uh[3];
+  double *vv1, *ww1, *uw, *vv3;
+  double *rdp, *dp;
+  double *sgs;
+  double *div;
+  double *ubc_u, *ubc_w;  // top ghost values U(I,N2,K,1), U(I,N2,K,3) from BCOND (NX*NZ)
+  double *tauw, *taub;    // bottom-wall stresses TAUW(I,K,1), TAUB(I,K,1) (NX*NZ)
+  double2 *spec;          // Poisson spectrum [jm][k][kx], NXH*NZ*N2M
+  double *pgam;           // Poisson Thomas scratch, NXH*NZ*N2M
+  double *sc;             // device scalars (SC_*)
+  double *red;            // reduction partials
+  int nred;               // number of partial slots
+};
+
+struct Ctx {
+  Geo g;
+  Metrics m;
+  Scal s;
+  Ops o;
+  FFTPlan f;
+  Fields d;
+  hipStream_t st;
+};
+
+// host launchers (c3d_kernels.hip)
+namespace c3dk {
+void bcond(const Ctx& c);
+void rhs(const Ctx& c);
+void getuh(const Ctx& c);
+void rhsdp(const Ctx& c);
+void getdp(const Ctx& c);
+void upcalc(const Ctx& c, bool fuse_checks);
+void divcheck(const Ctx& c);
+void cfl(const Ctx& c);
+void chkmf(const Ctx& c, bool reference_sums);
+void sgscs(const Ctx& c);
+void wallmodel(const Ctx& c);
+void apply_op(const Ctx& c, int op, int dir, const double* in, double* out);
+void zero(const Ctx& c, double* p, long long n);
+}  // namespace c3dk

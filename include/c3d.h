@@ -1,0 +1,130 @@
+/*
+ * c3d.h -- C-ABI of the MI355X-native compact3d hot-path library (libc3d.so).
+ *
+ * The reference (compact3d.f) has no plugin or operator API: its boundary is
+ * the set of Fortran subroutines the time loop calls (MAIN, compact3d.f:101-172)
+ * and the COMMON blocks they share.  Each entry point below replaces one of
+ * them; the project's Fortran interface module (compact3d_amd/fortran/) binds
+ * them through ISO_C_BINDING under the reference's subroutine names.
+ *
+ * Conventions
+ *  - Status codes: 0 = success, < 0 = error; the message is available from
+ *    c3d_last_error().  No exception or longjmp crosses the ABI.
+ *  - Host arrays use the reference's Fortran layouts with M1=N1, M2=N2,
+ *    M3=N3 (compact3d.f:55-60):
+ *      U, HH  (0:N1, 0:N2, 0:N3, 3)   column-major, I fastest
+ *      SGSVIS (0:N1, 0:N2, 0:N3)
+ *      P      (1:N1, 1:N2, 1:N3)
+ *    The state lives in HBM between calls; upload/download are the only
+ *    host<->device copies.
+ *  - One host thread per handle.  Handles are independent (an ensemble or a
+ *    server keeps many).
+ */
+#ifndef C3D_H
+#define C3D_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct c3d_handle c3d_handle;
+
+enum c3d_precision {
+  C3D_FP64 = 0,  /* every operator in FP64 */
+  C3D_MIXED = 1  /* FP32 operators, FP64 state / projection / reductions */
+};
+
+/* Run configuration.  Mirrors what SETUP reads from parame.ter and what MESH,
+ * INDICES, INIWAVE and METRICPOISSON derive from it (compact3d.f:192-494,
+ * 9182-9263). */
+typedef struct c3d_config {
+  int n1, n2, n3;     /* N1, N2, N3 (= M1, M2, M3 of PARAM.H); cells N1-1 x N2-1 x N3-1 */
+  double re;          /* RE = 1/nu in the reference's units */
+  double alx, aly, alz; /* box; ALY = 2 full channel, 1 half channel */
+  double dt;          /* DT */
+  double files;       /* SGS model switch FILES: 0 none (DNS), 2 Smagorinsky */
+  double wmodel;      /* WMODEL: 0 resolved walls, 1 instantaneous log-law wall model */
+  double cks;         /* tanh stretching of y (the reference hard-codes 2.5, :338) */
+  const double *y;    /* optional Y(0:N2) to use instead of the tanh grid, or NULL */
+  int precision;      /* enum c3d_precision */
+  int device;         /* HIP device ordinal */
+  void *stream;       /* hipStream_t to run on, or NULL for the library's own */
+} c3d_config;
+
+/* SETUP/MESH/INDICES/INIWAVE/METRICPOISSON (compact3d.f:192-299, 302-494,
+ * 9182-9263): allocates the resident state in HBM and precomputes metrics,
+ * line-solve factors and FFT twiddles on the device. */
+int c3d_create(const c3d_config *cfg, c3d_handle **out);
+/* frees the handle and all device memory */
+int c3d_destroy(c3d_handle *h);
+/* last error message of this handle (or of the last failed c3d_create) */
+const char *c3d_last_error(const c3d_handle *h);
+
+/* READUP (compact3d.f:729-749) followed by the in-memory state of MAIN:
+ * copies U, P, HH (AB2 history), SGSVIS and PRESG/PRESG3 to the device.
+ * HH and SGSVIS may be NULL (zero). */
+int c3d_upload_state(c3d_handle *h, const double *U, const double *P, const double *HH,
+                     const double *SGSVIS, double presg, double presg3);
+/* copies the resident state back (any pointer may be NULL) */
+int c3d_download_state(c3d_handle *h, double *U, double *P, double *HH, double *SGSVIS,
+                       double *presg, double *presg3);
+
+/* GETUP(U,P,HH,SS,NTIME,PRESG,PRESG3,SGSVIS) (compact3d.f:879-960): one full
+ * time step on the resident state. */
+int c3d_getup(c3d_handle *h, int ntime);
+/* GETUP + DIVCHECK + CFL as MAIN calls them after the step
+ * (compact3d.f:130-138); divmax / cflm may be NULL. */
+int c3d_step(c3d_handle *h, int ntime, double *divmax, double *cflm);
+
+/* Phase entry points of GETUP, for operator-level parity.  They act on the
+ * resident state exactly as the named reference routines do. */
+int c3d_bcond(c3d_handle *h);     /* BCOND  compact3d.f:963-1001 */
+int c3d_rhs(c3d_handle *h);       /* RHS1/RHS2/RHS3 compact3d.f:6063, 6784, 7399 (after the
+                                     GETUP prologue :895-923 and BCOND) */
+int c3d_getuh(c3d_handle *h);     /* GETUH1/2/3 compact3d.f:8218, 8344, 8478 */
+int c3d_dpcalc(c3d_handle *h);    /* DPCALC = RHSDP + GETDP compact3d.f:1037-1049 */
+int c3d_upcalc(c3d_handle *h);    /* UPCALC compact3d.f:1052-1238 */
+int c3d_sgs(c3d_handle *h);       /* SGSCS compact3d.f:1413-1488 */
+int c3d_wall_model(c3d_handle *h); /* WALL_MODEL -> WALLMODEL_INSTANT compact3d.f:1952, 2319 */
+
+/* DIVCHECK(U,TIME,DIVMAX,NTIME) compact3d.f:9049 */
+int c3d_divcheck(c3d_handle *h, double *divmax);
+/* CFL(U,CFLM) compact3d.f:9138 */
+int c3d_cfl(c3d_handle *h, double *cflm);
+/* CHKMF(U,TIME) compact3d.f:9463 (reproduces its sums over U(1,J,K,1) and
+ * U(I,J,1,3), :9484 and :9495) */
+int c3d_chkmf(c3d_handle *h, double *flow1, double *flow3);
+/* the flow rates CHKMF intends: sum over all cells */
+int c3d_flowrate(c3d_handle *h, double *flow1, double *flow3);
+
+/* Intermediate fields of the last step, for operator-level parity. */
+enum c3d_field {
+  C3D_F_RUH1 = 1, C3D_F_RUH2, C3D_F_RUH3, /* U-like single component */
+  C3D_F_UH1, C3D_F_UH2, C3D_F_UH3,
+  C3D_F_RDP, C3D_F_DP,                    /* P-like */
+  C3D_F_HH1, C3D_F_HH2, C3D_F_HH3,        /* current HH, U-like single component */
+  C3D_F_TAUW, C3D_F_TAUB                  /* (0:N1,0:N3,2) */
+};
+/* copies one device field into a host array in its Fortran layout */
+int c3d_get_field(c3d_handle *h, int which, double *out);
+/* device scalars: 0 PRESG 1 PRESG3 2 DMPRESG 3 DMPRESG3 4 FLOW1 5 FLOW3
+ * 6 WSM(1) 7 WSM(2) 8 WSM(3) */
+int c3d_get_scalar(c3d_handle *h, int which, double *out);
+int c3d_set_scalar(c3d_handle *h, int which, double value);
+int c3d_set_dt(c3d_handle *h, double dt);
+
+/* Operator test hook: applies one compact operator (CTDMA11/CTDMA33 with the
+ * reference's RHS builders, e.g. compact3d.f:6146-6151, 6200-6204, 6232-6236)
+ * along x (dir 1) or z (dir 3) to a host P-like array in (1:N1,1:N2,1:N3)
+ * layout, on every line J=1..N2-1.
+ * op: 0 D2 (b=10), 1 D1 centre->face (b=22), 2 D1 face->centre (b=22),
+ *     3 I centre->face (b=6), 4 I face->centre (b=6) */
+int c3d_apply_op(c3d_handle *h, int op, int dir, const double *in, double *out);
+
+/* device memory held by one handle, bytes (for admission by footprint) */
+long long c3d_footprint_bytes(const c3d_config *cfg);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
