@@ -72,6 +72,11 @@ def lib():
         L.c3d_set_scalar.argtypes = [C.c_void_p, C.c_int, C.c_double]
         L.c3d_set_dt.argtypes = [C.c_void_p, C.c_double]
         L.c3d_apply_op.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp]
+        L.c3d_get_mesh.argtypes = [C.c_void_p, _dp, _dp]
+        L.c3d_timing.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.c3d_kernel_times.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_int)]
+        L.c3d_kernel_name.restype = C.c_char_p
+        L.c3d_kernel_name.argtypes = [C.c_int]
         _lib = L
     return _lib
 
@@ -225,6 +230,25 @@ class Channel:
 
     def set_dt(self, dt):
         self._ck(self.L.c3d_set_dt(self.h, float(dt)), "set_dt")
+
+    def mesh(self):
+        """{'Y': Y(0:N2), 'DX1', 'DX3', 'ALY', 'VOL'} as the library built them"""
+        y = np.zeros(self.n2 + 1)
+        sc = np.zeros(4)
+        self._ck(self.L.c3d_get_mesh(self.h, _p(y), _p(sc)), "get_mesh")
+        return dict(Y=y, DX1=sc[0], DX3=sc[1], ALY=sc[2], VOL=sc[3])
+
+    def timing(self, on=True, capacity=4096):
+        """bracket every following kernel launch with HIP events on the
+        handle's stream (per-kernel durations, c3d_kernel_times)"""
+        self._ck(self.L.c3d_timing(self.h, 1 if on else 0, int(capacity)), "timing")
+
+    def kernel_times(self):
+        """{kernel name: (total ms, launches)} since timing() was enabled"""
+        ms = (C.c_double * 32)()
+        cnt = (C.c_int * 32)()
+        self._ck(self.L.c3d_kernel_times(self.h, ms, cnt), "kernel_times")
+        return {self.L.c3d_kernel_name(q).decode(): (ms[q], cnt[q]) for q in range(32) if cnt[q] > 0}
 
     def apply_op(self, op, direction, field):
         inp = np.asfortranarray(field, dtype=np.float64)

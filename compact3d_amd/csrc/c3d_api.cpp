@@ -29,6 +29,7 @@ struct c3d_handle {
   double* dtmp = nullptr;       // scratch field for apply_op
   double* dtmp2 = nullptr;
   std::vector<double> stage;    // host staging buffer
+  KTimer kt;
 };
 
 static std::string g_create_err;
@@ -326,6 +327,7 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   if (cf.re <= 0 || cf.dt <= 0 || cf.alx <= 0 || cf.alz <= 0) return bad("RE, DT, ALX, ALZ must be > 0");
   if (hipSetDevice(cf.device) != hipSuccess) return bad("hipSetDevice failed");
   Ctx& c = h->c;
+  c.kt = &h->kt;
   Geo& g = c.g;
   g.nx = cf.n1 - 1;
   g.nz = cf.n3 - 1;
@@ -445,9 +447,21 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   return 0;
 }
 
+static void kt_free(c3d_handle* h) {
+  KTimer& t = h->kt;
+  for (int q = 0; q < 2 * t.cap; q++) (void)hipEventDestroy(t.ev[q]);
+  delete[] t.ev;
+  delete[] t.id;
+  t.ev = nullptr;
+  t.id = nullptr;
+  t.cap = t.used = 0;
+  t.on = false;
+}
+
 extern "C" int c3d_destroy(c3d_handle* h) {
   if (!h) return 0;
   (void)hipStreamSynchronize(h->c.st);
+  kt_free(h);
   for (void* p : h->allocs) (void)hipFree(p);
   if (h->own_stream) (void)hipStreamDestroy(h->c.st);
   delete h;
@@ -731,6 +745,19 @@ extern "C" int c3d_set_dt(c3d_handle* h, double dt) {
   return upload_dt_terms(h);
 }
 
+extern "C" int c3d_get_mesh(c3d_handle* h, double* y, double* scal4) {
+  if (!h) return -1;
+  if (y)
+    for (int j = 0; j <= h->cfg.n2; j++) y[j] = h->Y[j];
+  if (scal4) {
+    scal4[0] = h->c.s.dx1;
+    scal4[1] = h->c.s.dx3;
+    scal4[2] = h->c.s.aly;
+    scal4[3] = h->c.s.vol;
+  }
+  return 0;
+}
+
 extern "C" int c3d_apply_op(c3d_handle* h, int op, int dir, const double* in, double* out) {
   if (!h || !in || !out) return -1;
   if (op < 0 || op > 4 || (dir != 1 && dir != 3)) return fail(h, "bad op/dir");
@@ -738,4 +765,48 @@ extern "C" int c3d_apply_op(c3d_handle* h, int op, int dir, const double* in, do
   c3dk::apply_op(h->c, op, dir, h->dtmp, h->dtmp2);
   if (launched(h)) return -1;
   return from_dev(h, out, h->dtmp2, LAY_P);
+}
+
+static const char* kid_names[C3D_NKID] = {
+    "bcond", "rhs_x", "rhs_z", "rhs_x2", "rhs_y", "adi_y", "adi_x", "adi_z", "rhsdp_x", "rhsdp_z",
+    "fft_x", "fft_z", "pois_y", "fft_xinv", "flow_red", "flow_fin", "upd_x", "ghosts", "upd_z",
+    "div_x", "div_z", "cfl", "chkmf", "zero", "apply_op", "sgs", "wall", "", "", "", "", ""};
+
+extern "C" const char* c3d_kernel_name(int id) {
+  if (id < 0 || id >= C3D_NKID) return "";
+  return kid_names[id];
+}
+
+extern "C" int c3d_timing(c3d_handle* h, int on, int capacity) {
+  if (!h) return -1;
+  HIPCHK(hipStreamSynchronize(h->c.st));
+  KTimer& t = h->kt;
+  if (on && capacity > t.cap) {
+    kt_free(h);
+    t.ev = new hipEvent_t[2 * (size_t)capacity];
+    t.id = new int[capacity];
+    for (int q = 0; q < 2 * capacity; q++) HIPCHK(hipEventCreate(&t.ev[q]));
+    t.cap = capacity;
+  }
+  t.used = 0;
+  t.on = on != 0;
+  return 0;
+}
+
+extern "C" int c3d_kernel_times(c3d_handle* h, double* ms, int* count) {
+  if (!h) return -1;
+  HIPCHK(hipStreamSynchronize(h->c.st));
+  KTimer& t = h->kt;
+  for (int q = 0; q < C3D_NKID; q++) {
+    if (ms) ms[q] = 0.0;
+    if (count) count[q] = 0;
+  }
+  for (int s2 = 0; s2 < t.used; s2++) {
+    float e = 0.f;
+    HIPCHK(hipEventElapsedTime(&e, t.ev[2 * s2], t.ev[2 * s2 + 1]));
+    if (ms) ms[t.id[s2]] += e;
+    if (count) count[t.id[s2]] += 1;
+  }
+  t.used = 0;
+  return 0;
 }

@@ -62,6 +62,24 @@ struct Fields {
   int nred;               // number of partial slots
 };
 
+// kernel ids for the per-kernel event timer
+enum {
+  KID_BCOND = 0, KID_RHS_X, KID_RHS_Z, KID_RHS_X2, KID_RHS_Y, KID_ADI_Y, KID_ADI_X, KID_ADI_Z,
+  KID_RHSDP_X, KID_RHSDP_Z, KID_FFT_X, KID_FFT_Z, KID_POIS_Y, KID_FFT_XINV, KID_FLOW_RED,
+  KID_FLOW_FIN, KID_UPD_X, KID_GHOSTS, KID_UPD_Z, KID_DIV_X, KID_DIV_Z, KID_CFL, KID_CHKMF,
+  KID_ZERO, KID_APPLY, KID_SGS, KID_WALL, KID_COUNT = 32
+};
+
+// host-side recorder: when `on`, every launch is bracketed by two events on
+// the launch stream; times are summed per kernel id on request
+struct KTimer {
+  bool on = false;
+  int used = 0;
+  hipEvent_t* ev = nullptr;  // 2*cap events
+  int* id = nullptr;         // kernel id per pair
+  int cap = 0;
+};
+
 struct Ctx {
   Geo g;
   Metrics m;
@@ -70,6 +88,24 @@ struct Ctx {
   FFTPlan f;
   Fields d;
   hipStream_t st;
+  KTimer* kt;
+};
+
+// brackets a launch with timing events when the recorder is on
+struct KScope {
+  const Ctx& c;
+  int slot;
+  KScope(const Ctx& cc, int kid) : c(cc), slot(-1) {
+    KTimer* t = c.kt;
+    if (t && t->on && t->used < t->cap) {
+      slot = t->used++;
+      t->id[slot] = kid;
+      (void)hipEventRecord(t->ev[2 * slot], c.st);
+    }
+  }
+  ~KScope() {
+    if (slot >= 0) (void)hipEventRecord(c.kt->ev[2 * slot + 1], c.st);
+  }
 };
 
 // host launchers (c3d_kernels.hip)

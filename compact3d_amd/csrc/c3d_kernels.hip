@@ -1298,39 +1298,39 @@ static int zgrid(const Ctx& c) { return ((c.g.nx + c.o.tz.nl - 1) / c.o.tz.nl) *
 static int pgrid(long long n) { return (int)((n + C3D_NT - 1) / C3D_NT); }
 
 void zero(const Ctx& c, double* p, long long n) {
-  hipLaunchKernelGGL(k_zero, dim3(pgrid(n)), dim3(C3D_NT), 0, c.st, p, n);
+  { KScope ks_(c, KID_ZERO); hipLaunchKernelGGL(k_zero, dim3(pgrid(n)), dim3(C3D_NT), 0, c.st, p, n); }
 }
 
 void bcond(const Ctx& c) {
-  hipLaunchKernelGGL(k_bcond, dim3(pgrid((long long)c.g.nx * c.g.nz)), dim3(C3D_NT), 0, c.st, c);
+  { KScope ks_(c, KID_BCOND); hipLaunchKernelGGL(k_bcond, dim3(pgrid((long long)c.g.nx * c.g.nz)), dim3(C3D_NT), 0, c.st, c); }
 }
 
 void rhs(const Ctx& c) {
-  hipLaunchKernelGGL(k_rhs_x, dim3(xgrid(c)), dim3(C3D_NT), tile_lds(c.o.tx, 2), c.st, c);
-  hipLaunchKernelGGL(k_rhs_z, dim3(zgrid(c)), dim3(C3D_NT), tile_lds(c.o.tz, 2), c.st, c);
-  hipLaunchKernelGGL(k_rhs_x2, dim3(xgrid(c)), dim3(C3D_NT), tile_lds(c.o.tx, 1), c.st, c);
-  hipLaunchKernelGGL(k_rhs_y, dim3(pgrid((long long)c.g.nx * c.g.n2m * c.g.nz)), dim3(C3D_NT), 0,
-                     c.st, c);
+  { KScope ks_(c, KID_RHS_X); hipLaunchKernelGGL(k_rhs_x, dim3(xgrid(c)), dim3(C3D_NT), tile_lds(c.o.tx, 2), c.st, c); }
+  { KScope ks_(c, KID_RHS_Z); hipLaunchKernelGGL(k_rhs_z, dim3(zgrid(c)), dim3(C3D_NT), tile_lds(c.o.tz, 2), c.st, c); }
+  { KScope ks_(c, KID_RHS_X2); hipLaunchKernelGGL(k_rhs_x2, dim3(xgrid(c)), dim3(C3D_NT), tile_lds(c.o.tx, 1), c.st, c); }
+  { KScope ks_(c, KID_RHS_Y); hipLaunchKernelGGL(k_rhs_y, dim3(pgrid((long long)c.g.nx * c.g.n2m * c.g.nz)), dim3(C3D_NT), 0,
+                     c.st, c); }
 }
 
 void getuh(const Ctx& c) {
   int tl = 32;  // i-columns per y tile; (N2M+1)*tl doubles of LDS <= 64 KB
   while (tl > 1 && sizeof(double) * (size_t)(c.g.n2m + 1) * tl > 65536) tl /= 2;
   const int nit = (c.g.nx + tl - 1) / tl;
-  hipLaunchKernelGGL(k_adi_y, dim3(nit * c.g.nz, 3), dim3(C3D_NT),
-                     sizeof(double) * (size_t)(c.g.n2m + 1) * tl, c.st, c, tl);
+  { KScope ks_(c, KID_ADI_Y); hipLaunchKernelGGL(k_adi_y, dim3(nit * c.g.nz, 3), dim3(C3D_NT),
+                     sizeof(double) * (size_t)(c.g.n2m + 1) * tl, c.st, c, tl); }
   // GETUH1/2: y, x, z;  GETUH3: y, z, x (compact3d.f:8508-8607)
   CompList x12{2, {0, 1, 0}, {0, 0, 0}};
-  hipLaunchKernelGGL(k_adi_line<1>, dim3(xgrid(c), 2), dim3(C3D_NT), tile_lds(c.o.tx, 1), c.st, c, x12);
+  { KScope ks_(c, KID_ADI_X); hipLaunchKernelGGL(k_adi_line<1>, dim3(xgrid(c), 2), dim3(C3D_NT), tile_lds(c.o.tx, 1), c.st, c, x12); }
   CompList z123{3, {0, 1, 2}, {1, 1, 0}};
-  hipLaunchKernelGGL(k_adi_line<3>, dim3(zgrid(c), 3), dim3(C3D_NT), tile_lds(c.o.tz, 1), c.st, c, z123);
+  { KScope ks_(c, KID_ADI_Z); hipLaunchKernelGGL(k_adi_line<3>, dim3(zgrid(c), 3), dim3(C3D_NT), tile_lds(c.o.tz, 1), c.st, c, z123); }
   CompList x3{1, {2, 0, 0}, {1, 0, 0}};
-  hipLaunchKernelGGL(k_adi_line<1>, dim3(xgrid(c), 1), dim3(C3D_NT), tile_lds(c.o.tx, 1), c.st, c, x3);
+  { KScope ks_(c, KID_ADI_X); hipLaunchKernelGGL(k_adi_line<1>, dim3(xgrid(c), 1), dim3(C3D_NT), tile_lds(c.o.tx, 1), c.st, c, x3); }
 }
 
 void rhsdp(const Ctx& c) {
-  hipLaunchKernelGGL(k_rhsdp_x, dim3(xgrid(c)), dim3(C3D_NT), tile_lds(c.o.tx, 1), c.st, c);
-  hipLaunchKernelGGL(k_rhsdp_z, dim3(zgrid(c)), dim3(C3D_NT), tile_lds(c.o.tz, 1), c.st, c);
+  { KScope ks_(c, KID_RHSDP_X); hipLaunchKernelGGL(k_rhsdp_x, dim3(xgrid(c)), dim3(C3D_NT), tile_lds(c.o.tx, 1), c.st, c); }
+  { KScope ks_(c, KID_RHSDP_Z); hipLaunchKernelGGL(k_rhsdp_z, dim3(zgrid(c)), dim3(C3D_NT), tile_lds(c.o.tz, 1), c.st, c); }
 }
 
 void getdp(const Ctx& c) {
@@ -1342,45 +1342,45 @@ void getdp(const Ctx& c) {
   const int gz = ((g.nxh + f.tlz - 1) / f.tlz) * g.n2m;
   const int gy = ((g.nxh + f.tly - 1) / f.tly) * g.nz;
   const size_t ldsy = (sizeof(zc) + sizeof(double)) * (size_t)g.n2m * f.tly;
-  hipLaunchKernelGGL(k_fft_x, dim3(gx), dim3(C3D_NT), ldsx, c.st, c);
-  hipLaunchKernelGGL(k_fft_z, dim3(gz), dim3(C3D_NT), ldsz, c.st, c, -1);
-  hipLaunchKernelGGL(k_pois_y, dim3(gy), dim3(C3D_NT), ldsy, c.st, c);
-  hipLaunchKernelGGL(k_fft_z, dim3(gz), dim3(C3D_NT), ldsz, c.st, c, +1);
-  hipLaunchKernelGGL(k_fft_xinv, dim3(gx), dim3(C3D_NT), ldsx, c.st, c);
+  { KScope ks_(c, KID_FFT_X); hipLaunchKernelGGL(k_fft_x, dim3(gx), dim3(C3D_NT), ldsx, c.st, c); }
+  { KScope ks_(c, KID_FFT_Z); hipLaunchKernelGGL(k_fft_z, dim3(gz), dim3(C3D_NT), ldsz, c.st, c, -1); }
+  { KScope ks_(c, KID_POIS_Y); hipLaunchKernelGGL(k_pois_y, dim3(gy), dim3(C3D_NT), ldsy, c.st, c); }
+  { KScope ks_(c, KID_FFT_Z); hipLaunchKernelGGL(k_fft_z, dim3(gz), dim3(C3D_NT), ldsz, c.st, c, +1); }
+  { KScope ks_(c, KID_FFT_XINV); hipLaunchKernelGGL(k_fft_xinv, dim3(gx), dim3(C3D_NT), ldsx, c.st, c); }
 }
 
 void upcalc(const Ctx& c, bool chk) {
   if (chk) zero(c, c.d.sc + SC_DIVMAX, 2);
-  hipLaunchKernelGGL(k_flow_red, dim3(RED_BLOCKS), dim3(C3D_NT), 0, c.st, c);
-  hipLaunchKernelGGL(k_flow_fin, dim3(1), dim3(C3D_NT), 0, c.st, c, 1);
-  hipLaunchKernelGGL(k_upd_x, dim3(xgrid(c)), dim3(C3D_NT), tile_lds(c.o.tx, 2), c.st, c, chk ? 1 : 0);
-  hipLaunchKernelGGL(k_ghosts, dim3(pgrid((long long)c.g.nx * c.g.nz)), dim3(C3D_NT), 0, c.st, c);
-  hipLaunchKernelGGL(k_upd_z, dim3(zgrid(c)), dim3(C3D_NT), tile_lds(c.o.tz, 2), c.st, c, chk ? 1 : 0);
+  { KScope ks_(c, KID_FLOW_RED); hipLaunchKernelGGL(k_flow_red, dim3(RED_BLOCKS), dim3(C3D_NT), 0, c.st, c); }
+  { KScope ks_(c, KID_FLOW_FIN); hipLaunchKernelGGL(k_flow_fin, dim3(1), dim3(C3D_NT), 0, c.st, c, 1); }
+  { KScope ks_(c, KID_UPD_X); hipLaunchKernelGGL(k_upd_x, dim3(xgrid(c)), dim3(C3D_NT), tile_lds(c.o.tx, 2), c.st, c, chk ? 1 : 0); }
+  { KScope ks_(c, KID_GHOSTS); hipLaunchKernelGGL(k_ghosts, dim3(pgrid((long long)c.g.nx * c.g.nz)), dim3(C3D_NT), 0, c.st, c); }
+  { KScope ks_(c, KID_UPD_Z); hipLaunchKernelGGL(k_upd_z, dim3(zgrid(c)), dim3(C3D_NT), tile_lds(c.o.tz, 2), c.st, c, chk ? 1 : 0); }
 }
 
 void divcheck(const Ctx& c) {
   zero(c, c.d.sc + SC_DIVMAX, 1);
-  hipLaunchKernelGGL(k_div_x, dim3(xgrid(c)), dim3(C3D_NT), tile_lds(c.o.tx, 1), c.st, c);
-  hipLaunchKernelGGL(k_div_z, dim3(zgrid(c)), dim3(C3D_NT), tile_lds(c.o.tz, 1), c.st, c);
+  { KScope ks_(c, KID_DIV_X); hipLaunchKernelGGL(k_div_x, dim3(xgrid(c)), dim3(C3D_NT), tile_lds(c.o.tx, 1), c.st, c); }
+  { KScope ks_(c, KID_DIV_Z); hipLaunchKernelGGL(k_div_z, dim3(zgrid(c)), dim3(C3D_NT), tile_lds(c.o.tz, 1), c.st, c); }
 }
 
 void cfl(const Ctx& c) {
   zero(c, c.d.sc + SC_CFLM, 1);
-  hipLaunchKernelGGL(k_cfl, dim3(RED_BLOCKS), dim3(C3D_NT), 0, c.st, c);
+  { KScope ks_(c, KID_CFL); hipLaunchKernelGGL(k_cfl, dim3(RED_BLOCKS), dim3(C3D_NT), 0, c.st, c); }
 }
 
 void chkmf(const Ctx& c, bool ref) {
-  hipLaunchKernelGGL(k_chkmf, dim3(RED_BLOCKS), dim3(C3D_NT), 0, c.st, c, ref ? 1 : 0);
-  hipLaunchKernelGGL(k_chkmf_fin, dim3(1), dim3(C3D_NT), 0, c.st, c);
+  { KScope ks_(c, KID_CHKMF); hipLaunchKernelGGL(k_chkmf, dim3(RED_BLOCKS), dim3(C3D_NT), 0, c.st, c, ref ? 1 : 0); }
+  { KScope ks_(c, KID_CHKMF); hipLaunchKernelGGL(k_chkmf_fin, dim3(1), dim3(C3D_NT), 0, c.st, c); }
 }
 
 void apply_op(const Ctx& c, int op, int dir, const double* in, double* out) {
   if (dir == 1)
-    hipLaunchKernelGGL(k_apply_op, dim3(xgrid(c)), dim3(C3D_NT), tile_lds(c.o.tx, 1), c.st, c, op, dir, in,
-                       out);
+    { KScope ks_(c, KID_APPLY); hipLaunchKernelGGL(k_apply_op, dim3(xgrid(c)), dim3(C3D_NT), tile_lds(c.o.tx, 1), c.st, c, op, dir, in,
+                       out); }
   else
-    hipLaunchKernelGGL(k_apply_op, dim3(zgrid(c)), dim3(C3D_NT), tile_lds(c.o.tz, 1), c.st, c, op, dir, in,
-                       out);
+    { KScope ks_(c, KID_APPLY); hipLaunchKernelGGL(k_apply_op, dim3(zgrid(c)), dim3(C3D_NT), tile_lds(c.o.tz, 1), c.st, c, op, dir, in,
+                       out); }
 }
 
 void sgscs(const Ctx& c) { (void)c; }

@@ -112,6 +112,8 @@ int c3d_get_field(c3d_handle *h, int which, double *out);
 int c3d_get_scalar(c3d_handle *h, int which, double *out);
 int c3d_set_scalar(c3d_handle *h, int which, double value);
 int c3d_set_dt(c3d_handle *h, double dt);
+/* the grid MESH built (compact3d.f:302-443): Y(0:N2) and {DX1, DX3, ALY, VOL} */
+int c3d_get_mesh(c3d_handle *h, double *y, double *scal4);
 
 /* Operator test hook: applies one compact operator (CTDMA11/CTDMA33 with the
  * reference's RHS builders, e.g. compact3d.f:6146-6151, 6200-6204, 6232-6236)
@@ -120,6 +122,17 @@ int c3d_set_dt(c3d_handle *h, double dt);
  * op: 0 D2 (b=10), 1 D1 centre->face (b=22), 2 D1 face->centre (b=22),
  *     3 I centre->face (b=6), 4 I face->centre (b=6) */
 int c3d_apply_op(c3d_handle *h, int op, int dir, const double *in, double *out);
+
+/* Per-kernel timing (no counterpart in the reference; it replaces the
+ * reference's wall-clock print of MAIN, compact3d.f:101-103, 160-163).
+ * c3d_timing(h, 1, cap) brackets every following launch with two HIP events
+ * on the handle's stream (up to cap launches); c3d_kernel_times()
+ * synchronises, returns per-kernel-id summed milliseconds and launch counts
+ * (arrays of C3D_NKID entries) and resets the recorder. */
+#define C3D_NKID 32
+int c3d_timing(c3d_handle *h, int on, int capacity);
+int c3d_kernel_times(c3d_handle *h, double *ms, int *count);
+const char *c3d_kernel_name(int id);
 
 /* device memory held by one handle, bytes (for admission by footprint) */
 long long c3d_footprint_bytes(const c3d_config *cfg);

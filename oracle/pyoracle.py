@@ -185,6 +185,11 @@ class Oracle:
     def sgscs(self, U, SGSVIS):
         self.L.c3o_sgscs(self.h, _p(U), _p(SGSVIS))
 
+    def strain(self, U, j):
+        SR = np.zeros((self.n1, self.n3, 6), order="F")
+        self.L.c3o_strain(self.h, _p(U), _p(SR), int(j))
+        return SR
+
     def wallmodel_instant(self, U):
         self.L.c3o_wallmodel_instant(self.h, _p(U))
 
@@ -207,10 +212,12 @@ class Oracle:
     @classmethod
     def dft2d(cls, a, sign):
         """complex 2-D DFT of a (n1, n3) column-major complex array"""
-        z = np.asfortranarray(a, dtype=np.complex128).copy(order="F")
-        buf = z.view(np.float64)
-        cls.lib().c3o_dft2d(a.shape[0], a.shape[1], buf.ctypes.data_as(_dp), sign)
-        return z
+        n1, n3 = a.shape
+        flat = np.asarray(a, dtype=np.complex128).reshape(-1, order="F")
+        buf = np.empty(2 * flat.size)
+        buf[0::2], buf[1::2] = flat.real, flat.imag
+        cls.lib().c3o_dft2d(n1, n3, buf.ctypes.data_as(_dp), sign)
+        return (buf[0::2] + 1j * buf[1::2]).reshape((n1, n3), order="F")
 
     @staticmethod
     def set_threads(n):
@@ -227,7 +234,25 @@ class RefLib:
     @classmethod
     def lib(cls):
         if cls._lib is None:
-            cls._lib = C.CDLL(REF_SO)
+            L = C.CDLL(REF_SO)
+            # zero the stack below the caller before every reference routine
+            # (see ref_stack.c): reads outside local arrays then return 0
+            zs = L.refh_zerostack_
+
+            class _Z:
+                def __init__(self, lib):
+                    self._l = lib
+
+                def __getattr__(self, name):
+                    f = getattr(self._l, name)
+                    if not name.endswith("_") or name.startswith("refh_"):
+                        return f
+
+                    def call(*a):
+                        zs()
+                        return f(*a)
+                    return call
+            cls._lib = _Z(L)
         return cls._lib
 
     @staticmethod

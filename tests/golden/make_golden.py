@@ -1,0 +1,128 @@
+#!/usr/bin/env python3
+"""Generate golden vectors from the reference's own subroutines.
+
+Runs the hot-path subroutines of compact3d.f, compiled unmodified into
+oracle/_ref/libc3dref.so by `make -C oracle ref` (the reference grid of its
+PARAM.H, 32 x 33 x 32), on seeded inputs, and stores a deterministic subsample
+of every output plus full-field checksums in tests/golden/ref_<case>.npz.
+
+GETDP needs FFTW, which this image does not have, so the reference cannot run
+it; the fixture's UPCALC input DP is the oracle's GETDP of the reference's RDP
+(GETDP itself is pinned by closed forms in tests/test_oracle_math.py).
+
+Usage:  python tests/golden/make_golden.py      (build container only)
+"""
+import math
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+from cases import inputs  # noqa: E402
+from pyoracle import Oracle, RefLib  # noqa: E402
+
+STRIDE = 13
+
+CASES = {
+    "dns": dict(re=4200.0, alx=2 * math.pi, aly=2.0, alz=math.pi, dt=1e-3, files=0.0, wmodel=0.0),
+    "les": dict(re=5200.0, alx=2 * math.pi, aly=2.0, alz=math.pi, dt=5e-4, files=2.0, wmodel=1.0),
+}
+
+
+def sub(a):
+    return np.ascontiguousarray(a, dtype=np.float64).reshape(-1, order="F")[::STRIDE].copy()
+
+
+def summ(a):
+    a = np.asarray(a, dtype=np.float64)
+    return np.array([a.sum(), (a * a).sum(), np.abs(a).max()])
+
+
+def run_case(name, cfg):
+    ref = RefLib(**cfg)
+    n1, n2, n3 = ref.n1, ref.n2, ref.n3
+    orc = Oracle(n1, n2, n3, cks=2.5, **cfg)  # inputs only; outputs come from ref
+    U, P, HH2 = inputs(orc, seed=7)
+    out = {}
+    presg, presg3 = 0.003, -0.001
+    S = np.zeros(U.shape[:3], order="F")
+    if cfg["files"] == 2.0:
+        ref.set_wsm([2.5e-3, 0.0, 4.0e-4])
+        ref.sgscs(U, S)
+        out["SGSVIS"] = S
+        out["STRAIN3"] = ref.strain(U, 3)
+    if cfg["files"] != 0.0 and cfg["wmodel"] == 1.0:
+        ref.wallmodel_instant(U, S)
+        ref.set_ifwall(1)
+        tw, tb = ref.get_tau()
+        out["TAUW"], out["TAUB"] = tw, tb
+        out["WSM"] = ref.get_wsm()
+    ref.bcond(U, S)
+    out["U_BCOND"] = U.copy(order="F")
+    if cfg["files"] != 0.0:
+        out["SGSVIS_BCOND"] = S.copy(order="F")
+    HH = np.zeros_like(U)
+    RUH = [np.zeros(U.shape[:3], order="F") for _ in range(3)]
+    for c in (1, 2, 3):
+        ref.rhs(c, U, P, HH, HH2, RUH[c - 1], presg if c == 1 else presg3, S)
+        out[f"RUH{c}"] = RUH[c - 1]
+    out["HH"] = HH
+    UH = np.zeros_like(U)
+    for c in (1, 2, 3):
+        ref.getuh(c, U, UH, RUH[c - 1], S)
+    out["UH"] = UH
+    RDP = np.zeros((n1, n2, n3), order="F")
+    ref.rhsdp(RDP, UH, U)
+    out["RDP"] = RDP
+    DP = np.zeros_like(RDP)
+    orc.getdp(DP, RDP)
+    Pn = P.copy(order="F")
+    pg, pg3 = ref.upcalc(U, Pn, UH, DP, presg, presg3)
+    out["U_UPCALC"] = U
+    out["P_UPCALC"] = Pn
+    out["PRESG"] = np.array([pg, pg3])
+    out["DIVMAX"] = np.array([ref.divcheck(U)])
+    out["CFL"] = np.array([ref.cfl(U)])
+    # line solvers and filters
+    rng = np.random.default_rng(99)
+    d = rng.uniform(-1, 1, n1 - 1)
+    for b in (6.0, 10.0, 22.0):
+        out[f"CTDMA11_b{int(b)}"] = ref.ctdma11(b, d)
+    out["CTDMA_RHS"] = d
+    a, bb, c3, r = (rng.uniform(0.2, 1, n2 - 1), rng.uniform(-4, -3, n2 - 1), rng.uniform(0.2, 1, n2 - 1),
+                    rng.uniform(-1, 1, n2 - 1))
+    out["TDMAP_IN"] = np.stack([a, bb, c3, r])
+    out["TDMAP"] = ref.tdmap(a, bb, c3, r)
+    F = np.zeros((n1, n3, 2), order="F")
+    F[: n1 - 1, : n3 - 1, :] = rng.uniform(-1, 1, (n1 - 1, n3 - 1, 2))
+    out["FILTER_IN"] = F.copy(order="F")
+    F1 = F.copy(order="F")
+    ref.filter(F1, four=False)
+    out["FILTER"] = F1
+    F4 = F.copy(order="F")
+    ref.filter(F4, four=True)
+    out["FILTER4"] = F4
+    store = {"config": np.array([n1, n2, n3, cfg["re"], cfg["alx"], cfg["aly"], cfg["alz"], cfg["dt"],
+                                 cfg["files"], cfg["wmodel"], 7, presg, presg3, STRIDE])}
+    small = {"PRESG", "DIVMAX", "CFL", "WSM", "CTDMA_RHS", "TDMAP_IN", "TDMAP", "FILTER_IN", "FILTER",
+             "FILTER4"} | {k for k in out if k.startswith("CTDMA11")}
+    for k, v in out.items():
+        if k in small:
+            store[k] = np.asarray(v, dtype=np.float64)
+        else:
+            store[k + "__sub"] = sub(v)
+            store[k + "__sum"] = summ(v)
+    path = os.path.join(HERE, f"ref_{name}.npz")
+    np.savez_compressed(path, **store)
+    print(path, os.path.getsize(path), "bytes")
+
+
+if __name__ == "__main__":
+    for name, cfg in CASES.items():
+        run_case(name, cfg)

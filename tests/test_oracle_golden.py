@@ -1,0 +1,112 @@
+"""The oracle against golden vectors produced by the reference's own
+subroutines (tests/golden/make_golden.py; compact3d.f compiled unmodified).
+
+Every phase the reference can run without FFTW is compared: BCOND, RHS1-3,
+GETUH1-3, RHSDP, UPCALC, DIVCHECK, CFL, SGSCS/STRAIN, WALLMODEL_INSTANT,
+CTDMA11, TDMAP, FILTER, FILTER4.  The oracle follows the reference's operation
+order, so the expected agreement is exact; the bound allows 4 ulps."""
+import os
+
+import numpy as np
+import pytest
+
+from cases import inputs
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TOL = 1e-15
+
+
+def _load(name):
+    return dict(np.load(os.path.join(HERE, "golden", f"ref_{name}.npz")))
+
+
+def _sub(a, stride):
+    return np.ascontiguousarray(a, dtype=np.float64).reshape(-1, order="F")[::stride]
+
+
+def _check(fx, key, arr, stride):
+    ref = fx[key + "__sub"]
+    got = _sub(arr, stride)
+    scale = max(np.max(np.abs(ref)), 1e-300)
+    err = np.max(np.abs(ref - got)) / scale
+    s = np.array([arr.sum(), (arr * arr).sum(), np.abs(arr).max()])
+    es = np.max(np.abs(s - fx[key + "__sum"]) / np.maximum(np.abs(fx[key + "__sum"]), 1e-300))
+    assert err <= TOL, (key, err)
+    assert es <= 1e-12, (key, es)
+
+
+@pytest.mark.parametrize("name", ["dns", "les"])
+def test_oracle_matches_reference_fixture(name):
+    from pyoracle import Oracle
+    fx = _load(name)
+    c = fx["config"]
+    n1, n2, n3 = int(c[0]), int(c[1]), int(c[2])
+    re, alx, aly, alz, dt, files, wmodel = (float(x) for x in c[3:10])
+    seed, presg, presg3, stride = int(c[10]), float(c[11]), float(c[12]), int(c[13])
+    o = Oracle(n1, n2, n3, re, alx, aly, alz, dt, files, wmodel, 2.5)
+    U, P, HH2 = inputs(o, seed=seed)
+    S = o.zeros_u(0)
+    if files == 2.0:
+        o.set_wsm([2.5e-3, 0.0, 4.0e-4])
+        o.sgscs(U, S)
+        _check(fx, "SGSVIS", S, stride)
+    if files != 0.0 and wmodel == 1.0:
+        o.wallmodel_instant(U)
+        o.set_ifwall(1)
+        tw, tb = o.get_tau()
+        _check(fx, "TAUW", tw, stride)
+        _check(fx, "TAUB", tb, stride)
+        assert abs(o.get_wsm()[0] - fx["WSM"][0]) <= TOL * abs(fx["WSM"][0])
+    o.bcond(U, S)
+    _check(fx, "U_BCOND", U, stride)
+    HH = o.zeros_u()
+    RUH = [o.zeros_u(0) for _ in range(3)]
+    for comp in (1, 2, 3):
+        o.rhs(comp, U, P, HH, HH2, RUH[comp - 1], presg if comp == 1 else presg3, S)
+        _check(fx, f"RUH{comp}", RUH[comp - 1], stride)
+    _check(fx, "HH", HH, stride)
+    UH = o.zeros_u()
+    for comp in (1, 2, 3):
+        o.getuh(comp, U, UH, RUH[comp - 1])
+    _check(fx, "UH", UH, stride)
+    RDP = o.zeros_p()
+    o.rhsdp(RDP, UH)
+    _check(fx, "RDP", RDP, stride)
+    DP = o.zeros_p()
+    o.getdp(DP, RDP)
+    Pn = P.copy(order="F")
+    pg, pg3 = o.upcalc(U, Pn, UH, DP, presg, presg3)
+    _check(fx, "U_UPCALC", U, stride)
+    _check(fx, "P_UPCALC", Pn, stride)
+    assert np.max(np.abs(np.array([pg, pg3]) - fx["PRESG"])) <= TOL * np.max(np.abs(fx["PRESG"]))
+    assert o.divcheck(U) == pytest.approx(fx["DIVMAX"][0], rel=1e-12, abs=1e-15)
+    assert o.cfl(U) == pytest.approx(fx["CFL"][0], rel=1e-15)
+
+
+def test_line_solvers_and_filters_match_reference():
+    from pyoracle import Oracle
+    fx = _load("dns")
+    d = fx["CTDMA_RHS"]
+    for b in (6, 10, 22):
+        x = Oracle.ctdma(float(b), d)
+        assert np.max(np.abs(x - fx[f"CTDMA11_b{b}"])) <= TOL * np.max(np.abs(x))
+    c = fx["config"]
+    n1, n2, n3 = int(c[0]), int(c[1]), int(c[2])
+    o = Oracle(n1, n2, n3, 4200.0, 2 * np.pi, 2.0, np.pi, 1e-3)
+    F = fx["FILTER_IN"].copy(order="F")
+    o.filter(F, four=False)
+    assert np.max(np.abs(F - fx["FILTER"])) <= TOL
+    F = fx["FILTER_IN"].copy(order="F")
+    o.filter(F, four=True)
+    assert np.max(np.abs(F - fx["FILTER4"])) <= TOL
+
+
+def test_strain_matches_reference_les():
+    from pyoracle import Oracle
+    fx = _load("les")
+    c = fx["config"]
+    n1, n2, n3 = int(c[0]), int(c[1]), int(c[2])
+    re, alx, aly, alz, dt, files, wmodel = (float(x) for x in c[3:10])
+    o = Oracle(n1, n2, n3, re, alx, aly, alz, dt, files, wmodel, 2.5)
+    U, _, _ = inputs(o, seed=int(c[10]))
+    _check(fx, "STRAIN3", o.strain(U, 3), int(c[13]))

@@ -1,0 +1,191 @@
+"""Closed-form pins of the oracle for what the reference cannot run here
+(GETDP needs FFTW) and independent checks of what it can.  Each check is
+chosen so that a plausible slip (sign, index shift, transposed operand,
+dropped term, wrong normalisation) fails it."""
+import math
+
+import numpy as np
+import pytest
+
+from cases import config, inputs
+
+
+def _o(**kw):
+    from pyoracle import Oracle
+    return Oracle(**config(**kw))
+
+
+@pytest.mark.parametrize("b", [6.0, 10.0, 22.0])
+@pytest.mark.parametrize("n", [16, 33, 64])
+def test_ctdma_equals_dense_cyclic_solve(b, n):
+    from pyoracle import Oracle
+    rng = np.random.default_rng(n)
+    d = rng.uniform(-1, 1, n)
+    A = np.diag(np.full(n, b)) + np.diag(np.ones(n - 1), 1) + np.diag(np.ones(n - 1), -1)
+    A[0, -1] = A[-1, 0] = 1.0
+    x = Oracle.ctdma(b, d)
+    assert np.max(np.abs(x - np.linalg.solve(A, d))) <= 1e-15 * np.max(np.abs(x)) * n
+
+
+def test_dft2d_matches_numpy_fft():
+    from pyoracle import Oracle
+    rng = np.random.default_rng(1)
+    for n1, n3 in ((8, 12), (32, 48), (20, 30)):
+        a = rng.standard_normal((n1, n3)) + 1j * rng.standard_normal((n1, n3))
+        assert np.allclose(Oracle.dft2d(a, -1), np.fft.fft2(a), rtol=0, atol=1e-12 * np.abs(a).sum())
+        assert np.allclose(Oracle.dft2d(a, +1), np.fft.ifft2(a) * n1 * n3, rtol=0, atol=1e-12 * np.abs(a).sum())
+
+
+def test_modified_wavenumber_is_square_of_compact_d1_symbol():
+    """AK1/AK3 (INIWAVE, compact3d.f:9211,9222) must equal |sigma|^2 of the
+    compact face<->centre derivative, so that the Poisson operator is exactly
+    D1x(D1x) and the projection leaves no divergence."""
+    o = _o(n1=49, n2=17, n3=33)
+    m = o.mesh()
+    n = 48
+    th = 2 * np.pi * np.arange(n) / n
+    sig = 24 * m["DX1"] * (np.exp(1j * th) - 1) / (22 + 2 * np.cos(th))
+    ak1 = m["AK1"][1:n + 1]
+    kk = np.where(np.arange(n) <= n // 2, np.arange(n), np.arange(n) - n)
+    assert np.allclose(ak1, np.abs(sig[np.abs(kk)]) ** 2, rtol=1e-13, atol=1e-12)
+
+
+def _apply_lh(o, DP):
+    """L_h DP in physical space, built independently with numpy: compact
+    D1(face->centre) D1(centre->face) in x and z (exact circulant symbols) plus
+    the wall-normal three-point operator of METRICPOISSON."""
+    m = o.mesh()
+    n1m, n2m, n3m = o.n1 - 1, o.n2 - 1, o.n3 - 1
+    f = DP[:n1m, :n2m, :n3m]
+    kx = 2 * np.pi * np.arange(n1m) / n1m
+    kz = 2 * np.pi * np.arange(n3m) / n3m
+    a1 = 288 * m["DX1Q"] * (1 - np.cos(kx)) / (11 + np.cos(kx)) ** 2
+    a3 = 288 * m["DX3Q"] * (1 - np.cos(kz)) / (11 + np.cos(kz)) ** 2
+    F = np.fft.fft2(f, axes=(0, 2))
+    lap = -(a1[:, None, None] + a3[None, None, :]) * F
+    out = np.real(np.fft.ifft2(lap, axes=(0, 2)))
+    pm, pc, pp = m["PMJ"], m["PCJ"], m["PPJ"]
+    for j in range(1, n2m + 1):
+        acc = pc[j] * f[:, j - 1, :]
+        if j > 1:
+            acc = acc + pm[j] * f[:, j - 2, :]
+        if j < n2m:
+            acc = acc + pp[j] * f[:, j, :]
+        out[:, j - 1, :] += acc
+    R = np.zeros_like(DP)
+    R[:n1m, :n2m, :n3m] = out
+    return R
+
+
+@pytest.mark.parametrize("grid", [(33, 33, 33), (49, 25, 25)])
+def test_getdp_recovers_manufactured_solution(grid):
+    """GETDP inverts L_h exactly; its gauge is DP's plane mean at J=1 = 0
+    (the pinned row of mode (0,0), compact3d.f:9363-9368)."""
+    o = _o(n1=grid[0], n2=grid[1], n3=grid[2])
+    rng = np.random.default_rng(5)
+    DP = o.zeros_p()
+    n1m, n2m, n3m = o.n1 - 1, o.n2 - 1, o.n3 - 1
+    DP[:n1m, :n2m, :n3m] = rng.uniform(-1, 1, (n1m, n2m, n3m))
+    DP[:n1m, 0, :n3m] -= DP[:n1m, 0, :n3m].mean()
+    R = _apply_lh(o, DP)
+    out = o.zeros_p()
+    o.getdp(out, R)
+    err = np.max(np.abs(out - DP)) / np.max(np.abs(DP))
+    assert err <= 1e-10
+    # a wrong sign, a dropped AK term or a missing 1/(N1M*N3M) fails by O(1)
+
+
+def test_projection_invariants_after_step():
+    """After a full step: divergence at roundoff (DIVCHECK) and the flow rate
+    through every x-plane equal to the flow rate before it (UPCALC's
+    constant-flow-rate constraint, compact3d.f:1079-1128)."""
+    o = _o(n1=33, n2=33, n3=33)
+    U, P, _ = inputs(o, seed=2)
+    HH = o.zeros_u()
+    S = o.zeros_u(0)
+    m = o.mesh()
+    dy = m["DY"][1:o.n2]
+
+    def flow(U):
+        return np.einsum("ijk,j->i", U[1:o.n1, 1:o.n2, 1:o.n3, 0], dy) / m["DX3"]
+
+    f0 = flow(U).mean()
+    pg = pg3 = 0.0
+    for n in range(3):
+        pg, pg3 = o.getup(U, P, HH, S, pg, pg3, n + 1)
+        assert o.divcheck(U) <= 1e-12 * np.max(np.abs(U))
+        assert np.max(np.abs(flow(U).mean() - f0)) <= 1e-13 * abs(f0)
+
+
+def test_wall_model_solves_log_law():
+    """WALLMODEL_INSTANT: tau_w = u_tau^2 with u_tau the damped-Newton root of
+    U/u_tau = ln(u_tau*Delta*Re)/kappa + B (kappa=0.41, B=5.2,
+    compact3d.f:2370-2371), sampled at the third cell, Delta = DY1+DY2+DY3/2,
+    and the stress points along (u, w)."""
+    o = _o(n1=17, n2=17, n3=17, re=20000.0, files=2.0, wmodel=1.0)
+    U, _, _ = inputs(o, seed=4)
+    o.wallmodel_instant(U)
+    tw, tb = o.get_tau()
+    m = o.mesh()
+    dl = m["DY"][1] + m["DY"][2] + m["DY"][3] / 2.0
+    u3 = U[1:17, 3, 1:17, 0]
+    w3 = U[1:17, 3, 1:17, 2]
+    ua = np.sqrt(u3 ** 2 + w3 ** 2)
+    tau = np.sqrt(tw[1:17, 1:17, 0] ** 2 + tb[1:17, 1:17, 0] ** 2)
+    ut = np.sqrt(tau)
+    res = ua / ut - np.log(ut * dl * 20000.0) / 0.41 - 5.2
+    # the iteration stops on |du_tau| <= 1e-6 and reports the previous iterate
+    assert np.max(np.abs(res)) <= 1e-6 * np.max(ua / ut ** 2 + 1 / (0.41 * ut))
+    assert np.allclose(tw[1:17, 1:17, 0] * w3, tb[1:17, 1:17, 0] * u3, rtol=0, atol=1e-15)
+    assert o.get_wsm()[0] == pytest.approx(np.mean(tau), rel=1e-13)
+
+
+def test_strain_of_linear_shear():
+    """STRAIN on u = a*y (interior rows): S12 = a/2, all other S_ij = 0."""
+    o = _o(n1=17, n2=17, n3=17, cks=1.0)
+    m = o.mesh()
+    U = o.zeros_u()
+    a = 0.7
+    yc = 0.5 * (m["Y"][1:17] + m["Y"][2:18])
+    U[1:17, 1:17, 1:17, 0] = a * yc[None, :, None]
+    U[1:17, 0, 1:17, 0] = -U[1:17, 1, 1:17, 0]
+    U[1:17, 17, 1:17, 0] = 2 * a * m["Y"][17] - U[1:17, 16, 1:17, 0]
+    for j in (4, 8, 12):
+        SR = o.strain(U, j)
+        assert np.allclose(SR[:16, :16, 1], a / 2, rtol=1e-12)
+        for q in (0, 2, 3, 4, 5):
+            assert np.max(np.abs(SR[:16, :16, q])) <= 1e-12
+
+
+def test_filters_on_fourier_modes():
+    """FILTER: Simpson (1,4,1)/6 in x then z -> gain (4+2cos t)/6 per
+    direction; FILTER4: (1,4,2,4,1)/12 -> (2+8cos t+2cos 2t)/12."""
+    o = _o(n1=33, n2=9, n3=17)
+    n1m, n3m = 32, 16
+    i = np.arange(n1m)[:, None]
+    k = np.arange(n3m)[None, :]
+    mx, mz = 3, 2
+    f = np.cos(2 * np.pi * mx * i / n1m) * np.cos(2 * np.pi * mz * k / n3m)
+    F = np.zeros((33, 17, 1), order="F")
+    F[:n1m, :n3m, 0] = f
+    G = F.copy(order="F")
+    o.filter(G, four=False)
+    tx, tz = 2 * np.pi * mx / n1m, 2 * np.pi * mz / n3m
+    g = (4 + 2 * np.cos(tx)) / 6 * (4 + 2 * np.cos(tz)) / 6
+    assert np.allclose(G[:n1m, :n3m, 0], g * f, atol=1e-14)
+    G = F.copy(order="F")
+    o.filter(G, four=True)
+    g4 = ((2 + 8 * np.cos(tx) + 2 * np.cos(2 * tx)) / 12) * ((2 + 8 * np.cos(tz) + 2 * np.cos(2 * tz)) / 12)
+    assert np.allclose(G[:n1m, :n3m, 0], g4 * f, atol=1e-14)
+
+
+def test_mesh_closed_form():
+    o = _o(n1=17, n2=33, n3=17, cks=1.8)
+    m = o.mesh()
+    j = np.arange(1, 34)
+    y = 1 - np.tanh(1.8 * (1 - 2.0 * (j - 1) / 32)) / np.tanh(1.8)
+    assert np.allclose(m["Y"][1:34], y, rtol=0, atol=1e-15)
+    assert m["ALY"] == 2.0
+    dy = np.diff(m["Y"][1:34])
+    assert np.allclose(m["DY"][1:33], dy, atol=1e-16)
+    assert m["DX1"] == pytest.approx(16 / (2 * math.pi))
