@@ -57,7 +57,7 @@ def lib():
         L.c3d_footprint_bytes.argtypes = [C.POINTER(Config)]
         vp = [C.c_void_p]
         for name in ("c3d_destroy", "c3d_bcond", "c3d_rhs", "c3d_getuh", "c3d_dpcalc", "c3d_upcalc",
-                     "c3d_sgs", "c3d_wall_model"):
+                     "c3d_sgs", "c3d_wall_model", "c3d_wallss"):
             getattr(L, name).argtypes = vp
         L.c3d_getup.argtypes = [C.c_void_p, C.c_int]
         L.c3d_step.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
@@ -73,6 +73,7 @@ def lib():
         L.c3d_set_dt.argtypes = [C.c_void_p, C.c_double]
         L.c3d_apply_op.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp]
         L.c3d_get_mesh.argtypes = [C.c_void_p, _dp, _dp]
+        L.c3d_filter.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int]
         L.c3d_timing.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.c3d_kernel_times.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_int)]
         L.c3d_kernel_name.restype = C.c_char_p
@@ -189,6 +190,16 @@ class Channel:
 
     def wall_model(self):
         self._ck(self.L.c3d_wall_model(self.h), "wall_model")
+
+    def wallss(self):
+        self._ck(self.L.c3d_wallss(self.h), "wallss")
+
+    def filter(self, filv, width=2):
+        """FILTER (width 2) / FILTER4 (width 4) of FILV(N1, N3, n), Fortran order"""
+        a = np.asfortranarray(filv, dtype=np.float64)
+        out = np.zeros_like(a, order="F")
+        self._ck(self.L.c3d_filter(self.h, int(width), _p(a), _p(out), a.shape[2]), "filter")
+        return out
 
     def divcheck(self):
         v = C.c_double()

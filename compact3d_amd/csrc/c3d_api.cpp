@@ -160,6 +160,7 @@ static void build_mesh(c3d_handle* h) {
   s.vol = cf.alx * aly * cf.alz;
   s.ifwall = 0;
   s.files = cf.files != 0.0;
+  s.wmodel = cf.wmodel;
   DY[1] = Y[2];
   for (int j = 2; j <= N2M; j++) {
     DY[j] = Y[j + 1] - Y[j];
@@ -323,7 +324,6 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   if (cf.precision != C3D_FP64) return bad("only C3D_FP64 is available in this build");
   if (cf.files != 0.0 && cf.files != 2.0) return bad("FILES must be 0 (DNS) or 2 (Smagorinsky)");
   if (cf.wmodel != 0.0 && cf.wmodel != 1.0) return bad("WMODEL must be 0 or 1");
-  if (cf.files != 0.0) return bad("LES (FILES=2) is not available in this build yet");
   if (cf.re <= 0 || cf.dt <= 0 || cf.alx <= 0 || cf.alz <= 0) return bad("RE, DT, ALX, ALZ must be > 0");
   if (hipSetDevice(cf.device) != hipSuccess) return bad("hipSetDevice failed");
   Ctx& c = h->c;
@@ -354,6 +354,8 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   o.s22 = make_solve(1.0, 22.0);
   o.tx = make_tile(g.nx, 2);
   o.tz = make_tile(g.nz, 2);
+  o.tx4 = make_tile(g.nx, 4);
+  o.tz4 = make_tile(g.nz, 4);
 
   if (cf.stream) {
     c.st = (hipStream_t)cf.stream;
@@ -439,6 +441,12 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
       dalloc(h, &F.taub, n2d))
     return bad(h->err);
   if (dalloc(h, &F.spec, (size_t)g.nxh * g.nz * g.n2m)) return bad(h->err);
+  if (cf.files != 0.0) {
+    double** les[] = {&F.nux, &F.nuz, &F.nue, &F.dvdx, &F.dwdx, &F.dudz, &F.dvdz};
+    for (auto pp : les)
+      if (dalloc(h, pp, fs)) return bad(h->err);
+  }
+  c3dk::init_attrs();
   F.pgam = nullptr;
   if (dalloc(h, &F.sc, SC_COUNT)) return bad(h->err);
   F.nred = 1024;
@@ -610,16 +618,40 @@ extern "C" int c3d_upcalc(c3d_handle* h) {
 
 extern "C" int c3d_sgs(c3d_handle* h) {
   if (!h) return -1;
-  return fail(h, "SGSCS is not available in this build yet");
+  if (!h->c.s.files) return fail(h, "SGSCS needs FILES != 0");
+  c3dk::sgscs(h->c);
+  return launched(h);
+}
+
+// WALL_MODEL (compact3d.f:1981-1991): WMODEL=0 -> WALLSS, IFWALL=0;
+// WMODEL=1 (Y0=0) -> WALLMODEL_INSTANT, IFWALL=1
+static void wall_model_impl(c3d_handle* h) {
+  if (h->cfg.wmodel == 1.0) {
+    c3dk::wallmodel(h->c);
+    h->c.s.ifwall = 1;
+  } else {
+    c3dk::wallss(h->c);
+    h->c.s.ifwall = 0;
+  }
 }
 
 extern "C" int c3d_wall_model(c3d_handle* h) {
   if (!h) return -1;
-  return fail(h, "the wall model is not available in this build yet");
+  wall_model_impl(h);
+  return launched(h);
+}
+
+extern "C" int c3d_wallss(c3d_handle* h) {
+  if (!h) return -1;
+  c3dk::wallss(h->c);
+  return launched(h);
 }
 
 static int step_impl(c3d_handle* h, bool chk) {
   swap_hh(h);
+  // GETUP (:925-929): SGS model, then the wall model
+  if (h->cfg.files == 2.0) c3dk::sgscs(h->c);
+  if (h->cfg.files != 0.0) wall_model_impl(h);
   c3dk::bcond(h->c);
   c3dk::rhs(h->c);
   c3dk::getuh(h->c);
@@ -646,6 +678,9 @@ static double bits_to_double(double v) { return v; }
 extern "C" int c3d_step(c3d_handle* h, int ntime, double* divmax, double* cflm) {
   (void)ntime;
   if (!h) return -1;
+  // MAIN (:125) evaluates WALLSS before GETUP when WMODEL=0; only the SGS
+  // model consumes it
+  if (h->cfg.wmodel == 0.0 && h->cfg.files != 0.0) c3dk::wallss(h->c);
   if (step_impl(h, true)) return -1;
   if (divmax && read_sc(h, SC_DIVMAX, divmax)) return -1;
   if (cflm && read_sc(h, SC_CFLM, cflm)) return -1;
@@ -743,6 +778,23 @@ extern "C" int c3d_set_dt(c3d_handle* h, double dt) {
   h->c.s.dt = dt;
   h->cfg.dt = dt;
   return upload_dt_terms(h);
+}
+
+extern "C" int c3d_filter(c3d_handle* h, int width, const double* in, double* out, int n) {
+  if (!h || !in || !out) return -1;
+  if (width != 2 && width != 4) return fail(h, "width must be 2 or 4");
+  if (n < 1) return fail(h, "n must be >= 1");
+  const size_t plane = (size_t)h->cfg.n1 * h->cfg.n3;
+  const size_t bytes = plane * n * sizeof(double);
+  if (bytes > (size_t)h->c.g.fsz * sizeof(double)) return fail(h, "too many planes for the scratch field");
+  HIPCHK(hipMemcpyAsync(h->dtmp, in, bytes, hipMemcpyHostToDevice, h->c.st));
+  // entries outside I<N1, K<N3 keep their input values, as in the reference
+  HIPCHK(hipMemcpyAsync(h->dtmp2, h->dtmp, bytes, hipMemcpyDeviceToDevice, h->c.st));
+  c3dk::filter(h->c, width, h->dtmp, h->dtmp2, n, h->cfg.n1, h->cfg.n3);
+  if (launched(h)) return -1;
+  HIPCHK(hipMemcpyAsync(out, h->dtmp2, bytes, hipMemcpyDeviceToHost, h->c.st));
+  HIPCHK(hipStreamSynchronize(h->c.st));
+  return 0;
 }
 
 extern "C" int c3d_get_mesh(c3d_handle* h, double* y, double* scal4) {

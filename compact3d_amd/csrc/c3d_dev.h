@@ -101,6 +101,44 @@ struct TileShape {
   int seg;  // segment length
 };
 
+// ------------------------------------------------- element loops, reductions
+#define MAXE 16  // elements per thread of a line tile (n*L <= 16*C3D_NT)
+
+#define EL_LOOP(nL)                                    \
+  _Pragma("unroll") for (int m = 0; m < MAXE; m++) {   \
+    const int e = threadIdx.x + m * C3D_NT;            \
+    if (e < (nL)) {
+#define EL_END \
+  }            \
+  }
+
+__device__ __forceinline__ int wrapm(int p, int n) { return p == 0 ? n - 1 : p - 1; }
+__device__ __forceinline__ int wrapp(int p, int n) { return p == n - 1 ? 0 : p + 1; }
+
+// atomic max of a non-negative double (NaN compares largest, so it propagates)
+__device__ __forceinline__ void atomic_max_pos(double* slot, double v) {
+  unsigned long long b = __double_as_longlong(fabs(v)) ;
+  if (v != v) b = 0x7ff8000000000000ull;
+  atomicMax((unsigned long long*)slot, b);
+}
+
+__device__ __forceinline__ double block_max(double v, double* red) {
+  const int t = threadIdx.x;
+  red[t] = v;
+  __syncthreads();
+  for (int s = C3D_NT / 2; s > 0; s >>= 1) {
+    if (t < s) {
+      double a = red[t], b = red[t + s];
+      red[t] = (a != a || b != b) ? NAN : fmax(a, b);
+    }
+    __syncthreads();
+  }
+  double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+
 // ---------------------------------------------------------------- FFT
 struct cpx {
   double x, y;

@@ -27,11 +27,13 @@ struct Scal {
   double re, rinv, dt, dx1, dx1q, dx3, dx3q, aly, vol;
   int ifwall;
   int files;  // FILES != 0
+  double wmodel;
 };
 
 struct Ops {
   LineSolve s6, s10, s22, adix, adiz;
-  TileShape tx, tz;  // x-line and z-line tile shapes
+  TileShape tx, tz;    // x-line and z-line tile shapes (2 tiles in 64 KB)
+  TileShape tx4, tz4;  // shapes for kernels holding 3-4 tiles
   double adix_a, adix_b, adiz_a, adiz_b;
 };
 
@@ -52,6 +54,10 @@ struct Fields {
   double *vv1, *ww1, *uw, *vv3;
   double *rdp, *dp;
   double *sgs;
+  // LES intermediates (allocated when FILES != 0): nu_t interpolated to
+  // x-faces (nux), z-faces (nuz), x-z edges (nue), and the compact
+  // derivatives dv/dx, dw/dx (at x-faces), du/dz, dv/dz (at z-faces)
+  double *nux, *nuz, *nue, *dvdx, *dwdx, *dudz, *dvdz;
   double *div;
   double *ubc_u, *ubc_w;  // top ghost values U(I,N2,K,1), U(I,N2,K,3) from BCOND (NX*NZ)
   double *tauw, *taub;    // bottom-wall stresses TAUW(I,K,1), TAUB(I,K,1) (NX*NZ)
@@ -67,7 +73,8 @@ enum {
   KID_BCOND = 0, KID_RHS_X, KID_RHS_Z, KID_RHS_X2, KID_RHS_Y, KID_ADI_Y, KID_ADI_X, KID_ADI_Z,
   KID_RHSDP_X, KID_RHSDP_Z, KID_FFT_X, KID_FFT_Z, KID_POIS_Y, KID_FFT_XINV, KID_FLOW_RED,
   KID_FLOW_FIN, KID_UPD_X, KID_GHOSTS, KID_UPD_Z, KID_DIV_X, KID_DIV_Z, KID_CFL, KID_CHKMF,
-  KID_ZERO, KID_APPLY, KID_SGS, KID_WALL, KID_COUNT = 32
+  KID_ZERO, KID_APPLY, KID_SGS, KID_WALL, KID_SGS_X, KID_SGS_Z, KID_SGS_Z2, KID_SGS_X2,
+  KID_WALLSS, KID_FILTER, KID_COUNT = 32
 };
 
 // host-side recorder: when `on`, every launch is bracketed by two events on
@@ -121,6 +128,10 @@ void cfl(const Ctx& c);
 void chkmf(const Ctx& c, bool reference_sums);
 void sgscs(const Ctx& c);
 void wallmodel(const Ctx& c);
+void wallss(const Ctx& c);
+void sgs_terms(const Ctx& c);
+void init_attrs();
+void filter(const Ctx& c, int width, const double* in, double* out, int nplanes, int ld1, int ld3);
 void apply_op(const Ctx& c, int op, int dir, const double* in, double* out);
 void zero(const Ctx& c, double* p, long long n);
 }  // namespace c3dk

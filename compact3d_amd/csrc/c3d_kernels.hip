@@ -18,42 +18,6 @@
 
 #include "c3d_impl.h"
 
-#define MAXE 16  // elements per thread of a line tile (n*L <= 16*C3D_NT)
-
-#define EL_LOOP(nL)                                    \
-  _Pragma("unroll") for (int m = 0; m < MAXE; m++) {   \
-    const int e = threadIdx.x + m * C3D_NT;            \
-    if (e < (nL)) {
-#define EL_END \
-  }            \
-  }
-
-__device__ __forceinline__ int wrapm(int p, int n) { return p == 0 ? n - 1 : p - 1; }
-__device__ __forceinline__ int wrapp(int p, int n) { return p == n - 1 ? 0 : p + 1; }
-
-// atomic max of a non-negative double (NaN compares largest, so it propagates)
-__device__ __forceinline__ void atomic_max_pos(double* slot, double v) {
-  unsigned long long b = __double_as_longlong(fabs(v)) ;
-  if (v != v) b = 0x7ff8000000000000ull;
-  atomicMax((unsigned long long*)slot, b);
-}
-
-__device__ __forceinline__ double block_max(double v, double* red) {
-  const int t = threadIdx.x;
-  red[t] = v;
-  __syncthreads();
-  for (int s = C3D_NT / 2; s > 0; s >>= 1) {
-    if (t < s) {
-      double a = red[t], b = red[t + s];
-      red[t] = (a != a || b != b) ? NAN : fmax(a, b);
-    }
-    __syncthreads();
-  }
-  double r = red[0];
-  __syncthreads();
-  return r;
-}
-
 // ------------------------------------------------------------------ BCOND
 __global__ void k_bcond(Ctx c) {
   const Geo g = c.g;
@@ -429,6 +393,23 @@ __global__ void k_rhs_y(Ctx c) {
     double v2 = (j == g.n2m) ? 0.0 : F.vv1[bp];
     double v1 = (j == 1) ? 0.0 : F.vv1[b];
     double hh = F.hh[0][b] + 1. / M.dy[j] * (v2 * u2 - v1 * u1);
+    if (s.files) {
+      // D(SGS*(DU/DY))/DY and D(SGS*(DV/DX))/DY (:6551-6680); nux outside
+      // J=1..N2M enters only multiplied by a zero wall flag and is taken as 0
+      const int im = wrapm(i, g.nx);
+      const double n0 = F.nux[b], np1 = (j < g.n2m) ? F.nux[bp] : 0.0, nm1 = (j > 1) ? F.nux[bm] : 0.0;
+      const double sgsj2 = fjup * 1. / M.h[jp] * (M.dy[j] / 2. * np1 + M.dy[jp] / 2. * n0) +
+                           (1. - fjup) * 0.5 * (F.sgs[bN2] + F.sgs[gidx(g, im, g.n2, k)]);
+      const double sgsj1 = fjum * 1. / M.h[j] * (M.dy[jm] / 2. * n0 + M.dy[j] / 2. * nm1) +
+                           (1. - fjum) * 0.5 * (F.sgs[b0] + F.sgs[gidx(g, im, 0, k)]);
+      const double wfu = 1. - s.ifwall * (1. - fjup) * (s.aly - 1.);
+      const double wfd = 1. - s.ifwall * (1. - fjum);
+      const double uu22 = (F.u[bp] - F.u[b]) / M.h[jp];
+      const double uu11 = (F.u[b] - F.u[bm]) / M.h[j];
+      hh = hh - 1. / M.dy[j] * (wfu * sgsj2 * uu22 - wfd * sgsj1 * uu11);
+      const double dvp = (j < g.n2m) ? F.dvdx[bp] : 0.0;
+      hh = hh - 1.0 / M.dy[j] * (wfu * sgsj2 * dvp - wfd * sgsj1 * F.dvdx[b]);
+    }
     F.hh[0][b] = hh;
     double bc_down = -F.tauw[t2] / M.dy[1] * s.ifwall;
     double bc = (1. - fjum) * bc_down;
@@ -461,6 +442,22 @@ __global__ void k_rhs_y(Ctx c) {
     double v2 = (j == g.n2m) ? 0.0 : F.vv3[bp];
     double v1 = (j == 1) ? 0.0 : F.vv3[b];
     double hh = F.hh[2][b] + 1. / M.dy[j] * (v2 * w2 - v1 * w1);
+    if (s.files) {
+      // D(SGS*(DW/DY))/DY and D(SGS*(DV/DZ))/DY (:7858-7908, :8042-8094)
+      const int km = wrapm(k, g.nz);
+      const double z0 = F.nuz[b], zp1 = (j < g.n2m) ? F.nuz[bp] : 0.0, zm1 = (j > 1) ? F.nuz[bm] : 0.0;
+      const double sgsj1 = fjum * 1. / M.h[j] * (M.dy[jm] / 2. * z0 + M.dy[j] / 2. * zm1) +
+                           (1. - fjum) * 0.5 * (F.sgs[b0] + F.sgs[gidx(g, i, 0, km)]);
+      const double sgsj2 = fjup * 1.0 / M.h[jp] * (M.dy[j] / 2. * zp1 + M.dy[jp] / 2. * z0) +
+                           (1. - fjup) * 0.5 * (F.sgs[bN2] + F.sgs[gidx(g, i, g.n2, km)]);
+      const double wfu = 1. - s.ifwall * (1. - fjup) * (s.aly - 1.);
+      const double wfd = 1. - s.ifwall * (1. - fjum);
+      const double ww22 = 1. / M.h[jp] * (F.w[bp] - F.w[b]);
+      const double ww11 = 1. / M.h[j] * (F.w[b] - F.w[bm]);
+      hh = hh - 1. / M.dy[j] * (wfu * sgsj2 * ww22 - wfd * sgsj1 * ww11);
+      const double dzp = (j < g.n2m) ? F.dvdz[bp] : 0.0;
+      hh = hh - 1.0 / M.dy[j] * (wfu * sgsj2 * dzp - wfd * sgsj1 * F.dvdz[b]);
+    }
     F.hh[2][b] = hh;
     double bc_down = -F.taub[t2] / M.dy[1] * s.ifwall;
     double bc = (1. - fjum) * bc_down;
@@ -1297,6 +1294,11 @@ static int xgrid(const Ctx& c) { return (c.g.n2m * c.g.nz + c.o.tx.nl - 1) / c.o
 static int zgrid(const Ctx& c) { return ((c.g.nx + c.o.tz.nl - 1) / c.o.tz.nl) * c.g.n2m; }
 static int pgrid(long long n) { return (int)((n + C3D_NT - 1) / C3D_NT); }
 
+void init_attrs() {
+  // every kernel keeps its dynamic LDS within 64 KB; clear any stale error
+  (void)hipGetLastError();
+}
+
 void zero(const Ctx& c, double* p, long long n) {
   { KScope ks_(c, KID_ZERO); hipLaunchKernelGGL(k_zero, dim3(pgrid(n)), dim3(C3D_NT), 0, c.st, p, n); }
 }
@@ -1309,6 +1311,7 @@ void rhs(const Ctx& c) {
   { KScope ks_(c, KID_RHS_X); hipLaunchKernelGGL(k_rhs_x, dim3(xgrid(c)), dim3(C3D_NT), tile_lds(c.o.tx, 2), c.st, c); }
   { KScope ks_(c, KID_RHS_Z); hipLaunchKernelGGL(k_rhs_z, dim3(zgrid(c)), dim3(C3D_NT), tile_lds(c.o.tz, 2), c.st, c); }
   { KScope ks_(c, KID_RHS_X2); hipLaunchKernelGGL(k_rhs_x2, dim3(xgrid(c)), dim3(C3D_NT), tile_lds(c.o.tx, 1), c.st, c); }
+  if (c.s.files) sgs_terms(c);
   { KScope ks_(c, KID_RHS_Y); hipLaunchKernelGGL(k_rhs_y, dim3(pgrid((long long)c.g.nx * c.g.n2m * c.g.nz)), dim3(C3D_NT), 0,
                      c.st, c); }
 }
@@ -1383,7 +1386,5 @@ void apply_op(const Ctx& c, int op, int dir, const double* in, double* out) {
                        out); }
 }
 
-void sgscs(const Ctx& c) { (void)c; }
-void wallmodel(const Ctx& c) { (void)c; }
 
 }  // namespace c3dk

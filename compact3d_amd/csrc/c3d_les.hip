@@ -1,0 +1,665 @@
+// c3d_les.hip -- LES parts of the hot path: Smagorinsky eddy viscosity
+// (SGSCS + STRAIN), the instantaneous log-law wall model, WALLSS, the SGS
+// stress-divergence line solves of RHS1-3 and the explicit test filters.
+#include <math.h>
+
+#include <algorithm>
+
+#include "c3d_impl.h"
+
+// ------------------------------------------------------------ SGSCS
+// SGSCS (compact3d.f:1413-1488) with STRAIN (:5558-5689): cell-centre strain
+// rate from second-order differences, |S| = sqrt(2 S:S),
+// nu_t = max((0.1 (1-exp(-y+/26)))^2 (DY/(DX1 DX3))^(2/3) |S|, -1/RE),
+// y+ = (Y(J)+DY(J)/2) u_tau RE, u_tau = (WSM1^2 + WSM3^2)^(1/4) (previous step).
+__global__ void k_sgscs(Ctx c) {
+  const Geo g = c.g;
+  const long long tot = (long long)g.nx * g.n2m * g.nz;
+  long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= tot) return;
+  const int i = t % g.nx;
+  const int j = 1 + (int)((t / g.nx) % g.n2m);
+  const int k = (int)(t / ((long long)g.nx * g.n2m));
+  const Scal s = c.s;
+  const Metrics M = c.m;
+  const Fields F = c.d;
+  const double* U = F.u;
+  const double* V = F.v;
+  const double* W = F.w;
+  const int ip = wrapp(i, g.nx), im = wrapm(i, g.nx), kp = wrapp(k, g.nz), km = wrapm(k, g.nz);
+  const int jp = j + 1, jm = j - 1;
+  const double DX1 = s.dx1, DX3 = s.dx3;
+  const double* DY = M.dy;
+  const double* H = M.h;
+  const int fjum = (int)M.fjmu[j], fjup = (int)M.fjpa[j];
+  const bool wm0 = (s.wmodel == 0.0);
+  auto u_ = [&](int ii, int jj, int kk) { return U[gidx(g, ii, jj, kk)]; };
+  auto v_ = [&](int ii, int jj, int kk) { return V[gidx(g, ii, jj, kk)]; };
+  auto w_ = [&](int ii, int jj, int kk) { return W[gidx(g, ii, jj, kk)]; };
+  const int N2 = g.n2;
+  double sr1, sr2, sr3, sr4, sr5, sr6;
+  double u2 = u_(ip, j, k), u1 = u_(i, j, k);
+  sr1 = DX1 * (u2 - u1);
+  double ujp = 0.5 * (u_(i, jp, k) + u_(ip, jp, k));
+  double ujc = 0.5 * (u_(i, j, k) + u_(ip, j, k));
+  double ujm = 0.5 * (u_(i, jm, k) + u_(ip, jm, k));
+  if (wm0) {
+    u2 = fjup * 1.0 / H[jp] * (DY[j] / 2.0 * ujp + DY[jp] / 2.0 * ujc) +
+         (1. - fjup) * 0.5 * (u_(i, N2, k) + u_(ip, N2, k));
+    u1 = fjum * 1.0 / H[j] * (DY[j] / 2.0 * ujm + DY[jm] / 2.0 * ujc) +
+         (1. - fjum) * 0.5 * (u_(i, 0, k) + u_(ip, 0, k));
+  } else {
+    u2 = fjup * 1.0 / H[jp] * (DY[j] / 2.0 * ujp + DY[jp] / 2.0 * ujc) + (1. - fjup) * ujc;
+    u1 = fjum * 1.0 / H[j] * (DY[j] / 2.0 * ujm + DY[jm] / 2.0 * ujc) + (1. - fjum) * ujc;
+  }
+  double vip = 0.5 * (v_(ip, j, k) + v_(ip, jp, k));
+  double vic = 0.5 * (v_(i, j, k) + v_(i, jp, k));
+  double vim = 0.5 * (v_(im, j, k) + v_(im, jp, k));
+  double v2 = 0.5 * (vip + vic), v1 = 0.5 * (vim + vic);
+  const double yfac = (!wm0 && (j == 1 || j == g.n2m)) ? 2.0 : 1.0;
+  sr2 = 0.5 * (DX1 * (v2 - v1) + yfac / DY[j] * (u2 - u1));
+  double ukp = 0.5 * (u_(i, j, kp) + u_(ip, j, kp));
+  double ukc = 0.5 * (u_(i, j, k) + u_(ip, j, k));
+  double ukm = 0.5 * (u_(i, j, km) + u_(ip, j, km));
+  u2 = 0.5 * (ukp + ukc);
+  u1 = 0.5 * (ukm + ukc);
+  double wip = 0.5 * (w_(ip, j, k) + w_(ip, j, kp));
+  double wic = 0.5 * (w_(i, j, k) + w_(i, j, kp));
+  double wim = 0.5 * (w_(im, j, k) + w_(im, j, kp));
+  double w2 = 0.5 * (wip + wic), w1 = 0.5 * (wim + wic);
+  sr3 = 0.5 * (DX1 * (w2 - w1) + DX3 * (u2 - u1));
+  v2 = v_(i, jp, k);
+  v1 = v_(i, j, k);
+  sr4 = 1.0 / DY[j] * (v2 - v1);
+  double wjp = 0.5 * (w_(i, jp, k) + w_(i, jp, kp));
+  double wjc = 0.5 * (w_(i, j, k) + w_(i, j, kp));
+  double wjm = 0.5 * (w_(i, jm, k) + w_(i, jm, kp));
+  if (wm0) {
+    w2 = fjup * 1.0 / H[jp] * (DY[j] / 2.0 * wjp + DY[jp] / 2.0 * wjc) +
+         (1. - fjup) * 0.5 * (w_(i, N2, k) + w_(i, N2, kp));
+    w1 = fjum * 1.0 / H[j] * (DY[j] / 2.0 * wjm + DY[jm] / 2.0 * wjc) +
+         (1. - fjum) * 0.5 * (w_(i, 0, k) + w_(i, 0, kp));
+  } else {
+    w2 = fjup * 1.0 / H[jp] * (DY[j] / 2.0 * wjp + DY[jp] / 2.0 * wjc) + (1. - fjup) * wjc;
+    w1 = fjum * 1.0 / H[j] * (DY[j] / 2.0 * wjm + DY[jm] / 2.0 * wjc) + (1. - fjum) * wjc;
+  }
+  double vkp = 0.5 * (v_(i, j, kp) + v_(i, jp, kp));
+  double vkc = 0.5 * (v_(i, j, k) + v_(i, jp, k));
+  double vkm = 0.5 * (v_(i, j, km) + v_(i, jp, km));
+  v2 = 0.5 * (vkp + vkc);
+  v1 = 0.5 * (vkm + vkc);
+  sr5 = 0.5 * (yfac / DY[j] * (w2 - w1) + DX3 * (v2 - v1));
+  w2 = w_(i, j, kp);
+  w1 = w_(i, j, k);
+  sr6 = DX3 * (w2 - w1);
+  double sumsr = 2. * (sr2 * sr2) + sr1 * sr1 + 2. * (sr3 * sr3) + sr4 * sr4 + 2. * (sr5 * sr5) + sr6 * sr6;
+  double abssr = sqrt(2. * sumsr);
+  const double w1s = F.sc[SC_WSM1], w3s = F.sc[SC_WSM3];
+  const double utau = sqrt(sqrt(w1s * w1s + w3s * w3s));
+  double cs = 0.1;
+  const double yplus = (M.y[j] + DY[j] * 0.5) * utau * s.re;
+  cs = cs * (1 - exp(-yplus / 26.0));
+  const double smagc = cs * cs;
+  const double dxyz = 1.0 / DX1 * DY[j] / DX3;
+  const double turbv = smagc * pow(dxyz, 2. / 3.) * abssr;
+  F.sgs[gidx(g, i, j, k)] = fmax(turbv, -s.rinv);
+}
+
+// --------------------------------------------------------- wall model
+// WALLMODEL_INSTANT (compact3d.f:2319-2493): per wall point the damped Newton
+// iteration of U/u_tau = ln(u_tau*DELTAP*RE)/kappa + B on the third cell,
+// tau_w = u_tau^2 of the last accepted iterate, TAUW = tau_w u/U,
+// TAUB = tau_w w/U; WSM(1) = TAOW = mean tau_w.  The reference iterates
+// without a bound; here the loop also stops after 10000 iterations (the
+// damped iteration converges in tens).
+#define WM_MAXIT 10000
+__global__ __launch_bounds__(C3D_NT) void k_wallmodel(Ctx c) {
+  __shared__ double red[C3D_NT];
+  const Geo g = c.g;
+  const Metrics M = c.m;
+  const Fields F = c.d;
+  const long long n = (long long)g.nx * g.nz;
+  const double RE = c.s.re, KAPA = 0.41, B = 5.2;
+  const double deltap = M.dy[1] + M.dy[2] + M.dy[3] / 2.0;
+  double acc = 0.0;
+  for (long long t = blockIdx.x * (long long)C3D_NT + threadIdx.x; t < n; t += (long long)gridDim.x * C3D_NT) {
+    const int i = t % g.nx, k = t / g.nx;
+    const double u3 = F.u[gidx(g, i, 3, k)], w3 = F.w[gidx(g, i, 3, k)];
+    const double u_ave = sqrt(u3 * u3 + w3 * w3);
+    double ut1 = 1.0, ut2 = 0.000001, f1, f2, f3, lam = 1.0;
+    int it = 0;
+    while (fabs(ut2 - ut1) > 1.0E-6 && it < WM_MAXIT) {
+      ut1 = ut2;
+      f1 = u_ave / ut1 - log(ut1 * deltap * RE) / KAPA - B;
+      f2 = -u_ave / (ut1 * ut1) - 1. / KAPA / ut1;
+      ut2 = ut1 - lam * f1 / f2;
+      f3 = u_ave / ut2 - log(ut2 * deltap * RE) / KAPA - B;
+      if (fabs(f3) > fabs(f1)) lam = 0.5 * lam;
+      it++;
+    }
+    const double tauw1 = ut1 * ut1;
+    acc += tauw1;
+    F.tauw[t] = tauw1 * u3 / u_ave;
+    F.taub[t] = tauw1 * w3 / u_ave;
+  }
+  const int t = threadIdx.x;
+  red[t] = acc;
+  __syncthreads();
+  for (int h = C3D_NT / 2; h > 0; h >>= 1) {
+    if (t < h) red[t] += red[t + h];
+    __syncthreads();
+  }
+  if (t == 0) F.red[blockIdx.x * 4] = red[0];
+}
+
+__global__ __launch_bounds__(C3D_NT) void k_wallmodel_fin(Ctx c, int nblk) {
+  __shared__ double red[C3D_NT];
+  const Fields F = c.d;
+  const int t = threadIdx.x;
+  double a = 0.0;
+  for (int b = t; b < nblk; b += C3D_NT) a += F.red[b * 4];
+  red[t] = a;
+  __syncthreads();
+  for (int h = C3D_NT / 2; h > 0; h >>= 1) {
+    if (t < h) red[t] += red[t + h];
+    __syncthreads();
+  }
+  if (t == 0) {
+    const double w = red[0] / (double)((long long)c.g.nx * c.g.nz);
+    F.sc[SC_WSM1] = w;
+    F.sc[SC_TAOW] = w;
+  }
+}
+
+// WALLSS (compact3d.f:11077-11143), including its use of WST(.,.,1) in WSM(2)
+// and WSM(3) (:11130-11132)
+__global__ __launch_bounds__(C3D_NT) void k_wallss(Ctx c) {
+  __shared__ double red[3][C3D_NT];
+  const Geo g = c.g;
+  const Metrics M = c.m;
+  const Fields F = c.d;
+  const long long n = (long long)g.nx * g.nz;
+  const double aly = c.s.aly;
+  double a1 = 0, a2 = 0, a3 = 0;
+  for (long long t = blockIdx.x * (long long)C3D_NT + threadIdx.x; t < n; t += (long long)gridDim.x * C3D_NT) {
+    const int i = t % g.nx, k = t / g.nx;
+    const int ip = wrapp(i, g.nx), kp = wrapp(k, g.nz);
+    double uhc = 0.5 * (F.u[gidx(g, i, 1, k)] + F.u[gidx(g, ip, 1, k)]);
+    double uhw = 0.5 * (F.u[gidx(g, i, 0, k)] + F.u[gidx(g, ip, 0, k)]);
+    double vhc = F.v[gidx(g, i, 2, k)], vhw = F.v[gidx(g, i, 1, k)];
+    double whc = 0.5 * (F.w[gidx(g, i, 1, k)] + F.w[gidx(g, i, 1, kp)]);
+    double whw = 0.5 * (F.w[gidx(g, i, 0, k)] + F.w[gidx(g, i, 0, kp)]);
+    const double wsb1 = (uhc - uhw) / (M.dy[1] * 0.5);
+    const double wsb2 = (vhc - vhw) / M.dy[1];
+    const double wsb3 = (whc - whw) / (M.dy[1] * 0.5);
+    uhc = 0.5 * (F.u[gidx(g, i, g.n2m, k)] + F.u[gidx(g, ip, g.n2m, k)]);
+    uhw = 0.5 * (F.u[gidx(g, i, g.n2, k)] + F.u[gidx(g, ip, g.n2, k)]);
+    const double wst1 = (uhc - uhw) / (M.dy[g.n2m] * 0.5);
+    a1 = a1 + wsb1 + wst1 * (aly - 1.0);
+    a2 = a2 + wsb2 + wst1 * (aly - 1.0);
+    a3 = a3 + wsb3 + wst1 * (aly - 1.0);
+  }
+  const int t = threadIdx.x;
+  red[0][t] = a1;
+  red[1][t] = a2;
+  red[2][t] = a3;
+  __syncthreads();
+  for (int h = C3D_NT / 2; h > 0; h >>= 1) {
+    if (t < h)
+      for (int q = 0; q < 3; q++) red[q][t] += red[q][t + h];
+    __syncthreads();
+  }
+  if (t < 3) F.red[blockIdx.x * 4 + t] = red[t][0];
+}
+
+__global__ __launch_bounds__(C3D_NT) void k_wallss_fin(Ctx c, int nblk) {
+  __shared__ double red[3][C3D_NT];
+  const Fields F = c.d;
+  const int t = threadIdx.x;
+  double a[3] = {0, 0, 0};
+  for (int b = t; b < nblk; b += C3D_NT)
+    for (int q = 0; q < 3; q++) a[q] += F.red[b * 4 + q];
+  for (int q = 0; q < 3; q++) red[q][t] = a[q];
+  __syncthreads();
+  for (int h = C3D_NT / 2; h > 0; h >>= 1) {
+    if (t < h)
+      for (int q = 0; q < 3; q++) red[q][t] += red[q][t + h];
+    __syncthreads();
+  }
+  if (t == 0) {
+    const double nxz = 1.0 / (double)((long long)c.g.nx * c.g.nz);
+    for (int q = 0; q < 3; q++) F.sc[SC_WSM1 + q] = red[q][0] * nxz / c.s.re / c.s.aly;
+  }
+}
+
+// --------------------------------------------------- SGS line-set terms
+// Intermediates shared by the 36 SGS line solves of RHS1-3:
+//   nux = Ix(SGSVIS)   (centre -> x-face)    SGSVIS1/12/13, SGSVISV1/V12
+//   nuz = Iz(SGSVIS)   (centre -> z-face)    SGSVIS3/V3/W2/W3/31/32
+//   nue = Iz(Ix(SGSVIS)) (x-z edge)          the (1,6,1) pair of :6512-6517,
+//         :6747-6752, :7819-7823, :8005-8009 (Ix Iz and Iz Ix commute)
+//   dvdx = D1x(v), dwdx = D1x(w) (centre -> x-face)   UU12, VV12, WW13, X11
+//   dudz = D1z(u), dvdz = D1z(v) (centre -> z-face)   X33 of :6519-6524, UU31,
+//                                                     VV23, VV32
+#define SG_TILE_LDS 4
+__global__ __launch_bounds__(C3D_NT) void k_sgs_x(Ctx c) {
+  extern __shared__ double sm[];
+  const Geo g = c.g;
+  const TileShape ts = c.o.tx;
+  const int n = ts.n, L = ts.nl, LS = ts.ls, nL = n * L;
+  double* Wt = sm;
+  double* Kt = sm + n * LS;
+  const int nlines = g.n2m * g.nz;
+  const int ln0 = blockIdx.x * L;
+  const Scal s = c.s;
+  const Fields F = c.d;
+#define SXDEC                             \
+  const int l = e / n, p = e % n;         \
+  const int ln = ln0 + l;                 \
+  const bool ok = ln < nlines;            \
+  const int j = 1 + (ok ? ln % g.n2m : 0); \
+  const long long b = gidx(g, 0, j, ok ? ln / g.n2m : 0); \
+  const int pm = wrapm(p, n), pp = wrapp(p, n);
+  // nux = Ix(SGSVIS)
+  EL_LOOP(nL) SXDEC;
+  (void)pp;
+  Wt[p * LS + l] = ok ? 4. * (F.sgs[b + p] + F.sgs[b + pm]) : 0.0;
+  EL_END;
+  __syncthreads();
+  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s6);
+  EL_LOOP(nL) SXDEC;
+  (void)pm; (void)pp; (void)j;
+  if (ok) F.nux[b + p] = Wt[p * LS + l];
+  EL_END;
+  __syncthreads();
+  // dvdx = D1x(v)
+  EL_LOOP(nL) SXDEC;
+  (void)pp; (void)j;
+  Wt[p * LS + l] = ok ? 24. * s.dx1 * (F.v[b + p] - F.v[b + pm]) : 0.0;
+  EL_END;
+  __syncthreads();
+  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
+  EL_LOOP(nL) SXDEC;
+  (void)pm; (void)pp; (void)j;
+  if (ok) F.dvdx[b + p] = Wt[p * LS + l];
+  EL_END;
+  __syncthreads();
+  // dwdx = D1x(w)
+  EL_LOOP(nL) SXDEC;
+  (void)pp; (void)j;
+  Wt[p * LS + l] = ok ? 24. * s.dx1 * (F.w[b + p] - F.w[b + pm]) : 0.0;
+  EL_END;
+  __syncthreads();
+  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
+  EL_LOOP(nL) SXDEC;
+  (void)pm; (void)pp; (void)j;
+  if (ok) F.dwdx[b + p] = Wt[p * LS + l];
+  EL_END;
+  __syncthreads();
+  // D(SGS*(DU/DX))/DX (:6445-6466): HH1 -= 2 D1x(SGSVIS * D1x(u))
+  EL_LOOP(nL) SXDEC;
+  (void)pm; (void)j;
+  Wt[p * LS + l] = ok ? 24. * s.dx1 * (F.u[b + pp] - F.u[b + p]) : 0.0;
+  EL_END;
+  __syncthreads();
+  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
+  EL_LOOP(nL) SXDEC;
+  (void)pp; (void)j;
+  Kt[p * LS + l] = ok ? F.sgs[b + p] * Wt[p * LS + l] : 0.0;
+  EL_END;
+  __syncthreads();
+  EL_LOOP(nL) SXDEC;
+  (void)pp; (void)j; (void)b;
+  Wt[p * LS + l] = ok ? 24. * s.dx1 * (Kt[p * LS + l] - Kt[pm * LS + l]) : 0.0;
+  EL_END;
+  __syncthreads();
+  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
+  EL_LOOP(nL) SXDEC;
+  (void)pm; (void)pp; (void)j;
+  if (ok) F.hh[0][b + p] = F.hh[0][b + p] - 2. * Wt[p * LS + l];
+  EL_END;
+#undef SXDEC
+}
+
+__global__ __launch_bounds__(C3D_NT) void k_sgs_z(Ctx c) {
+  extern __shared__ double sm[];
+  const Geo g = c.g;
+  const TileShape ts = c.o.tz4;
+  const int n = ts.n, L = ts.nl, LS = ts.ls, nL = n * L;
+  double* Wt = sm;
+  double* Et = sm + n * LS;      // nue
+  double* Dt = sm + 2 * n * LS;  // dudz
+  double* Kt = sm + 3 * n * LS;
+  const int nit = (g.nx + L - 1) / L;
+  const int it = blockIdx.x % nit;
+  const int j = 1 + blockIdx.x / nit;
+  const Scal s = c.s;
+  const Fields F = c.d;
+  const long long pl = g.plane;
+#define SZDEC                               \
+  const int l = e % L, p = e / L;           \
+  const int i = it * L + l;                 \
+  const bool ok = i < g.nx;                 \
+  const long long b = gidx(g, ok ? i : 0, j, 0); \
+  const int pm = wrapm(p, n), pp = wrapp(p, n);
+  // nuz = Iz(SGSVIS)
+  EL_LOOP(nL) SZDEC;
+  (void)pp;
+  Wt[p * LS + l] = ok ? 4. * (F.sgs[b + pm * pl] + F.sgs[b + p * pl]) : 0.0;
+  EL_END;
+  __syncthreads();
+  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s6);
+  EL_LOOP(nL) SZDEC;
+  (void)pm; (void)pp;
+  if (ok) F.nuz[b + p * pl] = Wt[p * LS + l];
+  EL_END;
+  __syncthreads();
+  // nue = Iz(nux)
+  EL_LOOP(nL) SZDEC;
+  (void)pp;
+  Et[p * LS + l] = ok ? 4. * (F.nux[b + p * pl] + F.nux[b + pm * pl]) : 0.0;
+  EL_END;
+  __syncthreads();
+  tile_solve(Et, n, L, LS, ts.g, ts.seg, c.o.s6);
+  EL_LOOP(nL) SZDEC;
+  (void)pm; (void)pp;
+  if (ok) F.nue[b + p * pl] = Et[p * LS + l];
+  EL_END;
+  // dudz = D1z(u)
+  EL_LOOP(nL) SZDEC;
+  (void)pp;
+  Dt[p * LS + l] = ok ? 24. * s.dx3 * (F.u[b + p * pl] - F.u[b + pm * pl]) : 0.0;
+  EL_END;
+  __syncthreads();
+  tile_solve(Dt, n, L, LS, ts.g, ts.seg, c.o.s22);
+  EL_LOOP(nL) SZDEC;
+  (void)pm; (void)pp;
+  if (ok) F.dudz[b + p * pl] = Dt[p * LS + l];
+  EL_END;
+  // dvdz = D1z(v)
+  EL_LOOP(nL) SZDEC;
+  (void)pp;
+  Wt[p * LS + l] = ok ? 24. * s.dx3 * (F.v[b + p * pl] - F.v[b + pm * pl]) : 0.0;
+  EL_END;
+  __syncthreads();
+  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
+  EL_LOOP(nL) SZDEC;
+  (void)pm; (void)pp;
+  if (ok) F.dvdz[b + p * pl] = Wt[p * LS + l];
+  EL_END;
+  __syncthreads();
+  // D(SGS*(DW/DZ))/DZ (:7919-7940): HH3 -= 2 D1z(SGSVIS * D1z(w))
+  EL_LOOP(nL) SZDEC;
+  (void)pm;
+  Wt[p * LS + l] = ok ? 24. * s.dx3 * (F.w[b + pp * pl] - F.w[b + p * pl]) : 0.0;
+  EL_END;
+  __syncthreads();
+  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
+  EL_LOOP(nL) SZDEC;
+  (void)pm; (void)pp;
+  Kt[p * LS + l] = ok ? Wt[p * LS + l] * F.sgs[b + p * pl] : 0.0;
+  EL_END;
+  __syncthreads();
+  EL_LOOP(nL) SZDEC;
+  (void)pp; (void)b;
+  Wt[p * LS + l] = ok ? 24. * s.dx3 * (Kt[p * LS + l] - Kt[pm * LS + l]) : 0.0;
+  EL_END;
+  __syncthreads();
+  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
+  EL_LOOP(nL) SZDEC;
+  (void)pm; (void)pp;
+  if (ok) F.hh[2][b + p * pl] = F.hh[2][b + p * pl] - 2. * Wt[p * LS + l];
+  EL_END;
+  __syncthreads();
+  // D(SGS*(DU/DZ))/DZ (:6491-6540): HH1 -= D1z(dudz * nue)
+  EL_LOOP(nL) SZDEC;
+  (void)pm; (void)b;
+  Wt[p * LS + l] = ok ? 24. * s.dx3 * (Dt[pp * LS + l] * Et[pp * LS + l] - Dt[p * LS + l] * Et[p * LS + l]) : 0.0;
+  EL_END;
+  __syncthreads();
+  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
+  EL_LOOP(nL) SZDEC;
+  (void)pm; (void)pp;
+  if (ok) F.hh[0][b + p * pl] = F.hh[0][b + p * pl] - Wt[p * LS + l];
+  EL_END;
+  __syncthreads();
+  // D(SGS*(DW/DX))/DZ (:6713-6764): HH1 -= D1z(dwdx * nue)
+  EL_LOOP(nL) SZDEC;
+  (void)pm;
+  Wt[p * LS + l] = ok ? 24. * s.dx3 * (F.dwdx[b + pp * pl] * Et[pp * LS + l] - F.dwdx[b + p * pl] * Et[p * LS + l])
+                      : 0.0;
+  EL_END;
+  __syncthreads();
+  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
+  EL_LOOP(nL) SZDEC;
+  (void)pm; (void)pp;
+  if (ok) F.hh[0][b + p * pl] = F.hh[0][b + p * pl] - Wt[p * LS + l];
+  EL_END;
+#undef SZDEC
+}
+
+// z-lines at J >= 2: the two RHS2 terms that need nuz at J-1 and J
+//   D(SGS*(DW/DY))/DZ (:7177-7226), D(SGS*(DV/DZ))/DZ (:7245-7304)
+__global__ __launch_bounds__(C3D_NT) void k_sgs_z2(Ctx c) {
+  extern __shared__ double sm[];
+  const Geo g = c.g;
+  const TileShape ts = c.o.tz4;
+  const int n = ts.n, L = ts.nl, LS = ts.ls, nL = n * L;
+  double* Wt = sm;
+  double* Yt = sm + n * LS;  // nuz interpolated to the v face J
+  double* Kt = sm + 2 * n * LS;
+  const int nit = (g.nx + L - 1) / L;
+  const int it = blockIdx.x % nit;
+  const int j = 2 + blockIdx.x / nit;
+  const Scal s = c.s;
+  const Metrics M = c.m;
+  const Fields F = c.d;
+  const long long pl = g.plane;
+#define SZDEC                               \
+  const int l = e % L, p = e / L;           \
+  const int i = it * L + l;                 \
+  const bool ok = i < g.nx;                 \
+  const long long b = gidx(g, ok ? i : 0, j, 0); \
+  const int pp = wrapp(p, n);
+  const double hinv = 1. / M.h[j], dyj = M.dy[j] / 2., dyjm = M.dy[j - 1] / 2.;
+  EL_LOOP(nL) SZDEC;
+  (void)pp;
+  double yn = 0.0, gw = 0.0;
+  if (ok) {
+    yn = hinv * (dyj * F.nuz[b - g.nx + p * pl] + dyjm * F.nuz[b + p * pl]);
+    gw = hinv * (F.w[b + p * pl] - F.w[b - g.nx + p * pl]);
+  }
+  Yt[p * LS + l] = yn;
+  Kt[p * LS + l] = yn * gw;
+  EL_END;
+  __syncthreads();
+  EL_LOOP(nL) SZDEC;
+  (void)b;
+  Wt[p * LS + l] = ok ? 24. * s.dx3 * (Kt[pp * LS + l] - Kt[p * LS + l]) : 0.0;
+  EL_END;
+  __syncthreads();
+  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
+  EL_LOOP(nL) SZDEC;
+  (void)pp;
+  if (ok) F.hh[1][b + p * pl] = F.hh[1][b + p * pl] - Wt[p * LS + l];
+  EL_END;
+  __syncthreads();
+  EL_LOOP(nL) SZDEC;
+  Wt[p * LS + l] = ok ? 24. * s.dx3 * (Yt[pp * LS + l] * F.dvdz[b + pp * pl] - Yt[p * LS + l] * F.dvdz[b + p * pl])
+                      : 0.0;
+  EL_END;
+  __syncthreads();
+  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
+  EL_LOOP(nL) SZDEC;
+  (void)pp;
+  if (ok) F.hh[1][b + p * pl] = F.hh[1][b + p * pl] - Wt[p * LS + l];
+  EL_END;
+#undef SZDEC
+}
+
+// x-lines: D(SGS*(DU/DY))/DX and D(SGS*(DV/DX))/DX of RHS2 (:7104-7158,
+// :7323-7384, J >= 2), D(SGS*(DW/DX))/DX and D(SGS*(DU/DZ))/DX of RHS3
+// (:7797-7847, :7972-8024)
+__global__ __launch_bounds__(C3D_NT) void k_sgs_x2(Ctx c) {
+  extern __shared__ double sm[];
+  const Geo g = c.g;
+  const TileShape ts = c.o.tx4;
+  const int n = ts.n, L = ts.nl, LS = ts.ls, nL = n * L;
+  double* Wt = sm;
+  double* Yt = sm + n * LS;
+  double* Kt = sm + 2 * n * LS;
+  const int nlines = g.n2m * g.nz;
+  const int ln0 = blockIdx.x * L;
+  const Scal s = c.s;
+  const Metrics M = c.m;
+  const Fields F = c.d;
+#define SXDEC                              \
+  const int l = e / n, p = e % n;          \
+  const int ln = ln0 + l;                  \
+  const bool ok = ln < nlines;             \
+  const int j = 1 + (ok ? ln % g.n2m : 0); \
+  const long long b = gidx(g, 0, j, ok ? ln / g.n2m : 0); \
+  const int pp = wrapp(p, n);
+  EL_LOOP(nL) SXDEC;
+  (void)pp;
+  double yn = 0.0, gu = 0.0;
+  if (ok && j >= 2) {
+    const double hinv = 1. / M.h[j];
+    yn = hinv * (M.dy[j - 1] / 2. * F.nux[b + p] + M.dy[j] / 2. * F.nux[b - g.nx + p]);
+    gu = hinv * (F.u[b + p] - F.u[b - g.nx + p]);
+  }
+  Yt[p * LS + l] = yn;
+  Kt[p * LS + l] = yn * gu;
+  EL_END;
+  __syncthreads();
+  EL_LOOP(nL) SXDEC;
+  (void)b;
+  Wt[p * LS + l] = ok ? 24. * s.dx1 * (Kt[pp * LS + l] - Kt[p * LS + l]) : 0.0;
+  EL_END;
+  __syncthreads();
+  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
+  EL_LOOP(nL) SXDEC;
+  (void)pp;
+  if (ok && j >= 2) F.hh[1][b + p] = F.hh[1][b + p] - Wt[p * LS + l];
+  EL_END;
+  __syncthreads();
+  EL_LOOP(nL) SXDEC;
+  Wt[p * LS + l] = ok ? 24. * s.dx1 * (Yt[pp * LS + l] * F.dvdx[b + pp] - Yt[p * LS + l] * F.dvdx[b + p]) : 0.0;
+  EL_END;
+  __syncthreads();
+  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
+  EL_LOOP(nL) SXDEC;
+  (void)pp;
+  if (ok && j >= 2) F.hh[1][b + p] = F.hh[1][b + p] - Wt[p * LS + l];
+  EL_END;
+  __syncthreads();
+  EL_LOOP(nL) SXDEC;
+  Wt[p * LS + l] = ok ? 24. * s.dx1 * (F.nue[b + pp] * F.dwdx[b + pp] - F.nue[b + p] * F.dwdx[b + p]) : 0.0;
+  EL_END;
+  __syncthreads();
+  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
+  EL_LOOP(nL) SXDEC;
+  (void)pp;
+  if (ok) F.hh[2][b + p] = F.hh[2][b + p] - Wt[p * LS + l];
+  EL_END;
+  __syncthreads();
+  EL_LOOP(nL) SXDEC;
+  Wt[p * LS + l] = ok ? 24. * s.dx1 * (F.nue[b + pp] * F.dudz[b + pp] - F.nue[b + p] * F.dudz[b + p]) : 0.0;
+  EL_END;
+  __syncthreads();
+  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
+  EL_LOOP(nL) SXDEC;
+  (void)pp;
+  if (ok) F.hh[2][b + p] = F.hh[2][b + p] - Wt[p * LS + l];
+  EL_END;
+#undef SXDEC
+}
+
+// ------------------------------------------------------- test filters
+// FILTER (compact3d.f:5190-5228, width 2) and FILTER4 (:5481-5546, width 4)
+// on a stack of x-z planes FILV(ld1, ld3, n): x pass then z pass, each output
+// recomputing the x-filtered neighbours it needs (same expressions).
+__global__ void k_filter(int nx, int nz, int ld1, int ld3, int np, int width, const double* in,
+                         double* out) {
+  const long long tot = (long long)nx * nz * np;
+  long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= tot) return;
+  const int i = t % nx, k = (t / nx) % nz, q = t / ((long long)nx * nz);
+  const double* f = in + (long long)q * ld1 * ld3;
+  auto F = [&](int ii, int kk) { return f[(long long)kk * ld1 + ii]; };
+  auto wx = [&](int n, int a) { int r = (a + n) % n; return r; };
+  auto tx = [&](int kk) {
+    if (width == 2)
+      return (F(wx(nx, i - 1), kk) + 4.0 * F(i, kk) + F(wx(nx, i + 1), kk)) / 6.0;
+    return (F(wx(nx, i - 2), kk) + 4.0 * F(wx(nx, i - 1), kk) + 2.0 * F(i, kk) + 4.0 * F(wx(nx, i + 1), kk) +
+            F(wx(nx, i + 2), kk)) / 12.0;
+  };
+  double r;
+  if (width == 2)
+    r = (tx(wx(nz, k - 1)) + 4.0 * tx(k) + tx(wx(nz, k + 1))) / 6.0;
+  else
+    r = (tx(wx(nz, k - 2)) + 4.0 * tx(wx(nz, k - 1)) + 2.0 * tx(k) + 4.0 * tx(wx(nz, k + 1)) + tx(wx(nz, k + 2))) /
+        12.0;
+  out[(long long)q * ld1 * ld3 + (long long)k * ld1 + i] = r;
+}
+
+// ================================================================ host
+namespace c3dk {
+
+static int pgrid(long long n) { return (int)((n + C3D_NT - 1) / C3D_NT); }
+static int xgrid(const Ctx& c, const TileShape& t) { return (c.g.n2m * c.g.nz + t.nl - 1) / t.nl; }
+static int zgrid(const Ctx& c, const TileShape& t, int jlo) {
+  return ((c.g.nx + t.nl - 1) / t.nl) * (c.g.n2m - jlo + 1);
+}
+static size_t lds(const TileShape& t, int ntiles) { return sizeof(double) * (size_t)t.n * t.ls * ntiles; }
+
+void sgscs(const Ctx& c) {
+  KScope ks_(c, KID_SGS);
+  hipLaunchKernelGGL(k_sgscs, dim3(pgrid((long long)c.g.nx * c.g.n2m * c.g.nz)), dim3(C3D_NT), 0, c.st, c);
+}
+
+void wallmodel(const Ctx& c) {
+  const int nb = std::min(1024, pgrid((long long)c.g.nx * c.g.nz));
+  {
+    KScope ks_(c, KID_WALL);
+    hipLaunchKernelGGL(k_wallmodel, dim3(nb), dim3(C3D_NT), 0, c.st, c);
+  }
+  KScope ks_(c, KID_WALL);
+  hipLaunchKernelGGL(k_wallmodel_fin, dim3(1), dim3(C3D_NT), 0, c.st, c, nb);
+}
+
+void wallss(const Ctx& c) {
+  const int nb = std::min(1024, pgrid((long long)c.g.nx * c.g.nz));
+  {
+    KScope ks_(c, KID_WALLSS);
+    hipLaunchKernelGGL(k_wallss, dim3(nb), dim3(C3D_NT), 0, c.st, c);
+  }
+  KScope ks_(c, KID_WALLSS);
+  hipLaunchKernelGGL(k_wallss_fin, dim3(1), dim3(C3D_NT), 0, c.st, c, nb);
+}
+
+void sgs_terms(const Ctx& c) {
+  {
+    KScope ks_(c, KID_SGS_X);
+    hipLaunchKernelGGL(k_sgs_x, dim3(xgrid(c, c.o.tx)), dim3(C3D_NT), lds(c.o.tx, 2), c.st, c);
+  }
+  {
+    KScope ks_(c, KID_SGS_Z);
+    hipLaunchKernelGGL(k_sgs_z, dim3(zgrid(c, c.o.tz4, 1)), dim3(C3D_NT), lds(c.o.tz4, 4), c.st, c);
+  }
+  {
+    KScope ks_(c, KID_SGS_Z2);
+    hipLaunchKernelGGL(k_sgs_z2, dim3(zgrid(c, c.o.tz4, 2)), dim3(C3D_NT), lds(c.o.tz4, 3), c.st, c);
+  }
+  KScope ks_(c, KID_SGS_X2);
+  hipLaunchKernelGGL(k_sgs_x2, dim3(xgrid(c, c.o.tx4)), dim3(C3D_NT), lds(c.o.tx4, 3), c.st, c);
+}
+
+void filter(const Ctx& c, int width, const double* in, double* out, int np, int ld1, int ld3) {
+  KScope ks_(c, KID_FILTER);
+  const long long tot = (long long)c.g.nx * c.g.nz * np;
+  hipLaunchKernelGGL(k_filter, dim3(pgrid(tot)), dim3(C3D_NT), 0, c.st, c.g.nx, c.g.nz, ld1, ld3, np, width, in,
+                     out);
+}
+
+}  // namespace c3dk

@@ -1,0 +1,108 @@
+! c3d_wrappers.f90 -- the reference's per-step entry points, redirected to the
+! device.  Argument lists are those of compact3d.f:
+!   GETUP(U,P,HH,SS,NTIME,PRESG,PRESG3,SGSVIS)   :879
+!   DIVCHECK(U,TIME,DIVMAX,NTIME)                :9049
+!   CFL(U,CFLM)                                  :9138
+!   CHKMF(U,TIME)                                :9463
+!   WALLSS(U,P,TIME)                             :11077
+! The state is resident in HBM between calls: C3D_BEGIN uploads it once and
+! C3D_END (or C3D_SYNC) copies it back into the caller's arrays.  The array
+! arguments of the per-step routines are therefore not read or written; they
+! are kept so that MAIN's calls stay unchanged.
+module c3d_state
+  use iso_c_binding
+  use c3d_iface
+  implicit none
+  type(c_ptr), save :: h3d = c_null_ptr
+  integer, save :: m1 = 0, m2 = 0, m3 = 0
+  real(c_double), save :: flow1 = 0.0d0, flow3 = 0.0d0
+end module c3d_state
+
+subroutine C3D_BEGIN(N1, N2, N3, RE, ALX, ALY, ALZ, DT, FILES, WMODEL, CKS, PRECISION, &
+                     U, P, HH, SGSVIS, PRESG, PRESG3)
+  use c3d_state
+  implicit none
+  integer, intent(in) :: N1, N2, N3, PRECISION
+  real(8), intent(in) :: RE, ALX, ALY, ALZ, DT, FILES, WMODEL, CKS, PRESG, PRESG3
+  real(8), intent(in) :: U(0:N1, 0:N2, 0:N3, 3), P(N1, N2, N3), HH(0:N1, 0:N2, 0:N3, 3)
+  real(8), intent(in) :: SGSVIS(0:N1, 0:N2, 0:N3)
+  type(c3d_config) :: cfg
+  integer(c_int) :: rc
+  cfg%n1 = N1; cfg%n2 = N2; cfg%n3 = N3
+  cfg%re = RE; cfg%alx = ALX; cfg%aly = ALY; cfg%alz = ALZ; cfg%dt = DT
+  cfg%files = FILES; cfg%wmodel = WMODEL; cfg%cks = CKS
+  cfg%y = c_null_ptr; cfg%precision = PRECISION; cfg%device = 0; cfg%stream = c_null_ptr
+  rc = c3d_create(cfg, h3d)
+  call c3d_check(rc, c_null_ptr, 'c3d_create')
+  m1 = N1; m2 = N2; m3 = N3
+  rc = c3d_upload_state(h3d, U, P, HH, SGSVIS, PRESG, PRESG3)
+  call c3d_check(rc, h3d, 'c3d_upload_state')
+end subroutine C3D_BEGIN
+
+subroutine C3D_SYNC(U, P, HH, SGSVIS, PRESG, PRESG3)
+  use c3d_state
+  implicit none
+  real(8) :: U(*), P(*), HH(*), SGSVIS(*), PRESG, PRESG3
+  integer(c_int) :: rc
+  rc = c3d_download_state(h3d, U, P, HH, SGSVIS, PRESG, PRESG3)
+  call c3d_check(rc, h3d, 'c3d_download_state')
+end subroutine C3D_SYNC
+
+subroutine C3D_END(U, P, HH, SGSVIS, PRESG, PRESG3)
+  use c3d_state
+  implicit none
+  real(8) :: U(*), P(*), HH(*), SGSVIS(*), PRESG, PRESG3
+  integer(c_int) :: rc
+  call C3D_SYNC(U, P, HH, SGSVIS, PRESG, PRESG3)
+  rc = c3d_destroy(h3d)
+  h3d = c_null_ptr
+end subroutine C3D_END
+
+subroutine GETUP(U, P, HH, SS, NTIME, PRESG, PRESG3, SGSVIS)
+  use c3d_state
+  implicit none
+  real(8) :: U(*), P(*), HH(*), SS(*), SGSVIS(*), PRESG, PRESG3
+  integer :: NTIME
+  integer(c_int) :: rc
+  rc = c3d_getup(h3d, int(NTIME, c_int))
+  call c3d_check(rc, h3d, 'GETUP')
+  rc = c3d_get_scalar(h3d, 0_c_int, PRESG)
+  rc = c3d_get_scalar(h3d, 1_c_int, PRESG3)
+end subroutine GETUP
+
+subroutine DIVCHECK(U, TIME, DIVMAX, NTIME)
+  use c3d_state
+  implicit none
+  real(8) :: U(*), TIME, DIVMAX
+  integer :: NTIME
+  integer(c_int) :: rc
+  rc = c3d_divcheck(h3d, DIVMAX)
+  call c3d_check(rc, h3d, 'DIVCHECK')
+end subroutine DIVCHECK
+
+subroutine CFL(U, CFLM)
+  use c3d_state
+  implicit none
+  real(8) :: U(*), CFLM
+  integer(c_int) :: rc
+  rc = c3d_cfl(h3d, CFLM)
+  call c3d_check(rc, h3d, 'CFL')
+end subroutine CFL
+
+subroutine CHKMF(U, TIME)
+  use c3d_state
+  implicit none
+  real(8) :: U(*), TIME
+  integer(c_int) :: rc
+  rc = c3d_chkmf(h3d, flow1, flow3)
+  call c3d_check(rc, h3d, 'CHKMF')
+end subroutine CHKMF
+
+subroutine WALLSS(U, P, TIME)
+  use c3d_state
+  implicit none
+  real(8) :: U(*), P(*), TIME
+  integer(c_int) :: rc
+  rc = c3d_wallss(h3d)
+  call c3d_check(rc, h3d, 'WALLSS')
+end subroutine WALLSS

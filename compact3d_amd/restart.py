@@ -1,0 +1,67 @@
+"""On-disk state formats.
+
+* READUP's CONTINUE file (compact3d.f:741-746): Fortran unformatted
+  sequential records  PRESG,PRESG3 / U(I,J,K,1:3) with K fastest, J=0..N2,
+  I=1..N1M / P(I,J,K) with K fastest, J=1..N2M, I=1..N1M.  (The reference's
+  own CWRITE writes a formatted file that READUP cannot read, :854-872; these
+  functions are the pair READUP expects.)
+* The build's own checkpoint: an .npz with U, P, HH (the AB2 history that
+  CONTINUE omits), SGSVIS, PRESG, PRESG3 and the grid, so that a resumed run
+  continues bit-for-bit.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+_MARK = np.dtype("<i4")
+
+
+def _rec(f, arr):
+    b = np.ascontiguousarray(arr, dtype="<f8").tobytes()
+    np.array([len(b)], _MARK).tofile(f)
+    f.write(b)
+    np.array([len(b)], _MARK).tofile(f)
+
+
+def _read_rec(f):
+    n = int(np.fromfile(f, _MARK, 1)[0])
+    data = np.frombuffer(f.read(n), dtype="<f8")
+    m = int(np.fromfile(f, _MARK, 1)[0])
+    if m != n:
+        raise ValueError("corrupt record markers")
+    return data
+
+
+def write_continue(path, U, P, presg, presg3):
+    """U (N1+1, N2+1, N3+1, 3) and P (N1, N2, N3) in the reference layout."""
+    n1, n2, n3 = P.shape
+    with open(path, "wb") as f:
+        _rec(f, np.array([presg, presg3]))
+        # (((U(I,J,K,1),U(I,J,K,2),U(I,J,K,3),K=1,N3M),J=0,N2),I=1,N1M)
+        _rec(f, np.transpose(U[1:n1, 0:n2 + 1, 1:n3, :], (0, 1, 2, 3)))
+        # (((P(I,J,K),K=1,N3M),J=1,N2M),I=1,N1M)
+        _rec(f, P[: n1 - 1, : n2 - 1, : n3 - 1])
+
+
+def read_continue(path, n1, n2, n3):
+    U = np.zeros((n1 + 1, n2 + 1, n3 + 1, 3), order="F")
+    P = np.zeros((n1, n2, n3), order="F")
+    with open(path, "rb") as f:
+        presg, presg3 = _read_rec(f)
+        U[1:n1, 0:n2 + 1, 1:n3, :] = _read_rec(f).reshape(n1 - 1, n2 + 1, n3 - 1, 3)
+        P[: n1 - 1, : n2 - 1, : n3 - 1] = _read_rec(f).reshape(n1 - 1, n2 - 1, n3 - 1)
+    return U, P, float(presg), float(presg3)
+
+
+def save_checkpoint(path, channel):
+    st = channel.download()
+    np.savez(path, U=st["U"], P=st["P"], HH=st["HH"], SGSVIS=st["SGSVIS"],
+             PRESG=st["PRESG"], PRESG3=st["PRESG3"], grid=np.array([channel.n1, channel.n2, channel.n3]))
+
+
+def load_checkpoint(path, channel):
+    z = np.load(path, allow_pickle=False)
+    if tuple(z["grid"]) != (channel.n1, channel.n2, channel.n3):
+        raise ValueError("checkpoint grid does not match the channel")
+    channel.upload(np.asfortranarray(z["U"]), np.asfortranarray(z["P"]), HH=np.asfortranarray(z["HH"]),
+                   SGSVIS=np.asfortranarray(z["SGSVIS"]), presg=float(z["PRESG"]), presg3=float(z["PRESG3"]))

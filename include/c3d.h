@@ -86,6 +86,11 @@ int c3d_dpcalc(c3d_handle *h);    /* DPCALC = RHSDP + GETDP compact3d.f:1037-104
 int c3d_upcalc(c3d_handle *h);    /* UPCALC compact3d.f:1052-1238 */
 int c3d_sgs(c3d_handle *h);       /* SGSCS compact3d.f:1413-1488 */
 int c3d_wall_model(c3d_handle *h); /* WALL_MODEL -> WALLMODEL_INSTANT compact3d.f:1952, 2319 */
+int c3d_wallss(c3d_handle *h);    /* WALLSS compact3d.f:11077 (sets WSM(1:3)) */
+
+/* FILTER (width 2, compact3d.f:5190) / FILTER4 (width 4, :5481) of a host
+ * stack of x-z planes FILV(1:N1,1:N3,n); in and out may alias */
+int c3d_filter(c3d_handle *h, int width, const double *filv_in, double *filv_out, int n);
 
 /* DIVCHECK(U,TIME,DIVMAX,NTIME) compact3d.f:9049 */
 int c3d_divcheck(c3d_handle *h, double *divmax);

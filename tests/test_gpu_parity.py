@@ -145,3 +145,100 @@ def test_apply_op_vs_dense(op, direction):
     print("op", op, "dir", direction, e)
     assert e <= 1e-13
     ch.close()
+
+
+def test_les_phases_match_oracle():
+    """Smagorinsky SGSCS, the log-law wall model and the SGS terms of RHS1-3
+    (FILES=2, WMODEL=1, uniform y grid as MESH makes it, compact3d.f:356-357)"""
+    from compact3d_amd import c3d
+    from pyoracle import Oracle
+    n1, n2, n3 = 33, 33, 33
+    cfg = config(n1, n2, n3, re=5200.0, dt=5e-4, files=2.0, wmodel=1.0)
+    orc = Oracle(**cfg)
+    ch = c3d.Channel(**cfg)
+    U, P, HH2 = inputs(orc, seed=7)
+    presg, presg3 = 0.003, -0.001
+    ch.upload(U, P, HH=HH2, presg=presg, presg3=presg3)
+    wsm = [2.5e-3, 0.0, 4.0e-4]
+    orc.set_wsm(wsm)
+    ch.set_scalar(c3d.S_WSM1, wsm[0])
+    ch.set_scalar(c3d.S_WSM3, wsm[2])
+    Uo = U.copy(order="F")
+    So = orc.zeros_u(0)
+    orc.sgscs(Uo, So)
+    ch.sgscs()
+    e = relerr(interior_u(So, n1, n2, n3), interior_u(ch.download()["SGSVIS"], n1, n2, n3))
+    print(f"SGSCS {e:.3e}")
+    assert e <= 1e-13
+    orc.wallmodel_instant(Uo)
+    orc.set_ifwall(1)
+    ch.wall_model()
+    tw, tb = orc.get_tau()
+    e1 = relerr(tw, ch.field(c3d.F_TAUW))
+    e2 = relerr(tb, ch.field(c3d.F_TAUB))
+    ew = abs(orc.get_wsm()[0] - ch.scalar(c3d.S_WSM1)) / orc.get_wsm()[0]
+    print(f"WALLMODEL TAUW {e1:.3e} TAUB {e2:.3e} WSM1 {ew:.3e}")
+    assert e1 <= 1e-12 and e2 <= 1e-12 and ew <= 1e-13
+    orc.bcond(Uo, So)
+    ch.bcond()
+    HHo = orc.zeros_u()
+    RUH = [orc.zeros_u(0) for _ in range(3)]
+    for c in (1, 2, 3):
+        orc.rhs(c, Uo, P, HHo, HH2, RUH[c - 1], presg if c == 1 else presg3, So)
+    ch.rhs()
+    for c in (1, 2, 3):
+        j0 = 2 if c == 2 else 1
+        e1 = relerr(interior_u(RUH[c - 1], n1, n2, n3, j0=j0),
+                    interior_u(ch.field(c3d.F_RUH1 + c - 1), n1, n2, n3, j0=j0))
+        e2 = relerr(interior_u(HHo, n1, n2, n3, c - 1, j0=j0),
+                    interior_u(ch.field(c3d.F_HH1 + c - 1), n1, n2, n3, j0=j0))
+        print(f"LES RHS{c} {e1:.3e}  HH{c} {e2:.3e}")
+        assert e1 <= 1e-12 and e2 <= 1e-12
+    ch.close()
+
+
+@pytest.mark.parametrize("wmodel", [1.0, 0.0])
+def test_les_trajectory_10_steps(wmodel):
+    from compact3d_amd import c3d
+    from pyoracle import Oracle
+    n1, n2, n3 = 49, 33, 33
+    cfg = config(n1, n2, n3, re=20000.0, dt=5e-4, files=2.0, wmodel=wmodel, cks=1.5)
+    orc = Oracle(**cfg)
+    ch = c3d.Channel(**cfg)
+    U, P, _ = inputs(orc, seed=9)
+    ch.upload(U, P)
+    wsm = [1e-3, 0.0, 1e-4]
+    orc.set_wsm(wsm)
+    ch.set_scalar(c3d.S_WSM1, wsm[0])
+    ch.set_scalar(c3d.S_WSM3, wsm[2])
+    HH = orc.zeros_u()
+    S = orc.zeros_u(0)
+    pg = pg3 = 0.0
+    Uo, Po = U.copy(order="F"), P.copy(order="F")
+    for n in range(10):
+        pg, pg3 = orc.getup(Uo, Po, HH, S, pg, pg3, n + 1)
+        ch.getup(n + 1)
+    st = ch.download()
+    eu = [relerr(interior_u(Uo, n1, n2, n3, c), interior_u(st["U"], n1, n2, n3, c)) for c in range(3)]
+    es = relerr(interior_u(S, n1, n2, n3), interior_u(st["SGSVIS"], n1, n2, n3))
+    print("LES 10 steps wmodel", wmodel, "U", ["%.3e" % x for x in eu], "SGSVIS %.3e" % es)
+    assert max(eu) <= 1e-10 and es <= 1e-9
+    ch.close()
+
+
+@pytest.mark.parametrize("width", [2, 4])
+def test_filters_match_oracle(width):
+    from compact3d_amd import c3d
+    from pyoracle import Oracle
+    n1, n2, n3 = 33, 9, 17
+    cfg = config(n1, n2, n3)
+    orc = Oracle(**cfg)
+    ch = c3d.Channel(**cfg)
+    rng = np.random.default_rng(width)
+    F = np.asfortranarray(rng.uniform(-1, 1, (n1, n3, 3)))
+    G = F.copy(order="F")
+    orc.filter(G, four=(width == 4))
+    out = ch.filter(F, width)
+    print("FILTER", width, np.max(np.abs(out - G)))
+    assert np.max(np.abs(out - G)) <= 1e-15
+    ch.close()
