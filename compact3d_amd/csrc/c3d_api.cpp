@@ -69,6 +69,67 @@ static LineSolve make_solve(double a, double b) {
   return s;
 }
 
+// carry-solve parameters of a periodic (a, b, a) system for segments of
+// `seg` points: hop = (-alpha)^seg, nhop terms with |alpha|^(nhop*seg) <= 2^-60
+static PSolve make_psolve(double a, double b, int seg) {
+  const LineSolve ls = make_solve(a, b);
+  PSolve p;
+  p.alpha = ls.alpha;
+  p.scale = ls.scale;
+  long double h = 1.0L;
+  for (int q = 0; q < seg; q++) h *= -(long double)ls.alpha;
+  p.hop = (double)h;
+  const long double aa = fabsl((long double)ls.alpha);
+  if (aa == 0.0L)
+    p.nhop = 0;
+  else
+    p.nhop = std::max(1, (int)ceill(logl(0x1p-60L) / (seg * logl(aa))));
+  return p;
+}
+
+// line-engine tile: seg = the largest compiled segment length (x-lines:
+// 8, 2; z-lines: 16, 8, 2) dividing n, G = n/seg segments, L lines with
+// L*G <= 256; L a multiple of 16 where possible (128-byte rows for z-lines);
+// staged tiles keep 2 LDS row tiles of n*(L+1) plus the exchange arrays
+// within 64 KB
+static bool make_lshape(int n, int segmax, bool staged, size_t es, LShape& sh) {
+  int seg = 2;
+  if (segmax >= 16 && n % 16 == 0)
+    seg = 16;
+  else if (n % 8 == 0)
+    seg = 8;
+  if (n % seg) return false;
+  const int G = n / seg;
+  if (G > 256) return false;
+  int L = std::min(64, 256 / G);
+  if (L >= 16) L -= L % 16;
+  if (staged)
+    while (L > 1 && es * (2 * (size_t)n * (L + 1) + 4 * (size_t)G * L) > 65536) L--;
+  sh.n = n;
+  sh.seg = seg;
+  sh.G = G;
+  sh.L = L;
+  return true;
+}
+
+static OpSet make_opset(int seg, double adi_a, double adi_b) {
+  OpSet o;
+  o.s6 = make_psolve(1.0, 6.0, seg);
+  o.s10 = make_psolve(1.0, 10.0, seg);
+  o.s22 = make_psolve(1.0, 22.0, seg);
+  o.adi = make_psolve(adi_a, adi_b, seg);
+  return o;
+}
+
+// y tiles: C chunks of at most 16 points (C a power of two), L = 256/C <= 64
+static YShape make_yshape(int m) {
+  YShape y;
+  y.C = 1;
+  while (y.C * 16 < m) y.C *= 2;
+  y.L = std::max(1, std::min(64, 256 / y.C));
+  return y;
+}
+
 static TileShape make_tile(int n, int ntiles) {
   TileShape t;
   t.n = n;
@@ -273,6 +334,8 @@ static void build_dt_terms(c3d_handle* h, std::vector<double>& ych) {
   o.adiz_b = 10.0 - DT / 2.0 * (1. / RE) * 12.0 * dx3q * (-2.0);
   o.adix = make_solve(o.adix_a, o.adix_b);
   o.adiz = make_solve(o.adiz_a, o.adiz_b);
+  o.px = make_opset(o.lx.seg, o.adix_a, o.adix_b);
+  o.pz = make_opset(o.lz.seg, o.adiz_a, o.adiz_b);
 }
 
 static int upload_dt_terms(c3d_handle* h) {
@@ -300,7 +363,7 @@ long long c3d_footprint_bytes(const c3d_config* cfg) {
   long long nx = cfg->n1 - 1, nz = cfg->n3 - 1, nj = cfg->n2 + 1, n2m = cfg->n2 - 1;
   long long fsz = nx * nj * nz;
   long long fields = 26;
-  long long spec = (nx / 2 + 1) * nz * n2m * 16;
+  long long spec = 2 * (nx / 2 + 1) * nz * n2m * 16;  // spectrum + TDMAP factors
   return fields * fsz * 8 + spec + 4 * nx * nz * 8 + (1 << 20);
 }
 
@@ -343,12 +406,19 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   f.nz = g.nz;
   if (!factor_fft(g.nx, f.radx, f.nradx) || !factor_fft(g.nz, f.radz, f.nradz))
     return bad("N1-1 and N3-1 must factor into 2, 3 and 5");
+  f.codex = f.codez = 0;
+  for (int q = 0; q < f.nradx; q++) f.codex |= f.radx[q] << (3 * q);
+  for (int q = 0; q < f.nradz; q++) f.codez |= f.radz[q] << (3 * q);
   f.tlx = std::max(1, 2048 / g.nx);
   f.tlz = std::max(1, 2048 / g.nz);
   f.tly = std::max(1, std::min(32, pow2_floor((int)(65536 / (24 * (size_t)g.n2m)))));
-  if (g.n2m > 1300) return bad("N2 too large for the y tiles");
+  if (g.n2m > 4096) return bad("N2 too large for the y tiles");
   build_mesh(h);
   Ops& o = c.o;
+  if (!make_lshape(g.nx, 8, true, sizeof(double), o.lx) || !make_lshape(g.nz, 16, false, sizeof(double), o.lz))
+    return bad("N1-1 and N3-1 must have a divisor <= 16 giving at most 256 segments");
+  o.ys = make_yshape(g.n2m);
+  o.yp = make_yshape(g.n2m);
   o.s6 = make_solve(1.0, 6.0);
   o.s10 = make_solve(1.0, 10.0);
   o.s22 = make_solve(1.0, 22.0);
@@ -429,6 +499,10 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
                    &F.rdp,    &F.dp,     &F.sgs,    &F.div,    &h->dtmp,  &h->dtmp2};
   for (auto pp : fl)
     if (dalloc(h, pp, fs)) return bad(h->err);
+  F.su = F.u;
+  F.sv = F.v;
+  F.sw = F.w;
+  F.sp = F.p;
   for (int s2 = 0; s2 < 2; s2++)
     for (int q = 0; q < 3; q++)
       if (dalloc(h, &h->hhbuf[s2][q], fs)) return bad(h->err);
@@ -447,7 +521,13 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
       if (dalloc(h, pp, fs)) return bad(h->err);
   }
   c3dk::init_attrs();
-  F.pgam = nullptr;
+  {
+    const size_t nmj = (size_t)g.nxh * g.nz * g.n2m;
+    if (dalloc(h, &F.pib, nmj) || dalloc(h, &F.pgt, nmj)) return bad(h->err);
+    c3dk::pois_table(c);
+    if (hipStreamSynchronize(c.st) != hipSuccess || hipGetLastError() != hipSuccess)
+      return bad("Poisson factor set-up failed");
+  }
   if (dalloc(h, &F.sc, SC_COUNT)) return bad(h->err);
   F.nred = 1024;
   if (dalloc(h, &F.red, 4 * (size_t)F.nred)) return bad(h->err);

@@ -235,3 +235,118 @@ __device__ cplx_t<T>* tile_fft(cplx_t<T>* a, cplx_t<T>* b, int n, int nl, int ls
   }
   return a;
 }
+
+// ===================================================== register line engine
+// A line tile is L lines x n points handled by L*G threads, thread
+// t = l + L*g owning the contiguous segment p in [g*SEG, (g+1)*SEG) of line
+// l in registers (n = G*SEG).
+//
+// The periodic system a x[p-1] + b x[p] + a x[p+1] = d[p] factors as
+// (a/alpha)(1 + alpha S^-1)(1 + alpha S) (see LineSolve).  Each recursion
+// y[p] = d[p] - alpha y[p-1] is solved per segment from a zero start; the
+// true start value is the previous segment's end value, which itself is its
+// local end value plus hop = (-alpha)^SEG times the start of the segment
+// before.  Summing nhop terms of that geometric series, with
+// |alpha|^(nhop*SEG) <= 2^-60, gives the periodic solution, after which
+// y[p0+q] = ylocal[q] + (-alpha)^(q+1) * carry.  The backward recursion is
+// the mirror image.  Two barriers per solve; no LDS traffic in the chains.
+#define SEGMAX 16
+
+struct PSolve {
+  double alpha;  // decay factor, |alpha| < 1
+  double scale;  // alpha / a
+  double hop;    // (-alpha)^seg
+  int nhop;      // carry terms
+};
+
+struct LShape {
+  int n;    // line length
+  int L;    // lines per tile
+  int G;    // segments per line
+  int seg;  // points per segment
+};
+
+template <typename T, int SEG>
+__device__ __forceinline__ void psolve(T (&v)[SEG], const int G, const int L, const int l, const int g,
+                                       const PSolve& op, T* xf, T* xb) {
+  const T a = (T)op.alpha;
+  const T na = -a;
+  T y = (T)0;
+#pragma unroll
+  for (int q = 0; q < SEG; q++) {
+    y = v[q] - a * y;
+    v[q] = y;
+  }
+  xf[g * L + l] = y;
+  __syncthreads();
+  const T hop = (T)op.hop;
+  T cy = (T)0, f = (T)1;
+  int gg = g;
+  for (int h = 0; h < op.nhop; h++) {
+    gg = (gg == 0) ? G - 1 : gg - 1;
+    cy += f * xf[gg * L + l];
+    f *= hop;
+  }
+  T pw = na;
+#pragma unroll
+  for (int q = 0; q < SEG; q++) {
+    v[q] += pw * cy;
+    pw *= na;
+  }
+  T z = (T)0;
+#pragma unroll
+  for (int q = SEG - 1; q >= 0; q--) {
+    z = v[q] - a * z;
+    v[q] = z;
+  }
+  xb[g * L + l] = z;
+  __syncthreads();
+  T cz = (T)0;
+  f = (T)1;
+  gg = g;
+  for (int h = 0; h < op.nhop; h++) {
+    gg = (gg == G - 1) ? 0 : gg + 1;
+    cz += f * xb[gg * L + l];
+    f *= hop;
+  }
+  const T sc = (T)op.scale;
+  pw = na;
+#pragma unroll
+  for (int q = SEG - 1; q >= 0; q--) {
+    v[q] = sc * (v[q] + pw * cz);
+    pw *= na;
+  }
+}
+
+// value at p0-1 (end of the left neighbour segment) / at p0+SEG (start of
+// the right one).  One barrier; the buffer must not be reused before the
+// next barrier.
+template <typename T>
+__device__ __forceinline__ T xch_left(const T last, const int G, const int L, const int l, const int g,
+                                      T* x) {
+  x[g * L + l] = last;
+  __syncthreads();
+  return x[(g == 0 ? G - 1 : g - 1) * L + l];
+}
+template <typename T>
+__device__ __forceinline__ T xch_right(const T first, const int G, const int L, const int l,
+                                       const int g, T* x) {
+  x[g * L + l] = first;
+  __syncthreads();
+  return x[(g == G - 1 ? 0 : g + 1) * L + l];
+}
+
+// NaN-propagating max over one wavefront (64 lanes)
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double w = __shfl_xor(v, o, 64);
+    v = (v != v || w != w) ? NAN : fmax(v, w);
+  }
+  return v;
+}
+
+// y-line tile: L columns x C chunks, thread (l, c) = l + L*c
+struct YShape {
+  int L, C;
+};

@@ -30,7 +30,16 @@ struct Scal {
   double wmodel;
 };
 
+// the four periodic solves of one direction, for that direction's segment
+struct OpSet {
+  PSolve s6, s10, s22, adi;
+};
+
 struct Ops {
+  LShape lx, lz;  // register line engine tiles (x-lines staged, z-lines direct)
+  OpSet px, pz;
+  YShape ys;      // ADI y-sweep tiles
+  YShape yp;      // Poisson y-solve tiles
   LineSolve s6, s10, s22, adix, adiz;
   TileShape tx, tz;    // x-line and z-line tile shapes (2 tiles in 64 KB)
   TileShape tx4, tz4;  // shapes for kernels holding 3-4 tiles
@@ -41,13 +50,16 @@ struct FFTPlan {
   int nx, nz;
   int radx[12], nradx, radz[12], nradz;
   const double2 *twxf, *twxb, *twzf, *twzb;  // exp(-+2 pi i m/n)
-  int tlx;  // lines per x-FFT tile
+  int codex, codez;  // radices, 3 bits per stage
+  int tlx;  // complex lines per x-FFT tile (2 rows each)
   int tlz;  // kx columns per z-FFT tile
   int tly;  // modes per y-solve tile
 };
 
 struct Fields {
   double *u, *v, *w, *p;
+  // the state as read by the line kernels (the FP64 arrays themselves)
+  const void *su, *sv, *sw, *sp;
   double *hh[3], *hhp[3];
   double *ruh[3];
   double *uh[3];
@@ -62,7 +74,7 @@ struct Fields {
   double *ubc_u, *ubc_w;  // top ghost values U(I,N2,K,1), U(I,N2,K,3) from BCOND (NX*NZ)
   double *tauw, *taub;    // bottom-wall stresses TAUW(I,K,1), TAUB(I,K,1) (NX*NZ)
   double2 *spec;          // Poisson spectrum [jm][k][kx], NXH*NZ*N2M
-  double *pgam;           // Poisson Thomas scratch, NXH*NZ*N2M
+  double *pib, *pgt;      // Poisson TDMAP factors per mode: 1/BET(J), GAM(J+1) [jm][kz][kx]
   double *sc;             // device scalars (SC_*)
   double *red;            // reduction partials
   int nred;               // number of partial slots
@@ -119,10 +131,14 @@ struct KScope {
 namespace c3dk {
 void bcond(const Ctx& c);
 void rhs(const Ctx& c);
+void rhs_lines_fp64(const Ctx& c);
 void getuh(const Ctx& c);
 void rhsdp(const Ctx& c);
 void getdp(const Ctx& c);
 void upcalc(const Ctx& c, bool fuse_checks);
+void upd_x(const Ctx& c, bool chk);
+void upd_z(const Ctx& c, bool chk);
+void pois_table(const Ctx& c);
 void divcheck(const Ctx& c);
 void cfl(const Ctx& c);
 void chkmf(const Ctx& c, bool reference_sums);

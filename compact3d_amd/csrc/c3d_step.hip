@@ -1,0 +1,1035 @@
+// c3d_step.hip -- the time step of the channel solver on gfx950 (GETUP,
+// compact3d.f:879-960), written on the register line engine of c3d_dev.h.
+//
+// Pass structure of one step:
+//   k_rhs_x, k_rhs_z, k_rhs_x2   RHS1/2/3 x- and z-line terms        :6063-8114
+//   k_rhs_y (c3d_kernels.hip)    pointwise y terms, AB2, wall BC
+//   k_adi_y, k_adi_x, k_adi_z    GETUH1/2/3 (TDMAI, CTDMA1J/3I)      :8218-8627
+//   k_rhsdp_x, k_rhsdp_z         RHSDP                               :8117-8214
+//   k_fft_x, k_fft_z, k_pois_y,  GETDP: real-to-complex x-DFT (two  :9271-9441
+//     k_fft_z, k_fft_xinv          rows per complex transform), z-DFT, one
+//                                  partitioned Thomas solve per mode
+//   k_flow_red/fin, k_upd_x,     UPCALC (+ DIVCHECK and CFL fused)   :1052-1238
+//     k_ghosts, k_upd_z
+//
+// Thread layouts.  z-lines: a tile is L consecutive i at one j; thread
+// (l, g) keeps points k in [g*seg, (g+1)*seg) of column i = i0+l in
+// registers and reads/writes HBM directly (16 lanes = one 128-byte row
+// piece).  x-lines: a tile is L consecutive (j,k) rows; rows are staged
+// through LDS with coalesced row loads, then each thread takes its segment.
+// y-lines (ADI TDMAI, Poisson TDMAP): thread (l, c) owns chunk c of column
+// l; the recurrences are split into chunk-local solves plus an exact
+// carry pass (partitioned Thomas).
+#include <math.h>
+
+#include "c3d_impl.h"
+
+// ------------------------------------------------------------ helpers
+template <typename T, int SEG>
+__device__ __forceinline__ T prv(const T (&v)[SEG], const int q, const T left) {
+  return q == 0 ? left : v[q > 0 ? q - 1 : 0];
+}
+template <typename T, int SEG>
+__device__ __forceinline__ T nxt(const T (&v)[SEG], const int q, const T right) {
+  return (q + 1 < SEG) ? v[q + 1 < SEG ? q + 1 : SEG - 1] : right;
+}
+
+#define SQ _Pragma("unroll") for (int q = 0; q < SEG; q++)
+#define SQR _Pragma("unroll") for (int q = SEG - 1; q >= 0; q--)
+
+// LDS carve-up of a line kernel: two staging tiles (x-lines only) and four
+// exchange arrays of G*L entries
+template <typename T>
+struct LBuf {
+  T *st0, *st1, *xf, *xb, *xl, *xr;
+};
+template <typename T>
+__device__ __forceinline__ LBuf<T> lbuf(unsigned char* sm, const LShape& sh, bool staged) {
+  LBuf<T> b;
+  const int tile = staged ? sh.n * (sh.L + 1) : 0;
+  b.st0 = (T*)sm;
+  b.st1 = b.st0 + tile;
+  b.xf = b.st1 + tile;
+  b.xb = b.xf + sh.G * sh.L;
+  b.xl = b.xb + sh.G * sh.L;
+  b.xr = b.xl + sh.G * sh.L;
+  return b;
+}
+
+// x-line staging.  Lines ln = ln0 + l (l < L) are rows (j, k) with
+// j = 1 + ln % N2M, k = ln / N2M.  Two LDS tiles alternate and every staging
+// operation starts with a barrier, so a tile stays readable until the
+// second staging operation after the one that filled it begins.
+template <typename T>
+struct XStage {
+  T *st0, *st1;
+  int sb;
+  int n, L, LS, ln0, nlines;
+  int l, p0, pm, pp;  // this thread's line, segment start, and periodic neighbours
+  Geo g;
+  __device__ __forceinline__ T* next() {
+    sb ^= 1;
+    return sb ? st1 : st0;
+  }
+  __device__ __forceinline__ long long row(int ll, int joff) const {
+    const int ln = ln0 + ll;
+    return gidx(g, 0, 1 + ln % g.n2m + joff, ln / g.n2m);
+  }
+  // coalesced load of the tile rows (j shifted by joff)
+  template <typename S>
+  __device__ __forceinline__ const T* in(const S* F, int joff) {
+    T* B = next();
+    __syncthreads();
+    const int lmax = min(L, nlines - ln0);
+    for (int ll = 0; ll < lmax; ll++) {
+      const long long r = row(ll, joff);
+      for (int p = threadIdx.x; p < n; p += blockDim.x) B[p * LS + ll] = (T)F[r + p];
+    }
+    __syncthreads();
+    return B;
+  }
+  // element at segment offset q in [-1, SEG] of this thread's line
+  template <int SEG>
+  __device__ __forceinline__ T at(const T* B, int q) const {
+    const int p = q < 0 ? pm : (q >= SEG ? pp : p0 + q);
+    return B[p * LS + l];
+  }
+  // rows with j < jmin are left untouched
+  template <typename S, int SEG>
+  __device__ __forceinline__ void out(S* F, const T (&v)[SEG], int jmin = 1) {
+    T* B = next();
+    __syncthreads();
+    SQ B[(p0 + q) * LS + l] = v[q];
+    __syncthreads();
+    const int lmax = min(L, nlines - ln0);
+    for (int ll = 0; ll < lmax; ll++) {
+      if (1 + (ln0 + ll) % g.n2m < jmin) continue;
+      const long long r = row(ll, 0);
+      for (int p = threadIdx.x; p < n; p += blockDim.x) F[r + p] = (S)B[p * LS + ll];
+    }
+  }
+};
+
+template <typename T>
+__device__ __forceinline__ XStage<T> xstage(const LBuf<T>& b, const Geo& g, const LShape& sh, int l, int p0,
+                                            int seg) {
+  XStage<T> x;
+  x.st0 = b.st0;
+  x.st1 = b.st1;
+  x.sb = 1;
+  x.n = sh.n;
+  x.L = sh.L;
+  x.LS = sh.L + 1;
+  x.ln0 = blockIdx.x * sh.L;
+  x.nlines = g.n2m * g.nz;
+  x.l = l;
+  x.p0 = p0;
+  x.pm = p0 == 0 ? sh.n - 1 : p0 - 1;
+  x.pp = p0 + seg == sh.n ? 0 : p0 + seg;
+  x.g = g;
+  return x;
+}
+
+#define XSETUP                                                        \
+  extern __shared__ __align__(16) unsigned char smem[];               \
+  const Geo g = c.g;                                                  \
+  const LShape sh = c.o.lx;                                           \
+  const int L = sh.L, G = sh.G;                                       \
+  const LBuf<T> bf = lbuf<T>(smem, sh, true);                         \
+  const int l = threadIdx.x % L, gs = threadIdx.x / L, p0 = gs * SEG; \
+  XStage<T> X = xstage<T>(bf, g, sh, l, p0, SEG);                     \
+  const int ln = X.ln0 + l;                                           \
+  const bool ok = ln < X.nlines;                                      \
+  const int j = 1 + (ok ? ln % g.n2m : 0);                            \
+  (void)j;
+#define XA(B, q) X.template at<SEG>(B, q)
+
+// z-lines: a tile is L consecutive i at one j; column i = i0 + l, k = p
+#define ZSETUP                                                        \
+  extern __shared__ __align__(16) unsigned char smem[];               \
+  const Geo g = c.g;                                                  \
+  const LShape sh = c.o.lz;                                           \
+  const int n = sh.n, L = sh.L, G = sh.G;                             \
+  const LBuf<T> bf = lbuf<T>(smem, sh, false);                        \
+  const int l = threadIdx.x % L, gs = threadIdx.x / L, p0 = gs * SEG; \
+  const int nit = (g.nx + L - 1) / L;                                 \
+  const int j = 1 + blockIdx.x / nit;                                 \
+  const int i = (blockIdx.x % nit) * L + l;                           \
+  const bool ok = i < g.nx;                                           \
+  const long long pl = g.plane;                                       \
+  const long long b = gidx(g, ok ? i : 0, j, 0);                      \
+  const long long bm = b + (long long)(p0 == 0 ? n - 1 : p0 - 1) * pl; \
+  const long long bp = b + (long long)(p0 + SEG == n ? 0 : p0 + SEG) * pl;
+#define ZI(q) (b + (long long)(p0 + (q)) * pl)
+
+template <typename T, int SEG, typename S>
+__device__ __forceinline__ void zload(const S* F, long long b, long long pl, int p0, long long bm, long long bp,
+                                      T (&v)[SEG], T& left, T& right) {
+  SQ v[q] = (T)F[b + (long long)(p0 + q) * pl];
+  left = (T)F[bm];
+  right = (T)F[bp];
+}
+
+// ------------------------------------------------------------ RHS x-pass
+// per x-line (j,k), j = 1..N2M:
+//   RUH2  = (1/RE) D2x v                         (j>=2)   (:6879-6895)
+//   VV1   = Ix v                 (stored)                  (:6228-6242, :6946-6959)
+//   HH2   = D1x(UU2 * VV1)                       (j>=2)   (:6961-7004)
+//   RUH1  = (1/RE) D2x u - D1x(P) - PRESG                 (:6142-6157, :6196-6210)
+//   HH1   = D1x((Ix u)^2)                                  (:6295-6334)
+//   RUH3  = (1/RE) D2x w                                   (:7483-7498)
+//   WW1   = Ix w                 (stored)                  (:6380-6394, :7718-7732)
+template <typename T, int SEG>
+__global__ __launch_bounds__(256) void k_rhs_x(Ctx c) {
+  XSETUP;
+  const Scal s = c.s;
+  const Metrics M = c.m;
+  const Fields F = c.d;
+  const PSolve s6 = c.o.px.s6, s10 = c.o.px.s10, s22 = c.o.px.s22;
+  const T dx1 = (T)s.dx1, dx1q = (T)s.dx1q, rinv = (T)s.rinv;
+  const bool j2 = j >= 2;
+  T d[SEG], acc[SEG], vv[SEG];
+  // v: RUH2, VV1
+  const T* B = X.in((const T*)F.sv, 0);
+  SQ d[q] = (T)12. * dx1q * (XA(B, q - 1) - (T)2. * XA(B, q) + XA(B, q + 1));
+  psolve<T, SEG>(d, G, L, l, gs, s10, bf.xf, bf.xb);
+  SQ d[q] = j2 ? rinv * d[q] : (T)0;
+  X.out((T*)F.ruh[1], d);
+  SQ vv[q] = (T)4. * (XA(B, q - 1) + XA(B, q));
+  psolve<T, SEG>(vv, G, L, l, gs, s6, bf.xf, bf.xb);
+  X.out((T*)F.vv1, vv);
+  // UU2 = (DY(J)/2 U(J-1) + DY(J-1)/2 U(J)) / H(J)  (:6966); HH2 = D1x(UU2 VV1)
+  const T hj = (T)(1. / M.h[j]), dyj = (T)(M.dy[j] / 2.), dyjm = (T)(M.dy[j - 1] / 2.);
+  B = X.in((const T*)F.su, -1);
+  SQ acc[q] = dyj * XA(B, q);
+  B = X.in((const T*)F.su, 0);  // u, readable until the second staging op from here
+  SQ acc[q] = j2 ? hj * (acc[q] + dyjm * XA(B, q)) * vv[q] : (T)0;
+  T rt = xch_right(acc[0], G, L, l, gs, bf.xr);
+  SQ d[q] = (T)24. * dx1 * (-acc[q] + nxt(acc, q, rt));
+  psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
+  SQ d[q] = j2 ? d[q] : (T)0;
+  X.out((T*)F.hh[1], d);
+  // u: viscous part of RUH1, HH1 = D1x((Ix u)^2)
+  SQ acc[q] = (T)12. * dx1q * (XA(B, q - 1) - (T)2. * XA(B, q) + XA(B, q + 1));
+  psolve<T, SEG>(acc, G, L, l, gs, s10, bf.xf, bf.xb);
+  SQ d[q] = (T)4. * (XA(B, q) + XA(B, q + 1));
+  psolve<T, SEG>(d, G, L, l, gs, s6, bf.xf, bf.xb);
+  SQ d[q] = d[q] * d[q];
+  const T lf = xch_left(d[SEG - 1], G, L, l, gs, bf.xl);
+  SQR d[q] = (T)24. * dx1 * (d[q] - prv(d, q, lf));
+  psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
+  X.out((T*)F.hh[0], d);
+  // P: RUH1 = (1/RE) D2x u - D1x P - PRESG
+  B = X.in((const T*)F.sp, 0);
+  SQ d[q] = (T)24. * dx1 * (-XA(B, q - 1) + XA(B, q));
+  psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
+  const T presg = (T)F.sc[SC_PRESG];
+  SQ d[q] = rinv * acc[q] - d[q] - presg;
+  X.out((T*)F.ruh[0], d);
+  // w: RUH3, WW1
+  B = X.in((const T*)F.sw, 0);
+  SQ d[q] = (T)12. * dx1q * (XA(B, q - 1) - (T)2. * XA(B, q) + XA(B, q + 1));
+  psolve<T, SEG>(d, G, L, l, gs, s10, bf.xf, bf.xb);
+  SQ d[q] = rinv * d[q];
+  X.out((T*)F.ruh[2], d);
+  SQ d[q] = (T)4. * (XA(B, q - 1) + XA(B, q));
+  psolve<T, SEG>(d, G, L, l, gs, s6, bf.xf, bf.xb);
+  X.out((T*)F.ww1, d);
+}
+
+// ------------------------------------------------------------ RHS z-pass
+// per z-line (i,j):
+//   RUH1 += (1/RE) D2z u                                    (:6170-6185)
+//   UW    = Iz(u) * WW1 (stored);  HH1 += D1z(UW)           (:6360-6433)
+//   RUH2 += (1/RE) D2z v                          (j>=2)   (:6907-6923)
+//   VV3   = Iz v (stored);  HH2 += D1z(VV3*WW2)   (j>=2)   (:7023-7084)
+//   RUH3 += (1/RE) D2z w - D1z(P) - PRESG3                   (:7510-7550)
+//   HH3   = D1z((Iz w)^2)                                    (:7633-7672)
+template <typename T, int SEG>
+__global__ __launch_bounds__(256) void k_rhs_z(Ctx c) {
+  ZSETUP;
+  const Scal s = c.s;
+  const Metrics M = c.m;
+  const Fields F = c.d;
+  const PSolve s6 = c.o.pz.s6, s10 = c.o.pz.s10, s22 = c.o.pz.s22;
+  const T* sw = (const T*)F.sw;
+  T* ruh0 = (T*)F.ruh[0];
+  T* ruh1 = (T*)F.ruh[1];
+  T* ruh2 = (T*)F.ruh[2];
+  T* hh0 = (T*)F.hh[0];
+  T* hh1 = (T*)F.hh[1];
+  T* hh2 = (T*)F.hh[2];
+  const T dx3 = (T)s.dx3, dx3q = (T)s.dx3q, rinv = (T)s.rinv;
+  T a[SEG], d[SEG], e[SEG], lf, rt;
+  // u
+  zload((const T*)F.su, b, pl, p0, bm, bp, a, lf, rt);
+  SQ d[q] = (T)12. * dx3q * (prv(a, q, lf) - (T)2. * a[q] + nxt(a, q, rt));
+  psolve<T, SEG>(d, G, L, l, gs, s10, bf.xf, bf.xb);
+  if (ok) SQ ruh0[ZI(q)] = ruh0[ZI(q)] + rinv * d[q];
+  SQ d[q] = (T)4. * (prv(a, q, lf) + a[q]);
+  psolve<T, SEG>(d, G, L, l, gs, s6, bf.xf, bf.xb);
+  SQ d[q] = d[q] * ((const T*)F.ww1)[ZI(q)];  // UW
+  if (ok) SQ((T*)F.uw)[ZI(q)] = d[q];
+  rt = xch_right(d[0], G, L, l, gs, bf.xr);
+  SQ d[q] = (T)24. * dx3 * (-d[q] + nxt(d, q, rt));
+  psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
+  if (ok) SQ hh0[ZI(q)] = hh0[ZI(q)] + d[q];
+  // v
+  zload((const T*)F.sv, b, pl, p0, bm, bp, a, lf, rt);
+  if (j >= 2) {
+    SQ d[q] = (T)12. * dx3q * (prv(a, q, lf) - (T)2. * a[q] + nxt(a, q, rt));
+    psolve<T, SEG>(d, G, L, l, gs, s10, bf.xf, bf.xb);
+    if (ok) SQ ruh1[ZI(q)] = ruh1[ZI(q)] + rinv * d[q];
+  }
+  SQ d[q] = (T)4. * (prv(a, q, lf) + a[q]);
+  psolve<T, SEG>(d, G, L, l, gs, s6, bf.xf, bf.xb);
+  if (ok) SQ((T*)F.vv3)[ZI(q)] = d[q];
+  if (j >= 2) {
+    // WW2 = (DY(J)/2 W(J-1) + DY(J-1)/2 W(J)) / H(J)  (:7028)
+    const T hj = (T)(1. / M.h[j]), dyj = (T)(M.dy[j] / 2.), dyjm = (T)(M.dy[j - 1] / 2.);
+    SQ d[q] = d[q] * (hj * (dyj * sw[ZI(q) - g.nx] + dyjm * sw[ZI(q)]));
+    rt = xch_right(d[0], G, L, l, gs, bf.xr);
+    SQ d[q] = (T)24. * dx3 * (-d[q] + nxt(d, q, rt));
+    psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
+    if (ok) SQ hh1[ZI(q)] = hh1[ZI(q)] + d[q];
+  }
+  // w, P: RUH3
+  zload(sw, b, pl, p0, bm, bp, a, lf, rt);
+  SQ e[q] = (T)12. * dx3q * (prv(a, q, lf) - (T)2. * a[q] + nxt(a, q, rt));
+  psolve<T, SEG>(e, G, L, l, gs, s10, bf.xf, bf.xb);
+  {
+    const T* sp = (const T*)F.sp;
+    T pv = sp[bm];
+    SQ {
+      const T pc = sp[ZI(q)];
+      d[q] = (T)24. * dx3 * (-pv + pc);
+      pv = pc;
+    }
+  }
+  psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
+  const T presg3 = (T)F.sc[SC_PRESG3];
+  if (ok) SQ ruh2[ZI(q)] = ruh2[ZI(q)] + rinv * e[q] - d[q] - presg3;
+  // HH3 = D1z((Iz w)^2), w face -> centre -> face
+  SQ d[q] = (T)4. * (a[q] + nxt(a, q, rt));
+  psolve<T, SEG>(d, G, L, l, gs, s6, bf.xf, bf.xb);
+  SQ d[q] = d[q] * d[q];
+  lf = xch_left(d[SEG - 1], G, L, l, gs, bf.xl);
+  SQR d[q] = (T)24. * dx3 * (d[q] - prv(d, q, lf));
+  psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
+  if (ok) SQ hh2[ZI(q)] = d[q];
+}
+
+// -------------------------------------------------------- RHS x-pass 2
+// HH3 += D1x(UU3*WW1) face -> centre (:7734-7769)
+template <typename T, int SEG>
+__global__ __launch_bounds__(256) void k_rhs_x2(Ctx c) {
+  XSETUP;
+  const T dx1 = (T)c.s.dx1;
+  T d[SEG];
+  const T* B = X.in((const T*)c.d.uw, 0);
+  SQ d[q] = (T)24. * dx1 * (-XA(B, q) + XA(B, q + 1));
+  psolve<T, SEG>(d, G, L, l, gs, c.o.px.s22, bf.xf, bf.xb);
+  B = X.in((const T*)c.d.hh[2], 0);
+  SQ d[q] = XA(B, q) + d[q];
+  X.out((T*)c.d.hh[2], d);
+}
+
+// ------------------------------------------------------------- GETUH y
+// TDMAI along y (compact3d.f:8630-8659) with the J-only chains (A, BET, GAM)
+// of the host, as a partitioned recurrence: thread (l, c) holds chunk c of
+// column i = i0 + l.  Forward x(J) = BET(J)(R(J) - A(J) x(J-1)) is
+// y = c + m y_prev with m = -A BET; backward x(J) -= GAM(J) x(J+1).
+#define CHMAX 16
+struct YPos {
+  int l, c, j0, len;
+};
+
+// chunk-parallel linear recurrence v[q] <- v[q] + m[q] * v[q-1] (forward)
+template <typename T>
+__device__ __forceinline__ void ychain_fwd(T (&v)[CHMAX], const T (&m)[CHMAX], const YPos& y, int C, int L,
+                                           T* xa, T* xm) {
+  T r = (T)0, P = (T)1, Pq[CHMAX];
+#pragma unroll
+  for (int q = 0; q < CHMAX; q++)
+    if (q < y.len) {
+      r = v[q] + m[q] * r;
+      v[q] = r;
+      P *= m[q];
+      Pq[q] = P;
+    }
+  xa[y.c * L + y.l] = r;
+  xm[y.c * L + y.l] = P;
+  __syncthreads();
+  T Y = (T)0;
+  for (int cc = 0; cc < y.c; cc++) Y = xa[cc * L + y.l] + xm[cc * L + y.l] * Y;
+#pragma unroll
+  for (int q = 0; q < CHMAX; q++)
+    if (q < y.len) v[q] += Pq[q] * Y;
+}
+// backward: v[q] <- v[q] + m[q] * v[q+1]
+template <typename T>
+__device__ __forceinline__ void ychain_bwd(T (&v)[CHMAX], const T (&m)[CHMAX], const YPos& y, int C, int L,
+                                           T* xa, T* xm) {
+  T r = (T)0, P = (T)1, Pq[CHMAX];
+#pragma unroll
+  for (int q = CHMAX - 1; q >= 0; q--)
+    if (q < y.len) {
+      r = v[q] + m[q] * r;
+      v[q] = r;
+      P *= m[q];
+      Pq[q] = P;
+    }
+  xa[y.c * L + y.l] = r;
+  xm[y.c * L + y.l] = P;
+  __syncthreads();
+  T Z = (T)0;
+  for (int cc = C - 1; cc > y.c; cc--) Z = xa[cc * L + y.l] + xm[cc * L + y.l] * Z;
+#pragma unroll
+  for (int q = 0; q < CHMAX; q++)
+    if (q < y.len) v[q] += Pq[q] * Z;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_adi_y(Ctx c) {
+  __shared__ T xs[4][256];
+  const Geo g = c.g;
+  const YShape ys = c.o.ys;
+  const int comp = blockIdx.y;
+  const int ns = (comp == 1) ? 2 : 1;
+  const int L = ys.L, C = ys.C;
+  const int t = threadIdx.x, l = t % L;
+  YPos y;
+  y.l = l;
+  y.c = t / L;
+  const int m = g.n2m - ns + 1;
+  const int ch = (m + C - 1) / C;
+  y.j0 = ns + y.c * ch;
+  y.len = max(0, min(ch, g.n2m + 1 - y.j0));
+  const int nit = (g.nx + L - 1) / L;
+  const int it = blockIdx.x % nit;
+  const int k = blockIdx.x / nit;
+  const int i = it * L + l;
+  const bool ok = i < g.nx;
+  const Fields F = c.d;
+  const Metrics M = c.m;
+  const T* R = (const T*)(comp == 0 ? F.ruh[0] : comp == 1 ? F.ruh[1] : F.ruh[2]);
+  T* X = (T*)(comp == 0 ? F.uh[0] : comp == 1 ? F.uh[1] : F.uh[2]);
+  const double* A = comp == 0 ? M.ya[0] : comp == 1 ? M.ya[1] : M.ya[2];
+  const double* BET = comp == 0 ? M.ybet[0] : comp == 1 ? M.ybet[1] : M.ybet[2];
+  const double* GAM = comp == 0 ? M.ygam[0] : comp == 1 ? M.ygam[1] : M.ygam[2];
+  const T dt = (T)c.s.dt;
+  const long long b = gidx(g, ok ? i : 0, 0, k);
+  T v[CHMAX], mm[CHMAX];
+#pragma unroll
+  for (int q = 0; q < CHMAX; q++)
+    if (q < y.len) {
+      const int J = y.j0 + q;
+      const T bet = (T)BET[J];
+      v[q] = bet * (R[b + (long long)J * g.nx] * dt);
+      mm[q] = (J == ns) ? (T)0 : -(T)A[J] * bet;
+    }
+  ychain_fwd(v, mm, y, C, L, xs[0], xs[1]);
+#pragma unroll
+  for (int q = 0; q < CHMAX; q++)
+    if (q < y.len) {
+      const int J = y.j0 + q;
+      mm[q] = (J == g.n2m) ? (T)0 : -(T)GAM[J];
+    }
+  ychain_bwd(v, mm, y, C, L, xs[2], xs[3]);
+  if (ok)
+#pragma unroll
+    for (int q = 0; q < CHMAX; q++)
+      if (q < y.len) X[b + (long long)(y.j0 + q) * g.nx] = v[q];
+}
+
+// --------------------------------------------------- GETUH x / z sweeps
+// CTDMA1J (x, :8285-8311) / CTDMA3I (z, :8314-8338): solve
+// (A, B, A) x = UH(p-1) + 10 UH(p) + UH(p+1); `fin` adds U (UH = U + delta,
+// :8610-8624).  blockIdx.y indexes the component list.
+struct CompList {
+  int n;
+  int comp[3];
+  int fin[3];
+};
+
+template <typename T, int SEG>
+__global__ __launch_bounds__(256) void k_adi_x(Ctx c, CompList cl) {
+  XSETUP;
+  const int sel = blockIdx.y;
+  const int comp = sel == 0 ? cl.comp[0] : sel == 1 ? cl.comp[1] : cl.comp[2];
+  const bool fin = (sel == 0 ? cl.fin[0] : sel == 1 ? cl.fin[1] : cl.fin[2]) != 0;
+  const Fields F = c.d;
+  T* UH = (T*)(comp == 0 ? F.uh[0] : comp == 1 ? F.uh[1] : F.uh[2]);
+  const double* U = comp == 0 ? F.u : comp == 1 ? F.v : F.w;
+  T d[SEG];
+  const T* B = X.in((const T*)UH, 0);
+  SQ d[q] = XA(B, q - 1) + (T)10. * XA(B, q) + XA(B, q + 1);
+  psolve<T, SEG>(d, G, L, l, gs, c.o.px.adi, bf.xf, bf.xb);
+  if (fin) {
+    B = X.in(U, 0);
+    SQ d[q] = XA(B, q) + d[q];
+  }
+  X.out(UH, d, comp == 1 ? 2 : 1);  // V rows J=1 are not part of the system
+}
+
+template <typename T, int SEG>
+__global__ __launch_bounds__(256) void k_adi_z(Ctx c, CompList cl) {
+  ZSETUP;
+  const int sel = blockIdx.y;
+  const int comp = sel == 0 ? cl.comp[0] : sel == 1 ? cl.comp[1] : cl.comp[2];
+  const bool fin = (sel == 0 ? cl.fin[0] : sel == 1 ? cl.fin[1] : cl.fin[2]) != 0;
+  if (comp == 1 && j < 2) return;  // uniform per block
+  const Fields F = c.d;
+  T* UH = (T*)(comp == 0 ? F.uh[0] : comp == 1 ? F.uh[1] : F.uh[2]);
+  const double* U = comp == 0 ? F.u : comp == 1 ? F.v : F.w;
+  T a[SEG], d[SEG], lf, rt;
+  zload((const T*)UH, b, pl, p0, bm, bp, a, lf, rt);
+  SQ d[q] = prv(a, q, lf) + (T)10. * a[q] + nxt(a, q, rt);
+  psolve<T, SEG>(d, G, L, l, gs, c.o.pz.adi, bf.xf, bf.xb);
+  if (ok) {
+    if (fin) {
+      SQ UH[ZI(q)] = (T)U[ZI(q)] + d[q];
+    } else {
+      SQ UH[ZI(q)] = d[q];
+    }
+  }
+}
+
+// ------------------------------------------------------------------ RHSDP
+// x-lines: RDP = (FJUP*UH2(J+1) - FJUM*UH2(J))/DY/DT + D1x(UH1)/DT  (:8138-8186)
+// (the CBC term is zero: UBC(.,.,2,.) = 0, :978, :986)
+template <typename T, int SEG>
+__global__ __launch_bounds__(256) void k_rhsdp_x(Ctx c) {
+  XSETUP;
+  const Fields F = c.d;
+  const Metrics M = c.m;
+  const T dx1 = (T)c.s.dx1, dt = (T)c.s.dt;
+  T d[SEG], e[SEG];
+  const T* B = X.in((const T*)F.uh[0], 0);
+  SQ d[q] = (T)24. * dx1 * (XA(B, q + 1) - XA(B, q));
+  psolve<T, SEG>(d, G, L, l, gs, c.o.px.s22, bf.xf, bf.xb);
+  const T fjpa = (T)M.fjpa[j], fjmu = (T)M.fjmu[j], dy = (T)M.dy[j];
+  B = X.in((const T*)F.uh[1], 1);
+  SQ e[q] = fjpa * XA(B, q);
+  B = X.in((const T*)F.uh[1], 0);
+  SQ d[q] = (e[q] - fjmu * XA(B, q)) / dy / dt + d[q] / dt;
+  X.out((T*)F.rdp, d);
+}
+
+// z-lines: RDP += D1z(UH3)/DT (:8196-8211)
+template <typename T, int SEG>
+__global__ __launch_bounds__(256) void k_rhsdp_z(Ctx c) {
+  ZSETUP;
+  const T dx3 = (T)c.s.dx3, dt = (T)c.s.dt;
+  T a[SEG], d[SEG], lf, rt;
+  zload((const T*)c.d.uh[2], b, pl, p0, bm, bp, a, lf, rt);
+  SQ d[q] = (T)24. * dx3 * (nxt(a, q, rt) - a[q]);
+  psolve<T, SEG>(d, G, L, l, gs, c.o.pz.s22, bf.xf, bf.xb);
+  T* rdp = (T*)c.d.rdp;
+  if (ok) SQ rdp[ZI(q)] = rdp[ZI(q)] + d[q] / dt;
+}
+
+// ------------------------------------------------------------------ GETDP
+// The plane DFT of :9325-9346 is factored as a real x-DFT (two rows packed
+// into one complex transform: z = a + i b, A[k] = (Z[k] + conj Z[n-k])/2,
+// B[k] = (Z[k] - conj Z[n-k])/(2i)), keeping kx = 0..NX/2, then complex
+// z-DFTs of that half spectrum.  Spectrum layout [jm][kz][kx].
+template <typename T, int R>
+__device__ __forceinline__ void dft_r(cplx_t<T> (&v)[R], int sign) {
+  if (R == 2) {
+    cplx_t<T> a = v[0], b = v[1];
+    v[0] = {a.x + b.x, a.y + b.y};
+    v[1] = {a.x - b.x, a.y - b.y};
+  } else if (R == 4) {
+    cplx_t<T> a0 = {v[0].x + v[2].x, v[0].y + v[2].y};
+    cplx_t<T> a1 = {v[0].x - v[2].x, v[0].y - v[2].y};
+    cplx_t<T> a2 = {v[1].x + v[3].x, v[1].y + v[3].y};
+    cplx_t<T> a3 = {v[1].x - v[3].x, v[1].y - v[3].y};
+    cplx_t<T> a3r = (sign < 0) ? cplx_t<T>{a3.y, -a3.x} : cplx_t<T>{-a3.y, a3.x};
+    v[0] = {a0.x + a2.x, a0.y + a2.y};
+    v[2] = {a0.x - a2.x, a0.y - a2.y};
+    v[1] = {a1.x + a3r.x, a1.y + a3r.y};
+    v[3] = {a1.x - a3r.x, a1.y - a3r.y};
+  } else if (R == 3) {
+    const T c1 = (T)-0.5;
+    const T s1 = (T)(sign < 0 ? -0.86602540378443864676 : 0.86602540378443864676);
+    cplx_t<T> a = v[0], b = v[1], cc = v[2];
+    cplx_t<T> t1 = {b.x + cc.x, b.y + cc.y};
+    cplx_t<T> t2 = {a.x + c1 * t1.x, a.y + c1 * t1.y};
+    cplx_t<T> t3 = {s1 * (b.x - cc.x), s1 * (b.y - cc.y)};
+    v[0] = {a.x + t1.x, a.y + t1.y};
+    v[1] = {t2.x - t3.y, t2.y + t3.x};
+    v[2] = {t2.x + t3.y, t2.y - t3.x};
+  } else if (R == 5) {
+    const T cs[5] = {(T)1.0, (T)0.30901699437494742410, (T)-0.80901699437494742410,
+                     (T)-0.80901699437494742410, (T)0.30901699437494742410};
+    const T sn[5] = {(T)0.0, (T)0.95105651629515357212, (T)0.58778525229247312917,
+                     (T)-0.58778525229247312917, (T)-0.95105651629515357212};
+    cplx_t<T> in[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) in[q] = v[q];
+#pragma unroll
+    for (int p = 0; p < 5; p++) {
+      T ax = 0, ay = 0;
+#pragma unroll
+      for (int q = 0; q < 5; q++) {
+        const int mq = (p * q) % 5;
+        const T cq = cs[mq], sq = (sign < 0 ? -sn[mq] : sn[mq]);
+        ax += in[q].x * cq - in[q].y * sq;
+        ay += in[q].x * sq + in[q].y * cq;
+      }
+      v[p] = {ax, ay};
+    }
+  }
+}
+
+// one Stockham stage of radix R over nl lines in LDS (a -> b)
+template <typename T, int R>
+__device__ __forceinline__ void fft_stage(const cplx_t<T>* a, cplx_t<T>* b, int n, int nl, int ls, int ns,
+                                          const cplx_t<T>* tw, int sign) {
+  const int nb = n / R;
+  const int tot = nb * nl;
+  const int tq = n / (ns * R);
+  for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+    const int l = e % nl;
+    const int jb = e / nl;
+    const int k = jb % ns;
+    const int ts = k * tq;  // twiddle step; ts*r < n
+    cplx_t<T> v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const cplx_t<T> x = a[(jb + r * nb) * ls + l];
+      v[r] = (r == 0) ? x : cmul(x, tw[ts * r]);
+    }
+    dft_r<T, R>(v, sign);
+    const int d = (jb / ns) * ns * R + k;
+#pragma unroll
+    for (int r = 0; r < R; r++) b[(d + r * ns) * ls + l] = v[r];
+  }
+}
+
+// mixed-radix transform; radices packed 3 bits each in `code`, `nrad` stages
+template <typename T>
+__device__ __forceinline__ cplx_t<T>* fft_lines(cplx_t<T>* a, cplx_t<T>* b, int n, int nl, int ls, int code, int nrad,
+                                const cplx_t<T>* tw, int sign) {
+  int ns = 1;
+  for (int s = 0; s < nrad; s++) {
+    const int R = (code >> (3 * s)) & 7;
+    if (R == 4)
+      fft_stage<T, 4>(a, b, n, nl, ls, ns, tw, sign);
+    else if (R == 2)
+      fft_stage<T, 2>(a, b, n, nl, ls, ns, tw, sign);
+    else if (R == 3)
+      fft_stage<T, 3>(a, b, n, nl, ls, ns, tw, sign);
+    else
+      fft_stage<T, 5>(a, b, n, nl, ls, ns, tw, sign);
+    __syncthreads();
+    cplx_t<T>* t = a;
+    a = b;
+    b = t;
+    ns *= R;
+  }
+  return a;
+}
+
+// forward x-DFT of RDP rows (J=1..N2M, K), 2*TL rows per block
+template <typename T>
+__global__ __launch_bounds__(256) void k_fft_x(Ctx c) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  typedef cplx_t<T> zc;
+  const Geo g = c.g;
+  const int n = g.nx, NL = c.f.tlx, LS = NL;  // NL complex lines = 2*NL rows
+  zc* A = (zc*)smem;
+  zc* B = A + n * LS;
+  const int nlines = g.n2m * g.nz;
+  const int ln0 = blockIdx.x * 2 * NL;
+  const T* src = (const T*)c.d.rdp;
+  for (int l = 0; l < NL; l++) {
+    const int la = ln0 + l, lb = ln0 + NL + l;
+    const long long ra = la < nlines ? gidx(g, 0, 1 + la % g.n2m, la / g.n2m) : -1;
+    const long long rb = lb < nlines ? gidx(g, 0, 1 + lb % g.n2m, lb / g.n2m) : -1;
+    for (int p = threadIdx.x; p < n; p += blockDim.x)
+      A[p * LS + l] = {ra >= 0 ? src[ra + p] : (T)0, rb >= 0 ? src[rb + p] : (T)0};
+  }
+  __syncthreads();
+  const zc* R = fft_lines<T>(A, B, n, NL, LS, c.f.codex, c.f.nradx, (const zc*)c.f.twxf, -1);
+  const int nxh = g.nxh;
+  for (int l = 0; l < NL; l++) {
+    const int la = ln0 + l, lb = ln0 + NL + l;
+    for (int kx = threadIdx.x; kx < nxh; kx += blockDim.x) {
+      const zc z1 = R[kx * LS + l];
+      const zc z2 = R[(kx == 0 ? 0 : n - kx) * LS + l];
+      // A = (z1 + conj z2)/2, B = (z1 - conj z2)/(2i)
+      const zc va = {(T)0.5 * (z1.x + z2.x), (T)0.5 * (z1.y - z2.y)};
+      const zc vb = {(T)0.5 * (z1.y + z2.y), (T)-0.5 * (z1.x - z2.x)};
+      if (la < nlines) {
+        const int jm = la % g.n2m, k = la / g.n2m;
+        ((zc*)c.d.spec)[((long long)jm * g.nz + k) * nxh + kx] = va;
+      }
+      if (lb < nlines) {
+        const int jm = lb % g.n2m, k = lb / g.n2m;
+        ((zc*)c.d.spec)[((long long)jm * g.nz + k) * nxh + kx] = vb;
+      }
+    }
+  }
+}
+
+// z-DFT of the half spectrum, columns (jm, kx); sign -1 forward, +1 backward
+template <typename T>
+__global__ __launch_bounds__(256) void k_fft_z(Ctx c, int sign) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  typedef cplx_t<T> zc;
+  const Geo g = c.g;
+  const int n = g.nz, L = c.f.tlz, LS = L;
+  zc* A = (zc*)smem;
+  zc* B = A + n * LS;
+  const int nit = (g.nxh + L - 1) / L;
+  const int it = blockIdx.x % nit;
+  const int jm = blockIdx.x / nit;
+  zc* S = (zc*)c.d.spec + (long long)jm * g.nz * g.nxh;
+  const int tot = n * L;
+  for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+    const int l = e % L, p = e / L;
+    const int kx = it * L + l;
+    A[p * LS + l] = kx < g.nxh ? S[(long long)p * g.nxh + kx] : zc{(T)0, (T)0};
+  }
+  __syncthreads();
+  const zc* R = fft_lines<T>(A, B, n, L, LS, c.f.codez, c.f.nradz,
+                             (const zc*)(sign < 0 ? c.f.twzf : c.f.twzb), sign);
+  for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+    const int l = e % L, p = e / L;
+    const int kx = it * L + l;
+    if (kx < g.nxh) S[(long long)p * g.nxh + kx] = R[p * LS + l];
+  }
+}
+
+// Thomas solve in y per Fourier mode (TDMAP, :9352-9406, :9444-9459) with
+// the reference's pinning: real part of mode (0,0) has row 1 replaced by the
+// identity with zero RHS; imaginary parts of the self-conjugate modes
+// (kx in {0, NX/2}) x (kz in {0, NZ/2}) are zero.  The elimination factors
+// depend on the mode only: IBET(J) = 1/BET(J) and GAM(J+1) are tabulated at
+// set-up (k_pois_table); here each thread (l, c) solves chunk c of mode
+// kx = kx0 + l by the partitioned recurrences.
+template <typename T>
+__global__ __launch_bounds__(256) void k_pois_y(Ctx c) {
+  __shared__ T xs[8][256];
+  typedef cplx_t<T> zc;
+  const Geo g = c.g;
+  const YShape ys = c.o.yp;
+  const int L = ys.L, C = ys.C;
+  const int t = threadIdx.x, l = t % L;
+  YPos y;
+  y.l = l;
+  y.c = t / L;
+  const int ch = (g.n2m + C - 1) / C;
+  y.j0 = y.c * ch;  // jm index (J - 1)
+  y.len = max(0, min(ch, g.n2m - y.j0));
+  const int nit = (g.nxh + L - 1) / L;
+  const int it = blockIdx.x % nit;
+  const int kz = blockIdx.x / nit;
+  const int kx = it * L + l;
+  const bool ok = kx < g.nxh;
+  const long long jstr = (long long)g.nz * g.nxh;
+  const long long b = (long long)kz * g.nxh + (ok ? kx : 0);
+  zc* S = (zc*)c.d.spec;
+  const double* IB = c.d.pib;
+  const double* GT = c.d.pgt;
+  const double* pmj = c.m.pmj;
+  const bool selfconj = (kx == 0 || 2 * kx == g.nx) && (kz == 0 || 2 * kz == g.nz);
+  const bool pin = (kx == 0 && kz == 0);
+  T vr[CHMAX], vi[CHMAX], mm[CHMAX];
+#pragma unroll
+  for (int q = 0; q < CHMAX; q++)
+    if (q < y.len) {
+      const int jm = y.j0 + q;
+      const long long a = jm * jstr + b;
+      const zc r = S[a];
+      const T ib = (T)IB[a];
+      vr[q] = (pin && jm == 0) ? (T)0 : r.x * ib;
+      vi[q] = selfconj ? (T)0 : r.y * ib;
+      mm[q] = (jm == 0) ? (T)0 : -(T)pmj[jm + 1] * ib;
+    }
+  ychain_fwd(vr, mm, y, C, L, xs[0], xs[1]);
+  ychain_fwd(vi, mm, y, C, L, xs[2], xs[3]);
+#pragma unroll
+  for (int q = 0; q < CHMAX; q++)
+    if (q < y.len) {
+      const int jm = y.j0 + q;
+      mm[q] = (jm == g.n2m - 1) ? (T)0 : -(T)GT[jm * jstr + b];
+    }
+  ychain_bwd(vr, mm, y, C, L, xs[4], xs[5]);
+  ychain_bwd(vi, mm, y, C, L, xs[6], xs[7]);
+  if (ok)
+#pragma unroll
+    for (int q = 0; q < CHMAX; q++)
+      if (q < y.len) S[(y.j0 + q) * jstr + b] = zc{vr[q], vi[q]};
+}
+
+// set-up: per mode, BET(J) and GAM(J) of TDMAP on (PMJ, PCJ - AK1 - AK3, PPJ)
+// with the pinned first row of mode (0,0); IB = 1/BET(J), GT(J) = GAM(J+1)
+__global__ void k_pois_table(Ctx c) {
+  const Geo g = c.g;
+  const long long nm = (long long)g.nxh * g.nz;
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= nm) return;
+  const int kx = t % g.nxh, kz = t / g.nxh;
+  const Metrics M = c.m;
+  const int kk = (kz <= g.nz / 2) ? kz : g.nz - kz;  // AK3 is even in kz
+  const double ak = M.ak3[kk + 1];
+  const double a1 = M.ak1[kx + 1];
+  const bool pin = (kx == 0 && kz == 0);
+  const long long jstr = (long long)g.nz * g.nxh;
+  double bet = pin ? 1.0 : (M.pcj[1] - ak - a1);
+  c.d.pib[t] = 1.0 / bet;
+  for (int jm = 1; jm < g.n2m; jm++) {
+    const int J = jm + 1;
+    const double cprev = (pin && J == 2) ? 0.0 : M.ppj[J - 1];
+    const double gam = cprev / bet;
+    bet = (M.pcj[J] - ak - a1) - M.pmj[J] * gam;
+    c.d.pib[jm * jstr + t] = 1.0 / bet;
+    c.d.pgt[(jm - 1) * jstr + t] = gam;
+  }
+  c.d.pgt[(g.n2m - 1) * jstr + t] = 0.0;
+}
+
+// inverse x-DFT from the half spectrum (Hermitian fill, :9419-9425), real
+// part / (N1M*N3M) into DP (:9429-9433); two rows per complex transform:
+// Z = A + iB with the imaginary parts of A and B at kx = 0 and NX/2 dropped
+// (they do not reach the real part of the reference's transform)
+template <typename T>
+__global__ __launch_bounds__(256) void k_fft_xinv(Ctx c) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  typedef cplx_t<T> zc;
+  const Geo g = c.g;
+  const int n = g.nx, NL = c.f.tlx, LS = NL;
+  zc* A = (zc*)smem;
+  zc* B = A + n * LS;
+  const int nlines = g.n2m * g.nz;
+  const int ln0 = blockIdx.x * 2 * NL;
+  const zc* S = (const zc*)c.d.spec;
+  const int nxh = g.nxh;
+  for (int l = 0; l < NL; l++) {
+    const int la = ln0 + l, lb = ln0 + NL + l;
+    const long long sa = la < nlines ? ((long long)(la % g.n2m) * g.nz + la / g.n2m) * nxh : -1;
+    const long long sb = lb < nlines ? ((long long)(lb % g.n2m) * g.nz + lb / g.n2m) * nxh : -1;
+    for (int kx = threadIdx.x; kx < n; kx += blockDim.x) {
+      const bool lo = kx < nxh;
+      const int kk = lo ? kx : n - kx;
+      zc xa = sa >= 0 ? S[sa + kk] : zc{(T)0, (T)0};
+      zc xb = sb >= 0 ? S[sb + kk] : zc{(T)0, (T)0};
+      if (kx == 0 || 2 * kx == n) {
+        xa.y = (T)0;
+        xb.y = (T)0;
+      }
+      if (!lo) {
+        xa.y = -xa.y;
+        xb.y = -xb.y;
+      }
+      A[kx * LS + l] = {xa.x - xb.y, xa.y + xb.x};
+    }
+  }
+  __syncthreads();
+  const zc* R = fft_lines<T>(A, B, n, NL, LS, c.f.codex, c.f.nradx, (const zc*)c.f.twxb, +1);
+  const T norm = (T)(1.0 / ((double)g.nx * (double)g.nz));
+  T* dp = (T*)c.d.dp;
+  for (int l = 0; l < NL; l++) {
+    const int la = ln0 + l, lb = ln0 + NL + l;
+    for (int p = threadIdx.x; p < n; p += blockDim.x) {
+      const zc z = R[p * LS + l];
+      if (la < nlines) dp[gidx(g, p, 1 + la % g.n2m, la / g.n2m)] = z.x * norm;
+      if (lb < nlines) dp[gidx(g, p, 1 + lb % g.n2m, lb / g.n2m)] = z.y * norm;
+    }
+  }
+}
+
+// ----------------------------------------------------------------- UPCALC
+// x-lines: U1 = UH1 - DT*D1x(DP) - DT*DMPRESG (:1113-1128); U2 = UH2 -
+// DT*(DP(J)-DP(J-1))/H(J) (:1139-1145); with `chk`, the x part of the
+// divergence of the new field, (V(J+1)-V(J))/DY + D1x(U) (:9066-9102).
+template <typename T, int SEG>
+__global__ __launch_bounds__(256) void k_upd_x(Ctx c, int chk) {
+  XSETUP;
+  const Scal s = c.s;
+  const Metrics M = c.m;
+  const Fields F = c.d;
+  const T dx1 = (T)s.dx1, dt = (T)s.dt;
+  const T dmp = (T)F.sc[SC_DMPRESG];
+  T dpr[SEG], d[SEG], un[SEG];
+  const T* B = X.in((const T*)F.dp, 0);
+  SQ {
+    dpr[q] = XA(B, q);
+    d[q] = (T)24. * dx1 * (-XA(B, q - 1) + XA(B, q));
+  }
+  psolve<T, SEG>(d, G, L, l, gs, c.o.px.s22, bf.xf, bf.xb);
+  B = X.in((const T*)F.uh[0], 0);
+  SQ un[q] = XA(B, q) - dt * d[q] - dt * dmp;
+  X.out(F.u, un);
+  const T hj = (T)M.h[j];
+  const T* B1 = X.in((const T*)F.uh[1], 0);
+  B = X.in((const T*)F.dp, -1);
+  SQ d[q] = (j >= 2) ? XA(B1, q) - dt * (dpr[q] - XA(B, q)) / hj : (T)0;
+  X.out(F.v, d, 2);  // V(I,1,K) is not written by UPCALC
+  if (!chk) return;
+  // new V at J+1 (zero at J+1 = N2: the wall value); d = (V(J+1) - V(J))/DY
+  const T hjp = (T)M.h[j + 1], dy = (T)M.dy[j];
+  B1 = X.in((const T*)F.uh[1], 1);
+  B = X.in((const T*)F.dp, 1);
+  SQ d[q] = (((j + 1 <= g.n2m) ? XA(B1, q) - dt * (XA(B, q) - dpr[q]) / hjp : (T)0) - d[q]) / dy;
+  const T rt = xch_right(un[0], G, L, l, gs, bf.xr);
+  SQ dpr[q] = (T)24. * dx1 * (nxt(un, q, rt) - un[q]);
+  psolve<T, SEG>(dpr, G, L, l, gs, c.o.px.s22, bf.xf, bf.xb);
+  SQ d[q] = d[q] + dpr[q];
+  X.out((T*)F.div, d);
+}
+
+// z-lines: U3 = UH3 - DT*D1z(DP) - DT*DMPRESG3 (:1193-1208); P += DP
+// (:1218-1230); with `chk`: DIV += D1z(U3), DIVMAX, and the CFL maximum of
+// the new field (:9112-9128, :9150-9162).
+template <typename T, int SEG>
+__global__ __launch_bounds__(256) void k_upd_z(Ctx c, int chk) {
+  ZSETUP;
+  const Scal s = c.s;
+  const Metrics M = c.m;
+  const Fields F = c.d;
+  const T dx3 = (T)s.dx3, dt = (T)s.dt;
+  const T dmp3 = (T)F.sc[SC_DMPRESG3];
+  T dpr[SEG], d[SEG], wn[SEG], lf, rt;
+  zload((const T*)F.dp, b, pl, p0, bm, bp, dpr, lf, rt);
+  SQ d[q] = (T)24. * dx3 * (-prv(dpr, q, lf) + dpr[q]);
+  psolve<T, SEG>(d, G, L, l, gs, c.o.pz.s22, bf.xf, bf.xb);
+  SQ wn[q] = ((const T*)F.uh[2])[ZI(q)] - dt * d[q] - dt * dmp3;
+  if (ok) {
+    SQ {
+      F.w[ZI(q)] = wn[q];
+      F.p[ZI(q)] = F.p[ZI(q)] + dpr[q];
+    }
+  }
+  if (!chk) return;
+  rt = xch_right(wn[0], G, L, l, gs, bf.xr);
+  SQ d[q] = (T)24. * dx3 * (nxt(wn, q, rt) - wn[q]);
+  psolve<T, SEG>(d, G, L, l, gs, c.o.pz.s22, bf.xf, bf.xb);
+  double dmax = 0.0, cmax = 0.0;
+  if (ok) {
+    const double dyj = M.dy[j];
+    const long long bi = gidx(g, i + 1 == g.nx ? 0 : i + 1, j, 0);
+    SQ {
+      const long long a = ZI(q);
+      const double dv = (double)((T)F.div[a] + d[q]);
+      dmax = (dv != dv || dmax != dmax) ? NAN : fmax(dmax, fabs(dv));
+      const double wq = (double)wn[q], wq1 = (double)nxt(wn, q, rt);
+      const double cfll = fabs(F.u[a] + F.u[bi + (long long)(p0 + q) * pl]) * 0.5 * s.dx1 +
+                          fabs(F.v[a] + F.v[a + g.nx]) * 0.5 / dyj + fabs(wq + wq1) * 0.5 * s.dx3;
+      cmax = (cfll != cfll || cmax != cmax) ? NAN : fmax(cmax, cfll);
+    }
+  }
+  dmax = wave_max(dmax);
+  cmax = wave_max(cmax);
+  if ((threadIdx.x & 63) == 0) {
+    atomic_max_pos(&F.sc[SC_DIVMAX], dmax);
+    atomic_max_pos(&F.sc[SC_CFLM], cmax);
+  }
+}
+
+// =================================================================== host
+namespace c3dk {
+
+// compile-time segment lengths: x-lines 8 or 2, z-lines 16, 8 or 2
+#define XSEG(seg, ...)                \
+  if ((seg) == 8) {                   \
+    constexpr int S_ = 8;             \
+    __VA_ARGS__;                      \
+  } else {                            \
+    constexpr int S_ = 2;             \
+    __VA_ARGS__;                      \
+  }
+#define ZSEG(seg, ...)                \
+  if ((seg) == 16) {                  \
+    constexpr int S_ = 16;            \
+    __VA_ARGS__;                      \
+  } else if ((seg) == 8) {            \
+    constexpr int S_ = 8;             \
+    __VA_ARGS__;                      \
+  } else {                            \
+    constexpr int S_ = 2;             \
+    __VA_ARGS__;                      \
+  }
+
+// LDS bytes of a line kernel
+static size_t lds_line(const LShape& sh, bool staged, size_t es) {
+  const size_t tile = staged ? (size_t)sh.n * (sh.L + 1) : 0;
+  return es * (2 * tile + 4 * (size_t)sh.G * sh.L);
+}
+static int xblocks(const Ctx& c) { return (c.g.n2m * c.g.nz + c.o.lx.L - 1) / c.o.lx.L; }
+static int zblocks(const Ctx& c) { return ((c.g.nx + c.o.lz.L - 1) / c.o.lz.L) * c.g.n2m; }
+static dim3 xthreads(const Ctx& c) { return dim3(c.o.lx.L * c.o.lx.G); }
+static dim3 zthreads(const Ctx& c) { return dim3(c.o.lz.L * c.o.lz.G); }
+
+template <typename T>
+static void rhs_lines(const Ctx& c) {
+  const size_t lx = lds_line(c.o.lx, true, sizeof(T)), lz = lds_line(c.o.lz, false, sizeof(T));
+  XSEG(c.o.lx.seg, { KScope ks_(c, KID_RHS_X); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rhs_x<T, S_>), dim3(xblocks(c)), xthreads(c), lx, c.st, c); });
+  ZSEG(c.o.lz.seg, { KScope ks_(c, KID_RHS_Z); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rhs_z<T, S_>), dim3(zblocks(c)), zthreads(c), lz, c.st, c); });
+  XSEG(c.o.lx.seg, { KScope ks_(c, KID_RHS_X2); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rhs_x2<T, S_>), dim3(xblocks(c)), xthreads(c), lx, c.st, c); });
+}
+void rhs_lines_fp64(const Ctx& c) { rhs_lines<double>(c); }
+
+template <typename T>
+static void getuh_t(const Ctx& c) {
+  const int ylx = (c.g.nx + c.o.ys.L - 1) / c.o.ys.L;
+  { KScope ks_(c, KID_ADI_Y); hipLaunchKernelGGL(k_adi_y<T>, dim3(ylx * c.g.nz, 3), dim3(c.o.ys.L * c.o.ys.C), 0, c.st, c); }
+  const size_t lx = lds_line(c.o.lx, true, sizeof(T)), lz = lds_line(c.o.lz, false, sizeof(T));
+  // GETUH1/2: y, x, z;  GETUH3: y, z, x (compact3d.f:8508-8607)
+  const CompList x12{2, {0, 1, 0}, {0, 0, 0}};
+  const CompList z123{3, {0, 1, 2}, {1, 1, 0}};
+  const CompList x3{1, {2, 0, 0}, {1, 0, 0}};
+  XSEG(c.o.lx.seg, { KScope ks_(c, KID_ADI_X); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adi_x<T, S_>), dim3(xblocks(c), 2), xthreads(c), lx, c.st, c, x12); });
+  ZSEG(c.o.lz.seg, { KScope ks_(c, KID_ADI_Z); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adi_z<T, S_>), dim3(zblocks(c), 3), zthreads(c), lz, c.st, c, z123); });
+  XSEG(c.o.lx.seg, { KScope ks_(c, KID_ADI_X); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adi_x<T, S_>), dim3(xblocks(c), 1), xthreads(c), lx, c.st, c, x3); });
+}
+void getuh(const Ctx& c) { getuh_t<double>(c); }
+
+template <typename T>
+static void rhsdp_t(const Ctx& c) {
+  const size_t lx = lds_line(c.o.lx, true, sizeof(T)), lz = lds_line(c.o.lz, false, sizeof(T));
+  XSEG(c.o.lx.seg, { KScope ks_(c, KID_RHSDP_X); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rhsdp_x<T, S_>), dim3(xblocks(c)), xthreads(c), lx, c.st, c); });
+  ZSEG(c.o.lz.seg, { KScope ks_(c, KID_RHSDP_Z); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rhsdp_z<T, S_>), dim3(zblocks(c)), zthreads(c), lz, c.st, c); });
+}
+void rhsdp(const Ctx& c) { rhsdp_t<double>(c); }
+
+template <typename T>
+static void getdp_t(const Ctx& c) {
+  typedef cplx_t<T> zc;
+  const Geo& g = c.g;
+  const FFTPlan& f = c.f;
+  const size_t ldsx = 2 * sizeof(zc) * (size_t)g.nx * f.tlx;
+  const size_t ldsz = 2 * sizeof(zc) * (size_t)g.nz * f.tlz;
+  const int gx = (g.n2m * g.nz + 2 * f.tlx - 1) / (2 * f.tlx);
+  const int gz = ((g.nxh + f.tlz - 1) / f.tlz) * g.n2m;
+  const int gy = ((g.nxh + c.o.yp.L - 1) / c.o.yp.L) * g.nz;
+  { KScope ks_(c, KID_FFT_X); hipLaunchKernelGGL(k_fft_x<T>, dim3(gx), dim3(256), ldsx, c.st, c); }
+  { KScope ks_(c, KID_FFT_Z); hipLaunchKernelGGL(k_fft_z<T>, dim3(gz), dim3(256), ldsz, c.st, c, -1); }
+  { KScope ks_(c, KID_POIS_Y); hipLaunchKernelGGL(k_pois_y<T>, dim3(gy), dim3(c.o.yp.L * c.o.yp.C), 0, c.st, c); }
+  { KScope ks_(c, KID_FFT_Z); hipLaunchKernelGGL(k_fft_z<T>, dim3(gz), dim3(256), ldsz, c.st, c, +1); }
+  { KScope ks_(c, KID_FFT_XINV); hipLaunchKernelGGL(k_fft_xinv<T>, dim3(gx), dim3(256), ldsx, c.st, c); }
+}
+void getdp(const Ctx& c) { getdp_t<double>(c); }
+
+void pois_table(const Ctx& c) {
+  const long long nm = (long long)c.g.nxh * c.g.nz;
+  hipLaunchKernelGGL(k_pois_table, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, c.st, c);
+}
+
+template <typename T>
+static void upd_x_t(const Ctx& c, bool chk) {
+  const size_t lx = lds_line(c.o.lx, true, sizeof(T));
+  XSEG(c.o.lx.seg, { KScope ks_(c, KID_UPD_X); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_upd_x<T, S_>), dim3(xblocks(c)), xthreads(c), lx, c.st, c, chk ? 1 : 0); });
+}
+template <typename T>
+static void upd_z_t(const Ctx& c, bool chk) {
+  const size_t lz = lds_line(c.o.lz, false, sizeof(T));
+  ZSEG(c.o.lz.seg, { KScope ks_(c, KID_UPD_Z); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_upd_z<T, S_>), dim3(zblocks(c)), zthreads(c), lz, c.st, c, chk ? 1 : 0); });
+}
+void upd_x(const Ctx& c, bool chk) { upd_x_t<double>(c, chk); }
+void upd_z(const Ctx& c, bool chk) { upd_z_t<double>(c, chk); }
+
+}  // namespace c3dk
