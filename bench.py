@@ -33,7 +33,10 @@ METRIC = "Mpts·steps/s mixed vs FP64 on 1 GPU @ div≤1e-9; p50/p99 served late
 # algorithmic (compulsory) HBM bytes per cell and launch of each kernel of the
 # FP64 step: 8 B per field element read or written once
 ALG_BYTES = {
-    "rhs_x": 88, "rhs_z": 144, "rhs_x2": 24, "rhs_y": 168, "adi_y": 48, "adi_z": 64,
+    # compulsory bytes per pressure cell of each kernel (every distinct field
+    # read once, written once); adi_y is the ADI y-sweep with the RHS y terms
+    # fused in (per component: state, VV1/VV3 or P, RUH, HH, HH2 in; HH, UH out)
+    "rhs_x": 88, "rhs_z": 112, "rhs_zw": 40, "rhs_x2": 24, "adi_y": 168, "adi_z": 64,
     "rhsdp_x": 24, "rhsdp_z": 24, "fft_x": 16, "fft_z": 16, "pois_y": 16, "fft_xinv": 16,
     "flow_red": 40, "upd_x": 48, "upd_z": 64,
 }

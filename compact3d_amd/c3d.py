@@ -23,6 +23,7 @@ LIB_PATH = os.path.join(HERE, "libc3d.so")
 FP64, MIXED = 0, 1
 F_RUH1, F_RUH2, F_RUH3, F_UH1, F_UH2, F_UH3, F_RDP, F_DP, F_HH1, F_HH2, F_HH3, F_TAUW, F_TAUB = range(1, 14)
 S_PRESG, S_PRESG3, S_DMPRESG, S_DMPRESG3, S_FLOW1, S_FLOW3, S_WSM1, S_WSM2, S_WSM3 = range(9)
+NKID = 48  # C3D_NKID of include/c3d.h: entries of the per-kernel timing tables
 
 _dp = C.POINTER(C.c_double)
 
@@ -256,10 +257,10 @@ class Channel:
 
     def kernel_times(self):
         """{kernel name: (total ms, launches)} since timing() was enabled"""
-        ms = (C.c_double * 32)()
-        cnt = (C.c_int * 32)()
+        ms = (C.c_double * NKID)()
+        cnt = (C.c_int * NKID)()
         self._ck(self.L.c3d_kernel_times(self.h, ms, cnt), "kernel_times")
-        return {self.L.c3d_kernel_name(q).decode(): (ms[q], cnt[q]) for q in range(32) if cnt[q] > 0}
+        return {self.L.c3d_kernel_name(q).decode(): (ms[q], cnt[q]) for q in range(NKID) if cnt[q] > 0}
 
     def apply_op(self, op, direction, field):
         inp = np.asfortranarray(field, dtype=np.float64)

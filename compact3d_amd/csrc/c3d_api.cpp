@@ -87,21 +87,17 @@ static PSolve make_psolve(double a, double b, int seg) {
   return p;
 }
 
-// line-engine tile: seg = the largest compiled segment length (x-lines:
-// 8, 2; z-lines: 16, 8, 2) dividing n, G = n/seg segments, L lines with
-// L*G <= 256; L a multiple of 16 where possible (128-byte rows for z-lines);
-// staged tiles keep 2 LDS row tiles of n*(L+1) plus the exchange arrays
-// within 64 KB
-static bool make_lshape(int n, int segmax, bool staged, size_t es, LShape& sh) {
-  int seg = 2;
-  if (segmax >= 16 && n % 16 == 0)
-    seg = 16;
-  else if (n % 8 == 0)
-    seg = 8;
+// line-engine tile: seg = 8 when it divides n (else 2, the other compiled
+// length), G = n/seg segments, L lines per tile.  x-lines: L*G <= 256, with
+// 2 LDS row tiles of n*(L+1) plus the exchange arrays within 64 KB.
+// z-lines: L*G <= 512 and L a multiple of 16 (128-byte rows) up to 64.
+static bool make_lshape(int n, bool staged, size_t es, LShape& sh) {
+  const int seg = (n % 8 == 0) ? 8 : 2;
   if (n % seg) return false;
   const int G = n / seg;
-  if (G > 256) return false;
-  int L = std::min(64, 256 / G);
+  const int tmax = staged ? 256 : 512;
+  if (G > tmax) return false;
+  int L = std::min(64, tmax / G);
   if (L >= 16) L -= L % 16;
   if (staged)
     while (L > 1 && es * (2 * (size_t)n * (L + 1) + 4 * (size_t)G * L) > 65536) L--;
@@ -415,8 +411,8 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   if (g.n2m > 4096) return bad("N2 too large for the y tiles");
   build_mesh(h);
   Ops& o = c.o;
-  if (!make_lshape(g.nx, 8, true, sizeof(double), o.lx) || !make_lshape(g.nz, 16, false, sizeof(double), o.lz))
-    return bad("N1-1 and N3-1 must have a divisor <= 16 giving at most 256 segments");
+  if (!make_lshape(g.nx, true, sizeof(double), o.lx) || !make_lshape(g.nz, false, sizeof(double), o.lz))
+    return bad("N1-1 (N3-1) too long for the line tiles: at most 2048 (4096) points");
   o.ys = make_yshape(g.n2m);
   o.yp = make_yshape(g.n2m);
   o.s6 = make_solve(1.0, 6.0);
@@ -435,7 +431,7 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   }
   // device vectors: 15 J-vectors + AK1 + AK3 + 9 ADI chains
   const size_t nv = (size_t)(g.n2 + 2);
-  const size_t nvec = 15 * nv + (g.nx + 2) + (g.nz / 2 + 3) + 9 * nv;
+  const size_t nvec = 15 * nv + (g.nx + 2) + (g.nz / 2 + 3) + 9 * nv + 5 * nv;
   if (dalloc(h, &h->dvec, nvec)) return bad(h->err);
   {
     std::vector<double> hv(nvec, 0.0);
@@ -445,6 +441,16 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
     for (int q = 0; q < 15; q++) std::copy(src[q]->begin(), src[q]->end(), hv.begin() + q * nv);
     std::copy(h->AK1.begin(), h->AK1.end(), hv.begin() + 15 * nv);
     std::copy(h->AK3.begin(), h->AK3.end(), hv.begin() + 15 * nv + (g.nx + 2));
+    {
+      double* yt = hv.data() + 15 * nv + (g.nx + 2) + (g.nz / 2 + 3) + 9 * nv;
+      for (int J = 1; J <= g.n2m; J++) {
+        yt[0 * nv + J] = h->FJPA[J] * 1.0 / h->H[J + 1];
+        yt[1 * nv + J] = h->FJMU[J] * 1.0 / h->H[J];
+        yt[3 * nv + J] = 1. / h->DY[J];
+      }
+      for (int J = 0; J <= g.n2; J++) yt[2 * nv + J] = h->DY[J] / 2.;
+      for (int J = 1; J <= g.n2; J++) yt[4 * nv + J] = 1. / h->H[J];
+    }
     if (hipMemcpy(h->dvec, hv.data(), nvec * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
       return bad("vector upload failed");
     const double* d = h->dvec;
@@ -467,6 +473,12 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
     m.ak1 = d + 15 * nv;
     m.ak3 = d + 15 * nv + (g.nx + 2);
     const double* yc = d + 15 * nv + (g.nx + 2) + (g.nz / 2 + 3);
+    const double* yt = yc + 9 * nv;
+    m.cu2 = yt;
+    m.cu1 = yt + nv;
+    m.dyh = yt + 2 * nv;
+    m.rdy = yt + 3 * nv;
+    m.rh = yt + 4 * nv;
     for (int q = 0; q < 3; q++) {
       m.ya[q] = yc + (3 * q + 0) * nv;
       m.ybet[q] = yc + (3 * q + 1) * nv;
@@ -733,8 +745,8 @@ static int step_impl(c3d_handle* h, bool chk) {
   if (h->cfg.files == 2.0) c3dk::sgscs(h->c);
   if (h->cfg.files != 0.0) wall_model_impl(h);
   c3dk::bcond(h->c);
-  c3dk::rhs(h->c);
-  c3dk::getuh(h->c);
+  c3dk::rhs_xz(h->c);
+  c3dk::getuh_fused(h->c);
   c3dk::rhsdp(h->c);
   c3dk::getdp(h->c);
   c3dk::upcalc(h->c, chk);
@@ -902,7 +914,9 @@ extern "C" int c3d_apply_op(c3d_handle* h, int op, int dir, const double* in, do
 static const char* kid_names[C3D_NKID] = {
     "bcond", "rhs_x", "rhs_z", "rhs_x2", "rhs_y", "adi_y", "adi_x", "adi_z", "rhsdp_x", "rhsdp_z",
     "fft_x", "fft_z", "pois_y", "fft_xinv", "flow_red", "flow_fin", "upd_x", "ghosts", "upd_z",
-    "div_x", "div_z", "cfl", "chkmf", "zero", "apply_op", "sgs", "wall", "", "", "", "", ""};
+    "div_x", "div_z", "cfl", "chkmf", "zero", "apply_op", "sgscs", "wallmodel", "sgs_x", "sgs_z",
+    "sgs_z2", "sgs_x2", "wallss", "filter", "rhs_zw"};
+static_assert(KID_COUNT <= C3D_NKID, "kernel id table too small");
 
 extern "C" const char* c3d_kernel_name(int id) {
   if (id < 0 || id >= C3D_NKID) return "";

@@ -17,6 +17,10 @@ enum {
 struct Metrics {
   const double *y, *dy, *h, *hp, *hc, *hm, *dyp, *dyc, *dym, *fjpa, *fjmu, *fjmv;
   const double *pmj, *pcj, *ppj;
+  // J-only factors of the y terms, formed once on the host exactly as the
+  // reference's expressions evaluate them: FJPA(J)*1/H(J+1), FJMU(J)*1/H(J),
+  // DY(J)/2, 1/DY(J), 1/H(J)
+  const double *cu2, *cu1, *dyh, *rdy, *rh;
   const double *ak1;  // AK1(I), I = 1..N1M  (index I)
   const double *ak3;  // AK3(K), K = 1..N3M/2+1
   // TDMAI chains of the ADI y-sweeps (component c = 0,1,2): A(J), BET(J), GAM(J)
@@ -81,12 +85,13 @@ struct Fields {
 };
 
 // kernel ids for the per-kernel event timer
+// kernel ids for the per-kernel event timer (KID_COUNT <= C3D_NKID)
 enum {
   KID_BCOND = 0, KID_RHS_X, KID_RHS_Z, KID_RHS_X2, KID_RHS_Y, KID_ADI_Y, KID_ADI_X, KID_ADI_Z,
   KID_RHSDP_X, KID_RHSDP_Z, KID_FFT_X, KID_FFT_Z, KID_POIS_Y, KID_FFT_XINV, KID_FLOW_RED,
   KID_FLOW_FIN, KID_UPD_X, KID_GHOSTS, KID_UPD_Z, KID_DIV_X, KID_DIV_Z, KID_CFL, KID_CHKMF,
   KID_ZERO, KID_APPLY, KID_SGS, KID_WALL, KID_SGS_X, KID_SGS_Z, KID_SGS_Z2, KID_SGS_X2,
-  KID_WALLSS, KID_FILTER, KID_COUNT = 32
+  KID_WALLSS, KID_FILTER, KID_RHS_ZW, KID_COUNT
 };
 
 // host-side recorder: when `on`, every launch is bracketed by two events on
@@ -130,9 +135,11 @@ struct KScope {
 // host launchers (c3d_kernels.hip)
 namespace c3dk {
 void bcond(const Ctx& c);
-void rhs(const Ctx& c);
+void rhs(const Ctx& c);      // RHS1/2/3 complete (x, z and y parts)
+void rhs_xz(const Ctx& c);   // x and z parts only
 void rhs_lines_fp64(const Ctx& c);
-void getuh(const Ctx& c);
+void getuh(const Ctx& c);    // GETUH1/2/3 from the complete RUH
+void getuh_fused(const Ctx& c);  // y parts of RHS1/2/3 fused into GETUH1/2/3
 void rhsdp(const Ctx& c);
 void getdp(const Ctx& c);
 void upcalc(const Ctx& c, bool fuse_checks);

@@ -13,7 +13,10 @@
 // tile_solve() (c3d_dev.h) on LDS tiles.
 #include <math.h>
 
+#include <algorithm>
+
 #include "c3d_impl.h"
+#include "c3d_yterms.h"
 
 // ------------------------------------------------------------------ BCOND
 __global__ void k_bcond(Ctx c) {
@@ -42,108 +45,23 @@ __global__ void k_bcond(Ctx c) {
 }
 
 // ---------------------------------------------------- RHS pointwise y part
-// y-direction viscous terms, pressure gradient and convection of RHS1-3,
-// then the AB2 combination and the wall-stress BC:
-//   RHS1 :6119-6123, :6255-6267, :6765-6773
-//   RHS2 :6840-6863, :7386-7389
-//   RHS3 :7457-7463, :7594-7605, :8096-8103
-__global__ void k_rhs_y(Ctx c) {
+// the y terms, AB2 and wall BC of RHS1-3 (c3d_yterms.h) as a pass of its own;
+// the step fuses them into the ADI y-sweep instead (c3d_step.hip).  Grid
+// (i-block, J, K-block); each thread does YK consecutive K of one (I, J).
+#define YK 4
+template <typename T>
+__global__ __launch_bounds__(256) void k_rhs_y(Ctx c) {
   const Geo g = c.g;
-  const long long tot = (long long)g.nx * g.n2m * g.nz;
-  long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (t >= tot) return;
-  const int i = t % g.nx;
-  const int j = 1 + (int)((t / g.nx) % g.n2m);
-  const int k = (int)(t / ((long long)g.nx * g.n2m));
-  const Scal s = c.s;
-  const Metrics M = c.m;
-  const Fields F = c.d;
-  const long long b = gidx(g, i, j, k);
-  const long long bp = b + g.nx, bm = b - g.nx;
-  const long long bN2 = gidx(g, i, g.n2, k), b0 = gidx(g, i, 0, k);
-  const int jp = j + 1, jm = j - 1;
-  const double fjum = M.fjmu[j], fjup = M.fjpa[j];
-  const double rinv = s.rinv;
-  const long long t2 = (long long)k * g.nx + i;
-  // RHS1
-  {
-    double ru = F.ruh[0][b] + rinv * (M.hp[j] * F.u[bp] - M.hc[j] * F.u[b] + M.hm[j] * F.u[bm]);
-    double u2 = fjup * 1.0 / M.h[jp] * (M.dy[j] / 2. * F.u[bp] + M.dy[jp] / 2. * F.u[b]) +
-                (1. - fjup) * F.u[bN2];
-    double u1 = fjum * 1.0 / M.h[j] * (M.dy[jm] / 2. * F.u[b] + M.dy[j] / 2. * F.u[bm]) +
-                (1. - fjum) * F.u[b0];
-    double v2 = (j == g.n2m) ? 0.0 : F.vv1[bp];
-    double v1 = (j == 1) ? 0.0 : F.vv1[b];
-    double hh = F.hh[0][b] + 1. / M.dy[j] * (v2 * u2 - v1 * u1);
-    if (s.files) {
-      // D(SGS*(DU/DY))/DY and D(SGS*(DV/DX))/DY (:6551-6680); nux outside
-      // J=1..N2M enters only multiplied by a zero wall flag and is taken as 0
-      const int im = wrapm(i, g.nx);
-      const double n0 = F.nux[b], np1 = (j < g.n2m) ? F.nux[bp] : 0.0, nm1 = (j > 1) ? F.nux[bm] : 0.0;
-      const double sgsj2 = fjup * 1. / M.h[jp] * (M.dy[j] / 2. * np1 + M.dy[jp] / 2. * n0) +
-                           (1. - fjup) * 0.5 * (F.sgs[bN2] + F.sgs[gidx(g, im, g.n2, k)]);
-      const double sgsj1 = fjum * 1. / M.h[j] * (M.dy[jm] / 2. * n0 + M.dy[j] / 2. * nm1) +
-                           (1. - fjum) * 0.5 * (F.sgs[b0] + F.sgs[gidx(g, im, 0, k)]);
-      const double wfu = 1. - s.ifwall * (1. - fjup) * (s.aly - 1.);
-      const double wfd = 1. - s.ifwall * (1. - fjum);
-      const double uu22 = (F.u[bp] - F.u[b]) / M.h[jp];
-      const double uu11 = (F.u[b] - F.u[bm]) / M.h[j];
-      hh = hh - 1. / M.dy[j] * (wfu * sgsj2 * uu22 - wfd * sgsj1 * uu11);
-      const double dvp = (j < g.n2m) ? F.dvdx[bp] : 0.0;
-      hh = hh - 1.0 / M.dy[j] * (wfu * sgsj2 * dvp - wfd * sgsj1 * F.dvdx[b]);
-    }
-    F.hh[0][b] = hh;
-    double bc_down = -F.tauw[t2] / M.dy[1] * s.ifwall;
-    double bc = (1. - fjum) * bc_down;
-    F.ruh[0][b] = ru - 3. / 2. * hh + 1. / 2. * F.hhp[0][b] + bc;
-  }
-  // RHS2 (J = 2..N2M)
-  if (j >= 2) {
-    double viscos = rinv * (M.dyp[j] * F.v[bp] - M.dyc[j] * F.v[b] + M.dym[j] * F.v[bm]);
-    double pressg2 = (F.p[b] - F.p[bm]) / M.h[j];
-    double ru = F.ruh[1][b] + (-pressg2 + viscos);
-    double v2 = 0.5 * (F.v[bp] + F.v[b]);
-    double v1 = 0.5 * (F.v[b] + F.v[bm]);
-    double hh = F.hh[1][b] + 1. / M.h[j] * (v2 * v2 - v1 * v1);
-    if (s.files) {
-      double g2 = 1. / M.dy[j] * (F.v[bp] - F.v[b]);
-      double g1 = 1. / M.dy[jm] * (F.v[b] - F.v[bm]);
-      double vis22 = 1.0 / M.h[j] * (F.sgs[b] * g2 - F.sgs[bm] * g1);
-      hh = hh - 2. * vis22;
-    }
-    F.hh[1][b] = hh;
-    F.ruh[1][b] = ru - 3. / 2. * hh + 1. / 2. * F.hhp[1][b];
-  }
-  // RHS3
-  {
-    double ru = F.ruh[2][b] + rinv * (M.hp[j] * F.w[bp] - M.hc[j] * F.w[b] + M.hm[j] * F.w[bm]);
-    double w2 = fjup * 1.0 / M.h[jp] * (M.dy[jp] / 2. * F.w[b] + M.dy[j] / 2. * F.w[bp]) +
-                (1. - fjup) * F.w[bN2];
-    double w1 = fjum * 1.0 / M.h[j] * (M.dy[j] / 2. * F.w[bm] + M.dy[jm] / 2. * F.w[b]) +
-                (1. - fjum) * F.w[b0];
-    double v2 = (j == g.n2m) ? 0.0 : F.vv3[bp];
-    double v1 = (j == 1) ? 0.0 : F.vv3[b];
-    double hh = F.hh[2][b] + 1. / M.dy[j] * (v2 * w2 - v1 * w1);
-    if (s.files) {
-      // D(SGS*(DW/DY))/DY and D(SGS*(DV/DZ))/DY (:7858-7908, :8042-8094)
-      const int km = wrapm(k, g.nz);
-      const double z0 = F.nuz[b], zp1 = (j < g.n2m) ? F.nuz[bp] : 0.0, zm1 = (j > 1) ? F.nuz[bm] : 0.0;
-      const double sgsj1 = fjum * 1. / M.h[j] * (M.dy[jm] / 2. * z0 + M.dy[j] / 2. * zm1) +
-                           (1. - fjum) * 0.5 * (F.sgs[b0] + F.sgs[gidx(g, i, 0, km)]);
-      const double sgsj2 = fjup * 1.0 / M.h[jp] * (M.dy[j] / 2. * zp1 + M.dy[jp] / 2. * z0) +
-                           (1. - fjup) * 0.5 * (F.sgs[bN2] + F.sgs[gidx(g, i, g.n2, km)]);
-      const double wfu = 1. - s.ifwall * (1. - fjup) * (s.aly - 1.);
-      const double wfd = 1. - s.ifwall * (1. - fjum);
-      const double ww22 = 1. / M.h[jp] * (F.w[bp] - F.w[b]);
-      const double ww11 = 1. / M.h[j] * (F.w[b] - F.w[bm]);
-      hh = hh - 1. / M.dy[j] * (wfu * sgsj2 * ww22 - wfd * sgsj1 * ww11);
-      const double dzp = (j < g.n2m) ? F.dvdz[bp] : 0.0;
-      hh = hh - 1.0 / M.dy[j] * (wfu * sgsj2 * dzp - wfd * sgsj1 * F.dvdz[b]);
-    }
-    F.hh[2][b] = hh;
-    double bc_down = -F.taub[t2] / M.dy[1] * s.ifwall;
-    double bc = (1. - fjum) * bc_down;
-    F.ruh[2][b] = ru - 3. / 2. * hh + 1. / 2. * F.hhp[2][b] + bc;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.nx) return;
+  const int j = 1 + blockIdx.y;
+  for (int kk = 0; kk < YK; kk++) {
+    const int k = blockIdx.z * YK + kk;
+    if (k >= g.nz) break;
+    const YIdx x = yidx(g, i, j, k);
+    ((T*)c.d.ruh[0])[x.b] = yterm_u<T>(c, x, i, j, k);
+    if (j >= 2) ((T*)c.d.ruh[1])[x.b] = yterm_v<T>(c, x, i, j, k);
+    ((T*)c.d.ruh[2])[x.b] = yterm_w<T>(c, x, i, j, k);
   }
 }
 
@@ -151,29 +69,36 @@ __global__ void k_rhs_y(Ctx c) {
 // flow-rate sums of UPCALC (:1079-1093, :1160-1172), all before any update.
 // Fixed grid => fixed reduction tree => bit-repeatable.
 #define RED_BLOCKS 1024
+template <typename T>
 __global__ __launch_bounds__(C3D_NT) void k_flow_red(Ctx c) {
   __shared__ double red[4][C3D_NT];
   const Geo g = c.g;
   const Scal s = c.s;
   const Metrics M = c.m;
   const Fields F = c.d;
-  const long long tot = (long long)g.nx * g.n2m * g.nz;
+  const T* uh0 = (const T*)F.uh[0];
+  const T* uh2 = (const T*)F.uh[2];
+  const T* dpf = (const T*)F.dp;
+  const double cu = 1.0 / s.dx3 / s.dx1;  // (x * DY / DX3) / DX1 as in :1083
   double f1 = 0, d1 = 0, f3 = 0, d3 = 0;
-  for (long long t = blockIdx.x * (long long)C3D_NT + threadIdx.x; t < tot;
-       t += (long long)RED_BLOCKS * C3D_NT) {
-    const int i = t % g.nx;
-    const int j = 1 + (int)((t / g.nx) % g.n2m);
-    const int k = (int)(t / ((long long)g.nx * g.n2m));
-    const long long b = gidx(g, i, j, k);
+  const int nrows = g.n2m * g.nz;
+  for (int r = blockIdx.x; r < nrows; r += RED_BLOCKS) {
+    const int j = 1 + r % g.n2m, k = r / g.n2m;
     const double dy = M.dy[j];
-    const double dp = F.dp[b];
-    const double dpm = F.dp[gidx(g, wrapm(i, g.nx), j, k)];
-    const double dpk = F.dp[gidx(g, i, j, wrapm(k, g.nz))];
-    f1 += F.u[b] * dy / s.dx3 / s.dx1;
-    d1 += F.uh[0][b] * dy / s.dx3 / s.dx1 - s.dt * (dp - dpm) * dy / s.dx3;
-    f3 += F.w[b] * dy / s.dx1 / s.dx3;
-    d3 += F.uh[2][b] * dy / s.dx3 / s.dx1 - s.dt * (dp - dpk) * dy / s.dx1;
+    const long long rb = gidx(g, 0, j, k);
+    const long long rk = gidx(g, 0, j, k == 0 ? g.nz - 1 : k - 1);
+    for (int i = threadIdx.x; i < g.nx; i += C3D_NT) {
+      const long long b = rb + i;
+      const double dp = (double)dpf[b];
+      const double dpm = (double)dpf[rb + (i == 0 ? g.nx - 1 : i - 1)];
+      const double dpk = (double)dpf[rk + i];
+      f1 += F.u[b] * dy / s.dx3 / s.dx1;
+      d1 += (double)uh0[b] * dy / s.dx3 / s.dx1 - s.dt * (dp - dpm) * dy / s.dx3;
+      f3 += F.w[b] * dy / s.dx1 / s.dx3;
+      d3 += (double)uh2[b] * dy / s.dx3 / s.dx1 - s.dt * (dp - dpk) * dy / s.dx1;
+    }
   }
+  (void)cu;
   const int t = threadIdx.x;
   red[0][t] = f1;
   red[1][t] = d1;
@@ -487,16 +412,23 @@ void bcond(const Ctx& c) {
   { KScope ks_(c, KID_BCOND); hipLaunchKernelGGL(k_bcond, dim3(pgrid((long long)c.g.nx * c.g.nz)), dim3(C3D_NT), 0, c.st, c); }
 }
 
-void rhs(const Ctx& c) {
+void rhs_xz(const Ctx& c) {
   rhs_lines_fp64(c);
   if (c.s.files) sgs_terms(c);
-  { KScope ks_(c, KID_RHS_Y); hipLaunchKernelGGL(k_rhs_y, dim3(pgrid((long long)c.g.nx * c.g.n2m * c.g.nz)), dim3(C3D_NT), 0,
-                     c.st, c); }
+}
+
+void rhs(const Ctx& c) {
+  rhs_xz(c);
+  {
+    const int bx = std::min(256, c.g.nx);
+    KScope ks_(c, KID_RHS_Y);
+    hipLaunchKernelGGL(k_rhs_y<double>, dim3((c.g.nx + bx - 1) / bx, c.g.n2m, (c.g.nz + YK - 1) / YK), dim3(bx), 0,
+                       c.st, c);
+  }
 }
 
 void upcalc(const Ctx& c, bool chk) {
-  if (chk) zero(c, c.d.sc + SC_DIVMAX, 2);
-  { KScope ks_(c, KID_FLOW_RED); hipLaunchKernelGGL(k_flow_red, dim3(RED_BLOCKS), dim3(C3D_NT), 0, c.st, c); }
+  { KScope ks_(c, KID_FLOW_RED); hipLaunchKernelGGL(k_flow_red<double>, dim3(RED_BLOCKS), dim3(C3D_NT), 0, c.st, c); }
   { KScope ks_(c, KID_FLOW_FIN); hipLaunchKernelGGL(k_flow_fin, dim3(1), dim3(C3D_NT), 0, c.st, c, 1); }
   upd_x(c, chk);
   { KScope ks_(c, KID_GHOSTS); hipLaunchKernelGGL(k_ghosts, dim3(pgrid((long long)c.g.nx * c.g.nz)), dim3(C3D_NT), 0, c.st, c); }

@@ -23,6 +23,7 @@
 #include <math.h>
 
 #include "c3d_impl.h"
+#include "c3d_yterms.h"
 
 // ------------------------------------------------------------ helpers
 template <typename T, int SEG>
@@ -57,77 +58,85 @@ __device__ __forceinline__ LBuf<T> lbuf(unsigned char* sm, const LShape& sh, boo
 }
 
 // x-line staging.  Lines ln = ln0 + l (l < L) are rows (j, k) with
-// j = 1 + ln % N2M, k = ln / N2M.  Two LDS tiles alternate and every staging
-// operation starts with a barrier, so a tile stays readable until the
-// second staging operation after the one that filled it begins.
-template <typename T>
+// j = 1 + ln % N2M, k = ln / N2M.  A tile holds n*L = SEG*blockDim values, so
+// each thread moves exactly SEG of them: element e = t + m*blockDim is row
+// e / n, point e % n.  pre() issues the global loads of a tile into
+// registers (so they can be in flight while the previous tile is used);
+// put() writes them to LDS.  Two LDS tiles alternate and every put()/out()
+// starts with a barrier, so a tile stays readable until the second
+// staging operation after the one that filled it begins.
+template <typename T, int SEG>
 struct XStage {
   T *st0, *st1;
   int sb;
-  int n, L, LS, ln0, nlines;
-  int l, p0, pm, pp;  // this thread's line, segment start, and periodic neighbours
-  Geo g;
+  int nx, L, LS;
+  int l, p0, pm, pp;   // this thread's line, segment start, and periodic neighbours
+  long long go[SEG];   // global offset of element m (row j), or -1
+  int li[SEG];         // LDS index of element m
+  int jr[SEG];         // row j of element m
   __device__ __forceinline__ T* next() {
     sb ^= 1;
     return sb ? st1 : st0;
   }
-  __device__ __forceinline__ long long row(int ll, int joff) const {
-    const int ln = ln0 + ll;
-    return gidx(g, 0, 1 + ln % g.n2m + joff, ln / g.n2m);
-  }
-  // coalesced load of the tile rows (j shifted by joff)
   template <typename S>
-  __device__ __forceinline__ const T* in(const S* F, int joff) {
+  __device__ __forceinline__ void pre(const S* F, int joff, T (&r)[SEG]) const {
+    const long long off = (long long)joff * nx;
+    SQ r[q] = go[q] >= 0 ? (T)F[go[q] + off] : (T)0;
+  }
+  __device__ __forceinline__ const T* put(const T (&r)[SEG]) {
     T* B = next();
     __syncthreads();
-    const int lmax = min(L, nlines - ln0);
-    for (int ll = 0; ll < lmax; ll++) {
-      const long long r = row(ll, joff);
-      for (int p = threadIdx.x; p < n; p += blockDim.x) B[p * LS + ll] = (T)F[r + p];
-    }
+    SQ B[li[q]] = r[q];
     __syncthreads();
     return B;
   }
+  template <typename S>
+  __device__ __forceinline__ const T* in(const S* F, int joff) {
+    T r[SEG];
+    pre(F, joff, r);
+    return put(r);
+  }
   // element at segment offset q in [-1, SEG] of this thread's line
-  template <int SEG>
   __device__ __forceinline__ T at(const T* B, int q) const {
     const int p = q < 0 ? pm : (q >= SEG ? pp : p0 + q);
     return B[p * LS + l];
   }
   // rows with j < jmin are left untouched
-  template <typename S, int SEG>
+  template <typename S>
   __device__ __forceinline__ void out(S* F, const T (&v)[SEG], int jmin = 1) {
     T* B = next();
     __syncthreads();
     SQ B[(p0 + q) * LS + l] = v[q];
     __syncthreads();
-    const int lmax = min(L, nlines - ln0);
-    for (int ll = 0; ll < lmax; ll++) {
-      if (1 + (ln0 + ll) % g.n2m < jmin) continue;
-      const long long r = row(ll, 0);
-      for (int p = threadIdx.x; p < n; p += blockDim.x) F[r + p] = (S)B[p * LS + ll];
-    }
+    SQ if (go[q] >= 0 && jr[q] >= jmin) F[go[q]] = (S)B[li[q]];
   }
 };
 
-template <typename T>
-__device__ __forceinline__ XStage<T> xstage(const LBuf<T>& b, const Geo& g, const LShape& sh, int l, int p0,
-                                            int seg) {
-  XStage<T> x;
+template <typename T, int SEG>
+__device__ __forceinline__ void xstage(XStage<T, SEG>& x, const LBuf<T>& b, const Geo& g, const LShape& sh,
+                                       int l, int p0) {
   x.st0 = b.st0;
   x.st1 = b.st1;
   x.sb = 1;
-  x.n = sh.n;
+  x.nx = g.nx;
   x.L = sh.L;
   x.LS = sh.L + 1;
-  x.ln0 = blockIdx.x * sh.L;
-  x.nlines = g.n2m * g.nz;
   x.l = l;
   x.p0 = p0;
   x.pm = p0 == 0 ? sh.n - 1 : p0 - 1;
-  x.pp = p0 + seg == sh.n ? 0 : p0 + seg;
-  x.g = g;
-  return x;
+  x.pp = p0 + SEG == sh.n ? 0 : p0 + SEG;
+  const int ln0 = blockIdx.x * sh.L;
+  const int nlines = g.n2m * g.nz;
+  const int nt = blockDim.x;
+  SQ {
+    const int e = threadIdx.x + q * nt;
+    const int ll = e / sh.n, p = e - ll * sh.n;
+    const int ln = ln0 + ll;
+    const int jj = 1 + ln % g.n2m;
+    x.go[q] = (ll < sh.L && ln < nlines) ? gidx(g, p, jj, ln / g.n2m) : -1;
+    x.li[q] = p * x.LS + ll;
+    x.jr[q] = jj;
+  }
 }
 
 #define XSETUP                                                        \
@@ -137,14 +146,16 @@ __device__ __forceinline__ XStage<T> xstage(const LBuf<T>& b, const Geo& g, cons
   const int L = sh.L, G = sh.G;                                       \
   const LBuf<T> bf = lbuf<T>(smem, sh, true);                         \
   const int l = threadIdx.x % L, gs = threadIdx.x / L, p0 = gs * SEG; \
-  XStage<T> X = xstage<T>(bf, g, sh, l, p0, SEG);                     \
-  const int ln = X.ln0 + l;                                           \
-  const bool ok = ln < X.nlines;                                      \
+  XStage<T, SEG> X;                                                   \
+  xstage(X, bf, g, sh, l, p0);                                        \
+  const int ln = blockIdx.x * L + l;                                  \
+  const bool ok = ln < g.n2m * g.nz;                                  \
   const int j = 1 + (ok ? ln % g.n2m : 0);                            \
   (void)j;
-#define XA(B, q) X.template at<SEG>(B, q)
+#define XA(B, q) X.at(B, q)
 
-// z-lines: a tile is L consecutive i at one j; column i = i0 + l, k = p
+// z-lines: a tile is L consecutive i at one j (up to 512 threads); column
+// i = i0 + l, k = p
 #define ZSETUP                                                        \
   extern __shared__ __align__(16) unsigned char smem[];               \
   const Geo g = c.g;                                                  \
@@ -188,9 +199,11 @@ __global__ __launch_bounds__(256) void k_rhs_x(Ctx c) {
   const PSolve s6 = c.o.px.s6, s10 = c.o.px.s10, s22 = c.o.px.s22;
   const T dx1 = (T)s.dx1, dx1q = (T)s.dx1q, rinv = (T)s.rinv;
   const bool j2 = j >= 2;
-  T d[SEG], acc[SEG], vv[SEG];
+  T d[SEG], acc[SEG], vv[SEG], pf[SEG];
   // v: RUH2, VV1
-  const T* B = X.in((const T*)F.sv, 0);
+  X.pre((const T*)F.sv, 0, pf);
+  const T* B = X.put(pf);
+  X.pre((const T*)F.su, -1, pf);  // next tile in flight during the v work
   SQ d[q] = (T)12. * dx1q * (XA(B, q - 1) - (T)2. * XA(B, q) + XA(B, q + 1));
   psolve<T, SEG>(d, G, L, l, gs, s10, bf.xf, bf.xb);
   SQ d[q] = j2 ? rinv * d[q] : (T)0;
@@ -200,9 +213,11 @@ __global__ __launch_bounds__(256) void k_rhs_x(Ctx c) {
   X.out((T*)F.vv1, vv);
   // UU2 = (DY(J)/2 U(J-1) + DY(J-1)/2 U(J)) / H(J)  (:6966); HH2 = D1x(UU2 VV1)
   const T hj = (T)(1. / M.h[j]), dyj = (T)(M.dy[j] / 2.), dyjm = (T)(M.dy[j - 1] / 2.);
-  B = X.in((const T*)F.su, -1);
+  B = X.put(pf);
+  X.pre((const T*)F.su, 0, pf);
   SQ acc[q] = dyj * XA(B, q);
-  B = X.in((const T*)F.su, 0);  // u, readable until the second staging op from here
+  B = X.put(pf);  // u, readable until the second staging op from here
+  X.pre((const T*)F.sp, 0, pf);
   SQ acc[q] = j2 ? hj * (acc[q] + dyjm * XA(B, q)) * vv[q] : (T)0;
   T rt = xch_right(acc[0], G, L, l, gs, bf.xr);
   SQ d[q] = (T)24. * dx1 * (-acc[q] + nxt(acc, q, rt));
@@ -220,14 +235,15 @@ __global__ __launch_bounds__(256) void k_rhs_x(Ctx c) {
   psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
   X.out((T*)F.hh[0], d);
   // P: RUH1 = (1/RE) D2x u - D1x P - PRESG
-  B = X.in((const T*)F.sp, 0);
+  B = X.put(pf);
+  X.pre((const T*)F.sw, 0, pf);
   SQ d[q] = (T)24. * dx1 * (-XA(B, q - 1) + XA(B, q));
   psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
   const T presg = (T)F.sc[SC_PRESG];
   SQ d[q] = rinv * acc[q] - d[q] - presg;
   X.out((T*)F.ruh[0], d);
   // w: RUH3, WW1
-  B = X.in((const T*)F.sw, 0);
+  B = X.put(pf);
   SQ d[q] = (T)12. * dx1q * (XA(B, q - 1) - (T)2. * XA(B, q) + XA(B, q + 1));
   psolve<T, SEG>(d, G, L, l, gs, s10, bf.xf, bf.xb);
   SQ d[q] = rinv * d[q];
@@ -238,15 +254,13 @@ __global__ __launch_bounds__(256) void k_rhs_x(Ctx c) {
 }
 
 // ------------------------------------------------------------ RHS z-pass
-// per z-line (i,j):
+// per z-line (i,j), first kernel (u and v):
 //   RUH1 += (1/RE) D2z u                                    (:6170-6185)
 //   UW    = Iz(u) * WW1 (stored);  HH1 += D1z(UW)           (:6360-6433)
 //   RUH2 += (1/RE) D2z v                          (j>=2)   (:6907-6923)
 //   VV3   = Iz v (stored);  HH2 += D1z(VV3*WW2)   (j>=2)   (:7023-7084)
-//   RUH3 += (1/RE) D2z w - D1z(P) - PRESG3                   (:7510-7550)
-//   HH3   = D1z((Iz w)^2)                                    (:7633-7672)
 template <typename T, int SEG>
-__global__ __launch_bounds__(256) void k_rhs_z(Ctx c) {
+__global__ __launch_bounds__(512) void k_rhs_z(Ctx c) {
   ZSETUP;
   const Scal s = c.s;
   const Metrics M = c.m;
@@ -293,8 +307,22 @@ __global__ __launch_bounds__(256) void k_rhs_z(Ctx c) {
     psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
     if (ok) SQ hh1[ZI(q)] = hh1[ZI(q)] + d[q];
   }
+}
+
+// z-lines, second kernel: RUH3 += (1/RE) D2z w - D1z(P) - PRESG3
+// (:7510-7550) and HH3 = D1z((Iz w)^2) (:7633-7672)
+template <typename T, int SEG>
+__global__ __launch_bounds__(512) void k_rhs_zw(Ctx c) {
+  ZSETUP;
+  const Scal s = c.s;
+  const Fields F = c.d;
+  const PSolve s6 = c.o.pz.s6, s10 = c.o.pz.s10, s22 = c.o.pz.s22;
+  T* ruh2 = (T*)F.ruh[2];
+  T* hh2 = (T*)F.hh[2];
+  const T dx3 = (T)s.dx3, dx3q = (T)s.dx3q, rinv = (T)s.rinv;
+  T a[SEG], d[SEG], e[SEG], lf, rt;
   // w, P: RUH3
-  zload(sw, b, pl, p0, bm, bp, a, lf, rt);
+  zload((const T*)F.sw, b, pl, p0, bm, bp, a, lf, rt);
   SQ e[q] = (T)12. * dx3q * (prv(a, q, lf) - (T)2. * a[q] + nxt(a, q, rt));
   psolve<T, SEG>(e, G, L, l, gs, s10, bf.xf, bf.xb);
   {
@@ -319,17 +347,20 @@ __global__ __launch_bounds__(256) void k_rhs_z(Ctx c) {
   if (ok) SQ hh2[ZI(q)] = d[q];
 }
 
+
 // -------------------------------------------------------- RHS x-pass 2
 // HH3 += D1x(UU3*WW1) face -> centre (:7734-7769)
 template <typename T, int SEG>
 __global__ __launch_bounds__(256) void k_rhs_x2(Ctx c) {
   XSETUP;
   const T dx1 = (T)c.s.dx1;
-  T d[SEG];
-  const T* B = X.in((const T*)c.d.uw, 0);
+  T d[SEG], pf[SEG];
+  X.pre((const T*)c.d.uw, 0, pf);
+  const T* B = X.put(pf);
+  X.pre((const T*)c.d.hh[2], 0, pf);
   SQ d[q] = (T)24. * dx1 * (-XA(B, q) + XA(B, q + 1));
   psolve<T, SEG>(d, G, L, l, gs, c.o.px.s22, bf.xf, bf.xb);
-  B = X.in((const T*)c.d.hh[2], 0);
+  B = X.put(pf);
   SQ d[q] = XA(B, q) + d[q];
   X.out((T*)c.d.hh[2], d);
 }
@@ -442,6 +473,200 @@ __global__ __launch_bounds__(256) void k_adi_y(Ctx c) {
       if (q < y.len) X[b + (long long)(y.j0 + q) * g.nx] = v[q];
 }
 
+// RHS y parts fused into the ADI y-sweep: each thread forms the final RUH of
+// its chunk points and feeds it straight into the TDMAI recurrences; RUH is
+// never stored, HH is.  Without SGS terms the thread walks its chunk in J
+// with the J-1, J, J+1 values of its component's fields in registers (the
+// expressions are those of yterm_u/v/w); with them it calls yterm_*.
+// Returns TDMAI's right-hand side BET(J) * RUH(J) * DT.
+template <typename T>
+__device__ __forceinline__ void yfuse_u(const Ctx& c, long long b, long long t2, const YPos& y, T (&v)[CHMAX]) {
+  const Geo& g = c.g;
+  const Metrics& M = c.m;
+  const Fields& F = c.d;
+  const T* su = (const T*)F.su;
+  const T* vv1 = (const T*)F.vv1;
+  const T* ruh = (const T*)F.ruh[0];
+  T* hh = (T*)F.hh[0];
+  const T* hp = (const T*)F.hhp[0];
+  const T rinv = (T)c.s.rinv, dt = (T)c.s.dt;
+  const long long nx = g.nx;
+  const T uN2 = su[b + g.n2 * nx], uW = su[b];
+  const T bcw = -(T)F.tauw[t2] / (T)M.dy[1] * (T)c.s.ifwall;
+  int J = y.j0;
+  T um = su[b + (J - 1) * nx], u0 = su[b + J * nx];
+  T w1 = vv1[b + J * nx];
+#pragma unroll
+  for (int q = 0; q < CHMAX; q++)
+    if (q < y.len) {
+      J = y.j0 + q;
+      const long long a = b + J * nx;
+      const T up = su[a + nx];
+      const T w2 = vv1[a + nx];
+      const T ru = ruh[a] + rinv * ((T)M.hp[J] * up - (T)M.hc[J] * u0 + (T)M.hm[J] * um);
+      const double fjup = M.fjpa[J], fjum = M.fjmu[J];
+      const T u2 = (T)M.cu2[J] * ((T)M.dyh[J] * up + (T)M.dyh[J + 1] * u0) + (T)(1. - fjup) * uN2;
+      const T u1 = (T)M.cu1[J] * ((T)M.dyh[J - 1] * u0 + (T)M.dyh[J] * um) + (T)(1. - fjum) * uW;
+      const T v2 = (J == g.n2m) ? (T)0 : w2;
+      const T v1 = (J == 1) ? (T)0 : w1;
+      const T h = hh[a] + (T)M.rdy[J] * (v2 * u2 - v1 * u1);
+      hh[a] = h;
+      const T r = ru - (T)(3. / 2.) * h + (T)(1. / 2.) * hp[a] + (T)(1. - fjum) * bcw;
+      v[q] = (T)M.ybet[0][J] * (r * dt);
+      um = u0;
+      u0 = up;
+      w1 = w2;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void yfuse_v(const Ctx& c, long long b, const YPos& y, T (&v)[CHMAX]) {
+  const Metrics& M = c.m;
+  const Fields& F = c.d;
+  const T* sv = (const T*)F.sv;
+  const T* sp = (const T*)F.sp;
+  const T* ruh = (const T*)F.ruh[1];
+  T* hh = (T*)F.hh[1];
+  const T* hp = (const T*)F.hhp[1];
+  const T rinv = (T)c.s.rinv, dt = (T)c.s.dt;
+  const long long nx = c.g.nx;
+  int J = y.j0;
+  T vm = sv[b + (J - 1) * nx], v0 = sv[b + J * nx];
+  T pm = sp[b + (J - 1) * nx];
+#pragma unroll
+  for (int q = 0; q < CHMAX; q++)
+    if (q < y.len) {
+      J = y.j0 + q;
+      const long long a = b + J * nx;
+      const T vp = sv[a + nx];
+      const T p0 = sp[a];
+      const T viscos = rinv * ((T)M.dyp[J] * vp - (T)M.dyc[J] * v0 + (T)M.dym[J] * vm);
+      const T pressg2 = (p0 - pm) / (T)M.h[J];
+      const T ru = ruh[a] + (-pressg2 + viscos);
+      const T v2 = (T)0.5 * (vp + v0);
+      const T v1 = (T)0.5 * (v0 + vm);
+      const T h = hh[a] + (T)M.rh[J] * (v2 * v2 - v1 * v1);
+      hh[a] = h;
+      const T r = ru - (T)(3. / 2.) * h + (T)(1. / 2.) * hp[a];
+      v[q] = (T)M.ybet[1][J] * (r * dt);
+      vm = v0;
+      v0 = vp;
+      pm = p0;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void yfuse_w(const Ctx& c, long long b, long long t2, const YPos& y, T (&v)[CHMAX]) {
+  const Geo& g = c.g;
+  const Metrics& M = c.m;
+  const Fields& F = c.d;
+  const T* sw = (const T*)F.sw;
+  const T* vv3 = (const T*)F.vv3;
+  const T* ruh = (const T*)F.ruh[2];
+  T* hh = (T*)F.hh[2];
+  const T* hp = (const T*)F.hhp[2];
+  const T rinv = (T)c.s.rinv, dt = (T)c.s.dt;
+  const long long nx = g.nx;
+  const T wN2 = sw[b + g.n2 * nx], wW = sw[b];
+  const T bcw = -(T)F.taub[t2] / (T)M.dy[1] * (T)c.s.ifwall;
+  int J = y.j0;
+  T wm = sw[b + (J - 1) * nx], w0 = sw[b + J * nx];
+  T s1 = vv3[b + J * nx];
+#pragma unroll
+  for (int q = 0; q < CHMAX; q++)
+    if (q < y.len) {
+      J = y.j0 + q;
+      const long long a = b + J * nx;
+      const T wp = sw[a + nx];
+      const T s2 = vv3[a + nx];
+      const T ru = ruh[a] + rinv * ((T)M.hp[J] * wp - (T)M.hc[J] * w0 + (T)M.hm[J] * wm);
+      const double fjup = M.fjpa[J], fjum = M.fjmu[J];
+      const T w2 = (T)M.cu2[J] * ((T)M.dyh[J + 1] * w0 + (T)M.dyh[J] * wp) + (T)(1. - fjup) * wN2;
+      const T w1 = (T)M.cu1[J] * ((T)M.dyh[J] * wm + (T)M.dyh[J - 1] * w0) + (T)(1. - fjum) * wW;
+      const T v2 = (J == g.n2m) ? (T)0 : s2;
+      const T v1 = (J == 1) ? (T)0 : s1;
+      const T h = hh[a] + (T)M.rdy[J] * (v2 * w2 - v1 * w1);
+      hh[a] = h;
+      const T r = ru - (T)(3. / 2.) * h + (T)(1. / 2.) * hp[a] + (T)(1. - fjum) * bcw;
+      v[q] = (T)M.ybet[2][J] * (r * dt);
+      wm = w0;
+      w0 = wp;
+      s1 = s2;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_rhsy_adiy(Ctx c) {
+  __shared__ T xs[4][256];
+  const Geo g = c.g;
+  const YShape ys = c.o.ys;
+  const int comp = blockIdx.y;
+  const int ns = (comp == 1) ? 2 : 1;
+  const int L = ys.L, C = ys.C;
+  const int t = threadIdx.x, l = t % L;
+  YPos y;
+  y.l = l;
+  y.c = t / L;
+  const int m = g.n2m - ns + 1;
+  const int ch = (m + C - 1) / C;
+  y.j0 = ns + y.c * ch;
+  y.len = max(0, min(ch, g.n2m + 1 - y.j0));
+  const int nit = (g.nx + L - 1) / L;
+  const int it = blockIdx.x % nit;
+  const int k = blockIdx.x / nit;
+  const int i = it * L + l;
+  const bool ok = i < g.nx;
+  const Fields F = c.d;
+  const Metrics M = c.m;
+  T* X = (T*)(comp == 0 ? F.uh[0] : comp == 1 ? F.uh[1] : F.uh[2]);
+  const double* A = comp == 0 ? M.ya[0] : comp == 1 ? M.ya[1] : M.ya[2];
+  const double* BET = comp == 0 ? M.ybet[0] : comp == 1 ? M.ybet[1] : M.ybet[2];
+  const double* GAM = comp == 0 ? M.ygam[0] : comp == 1 ? M.ygam[1] : M.ygam[2];
+  const T dt = (T)c.s.dt;
+  const long long b = gidx(g, ok ? i : 0, 0, k);
+  const long long t2 = (long long)k * g.nx + (ok ? i : 0);
+  T v[CHMAX], mm[CHMAX];
+#pragma unroll
+  for (int q = 0; q < CHMAX; q++) v[q] = (T)0;
+  if (ok && y.len > 0) {
+    if (c.s.files) {
+#pragma unroll
+      for (int q = 0; q < CHMAX; q++)
+        if (q < y.len) {
+          const int J = y.j0 + q;
+          const YIdx x = yidx(g, i, J, k);
+          const T r = comp == 0 ? yterm_u<T>(c, x, i, J, k)
+                                : comp == 1 ? yterm_v<T>(c, x, i, J, k) : yterm_w<T>(c, x, i, J, k);
+          v[q] = (T)BET[J] * (r * dt);
+        }
+    } else if (comp == 0) {
+      yfuse_u<T>(c, b, t2, y, v);
+    } else if (comp == 1) {
+      yfuse_v<T>(c, b, y, v);
+    } else {
+      yfuse_w<T>(c, b, t2, y, v);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < CHMAX; q++)
+    if (q < y.len) {
+      const int J = y.j0 + q;
+      mm[q] = (J == ns) ? (T)0 : -(T)A[J] * (T)BET[J];
+    }
+  ychain_fwd(v, mm, y, C, L, xs[0], xs[1]);
+#pragma unroll
+  for (int q = 0; q < CHMAX; q++)
+    if (q < y.len) {
+      const int J = y.j0 + q;
+      mm[q] = (J == g.n2m) ? (T)0 : -(T)GAM[J];
+    }
+  ychain_bwd(v, mm, y, C, L, xs[2], xs[3]);
+  if (ok)
+#pragma unroll
+    for (int q = 0; q < CHMAX; q++)
+      if (q < y.len) X[b + (long long)(y.j0 + q) * g.nx] = v[q];
+}
+
 // --------------------------------------------------- GETUH x / z sweeps
 // CTDMA1J (x, :8285-8311) / CTDMA3I (z, :8314-8338): solve
 // (A, B, A) x = UH(p-1) + 10 UH(p) + UH(p+1); `fin` adds U (UH = U + delta,
@@ -461,19 +686,21 @@ __global__ __launch_bounds__(256) void k_adi_x(Ctx c, CompList cl) {
   const Fields F = c.d;
   T* UH = (T*)(comp == 0 ? F.uh[0] : comp == 1 ? F.uh[1] : F.uh[2]);
   const double* U = comp == 0 ? F.u : comp == 1 ? F.v : F.w;
-  T d[SEG];
-  const T* B = X.in((const T*)UH, 0);
+  T d[SEG], pf[SEG];
+  X.pre((const T*)UH, 0, pf);
+  const T* B = X.put(pf);
+  if (fin) X.pre(U, 0, pf);
   SQ d[q] = XA(B, q - 1) + (T)10. * XA(B, q) + XA(B, q + 1);
   psolve<T, SEG>(d, G, L, l, gs, c.o.px.adi, bf.xf, bf.xb);
   if (fin) {
-    B = X.in(U, 0);
+    B = X.put(pf);
     SQ d[q] = XA(B, q) + d[q];
   }
   X.out(UH, d, comp == 1 ? 2 : 1);  // V rows J=1 are not part of the system
 }
 
 template <typename T, int SEG>
-__global__ __launch_bounds__(256) void k_adi_z(Ctx c, CompList cl) {
+__global__ __launch_bounds__(512) void k_adi_z(Ctx c, CompList cl) {
   ZSETUP;
   const int sel = blockIdx.y;
   const int comp = sel == 0 ? cl.comp[0] : sel == 1 ? cl.comp[1] : cl.comp[2];
@@ -504,21 +731,24 @@ __global__ __launch_bounds__(256) void k_rhsdp_x(Ctx c) {
   const Fields F = c.d;
   const Metrics M = c.m;
   const T dx1 = (T)c.s.dx1, dt = (T)c.s.dt;
-  T d[SEG], e[SEG];
-  const T* B = X.in((const T*)F.uh[0], 0);
+  T d[SEG], e[SEG], pf[SEG];
+  X.pre((const T*)F.uh[0], 0, pf);
+  const T* B = X.put(pf);
+  X.pre((const T*)F.uh[1], 1, pf);
   SQ d[q] = (T)24. * dx1 * (XA(B, q + 1) - XA(B, q));
   psolve<T, SEG>(d, G, L, l, gs, c.o.px.s22, bf.xf, bf.xb);
   const T fjpa = (T)M.fjpa[j], fjmu = (T)M.fjmu[j], dy = (T)M.dy[j];
-  B = X.in((const T*)F.uh[1], 1);
+  B = X.put(pf);
+  X.pre((const T*)F.uh[1], 0, pf);
   SQ e[q] = fjpa * XA(B, q);
-  B = X.in((const T*)F.uh[1], 0);
+  B = X.put(pf);
   SQ d[q] = (e[q] - fjmu * XA(B, q)) / dy / dt + d[q] / dt;
   X.out((T*)F.rdp, d);
 }
 
 // z-lines: RDP += D1z(UH3)/DT (:8196-8211)
 template <typename T, int SEG>
-__global__ __launch_bounds__(256) void k_rhsdp_z(Ctx c) {
+__global__ __launch_bounds__(512) void k_rhsdp_z(Ctx c) {
   ZSETUP;
   const T dx3 = (T)c.s.dx3, dt = (T)c.s.dt;
   T a[SEG], d[SEG], lf, rt;
@@ -854,26 +1084,32 @@ __global__ __launch_bounds__(256) void k_upd_x(Ctx c, int chk) {
   const Fields F = c.d;
   const T dx1 = (T)s.dx1, dt = (T)s.dt;
   const T dmp = (T)F.sc[SC_DMPRESG];
-  T dpr[SEG], d[SEG], un[SEG];
-  const T* B = X.in((const T*)F.dp, 0);
+  T dpr[SEG], d[SEG], un[SEG], pf[SEG];
+  X.pre((const T*)F.dp, 0, pf);
+  const T* B = X.put(pf);
+  X.pre((const T*)F.uh[0], 0, pf);
   SQ {
     dpr[q] = XA(B, q);
     d[q] = (T)24. * dx1 * (-XA(B, q - 1) + XA(B, q));
   }
   psolve<T, SEG>(d, G, L, l, gs, c.o.px.s22, bf.xf, bf.xb);
-  B = X.in((const T*)F.uh[0], 0);
+  B = X.put(pf);
+  X.pre((const T*)F.uh[1], 0, pf);
   SQ un[q] = XA(B, q) - dt * d[q] - dt * dmp;
   X.out(F.u, un);
   const T hj = (T)M.h[j];
-  const T* B1 = X.in((const T*)F.uh[1], 0);
-  B = X.in((const T*)F.dp, -1);
+  const T* B1 = X.put(pf);
+  X.pre((const T*)F.dp, -1, pf);
+  B = X.put(pf);
+  if (chk) X.pre((const T*)F.uh[1], 1, pf);
   SQ d[q] = (j >= 2) ? XA(B1, q) - dt * (dpr[q] - XA(B, q)) / hj : (T)0;
   X.out(F.v, d, 2);  // V(I,1,K) is not written by UPCALC
   if (!chk) return;
   // new V at J+1 (zero at J+1 = N2: the wall value); d = (V(J+1) - V(J))/DY
   const T hjp = (T)M.h[j + 1], dy = (T)M.dy[j];
-  B1 = X.in((const T*)F.uh[1], 1);
-  B = X.in((const T*)F.dp, 1);
+  B1 = X.put(pf);
+  X.pre((const T*)F.dp, 1, pf);
+  B = X.put(pf);
   SQ d[q] = (((j + 1 <= g.n2m) ? XA(B1, q) - dt * (XA(B, q) - dpr[q]) / hjp : (T)0) - d[q]) / dy;
   const T rt = xch_right(un[0], G, L, l, gs, bf.xr);
   SQ dpr[q] = (T)24. * dx1 * (nxt(un, q, rt) - un[q]);
@@ -886,7 +1122,7 @@ __global__ __launch_bounds__(256) void k_upd_x(Ctx c, int chk) {
 // (:1218-1230); with `chk`: DIV += D1z(U3), DIVMAX, and the CFL maximum of
 // the new field (:9112-9128, :9150-9162).
 template <typename T, int SEG>
-__global__ __launch_bounds__(256) void k_upd_z(Ctx c, int chk) {
+__global__ __launch_bounds__(512) void k_upd_z(Ctx c, int chk) {
   ZSETUP;
   const Scal s = c.s;
   const Metrics M = c.m;
@@ -922,18 +1158,48 @@ __global__ __launch_bounds__(256) void k_upd_z(Ctx c, int chk) {
       cmax = (cfll != cfll || cmax != cmax) ? NAN : fmax(cmax, cfll);
     }
   }
+  // block maxima into 64 spread slots (bit patterns of non-negative
+  // doubles order like the values); k_max_fin reduces the slots
+  __shared__ double red[2][8];
   dmax = wave_max(dmax);
   cmax = wave_max(cmax);
+  const int wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
   if ((threadIdx.x & 63) == 0) {
-    atomic_max_pos(&F.sc[SC_DIVMAX], dmax);
-    atomic_max_pos(&F.sc[SC_CFLM], cmax);
+    red[0][wv] = dmax;
+    red[1][wv] = cmax;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < nw; q++) {
+      dmax = (dmax != dmax || red[0][q] != red[0][q]) ? NAN : fmax(dmax, red[0][q]);
+      cmax = (cmax != cmax || red[1][q] != red[1][q]) ? NAN : fmax(cmax, red[1][q]);
+    }
+    atomic_max_pos(&F.red[blockIdx.x & 63], dmax);
+    atomic_max_pos(&F.red[64 + (blockIdx.x & 63)], cmax);
+  }
+}
+
+// final maxima of the 64 slots of k_upd_z into SC_DIVMAX, SC_CFLM
+__global__ void k_max_fin(Ctx c) {
+  const unsigned long long* r = (const unsigned long long*)c.d.red;
+  unsigned long long a = r[threadIdx.x], b = r[64 + threadIdx.x];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long x = __shfl_xor(a, o, 64), y = __shfl_xor(b, o, 64);
+    a = x > a ? x : a;
+    b = y > b ? y : b;
+  }
+  if (threadIdx.x == 0) {
+    ((unsigned long long*)c.d.sc)[SC_DIVMAX] = a;
+    ((unsigned long long*)c.d.sc)[SC_CFLM] = b;
   }
 }
 
 // =================================================================== host
 namespace c3dk {
 
-// compile-time segment lengths: x-lines 8 or 2, z-lines 16, 8 or 2
+// compile-time segment lengths: 8, or 2 when the line length is not a
+// multiple of 8
 #define XSEG(seg, ...)                \
   if ((seg) == 8) {                   \
     constexpr int S_ = 8;             \
@@ -942,17 +1208,7 @@ namespace c3dk {
     constexpr int S_ = 2;             \
     __VA_ARGS__;                      \
   }
-#define ZSEG(seg, ...)                \
-  if ((seg) == 16) {                  \
-    constexpr int S_ = 16;            \
-    __VA_ARGS__;                      \
-  } else if ((seg) == 8) {            \
-    constexpr int S_ = 8;             \
-    __VA_ARGS__;                      \
-  } else {                            \
-    constexpr int S_ = 2;             \
-    __VA_ARGS__;                      \
-  }
+#define ZSEG(seg, ...) XSEG(seg, __VA_ARGS__)
 
 // LDS bytes of a line kernel
 static size_t lds_line(const LShape& sh, bool staged, size_t es) {
@@ -969,14 +1225,21 @@ static void rhs_lines(const Ctx& c) {
   const size_t lx = lds_line(c.o.lx, true, sizeof(T)), lz = lds_line(c.o.lz, false, sizeof(T));
   XSEG(c.o.lx.seg, { KScope ks_(c, KID_RHS_X); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rhs_x<T, S_>), dim3(xblocks(c)), xthreads(c), lx, c.st, c); });
   ZSEG(c.o.lz.seg, { KScope ks_(c, KID_RHS_Z); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rhs_z<T, S_>), dim3(zblocks(c)), zthreads(c), lz, c.st, c); });
+  ZSEG(c.o.lz.seg, { KScope ks_(c, KID_RHS_ZW); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rhs_zw<T, S_>), dim3(zblocks(c)), zthreads(c), lz, c.st, c); });
   XSEG(c.o.lx.seg, { KScope ks_(c, KID_RHS_X2); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rhs_x2<T, S_>), dim3(xblocks(c)), xthreads(c), lx, c.st, c); });
 }
 void rhs_lines_fp64(const Ctx& c) { rhs_lines<double>(c); }
 
 template <typename T>
-static void getuh_t(const Ctx& c) {
+static void getuh_t(const Ctx& c, bool fused) {
   const int ylx = (c.g.nx + c.o.ys.L - 1) / c.o.ys.L;
-  { KScope ks_(c, KID_ADI_Y); hipLaunchKernelGGL(k_adi_y<T>, dim3(ylx * c.g.nz, 3), dim3(c.o.ys.L * c.o.ys.C), 0, c.st, c); }
+  if (fused) {
+    KScope ks_(c, KID_ADI_Y);
+    hipLaunchKernelGGL(k_rhsy_adiy<T>, dim3(ylx * c.g.nz, 3), dim3(c.o.ys.L * c.o.ys.C), 0, c.st, c);
+  } else {
+    KScope ks_(c, KID_ADI_Y);
+    hipLaunchKernelGGL(k_adi_y<T>, dim3(ylx * c.g.nz, 3), dim3(c.o.ys.L * c.o.ys.C), 0, c.st, c);
+  }
   const size_t lx = lds_line(c.o.lx, true, sizeof(T)), lz = lds_line(c.o.lz, false, sizeof(T));
   // GETUH1/2: y, x, z;  GETUH3: y, z, x (compact3d.f:8508-8607)
   const CompList x12{2, {0, 1, 0}, {0, 0, 0}};
@@ -986,7 +1249,8 @@ static void getuh_t(const Ctx& c) {
   ZSEG(c.o.lz.seg, { KScope ks_(c, KID_ADI_Z); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adi_z<T, S_>), dim3(zblocks(c), 3), zthreads(c), lz, c.st, c, z123); });
   XSEG(c.o.lx.seg, { KScope ks_(c, KID_ADI_X); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adi_x<T, S_>), dim3(xblocks(c), 1), xthreads(c), lx, c.st, c, x3); });
 }
-void getuh(const Ctx& c) { getuh_t<double>(c); }
+void getuh(const Ctx& c) { getuh_t<double>(c, false); }
+void getuh_fused(const Ctx& c) { getuh_t<double>(c, true); }
 
 template <typename T>
 static void rhsdp_t(const Ctx& c) {
@@ -1027,7 +1291,12 @@ static void upd_x_t(const Ctx& c, bool chk) {
 template <typename T>
 static void upd_z_t(const Ctx& c, bool chk) {
   const size_t lz = lds_line(c.o.lz, false, sizeof(T));
+  if (chk) zero(c, c.d.red, 128);
   ZSEG(c.o.lz.seg, { KScope ks_(c, KID_UPD_Z); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_upd_z<T, S_>), dim3(zblocks(c)), zthreads(c), lz, c.st, c, chk ? 1 : 0); });
+  if (chk) {
+    KScope ks_(c, KID_FLOW_FIN);
+    hipLaunchKernelGGL(k_max_fin, dim3(1), dim3(64), 0, c.st, c);
+  }
 }
 void upd_x(const Ctx& c, bool chk) { upd_x_t<double>(c, chk); }
 void upd_z(const Ctx& c, bool chk) { upd_z_t<double>(c, chk); }

@@ -134,7 +134,7 @@ int c3d_apply_op(c3d_handle *h, int op, int dir, const double *in, double *out);
  * on the handle's stream (up to cap launches); c3d_kernel_times()
  * synchronises, returns per-kernel-id summed milliseconds and launch counts
  * (arrays of C3D_NKID entries) and resets the recorder. */
-#define C3D_NKID 32
+#define C3D_NKID 48
 int c3d_timing(c3d_handle *h, int on, int capacity);
 int c3d_kernel_times(c3d_handle *h, double *ms, int *count);
 const char *c3d_kernel_name(int id);
