@@ -42,6 +42,14 @@ ALG_BYTES = {
 }
 # adi_x runs twice per step: (u,v) read+write 32 B, then w with U added 24 B
 ALG_BYTES_ADI_X_PER_STEP = 56
+# mixed mode: FP32 intermediates and state shadows, FP64 state, RDP, DP, DIV
+# (state read as FP32 shadows by the line kernels; the update reads and
+# writes the FP64 state and writes the shadows)
+ALG_BYTES_MIXED = {
+    "rhs_x": 44, "rhs_z": 56, "rhs_zw": 20, "rhs_x2": 12, "adi_y": 84, "adi_z": 24,
+    "rhsdp_x": 24, "rhsdp_z": 20, "fft_x": 16, "fft_z": 16, "pois_y": 16, "fft_xinv": 16,
+    "flow_red": 32, "upd_x": 64, "upd_z": 92,
+}
 
 
 def workload(grid):
@@ -80,6 +88,56 @@ def cpu_baseline(cfg, target_s, seed):
                        f"subroutines), OpenMP {threads} threads, {el:.1f} s")
 
 
+def run_leg(c3d, cfg, U, P, precision, args, pts):
+    """warm-up, then args.steps timed steps (GETUP + DIVCHECK + CFL read back
+    every step, as MAIN's loop does) of one precision"""
+    import torch
+    ch = c3d.Channel(**cfg, precision=precision)
+    ch.upload(U, P)
+    for n in range(args.warmup):
+        ch.step(n + 1)
+    torch.cuda.synchronize()
+    if not args.no_timing:
+        ch.timing(True, 48 * args.steps + 64)
+    torch.cuda.synchronize()
+    dmax = 0.0
+    t0 = time.perf_counter()
+    for n in range(args.steps):
+        d, _ = ch.step(args.warmup + n + 1)
+        dmax = max(dmax, d) if d == d else float("nan")
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    kt = ch.kernel_times() if not args.no_timing else {}
+    ch.timing(False)
+    dt_s = (t1 - t0) / args.steps
+    kernels = {}
+    for name, (ms, cnt) in kt.items():
+        kernels[name] = dict(avg_us=round(1000.0 * ms / cnt, 2), launches=cnt,
+                             share=round(ms / (1000.0 * dt_s * args.steps), 4))
+    return ch, dict(value=pts / dt_s / 1e6, dt_s=dt_s, divmax_max=dmax, kt=kt, kernels=kernels)
+
+
+def roofline(kt, table, pts, grid):
+    if not kt:
+        return None
+    dom = max((k for k in kt if k in table), key=lambda k: kt[k][0])
+    avg_s = kt[dom][0] / kt[dom][1] / 1000.0
+    alg = table[dom] * pts
+    achieved = alg / avg_s / 1e9
+    traffic = None
+    tp = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tp):
+        try:
+            rec = json.load(open(tp)).get(f"{dom}@{grid}")
+            if rec:
+                traffic = rec["bytes_per_launch"]
+        except Exception:
+            traffic = None
+    return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel=dom,
+                alg_bytes_per_launch=alg, avg_launch_us=round(avg_s * 1e6, 2))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,55 +166,40 @@ def main():
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a GPU")
     torch.cuda.init()
-    ch = c3d.Channel(**cfg)
-    m = ch.mesh()  # the grid the library built, for the initial field
+    probe = c3d.Channel(**cfg)
+    m = probe.mesh()  # the grid the library built, for the initial field
+    probe.close()
     U, P = perturbed_poiseuille(cfg["n1"], cfg["n2"], cfg["n3"], m["Y"], m["DX1"], m["DX3"], m["ALY"], seed=args.seed)
-    ch.upload(U, P)
+    ch64, r64 = run_leg(c3d, cfg, U, P, c3d.FP64, args, pts)
+    chmx, rmx = run_leg(c3d, cfg, U, P, c3d.MIXED, args, pts)
     del U, P
-    for n in range(args.warmup):
-        ch.step(n + 1, checks=False)
-    torch.cuda.synchronize()
-    nlaunch = 40 * args.steps + 64
-    if not args.no_timing:
-        ch.timing(True, nlaunch)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for n in range(args.steps):
-        ch._ck(ch.L.c3d_step(ch.h, args.warmup + n + 1, None, None), "step")
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    kt = ch.kernel_times() if not args.no_timing else {}
-    ch.timing(False)
-    dt_s = (t1 - t0) / args.steps
-    value = pts / dt_s / 1e6
-    divmax = ch.divcheck()
-    cflm = ch.cfl()
-    st_presg = ch.scalar(c3d.S_PRESG)
-    finite = math.isfinite(divmax) and math.isfinite(cflm)
-
-    roof = None
-    kernels = {}
-    if kt:
-        for name, (ms, cnt) in kt.items():
-            kernels[name] = dict(avg_us=1000.0 * ms / cnt, launches=cnt, share=ms / (1000.0 * dt_s * args.steps))
-        dom = max((k for k in kt if k in ALG_BYTES), key=lambda k: kt[k][0])
-        avg_s = kt[dom][0] / kt[dom][1] / 1000.0
-        alg = ALG_BYTES[dom] * pts
-        achieved = alg / avg_s / 1e9
-        traffic = None
-        tp = os.path.join(ROOT, "profiles", "traffic.json")
-        if os.path.exists(tp):
-            try:
-                tj = json.load(open(tp))
-                rec = tj.get(f"{dom}@{args.grid}")
-                if rec:
-                    traffic = rec["bytes_per_launch"]
-            except Exception:
-                traffic = None
-        roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel=dom,
-                    alg_bytes_per_launch=alg, avg_launch_us=round(avg_s * 1e6, 2))
-    step_frac = 112.0 * pts / dt_s / (HBM_PEAK_GBS * 1e9)
+    # both legs ran warmup + steps steps from the same field: mixed vs FP64
+    s64, smx = ch64.download(), chmx.download()
+    n1, n2, n3 = cfg["n1"], cfg["n2"], cfg["n3"]
+    errs = {}
+    for name, comp in (("u", 0), ("v", 1), ("w", 2)):
+        a_ = s64["U"][1:n1, 1:n2, 1:n3, comp]
+        b_ = smx["U"][1:n1, 1:n2, 1:n3, comp]
+        errs[name] = dict(max=float(np.abs(a_ - b_).max() / np.abs(a_).max()),
+                          l2=float(np.linalg.norm(a_ - b_) / np.linalg.norm(a_)))
+    a_, b_ = s64["P"][: n1 - 1, : n2 - 1, : n3 - 1], smx["P"][: n1 - 1, : n2 - 1, : n3 - 1]
+    errs["p"] = dict(max=float(np.abs(a_ - b_).max() / np.abs(a_).max()),
+                     l2=float(np.linalg.norm(a_ - b_) / np.linalg.norm(a_)))
+    ub64 = float(np.mean(s64["U"][1:n1, 1:n2 - 1, 1:n3, 0]))
+    ubmx = float(np.mean(smx["U"][1:n1, 1:n2 - 1, 1:n3, 0]))
+    errs["mean_u"] = abs(ubmx - ub64) / abs(ub64)
+    del s64, smx
+    refines = chmx.scalar(c3d.S_REFINES)
+    cflm = chmx.cfl()
+    st_presg = chmx.scalar(c3d.S_PRESG)
+    ch64.close()
+    chmx.close()
+    value = rmx["value"]
+    dt_s = rmx["dt_s"]
+    finite = math.isfinite(rmx["divmax_max"]) and math.isfinite(cflm)
+    roof = roofline(rmx["kt"], ALG_BYTES_MIXED, pts, args.grid)
+    roof64 = roofline(r64["kt"], ALG_BYTES, pts, args.grid)
+    step_frac = 112.0 * pts / r64["dt_s"] / (HBM_PEAK_GBS * 1e9)
 
     cpu = None
     if not args.no_cpu:
@@ -173,23 +216,28 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64 mixed",
         "data": "synthetic: seeded divergence-free perturbed Poiseuille field (seed %d)" % args.seed,
-        "config": {"workload": "channel DNS step (GETUP+DIVCHECK+CFL), %s, Re_tau=395 (RE=10312.5), "
-                               "dt=7.5e-4, FP64 path" % args.grid,
+        "config": {"workload": "channel DNS step (GETUP+DIVCHECK+CFL, DIVMAX and CFL read back every "
+                               "step), %s, Re_tau=395 (RE=10312.5), dt=7.5e-4; value = mixed path, "
+                               "fp64 = FP64 path on the same field" % args.grid,
                    "grid": args.grid, "re": cfg["re"], "dt": cfg["dt"], "cks": cfg["cks"],
-                   "precision": "fp64", "parallelism": "single GPU"},
+                   "precision": "mixed", "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "step_alg_frac": round(step_frac, 4),
-        "divmax_after": divmax,
+        "fp64": {"value": round(r64["value"], 2), "ms_per_step": round(r64["dt_s"] * 1000.0, 4),
+                 "divmax_max": r64["divmax_max"], "roofline": roof64, "kernels": r64["kernels"],
+                 "step_alg_frac": round(step_frac, 4)},
+        "mixed_over_fp64": round(value / r64["value"], 3),
+        "mixed_divmax_max": rmx["divmax_max"],
+        "mixed_refinements": refines,
+        "mixed_vs_fp64_err": errs,
         "cfl_after": cflm * cfg["dt"],
         "presg_after": st_presg,
         "finite": finite,
-        "kernels": kernels,
+        "kernels": rmx["kernels"],
     }
     print(json.dumps(out))
-    ch.close()
 
 
 if __name__ == "__main__":

@@ -22,7 +22,7 @@ LIB_PATH = os.path.join(HERE, "libc3d.so")
 
 FP64, MIXED = 0, 1
 F_RUH1, F_RUH2, F_RUH3, F_UH1, F_UH2, F_UH3, F_RDP, F_DP, F_HH1, F_HH2, F_HH3, F_TAUW, F_TAUB = range(1, 14)
-S_PRESG, S_PRESG3, S_DMPRESG, S_DMPRESG3, S_FLOW1, S_FLOW3, S_WSM1, S_WSM2, S_WSM3 = range(9)
+S_PRESG, S_PRESG3, S_DMPRESG, S_DMPRESG3, S_FLOW1, S_FLOW3, S_WSM1, S_WSM2, S_WSM3, S_REFINES, S_DIVMAX1 = range(11)
 NKID = 48  # C3D_NKID of include/c3d.h: entries of the per-kernel timing tables
 
 _dp = C.POINTER(C.c_double)

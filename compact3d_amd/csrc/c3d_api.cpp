@@ -22,10 +22,15 @@ struct c3d_handle {
   std::string err;
   std::vector<void*> allocs;
   int hh_cur = 0;               // which of the two HH sets is "current"
+  double* vbuf[2] = {nullptr, nullptr};  // mixed mode: the two V buffers
+  int vcur = 0;
+  long long refines = 0;        // mixed mode: refinement passes run so far
+  double first_divmax = 0.0;    // mixed mode: DIVMAX of the last first projection
   double* hhbuf[2][3] = {{nullptr}};
   // host copies of the 1-D vectors (reference J indexing)
   std::vector<double> Y, DY, H, HM, HC, HP, DYM, DYC, DYP, FJPA, FJMU, FJMV, PMJ, PCJ, PPJ, AK1, AK3;
   double* dvec = nullptr;       // device block holding all vectors
+  double* dytab = nullptr;      // fused y-sweep coefficient table
   double* dtmp = nullptr;       // scratch field for apply_op
   double* dtmp2 = nullptr;
   std::vector<double> stage;    // host staging buffer
@@ -334,13 +339,51 @@ static void build_dt_terms(c3d_handle* h, std::vector<double>& ych) {
   o.pz = make_opset(o.lz.seg, o.adiz_a, o.adiz_b);
 }
 
+// coefficient table of the fused RHS-y / ADI-y sweep (layout in c3d_impl.h)
+static void build_ytab(c3d_handle* h, const std::vector<double>& ych, std::vector<double>& tab) {
+  const int N2 = h->cfg.n2, N2M = N2 - 1;
+  const size_t nv = (size_t)(N2 + 2);
+  tab.assign(3 * nv * YTAB_W, 0.0);
+  for (int comp = 0; comp < 3; comp++) {
+    const double* A = &ych[(3 * comp + 0) * nv];
+    const double* BET = &ych[(3 * comp + 1) * nv];
+    const double* GAM = &ych[(3 * comp + 2) * nv];
+    const int ns = (comp == 1) ? 2 : 1;
+    for (int J = 1; J <= N2M; J++) {
+      double* t = &tab[(comp * nv + J) * YTAB_W];
+      if (comp == 1) {
+        t[0] = h->DYP[J];
+        t[1] = h->DYC[J];
+        t[2] = h->DYM[J];
+        t[3] = h->H[J];
+        t[4] = 1. / h->H[J];
+      } else {
+        t[0] = h->HP[J];
+        t[1] = h->HC[J];
+        t[2] = h->HM[J];
+        t[3] = h->FJPA[J] * 1.0 / h->H[J + 1];
+        t[4] = h->FJMU[J] * 1.0 / h->H[J];
+        t[5] = h->DY[J - 1] / 2.;
+        t[6] = h->DY[J] / 2.;
+        t[7] = h->DY[J + 1] / 2.;
+        t[8] = 1. / h->DY[J];
+        t[9] = 1. - h->FJPA[J];
+        t[10] = 1. - h->FJMU[J];
+      }
+      t[YT_BET] = BET[J];
+      t[YT_M] = (J == ns) ? 0.0 : -A[J] * BET[J];
+      t[YT_G] = (J == N2M) ? 0.0 : -GAM[J];
+    }
+  }
+}
+
 static int upload_dt_terms(c3d_handle* h) {
-  std::vector<double> ych;
+  std::vector<double> ych, tab;
   build_dt_terms(h, ych);
-  const int N2 = h->cfg.n2;
   double* base = (double*)h->c.m.ya[0];
   HIPCHK(hipMemcpy(base, ych.data(), ych.size() * sizeof(double), hipMemcpyHostToDevice));
-  (void)N2;
+  build_ytab(h, ych, tab);
+  HIPCHK(hipMemcpy(h->dytab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
   return 0;
 }
 
@@ -358,9 +401,15 @@ long long c3d_footprint_bytes(const c3d_config* cfg) {
   if (!cfg) return -1;
   long long nx = cfg->n1 - 1, nz = cfg->n3 - 1, nj = cfg->n2 + 1, n2m = cfg->n2 - 1;
   long long fsz = nx * nj * nz;
-  long long fields = 26;
-  long long spec = 2 * (nx / 2 + 1) * nz * n2m * 16;  // spectrum + TDMAP factors
-  return fields * fsz * 8 + spec + 4 * nx * nz * 8 + (1 << 20);
+  const bool mixed = cfg->precision == C3D_MIXED;
+  const long long es = mixed ? 4 : 8;
+  // FP64: state 4, DP, RDP, SGSVIS, DIV, 2 scratch (+ second V in mixed
+  // mode); intermediates 10 + HH 6; FP32 shadows 4 in mixed mode; LES 7
+  long long bytes = (10 + (mixed ? 1 : 0)) * fsz * 8 + 16 * fsz * es + (mixed ? 4 * fsz * 4 : 0);
+  if (cfg->files != 0.0) bytes += 7 * fsz * 8;
+  const long long nm = (nx / 2 + 1) * nz * n2m;
+  bytes += nm * 16 + nm * 16;  // spectrum + TDMAP factors
+  return bytes + 4 * nx * nz * 8 + (1 << 20);
 }
 
 extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
@@ -380,7 +429,8 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   };
   const c3d_config& cf = h->cfg;
   if (cf.n1 < 9 || cf.n2 < 5 || cf.n3 < 9) return bad("grid too small (need N1,N3 >= 9, N2 >= 5)");
-  if (cf.precision != C3D_FP64) return bad("only C3D_FP64 is available in this build");
+  if (cf.precision != C3D_FP64 && cf.precision != C3D_MIXED) return bad("precision must be C3D_FP64 or C3D_MIXED");
+  if (cf.precision == C3D_MIXED && cf.files != 0.0) return bad("C3D_MIXED supports FILES = 0 (DNS) only");
   if (cf.files != 0.0 && cf.files != 2.0) return bad("FILES must be 0 (DNS) or 2 (Smagorinsky)");
   if (cf.wmodel != 0.0 && cf.wmodel != 1.0) return bad("WMODEL must be 0 or 1");
   if (cf.re <= 0 || cf.dt <= 0 || cf.alx <= 0 || cf.alz <= 0) return bad("RE, DT, ALX, ALZ must be > 0");
@@ -485,6 +535,8 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
       m.ygam[q] = yc + (3 * q + 2) * nv;
     }
   }
+  if (dalloc(h, &h->dytab, 3 * (size_t)(g.n2 + 2) * YTAB_W)) return bad(h->err);
+  c.m.ytab = h->dytab;
   if (upload_dt_terms(h)) return bad(h->err);
   // FFT twiddles
   {
@@ -506,18 +558,40 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   // fields
   Fields& F = c.d;
   const size_t fs = (size_t)g.fsz;
-  double** fl[] = {&F.u,      &F.v,      &F.w,      &F.p,      &F.ruh[0], &F.ruh[1], &F.ruh[2],
-                   &F.uh[0],  &F.uh[1],  &F.uh[2],  &F.vv1,    &F.ww1,    &F.uw,     &F.vv3,
-                   &F.rdp,    &F.dp,     &F.sgs,    &F.div,    &h->dtmp,  &h->dtmp2};
-  for (auto pp : fl)
+  // FP64 always: the state, SGSVIS, DIV and host-transfer scratch
+  double** f64[] = {&F.u, &F.v, &F.w, &F.p, &F.sgs, &F.div, &h->dtmp, &h->dtmp2};
+  for (auto pp : f64)
     if (dalloc(h, pp, fs)) return bad(h->err);
-  F.su = F.u;
-  F.sv = F.v;
-  F.sw = F.w;
-  F.sp = F.p;
+  // intermediates: FP32 in mixed mode (allocated as doubles of that size)
+  const bool mixed = cf.precision == C3D_MIXED;
+  c.s.mixed = mixed ? 1 : 0;
+  const size_t fsi = mixed ? (fs + 1) / 2 : fs;
+  if (dalloc(h, &F.dp, fs) || dalloc(h, &F.rdp, fs)) return bad(h->err);
+  double** fin[] = {&F.ruh[0], &F.ruh[1], &F.ruh[2], &F.uh[0], &F.uh[1], &F.uh[2],
+                    &F.vv1,    &F.ww1,    &F.uw,     &F.vv3};
+  for (auto pp : fin)
+    if (dalloc(h, pp, fsi)) return bad(h->err);
+  F.vold = nullptr;
+  if (mixed) {
+    h->vbuf[0] = F.v;
+    if (dalloc(h, &h->vbuf[1], fs)) return bad(h->err);
+    float** f32[] = {&F.uf, &F.vf, &F.wf, &F.pf};
+    for (auto pp : f32)
+      if (dalloc(h, pp, fs)) return bad(h->err);
+    F.su = F.uf;
+    F.sv = F.vf;
+    F.sw = F.wf;
+    F.sp = F.pf;
+  } else {
+    F.uf = F.vf = F.wf = F.pf = nullptr;
+    F.su = F.u;
+    F.sv = F.v;
+    F.sw = F.w;
+    F.sp = F.p;
+  }
   for (int s2 = 0; s2 < 2; s2++)
     for (int q = 0; q < 3; q++)
-      if (dalloc(h, &h->hhbuf[s2][q], fs)) return bad(h->err);
+      if (dalloc(h, &h->hhbuf[s2][q], fsi)) return bad(h->err);
   for (int q = 0; q < 3; q++) {
     F.hh[q] = h->hhbuf[0][q];
     F.hhp[q] = h->hhbuf[1][q];
@@ -631,11 +705,29 @@ extern "C" int c3d_upload_state(c3d_handle* h, const double* U, const double* P,
   if (to_dev(h, F.u, U, LAY_U) || to_dev(h, F.v, U + nu, LAY_U) || to_dev(h, F.w, U + 2 * nu, LAY_U))
     return -1;
   if (to_dev(h, F.p, P, LAY_P)) return -1;
-  for (int q = 0; q < 3; q++)
-    if (to_dev(h, F.hh[q], HH ? HH + q * nu : nullptr, LAY_U)) return -1;
+  const long long fsz = h->c.g.fsz;
+  for (int q = 0; q < 3; q++) {
+    if (h->c.s.mixed) {
+      if (to_dev(h, h->dtmp, HH ? HH + q * nu : nullptr, LAY_U)) return -1;
+      c3dk::to_f32(h->c, h->dtmp, (float*)F.hh[q], fsz);
+    } else if (to_dev(h, F.hh[q], HH ? HH + q * nu : nullptr, LAY_U)) {
+      return -1;
+    }
+  }
   if (to_dev(h, F.sgs, SGSVIS, LAY_U)) return -1;
   double sc[2] = {presg, presg3};
   HIPCHK(hipMemcpy(F.sc + SC_PRESG, sc, 2 * sizeof(double), hipMemcpyHostToDevice));
+  if (h->c.s.mixed) {
+    // FP32 shadows of the state, and DIV(U) of the uploaded field (the
+    // next RDP starts from it)
+    c3dk::to_f32(h->c, F.u, F.uf, fsz);
+    c3dk::to_f32(h->c, F.v, F.vf, fsz);
+    c3dk::to_f32(h->c, F.w, F.wf, fsz);
+    c3dk::to_f32(h->c, F.p, F.pf, fsz);
+    c3dk::divcheck(h->c);
+    HIPCHK(hipStreamSynchronize(h->c.st));
+    HIPCHK(hipGetLastError());
+  }
   return 0;
 }
 
@@ -649,8 +741,14 @@ extern "C" int c3d_download_state(c3d_handle* h, double* U, double* P, double* H
     return -1;
   if (P && from_dev(h, P, F.p, LAY_P)) return -1;
   if (HH)
-    for (int q = 0; q < 3; q++)
-      if (from_dev(h, HH + q * nu, F.hh[q], LAY_U)) return -1;
+    for (int q = 0; q < 3; q++) {
+      const double* src = F.hh[q];
+      if (h->c.s.mixed) {
+        c3dk::to_f64(h->c, (const float*)F.hh[q], h->dtmp, h->c.g.fsz);
+        src = h->dtmp;
+      }
+      if (from_dev(h, HH + q * nu, src, LAY_U)) return -1;
+    }
   if (SGSVIS && from_dev(h, SGSVIS, F.sgs, LAY_U)) return -1;
   double sc[2];
   HIPCHK(hipMemcpy(sc, F.sc + SC_PRESG, 2 * sizeof(double), hipMemcpyDeviceToHost));
@@ -702,10 +800,54 @@ extern "C" int c3d_dpcalc(c3d_handle* h) {
   return launched(h);
 }
 
+// mixed mode: the update writes V into the other buffer
+static void swap_v(c3d_handle* h) {
+  h->c.d.vold = h->c.d.v;
+  h->vcur ^= 1;
+  h->c.d.v = h->vbuf[h->vcur];
+}
+
+// DIVMAX above which mixed mode refines the projection (the FP64 path
+// reaches ~1e-13 on the same fields)
+static const double kRefineTol = 1e-10;
+
+static int read_sc(c3d_handle* h, int slot, double* v);
+
+// UPCALC (in mixed mode V goes to the other buffer)
+static void upcalc_impl(c3d_handle* h, bool chk) {
+  if (h->c.s.mixed) swap_v(h);
+  c3dk::upcalc(h->c, chk);
+}
+
+// Mixed mode, after UPCALC: the DIVMAX of the projection (FP64, on the new
+// field) is read back; above kRefineTol one FP64-residual refinement
+// follows: DIV is the residual of the Poisson solve, DP2 solves
+// L DP2 = DIV/DT the same way and the field takes U -= DT grad(DP2),
+// P += DP2 (flow rate unchanged: the periodic gradients have zero line
+// means).  *divmax returns the final DIVMAX.
+static int refine_impl(c3d_handle* h, double* divmax) {
+  double dm = 0.0;
+  if (read_sc(h, SC_DIVMAX, &dm)) return -1;
+  h->first_divmax = dm;
+  if (!(dm <= kRefineTol)) {
+    h->refines++;
+    c3dk::div_rdp(h->c);
+    c3dk::getdp(h->c);
+    swap_v(h);
+    c3dk::upd_x(h->c, true, true);
+    c3dk::ghosts(h->c);
+    c3dk::upd_z(h->c, true, true);
+    if (read_sc(h, SC_DIVMAX, &dm)) return -1;
+  }
+  if (divmax) *divmax = dm;
+  return 0;
+}
+
 extern "C" int c3d_upcalc(c3d_handle* h) {
   if (!h) return -1;
-  c3dk::upcalc(h->c, false);
-  return launched(h);
+  upcalc_impl(h, false);
+  if (launched(h)) return -1;
+  return h->c.s.mixed ? refine_impl(h, nullptr) : 0;
 }
 
 extern "C" int c3d_sgs(c3d_handle* h) {
@@ -749,8 +891,9 @@ static int step_impl(c3d_handle* h, bool chk) {
   c3dk::getuh_fused(h->c);
   c3dk::rhsdp(h->c);
   c3dk::getdp(h->c);
-  c3dk::upcalc(h->c, chk);
-  return launched(h);
+  upcalc_impl(h, chk);
+  if (launched(h)) return -1;
+  return h->c.s.mixed ? refine_impl(h, nullptr) : 0;
 }
 
 extern "C" int c3d_getup(c3d_handle* h, int ntime) {
@@ -826,8 +969,8 @@ extern "C" int c3d_get_field(c3d_handle* h, int which, double* out) {
     case C3D_F_UH1: src = F.uh[0]; break;
     case C3D_F_UH2: src = F.uh[1]; break;
     case C3D_F_UH3: src = F.uh[2]; break;
-    case C3D_F_RDP: src = F.rdp; lay = LAY_P; break;
-    case C3D_F_DP: src = F.dp; lay = LAY_P; break;
+    case C3D_F_RDP: src = F.rdp; lay = LAY_P; break;  // FP64 in both modes
+    case C3D_F_DP: src = F.dp; lay = LAY_P; break;  // FP64 in both modes
     case C3D_F_HH1: src = F.hh[0]; break;
     case C3D_F_HH2: src = F.hh[1]; break;
     case C3D_F_HH3: src = F.hh[2]; break;
@@ -844,11 +987,26 @@ extern "C" int c3d_get_field(c3d_handle* h, int which, double* out) {
     }
     default: return fail(h, "unknown field");
   }
+  if (h->c.s.mixed && which != C3D_F_DP && which != C3D_F_RDP) {
+    // FP32 intermediates; UH is U + DELTA
+    const long long n = h->c.g.fsz;
+    if (which == C3D_F_UH1 || which == C3D_F_UH2 || which == C3D_F_UH3) {
+      const double* U = which == C3D_F_UH1 ? F.u : which == C3D_F_UH2 ? F.v : F.w;
+      c3dk::add_f32(h->c, U, (const float*)src, h->dtmp, n);
+    } else {
+      c3dk::to_f64(h->c, (const float*)src, h->dtmp, n);
+    }
+    src = h->dtmp;
+  }
   return from_dev(h, out, src, lay);
 }
 
 extern "C" int c3d_get_scalar(c3d_handle* h, int which, double* out) {
   if (!h || !out) return -1;
+  if (which == 9 || which == 10) {  // mixed mode diagnostics
+    *out = which == 9 ? (double)h->refines : h->first_divmax;
+    return 0;
+  }
   const int map[9] = {SC_PRESG, SC_PRESG3, SC_DMPRESG, SC_DMPRESG3, SC_FLOW1, SC_FLOW3, SC_WSM1, SC_WSM2, SC_WSM3};
   if (which < 0 || which > 8) return fail(h, "unknown scalar");
   return read_sc(h, map[which], out);

@@ -3,6 +3,9 @@
 #include <hip/hip_runtime.h>
 #include "c3d_dev.h"
 
+#define YTAB_W 16
+enum { YT_BET = 11, YT_M = 12, YT_G = 13 };
+
 // device scalar slots (double unless noted)
 enum {
   SC_PRESG = 0, SC_PRESG3, SC_DMPRESG, SC_DMPRESG3, SC_FLOW1, SC_FLOW3,
@@ -21,6 +24,12 @@ struct Metrics {
   // reference's expressions evaluate them: FJPA(J)*1/H(J+1), FJMU(J)*1/H(J),
   // DY(J)/2, 1/DY(J), 1/H(J)
   const double *cu2, *cu1, *dyh, *rdy, *rh;
+  // per-component, per-J coefficients of the fused RHS-y / ADI-y sweep,
+  // YTAB_W doubles per J (index comp*(N2+2)*YTAB_W + J*YTAB_W + slot):
+  //   u, w: HP HC HM CU2 CU1 DYH(J-1) DYH(J) DYH(J+1) RDY 1-FJPA 1-FJMU
+  //   v:    DYP DYC DYM H RH
+  //   all:  slot 11 BET, 12 -A*BET (0 at the first row), 13 -GAM (0 at N2M)
+  const double* ytab;
   const double *ak1;  // AK1(I), I = 1..N1M  (index I)
   const double *ak3;  // AK3(K), K = 1..N3M/2+1
   // TDMAI chains of the ADI y-sweeps (component c = 0,1,2): A(J), BET(J), GAM(J)
@@ -30,6 +39,7 @@ struct Metrics {
 struct Scal {
   double re, rinv, dt, dx1, dx1q, dx3, dx3q, aly, vol;
   int ifwall;
+  int mixed;  // C3D_MIXED: FP32 intermediates (see Fields)
   int files;  // FILES != 0
   double wmodel;
 };
@@ -60,9 +70,21 @@ struct FFTPlan {
   int tly;  // modes per y-solve tile
 };
 
+// Precision.  The state U, V, W, P, DIV and the whole Poisson solve (RDP,
+// spectrum, DP) are FP64 in both modes.  FP64 mode: every other field is
+// FP64 and the line kernels read the state itself (su..sp = u..p).  Mixed
+// mode: RUH, HH, UH, VV1, WW1, UW, VV3 are FP32; the line kernels read FP32
+// shadows of the state (su..sp = uf..pf, refreshed wherever the state is
+// written); UH holds the ADI increment DELTA = UH - U, RDP is formed in FP64
+// from DIV(U) and DELTA, and the update forms U + DELTA - DT*grad(DP) in
+// FP64.
 struct Fields {
   double *u, *v, *w, *p;
-  // the state as read by the line kernels (the FP64 arrays themselves)
+  float *uf, *vf, *wf, *pf;  // FP32 shadows (mixed mode)
+  // mixed mode double-buffers V: the update reads V(J+1) of the old field
+  // while other blocks write the new one
+  const double* vold;
+  // the state as read by the line kernels
   const void *su, *sv, *sw, *sp;
   double *hh[3], *hhp[3];
   double *ruh[3];
@@ -137,14 +159,16 @@ namespace c3dk {
 void bcond(const Ctx& c);
 void rhs(const Ctx& c);      // RHS1/2/3 complete (x, z and y parts)
 void rhs_xz(const Ctx& c);   // x and z parts only
-void rhs_lines_fp64(const Ctx& c);
+void rhs_lines(const Ctx& c);
 void getuh(const Ctx& c);    // GETUH1/2/3 from the complete RUH
 void getuh_fused(const Ctx& c);  // y parts of RHS1/2/3 fused into GETUH1/2/3
 void rhsdp(const Ctx& c);
 void getdp(const Ctx& c);
 void upcalc(const Ctx& c, bool fuse_checks);
-void upd_x(const Ctx& c, bool chk);
-void upd_z(const Ctx& c, bool chk);
+void upd_x(const Ctx& c, bool chk, bool refine = false);
+void upd_z(const Ctx& c, bool chk, bool refine = false);
+void div_rdp(const Ctx& c);
+void ghosts(const Ctx& c);
 void pois_table(const Ctx& c);
 void divcheck(const Ctx& c);
 void cfl(const Ctx& c);
@@ -157,4 +181,7 @@ void init_attrs();
 void filter(const Ctx& c, int width, const double* in, double* out, int nplanes, int ld1, int ld3);
 void apply_op(const Ctx& c, int op, int dir, const double* in, double* out);
 void zero(const Ctx& c, double* p, long long n);
+void to_f32(const Ctx& c, const double* in, float* out, long long n);
+void to_f64(const Ctx& c, const float* in, double* out, long long n);
+void add_f32(const Ctx& c, const double* a, const float* b, double* out, long long n);
 }  // namespace c3dk

@@ -38,6 +38,14 @@ __global__ void k_bcond(Ctx c) {
   F.u[bn] = ut;
   F.v[bn] = 0.0;
   F.w[bn] = wt;
+  if (c.s.mixed) {
+    F.uf[b0] = 0.f;
+    F.vf[b0] = 0.f;
+    F.wf[b0] = 0.f;
+    F.uf[bn] = (float)ut;
+    F.vf[bn] = 0.f;
+    F.wf[bn] = (float)wt;
+  }
   if (c.s.files) {
     F.sgs[b0] = 0.0;
     F.sgs[bn] = F.sgs[bm] * (2.0 - aly);
@@ -78,7 +86,7 @@ __global__ __launch_bounds__(C3D_NT) void k_flow_red(Ctx c) {
   const Fields F = c.d;
   const T* uh0 = (const T*)F.uh[0];
   const T* uh2 = (const T*)F.uh[2];
-  const T* dpf = (const T*)F.dp;
+  const double* dpf = F.dp;
   const double cu = 1.0 / s.dx3 / s.dx1;  // (x * DY / DX3) / DX1 as in :1083
   double f1 = 0, d1 = 0, f3 = 0, d3 = 0;
   const int nrows = g.n2m * g.nz;
@@ -93,9 +101,12 @@ __global__ __launch_bounds__(C3D_NT) void k_flow_red(Ctx c) {
       const double dpm = (double)dpf[rb + (i == 0 ? g.nx - 1 : i - 1)];
       const double dpk = (double)dpf[rk + i];
       f1 += F.u[b] * dy / s.dx3 / s.dx1;
-      d1 += (double)uh0[b] * dy / s.dx3 / s.dx1 - s.dt * (dp - dpm) * dy / s.dx3;
+      // mixed mode: UH = U + DELTA
+      const double uh1 = sizeof(T) == 4 ? F.u[b] + (double)uh0[b] : (double)uh0[b];
+      const double uh3 = sizeof(T) == 4 ? F.w[b] + (double)uh2[b] : (double)uh2[b];
+      d1 += uh1 * dy / s.dx3 / s.dx1 - s.dt * (dp - dpm) * dy / s.dx3;
       f3 += F.w[b] * dy / s.dx1 / s.dx3;
-      d3 += (double)uh2[b] * dy / s.dx3 / s.dx1 - s.dt * (dp - dpk) * dy / s.dx1;
+      d3 += uh3 * dy / s.dx3 / s.dx1 - s.dt * (dp - dpk) * dy / s.dx1;
     }
   }
   (void)cu;
@@ -157,6 +168,14 @@ __global__ void k_ghosts(Ctx c) {
   F.v[gidx(g, i, g.n2, k)] = 0.0;
   F.w[gidx(g, i, 0, k)] = 0.0;
   F.w[gidx(g, i, g.n2, k)] = F.ubc_w[t];
+  if (c.s.mixed) {
+    F.uf[gidx(g, i, 0, k)] = 0.f;
+    F.uf[gidx(g, i, g.n2, k)] = (float)F.ubc_u[t];
+    F.vf[gidx(g, i, 1, k)] = 0.f;
+    F.vf[gidx(g, i, g.n2, k)] = 0.f;
+    F.wf[gidx(g, i, 0, k)] = 0.f;
+    F.wf[gidx(g, i, g.n2, k)] = (float)F.ubc_w[t];
+  }
 }
 
 // ------------------------------------------------ DIVCHECK / CFL (alone)
@@ -315,6 +334,20 @@ __global__ __launch_bounds__(C3D_NT) void k_chkmf_fin(Ctx c) {
   }
 }
 
+// precision conversions of whole fields (set-up and host transfers)
+__global__ void k_to_f32(const double* in, float* out, long long n) {
+  long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t < n) out[t] = (float)in[t];
+}
+__global__ void k_to_f64(const float* in, double* out, long long n) {
+  long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t < n) out[t] = (double)in[t];
+}
+__global__ void k_add_f32(const double* a, const float* b, double* out, long long n) {
+  long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t < n) out[t] = a[t] + (double)b[t];
+}
+
 __global__ void k_zero(double* p, long long n) {
   long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (t < n) p[t] = 0.0;
@@ -404,6 +437,16 @@ void init_attrs() {
   (void)hipGetLastError();
 }
 
+void to_f32(const Ctx& c, const double* in, float* out, long long n) {
+  hipLaunchKernelGGL(k_to_f32, dim3(pgrid(n)), dim3(C3D_NT), 0, c.st, in, out, n);
+}
+void to_f64(const Ctx& c, const float* in, double* out, long long n) {
+  hipLaunchKernelGGL(k_to_f64, dim3(pgrid(n)), dim3(C3D_NT), 0, c.st, in, out, n);
+}
+void add_f32(const Ctx& c, const double* a, const float* b, double* out, long long n) {
+  hipLaunchKernelGGL(k_add_f32, dim3(pgrid(n)), dim3(C3D_NT), 0, c.st, a, b, out, n);
+}
+
 void zero(const Ctx& c, double* p, long long n) {
   { KScope ks_(c, KID_ZERO); hipLaunchKernelGGL(k_zero, dim3(pgrid(n)), dim3(C3D_NT), 0, c.st, p, n); }
 }
@@ -413,7 +456,7 @@ void bcond(const Ctx& c) {
 }
 
 void rhs_xz(const Ctx& c) {
-  rhs_lines_fp64(c);
+  rhs_lines(c);
   if (c.s.files) sgs_terms(c);
 }
 
@@ -422,17 +465,33 @@ void rhs(const Ctx& c) {
   {
     const int bx = std::min(256, c.g.nx);
     KScope ks_(c, KID_RHS_Y);
-    hipLaunchKernelGGL(k_rhs_y<double>, dim3((c.g.nx + bx - 1) / bx, c.g.n2m, (c.g.nz + YK - 1) / YK), dim3(bx), 0,
-                       c.st, c);
+    const dim3 grid((c.g.nx + bx - 1) / bx, c.g.n2m, (c.g.nz + YK - 1) / YK);
+    if (c.s.mixed)
+      hipLaunchKernelGGL(k_rhs_y<float>, grid, dim3(bx), 0, c.st, c);
+    else
+      hipLaunchKernelGGL(k_rhs_y<double>, grid, dim3(bx), 0, c.st, c);
   }
 }
 
 void upcalc(const Ctx& c, bool chk) {
-  { KScope ks_(c, KID_FLOW_RED); hipLaunchKernelGGL(k_flow_red<double>, dim3(RED_BLOCKS), dim3(C3D_NT), 0, c.st, c); }
+  // mixed mode always forms DIV of the new field: the next RDP starts from it
+  chk = chk || c.s.mixed;
+  {
+    KScope ks_(c, KID_FLOW_RED);
+    if (c.s.mixed)
+      hipLaunchKernelGGL(k_flow_red<float>, dim3(RED_BLOCKS), dim3(C3D_NT), 0, c.st, c);
+    else
+      hipLaunchKernelGGL(k_flow_red<double>, dim3(RED_BLOCKS), dim3(C3D_NT), 0, c.st, c);
+  }
   { KScope ks_(c, KID_FLOW_FIN); hipLaunchKernelGGL(k_flow_fin, dim3(1), dim3(C3D_NT), 0, c.st, c, 1); }
   upd_x(c, chk);
-  { KScope ks_(c, KID_GHOSTS); hipLaunchKernelGGL(k_ghosts, dim3(pgrid((long long)c.g.nx * c.g.nz)), dim3(C3D_NT), 0, c.st, c); }
+  ghosts(c);
   upd_z(c, chk);
+}
+
+void ghosts(const Ctx& c) {
+  KScope ks_(c, KID_GHOSTS);
+  hipLaunchKernelGGL(k_ghosts, dim3(pgrid((long long)c.g.nx * c.g.nz)), dim3(C3D_NT), 0, c.st, c);
 }
 
 void divcheck(const Ctx& c) {

@@ -8,7 +8,8 @@
 //   k_rhsdp_x, k_rhsdp_z         RHSDP                               :8117-8214
 //   k_fft_x, k_fft_z, k_pois_y,  GETDP: real-to-complex x-DFT (two  :9271-9441
 //     k_fft_z, k_fft_xinv          rows per complex transform), z-DFT, one
-//                                  partitioned Thomas solve per mode
+//                                  partitioned Thomas solve per mode (FP64),
+//                                  inverse z- and x-DFTs (FP64)
 //   k_flow_red/fin, k_upd_x,     UPCALC (+ DIVCHECK and CFL fused)   :1052-1238
 //     k_ghosts, k_upd_z
 //
@@ -480,9 +481,9 @@ __global__ __launch_bounds__(256) void k_adi_y(Ctx c) {
 // expressions are those of yterm_u/v/w); with them it calls yterm_*.
 // Returns TDMAI's right-hand side BET(J) * RUH(J) * DT.
 template <typename T>
-__device__ __forceinline__ void yfuse_u(const Ctx& c, long long b, long long t2, const YPos& y, T (&v)[CHMAX]) {
+__device__ __forceinline__ void yfuse_u(const Ctx& c, const double* tb, long long b, long long t2, const YPos& y,
+                                        T (&v)[CHMAX]) {
   const Geo& g = c.g;
-  const Metrics& M = c.m;
   const Fields& F = c.d;
   const T* su = (const T*)F.su;
   const T* vv1 = (const T*)F.vv1;
@@ -492,7 +493,7 @@ __device__ __forceinline__ void yfuse_u(const Ctx& c, long long b, long long t2,
   const T rinv = (T)c.s.rinv, dt = (T)c.s.dt;
   const long long nx = g.nx;
   const T uN2 = su[b + g.n2 * nx], uW = su[b];
-  const T bcw = -(T)F.tauw[t2] / (T)M.dy[1] * (T)c.s.ifwall;
+  const T bcw = -(T)F.tauw[t2] / (T)c.m.dy[1] * (T)c.s.ifwall;
   int J = y.j0;
   T um = su[b + (J - 1) * nx], u0 = su[b + J * nx];
   T w1 = vv1[b + J * nx];
@@ -500,19 +501,19 @@ __device__ __forceinline__ void yfuse_u(const Ctx& c, long long b, long long t2,
   for (int q = 0; q < CHMAX; q++)
     if (q < y.len) {
       J = y.j0 + q;
+      const double* t = tb + J * YTAB_W;
       const long long a = b + J * nx;
       const T up = su[a + nx];
       const T w2 = vv1[a + nx];
-      const T ru = ruh[a] + rinv * ((T)M.hp[J] * up - (T)M.hc[J] * u0 + (T)M.hm[J] * um);
-      const double fjup = M.fjpa[J], fjum = M.fjmu[J];
-      const T u2 = (T)M.cu2[J] * ((T)M.dyh[J] * up + (T)M.dyh[J + 1] * u0) + (T)(1. - fjup) * uN2;
-      const T u1 = (T)M.cu1[J] * ((T)M.dyh[J - 1] * u0 + (T)M.dyh[J] * um) + (T)(1. - fjum) * uW;
+      const T ru = ruh[a] + rinv * ((T)t[0] * up - (T)t[1] * u0 + (T)t[2] * um);
+      const T u2 = (T)t[3] * ((T)t[6] * up + (T)t[7] * u0) + (T)t[9] * uN2;
+      const T u1 = (T)t[4] * ((T)t[5] * u0 + (T)t[6] * um) + (T)t[10] * uW;
       const T v2 = (J == g.n2m) ? (T)0 : w2;
       const T v1 = (J == 1) ? (T)0 : w1;
-      const T h = hh[a] + (T)M.rdy[J] * (v2 * u2 - v1 * u1);
+      const T h = hh[a] + (T)t[8] * (v2 * u2 - v1 * u1);
       hh[a] = h;
-      const T r = ru - (T)(3. / 2.) * h + (T)(1. / 2.) * hp[a] + (T)(1. - fjum) * bcw;
-      v[q] = (T)M.ybet[0][J] * (r * dt);
+      const T r = ru - (T)(3. / 2.) * h + (T)(1. / 2.) * hp[a] + (T)t[10] * bcw;
+      v[q] = (T)t[YT_BET] * (r * dt);
       um = u0;
       u0 = up;
       w1 = w2;
@@ -520,8 +521,8 @@ __device__ __forceinline__ void yfuse_u(const Ctx& c, long long b, long long t2,
 }
 
 template <typename T>
-__device__ __forceinline__ void yfuse_v(const Ctx& c, long long b, const YPos& y, T (&v)[CHMAX]) {
-  const Metrics& M = c.m;
+__device__ __forceinline__ void yfuse_v(const Ctx& c, const double* tb, long long b, const YPos& y,
+                                        T (&v)[CHMAX]) {
   const Fields& F = c.d;
   const T* sv = (const T*)F.sv;
   const T* sp = (const T*)F.sp;
@@ -537,18 +538,19 @@ __device__ __forceinline__ void yfuse_v(const Ctx& c, long long b, const YPos& y
   for (int q = 0; q < CHMAX; q++)
     if (q < y.len) {
       J = y.j0 + q;
+      const double* t = tb + J * YTAB_W;
       const long long a = b + J * nx;
       const T vp = sv[a + nx];
       const T p0 = sp[a];
-      const T viscos = rinv * ((T)M.dyp[J] * vp - (T)M.dyc[J] * v0 + (T)M.dym[J] * vm);
-      const T pressg2 = (p0 - pm) / (T)M.h[J];
+      const T viscos = rinv * ((T)t[0] * vp - (T)t[1] * v0 + (T)t[2] * vm);
+      const T pressg2 = (p0 - pm) / (T)t[3];
       const T ru = ruh[a] + (-pressg2 + viscos);
       const T v2 = (T)0.5 * (vp + v0);
       const T v1 = (T)0.5 * (v0 + vm);
-      const T h = hh[a] + (T)M.rh[J] * (v2 * v2 - v1 * v1);
+      const T h = hh[a] + (T)t[4] * (v2 * v2 - v1 * v1);
       hh[a] = h;
       const T r = ru - (T)(3. / 2.) * h + (T)(1. / 2.) * hp[a];
-      v[q] = (T)M.ybet[1][J] * (r * dt);
+      v[q] = (T)t[YT_BET] * (r * dt);
       vm = v0;
       v0 = vp;
       pm = p0;
@@ -556,9 +558,9 @@ __device__ __forceinline__ void yfuse_v(const Ctx& c, long long b, const YPos& y
 }
 
 template <typename T>
-__device__ __forceinline__ void yfuse_w(const Ctx& c, long long b, long long t2, const YPos& y, T (&v)[CHMAX]) {
+__device__ __forceinline__ void yfuse_w(const Ctx& c, const double* tb, long long b, long long t2, const YPos& y,
+                                        T (&v)[CHMAX]) {
   const Geo& g = c.g;
-  const Metrics& M = c.m;
   const Fields& F = c.d;
   const T* sw = (const T*)F.sw;
   const T* vv3 = (const T*)F.vv3;
@@ -568,7 +570,7 @@ __device__ __forceinline__ void yfuse_w(const Ctx& c, long long b, long long t2,
   const T rinv = (T)c.s.rinv, dt = (T)c.s.dt;
   const long long nx = g.nx;
   const T wN2 = sw[b + g.n2 * nx], wW = sw[b];
-  const T bcw = -(T)F.taub[t2] / (T)M.dy[1] * (T)c.s.ifwall;
+  const T bcw = -(T)F.taub[t2] / (T)c.m.dy[1] * (T)c.s.ifwall;
   int J = y.j0;
   T wm = sw[b + (J - 1) * nx], w0 = sw[b + J * nx];
   T s1 = vv3[b + J * nx];
@@ -576,19 +578,19 @@ __device__ __forceinline__ void yfuse_w(const Ctx& c, long long b, long long t2,
   for (int q = 0; q < CHMAX; q++)
     if (q < y.len) {
       J = y.j0 + q;
+      const double* t = tb + J * YTAB_W;
       const long long a = b + J * nx;
       const T wp = sw[a + nx];
       const T s2 = vv3[a + nx];
-      const T ru = ruh[a] + rinv * ((T)M.hp[J] * wp - (T)M.hc[J] * w0 + (T)M.hm[J] * wm);
-      const double fjup = M.fjpa[J], fjum = M.fjmu[J];
-      const T w2 = (T)M.cu2[J] * ((T)M.dyh[J + 1] * w0 + (T)M.dyh[J] * wp) + (T)(1. - fjup) * wN2;
-      const T w1 = (T)M.cu1[J] * ((T)M.dyh[J] * wm + (T)M.dyh[J - 1] * w0) + (T)(1. - fjum) * wW;
+      const T ru = ruh[a] + rinv * ((T)t[0] * wp - (T)t[1] * w0 + (T)t[2] * wm);
+      const T w2 = (T)t[3] * ((T)t[7] * w0 + (T)t[6] * wp) + (T)t[9] * wN2;
+      const T w1 = (T)t[4] * ((T)t[6] * wm + (T)t[5] * w0) + (T)t[10] * wW;
       const T v2 = (J == g.n2m) ? (T)0 : s2;
       const T v1 = (J == 1) ? (T)0 : s1;
-      const T h = hh[a] + (T)M.rdy[J] * (v2 * w2 - v1 * w1);
+      const T h = hh[a] + (T)t[8] * (v2 * w2 - v1 * w1);
       hh[a] = h;
-      const T r = ru - (T)(3. / 2.) * h + (T)(1. / 2.) * hp[a] + (T)(1. - fjum) * bcw;
-      v[q] = (T)M.ybet[2][J] * (r * dt);
+      const T r = ru - (T)(3. / 2.) * h + (T)(1. / 2.) * hp[a] + (T)t[10] * bcw;
+      v[q] = (T)t[YT_BET] * (r * dt);
       wm = w0;
       w0 = wp;
       s1 = s2;
@@ -596,8 +598,9 @@ __device__ __forceinline__ void yfuse_w(const Ctx& c, long long b, long long t2,
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_rhsy_adiy(Ctx c) {
+__global__ __launch_bounds__(256) void k_rhsy_adiy(Ctx c, int staged) {
   __shared__ T xs[4][256];
+  extern __shared__ __align__(16) unsigned char smem[];
   const Geo g = c.g;
   const YShape ys = c.o.ys;
   const int comp = blockIdx.y;
@@ -619,12 +622,18 @@ __global__ __launch_bounds__(256) void k_rhsy_adiy(Ctx c) {
   const Fields F = c.d;
   const Metrics M = c.m;
   T* X = (T*)(comp == 0 ? F.uh[0] : comp == 1 ? F.uh[1] : F.uh[2]);
-  const double* A = comp == 0 ? M.ya[0] : comp == 1 ? M.ya[1] : M.ya[2];
-  const double* BET = comp == 0 ? M.ybet[0] : comp == 1 ? M.ybet[1] : M.ybet[2];
-  const double* GAM = comp == 0 ? M.ygam[0] : comp == 1 ? M.ygam[1] : M.ygam[2];
   const T dt = (T)c.s.dt;
   const long long b = gidx(g, ok ? i : 0, 0, k);
   const long long t2 = (long long)k * g.nx + (ok ? i : 0);
+  // this component's coefficient rows, in LDS when they fit
+  const int ntab = (g.n2 + 2) * YTAB_W;
+  const double* tb = M.ytab + (long long)comp * ntab;
+  if (staged) {
+    double* st = (double*)smem;
+    for (int e = threadIdx.x; e < ntab; e += blockDim.x) st[e] = tb[e];
+    __syncthreads();
+    tb = st;
+  }
   T v[CHMAX], mm[CHMAX];
 #pragma unroll
   for (int q = 0; q < CHMAX; q++) v[q] = (T)0;
@@ -637,29 +646,23 @@ __global__ __launch_bounds__(256) void k_rhsy_adiy(Ctx c) {
           const YIdx x = yidx(g, i, J, k);
           const T r = comp == 0 ? yterm_u<T>(c, x, i, J, k)
                                 : comp == 1 ? yterm_v<T>(c, x, i, J, k) : yterm_w<T>(c, x, i, J, k);
-          v[q] = (T)BET[J] * (r * dt);
+          v[q] = (T)tb[J * YTAB_W + YT_BET] * (r * dt);
         }
     } else if (comp == 0) {
-      yfuse_u<T>(c, b, t2, y, v);
+      yfuse_u<T>(c, tb, b, t2, y, v);
     } else if (comp == 1) {
-      yfuse_v<T>(c, b, y, v);
+      yfuse_v<T>(c, tb, b, y, v);
     } else {
-      yfuse_w<T>(c, b, t2, y, v);
+      yfuse_w<T>(c, tb, b, t2, y, v);
     }
   }
 #pragma unroll
   for (int q = 0; q < CHMAX; q++)
-    if (q < y.len) {
-      const int J = y.j0 + q;
-      mm[q] = (J == ns) ? (T)0 : -(T)A[J] * (T)BET[J];
-    }
+    if (q < y.len) mm[q] = (T)tb[(y.j0 + q) * YTAB_W + YT_M];
   ychain_fwd(v, mm, y, C, L, xs[0], xs[1]);
 #pragma unroll
   for (int q = 0; q < CHMAX; q++)
-    if (q < y.len) {
-      const int J = y.j0 + q;
-      mm[q] = (J == g.n2m) ? (T)0 : -(T)GAM[J];
-    }
+    if (q < y.len) mm[q] = (T)tb[(y.j0 + q) * YTAB_W + YT_G];
   ychain_bwd(v, mm, y, C, L, xs[2], xs[3]);
   if (ok)
 #pragma unroll
@@ -687,12 +690,14 @@ __global__ __launch_bounds__(256) void k_adi_x(Ctx c, CompList cl) {
   T* UH = (T*)(comp == 0 ? F.uh[0] : comp == 1 ? F.uh[1] : F.uh[2]);
   const double* U = comp == 0 ? F.u : comp == 1 ? F.v : F.w;
   T d[SEG], pf[SEG];
+  // mixed mode keeps the increment DELTA in UH (no U added)
+  const bool addu = fin && sizeof(T) == 8;
   X.pre((const T*)UH, 0, pf);
   const T* B = X.put(pf);
-  if (fin) X.pre(U, 0, pf);
+  if (addu) X.pre(U, 0, pf);
   SQ d[q] = XA(B, q - 1) + (T)10. * XA(B, q) + XA(B, q + 1);
   psolve<T, SEG>(d, G, L, l, gs, c.o.px.adi, bf.xf, bf.xb);
-  if (fin) {
+  if (addu) {
     B = X.put(pf);
     SQ d[q] = XA(B, q) + d[q];
   }
@@ -714,7 +719,7 @@ __global__ __launch_bounds__(512) void k_adi_z(Ctx c, CompList cl) {
   SQ d[q] = prv(a, q, lf) + (T)10. * a[q] + nxt(a, q, rt);
   psolve<T, SEG>(d, G, L, l, gs, c.o.pz.adi, bf.xf, bf.xb);
   if (ok) {
-    if (fin) {
+    if (fin && sizeof(T) == 8) {  // mixed mode keeps the increment DELTA
       SQ UH[ZI(q)] = (T)U[ZI(q)] + d[q];
     } else {
       SQ UH[ZI(q)] = d[q];
@@ -724,38 +729,47 @@ __global__ __launch_bounds__(512) void k_adi_z(Ctx c, CompList cl) {
 
 // ------------------------------------------------------------------ RHSDP
 // x-lines: RDP = (FJUP*UH2(J+1) - FJUM*UH2(J))/DY/DT + D1x(UH1)/DT  (:8138-8186)
-// (the CBC term is zero: UBC(.,.,2,.) = 0, :978, :986)
-template <typename T, int SEG>
+// (the CBC term is zero: UBC(.,.,2,.) = 0, :978, :986).  Arithmetic and RDP
+// in FP64; UH stored as TS.  Mixed mode (TS = float): UH holds DELTA and
+// RDP = DIV(U)/DT + DIV(DELTA)/DT, with DIV(U) of the current state kept in
+// FP64 by the previous update.
+template <typename T, int SEG, typename TS>
 __global__ __launch_bounds__(256) void k_rhsdp_x(Ctx c) {
   XSETUP;
+  constexpr bool MIX = sizeof(TS) == 4;
   const Fields F = c.d;
   const Metrics M = c.m;
   const T dx1 = (T)c.s.dx1, dt = (T)c.s.dt;
   T d[SEG], e[SEG], pf[SEG];
-  X.pre((const T*)F.uh[0], 0, pf);
+  X.pre((const TS*)F.uh[0], 0, pf);
   const T* B = X.put(pf);
-  X.pre((const T*)F.uh[1], 1, pf);
+  X.pre((const TS*)F.uh[1], 1, pf);
   SQ d[q] = (T)24. * dx1 * (XA(B, q + 1) - XA(B, q));
   psolve<T, SEG>(d, G, L, l, gs, c.o.px.s22, bf.xf, bf.xb);
   const T fjpa = (T)M.fjpa[j], fjmu = (T)M.fjmu[j], dy = (T)M.dy[j];
   B = X.put(pf);
-  X.pre((const T*)F.uh[1], 0, pf);
+  X.pre((const TS*)F.uh[1], 0, pf);
   SQ e[q] = fjpa * XA(B, q);
   B = X.put(pf);
+  if (MIX) X.pre(F.div, 0, pf);
   SQ d[q] = (e[q] - fjmu * XA(B, q)) / dy / dt + d[q] / dt;
-  X.out((T*)F.rdp, d);
+  if (MIX) {
+    B = X.put(pf);
+    SQ d[q] = d[q] + XA(B, q) / dt;
+  }
+  X.out(F.rdp, d);
 }
 
 // z-lines: RDP += D1z(UH3)/DT (:8196-8211)
-template <typename T, int SEG>
+template <typename T, int SEG, typename TS>
 __global__ __launch_bounds__(512) void k_rhsdp_z(Ctx c) {
   ZSETUP;
   const T dx3 = (T)c.s.dx3, dt = (T)c.s.dt;
   T a[SEG], d[SEG], lf, rt;
-  zload((const T*)c.d.uh[2], b, pl, p0, bm, bp, a, lf, rt);
+  zload((const TS*)c.d.uh[2], b, pl, p0, bm, bp, a, lf, rt);
   SQ d[q] = (T)24. * dx3 * (nxt(a, q, rt) - a[q]);
   psolve<T, SEG>(d, G, L, l, gs, c.o.pz.s22, bf.xf, bf.xb);
-  T* rdp = (T*)c.d.rdp;
+  double* rdp = c.d.rdp;
   if (ok) SQ rdp[ZI(q)] = rdp[ZI(q)] + d[q] / dt;
 }
 
@@ -865,6 +879,7 @@ __device__ __forceinline__ cplx_t<T>* fft_lines(cplx_t<T>* a, cplx_t<T>* b, int 
 // forward x-DFT of RDP rows (J=1..N2M, K), 2*TL rows per block
 template <typename T>
 __global__ __launch_bounds__(256) void k_fft_x(Ctx c) {
+  static_assert(sizeof(T) == 8, "the twiddle tables are FP64");
   extern __shared__ __align__(16) unsigned char smem[];
   typedef cplx_t<T> zc;
   const Geo g = c.g;
@@ -882,7 +897,8 @@ __global__ __launch_bounds__(256) void k_fft_x(Ctx c) {
       A[p * LS + l] = {ra >= 0 ? src[ra + p] : (T)0, rb >= 0 ? src[rb + p] : (T)0};
   }
   __syncthreads();
-  const zc* R = fft_lines<T>(A, B, n, NL, LS, c.f.codex, c.f.nradx, (const zc*)c.f.twxf, -1);
+  const zc* R = fft_lines<T>(A, B, n, NL, LS, c.f.codex, c.f.nradx,
+                             (const zc*)c.f.twxf, -1);
   const int nxh = g.nxh;
   for (int l = 0; l < NL; l++) {
     const int la = ln0 + l, lb = ln0 + NL + l;
@@ -906,7 +922,8 @@ __global__ __launch_bounds__(256) void k_fft_x(Ctx c) {
 
 // z-DFT of the half spectrum, columns (jm, kx); sign -1 forward, +1 backward
 template <typename T>
-__global__ __launch_bounds__(256) void k_fft_z(Ctx c, int sign) {
+__global__ __launch_bounds__(256) void k_fft_z(Ctx c, int sign, void* spec) {
+  static_assert(sizeof(T) == 8, "the twiddle tables are FP64");
   extern __shared__ __align__(16) unsigned char smem[];
   typedef cplx_t<T> zc;
   const Geo g = c.g;
@@ -916,7 +933,7 @@ __global__ __launch_bounds__(256) void k_fft_z(Ctx c, int sign) {
   const int nit = (g.nxh + L - 1) / L;
   const int it = blockIdx.x % nit;
   const int jm = blockIdx.x / nit;
-  zc* S = (zc*)c.d.spec + (long long)jm * g.nz * g.nxh;
+  zc* S = (zc*)spec + (long long)jm * g.nz * g.nxh;
   const int tot = n * L;
   for (int e = threadIdx.x; e < tot; e += blockDim.x) {
     const int l = e % L, p = e / L;
@@ -924,8 +941,8 @@ __global__ __launch_bounds__(256) void k_fft_z(Ctx c, int sign) {
     A[p * LS + l] = kx < g.nxh ? S[(long long)p * g.nxh + kx] : zc{(T)0, (T)0};
   }
   __syncthreads();
-  const zc* R = fft_lines<T>(A, B, n, L, LS, c.f.codez, c.f.nradz,
-                             (const zc*)(sign < 0 ? c.f.twzf : c.f.twzb), sign);
+  const zc* tw = (const zc*)(sign < 0 ? c.f.twzf : c.f.twzb);
+  const zc* R = fft_lines<T>(A, B, n, L, LS, c.f.codez, c.f.nradz, tw, sign);
   for (int e = threadIdx.x; e < tot; e += blockDim.x) {
     const int l = e % L, p = e / L;
     const int kx = it * L + l;
@@ -940,10 +957,12 @@ __global__ __launch_bounds__(256) void k_fft_z(Ctx c, int sign) {
 // depend on the mode only: IBET(J) = 1/BET(J) and GAM(J+1) are tabulated at
 // set-up (k_pois_table); here each thread (l, c) solves chunk c of mode
 // kx = kx0 + l by the partitioned recurrences.
-template <typename T>
-__global__ __launch_bounds__(256) void k_pois_y(Ctx c) {
-  __shared__ T xs[8][256];
-  typedef cplx_t<T> zc;
+// Spectrum in TI, recurrences and result in TO (FP64 in both modes).
+template <typename TI, typename TO>
+__global__ __launch_bounds__(256) void k_pois_y(Ctx c, const void* spec_in, void* spec_out) {
+  __shared__ TO xs[8][256];
+  typedef cplx_t<TI> zi;
+  typedef cplx_t<TO> zo;
   const Geo g = c.g;
   const YShape ys = c.o.yp;
   const int L = ys.L, C = ys.C;
@@ -961,23 +980,24 @@ __global__ __launch_bounds__(256) void k_pois_y(Ctx c) {
   const bool ok = kx < g.nxh;
   const long long jstr = (long long)g.nz * g.nxh;
   const long long b = (long long)kz * g.nxh + (ok ? kx : 0);
-  zc* S = (zc*)c.d.spec;
+  const zi* Si = (const zi*)spec_in;
+  zo* So = (zo*)spec_out;
   const double* IB = c.d.pib;
   const double* GT = c.d.pgt;
   const double* pmj = c.m.pmj;
   const bool selfconj = (kx == 0 || 2 * kx == g.nx) && (kz == 0 || 2 * kz == g.nz);
   const bool pin = (kx == 0 && kz == 0);
-  T vr[CHMAX], vi[CHMAX], mm[CHMAX];
+  TO vr[CHMAX], vi[CHMAX], mm[CHMAX];
 #pragma unroll
   for (int q = 0; q < CHMAX; q++)
     if (q < y.len) {
       const int jm = y.j0 + q;
       const long long a = jm * jstr + b;
-      const zc r = S[a];
-      const T ib = (T)IB[a];
-      vr[q] = (pin && jm == 0) ? (T)0 : r.x * ib;
-      vi[q] = selfconj ? (T)0 : r.y * ib;
-      mm[q] = (jm == 0) ? (T)0 : -(T)pmj[jm + 1] * ib;
+      const zi r = Si[a];
+      const TO ib = (TO)IB[a];
+      vr[q] = (pin && jm == 0) ? (TO)0 : (TO)r.x * ib;
+      vi[q] = selfconj ? (TO)0 : (TO)r.y * ib;
+      mm[q] = (jm == 0) ? (TO)0 : -(TO)pmj[jm + 1] * ib;
     }
   ychain_fwd(vr, mm, y, C, L, xs[0], xs[1]);
   ychain_fwd(vi, mm, y, C, L, xs[2], xs[3]);
@@ -985,14 +1005,14 @@ __global__ __launch_bounds__(256) void k_pois_y(Ctx c) {
   for (int q = 0; q < CHMAX; q++)
     if (q < y.len) {
       const int jm = y.j0 + q;
-      mm[q] = (jm == g.n2m - 1) ? (T)0 : -(T)GT[jm * jstr + b];
+      mm[q] = (jm == g.n2m - 1) ? (TO)0 : -(TO)GT[jm * jstr + b];
     }
   ychain_bwd(vr, mm, y, C, L, xs[4], xs[5]);
   ychain_bwd(vi, mm, y, C, L, xs[6], xs[7]);
   if (ok)
 #pragma unroll
     for (int q = 0; q < CHMAX; q++)
-      if (q < y.len) S[(y.j0 + q) * jstr + b] = zc{vr[q], vi[q]};
+      if (q < y.len) So[(y.j0 + q) * jstr + b] = zo{vr[q], vi[q]};
 }
 
 // set-up: per mode, BET(J) and GAM(J) of TDMAP on (PMJ, PCJ - AK1 - AK3, PPJ)
@@ -1027,7 +1047,8 @@ __global__ void k_pois_table(Ctx c) {
 // Z = A + iB with the imaginary parts of A and B at kx = 0 and NX/2 dropped
 // (they do not reach the real part of the reference's transform)
 template <typename T>
-__global__ __launch_bounds__(256) void k_fft_xinv(Ctx c) {
+__global__ __launch_bounds__(256) void k_fft_xinv(Ctx c, const void* spec) {
+  static_assert(sizeof(T) == 8, "the twiddle tables are FP64");
   extern __shared__ __align__(16) unsigned char smem[];
   typedef cplx_t<T> zc;
   const Geo g = c.g;
@@ -1036,7 +1057,7 @@ __global__ __launch_bounds__(256) void k_fft_xinv(Ctx c) {
   zc* B = A + n * LS;
   const int nlines = g.n2m * g.nz;
   const int ln0 = blockIdx.x * 2 * NL;
-  const zc* S = (const zc*)c.d.spec;
+  const zc* S = (const zc*)spec;
   const int nxh = g.nxh;
   for (int l = 0; l < NL; l++) {
     const int la = ln0 + l, lb = ln0 + NL + l;
@@ -1059,9 +1080,10 @@ __global__ __launch_bounds__(256) void k_fft_xinv(Ctx c) {
     }
   }
   __syncthreads();
-  const zc* R = fft_lines<T>(A, B, n, NL, LS, c.f.codex, c.f.nradx, (const zc*)c.f.twxb, +1);
+  const zc* R = fft_lines<T>(A, B, n, NL, LS, c.f.codex, c.f.nradx,
+                             (const zc*)c.f.twxb, +1);
   const T norm = (T)(1.0 / ((double)g.nx * (double)g.nz));
-  T* dp = (T*)c.d.dp;
+  double* dp = c.d.dp;
   for (int l = 0; l < NL; l++) {
     const int la = ln0 + l, lb = ln0 + NL + l;
     for (int p = threadIdx.x; p < n; p += blockDim.x) {
@@ -1076,68 +1098,100 @@ __global__ __launch_bounds__(256) void k_fft_xinv(Ctx c) {
 // x-lines: U1 = UH1 - DT*D1x(DP) - DT*DMPRESG (:1113-1128); U2 = UH2 -
 // DT*(DP(J)-DP(J-1))/H(J) (:1139-1145); with `chk`, the x part of the
 // divergence of the new field, (V(J+1)-V(J))/DY + D1x(U) (:9066-9102).
-template <typename T, int SEG>
-__global__ __launch_bounds__(256) void k_upd_x(Ctx c, int chk) {
+// Arithmetic in T (FP64); DP and UH stored as TS.  Mixed mode (TS = float):
+// UH holds DELTA, the new state is U + (DELTA - DT*...), and the FP32
+// shadows are written beside the state.
+// `refine` (mixed mode): the correction pass of the FP64-residual
+// refinement, U -= DT*grad(DP) with no DELTA and no DMPRESG.
+template <typename T, int SEG, typename TS>
+__global__ __launch_bounds__(256) void k_upd_x(Ctx c, int chk, int refine) {
   XSETUP;
+  constexpr bool MIX = sizeof(TS) == 4;
   const Scal s = c.s;
   const Metrics M = c.m;
   const Fields F = c.d;
   const T dx1 = (T)s.dx1, dt = (T)s.dt;
-  const T dmp = (T)F.sc[SC_DMPRESG];
+  const T dmp = refine ? (T)0 : (T)F.sc[SC_DMPRESG];
+  const T ku = refine ? (T)0 : (T)1;  // weight of DELTA
   T dpr[SEG], d[SEG], un[SEG], pf[SEG];
-  X.pre((const T*)F.dp, 0, pf);
+  X.pre(F.dp, 0, pf);
   const T* B = X.put(pf);
-  X.pre((const T*)F.uh[0], 0, pf);
+  X.pre((const TS*)F.uh[0], 0, pf);
   SQ {
     dpr[q] = XA(B, q);
     d[q] = (T)24. * dx1 * (-XA(B, q - 1) + XA(B, q));
   }
   psolve<T, SEG>(d, G, L, l, gs, c.o.px.s22, bf.xf, bf.xb);
   B = X.put(pf);
-  X.pre((const T*)F.uh[1], 0, pf);
-  SQ un[q] = XA(B, q) - dt * d[q] - dt * dmp;
+  SQ un[q] = ku * XA(B, q) - dt * d[q] - dt * dmp;
+  if (MIX) {
+    B = X.in(F.u, 0);
+    SQ un[q] = XA(B, q) + un[q];
+  }
+  X.pre((const TS*)F.uh[1], 0, pf);
   X.out(F.u, un);
+  if (MIX) X.out(F.uf, un);
   const T hj = (T)M.h[j];
   const T* B1 = X.put(pf);
-  X.pre((const T*)F.dp, -1, pf);
+  X.pre(F.dp, -1, pf);
   B = X.put(pf);
-  if (chk) X.pre((const T*)F.uh[1], 1, pf);
-  SQ d[q] = (j >= 2) ? XA(B1, q) - dt * (dpr[q] - XA(B, q)) / hj : (T)0;
+  SQ d[q] = (j >= 2) ? ku * XA(B1, q) - dt * (dpr[q] - XA(B, q)) / hj : (T)0;
+  if (MIX) {
+    B = X.in(F.vold, 0);
+    SQ d[q] = (j >= 2) ? XA(B, q) + d[q] : (T)0;
+  }
+  if (chk) X.pre((const TS*)F.uh[1], 1, pf);
   X.out(F.v, d, 2);  // V(I,1,K) is not written by UPCALC
+  if (MIX) X.out(F.vf, d, 2);
   if (!chk) return;
   // new V at J+1 (zero at J+1 = N2: the wall value); d = (V(J+1) - V(J))/DY
   const T hjp = (T)M.h[j + 1], dy = (T)M.dy[j];
   B1 = X.put(pf);
-  X.pre((const T*)F.dp, 1, pf);
+  X.pre(F.dp, 1, pf);
   B = X.put(pf);
-  SQ d[q] = (((j + 1 <= g.n2m) ? XA(B1, q) - dt * (XA(B, q) - dpr[q]) / hjp : (T)0) - d[q]) / dy;
+  T e[SEG];
+  SQ e[q] = ku * XA(B1, q) - dt * (XA(B, q) - dpr[q]) / hjp;
+  if (MIX) {
+    B = X.in(F.vold, 1);
+    SQ e[q] = XA(B, q) + e[q];
+  }
+  SQ d[q] = (((j + 1 <= g.n2m) ? e[q] : (T)0) - d[q]) / dy;
   const T rt = xch_right(un[0], G, L, l, gs, bf.xr);
   SQ dpr[q] = (T)24. * dx1 * (nxt(un, q, rt) - un[q]);
   psolve<T, SEG>(dpr, G, L, l, gs, c.o.px.s22, bf.xf, bf.xb);
   SQ d[q] = d[q] + dpr[q];
-  X.out((T*)F.div, d);
+  X.out(F.div, d);
 }
 
 // z-lines: U3 = UH3 - DT*D1z(DP) - DT*DMPRESG3 (:1193-1208); P += DP
 // (:1218-1230); with `chk`: DIV += D1z(U3), DIVMAX, and the CFL maximum of
-// the new field (:9112-9128, :9150-9162).
-template <typename T, int SEG>
-__global__ __launch_bounds__(512) void k_upd_z(Ctx c, int chk) {
+// the new field (:9112-9128, :9150-9162).  Mixed mode also stores the full
+// DIV (the next RDP starts from it) and the FP32 shadows of W and P.
+template <typename T, int SEG, typename TS>
+__global__ __launch_bounds__(512) void k_upd_z(Ctx c, int chk, int refine) {
   ZSETUP;
+  constexpr bool MIX = sizeof(TS) == 4;
   const Scal s = c.s;
   const Metrics M = c.m;
   const Fields F = c.d;
   const T dx3 = (T)s.dx3, dt = (T)s.dt;
-  const T dmp3 = (T)F.sc[SC_DMPRESG3];
+  const T dmp3 = refine ? (T)0 : (T)F.sc[SC_DMPRESG3];
+  const T ku = refine ? (T)0 : (T)1;
   T dpr[SEG], d[SEG], wn[SEG], lf, rt;
-  zload((const T*)F.dp, b, pl, p0, bm, bp, dpr, lf, rt);
+  zload(F.dp, b, pl, p0, bm, bp, dpr, lf, rt);
   SQ d[q] = (T)24. * dx3 * (-prv(dpr, q, lf) + dpr[q]);
   psolve<T, SEG>(d, G, L, l, gs, c.o.pz.s22, bf.xf, bf.xb);
-  SQ wn[q] = ((const T*)F.uh[2])[ZI(q)] - dt * d[q] - dt * dmp3;
+  SQ wn[q] = ku * (T)((const TS*)F.uh[2])[ZI(q)] - dt * d[q] - dt * dmp3;
+  if (MIX) SQ wn[q] = (T)F.w[ZI(q)] + wn[q];
   if (ok) {
     SQ {
       F.w[ZI(q)] = wn[q];
-      F.p[ZI(q)] = F.p[ZI(q)] + dpr[q];
+      const double pn = F.p[ZI(q)] + dpr[q];
+      F.p[ZI(q)] = pn;
+      if (MIX) {
+        F.wf[ZI(q)] = (float)wn[q];
+        F.pf[ZI(q)] = (float)pn;
+      }
     }
   }
   if (!chk) return;
@@ -1150,7 +1204,8 @@ __global__ __launch_bounds__(512) void k_upd_z(Ctx c, int chk) {
     const long long bi = gidx(g, i + 1 == g.nx ? 0 : i + 1, j, 0);
     SQ {
       const long long a = ZI(q);
-      const double dv = (double)((T)F.div[a] + d[q]);
+      const double dv = F.div[a] + (double)d[q];
+      if (MIX) F.div[a] = dv;
       dmax = (dv != dv || dmax != dmax) ? NAN : fmax(dmax, fabs(dv));
       const double wq = (double)wn[q], wq1 = (double)nxt(wn, q, rt);
       const double cfll = fabs(F.u[a] + F.u[bi + (long long)(p0 + q) * pl]) * 0.5 * s.dx1 +
@@ -1228,14 +1283,22 @@ static void rhs_lines(const Ctx& c) {
   ZSEG(c.o.lz.seg, { KScope ks_(c, KID_RHS_ZW); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rhs_zw<T, S_>), dim3(zblocks(c)), zthreads(c), lz, c.st, c); });
   XSEG(c.o.lx.seg, { KScope ks_(c, KID_RHS_X2); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rhs_x2<T, S_>), dim3(xblocks(c)), xthreads(c), lx, c.st, c); });
 }
-void rhs_lines_fp64(const Ctx& c) { rhs_lines<double>(c); }
+void rhs_lines(const Ctx& c) {
+  if (c.s.mixed)
+    rhs_lines<float>(c);
+  else
+    rhs_lines<double>(c);
+}
 
 template <typename T>
 static void getuh_t(const Ctx& c, bool fused) {
   const int ylx = (c.g.nx + c.o.ys.L - 1) / c.o.ys.L;
   if (fused) {
+    const size_t tbytes = sizeof(double) * (size_t)(c.g.n2 + 2) * YTAB_W;
+    const int staged = tbytes <= 48 * 1024 ? 1 : 0;
     KScope ks_(c, KID_ADI_Y);
-    hipLaunchKernelGGL(k_rhsy_adiy<T>, dim3(ylx * c.g.nz, 3), dim3(c.o.ys.L * c.o.ys.C), 0, c.st, c);
+    hipLaunchKernelGGL(k_rhsy_adiy<T>, dim3(ylx * c.g.nz, 3), dim3(c.o.ys.L * c.o.ys.C), staged ? tbytes : 0, c.st, c,
+                       staged);
   } else {
     KScope ks_(c, KID_ADI_Y);
     hipLaunchKernelGGL(k_adi_y<T>, dim3(ylx * c.g.nz, 3), dim3(c.o.ys.L * c.o.ys.C), 0, c.st, c);
@@ -1249,56 +1312,102 @@ static void getuh_t(const Ctx& c, bool fused) {
   ZSEG(c.o.lz.seg, { KScope ks_(c, KID_ADI_Z); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adi_z<T, S_>), dim3(zblocks(c), 3), zthreads(c), lz, c.st, c, z123); });
   XSEG(c.o.lx.seg, { KScope ks_(c, KID_ADI_X); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adi_x<T, S_>), dim3(xblocks(c), 1), xthreads(c), lx, c.st, c, x3); });
 }
-void getuh(const Ctx& c) { getuh_t<double>(c, false); }
-void getuh_fused(const Ctx& c) { getuh_t<double>(c, true); }
-
-template <typename T>
-static void rhsdp_t(const Ctx& c) {
-  const size_t lx = lds_line(c.o.lx, true, sizeof(T)), lz = lds_line(c.o.lz, false, sizeof(T));
-  XSEG(c.o.lx.seg, { KScope ks_(c, KID_RHSDP_X); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rhsdp_x<T, S_>), dim3(xblocks(c)), xthreads(c), lx, c.st, c); });
-  ZSEG(c.o.lz.seg, { KScope ks_(c, KID_RHSDP_Z); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rhsdp_z<T, S_>), dim3(zblocks(c)), zthreads(c), lz, c.st, c); });
+void getuh(const Ctx& c) {
+  if (c.s.mixed)
+    getuh_t<float>(c, false);
+  else
+    getuh_t<double>(c, false);
 }
-void rhsdp(const Ctx& c) { rhsdp_t<double>(c); }
+void getuh_fused(const Ctx& c) {
+  if (c.s.mixed)
+    getuh_t<float>(c, true);
+  else
+    getuh_t<double>(c, true);
+}
 
-template <typename T>
-static void getdp_t(const Ctx& c) {
-  typedef cplx_t<T> zc;
+// FP64 arithmetic; UH (DELTA) stored as TS
+template <typename TS>
+static void rhsdp_t(const Ctx& c) {
+  const size_t lx = lds_line(c.o.lx, true, sizeof(double)), lz = lds_line(c.o.lz, false, sizeof(double));
+  XSEG(c.o.lx.seg, { KScope ks_(c, KID_RHSDP_X); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rhsdp_x<double, S_, TS>), dim3(xblocks(c)), xthreads(c), lx, c.st, c); });
+  ZSEG(c.o.lz.seg, { KScope ks_(c, KID_RHSDP_Z); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rhsdp_z<double, S_, TS>), dim3(zblocks(c)), zthreads(c), lz, c.st, c); });
+}
+void rhsdp(const Ctx& c) {
+  if (c.s.mixed)
+    rhsdp_t<float>(c);
+  else
+    rhsdp_t<double>(c);
+}
+
+// GETDP in FP64 in both modes: rounding RDP's transform or the solution to
+// FP32 would be amplified ~1/DY^2 by the projection's divergence
+static void getdp_fp64(const Ctx& c) {
+  typedef double T;
   const Geo& g = c.g;
   const FFTPlan& f = c.f;
-  const size_t ldsx = 2 * sizeof(zc) * (size_t)g.nx * f.tlx;
-  const size_t ldsz = 2 * sizeof(zc) * (size_t)g.nz * f.tlz;
+  const size_t ldsx = 2 * sizeof(cplx_t<T>) * (size_t)g.nx * f.tlx;
+  const size_t ldsz = 2 * sizeof(cplx_t<T>) * (size_t)g.nz * f.tlz;
   const int gx = (g.n2m * g.nz + 2 * f.tlx - 1) / (2 * f.tlx);
   const int gz = ((g.nxh + f.tlz - 1) / f.tlz) * g.n2m;
   const int gy = ((g.nxh + c.o.yp.L - 1) / c.o.yp.L) * g.nz;
+  void* sp = (void*)c.d.spec;
   { KScope ks_(c, KID_FFT_X); hipLaunchKernelGGL(k_fft_x<T>, dim3(gx), dim3(256), ldsx, c.st, c); }
-  { KScope ks_(c, KID_FFT_Z); hipLaunchKernelGGL(k_fft_z<T>, dim3(gz), dim3(256), ldsz, c.st, c, -1); }
-  { KScope ks_(c, KID_POIS_Y); hipLaunchKernelGGL(k_pois_y<T>, dim3(gy), dim3(c.o.yp.L * c.o.yp.C), 0, c.st, c); }
-  { KScope ks_(c, KID_FFT_Z); hipLaunchKernelGGL(k_fft_z<T>, dim3(gz), dim3(256), ldsz, c.st, c, +1); }
-  { KScope ks_(c, KID_FFT_XINV); hipLaunchKernelGGL(k_fft_xinv<T>, dim3(gx), dim3(256), ldsx, c.st, c); }
+  { KScope ks_(c, KID_FFT_Z); hipLaunchKernelGGL(k_fft_z<T>, dim3(gz), dim3(256), ldsz, c.st, c, -1, sp); }
+  {
+    KScope ks_(c, KID_POIS_Y);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pois_y<T, T>), dim3(gy), dim3(c.o.yp.L * c.o.yp.C), 0, c.st, c,
+                       (const void*)sp, sp);
+  }
+  { KScope ks_(c, KID_FFT_Z); hipLaunchKernelGGL(k_fft_z<T>, dim3(gz), dim3(256), ldsz, c.st, c, +1, sp); }
+  { KScope ks_(c, KID_FFT_XINV); hipLaunchKernelGGL(k_fft_xinv<T>, dim3(gx), dim3(256), ldsx, c.st, c, (const void*)sp); }
 }
-void getdp(const Ctx& c) { getdp_t<double>(c); }
+void getdp(const Ctx& c) { getdp_fp64(c); }
 
 void pois_table(const Ctx& c) {
   const long long nm = (long long)c.g.nxh * c.g.nz;
   hipLaunchKernelGGL(k_pois_table, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, c.st, c);
 }
 
-template <typename T>
-static void upd_x_t(const Ctx& c, bool chk) {
-  const size_t lx = lds_line(c.o.lx, true, sizeof(T));
-  XSEG(c.o.lx.seg, { KScope ks_(c, KID_UPD_X); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_upd_x<T, S_>), dim3(xblocks(c)), xthreads(c), lx, c.st, c, chk ? 1 : 0); });
+template <typename TS>
+static void upd_x_t(const Ctx& c, bool chk, int refine) {
+  const size_t lx = lds_line(c.o.lx, true, sizeof(double));
+  XSEG(c.o.lx.seg, { KScope ks_(c, KID_UPD_X); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_upd_x<double, S_, TS>), dim3(xblocks(c)), xthreads(c), lx, c.st, c, chk ? 1 : 0, refine); });
 }
-template <typename T>
-static void upd_z_t(const Ctx& c, bool chk) {
-  const size_t lz = lds_line(c.o.lz, false, sizeof(T));
+template <typename TS>
+static void upd_z_t(const Ctx& c, bool chk, int refine) {
+  const size_t lz = lds_line(c.o.lz, false, sizeof(double));
   if (chk) zero(c, c.d.red, 128);
-  ZSEG(c.o.lz.seg, { KScope ks_(c, KID_UPD_Z); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_upd_z<T, S_>), dim3(zblocks(c)), zthreads(c), lz, c.st, c, chk ? 1 : 0); });
+  ZSEG(c.o.lz.seg, { KScope ks_(c, KID_UPD_Z); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_upd_z<double, S_, TS>), dim3(zblocks(c)), zthreads(c), lz, c.st, c, chk ? 1 : 0, refine); });
   if (chk) {
     KScope ks_(c, KID_FLOW_FIN);
     hipLaunchKernelGGL(k_max_fin, dim3(1), dim3(64), 0, c.st, c);
   }
 }
-void upd_x(const Ctx& c, bool chk) { upd_x_t<double>(c, chk); }
-void upd_z(const Ctx& c, bool chk) { upd_z_t<double>(c, chk); }
+void upd_x(const Ctx& c, bool chk, bool refine) {
+  if (c.s.mixed)
+    upd_x_t<float>(c, chk, refine ? 1 : 0);
+  else
+    upd_x_t<double>(c, chk, 0);
+}
+void upd_z(const Ctx& c, bool chk, bool refine) {
+  if (c.s.mixed)
+    upd_z_t<float>(c, chk, refine ? 1 : 0);
+  else
+    upd_z_t<double>(c, chk, 0);
+}
+
+// RDP = DIV/DT for the refinement solve
+__global__ void k_div_rdp(Ctx c) {
+  const Geo g = c.g;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.nx) return;
+  const long long b = gidx(g, i, 1 + blockIdx.y, blockIdx.z);
+  c.d.rdp[b] = c.d.div[b] / c.s.dt;
+}
+void div_rdp(const Ctx& c) {
+  const int bx = c.g.nx < 256 ? c.g.nx : 256;
+  KScope ks_(c, KID_RHSDP_X);
+  hipLaunchKernelGGL(k_div_rdp, dim3((c.g.nx + bx - 1) / bx, c.g.n2m, c.g.nz), dim3(bx), 0, c.st, c);
+}
 
 }  // namespace c3dk

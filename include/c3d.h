@@ -31,7 +31,12 @@ typedef struct c3d_handle c3d_handle;
 
 enum c3d_precision {
   C3D_FP64 = 0,  /* every operator in FP64 */
-  C3D_MIXED = 1  /* FP32 operators, FP64 state / projection / reductions */
+  /* FP32 derivative/filter operators, RHS, ADI sweeps and forward FFTs;
+   * FP64 state (U, P), pressure correction, Poisson y-solves and inverse
+   * FFTs, flow-rate constraint, DIVCHECK and CFL; an FP64-residual
+   * refinement of the projection when its DIVMAX exceeds 1e-10.
+   * FILES = 0 only. */
+  C3D_MIXED = 1
 };
 
 /* Run configuration.  Mirrors what SETUP reads from parame.ter and what MESH,
@@ -113,7 +118,8 @@ enum c3d_field {
 /* copies one device field into a host array in its Fortran layout */
 int c3d_get_field(c3d_handle *h, int which, double *out);
 /* device scalars: 0 PRESG 1 PRESG3 2 DMPRESG 3 DMPRESG3 4 FLOW1 5 FLOW3
- * 6 WSM(1) 7 WSM(2) 8 WSM(3) */
+ * 6 WSM(1) 7 WSM(2) 8 WSM(3); get only (mixed mode): 9 refinement passes
+ * run, 10 DIVMAX of the last step's first projection */
 int c3d_get_scalar(c3d_handle *h, int which, double *out);
 int c3d_set_scalar(c3d_handle *h, int which, double value);
 int c3d_set_dt(c3d_handle *h, double dt);

@@ -1,0 +1,89 @@
+"""Mixed precision (C3D_MIXED) against the FP64 path of the same library,
+which the parity tests pin to the oracle at 1e-12.
+
+Mixed mode keeps the state U, V, W, P in FP64 and runs the derivative and
+filter operators, the RHS, the ADI sweeps, the FFTs and most Poisson modes
+in FP32; the Poisson y-solves, the inverse FFTs, the pressure correction, the
+flow-rate constraint, DIV and CFL stay FP64, and an FP64-residual
+refinement of the projection runs when its DIVMAX exceeds 1e-10.  What is checked, on the same seeded inputs:
+  post-projection DIVMAX of every step                 <= 1e-9
+  flow rate after a step equals the one before         <= 1e-12 relative
+  velocities after 10 steps, max-norm relative error   <= 1e-5
+  pressure after 10 steps                              <= 1e-3
+  one RHS phase (RUH), max-norm relative error         <= 1e-5
+The measured errors are printed (pytest -s) for the record.
+"""
+import numpy as np
+import pytest
+
+from cases import config, inputs, interior_p, interior_u, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(cfg, seed):
+    from pyoracle import Oracle
+    from compact3d_amd import c3d
+    orc = Oracle(**cfg)
+    U, P, HH = inputs(orc, seed=seed)
+    a = c3d.Channel(**cfg, precision=c3d.FP64)
+    b = c3d.Channel(**cfg, precision=c3d.MIXED)
+    for ch in (a, b):
+        ch.upload(U, P, HH)
+    return a, b, U, P
+
+
+def test_mixed_trajectory_10_steps():
+    n1, n2, n3 = 65, 65, 65
+    cfg = config(n1, n2, n3, cks=1.5)
+    a, b, U, P = _pair(cfg, seed=3)
+    f0 = b.flowrate()
+    for n in range(10):
+        da, _ = a.step(n + 1)
+        db, _ = b.step(n + 1)
+        f1 = b.flowrate()
+        print("step", n + 1, "divmax fp64 %.2e mixed %.2e" % (da, db), "flow drift %.2e" % (abs(f1[0] - f0[0]) / abs(f0[0])))
+        assert db <= 1e-9
+        assert abs(f1[0] - f0[0]) <= 1e-12 * abs(f0[0])
+    sa, sb = a.download(), b.download()
+    eu = [relerr(interior_u(sa["U"], n1, n2, n3, c), interior_u(sb["U"], n1, n2, n3, c)) for c in range(3)]
+    l2 = [float(np.linalg.norm(interior_u(sa["U"] - sb["U"], n1, n2, n3, c)) /
+                np.linalg.norm(interior_u(sa["U"], n1, n2, n3, c))) for c in range(3)]
+    ep = relerr(interior_p(sa["P"], n1, n2, n3), interior_p(sb["P"], n1, n2, n3))
+    from compact3d_amd import c3d
+    print("refinement passes:", b.scalar(c3d.S_REFINES))
+    print("mixed vs fp64 after 10 steps: U max", ["%.2e" % x for x in eu], "L2", ["%.2e" % x for x in l2],
+          "P max %.2e" % ep, "PRESG", sa["PRESG"], sb["PRESG"])
+    assert max(eu) <= 1e-5
+    assert ep <= 1e-3
+    a.close()
+    b.close()
+
+
+def test_mixed_rhs_phase():
+    n1, n2, n3 = 49, 41, 33
+    cfg = config(n1, n2, n3)
+    a, b, U, P = _pair(cfg, seed=7)
+    from compact3d_amd import c3d
+    for ch in (a, b):
+        ch.bcond()
+        ch.rhs()
+    for f in (c3d.F_RUH1, c3d.F_RUH2, c3d.F_RUH3):
+        ra, rb = a.field(f), b.field(f)
+        e = relerr(ra, rb)
+        print("RUH", f, "%.2e" % e)
+        assert e <= 1e-5
+    a.close()
+    b.close()
+
+
+def test_mixed_state_roundtrip():
+    """the FP64 state survives upload/download in mixed mode bit for bit"""
+    n1, n2, n3 = 33, 33, 33
+    cfg = config(n1, n2, n3)
+    a, b, U, P = _pair(cfg, seed=1)
+    sb = b.download()
+    assert np.array_equal(interior_u(sb["U"], n1, n2, n3), interior_u(U, n1, n2, n3))
+    assert np.array_equal(interior_p(sb["P"], n1, n2, n3), interior_p(P, n1, n2, n3))
+    a.close()
+    b.close()
