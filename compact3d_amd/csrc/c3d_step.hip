@@ -102,6 +102,19 @@ struct XStage {
     const int p = q < 0 ? pm : (q >= SEG ? pp : p0 + q);
     return B[p * LS + l];
   }
+  // the same tile to two arrays (a field and its FP32 shadow)
+  template <typename S1, typename S2>
+  __device__ __forceinline__ void out2(S1* F1, S2* F2, const T (&v)[SEG], int jmin = 1) {
+    T* B = next();
+    __syncthreads();
+    SQ B[(p0 + q) * LS + l] = v[q];
+    __syncthreads();
+    SQ if (go[q] >= 0 && jr[q] >= jmin) {
+      const T x = B[li[q]];
+      F1[go[q]] = (S1)x;
+      F2[go[q]] = (S2)x;
+    }
+  }
   // rows with j < jmin are left untouched
   template <typename S>
   __device__ __forceinline__ void out(S* F, const T (&v)[SEG], int jmin = 1) {
@@ -275,38 +288,44 @@ __global__ __launch_bounds__(512) void k_rhs_z(Ctx c) {
   T* hh1 = (T*)F.hh[1];
   T* hh2 = (T*)F.hh[2];
   const T dx3 = (T)s.dx3, dx3q = (T)s.dx3q, rinv = (T)s.rinv;
-  T a[SEG], d[SEG], e[SEG], lf, rt;
+  // each read-modify-write operand is loaded one solve ahead of its use
+  T a[SEG], d[SEG], e[SEG], pr[SEG], lf, rt;
   // u
   zload((const T*)F.su, b, pl, p0, bm, bp, a, lf, rt);
   SQ d[q] = (T)12. * dx3q * (prv(a, q, lf) - (T)2. * a[q] + nxt(a, q, rt));
+  SQ pr[q] = ruh0[ZI(q)];
   psolve<T, SEG>(d, G, L, l, gs, s10, bf.xf, bf.xb);
-  if (ok) SQ ruh0[ZI(q)] = ruh0[ZI(q)] + rinv * d[q];
+  if (ok) SQ ruh0[ZI(q)] = pr[q] + rinv * d[q];
   SQ d[q] = (T)4. * (prv(a, q, lf) + a[q]);
+  SQ e[q] = ((const T*)F.ww1)[ZI(q)];
   psolve<T, SEG>(d, G, L, l, gs, s6, bf.xf, bf.xb);
-  SQ d[q] = d[q] * ((const T*)F.ww1)[ZI(q)];  // UW
+  SQ d[q] = d[q] * e[q];  // UW
   if (ok) SQ((T*)F.uw)[ZI(q)] = d[q];
+  SQ pr[q] = hh0[ZI(q)];
   rt = xch_right(d[0], G, L, l, gs, bf.xr);
   SQ d[q] = (T)24. * dx3 * (-d[q] + nxt(d, q, rt));
-  psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
-  if (ok) SQ hh0[ZI(q)] = hh0[ZI(q)] + d[q];
-  // v
   zload((const T*)F.sv, b, pl, p0, bm, bp, a, lf, rt);
+  psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
+  if (ok) SQ hh0[ZI(q)] = pr[q] + d[q];
+  // v
   if (j >= 2) {
     SQ d[q] = (T)12. * dx3q * (prv(a, q, lf) - (T)2. * a[q] + nxt(a, q, rt));
+    SQ pr[q] = ruh1[ZI(q)];
     psolve<T, SEG>(d, G, L, l, gs, s10, bf.xf, bf.xb);
-    if (ok) SQ ruh1[ZI(q)] = ruh1[ZI(q)] + rinv * d[q];
+    if (ok) SQ ruh1[ZI(q)] = pr[q] + rinv * d[q];
   }
   SQ d[q] = (T)4. * (prv(a, q, lf) + a[q]);
+  const T hj = (T)(1. / M.h[j]), dyj = (T)(M.dy[j] / 2.), dyjm = (T)(M.dy[j - 1] / 2.);
+  if (j >= 2) SQ e[q] = hj * (dyj * sw[ZI(q) - g.nx] + dyjm * sw[ZI(q)]);  // WW2 (:7028)
   psolve<T, SEG>(d, G, L, l, gs, s6, bf.xf, bf.xb);
   if (ok) SQ((T*)F.vv3)[ZI(q)] = d[q];
   if (j >= 2) {
-    // WW2 = (DY(J)/2 W(J-1) + DY(J-1)/2 W(J)) / H(J)  (:7028)
-    const T hj = (T)(1. / M.h[j]), dyj = (T)(M.dy[j] / 2.), dyjm = (T)(M.dy[j - 1] / 2.);
-    SQ d[q] = d[q] * (hj * (dyj * sw[ZI(q) - g.nx] + dyjm * sw[ZI(q)]));
+    SQ d[q] = d[q] * e[q];
+    SQ pr[q] = hh1[ZI(q)];
     rt = xch_right(d[0], G, L, l, gs, bf.xr);
     SQ d[q] = (T)24. * dx3 * (-d[q] + nxt(d, q, rt));
     psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
-    if (ok) SQ hh1[ZI(q)] = hh1[ZI(q)] + d[q];
+    if (ok) SQ hh1[ZI(q)] = pr[q] + d[q];
   }
 }
 
@@ -480,39 +499,50 @@ __global__ __launch_bounds__(256) void k_adi_y(Ctx c) {
 // with the J-1, J, J+1 values of its component's fields in registers (the
 // expressions are those of yterm_u/v/w); with them it calls yterm_*.
 // Returns TDMAI's right-hand side BET(J) * RUH(J) * DT.
+// (each iteration issues the next J's loads before its own arithmetic and
+// HH store, so one row of loads is always in flight)
 template <typename T>
 __device__ __forceinline__ void yfuse_u(const Ctx& c, const double* tb, long long b, long long t2, const YPos& y,
                                         T (&v)[CHMAX]) {
   const Geo& g = c.g;
   const Fields& F = c.d;
-  const T* su = (const T*)F.su;
-  const T* vv1 = (const T*)F.vv1;
-  const T* ruh = (const T*)F.ruh[0];
-  T* hh = (T*)F.hh[0];
-  const T* hp = (const T*)F.hhp[0];
+  const T* __restrict__ su = (const T*)F.su;
+  const T* __restrict__ vv1 = (const T*)F.vv1;
+  const T* __restrict__ ruh = (const T*)F.ruh[0];
+  T* __restrict__ hh = (T*)F.hh[0];
+  const T* __restrict__ hp = (const T*)F.hhp[0];
   const T rinv = (T)c.s.rinv, dt = (T)c.s.dt;
   const long long nx = g.nx;
   const T uN2 = su[b + g.n2 * nx], uW = su[b];
   const T bcw = -(T)F.tauw[t2] / (T)c.m.dy[1] * (T)c.s.ifwall;
   int J = y.j0;
-  T um = su[b + (J - 1) * nx], u0 = su[b + J * nx];
-  T w1 = vv1[b + J * nx];
+  long long a = b + J * nx;
+  T um = su[a - nx], u0 = su[a];
+  T w1 = vv1[a];
+  T up_n = su[a + nx], w2_n = vv1[a + nx], ru_n = ruh[a], hh_n = hh[a], hp_n = hp[a];
 #pragma unroll
   for (int q = 0; q < CHMAX; q++)
     if (q < y.len) {
       J = y.j0 + q;
+      a = b + J * nx;
+      const T up = up_n, w2 = w2_n, rux = ru_n, hhx = hh_n, hpx = hp_n;
+      if (q + 1 < y.len) {
+        const long long a1 = a + nx;
+        up_n = su[a1 + nx];
+        w2_n = vv1[a1 + nx];
+        ru_n = ruh[a1];
+        hh_n = hh[a1];
+        hp_n = hp[a1];
+      }
       const double* t = tb + J * YTAB_W;
-      const long long a = b + J * nx;
-      const T up = su[a + nx];
-      const T w2 = vv1[a + nx];
-      const T ru = ruh[a] + rinv * ((T)t[0] * up - (T)t[1] * u0 + (T)t[2] * um);
+      const T ru = rux + rinv * ((T)t[0] * up - (T)t[1] * u0 + (T)t[2] * um);
       const T u2 = (T)t[3] * ((T)t[6] * up + (T)t[7] * u0) + (T)t[9] * uN2;
       const T u1 = (T)t[4] * ((T)t[5] * u0 + (T)t[6] * um) + (T)t[10] * uW;
       const T v2 = (J == g.n2m) ? (T)0 : w2;
       const T v1 = (J == 1) ? (T)0 : w1;
-      const T h = hh[a] + (T)t[8] * (v2 * u2 - v1 * u1);
+      const T h = hhx + (T)t[8] * (v2 * u2 - v1 * u1);
       hh[a] = h;
-      const T r = ru - (T)(3. / 2.) * h + (T)(1. / 2.) * hp[a] + (T)t[10] * bcw;
+      const T r = ru - (T)(3. / 2.) * h + (T)(1. / 2.) * hpx + (T)t[10] * bcw;
       v[q] = (T)t[YT_BET] * (r * dt);
       um = u0;
       u0 = up;
@@ -524,32 +554,41 @@ template <typename T>
 __device__ __forceinline__ void yfuse_v(const Ctx& c, const double* tb, long long b, const YPos& y,
                                         T (&v)[CHMAX]) {
   const Fields& F = c.d;
-  const T* sv = (const T*)F.sv;
-  const T* sp = (const T*)F.sp;
-  const T* ruh = (const T*)F.ruh[1];
-  T* hh = (T*)F.hh[1];
-  const T* hp = (const T*)F.hhp[1];
+  const T* __restrict__ sv = (const T*)F.sv;
+  const T* __restrict__ sp = (const T*)F.sp;
+  const T* __restrict__ ruh = (const T*)F.ruh[1];
+  T* __restrict__ hh = (T*)F.hh[1];
+  const T* __restrict__ hp = (const T*)F.hhp[1];
   const T rinv = (T)c.s.rinv, dt = (T)c.s.dt;
   const long long nx = c.g.nx;
   int J = y.j0;
-  T vm = sv[b + (J - 1) * nx], v0 = sv[b + J * nx];
-  T pm = sp[b + (J - 1) * nx];
+  long long a = b + J * nx;
+  T vm = sv[a - nx], v0 = sv[a];
+  T pm = sp[a - nx];
+  T vp_n = sv[a + nx], p0_n = sp[a], ru_n = ruh[a], hh_n = hh[a], hp_n = hp[a];
 #pragma unroll
   for (int q = 0; q < CHMAX; q++)
     if (q < y.len) {
       J = y.j0 + q;
+      a = b + J * nx;
+      const T vp = vp_n, p0 = p0_n, rux = ru_n, hhx = hh_n, hpx = hp_n;
+      if (q + 1 < y.len) {
+        const long long a1 = a + nx;
+        vp_n = sv[a1 + nx];
+        p0_n = sp[a1];
+        ru_n = ruh[a1];
+        hh_n = hh[a1];
+        hp_n = hp[a1];
+      }
       const double* t = tb + J * YTAB_W;
-      const long long a = b + J * nx;
-      const T vp = sv[a + nx];
-      const T p0 = sp[a];
       const T viscos = rinv * ((T)t[0] * vp - (T)t[1] * v0 + (T)t[2] * vm);
       const T pressg2 = (p0 - pm) / (T)t[3];
-      const T ru = ruh[a] + (-pressg2 + viscos);
+      const T ru = rux + (-pressg2 + viscos);
       const T v2 = (T)0.5 * (vp + v0);
       const T v1 = (T)0.5 * (v0 + vm);
-      const T h = hh[a] + (T)t[4] * (v2 * v2 - v1 * v1);
+      const T h = hhx + (T)t[4] * (v2 * v2 - v1 * v1);
       hh[a] = h;
-      const T r = ru - (T)(3. / 2.) * h + (T)(1. / 2.) * hp[a];
+      const T r = ru - (T)(3. / 2.) * h + (T)(1. / 2.) * hpx;
       v[q] = (T)t[YT_BET] * (r * dt);
       vm = v0;
       v0 = vp;
@@ -562,34 +601,43 @@ __device__ __forceinline__ void yfuse_w(const Ctx& c, const double* tb, long lon
                                         T (&v)[CHMAX]) {
   const Geo& g = c.g;
   const Fields& F = c.d;
-  const T* sw = (const T*)F.sw;
-  const T* vv3 = (const T*)F.vv3;
-  const T* ruh = (const T*)F.ruh[2];
-  T* hh = (T*)F.hh[2];
-  const T* hp = (const T*)F.hhp[2];
+  const T* __restrict__ sw = (const T*)F.sw;
+  const T* __restrict__ vv3 = (const T*)F.vv3;
+  const T* __restrict__ ruh = (const T*)F.ruh[2];
+  T* __restrict__ hh = (T*)F.hh[2];
+  const T* __restrict__ hp = (const T*)F.hhp[2];
   const T rinv = (T)c.s.rinv, dt = (T)c.s.dt;
   const long long nx = g.nx;
   const T wN2 = sw[b + g.n2 * nx], wW = sw[b];
   const T bcw = -(T)F.taub[t2] / (T)c.m.dy[1] * (T)c.s.ifwall;
   int J = y.j0;
-  T wm = sw[b + (J - 1) * nx], w0 = sw[b + J * nx];
-  T s1 = vv3[b + J * nx];
+  long long a = b + J * nx;
+  T wm = sw[a - nx], w0 = sw[a];
+  T s1 = vv3[a];
+  T wp_n = sw[a + nx], s2_n = vv3[a + nx], ru_n = ruh[a], hh_n = hh[a], hp_n = hp[a];
 #pragma unroll
   for (int q = 0; q < CHMAX; q++)
     if (q < y.len) {
       J = y.j0 + q;
+      a = b + J * nx;
+      const T wp = wp_n, s2 = s2_n, rux = ru_n, hhx = hh_n, hpx = hp_n;
+      if (q + 1 < y.len) {
+        const long long a1 = a + nx;
+        wp_n = sw[a1 + nx];
+        s2_n = vv3[a1 + nx];
+        ru_n = ruh[a1];
+        hh_n = hh[a1];
+        hp_n = hp[a1];
+      }
       const double* t = tb + J * YTAB_W;
-      const long long a = b + J * nx;
-      const T wp = sw[a + nx];
-      const T s2 = vv3[a + nx];
-      const T ru = ruh[a] + rinv * ((T)t[0] * wp - (T)t[1] * w0 + (T)t[2] * wm);
+      const T ru = rux + rinv * ((T)t[0] * wp - (T)t[1] * w0 + (T)t[2] * wm);
       const T w2 = (T)t[3] * ((T)t[7] * w0 + (T)t[6] * wp) + (T)t[9] * wN2;
       const T w1 = (T)t[4] * ((T)t[6] * wm + (T)t[5] * w0) + (T)t[10] * wW;
       const T v2 = (J == g.n2m) ? (T)0 : s2;
       const T v1 = (J == 1) ? (T)0 : s1;
-      const T h = hh[a] + (T)t[8] * (v2 * w2 - v1 * w1);
+      const T h = hhx + (T)t[8] * (v2 * w2 - v1 * w1);
       hh[a] = h;
-      const T r = ru - (T)(3. / 2.) * h + (T)(1. / 2.) * hp[a] + (T)t[10] * bcw;
+      const T r = ru - (T)(3. / 2.) * h + (T)(1. / 2.) * hpx + (T)t[10] * bcw;
       v[q] = (T)t[YT_BET] * (r * dt);
       wm = w0;
       w0 = wp;
@@ -1112,11 +1160,13 @@ __global__ __launch_bounds__(256) void k_upd_x(Ctx c, int chk, int refine) {
   const Fields F = c.d;
   const T dx1 = (T)s.dx1, dt = (T)s.dt;
   const T dmp = refine ? (T)0 : (T)F.sc[SC_DMPRESG];
-  const T ku = refine ? (T)0 : (T)1;  // weight of DELTA
-  T dpr[SEG], d[SEG], un[SEG], pf[SEG];
+  const T ku = refine ? (T)0 : (T)1;  // weight of UH (DELTA)
+  // pf: next UH/DP tile in flight; pg: next FP64 state tile (mixed mode)
+  T dpr[SEG], d[SEG], un[SEG], pf[SEG], pg[SEG];
   X.pre(F.dp, 0, pf);
   const T* B = X.put(pf);
   X.pre((const TS*)F.uh[0], 0, pf);
+  if (MIX) X.pre(F.u, 0, pg);
   SQ {
     dpr[q] = XA(B, q);
     d[q] = (T)24. * dx1 * (-XA(B, q - 1) + XA(B, q));
@@ -1124,25 +1174,29 @@ __global__ __launch_bounds__(256) void k_upd_x(Ctx c, int chk, int refine) {
   psolve<T, SEG>(d, G, L, l, gs, c.o.px.s22, bf.xf, bf.xb);
   B = X.put(pf);
   SQ un[q] = ku * XA(B, q) - dt * d[q] - dt * dmp;
-  if (MIX) {
-    B = X.in(F.u, 0);
-    SQ un[q] = XA(B, q) + un[q];
-  }
   X.pre((const TS*)F.uh[1], 0, pf);
-  X.out(F.u, un);
-  if (MIX) X.out(F.uf, un);
+  if (MIX) {
+    B = X.put(pg);
+    SQ un[q] = XA(B, q) + un[q];
+    X.pre(F.vold, 0, pg);
+    X.out2(F.u, F.uf, un);
+  } else {
+    X.out(F.u, un);
+  }
   const T hj = (T)M.h[j];
   const T* B1 = X.put(pf);
   X.pre(F.dp, -1, pf);
   B = X.put(pf);
   SQ d[q] = (j >= 2) ? ku * XA(B1, q) - dt * (dpr[q] - XA(B, q)) / hj : (T)0;
-  if (MIX) {
-    B = X.in(F.vold, 0);
-    SQ d[q] = (j >= 2) ? XA(B, q) + d[q] : (T)0;
-  }
   if (chk) X.pre((const TS*)F.uh[1], 1, pf);
-  X.out(F.v, d, 2);  // V(I,1,K) is not written by UPCALC
-  if (MIX) X.out(F.vf, d, 2);
+  if (MIX) {
+    B = X.put(pg);
+    SQ d[q] = (j >= 2) ? XA(B, q) + d[q] : (T)0;
+    if (chk) X.pre(F.vold, 1, pg);
+    X.out2(F.v, F.vf, d, 2);  // V(I,1,K) is not written by UPCALC
+  } else {
+    X.out(F.v, d, 2);
+  }
   if (!chk) return;
   // new V at J+1 (zero at J+1 = N2: the wall value); d = (V(J+1) - V(J))/DY
   const T hjp = (T)M.h[j + 1], dy = (T)M.dy[j];
@@ -1152,7 +1206,7 @@ __global__ __launch_bounds__(256) void k_upd_x(Ctx c, int chk, int refine) {
   T e[SEG];
   SQ e[q] = ku * XA(B1, q) - dt * (XA(B, q) - dpr[q]) / hjp;
   if (MIX) {
-    B = X.in(F.vold, 1);
+    B = X.put(pg);
     SQ e[q] = XA(B, q) + e[q];
   }
   SQ d[q] = (((j + 1 <= g.n2m) ? e[q] : (T)0) - d[q]) / dy;
