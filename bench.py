@@ -117,7 +117,7 @@ def run_leg(c3d, cfg, U, P, precision, args, pts):
     return ch, dict(value=pts / dt_s / 1e6, dt_s=dt_s, divmax_max=dmax, kt=kt, kernels=kernels)
 
 
-def roofline(kt, table, pts, grid):
+def roofline(kt, table, pts, grid, precision):
     if not kt:
         return None
     dom = max((k for k in kt if k in table), key=lambda k: kt[k][0])
@@ -128,7 +128,7 @@ def roofline(kt, table, pts, grid):
     tp = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tp):
         try:
-            rec = json.load(open(tp)).get(f"{dom}@{grid}")
+            rec = json.load(open(tp)).get(f"{dom}@{grid}@{precision}")
             if rec:
                 traffic = rec["bytes_per_launch"]
         except Exception:
@@ -148,6 +148,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-timing", action="store_true", help="no per-kernel events (for profilers)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain device copy of 256 MiB three times (the reference "
+                         "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU decomposition is a design "
@@ -166,6 +169,13 @@ def main():
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a GPU")
     torch.cuda.init()
+    if args.copy_calibration:
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     probe = c3d.Channel(**cfg)
     m = probe.mesh()  # the grid the library built, for the initial field
     probe.close()
@@ -197,8 +207,8 @@ def main():
     value = rmx["value"]
     dt_s = rmx["dt_s"]
     finite = math.isfinite(rmx["divmax_max"]) and math.isfinite(cflm)
-    roof = roofline(rmx["kt"], ALG_BYTES_MIXED, pts, args.grid)
-    roof64 = roofline(r64["kt"], ALG_BYTES, pts, args.grid)
+    roof = roofline(rmx["kt"], ALG_BYTES_MIXED, pts, args.grid, "mixed")
+    roof64 = roofline(r64["kt"], ALG_BYTES, pts, args.grid, "fp64")
     step_frac = 112.0 * pts / r64["dt_s"] / (HBM_PEAK_GBS * 1e9)
 
     cpu = None

@@ -113,6 +113,24 @@ static bool make_lshape(int n, bool staged, size_t es, LShape& sh) {
   return true;
 }
 
+// FP32 z-line tile: 128-byte rows need 32 columns, so 16-point segments
+// (8 or 2 when they do not divide n) keep L*G <= 512
+static bool make_lshape32(int n, LShape& sh) {
+  const int seg = (n % 16 == 0) ? 16 : (n % 8 == 0) ? 8 : 2;
+  const int G = n / seg;
+  if (G > 512) return false;
+  int L = std::min(64, 512 / G);
+  if (L >= 32)
+    L -= L % 32;
+  else if (L >= 16)
+    L -= L % 16;
+  sh.n = n;
+  sh.seg = seg;
+  sh.G = G;
+  sh.L = L;
+  return true;
+}
+
 static OpSet make_opset(int seg, double adi_a, double adi_b) {
   OpSet o;
   o.s6 = make_psolve(1.0, 6.0, seg);
@@ -122,12 +140,14 @@ static OpSet make_opset(int seg, double adi_a, double adi_b) {
   return o;
 }
 
-// y tiles: C chunks of at most 16 points (C a power of two), L = 256/C <= 64
-static YShape make_yshape(int m) {
+// y tiles: C chunks of at most 16 points (C a power of two) and L columns,
+// L*C <= nt; L a multiple of `row` columns where possible
+static YShape make_yshape(int m, int nt, int row) {
   YShape y;
   y.C = 1;
   while (y.C * 16 < m) y.C *= 2;
-  y.L = std::max(1, std::min(64, 256 / y.C));
+  y.L = std::max(1, std::min(64, nt / y.C));
+  if (y.L >= row) y.L -= y.L % row;
   return y;
 }
 
@@ -337,6 +357,7 @@ static void build_dt_terms(c3d_handle* h, std::vector<double>& ych) {
   o.adiz = make_solve(o.adiz_a, o.adiz_b);
   o.px = make_opset(o.lx.seg, o.adix_a, o.adix_b);
   o.pz = make_opset(o.lz.seg, o.adiz_a, o.adiz_b);
+  o.pz32 = make_opset(o.lz32.seg, o.adiz_a, o.adiz_b);
 }
 
 // coefficient table of the fused RHS-y / ADI-y sweep (layout in c3d_impl.h)
@@ -463,8 +484,10 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   Ops& o = c.o;
   if (!make_lshape(g.nx, true, sizeof(double), o.lx) || !make_lshape(g.nz, false, sizeof(double), o.lz))
     return bad("N1-1 (N3-1) too long for the line tiles: at most 2048 (4096) points");
-  o.ys = make_yshape(g.n2m);
-  o.yp = make_yshape(g.n2m);
+  if (!make_lshape32(g.nz, o.lz32)) return bad("N3-1 too long for the FP32 line tiles");
+  o.ys = make_yshape(g.n2m, 256, 16);
+  o.ys32 = make_yshape(g.n2m, 512, 32);
+  o.yp = make_yshape(g.n2m, 256, 16);
   o.s6 = make_solve(1.0, 6.0);
   o.s10 = make_solve(1.0, 10.0);
   o.s22 = make_solve(1.0, 22.0);

@@ -52,7 +52,11 @@ struct OpSet {
 struct Ops {
   LShape lx, lz;  // register line engine tiles (x-lines staged, z-lines direct)
   OpSet px, pz;
-  YShape ys;      // ADI y-sweep tiles
+  // FP32 z-line tiles: 32 columns (128-byte rows), 16-point segments
+  LShape lz32;
+  OpSet pz32;
+  YShape ys;      // ADI y-sweep tiles (16 columns of FP64: 128-byte rows)
+  YShape ys32;    // the same for FP32 (32 columns)
   YShape yp;      // Poisson y-solve tiles
   LineSolve s6, s10, s22, adix, adiz;
   TileShape tx, tz;    // x-line and z-line tile shapes (2 tiles in 64 KB)

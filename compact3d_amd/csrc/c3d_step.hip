@@ -173,7 +173,7 @@ __device__ __forceinline__ void xstage(XStage<T, SEG>& x, const LBuf<T>& b, cons
 #define ZSETUP                                                        \
   extern __shared__ __align__(16) unsigned char smem[];               \
   const Geo g = c.g;                                                  \
-  const LShape sh = c.o.lz;                                           \
+  const LShape sh = sizeof(T) == 4 ? c.o.lz32 : c.o.lz;             \
   const int n = sh.n, L = sh.L, G = sh.G;                             \
   const LBuf<T> bf = lbuf<T>(smem, sh, false);                        \
   const int l = threadIdx.x % L, gs = threadIdx.x / L, p0 = gs * SEG; \
@@ -186,6 +186,7 @@ __device__ __forceinline__ void xstage(XStage<T, SEG>& x, const LBuf<T>& b, cons
   const long long bm = b + (long long)(p0 == 0 ? n - 1 : p0 - 1) * pl; \
   const long long bp = b + (long long)(p0 + SEG == n ? 0 : p0 + SEG) * pl;
 #define ZI(q) (b + (long long)(p0 + (q)) * pl)
+#define PZ (sizeof(T) == 4 ? c.o.pz32 : c.o.pz)
 
 template <typename T, int SEG, typename S>
 __device__ __forceinline__ void zload(const S* F, long long b, long long pl, int p0, long long bm, long long bp,
@@ -279,7 +280,7 @@ __global__ __launch_bounds__(512) void k_rhs_z(Ctx c) {
   const Scal s = c.s;
   const Metrics M = c.m;
   const Fields F = c.d;
-  const PSolve s6 = c.o.pz.s6, s10 = c.o.pz.s10, s22 = c.o.pz.s22;
+  const PSolve s6 = PZ.s6, s10 = PZ.s10, s22 = PZ.s22;
   const T* sw = (const T*)F.sw;
   T* ruh0 = (T*)F.ruh[0];
   T* ruh1 = (T*)F.ruh[1];
@@ -336,7 +337,7 @@ __global__ __launch_bounds__(512) void k_rhs_zw(Ctx c) {
   ZSETUP;
   const Scal s = c.s;
   const Fields F = c.d;
-  const PSolve s6 = c.o.pz.s6, s10 = c.o.pz.s10, s22 = c.o.pz.s22;
+  const PSolve s6 = PZ.s6, s10 = PZ.s10, s22 = PZ.s22;
   T* ruh2 = (T*)F.ruh[2];
   T* hh2 = (T*)F.hh[2];
   const T dx3 = (T)s.dx3, dx3q = (T)s.dx3q, rinv = (T)s.rinv;
@@ -441,10 +442,10 @@ __device__ __forceinline__ void ychain_bwd(T (&v)[CHMAX], const T (&m)[CHMAX], c
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_adi_y(Ctx c) {
-  __shared__ T xs[4][256];
+__global__ __launch_bounds__(512) void k_adi_y(Ctx c) {
+  __shared__ T xs[4][512];
   const Geo g = c.g;
-  const YShape ys = c.o.ys;
+  const YShape ys = sizeof(T) == 4 ? c.o.ys32 : c.o.ys;
   const int comp = blockIdx.y;
   const int ns = (comp == 1) ? 2 : 1;
   const int L = ys.L, C = ys.C;
@@ -646,11 +647,11 @@ __device__ __forceinline__ void yfuse_w(const Ctx& c, const double* tb, long lon
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_rhsy_adiy(Ctx c, int staged) {
-  __shared__ T xs[4][256];
+__global__ __launch_bounds__(512) void k_rhsy_adiy(Ctx c, int staged) {
+  __shared__ T xs[4][512];
   extern __shared__ __align__(16) unsigned char smem[];
   const Geo g = c.g;
-  const YShape ys = c.o.ys;
+  const YShape ys = sizeof(T) == 4 ? c.o.ys32 : c.o.ys;
   const int comp = blockIdx.y;
   const int ns = (comp == 1) ? 2 : 1;
   const int L = ys.L, C = ys.C;
@@ -765,7 +766,7 @@ __global__ __launch_bounds__(512) void k_adi_z(Ctx c, CompList cl) {
   T a[SEG], d[SEG], lf, rt;
   zload((const T*)UH, b, pl, p0, bm, bp, a, lf, rt);
   SQ d[q] = prv(a, q, lf) + (T)10. * a[q] + nxt(a, q, rt);
-  psolve<T, SEG>(d, G, L, l, gs, c.o.pz.adi, bf.xf, bf.xb);
+  psolve<T, SEG>(d, G, L, l, gs, PZ.adi, bf.xf, bf.xb);
   if (ok) {
     if (fin && sizeof(T) == 8) {  // mixed mode keeps the increment DELTA
       SQ UH[ZI(q)] = (T)U[ZI(q)] + d[q];
@@ -816,7 +817,7 @@ __global__ __launch_bounds__(512) void k_rhsdp_z(Ctx c) {
   T a[SEG], d[SEG], lf, rt;
   zload((const TS*)c.d.uh[2], b, pl, p0, bm, bp, a, lf, rt);
   SQ d[q] = (T)24. * dx3 * (nxt(a, q, rt) - a[q]);
-  psolve<T, SEG>(d, G, L, l, gs, c.o.pz.s22, bf.xf, bf.xb);
+  psolve<T, SEG>(d, G, L, l, gs, PZ.s22, bf.xf, bf.xb);
   double* rdp = c.d.rdp;
   if (ok) SQ rdp[ZI(q)] = rdp[ZI(q)] + d[q] / dt;
 }
@@ -1234,7 +1235,7 @@ __global__ __launch_bounds__(512) void k_upd_z(Ctx c, int chk, int refine) {
   T dpr[SEG], d[SEG], wn[SEG], lf, rt;
   zload(F.dp, b, pl, p0, bm, bp, dpr, lf, rt);
   SQ d[q] = (T)24. * dx3 * (-prv(dpr, q, lf) + dpr[q]);
-  psolve<T, SEG>(d, G, L, l, gs, c.o.pz.s22, bf.xf, bf.xb);
+  psolve<T, SEG>(d, G, L, l, gs, PZ.s22, bf.xf, bf.xb);
   SQ wn[q] = ku * (T)((const TS*)F.uh[2])[ZI(q)] - dt * d[q] - dt * dmp3;
   if (MIX) SQ wn[q] = (T)F.w[ZI(q)] + wn[q];
   if (ok) {
@@ -1251,7 +1252,7 @@ __global__ __launch_bounds__(512) void k_upd_z(Ctx c, int chk, int refine) {
   if (!chk) return;
   rt = xch_right(wn[0], G, L, l, gs, bf.xr);
   SQ d[q] = (T)24. * dx3 * (nxt(wn, q, rt) - wn[q]);
-  psolve<T, SEG>(d, G, L, l, gs, c.o.pz.s22, bf.xf, bf.xb);
+  psolve<T, SEG>(d, G, L, l, gs, PZ.s22, bf.xf, bf.xb);
   double dmax = 0.0, cmax = 0.0;
   if (ok) {
     const double dyj = M.dy[j];
@@ -1325,16 +1326,47 @@ static size_t lds_line(const LShape& sh, bool staged, size_t es) {
   return es * (2 * tile + 4 * (size_t)sh.G * sh.L);
 }
 static int xblocks(const Ctx& c) { return (c.g.n2m * c.g.nz + c.o.lx.L - 1) / c.o.lx.L; }
-static int zblocks(const Ctx& c) { return ((c.g.nx + c.o.lz.L - 1) / c.o.lz.L) * c.g.n2m; }
+static int zblocks(const Ctx& c, const LShape& z) { return ((c.g.nx + z.L - 1) / z.L) * c.g.n2m; }
+static int zblocks(const Ctx& c) { return zblocks(c, c.o.lz); }
 static dim3 xthreads(const Ctx& c) { return dim3(c.o.lx.L * c.o.lx.G); }
 static dim3 zthreads(const Ctx& c) { return dim3(c.o.lz.L * c.o.lz.G); }
+// the z-line shape of compute type T
+template <typename T>
+static const LShape& zshape(const Ctx& c) {
+  return sizeof(T) == 4 ? c.o.lz32 : c.o.lz;
+}
+// compile-time segment of the FP32 z-line kernels: 16, 8 or 2
+#define ZSEG32(seg, ...)              \
+  if ((seg) == 16) {                  \
+    constexpr int S_ = 16;            \
+    __VA_ARGS__;                      \
+  } else if ((seg) == 8) {            \
+    constexpr int S_ = 8;             \
+    __VA_ARGS__;                      \
+  } else {                            \
+    constexpr int S_ = 2;             \
+    __VA_ARGS__;                      \
+  }
+
+// z-line launches of compute type T (the FP32 kernels also have SEG 16)
+#define ZLAUNCH(T, KID, KERNEL, GY, ...)                                                              \
+  {                                                                                                   \
+    const LShape& z_ = zshape<T>(c);                                                                  \
+    const size_t lz_ = lds_line(z_, false, sizeof(T));                                                \
+    const dim3 gr_(zblocks(c, z_), GY), th_(z_.L * z_.G);                                             \
+    if (sizeof(T) == 4) {                                                                             \
+      ZSEG32(z_.seg, { KScope ks_(c, KID); hipLaunchKernelGGL(HIP_KERNEL_NAME(KERNEL<float, S_>), gr_, th_, lz_, c.st, c, ##__VA_ARGS__); }); \
+    } else {                                                                                          \
+      ZSEG(z_.seg, { KScope ks_(c, KID); hipLaunchKernelGGL(HIP_KERNEL_NAME(KERNEL<double, S_>), gr_, th_, lz_, c.st, c, ##__VA_ARGS__); }); \
+    }                                                                                                 \
+  }
 
 template <typename T>
 static void rhs_lines(const Ctx& c) {
-  const size_t lx = lds_line(c.o.lx, true, sizeof(T)), lz = lds_line(c.o.lz, false, sizeof(T));
+  const size_t lx = lds_line(c.o.lx, true, sizeof(T));
   XSEG(c.o.lx.seg, { KScope ks_(c, KID_RHS_X); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rhs_x<T, S_>), dim3(xblocks(c)), xthreads(c), lx, c.st, c); });
-  ZSEG(c.o.lz.seg, { KScope ks_(c, KID_RHS_Z); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rhs_z<T, S_>), dim3(zblocks(c)), zthreads(c), lz, c.st, c); });
-  ZSEG(c.o.lz.seg, { KScope ks_(c, KID_RHS_ZW); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rhs_zw<T, S_>), dim3(zblocks(c)), zthreads(c), lz, c.st, c); });
+  ZLAUNCH(T, KID_RHS_Z, k_rhs_z, 1);
+  ZLAUNCH(T, KID_RHS_ZW, k_rhs_zw, 1);
   XSEG(c.o.lx.seg, { KScope ks_(c, KID_RHS_X2); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rhs_x2<T, S_>), dim3(xblocks(c)), xthreads(c), lx, c.st, c); });
 }
 void rhs_lines(const Ctx& c) {
@@ -1346,24 +1378,25 @@ void rhs_lines(const Ctx& c) {
 
 template <typename T>
 static void getuh_t(const Ctx& c, bool fused) {
-  const int ylx = (c.g.nx + c.o.ys.L - 1) / c.o.ys.L;
+  const YShape& ys = sizeof(T) == 4 ? c.o.ys32 : c.o.ys;
+  const int ylx = (c.g.nx + ys.L - 1) / ys.L;
   if (fused) {
     const size_t tbytes = sizeof(double) * (size_t)(c.g.n2 + 2) * YTAB_W;
-    const int staged = tbytes <= 48 * 1024 ? 1 : 0;
+    const int staged = tbytes <= 40 * 1024 ? 1 : 0;
     KScope ks_(c, KID_ADI_Y);
-    hipLaunchKernelGGL(k_rhsy_adiy<T>, dim3(ylx * c.g.nz, 3), dim3(c.o.ys.L * c.o.ys.C), staged ? tbytes : 0, c.st, c,
+    hipLaunchKernelGGL(k_rhsy_adiy<T>, dim3(ylx * c.g.nz, 3), dim3(ys.L * ys.C), staged ? tbytes : 0, c.st, c,
                        staged);
   } else {
     KScope ks_(c, KID_ADI_Y);
-    hipLaunchKernelGGL(k_adi_y<T>, dim3(ylx * c.g.nz, 3), dim3(c.o.ys.L * c.o.ys.C), 0, c.st, c);
+    hipLaunchKernelGGL(k_adi_y<T>, dim3(ylx * c.g.nz, 3), dim3(ys.L * ys.C), 0, c.st, c);
   }
-  const size_t lx = lds_line(c.o.lx, true, sizeof(T)), lz = lds_line(c.o.lz, false, sizeof(T));
+  const size_t lx = lds_line(c.o.lx, true, sizeof(T));
   // GETUH1/2: y, x, z;  GETUH3: y, z, x (compact3d.f:8508-8607)
   const CompList x12{2, {0, 1, 0}, {0, 0, 0}};
   const CompList z123{3, {0, 1, 2}, {1, 1, 0}};
   const CompList x3{1, {2, 0, 0}, {1, 0, 0}};
   XSEG(c.o.lx.seg, { KScope ks_(c, KID_ADI_X); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adi_x<T, S_>), dim3(xblocks(c), 2), xthreads(c), lx, c.st, c, x12); });
-  ZSEG(c.o.lz.seg, { KScope ks_(c, KID_ADI_Z); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adi_z<T, S_>), dim3(zblocks(c), 3), zthreads(c), lz, c.st, c, z123); });
+  ZLAUNCH(T, KID_ADI_Z, k_adi_z, 3, z123);
   XSEG(c.o.lx.seg, { KScope ks_(c, KID_ADI_X); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adi_x<T, S_>), dim3(xblocks(c), 1), xthreads(c), lx, c.st, c, x3); });
 }
 void getuh(const Ctx& c) {
