@@ -138,6 +138,42 @@ def roofline(kt, table, pts, grid, precision):
                 alg_bytes_per_launch=alg, avg_launch_us=round(avg_s * 1e6, 2))
 
 
+def served_leg(args):
+    """The serving metric: a seeded Poisson stream of channel requests served
+    by the continuous-batching loop (compact3d_amd.serve), and the same
+    requests all present at t=0 (the batched throughput it is compared to).
+    Step counts are scaled down from 100-2000 so the leg takes seconds; the
+    offered load stays above what the GPU completes, so the loop is busy."""
+    import copy
+    from compact3d_amd import c3d
+    from compact3d_amd.serve import FieldCache, Server, poisson_requests, stats
+    grids = [(64, 49, 64), (96, 65, 96), (128, 65, 128), (192, 129, 192), (256, 129, 256)]
+    reqs = poisson_requests(args.serve_requests, 20.0, grids, [180, 395, 1000], (100, 400), seed=4)
+    fields = FieldCache()
+    for r in reqs:  # seeded initial fields, generated before any timing
+        ch = c3d.Channel(**r.config(), precision=r.precision)
+        fields.get(r, ch.mesh())
+        ch.close()
+    budget = 240 << 30
+    out = {}
+    for name, stream in (("batched", [copy.copy(r) for r in reqs]), ("served", reqs)):
+        if name == "batched":
+            for r in stream:
+                r.t_arrive = 0.0
+        srv = Server(budget, max_live=args.serve_slots, chunk=10, use_graph=True, fields=fields)
+        done = srv.run(stream)
+        wall = max(r.t_done for r in done) - min(r.t_arrive for r in done)
+        st = stats(done, wall)
+        st["peak_resident_gb"] = srv.peak_bytes / 1e9
+        st["divmax_max"] = max(r.result["divmax"] for r in done)
+        out[name] = st
+    out["served_over_batched"] = out["served"]["throughput_mpts_steps_s"] / out["batched"]["throughput_mpts_steps_s"]
+    out["stream"] = ("%d requests, Poisson arrivals 20/s (seed 4), grids 64x49x64..256x129x256, Re_tau "
+                     "{180, 395 mixed precision; 1000 wall-modelled LES FP64}, 100-400 steps, dt 5e-4, "
+                     "%d slots, 10 steps per turn, captured graphs" % (len(reqs), args.serve_slots))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,6 +184,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-timing", action="store_true", help="no per-kernel events (for profilers)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--serve-slots", type=int, default=16)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain device copy of 256 MiB three times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -211,6 +250,8 @@ def main():
     roof64 = roofline(r64["kt"], ALG_BYTES, pts, args.grid, "fp64")
     step_frac = 112.0 * pts / r64["dt_s"] / (HBM_PEAK_GBS * 1e9)
 
+    served = None if args.no_serve else served_leg(args)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(cfg, args.cpu_seconds, args.seed)
@@ -242,6 +283,7 @@ def main():
         "mixed_divmax_max": rmx["divmax_max"],
         "mixed_refinements": refines,
         "mixed_vs_fp64_err": errs,
+        "served": served,
         "cfl_after": cflm * cfg["dt"],
         "presg_after": st_presg,
         "finite": finite,

@@ -62,6 +62,8 @@ def lib():
             getattr(L, name).argtypes = vp
         L.c3d_getup.argtypes = [C.c_void_p, C.c_int]
         L.c3d_step.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        L.c3d_advance.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.c3d_sync.argtypes = [C.c_void_p, _dp, _dp]
         L.c3d_upload_state.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, C.c_double, C.c_double]
         L.c3d_download_state.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp]
         L.c3d_divcheck.argtypes = [C.c_void_p, _dp]
@@ -170,6 +172,16 @@ class Channel:
             return d.value, c.value
         self._ck(self.L.c3d_step(self.h, ntime, None, None), "step")
         return None
+
+    def advance(self, nsteps, use_graph=True):
+        """queue nsteps steps on the handle's stream without waiting"""
+        self._ck(self.L.c3d_advance(self.h, int(nsteps), 1 if use_graph else 0), "advance")
+
+    def sync(self):
+        """wait for the queued steps; (DIVMAX, CFLM) of the last one"""
+        d, c = C.c_double(), C.c_double()
+        self._ck(self.L.c3d_sync(self.h, C.byref(d), C.byref(c)), "sync")
+        return d.value, c.value
 
     def bcond(self):
         self._ck(self.L.c3d_bcond(self.h), "bcond")

@@ -25,6 +25,10 @@ struct c3d_handle {
   double* vbuf[2] = {nullptr, nullptr};  // mixed mode: the two V buffers
   int vcur = 0;
   long long refines = 0;        // mixed mode: refinement passes run so far
+  // captured steps for c3d_advance: graph p was captured with the buffer
+  // state gkey[p] (HH set, V buffer); -1 = none
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  int gkey[2] = {-1, -1};
   double first_divmax = 0.0;    // mixed mode: DIVMAX of the last first projection
   double* hhbuf[2][3] = {{nullptr}};
   // host copies of the 1-D vectors (reference J indexing)
@@ -644,6 +648,8 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   return 0;
 }
 
+static void graphs_free(c3d_handle* h);
+
 static void kt_free(c3d_handle* h) {
   KTimer& t = h->kt;
   for (int q = 0; q < 2 * t.cap; q++) (void)hipEventDestroy(t.ev[q]);
@@ -658,6 +664,7 @@ static void kt_free(c3d_handle* h) {
 extern "C" int c3d_destroy(c3d_handle* h) {
   if (!h) return 0;
   (void)hipStreamSynchronize(h->c.st);
+  graphs_free(h);
   kt_free(h);
   for (void* p : h->allocs) (void)hipFree(p);
   if (h->own_stream) (void)hipStreamDestroy(h->c.st);
@@ -931,6 +938,95 @@ static int read_sc(c3d_handle* h, int slot, double* v) {
   return 0;
 }
 
+// ------------------------------------------------------- multi-step driving
+// The kernels of one MAIN-loop step (WALLSS first when WMODEL=0 and
+// FILES != 0, then GETUP with DIV, DIVMAX and CFL fused), with no host
+// synchronisation and no mixed-mode refinement check.
+static void step_launch(c3d_handle* h) {
+  if (h->cfg.wmodel == 0.0 && h->cfg.files != 0.0) c3dk::wallss(h->c);
+  swap_hh(h);
+  if (h->cfg.files == 2.0) c3dk::sgscs(h->c);
+  if (h->cfg.files != 0.0) wall_model_impl(h);
+  c3dk::bcond(h->c);
+  c3dk::rhs_xz(h->c);
+  c3dk::getuh_fused(h->c);
+  c3dk::rhsdp(h->c);
+  c3dk::getdp(h->c);
+  upcalc_impl(h, true);
+}
+
+static int buf_key(const c3d_handle* h) { return h->hh_cur * 2 + h->vcur; }
+
+static void graphs_free(c3d_handle* h) {
+  for (int p = 0; p < 2; p++) {
+    if (h->gexec[p]) (void)hipGraphExecDestroy(h->gexec[p]);
+    h->gexec[p] = nullptr;
+    h->gkey[p] = -1;
+  }
+}
+
+// Each step swaps the HH sets (and, in mixed mode, the V buffers), so the
+// kernel arguments repeat every two steps: two graphs, captured from the
+// current state; capturing advances the host-side buffer state by two steps,
+// which returns it to where it was.
+static int graphs_capture(c3d_handle* h) {
+  graphs_free(h);
+  const bool timing = h->kt.on;
+  h->kt.on = false;  // no event nodes in the graphs
+  for (int p = 0; p < 2; p++) {
+    const int key = buf_key(h);
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(h->c.st, hipStreamCaptureModeThreadLocal));
+    step_launch(h);
+    HIPCHK(hipStreamEndCapture(h->c.st, &g));
+    const hipError_t e = hipGraphInstantiate(&h->gexec[p], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      h->kt.on = timing;
+      graphs_free(h);
+      return fail(h, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    h->gkey[p] = key;
+  }
+  h->kt.on = timing;
+  return 0;
+}
+
+extern "C" int c3d_advance(c3d_handle* h, int nsteps, int use_graph) {
+  if (!h) return -1;
+  if (nsteps < 0) return fail(h, "nsteps must be >= 0");
+  for (int n = 0; n < nsteps; n++) {
+    if (!use_graph) {
+      step_launch(h);
+      continue;
+    }
+    int p = (h->gkey[0] == buf_key(h)) ? 0 : (h->gkey[1] == buf_key(h)) ? 1 : -1;
+    if (p < 0) {
+      if (graphs_capture(h)) return -1;
+      p = 0;
+    }
+    HIPCHK(hipGraphLaunch(h->gexec[p], h->c.st));
+    // the host-side buffer state follows the step the graph performed
+    swap_hh(h);
+    if (h->c.s.mixed) swap_v(h);
+  }
+  return launched(h);
+}
+
+extern "C" int c3d_sync(c3d_handle* h, double* divmax, double* cflm) {
+  if (!h) return -1;
+  HIPCHK(hipStreamSynchronize(h->c.st));
+  double dm = 0.0;
+  if (h->c.s.mixed) {
+    if (refine_impl(h, &dm)) return -1;
+  } else if (read_sc(h, SC_DIVMAX, &dm)) {
+    return -1;
+  }
+  if (divmax) *divmax = dm;
+  if (cflm && read_sc(h, SC_CFLM, cflm)) return -1;
+  return 0;
+}
+
 static double bits_to_double(double v) { return v; }
 
 extern "C" int c3d_step(c3d_handle* h, int ntime, double* divmax, double* cflm) {
@@ -1050,6 +1146,7 @@ extern "C" int c3d_set_dt(c3d_handle* h, double dt) {
   HIPCHK(hipStreamSynchronize(h->c.st));
   h->c.s.dt = dt;
   h->cfg.dt = dt;
+  graphs_free(h);
   return upload_dt_terms(h);
 }
 

@@ -1,0 +1,198 @@
+"""Serving many channel instances on one GPU: a continuous-batching request
+loop over resident states.
+
+A request asks for `steps` time steps of one channel (grid, Reynolds
+number, optional Smagorinsky + wall model, precision) from a seeded initial
+field.  The loop admits waiting requests while their HBM footprint
+(`c3d_footprint_bytes`) fits the budget and a slot is free, keeps every
+admitted state resident, and advances all live instances together: each turn
+queues up to `chunk` steps of every live instance on that instance's own HIP
+stream (`c3d_advance`, replaying captured graphs), then waits for each.
+Work of different instances therefore overlaps on the device, requests that
+arrive meanwhile are admitted at the next turn, and a finished instance
+frees its memory and slot at once.  Each instance's arithmetic does not
+depend on what else is resident, so a result is the same alone and in the
+mix.
+
+The reference runs one channel per process and has no counterpart; this is
+the ensemble/serving driver over the library's C-ABI.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+PI = math.pi
+
+
+def re_from_re_tau(re_tau: float) -> float:
+    """RE in the reference's units (centreline velocity 1, half height 1)
+    for a friction Reynolds number, via Dean's correlation
+    Re_tau = 0.09 Re_b^0.88 (Re_b on the full height and bulk velocity) and
+    RE = 0.75 Re_b (compact3d.f:627, :656)."""
+    re_b = (re_tau / 0.09) ** (1.0 / 0.88)
+    return 0.75 * re_b
+
+
+@dataclass
+class Request:
+    rid: int
+    grid: tuple          # (N1M, N2, N3M)
+    re: float
+    steps: int
+    seed: int = 0
+    precision: int = 0   # 0 FP64, 1 mixed
+    les: bool = False    # Smagorinsky SGS (FILES=2) + log-law wall model (WMODEL=1)
+    dt: float = 5e-4
+    cks: float = 1.5
+    t_arrive: float = 0.0
+    # filled in by the loop
+    t_admit: float = float("nan")
+    t_done: float = float("nan")
+    result: dict = field(default_factory=dict)
+
+    def config(self):
+        n1m, n2, n3m = self.grid
+        return dict(n1=n1m + 1, n2=n2, n3=n3m + 1, re=self.re, alx=2 * PI, aly=2.0, alz=PI, dt=self.dt,
+                    files=2.0 if self.les else 0.0, wmodel=1.0 if self.les else 0.0, cks=self.cks)
+
+    @property
+    def pts(self):
+        n1m, n2, n3m = self.grid
+        return n1m * (n2 - 1) * n3m
+
+
+class FieldCache:
+    """Seeded initial fields, one per (grid, seed): the divergence-free
+    perturbed Poiseuille field of compact3d_amd.initfield on the grid the
+    library builds."""
+
+    def __init__(self):
+        self._f = {}
+
+    def get(self, req: Request, mesh):
+        from .initfield import perturbed_poiseuille
+        key = (req.grid, req.seed, req.les, req.cks)
+        if key not in self._f:
+            cfg = req.config()
+            self._f[key] = perturbed_poiseuille(cfg["n1"], cfg["n2"], cfg["n3"], mesh["Y"], mesh["DX1"],
+                                                mesh["DX3"], mesh["ALY"], seed=req.seed)
+        return self._f[key]
+
+
+def _channel_factory(req: Request):
+    from . import c3d
+    return c3d.Channel(**req.config(), precision=req.precision)
+
+
+def _footprint(req: Request) -> int:
+    from . import c3d
+    return c3d.footprint_bytes(**req.config(), precision=req.precision)
+
+
+class Server:
+    """Continuous-batching loop.  `make_channel(req)` and `footprint(req)`
+    default to the library; tests may pass stand-ins."""
+
+    def __init__(self, budget_bytes: int, max_live: int = 64, chunk: int = 10, use_graph: bool = True,
+                 make_channel=None, footprint=None, fields: FieldCache | None = None, keep_results: bool = False,
+                 clock=time.perf_counter, sleep=time.sleep):
+        self.budget = int(budget_bytes)
+        self.max_live = int(max_live)
+        self.chunk = int(chunk)
+        self.use_graph = use_graph
+        self.make_channel = make_channel or _channel_factory
+        self.footprint = footprint or _footprint
+        self.fields = fields or FieldCache()
+        self.keep_results = keep_results
+        self.clock = clock
+        self.sleep = sleep
+        self.peak_bytes = 0
+
+    def _admit(self, req, now):
+        ch = self.make_channel(req)
+        U, P = self.fields.get(req, ch.mesh())
+        ch.upload(U, P)
+        req.t_admit = now
+        return dict(req=req, ch=ch, left=req.steps, bytes=self.footprint(req))
+
+    def run(self, requests, t0=None):
+        """Serve `requests` (t_arrive relative to the start) to completion;
+        returns the finished requests in completion order."""
+        pending = sorted(requests, key=lambda r: r.t_arrive)
+        for r in pending:
+            if self.footprint(r) > self.budget:
+                raise ValueError(f"request {r.rid} needs {self.footprint(r)} B, more than the budget")
+        waiting, live, done = [], [], []
+        used = 0
+        start = self.clock() if t0 is None else t0
+        while pending or waiting or live:
+            now = self.clock() - start
+            while pending and pending[0].t_arrive <= now:
+                waiting.append(pending.pop(0))
+            # admission in arrival order while memory and slots allow
+            while waiting and len(live) < self.max_live and used + self.footprint(waiting[0]) <= self.budget:
+                rec = self._admit(waiting.pop(0), self.clock() - start)
+                live.append(rec)
+                used += rec["bytes"]
+                self.peak_bytes = max(self.peak_bytes, used)
+            if not live:
+                if pending:
+                    self.sleep(max(0.0, pending[0].t_arrive - (self.clock() - start)))
+                continue
+            # one turn: queue work on every live instance, then wait for each
+            for rec in live:
+                n = min(self.chunk, rec["left"])
+                rec["ch"].advance(n, self.use_graph)
+                rec["n"] = n
+            for rec in live:
+                divmax, cflm = rec["ch"].sync()
+                rec["left"] -= rec["n"]
+                rec["divmax"], rec["cflm"] = divmax, cflm
+            still = []
+            for rec in live:
+                if rec["left"] > 0:
+                    still.append(rec)
+                    continue
+                req = rec["req"]
+                req.t_done = self.clock() - start
+                req.result = dict(divmax=rec["divmax"], cflm=rec["cflm"])
+                if self.keep_results:
+                    st = rec["ch"].download()
+                    req.result.update(U=st["U"], P=st["P"], PRESG=st["PRESG"])
+                rec["ch"].close()
+                used -= rec["bytes"]
+                done.append(req)
+            live = still
+        return done
+
+
+def stats(done, wall_s):
+    """p50/p99 latency (arrival to completion), throughput and the HBM
+    fraction of the compulsory state traffic (112 B per cell and step)."""
+    lat = np.array([r.t_done - r.t_arrive for r in done])
+    work = sum(r.pts * r.steps for r in done)
+    thr = work / wall_s / 1e6
+    return dict(requests=len(done), p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
+                mean_s=float(lat.mean()), throughput_mpts_steps_s=thr,
+                hbm_state_frac=112.0 * thr * 1e6 / 8e12, wall_s=wall_s)
+
+
+def poisson_requests(n, rate_per_s, grids, re_taus, steps_range, seed=4, precision=1):
+    """A seeded Poisson arrival stream: grids and Re_tau drawn uniformly;
+    Re_tau >= 1000 requests are wall-modelled LES (FP64)."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for q in range(n):
+        t += rng.exponential(1.0 / rate_per_s)
+        g = tuple(grids[rng.integers(len(grids))])
+        rt = re_taus[rng.integers(len(re_taus))]
+        les = rt >= 1000
+        steps = int(rng.integers(steps_range[0], steps_range[1] + 1))
+        out.append(Request(rid=q, grid=g, re=re_from_re_tau(rt), steps=steps, seed=q % 3,
+                           precision=0 if les else precision, les=les, t_arrive=t))
+    return out

@@ -80,6 +80,16 @@ int c3d_getup(c3d_handle *h, int ntime);
 /* GETUP + DIVCHECK + CFL as MAIN calls them after the step
  * (compact3d.f:130-138); divmax / cflm may be NULL. */
 int c3d_step(c3d_handle *h, int ntime, double *divmax, double *cflm);
+/* nsteps of MAIN's loop (GETUP with DIV, DIVMAX and CFL formed) queued on
+ * the handle's stream without waiting; use_graph != 0 replays two captured
+ * HIP graphs (even and odd steps) instead of launching kernel by kernel.
+ * No counterpart in the reference: it is the batched driver the ensemble
+ * and serving loops use.  Mixed mode skips the per-step refinement check;
+ * c3d_sync applies it to the last step. */
+int c3d_advance(c3d_handle *h, int nsteps, int use_graph);
+/* waits for the handle's stream and returns DIVMAX and CFLM of the last
+ * step (mixed mode: after the refinement check); either may be NULL */
+int c3d_sync(c3d_handle *h, double *divmax, double *cflm);
 
 /* Phase entry points of GETUP, for operator-level parity.  They act on the
  * resident state exactly as the named reference routines do. */

@@ -1,0 +1,114 @@
+"""The serving loop's scheduling on the CPU, with stand-in channels: admission
+by footprint and slots, continuous admission of late arrivals, latency
+bookkeeping, and refusal of requests larger than the budget."""
+import pytest
+
+from compact3d_amd.serve import Request, Server, poisson_requests, re_from_re_tau, stats
+
+
+class FakeClock:
+    def __init__(self):
+        self.t = 0.0
+
+    def __call__(self):
+        return self.t
+
+    def sleep(self, dt):
+        self.t += max(dt, 1e-6)
+
+
+class FakeChannel:
+    """advance() costs 1 ms of clock per step; records what ran"""
+
+    live = 0
+    peak = 0
+
+    def __init__(self, req, clock, log):
+        self.req, self.clock, self.log = req, clock, log
+        FakeChannel.live += 1
+        FakeChannel.peak = max(FakeChannel.peak, FakeChannel.live)
+        self.steps = 0
+
+    def mesh(self):
+        return None
+
+    def upload(self, U, P):
+        pass
+
+    def advance(self, n, use_graph=True):
+        self.clock.t += 1e-3 * n
+        self.steps += n
+        self.log.append((self.req.rid, n))
+
+    def sync(self):
+        return 1e-14, 0.1
+
+    def close(self):
+        FakeChannel.live -= 1
+
+
+class NoFields:
+    def get(self, req, mesh):
+        return None, None
+
+
+def _server(budget, max_live, chunk=10):
+    clock = FakeClock()
+    log = []
+    srv = Server(budget, max_live=max_live, chunk=chunk, make_channel=lambda r: FakeChannel(r, clock, log),
+                 footprint=lambda r: r.pts, fields=NoFields(), clock=clock, sleep=clock.sleep)
+    return srv, clock, log
+
+
+def test_every_request_finishes_with_its_steps():
+    FakeChannel.live = FakeChannel.peak = 0
+    reqs = [Request(rid=q, grid=(8, 9, 8), re=100.0, steps=25 + 5 * q, t_arrive=0.01 * q) for q in range(6)]
+    srv, clock, log = _server(budget=10 ** 9, max_live=4)
+    done = srv.run(reqs)
+    assert sorted(r.rid for r in done) == list(range(6))
+    ran = {}
+    for rid, n in log:
+        ran[rid] = ran.get(rid, 0) + n
+    assert all(ran[r.rid] == r.steps for r in reqs)
+    assert FakeChannel.peak <= 4 and FakeChannel.live == 0
+    for r in done:
+        assert r.t_arrive <= r.t_admit <= r.t_done
+
+
+def test_admission_respects_the_memory_budget():
+    FakeChannel.live = FakeChannel.peak = 0
+    big = Request(rid=0, grid=(8, 9, 8), re=100.0, steps=30, t_arrive=0.0)      # 512 "bytes"
+    reqs = [big] + [Request(rid=q, grid=(8, 9, 8), re=100.0, steps=10, t_arrive=0.0) for q in range(1, 4)]
+    srv, clock, log = _server(budget=2 * big.pts, max_live=64)
+    done = srv.run(reqs)
+    assert srv.peak_bytes <= 2 * big.pts
+    assert FakeChannel.peak == 2
+    assert len(done) == 4
+
+
+def test_late_arrivals_join_while_others_run():
+    FakeChannel.live = FakeChannel.peak = 0
+    long = Request(rid=0, grid=(8, 9, 8), re=100.0, steps=200, t_arrive=0.0)
+    late = Request(rid=1, grid=(8, 9, 8), re=100.0, steps=10, t_arrive=0.05)
+    srv, clock, log = _server(budget=10 ** 9, max_live=8)
+    done = srv.run([long, late])
+    assert [r.rid for r in done] == [1, 0]  # the short late request finishes first
+    assert late.t_admit < long.t_done
+
+
+def test_oversized_request_is_refused():
+    srv, clock, log = _server(budget=10, max_live=8)
+    with pytest.raises(ValueError):
+        srv.run([Request(rid=0, grid=(8, 9, 8), re=100.0, steps=5)])
+
+
+def test_stats_and_request_stream():
+    reqs = poisson_requests(50, 20.0, [(64, 49, 64), (96, 65, 96)], [180, 395, 1000], (20, 40), seed=4)
+    assert all(b.t_arrive >= a.t_arrive for a, b in zip(reqs, reqs[1:]))
+    assert all(r.precision == (0 if r.les else 1) for r in reqs)
+    assert abs(re_from_re_tau(180.0) - 4200) / 4200 < 0.05  # the reference's RE for Re_tau 180
+    for r in reqs:
+        r.t_done = r.t_arrive + 0.5
+    s = stats(reqs, 10.0)
+    assert s["p50_s"] == pytest.approx(0.5) and s["p99_s"] == pytest.approx(0.5)
+    assert s["throughput_mpts_steps_s"] > 0
