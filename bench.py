@@ -35,9 +35,11 @@ METRIC = "Mpts·steps/s mixed vs FP64 on 1 GPU @ div≤1e-9; p50/p99 served late
 ALG_BYTES = {
     # compulsory bytes per pressure cell of each kernel (every distinct field
     # read once, written once); adi_y is the ADI y-sweep with the RHS y terms
-    # fused in (per component: state, VV1/VV3 or P, RUH, HH, HH2 in; HH, UH out)
+    # fused in (per component: state, VV1/VV3 or P, RUH, HH, HH2 in; HH, UH out);
+    # the spectral kernels move (NX/2+1)/NX ~ 1/2 a complex per cell, pois_y
+    # also its two per-mode factor tables
     "rhs_x": 88, "rhs_z": 112, "rhs_zw": 40, "rhs_x2": 24, "adi_y": 168, "adi_z": 64,
-    "rhsdp_x": 24, "rhsdp_z": 24, "fft_x": 16, "fft_z": 16, "pois_y": 16, "fft_xinv": 16,
+    "rhsdp_x": 24, "rhsdp_z": 24, "fft_x": 16, "fft_z": 16, "pois_y": 24, "fft_xinv": 16,
     "flow_red": 40, "upd_x": 48, "upd_z": 64,
 }
 # adi_x runs twice per step: (u,v) read+write 32 B, then w with U added 24 B
@@ -47,7 +49,7 @@ ALG_BYTES_ADI_X_PER_STEP = 56
 # writes the FP64 state and writes the shadows)
 ALG_BYTES_MIXED = {
     "rhs_x": 44, "rhs_z": 56, "rhs_zw": 20, "rhs_x2": 12, "adi_y": 84, "adi_z": 24,
-    "rhsdp_x": 24, "rhsdp_z": 20, "fft_x": 16, "fft_z": 16, "pois_y": 16, "fft_xinv": 16,
+    "rhsdp_x": 24, "rhsdp_z": 20, "fft_x": 16, "fft_z": 16, "pois_y": 24, "fft_xinv": 16,
     "flow_red": 32, "upd_x": 64, "upd_z": 92,
 }
 

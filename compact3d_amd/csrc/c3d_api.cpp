@@ -432,7 +432,7 @@ long long c3d_footprint_bytes(const c3d_config* cfg) {
   // mode); intermediates 10 + HH 6; FP32 shadows 4 in mixed mode; LES 7
   long long bytes = (10 + (mixed ? 1 : 0)) * fsz * 8 + 16 * fsz * es + (mixed ? 4 * fsz * 4 : 0);
   if (cfg->files != 0.0) bytes += 7 * fsz * 8;
-  const long long nm = (nx / 2 + 1) * nz * n2m;
+  const long long nm = ((nx / 2 + 1 + 7) / 8 * 8) * nz * n2m;
   bytes += nm * 16 + nm * 16;  // spectrum + TDMAP factors
   return bytes + 4 * nx * nz * 8 + (1 << 20);
 }
@@ -469,6 +469,7 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   g.n2m = cf.n2 - 1;
   g.nj = cf.n2 + 1;
   g.nxh = g.nx / 2 + 1;
+  g.nxhp = (g.nxh + 7) / 8 * 8;
   g.plane = (long long)g.nj * g.nx;
   g.fsz = g.plane * g.nz;
   if (g.nx % 2 || g.nz % 2) return bad("N1-1 and N3-1 must be even");
@@ -627,7 +628,7 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   if (dalloc(h, &F.ubc_u, n2d) || dalloc(h, &F.ubc_w, n2d) || dalloc(h, &F.tauw, n2d) ||
       dalloc(h, &F.taub, n2d))
     return bad(h->err);
-  if (dalloc(h, &F.spec, (size_t)g.nxh * g.nz * g.n2m)) return bad(h->err);
+  if (dalloc(h, &F.spec, (size_t)g.nxhp * g.nz * g.n2m)) return bad(h->err);
   if (cf.files != 0.0) {
     double** les[] = {&F.nux, &F.nuz, &F.nue, &F.dvdx, &F.dwdx, &F.dudz, &F.dvdz};
     for (auto pp : les)
@@ -635,7 +636,7 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   }
   c3dk::init_attrs();
   {
-    const size_t nmj = (size_t)g.nxh * g.nz * g.n2m;
+    const size_t nmj = (size_t)g.nxhp * g.nz * g.n2m;
     if (dalloc(h, &F.pib, nmj) || dalloc(h, &F.pgt, nmj)) return bad(h->err);
     c3dk::pois_table(c);
     if (hipStreamSynchronize(c.st) != hipSuccess || hipGetLastError() != hipSuccess)

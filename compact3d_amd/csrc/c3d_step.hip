@@ -826,7 +826,8 @@ __global__ __launch_bounds__(512) void k_rhsdp_z(Ctx c) {
 // The plane DFT of :9325-9346 is factored as a real x-DFT (two rows packed
 // into one complex transform: z = a + i b, A[k] = (Z[k] + conj Z[n-k])/2,
 // B[k] = (Z[k] - conj Z[n-k])/(2i)), keeping kx = 0..NX/2, then complex
-// z-DFTs of that half spectrum.  Spectrum layout [jm][kz][kx].
+// z-DFTs of that half spectrum.  Spectrum layout [jm][kz][kx], rows of
+// NXHP >= NX/2+1 (padded to 128 bytes).
 template <typename T, int R>
 __device__ __forceinline__ void dft_r(cplx_t<T> (&v)[R], int sign) {
   if (R == 2) {
@@ -959,11 +960,11 @@ __global__ __launch_bounds__(256) void k_fft_x(Ctx c) {
       const zc vb = {(T)0.5 * (z1.y + z2.y), (T)-0.5 * (z1.x - z2.x)};
       if (la < nlines) {
         const int jm = la % g.n2m, k = la / g.n2m;
-        ((zc*)c.d.spec)[((long long)jm * g.nz + k) * nxh + kx] = va;
+        ((zc*)c.d.spec)[((long long)jm * g.nz + k) * g.nxhp + kx] = va;
       }
       if (lb < nlines) {
         const int jm = lb % g.n2m, k = lb / g.n2m;
-        ((zc*)c.d.spec)[((long long)jm * g.nz + k) * nxh + kx] = vb;
+        ((zc*)c.d.spec)[((long long)jm * g.nz + k) * g.nxhp + kx] = vb;
       }
     }
   }
@@ -982,12 +983,12 @@ __global__ __launch_bounds__(256) void k_fft_z(Ctx c, int sign, void* spec) {
   const int nit = (g.nxh + L - 1) / L;
   const int it = blockIdx.x % nit;
   const int jm = blockIdx.x / nit;
-  zc* S = (zc*)spec + (long long)jm * g.nz * g.nxh;
+  zc* S = (zc*)spec + (long long)jm * g.nz * g.nxhp;
   const int tot = n * L;
   for (int e = threadIdx.x; e < tot; e += blockDim.x) {
     const int l = e % L, p = e / L;
     const int kx = it * L + l;
-    A[p * LS + l] = kx < g.nxh ? S[(long long)p * g.nxh + kx] : zc{(T)0, (T)0};
+    A[p * LS + l] = kx < g.nxh ? S[(long long)p * g.nxhp + kx] : zc{(T)0, (T)0};
   }
   __syncthreads();
   const zc* tw = (const zc*)(sign < 0 ? c.f.twzf : c.f.twzb);
@@ -995,7 +996,7 @@ __global__ __launch_bounds__(256) void k_fft_z(Ctx c, int sign, void* spec) {
   for (int e = threadIdx.x; e < tot; e += blockDim.x) {
     const int l = e % L, p = e / L;
     const int kx = it * L + l;
-    if (kx < g.nxh) S[(long long)p * g.nxh + kx] = R[p * LS + l];
+    if (kx < g.nxh) S[(long long)p * g.nxhp + kx] = R[p * LS + l];
   }
 }
 
@@ -1027,8 +1028,8 @@ __global__ __launch_bounds__(256) void k_pois_y(Ctx c, const void* spec_in, void
   const int kz = blockIdx.x / nit;
   const int kx = it * L + l;
   const bool ok = kx < g.nxh;
-  const long long jstr = (long long)g.nz * g.nxh;
-  const long long b = (long long)kz * g.nxh + (ok ? kx : 0);
+  const long long jstr = (long long)g.nz * g.nxhp;
+  const long long b = (long long)kz * g.nxhp + (ok ? kx : 0);
   const zi* Si = (const zi*)spec_in;
   zo* So = (zo*)spec_out;
   const double* IB = c.d.pib;
@@ -1069,15 +1070,16 @@ __global__ __launch_bounds__(256) void k_pois_y(Ctx c, const void* spec_in, void
 __global__ void k_pois_table(Ctx c) {
   const Geo g = c.g;
   const long long nm = (long long)g.nxh * g.nz;
-  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (t >= nm) return;
-  const int kx = t % g.nxh, kz = t / g.nxh;
+  const long long tm = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (tm >= nm) return;
+  const int kx = tm % g.nxh, kz = tm / g.nxh;
+  const long long t = (long long)kz * g.nxhp + kx;  // the spectrum's layout
   const Metrics M = c.m;
   const int kk = (kz <= g.nz / 2) ? kz : g.nz - kz;  // AK3 is even in kz
   const double ak = M.ak3[kk + 1];
   const double a1 = M.ak1[kx + 1];
   const bool pin = (kx == 0 && kz == 0);
-  const long long jstr = (long long)g.nz * g.nxh;
+  const long long jstr = (long long)g.nz * g.nxhp;
   double bet = pin ? 1.0 : (M.pcj[1] - ak - a1);
   c.d.pib[t] = 1.0 / bet;
   for (int jm = 1; jm < g.n2m; jm++) {
@@ -1110,8 +1112,8 @@ __global__ __launch_bounds__(256) void k_fft_xinv(Ctx c, const void* spec) {
   const int nxh = g.nxh;
   for (int l = 0; l < NL; l++) {
     const int la = ln0 + l, lb = ln0 + NL + l;
-    const long long sa = la < nlines ? ((long long)(la % g.n2m) * g.nz + la / g.n2m) * nxh : -1;
-    const long long sb = lb < nlines ? ((long long)(lb % g.n2m) * g.nz + lb / g.n2m) * nxh : -1;
+    const long long sa = la < nlines ? ((long long)(la % g.n2m) * g.nz + la / g.n2m) * g.nxhp : -1;
+    const long long sb = lb < nlines ? ((long long)(lb % g.n2m) * g.nz + lb / g.n2m) * g.nxhp : -1;
     for (int kx = threadIdx.x; kx < n; kx += blockDim.x) {
       const bool lo = kx < nxh;
       const int kk = lo ? kx : n - kx;
