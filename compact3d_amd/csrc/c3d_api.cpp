@@ -144,12 +144,12 @@ static OpSet make_opset(int seg, double adi_a, double adi_b) {
   return o;
 }
 
-// y tiles: C chunks of at most 16 points (C a power of two) and L columns,
+// y tiles: C chunks of at most `ch` points (C a power of two) and L columns,
 // L*C <= nt; L a multiple of `row` columns where possible
-static YShape make_yshape(int m, int nt, int row) {
+static YShape make_yshape(int m, int nt, int row, int ch = 12) {
   YShape y;
   y.C = 1;
-  while (y.C * 16 < m) y.C *= 2;
+  while (y.C * ch < m) y.C *= 2;
   y.L = std::max(1, std::min(64, nt / y.C));
   if (y.L >= row) y.L -= y.L % row;
   return y;
@@ -469,7 +469,7 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   g.n2m = cf.n2 - 1;
   g.nj = cf.n2 + 1;
   g.nxh = g.nx / 2 + 1;
-  g.nxhp = (g.nxh + 7) / 8 * 8;
+  g.nxhp = (g.nxh + 15) / 16 * 16;
   g.plane = (long long)g.nj * g.nx;
   g.fsz = g.plane * g.nz;
   if (g.nx % 2 || g.nz % 2) return bad("N1-1 and N3-1 must be even");
@@ -484,7 +484,7 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   f.tlx = std::max(1, 2048 / g.nx);
   f.tlz = std::max(1, 2048 / g.nz);
   f.tly = std::max(1, std::min(32, pow2_floor((int)(65536 / (24 * (size_t)g.n2m)))));
-  if (g.n2m > 4096) return bad("N2 too large for the y tiles");
+  if (g.n2m > 3072) return bad("N2 too large for the y tiles (N2-1 <= 3072)");
   build_mesh(h);
   Ops& o = c.o;
   if (!make_lshape(g.nx, true, sizeof(double), o.lx) || !make_lshape(g.nz, false, sizeof(double), o.lz))
@@ -492,7 +492,9 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   if (!make_lshape32(g.nz, o.lz32)) return bad("N3-1 too long for the FP32 line tiles");
   o.ys = make_yshape(g.n2m, 256, 16);
   o.ys32 = make_yshape(g.n2m, 512, 32);
-  o.yp = make_yshape(g.n2m, 256, 16);
+  // Poisson: 8 modes of 16 bytes per row and chunks of 6 points keep its
+  // three FP64 chunk arrays small; 12-point chunks beyond N2-1 = 1536
+  o.yp = g.n2m <= 6 * 256 ? make_yshape(g.n2m, 256, 8, 6) : make_yshape(g.n2m, 256, 16);
   o.s6 = make_solve(1.0, 6.0);
   o.s10 = make_solve(1.0, 10.0);
   o.s22 = make_solve(1.0, 22.0);

@@ -15,7 +15,7 @@
 
 struct Geo {
   int nx, nz, n2, n2m, nj, nxh;  // nxh = NX/2+1 (half spectrum)
-  int nxhp;                      // spectrum row stride: nxh rounded up to 8 (128-byte rows)
+  int nxhp;                      // spectrum row stride: nxh rounded up to 16 (128-byte rows of the complex spectrum and of the FP64 factor tables)
   long long plane;               // NJ*NX
   long long fsz;                 // NZ*NJ*NX
 };

@@ -391,23 +391,27 @@ __global__ __launch_bounds__(256) void k_rhs_x2(Ctx c) {
 // of the host, as a partitioned recurrence: thread (l, c) holds chunk c of
 // column i = i0 + l.  Forward x(J) = BET(J)(R(J) - A(J) x(J-1)) is
 // y = c + m y_prev with m = -A BET; backward x(J) -= GAM(J) x(J+1).
-#define CHMAX 16
+#define CHMAX 12
 struct YPos {
   int l, c, j0, len;
 };
 
-// chunk-parallel linear recurrence v[q] <- v[q] + m[q] * v[q-1] (forward)
-template <typename T>
-__device__ __forceinline__ void ychain_fwd(T (&v)[CHMAX], const T (&m)[CHMAX], const YPos& y, int C, int L,
-                                           T* xa, T* xm) {
-  T r = (T)0, P = (T)1, Pq[CHMAX];
+// chunk-parallel linear recurrence v[q] <- v[q] + m(q) * v[q-1] (forward):
+// each chunk runs it from zero, the chunks' end values and multiplier
+// products are combined across the block through LDS, and the incoming
+// value Y reaches point q of the chunk as Y * m(0) * ... * m(q).  m(q) is
+// evaluated twice, so a multiplier held in LDS or recomputed costs no
+// registers across the barrier.
+template <typename T, int N, typename MF>
+__device__ __forceinline__ void ychain_fwd(T (&v)[N], MF m, const YPos& y, int L, T* xa, T* xm) {
+  T r = (T)0, P = (T)1;
 #pragma unroll
-  for (int q = 0; q < CHMAX; q++)
+  for (int q = 0; q < N; q++)
     if (q < y.len) {
-      r = v[q] + m[q] * r;
+      const T mq = m(q);
+      r = v[q] + mq * r;
       v[q] = r;
-      P *= m[q];
-      Pq[q] = P;
+      P *= mq;
     }
   xa[y.c * L + y.l] = r;
   xm[y.c * L + y.l] = P;
@@ -415,21 +419,23 @@ __device__ __forceinline__ void ychain_fwd(T (&v)[CHMAX], const T (&m)[CHMAX], c
   T Y = (T)0;
   for (int cc = 0; cc < y.c; cc++) Y = xa[cc * L + y.l] + xm[cc * L + y.l] * Y;
 #pragma unroll
-  for (int q = 0; q < CHMAX; q++)
-    if (q < y.len) v[q] += Pq[q] * Y;
-}
-// backward: v[q] <- v[q] + m[q] * v[q+1]
-template <typename T>
-__device__ __forceinline__ void ychain_bwd(T (&v)[CHMAX], const T (&m)[CHMAX], const YPos& y, int C, int L,
-                                           T* xa, T* xm) {
-  T r = (T)0, P = (T)1, Pq[CHMAX];
-#pragma unroll
-  for (int q = CHMAX - 1; q >= 0; q--)
+  for (int q = 0; q < N; q++)
     if (q < y.len) {
-      r = v[q] + m[q] * r;
+      Y *= m(q);
+      v[q] += Y;
+    }
+}
+// backward: v[q] <- v[q] + m(q) * v[q+1]
+template <typename T, int N, typename MF>
+__device__ __forceinline__ void ychain_bwd(T (&v)[N], MF m, const YPos& y, int C, int L, T* xa, T* xm) {
+  T r = (T)0, P = (T)1;
+#pragma unroll
+  for (int q = N - 1; q >= 0; q--)
+    if (q < y.len) {
+      const T mq = m(q);
+      r = v[q] + mq * r;
       v[q] = r;
-      P *= m[q];
-      Pq[q] = P;
+      P *= mq;
     }
   xa[y.c * L + y.l] = r;
   xm[y.c * L + y.l] = P;
@@ -437,13 +443,16 @@ __device__ __forceinline__ void ychain_bwd(T (&v)[CHMAX], const T (&m)[CHMAX], c
   T Z = (T)0;
   for (int cc = C - 1; cc > y.c; cc--) Z = xa[cc * L + y.l] + xm[cc * L + y.l] * Z;
 #pragma unroll
-  for (int q = 0; q < CHMAX; q++)
-    if (q < y.len) v[q] += Pq[q] * Z;
+  for (int q = N - 1; q >= 0; q--)
+    if (q < y.len) {
+      Z *= m(q);
+      v[q] += Z;
+    }
 }
 
 template <typename T>
 __global__ __launch_bounds__(512) void k_adi_y(Ctx c) {
-  __shared__ T xs[4][512];
+  __shared__ T xs[4][sizeof(T) == 4 ? 512 : 256];
   const Geo g = c.g;
   const YShape ys = sizeof(T) == 4 ? c.o.ys32 : c.o.ys;
   const int comp = blockIdx.y;
@@ -471,23 +480,17 @@ __global__ __launch_bounds__(512) void k_adi_y(Ctx c) {
   const double* GAM = comp == 0 ? M.ygam[0] : comp == 1 ? M.ygam[1] : M.ygam[2];
   const T dt = (T)c.s.dt;
   const long long b = gidx(g, ok ? i : 0, 0, k);
-  T v[CHMAX], mm[CHMAX];
+  T v[CHMAX];
 #pragma unroll
   for (int q = 0; q < CHMAX; q++)
     if (q < y.len) {
       const int J = y.j0 + q;
-      const T bet = (T)BET[J];
-      v[q] = bet * (R[b + (long long)J * g.nx] * dt);
-      mm[q] = (J == ns) ? (T)0 : -(T)A[J] * bet;
+      v[q] = (T)BET[J] * (R[b + (long long)J * g.nx] * dt);
     }
-  ychain_fwd(v, mm, y, C, L, xs[0], xs[1]);
-#pragma unroll
-  for (int q = 0; q < CHMAX; q++)
-    if (q < y.len) {
-      const int J = y.j0 + q;
-      mm[q] = (J == g.n2m) ? (T)0 : -(T)GAM[J];
-    }
-  ychain_bwd(v, mm, y, C, L, xs[2], xs[3]);
+  ychain_fwd(
+      v, [&](int q) { const int J = y.j0 + q; return (J == ns) ? (T)0 : -(T)A[J] * (T)BET[J]; }, y, L, xs[0], xs[1]);
+  ychain_bwd(
+      v, [&](int q) { const int J = y.j0 + q; return (J == g.n2m) ? (T)0 : -(T)GAM[J]; }, y, C, L, xs[2], xs[3]);
   if (ok)
 #pragma unroll
     for (int q = 0; q < CHMAX; q++)
@@ -646,9 +649,9 @@ __device__ __forceinline__ void yfuse_w(const Ctx& c, const double* tb, long lon
     }
 }
 
-template <typename T>
+template <typename T, bool SGS>
 __global__ __launch_bounds__(512) void k_rhsy_adiy(Ctx c, int staged) {
-  __shared__ T xs[4][512];
+  __shared__ T xs[4][sizeof(T) == 4 ? 512 : 256];
   extern __shared__ __align__(16) unsigned char smem[];
   const Geo g = c.g;
   const YShape ys = sizeof(T) == 4 ? c.o.ys32 : c.o.ys;
@@ -683,11 +686,11 @@ __global__ __launch_bounds__(512) void k_rhsy_adiy(Ctx c, int staged) {
     __syncthreads();
     tb = st;
   }
-  T v[CHMAX], mm[CHMAX];
+  T v[CHMAX];
 #pragma unroll
   for (int q = 0; q < CHMAX; q++) v[q] = (T)0;
   if (ok && y.len > 0) {
-    if (c.s.files) {
+    if constexpr (SGS) {
 #pragma unroll
       for (int q = 0; q < CHMAX; q++)
         if (q < y.len) {
@@ -705,14 +708,8 @@ __global__ __launch_bounds__(512) void k_rhsy_adiy(Ctx c, int staged) {
       yfuse_w<T>(c, tb, b, t2, y, v);
     }
   }
-#pragma unroll
-  for (int q = 0; q < CHMAX; q++)
-    if (q < y.len) mm[q] = (T)tb[(y.j0 + q) * YTAB_W + YT_M];
-  ychain_fwd(v, mm, y, C, L, xs[0], xs[1]);
-#pragma unroll
-  for (int q = 0; q < CHMAX; q++)
-    if (q < y.len) mm[q] = (T)tb[(y.j0 + q) * YTAB_W + YT_G];
-  ychain_bwd(v, mm, y, C, L, xs[2], xs[3]);
+  ychain_fwd(v, [&](int q) { return (T)tb[(y.j0 + q) * YTAB_W + YT_M]; }, y, L, xs[0], xs[1]);
+  ychain_bwd(v, [&](int q) { return (T)tb[(y.j0 + q) * YTAB_W + YT_G]; }, y, C, L, xs[2], xs[3]);
   if (ok)
 #pragma unroll
     for (int q = 0; q < CHMAX; q++)
@@ -1007,8 +1004,9 @@ __global__ __launch_bounds__(256) void k_fft_z(Ctx c, int sign, void* spec) {
 // depend on the mode only: IBET(J) = 1/BET(J) and GAM(J+1) are tabulated at
 // set-up (k_pois_table); here each thread (l, c) solves chunk c of mode
 // kx = kx0 + l by the partitioned recurrences.
-// Spectrum in TI, recurrences and result in TO (FP64 in both modes).
-template <typename TI, typename TO>
+// Spectrum in TI, recurrences and result in TO (FP64 in both modes); chunks
+// of at most CH points.
+template <typename TI, typename TO, int CH>
 __global__ __launch_bounds__(256) void k_pois_y(Ctx c, const void* spec_in, void* spec_out) {
   __shared__ TO xs[8][256];
   typedef cplx_t<TI> zi;
@@ -1037,9 +1035,9 @@ __global__ __launch_bounds__(256) void k_pois_y(Ctx c, const void* spec_in, void
   const double* pmj = c.m.pmj;
   const bool selfconj = (kx == 0 || 2 * kx == g.nx) && (kz == 0 || 2 * kz == g.nz);
   const bool pin = (kx == 0 && kz == 0);
-  TO vr[CHMAX], vi[CHMAX], mm[CHMAX];
+  TO vr[CH], vi[CH], mm[CH];
 #pragma unroll
-  for (int q = 0; q < CHMAX; q++)
+  for (int q = 0; q < CH; q++)
     if (q < y.len) {
       const int jm = y.j0 + q;
       const long long a = jm * jstr + b;
@@ -1049,19 +1047,20 @@ __global__ __launch_bounds__(256) void k_pois_y(Ctx c, const void* spec_in, void
       vi[q] = selfconj ? (TO)0 : (TO)r.y * ib;
       mm[q] = (jm == 0) ? (TO)0 : -(TO)pmj[jm + 1] * ib;
     }
-  ychain_fwd(vr, mm, y, C, L, xs[0], xs[1]);
-  ychain_fwd(vi, mm, y, C, L, xs[2], xs[3]);
+  auto mf = [&](int q) { return mm[q]; };
+  ychain_fwd(vr, mf, y, L, xs[0], xs[1]);
+  ychain_fwd(vi, mf, y, L, xs[2], xs[3]);
 #pragma unroll
-  for (int q = 0; q < CHMAX; q++)
+  for (int q = 0; q < CH; q++)
     if (q < y.len) {
       const int jm = y.j0 + q;
       mm[q] = (jm == g.n2m - 1) ? (TO)0 : -(TO)GT[jm * jstr + b];
     }
-  ychain_bwd(vr, mm, y, C, L, xs[4], xs[5]);
-  ychain_bwd(vi, mm, y, C, L, xs[6], xs[7]);
+  ychain_bwd(vr, mf, y, C, L, xs[4], xs[5]);
+  ychain_bwd(vi, mf, y, C, L, xs[6], xs[7]);
   if (ok)
 #pragma unroll
-    for (int q = 0; q < CHMAX; q++)
+    for (int q = 0; q < CH; q++)
       if (q < y.len) So[(y.j0 + q) * jstr + b] = zo{vr[q], vi[q]};
 }
 
@@ -1386,8 +1385,12 @@ static void getuh_t(const Ctx& c, bool fused) {
     const size_t tbytes = sizeof(double) * (size_t)(c.g.n2 + 2) * YTAB_W;
     const int staged = tbytes <= 40 * 1024 ? 1 : 0;
     KScope ks_(c, KID_ADI_Y);
-    hipLaunchKernelGGL(k_rhsy_adiy<T>, dim3(ylx * c.g.nz, 3), dim3(ys.L * ys.C), staged ? tbytes : 0, c.st, c,
-                       staged);
+    if (c.s.files)
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rhsy_adiy<T, true>), dim3(ylx * c.g.nz, 3), dim3(ys.L * ys.C),
+                         staged ? tbytes : 0, c.st, c, staged);
+    else
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rhsy_adiy<T, false>), dim3(ylx * c.g.nz, 3), dim3(ys.L * ys.C),
+                         staged ? tbytes : 0, c.st, c, staged);
   } else {
     KScope ks_(c, KID_ADI_Y);
     hipLaunchKernelGGL(k_adi_y<T>, dim3(ylx * c.g.nz, 3), dim3(ys.L * ys.C), 0, c.st, c);
@@ -1444,8 +1447,12 @@ static void getdp_fp64(const Ctx& c) {
   { KScope ks_(c, KID_FFT_Z); hipLaunchKernelGGL(k_fft_z<T>, dim3(gz), dim3(256), ldsz, c.st, c, -1, sp); }
   {
     KScope ks_(c, KID_POIS_Y);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pois_y<T, T>), dim3(gy), dim3(c.o.yp.L * c.o.yp.C), 0, c.st, c,
-                       (const void*)sp, sp);
+    if ((g.n2m + c.o.yp.C - 1) / c.o.yp.C <= 6)
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pois_y<T, T, 6>), dim3(gy), dim3(c.o.yp.L * c.o.yp.C), 0, c.st, c,
+                         (const void*)sp, sp);
+    else
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pois_y<T, T, CHMAX>), dim3(gy), dim3(c.o.yp.L * c.o.yp.C), 0, c.st,
+                         c, (const void*)sp, sp);
   }
   { KScope ks_(c, KID_FFT_Z); hipLaunchKernelGGL(k_fft_z<T>, dim3(gz), dim3(256), ldsz, c.st, c, +1, sp); }
   { KScope ks_(c, KID_FFT_XINV); hipLaunchKernelGGL(k_fft_xinv<T>, dim3(gx), dim3(256), ldsx, c.st, c, (const void*)sp); }
