@@ -874,126 +874,137 @@ __device__ __forceinline__ void dft_r(cplx_t<T> (&v)[R], int sign) {
   }
 }
 
-// one Stockham stage of radix R over nl lines in LDS (a -> b)
-template <typename T, int R>
-__device__ __forceinline__ void fft_stage(const cplx_t<T>* a, cplx_t<T>* b, int n, int nl, int ls, int ns,
-                                          const cplx_t<T>* tw, int sign) {
+// one Stockham stage of radix R over nl lines of n points held in LDS at
+// a[p * ls + l], in place: every thread first gathers its butterflies into
+// registers (n*nl <= NE complex values per block of FFT_THREADS
+// threads), then all write back after a barrier.  One buffer instead of two
+// halves the LDS of a block.
+#define FFT_THREADS 512
+template <typename T, int R, int NE>
+__device__ __forceinline__ void fft_stage(cplx_t<T>* a, int n, int nl, int ls, int ns, const cplx_t<T>* tw,
+                                          int sign) {
+  constexpr int IT = (NE / R + FFT_THREADS - 1) / FFT_THREADS;
   const int nb = n / R;
   const int tot = nb * nl;
   const int tq = n / (ns * R);
-  for (int e = threadIdx.x; e < tot; e += blockDim.x) {
-    const int l = e % nl;
-    const int jb = e / nl;
-    const int k = jb % ns;
-    const int ts = k * tq;  // twiddle step; ts*r < n
-    cplx_t<T> v[R];
+  cplx_t<T> v[IT][R];
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-      const cplx_t<T> x = a[(jb + r * nb) * ls + l];
-      v[r] = (r == 0) ? x : cmul(x, tw[ts * r]);
+  for (int it = 0; it < IT; it++) {
+    const int e = threadIdx.x + it * FFT_THREADS;
+    if (e < tot) {
+      const int l = e % nl;
+      const int jb = e / nl;
+      const int ts = (jb % ns) * tq;  // twiddle step; ts*r < n
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const cplx_t<T> x = a[(jb + r * nb) * ls + l];
+        v[it][r] = (r == 0) ? x : cmul(x, tw[ts * r]);
+      }
+      dft_r<T, R>(v[it], sign);
     }
-    dft_r<T, R>(v, sign);
-    const int d = (jb / ns) * ns * R + k;
-#pragma unroll
-    for (int r = 0; r < R; r++) b[(d + r * ns) * ls + l] = v[r];
   }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < IT; it++) {
+    const int e = threadIdx.x + it * FFT_THREADS;
+    if (e < tot) {
+      const int l = e % nl;
+      const int jb = e / nl;
+      const int d = (jb / ns) * ns * R + jb % ns;
+#pragma unroll
+      for (int r = 0; r < R; r++) a[(d + r * ns) * ls + l] = v[it][r];
+    }
+  }
+  __syncthreads();
 }
 
-// mixed-radix transform; radices packed 3 bits each in `code`, `nrad` stages
-template <typename T>
-__device__ __forceinline__ cplx_t<T>* fft_lines(cplx_t<T>* a, cplx_t<T>* b, int n, int nl, int ls, int code, int nrad,
-                                const cplx_t<T>* tw, int sign) {
+// mixed-radix transform in place; radices packed 3 bits each in `code`,
+// `nrad` stages
+template <typename T, int NE>
+__device__ __forceinline__ void fft_lines(cplx_t<T>* a, int n, int nl, int ls, int code, int nrad,
+                                          const cplx_t<T>* tw, int sign) {
   int ns = 1;
   for (int s = 0; s < nrad; s++) {
     const int R = (code >> (3 * s)) & 7;
     if (R == 4)
-      fft_stage<T, 4>(a, b, n, nl, ls, ns, tw, sign);
+      fft_stage<T, 4, NE>(a, n, nl, ls, ns, tw, sign);
     else if (R == 2)
-      fft_stage<T, 2>(a, b, n, nl, ls, ns, tw, sign);
+      fft_stage<T, 2, NE>(a, n, nl, ls, ns, tw, sign);
     else if (R == 3)
-      fft_stage<T, 3>(a, b, n, nl, ls, ns, tw, sign);
+      fft_stage<T, 3, NE>(a, n, nl, ls, ns, tw, sign);
     else
-      fft_stage<T, 5>(a, b, n, nl, ls, ns, tw, sign);
-    __syncthreads();
-    cplx_t<T>* t = a;
-    a = b;
-    b = t;
+      fft_stage<T, 5, NE>(a, n, nl, ls, ns, tw, sign);
     ns *= R;
   }
-  return a;
 }
 
-// forward x-DFT of RDP rows (J=1..N2M, K), 2*TL rows per block
-template <typename T>
-__global__ __launch_bounds__(256) void k_fft_x(Ctx c) {
+// forward x-DFT of RDP rows (J=1..N2M, K), 2*TL rows per block; line
+// stride TL+1 in LDS so that the row-wise fill and drain hit distinct banks
+template <typename T, int NE>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_x(Ctx c) {
   static_assert(sizeof(T) == 8, "the twiddle tables are FP64");
   extern __shared__ __align__(16) unsigned char smem[];
   typedef cplx_t<T> zc;
   const Geo g = c.g;
-  const int n = g.nx, NL = c.f.tlx, LS = NL;  // NL complex lines = 2*NL rows
+  const int n = g.nx, NL = c.f.tlx, LS = NL + 1;  // NL complex lines = 2*NL rows
   zc* A = (zc*)smem;
-  zc* B = A + n * LS;
   const int nlines = g.n2m * g.nz;
   const int ln0 = blockIdx.x * 2 * NL;
   const T* src = (const T*)c.d.rdp;
-  for (int l = 0; l < NL; l++) {
+#pragma unroll 4
+  for (int e = threadIdx.x; e < NL * n; e += FFT_THREADS) {
+    const int l = e / n, p = e - l * n;
     const int la = ln0 + l, lb = ln0 + NL + l;
-    const long long ra = la < nlines ? gidx(g, 0, 1 + la % g.n2m, la / g.n2m) : -1;
-    const long long rb = lb < nlines ? gidx(g, 0, 1 + lb % g.n2m, lb / g.n2m) : -1;
-    for (int p = threadIdx.x; p < n; p += blockDim.x)
-      A[p * LS + l] = {ra >= 0 ? src[ra + p] : (T)0, rb >= 0 ? src[rb + p] : (T)0};
+    const T xa = la < nlines ? src[gidx(g, p, 1 + la % g.n2m, la / g.n2m)] : (T)0;
+    const T xb = lb < nlines ? src[gidx(g, p, 1 + lb % g.n2m, lb / g.n2m)] : (T)0;
+    A[p * LS + l] = {xa, xb};
   }
   __syncthreads();
-  const zc* R = fft_lines<T>(A, B, n, NL, LS, c.f.codex, c.f.nradx,
-                             (const zc*)c.f.twxf, -1);
+  fft_lines<T, NE>(A, n, NL, LS, c.f.codex, c.f.nradx, (const zc*)c.f.twxf, -1);
   const int nxh = g.nxh;
-  for (int l = 0; l < NL; l++) {
+  zc* S = (zc*)c.d.spec;
+#pragma unroll 2
+  for (int e = threadIdx.x; e < NL * nxh; e += FFT_THREADS) {
+    const int l = e / nxh, kx = e - l * nxh;
     const int la = ln0 + l, lb = ln0 + NL + l;
-    for (int kx = threadIdx.x; kx < nxh; kx += blockDim.x) {
-      const zc z1 = R[kx * LS + l];
-      const zc z2 = R[(kx == 0 ? 0 : n - kx) * LS + l];
-      // A = (z1 + conj z2)/2, B = (z1 - conj z2)/(2i)
-      const zc va = {(T)0.5 * (z1.x + z2.x), (T)0.5 * (z1.y - z2.y)};
-      const zc vb = {(T)0.5 * (z1.y + z2.y), (T)-0.5 * (z1.x - z2.x)};
-      if (la < nlines) {
-        const int jm = la % g.n2m, k = la / g.n2m;
-        ((zc*)c.d.spec)[((long long)jm * g.nz + k) * g.nxhp + kx] = va;
-      }
-      if (lb < nlines) {
-        const int jm = lb % g.n2m, k = lb / g.n2m;
-        ((zc*)c.d.spec)[((long long)jm * g.nz + k) * g.nxhp + kx] = vb;
-      }
-    }
+    const zc z1 = A[kx * LS + l];
+    const zc z2 = A[(kx == 0 ? 0 : n - kx) * LS + l];
+    // A = (z1 + conj z2)/2, B = (z1 - conj z2)/(2i)
+    const zc va = {(T)0.5 * (z1.x + z2.x), (T)0.5 * (z1.y - z2.y)};
+    const zc vb = {(T)0.5 * (z1.y + z2.y), (T)-0.5 * (z1.x - z2.x)};
+    if (la < nlines) S[((long long)(la % g.n2m) * g.nz + la / g.n2m) * g.nxhp + kx] = va;
+    if (lb < nlines) S[((long long)(lb % g.n2m) * g.nz + lb / g.n2m) * g.nxhp + kx] = vb;
   }
 }
 
 // z-DFT of the half spectrum, columns (jm, kx); sign -1 forward, +1 backward
-template <typename T>
-__global__ __launch_bounds__(256) void k_fft_z(Ctx c, int sign, void* spec) {
+template <typename T, int NE>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_z(Ctx c, int sign, void* spec) {
   static_assert(sizeof(T) == 8, "the twiddle tables are FP64");
   extern __shared__ __align__(16) unsigned char smem[];
   typedef cplx_t<T> zc;
   const Geo g = c.g;
   const int n = g.nz, L = c.f.tlz, LS = L;
   zc* A = (zc*)smem;
-  zc* B = A + n * LS;
   const int nit = (g.nxh + L - 1) / L;
   const int it = blockIdx.x % nit;
   const int jm = blockIdx.x / nit;
   zc* S = (zc*)spec + (long long)jm * g.nz * g.nxhp;
   const int tot = n * L;
-  for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+#pragma unroll 4
+  for (int e = threadIdx.x; e < tot; e += FFT_THREADS) {
     const int l = e % L, p = e / L;
     const int kx = it * L + l;
     A[p * LS + l] = kx < g.nxh ? S[(long long)p * g.nxhp + kx] : zc{(T)0, (T)0};
   }
   __syncthreads();
   const zc* tw = (const zc*)(sign < 0 ? c.f.twzf : c.f.twzb);
-  const zc* R = fft_lines<T>(A, B, n, L, LS, c.f.codez, c.f.nradz, tw, sign);
-  for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+  fft_lines<T, NE>(A, n, L, LS, c.f.codez, c.f.nradz, tw, sign);
+#pragma unroll 4
+  for (int e = threadIdx.x; e < tot; e += FFT_THREADS) {
     const int l = e % L, p = e / L;
     const int kx = it * L + l;
-    if (kx < g.nxh) S[(long long)p * g.nxhp + kx] = R[p * LS + l];
+    if (kx < g.nxh) S[(long long)p * g.nxhp + kx] = A[p * LS + l];
   }
 }
 
@@ -1096,51 +1107,49 @@ __global__ void k_pois_table(Ctx c) {
 // part / (N1M*N3M) into DP (:9429-9433); two rows per complex transform:
 // Z = A + iB with the imaginary parts of A and B at kx = 0 and NX/2 dropped
 // (they do not reach the real part of the reference's transform)
-template <typename T>
-__global__ __launch_bounds__(256) void k_fft_xinv(Ctx c, const void* spec) {
+template <typename T, int NE>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_xinv(Ctx c, const void* spec) {
   static_assert(sizeof(T) == 8, "the twiddle tables are FP64");
   extern __shared__ __align__(16) unsigned char smem[];
   typedef cplx_t<T> zc;
   const Geo g = c.g;
-  const int n = g.nx, NL = c.f.tlx, LS = NL;
+  const int n = g.nx, NL = c.f.tlx, LS = NL + 1;
   zc* A = (zc*)smem;
-  zc* B = A + n * LS;
   const int nlines = g.n2m * g.nz;
   const int ln0 = blockIdx.x * 2 * NL;
   const zc* S = (const zc*)spec;
   const int nxh = g.nxh;
-  for (int l = 0; l < NL; l++) {
+#pragma unroll 4
+  for (int e = threadIdx.x; e < NL * n; e += FFT_THREADS) {
+    const int l = e / n, kx = e - l * n;
     const int la = ln0 + l, lb = ln0 + NL + l;
     const long long sa = la < nlines ? ((long long)(la % g.n2m) * g.nz + la / g.n2m) * g.nxhp : -1;
     const long long sb = lb < nlines ? ((long long)(lb % g.n2m) * g.nz + lb / g.n2m) * g.nxhp : -1;
-    for (int kx = threadIdx.x; kx < n; kx += blockDim.x) {
-      const bool lo = kx < nxh;
-      const int kk = lo ? kx : n - kx;
-      zc xa = sa >= 0 ? S[sa + kk] : zc{(T)0, (T)0};
-      zc xb = sb >= 0 ? S[sb + kk] : zc{(T)0, (T)0};
-      if (kx == 0 || 2 * kx == n) {
-        xa.y = (T)0;
-        xb.y = (T)0;
-      }
-      if (!lo) {
-        xa.y = -xa.y;
-        xb.y = -xb.y;
-      }
-      A[kx * LS + l] = {xa.x - xb.y, xa.y + xb.x};
+    const bool lo = kx < nxh;
+    const int kk = lo ? kx : n - kx;
+    zc xa = sa >= 0 ? S[sa + kk] : zc{(T)0, (T)0};
+    zc xb = sb >= 0 ? S[sb + kk] : zc{(T)0, (T)0};
+    if (kx == 0 || 2 * kx == n) {
+      xa.y = (T)0;
+      xb.y = (T)0;
     }
+    if (!lo) {
+      xa.y = -xa.y;
+      xb.y = -xb.y;
+    }
+    A[kx * LS + l] = {xa.x - xb.y, xa.y + xb.x};
   }
   __syncthreads();
-  const zc* R = fft_lines<T>(A, B, n, NL, LS, c.f.codex, c.f.nradx,
-                             (const zc*)c.f.twxb, +1);
+  fft_lines<T, NE>(A, n, NL, LS, c.f.codex, c.f.nradx, (const zc*)c.f.twxb, +1);
   const T norm = (T)(1.0 / ((double)g.nx * (double)g.nz));
   double* dp = c.d.dp;
-  for (int l = 0; l < NL; l++) {
+#pragma unroll 4
+  for (int e = threadIdx.x; e < NL * n; e += FFT_THREADS) {
+    const int l = e / n, p = e - l * n;
     const int la = ln0 + l, lb = ln0 + NL + l;
-    for (int p = threadIdx.x; p < n; p += blockDim.x) {
-      const zc z = R[p * LS + l];
-      if (la < nlines) dp[gidx(g, p, 1 + la % g.n2m, la / g.n2m)] = z.x * norm;
-      if (lb < nlines) dp[gidx(g, p, 1 + lb % g.n2m, lb / g.n2m)] = z.y * norm;
-    }
+    const zc z = A[p * LS + l];
+    if (la < nlines) dp[gidx(g, p, 1 + la % g.n2m, la / g.n2m)] = z.x * norm;
+    if (lb < nlines) dp[gidx(g, p, 1 + lb % g.n2m, lb / g.n2m)] = z.y * norm;
   }
 }
 
@@ -1437,14 +1446,25 @@ static void getdp_fp64(const Ctx& c) {
   typedef double T;
   const Geo& g = c.g;
   const FFTPlan& f = c.f;
-  const size_t ldsx = 2 * sizeof(cplx_t<T>) * (size_t)g.nx * f.tlx;
-  const size_t ldsz = 2 * sizeof(cplx_t<T>) * (size_t)g.nz * f.tlz;
+  // one in-place buffer per block; x lines padded to a stride of TL+1
+  const size_t ldsx = sizeof(cplx_t<T>) * (size_t)g.nx * (f.tlx + 1);
+  const size_t ldsz = sizeof(cplx_t<T>) * (size_t)g.nz * f.tlz;
   const int gx = (g.n2m * g.nz + 2 * f.tlx - 1) / (2 * f.tlx);
   const int gz = ((g.nxh + f.tlz - 1) / f.tlz) * g.n2m;
   const int gy = ((g.nxh + c.o.yp.L - 1) / c.o.yp.L) * g.nz;
   void* sp = (void*)c.d.spec;
-  { KScope ks_(c, KID_FFT_X); hipLaunchKernelGGL(k_fft_x<T>, dim3(gx), dim3(256), ldsx, c.st, c); }
-  { KScope ks_(c, KID_FFT_Z); hipLaunchKernelGGL(k_fft_z<T>, dim3(gz), dim3(256), ldsz, c.st, c, -1, sp); }
+  // lines of up to 2048 points per block use the lean stage; longer z lines
+  // (N3-1 up to 4096, one per block) the wide one
+  const bool wx = g.nx * f.tlx > 2048, wz = g.nz * f.tlz > 2048;
+#define FFTW_(K, wide, ...)                                                             \
+  do {                                                                                  \
+    if (wide)                                                                           \
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(K<T, 4096>), __VA_ARGS__);                    \
+    else                                                                                \
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(K<T, 2048>), __VA_ARGS__);                    \
+  } while (0)
+  { KScope ks_(c, KID_FFT_X); FFTW_(k_fft_x, wx, dim3(gx), dim3(FFT_THREADS), ldsx, c.st, c); }
+  { KScope ks_(c, KID_FFT_Z); FFTW_(k_fft_z, wz, dim3(gz), dim3(FFT_THREADS), ldsz, c.st, c, -1, sp); }
   {
     KScope ks_(c, KID_POIS_Y);
     if ((g.n2m + c.o.yp.C - 1) / c.o.yp.C <= 6)
@@ -1454,8 +1474,9 @@ static void getdp_fp64(const Ctx& c) {
       hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pois_y<T, T, CHMAX>), dim3(gy), dim3(c.o.yp.L * c.o.yp.C), 0, c.st,
                          c, (const void*)sp, sp);
   }
-  { KScope ks_(c, KID_FFT_Z); hipLaunchKernelGGL(k_fft_z<T>, dim3(gz), dim3(256), ldsz, c.st, c, +1, sp); }
-  { KScope ks_(c, KID_FFT_XINV); hipLaunchKernelGGL(k_fft_xinv<T>, dim3(gx), dim3(256), ldsx, c.st, c, (const void*)sp); }
+  { KScope ks_(c, KID_FFT_Z); FFTW_(k_fft_z, wz, dim3(gz), dim3(FFT_THREADS), ldsz, c.st, c, +1, sp); }
+  { KScope ks_(c, KID_FFT_XINV); FFTW_(k_fft_xinv, wx, dim3(gx), dim3(FFT_THREADS), ldsx, c.st, c, (const void*)sp); }
+#undef FFTW_
 }
 void getdp(const Ctx& c) { getdp_fp64(c); }
 
