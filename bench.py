@@ -13,9 +13,11 @@ Metric: Mpts.steps/s = (N1M*N2M*N3M) * steps / wall-seconds / 1e6 over the
 timed steps (state resident, no host transfers).
 
 Prints one JSON line (see the keys below).  `roofline` is for the kernel with
-the largest share of the step, from HIP events recorded by the library around
-each launch on its own stream; `cpu_baseline` times the FP64 oracle
-(oracle/liboracle.so, OpenMP) on a few steps of the same grid.
+the largest share of the step (found in a short all-kernel timing pass after
+the warm-up): HIP events recorded by the library on its own stream around
+each of that kernel's launches inside the timed steps give its average
+duration.  `cpu_baseline` times the FP64 oracle (oracle/liboracle.so,
+OpenMP) on a few steps of the same grid.
 """
 import argparse
 import json
@@ -31,7 +33,7 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "Mpts·steps/s mixed vs FP64 on 1 GPU @ div≤1e-9; p50/p99 served latency"
 
 # algorithmic (compulsory) HBM bytes per cell and launch of each kernel of the
-# FP64 step: 8 B per field element read or written once (see DESIGN.md)
+# FP64 step: 8 B per field element read or written once
 ALG_BYTES = {
     # compulsory bytes per pressure cell of each kernel (every distinct field
     # read once, written once); adi_y is the ADI y-sweep with the RHS y terms
@@ -40,7 +42,7 @@ ALG_BYTES = {
     # also its two per-mode factor tables
     "rhs_x": 88, "rhs_z": 112, "rhs_zw": 40, "rhs_x2": 24, "adi_y": 168, "adi_z": 64,
     "rhsdp_x": 24, "rhsdp_z": 24, "fft_x": 16, "fft_z": 16, "pois_y": 24, "fft_xinv": 16,
-    "flow_red": 40, "upd_x": 48, "upd_z": 64,
+    "upd_x": 48, "upd_z": 64,
 }
 # adi_x runs twice per step: (u,v) read+write 32 B, then w with U added 24 B
 ALG_BYTES_ADI_X_PER_STEP = 56
@@ -50,7 +52,7 @@ ALG_BYTES_ADI_X_PER_STEP = 56
 ALG_BYTES_MIXED = {
     "rhs_x": 44, "rhs_z": 56, "rhs_zw": 20, "rhs_x2": 12, "adi_y": 84, "adi_z": 24,
     "rhsdp_x": 24, "rhsdp_z": 20, "fft_x": 16, "fft_z": 16, "pois_y": 24, "fft_xinv": 16,
-    "flow_red": 32, "upd_x": 64, "upd_z": 92,
+    "upd_x": 64, "upd_z": 92,
 }
 
 
@@ -90,33 +92,51 @@ def cpu_baseline(cfg, target_s, seed):
                        f"subroutines), OpenMP {threads} threads, {el:.1f} s")
 
 
-def run_leg(c3d, cfg, U, P, precision, args, pts):
-    """warm-up, then args.steps timed steps (GETUP + DIVCHECK + CFL read back
-    every step, as MAIN's loop does) of one precision"""
+def run_leg(c3d, cfg, U, P, precision, args, pts, table):
+    """warm-up, a short pass with every kernel timed (the per-kernel table
+    and the dominant kernel), then args.steps timed steps (GETUP + DIVCHECK +
+    CFL read back every step, as MAIN's loop does) of one precision, with
+    HIP events around the dominant kernel's launches only"""
     import torch
     ch = c3d.Channel(**cfg, precision=precision)
     ch.upload(U, P)
-    for n in range(args.warmup):
-        ch.step(n + 1)
+    n = 0
+    for _ in range(args.warmup):
+        n += 1
+        ch.step(n)
     torch.cuda.synchronize()
+    kt_all = {}
     if not args.no_timing:
-        ch.timing(True, 48 * args.steps + 64)
+        nprof = max(5, args.steps // 5)
+        ch.timing(True, 48 * nprof + 64)
+        for _ in range(nprof):
+            n += 1
+            ch.step(n)
+        kt_all = ch.kernel_times()
+        ch.timing(False)
+    dom = max((k for k in kt_all if k in table), key=lambda k: kt_all[k][0]) if kt_all else None
+    if dom is not None:
+        ch.timing_select(dom)
+        ch.timing(True, 4 * args.steps + 64)
     torch.cuda.synchronize()
     dmax = 0.0
     t0 = time.perf_counter()
-    for n in range(args.steps):
-        d, _ = ch.step(args.warmup + n + 1)
+    for _ in range(args.steps):
+        n += 1
+        d, _ = ch.step(n)
         dmax = max(dmax, d) if d == d else float("nan")
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    kt = ch.kernel_times() if not args.no_timing else {}
+    kt_dom = ch.kernel_times() if dom is not None else {}
     ch.timing(False)
+    ch.timing_select(None)
     dt_s = (t1 - t0) / args.steps
+    tot_ms = sum(ms for ms, _ in kt_all.values())
     kernels = {}
-    for name, (ms, cnt) in kt.items():
-        kernels[name] = dict(avg_us=round(1000.0 * ms / cnt, 2), launches=cnt,
-                             share=round(ms / (1000.0 * dt_s * args.steps), 4))
-    return ch, dict(value=pts / dt_s / 1e6, dt_s=dt_s, divmax_max=dmax, kt=kt, kernels=kernels)
+    for name, (ms, cnt) in kt_all.items():
+        kernels[name] = dict(avg_us=round(1000.0 * ms / cnt, 2), launches=cnt, share=round(ms / tot_ms, 4))
+    return ch, dict(value=pts / dt_s / 1e6, dt_s=dt_s, divmax_max=dmax, kt=kt_dom, kernels=kernels,
+                    steps_run=n)
 
 
 def roofline(kt, table, pts, grid, precision):
@@ -195,7 +215,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU decomposition is a design "
-                                   "only (DESIGN.md, 'Sharding').  Use --gpus 1."}))
+                                   "only.  Use --gpus 1."}))
         sys.exit(2)
 
     import numpy as np
@@ -221,10 +241,10 @@ def main():
     m = probe.mesh()  # the grid the library built, for the initial field
     probe.close()
     U, P = perturbed_poiseuille(cfg["n1"], cfg["n2"], cfg["n3"], m["Y"], m["DX1"], m["DX3"], m["ALY"], seed=args.seed)
-    ch64, r64 = run_leg(c3d, cfg, U, P, c3d.FP64, args, pts)
-    chmx, rmx = run_leg(c3d, cfg, U, P, c3d.MIXED, args, pts)
+    ch64, r64 = run_leg(c3d, cfg, U, P, c3d.FP64, args, pts, ALG_BYTES)
+    chmx, rmx = run_leg(c3d, cfg, U, P, c3d.MIXED, args, pts, ALG_BYTES_MIXED)
     del U, P
-    # both legs ran warmup + steps steps from the same field: mixed vs FP64
+    # both legs ran the same number of steps from the same field: mixed vs FP64
     s64, smx = ch64.download(), chmx.download()
     n1, n2, n3 = cfg["n1"], cfg["n2"], cfg["n3"]
     errs = {}

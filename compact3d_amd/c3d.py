@@ -78,6 +78,7 @@ def lib():
         L.c3d_get_mesh.argtypes = [C.c_void_p, _dp, _dp]
         L.c3d_filter.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int]
         L.c3d_timing.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.c3d_timing_select.argtypes = [C.c_void_p, C.c_int]
         L.c3d_kernel_times.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_int)]
         L.c3d_kernel_name.restype = C.c_char_p
         L.c3d_kernel_name.argtypes = [C.c_int]
@@ -266,6 +267,17 @@ class Channel:
         """bracket every following kernel launch with HIP events on the
         handle's stream (per-kernel durations, c3d_kernel_times)"""
         self._ck(self.L.c3d_timing(self.h, 1 if on else 0, int(capacity)), "timing")
+
+    def timing_select(self, name=None):
+        """restrict the event recorder to the launches of one kernel (by
+        name, as kernel_times() reports it); None: all kernels"""
+        kid = -1
+        if name is not None:
+            names = [self.L.c3d_kernel_name(q).decode() for q in range(NKID)]
+            if name not in names:
+                raise C3DError("no kernel named %r" % name)
+            kid = names.index(name)
+        self._ck(self.L.c3d_timing_select(self.h, kid), "timing_select")
 
     def kernel_times(self):
         """{kernel name: (total ms, launches)} since timing() was enabled"""

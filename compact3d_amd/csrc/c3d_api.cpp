@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -38,6 +39,7 @@ struct c3d_handle {
   double* dtmp = nullptr;       // scratch field for apply_op
   double* dtmp2 = nullptr;
   std::vector<double> stage;    // host staging buffer
+  double* hread = nullptr;      // pinned host slots for the per-step DIVMAX/CFLM read-back
   KTimer kt;
 };
 
@@ -434,6 +436,7 @@ long long c3d_footprint_bytes(const c3d_config* cfg) {
   if (cfg->files != 0.0) bytes += 7 * fsz * 8;
   const long long nm = ((nx / 2 + 1 + 7) / 8 * 8) * nz * n2m;
   bytes += nm * 16 + nm * 16;  // spectrum + TDMAP factors
+  bytes += 4 * std::max(n2m * nz, nx * n2m) * 8 + 128 * 8;  // flow-rate partial sums, max slots
   return bytes + 4 * nx * nz * 8 + (1 << 20);
 }
 
@@ -449,6 +452,7 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   auto bad = [&](const std::string& m) {
     g_create_err = m;
     for (void* p : h->allocs) (void)hipFree(p);
+    if (h->hread) (void)hipHostFree(h->hread);
     delete h;
     return -1;
   };
@@ -645,8 +649,17 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
       return bad("Poisson factor set-up failed");
   }
   if (dalloc(h, &F.sc, SC_COUNT)) return bad(h->err);
+  if (hipHostMalloc((void**)&h->hread, 2 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    h->hread = nullptr;
+    return bad("hipHostMalloc of the read-back slots failed");
+  }
   F.nred = 1024;
   if (dalloc(h, &F.red, 4 * (size_t)F.nred)) return bad(h->err);
+  // flow-rate partial sums: one entry per block of any line kernel (x tiles
+  // hold >= 1 line of N2M*N3M, z tiles >= 1 column of N1M*N2M)
+  F.flowr = std::max(g.n2m * g.nz, g.nx * g.n2m);
+  if (dalloc(h, &F.flowp, 4 * (size_t)F.flowr)) return bad(h->err);
+  if (dalloc(h, &F.mxs, 128)) return bad(h->err);  // zeroed
   *out = h;
   return 0;
 }
@@ -670,6 +683,7 @@ extern "C" int c3d_destroy(c3d_handle* h) {
   graphs_free(h);
   kt_free(h);
   for (void* p : h->allocs) (void)hipFree(p);
+  if (h->hread) (void)hipHostFree(h->hread);
   if (h->own_stream) (void)hipStreamDestroy(h->c.st);
   delete h;
   return 0;
@@ -852,15 +866,24 @@ static void upcalc_impl(c3d_handle* h, bool chk) {
   c3dk::upcalc(h->c, chk);
 }
 
-// Mixed mode, after UPCALC: the DIVMAX of the projection (FP64, on the new
-// field) is read back; above kRefineTol one FP64-residual refinement
-// follows: DIV is the residual of the Poisson solve, DP2 solves
-// L DP2 = DIV/DT the same way and the field takes U -= DT grad(DP2),
-// P += DP2 (flow rate unchanged: the periodic gradients have zero line
-// means).  *divmax returns the final DIVMAX.
-static int refine_impl(c3d_handle* h, double* divmax) {
-  double dm = 0.0;
-  if (read_sc(h, SC_DIVMAX, &dm)) return -1;
+// DIVMAX and CFLM of the last step (adjacent device slots) in one copy to
+// pinned host memory; either pointer may be null
+static int read_checks(c3d_handle* h, double* divmax, double* cflm) {
+  static_assert(SC_CFLM == SC_DIVMAX + 1, "DIVMAX and CFLM slots must be adjacent");
+  HIPCHK(hipMemcpyAsync(h->hread, h->c.d.sc + SC_DIVMAX, 2 * sizeof(double), hipMemcpyDeviceToHost, h->c.st));
+  HIPCHK(hipStreamSynchronize(h->c.st));
+  if (divmax) *divmax = h->hread[0];
+  if (cflm) *cflm = h->hread[1];
+  return 0;
+}
+
+// Mixed mode, after UPCALC, given the DIVMAX dm of the projection (FP64, on
+// the new field): above kRefineTol one FP64-residual refinement follows: DIV
+// is the residual of the Poisson solve, DP2 solves L DP2 = DIV/DT the same
+// way and the field takes U -= DT grad(DP2), P += DP2 (flow rate unchanged:
+// the periodic gradients have zero line means).  *divmax / *cflm return the
+// final DIVMAX and CFLM (unchanged when no refinement ran).
+static int refine_from(c3d_handle* h, double dm, double* divmax, double* cflm) {
   h->first_divmax = dm;
   if (!(dm <= kRefineTol)) {
     h->refines++;
@@ -870,10 +893,17 @@ static int refine_impl(c3d_handle* h, double* divmax) {
     c3dk::upd_x(h->c, true, true);
     c3dk::ghosts(h->c);
     c3dk::upd_z(h->c, true, true);
-    if (read_sc(h, SC_DIVMAX, &dm)) return -1;
+    if (read_checks(h, &dm, cflm)) return -1;
   }
   if (divmax) *divmax = dm;
   return 0;
+}
+
+// the same, reading DIVMAX first
+static int refine_impl(c3d_handle* h, double* divmax) {
+  double dm = 0.0;
+  if (read_checks(h, &dm, nullptr)) return -1;
+  return refine_from(h, dm, divmax, nullptr);
 }
 
 extern "C" int c3d_upcalc(c3d_handle* h) {
@@ -914,6 +944,7 @@ extern "C" int c3d_wallss(c3d_handle* h) {
   return launched(h);
 }
 
+// the kernels of GETUP (no host synchronisation)
 static int step_impl(c3d_handle* h, bool chk) {
   swap_hh(h);
   // GETUP (:925-929): SGS model, then the wall model
@@ -925,14 +956,14 @@ static int step_impl(c3d_handle* h, bool chk) {
   c3dk::rhsdp(h->c);
   c3dk::getdp(h->c);
   upcalc_impl(h, chk);
-  if (launched(h)) return -1;
-  return h->c.s.mixed ? refine_impl(h, nullptr) : 0;
+  return launched(h);
 }
 
 extern "C" int c3d_getup(c3d_handle* h, int ntime) {
   (void)ntime;
   if (!h) return -1;
-  return step_impl(h, false);
+  if (step_impl(h, false)) return -1;
+  return h->c.s.mixed ? refine_impl(h, nullptr) : 0;
 }
 
 static int read_sc(c3d_handle* h, int slot, double* v) {
@@ -1018,19 +1049,13 @@ extern "C" int c3d_advance(c3d_handle* h, int nsteps, int use_graph) {
 
 extern "C" int c3d_sync(c3d_handle* h, double* divmax, double* cflm) {
   if (!h) return -1;
-  HIPCHK(hipStreamSynchronize(h->c.st));
-  double dm = 0.0;
-  if (h->c.s.mixed) {
-    if (refine_impl(h, &dm)) return -1;
-  } else if (read_sc(h, SC_DIVMAX, &dm)) {
-    return -1;
-  }
+  double dm = 0.0, cf = 0.0;
+  if (read_checks(h, &dm, &cf)) return -1;
+  if (h->c.s.mixed && refine_from(h, dm, &dm, &cf)) return -1;
   if (divmax) *divmax = dm;
-  if (cflm && read_sc(h, SC_CFLM, cflm)) return -1;
+  if (cflm) *cflm = cf;
   return 0;
 }
-
-static double bits_to_double(double v) { return v; }
 
 extern "C" int c3d_step(c3d_handle* h, int ntime, double* divmax, double* cflm) {
   (void)ntime;
@@ -1039,9 +1064,13 @@ extern "C" int c3d_step(c3d_handle* h, int ntime, double* divmax, double* cflm) 
   // model consumes it
   if (h->cfg.wmodel == 0.0 && h->cfg.files != 0.0) c3dk::wallss(h->c);
   if (step_impl(h, true)) return -1;
-  if (divmax && read_sc(h, SC_DIVMAX, divmax)) return -1;
-  if (cflm && read_sc(h, SC_CFLM, cflm)) return -1;
-  if (divmax) *divmax = bits_to_double(*divmax);
+  // one read-back of DIVMAX and CFLM per step (mixed mode: the refinement
+  // check uses the same DIVMAX)
+  double dm = 0.0, cf = 0.0;
+  if (read_checks(h, &dm, &cf)) return -1;
+  if (h->c.s.mixed && refine_from(h, dm, &dm, &cf)) return -1;
+  if (divmax) *divmax = dm;
+  if (cflm) *cflm = cf;
   return 0;
 }
 
@@ -1200,7 +1229,7 @@ static const char* kid_names[C3D_NKID] = {
 static_assert(KID_COUNT <= C3D_NKID, "kernel id table too small");
 
 extern "C" const char* c3d_kernel_name(int id) {
-  if (id < 0 || id >= C3D_NKID) return "";
+  if (id < 0 || id >= KID_COUNT) return "";
   return kid_names[id];
 }
 
@@ -1217,6 +1246,13 @@ extern "C" int c3d_timing(c3d_handle* h, int on, int capacity) {
   }
   t.used = 0;
   t.on = on != 0;
+  return 0;
+}
+
+extern "C" int c3d_timing_select(c3d_handle* h, int kid) {
+  if (!h) return -1;
+  if (kid >= KID_COUNT) return fail(h, "no such kernel id");
+  h->kt.only = kid < 0 ? -1 : kid;
   return 0;
 }
 

@@ -347,6 +347,38 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
+// sum over one wavefront; the butterfly leaves the same value in every lane
+// (a + b == b + a), so the result does not depend on the lane
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// block sums of a and b in a fixed order (lanes, then wavefronts in index
+// order); thread 0 stores them to *oa and *ob.  Every thread of the block
+// must call it.
+__device__ __forceinline__ void block_sum2(double a, double b, double* oa, double* ob) {
+  __shared__ double rs[2][16];
+  a = wave_sum(a);
+  b = wave_sum(b);
+  const int wv = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    rs[0][wv] = a;
+    rs[1][wv] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sa = 0.0, sb = 0.0;
+    for (int q = 0; q < nw; q++) {
+      sa += rs[0][q];
+      sb += rs[1][q];
+    }
+    *oa = sa;
+    if (ob) *ob = sb;
+  }
+}
+
 // y-line tile: L columns x C chunks, thread (l, c) = l + L*c
 struct YShape {
   int L, C;

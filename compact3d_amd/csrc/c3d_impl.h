@@ -108,7 +108,17 @@ struct Fields {
   double *sc;             // device scalars (SC_*)
   double *red;            // reduction partials
   int nred;               // number of partial slots
+  // per-block partial sums of UPCALC's flow-rate terms, four regions of
+  // flowr entries (flowr >= the block count of any line kernel):
+  //   FP_DU: sum DELTA_u*DY (ADI z-sweep of u)   FP_U: sum U_u*DY (before the update)
+  //   FP_DW: sum DELTA_w*DY (ADI x-sweep of w)   FP_W: sum U_w*DY
+  double *flowp;
+  int flowr;
+  // the 2 x 64 max slots (DIVMAX, CFL bit patterns) of the update's checks;
+  // zero between steps (cleared at set-up and by the kernel that reads them)
+  double *mxs;
 };
+enum { FP_DU = 0, FP_U = 1, FP_DW = 2, FP_W = 3 };
 
 // kernel ids for the per-kernel event timer
 // kernel ids for the per-kernel event timer (KID_COUNT <= C3D_NKID)
@@ -124,6 +134,7 @@ enum {
 // the launch stream; times are summed per kernel id on request
 struct KTimer {
   bool on = false;
+  int only = -1;  // >= 0: bracket only launches of this kernel id
   int used = 0;
   hipEvent_t* ev = nullptr;  // 2*cap events
   int* id = nullptr;         // kernel id per pair
@@ -147,7 +158,7 @@ struct KScope {
   int slot;
   KScope(const Ctx& cc, int kid) : c(cc), slot(-1) {
     KTimer* t = c.kt;
-    if (t && t->on && t->used < t->cap) {
+    if (t && t->on && t->used < t->cap && (t->only < 0 || t->only == kid)) {
       slot = t->used++;
       t->id[slot] = kid;
       (void)hipEventRecord(t->ev[2 * slot], c.st);
@@ -169,6 +180,7 @@ void getuh_fused(const Ctx& c);  // y parts of RHS1/2/3 fused into GETUH1/2/3
 void rhsdp(const Ctx& c);
 void getdp(const Ctx& c);
 void upcalc(const Ctx& c, bool fuse_checks);
+void flow_fin(const Ctx& c);  // DMPRESG/DMPRESG3 from the ADI sweeps' partial sums
 void upd_x(const Ctx& c, bool chk, bool refine = false);
 void upd_z(const Ctx& c, bool chk, bool refine = false);
 void div_rdp(const Ctx& c);

@@ -3,7 +3,6 @@
 //
 //   k_bcond                 BCOND                                 :963-1001
 //   k_rhs_y                 RHS1/2/3 pointwise y parts, AB2, wall BC :6063-8114
-//   k_flow_red/k_flow_fin   UPCALC flow-rate sums and DMPRESG       :1079-1093
 //   k_ghosts                ghost planes after the update           :1130-1215
 //   k_div_x/k_div_z, k_cfl  DIVCHECK and CFL on the resident field  :9049-9169
 //   k_chkmf                 CHKMF                                   :9463-9503
@@ -17,6 +16,9 @@
 
 #include "c3d_impl.h"
 #include "c3d_yterms.h"
+
+// blocks of the grid-stride reductions (fixed, so every sum has a fixed order)
+#define RED_BLOCKS 1024
 
 // ------------------------------------------------------------------ BCOND
 __global__ void k_bcond(Ctx c) {
@@ -70,87 +72,6 @@ __global__ __launch_bounds__(256) void k_rhs_y(Ctx c) {
     ((T*)c.d.ruh[0])[x.b] = yterm_u<T>(c, x, i, j, k);
     if (j >= 2) ((T*)c.d.ruh[1])[x.b] = yterm_v<T>(c, x, i, j, k);
     ((T*)c.d.ruh[2])[x.b] = yterm_w<T>(c, x, i, j, k);
-  }
-}
-
-// ----------------------------------------------------------------- UPCALC
-// flow-rate sums of UPCALC (:1079-1093, :1160-1172), all before any update.
-// Fixed grid => fixed reduction tree => bit-repeatable.
-#define RED_BLOCKS 1024
-template <typename T>
-__global__ __launch_bounds__(C3D_NT) void k_flow_red(Ctx c) {
-  __shared__ double red[4][C3D_NT];
-  const Geo g = c.g;
-  const Scal s = c.s;
-  const Metrics M = c.m;
-  const Fields F = c.d;
-  const T* uh0 = (const T*)F.uh[0];
-  const T* uh2 = (const T*)F.uh[2];
-  const double* dpf = F.dp;
-  const double cu = 1.0 / s.dx3 / s.dx1;  // (x * DY / DX3) / DX1 as in :1083
-  double f1 = 0, d1 = 0, f3 = 0, d3 = 0;
-  const int nrows = g.n2m * g.nz;
-  for (int r = blockIdx.x; r < nrows; r += RED_BLOCKS) {
-    const int j = 1 + r % g.n2m, k = r / g.n2m;
-    const double dy = M.dy[j];
-    const long long rb = gidx(g, 0, j, k);
-    const long long rk = gidx(g, 0, j, k == 0 ? g.nz - 1 : k - 1);
-    for (int i = threadIdx.x; i < g.nx; i += C3D_NT) {
-      const long long b = rb + i;
-      const double dp = (double)dpf[b];
-      const double dpm = (double)dpf[rb + (i == 0 ? g.nx - 1 : i - 1)];
-      const double dpk = (double)dpf[rk + i];
-      f1 += F.u[b] * dy / s.dx3 / s.dx1;
-      // mixed mode: UH = U + DELTA
-      const double uh1 = sizeof(T) == 4 ? F.u[b] + (double)uh0[b] : (double)uh0[b];
-      const double uh3 = sizeof(T) == 4 ? F.w[b] + (double)uh2[b] : (double)uh2[b];
-      d1 += uh1 * dy / s.dx3 / s.dx1 - s.dt * (dp - dpm) * dy / s.dx3;
-      f3 += F.w[b] * dy / s.dx1 / s.dx3;
-      d3 += uh3 * dy / s.dx3 / s.dx1 - s.dt * (dp - dpk) * dy / s.dx1;
-    }
-  }
-  (void)cu;
-  const int t = threadIdx.x;
-  red[0][t] = f1;
-  red[1][t] = d1;
-  red[2][t] = f3;
-  red[3][t] = d3;
-  __syncthreads();
-  for (int h = C3D_NT / 2; h > 0; h >>= 1) {
-    if (t < h)
-      for (int q = 0; q < 4; q++) red[q][t] += red[q][t + h];
-    __syncthreads();
-  }
-  if (t < 4) F.red[(long long)blockIdx.x * 4 + t] = red[t][0];
-}
-
-__global__ __launch_bounds__(C3D_NT) void k_flow_fin(Ctx c, int update) {
-  __shared__ double red[4][C3D_NT];
-  const Fields F = c.d;
-  const int t = threadIdx.x;
-  double a[4] = {0, 0, 0, 0};
-  for (int b = t; b < RED_BLOCKS; b += C3D_NT)
-    for (int q = 0; q < 4; q++) a[q] += F.red[(long long)b * 4 + q];
-  for (int q = 0; q < 4; q++) red[q][t] = a[q];
-  __syncthreads();
-  for (int h = C3D_NT / 2; h > 0; h >>= 1) {
-    if (t < h)
-      for (int q = 0; q < 4; q++) red[q][t] += red[q][t + h];
-    __syncthreads();
-  }
-  if (t == 0) {
-    const Scal s = c.s;
-    const double flow1 = red[0][0], flow3 = red[2][0];
-    const double dmp = (red[1][0] - flow1) / s.vol / s.dt;
-    const double dmp3 = (red[3][0] - flow3) / s.vol / s.dt;
-    F.sc[SC_FLOW1] = flow1;
-    F.sc[SC_FLOW3] = flow3;
-    if (update) {
-      F.sc[SC_DMPRESG] = dmp;
-      F.sc[SC_DMPRESG3] = dmp3;
-      F.sc[SC_PRESG] = F.sc[SC_PRESG] + dmp;
-      F.sc[SC_PRESG3] = F.sc[SC_PRESG3] + dmp3;
-    }
   }
 }
 
@@ -476,14 +397,7 @@ void rhs(const Ctx& c) {
 void upcalc(const Ctx& c, bool chk) {
   // mixed mode always forms DIV of the new field: the next RDP starts from it
   chk = chk || c.s.mixed;
-  {
-    KScope ks_(c, KID_FLOW_RED);
-    if (c.s.mixed)
-      hipLaunchKernelGGL(k_flow_red<float>, dim3(RED_BLOCKS), dim3(C3D_NT), 0, c.st, c);
-    else
-      hipLaunchKernelGGL(k_flow_red<double>, dim3(RED_BLOCKS), dim3(C3D_NT), 0, c.st, c);
-  }
-  { KScope ks_(c, KID_FLOW_FIN); hipLaunchKernelGGL(k_flow_fin, dim3(1), dim3(C3D_NT), 0, c.st, c, 1); }
+  flow_fin(c);
   upd_x(c, chk);
   ghosts(c);
   upd_z(c, chk);

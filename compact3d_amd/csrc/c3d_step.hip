@@ -353,23 +353,20 @@ __global__ __launch_bounds__(512) void k_rhs_zw(Ctx c) {
   T* ruh2 = (T*)F.ruh[2];
   T* hh2 = (T*)F.hh[2];
   const T dx3 = (T)s.dx3, dx3q = (T)s.dx3q, rinv = (T)s.rinv;
-  T a[SEG], d[SEG], e[SEG], lf, rt;
-  // w, P: RUH3
+  T a[SEG], d[SEG], e[SEG], r[SEG], lf, rt;
+  // w, P and the RUH3 operand are all loaded before the first solve
   zload((const T*)F.sw, b, pl, p0, bm, bp, a, lf, rt);
+  const T* sp = (const T*)F.sp;
+  const T pv0 = sp[bm];
+  SQ d[q] = sp[ZI(q)];
+  SQ r[q] = ruh2[ZI(q)];
+  // w, P: RUH3
   SQ e[q] = (T)12. * dx3q * (prv(a, q, lf) - (T)2. * a[q] + nxt(a, q, rt));
   psolve<T, SEG>(e, G, L, l, gs, s10, bf.xf, bf.xb);
-  {
-    const T* sp = (const T*)F.sp;
-    T pv = sp[bm];
-    SQ {
-      const T pc = sp[ZI(q)];
-      d[q] = (T)24. * dx3 * (-pv + pc);
-      pv = pc;
-    }
-  }
+  SQR d[q] = (T)24. * dx3 * (-(q == 0 ? pv0 : d[q > 0 ? q - 1 : 0]) + d[q]);
   psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
   const T presg3 = (T)F.sc[SC_PRESG3];
-  if (ok) SQ ruh2[ZI(q)] = ruh2[ZI(q)] + rinv * e[q] - d[q] - presg3;
+  if (ok) SQ ruh2[ZI(q)] = r[q] + rinv * e[q] - d[q] - presg3;
   // HH3 = D1z((Iz w)^2), w face -> centre -> face
   SQ d[q] = (T)4. * (a[q] + nxt(a, q, rt));
   psolve<T, SEG>(d, G, L, l, gs, s6, bf.xf, bf.xb);
@@ -755,11 +752,22 @@ __global__ __launch_bounds__(256) void k_adi_x(Ctx c, CompList cl) {
   if (addu) X.pre(U, 0, pf);
   SQ d[q] = XA(B, q - 1) + (T)10. * XA(B, q) + XA(B, q + 1);
   psolve<T, SEG>(d, G, L, l, gs, c.o.px.adi, bf.xf, bf.xb);
+  // the flow-rate terms of UPCALC for w (:1160-1172): sum DELTA*DY and U*DY
+  double sd = 0.0, su = 0.0;
+  SQ sd += (double)d[q];
   if (addu) {
     B = X.put(pf);
-    SQ d[q] = XA(B, q) + d[q];
+    SQ {
+      const T uq = XA(B, q);
+      su += (double)uq;
+      d[q] = uq + d[q];
+    }
   }
   X.out(UH, d, comp == 1 ? 2 : 1);  // V rows J=1 are not part of the system
+  if (fin && comp == 2) {
+    const double dy = ok ? c.m.dy[j] : 0.0;
+    block_sum2(sd * dy, su * dy, F.flowp + FP_DW * F.flowr + blockIdx.x, F.flowp + FP_W * F.flowr + blockIdx.x);
+  }
 }
 
 template <typename T, int SEG>
@@ -776,12 +784,26 @@ __global__ __launch_bounds__(512) void k_adi_z(Ctx c, CompList cl) {
   zload((const T*)UH, b, pl, p0, bm, bp, a, lf, rt);
   SQ d[q] = prv(a, q, lf) + (T)10. * a[q] + nxt(a, q, rt);
   psolve<T, SEG>(d, G, L, l, gs, PZ.adi, bf.xf, bf.xb);
+  // the flow-rate terms of UPCALC for u (:1079-1090): sum DELTA*DY and U*DY
+  double sd = 0.0, su = 0.0;
   if (ok) {
     if (fin && sizeof(T) == 8) {  // mixed mode keeps the increment DELTA
-      SQ UH[ZI(q)] = (T)U[ZI(q)] + d[q];
+      SQ {
+        const T uq = (T)U[ZI(q)];
+        sd += (double)d[q];
+        su += (double)uq;
+        UH[ZI(q)] = uq + d[q];
+      }
     } else {
-      SQ UH[ZI(q)] = d[q];
+      SQ {
+        sd += (double)d[q];
+        UH[ZI(q)] = d[q];
+      }
     }
+  }
+  if (fin && comp == 0) {
+    const double dy = c.m.dy[j];
+    block_sum2(sd * dy, su * dy, F.flowp + FP_DU * F.flowr + blockIdx.x, F.flowp + FP_U * F.flowr + blockIdx.x);
   }
 }
 
@@ -1198,13 +1220,23 @@ __global__ __launch_bounds__(256) void k_upd_x(Ctx c, int chk, int refine) {
   B = X.put(pf);
   SQ un[q] = ku * XA(B, q) - dt * d[q] - dt * dmp;
   X.pre((const TS*)F.uh[1], 0, pf);
+  // mixed mode: the flow rate of U before the update (FLOW1, :1081)
+  double so = 0.0;
   if (MIX) {
     B = X.put(pg);
-    SQ un[q] = XA(B, q) + un[q];
+    SQ {
+      const T uo = XA(B, q);
+      so += (double)uo;
+      un[q] = uo + un[q];
+    }
     X.pre(F.vold, 0, pg);
     X.out2(F.u, F.uf, un);
   } else {
     X.out(F.u, un);
+  }
+  if (MIX) {
+    const double dy = ok ? M.dy[j] : 0.0;
+    block_sum2(so * dy, 0.0, F.flowp + FP_U * F.flowr + blockIdx.x, nullptr);
   }
   const T hj = (T)M.h[j];
   const T* B1 = X.put(pf);
@@ -1259,7 +1291,16 @@ __global__ __launch_bounds__(512) void k_upd_z(Ctx c, int chk, int refine) {
   SQ d[q] = (T)24. * dx3 * (-prv(dpr, q, lf) + dpr[q]);
   psolve<T, SEG>(d, G, L, l, gs, PZ.s22, bf.xf, bf.xb);
   SQ wn[q] = ku * (T)((const TS*)F.uh[2])[ZI(q)] - dt * d[q] - dt * dmp3;
-  if (MIX) SQ wn[q] = (T)F.w[ZI(q)] + wn[q];
+  if (MIX) {
+    // the flow rate of W before the update (FLOW3, :1168)
+    double so = 0.0;
+    SQ {
+      const T wo = (T)F.w[ZI(q)];
+      so += (double)wo;
+      wn[q] = wo + wn[q];
+    }
+    block_sum2(ok ? so * M.dy[j] : 0.0, 0.0, F.flowp + FP_W * F.flowr + blockIdx.x, nullptr);
+  }
   if (ok) {
     SQ {
       F.w[ZI(q)] = wn[q];
@@ -1306,15 +1347,110 @@ __global__ __launch_bounds__(512) void k_upd_z(Ctx c, int chk, int refine) {
       dmax = (dmax != dmax || red[0][q] != red[0][q]) ? NAN : fmax(dmax, red[0][q]);
       cmax = (cmax != cmax || red[1][q] != red[1][q]) ? NAN : fmax(cmax, red[1][q]);
     }
-    atomic_max_pos(&F.red[blockIdx.x & 63], dmax);
-    atomic_max_pos(&F.red[64 + (blockIdx.x & 63)], cmax);
+    atomic_max_pos(&F.mxs[blockIdx.x & 63], dmax);
+    atomic_max_pos(&F.mxs[64 + (blockIdx.x & 63)], cmax);
   }
 }
 
-// final maxima of the 64 slots of k_upd_z into SC_DIVMAX, SC_CFLM
-__global__ void k_max_fin(Ctx c) {
-  const unsigned long long* r = (const unsigned long long*)c.d.red;
+// Sums of the first n[r] entries of NR partial-sum regions (region r at
+// F.flowp + reg[r]*F.flowr) by one block of FIN_THREADS threads, in a fixed
+// order: thread t adds entries t, t+FIN_THREADS, ... (FIN_PER loads of each
+// region in flight before their adds), then the wavefronts' butterflies,
+// then the wavefronts in index order.  The results are valid in thread 0.
+#define FIN_THREADS 1024
+#define FIN_PER 8
+template <int NR>
+__device__ __forceinline__ void region_sums(const Fields& F, const int (&reg)[NR], const int (&n)[NR],
+                                            double (&out)[NR]) {
+  __shared__ double rs[NR][FIN_THREADS / 64];
+  const int t = threadIdx.x;
+  int nmax = 0;
+#pragma unroll
+  for (int r = 0; r < NR; r++) nmax = max(nmax, n[r]);
+  double acc[NR];
+#pragma unroll
+  for (int r = 0; r < NR; r++) acc[r] = 0.0;
+  for (int base = 0; base < nmax; base += FIN_THREADS * FIN_PER) {
+    double v[NR][FIN_PER];
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+#pragma unroll
+      for (int m = 0; m < FIN_PER; m++) {
+        const int e = base + t + m * FIN_THREADS;
+        v[r][m] = e < n[r] ? F.flowp[(long long)reg[r] * F.flowr + e] : 0.0;
+      }
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+#pragma unroll
+      for (int m = 0; m < FIN_PER; m++) acc[r] += v[r][m];
+  }
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    const double a = wave_sum(acc[r]);
+    if ((t & 63) == 0) rs[r][t >> 6] = a;
+  }
+  __syncthreads();
+  if (t == 0)
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      double a = 0.0;
+      for (int w = 0; w < FIN_THREADS / 64; w++) a += rs[r][w];
+      out[r] = a;
+    }
+}
+
+// UPCALC's mean-pressure-gradient corrections (:1079-1093, :1160-1172) from
+// the ADI sweeps' partial sums.  The reference's sums are
+//   DMPRESG = [sum(UH*DY/DX3/DX1 - DT*(DP(I)-DP(I-1))*DY/DX3) - FLOW1]/VOL/DT
+// with FLOW1 = sum(U*DY/DX3/DX1).  The DP differences sum to zero over each
+// periodic x-line, and UH - U is the ADI increment DELTA, so
+//   DMPRESG = sum(DELTA*DY)/DX3/DX1/VOL/DT
+// (likewise DMPRESG3 along z): the same quantity without the cancellation of
+// the two flow sums.  nu / nw: partial counts of the u and w regions; with
+// `flows` the flow rates FLOW1/FLOW3 of U are formed too (FP64 mode, where
+// the ADI sweeps read U).
+__global__ __launch_bounds__(FIN_THREADS) void k_flow_fin(Ctx c, int nu, int nw, int flows) {
+  const Fields F = c.d;
+  const Scal s = c.s;
+  const int reg[4] = {FP_DU, FP_DW, FP_U, FP_W};
+  const int n[4] = {nu, nw, flows ? nu : 0, flows ? nw : 0};
+  double a[4];
+  region_sums<4>(F, reg, n, a);
+  if (threadIdx.x == 0) {
+    const double dmp = a[0] / s.dx3 / s.dx1 / s.vol / s.dt;
+    const double dmp3 = a[1] / s.dx3 / s.dx1 / s.vol / s.dt;
+    F.sc[SC_DMPRESG] = dmp;
+    F.sc[SC_DMPRESG3] = dmp3;
+    F.sc[SC_PRESG] = F.sc[SC_PRESG] + dmp;
+    F.sc[SC_PRESG3] = F.sc[SC_PRESG3] + dmp3;
+    if (flows) {
+      F.sc[SC_FLOW1] = a[2] / s.dx3 / s.dx1;
+      F.sc[SC_FLOW3] = a[3] / s.dx1 / s.dx3;
+    }
+  }
+}
+
+// final maxima of the 64 max slots of k_upd_z into SC_DIVMAX, SC_CFLM, which
+// are cleared again for the next k_upd_z; with `flows` (mixed mode, where the
+// update kernels read the old U) also the flow rates FLOW1/FLOW3 from nu /
+// nw partial sums
+__global__ __launch_bounds__(FIN_THREADS) void k_max_fin(Ctx c, int nu, int nw, int flows) {
+  const Fields F = c.d;
+  if (flows) {
+    const int reg[2] = {FP_U, FP_W};
+    const int n[2] = {nu, nw};
+    double a[2];
+    region_sums<2>(F, reg, n, a);
+    if (threadIdx.x == 0) {
+      F.sc[SC_FLOW1] = a[0] / c.s.dx3 / c.s.dx1;
+      F.sc[SC_FLOW3] = a[1] / c.s.dx1 / c.s.dx3;
+    }
+  }
+  if (threadIdx.x >= 64) return;
+  unsigned long long* r = (unsigned long long*)F.mxs;
   unsigned long long a = r[threadIdx.x], b = r[64 + threadIdx.x];
+  r[threadIdx.x] = 0ull;
+  r[64 + threadIdx.x] = 0ull;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
     const unsigned long long x = __shfl_xor(a, o, 64), y = __shfl_xor(b, o, 64);
@@ -1322,8 +1458,8 @@ __global__ void k_max_fin(Ctx c) {
     b = y > b ? y : b;
   }
   if (threadIdx.x == 0) {
-    ((unsigned long long*)c.d.sc)[SC_DIVMAX] = a;
-    ((unsigned long long*)c.d.sc)[SC_CFLM] = b;
+    ((unsigned long long*)F.sc)[SC_DIVMAX] = a;
+    ((unsigned long long*)F.sc)[SC_CFLM] = b;
   }
 }
 
@@ -1505,12 +1641,19 @@ static void upd_x_t(const Ctx& c, bool chk, int refine) {
 template <typename TS>
 static void upd_z_t(const Ctx& c, bool chk, int refine) {
   const size_t lz = lds_line(c.o.lz, false, sizeof(double));
-  if (chk) zero(c, c.d.red, 128);
   ZSEG(c.o.lz.seg, { KScope ks_(c, KID_UPD_Z); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_upd_z<double, S_, TS>), dim3(zblocks(c)), zthreads(c), lz, c.st, c, chk ? 1 : 0, refine); });
   if (chk) {
     KScope ks_(c, KID_FLOW_FIN);
-    hipLaunchKernelGGL(k_max_fin, dim3(1), dim3(64), 0, c.st, c);
+    hipLaunchKernelGGL(k_max_fin, dim3(1), dim3(FIN_THREADS), 0, c.st, c, xblocks(c), zblocks(c), c.s.mixed ? 1 : 0);
   }
+}
+// DMPRESG/DMPRESG3 (and PRESG += ...) from the partial sums of the last
+// ADI sweeps: u in the z-sweep (blocks of the compute type's z shape), w in
+// the final x-sweep
+void flow_fin(const Ctx& c) {
+  const int nu = c.s.mixed ? zblocks(c, c.o.lz32) : zblocks(c, c.o.lz);
+  KScope ks_(c, KID_FLOW_FIN);
+  hipLaunchKernelGGL(k_flow_fin, dim3(1), dim3(FIN_THREADS), 0, c.st, c, nu, xblocks(c), c.s.mixed ? 0 : 1);
 }
 void upd_x(const Ctx& c, bool chk, bool refine) {
   if (c.s.mixed)

@@ -152,6 +152,9 @@ int c3d_apply_op(c3d_handle *h, int op, int dir, const double *in, double *out);
  * (arrays of C3D_NKID entries) and resets the recorder. */
 #define C3D_NKID 48
 int c3d_timing(c3d_handle *h, int on, int capacity);
+/* restricts the recorder to launches of one kernel id (kid < 0: all), so a
+ * timed loop carries two events per launch of that kernel only */
+int c3d_timing_select(c3d_handle *h, int kid);
 int c3d_kernel_times(c3d_handle *h, double *ms, int *count);
 const char *c3d_kernel_name(int id);
 
