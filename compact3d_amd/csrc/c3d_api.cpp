@@ -99,9 +99,11 @@ static PSolve make_psolve(double a, double b, int seg) {
 }
 
 // line-engine tile: seg = 8 when it divides n (else 2, the other compiled
-// length), G = n/seg segments, L lines per tile.  x-lines: L*G <= 256, with
-// 2 LDS row tiles of n*(L+1) plus the exchange arrays within 64 KB.
-// z-lines: L*G <= 512 and L a multiple of 16 (128-byte rows) up to 64.
+// length), G = n/seg segments, L lines per tile.  x-lines: L*G <= 256 with L
+// a multiple of seg (so the block size is a multiple of n, which the
+// staging's index arithmetic relies on), 2 LDS row tiles of n*(L+1) plus
+// the exchange arrays within 64 KB.  z-lines: L*G <= 512 and L a multiple of
+// 16 (128-byte rows) up to 64.
 static bool make_lshape(int n, bool staged, size_t es, LShape& sh) {
   const int seg = (n % 8 == 0) ? 8 : 2;
   if (n % seg) return false;
@@ -109,9 +111,13 @@ static bool make_lshape(int n, bool staged, size_t es, LShape& sh) {
   const int tmax = staged ? 256 : 512;
   if (G > tmax) return false;
   int L = std::min(64, tmax / G);
-  if (L >= 16) L -= L % 16;
-  if (staged)
-    while (L > 1 && es * (2 * (size_t)n * (L + 1) + 4 * (size_t)G * L) > 65536) L--;
+  if (staged) {
+    L -= L % seg;
+    while (L > seg && es * (2 * (size_t)n * (L + 1) + 4 * (size_t)G * L) > 65536) L -= seg;
+    if (L < seg) return false;
+  } else if (L >= 16) {
+    L -= L % 16;
+  }
   sh.n = n;
   sh.seg = seg;
   sh.G = G;
@@ -477,6 +483,8 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   g.plane = (long long)g.nj * g.nx;
   g.fsz = g.plane * g.nz;
   if (g.nx % 2 || g.nz % 2) return bad("N1-1 and N3-1 must be even");
+  // x-line staging addresses elements with 32-bit offsets
+  if (g.fsz + 2 * g.plane >= (1ll << 31)) return bad("grid too large: a field must hold fewer than 2^31 elements");
   FFTPlan& f = c.f;
   f.nx = g.nx;
   f.nz = g.nz;
@@ -492,7 +500,7 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   build_mesh(h);
   Ops& o = c.o;
   if (!make_lshape(g.nx, true, sizeof(double), o.lx) || !make_lshape(g.nz, false, sizeof(double), o.lz))
-    return bad("N1-1 (N3-1) too long for the line tiles: at most 2048 (4096) points");
+    return bad("N1-1 (N3-1) too long for the line tiles: at most 256 (4096) points");
   if (!make_lshape32(g.nz, o.lz32)) return bad("N3-1 too long for the FP32 line tiles");
   o.ys = make_yshape(g.n2m, 256, 16);
   o.ys32 = make_yshape(g.n2m, 512, 32);

@@ -61,33 +61,37 @@ __device__ __forceinline__ LBuf<T> lbuf(unsigned char* sm, const LShape& sh, boo
 // x-line staging.  Lines ln = ln0 + l (l < L) are rows (j, k) with
 // j = 1 + ln % N2M, k = ln / N2M.  A tile holds n*L = SEG*blockDim values, so
 // each thread moves exactly SEG of them: element e = t + m*blockDim is row
-// e / n, point e % n.  pre() issues the global loads of a tile into
-// registers (so they can be in flight while the previous tile is used);
-// put() writes them to LDS.  Two LDS tiles alternate and every put()/out()
-// starts with a barrier, so a tile stays readable until the second
-// staging operation after the one that filled it begins.
+// e / n, point e % n.  The host makes blockDim a multiple of n (L a multiple
+// of SEG), so a thread's elements share the point p = t % n and lie in rows
+// t/n + m*(blockDim/n): their LDS indices are li0 + m*lstep.  pre() issues
+// the global loads of a tile into registers (so they can be in flight while
+// the previous tile is used); put() writes them to LDS.  Two LDS tiles
+// alternate and every put()/out() starts with a barrier, so a tile stays
+// readable until the second staging operation after the one that filled it
+// begins.  Offsets are 32-bit (fields hold fewer than 2^31 elements).
 template <typename T, int SEG>
 struct XStage {
   T *st0, *st1;
   int sb;
-  int nx, L, LS;
+  int nx, LS;
   int l, p0, pm, pp;   // this thread's line, segment start, and periodic neighbours
-  long long go[SEG];   // global offset of element m (row j), or -1
-  int li[SEG];         // LDS index of element m
-  int jr[SEG];         // row j of element m
+  unsigned go[SEG];    // element offset of element m (row j)
+  unsigned vm;         // bit m set: element m exists (its line is in the grid)
+  int li0, lstep;      // LDS index of element m: li0 + m*lstep
+  unsigned j1;         // bit m set: element m lies in row j = 1
   __device__ __forceinline__ T* next() {
     sb ^= 1;
     return sb ? st1 : st0;
   }
   template <typename S>
   __device__ __forceinline__ void pre(const S* F, int joff, T (&r)[SEG]) const {
-    const long long off = (long long)joff * nx;
-    SQ r[q] = go[q] >= 0 ? (T)F[go[q] + off] : (T)0;
+    const unsigned off = (unsigned)(joff * nx);  // row j+joff exists for every j (ghost planes)
+    SQ r[q] = ((vm >> q) & 1u) ? (T)F[go[q] + off] : (T)0;
   }
   __device__ __forceinline__ const T* put(const T (&r)[SEG]) {
     T* B = next();
     __syncthreads();
-    SQ B[li[q]] = r[q];
+    SQ B[li0 + q * lstep] = r[q];
     __syncthreads();
     return B;
   }
@@ -102,6 +106,9 @@ struct XStage {
     const int p = q < 0 ? pm : (q >= SEG ? pp : p0 + q);
     return B[p * LS + l];
   }
+  __device__ __forceinline__ bool keep(int q, int jmin) const {
+    return ((vm >> q) & 1u) && (jmin <= 1 || !((j1 >> q) & 1u));
+  }
   // the same tile to two arrays (a field and its FP32 shadow)
   template <typename S1, typename S2>
   __device__ __forceinline__ void out2(S1* F1, S2* F2, const T (&v)[SEG], int jmin = 1) {
@@ -109,8 +116,8 @@ struct XStage {
     __syncthreads();
     SQ B[(p0 + q) * LS + l] = v[q];
     __syncthreads();
-    SQ if (go[q] >= 0 && jr[q] >= jmin) {
-      const T x = B[li[q]];
+    SQ if (keep(q, jmin)) {
+      const T x = B[li0 + q * lstep];
       F1[go[q]] = (S1)x;
       F2[go[q]] = (S2)x;
     }
@@ -122,7 +129,7 @@ struct XStage {
     __syncthreads();
     SQ B[(p0 + q) * LS + l] = v[q];
     __syncthreads();
-    SQ if (go[q] >= 0 && jr[q] >= jmin) F[go[q]] = (S)B[li[q]];
+    SQ if (keep(q, jmin)) F[go[q]] = (S)B[li0 + q * lstep];
   }
 };
 
@@ -144,7 +151,6 @@ __device__ __forceinline__ void xstage(XStage<T, SEG>& x, const LBuf<T>& b, cons
   x.st1 = b.st1;
   x.sb = 1;
   x.nx = g.nx;
-  x.L = sh.L;
   x.LS = sh.L + 1;
   x.l = l;
   x.p0 = p0;
@@ -153,14 +159,20 @@ __device__ __forceinline__ void xstage(XStage<T, SEG>& x, const LBuf<T>& b, cons
   const int ln0 = blockIdx.x * sh.L;
   const int nlines = g.n2m * g.nz;
   const int nt = blockDim.x;
+  const int t = threadIdx.x;
+  const int p = t % sh.n, ll0 = t / sh.n;
+  x.lstep = nt / sh.n;
+  x.li0 = p * x.LS + ll0;
+  x.j1 = 0u;
+  x.vm = 0u;
   SQ {
-    const int e = threadIdx.x + q * nt;
-    const int ll = e / sh.n, p = e - ll * sh.n;
+    const int ll = ll0 + q * x.lstep;
     const int ln = ln0 + ll;
     const int jj = 1 + ln % g.n2m;
-    x.go[q] = (ll < sh.L && ln < nlines) ? gidx(g, p, jj, ln / g.n2m) : -1;
-    x.li[q] = p * x.LS + ll;
-    x.jr[q] = jj;
+    const bool v = ll < sh.L && ln < nlines;
+    x.go[q] = v ? (unsigned)gidx(g, p, jj, ln / g.n2m) : 0u;
+    if (v) x.vm |= 1u << q;
+    if (jj == 1) x.j1 |= 1u << q;
   }
 }
 
@@ -193,17 +205,19 @@ __device__ __forceinline__ void xstage(XStage<T, SEG>& x, const LBuf<T>& b, cons
   const int j = 1 + tile / nit;                                       \
   const int i = (tile % nit) * L + l;                                 \
   const bool ok = i < g.nx;                                           \
-  const long long pl = g.plane;                                       \
-  const long long b = gidx(g, ok ? i : 0, j, 0);                      \
-  const long long bm = b + (long long)(p0 == 0 ? n - 1 : p0 - 1) * pl; \
-  const long long bp = b + (long long)(p0 + SEG == n ? 0 : p0 + SEG) * pl;
-#define ZI(q) (b + (long long)(p0 + (q)) * pl)
+  const unsigned pl = (unsigned)g.plane;                              \
+  const unsigned b = (unsigned)gidx(g, ok ? i : 0, j, 0);             \
+  const unsigned bm = b + (unsigned)(p0 == 0 ? n - 1 : p0 - 1) * pl;  \
+  const unsigned bp = b + (unsigned)(p0 + SEG == n ? 0 : p0 + SEG) * pl;
+// element offsets are 32-bit unsigned (fields hold fewer than 2^31
+// elements), which lets loads use a uniform base plus a 32-bit lane offset
+#define ZI(q) (b + (unsigned)(p0 + (q)) * pl)
 #define PZ (sizeof(T) == 4 ? c.o.pz32 : c.o.pz)
 
 template <typename T, int SEG, typename S>
-__device__ __forceinline__ void zload(const S* F, long long b, long long pl, int p0, long long bm, long long bp,
+__device__ __forceinline__ void zload(const S* F, unsigned b, unsigned pl, int p0, unsigned bm, unsigned bp,
                                       T (&v)[SEG], T& left, T& right) {
-  SQ v[q] = (T)F[b + (long long)(p0 + q) * pl];
+  SQ v[q] = (T)F[b + (unsigned)(p0 + q) * pl];
   left = (T)F[bm];
   right = (T)F[bp];
 }
@@ -515,7 +529,7 @@ __global__ __launch_bounds__(512) void k_adi_y(Ctx c) {
 // (each iteration issues the next J's loads before its own arithmetic and
 // HH store, so one row of loads is always in flight)
 template <typename T>
-__device__ __forceinline__ void yfuse_u(const Ctx& c, const double* tb, long long b, long long t2, const YPos& y,
+__device__ __forceinline__ void yfuse_u(const Ctx& c, const double* tb, unsigned b, unsigned t2, const YPos& y,
                                         T (&v)[CHMAX]) {
   const Geo& g = c.g;
   const Fields& F = c.d;
@@ -525,11 +539,11 @@ __device__ __forceinline__ void yfuse_u(const Ctx& c, const double* tb, long lon
   T* __restrict__ hh = (T*)F.hh[0];
   const T* __restrict__ hp = (const T*)F.hhp[0];
   const T rinv = (T)c.s.rinv, dt = (T)c.s.dt;
-  const long long nx = g.nx;
+  const unsigned nx = g.nx;
   const T uN2 = su[b + g.n2 * nx], uW = su[b];
   const T bcw = -(T)F.tauw[t2] / (T)c.m.dy[1] * (T)c.s.ifwall;
   int J = y.j0;
-  long long a = b + J * nx;
+  unsigned a = b + J * nx;
   T um = su[a - nx], u0 = su[a];
   T w1 = vv1[a];
   T up_n = su[a + nx], w2_n = vv1[a + nx], ru_n = ruh[a], hh_n = hh[a], hp_n = hp[a];
@@ -540,7 +554,7 @@ __device__ __forceinline__ void yfuse_u(const Ctx& c, const double* tb, long lon
       a = b + J * nx;
       const T up = up_n, w2 = w2_n, rux = ru_n, hhx = hh_n, hpx = hp_n;
       if (q + 1 < y.len) {
-        const long long a1 = a + nx;
+        const unsigned a1 = a + nx;
         up_n = su[a1 + nx];
         w2_n = vv1[a1 + nx];
         ru_n = ruh[a1];
@@ -564,7 +578,7 @@ __device__ __forceinline__ void yfuse_u(const Ctx& c, const double* tb, long lon
 }
 
 template <typename T>
-__device__ __forceinline__ void yfuse_v(const Ctx& c, const double* tb, long long b, const YPos& y,
+__device__ __forceinline__ void yfuse_v(const Ctx& c, const double* tb, unsigned b, const YPos& y,
                                         T (&v)[CHMAX]) {
   const Fields& F = c.d;
   const T* __restrict__ sv = (const T*)F.sv;
@@ -573,9 +587,9 @@ __device__ __forceinline__ void yfuse_v(const Ctx& c, const double* tb, long lon
   T* __restrict__ hh = (T*)F.hh[1];
   const T* __restrict__ hp = (const T*)F.hhp[1];
   const T rinv = (T)c.s.rinv, dt = (T)c.s.dt;
-  const long long nx = c.g.nx;
+  const unsigned nx = c.g.nx;
   int J = y.j0;
-  long long a = b + J * nx;
+  unsigned a = b + J * nx;
   T vm = sv[a - nx], v0 = sv[a];
   T pm = sp[a - nx];
   T vp_n = sv[a + nx], p0_n = sp[a], ru_n = ruh[a], hh_n = hh[a], hp_n = hp[a];
@@ -586,7 +600,7 @@ __device__ __forceinline__ void yfuse_v(const Ctx& c, const double* tb, long lon
       a = b + J * nx;
       const T vp = vp_n, p0 = p0_n, rux = ru_n, hhx = hh_n, hpx = hp_n;
       if (q + 1 < y.len) {
-        const long long a1 = a + nx;
+        const unsigned a1 = a + nx;
         vp_n = sv[a1 + nx];
         p0_n = sp[a1];
         ru_n = ruh[a1];
@@ -610,7 +624,7 @@ __device__ __forceinline__ void yfuse_v(const Ctx& c, const double* tb, long lon
 }
 
 template <typename T>
-__device__ __forceinline__ void yfuse_w(const Ctx& c, const double* tb, long long b, long long t2, const YPos& y,
+__device__ __forceinline__ void yfuse_w(const Ctx& c, const double* tb, unsigned b, unsigned t2, const YPos& y,
                                         T (&v)[CHMAX]) {
   const Geo& g = c.g;
   const Fields& F = c.d;
@@ -620,11 +634,11 @@ __device__ __forceinline__ void yfuse_w(const Ctx& c, const double* tb, long lon
   T* __restrict__ hh = (T*)F.hh[2];
   const T* __restrict__ hp = (const T*)F.hhp[2];
   const T rinv = (T)c.s.rinv, dt = (T)c.s.dt;
-  const long long nx = g.nx;
+  const unsigned nx = g.nx;
   const T wN2 = sw[b + g.n2 * nx], wW = sw[b];
   const T bcw = -(T)F.taub[t2] / (T)c.m.dy[1] * (T)c.s.ifwall;
   int J = y.j0;
-  long long a = b + J * nx;
+  unsigned a = b + J * nx;
   T wm = sw[a - nx], w0 = sw[a];
   T s1 = vv3[a];
   T wp_n = sw[a + nx], s2_n = vv3[a + nx], ru_n = ruh[a], hh_n = hh[a], hp_n = hp[a];
@@ -635,7 +649,7 @@ __device__ __forceinline__ void yfuse_w(const Ctx& c, const double* tb, long lon
       a = b + J * nx;
       const T wp = wp_n, s2 = s2_n, rux = ru_n, hhx = hh_n, hpx = hp_n;
       if (q + 1 < y.len) {
-        const long long a1 = a + nx;
+        const unsigned a1 = a + nx;
         wp_n = sw[a1 + nx];
         s2_n = vv3[a1 + nx];
         ru_n = ruh[a1];
@@ -684,8 +698,8 @@ __global__ __launch_bounds__(512) void k_rhsy_adiy(Ctx c, int staged) {
   const Metrics M = c.m;
   T* X = (T*)(comp == 0 ? F.uh[0] : comp == 1 ? F.uh[1] : F.uh[2]);
   const T dt = (T)c.s.dt;
-  const long long b = gidx(g, ok ? i : 0, 0, k);
-  const long long t2 = (long long)k * g.nx + (ok ? i : 0);
+  const unsigned b = (unsigned)gidx(g, ok ? i : 0, 0, k);
+  const unsigned t2 = (unsigned)k * g.nx + (ok ? i : 0);
   // this component's coefficient rows, in LDS when they fit
   const int ntab = (g.n2 + 2) * YTAB_W;
   const double* tb = M.ytab + (long long)comp * ntab;
@@ -722,7 +736,7 @@ __global__ __launch_bounds__(512) void k_rhsy_adiy(Ctx c, int staged) {
   if (ok)
 #pragma unroll
     for (int q = 0; q < CHMAX; q++)
-      if (q < y.len) X[b + (long long)(y.j0 + q) * g.nx] = v[q];
+      if (q < y.len) X[b + (unsigned)(y.j0 + q) * g.nx] = v[q];
 }
 
 // --------------------------------------------------- GETUH x / z sweeps
@@ -1319,14 +1333,14 @@ __global__ __launch_bounds__(512) void k_upd_z(Ctx c, int chk, int refine) {
   double dmax = 0.0, cmax = 0.0;
   if (ok) {
     const double dyj = M.dy[j];
-    const long long bi = gidx(g, i + 1 == g.nx ? 0 : i + 1, j, 0);
+    const unsigned bi = (unsigned)gidx(g, i + 1 == g.nx ? 0 : i + 1, j, 0);
     SQ {
-      const long long a = ZI(q);
+      const unsigned a = ZI(q);
       const double dv = F.div[a] + (double)d[q];
       if (MIX) F.div[a] = dv;
       dmax = (dv != dv || dmax != dmax) ? NAN : fmax(dmax, fabs(dv));
       const double wq = (double)wn[q], wq1 = (double)nxt(wn, q, rt);
-      const double cfll = fabs(F.u[a] + F.u[bi + (long long)(p0 + q) * pl]) * 0.5 * s.dx1 +
+      const double cfll = fabs(F.u[a] + F.u[bi + (unsigned)(p0 + q) * pl]) * 0.5 * s.dx1 +
                           fabs(F.v[a] + F.v[a + g.nx]) * 0.5 / dyj + fabs(wq + wq1) * 0.5 * s.dx3;
       cmax = (cfll != cfll || cmax != cmax) ? NAN : fmax(cmax, cfll);
     }
