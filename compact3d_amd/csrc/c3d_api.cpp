@@ -36,6 +36,7 @@ struct c3d_handle {
   std::vector<double> Y, DY, H, HM, HC, HP, DYM, DYC, DYP, FJPA, FJMU, FJMV, PMJ, PCJ, PPJ, AK1, AK3;
   double* dvec = nullptr;       // device block holding all vectors
   double* dytab = nullptr;      // fused y-sweep coefficient table
+  float* dytab32 = nullptr;     // the same rounded to FP32 (mixed mode's FP32 sweep)
   double* dtmp = nullptr;       // scratch field for apply_op
   double* dtmp2 = nullptr;
   std::vector<double> stage;    // host staging buffer
@@ -417,6 +418,8 @@ static int upload_dt_terms(c3d_handle* h) {
   HIPCHK(hipMemcpy(base, ych.data(), ych.size() * sizeof(double), hipMemcpyHostToDevice));
   build_ytab(h, ych, tab);
   HIPCHK(hipMemcpy(h->dytab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+  std::vector<float> tab32(tab.begin(), tab.end());
+  HIPCHK(hipMemcpy(h->dytab32, tab32.data(), tab32.size() * sizeof(float), hipMemcpyHostToDevice));
   return 0;
 }
 
@@ -465,7 +468,6 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   const c3d_config& cf = h->cfg;
   if (cf.n1 < 9 || cf.n2 < 5 || cf.n3 < 9) return bad("grid too small (need N1,N3 >= 9, N2 >= 5)");
   if (cf.precision != C3D_FP64 && cf.precision != C3D_MIXED) return bad("precision must be C3D_FP64 or C3D_MIXED");
-  if (cf.precision == C3D_MIXED && cf.files != 0.0) return bad("C3D_MIXED supports FILES = 0 (DNS) only");
   if (cf.files != 0.0 && cf.files != 2.0) return bad("FILES must be 0 (DNS) or 2 (Smagorinsky)");
   if (cf.wmodel != 0.0 && cf.wmodel != 1.0) return bad("WMODEL must be 0 or 1");
   if (cf.re <= 0 || cf.dt <= 0 || cf.alx <= 0 || cf.alz <= 0) return bad("RE, DT, ALX, ALZ must be > 0");
@@ -502,7 +504,9 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   if (!make_lshape(g.nx, true, sizeof(double), o.lx) || !make_lshape(g.nz, false, sizeof(double), o.lz))
     return bad("N1-1 (N3-1) too long for the line tiles: at most 256 (4096) points");
   if (!make_lshape32(g.nz, o.lz32)) return bad("N3-1 too long for the FP32 line tiles");
-  o.ys = make_yshape(g.n2m, 256, 16);
+  // FP64 y-sweep: 6-point chunks (every operand of a chunk is loaded before
+  // its arithmetic), 16 columns (128-byte rows), up to 512 threads
+  o.ys = make_yshape(g.n2m, 512, 16, 6);
   o.ys32 = make_yshape(g.n2m, 512, 32);
   // Poisson: 8 modes of 16 bytes per row and chunks of 6 points keep its
   // three FP64 chunk arrays small; 12-point chunks beyond N2-1 = 1536
@@ -577,8 +581,11 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
       m.ygam[q] = yc + (3 * q + 2) * nv;
     }
   }
-  if (dalloc(h, &h->dytab, 3 * (size_t)(g.n2 + 2) * YTAB_W)) return bad(h->err);
+  if (dalloc(h, &h->dytab, 3 * (size_t)(g.n2 + 2) * YTAB_W) ||
+      dalloc(h, &h->dytab32, 3 * (size_t)(g.n2 + 2) * YTAB_W))
+    return bad(h->err);
   c.m.ytab = h->dytab;
+  c.m.ytab32 = h->dytab32;
   if (upload_dt_terms(h)) return bad(h->err);
   // FFT twiddles
   {

@@ -30,6 +30,7 @@ struct Metrics {
   //   v:    DYP DYC DYM H RH
   //   all:  slot 11 BET, 12 -A*BET (0 at the first row), 13 -GAM (0 at N2M)
   const double* ytab;
+  const float* ytab32;  // ytab rounded to FP32, for the FP32 sweep
   const double *ak1;  // AK1(I), I = 1..N1M  (index I)
   const double *ak3;  // AK3(K), K = 1..N3M/2+1
   // TDMAI chains of the ADI y-sweeps (component c = 0,1,2): A(J), BET(J), GAM(J)

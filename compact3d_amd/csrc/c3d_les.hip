@@ -242,6 +242,7 @@ __global__ __launch_bounds__(C3D_NT) void k_wallss_fin(Ctx c, int nblk) {
 //   dudz = D1z(u), dvdz = D1z(v) (centre -> z-face)   X33 of :6519-6524, UU31,
 //                                                     VV23, VV32
 #define SG_TILE_LDS 4
+template <typename TH>
 __global__ __launch_bounds__(C3D_NT) void k_sgs_x(Ctx c) {
   extern __shared__ double sm[];
   const Geo g = c.g;
@@ -316,11 +317,12 @@ __global__ __launch_bounds__(C3D_NT) void k_sgs_x(Ctx c) {
   tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
   EL_LOOP(nL) SXDEC;
   (void)pm; (void)pp; (void)j;
-  if (ok) F.hh[0][b + p] = F.hh[0][b + p] - 2. * Wt[p * LS + l];
+  if (ok) ((TH*)F.hh[0])[b + p] = (TH)((double)((TH*)F.hh[0])[b + p] - 2. * Wt[p * LS + l]);
   EL_END;
 #undef SXDEC
 }
 
+template <typename TH>
 __global__ __launch_bounds__(C3D_NT) void k_sgs_z(Ctx c) {
   extern __shared__ double sm[];
   const Geo g = c.g;
@@ -408,7 +410,7 @@ __global__ __launch_bounds__(C3D_NT) void k_sgs_z(Ctx c) {
   tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
   EL_LOOP(nL) SZDEC;
   (void)pm; (void)pp;
-  if (ok) F.hh[2][b + p * pl] = F.hh[2][b + p * pl] - 2. * Wt[p * LS + l];
+  if (ok) ((TH*)F.hh[2])[b + p * pl] = (TH)((double)((TH*)F.hh[2])[b + p * pl] - 2. * Wt[p * LS + l]);
   EL_END;
   __syncthreads();
   // D(SGS*(DU/DZ))/DZ (:6491-6540): HH1 -= D1z(dudz * nue)
@@ -420,7 +422,7 @@ __global__ __launch_bounds__(C3D_NT) void k_sgs_z(Ctx c) {
   tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
   EL_LOOP(nL) SZDEC;
   (void)pm; (void)pp;
-  if (ok) F.hh[0][b + p * pl] = F.hh[0][b + p * pl] - Wt[p * LS + l];
+  if (ok) ((TH*)F.hh[0])[b + p * pl] = (TH)((double)((TH*)F.hh[0])[b + p * pl] - Wt[p * LS + l]);
   EL_END;
   __syncthreads();
   // D(SGS*(DW/DX))/DZ (:6713-6764): HH1 -= D1z(dwdx * nue)
@@ -433,13 +435,14 @@ __global__ __launch_bounds__(C3D_NT) void k_sgs_z(Ctx c) {
   tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
   EL_LOOP(nL) SZDEC;
   (void)pm; (void)pp;
-  if (ok) F.hh[0][b + p * pl] = F.hh[0][b + p * pl] - Wt[p * LS + l];
+  if (ok) ((TH*)F.hh[0])[b + p * pl] = (TH)((double)((TH*)F.hh[0])[b + p * pl] - Wt[p * LS + l]);
   EL_END;
 #undef SZDEC
 }
 
 // z-lines at J >= 2: the two RHS2 terms that need nuz at J-1 and J
 //   D(SGS*(DW/DY))/DZ (:7177-7226), D(SGS*(DV/DZ))/DZ (:7245-7304)
+template <typename TH>
 __global__ __launch_bounds__(C3D_NT) void k_sgs_z2(Ctx c) {
   extern __shared__ double sm[];
   const Geo g = c.g;
@@ -481,7 +484,7 @@ __global__ __launch_bounds__(C3D_NT) void k_sgs_z2(Ctx c) {
   tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
   EL_LOOP(nL) SZDEC;
   (void)pp;
-  if (ok) F.hh[1][b + p * pl] = F.hh[1][b + p * pl] - Wt[p * LS + l];
+  if (ok) ((TH*)F.hh[1])[b + p * pl] = (TH)((double)((TH*)F.hh[1])[b + p * pl] - Wt[p * LS + l]);
   EL_END;
   __syncthreads();
   EL_LOOP(nL) SZDEC;
@@ -492,7 +495,7 @@ __global__ __launch_bounds__(C3D_NT) void k_sgs_z2(Ctx c) {
   tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
   EL_LOOP(nL) SZDEC;
   (void)pp;
-  if (ok) F.hh[1][b + p * pl] = F.hh[1][b + p * pl] - Wt[p * LS + l];
+  if (ok) ((TH*)F.hh[1])[b + p * pl] = (TH)((double)((TH*)F.hh[1])[b + p * pl] - Wt[p * LS + l]);
   EL_END;
 #undef SZDEC
 }
@@ -500,6 +503,7 @@ __global__ __launch_bounds__(C3D_NT) void k_sgs_z2(Ctx c) {
 // x-lines: D(SGS*(DU/DY))/DX and D(SGS*(DV/DX))/DX of RHS2 (:7104-7158,
 // :7323-7384, J >= 2), D(SGS*(DW/DX))/DX and D(SGS*(DU/DZ))/DX of RHS3
 // (:7797-7847, :7972-8024)
+template <typename TH>
 __global__ __launch_bounds__(C3D_NT) void k_sgs_x2(Ctx c) {
   extern __shared__ double sm[];
   const Geo g = c.g;
@@ -540,7 +544,7 @@ __global__ __launch_bounds__(C3D_NT) void k_sgs_x2(Ctx c) {
   tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
   EL_LOOP(nL) SXDEC;
   (void)pp;
-  if (ok && j >= 2) F.hh[1][b + p] = F.hh[1][b + p] - Wt[p * LS + l];
+  if (ok && j >= 2) ((TH*)F.hh[1])[b + p] = (TH)((double)((TH*)F.hh[1])[b + p] - Wt[p * LS + l]);
   EL_END;
   __syncthreads();
   EL_LOOP(nL) SXDEC;
@@ -550,7 +554,7 @@ __global__ __launch_bounds__(C3D_NT) void k_sgs_x2(Ctx c) {
   tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
   EL_LOOP(nL) SXDEC;
   (void)pp;
-  if (ok && j >= 2) F.hh[1][b + p] = F.hh[1][b + p] - Wt[p * LS + l];
+  if (ok && j >= 2) ((TH*)F.hh[1])[b + p] = (TH)((double)((TH*)F.hh[1])[b + p] - Wt[p * LS + l]);
   EL_END;
   __syncthreads();
   EL_LOOP(nL) SXDEC;
@@ -560,7 +564,7 @@ __global__ __launch_bounds__(C3D_NT) void k_sgs_x2(Ctx c) {
   tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
   EL_LOOP(nL) SXDEC;
   (void)pp;
-  if (ok) F.hh[2][b + p] = F.hh[2][b + p] - Wt[p * LS + l];
+  if (ok) ((TH*)F.hh[2])[b + p] = (TH)((double)((TH*)F.hh[2])[b + p] - Wt[p * LS + l]);
   EL_END;
   __syncthreads();
   EL_LOOP(nL) SXDEC;
@@ -570,7 +574,7 @@ __global__ __launch_bounds__(C3D_NT) void k_sgs_x2(Ctx c) {
   tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
   EL_LOOP(nL) SXDEC;
   (void)pp;
-  if (ok) F.hh[2][b + p] = F.hh[2][b + p] - Wt[p * LS + l];
+  if (ok) ((TH*)F.hh[2])[b + p] = (TH)((double)((TH*)F.hh[2])[b + p] - Wt[p * LS + l]);
   EL_END;
 #undef SXDEC
 }
@@ -638,21 +642,30 @@ void wallss(const Ctx& c) {
   hipLaunchKernelGGL(k_wallss_fin, dim3(1), dim3(C3D_NT), 0, c.st, c, nb);
 }
 
-void sgs_terms(const Ctx& c) {
+// TH: storage type of HH (FP32 in mixed mode); the SGS terms are formed in
+// FP64 from the FP64 state and added to HH in FP64 arithmetic
+template <typename TH>
+static void sgs_terms_t(const Ctx& c) {
   {
     KScope ks_(c, KID_SGS_X);
-    hipLaunchKernelGGL(k_sgs_x, dim3(xgrid(c, c.o.tx)), dim3(C3D_NT), lds(c.o.tx, 2), c.st, c);
+    hipLaunchKernelGGL(k_sgs_x<TH>, dim3(xgrid(c, c.o.tx)), dim3(C3D_NT), lds(c.o.tx, 2), c.st, c);
   }
   {
     KScope ks_(c, KID_SGS_Z);
-    hipLaunchKernelGGL(k_sgs_z, dim3(zgrid(c, c.o.tz4, 1)), dim3(C3D_NT), lds(c.o.tz4, 4), c.st, c);
+    hipLaunchKernelGGL(k_sgs_z<TH>, dim3(zgrid(c, c.o.tz4, 1)), dim3(C3D_NT), lds(c.o.tz4, 4), c.st, c);
   }
   {
     KScope ks_(c, KID_SGS_Z2);
-    hipLaunchKernelGGL(k_sgs_z2, dim3(zgrid(c, c.o.tz4, 2)), dim3(C3D_NT), lds(c.o.tz4, 3), c.st, c);
+    hipLaunchKernelGGL(k_sgs_z2<TH>, dim3(zgrid(c, c.o.tz4, 2)), dim3(C3D_NT), lds(c.o.tz4, 3), c.st, c);
   }
   KScope ks_(c, KID_SGS_X2);
-  hipLaunchKernelGGL(k_sgs_x2, dim3(xgrid(c, c.o.tx4)), dim3(C3D_NT), lds(c.o.tx4, 3), c.st, c);
+  hipLaunchKernelGGL(k_sgs_x2<TH>, dim3(xgrid(c, c.o.tx4)), dim3(C3D_NT), lds(c.o.tx4, 3), c.st, c);
+}
+void sgs_terms(const Ctx& c) {
+  if (c.s.mixed)
+    sgs_terms_t<float>(c);
+  else
+    sgs_terms_t<double>(c);
 }
 
 void filter(const Ctx& c, int width, const double* in, double* out, int np, int ld1, int ld3) {

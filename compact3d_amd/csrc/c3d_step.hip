@@ -475,7 +475,7 @@ __device__ __forceinline__ void ychain_bwd(T (&v)[N], MF m, const YPos& y, int C
 
 template <typename T>
 __global__ __launch_bounds__(512) void k_adi_y(Ctx c) {
-  __shared__ T xs[4][sizeof(T) == 4 ? 512 : 256];
+  __shared__ T xs[4][512];  // C*L <= 512 entries each
   const Geo g = c.g;
   const YShape ys = sizeof(T) == 4 ? c.o.ys32 : c.o.ys;
   const int comp = blockIdx.y;
@@ -522,15 +522,15 @@ __global__ __launch_bounds__(512) void k_adi_y(Ctx c) {
 
 // RHS y parts fused into the ADI y-sweep: each thread forms the final RUH of
 // its chunk points and feeds it straight into the TDMAI recurrences; RUH is
-// never stored, HH is.  Without SGS terms the thread walks its chunk in J
-// with the J-1, J, J+1 values of its component's fields in registers (the
-// expressions are those of yterm_u/v/w); with them it calls yterm_*.
-// Returns TDMAI's right-hand side BET(J) * RUH(J) * DT.
-// (each iteration issues the next J's loads before its own arithmetic and
-// HH store, so one row of loads is always in flight)
-template <typename T>
-__device__ __forceinline__ void yfuse_u(const Ctx& c, const double* tb, unsigned b, unsigned t2, const YPos& y,
-                                        T (&v)[CHMAX]) {
+// never stored, HH is.  Without SGS terms every operand of the thread's
+// chunk (its component's field at J0-1..J0+len, the VV1/P/VV3 rows, RUH, HH
+// and HH2) is loaded into registers before any arithmetic, so a thread has
+// all of its loads in flight at once; the expressions are those of
+// yterm_u/v/w.  With SGS terms it calls yterm_*.  Returns TDMAI's right-hand
+// side BET(J) * RUH(J) * DT.
+template <typename T, int CH>
+__device__ __forceinline__ void yfuse_u(const Ctx& c, const T* tb, unsigned b, unsigned t2, const YPos& y,
+                                        T (&v)[CH]) {
   const Geo& g = c.g;
   const Fields& F = c.d;
   const T* __restrict__ su = (const T*)F.su;
@@ -542,44 +542,39 @@ __device__ __forceinline__ void yfuse_u(const Ctx& c, const double* tb, unsigned
   const unsigned nx = g.nx;
   const T uN2 = su[b + g.n2 * nx], uW = su[b];
   const T bcw = -(T)F.tauw[t2] / (T)c.m.dy[1] * (T)c.s.ifwall;
-  int J = y.j0;
-  unsigned a = b + J * nx;
-  T um = su[a - nx], u0 = su[a];
-  T w1 = vv1[a];
-  T up_n = su[a + nx], w2_n = vv1[a + nx], ru_n = ruh[a], hh_n = hh[a], hp_n = hp[a];
+  const unsigned a0 = b + (unsigned)y.j0 * nx;  // row J0
+  T us[CH + 2], w[CH + 1], ru[CH], hx[CH], hq[CH];
 #pragma unroll
-  for (int q = 0; q < CHMAX; q++)
+  for (int q = 0; q < CH + 2; q++) us[q] = (q < y.len + 2) ? su[a0 - nx + q * nx] : (T)0;
+#pragma unroll
+  for (int q = 0; q < CH + 1; q++) w[q] = (q < y.len + 1) ? vv1[a0 + q * nx] : (T)0;
+#pragma unroll
+  for (int q = 0; q < CH; q++) {
+    const bool in = q < y.len;
+    ru[q] = in ? ruh[a0 + q * nx] : (T)0;
+    hx[q] = in ? hh[a0 + q * nx] : (T)0;
+    hq[q] = in ? hp[a0 + q * nx] : (T)0;
+  }
+#pragma unroll
+  for (int q = 0; q < CH; q++)
     if (q < y.len) {
-      J = y.j0 + q;
-      a = b + J * nx;
-      const T up = up_n, w2 = w2_n, rux = ru_n, hhx = hh_n, hpx = hp_n;
-      if (q + 1 < y.len) {
-        const unsigned a1 = a + nx;
-        up_n = su[a1 + nx];
-        w2_n = vv1[a1 + nx];
-        ru_n = ruh[a1];
-        hh_n = hh[a1];
-        hp_n = hp[a1];
-      }
-      const double* t = tb + J * YTAB_W;
-      const T ru = rux + rinv * ((T)t[0] * up - (T)t[1] * u0 + (T)t[2] * um);
+      const int J = y.j0 + q;
+      const T um = us[q], u0 = us[q + 1], up = us[q + 2];
+      const T* t = tb + J * YTAB_W;
+      const T r0 = ru[q] + rinv * ((T)t[0] * up - (T)t[1] * u0 + (T)t[2] * um);
       const T u2 = (T)t[3] * ((T)t[6] * up + (T)t[7] * u0) + (T)t[9] * uN2;
       const T u1 = (T)t[4] * ((T)t[5] * u0 + (T)t[6] * um) + (T)t[10] * uW;
-      const T v2 = (J == g.n2m) ? (T)0 : w2;
-      const T v1 = (J == 1) ? (T)0 : w1;
-      const T h = hhx + (T)t[8] * (v2 * u2 - v1 * u1);
-      hh[a] = h;
-      const T r = ru - (T)(3. / 2.) * h + (T)(1. / 2.) * hpx + (T)t[10] * bcw;
+      const T v2 = (J == g.n2m) ? (T)0 : w[q + 1];
+      const T v1 = (J == 1) ? (T)0 : w[q];
+      const T h = hx[q] + (T)t[8] * (v2 * u2 - v1 * u1);
+      hh[a0 + q * nx] = h;
+      const T r = r0 - (T)(3. / 2.) * h + (T)(1. / 2.) * hq[q] + (T)t[10] * bcw;
       v[q] = (T)t[YT_BET] * (r * dt);
-      um = u0;
-      u0 = up;
-      w1 = w2;
     }
 }
 
-template <typename T>
-__device__ __forceinline__ void yfuse_v(const Ctx& c, const double* tb, unsigned b, const YPos& y,
-                                        T (&v)[CHMAX]) {
+template <typename T, int CH>
+__device__ __forceinline__ void yfuse_v(const Ctx& c, const T* tb, unsigned b, const YPos& y, T (&v)[CH]) {
   const Fields& F = c.d;
   const T* __restrict__ sv = (const T*)F.sv;
   const T* __restrict__ sp = (const T*)F.sp;
@@ -588,44 +583,40 @@ __device__ __forceinline__ void yfuse_v(const Ctx& c, const double* tb, unsigned
   const T* __restrict__ hp = (const T*)F.hhp[1];
   const T rinv = (T)c.s.rinv, dt = (T)c.s.dt;
   const unsigned nx = c.g.nx;
-  int J = y.j0;
-  unsigned a = b + J * nx;
-  T vm = sv[a - nx], v0 = sv[a];
-  T pm = sp[a - nx];
-  T vp_n = sv[a + nx], p0_n = sp[a], ru_n = ruh[a], hh_n = hh[a], hp_n = hp[a];
+  const unsigned a0 = b + (unsigned)y.j0 * nx;
+  T vs[CH + 2], ps[CH + 1], ru[CH], hx[CH], hq[CH];
 #pragma unroll
-  for (int q = 0; q < CHMAX; q++)
+  for (int q = 0; q < CH + 2; q++) vs[q] = (q < y.len + 2) ? sv[a0 - nx + q * nx] : (T)0;
+#pragma unroll
+  for (int q = 0; q < CH + 1; q++) ps[q] = (q < y.len + 1) ? sp[a0 - nx + q * nx] : (T)0;
+#pragma unroll
+  for (int q = 0; q < CH; q++) {
+    const bool in = q < y.len;
+    ru[q] = in ? ruh[a0 + q * nx] : (T)0;
+    hx[q] = in ? hh[a0 + q * nx] : (T)0;
+    hq[q] = in ? hp[a0 + q * nx] : (T)0;
+  }
+#pragma unroll
+  for (int q = 0; q < CH; q++)
     if (q < y.len) {
-      J = y.j0 + q;
-      a = b + J * nx;
-      const T vp = vp_n, p0 = p0_n, rux = ru_n, hhx = hh_n, hpx = hp_n;
-      if (q + 1 < y.len) {
-        const unsigned a1 = a + nx;
-        vp_n = sv[a1 + nx];
-        p0_n = sp[a1];
-        ru_n = ruh[a1];
-        hh_n = hh[a1];
-        hp_n = hp[a1];
-      }
-      const double* t = tb + J * YTAB_W;
+      const int J = y.j0 + q;
+      const T vm = vs[q], v0 = vs[q + 1], vp = vs[q + 2];
+      const T* t = tb + J * YTAB_W;
       const T viscos = rinv * ((T)t[0] * vp - (T)t[1] * v0 + (T)t[2] * vm);
-      const T pressg2 = (p0 - pm) / (T)t[3];
-      const T ru = rux + (-pressg2 + viscos);
+      const T pressg2 = (ps[q + 1] - ps[q]) / (T)t[3];
+      const T r0 = ru[q] + (-pressg2 + viscos);
       const T v2 = (T)0.5 * (vp + v0);
       const T v1 = (T)0.5 * (v0 + vm);
-      const T h = hhx + (T)t[4] * (v2 * v2 - v1 * v1);
-      hh[a] = h;
-      const T r = ru - (T)(3. / 2.) * h + (T)(1. / 2.) * hpx;
+      const T h = hx[q] + (T)t[4] * (v2 * v2 - v1 * v1);
+      hh[a0 + q * nx] = h;
+      const T r = r0 - (T)(3. / 2.) * h + (T)(1. / 2.) * hq[q];
       v[q] = (T)t[YT_BET] * (r * dt);
-      vm = v0;
-      v0 = vp;
-      pm = p0;
     }
 }
 
-template <typename T>
-__device__ __forceinline__ void yfuse_w(const Ctx& c, const double* tb, unsigned b, unsigned t2, const YPos& y,
-                                        T (&v)[CHMAX]) {
+template <typename T, int CH>
+__device__ __forceinline__ void yfuse_w(const Ctx& c, const T* tb, unsigned b, unsigned t2, const YPos& y,
+                                        T (&v)[CH]) {
   const Geo& g = c.g;
   const Fields& F = c.d;
   const T* __restrict__ sw = (const T*)F.sw;
@@ -637,44 +628,40 @@ __device__ __forceinline__ void yfuse_w(const Ctx& c, const double* tb, unsigned
   const unsigned nx = g.nx;
   const T wN2 = sw[b + g.n2 * nx], wW = sw[b];
   const T bcw = -(T)F.taub[t2] / (T)c.m.dy[1] * (T)c.s.ifwall;
-  int J = y.j0;
-  unsigned a = b + J * nx;
-  T wm = sw[a - nx], w0 = sw[a];
-  T s1 = vv3[a];
-  T wp_n = sw[a + nx], s2_n = vv3[a + nx], ru_n = ruh[a], hh_n = hh[a], hp_n = hp[a];
+  const unsigned a0 = b + (unsigned)y.j0 * nx;
+  T ws[CH + 2], s3[CH + 1], ru[CH], hx[CH], hq[CH];
 #pragma unroll
-  for (int q = 0; q < CHMAX; q++)
+  for (int q = 0; q < CH + 2; q++) ws[q] = (q < y.len + 2) ? sw[a0 - nx + q * nx] : (T)0;
+#pragma unroll
+  for (int q = 0; q < CH + 1; q++) s3[q] = (q < y.len + 1) ? vv3[a0 + q * nx] : (T)0;
+#pragma unroll
+  for (int q = 0; q < CH; q++) {
+    const bool in = q < y.len;
+    ru[q] = in ? ruh[a0 + q * nx] : (T)0;
+    hx[q] = in ? hh[a0 + q * nx] : (T)0;
+    hq[q] = in ? hp[a0 + q * nx] : (T)0;
+  }
+#pragma unroll
+  for (int q = 0; q < CH; q++)
     if (q < y.len) {
-      J = y.j0 + q;
-      a = b + J * nx;
-      const T wp = wp_n, s2 = s2_n, rux = ru_n, hhx = hh_n, hpx = hp_n;
-      if (q + 1 < y.len) {
-        const unsigned a1 = a + nx;
-        wp_n = sw[a1 + nx];
-        s2_n = vv3[a1 + nx];
-        ru_n = ruh[a1];
-        hh_n = hh[a1];
-        hp_n = hp[a1];
-      }
-      const double* t = tb + J * YTAB_W;
-      const T ru = rux + rinv * ((T)t[0] * wp - (T)t[1] * w0 + (T)t[2] * wm);
+      const int J = y.j0 + q;
+      const T wm = ws[q], w0 = ws[q + 1], wp = ws[q + 2];
+      const T* t = tb + J * YTAB_W;
+      const T r0 = ru[q] + rinv * ((T)t[0] * wp - (T)t[1] * w0 + (T)t[2] * wm);
       const T w2 = (T)t[3] * ((T)t[7] * w0 + (T)t[6] * wp) + (T)t[9] * wN2;
       const T w1 = (T)t[4] * ((T)t[6] * wm + (T)t[5] * w0) + (T)t[10] * wW;
-      const T v2 = (J == g.n2m) ? (T)0 : s2;
-      const T v1 = (J == 1) ? (T)0 : s1;
-      const T h = hhx + (T)t[8] * (v2 * w2 - v1 * w1);
-      hh[a] = h;
-      const T r = ru - (T)(3. / 2.) * h + (T)(1. / 2.) * hpx + (T)t[10] * bcw;
+      const T v2 = (J == g.n2m) ? (T)0 : s3[q + 1];
+      const T v1 = (J == 1) ? (T)0 : s3[q];
+      const T h = hx[q] + (T)t[8] * (v2 * w2 - v1 * w1);
+      hh[a0 + q * nx] = h;
+      const T r = r0 - (T)(3. / 2.) * h + (T)(1. / 2.) * hq[q] + (T)t[10] * bcw;
       v[q] = (T)t[YT_BET] * (r * dt);
-      wm = w0;
-      w0 = wp;
-      s1 = s2;
     }
 }
 
-template <typename T, bool SGS>
+template <typename T, bool SGS, int CH>
 __global__ __launch_bounds__(512) void k_rhsy_adiy(Ctx c, int staged) {
-  __shared__ T xs[4][sizeof(T) == 4 ? 512 : 256];
+  __shared__ T xs[4][512];  // C*L <= 512 entries each
   extern __shared__ __align__(16) unsigned char smem[];
   const Geo g = c.g;
   const YShape ys = sizeof(T) == 4 ? c.o.ys32 : c.o.ys;
@@ -700,22 +687,23 @@ __global__ __launch_bounds__(512) void k_rhsy_adiy(Ctx c, int staged) {
   const T dt = (T)c.s.dt;
   const unsigned b = (unsigned)gidx(g, ok ? i : 0, 0, k);
   const unsigned t2 = (unsigned)k * g.nx + (ok ? i : 0);
-  // this component's coefficient rows, in LDS when they fit
+  // this component's coefficient rows (in the sweep's precision), in LDS
+  // when they fit
   const int ntab = (g.n2 + 2) * YTAB_W;
-  const double* tb = M.ytab + (long long)comp * ntab;
+  const T* tb = (sizeof(T) == 4 ? (const T*)M.ytab32 : (const T*)M.ytab) + (long long)comp * ntab;
   if (staged) {
-    double* st = (double*)smem;
+    T* st = (T*)smem;
     for (int e = threadIdx.x; e < ntab; e += blockDim.x) st[e] = tb[e];
     __syncthreads();
     tb = st;
   }
-  T v[CHMAX];
+  T v[CH];
 #pragma unroll
-  for (int q = 0; q < CHMAX; q++) v[q] = (T)0;
+  for (int q = 0; q < CH; q++) v[q] = (T)0;
   if (ok && y.len > 0) {
     if constexpr (SGS) {
 #pragma unroll
-      for (int q = 0; q < CHMAX; q++)
+      for (int q = 0; q < CH; q++)
         if (q < y.len) {
           const int J = y.j0 + q;
           const YIdx x = yidx(g, i, J, k);
@@ -724,18 +712,18 @@ __global__ __launch_bounds__(512) void k_rhsy_adiy(Ctx c, int staged) {
           v[q] = (T)tb[J * YTAB_W + YT_BET] * (r * dt);
         }
     } else if (comp == 0) {
-      yfuse_u<T>(c, tb, b, t2, y, v);
+      yfuse_u<T, CH>(c, tb, b, t2, y, v);
     } else if (comp == 1) {
-      yfuse_v<T>(c, tb, b, y, v);
+      yfuse_v<T, CH>(c, tb, b, y, v);
     } else {
-      yfuse_w<T>(c, tb, b, t2, y, v);
+      yfuse_w<T, CH>(c, tb, b, t2, y, v);
     }
   }
   ychain_fwd(v, [&](int q) { return (T)tb[(y.j0 + q) * YTAB_W + YT_M]; }, y, L, xs[0], xs[1]);
   ychain_bwd(v, [&](int q) { return (T)tb[(y.j0 + q) * YTAB_W + YT_G]; }, y, C, L, xs[2], xs[3]);
   if (ok)
 #pragma unroll
-    for (int q = 0; q < CHMAX; q++)
+    for (int q = 0; q < CH; q++)
       if (q < y.len) X[b + (unsigned)(y.j0 + q) * g.nx] = v[q];
 }
 
@@ -1553,15 +1541,20 @@ static void getuh_t(const Ctx& c, bool fused) {
   const YShape& ys = sizeof(T) == 4 ? c.o.ys32 : c.o.ys;
   const int ylx = (c.g.nx + ys.L - 1) / ys.L;
   if (fused) {
-    const size_t tbytes = sizeof(double) * (size_t)(c.g.n2 + 2) * YTAB_W;
+    const size_t tbytes = sizeof(T) * (size_t)(c.g.n2 + 2) * YTAB_W;
     const int staged = tbytes <= 40 * 1024 ? 1 : 0;
+    // chunk length of the shape (u and w have the most points, N2M)
+    const int ch = (c.g.n2m + ys.C - 1) / ys.C;
+    const dim3 gr(ylx * c.g.nz, 3), th(ys.L * ys.C);
+    const size_t lb = staged ? tbytes : 0;
     KScope ks_(c, KID_ADI_Y);
-    if (c.s.files)
-      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rhsy_adiy<T, true>), dim3(ylx * c.g.nz, 3), dim3(ys.L * ys.C),
-                         staged ? tbytes : 0, c.st, c, staged);
-    else
-      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rhsy_adiy<T, false>), dim3(ylx * c.g.nz, 3), dim3(ys.L * ys.C),
-                         staged ? tbytes : 0, c.st, c, staged);
+#define YF_(SGS, CH) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rhsy_adiy<T, SGS, CH>), gr, th, lb, c.st, c, staged)
+    if (c.s.files) {
+      if (ch <= 6) YF_(true, 6); else YF_(true, CHMAX);
+    } else {
+      if (ch <= 6) YF_(false, 6); else YF_(false, CHMAX);
+    }
+#undef YF_
   } else {
     KScope ks_(c, KID_ADI_Y);
     hipLaunchKernelGGL(k_adi_y<T>, dim3(ylx * c.g.nz, 3), dim3(ys.L * ys.C), 0, c.st, c);

@@ -1,16 +1,21 @@
 """Mixed precision (C3D_MIXED) against the FP64 path of the same library,
 which the parity tests pin to the oracle at 1e-12.
 
-Mixed mode keeps the state U, V, W, P in FP64 and runs the derivative and
-filter operators, the RHS, the ADI sweeps, the FFTs and most Poisson modes
-in FP32; the Poisson y-solves, the inverse FFTs, the pressure correction, the
-flow-rate constraint, DIV and CFL stay FP64, and an FP64-residual
-refinement of the projection runs when its DIVMAX exceeds 1e-10.  What is checked, on the same seeded inputs:
+Mixed mode keeps the state U, V, W, P in FP64 and runs the compact
+derivative and interpolation operators, the RHS, the ADI sweeps and their
+intermediates in FP32; RDP, the Poisson solve (FFTs and Thomas solves),
+the pressure correction, the flow-rate constraint, DIV, DIVMAX and CFL stay
+FP64, and an FP64-residual refinement of the projection runs when its
+DIVMAX exceeds 1e-10.  The LES terms (SGSVIS, wall model, SGS fluxes) are
+formed in FP64 from the FP64 state and added to the FP32 HH.  What is
+checked, on the same seeded inputs:
   post-projection DIVMAX of every step                 <= 1e-9
   flow rate after a step equals the one before         <= 1e-12 relative
   velocities after 10 steps, max-norm relative error   <= 1e-5
   pressure after 10 steps                              <= 1e-3
   one RHS phase (RUH), max-norm relative error         <= 1e-5
+  LES (Smagorinsky, both wall models), 10 steps: velocities <= 1e-5,
+  SGSVIS <= 1e-4
 The measured errors are printed (pytest -s) for the record.
 """
 import numpy as np
@@ -85,5 +90,36 @@ def test_mixed_state_roundtrip():
     sb = b.download()
     assert np.array_equal(interior_u(sb["U"], n1, n2, n3), interior_u(U, n1, n2, n3))
     assert np.array_equal(interior_p(sb["P"], n1, n2, n3), interior_p(P, n1, n2, n3))
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("wmodel", [1.0, 0.0])
+def test_mixed_les_trajectory_10_steps(wmodel):
+    """wall-modelled / wall-resolved Smagorinsky LES: mixed against FP64"""
+    from compact3d_amd import c3d
+    from pyoracle import Oracle
+    n1, n2, n3 = 49, 33, 33
+    cfg = config(n1, n2, n3, re=20000.0, dt=5e-4, files=2.0, wmodel=wmodel, cks=1.5)
+    orc = Oracle(**cfg)
+    U, P, _ = inputs(orc, seed=9)
+    chs = []
+    for prec in (c3d.FP64, c3d.MIXED):
+        ch = c3d.Channel(**cfg, precision=prec)
+        ch.upload(U, P)
+        ch.set_scalar(c3d.S_WSM1, 1e-3)
+        ch.set_scalar(c3d.S_WSM3, 1e-4)
+        chs.append(ch)
+    a, b = chs
+    for n in range(10):
+        a.step(n + 1)
+        db, _ = b.step(n + 1)
+        assert db <= 1e-9
+    sa, sb = a.download(), b.download()
+    eu = [relerr(interior_u(sa["U"], n1, n2, n3, c), interior_u(sb["U"], n1, n2, n3, c)) for c in range(3)]
+    es = relerr(interior_u(sa["SGSVIS"], n1, n2, n3), interior_u(sb["SGSVIS"], n1, n2, n3))
+    print("mixed LES wmodel", wmodel, "U max", ["%.2e" % x for x in eu], "SGSVIS %.2e" % es)
+    assert max(eu) <= 1e-5
+    assert es <= 1e-4
     a.close()
     b.close()
