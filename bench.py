@@ -191,9 +191,70 @@ def served_leg(args):
         out[name] = st
     out["served_over_batched"] = out["served"]["throughput_mpts_steps_s"] / out["batched"]["throughput_mpts_steps_s"]
     out["stream"] = ("%d requests, Poisson arrivals 20/s (seed 4), grids 64x49x64..256x129x256, Re_tau "
-                     "{180, 395 mixed precision; 1000 wall-modelled LES FP64}, 100-400 steps, dt 5e-4, "
-                     "%d slots, 10 steps per turn, captured graphs" % (len(reqs), args.serve_slots))
+                     "{180, 395; 1000 wall-modelled LES}, all mixed precision, 100-400 steps, dt 5e-4, "
+                     "%d slots, up to 10 steps per turn, captured graphs on each instance's own stream" % (
+                         len(reqs), args.serve_slots))
     return out
+
+
+def ensemble_leg(args):
+    """Batched uneven instances: wall-modelled Smagorinsky LES channels
+    (Cs 0.1 with van Driest damping, log-law wall stress by Newton iteration
+    at the first off-wall points) with grids drawn by seed 3 from
+    {48x33x48, 64x49x64, 96x65x96, 128x65x128} and Re_tau from
+    {1000, 2000, 5200}, mixed precision, advanced args.ens_steps steps by one
+    captured-graph launch per step for all of them (c3d_ensemble), then the
+    same instances on their own streams (c3d_advance) for comparison."""
+    import numpy as np
+    from compact3d_amd import c3d
+    from compact3d_amd.serve import FieldCache, Request, re_from_re_tau
+    grids = [(48, 33, 48), (64, 49, 64), (96, 65, 96), (128, 65, 128)]
+    re_taus = [1000, 2000, 5200]
+    rng = np.random.default_rng(3)
+    reqs = []
+    for q in range(args.ens_members):
+        g = grids[int(rng.integers(len(grids)))]
+        rt = re_taus[int(rng.integers(len(re_taus)))]
+        reqs.append(Request(rid=q, grid=g, re=re_from_re_tau(rt), steps=args.ens_steps, seed=q % 3,
+                            precision=c3d.MIXED, les=True, dt=5e-4))
+    fields = FieldCache()
+    chans = []
+    for r in reqs:
+        ch = c3d.Channel(**r.config(), precision=r.precision)
+        U, P = fields.get(r, ch.mesh())
+        ch.upload(U, P)
+        chans.append(ch)
+    pts = sum(r.pts for r in reqs)
+    n = args.ens_steps
+    ens = c3d.Ensemble(chans)
+    ens.advance(2)  # capture + warm-up
+    ens.sync()
+    t0 = time.perf_counter()
+    ens.advance(n)
+    dm, cf = ens.sync()
+    t1 = time.perf_counter()
+    ens.close()
+    thr_ens = pts * n / (t1 - t0) / 1e6
+    for ch in chans:
+        ch.advance(2, True)
+    for ch in chans:
+        ch.sync()
+    t2 = time.perf_counter()
+    for ch in chans:
+        ch.advance(n, True)
+    dm2 = [ch.sync()[0] for ch in chans]
+    t3 = time.perf_counter()
+    thr_str = pts * n / (t3 - t2) / 1e6
+    for ch in chans:
+        ch.close()
+    counts = {}
+    for r in reqs:
+        key = "%dx%dx%d" % r.grid
+        counts[key] = counts.get(key, 0) + 1
+    return dict(members=len(reqs), grids=counts, steps=n, precision="mixed", model="Smagorinsky + log-law wall model",
+                pts_per_step=pts, ensemble_ms_per_step=round(1000.0 * (t1 - t0) / n, 4),
+                ensemble_mpts_steps_s=round(thr_ens, 1), streams_mpts_steps_s=round(thr_str, 1),
+                divmax_max=float(max(max(dm), max(dm2))), finite=bool(np.all(np.isfinite(dm)) and np.all(np.isfinite(cf))))
 
 
 def main():
@@ -209,6 +270,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--serve-slots", type=int, default=16)
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched LES ensemble leg")
+    ap.add_argument("--ens-members", type=int, default=48)
+    ap.add_argument("--ens-steps", type=int, default=200)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain device copy of 256 MiB three times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -273,6 +337,7 @@ def main():
     step_frac = 112.0 * pts / r64["dt_s"] / (HBM_PEAK_GBS * 1e9)
 
     served = None if args.no_serve else served_leg(args)
+    ensemble = None if args.no_ensemble else ensemble_leg(args)
 
     cpu = None
     if not args.no_cpu:
@@ -306,6 +371,7 @@ def main():
         "mixed_refinements": refines,
         "mixed_vs_fp64_err": errs,
         "served": served,
+        "ensemble": ensemble,
         "cfl_after": cflm * cfg["dt"],
         "presg_after": st_presg,
         "finite": finite,

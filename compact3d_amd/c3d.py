@@ -78,6 +78,12 @@ def lib():
         L.c3d_get_mesh.argtypes = [C.c_void_p, _dp, _dp]
         L.c3d_filter.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int]
         L.c3d_timing.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.c3d_ensemble_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]
+        L.c3d_ensemble_advance.argtypes = [C.c_void_p, C.c_int]
+        L.c3d_ensemble_sync.argtypes = [C.c_void_p, _dp, _dp]
+        L.c3d_ensemble_destroy.argtypes = [C.c_void_p]
+        L.c3d_ensemble_last_error.restype = C.c_char_p
+        L.c3d_ensemble_last_error.argtypes = [C.c_void_p]
         L.c3d_timing_select.argtypes = [C.c_void_p, C.c_int]
         L.c3d_kernel_times.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_int)]
         L.c3d_kernel_name.restype = C.c_char_p
@@ -291,3 +297,44 @@ class Channel:
         out = np.zeros(self.pshape(), order="F")
         self._ck(self.L.c3d_apply_op(self.h, op, direction, _p(inp), _p(out)), "apply_op")
         return out
+
+
+class Ensemble:
+    """Independent channels advanced together: each step of every member is
+    one branch of a single captured HIP graph, so one launch per step
+    advances all of them and their kernels overlap on the device."""
+
+    def __init__(self, channels):
+        self.L = lib()
+        self.members = list(channels)
+        arr = (C.c_void_p * len(self.members))(*[ch.h for ch in self.members])
+        e = C.c_void_p()
+        if self.L.c3d_ensemble_create(arr, len(self.members), C.byref(e)) != 0:
+            raise C3DError("c3d_ensemble_create: " + self.L.c3d_ensemble_last_error(None).decode())
+        self.e = e
+
+    def _ck(self, rc, what):
+        if rc != 0:
+            raise C3DError(f"{what}: " + self.L.c3d_ensemble_last_error(self.e).decode())
+
+    def advance(self, nsteps):
+        """queue nsteps steps of every member without waiting"""
+        self._ck(self.L.c3d_ensemble_advance(self.e, int(nsteps)), "ensemble_advance")
+
+    def sync(self):
+        """wait; arrays of every member's (DIVMAX, CFLM) after its last step"""
+        d = np.zeros(len(self.members))
+        c = np.zeros(len(self.members))
+        self._ck(self.L.c3d_ensemble_sync(self.e, _p(d), _p(c)), "ensemble_sync")
+        return d, c
+
+    def close(self):
+        if getattr(self, "e", None):
+            self.L.c3d_ensemble_destroy(self.e)
+            self.e = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

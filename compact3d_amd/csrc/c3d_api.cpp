@@ -1072,6 +1072,188 @@ extern "C" int c3d_sync(c3d_handle* h, double* divmax, double* cflm) {
   return 0;
 }
 
+// ------------------------------------------------------------- ensembles
+// Independent channels (any grids, precisions and models) advanced by one
+// graph launch per step: the step of every member is captured as one branch
+// of a single HIP graph (fork from the ensemble's stream, each branch on its
+// member's stream, join back), so the device overlaps the members' kernels
+// and the host issues one launch for all of them.  As for c3d_advance, two
+// graphs cover the two buffer states a step alternates between.
+struct c3d_ensemble {
+  std::vector<c3d_handle*> m;
+  hipStream_t st = nullptr;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> join;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  std::vector<int> gkey[2];
+  std::string err;
+};
+
+static std::vector<int> ens_key(const c3d_ensemble* e) {
+  std::vector<int> k;
+  for (const c3d_handle* h : e->m) k.push_back(buf_key(h));
+  return k;
+}
+
+static void ens_graphs_free(c3d_ensemble* e) {
+  for (int p = 0; p < 2; p++) {
+    if (e->gexec[p]) (void)hipGraphExecDestroy(e->gexec[p]);
+    e->gexec[p] = nullptr;
+    e->gkey[p].clear();
+  }
+}
+
+#define EHIPCHK(x)                                                                        \
+  do {                                                                                    \
+    hipError_t e_ = (x);                                                                  \
+    if (e_ != hipSuccess) {                                                               \
+      e->err = std::string(#x) + ": " + hipGetErrorString(e_);                            \
+      return -1;                                                                          \
+    }                                                                                     \
+  } while (0)
+
+static int ens_capture(c3d_ensemble* e) {
+  ens_graphs_free(e);
+  std::vector<bool> timing;
+  for (c3d_handle* h : e->m) {
+    timing.push_back(h->kt.on);
+    h->kt.on = false;  // no event nodes in the graphs
+  }
+  int rc = 0;
+  for (int p = 0; p < 2 && rc == 0; p++) {
+    e->gkey[p] = ens_key(e);
+    hipGraph_t g = nullptr;
+    if (hipStreamBeginCapture(e->st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      e->err = "hipStreamBeginCapture failed";
+      rc = -1;
+      break;
+    }
+    hipError_t r = hipEventRecord(e->fork, e->st);
+    for (size_t q = 0; q < e->m.size() && r == hipSuccess; q++) {
+      c3d_handle* h = e->m[q];
+      r = hipStreamWaitEvent(h->c.st, e->fork, 0);
+      if (r != hipSuccess) break;
+      step_launch(h);
+      r = hipEventRecord(e->join[q], h->c.st);
+      if (r == hipSuccess) r = hipStreamWaitEvent(e->st, e->join[q], 0);
+    }
+    const hipError_t r2 = hipStreamEndCapture(e->st, &g);
+    if (r == hipSuccess) r = r2;
+    if (r == hipSuccess) r = hipGraphInstantiate(&e->gexec[p], g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    if (r != hipSuccess) {
+      e->err = std::string("ensemble capture: ") + hipGetErrorString(r);
+      rc = -1;
+    }
+  }
+  for (size_t q = 0; q < e->m.size(); q++) e->m[q]->kt.on = timing[q];
+  if (rc) ens_graphs_free(e);
+  return rc;
+}
+
+extern "C" int c3d_ensemble_create(c3d_handle* const* members, int n, c3d_ensemble** out) {
+  if (!out) return -1;
+  *out = nullptr;
+  if (!members || n <= 0) {
+    g_create_err = "an ensemble needs at least one member";
+    return -1;
+  }
+  c3d_ensemble* e = new c3d_ensemble();
+  for (int q = 0; q < n; q++) {
+    if (!members[q]) {
+      delete e;
+      g_create_err = "null ensemble member";
+      return -1;
+    }
+    for (int r = 0; r < q; r++)
+      if (members[r] == members[q]) {
+        delete e;
+        g_create_err = "an ensemble member appears twice";
+        return -1;
+      }
+    e->m.push_back(members[q]);
+  }
+  bool ok = hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) == hipSuccess;
+  e->join.assign(n, nullptr);
+  for (int q = 0; q < n && ok; q++) ok = hipEventCreateWithFlags(&e->join[q], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    (void)c3d_ensemble_destroy(e);
+    g_create_err = "ensemble stream/event creation failed";
+    return -1;
+  }
+  *out = e;
+  return 0;
+}
+
+extern "C" int c3d_ensemble_destroy(c3d_ensemble* e) {
+  if (!e) return 0;
+  if (e->st) (void)hipStreamSynchronize(e->st);
+  ens_graphs_free(e);
+  for (hipEvent_t v : e->join)
+    if (v) (void)hipEventDestroy(v);
+  if (e->fork) (void)hipEventDestroy(e->fork);
+  if (e->st) (void)hipStreamDestroy(e->st);
+  delete e;
+  return 0;
+}
+
+extern "C" const char* c3d_ensemble_last_error(const c3d_ensemble* e) {
+  if (!e) return g_create_err.c_str();
+  return e->err.c_str();
+}
+
+extern "C" int c3d_ensemble_advance(c3d_ensemble* e, int nsteps) {
+  if (!e) return -1;
+  if (nsteps < 0) {
+    e->err = "nsteps must be >= 0";
+    return -1;
+  }
+  if (nsteps == 0) return 0;
+  // the ensemble's stream follows whatever each member has queued
+  for (size_t q = 0; q < e->m.size(); q++) {
+    EHIPCHK(hipEventRecord(e->join[q], e->m[q]->c.st));
+    EHIPCHK(hipStreamWaitEvent(e->st, e->join[q], 0));
+  }
+  for (int n = 0; n < nsteps; n++) {
+    const std::vector<int> key = ens_key(e);
+    int p = (e->gexec[0] && e->gkey[0] == key) ? 0 : (e->gexec[1] && e->gkey[1] == key) ? 1 : -1;
+    if (p < 0) {
+      if (ens_capture(e)) return -1;
+      p = 0;
+    }
+    EHIPCHK(hipGraphLaunch(e->gexec[p], e->st));
+    for (c3d_handle* h : e->m) {
+      swap_hh(h);
+      if (h->c.s.mixed) swap_v(h);
+    }
+  }
+  // and every member's stream follows the ensemble (for c3d_sync, c3d_step, ...)
+  EHIPCHK(hipEventRecord(e->fork, e->st));
+  for (c3d_handle* h : e->m) EHIPCHK(hipStreamWaitEvent(h->c.st, e->fork, 0));
+  const hipError_t r = hipGetLastError();
+  if (r != hipSuccess) {
+    e->err = std::string("ensemble launch: ") + hipGetErrorString(r);
+    return -1;
+  }
+  return 0;
+}
+
+extern "C" int c3d_ensemble_sync(c3d_ensemble* e, double* divmax, double* cflm) {
+  if (!e) return -1;
+  EHIPCHK(hipStreamSynchronize(e->st));
+  for (size_t q = 0; q < e->m.size(); q++) {
+    double dm = 0.0, cf = 0.0;
+    if (c3d_sync(e->m[q], &dm, &cf)) {
+      e->err = std::string("member ") + std::to_string(q) + ": " + e->m[q]->err;
+      return -1;
+    }
+    if (divmax) divmax[q] = dm;
+    if (cflm) cflm[q] = cf;
+  }
+  return 0;
+}
+
 extern "C" int c3d_step(c3d_handle* h, int ntime, double* divmax, double* cflm) {
   (void)ntime;
   if (!h) return -1;

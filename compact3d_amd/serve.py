@@ -181,9 +181,10 @@ def stats(done, wall_s):
                 hbm_state_frac=112.0 * thr * 1e6 / 8e12, wall_s=wall_s)
 
 
-def poisson_requests(n, rate_per_s, grids, re_taus, steps_range, seed=4, precision=1):
+def poisson_requests(n, rate_per_s, grids, re_taus, steps_range, seed=4, precision=1, les_precision=None):
     """A seeded Poisson arrival stream: grids and Re_tau drawn uniformly;
-    Re_tau >= 1000 requests are wall-modelled LES (FP64)."""
+    Re_tau >= 1000 requests are wall-modelled LES (in `les_precision`,
+    default `precision`)."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
@@ -194,5 +195,6 @@ def poisson_requests(n, rate_per_s, grids, re_taus, steps_range, seed=4, precisi
         les = rt >= 1000
         steps = int(rng.integers(steps_range[0], steps_range[1] + 1))
         out.append(Request(rid=q, grid=g, re=re_from_re_tau(rt), steps=steps, seed=q % 3,
-                           precision=0 if les else precision, les=les, t_arrive=t))
+                           precision=(precision if les_precision is None else les_precision) if les else precision,
+                           les=les, t_arrive=t))
     return out

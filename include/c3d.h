@@ -91,6 +91,25 @@ int c3d_advance(c3d_handle *h, int nsteps, int use_graph);
  * step (mixed mode: after the refinement check); either may be NULL */
 int c3d_sync(c3d_handle *h, double *divmax, double *cflm);
 
+/* Ensembles (no counterpart in the reference, which runs one channel per
+ * process): independent handles of any grids, precisions and models whose
+ * steps are captured together as the parallel branches of one HIP graph, so
+ * one launch advances every member by one step (GETUP with DIV, DIVMAX and
+ * CFL, as c3d_advance) and the device overlaps their kernels.  Members stay
+ * owned by the caller and usable on their own; create an ensemble once for
+ * a fixed set of members and keep it while they run (a member must not be
+ * in two ensembles at the same time).  c3d_ensemble_advance queues nsteps
+ * steps without waiting (each member's stream is ordered before and after
+ * them); c3d_ensemble_sync waits and returns every member's DIVMAX and CFLM
+ * (arrays of n, either may be NULL; mixed-mode members apply their
+ * refinement check as in c3d_sync). */
+typedef struct c3d_ensemble c3d_ensemble;
+int c3d_ensemble_create(c3d_handle *const *members, int n, c3d_ensemble **out);
+int c3d_ensemble_advance(c3d_ensemble *e, int nsteps);
+int c3d_ensemble_sync(c3d_ensemble *e, double *divmax, double *cflm);
+int c3d_ensemble_destroy(c3d_ensemble *e);
+const char *c3d_ensemble_last_error(const c3d_ensemble *e);
+
 /* Phase entry points of GETUP, for operator-level parity.  They act on the
  * resident state exactly as the named reference routines do. */
 int c3d_bcond(c3d_handle *h);     /* BCOND  compact3d.f:963-1001 */

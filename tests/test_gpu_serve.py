@@ -1,7 +1,9 @@
 """Ensemble and serving on the GPU: the batched driver (c3d_advance, with and
-without captured graphs) equals stepping one call at a time, and a request's
-result is the same alone and served in a mix of other instances (bit for
-bit: an instance's arithmetic does not depend on what else is resident)."""
+without captured graphs) and the ensemble graph (every member's step a
+branch of one graph launch) equal stepping one call at a time, and a
+request's result is the same alone and served in a mix of other instances
+(bit for bit: an instance's arithmetic does not depend on what else is
+resident)."""
 import numpy as np
 import pytest
 
@@ -66,3 +68,37 @@ def test_served_result_equals_alone():
         assert np.array_equal(st["U"], r.result["U"]), f"request {r.rid}: U differs alone vs served"
         assert np.array_equal(st["P"], r.result["P"]), f"request {r.rid}: P differs alone vs served"
         assert r.result["divmax"] <= 1e-9
+
+
+def test_ensemble_equals_alone():
+    """members of different grids, precisions and models, advanced by one
+    graph launch per step, end bit for bit where each ends stepped alone"""
+    from compact3d_amd import c3d
+    specs = [(_cfg(49, 41, 33), 0, 5), (_cfg(33, 33, 33, les=True, re=20000.0), 1, 6),
+             (_cfg(65, 33, 49), 1, 7), (_cfg(33, 33, 33, les=True, re=20000.0), 0, 8)]
+    ens_ch, alone = [], []
+    for cfg, prec, seed in specs:
+        pair = [c3d.Channel(**cfg, precision=prec) for _ in range(2)]
+        U, P = _field(pair[0], cfg, seed)
+        for ch in pair:
+            ch.upload(U, P)
+        ens_ch.append(pair[0])
+        alone.append(pair[1])
+    ens = c3d.Ensemble(ens_ch)
+    ens.advance(3)
+    ens.advance(4)   # odd counts: the two captured graphs alternate across calls
+    dm, cf = ens.sync()
+    ens.close()
+    for q, ch in enumerate(alone):
+        for n in range(7):
+            d, c = ch.step(n + 1)
+        a, b = ch.download(), ens_ch[q].download()
+        print("member", q, "divmax ensemble %.2e alone %.2e" % (dm[q], d))
+        assert np.array_equal(a["U"], b["U"]) and np.array_equal(a["P"], b["P"]), f"member {q} differs"
+        assert a["PRESG"] == b["PRESG"]
+        assert dm[q] == d and cf[q] == c
+        assert dm[q] <= 1e-9
+    # a member stays usable on its own after the ensemble is released
+    ens_ch[0].step(8)
+    for ch in ens_ch + alone:
+        ch.close()

@@ -105,7 +105,9 @@ def test_oversized_request_is_refused():
 def test_stats_and_request_stream():
     reqs = poisson_requests(50, 20.0, [(64, 49, 64), (96, 65, 96)], [180, 395, 1000], (20, 40), seed=4)
     assert all(b.t_arrive >= a.t_arrive for a, b in zip(reqs, reqs[1:]))
-    assert all(r.precision == (0 if r.les else 1) for r in reqs)
+    assert all(r.precision == 1 for r in reqs)
+    reqs64 = poisson_requests(50, 20.0, [(64, 49, 64)], [180, 395, 1000], (20, 40), seed=4, les_precision=0)
+    assert all(r.precision == (0 if r.les else 1) for r in reqs64)
     assert abs(re_from_re_tau(180.0) - 4200) / 4200 < 0.05  # the reference's RE for Re_tau 180
     for r in reqs:
         r.t_done = r.t_arrive + 0.5
