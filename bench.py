@@ -171,6 +171,13 @@ def served_leg(args):
     from compact3d_amd.serve import FieldCache, Server, poisson_requests, stats
     grids = [(64, 49, 64), (96, 65, 96), (128, 65, 128), (192, 129, 192), (256, 129, 256)]
     reqs = poisson_requests(args.serve_requests, 20.0, grids, [180, 395, 1000], (100, 400), seed=4)
+    if not args.no_autotune:  # tile shapes of every configuration in the stream, before any timing
+        seen = set()
+        for r in reqs:
+            key = (r.grid, r.precision, r.les)
+            if key not in seen:
+                seen.add(key)
+                c3d.autotune(steps=3, **r.config(), precision=r.precision)
     fields = FieldCache()
     for r in reqs:  # seeded initial fields, generated before any timing
         ch = c3d.Channel(**r.config(), precision=r.precision)
@@ -217,6 +224,13 @@ def ensemble_leg(args):
         rt = re_taus[int(rng.integers(len(re_taus)))]
         reqs.append(Request(rid=q, grid=g, re=re_from_re_tau(rt), steps=args.ens_steps, seed=q % 3,
                             precision=c3d.MIXED, les=True, dt=5e-4))
+    tuned = {}
+    if not args.no_autotune:  # one shape set per distinct configuration, before any channel exists
+        for r in reqs:
+            key = "%dx%dx%d" % r.grid
+            if key not in tuned:
+                t = c3d.autotune(steps=4, **r.config(), precision=r.precision)
+                tuned[key] = dict(ms_rule=round(t["ms_rule"], 4), ms_tuned=round(t["ms_tuned"], 4))
     fields = FieldCache()
     chans = []
     for r in reqs:
@@ -252,6 +266,7 @@ def ensemble_leg(args):
         key = "%dx%dx%d" % r.grid
         counts[key] = counts.get(key, 0) + 1
     return dict(members=len(reqs), grids=counts, steps=n, precision="mixed", model="Smagorinsky + log-law wall model",
+                autotune=tuned or None,
                 pts_per_step=pts, ensemble_ms_per_step=round(1000.0 * (t1 - t0) / n, 4),
                 ensemble_mpts_steps_s=round(thr_ens, 1), streams_mpts_steps_s=round(thr_str, 1),
                 divmax_max=float(max(max(dm), max(dm2))), finite=bool(np.all(np.isfinite(dm)) and np.all(np.isfinite(cf))))
@@ -271,6 +286,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--serve-slots", type=int, default=16)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched LES ensemble leg")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the rule-derived tile shapes")
     ap.add_argument("--ens-members", type=int, default=48)
     ap.add_argument("--ens-steps", type=int, default=200)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -301,6 +317,11 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    tune = {}
+    if not args.no_autotune:
+        for name, prec in (("fp64", c3d.FP64), ("mixed", c3d.MIXED)):
+            t = c3d.autotune(steps=4, **cfg, precision=prec)
+            tune[name] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in t.items()}
     probe = c3d.Channel(**cfg)
     m = probe.mesh()  # the grid the library built, for the initial field
     probe.close()
@@ -370,6 +391,7 @@ def main():
         "mixed_divmax_max": rmx["divmax_max"],
         "mixed_refinements": refines,
         "mixed_vs_fp64_err": errs,
+        "autotune": tune or None,
         "served": served,
         "ensemble": ensemble,
         "cfl_after": cflm * cfg["dt"],

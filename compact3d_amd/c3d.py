@@ -78,6 +78,9 @@ def lib():
         L.c3d_get_mesh.argtypes = [C.c_void_p, _dp, _dp]
         L.c3d_filter.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int]
         L.c3d_timing.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.c3d_autotune.argtypes = [C.POINTER(Config), C.c_int, C.POINTER(C.c_int), _dp]
+        L.c3d_autotune_reset.argtypes = []
+        L.c3d_autotune_reset.restype = None
         L.c3d_ensemble_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]
         L.c3d_ensemble_advance.argtypes = [C.c_void_p, C.c_int]
         L.c3d_ensemble_sync.argtypes = [C.c_void_p, _dp, _dp]
@@ -112,6 +115,28 @@ def make_config(n1, n2, n3, re, alx, aly, alz, dt, files=0.0, wmodel=0.0, cks=2.
         cfg.y = cfg._y.ctypes.data_as(_dp)
     cfg.precision, cfg.device, cfg.stream = precision, device, None
     return cfg
+
+
+SHAPE_KEYS = ("x_lines", "z_cols_fp64", "z_cols_fp32", "z_seg_fp32", "y_cols_fp64", "y_chunks_fp64",
+              "y_cols_fp32", "y_chunks_fp32", "pois_cols", "pois_chunks", "fft_x_lines", "fft_z_lines")
+
+
+def autotune(steps=4, **kw):
+    """c3d_autotune: time candidate tile shapes for this configuration and
+    record the fastest for handles created afterwards.  Returns the chosen
+    shapes and ms per step before ('ms_rule') and after ('ms_tuned')."""
+    L = lib()
+    sh = (C.c_int * 12)()
+    ms = np.zeros(2)
+    if L.c3d_autotune(C.byref(make_config(**kw)), int(steps), sh, _p(ms)) != 0:
+        raise C3DError("c3d_autotune: " + L.c3d_last_error(None).decode())
+    out = dict(zip(SHAPE_KEYS, list(sh)))
+    out.update(ms_rule=float(ms[0]), ms_tuned=float(ms[1]))
+    return out
+
+
+def autotune_reset():
+    lib().c3d_autotune_reset()
 
 
 def footprint_bytes(**kw) -> int:

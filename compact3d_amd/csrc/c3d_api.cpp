@@ -10,6 +10,8 @@
 #include <cstdlib>
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -449,6 +451,48 @@ long long c3d_footprint_bytes(const c3d_config* cfg) {
   return bytes + 4 * nx * nz * 8 + (1 << 20);
 }
 
+// ------------------------------------------------------ tuned tile shapes
+// The line-group and tile shapes of the step's kernels: x-line tiles (lx),
+// FP64 and FP32 z-line tiles (lz, lz32), FP64/FP32 fused y-sweep tiles (ys,
+// ys32), Poisson y-solve tiles (yp) and the FFT tiles (tlx, tlz).  c3d_create
+// derives them from the grid by rule; c3d_autotune times candidates and
+// records the fastest in a process-wide table keyed by the configuration,
+// which later c3d_create calls of the same key apply.
+struct Shapes {
+  LShape lx, lz, lz32;
+  YShape ys, ys32, yp;
+  int tlx, tlz;
+};
+static std::mutex g_tune_mu;
+static std::map<std::string, Shapes> g_tuned;
+
+static std::string tune_key(const c3d_config& c) {
+  char b[96];
+  snprintf(b, sizeof b, "%d:%d:%d:%d:%d", c.n1, c.n2, c.n3, c.precision, c.files != 0.0 ? 1 : 0);
+  return b;
+}
+
+static Shapes get_shapes(const c3d_handle* h) {
+  const Ops& o = h->c.o;
+  return Shapes{o.lx, o.lz, o.lz32, o.ys, o.ys32, o.yp, h->c.f.tlx, h->c.f.tlz};
+}
+
+// installs shapes and the segment-dependent periodic solves that go with them
+static void apply_shapes(c3d_handle* h, const Shapes& t) {
+  Ops& o = h->c.o;
+  o.lx = t.lx;
+  o.lz = t.lz;
+  o.lz32 = t.lz32;
+  o.ys = t.ys;
+  o.ys32 = t.ys32;
+  o.yp = t.yp;
+  h->c.f.tlx = t.tlx;
+  h->c.f.tlz = t.tlz;
+  o.px = make_opset(o.lx.seg, o.adix_a, o.adix_b);
+  o.pz = make_opset(o.lz.seg, o.adiz_a, o.adiz_b);
+  o.pz32 = make_opset(o.lz32.seg, o.adiz_a, o.adiz_b);
+}
+
 extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   g_create_err.clear();
   if (!cfg || !out) {
@@ -587,6 +631,11 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   c.m.ytab = h->dytab;
   c.m.ytab32 = h->dytab32;
   if (upload_dt_terms(h)) return bad(h->err);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(tune_key(cf));
+    if (it != g_tuned.end()) apply_shapes(h, it->second);
+  }
   // FFT twiddles
   {
     std::vector<double> a = twiddles(g.nx, -1), b = twiddles(g.nx, +1), cz = twiddles(g.nz, -1),
@@ -1070,6 +1119,165 @@ extern "C" int c3d_sync(c3d_handle* h, double* divmax, double* cflm) {
   if (divmax) *divmax = dm;
   if (cflm) *cflm = cf;
   return 0;
+}
+
+// ------------------------------------------------------------- autotuner
+// Candidate shapes per knob, each within what its kernels assume: x tiles a
+// multiple of the segment with L*G <= 256 and two FP64 staging tiles in
+// 64 KB; z tiles L*G <= 512 with a compiled segment (FP64 8/2, FP32
+// 16/8/2); y tiles of <= 12-point chunks and <= 512 threads (Poisson
+// <= 256); FFT tiles of <= 4096 complex values and <= 64 KB of LDS.
+static std::vector<LShape> cand_lx(int n) {
+  std::vector<LShape> v;
+  const int seg = (n % 8 == 0) ? 8 : 2, G = n / seg;
+  for (int L = seg; L * G <= 256 && L <= 64; L += seg)
+    if (8 * (2 * (size_t)n * (L + 1) + 4 * (size_t)G * L) <= 65536) v.push_back(LShape{n, L, G, seg});
+  return v;
+}
+static std::vector<LShape> cand_lz(int n, bool fp32) {
+  std::vector<LShape> v;
+  const int segs64[] = {8, 2}, segs32[] = {16, 8, 2};
+  const int* segs = fp32 ? segs32 : segs64;
+  const int ns = fp32 ? 3 : 2;
+  for (int q = 0; q < ns; q++) {
+    const int seg = segs[q];
+    if (seg == 2 && !v.empty()) break;  // 2-point segments only when nothing longer fits
+    if (n % seg) continue;
+    const int G = n / seg;
+    for (int L : {16, 32, 64})
+      if (L * G <= 512) v.push_back(LShape{n, L, G, seg});
+  }
+  return v;
+}
+static std::vector<YShape> cand_y(int m, int row, int ntmax) {
+  std::vector<YShape> v;
+  for (int ch : {6, 12})
+    for (int nt : {256, 512}) {
+      if (nt > ntmax) continue;
+      const YShape y = make_yshape(m, nt, row, ch);
+      if (y.L * y.C > ntmax || (m + y.C - 1) / y.C > 12) continue;
+      bool dup = false;
+      for (const YShape& w : v) dup = dup || (w.L == y.L && w.C == y.C);
+      if (!dup) v.push_back(y);
+    }
+  return v;
+}
+static std::vector<int> cand_tl(int n, bool padded) {
+  std::vector<int> v;
+  for (int t = 1; t <= 16; t *= 2)
+    if (n * t <= 4096 && 16 * (size_t)n * (t + (padded ? 1 : 0)) <= 65536) v.push_back(t);
+  return v;
+}
+
+// ms per step of n steps (MAIN's loop: GETUP with DIV, DIVMAX and CFL),
+// queued back to back on the handle's stream
+static double time_steps(c3d_handle* h, int n) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return 1e30;
+  (void)hipEventRecord(a, h->c.st);
+  for (int q = 0; q < n; q++) step_launch(h);
+  (void)hipEventRecord(b, h->c.st);
+  float ms = 0.f;
+  const bool ok = hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&ms, a, b) == hipSuccess &&
+                  hipGetLastError() == hipSuccess;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ok ? (double)ms / n : 1e30;
+}
+
+// best of two timings after one warm-up step
+static double trial(c3d_handle* h, const Shapes& s, int n) {
+  apply_shapes(h, s);
+  step_launch(h);
+  return std::min(time_steps(h, n), time_steps(h, n));
+}
+
+extern "C" int c3d_autotune(const c3d_config* cfg, int steps, int* shapes, double* ms) {
+  g_create_err.clear();
+  if (!cfg) return -1;
+  if (steps < 1) steps = 1;
+  // a scratch instance of the configuration, from the rule-derived shapes
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned.erase(tune_key(*cfg));
+  }
+  c3d_handle* h = nullptr;
+  if (c3d_create(cfg, &h)) return -1;
+  // laminar Poiseuille state u = 1 - y^2 (the step's work does not depend
+  // on the values; this keeps every model's iterations in their usual range)
+  {
+    const int n1 = cfg->n1, n2 = cfg->n2, n3 = cfg->n3;
+    const size_t nu = (size_t)(n1 + 1) * (n2 + 1) * (n3 + 1);
+    std::vector<double> U(3 * nu, 0.0), P((size_t)n1 * n2 * n3, 0.0);
+    const double aly = h->c.s.aly;
+    for (int K = 1; K < n3; K++)
+      for (int J = 1; J < n2; J++)
+        for (int I = 1; I < n1; I++) {
+          const double yc = 0.5 * (h->Y[J] + h->Y[J + 1]) - aly / 2.0;
+          U[fu(h, I, J, K)] = 1.0 - yc * yc;
+        }
+    if (c3d_upload_state(h, U.data(), P.data(), nullptr, nullptr, 0.0, 0.0)) {
+      g_create_err = "autotune: " + h->err;
+      c3d_destroy(h);
+      return -1;
+    }
+  }
+  const Geo& g = h->c.g;
+  const bool mixed = h->c.s.mixed != 0;
+  Shapes best = get_shapes(h);
+  const double t0 = trial(h, best, steps);
+  double tb = t0;
+  auto knob = [&](auto cands, auto set) {
+    for (const auto& cnd : cands) {
+      Shapes s = best;
+      set(s, cnd);
+      const double t = trial(h, s, steps);
+      if (t < tb) {
+        tb = t;
+        best = s;
+      }
+    }
+  };
+  if (mixed)
+    knob(cand_y(g.n2m, 32, 512), [](Shapes& s, const YShape& y) { s.ys32 = y; });
+  else
+    knob(cand_y(g.n2m, 16, 512), [](Shapes& s, const YShape& y) { s.ys = y; });
+  if (mixed) knob(cand_lz(g.nz, true), [](Shapes& s, const LShape& z) { s.lz32 = z; });
+  knob(cand_lz(g.nz, false), [](Shapes& s, const LShape& z) { s.lz = z; });
+  knob(cand_lx(g.nx), [](Shapes& s, const LShape& x) { s.lx = x; });
+  std::vector<YShape> yp;
+  for (int ch : {6, 12})
+    for (int row : {8, 16}) {
+      const YShape y = make_yshape(g.n2m, 256, row, ch);
+      if (y.L * y.C <= 256 && (g.n2m + y.C - 1) / y.C <= 12) yp.push_back(y);
+    }
+  knob(yp, [](Shapes& s, const YShape& y) { s.yp = y; });
+  knob(cand_tl(g.nx, true), [](Shapes& s, int t) { s.tlx = t; });
+  knob(cand_tl(g.nz, false), [](Shapes& s, int t) { s.tlz = t; });
+  c3d_destroy(h);
+  if (!(tb < 1e29)) {
+    g_create_err = "autotune: timing failed";
+    return -1;
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[tune_key(*cfg)] = best;
+  }
+  if (shapes) {
+    const int v[12] = {best.lx.L, best.lz.L, best.lz32.L, best.lz32.seg, best.ys.L, best.ys.C,
+                       best.ys32.L, best.ys32.C, best.yp.L, best.yp.C, best.tlx, best.tlz};
+    for (int q = 0; q < 12; q++) shapes[q] = v[q];
+  }
+  if (ms) {
+    ms[0] = t0;
+    ms[1] = tb;
+  }
+  return 0;
+}
+
+extern "C" void c3d_autotune_reset(void) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tuned.clear();
 }
 
 // ------------------------------------------------------------- ensembles

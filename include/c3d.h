@@ -177,6 +177,22 @@ int c3d_timing_select(c3d_handle *h, int kid);
 int c3d_kernel_times(c3d_handle *h, double *ms, int *count);
 const char *c3d_kernel_name(int id);
 
+/* Tile-shape autotuner (no counterpart in the reference).  Times candidate
+ * line-group and tile shapes of the step's kernels (x-, z- and y-line
+ * tiles, Poisson y tiles, FFT tiles) on a scratch instance of this
+ * configuration, `steps` steps per candidate, one knob after another from
+ * the rule-derived shapes, and records the fastest set in a process-wide
+ * table keyed by (N1, N2, N3, precision, FILES != 0).  Handles created
+ * afterwards with that key use it; existing handles keep theirs.  Shapes
+ * change only the rounding order of the line solves.  shapes (12 ints, may
+ * be NULL): x lines per tile, FP64 z columns, FP32 z columns, FP32 z
+ * segment, FP64 y columns and chunks, FP32 y columns and chunks, Poisson y
+ * columns and chunks, x- and z-FFT lines per tile; ms (2, may be NULL):
+ * ms per step with the rule-derived and with the chosen shapes. */
+int c3d_autotune(const c3d_config *cfg, int steps, int *shapes, double *ms);
+/* forgets every tuned shape set */
+void c3d_autotune_reset(void);
+
 /* device memory held by one handle, bytes (for admission by footprint) */
 long long c3d_footprint_bytes(const c3d_config *cfg);
 
