@@ -272,6 +272,82 @@ def ensemble_leg(args):
                 divmax_max=float(max(max(dm), max(dm2))), finite=bool(np.all(np.isfinite(dm)) and np.all(np.isfinite(cf))))
 
 
+# kernel base names of the step -> bench keys
+KERNEL_KEYS = {"k_rhsy_adiy": "adi_y", "k_rhs_x": "rhs_x", "k_rhs_x2": "rhs_x2", "k_rhs_z": "rhs_z",
+               "k_rhs_zw": "rhs_zw", "k_adi_x": "adi_x", "k_adi_z": "adi_z", "k_rhsdp_x": "rhsdp_x",
+               "k_rhsdp_z": "rhsdp_z", "k_fft_x": "fft_x", "k_fft_z": "fft_z", "k_pois_y": "pois_y",
+               "k_fft_xinv": "fft_xinv", "k_upd_x": "upd_x", "k_upd_z": "upd_z"}
+# adi_x runs twice per step; its per-launch algorithmic bytes are the mean
+ADI_X_ALG = {"fp64": ALG_BYTES_ADI_X_PER_STEP / 2.0, "mixed": 12.0}
+
+
+def summarize_counters(rd_csv, wr_csv, grid, out_json):
+    """HBM bytes per launch of every step kernel from two rocprofv3 --pmc
+    runs of `bench.py --copy-calibration --no-autotune --no-cpu --no-serve
+    --no-ensemble` (reads: TCC_EA0_RDREQ_{32,64,128}B request counts; writes:
+    WRITE_SIZE in KiB), scaled by what the same counters read for the plain
+    256 MiB copy that --copy-calibration runs first.  Dispatches before the
+    first FP32 kernel belong to the FP64 leg, the rest to the mixed leg."""
+    import csv
+    import re
+
+    def load(path):
+        rows = {}
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                d = int(r["Dispatch_Id"])
+                rec = rows.setdefault(d, {"name": r["Kernel_Name"]})
+                rec[r["Counter_Name"]] = rec.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+        return rows
+
+    rd, wr = load(rd_csv), load(wr_csv)
+
+    def rbytes(r):
+        return sum(float(r.get("TCC_EA0_RDREQ_%dB_sum" % b, r.get("TCC_EA0_RDREQ_%dB" % b, 0.0))) * b
+                   for b in (32, 64, 128))
+
+    def wbytes(r):
+        return float(r.get("WRITE_SIZE", 0.0)) * 1024.0
+
+    copy_bytes = 256 * 1024 * 1024
+    cr = [d for d in sorted(rd) if "copyBuffer" in rd[d]["name"] or "copy" in rd[d]["name"].lower()]
+    cw = [d for d in sorted(wr) if "copyBuffer" in wr[d]["name"] or "copy" in wr[d]["name"].lower()]
+    rcopy = max(rbytes(rd[d]) for d in cr)
+    wcopy = max(wbytes(wr[d]) for d in cw)
+    rf, wf = copy_bytes / rcopy, copy_bytes / wcopy
+
+    def legs(rows):
+        first32 = min((d for d in rows if "<float" in rows[d]["name"]), default=None)
+        out = {}
+        for d, r in rows.items():
+            m = re.search(r"(k_\w+)", r["name"])
+            if not m or m.group(1) not in KERNEL_KEYS:
+                continue
+            leg = "fp64" if (first32 is None or d < first32) else "mixed"
+            out.setdefault((KERNEL_KEYS[m.group(1)], leg), []).append(r)
+        return out
+
+    lr, lw = legs(rd), legs(wr)
+    n1m, n2, n3m = grid
+    pts = n1m * (n2 - 1) * n3m
+    res = {"_calibration": {"copy_bytes": copy_bytes, "read_bytes_of_copy": rcopy, "write_counter_bytes_of_copy": wcopy,
+                            "read_factor": round(rf, 4), "write_factor": round(wf, 4)}}
+    gname = "%dx%dx%d" % grid
+    for key in sorted(lr):
+        if key not in lw:
+            continue
+        k, leg = key
+        r = rf * sum(rbytes(x) for x in lr[key]) / len(lr[key])
+        w = wf * sum(wbytes(x) for x in lw[key]) / len(lw[key])
+        table = ALG_BYTES if leg == "fp64" else ALG_BYTES_MIXED
+        alg = (ADI_X_ALG[leg] if k == "adi_x" else table[k]) * pts
+        res["%s@%s@%s" % (k, gname, leg)] = {"bytes_per_launch": int(r + w), "fetch": int(r), "write": int(w),
+                                              "alg_bytes": int(alg), "traffic_over_alg": round((r + w) / alg, 3)}
+    with open(out_json, "w") as f:
+        json.dump(res, f, indent=1)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -292,7 +368,18 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain device copy of 256 MiB three times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
+    ap.add_argument("--summarize-counters", nargs=2, metavar=("READ_CSV", "WRITE_CSV"),
+                    help="write profiles/traffic.json from two rocprofv3 --pmc counter CSVs and exit")
     args = ap.parse_args()
+    if args.summarize_counters:
+        grid = tuple(int(x) for x in args.grid.lower().split("x"))
+        res = summarize_counters(args.summarize_counters[0], args.summarize_counters[1], grid,
+                                 os.path.join(ROOT, "profiles", "traffic.json"))
+        for k, v in res.items():
+            if not k.startswith("_"):
+                print("%-28s %8.1f MB  (alg %8.1f MB, %.3f)" % (k, v["bytes_per_launch"] / 1e6, v["alg_bytes"] / 1e6,
+                                                              v["traffic_over_alg"]))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU decomposition is a design "
                                    "only.  Use --gpus 1."}))

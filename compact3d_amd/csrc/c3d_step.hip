@@ -1068,9 +1068,13 @@ __global__ __launch_bounds__(256) void k_pois_y(Ctx c, const void* spec_in, void
   const int ch = (g.n2m + C - 1) / C;
   y.j0 = y.c * ch;  // jm index (J - 1)
   y.len = max(0, min(ch, g.n2m - y.j0));
+  // XCD-aware tile order: the FP64 factor rows of 8 modes are half a
+  // 128-byte line, shared with the neighbouring tile, which then runs on the
+  // same L2
   const int nit = (g.nxh + L - 1) / L;
-  const int it = blockIdx.x % nit;
-  const int kz = blockIdx.x / nit;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int it = tile % nit;
+  const int kz = tile / nit;
   const int kx = it * L + l;
   const bool ok = kx < g.nxh;
   const long long jstr = (long long)g.nz * g.nxhp;

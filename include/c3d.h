@@ -31,11 +31,12 @@ typedef struct c3d_handle c3d_handle;
 
 enum c3d_precision {
   C3D_FP64 = 0,  /* every operator in FP64 */
-  /* FP32 derivative/filter operators, RHS, ADI sweeps and forward FFTs;
-   * FP64 state (U, P), pressure correction, Poisson y-solves and inverse
-   * FFTs, flow-rate constraint, DIVCHECK and CFL; an FP64-residual
-   * refinement of the projection when its DIVMAX exceeds 1e-10.
-   * FILES = 0 only. */
+  /* FP32 compact derivative and interpolation operators, RHS, ADI sweeps
+   * and their intermediates (the line kernels read FP32 shadows of the
+   * state); FP64 state (U, P), RDP, the whole Poisson solve (FFTs and
+   * Thomas solves), the pressure correction, the flow-rate constraint,
+   * DIVCHECK, CFL and the LES terms; an FP64-residual refinement of the
+   * projection when its DIVMAX exceeds 1e-10. */
   C3D_MIXED = 1
 };
 
