@@ -975,7 +975,9 @@ __device__ __forceinline__ void fft_lines(cplx_t<T>* a, int n, int nl, int ls, i
 }
 
 // forward x-DFT of RDP rows (J=1..N2M, K), 2*TL rows per block; line
-// stride TL+1 in LDS so that the row-wise fill and drain hit distinct banks
+// stride TL+1 in LDS so that the row-wise fill and drain hit distinct banks.
+// The FFT kernels copy their twiddle table (n complex values) into LDS
+// behind the tile, so the stages read twiddles from LDS.
 template <typename T, int NE>
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_x(Ctx c) {
   static_assert(sizeof(T) == 8, "the twiddle tables are FP64");
@@ -984,6 +986,8 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x(Ctx c) {
   const Geo g = c.g;
   const int n = g.nx, NL = c.f.tlx, LS = NL + 1;  // NL complex lines = 2*NL rows
   zc* A = (zc*)smem;
+  zc* TW = A + (size_t)n * LS;
+  for (int e = threadIdx.x; e < n; e += FFT_THREADS) TW[e] = ((const zc*)c.f.twxf)[e];
   const int nlines = g.n2m * g.nz;
   const int ln0 = blockIdx.x * 2 * NL;
   const T* src = (const T*)c.d.rdp;
@@ -996,7 +1000,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x(Ctx c) {
     A[p * LS + l] = {xa, xb};
   }
   __syncthreads();
-  fft_lines<T, NE>(A, n, NL, LS, c.f.codex, c.f.nradx, (const zc*)c.f.twxf, -1);
+  fft_lines<T, NE>(A, n, NL, LS, c.f.codex, c.f.nradx, TW, -1);
   const int nxh = g.nxh;
   zc* S = (zc*)c.d.spec;
 #pragma unroll 2
@@ -1022,6 +1026,11 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_z(Ctx c, int sign, void* sp
   const Geo g = c.g;
   const int n = g.nz, L = c.f.tlz, LS = L;
   zc* A = (zc*)smem;
+  zc* TW = A + (size_t)n * LS;
+  {
+    const zc* twg = (const zc*)(sign < 0 ? c.f.twzf : c.f.twzb);
+    for (int e = threadIdx.x; e < n; e += FFT_THREADS) TW[e] = twg[e];
+  }
   const int nit = (g.nxh + L - 1) / L;
   const int it = blockIdx.x % nit;
   const int jm = blockIdx.x / nit;
@@ -1034,8 +1043,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_z(Ctx c, int sign, void* sp
     A[p * LS + l] = kx < g.nxh ? S[(long long)p * g.nxhp + kx] : zc{(T)0, (T)0};
   }
   __syncthreads();
-  const zc* tw = (const zc*)(sign < 0 ? c.f.twzf : c.f.twzb);
-  fft_lines<T, NE>(A, n, L, LS, c.f.codez, c.f.nradz, tw, sign);
+  fft_lines<T, NE>(A, n, L, LS, c.f.codez, c.f.nradz, TW, sign);
 #pragma unroll 4
   for (int e = threadIdx.x; e < tot; e += FFT_THREADS) {
     const int l = e % L, p = e / L;
@@ -1155,6 +1163,8 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_xinv(Ctx c, const void* spe
   const Geo g = c.g;
   const int n = g.nx, NL = c.f.tlx, LS = NL + 1;
   zc* A = (zc*)smem;
+  zc* TW = A + (size_t)n * LS;
+  for (int e = threadIdx.x; e < n; e += FFT_THREADS) TW[e] = ((const zc*)c.f.twxb)[e];
   const int nlines = g.n2m * g.nz;
   const int ln0 = blockIdx.x * 2 * NL;
   const zc* S = (const zc*)spec;
@@ -1180,7 +1190,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_xinv(Ctx c, const void* spe
     A[kx * LS + l] = {xa.x - xb.y, xa.y + xb.x};
   }
   __syncthreads();
-  fft_lines<T, NE>(A, n, NL, LS, c.f.codex, c.f.nradx, (const zc*)c.f.twxb, +1);
+  fft_lines<T, NE>(A, n, NL, LS, c.f.codex, c.f.nradx, TW, +1);
   const T norm = (T)(1.0 / ((double)g.nx * (double)g.nz));
   double* dp = c.d.dp;
 #pragma unroll 4
@@ -1606,8 +1616,8 @@ static void getdp_fp64(const Ctx& c) {
   const Geo& g = c.g;
   const FFTPlan& f = c.f;
   // one in-place buffer per block; x lines padded to a stride of TL+1
-  const size_t ldsx = sizeof(cplx_t<T>) * (size_t)g.nx * (f.tlx + 1);
-  const size_t ldsz = sizeof(cplx_t<T>) * (size_t)g.nz * f.tlz;
+  const size_t ldsx = sizeof(cplx_t<T>) * (size_t)g.nx * (f.tlx + 2);
+  const size_t ldsz = sizeof(cplx_t<T>) * (size_t)g.nz * (f.tlz + 1);
   const int gx = (g.n2m * g.nz + 2 * f.tlx - 1) / (2 * f.tlx);
   const int gz = ((g.nxh + f.tlz - 1) / f.tlz) * g.n2m;
   const int gy = ((g.nxh + c.o.yp.L - 1) / c.o.yp.L) * g.nz;
