@@ -539,7 +539,7 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   f.codex = f.codez = 0;
   for (int q = 0; q < f.nradx; q++) f.codex |= f.radx[q] << (3 * q);
   for (int q = 0; q < f.nradz; q++) f.codez |= f.radz[q] << (3 * q);
-  f.tlx = std::max(1, 2048 / g.nx);
+  f.tlx = std::max(1, 1024 / g.nx);  // x-FFT tiles of <= 1024 values on 256 threads
   f.tlz = std::max(1, 2048 / g.nz);
   f.tly = std::max(1, std::min(32, pow2_floor((int)(65536 / (24 * (size_t)g.n2m)))));
   if (g.n2m > 3072) return bad("N2 too large for the y tiles (N2-1 <= 3072)");
@@ -1126,7 +1126,9 @@ extern "C" int c3d_sync(c3d_handle* h, double* divmax, double* cflm) {
 // multiple of the segment with L*G <= 256 and two FP64 staging tiles in
 // 64 KB; z tiles L*G <= 512 with a compiled segment (FP64 8/2, FP32
 // 16/8/2); y tiles of <= 12-point chunks and <= 512 threads (Poisson
-// <= 256); FFT tiles of <= 4096 complex values and <= 64 KB of LDS.
+// <= 256); FFT tiles of <= 4096 complex values and <= 64 KB of LDS (the
+// tile, its padding and the twiddle table), on 256 threads up to 1024
+// values.
 static std::vector<LShape> cand_lx(int n) {
   std::vector<LShape> v;
   const int seg = (n % 8 == 0) ? 8 : 2, G = n / seg;
@@ -1165,7 +1167,7 @@ static std::vector<YShape> cand_y(int m, int row, int ntmax) {
 static std::vector<int> cand_tl(int n, bool padded) {
   std::vector<int> v;
   for (int t = 1; t <= 16; t *= 2)
-    if (n * t <= 4096 && 16 * (size_t)n * (t + (padded ? 1 : 0)) <= 65536) v.push_back(t);
+    if (n * t <= 4096 && 16 * (size_t)n * (t + (padded ? 2 : 1)) <= 65536) v.push_back(t);
   return v;
 }
 

@@ -912,21 +912,20 @@ __device__ __forceinline__ void dft_r(cplx_t<T> (&v)[R], int sign) {
 
 // one Stockham stage of radix R over nl lines of n points held in LDS at
 // a[p * ls + l], in place: every thread first gathers its butterflies into
-// registers (n*nl <= NE complex values per block of FFT_THREADS
+// registers (n*nl <= NE complex values per block of NT
 // threads), then all write back after a barrier.  One buffer instead of two
 // halves the LDS of a block.
-#define FFT_THREADS 512
-template <typename T, int R, int NE>
+template <typename T, int R, int NE, int NT>
 __device__ __forceinline__ void fft_stage(cplx_t<T>* a, int n, int nl, int ls, int ns, const cplx_t<T>* tw,
                                           int sign) {
-  constexpr int IT = (NE / R + FFT_THREADS - 1) / FFT_THREADS;
+  constexpr int IT = (NE / R + NT - 1) / NT;
   const int nb = n / R;
   const int tot = nb * nl;
   const int tq = n / (ns * R);
   cplx_t<T> v[IT][R];
 #pragma unroll
   for (int it = 0; it < IT; it++) {
-    const int e = threadIdx.x + it * FFT_THREADS;
+    const int e = threadIdx.x + it * NT;
     if (e < tot) {
       const int l = e % nl;
       const int jb = e / nl;
@@ -942,7 +941,7 @@ __device__ __forceinline__ void fft_stage(cplx_t<T>* a, int n, int nl, int ls, i
   __syncthreads();
 #pragma unroll
   for (int it = 0; it < IT; it++) {
-    const int e = threadIdx.x + it * FFT_THREADS;
+    const int e = threadIdx.x + it * NT;
     if (e < tot) {
       const int l = e % nl;
       const int jb = e / nl;
@@ -956,20 +955,20 @@ __device__ __forceinline__ void fft_stage(cplx_t<T>* a, int n, int nl, int ls, i
 
 // mixed-radix transform in place; radices packed 3 bits each in `code`,
 // `nrad` stages
-template <typename T, int NE>
+template <typename T, int NE, int NT>
 __device__ __forceinline__ void fft_lines(cplx_t<T>* a, int n, int nl, int ls, int code, int nrad,
                                           const cplx_t<T>* tw, int sign) {
   int ns = 1;
   for (int s = 0; s < nrad; s++) {
     const int R = (code >> (3 * s)) & 7;
     if (R == 4)
-      fft_stage<T, 4, NE>(a, n, nl, ls, ns, tw, sign);
+      fft_stage<T, 4, NE, NT>(a, n, nl, ls, ns, tw, sign);
     else if (R == 2)
-      fft_stage<T, 2, NE>(a, n, nl, ls, ns, tw, sign);
+      fft_stage<T, 2, NE, NT>(a, n, nl, ls, ns, tw, sign);
     else if (R == 3)
-      fft_stage<T, 3, NE>(a, n, nl, ls, ns, tw, sign);
+      fft_stage<T, 3, NE, NT>(a, n, nl, ls, ns, tw, sign);
     else
-      fft_stage<T, 5, NE>(a, n, nl, ls, ns, tw, sign);
+      fft_stage<T, 5, NE, NT>(a, n, nl, ls, ns, tw, sign);
     ns *= R;
   }
 }
@@ -978,8 +977,8 @@ __device__ __forceinline__ void fft_lines(cplx_t<T>* a, int n, int nl, int ls, i
 // stride TL+1 in LDS so that the row-wise fill and drain hit distinct banks.
 // The FFT kernels copy their twiddle table (n complex values) into LDS
 // behind the tile, so the stages read twiddles from LDS.
-template <typename T, int NE>
-__global__ __launch_bounds__(FFT_THREADS) void k_fft_x(Ctx c) {
+template <typename T, int NE, int NT>
+__global__ __launch_bounds__(NT) void k_fft_x(Ctx c) {
   static_assert(sizeof(T) == 8, "the twiddle tables are FP64");
   extern __shared__ __align__(16) unsigned char smem[];
   typedef cplx_t<T> zc;
@@ -987,12 +986,12 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x(Ctx c) {
   const int n = g.nx, NL = c.f.tlx, LS = NL + 1;  // NL complex lines = 2*NL rows
   zc* A = (zc*)smem;
   zc* TW = A + (size_t)n * LS;
-  for (int e = threadIdx.x; e < n; e += FFT_THREADS) TW[e] = ((const zc*)c.f.twxf)[e];
+  for (int e = threadIdx.x; e < n; e += NT) TW[e] = ((const zc*)c.f.twxf)[e];
   const int nlines = g.n2m * g.nz;
   const int ln0 = blockIdx.x * 2 * NL;
   const T* src = (const T*)c.d.rdp;
 #pragma unroll 4
-  for (int e = threadIdx.x; e < NL * n; e += FFT_THREADS) {
+  for (int e = threadIdx.x; e < NL * n; e += NT) {
     const int l = e / n, p = e - l * n;
     const int la = ln0 + l, lb = ln0 + NL + l;
     const T xa = la < nlines ? src[gidx(g, p, 1 + la % g.n2m, la / g.n2m)] : (T)0;
@@ -1000,11 +999,11 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x(Ctx c) {
     A[p * LS + l] = {xa, xb};
   }
   __syncthreads();
-  fft_lines<T, NE>(A, n, NL, LS, c.f.codex, c.f.nradx, TW, -1);
+  fft_lines<T, NE, NT>(A, n, NL, LS, c.f.codex, c.f.nradx, TW, -1);
   const int nxh = g.nxh;
   zc* S = (zc*)c.d.spec;
 #pragma unroll 2
-  for (int e = threadIdx.x; e < NL * nxh; e += FFT_THREADS) {
+  for (int e = threadIdx.x; e < NL * nxh; e += NT) {
     const int l = e / nxh, kx = e - l * nxh;
     const int la = ln0 + l, lb = ln0 + NL + l;
     const zc z1 = A[kx * LS + l];
@@ -1018,8 +1017,8 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_x(Ctx c) {
 }
 
 // z-DFT of the half spectrum, columns (jm, kx); sign -1 forward, +1 backward
-template <typename T, int NE>
-__global__ __launch_bounds__(FFT_THREADS) void k_fft_z(Ctx c, int sign, void* spec) {
+template <typename T, int NE, int NT>
+__global__ __launch_bounds__(NT) void k_fft_z(Ctx c, int sign, void* spec) {
   static_assert(sizeof(T) == 8, "the twiddle tables are FP64");
   extern __shared__ __align__(16) unsigned char smem[];
   typedef cplx_t<T> zc;
@@ -1029,7 +1028,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_z(Ctx c, int sign, void* sp
   zc* TW = A + (size_t)n * LS;
   {
     const zc* twg = (const zc*)(sign < 0 ? c.f.twzf : c.f.twzb);
-    for (int e = threadIdx.x; e < n; e += FFT_THREADS) TW[e] = twg[e];
+    for (int e = threadIdx.x; e < n; e += NT) TW[e] = twg[e];
   }
   const int nit = (g.nxh + L - 1) / L;
   const int it = blockIdx.x % nit;
@@ -1037,15 +1036,15 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_z(Ctx c, int sign, void* sp
   zc* S = (zc*)spec + (long long)jm * g.nz * g.nxhp;
   const int tot = n * L;
 #pragma unroll 4
-  for (int e = threadIdx.x; e < tot; e += FFT_THREADS) {
+  for (int e = threadIdx.x; e < tot; e += NT) {
     const int l = e % L, p = e / L;
     const int kx = it * L + l;
     A[p * LS + l] = kx < g.nxh ? S[(long long)p * g.nxhp + kx] : zc{(T)0, (T)0};
   }
   __syncthreads();
-  fft_lines<T, NE>(A, n, L, LS, c.f.codez, c.f.nradz, TW, sign);
+  fft_lines<T, NE, NT>(A, n, L, LS, c.f.codez, c.f.nradz, TW, sign);
 #pragma unroll 4
-  for (int e = threadIdx.x; e < tot; e += FFT_THREADS) {
+  for (int e = threadIdx.x; e < tot; e += NT) {
     const int l = e % L, p = e / L;
     const int kx = it * L + l;
     if (kx < g.nxh) S[(long long)p * g.nxhp + kx] = A[p * LS + l];
@@ -1155,8 +1154,8 @@ __global__ void k_pois_table(Ctx c) {
 // part / (N1M*N3M) into DP (:9429-9433); two rows per complex transform:
 // Z = A + iB with the imaginary parts of A and B at kx = 0 and NX/2 dropped
 // (they do not reach the real part of the reference's transform)
-template <typename T, int NE>
-__global__ __launch_bounds__(FFT_THREADS) void k_fft_xinv(Ctx c, const void* spec) {
+template <typename T, int NE, int NT>
+__global__ __launch_bounds__(NT) void k_fft_xinv(Ctx c, const void* spec) {
   static_assert(sizeof(T) == 8, "the twiddle tables are FP64");
   extern __shared__ __align__(16) unsigned char smem[];
   typedef cplx_t<T> zc;
@@ -1164,13 +1163,13 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_xinv(Ctx c, const void* spe
   const int n = g.nx, NL = c.f.tlx, LS = NL + 1;
   zc* A = (zc*)smem;
   zc* TW = A + (size_t)n * LS;
-  for (int e = threadIdx.x; e < n; e += FFT_THREADS) TW[e] = ((const zc*)c.f.twxb)[e];
+  for (int e = threadIdx.x; e < n; e += NT) TW[e] = ((const zc*)c.f.twxb)[e];
   const int nlines = g.n2m * g.nz;
   const int ln0 = blockIdx.x * 2 * NL;
   const zc* S = (const zc*)spec;
   const int nxh = g.nxh;
 #pragma unroll 4
-  for (int e = threadIdx.x; e < NL * n; e += FFT_THREADS) {
+  for (int e = threadIdx.x; e < NL * n; e += NT) {
     const int l = e / n, kx = e - l * n;
     const int la = ln0 + l, lb = ln0 + NL + l;
     const long long sa = la < nlines ? ((long long)(la % g.n2m) * g.nz + la / g.n2m) * g.nxhp : -1;
@@ -1190,11 +1189,11 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_xinv(Ctx c, const void* spe
     A[kx * LS + l] = {xa.x - xb.y, xa.y + xb.x};
   }
   __syncthreads();
-  fft_lines<T, NE>(A, n, NL, LS, c.f.codex, c.f.nradx, TW, +1);
+  fft_lines<T, NE, NT>(A, n, NL, LS, c.f.codex, c.f.nradx, TW, +1);
   const T norm = (T)(1.0 / ((double)g.nx * (double)g.nz));
   double* dp = c.d.dp;
 #pragma unroll 4
-  for (int e = threadIdx.x; e < NL * n; e += FFT_THREADS) {
+  for (int e = threadIdx.x; e < NL * n; e += NT) {
     const int l = e / n, p = e - l * n;
     const int la = ln0 + l, lb = ln0 + NL + l;
     const zc z = A[p * LS + l];
@@ -1622,18 +1621,21 @@ static void getdp_fp64(const Ctx& c) {
   const int gz = ((g.nxh + f.tlz - 1) / f.tlz) * g.n2m;
   const int gy = ((g.nxh + c.o.yp.L - 1) / c.o.yp.L) * g.nz;
   void* sp = (void*)c.d.spec;
-  // lines of up to 2048 points per block use the lean stage; longer z lines
-  // (N3-1 up to 4096, one per block) the wide one
-  const bool wx = g.nx * f.tlx > 2048, wz = g.nz * f.tlz > 2048;
-#define FFTW_(K, wide, ...)                                                             \
+  // tile sizes (complex values per block): up to 1024 on 256 threads, up
+  // to 2048 or 4096 on 512 (the autotuner picks the lines per tile)
+  const int ex = g.nx * f.tlx, ez = g.nz * f.tlz;
+  const int nx_t = ex <= 1024 ? 256 : 512, nz_t = ez <= 1024 ? 256 : 512;
+#define FFTW_(K, e, nt, ...)                                                            \
   do {                                                                                  \
-    if (wide)                                                                           \
-      hipLaunchKernelGGL(HIP_KERNEL_NAME(K<T, 4096>), __VA_ARGS__);                    \
+    if ((e) > 2048)                                                                     \
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(K<T, 4096, 512>), __VA_ARGS__);               \
+    else if ((e) > 1024)                                                                \
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(K<T, 2048, 512>), __VA_ARGS__);               \
     else                                                                                \
-      hipLaunchKernelGGL(HIP_KERNEL_NAME(K<T, 2048>), __VA_ARGS__);                    \
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(K<T, 1024, 256>), __VA_ARGS__);               \
   } while (0)
-  { KScope ks_(c, KID_FFT_X); FFTW_(k_fft_x, wx, dim3(gx), dim3(FFT_THREADS), ldsx, c.st, c); }
-  { KScope ks_(c, KID_FFT_Z); FFTW_(k_fft_z, wz, dim3(gz), dim3(FFT_THREADS), ldsz, c.st, c, -1, sp); }
+  { KScope ks_(c, KID_FFT_X); FFTW_(k_fft_x, ex, nx_t, dim3(gx), dim3(nx_t), ldsx, c.st, c); }
+  { KScope ks_(c, KID_FFT_Z); FFTW_(k_fft_z, ez, nz_t, dim3(gz), dim3(nz_t), ldsz, c.st, c, -1, sp); }
   {
     KScope ks_(c, KID_POIS_Y);
     if ((g.n2m + c.o.yp.C - 1) / c.o.yp.C <= 6)
@@ -1643,8 +1645,8 @@ static void getdp_fp64(const Ctx& c) {
       hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pois_y<T, T, CHMAX>), dim3(gy), dim3(c.o.yp.L * c.o.yp.C), 0, c.st,
                          c, (const void*)sp, sp);
   }
-  { KScope ks_(c, KID_FFT_Z); FFTW_(k_fft_z, wz, dim3(gz), dim3(FFT_THREADS), ldsz, c.st, c, +1, sp); }
-  { KScope ks_(c, KID_FFT_XINV); FFTW_(k_fft_xinv, wx, dim3(gx), dim3(FFT_THREADS), ldsx, c.st, c, (const void*)sp); }
+  { KScope ks_(c, KID_FFT_Z); FFTW_(k_fft_z, ez, nz_t, dim3(gz), dim3(nz_t), ldsz, c.st, c, +1, sp); }
+  { KScope ks_(c, KID_FFT_XINV); FFTW_(k_fft_xinv, ex, nx_t, dim3(gx), dim3(nx_t), ldsx, c.st, c, (const void*)sp); }
 #undef FFTW_
 }
 void getdp(const Ctx& c) { getdp_fp64(c); }
