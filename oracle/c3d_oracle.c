@@ -1885,6 +1885,101 @@ void c3o_sgscs(c3o *o, const double *U, double *SGSVIS) {
   }
 }
 
+/* ----------------------------------------------------------------- SGSDY */
+/* compact3d.f:1250-1400: dynamic Smagorinsky model.  Per J plane: STRAIN,
+ * |S| (kept in SGSVIS), COM = DELTA^2 |S| S_ij; test filter (FILTER) of COM
+ * and S_ij; M_ij = 4 DELTA^2 |S~| S~_ij - (DELTA^2 |S| S_ij)~; cell-centre
+ * velocities UC and their products, both filtered; L_ij = (UC_i UC_j)~ -
+ * UC~_i UC~_j; C(J) = -0.5 <L:M>/<M:M> over the plane (serial sums in the
+ * reference's K, I order); SGSVIS = max(C DELTA^2 |S|, -1/RE). */
+void c3o_sgsdy(c3o *o, const double *U, double *SGSVIS, double *smagc_out) {
+  const int N1M = o->n1m, N2M = o->n2m, N3M = o->n3m;
+  const double RE = o->re, DX1 = o->dx1, DX3 = o->dx3;
+  const double *DY = o->dy;
+  const double nxz = 1.0 / (double)(N1M * N3M);
+  const double reninv = 1. / (double)RE;
+  const double *V = U + NU, *W = U + 2 * NU;
+#pragma omp parallel for
+  for (int j = 1; j <= N2M; j++) {
+    const size_t pl = (size_t)o->n1 * o->n3;
+    double *SR = dalloc(pl * 6), *COM = dalloc(pl * 6), *UC = dalloc(pl * 3);
+#define DYX(A, i, k, n) A[((size_t)(n) - 1) * pl + ((size_t)(k) - 1) * o->n1 + ((i) - 1)]
+    const double dxyz = 1.0 / DX1 * DY[j] / DX3;
+    c3o_strain(o, U, SR, j);
+    for (int k = 1; k <= N3M; k++)
+      for (int i = 1; i <= N1M; i++) {
+        const double delta2 = pow(dxyz, 2. / 3.);
+        const double sumsr = 2. * (DYX(SR, i, k, 2) * DYX(SR, i, k, 2)) + DYX(SR, i, k, 1) * DYX(SR, i, k, 1) +
+                             2. * (DYX(SR, i, k, 3) * DYX(SR, i, k, 3)) + DYX(SR, i, k, 4) * DYX(SR, i, k, 4) +
+                             2. * (DYX(SR, i, k, 5) * DYX(SR, i, k, 5)) + DYX(SR, i, k, 6) * DYX(SR, i, k, 6);
+        const double abssr = sqrt(2. * sumsr);
+        SGSVIS[IX(i, j, k)] = abssr;
+        const double delabs = delta2 * abssr;
+        for (int m = 1; m <= 6; m++) DYX(COM, i, k, m) = delabs * DYX(SR, i, k, m);
+      }
+    c3o_filter(o, COM, 6);
+    c3o_filter(o, SR, 6);
+    for (int k = 1; k <= N3M; k++)
+      for (int i = 1; i <= N1M; i++) {
+        const double sumfsr = 2. * (DYX(SR, i, k, 2) * DYX(SR, i, k, 2)) + DYX(SR, i, k, 1) * DYX(SR, i, k, 1) +
+                              2. * (DYX(SR, i, k, 3) * DYX(SR, i, k, 3)) + DYX(SR, i, k, 4) * DYX(SR, i, k, 4) +
+                              2. * (DYX(SR, i, k, 5) * DYX(SR, i, k, 5)) + DYX(SR, i, k, 6) * DYX(SR, i, k, 6);
+        const double absfsr = sqrt(2. * sumfsr);
+        const double fdxyz = 4.0 * pow(dxyz, 2. / 3.);
+        const double fdabs = fdxyz * absfsr;
+        for (int m = 1; m <= 6; m++) DYX(SR, i, k, m) = fdabs * DYX(SR, i, k, m) - DYX(COM, i, k, m);
+      }
+    const int jp = j + 1;
+    for (int k = 1; k <= N3M; k++) {
+      const int kp = o->kpa[k];
+      for (int i = 1; i <= N1M; i++) {
+        const int ip = o->ipa[i];
+        const double u1 = (U[IX(i, j, k)] + U[IX(ip, j, k)]) * 0.5;
+        const double u2 = (V[IX(i, j, k)] + V[IX(i, jp, k)]) * 0.5;
+        const double u3 = (W[IX(i, j, k)] + W[IX(i, j, kp)]) * 0.5;
+        DYX(UC, i, k, 1) = u1;
+        DYX(UC, i, k, 2) = u2;
+        DYX(UC, i, k, 3) = u3;
+        DYX(COM, i, k, 1) = u1 * u1;
+        DYX(COM, i, k, 2) = u1 * u2;
+        DYX(COM, i, k, 3) = u1 * u3;
+        DYX(COM, i, k, 4) = u2 * u2;
+        DYX(COM, i, k, 5) = u2 * u3;
+        DYX(COM, i, k, 6) = u3 * u3;
+      }
+    }
+    c3o_filter(o, UC, 3);
+    c3o_filter(o, COM, 6);
+    double sumnum = 0., sumden = 0.;
+    for (int k = 1; k <= N3M; k++)
+      for (int i = 1; i <= N1M; i++) {
+        const double c1 = DYX(UC, i, k, 1), c2 = DYX(UC, i, k, 2), c3 = DYX(UC, i, k, 3);
+        const double xl11 = DYX(COM, i, k, 1) - c1 * c1, xl12 = DYX(COM, i, k, 2) - c1 * c2;
+        const double xl13 = DYX(COM, i, k, 3) - c1 * c3, xl22 = DYX(COM, i, k, 4) - c2 * c2;
+        const double xl23 = DYX(COM, i, k, 5) - c2 * c3, xl33 = DYX(COM, i, k, 6) - c3 * c3;
+        const double s1 = DYX(SR, i, k, 1), s2 = DYX(SR, i, k, 2), s3 = DYX(SR, i, k, 3);
+        const double s4 = DYX(SR, i, k, 4), s5 = DYX(SR, i, k, 5), s6 = DYX(SR, i, k, 6);
+        const double sqmm = 2. * (s2 * s2) + s1 * s1 + 2. * (s3 * s3) + s4 * s4 + 2. * (s5 * s5) + s6 * s6;
+        const double sqlm = 2. * xl12 * s2 + xl11 * s1 + 2. * xl13 * s3 + xl22 * s4 + 2. * xl23 * s5 + xl33 * s6;
+        sumden = sumden + sqmm;
+        sumnum = sumnum + sqlm;
+      }
+    sumnum = sumnum * nxz;
+    sumden = sumden * nxz;
+    const double smagc = -0.5 * sumnum / sumden;
+    if (smagc_out) smagc_out[j] = smagc;
+    for (int k = 1; k <= N3M; k++)
+      for (int i = 1; i <= N1M; i++) {
+        const double turbv = smagc * pow(dxyz, 2. / 3.) * SGSVIS[IX(i, j, k)];
+        SGSVIS[IX(i, j, k)] = fmax(turbv, -reninv);
+      }
+    free(SR);
+    free(COM);
+    free(UC);
+#undef DYX
+  }
+}
+
 /* ------------------------------------------------------ WALLMODEL_INSTANT */
 /* compact3d.f:2319-2493 */
 void c3o_wallmodel_instant(c3o *o, const double *U) {
@@ -1982,6 +2077,7 @@ void c3o_getup(c3o *o, double *U, double *P, double *HH, double *SGSVIS,
   for (size_t t = 0; t < nu; t++) SGSVIS[t] = 0.0;
   memcpy(HH2, HH, sizeof(double) * 3 * nu);
   memset(HH, 0, sizeof(double) * 3 * nu);
+  if (o->files == 1.0) c3o_sgsdy(o, U, SGSVIS, NULL);
   if (o->files == 2.0) c3o_sgscs(o, U, SGSVIS);
   if (o->files != 0.0) {
     if (o->wmodel == 0.0) {

@@ -63,6 +63,9 @@ void c3o_chkmf(c3o *o, const double *U, double *flow1, double *flow3);
 void c3o_wallss(c3o *o, const double *U);
 void c3o_strain(c3o *o, const double *U, double *SR, int j);
 void c3o_sgscs(c3o *o, const double *U, double *SGSVIS);
+/* SGSDY (compact3d.f:1250-1400); smagc_out (may be NULL) receives the
+ * dynamic coefficient C(J) at [J], J = 1..N2M */
+void c3o_sgsdy(c3o *o, const double *U, double *SGSVIS, double *smagc_out);
 void c3o_wallmodel_instant(c3o *o, const double *U);
 void c3o_filter(c3o *o, double *FILV, int n);
 void c3o_filter4(c3o *o, double *FILV, int n);

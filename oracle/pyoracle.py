@@ -185,6 +185,12 @@ class Oracle:
     def sgscs(self, U, SGSVIS):
         self.L.c3o_sgscs(self.h, _p(U), _p(SGSVIS))
 
+    def sgsdy(self, U, SGSVIS):
+        """SGSDY; returns the dynamic coefficient C(J), J = 0..N2 (1..N2M set)"""
+        c = np.zeros(self.n2 + 1)
+        self.L.c3o_sgsdy(self.h, _p(U), _p(SGSVIS), _p(c))
+        return c
+
     def strain(self, U, j):
         SR = np.zeros((self.n1, self.n3, 6), order="F")
         self.L.c3o_strain(self.h, _p(U), _p(SR), int(j))
@@ -370,6 +376,13 @@ class RefLib:
 
     def sgscs(self, U, SGSVIS, ntime=1):
         self.L.sgscs_(_p(U), _p(SGSVIS), self._i(ntime))
+
+    def sgsdy(self, U, SGSVIS, ntime=1):
+        """the reference's SGSDY; returns SMAGC(1:M2) as C(J) at [J], J = 0..N2"""
+        self.L.sgsdy_(_p(U), _p(SGSVIS), self._i(ntime))
+        c = np.zeros(self.n2)
+        self.L.refh_getsmagc_(_p(c))
+        return np.concatenate([[0.0], c])
 
     def wallmodel_instant(self, U, SGSVIS):
         tw, tb = self.get_tau()

@@ -125,6 +125,15 @@ C
       RETURN
       END
 
+      SUBROUTINE REFH_GETSMAGC(C)
+      INCLUDE 'PARAM.H'
+      COMMON/SGSMEAN/STRM(M2,6),SGSNUT(M2),SGSDS(M2),SMAGC(M2)
+      REAL C(M2)
+      DO 10 J=1,M2
+   10 C(J)=SMAGC(J)
+      RETURN
+      END
+
       SUBROUTINE REFH_GETWSM(W)
       COMMON/WSMEM/WSM(3),WSMO(3)
       COMMON/WALLTAO/TAOW

@@ -32,6 +32,7 @@ STRIDE = 13
 CASES = {
     "dns": dict(re=4200.0, alx=2 * math.pi, aly=2.0, alz=math.pi, dt=1e-3, files=0.0, wmodel=0.0),
     "les": dict(re=5200.0, alx=2 * math.pi, aly=2.0, alz=math.pi, dt=5e-4, files=2.0, wmodel=1.0),
+    "dyn": dict(re=5200.0, alx=2 * math.pi, aly=2.0, alz=math.pi, dt=5e-4, files=1.0, wmodel=1.0),
 }
 
 
@@ -57,6 +58,9 @@ def run_case(name, cfg):
         ref.sgscs(U, S)
         out["SGSVIS"] = S
         out["STRAIN3"] = ref.strain(U, 3)
+    if cfg["files"] == 1.0:
+        out["SMAGC"] = ref.sgsdy(U, S)
+        out["SGSVIS"] = S
     if cfg["files"] != 0.0 and cfg["wmodel"] == 1.0:
         ref.wallmodel_instant(U, S)
         ref.set_ifwall(1)
@@ -110,7 +114,7 @@ def run_case(name, cfg):
     out["FILTER4"] = F4
     store = {"config": np.array([n1, n2, n3, cfg["re"], cfg["alx"], cfg["aly"], cfg["alz"], cfg["dt"],
                                  cfg["files"], cfg["wmodel"], 7, presg, presg3, STRIDE])}
-    small = {"PRESG", "DIVMAX", "CFL", "WSM", "CTDMA_RHS", "TDMAP_IN", "TDMAP", "FILTER_IN", "FILTER",
+    small = {"PRESG", "DIVMAX", "CFL", "WSM", "SMAGC", "CTDMA_RHS", "TDMAP_IN", "TDMAP", "FILTER_IN", "FILTER",
              "FILTER4"} | {k for k in out if k.startswith("CTDMA11")}
     for k, v in out.items():
         if k in small:

@@ -2,8 +2,9 @@
 subroutines (tests/golden/make_golden.py; compact3d.f compiled unmodified).
 
 Every phase the reference can run without FFTW is compared: BCOND, RHS1-3,
-GETUH1-3, RHSDP, UPCALC, DIVCHECK, CFL, SGSCS/STRAIN, WALLMODEL_INSTANT,
-CTDMA11, TDMAP, FILTER, FILTER4.  The oracle follows the reference's operation
+GETUH1-3, RHSDP, UPCALC, DIVCHECK, CFL, SGSCS/STRAIN, SGSDY (the dynamic
+model, its per-plane coefficient included), WALLMODEL_INSTANT, CTDMA11,
+TDMAP, FILTER, FILTER4.  The oracle follows the reference's operation
 order, so the expected agreement is exact; the bound allows 4 ulps."""
 import os
 
@@ -35,7 +36,7 @@ def _check(fx, key, arr, stride):
     assert es <= 1e-12, (key, es)
 
 
-@pytest.mark.parametrize("name", ["dns", "les"])
+@pytest.mark.parametrize("name", ["dns", "les", "dyn"])
 def test_oracle_matches_reference_fixture(name):
     from pyoracle import Oracle
     fx = _load(name)
@@ -50,6 +51,10 @@ def test_oracle_matches_reference_fixture(name):
         o.set_wsm([2.5e-3, 0.0, 4.0e-4])
         o.sgscs(U, S)
         _check(fx, "SGSVIS", S, stride)
+    if files == 1.0:
+        smagc = o.sgsdy(U, S)
+        _check(fx, "SGSVIS", S, stride)
+        assert np.max(np.abs(smagc - fx["SMAGC"])) <= TOL * np.max(np.abs(fx["SMAGC"]))
     if files != 0.0 and wmodel == 1.0:
         o.wallmodel_instant(U)
         o.set_ifwall(1)
