@@ -445,6 +445,7 @@ long long c3d_footprint_bytes(const c3d_config* cfg) {
   // mode); intermediates 10 + HH 6; FP32 shadows 4 in mixed mode; LES 7
   long long bytes = (10 + (mixed ? 1 : 0)) * fsz * 8 + 16 * fsz * es + (mixed ? 4 * fsz * 4 : 0);
   if (cfg->files != 0.0) bytes += 7 * fsz * 8;
+  if (cfg->files == 1.0) bytes += (2 * n2m * ((nx + 15) / 16) * ((nz + 15) / 16) + n2m + 3) * 8;
   const long long nm = ((nx / 2 + 1 + 7) / 8 * 8) * nz * n2m;
   bytes += nm * 16 + nm * 16;  // spectrum + TDMAP factors
   bytes += 4 * std::max(n2m * nz, nx * n2m) * 8 + 128 * 8;  // flow-rate partial sums, max slots
@@ -512,7 +513,8 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   const c3d_config& cf = h->cfg;
   if (cf.n1 < 9 || cf.n2 < 5 || cf.n3 < 9) return bad("grid too small (need N1,N3 >= 9, N2 >= 5)");
   if (cf.precision != C3D_FP64 && cf.precision != C3D_MIXED) return bad("precision must be C3D_FP64 or C3D_MIXED");
-  if (cf.files != 0.0 && cf.files != 2.0) return bad("FILES must be 0 (DNS) or 2 (Smagorinsky)");
+  if (cf.files != 0.0 && cf.files != 1.0 && cf.files != 2.0)
+    return bad("FILES must be 0 (DNS), 1 (dynamic model) or 2 (Smagorinsky)");
   if (cf.wmodel != 0.0 && cf.wmodel != 1.0) return bad("WMODEL must be 0 or 1");
   if (cf.re <= 0 || cf.dt <= 0 || cf.alx <= 0 || cf.alz <= 0) return bad("RE, DT, ALX, ALZ must be > 0");
   if (hipSetDevice(cf.device) != hipSuccess) return bad("hipSetDevice failed");
@@ -703,6 +705,11 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
     double** les[] = {&F.nux, &F.nuz, &F.nue, &F.dvdx, &F.dwdx, &F.dudz, &F.dvdz};
     for (auto pp : les)
       if (dalloc(h, pp, fs)) return bad(h->err);
+  }
+  F.dyp = F.smagc = nullptr;
+  if (cf.files == 1.0) {
+    if (dalloc(h, &F.dyp, 2 * (size_t)g.n2m * c3dk::sgsdy_tiles(g)) || dalloc(h, &F.smagc, (size_t)g.n2 + 2))
+      return bad(h->err);
   }
   c3dk::init_attrs();
   {
@@ -979,8 +986,11 @@ extern "C" int c3d_upcalc(c3d_handle* h) {
 
 extern "C" int c3d_sgs(c3d_handle* h) {
   if (!h) return -1;
-  if (!h->c.s.files) return fail(h, "SGSCS needs FILES != 0");
-  c3dk::sgscs(h->c);
+  if (!h->c.s.files) return fail(h, "the SGS model needs FILES != 0");
+  if (h->cfg.files == 1.0)
+    c3dk::sgsdy(h->c);
+  else
+    c3dk::sgscs(h->c);
   return launched(h);
 }
 
@@ -1012,6 +1022,7 @@ extern "C" int c3d_wallss(c3d_handle* h) {
 static int step_impl(c3d_handle* h, bool chk) {
   swap_hh(h);
   // GETUP (:925-929): SGS model, then the wall model
+  if (h->cfg.files == 1.0) c3dk::sgsdy(h->c);
   if (h->cfg.files == 2.0) c3dk::sgscs(h->c);
   if (h->cfg.files != 0.0) wall_model_impl(h);
   c3dk::bcond(h->c);
@@ -1043,6 +1054,7 @@ static int read_sc(c3d_handle* h, int slot, double* v) {
 static void step_launch(c3d_handle* h) {
   if (h->cfg.wmodel == 0.0 && h->cfg.files != 0.0) c3dk::wallss(h->c);
   swap_hh(h);
+  if (h->cfg.files == 1.0) c3dk::sgsdy(h->c);
   if (h->cfg.files == 2.0) c3dk::sgscs(h->c);
   if (h->cfg.files != 0.0) wall_model_impl(h);
   c3dk::bcond(h->c);

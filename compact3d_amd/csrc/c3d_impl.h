@@ -115,6 +115,9 @@ struct Fields {
   //   FP_DW: sum DELTA_w*DY (ADI x-sweep of w)   FP_W: sum U_w*DY
   double *flowp;
   int flowr;
+  // dynamic SGS model (FILES = 1): per-plane tile partial sums of L:M and
+  // M:M, and the coefficient C(J) at [J]
+  double *dyp, *smagc;
   // the 2 x 64 max slots (DIVMAX, CFL bit patterns) of the update's checks;
   // zero between steps (cleared at set-up and by the kernel that reads them)
   double *mxs;
@@ -191,6 +194,8 @@ void divcheck(const Ctx& c);
 void cfl(const Ctx& c);
 void chkmf(const Ctx& c, bool reference_sums);
 void sgscs(const Ctx& c);
+void sgsdy(const Ctx& c);            // SGSDY, the dynamic model (FILES = 1)
+int sgsdy_tiles(const Geo& g);      // tiles per plane of its partial sums
 void wallmodel(const Ctx& c);
 void wallss(const Ctx& c);
 void sgs_terms(const Ctx& c);

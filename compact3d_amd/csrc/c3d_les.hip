@@ -7,25 +7,17 @@
 
 #include "c3d_impl.h"
 
-// ------------------------------------------------------------ SGSCS
-// SGSCS (compact3d.f:1413-1488) with STRAIN (:5558-5689): cell-centre strain
-// rate from second-order differences, |S| = sqrt(2 S:S),
-// nu_t = max((0.1 (1-exp(-y+/26)))^2 (DY/(DX1 DX3))^(2/3) |S|, -1/RE),
-// y+ = (Y(J)+DY(J)/2) u_tau RE, u_tau = (WSM1^2 + WSM3^2)^(1/4) (previous step).
-__global__ void k_sgscs(Ctx c) {
-  const Geo g = c.g;
-  const long long tot = (long long)g.nx * g.n2m * g.nz;
-  long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (t >= tot) return;
-  const int i = t % g.nx;
-  const int j = 1 + (int)((t / g.nx) % g.n2m);
-  const int k = (int)(t / ((long long)g.nx * g.n2m));
-  const Scal s = c.s;
-  const Metrics M = c.m;
-  const Fields F = c.d;
-  const double* U = F.u;
-  const double* V = F.v;
-  const double* W = F.w;
+// ------------------------------------------------------------ STRAIN
+// STRAIN (compact3d.f:5558-5689) at one cell centre: the six components
+// S11 S12 S13 S22 S23 S33 from second-order differences (wall rows by the
+// FJPA/FJMU flags, WMODEL != 0 doubling the y differences at J = 1, N2M).
+__device__ __forceinline__ void strain_at(const Ctx& c, int i, int j, int k, double sr[6]) {
+  const Geo& g = c.g;
+  const Scal& s = c.s;
+  const Metrics& M = c.m;
+  const double* U = c.d.u;
+  const double* V = c.d.v;
+  const double* W = c.d.w;
   const int ip = wrapp(i, g.nx), im = wrapm(i, g.nx), kp = wrapp(k, g.nz), km = wrapm(k, g.nz);
   const int jp = j + 1, jm = j - 1;
   const double DX1 = s.dx1, DX3 = s.dx3;
@@ -37,9 +29,8 @@ __global__ void k_sgscs(Ctx c) {
   auto v_ = [&](int ii, int jj, int kk) { return V[gidx(g, ii, jj, kk)]; };
   auto w_ = [&](int ii, int jj, int kk) { return W[gidx(g, ii, jj, kk)]; };
   const int N2 = g.n2;
-  double sr1, sr2, sr3, sr4, sr5, sr6;
   double u2 = u_(ip, j, k), u1 = u_(i, j, k);
-  sr1 = DX1 * (u2 - u1);
+  sr[0] = DX1 * (u2 - u1);
   double ujp = 0.5 * (u_(i, jp, k) + u_(ip, jp, k));
   double ujc = 0.5 * (u_(i, j, k) + u_(ip, j, k));
   double ujm = 0.5 * (u_(i, jm, k) + u_(ip, jm, k));
@@ -57,7 +48,7 @@ __global__ void k_sgscs(Ctx c) {
   double vim = 0.5 * (v_(im, j, k) + v_(im, jp, k));
   double v2 = 0.5 * (vip + vic), v1 = 0.5 * (vim + vic);
   const double yfac = (!wm0 && (j == 1 || j == g.n2m)) ? 2.0 : 1.0;
-  sr2 = 0.5 * (DX1 * (v2 - v1) + yfac / DY[j] * (u2 - u1));
+  sr[1] = 0.5 * (DX1 * (v2 - v1) + yfac / DY[j] * (u2 - u1));
   double ukp = 0.5 * (u_(i, j, kp) + u_(ip, j, kp));
   double ukc = 0.5 * (u_(i, j, k) + u_(ip, j, k));
   double ukm = 0.5 * (u_(i, j, km) + u_(ip, j, km));
@@ -67,10 +58,10 @@ __global__ void k_sgscs(Ctx c) {
   double wic = 0.5 * (w_(i, j, k) + w_(i, j, kp));
   double wim = 0.5 * (w_(im, j, k) + w_(im, j, kp));
   double w2 = 0.5 * (wip + wic), w1 = 0.5 * (wim + wic);
-  sr3 = 0.5 * (DX1 * (w2 - w1) + DX3 * (u2 - u1));
+  sr[2] = 0.5 * (DX1 * (w2 - w1) + DX3 * (u2 - u1));
   v2 = v_(i, jp, k);
   v1 = v_(i, j, k);
-  sr4 = 1.0 / DY[j] * (v2 - v1);
+  sr[3] = 1.0 / DY[j] * (v2 - v1);
   double wjp = 0.5 * (w_(i, jp, k) + w_(i, jp, kp));
   double wjc = 0.5 * (w_(i, j, k) + w_(i, j, kp));
   double wjm = 0.5 * (w_(i, jm, k) + w_(i, jm, kp));
@@ -88,11 +79,36 @@ __global__ void k_sgscs(Ctx c) {
   double vkm = 0.5 * (v_(i, j, km) + v_(i, jp, km));
   v2 = 0.5 * (vkp + vkc);
   v1 = 0.5 * (vkm + vkc);
-  sr5 = 0.5 * (yfac / DY[j] * (w2 - w1) + DX3 * (v2 - v1));
+  sr[4] = 0.5 * (yfac / DY[j] * (w2 - w1) + DX3 * (v2 - v1));
   w2 = w_(i, j, kp);
   w1 = w_(i, j, k);
-  sr6 = DX3 * (w2 - w1);
-  double sumsr = 2. * (sr2 * sr2) + sr1 * sr1 + 2. * (sr3 * sr3) + sr4 * sr4 + 2. * (sr5 * sr5) + sr6 * sr6;
+  sr[5] = DX3 * (w2 - w1);
+}
+
+// 2 S:S with the off-diagonal components counted twice, in STRAIN's order
+__device__ __forceinline__ double sdots(const double s[6]) {
+  return 2. * (s[1] * s[1]) + s[0] * s[0] + 2. * (s[2] * s[2]) + s[3] * s[3] + 2. * (s[4] * s[4]) + s[5] * s[5];
+}
+
+// ------------------------------------------------------------ SGSCS
+// SGSCS (compact3d.f:1413-1488) with STRAIN: |S| = sqrt(2 S:S),
+// nu_t = max((0.1 (1-exp(-y+/26)))^2 (DY/(DX1 DX3))^(2/3) |S|, -1/RE),
+// y+ = (Y(J)+DY(J)/2) u_tau RE, u_tau = (WSM1^2 + WSM3^2)^(1/4) (previous step).
+__global__ void k_sgscs(Ctx c) {
+  const Geo g = c.g;
+  const long long tot = (long long)g.nx * g.n2m * g.nz;
+  long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= tot) return;
+  const int i = t % g.nx;
+  const int j = 1 + (int)((t / g.nx) % g.n2m);
+  const int k = (int)(t / ((long long)g.nx * g.n2m));
+  const Scal s = c.s;
+  const Metrics M = c.m;
+  const Fields F = c.d;
+  const double* DY = M.dy;
+  double sr[6];
+  strain_at(c, i, j, k, sr);
+  double sumsr = sdots(sr);
   double abssr = sqrt(2. * sumsr);
   const double w1s = F.sc[SC_WSM1], w3s = F.sc[SC_WSM3];
   const double utau = sqrt(sqrt(w1s * w1s + w3s * w3s));
@@ -100,9 +116,148 @@ __global__ void k_sgscs(Ctx c) {
   const double yplus = (M.y[j] + DY[j] * 0.5) * utau * s.re;
   cs = cs * (1 - exp(-yplus / 26.0));
   const double smagc = cs * cs;
-  const double dxyz = 1.0 / DX1 * DY[j] / DX3;
+  const double dxyz = 1.0 / s.dx1 * DY[j] / s.dx3;
   const double turbv = smagc * pow(dxyz, 2. / 3.) * abssr;
   F.sgs[gidx(g, i, j, k)] = fmax(turbv, -s.rinv);
+}
+
+// ------------------------------------------------------------ SGSDY
+// SGSDY (compact3d.f:1250-1400), the dynamic model, per J plane in three
+// kernels.  k_sgsdy_tile: a block takes a DT x DT tile of (I,K) with a
+// one-point periodic halo and forms at every point of it STRAIN, |S| (kept
+// in SGSVIS), COM1 = DELTA^2 |S| S_ij, the cell-centre velocities UC and
+// their products COM2; after a barrier each interior point applies the
+// test filter FILTER (Simpson (1,4,1)/6 in x, then in z, :5190-5228) to
+// the 21 quantities from LDS, forms M_ij = 4 DELTA^2 |S~| S~_ij - COM1~_ij
+// and L_ij = COM2~_ij - UC~_i UC~_j, and the block's sums of L:M and M:M go
+// to a per-plane partial array.  k_sgsdy_coef: C(J) = -0.5 <L:M>/<M:M> from
+// the partials in a fixed order.  k_sgsdy_clip: SGSVIS = max(C(J)
+// DELTA^2 |S|, -1/RE).
+#define DT_TILE 16
+#define DT_HALO (DT_TILE + 2)
+#define DT_NQ 21
+__global__ __launch_bounds__(DT_TILE* DT_TILE) void k_sgsdy_tile(Ctx c) {
+  __shared__ double Q[DT_NQ][DT_HALO * DT_HALO];
+  const Geo g = c.g;
+  const Scal s = c.s;
+  const Fields F = c.d;
+  const int tix = (g.nx + DT_TILE - 1) / DT_TILE, tiz = (g.nz + DT_TILE - 1) / DT_TILE;
+  const int ntile = tix * tiz;
+  const int j = 1 + blockIdx.x / ntile;
+  const int tile = blockIdx.x % ntile;
+  const int i0 = (tile % tix) * DT_TILE, k0 = (tile / tix) * DT_TILE;
+  const double dxyz = 1.0 / s.dx1 * c.m.dy[j] / s.dx3;
+  const double delta2 = pow(dxyz, 2. / 3.);
+  // every point of the tile and its halo
+  for (int e = threadIdx.x; e < DT_HALO * DT_HALO; e += blockDim.x) {
+    const int px = e % DT_HALO, pz = e / DT_HALO;
+    int i = i0 - 1 + px, k = k0 - 1 + pz;
+    i = ((i % g.nx) + g.nx) % g.nx;
+    k = ((k % g.nz) + g.nz) % g.nz;
+    double sr[6];
+    strain_at(c, i, j, k, sr);
+    const double abssr = sqrt(2. * sdots(sr));
+    const bool inner = px >= 1 && px <= DT_TILE && pz >= 1 && pz <= DT_TILE && i0 - 1 + px < g.nx &&
+                       k0 - 1 + pz < g.nz;
+    if (inner) F.sgs[gidx(g, i, j, k)] = abssr;
+    const double delabs = delta2 * abssr;
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+      Q[m][e] = delabs * sr[m];  // COM1
+      Q[6 + m][e] = sr[m];       // S
+    }
+    const int ip = wrapp(i, g.nx), kp = wrapp(k, g.nz);
+    const double u1 = (F.u[gidx(g, i, j, k)] + F.u[gidx(g, ip, j, k)]) * 0.5;
+    const double u2 = (F.v[gidx(g, i, j, k)] + F.v[gidx(g, i, j + 1, k)]) * 0.5;
+    const double u3 = (F.w[gidx(g, i, j, k)] + F.w[gidx(g, i, j, kp)]) * 0.5;
+    Q[12][e] = u1;
+    Q[13][e] = u2;
+    Q[14][e] = u3;
+    Q[15][e] = u1 * u1;
+    Q[16][e] = u1 * u2;
+    Q[17][e] = u1 * u3;
+    Q[18][e] = u2 * u2;
+    Q[19][e] = u2 * u3;
+    Q[20][e] = u3 * u3;
+  }
+  __syncthreads();
+  const int tx = threadIdx.x % DT_TILE, tz = threadIdx.x / DT_TILE;
+  const bool ok = i0 + tx < g.nx && k0 + tz < g.nz;
+  // FILTER: x pass on rows z-1, z, z+1, then the z pass
+  auto filt = [&](int q) {
+    const double* A = Q[q];
+    double fx[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      const int row = (tz + r) * DT_HALO + tx + 1;
+      fx[r] = (A[row - 1] + 4.0 * A[row] + A[row + 1]) / 6.0;
+    }
+    return (fx[0] + 4.0 * fx[1] + fx[2]) / 6.0;
+  };
+  double num = 0.0, den = 0.0;
+  if (ok) {
+    double fc[6], fs[6];
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+      fc[m] = filt(m);
+      fs[m] = filt(6 + m);
+    }
+    const double absfsr = sqrt(2. * sdots(fs));
+    const double fdxyz = 4.0 * pow(dxyz, 2. / 3.);
+    const double fdabs = fdxyz * absfsr;
+    double mm[6];
+#pragma unroll
+    for (int m = 0; m < 6; m++) mm[m] = fdabs * fs[m] - fc[m];
+    const double c1 = filt(12), c2 = filt(13), c3 = filt(14);
+    const double xl11 = filt(15) - c1 * c1, xl12 = filt(16) - c1 * c2, xl13 = filt(17) - c1 * c3;
+    const double xl22 = filt(18) - c2 * c2, xl23 = filt(19) - c2 * c3, xl33 = filt(20) - c3 * c3;
+    den = sdots(mm);
+    num = 2. * xl12 * mm[1] + xl11 * mm[0] + 2. * xl13 * mm[2] + xl22 * mm[3] + 2. * xl23 * mm[4] + xl33 * mm[5];
+  }
+  block_sum2(num, den, F.dyp + 2 * ((long long)(j - 1) * ntile + tile),
+             F.dyp + 2 * ((long long)(j - 1) * ntile + tile) + 1);
+}
+
+// C(J) per plane from the tile partials, summed in tile order
+__global__ __launch_bounds__(256) void k_sgsdy_coef(Ctx c, int ntile) {
+  __shared__ double red[2][256];
+  const Geo g = c.g;
+  const int j = 1 + blockIdx.x;
+  const double* P = c.d.dyp + 2 * (long long)(j - 1) * ntile;
+  double a = 0.0, b = 0.0;
+  for (int t = threadIdx.x; t < ntile; t += blockDim.x) {
+    a += P[2 * t];
+    b += P[2 * t + 1];
+  }
+  red[0][threadIdx.x] = a;
+  red[1][threadIdx.x] = b;
+  __syncthreads();
+  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + h];
+      red[1][threadIdx.x] += red[1][threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double nxz = 1.0 / (double)(g.nx * g.nz);
+    const double sumnum = red[0][0] * nxz, sumden = red[1][0] * nxz;
+    c.d.smagc[j] = -0.5 * sumnum / sumden;
+  }
+}
+
+__global__ void k_sgsdy_clip(Ctx c) {
+  const Geo g = c.g;
+  const long long tot = (long long)g.nx * g.n2m * g.nz;
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= tot) return;
+  const int i = t % g.nx;
+  const int j = 1 + (int)((t / g.nx) % g.n2m);
+  const int k = (int)(t / ((long long)g.nx * g.n2m));
+  const double dxyz = 1.0 / c.s.dx1 * c.m.dy[j] / c.s.dx3;
+  const long long a = gidx(g, i, j, k);
+  const double turbv = c.d.smagc[j] * pow(dxyz, 2. / 3.) * c.d.sgs[a];
+  c.d.sgs[a] = fmax(turbv, -c.s.rinv);
 }
 
 // --------------------------------------------------------- wall model
@@ -620,6 +775,18 @@ static size_t lds(const TileShape& t, int ntiles) { return sizeof(double) * (siz
 void sgscs(const Ctx& c) {
   KScope ks_(c, KID_SGS);
   hipLaunchKernelGGL(k_sgscs, dim3(pgrid((long long)c.g.nx * c.g.n2m * c.g.nz)), dim3(C3D_NT), 0, c.st, c);
+}
+
+int sgsdy_tiles(const Geo& g) {
+  return ((g.nx + DT_TILE - 1) / DT_TILE) * ((g.nz + DT_TILE - 1) / DT_TILE);
+}
+
+void sgsdy(const Ctx& c) {
+  const int nt = sgsdy_tiles(c.g);
+  KScope ks_(c, KID_SGS);
+  hipLaunchKernelGGL(k_sgsdy_tile, dim3(nt * c.g.n2m), dim3(DT_TILE * DT_TILE), 0, c.st, c);
+  hipLaunchKernelGGL(k_sgsdy_coef, dim3(c.g.n2m), dim3(256), 0, c.st, c, nt);
+  hipLaunchKernelGGL(k_sgsdy_clip, dim3(pgrid((long long)c.g.nx * c.g.n2m * c.g.nz)), dim3(C3D_NT), 0, c.st, c);
 }
 
 void wallmodel(const Ctx& c) {

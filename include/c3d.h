@@ -48,7 +48,7 @@ typedef struct c3d_config {
   double re;          /* RE = 1/nu in the reference's units */
   double alx, aly, alz; /* box; ALY = 2 full channel, 1 half channel */
   double dt;          /* DT */
-  double files;       /* SGS model switch FILES: 0 none (DNS), 2 Smagorinsky */
+  double files;       /* SGS model switch FILES: 0 none (DNS), 1 dynamic (SGSDY), 2 Smagorinsky (SGSCS) */
   double wmodel;      /* WMODEL: 0 resolved walls, 1 instantaneous log-law wall model */
   double cks;         /* tanh stretching of y (the reference hard-codes 2.5, :338) */
   const double *y;    /* optional Y(0:N2) to use instead of the tanh grid, or NULL */
@@ -119,7 +119,7 @@ int c3d_rhs(c3d_handle *h);       /* RHS1/RHS2/RHS3 compact3d.f:6063, 6784, 7399
 int c3d_getuh(c3d_handle *h);     /* GETUH1/2/3 compact3d.f:8218, 8344, 8478 */
 int c3d_dpcalc(c3d_handle *h);    /* DPCALC = RHSDP + GETDP compact3d.f:1037-1049 */
 int c3d_upcalc(c3d_handle *h);    /* UPCALC compact3d.f:1052-1238 */
-int c3d_sgs(c3d_handle *h);       /* SGSCS compact3d.f:1413-1488 */
+int c3d_sgs(c3d_handle *h);       /* SGSDY (FILES=1) compact3d.f:1250 / SGSCS (FILES=2) :1413 */
 int c3d_wall_model(c3d_handle *h); /* WALL_MODEL -> WALLMODEL_INSTANT compact3d.f:1952, 2319 */
 int c3d_wallss(c3d_handle *h);    /* WALLSS compact3d.f:11077 (sets WSM(1:3)) */
 

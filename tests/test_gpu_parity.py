@@ -242,3 +242,68 @@ def test_filters_match_oracle(width):
     print("FILTER", width, np.max(np.abs(out - G)))
     assert np.max(np.abs(out - G)) <= 1e-15
     ch.close()
+
+
+@pytest.mark.parametrize("grid", [(33, 33, 33), (41, 33, 49)])
+def test_dynamic_sgs_matches_oracle(grid):
+    """SGSDY, the dynamic model (FILES=1): SGSVIS after the model (tiles of
+    16 x 16 with a filter halo, including grids that are not a multiple of
+    the tile) and the phases that consume it"""
+    from compact3d_amd import c3d
+    from pyoracle import Oracle
+    n1, n2, n3 = grid
+    cfg = config(n1, n2, n3, re=5200.0, dt=5e-4, files=1.0, wmodel=1.0)
+    orc = Oracle(**cfg)
+    ch = c3d.Channel(**cfg)
+    U, P, HH2 = inputs(orc, seed=7)
+    presg, presg3 = 0.003, -0.001
+    ch.upload(U, P, HH=HH2, presg=presg, presg3=presg3)
+    Uo = U.copy(order="F")
+    So = orc.zeros_u(0)
+    orc.sgsdy(Uo, So)
+    ch.sgscs()  # the handle's SGS model: SGSDY for FILES=1
+    e = relerr(interior_u(So, n1, n2, n3), interior_u(ch.download()["SGSVIS"], n1, n2, n3))
+    print(f"SGSDY {grid} {e:.3e}")
+    assert e <= 1e-12
+    orc.wallmodel_instant(Uo)
+    orc.set_ifwall(1)
+    ch.wall_model()
+    orc.bcond(Uo, So)
+    ch.bcond()
+    HHo = orc.zeros_u()
+    RUH = [orc.zeros_u(0) for _ in range(3)]
+    for c in (1, 2, 3):
+        orc.rhs(c, Uo, P, HHo, HH2, RUH[c - 1], presg if c == 1 else presg3, So)
+    ch.rhs()
+    for c in (1, 2, 3):
+        j0 = 2 if c == 2 else 1
+        e1 = relerr(interior_u(RUH[c - 1], n1, n2, n3, j0=j0),
+                    interior_u(ch.field(c3d.F_RUH1 + c - 1), n1, n2, n3, j0=j0))
+        print(f"dynamic LES RHS{c} {e1:.3e}")
+        assert e1 <= 1e-11
+    ch.close()
+
+
+@pytest.mark.parametrize("wmodel", [1.0, 0.0])
+def test_dynamic_les_trajectory_10_steps(wmodel):
+    from compact3d_amd import c3d
+    from pyoracle import Oracle
+    n1, n2, n3 = 49, 33, 33
+    cfg = config(n1, n2, n3, re=20000.0, dt=5e-4, files=1.0, wmodel=wmodel, cks=1.5)
+    orc = Oracle(**cfg)
+    ch = c3d.Channel(**cfg)
+    U, P, _ = inputs(orc, seed=9)
+    ch.upload(U, P)
+    HH = orc.zeros_u()
+    S = orc.zeros_u(0)
+    pg = pg3 = 0.0
+    Uo, Po = U.copy(order="F"), P.copy(order="F")
+    for n in range(10):
+        pg, pg3 = orc.getup(Uo, Po, HH, S, pg, pg3, n + 1)
+        ch.getup(n + 1)
+    st = ch.download()
+    eu = [relerr(interior_u(Uo, n1, n2, n3, c), interior_u(st["U"], n1, n2, n3, c)) for c in range(3)]
+    es = relerr(interior_u(S, n1, n2, n3), interior_u(st["SGSVIS"], n1, n2, n3))
+    print("dynamic LES 10 steps wmodel", wmodel, "U", ["%.3e" % x for x in eu], "SGSVIS %.3e" % es)
+    assert max(eu) <= 1e-10 and es <= 1e-9
+    ch.close()
