@@ -70,6 +70,7 @@ def lib():
         L.c3d_cfl.argtypes = [C.c_void_p, _dp]
         L.c3d_chkmf.argtypes = [C.c_void_p, _dp, _dp]
         L.c3d_flowrate.argtypes = [C.c_void_p, _dp, _dp]
+        L.c3d_stats.argtypes = [C.c_void_p, _dp]
         L.c3d_get_field.argtypes = [C.c_void_p, C.c_int, _dp]
         L.c3d_get_scalar.argtypes = [C.c_void_p, C.c_int, _dp]
         L.c3d_set_scalar.argtypes = [C.c_void_p, C.c_int, C.c_double]
@@ -316,6 +317,16 @@ class Channel:
         cnt = (C.c_int * NKID)()
         self._ck(self.L.c3d_kernel_times(self.h, ms, cnt), "kernel_times")
         return {self.L.c3d_kernel_name(q).decode(): (ms[q], cnt[q]) for q in range(NKID) if cnt[q] > 0}
+
+    STAT_NAMES = ("VM1", "VM2", "VM3", "PM", "WM1", "WM2", "WM3", "VRMS1", "VRMS2", "VRMS3", "VRMS4",
+                  "VRMS5", "VRMS6", "PRMS", "PV", "WRMS1", "WRMS2", "WRMS3", "U2V", "VW2")
+
+    def stats(self):
+        """ENERGY's plane statistics of the resident field: array (20, N2-1),
+        rows in the order of STAT_NAMES (J = 1..N2-1 along each row)"""
+        out = np.zeros(20 * (self.n2 - 1))
+        self._ck(self.L.c3d_stats(self.h, _p(out)), "stats")
+        return out.reshape(20, self.n2 - 1)
 
     def apply_op(self, op, direction, field):
         inp = np.asfortranarray(field, dtype=np.float64)

@@ -1518,6 +1518,17 @@ extern "C" int c3d_chkmf(c3d_handle* h, double* flow1, double* flow3) {
   return 0;
 }
 
+extern "C" int c3d_stats(c3d_handle* h, double* out) {
+  if (!h) return -1;
+  if (!out) return fail(h, "c3d_stats needs an output array of 20*(N2-1) values");
+  const size_t n = 20 * (size_t)h->c.g.n2m;
+  c3dk::plane_stats(h->c, h->dtmp);
+  if (launched(h)) return -1;
+  HIPCHK(hipMemcpyAsync(out, h->dtmp, n * sizeof(double), hipMemcpyDeviceToHost, h->c.st));
+  HIPCHK(hipStreamSynchronize(h->c.st));
+  return 0;
+}
+
 extern "C" int c3d_flowrate(c3d_handle* h, double* flow1, double* flow3) {
   if (!h) return -1;
   c3dk::chkmf(h->c, false);

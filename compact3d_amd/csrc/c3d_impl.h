@@ -193,6 +193,7 @@ void pois_table(const Ctx& c);
 void divcheck(const Ctx& c);
 void cfl(const Ctx& c);
 void chkmf(const Ctx& c, bool reference_sums);
+void plane_stats(const Ctx& c, double* out);  // ENERGY's plane statistics, 20 x N2M on the device
 void sgscs(const Ctx& c);
 void sgsdy(const Ctx& c);            // SGSDY, the dynamic model (FILES = 1)
 int sgsdy_tiles(const Geo& g);      // tiles per plane of its partial sums

@@ -75,6 +75,103 @@ __global__ __launch_bounds__(256) void k_rhs_y(Ctx c) {
   }
 }
 
+// ------------------------------------------------------------ statistics
+// The plane statistics of ENERGY (compact3d.f:9902): the cell-centre
+// vorticity of loop 10 (:10005-10069) and the means and moments of loop 20
+// (:10073-10158), one block per J plane; thread t sums the points
+// t, t+blockDim, ... of the plane, then a tree over the block (fixed
+// order).  out[s*N2M + J-1] for the 20 quantities: VM1-3, PM, WM1-3,
+// VRMS1-6 (uu vv ww uv uw vw), PRMS, PV, WRMS1-3, U2V, VW2.
+#define ST_NQ 20
+__global__ __launch_bounds__(256) void k_plane_stats(Ctx c, double* out) {
+  __shared__ double red[ST_NQ][256];
+  const Geo g = c.g;
+  const Scal s = c.s;
+  const Metrics M = c.m;
+  const Fields F = c.d;
+  const int j = 1 + blockIdx.x;
+  const int jp = j + 1, jm = j - 1, N2 = g.n2;
+  const double fjum = M.fjmu[j], fjup = M.fjpa[j];
+  const double* DY = M.dy;
+  const double* H = M.h;
+  double acc[ST_NQ];
+#pragma unroll
+  for (int q = 0; q < ST_NQ; q++) acc[q] = 0.0;
+  const int npl = g.nx * g.nz;
+  for (int e = threadIdx.x; e < npl; e += blockDim.x) {
+    const int i = e % g.nx, k = e / g.nx;
+    const int ip = wrapp(i, g.nx), im = wrapm(i, g.nx), kp = wrapp(k, g.nz), km = wrapm(k, g.nz);
+    auto U = [&](int ii, int jj, int kk) { return F.u[gidx(g, ii, jj, kk)]; };
+    auto V = [&](int ii, int jj, int kk) { return F.v[gidx(g, ii, jj, kk)]; };
+    auto W = [&](int ii, int jj, int kk) { return F.w[gidx(g, ii, jj, kk)]; };
+    const double u1kp = 0.5 * (U(i, j, kp) + U(ip, j, kp));
+    const double u1km = 0.5 * (U(i, j, km) + U(ip, j, km));
+    const double u2kp = 0.5 * (V(i, j, kp) + V(i, jp, kp));
+    const double u2km = 0.5 * (V(i, j, km) + V(i, jp, km));
+    const double u2ip = 0.5 * (V(ip, j, k) + V(ip, jp, k));
+    const double u2im = 0.5 * (V(im, j, k) + V(im, jp, k));
+    const double u3ip = 0.5 * (W(ip, j, k) + W(ip, j, kp));
+    const double u3im = 0.5 * (W(im, j, k) + W(im, j, kp));
+    const double u1jp = 0.5 * (U(i, jp, k) + U(ip, jp, k));
+    const double u1jc = 0.5 * (U(i, j, k) + U(ip, j, k));
+    const double u1jm = 0.5 * (U(i, jm, k) + U(ip, jm, k));
+    double u12 = 0.5 / H[jp] * (DY[jp] * u1jc + DY[j] * u1jp);
+    double u11 = 0.5 / H[j] * (DY[j] * u1jm + DY[jm] * u1jc);
+    u12 = u12 * fjup + (1. - fjup) * (U(i, N2, k) + U(ip, N2, k)) * 0.5;
+    u11 = u11 * fjum + (1. - fjum) * (U(i, 0, k) + U(ip, 0, k)) * 0.5;
+    const double u3jp = 0.5 * (W(i, jp, k) + W(i, jp, kp));
+    const double u3jc = 0.5 * (W(i, j, k) + W(i, j, kp));
+    const double u3jm = 0.5 * (W(i, jm, k) + W(i, jm, kp));
+    double u32 = 0.5 / H[jp] * (DY[jp] * u3jc + DY[j] * u3jp);
+    double u31 = 0.5 / H[j] * (DY[j] * u3jm + DY[jm] * u3jc);
+    u32 = u32 * fjup + (1. - fjup) * (W(i, N2, k) + W(i, N2, kp)) * 0.5;
+    u31 = u31 * fjum + (1. - fjum) * (W(i, 0, k) + W(i, 0, kp)) * 0.5;
+    const double dv3dx2 = (u32 - u31) / DY[j];
+    const double dv2dx3 = (u2kp - u2km) * 0.5 * s.dx3;
+    const double dv3dx1 = (u3ip - u3im) * 0.5 * s.dx1;
+    const double dv1dx3 = (u1kp - u1km) * 0.5 * s.dx3;
+    const double dv2dx1 = (u2ip - u2im) * 0.5 * s.dx1;
+    const double dv1dx2 = (u12 - u11) / DY[j];
+    const double w1 = dv3dx2 - dv2dx3, w2 = dv1dx3 - dv3dx1, w3 = dv2dx1 - dv1dx2;
+    const double v1 = (U(i, j, k) + U(ip, j, k)) * 0.5;
+    const double v2 = (V(i, j, k) + V(i, jp, k)) * 0.5;
+    const double v3 = (W(i, j, k) + W(i, j, kp)) * 0.5;
+    const double p = F.p[gidx(g, i, j, k)];
+    acc[0] += v1;
+    acc[1] += v2;
+    acc[2] += v3;
+    acc[3] += p;
+    acc[4] += w1;
+    acc[5] += w2;
+    acc[6] += w3;
+    acc[7] += v1 * v1;
+    acc[8] += v2 * v2;
+    acc[9] += v3 * v3;
+    acc[10] += v1 * v2;
+    acc[11] += v1 * v3;
+    acc[12] += v2 * v3;
+    acc[13] += p * p;
+    acc[14] += p * v2;
+    acc[15] += w1 * w1;
+    acc[16] += w2 * w2;
+    acc[17] += w3 * w3;
+    acc[18] += v1 * v1 * v2;
+    acc[19] += v2 * (v3 * v3);
+  }
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < ST_NQ; q++) red[q][t] = acc[q];
+  __syncthreads();
+  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+    if (t < h)
+#pragma unroll
+      for (int q = 0; q < ST_NQ; q++) red[q][t] += red[q][t + h];
+    __syncthreads();
+  }
+  const double nxz = 1.0 / (double)npl;
+  if (t < ST_NQ) out[(long long)t * g.n2m + (j - 1)] = red[t][0] * nxz;
+}
+
 // ghost planes after the update (:1130-1135, :1147-1152, :1210-1215)
 __global__ void k_ghosts(Ctx c) {
   const Geo g = c.g;
@@ -401,6 +498,11 @@ void upcalc(const Ctx& c, bool chk) {
   upd_x(c, chk);
   ghosts(c);
   upd_z(c, chk);
+}
+
+void plane_stats(const Ctx& c, double* out) {
+  KScope ks_(c, KID_CHKMF);
+  hipLaunchKernelGGL(k_plane_stats, dim3(c.g.n2m), dim3(256), 0, c.st, c, out);
 }
 
 void ghosts(const Ctx& c) {

@@ -136,6 +136,12 @@ int c3d_cfl(c3d_handle *h, double *cflm);
 int c3d_chkmf(c3d_handle *h, double *flow1, double *flow3);
 /* the flow rates CHKMF intends: sum over all cells */
 int c3d_flowrate(c3d_handle *h, double *flow1, double *flow3);
+/* the plane statistics ENERGY forms each output interval (compact3d.f:9902,
+ * loops 10 and 20 at :10005-10158) of the resident field:
+ * out[s*(N2-1) + J-1], J = 1..N2-1, s = 0 VM(J,1..3) 3 PM 4 WM(J,1..3)
+ * (mean vorticity) 7 VRMS(J,1..6) (uu vv ww uv uw vw) 13 PRMS 14 PV
+ * 15 WRMS(J,1..3) 18 U2V 19 VW2; 20*(N2-1) values */
+int c3d_stats(c3d_handle *h, double *out);
 
 /* Intermediate fields of the last step, for operator-level parity. */
 enum c3d_field {

@@ -2066,6 +2066,91 @@ void c3o_filter4(c3o *o, double *FILV, int n) {
 #undef TT
 }
 
+/* ------------------------------------------------------------- ENERGY */
+/* The plane statistics of ENERGY (compact3d.f:9902): cell-centre vorticity
+ * (:10005-10069) and, per J plane, the means and second moments of loop 20
+ * (:10073-10158), in the reference's K, I summation order.  out[s*N2M +
+ * J-1], s = 0 VM1 1 VM2 2 VM3 3 PM 4 WM1 5 WM2 6 WM3 7-12 VRMS1..6 (uu vv ww
+ * uv uw vw) 13 PRMS 14 PV 15-17 WRMS1..3 18 U2V 19 VW2. */
+void c3o_plane_stats(c3o *o, const double *U, const double *P, double *out) {
+  const int N1M = o->n1m, N2M = o->n2m, N3M = o->n3m, N2 = o->n2;
+  const double DX1 = o->dx1, DX3 = o->dx3;
+  const double *DY = o->dy, *H = o->h;
+  const double *V = U + NU, *W = U + 2 * NU;
+  const double nxz = 1.0 / (double)(N1M * N3M);
+#define PP(i, j, k) P[((size_t)(k) - 1) * o->n1 * o->n2 + ((size_t)(j) - 1) * o->n1 + ((i) - 1)]
+#pragma omp parallel for
+  for (int j = 1; j <= N2M; j++) {
+    const int jp = j + 1, jm = j - 1;
+    const double fjum = o->fjmu[j], fjup = o->fjpa[j];
+    double acc[20];
+    for (int q = 0; q < 20; q++) acc[q] = 0.0;
+    for (int k = 1; k <= N3M; k++) {
+      const int kp = o->kpa[k], km = o->kma[k];
+      for (int i = 1; i <= N1M; i++) {
+        const int ip = o->ipa[i], im = o->ima[i];
+        /* vorticity at the cell centre (:10024-10069) */
+        const double u1kp = 0.5 * (U[IX(i, j, kp)] + U[IX(ip, j, kp)]);
+        const double u1km = 0.5 * (U[IX(i, j, km)] + U[IX(ip, j, km)]);
+        const double u2kp = 0.5 * (V[IX(i, j, kp)] + V[IX(i, jp, kp)]);
+        const double u2km = 0.5 * (V[IX(i, j, km)] + V[IX(i, jp, km)]);
+        const double u2ip = 0.5 * (V[IX(ip, j, k)] + V[IX(ip, jp, k)]);
+        const double u2im = 0.5 * (V[IX(im, j, k)] + V[IX(im, jp, k)]);
+        const double u3ip = 0.5 * (W[IX(ip, j, k)] + W[IX(ip, j, kp)]);
+        const double u3im = 0.5 * (W[IX(im, j, k)] + W[IX(im, j, kp)]);
+        const double u1jp = 0.5 * (U[IX(i, jp, k)] + U[IX(ip, jp, k)]);
+        const double u1jc = 0.5 * (U[IX(i, j, k)] + U[IX(ip, j, k)]);
+        const double u1jm = 0.5 * (U[IX(i, jm, k)] + U[IX(ip, jm, k)]);
+        double u12 = 0.5 / H[jp] * (DY[jp] * u1jc + DY[j] * u1jp);
+        double u11 = 0.5 / H[j] * (DY[j] * u1jm + DY[jm] * u1jc);
+        u12 = u12 * fjup + (1. - fjup) * (U[IX(i, N2, k)] + U[IX(ip, N2, k)]) * 0.5;
+        u11 = u11 * fjum + (1. - fjum) * (U[IX(i, 0, k)] + U[IX(ip, 0, k)]) * 0.5;
+        const double u3jp = 0.5 * (W[IX(i, jp, k)] + W[IX(i, jp, kp)]);
+        const double u3jc = 0.5 * (W[IX(i, j, k)] + W[IX(i, j, kp)]);
+        const double u3jm = 0.5 * (W[IX(i, jm, k)] + W[IX(i, jm, kp)]);
+        double u32 = 0.5 / H[jp] * (DY[jp] * u3jc + DY[j] * u3jp);
+        double u31 = 0.5 / H[j] * (DY[j] * u3jm + DY[jm] * u3jc);
+        u32 = u32 * fjup + (1. - fjup) * (W[IX(i, N2, k)] + W[IX(i, N2, kp)]) * 0.5;
+        u31 = u31 * fjum + (1. - fjum) * (W[IX(i, 0, k)] + W[IX(i, 0, kp)]) * 0.5;
+        const double dv3dx2 = (u32 - u31) / DY[j];
+        const double dv2dx3 = (u2kp - u2km) * 0.5 * DX3;
+        const double dv3dx1 = (u3ip - u3im) * 0.5 * DX1;
+        const double dv1dx3 = (u1kp - u1km) * 0.5 * DX3;
+        const double dv2dx1 = (u2ip - u2im) * 0.5 * DX1;
+        const double dv1dx2 = (u12 - u11) / DY[j];
+        const double w1 = dv3dx2 - dv2dx3, w2 = dv1dx3 - dv3dx1, w3 = dv2dx1 - dv1dx2;
+        /* loop 20 (:10114-10139) */
+        const double v1 = (U[IX(i, j, k)] + U[IX(ip, j, k)]) * 0.5;
+        const double v2 = (V[IX(i, j, k)] + V[IX(i, jp, k)]) * 0.5;
+        const double v3 = (W[IX(i, j, k)] + W[IX(i, j, kp)]) * 0.5;
+        const double p = PP(i, j, k);
+        acc[0] += v1;
+        acc[1] += v2;
+        acc[2] += v3;
+        acc[3] += p;
+        acc[4] += w1;
+        acc[5] += w2;
+        acc[6] += w3;
+        acc[7] += v1 * v1;
+        acc[8] += v2 * v2;
+        acc[9] += v3 * v3;
+        acc[10] += v1 * v2;
+        acc[11] += v1 * v3;
+        acc[12] += v2 * v3;
+        acc[13] += p * p;
+        acc[14] += p * v2;
+        acc[15] += w1 * w1;
+        acc[16] += w2 * w2;
+        acc[17] += w3 * w3;
+        acc[18] += v1 * v1 * v2;
+        acc[19] += v2 * (v3 * v3);
+      }
+    }
+    for (int q = 0; q < 20; q++) out[(size_t)q * N2M + (j - 1)] = acc[q] * nxz;
+  }
+#undef PP
+}
+
 /* ------------------------------------------------------------------ GETUP */
 /* compact3d.f:879-960 */
 void c3o_getup(c3o *o, double *U, double *P, double *HH, double *SGSVIS,

@@ -70,6 +70,11 @@ void c3o_wallmodel_instant(c3o *o, const double *U);
 void c3o_filter(c3o *o, double *FILV, int n);
 void c3o_filter4(c3o *o, double *FILV, int n);
 
+/* the plane statistics of ENERGY (compact3d.f:9902, loops 10 and 20 at
+ * :10005-10158): out[s*N2M + J-1] for the 20 quantities listed at the
+ * definition */
+void c3o_plane_stats(c3o *o, const double *U, const double *P, double *out);
+
 /* GETUP (compact3d.f:879-960): one full step, scratch allocated inside */
 void c3o_getup(c3o *o, double *U, double *P, double *HH, double *SGSVIS,
                double *presg, double *presg3, int ntime);

@@ -185,6 +185,12 @@ class Oracle:
     def sgscs(self, U, SGSVIS):
         self.L.c3o_sgscs(self.h, _p(U), _p(SGSVIS))
 
+    def plane_stats(self, U, P):
+        """ENERGY's plane statistics: array (20, N2M)"""
+        out = np.zeros(20 * (self.n2 - 1))
+        self.L.c3o_plane_stats(self.h, _p(U), _p(P), _p(out))
+        return out.reshape(20, self.n2 - 1)
+
     def sgsdy(self, U, SGSVIS):
         """SGSDY; returns the dynamic coefficient C(J), J = 0..N2 (1..N2M set)"""
         c = np.zeros(self.n2 + 1)

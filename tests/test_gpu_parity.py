@@ -307,3 +307,30 @@ def test_dynamic_les_trajectory_10_steps(wmodel):
     print("dynamic LES 10 steps wmodel", wmodel, "U", ["%.3e" % x for x in eu], "SGSVIS %.3e" % es)
     assert max(eu) <= 1e-10 and es <= 1e-9
     ch.close()
+
+
+def test_plane_statistics_match_oracle():
+    """ENERGY's plane statistics (means, moments, cell-centre vorticity) of
+    the resident field against the oracle, which closed forms pin
+    (tests/test_oracle_math.py)"""
+    from compact3d_amd import c3d
+    from pyoracle import Oracle
+    n1, n2, n3 = 49, 41, 33
+    cfg = config(n1, n2, n3, cks=1.5)
+    orc = Oracle(**cfg)
+    U, P, _ = inputs(orc, seed=11)
+    ch = c3d.Channel(**cfg)
+    ch.upload(U, P)
+    g = ch.stats()
+    o = orc.plane_stats(U, P)
+    # a mean that is zero up to rounding (the mean wall-normal velocity of a
+    # divergence-free field) is measured against its rms
+    rms = {0: 7, 1: 8, 2: 9, 3: 13, 4: 15, 5: 16, 6: 17}
+    for q, name in enumerate(c3d.Channel.STAT_NAMES):
+        scale = np.max(np.abs(o[q]))
+        if q in rms:
+            scale = max(scale, float(np.sqrt(np.max(o[rms[q]]))))
+        e = float(np.max(np.abs(g[q] - o[q])) / max(scale, 1e-300))
+        print("stat", name, "%.2e" % e)
+        assert e <= 1e-12, (name, e)
+    ch.close()

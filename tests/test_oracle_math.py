@@ -189,3 +189,47 @@ def test_mesh_closed_form():
     dy = np.diff(m["Y"][1:34])
     assert np.allclose(m["DY"][1:33], dy, atol=1e-16)
     assert m["DX1"] == pytest.approx(16 / (2 * math.pi))
+
+
+def _stats_field(n1, n2, n3, Y, alx, alpha=0.7, beta=0.2, b0=0.3, c=0.9, c2=-0.4, m=3):
+    """u = alpha*y + beta (linear in the cell-centre height, wall ghosts on
+    the same line), v = b0, w = c sin(2 pi m x/ALX) + c2 (independent of y
+    and z), P(J) = J: every plane statistic of ENERGY has a closed form"""
+    U = np.zeros((n1 + 1, n2 + 1, n3 + 1, 3), order="F")
+    P = np.zeros((n1, n2, n3), order="F")
+    yc = 0.5 * (Y[:-1] + Y[1:])  # cell centres, J = 0..N2-1
+    for J in range(1, n2):
+        U[:, J, :, 0] = alpha * yc[J] + beta
+        P[:, J - 1, :] = J
+    U[:, 0, :, 0] = alpha * Y[1] + beta        # wall values the face interpolation takes
+    U[:, n2, :, 0] = alpha * Y[n2] + beta
+    U[:, :, :, 1] = b0
+    x = (np.arange(n1 + 1) - 1) * alx / (n1 - 1)
+    U[:, :, :, 2] = (c * np.sin(2 * np.pi * m * x / alx) + c2)[:, None, None]
+    return U, P, yc
+
+
+def test_plane_statistics_closed_forms():
+    """ENERGY's plane statistics (compact3d.f:10005-10158): means, second
+    moments, third moments and the cell-centre vorticity of a field whose
+    statistics are known exactly"""
+    n1, n2, n3 = 33, 25, 17
+    alx = 2 * math.pi
+    o = _o(n1=n1, n2=n2, n3=n3, alx=alx, cks=1.5)
+    Y = o.vec(0)
+    alpha, beta, b0, c, c2, m = 0.7, 0.2, 0.3, 0.9, -0.4, 3
+    U, P, yc = _stats_field(n1, n2, n3, Y, alx, alpha, beta, b0, c, c2, m)
+    s = o.plane_stats(U, P)
+    J = np.arange(1, n2)
+    u = alpha * yc[1:n2] + beta
+    dx = alx / (n1 - 1)
+    ww = 0.5 * c * c + c2 * c2
+    g2 = 0.5 * (c * math.sin(2 * math.pi * m * dx / alx) / dx) ** 2  # <(dw/dx)^2>, central difference
+    expect = {0: u, 1: np.full(n2 - 1, b0), 2: np.full(n2 - 1, c2), 3: J.astype(float),
+              4: np.zeros(n2 - 1), 5: np.zeros(n2 - 1), 6: np.full(n2 - 1, -alpha),
+              7: u * u, 8: np.full(n2 - 1, b0 * b0), 9: np.full(n2 - 1, ww), 10: u * b0, 11: u * c2,
+              12: np.full(n2 - 1, b0 * c2), 13: (J * J).astype(float), 14: J * b0,
+              15: np.zeros(n2 - 1), 16: np.full(n2 - 1, g2), 17: np.full(n2 - 1, alpha * alpha),
+              18: u * u * b0, 19: np.full(n2 - 1, b0 * ww)}
+    for q, e in expect.items():
+        assert np.max(np.abs(s[q] - e)) <= 1e-12 * max(1.0, np.max(np.abs(e))), (q, s[q][:3], e[:3])
