@@ -71,6 +71,11 @@ def lib():
         L.c3d_chkmf.argtypes = [C.c_void_p, _dp, _dp]
         L.c3d_flowrate.argtypes = [C.c_void_p, _dp, _dp]
         L.c3d_stats.argtypes = [C.c_void_p, _dp]
+        L.c3d_cellcentre.argtypes = [C.c_void_p, _dp]
+        L.c3d_write_flow.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
+        L.c3d_write_strain.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_double]
+        L.c3d_format_flow.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
+        L.c3d_format_strain.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, C.c_double]
         L.c3d_get_field.argtypes = [C.c_void_p, C.c_int, _dp]
         L.c3d_get_scalar.argtypes = [C.c_void_p, C.c_int, _dp]
         L.c3d_set_scalar.argtypes = [C.c_void_p, C.c_int, C.c_double]
@@ -102,6 +107,32 @@ def _p(a):
     if a.dtype != np.float64 or not (a.flags.f_contiguous or a.ndim == 1):
         raise C3DError("arrays must be float64 in Fortran order")
     return a.ctypes.data_as(_dp)
+
+
+def format_flow(path, ntime, xc, yc, zc, cell):
+    """WRITEUP's FLOW-*.plt text from host arrays (no GPU): xc, yc, zc =
+    XC(1:N1-1), YC(1:N2-1), ZC(1:N3-1); cell as Channel.cellcentre()"""
+    L = lib()
+    xc, yc, zc = (np.ascontiguousarray(a, dtype=np.float64) for a in (xc, yc, zc))
+    cell = np.ascontiguousarray(cell, dtype=np.float64)
+    if cell.size != 4 * xc.size * yc.size * zc.size:
+        raise C3DError("cell must hold 4*(N1-1)*(N2-1)*(N3-1) values")
+    if L.c3d_format_flow(os.fsencode(path), int(ntime), xc.size, yc.size, zc.size, _p(xc), _p(yc), _p(zc),
+                         _p(cell.reshape(-1))):
+        raise C3DError(L.c3d_last_error(None).decode())
+
+
+def format_strain(path, ntime, xc, zc, tauw, utau):
+    """WRITEUP's STRAIN-*.plt text from host arrays (no GPU): tauw indexed
+    [K-1, I-1] = TAUW(I,K,1)"""
+    L = lib()
+    xc, zc = (np.ascontiguousarray(a, dtype=np.float64) for a in (xc, zc))
+    tauw = np.ascontiguousarray(tauw, dtype=np.float64)
+    if tauw.size != xc.size * zc.size:
+        raise C3DError("tauw must hold (N1-1)*(N3-1) values")
+    if L.c3d_format_strain(os.fsencode(path), int(ntime), xc.size, zc.size, _p(xc), _p(zc),
+                           _p(tauw.reshape(-1)), float(utau)):
+        raise C3DError(L.c3d_last_error(None).decode())
 
 
 def make_config(n1, n2, n3, re, alx, aly, alz, dt, files=0.0, wmodel=0.0, cks=2.5, y=None,
@@ -327,6 +358,22 @@ class Channel:
         out = np.zeros(20 * (self.n2 - 1))
         self._ck(self.L.c3d_stats(self.h, _p(out)), "stats")
         return out.reshape(20, self.n2 - 1)
+
+    def cellcentre(self):
+        """WRITEUP's cell-centre values of the resident field: array
+        (4, N3-1, N2-1, N1-1) of UF, VF, WF, P indexed [q, K-1, J-1, I-1]"""
+        n = (self.n1 - 1) * (self.n2 - 1) * (self.n3 - 1)
+        out = np.zeros(4 * n)
+        self._ck(self.L.c3d_cellcentre(self.h, _p(out)), "cellcentre")
+        return out.reshape(4, self.n3 - 1, self.n2 - 1, self.n1 - 1)
+
+    def write_flow(self, path, ntime):
+        """WRITEUP's FLOW-*.plt of the resident field to `path`"""
+        self._ck(self.L.c3d_write_flow(self.h, os.fsencode(path), int(ntime)), "write_flow")
+
+    def write_strain(self, path, ntime, utau):
+        """WRITEUP's STRAIN-*.plt (bottom-wall TAUW) to `path`"""
+        self._ck(self.L.c3d_write_strain(self.h, os.fsencode(path), int(ntime), float(utau)), "write_strain")
 
     def apply_op(self, op, direction, field):
         inp = np.asfortranarray(field, dtype=np.float64)

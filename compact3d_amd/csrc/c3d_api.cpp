@@ -13,6 +13,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/c3d.h"
@@ -1526,6 +1527,167 @@ extern "C" int c3d_stats(c3d_handle* h, double* out) {
   if (launched(h)) return -1;
   HIPCHK(hipMemcpyAsync(out, h->dtmp, n * sizeof(double), hipMemcpyDeviceToHost, h->c.st));
   HIPCHK(hipStreamSynchronize(h->c.st));
+  return 0;
+}
+
+// ------------------------------------------------------- WRITEUP output
+// The Tecplot files of WRITEUP (compact3d.f:791-832), text as the
+// reference's Fortran runtime writes it: list-directed header lines (a
+// leading blank, items separated by one blank) and '(7F20.10)' records.
+
+// F20.10: 10 decimals right-justified in 20 characters; 20 asterisks when
+// the value does not fit; NaN, Inf, -Inf right-justified
+static void f20_10(char* o, double x) {
+  char b[400];
+  int n;
+  if (std::isnan(x))
+    n = snprintf(b, sizeof b, "NaN");
+  else if (std::isinf(x))
+    n = snprintf(b, sizeof b, x < 0 ? "-Inf" : "Inf");
+  else
+    n = snprintf(b, sizeof b, "%.10f", x);
+  if (n < 0 || n > 20) {
+    memset(o, '*', 20);
+    return;
+  }
+  memset(o, ' ', 20 - n);
+  memcpy(o + 20 - n, b, n);
+}
+
+// records of nv F20.10 values, one per row, formatted by up to 16 threads
+template <typename RowFn>
+static bool write_rows(FILE* f, long long nrows, int nv, RowFn row) {
+  const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const long long w = 20LL * nv + 1, batch = 1 << 16;
+  std::vector<char> buf((size_t)(batch * nth * w));
+  for (long long r0 = 0; r0 < nrows; r0 += batch * nth) {
+    const long long nr = std::min(nrows - r0, batch * nth);
+    std::vector<std::thread> th;
+    for (int t = 0; t < nth; t++)
+      th.emplace_back([&, t] {
+        double v[8];
+        for (long long r = t; r < nr; r += nth) {
+          char* o = buf.data() + r * w;
+          row(r0 + r, v);
+          for (int q = 0; q < nv; q++) f20_10(o + 20 * q, v[q]);
+          o[20 * nv] = '\n';
+        }
+      });
+    for (auto& x : th) x.join();
+    if (fwrite(buf.data(), 1, (size_t)(nr * w), f) != (size_t)(nr * w)) return false;
+  }
+  return true;
+}
+
+extern "C" int c3d_format_flow(const char* path, int ntime, int n1m, int n2m, int n3m, const double* xc,
+                               const double* yc, const double* zc, const double* cell) {
+  if (!path || !xc || !yc || !zc || !cell || n1m < 1 || n2m < 1 || n3m < 1) {
+    g_create_err = "c3d_format_flow: bad arguments";
+    return -1;
+  }
+  FILE* f = fopen(path, "wb");
+  if (!f) {
+    g_create_err = std::string("c3d_format_flow: cannot open ") + path;
+    return -1;
+  }
+  fprintf(f, " VARIABLES=\"x\",\"y\",\"z\",\"U\",\"V\",\"W\",\"P\" \n");
+  fprintf(f, " ZONE T=\" %d \"I=  %d J= %d K= %d\n", ntime, n1m, n2m, n3m);
+  const long long nc = (long long)n1m * n2m * n3m;
+  const bool ok = write_rows(f, nc, 7, [&](long long e, double* v) {
+    const int i = (int)(e % n1m);
+    const long long r = e / n1m;
+    const int j = (int)(r % n2m), k = (int)(r / n2m);
+    v[0] = xc[i];
+    v[1] = yc[j];
+    v[2] = zc[k];
+    for (int q = 0; q < 4; q++) v[3 + q] = cell[q * nc + e];
+  });
+  if (fclose(f) != 0 || !ok) {
+    g_create_err = std::string("c3d_format_flow: write failed: ") + path;
+    return -1;
+  }
+  return 0;
+}
+
+extern "C" int c3d_format_strain(const char* path, int ntime, int n1m, int n3m, const double* xc, const double* zc,
+                                 const double* tauw, double utau) {
+  if (!path || !xc || !zc || !tauw || n1m < 1 || n3m < 1) {
+    g_create_err = "c3d_format_strain: bad arguments";
+    return -1;
+  }
+  // TAUDOWN: the plane mean of TAUW(I,K,1), summed K outer, I inner (:815-821)
+  double td = 0.0;
+  for (int k = 0; k < n3m; k++)
+    for (int i = 0; i < n1m; i++) td += tauw[(size_t)k * n1m + i];
+  td = td / (double)(n1m * n3m);
+  const double u2 = utau * utau;
+  FILE* f = fopen(path, "wb");
+  if (!f) {
+    g_create_err = std::string("c3d_format_strain: cannot open ") + path;
+    return -1;
+  }
+  fprintf(f, " VARIABLES=\"x\",\"z\",\"strain\"\n");
+  fprintf(f, " ZONE T=\" %d \"I=  %d K= %d\n", ntime, n1m, n3m);
+  const bool ok = write_rows(f, (long long)n1m * n3m, 3, [&](long long e, double* v) {
+    const int i = (int)(e % n1m), k = (int)(e / n1m);
+    v[0] = xc[i];
+    v[1] = zc[k];
+    v[2] = (tauw[e] - td) / u2;
+  });
+  if (fclose(f) != 0 || !ok) {
+    g_create_err = std::string("c3d_format_strain: write failed: ") + path;
+    return -1;
+  }
+  return 0;
+}
+
+// XC(1:N1M), YC(1:N2M), ZC(1:N3M) of MESH (:322-333, :363-381)
+static void cell_coords(const c3d_handle* h, std::vector<double>& xc, std::vector<double>& yc,
+                        std::vector<double>& zc) {
+  const int n1 = h->cfg.n1, n2 = h->cfg.n2, n3 = h->cfg.n3;
+  auto grid = [](int i, int n, double al) { return i == n ? al : (double)(i - 1) * al / (double)(n - 1); };
+  xc.resize(n1 - 1);
+  zc.resize(n3 - 1);
+  yc.resize(n2 - 1);
+  for (int i = 1; i < n1; i++) xc[i - 1] = 0.5 * (grid(i + 1, n1, h->cfg.alx) + grid(i, n1, h->cfg.alx));
+  for (int k = 1; k < n3; k++) zc[k - 1] = 0.5 * (grid(k + 1, n3, h->cfg.alz) + grid(k, n3, h->cfg.alz));
+  for (int j = 1; j < n2; j++) yc[j - 1] = 0.5 * (h->Y[j + 1] + h->Y[j]);
+}
+
+extern "C" int c3d_cellcentre(c3d_handle* h, double* out) {
+  if (!h) return -1;
+  if (!out) return fail(h, "c3d_cellcentre needs an output array of 4*(N1-1)*(N2-1)*(N3-1) values");
+  const Geo& g = h->c.g;
+  const size_t nc = (size_t)g.nx * g.n2m * g.nz;
+  for (int q = 0; q < 4; q++) {
+    c3dk::cellcentre(h->c, q, h->dtmp);
+    if (launched(h)) return -1;
+    HIPCHK(hipMemcpyAsync(out + q * nc, h->dtmp, nc * sizeof(double), hipMemcpyDeviceToHost, h->c.st));
+    HIPCHK(hipStreamSynchronize(h->c.st));
+  }
+  return 0;
+}
+
+extern "C" int c3d_write_flow(c3d_handle* h, const char* path, int ntime) {
+  if (!h) return -1;
+  const Geo& g = h->c.g;
+  std::vector<double> cell(4 * (size_t)g.nx * g.n2m * g.nz), xc, yc, zc;
+  if (c3d_cellcentre(h, cell.data())) return -1;
+  cell_coords(h, xc, yc, zc);
+  if (c3d_format_flow(path, ntime, g.nx, g.n2m, g.nz, xc.data(), yc.data(), zc.data(), cell.data()))
+    return fail(h, g_create_err.c_str());
+  return 0;
+}
+
+extern "C" int c3d_write_strain(c3d_handle* h, const char* path, int ntime, double utau) {
+  if (!h) return -1;
+  const Geo& g = h->c.g;
+  std::vector<double> t((size_t)g.nx * g.nz), xc, yc, zc;
+  HIPCHK(hipMemcpyAsync(t.data(), h->c.d.tauw, t.size() * sizeof(double), hipMemcpyDeviceToHost, h->c.st));
+  HIPCHK(hipStreamSynchronize(h->c.st));
+  cell_coords(h, xc, yc, zc);
+  if (c3d_format_strain(path, ntime, g.nx, g.nz, xc.data(), zc.data(), t.data(), utau))
+    return fail(h, g_create_err.c_str());
   return 0;
 }
 

@@ -194,6 +194,7 @@ void divcheck(const Ctx& c);
 void cfl(const Ctx& c);
 void chkmf(const Ctx& c, bool reference_sums);
 void plane_stats(const Ctx& c, double* out);  // ENERGY's plane statistics, 20 x N2M on the device
+void cellcentre(const Ctx& c, int q, double* out);  // WRITEUP's cell-centre quantity q, N1M x N2M x N3M
 void sgscs(const Ctx& c);
 void sgsdy(const Ctx& c);            // SGSDY, the dynamic model (FILES = 1)
 int sgsdy_tiles(const Geo& g);      // tiles per plane of its partial sums

@@ -505,6 +505,40 @@ void plane_stats(const Ctx& c, double* out) {
   hipLaunchKernelGGL(k_plane_stats, dim3(c.g.n2m), dim3(256), 0, c.st, c, out);
 }
 
+// WRITEUP's cell-centre values (compact3d.f:796-808): quantity q (0 UF,
+// 1 VF, 2 WF, 3 P) of every cell, I fastest, then J, then K.  VF averages
+// V(J) with V(JPA(J)), and JPA(N2M) = N2M (:473), so the top row is V(N2M)
+// itself.  TERMP = 1 (:794).
+__global__ __launch_bounds__(C3D_NT) void k_cellcentre(Ctx c, int q, double* out) {
+  const Geo g = c.g;
+  const Fields F = c.d;
+  const long long n = (long long)g.nx * g.n2m * g.nz;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(e % g.nx);
+    const long long r = e / g.nx;
+    const int j = 1 + (int)(r % g.n2m), k = (int)(r / g.n2m);
+    double v;
+    if (q == 0) {
+      v = 0.5 * (F.u[gidx(g, i, j, k)] + F.u[gidx(g, wrapp(i, g.nx), j, k)]);
+    } else if (q == 1) {
+      const int jp = j < g.n2m ? j + 1 : g.n2m;
+      v = 0.5 * (F.v[gidx(g, i, j, k)] + F.v[gidx(g, i, jp, k)]);
+    } else if (q == 2) {
+      v = 0.5 * (F.w[gidx(g, i, j, k)] + F.w[gidx(g, i, j, wrapp(k, g.nz))]);
+    } else {
+      v = F.p[gidx(g, i, j, k)];
+    }
+    out[e] = v;
+  }
+}
+
+void cellcentre(const Ctx& c, int q, double* out) {
+  KScope ks_(c, KID_CHKMF);
+  hipLaunchKernelGGL(k_cellcentre, dim3(pgrid((long long)c.g.nx * c.g.n2m * c.g.nz)), dim3(C3D_NT), 0, c.st, c,
+                     q, out);
+}
+
 void ghosts(const Ctx& c) {
   KScope ks_(c, KID_GHOSTS);
   hipLaunchKernelGGL(k_ghosts, dim3(pgrid((long long)c.g.nx * c.g.nz)), dim3(C3D_NT), 0, c.st, c);

@@ -5,6 +5,10 @@
   I=1..N1M / P(I,J,K) with K fastest, J=1..N2M, I=1..N1M.  (The reference's
   own CWRITE writes a formatted file that READUP cannot read, :854-872; these
   functions are the pair READUP expects.)
+* WRITEUP's Tecplot output (compact3d.f:757-835): RESULT/FLOW-<100000+NTIME>.plt
+  (cell-centre U, V, W, P) and RESULT/STRAIN-<100000+NTIME>.plt (the
+  bottom-wall shear stress fluctuation over UTAU**2), text as the
+  reference's Fortran runtime writes it (write_result).
 * The build's own checkpoint: an .npz with U, P, HH (the AB2 history that
   CONTINUE omits), SGSVIS, PRESG, PRESG3 and the grid, so that a resumed run
   continues bit-for-bit.
@@ -65,3 +69,27 @@ def load_checkpoint(path, channel):
         raise ValueError("checkpoint grid does not match the channel")
     channel.upload(np.asfortranarray(z["U"]), np.asfortranarray(z["P"]), HH=np.asfortranarray(z["HH"]),
                    SGSVIS=np.asfortranarray(z["SGSVIS"]), presg=float(z["PRESG"]), presg3=float(z["PRESG3"]))
+
+
+def result_names(ntime):
+    """WRITEUP's file names: FILENAME = 100000+NTIME written '(I6)' (:782)"""
+    v = 100000 + int(ntime)
+    tag = "%6d" % v
+    if len(tag) > 6:
+        tag = "*" * 6
+    return "FLOW-%s.plt" % tag, "STRAIN-%s.plt" % tag
+
+
+def write_result(channel, ntime, utau, root="."):
+    """WRITEUP(U,P,PRESG,PRESG3,NTIME) of the resident field: writes
+    root/RESULT/FLOW-*.plt and root/RESULT/STRAIN-*.plt.  utau is WRITEUP's
+    UTAU: SQRT(WSMO(1)) without a wall model, else SQRT(TAOWO) (:785-789).
+    Returns the two paths."""
+    import os
+    d = os.path.join(root, "RESULT")
+    os.makedirs(d, exist_ok=True)
+    flow, strain = (os.path.join(d, n) for n in result_names(ntime))
+    channel.write_flow(flow, ntime)
+    channel.write_strain(strain, ntime, utau)
+    return flow, strain
+

@@ -143,6 +143,26 @@ int c3d_flowrate(c3d_handle *h, double *flow1, double *flow3);
  * 15 WRMS(J,1..3) 18 U2V 19 VW2; 20*(N2-1) values */
 int c3d_stats(c3d_handle *h, double *out);
 
+/* WRITEUP's Tecplot output (compact3d.f:757-835).
+ * c3d_cellcentre: the cell-centre values WRITEUP writes (:796-808),
+ * out[q*n + (K-1)*(N2-1)*(N1-1) + (J-1)*(N1-1) + I-1], n = (N1-1)(N2-1)(N3-1),
+ * q = 0 UF 1 VF 2 WF 3 P.
+ * c3d_write_flow: RESULT/FLOW-*.plt of the resident field (:791-812) to
+ * `path`, text as the reference's Fortran runtime writes it.
+ * c3d_write_strain: RESULT/STRAIN-*.plt (:814-832) from the bottom-wall
+ * TAUW; utau = SQRT(WSMO(1)) (WMODEL=0) or SQRT(TAOWO) as WRITEUP forms it.
+ * c3d_format_flow / c3d_format_strain: the same files from host arrays
+ * (no handle, no GPU): xc[N1-1], yc[N2-1], zc[N3-1] = XC(1:), YC(1:),
+ * ZC(1:); cell as c3d_cellcentre's out; tauw[(K-1)*(N1-1) + I-1] =
+ * TAUW(I,K,1).  Errors: c3d_last_error(NULL). */
+int c3d_cellcentre(c3d_handle *h, double *out);
+int c3d_write_flow(c3d_handle *h, const char *path, int ntime);
+int c3d_write_strain(c3d_handle *h, const char *path, int ntime, double utau);
+int c3d_format_flow(const char *path, int ntime, int n1m, int n2m, int n3m, const double *xc, const double *yc,
+                    const double *zc, const double *cell);
+int c3d_format_strain(const char *path, int ntime, int n1m, int n3m, const double *xc, const double *zc,
+                      const double *tauw, double utau);
+
 /* Intermediate fields of the last step, for operator-level parity. */
 enum c3d_field {
   C3D_F_RUH1 = 1, C3D_F_RUH2, C3D_F_RUH3, /* U-like single component */

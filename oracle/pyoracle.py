@@ -191,6 +191,36 @@ class Oracle:
         self.L.c3o_plane_stats(self.h, _p(U), _p(P), _p(out))
         return out.reshape(20, self.n2 - 1)
 
+    # ---- WRITEUP (compact3d.f:757-835), restated in numpy
+    @staticmethod
+    def writeup_cells(U, P):
+        """WRITEUP's cell-centre values (compact3d.f:796-808): array
+        (4, N3M, N2M, N1M) of UF, VF, WF, P indexed [q, K-1, J-1, I-1].
+        IP, KP wrap periodically (INDICES :446-494); JPA(N2M) = N2M (:473),
+        so VF at J = N2M is V(N2M) itself; TERMP = 1 (:794)."""
+        n1m, n2m, n3m = P.shape[0] - 1, P.shape[1] - 1, P.shape[2] - 1
+        I, J, K = np.arange(1, n1m + 1), np.arange(1, n2m + 1), np.arange(1, n3m + 1)
+        IP = np.where(I < n1m, I + 1, 1)
+        JP = np.where(J < n2m, J + 1, n2m)
+        KP = np.where(K < n3m, K + 1, 1)
+        u, v, w = U[..., 0], U[..., 1], U[..., 2]
+        uf = 0.5 * (u[np.ix_(I, J, K)] + u[np.ix_(IP, J, K)])
+        vf = 0.5 * (v[np.ix_(I, J, K)] + v[np.ix_(I, JP, K)])
+        wf = 0.5 * (w[np.ix_(I, J, K)] + w[np.ix_(I, J, KP)])
+        pp = P[np.ix_(I - 1, J - 1, K - 1)]
+        return np.stack([a.transpose(2, 1, 0) for a in (uf, vf, wf, pp)])
+
+    @staticmethod
+    def writeup_coords(n1, n3, alx, alz, Y):
+        """XC(1:N1M), YC(1:N2M), ZC(1:N3M) of MESH (:322-333, :363-381):
+        XG(I) = (I-1)*ALX/(M1-1) with XG(M1) = ALX, XC(I) = (XG(I+1)+XG(I))/2"""
+        def centres(n, al):
+            g = lambda i: np.where(i == n, al, (i - 1).astype(np.float64) * al / float(n - 1))
+            i = np.arange(1, n)
+            return 0.5 * (g(i + 1) + g(i))
+        Y = np.asarray(Y, dtype=np.float64)
+        return centres(n1, alx), 0.5 * (Y[2:] + Y[1:-1]), centres(n3, alz)
+
     def sgsdy(self, U, SGSVIS):
         """SGSDY; returns the dynamic coefficient C(J), J = 0..N2 (1..N2M set)"""
         c = np.zeros(self.n2 + 1)
@@ -389,6 +419,24 @@ class RefLib:
         c = np.zeros(self.n2)
         self.L.refh_getsmagc_(_p(c))
         return np.concatenate([[0.0], c])
+
+    def writeup(self, U, P, presg, presg3, ntime, wsmo=(0.0, 0.0, 0.0), taowo=0.0):
+        """the reference's WRITEUP, run in a scratch directory; returns the
+        text of RESULT/FLOW-*.plt and RESULT/STRAIN-*.plt"""
+        self.L.refh_setwsmo_(_p(np.ascontiguousarray(wsmo, dtype=np.float64)), C.byref(C.c_double(taowo)))
+        a, b = C.c_double(presg), C.c_double(presg3)
+        cwd = os.getcwd()
+        with tempfile.TemporaryDirectory() as td:
+            os.makedirs(os.path.join(td, "RESULT"))
+            os.chdir(td)
+            try:
+                self.L.writeup_(_p(U), _p(P), C.byref(a), C.byref(b), self._i(ntime))
+            finally:
+                os.chdir(cwd)
+            name = "%06d.plt" % (100000 + ntime)
+            flow = open(os.path.join(td, "RESULT", "FLOW-" + name)).read()
+            strain = open(os.path.join(td, "RESULT", "STRAIN-" + name)).read()
+        return flow, strain
 
     def wallmodel_instant(self, U, SGSVIS):
         tw, tb = self.get_tau()

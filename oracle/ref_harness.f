@@ -125,6 +125,17 @@ C
       RETURN
       END
 
+      SUBROUTINE REFH_SETWSMO(W,XTAOWO)
+      COMMON/WSMEM/WSM(3),WSMO(3)
+      COMMON/WALLTAOAVE/TAOWO
+      REAL W(3)
+      WSMO(1)=W(1)
+      WSMO(2)=W(2)
+      WSMO(3)=W(3)
+      TAOWO=XTAOWO
+      RETURN
+      END
+
       SUBROUTINE REFH_GETSMAGC(C)
       INCLUDE 'PARAM.H'
       COMMON/SGSMEAN/STRM(M2,6),SGSNUT(M2),SGSDS(M2),SMAGC(M2)

@@ -52,3 +52,22 @@ def interior_u(a, n1, n2, n3, comp=None, j0=1):
 
 def interior_p(a, n1, n2, n3):
     return a[: n1 - 1, : n2 - 1, : n3 - 1]
+
+
+# values that exercise the F20.10 edge cases of WRITEUP's records: signed
+# zero, rounding at the 10th decimal, NaN, infinities and overflow to
+# asterisks (|x| >= 1e9, or <= -1e8)
+F20_SPECIAL = (-0.0, -1e-12, 5e-11, 4.9999e-11, 2.5e-10, 1.00000000005, 0.125, float("nan"), float("inf"),
+               -float("inf"), 2e9, 999999999.9999, 1e9, -99999999.99, -1e8, -123456789.5)
+
+
+def writeup_inputs(orc, seed=7):
+    """U, P for the WRITEUP cases: the seeded channel field of inputs(), with
+    F20_SPECIAL planted in P(1:16,1,1), and a random bottom-wall TAUW(I,K,1)
+    (I = 1..N1-1, K = 1..N3-1) as an array [K-1, I-1]"""
+    U, P, _ = inputs(orc, seed=seed)
+    for q, v in enumerate(F20_SPECIAL):
+        P[q, 0, 0] = v
+    rng = np.random.default_rng(2000 + seed)
+    tauw = rng.standard_normal((orc.n3 - 1, orc.n1 - 1)) * 1e-3 + 2.5e-3
+    return U, P, tauw

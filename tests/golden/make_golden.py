@@ -10,6 +10,9 @@ GETDP needs FFTW, which this image does not have, so the reference cannot run
 it; the fixture's UPCALC input DP is the oracle's GETDP of the reference's RDP
 (GETDP itself is pinned by closed forms in tests/test_oracle_math.py).
 
+WRITEUP's Tecplot text is stored as checksums and sample lines
+(ref_writeup.json).
+
 Usage:  python tests/golden/make_golden.py      (build container only)
 """
 import math
@@ -127,6 +130,32 @@ def run_case(name, cfg):
     print(path, os.path.getsize(path), "bytes")
 
 
+def run_writeup():
+    """WRITEUP's FLOW and STRAIN files for the writeup_inputs() field:
+    sha256 of each text, its line count and a few of its lines"""
+    import hashlib
+    import json
+    from cases import writeup_inputs
+    cfg = CASES["dns"]
+    ref = RefLib(**cfg)
+    orc = Oracle(ref.n1, ref.n2, ref.n3, cks=2.5, **cfg)
+    U, P, tauw = writeup_inputs(orc)
+    tw, tb = ref.get_tau()
+    tw[1:ref.n1, 1:ref.n3, 0] = tauw.T
+    ref.set_tau(tw, tb)
+    out = {"ntime": 4321, "wsmo1": 2.7e-3}
+    flow, strain = ref.writeup(U, P, 0.003, -0.001, out["ntime"], wsmo=(out["wsmo1"], 0.0, 0.0))
+    for name, text in (("flow", flow), ("strain", strain)):
+        lines = text.splitlines(keepends=True)
+        out[name] = {"sha256": hashlib.sha256(text.encode()).hexdigest(), "lines": len(lines),
+                     "head": lines[:4], "special": lines[2:2 + 16] if name == "flow" else []}
+    path = os.path.join(HERE, "ref_writeup.json")
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1)
+    print(path, os.path.getsize(path), "bytes")
+
+
 if __name__ == "__main__":
     for name, cfg in CASES.items():
         run_case(name, cfg)
+    run_writeup()

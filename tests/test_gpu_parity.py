@@ -334,3 +334,48 @@ def test_plane_statistics_match_oracle():
         print("stat", name, "%.2e" % e)
         assert e <= 1e-12, (name, e)
     ch.close()
+
+
+def test_writeup_output_matches_reference(tmp_path):
+    """WRITEUP's cell-centre values on the GPU equal the oracle's bit for
+    bit, and the FLOW file written from the resident field is the reference
+    WRITEUP's text (committed checksum of the reference's own file); the
+    STRAIN file of a wall-modelled step equals the formatter applied to the
+    resident TAUW"""
+    import hashlib
+    import json
+    import os
+    from compact3d_amd import c3d
+    from compact3d_amd.restart import write_result
+    from pyoracle import Oracle
+    from cases import writeup_inputs
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "ref_writeup.json")))
+    cfg = config(*REF_GRID)
+    orc = Oracle(**cfg)
+    U, P, _ = writeup_inputs(orc)
+    ch = c3d.Channel(**cfg)
+    ch.upload(U, P)
+    cells = ch.cellcentre()
+    assert np.array_equal(cells, Oracle.writeup_cells(U, P), equal_nan=True)
+    flow, strain = write_result(ch, gold["ntime"], np.sqrt(gold["wsmo1"]), root=str(tmp_path))
+    text = open(flow, "rb").read()
+    assert text.decode().splitlines(keepends=True)[:4] == gold["flow"]["head"]
+    assert hashlib.sha256(text).hexdigest() == gold["flow"]["sha256"]
+    ch.close()
+    # wall-modelled LES: the STRAIN file from the TAUW the step left on the device
+    cfg = config(*REF_GRID, re=20000.0, dt=5e-4, files=2.0, wmodel=1.0)
+    orc = Oracle(**cfg)
+    U, P, _ = inputs(orc, seed=3)
+    ch = c3d.Channel(**cfg)
+    ch.upload(U, P)
+    ch.step(1)
+    tw = ch.field(c3d.F_TAUW)
+    n1, n3 = cfg["n1"], cfg["n3"]
+    tauw = tw[1:n1, 1:n3, 0].T
+    assert np.max(np.abs(tauw)) > 0
+    s_gpu, s_host = tmp_path / "s_gpu.plt", tmp_path / "s_host.plt"
+    ch.write_strain(str(s_gpu), 5, 0.05)
+    xc, yc, zc = Oracle.writeup_coords(n1, n3, cfg["alx"], cfg["alz"], ch.mesh()["Y"])
+    c3d.format_strain(str(s_host), 5, xc, zc, tauw, 0.05)
+    assert s_gpu.read_bytes() == s_host.read_bytes()
+    ch.close()
