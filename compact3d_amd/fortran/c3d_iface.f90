@@ -70,6 +70,19 @@ module c3d_iface
       import :: c_ptr, c_int
       type(c_ptr), value :: h
     end function
+    integer(c_int) function c3d_write_flow(h, path, ntime) bind(C, name='c3d_write_flow')
+      import :: c_ptr, c_int, c_char
+      type(c_ptr), value :: h
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: ntime
+    end function
+    integer(c_int) function c3d_write_strain(h, path, ntime, utau) bind(C, name='c3d_write_strain')
+      import :: c_ptr, c_int, c_char, c_double
+      type(c_ptr), value :: h
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: ntime
+      real(c_double), value :: utau
+    end function
     integer(c_int) function c3d_get_scalar(h, which, v) bind(C, name='c3d_get_scalar')
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: h

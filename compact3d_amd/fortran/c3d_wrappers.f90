@@ -5,6 +5,7 @@
 !   CFL(U,CFLM)                                  :9138
 !   CHKMF(U,TIME)                                :9463
 !   WALLSS(U,P,TIME)                             :11077
+!   WRITEUP(U,P,PRESG,PRESG3,NTIME)              :757
 ! The state is resident in HBM between calls: C3D_BEGIN uploads it once and
 ! C3D_END (or C3D_SYNC) copies it back into the caller's arrays.  The array
 ! arguments of the per-step routines are therefore not read or written; they
@@ -106,3 +107,30 @@ subroutine WALLSS(U, P, TIME)
   rc = c3d_wallss(h3d)
   call c3d_check(rc, h3d, 'WALLSS')
 end subroutine WALLSS
+
+! RESULT/FLOW-*.plt and RESULT/STRAIN-*.plt of the resident field.  UTAU is
+! formed as WRITEUP forms it (:785-789), from the caller's COMMON blocks
+! WSMEM (WSMO) and WALLTAOAVE (TAOWO); the RESULT directory must exist, as
+! for the reference's OPEN.
+subroutine WRITEUP(U, P, PRESG, PRESG3, NTIME)
+  use c3d_state
+  implicit none
+  real(8) :: U(*), P(*), PRESG, PRESG3
+  integer :: NTIME
+  real(8) :: WSM, WSMO, TAOWO, WMODEL, UTAU
+  common /WSMEM/ WSM(3), WSMO(3)
+  common /WALLTAOAVE/ TAOWO
+  common /WALLMODEL/ WMODEL
+  character(len=6) :: tag
+  integer(c_int) :: rc
+  write(tag, '(I6)') 100000 + NTIME
+  if (WMODEL == 0.0d0) then
+    UTAU = sqrt(WSMO(1))
+  else
+    UTAU = sqrt(TAOWO)
+  end if
+  rc = c3d_write_flow(h3d, 'RESULT/FLOW-'//tag//'.plt'//c_null_char, int(NTIME, c_int))
+  call c3d_check(rc, h3d, 'WRITEUP')
+  rc = c3d_write_strain(h3d, 'RESULT/STRAIN-'//tag//'.plt'//c_null_char, int(NTIME, c_int), UTAU)
+  call c3d_check(rc, h3d, 'WRITEUP')
+end subroutine WRITEUP

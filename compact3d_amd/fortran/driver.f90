@@ -57,6 +57,8 @@ program c3d_driver
   end do
   call system_clock(c1)
   write(*, '(A,F12.6)') 'ms_per_step=', 1000.0d0 * dble(c1 - c0) / dble(cr) / dble(NTST)
+  ! the final WRITEUP of MAIN (:177), into ./RESULT
+  call WRITEUP(U, P, PRESG, PRESG3, NTIME)
   call C3D_END(U, P, HH, SGSVIS, PRESG, PRESG3)
   write(*, '(A,ES22.14)') 'U(5,5,5,1)=', U(5,5,5,1)
 end program c3d_driver
