@@ -1980,6 +1980,352 @@ void c3o_sgsdy(c3o *o, const double *U, double *SGSVIS, double *smagc_out) {
   }
 }
 
+/* ----------------------------------------------------------------- SGSSD */
+/* The scale-dependent dynamic model, compact3d.f:1502-1948, with
+ * COEFFICIENT (:2079-2097), DQRRT (:11548-11570) and CHHQR (:11571-11677).
+ *
+ * CHHQR finds the eigenvalues of an upper Hessenberg matrix by the
+ * double-shift QR iteration (Francis; the "hqr" procedure of Wilkinson and
+ * Reinsch), deflating where a subdiagonal entry is at most EPS times its two
+ * diagonal neighbours.  DQRRT calls it with EPS = 1e-5 on the companion
+ * matrix of x^5 + PA(1) x^4 + ... + PA(5), so the roots carry the error of
+ * that loose deflation and depend on the iteration path: the restatement
+ * keeps the reference's steps (no exceptional shifts, its reflector
+ * formulas, at most 60 iterations per root) so that it reproduces them.
+ * A (n x n) column-major, wr/wi the real and imaginary parts; returns 0 when
+ * an eigenvalue took 60 iterations ('FAIL'), leaving the rest unset. */
+static int chhqr(double *A, int n, double *wr, double *wi, double eps) {
+#define HA(i, j) A[((j) - 1) * n + ((i) - 1)]
+  int m = n, it = 0;
+  for (;;) {
+    if (m == 0) return 1;
+    int l = m;
+    while (l > 1 && fabs(HA(l, l - 1)) > eps * (fabs(HA(l - 1, l - 1)) + fabs(HA(l, l)))) l--;
+    if (l == m) {
+      wr[m - 1] = HA(m, m);
+      wi[m - 1] = 0.0;
+      m -= 1;
+      it = 0;
+      continue;
+    }
+    if (l == m - 1) {
+      const double b = -(HA(m, m) + HA(m - 1, m - 1));
+      const double c = HA(m, m) * HA(m - 1, m - 1) - HA(m, m - 1) * HA(m - 1, m);
+      const double w = b * b - 4 * c;
+      const double y = sqrt(fabs(w));
+      if (w > 0.0) {
+        const double xy = (b < 0.0) ? -1.0 : 1.0;
+        wr[m - 1] = (-b - xy * y) / 2.0;
+        wr[m - 2] = c / wr[m - 1];
+        wi[m - 1] = 0.0;
+        wi[m - 2] = 0.0;
+      } else {
+        wr[m - 1] = -b / 2.0;
+        wr[m - 2] = wr[m - 1];
+        wi[m - 1] = y / 2.0;
+        wi[m - 2] = -wi[m - 1];
+      }
+      m -= 2;
+      it = 0;
+      continue;
+    }
+    if (it >= 60) return 0;
+    it += 1;
+    for (int j = l + 2; j <= m; j++) HA(j, j - 2) = 0.0;
+    for (int j = l + 3; j <= m; j++) HA(j, j - 3) = 0.0;
+    for (int k = l; k <= m - 1; k++) {
+      double p, q, r, x, y;
+      if (k != l) {
+        p = HA(k, k - 1);
+        q = HA(k + 1, k - 1);
+        r = 0.0;
+        if (k != m - 1) r = HA(k + 2, k - 1);
+      } else {
+        x = HA(m, m) + HA(m - 1, m - 1);
+        y = HA(m - 1, m - 1) * HA(m, m) - HA(m - 1, m) * HA(m, m - 1);
+        p = HA(l, l) * (HA(l, l) - x) + HA(l, l + 1) * HA(l + 1, l) + y;
+        q = HA(l + 1, l) * (HA(l, l) + HA(l + 1, l + 1) - x);
+        r = HA(l + 1, l) * HA(l + 2, l + 1);
+      }
+      if (fabs(p) + fabs(q) + fabs(r) != 0.0) {
+        const double xy = (p < 0.0) ? -1.0 : 1.0;
+        const double s = xy * sqrt(p * p + q * q + r * r);
+        if (k != l) HA(k, k - 1) = -s;
+        const double e = -q / s, f = -r / s;
+        x = -p / s;
+        y = -x - f * r / (p + s);
+        const double g = e * r / (p + s);
+        const double z = -x - e * q / (p + s);
+        for (int j = k; j <= m; j++) {
+          p = x * HA(k, j) + e * HA(k + 1, j);
+          q = e * HA(k, j) + y * HA(k + 1, j);
+          r = f * HA(k, j) + g * HA(k + 1, j);
+          if (k != m - 1) {
+            p = p + f * HA(k + 2, j);
+            q = q + g * HA(k + 2, j);
+            r = r + z * HA(k + 2, j);
+            HA(k + 2, j) = r;
+          }
+          HA(k + 1, j) = q;
+          HA(k, j) = p;
+        }
+        int jj = k + 3;
+        if (jj >= m) jj = m;
+        for (int i = l; i <= jj; i++) {
+          p = x * HA(i, k) + e * HA(i, k + 1);
+          q = e * HA(i, k) + y * HA(i, k + 1);
+          r = f * HA(i, k) + g * HA(i, k + 1);
+          if (k != m - 1) {
+            p = p + f * HA(i, k + 2);
+            q = q + g * HA(i, k + 2);
+            r = r + z * HA(i, k + 2);
+            HA(i, k + 2) = r;
+          }
+          HA(i, k + 1) = q;
+          HA(i, k) = p;
+        }
+      }
+    }
+  }
+#undef HA
+}
+
+/* DQRRT (:11548-11570): BETA = the largest real part of a root of
+ * x^5 + PA(1) x^4 + ... + PA(5) in (0, 2), else 0.  Real parts of complex
+ * roots count too.  Roots CHHQR leaves unset after a failure are taken as
+ * 0 (not selected); in the reference they are uninitialised. */
+double c3o_dqrrt(const double *pa, int n) {
+  double B[25], xr[5] = {0, 0, 0, 0, 0}, xi[5];
+  if (n > 5) return 0.0;
+  for (int j = 1; j <= n; j++) B[(j - 1) * n] = -pa[j - 1];
+  for (int i = 2; i <= n; i++)
+    for (int j = 1; j <= n; j++) B[(j - 1) * n + (i - 1)] = 0.0;
+  for (int i = 1; i <= n - 1; i++) B[(i - 1) * n + i] = 1.0;
+  chhqr(B, n, xr, xi, 1.0E-5);
+  double beta = 0.;
+  for (int j = 0; j < n; j++)
+    if (xr[j] > beta && xr[j] < 2.0) beta = xr[j];
+  return beta;
+}
+
+/* one test-filter level of SGSSD's coefficient sums (:1544-1667 with FILTER,
+ * :1672-1800 with FILTER4): out = A, B, C, D, E scaled as the reference */
+static void sgssd_sums(c3o *o, const double *U, int j, int four, double *SR, double *COM, double *COM1,
+                       double *SR1, double *UC, double out[5]) {
+  const int N1M = o->n1m, N3M = o->n3m;
+  const double DX1 = o->dx1, DX3 = o->dx3;
+  const double nxz = 1.0 / (double)(N1M * N3M);
+  const double *V = U + NU, *W = U + 2 * NU;
+  const size_t pl = (size_t)o->n1 * o->n3;
+#define DYX(A, i, k, n) A[((size_t)(n) - 1) * pl + ((size_t)(k) - 1) * o->n1 + ((i) - 1)]
+  const double dxyz = 1.0 / DX1 * o->dy[j] / DX3;
+  c3o_strain(o, U, SR, j);
+  for (int k = 1; k <= N3M; k++)
+    for (int i = 1; i <= N1M; i++) {
+      const double delta2 = pow(dxyz, 2. / 3.);
+      const double sumsr = 2. * (DYX(SR, i, k, 2) * DYX(SR, i, k, 2)) + DYX(SR, i, k, 1) * DYX(SR, i, k, 1) +
+                           2. * (DYX(SR, i, k, 3) * DYX(SR, i, k, 3)) + DYX(SR, i, k, 4) * DYX(SR, i, k, 4) +
+                           2. * (DYX(SR, i, k, 5) * DYX(SR, i, k, 5)) + DYX(SR, i, k, 6) * DYX(SR, i, k, 6);
+      const double abssr = sqrt(2. * sumsr);
+      const double delabs = delta2 * abssr;
+      for (int m = 1; m <= 6; m++) DYX(COM1, i, k, m) = delabs * DYX(SR, i, k, m);
+    }
+  if (four) {
+    c3o_filter4(o, COM1, 6);
+    c3o_filter4(o, SR, 6);
+  } else {
+    c3o_filter(o, COM1, 6);
+    c3o_filter(o, SR, 6);
+  }
+  for (int k = 1; k <= N3M; k++)
+    for (int i = 1; i <= N1M; i++) {
+      const double sumfsr = 2. * (DYX(SR, i, k, 2) * DYX(SR, i, k, 2)) + DYX(SR, i, k, 1) * DYX(SR, i, k, 1) +
+                            2. * (DYX(SR, i, k, 3) * DYX(SR, i, k, 3)) + DYX(SR, i, k, 4) * DYX(SR, i, k, 4) +
+                            2. * (DYX(SR, i, k, 5) * DYX(SR, i, k, 5)) + DYX(SR, i, k, 6) * DYX(SR, i, k, 6);
+      const double absfsr = sqrt(2. * sumfsr);
+      const double fdxyz = pow(dxyz, 2. / 3.);
+      const double fdabs = fdxyz * absfsr;
+      for (int m = 1; m <= 6; m++) DYX(SR1, i, k, m) = fdabs * DYX(SR, i, k, m);
+    }
+  const int jp = j + 1;
+  for (int k = 1; k <= N3M; k++) {
+    const int kp = o->kpa[k];
+    for (int i = 1; i <= N1M; i++) {
+      const int ip = o->ipa[i];
+      const double u1 = (U[IX(i, j, k)] + U[IX(ip, j, k)]) * 0.5;
+      const double u2 = (V[IX(i, j, k)] + V[IX(i, jp, k)]) * 0.5;
+      const double u3 = (W[IX(i, j, k)] + W[IX(i, j, kp)]) * 0.5;
+      DYX(UC, i, k, 1) = u1;
+      DYX(UC, i, k, 2) = u2;
+      DYX(UC, i, k, 3) = u3;
+      DYX(COM, i, k, 1) = u1 * u1;
+      DYX(COM, i, k, 2) = u1 * u2;
+      DYX(COM, i, k, 3) = u1 * u3;
+      DYX(COM, i, k, 4) = u2 * u2;
+      DYX(COM, i, k, 5) = u2 * u3;
+      DYX(COM, i, k, 6) = u3 * u3;
+    }
+  }
+  if (four) {
+    c3o_filter4(o, UC, 3);
+    c3o_filter4(o, COM, 6);
+  } else {
+    c3o_filter(o, UC, 3);
+    c3o_filter(o, COM, 6);
+  }
+  double a = 0.0, b = 0.0;
+  for (int k = 1; k <= N3M; k++)
+    for (int i = 1; i <= N1M; i++) {
+      const double c1 = DYX(UC, i, k, 1), c2 = DYX(UC, i, k, 2), c3 = DYX(UC, i, k, 3);
+      const double xl11 = DYX(COM, i, k, 1) - c1 * c1, xl12 = DYX(COM, i, k, 2) - c1 * c2;
+      const double xl13 = DYX(COM, i, k, 3) - c1 * c3, xl22 = DYX(COM, i, k, 4) - c2 * c2;
+      const double xl23 = DYX(COM, i, k, 5) - c2 * c3, xl33 = DYX(COM, i, k, 6) - c3 * c3;
+      /* the accumulator is the first operand of the reference's sum */
+      a = a + 2. * xl12 * DYX(SR1, i, k, 2) + xl11 * DYX(SR1, i, k, 1) + 2. * xl13 * DYX(SR1, i, k, 3) +
+          xl22 * DYX(SR1, i, k, 4) + 2. * xl23 * DYX(SR1, i, k, 5) + xl33 * DYX(SR1, i, k, 6);
+      b = b + 2. * xl12 * DYX(COM1, i, k, 2) + xl11 * DYX(COM1, i, k, 1) + 2. * xl13 * DYX(COM1, i, k, 3) +
+          xl22 * DYX(COM1, i, k, 4) + 2. * xl23 * DYX(COM1, i, k, 5) + xl33 * DYX(COM1, i, k, 6);
+    }
+  double c = 0.0, d = 0.0, e = 0.0;
+  for (int k = 1; k <= N3M; k++)
+    for (int i = 1; i <= N1M; i++) {
+#define C1(m) DYX(COM1, i, k, m)
+#define S1(m) DYX(SR1, i, k, m)
+      c = c + 2. * C1(2) * C1(2) + C1(1) * C1(1) + 2. * C1(3) * C1(3) + C1(4) * C1(4) + 2. * C1(5) * C1(5) +
+          C1(6) * C1(6);
+      d = d + 2. * S1(2) * S1(2) + S1(1) * S1(1) + 2. * S1(3) * S1(3) + S1(4) * S1(4) + 2. * S1(5) * S1(5) +
+          S1(6) * S1(6);
+      e = e + 2. * S1(2) * C1(2) + S1(1) * C1(1) + 2. * S1(3) * C1(3) + S1(4) * C1(4) + 2. * S1(5) * C1(5) +
+          S1(6) * C1(6);
+#undef C1
+#undef S1
+    }
+  if (four) {
+    out[0] = -32.0 * a * nxz;
+    out[1] = -2.0 * b * nxz;
+    out[2] = 4.0 * c * nxz;
+    out[3] = 1024.0 * d * nxz;
+    out[4] = 128.0 * e * nxz;
+  } else {
+    out[0] = -8.0 * a * nxz;
+    out[1] = -2.0 * b * nxz;
+    out[2] = 4.0 * c * nxz;
+    out[3] = 64.0 * d * nxz;
+    out[4] = 32.0 * e * nxz;
+  }
+#undef DYX
+}
+
+/* COEFFICIENT (:2089-2093): PA(1..5) from the two filter levels' sums */
+void c3o_sd_coefficient(const double s1[5], const double s2[5], double pa[5]) {
+  const double A1 = s1[0], B1 = s1[1], C1 = s1[2], D1 = s1[3], E1 = s1[4];
+  const double A2 = s2[0], B2 = s2[1], C2 = s2[2], D2 = s2[3], E2 = s2[4];
+  pa[4] = (B2 * C1 - B1 * C2) / (A1 * D2);
+  pa[3] = (A1 * C2 - B2 * E1) / (A1 * D2);
+  pa[2] = (B2 * D1 + B1 * E2 - A2 * C1) / (A1 * D2);
+  pa[1] = (A2 * E1 - A1 * E2) / (A1 * D2);
+  pa[0] = -(A2 * D1 + B1 * D2) / (A1 * D2);
+}
+
+/* SGSSD: per plane the sums of both filter levels, BETA from the quintic,
+ * then SGSDY's Germano step with the test-filter factor 4*BETA (:1804-1930).
+ * smagc_out[J], beta_out[J] (J = 1..N2M) when not NULL. */
+void c3o_sgssd(c3o *o, const double *U, double *SGSVIS, double *smagc_out, double *beta_out) {
+  const int N1M = o->n1m, N2M = o->n2m, N3M = o->n3m;
+  const double RE = o->re, DX1 = o->dx1, DX3 = o->dx3;
+  const double nxz = 1.0 / (double)(N1M * N3M);
+  const double reninv = 1. / (double)RE;
+  const double *V = U + NU, *W = U + 2 * NU;
+#pragma omp parallel for
+  for (int j = 1; j <= N2M; j++) {
+    const size_t pl = (size_t)o->n1 * o->n3;
+    double *SR = dalloc(pl * 6), *COM = dalloc(pl * 6), *COM1 = dalloc(pl * 6), *SR1 = dalloc(pl * 6);
+    double *UC = dalloc(pl * 3);
+#define DYX(A, i, k, n) A[((size_t)(n) - 1) * pl + ((size_t)(k) - 1) * o->n1 + ((i) - 1)]
+    double s1[5], s2[5], pa[5];
+    sgssd_sums(o, U, j, 0, SR, COM, COM1, SR1, UC, s1);
+    sgssd_sums(o, U, j, 1, SR, COM, COM1, SR1, UC, s2);
+    c3o_sd_coefficient(s1, s2, pa);
+    const double beta = c3o_dqrrt(pa, 5);
+    if (beta_out) beta_out[j] = beta;
+    const double dxyz = 1.0 / DX1 * o->dy[j] / DX3;
+    c3o_strain(o, U, SR, j);
+    for (int k = 1; k <= N3M; k++)
+      for (int i = 1; i <= N1M; i++) {
+        const double delta2 = pow(dxyz, 2. / 3.);
+        const double sumsr = 2. * (DYX(SR, i, k, 2) * DYX(SR, i, k, 2)) + DYX(SR, i, k, 1) * DYX(SR, i, k, 1) +
+                             2. * (DYX(SR, i, k, 3) * DYX(SR, i, k, 3)) + DYX(SR, i, k, 4) * DYX(SR, i, k, 4) +
+                             2. * (DYX(SR, i, k, 5) * DYX(SR, i, k, 5)) + DYX(SR, i, k, 6) * DYX(SR, i, k, 6);
+        const double abssr = sqrt(2. * sumsr);
+        SGSVIS[IX(i, j, k)] = abssr;
+        const double delabs = delta2 * abssr;
+        for (int m = 1; m <= 6; m++) DYX(COM, i, k, m) = delabs * DYX(SR, i, k, m);
+      }
+    c3o_filter(o, COM, 6);
+    c3o_filter(o, SR, 6);
+    for (int k = 1; k <= N3M; k++)
+      for (int i = 1; i <= N1M; i++) {
+        const double sumfsr = 2. * (DYX(SR, i, k, 2) * DYX(SR, i, k, 2)) + DYX(SR, i, k, 1) * DYX(SR, i, k, 1) +
+                              2. * (DYX(SR, i, k, 3) * DYX(SR, i, k, 3)) + DYX(SR, i, k, 4) * DYX(SR, i, k, 4) +
+                              2. * (DYX(SR, i, k, 5) * DYX(SR, i, k, 5)) + DYX(SR, i, k, 6) * DYX(SR, i, k, 6);
+        const double absfsr = sqrt(2. * sumfsr);
+        const double fdxyz = 4.0 * pow(dxyz, 2. / 3.) * beta;
+        const double fdabs = fdxyz * absfsr;
+        for (int m = 1; m <= 6; m++) DYX(SR, i, k, m) = fdabs * DYX(SR, i, k, m) - DYX(COM, i, k, m);
+      }
+    const int jp = j + 1;
+    for (int k = 1; k <= N3M; k++) {
+      const int kp = o->kpa[k];
+      for (int i = 1; i <= N1M; i++) {
+        const int ip = o->ipa[i];
+        const double u1 = (U[IX(i, j, k)] + U[IX(ip, j, k)]) * 0.5;
+        const double u2 = (V[IX(i, j, k)] + V[IX(i, jp, k)]) * 0.5;
+        const double u3 = (W[IX(i, j, k)] + W[IX(i, j, kp)]) * 0.5;
+        DYX(UC, i, k, 1) = u1;
+        DYX(UC, i, k, 2) = u2;
+        DYX(UC, i, k, 3) = u3;
+        DYX(COM, i, k, 1) = u1 * u1;
+        DYX(COM, i, k, 2) = u1 * u2;
+        DYX(COM, i, k, 3) = u1 * u3;
+        DYX(COM, i, k, 4) = u2 * u2;
+        DYX(COM, i, k, 5) = u2 * u3;
+        DYX(COM, i, k, 6) = u3 * u3;
+      }
+    }
+    c3o_filter(o, UC, 3);
+    c3o_filter(o, COM, 6);
+    double sumnum = 0., sumden = 0.;
+    for (int k = 1; k <= N3M; k++)
+      for (int i = 1; i <= N1M; i++) {
+        const double c1 = DYX(UC, i, k, 1), c2 = DYX(UC, i, k, 2), c3 = DYX(UC, i, k, 3);
+        const double xl11 = DYX(COM, i, k, 1) - c1 * c1, xl12 = DYX(COM, i, k, 2) - c1 * c2;
+        const double xl13 = DYX(COM, i, k, 3) - c1 * c3, xl22 = DYX(COM, i, k, 4) - c2 * c2;
+        const double xl23 = DYX(COM, i, k, 5) - c2 * c3, xl33 = DYX(COM, i, k, 6) - c3 * c3;
+        const double m1 = DYX(SR, i, k, 1), m2 = DYX(SR, i, k, 2), m3 = DYX(SR, i, k, 3);
+        const double m4 = DYX(SR, i, k, 4), m5 = DYX(SR, i, k, 5), m6 = DYX(SR, i, k, 6);
+        const double sqmm = 2. * (m2 * m2) + m1 * m1 + 2. * (m3 * m3) + m4 * m4 + 2. * (m5 * m5) + m6 * m6;
+        const double sqlm = 2. * xl12 * m2 + xl11 * m1 + 2. * xl13 * m3 + xl22 * m4 + 2. * xl23 * m5 + xl33 * m6;
+        sumden = sumden + sqmm;
+        sumnum = sumnum + sqlm;
+      }
+    sumnum = sumnum * nxz;
+    sumden = sumden * nxz;
+    const double smagc = -0.5 * sumnum / sumden;
+    if (smagc_out) smagc_out[j] = smagc;
+    for (int k = 1; k <= N3M; k++)
+      for (int i = 1; i <= N1M; i++) {
+        const double turbv = smagc * pow(dxyz, 2. / 3.) * SGSVIS[IX(i, j, k)];
+        SGSVIS[IX(i, j, k)] = fmax(turbv, -reninv) * 1.0;
+      }
+    free(SR);
+    free(COM);
+    free(COM1);
+    free(SR1);
+    free(UC);
+#undef DYX
+  }
+}
+
 /* ------------------------------------------------------ WALLMODEL_INSTANT */
 /* compact3d.f:2319-2493 */
 void c3o_wallmodel_instant(c3o *o, const double *U) {
@@ -2164,6 +2510,7 @@ void c3o_getup(c3o *o, double *U, double *P, double *HH, double *SGSVIS,
   memset(HH, 0, sizeof(double) * 3 * nu);
   if (o->files == 1.0) c3o_sgsdy(o, U, SGSVIS, NULL);
   if (o->files == 2.0) c3o_sgscs(o, U, SGSVIS);
+  if (o->files == 3.0) c3o_sgssd(o, U, SGSVIS, NULL, NULL);
   if (o->files != 0.0) {
     if (o->wmodel == 0.0) {
       o->ifwall = 0;

@@ -66,6 +66,11 @@ void c3o_sgscs(c3o *o, const double *U, double *SGSVIS);
 /* SGSDY (compact3d.f:1250-1400); smagc_out (may be NULL) receives the
  * dynamic coefficient C(J) at [J], J = 1..N2M */
 void c3o_sgsdy(c3o *o, const double *U, double *SGSVIS, double *smagc_out);
+/* SGSSD compact3d.f:1502-1948 (scale-dependent dynamic model, FILES=3) */
+void c3o_sgssd(c3o *o, const double *U, double *SGSVIS, double *smagc_out, double *beta_out);
+/* COEFFICIENT :2079-2097 and DQRRT/CHHQR :11548-11677 */
+void c3o_sd_coefficient(const double s1[5], const double s2[5], double pa[5]);
+double c3o_dqrrt(const double *pa, int n);
 void c3o_wallmodel_instant(c3o *o, const double *U);
 void c3o_filter(c3o *o, double *FILV, int n);
 void c3o_filter4(c3o *o, double *FILV, int n);

@@ -71,6 +71,8 @@ class Oracle:
             L.c3o_upcalc.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp]
             L.c3o_getup.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int]
             L.c3o_dft2d.argtypes = [C.c_int, C.c_int, _dp, C.c_int]
+            L.c3o_dqrrt.restype = C.c_double
+            L.c3o_dqrrt.argtypes = [_dp, C.c_int]
             cls._lib = L
         return cls._lib
 
@@ -220,6 +222,25 @@ class Oracle:
             return 0.5 * (g(i + 1) + g(i))
         Y = np.asarray(Y, dtype=np.float64)
         return centres(n1, alx), 0.5 * (Y[2:] + Y[1:-1]), centres(n3, alz)
+
+    def sgssd(self, U, SGSVIS):
+        """SGSSD; returns (C(J), BETA(J)), J = 0..N2 (1..N2M set)"""
+        c = np.zeros(self.n2 + 1)
+        b = np.zeros(self.n2 + 1)
+        self.L.c3o_sgssd(self.h, _p(U), _p(SGSVIS), _p(c), _p(b))
+        return c, b
+
+    @classmethod
+    def sd_coefficient(cls, s1, s2):
+        pa = np.zeros(5)
+        cls.lib().c3o_sd_coefficient(_p(np.ascontiguousarray(s1, dtype=np.float64)),
+                                     _p(np.ascontiguousarray(s2, dtype=np.float64)), _p(pa))
+        return pa
+
+    @classmethod
+    def dqrrt(cls, pa):
+        pa = np.ascontiguousarray(pa, dtype=np.float64)
+        return cls.lib().c3o_dqrrt(_p(pa), len(pa))
 
     def sgsdy(self, U, SGSVIS):
         """SGSDY; returns the dynamic coefficient C(J), J = 0..N2 (1..N2M set)"""
@@ -437,6 +458,27 @@ class RefLib:
             flow = open(os.path.join(td, "RESULT", "FLOW-" + name)).read()
             strain = open(os.path.join(td, "RESULT", "STRAIN-" + name)).read()
         return flow, strain
+
+    def sgssd(self, U, SGSVIS, ntime=1):
+        """the reference's SGSSD; returns SMAGC(1:M2) as C(J) at [J], J = 0..N2"""
+        self.L.sgssd_(_p(U), _p(SGSVIS), self._i(ntime))
+        c = np.zeros(self.n2)
+        self.L.refh_getsmagc_(_p(c))
+        return np.concatenate([[0.0], c])
+
+    def coefficient(self, s1, s2):
+        """the reference's COEFFICIENT: PA(1:5) from A1..E1, A2..E2"""
+        d = [C.byref(C.c_double(float(v))) for v in list(s1) + list(s2)]
+        pa = np.zeros(5)
+        self.L.coefficient_(*d, _p(pa))
+        return pa
+
+    def dqrrt(self, pa, eps=1e-5):
+        """the reference's DQRRT on PA(1:N): BETA"""
+        pa = np.ascontiguousarray(pa, dtype=np.float64)
+        beta = C.c_double(0.0)
+        self.L.dqrrt_(_p(pa), self._i(len(pa)), C.byref(beta), C.byref(C.c_double(eps)))
+        return beta.value
 
     def wallmodel_instant(self, U, SGSVIS):
         tw, tb = self.get_tau()

@@ -36,6 +36,7 @@ CASES = {
     "dns": dict(re=4200.0, alx=2 * math.pi, aly=2.0, alz=math.pi, dt=1e-3, files=0.0, wmodel=0.0),
     "les": dict(re=5200.0, alx=2 * math.pi, aly=2.0, alz=math.pi, dt=5e-4, files=2.0, wmodel=1.0),
     "dyn": dict(re=5200.0, alx=2 * math.pi, aly=2.0, alz=math.pi, dt=5e-4, files=1.0, wmodel=1.0),
+    "sd": dict(re=5200.0, alx=2 * math.pi, aly=2.0, alz=math.pi, dt=5e-4, files=3.0, wmodel=1.0),
 }
 
 
@@ -64,6 +65,17 @@ def run_case(name, cfg):
     if cfg["files"] == 1.0:
         out["SMAGC"] = ref.sgsdy(U, S)
         out["SGSVIS"] = S
+    if cfg["files"] == 3.0:
+        out["SMAGC"] = ref.sgssd(U, S)
+        out["SGSVIS"] = S
+        # the quintic's root selection (DQRRT) on seeded coefficients
+        rq = np.random.default_rng(31)
+        pa = rq.standard_normal((16, 5)) * np.array([1.0, 3.0, 5.0, 3.0, 1.0])
+        out["DQRRT_PA"] = pa
+        out["DQRRT_BETA"] = np.array([ref.dqrrt(q) for q in pa])
+        s1, s2 = rq.standard_normal(5), rq.standard_normal(5)
+        out["COEF_IN"] = np.concatenate([s1, s2])
+        out["COEF_PA"] = ref.coefficient(s1, s2)
     if cfg["files"] != 0.0 and cfg["wmodel"] == 1.0:
         ref.wallmodel_instant(U, S)
         ref.set_ifwall(1)
@@ -118,7 +130,7 @@ def run_case(name, cfg):
     store = {"config": np.array([n1, n2, n3, cfg["re"], cfg["alx"], cfg["aly"], cfg["alz"], cfg["dt"],
                                  cfg["files"], cfg["wmodel"], 7, presg, presg3, STRIDE])}
     small = {"PRESG", "DIVMAX", "CFL", "WSM", "SMAGC", "CTDMA_RHS", "TDMAP_IN", "TDMAP", "FILTER_IN", "FILTER",
-             "FILTER4"} | {k for k in out if k.startswith("CTDMA11")}
+             "FILTER4", "DQRRT_PA", "DQRRT_BETA", "COEF_IN", "COEF_PA"} | {k for k in out if k.startswith("CTDMA11")}
     for k, v in out.items():
         if k in small:
             store[k] = np.asarray(v, dtype=np.float64)

@@ -3,8 +3,9 @@ subroutines (tests/golden/make_golden.py; compact3d.f compiled unmodified).
 
 Every phase the reference can run without FFTW is compared: BCOND, RHS1-3,
 GETUH1-3, RHSDP, UPCALC, DIVCHECK, CFL, SGSCS/STRAIN, SGSDY (the dynamic
-model, its per-plane coefficient included), WALLMODEL_INSTANT, CTDMA11,
-TDMAP, FILTER, FILTER4.  The oracle follows the reference's operation
+model, its per-plane coefficient included), SGSSD (the scale-dependent
+model) with COEFFICIENT and the DQRRT/CHHQR root selection,
+WALLMODEL_INSTANT, CTDMA11, TDMAP, FILTER, FILTER4.  The oracle follows the reference's operation
 order, so the expected agreement is exact; the bound allows 4 ulps."""
 import os
 
@@ -36,7 +37,7 @@ def _check(fx, key, arr, stride):
     assert es <= 1e-12, (key, es)
 
 
-@pytest.mark.parametrize("name", ["dns", "les", "dyn"])
+@pytest.mark.parametrize("name", ["dns", "les", "dyn", "sd"])
 def test_oracle_matches_reference_fixture(name):
     from pyoracle import Oracle
     fx = _load(name)
@@ -55,6 +56,13 @@ def test_oracle_matches_reference_fixture(name):
         smagc = o.sgsdy(U, S)
         _check(fx, "SGSVIS", S, stride)
         assert np.max(np.abs(smagc - fx["SMAGC"])) <= TOL * np.max(np.abs(fx["SMAGC"]))
+    if files == 3.0:
+        smagc, _ = o.sgssd(U, S)
+        _check(fx, "SGSVIS", S, stride)
+        assert np.max(np.abs(smagc - fx["SMAGC"])) <= TOL * np.max(np.abs(fx["SMAGC"]))
+        for pa, beta in zip(fx["DQRRT_PA"], fx["DQRRT_BETA"]):
+            assert Oracle.dqrrt(pa) == beta
+        assert np.array_equal(Oracle.sd_coefficient(fx["COEF_IN"][:5], fx["COEF_IN"][5:]), fx["COEF_PA"])
     if files != 0.0 and wmodel == 1.0:
         o.wallmodel_instant(U)
         o.set_ifwall(1)

@@ -233,3 +233,20 @@ def test_plane_statistics_closed_forms():
               18: u * u * b0, 19: np.full(n2 - 1, b0 * ww)}
     for q, e in expect.items():
         assert np.max(np.abs(s[q] - e)) <= 1e-12 * max(1.0, np.max(np.abs(e))), (q, s[q][:3], e[:3])
+
+
+def test_dqrrt_selects_the_largest_real_part_in_0_2():
+    """DQRRT/CHHQR (compact3d.f:11548-11677): the roots of a quintic with
+    known roots; BETA is the largest real part in (0, 2), complex pairs
+    included, else 0.  CHHQR deflates at EPS = 1e-5, which leaves errors of
+    a few 1e-6 in the roots (the reference's own DQRRT returns the same
+    values bit for bit, tests/test_oracle_golden.py), so roots are checked
+    to 2e-5."""
+    from pyoracle import Oracle
+    cases = [([0.3, 1.7, -0.5, 2.5, -4.0], 1.7),
+             ([0.9 + 0.4j, 0.9 - 0.4j, -1.0, 3.0, 5.0], 0.9),
+             ([-0.2, -1.0, 2.5, 3.0 + 1j, 3.0 - 1j], 0.0),
+             ([0.125, 0.25, 0.5, 1.0, 1.5], 1.5)]
+    for roots, beta in cases:
+        pa = np.real(np.poly(roots))[1:]
+        assert abs(Oracle.dqrrt(pa) - beta) <= 2e-5, (roots, Oracle.dqrrt(pa))
