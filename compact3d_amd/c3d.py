@@ -72,6 +72,7 @@ def lib():
         L.c3d_flowrate.argtypes = [C.c_void_p, _dp, _dp]
         L.c3d_stats.argtypes = [C.c_void_p, _dp]
         L.c3d_cellcentre.argtypes = [C.c_void_p, _dp]
+        L.c3d_sgs_profiles.argtypes = [C.c_void_p, _dp, _dp]
         L.c3d_write_flow.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
         L.c3d_write_strain.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_double]
         L.c3d_format_flow.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
@@ -188,6 +189,7 @@ class Channel:
             raise C3DError("c3d_create: " + self.L.c3d_last_error(None).decode())
         self.h = h
         self.n1, self.n2, self.n3 = n1, n2, n3
+        self.files = float(files)
 
     def close(self):
         if getattr(self, "h", None):
@@ -358,6 +360,14 @@ class Channel:
         out = np.zeros(20 * (self.n2 - 1))
         self._ck(self.L.c3d_stats(self.h, _p(out)), "stats")
         return out.reshape(20, self.n2 - 1)
+
+    def sgs_profiles(self):
+        """(SMAGC(J), BETA(J)) of the last dynamic-model evaluation, J = 1..N2-1
+        (BETA only for FILES = 3, else None)"""
+        c = np.zeros(self.n2 - 1)
+        b = np.zeros(self.n2 - 1) if self.files == 3.0 else None
+        self._ck(self.L.c3d_sgs_profiles(self.h, _p(c), _p(b)), "sgs_profiles")
+        return c, b
 
     def cellcentre(self):
         """WRITEUP's cell-centre values of the resident field: array

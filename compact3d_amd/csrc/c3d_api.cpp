@@ -418,11 +418,13 @@ static int upload_dt_terms(c3d_handle* h) {
   std::vector<double> ych, tab;
   build_dt_terms(h, ych);
   double* base = (double*)h->c.m.ya[0];
-  HIPCHK(hipMemcpy(base, ych.data(), ych.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpyAsync(base, ych.data(), ych.size() * sizeof(double), hipMemcpyHostToDevice, h->c.st));
   build_ytab(h, ych, tab);
-  HIPCHK(hipMemcpy(h->dytab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpyAsync(h->dytab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, h->c.st));
   std::vector<float> tab32(tab.begin(), tab.end());
-  HIPCHK(hipMemcpy(h->dytab32, tab32.data(), tab32.size() * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpyAsync(h->dytab32, tab32.data(), tab32.size() * sizeof(float), hipMemcpyHostToDevice, h->c.st));
+  // the host vectors go out of scope: wait for the copies
+  HIPCHK(hipStreamSynchronize(h->c.st));
   return 0;
 }
 
@@ -430,7 +432,9 @@ template <typename T>
 static int dalloc(c3d_handle* h, T** p, size_t n) {
   void* q = nullptr;
   HIPCHK(hipMalloc(&q, n * sizeof(T) + 64));
-  HIPCHK(hipMemset(q, 0, n * sizeof(T) + 64));
+  // on the handle's stream: it is non-blocking, so the null stream would not
+  // order the fill before the kernels and copies that follow on it
+  HIPCHK(hipMemsetAsync(q, 0, n * sizeof(T) + 64, h->c.st));
   h->allocs.push_back(q);
   *p = (T*)q;
   return 0;
@@ -447,6 +451,7 @@ long long c3d_footprint_bytes(const c3d_config* cfg) {
   long long bytes = (10 + (mixed ? 1 : 0)) * fsz * 8 + 16 * fsz * es + (mixed ? 4 * fsz * 4 : 0);
   if (cfg->files != 0.0) bytes += 7 * fsz * 8;
   if (cfg->files == 1.0) bytes += (2 * n2m * ((nx + 15) / 16) * ((nz + 15) / 16) + n2m + 3) * 8;
+  if (cfg->files == 3.0) bytes += (10 * n2m * ((nx + 15) / 16) * ((nz + 15) / 16) + 2 * (n2m + 3)) * 8;
   const long long nm = ((nx / 2 + 1 + 7) / 8 * 8) * nz * n2m;
   bytes += nm * 16 + nm * 16;  // spectrum + TDMAP factors
   bytes += 4 * std::max(n2m * nz, nx * n2m) * 8 + 128 * 8;  // flow-rate partial sums, max slots
@@ -514,8 +519,8 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   const c3d_config& cf = h->cfg;
   if (cf.n1 < 9 || cf.n2 < 5 || cf.n3 < 9) return bad("grid too small (need N1,N3 >= 9, N2 >= 5)");
   if (cf.precision != C3D_FP64 && cf.precision != C3D_MIXED) return bad("precision must be C3D_FP64 or C3D_MIXED");
-  if (cf.files != 0.0 && cf.files != 1.0 && cf.files != 2.0)
-    return bad("FILES must be 0 (DNS), 1 (dynamic model) or 2 (Smagorinsky)");
+  if (cf.files != 0.0 && cf.files != 1.0 && cf.files != 2.0 && cf.files != 3.0)
+    return bad("FILES must be 0 (DNS), 1 (dynamic model), 2 (Smagorinsky) or 3 (scale-dependent dynamic model)");
   if (cf.wmodel != 0.0 && cf.wmodel != 1.0) return bad("WMODEL must be 0 or 1");
   if (cf.re <= 0 || cf.dt <= 0 || cf.alx <= 0 || cf.alz <= 0) return bad("RE, DT, ALX, ALZ must be > 0");
   if (hipSetDevice(cf.device) != hipSuccess) return bad("hipSetDevice failed");
@@ -594,7 +599,8 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
       for (int J = 0; J <= g.n2; J++) yt[2 * nv + J] = h->DY[J] / 2.;
       for (int J = 1; J <= g.n2; J++) yt[4 * nv + J] = 1. / h->H[J];
     }
-    if (hipMemcpy(h->dvec, hv.data(), nvec * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    if (hipMemcpyAsync(h->dvec, hv.data(), nvec * sizeof(double), hipMemcpyHostToDevice, c.st) != hipSuccess ||
+        hipStreamSynchronize(c.st) != hipSuccess)
       return bad("vector upload failed");
     const double* d = h->dvec;
     Metrics& m = c.m;
@@ -646,10 +652,11 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
     double2 *p1, *p2, *p3, *p4;
     if (dalloc(h, &p1, g.nx) || dalloc(h, &p2, g.nx) || dalloc(h, &p3, g.nz) || dalloc(h, &p4, g.nz))
       return bad(h->err);
-    if (hipMemcpy(p1, a.data(), a.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(p2, b.data(), b.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(p3, cz.data(), cz.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(p4, dz.data(), dz.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+    if (hipMemcpyAsync(p1, a.data(), a.size() * 8, hipMemcpyHostToDevice, c.st) != hipSuccess ||
+        hipMemcpyAsync(p2, b.data(), b.size() * 8, hipMemcpyHostToDevice, c.st) != hipSuccess ||
+        hipMemcpyAsync(p3, cz.data(), cz.size() * 8, hipMemcpyHostToDevice, c.st) != hipSuccess ||
+        hipMemcpyAsync(p4, dz.data(), dz.size() * 8, hipMemcpyHostToDevice, c.st) != hipSuccess ||
+        hipStreamSynchronize(c.st) != hipSuccess)
       return bad("twiddle upload failed");
     f.twxf = p1;
     f.twxb = p2;
@@ -707,10 +714,12 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
     for (auto pp : les)
       if (dalloc(h, pp, fs)) return bad(h->err);
   }
-  F.dyp = F.smagc = nullptr;
-  if (cf.files == 1.0) {
-    if (dalloc(h, &F.dyp, 2 * (size_t)g.n2m * c3dk::sgsdy_tiles(g)) || dalloc(h, &F.smagc, (size_t)g.n2 + 2))
+  F.dyp = F.smagc = F.sdbeta = nullptr;
+  if (cf.files == 1.0 || cf.files == 3.0) {
+    const size_t per = cf.files == 3.0 ? 10 : 2;
+    if (dalloc(h, &F.dyp, per * (size_t)g.n2m * c3dk::sgsdy_tiles(g)) || dalloc(h, &F.smagc, (size_t)g.n2 + 2))
       return bad(h->err);
+    if (cf.files == 3.0 && dalloc(h, &F.sdbeta, (size_t)g.n2 + 2)) return bad(h->err);
   }
   c3dk::init_attrs();
   {
@@ -732,6 +741,7 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   F.flowr = std::max(g.n2m * g.nz, g.nx * g.n2m);
   if (dalloc(h, &F.flowp, 4 * (size_t)F.flowr)) return bad(h->err);
   if (dalloc(h, &F.mxs, 128)) return bad(h->err);  // zeroed
+  if (hipStreamSynchronize(c.st) != hipSuccess) return bad("set-up failed");
   *out = h;
   return 0;
 }
@@ -835,7 +845,8 @@ extern "C" int c3d_upload_state(c3d_handle* h, const double* U, const double* P,
   }
   if (to_dev(h, F.sgs, SGSVIS, LAY_U)) return -1;
   double sc[2] = {presg, presg3};
-  HIPCHK(hipMemcpy(F.sc + SC_PRESG, sc, 2 * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpyAsync(F.sc + SC_PRESG, sc, 2 * sizeof(double), hipMemcpyHostToDevice, h->c.st));
+  HIPCHK(hipStreamSynchronize(h->c.st));
   if (h->c.s.mixed) {
     // FP32 shadows of the state, and DIV(U) of the uploaded field (the
     // next RDP starts from it)
@@ -870,7 +881,8 @@ extern "C" int c3d_download_state(c3d_handle* h, double* U, double* P, double* H
     }
   if (SGSVIS && from_dev(h, SGSVIS, F.sgs, LAY_U)) return -1;
   double sc[2];
-  HIPCHK(hipMemcpy(sc, F.sc + SC_PRESG, 2 * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpyAsync(sc, F.sc + SC_PRESG, 2 * sizeof(double), hipMemcpyDeviceToHost, h->c.st));
+  HIPCHK(hipStreamSynchronize(h->c.st));
   if (presg) *presg = sc[0];
   if (presg3) *presg3 = sc[1];
   return 0;
@@ -990,6 +1002,8 @@ extern "C" int c3d_sgs(c3d_handle* h) {
   if (!h->c.s.files) return fail(h, "the SGS model needs FILES != 0");
   if (h->cfg.files == 1.0)
     c3dk::sgsdy(h->c);
+  else if (h->cfg.files == 3.0)
+    c3dk::sgssd(h->c);
   else
     c3dk::sgscs(h->c);
   return launched(h);
@@ -1025,6 +1039,7 @@ static int step_impl(c3d_handle* h, bool chk) {
   // GETUP (:925-929): SGS model, then the wall model
   if (h->cfg.files == 1.0) c3dk::sgsdy(h->c);
   if (h->cfg.files == 2.0) c3dk::sgscs(h->c);
+  if (h->cfg.files == 3.0) c3dk::sgssd(h->c);
   if (h->cfg.files != 0.0) wall_model_impl(h);
   c3dk::bcond(h->c);
   c3dk::rhs_xz(h->c);
@@ -1057,6 +1072,7 @@ static void step_launch(c3d_handle* h) {
   swap_hh(h);
   if (h->cfg.files == 1.0) c3dk::sgsdy(h->c);
   if (h->cfg.files == 2.0) c3dk::sgscs(h->c);
+  if (h->cfg.files == 3.0) c3dk::sgssd(h->c);
   if (h->cfg.files != 0.0) wall_model_impl(h);
   c3dk::bcond(h->c);
   c3dk::rhs_xz(h->c);
@@ -1691,6 +1707,18 @@ extern "C" int c3d_write_strain(c3d_handle* h, const char* path, int ntime, doub
   return 0;
 }
 
+extern "C" int c3d_sgs_profiles(c3d_handle* h, double* smagc, double* beta) {
+  if (!h) return -1;
+  const Fields& F = h->c.d;
+  const int n2m = h->c.g.n2m;
+  if ((smagc && !F.smagc) || (beta && !F.sdbeta))
+    return fail(h, "c3d_sgs_profiles: C(J) exists for FILES = 1 or 3, BETA(J) for FILES = 3");
+  if (smagc) HIPCHK(hipMemcpyAsync(smagc, F.smagc + 1, n2m * sizeof(double), hipMemcpyDeviceToHost, h->c.st));
+  if (beta) HIPCHK(hipMemcpyAsync(beta, F.sdbeta + 1, n2m * sizeof(double), hipMemcpyDeviceToHost, h->c.st));
+  HIPCHK(hipStreamSynchronize(h->c.st));
+  return 0;
+}
+
 extern "C" int c3d_flowrate(c3d_handle* h, double* flow1, double* flow3) {
   if (!h) return -1;
   c3dk::chkmf(h->c, false);
@@ -1721,7 +1749,9 @@ extern "C" int c3d_get_field(c3d_handle* h, int which, double* out) {
     case C3D_F_TAUB: {
       const Geo& g = h->c.g;
       std::vector<double> t((size_t)g.nx * g.nz);
-      HIPCHK(hipMemcpy(t.data(), which == C3D_F_TAUW ? F.tauw : F.taub, t.size() * 8, hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpyAsync(t.data(), which == C3D_F_TAUW ? F.tauw : F.taub, t.size() * 8, hipMemcpyDeviceToHost,
+                            h->c.st));
+      HIPCHK(hipStreamSynchronize(h->c.st));
       const int n1 = h->cfg.n1, n3 = h->cfg.n3;
       for (size_t q = 0; q < (size_t)(n1 + 1) * (n3 + 1) * 2; q++) out[q] = 0.0;
       for (int k = 0; k < g.nz; k++)

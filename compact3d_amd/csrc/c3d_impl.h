@@ -115,9 +115,10 @@ struct Fields {
   //   FP_DW: sum DELTA_w*DY (ADI x-sweep of w)   FP_W: sum U_w*DY
   double *flowp;
   int flowr;
-  // dynamic SGS model (FILES = 1): per-plane tile partial sums of L:M and
-  // M:M, and the coefficient C(J) at [J]
-  double *dyp, *smagc;
+  // dynamic SGS models (FILES = 1, 3): per-plane tile partial sums (L:M and
+  // M:M; FILES = 3 also the five sums of each of its two filter levels), the
+  // coefficient C(J) and, for FILES = 3, BETA(J) at [J]
+  double *dyp, *smagc, *sdbeta;
   // the 2 x 64 max slots (DIVMAX, CFL bit patterns) of the update's checks;
   // zero between steps (cleared at set-up and by the kernel that reads them)
   double *mxs;
@@ -197,6 +198,7 @@ void plane_stats(const Ctx& c, double* out);  // ENERGY's plane statistics, 20 x
 void cellcentre(const Ctx& c, int q, double* out);  // WRITEUP's cell-centre quantity q, N1M x N2M x N3M
 void sgscs(const Ctx& c);
 void sgsdy(const Ctx& c);            // SGSDY, the dynamic model (FILES = 1)
+void sgssd(const Ctx& c);            // SGSSD, the scale-dependent dynamic model (FILES = 3)
 int sgsdy_tiles(const Geo& g);      // tiles per plane of its partial sums
 void wallmodel(const Ctx& c);
 void wallss(const Ctx& c);

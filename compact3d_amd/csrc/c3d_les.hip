@@ -132,11 +132,12 @@ __global__ void k_sgscs(Ctx c) {
 // and L_ij = COM2~_ij - UC~_i UC~_j, and the block's sums of L:M and M:M go
 // to a per-plane partial array.  k_sgsdy_coef: C(J) = -0.5 <L:M>/<M:M> from
 // the partials in a fixed order.  k_sgsdy_clip: SGSVIS = max(C(J)
-// DELTA^2 |S|, -1/RE).
+// DELTA^2 |S|, -1/RE).  SGSSD's last stage is the same with the factor
+// 4 BETA(J) in place of 4 (beta != NULL).
 #define DT_TILE 16
 #define DT_HALO (DT_TILE + 2)
 #define DT_NQ 21
-__global__ __launch_bounds__(DT_TILE* DT_TILE) void k_sgsdy_tile(Ctx c) {
+__global__ __launch_bounds__(DT_TILE* DT_TILE) void k_sgsdy_tile(Ctx c, const double* beta) {
   __shared__ double Q[DT_NQ][DT_HALO * DT_HALO];
   const Geo g = c.g;
   const Scal s = c.s;
@@ -203,7 +204,8 @@ __global__ __launch_bounds__(DT_TILE* DT_TILE) void k_sgsdy_tile(Ctx c) {
       fs[m] = filt(6 + m);
     }
     const double absfsr = sqrt(2. * sdots(fs));
-    const double fdxyz = 4.0 * pow(dxyz, 2. / 3.);
+    double fdxyz = 4.0 * pow(dxyz, 2. / 3.);
+    if (beta) fdxyz = fdxyz * beta[j];  // SGSSD's scale-dependent factor (:1823)
     const double fdabs = fdxyz * absfsr;
     double mm[6];
 #pragma unroll
@@ -258,6 +260,292 @@ __global__ void k_sgsdy_clip(Ctx c) {
   const long long a = gidx(g, i, j, k);
   const double turbv = c.d.smagc[j] * pow(dxyz, 2. / 3.) * c.d.sgs[a];
   c.d.sgs[a] = fmax(turbv, -c.s.rinv);
+}
+
+// ------------------------------------------------------------ SGSSD
+// SGSSD (compact3d.f:1502-1948), the scale-dependent dynamic model, per J
+// plane: the sums A..E of the Germano terms at the test-filter levels
+// FILTER (2 DELTA) and FILTER4 (4 DELTA) (:1544-1800), the quintic of
+// COEFFICIENT (:2089-2093) and its root BETA (DQRRT/CHHQR), then SGSDY's
+// stage with the factor 4 BETA (:1804-1930).
+//
+// k_sgssd_tile<H>: a block takes a 16 x 16 tile of (I,K) with an H-point
+// periodic halo (H = 1 for FILTER, 2 for FILTER4) in two phases that share
+// one LDS buffer: STRAIN -> COM1 = DELTA^2 |S| S_ij and S_ij at every point,
+// filtered at the interior points to COM1~ and SR1 = DELTA^2 |S~| S~_ij;
+// then the cell-centre velocities UC and their products, filtered, give
+// L_ij.  The block's five sums go to a per-plane partial array.
+template <int H>
+__global__ __launch_bounds__(DT_TILE* DT_TILE) void k_sgssd_tile(Ctx c, double* part) {
+  constexpr int HD = DT_TILE + 2 * H;
+  __shared__ double Q[12][HD * HD];
+  const Geo g = c.g;
+  const Scal s = c.s;
+  const Fields F = c.d;
+  const int tix = (g.nx + DT_TILE - 1) / DT_TILE, tiz = (g.nz + DT_TILE - 1) / DT_TILE;
+  const int ntile = tix * tiz;
+  const int j = 1 + blockIdx.x / ntile;
+  const int tile = blockIdx.x % ntile;
+  const int i0 = (tile % tix) * DT_TILE, k0 = (tile / tix) * DT_TILE;
+  const double dxyz = 1.0 / s.dx1 * c.m.dy[j] / s.dx3;
+  const double delta2 = pow(dxyz, 2. / 3.);
+  auto wrap_ik = [&](int e, int& i, int& k) {
+    i = i0 - H + e % HD;
+    k = k0 - H + e / HD;
+    i = ((i % g.nx) + g.nx) % g.nx;
+    k = ((k % g.nz) + g.nz) % g.nz;
+  };
+  for (int e = threadIdx.x; e < HD * HD; e += blockDim.x) {
+    int i, k;
+    wrap_ik(e, i, k);
+    double sr[6];
+    strain_at(c, i, j, k, sr);
+    const double delabs = delta2 * sqrt(2. * sdots(sr));
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+      Q[m][e] = delabs * sr[m];
+      Q[6 + m][e] = sr[m];
+    }
+  }
+  __syncthreads();
+  const int tx = threadIdx.x % DT_TILE, tz = threadIdx.x / DT_TILE;
+  const bool ok = i0 + tx < g.nx && k0 + tz < g.nz;
+  // FILTER (:5190-5228) / FILTER4 (:5481-5546): x pass on the 2H+1 rows
+  // around the point, then the z pass
+  auto filt = [&](int q) {
+    const double* A = Q[q];
+    if constexpr (H == 1) {
+      double fx[3];
+#pragma unroll
+      for (int r = 0; r < 3; r++) {
+        const int row = (tz + r) * HD + tx + 1;
+        fx[r] = (A[row - 1] + 4.0 * A[row] + A[row + 1]) / 6.0;
+      }
+      return (fx[0] + 4.0 * fx[1] + fx[2]) / 6.0;
+    } else {
+      double fx[5];
+#pragma unroll
+      for (int r = 0; r < 5; r++) {
+        const int row = (tz + r) * HD + tx + 2;
+        fx[r] = (A[row - 2] + 4.0 * A[row - 1] + 2.0 * A[row] + 4.0 * A[row + 1] + A[row + 2]) / 12.0;
+      }
+      return (fx[0] + 4.0 * fx[1] + 2.0 * fx[2] + 4.0 * fx[3] + fx[4]) / 12.0;
+    }
+  };
+  double fc[6], sr1[6];
+  if (ok) {
+    double fs[6];
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+      fc[m] = filt(m);
+      fs[m] = filt(6 + m);
+    }
+    const double fdabs = pow(dxyz, 2. / 3.) * sqrt(2. * sdots(fs));
+#pragma unroll
+    for (int m = 0; m < 6; m++) sr1[m] = fdabs * fs[m];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < HD * HD; e += blockDim.x) {
+    int i, k;
+    wrap_ik(e, i, k);
+    const int ip = wrapp(i, g.nx), kp = wrapp(k, g.nz);
+    const double u1 = (F.u[gidx(g, i, j, k)] + F.u[gidx(g, ip, j, k)]) * 0.5;
+    const double u2 = (F.v[gidx(g, i, j, k)] + F.v[gidx(g, i, j + 1, k)]) * 0.5;
+    const double u3 = (F.w[gidx(g, i, j, k)] + F.w[gidx(g, i, j, kp)]) * 0.5;
+    Q[0][e] = u1;
+    Q[1][e] = u2;
+    Q[2][e] = u3;
+    Q[3][e] = u1 * u1;
+    Q[4][e] = u1 * u2;
+    Q[5][e] = u1 * u3;
+    Q[6][e] = u2 * u2;
+    Q[7][e] = u2 * u3;
+    Q[8][e] = u3 * u3;
+  }
+  __syncthreads();
+  double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (ok) {
+    const double c1 = filt(0), c2 = filt(1), c3 = filt(2);
+    const double xl11 = filt(3) - c1 * c1, xl12 = filt(4) - c1 * c2, xl13 = filt(5) - c1 * c3;
+    const double xl22 = filt(6) - c2 * c2, xl23 = filt(7) - c2 * c3, xl33 = filt(8) - c3 * c3;
+    v[0] = 2. * xl12 * sr1[1] + xl11 * sr1[0] + 2. * xl13 * sr1[2] + xl22 * sr1[3] + 2. * xl23 * sr1[4] +
+           xl33 * sr1[5];
+    v[1] = 2. * xl12 * fc[1] + xl11 * fc[0] + 2. * xl13 * fc[2] + xl22 * fc[3] + 2. * xl23 * fc[4] + xl33 * fc[5];
+    v[2] = 2. * fc[1] * fc[1] + fc[0] * fc[0] + 2. * fc[2] * fc[2] + fc[3] * fc[3] + 2. * fc[4] * fc[4] +
+           fc[5] * fc[5];
+    v[3] = 2. * sr1[1] * sr1[1] + sr1[0] * sr1[0] + 2. * sr1[2] * sr1[2] + sr1[3] * sr1[3] +
+           2. * sr1[4] * sr1[4] + sr1[5] * sr1[5];
+    v[4] = 2. * sr1[1] * fc[1] + sr1[0] * fc[0] + 2. * sr1[2] * fc[2] + sr1[3] * fc[3] + 2. * sr1[4] * fc[4] +
+           sr1[5] * fc[5];
+  }
+  // block sums in a fixed order: wavefront butterflies, then the
+  // wavefronts in index order (the block is 4 full wavefronts)
+  __shared__ double rs[5][DT_TILE * DT_TILE / 64];
+#pragma unroll
+  for (int q = 0; q < 5; q++) {
+    const double a = wave_sum(v[q]);
+    if ((threadIdx.x & 63) == 0) rs[q][threadIdx.x >> 6] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x < 5) {
+    double a = 0.0;
+    for (int w = 0; w < DT_TILE * DT_TILE / 64; w++) a += rs[threadIdx.x][w];
+    part[5 * ((long long)(j - 1) * ntile + tile) + threadIdx.x] = a;
+  }
+}
+
+// CHHQR (compact3d.f:11571-11677): eigenvalues of an upper Hessenberg
+// matrix by the double-shift QR iteration, deflating where a subdiagonal
+// entry is at most eps times its diagonal neighbours.  With eps = 1e-5 the
+// roots carry that deflation's error and depend on the iteration path, so
+// the steps are the reference's: no exceptional shifts, its reflector
+// formulas, at most 60 iterations per eigenvalue (returns 0 when reached,
+// leaving the rest unset).  A: n x n column-major, n <= 5.
+__device__ int sd_chhqr(double* A, int n, double* wr, double* wi, double eps) {
+#pragma clang fp contract(off)
+#define HA(i, j) A[((j) - 1) * n + ((i) - 1)]
+  int m = n, it = 0;
+  for (;;) {
+    if (m == 0) return 1;
+    int l = m;
+    while (l > 1 && fabs(HA(l, l - 1)) > eps * (fabs(HA(l - 1, l - 1)) + fabs(HA(l, l)))) l--;
+    if (l == m) {
+      wr[m - 1] = HA(m, m);
+      wi[m - 1] = 0.0;
+      m -= 1;
+      it = 0;
+      continue;
+    }
+    if (l == m - 1) {
+      const double b = -(HA(m, m) + HA(m - 1, m - 1));
+      const double cc = HA(m, m) * HA(m - 1, m - 1) - HA(m, m - 1) * HA(m - 1, m);
+      const double w = b * b - 4 * cc;
+      const double y = sqrt(fabs(w));
+      if (w > 0.0) {
+        const double xy = (b < 0.0) ? -1.0 : 1.0;
+        wr[m - 1] = (-b - xy * y) / 2.0;
+        wr[m - 2] = cc / wr[m - 1];
+        wi[m - 1] = 0.0;
+        wi[m - 2] = 0.0;
+      } else {
+        wr[m - 1] = -b / 2.0;
+        wr[m - 2] = wr[m - 1];
+        wi[m - 1] = y / 2.0;
+        wi[m - 2] = -wi[m - 1];
+      }
+      m -= 2;
+      it = 0;
+      continue;
+    }
+    if (it >= 60) return 0;
+    it += 1;
+    for (int jj = l + 2; jj <= m; jj++) HA(jj, jj - 2) = 0.0;
+    for (int jj = l + 3; jj <= m; jj++) HA(jj, jj - 3) = 0.0;
+    for (int k = l; k <= m - 1; k++) {
+      double p, q, r, x = 0.0, y = 0.0;
+      if (k != l) {
+        p = HA(k, k - 1);
+        q = HA(k + 1, k - 1);
+        r = 0.0;
+        if (k != m - 1) r = HA(k + 2, k - 1);
+      } else {
+        x = HA(m, m) + HA(m - 1, m - 1);
+        y = HA(m - 1, m - 1) * HA(m, m) - HA(m - 1, m) * HA(m, m - 1);
+        p = HA(l, l) * (HA(l, l) - x) + HA(l, l + 1) * HA(l + 1, l) + y;
+        q = HA(l + 1, l) * (HA(l, l) + HA(l + 1, l + 1) - x);
+        r = HA(l + 1, l) * HA(l + 2, l + 1);
+      }
+      if (fabs(p) + fabs(q) + fabs(r) != 0.0) {
+        const double xy = (p < 0.0) ? -1.0 : 1.0;
+        const double sg = xy * sqrt(p * p + q * q + r * r);
+        if (k != l) HA(k, k - 1) = -sg;
+        const double e = -q / sg, f = -r / sg;
+        x = -p / sg;
+        y = -x - f * r / (p + sg);
+        const double gg = e * r / (p + sg);
+        const double z = -x - e * q / (p + sg);
+        for (int jj = k; jj <= m; jj++) {
+          p = x * HA(k, jj) + e * HA(k + 1, jj);
+          q = e * HA(k, jj) + y * HA(k + 1, jj);
+          r = f * HA(k, jj) + gg * HA(k + 1, jj);
+          if (k != m - 1) {
+            p = p + f * HA(k + 2, jj);
+            q = q + gg * HA(k + 2, jj);
+            r = r + z * HA(k + 2, jj);
+            HA(k + 2, jj) = r;
+          }
+          HA(k + 1, jj) = q;
+          HA(k, jj) = p;
+        }
+        const int imax = (k + 3 >= m) ? m : k + 3;
+        for (int i = l; i <= imax; i++) {
+          p = x * HA(i, k) + e * HA(i, k + 1);
+          q = e * HA(i, k) + y * HA(i, k + 1);
+          r = f * HA(i, k) + gg * HA(i, k + 1);
+          if (k != m - 1) {
+            p = p + f * HA(i, k + 2);
+            q = q + gg * HA(i, k + 2);
+            r = r + z * HA(i, k + 2);
+            HA(i, k + 2) = r;
+          }
+          HA(i, k + 1) = q;
+          HA(i, k) = p;
+        }
+      }
+    }
+  }
+#undef HA
+}
+
+// per plane: the two levels' sums from the tile partials (tile order),
+// scaled as :1643-1667 and :1772-1800; COEFFICIENT's quintic; BETA = the
+// largest real part of a root in (0, 2) (DQRRT :11548-11570), else 0
+__global__ __launch_bounds__(256) void k_sgssd_coef(Ctx c, int ntile, const double* p1, const double* p2) {
+  __shared__ double red[10][256];
+  const Geo g = c.g;
+  const int j = 1 + blockIdx.x;
+  double a[10];
+#pragma unroll
+  for (int q = 0; q < 10; q++) a[q] = 0.0;
+  for (int t = threadIdx.x; t < ntile; t += blockDim.x) {
+    const long long o = 5 * ((long long)(j - 1) * ntile + t);
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+      a[q] += p1[o + q];
+      a[5 + q] += p2[o + q];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 10; q++) red[q][threadIdx.x] = a[q];
+  __syncthreads();
+  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h)
+#pragma unroll
+      for (int q = 0; q < 10; q++) red[q][threadIdx.x] += red[q][threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  const double nxz = 1.0 / (double)(g.nx * g.nz);
+  const double A1 = -8.0 * red[0][0] * nxz, B1 = -2.0 * red[1][0] * nxz, C1 = 4.0 * red[2][0] * nxz;
+  const double D1 = 64.0 * red[3][0] * nxz, E1 = 32.0 * red[4][0] * nxz;
+  const double A2 = -32.0 * red[5][0] * nxz, B2 = -2.0 * red[6][0] * nxz, C2 = 4.0 * red[7][0] * nxz;
+  const double D2 = 1024.0 * red[8][0] * nxz, E2 = 128.0 * red[9][0] * nxz;
+  double pa[5];
+  pa[4] = (B2 * C1 - B1 * C2) / (A1 * D2);
+  pa[3] = (A1 * C2 - B2 * E1) / (A1 * D2);
+  pa[2] = (B2 * D1 + B1 * E2 - A2 * C1) / (A1 * D2);
+  pa[1] = (A2 * E1 - A1 * E2) / (A1 * D2);
+  pa[0] = -(A2 * D1 + B1 * D2) / (A1 * D2);
+  // companion matrix of x^5 + pa[0] x^4 + ... + pa[4]
+  double B[25], wr[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, wi[5];
+  for (int q = 0; q < 25; q++) B[q] = 0.0;
+  for (int q = 0; q < 5; q++) B[q * 5] = -pa[q];
+  for (int q = 0; q < 4; q++) B[q * 5 + q + 1] = 1.0;
+  sd_chhqr(B, 5, wr, wi, 1.0E-5);
+  double beta = 0.0;
+  for (int q = 0; q < 5; q++)
+    if (wr[q] > beta && wr[q] < 2.0) beta = wr[q];
+  c.d.sdbeta[j] = beta;
 }
 
 // --------------------------------------------------------- wall model
@@ -784,7 +1072,23 @@ int sgsdy_tiles(const Geo& g) {
 void sgsdy(const Ctx& c) {
   const int nt = sgsdy_tiles(c.g);
   KScope ks_(c, KID_SGS);
-  hipLaunchKernelGGL(k_sgsdy_tile, dim3(nt * c.g.n2m), dim3(DT_TILE * DT_TILE), 0, c.st, c);
+  hipLaunchKernelGGL(k_sgsdy_tile, dim3(nt * c.g.n2m), dim3(DT_TILE * DT_TILE), 0, c.st, c, nullptr);
+  hipLaunchKernelGGL(k_sgsdy_coef, dim3(c.g.n2m), dim3(256), 0, c.st, c, nt);
+  hipLaunchKernelGGL(k_sgsdy_clip, dim3(pgrid((long long)c.g.nx * c.g.n2m * c.g.nz)), dim3(C3D_NT), 0, c.st, c);
+}
+
+// the partials of the two filter levels sit at dyp and dyp + 5*N2M*tiles;
+// the last stage's L:M, M:M partials reuse the start of dyp once BETA is set
+void sgssd(const Ctx& c) {
+  const int nt = sgsdy_tiles(c.g);
+  double* p1 = c.d.dyp;
+  double* p2 = c.d.dyp + 5 * (size_t)c.g.n2m * nt;
+  KScope ks_(c, KID_SGS);
+  hipLaunchKernelGGL(k_sgssd_tile<1>, dim3(nt * c.g.n2m), dim3(DT_TILE * DT_TILE), 0, c.st, c, p1);
+  hipLaunchKernelGGL(k_sgssd_tile<2>, dim3(nt * c.g.n2m), dim3(DT_TILE * DT_TILE), 0, c.st, c, p2);
+  hipLaunchKernelGGL(k_sgssd_coef, dim3(c.g.n2m), dim3(256), 0, c.st, c, nt, (const double*)p1, (const double*)p2);
+  hipLaunchKernelGGL(k_sgsdy_tile, dim3(nt * c.g.n2m), dim3(DT_TILE * DT_TILE), 0, c.st, c,
+                     (const double*)c.d.sdbeta);
   hipLaunchKernelGGL(k_sgsdy_coef, dim3(c.g.n2m), dim3(256), 0, c.st, c, nt);
   hipLaunchKernelGGL(k_sgsdy_clip, dim3(pgrid((long long)c.g.nx * c.g.n2m * c.g.nz)), dim3(C3D_NT), 0, c.st, c);
 }

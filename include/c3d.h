@@ -48,7 +48,8 @@ typedef struct c3d_config {
   double re;          /* RE = 1/nu in the reference's units */
   double alx, aly, alz; /* box; ALY = 2 full channel, 1 half channel */
   double dt;          /* DT */
-  double files;       /* SGS model switch FILES: 0 none (DNS), 1 dynamic (SGSDY), 2 Smagorinsky (SGSCS) */
+  double files;       /* SGS model switch FILES: 0 none (DNS), 1 dynamic (SGSDY), 2 Smagorinsky (SGSCS),
+                         3 scale-dependent dynamic (SGSSD) */
   double wmodel;      /* WMODEL: 0 resolved walls, 1 instantaneous log-law wall model */
   double cks;         /* tanh stretching of y (the reference hard-codes 2.5, :338) */
   const double *y;    /* optional Y(0:N2) to use instead of the tanh grid, or NULL */
@@ -119,7 +120,8 @@ int c3d_rhs(c3d_handle *h);       /* RHS1/RHS2/RHS3 compact3d.f:6063, 6784, 7399
 int c3d_getuh(c3d_handle *h);     /* GETUH1/2/3 compact3d.f:8218, 8344, 8478 */
 int c3d_dpcalc(c3d_handle *h);    /* DPCALC = RHSDP + GETDP compact3d.f:1037-1049 */
 int c3d_upcalc(c3d_handle *h);    /* UPCALC compact3d.f:1052-1238 */
-int c3d_sgs(c3d_handle *h);       /* SGSDY (FILES=1) compact3d.f:1250 / SGSCS (FILES=2) :1413 */
+int c3d_sgs(c3d_handle *h);       /* SGSDY (FILES=1) compact3d.f:1250 / SGSCS (FILES=2) :1413 /
+                                     SGSSD (FILES=3) :1502 */
 int c3d_wall_model(c3d_handle *h); /* WALL_MODEL -> WALLMODEL_INSTANT compact3d.f:1952, 2319 */
 int c3d_wallss(c3d_handle *h);    /* WALLSS compact3d.f:11077 (sets WSM(1:3)) */
 
@@ -142,6 +144,11 @@ int c3d_flowrate(c3d_handle *h, double *flow1, double *flow3);
  * (mean vorticity) 7 VRMS(J,1..6) (uu vv ww uv uw vw) 13 PRMS 14 PV
  * 15 WRMS(J,1..3) 18 U2V 19 VW2; 20*(N2-1) values */
 int c3d_stats(c3d_handle *h, double *out);
+/* the dynamic models' per-plane profiles of the last SGS evaluation:
+ * smagc[J-1] = SMAGC(J) (COMMON/SGSMEAN, FILES = 1 or 3) and beta[J-1] =
+ * SGSSD's BETA(J) (the BETAZ(J) of BETA.plt before its 0.125 floor,
+ * compact3d.f:1802-1804; FILES = 3), J = 1..N2-1; either may be NULL */
+int c3d_sgs_profiles(c3d_handle *h, double *smagc, double *beta);
 
 /* WRITEUP's Tecplot output (compact3d.f:757-835).
  * c3d_cellcentre: the cell-centre values WRITEUP writes (:796-808),

@@ -2227,6 +2227,22 @@ void c3o_sd_coefficient(const double s1[5], const double s2[5], double pa[5]) {
   pa[0] = -(A2 * D1 + B1 * D2) / (A1 * D2);
 }
 
+/* one plane's sums of both filter levels (s1, s2: A..E), PA and BETA */
+double c3o_sgssd_plane(c3o *o, const double *U, int j, double *s1, double *s2, double *pa) {
+  const size_t pl = (size_t)o->n1 * o->n3;
+  double *SR = dalloc(pl * 6), *COM = dalloc(pl * 6), *COM1 = dalloc(pl * 6), *SR1 = dalloc(pl * 6);
+  double *UC = dalloc(pl * 3);
+  sgssd_sums(o, U, j, 0, SR, COM, COM1, SR1, UC, s1);
+  sgssd_sums(o, U, j, 1, SR, COM, COM1, SR1, UC, s2);
+  c3o_sd_coefficient(s1, s2, pa);
+  free(SR);
+  free(COM);
+  free(COM1);
+  free(SR1);
+  free(UC);
+  return c3o_dqrrt(pa, 5);
+}
+
 /* SGSSD: per plane the sums of both filter levels, BETA from the quintic,
  * then SGSDY's Germano step with the test-filter factor 4*BETA (:1804-1930).
  * smagc_out[J], beta_out[J] (J = 1..N2M) when not NULL. */

@@ -71,6 +71,8 @@ void c3o_sgssd(c3o *o, const double *U, double *SGSVIS, double *smagc_out, doubl
 /* COEFFICIENT :2079-2097 and DQRRT/CHHQR :11548-11677 */
 void c3o_sd_coefficient(const double s1[5], const double s2[5], double pa[5]);
 double c3o_dqrrt(const double *pa, int n);
+/* one plane of SGSSD: its two levels' sums s1, s2 (A..E), PA; returns BETA */
+double c3o_sgssd_plane(c3o *o, const double *U, int j, double *s1, double *s2, double *pa);
 void c3o_wallmodel_instant(c3o *o, const double *U);
 void c3o_filter(c3o *o, double *FILV, int n);
 void c3o_filter4(c3o *o, double *FILV, int n);

@@ -73,6 +73,8 @@ class Oracle:
             L.c3o_dft2d.argtypes = [C.c_int, C.c_int, _dp, C.c_int]
             L.c3o_dqrrt.restype = C.c_double
             L.c3o_dqrrt.argtypes = [_dp, C.c_int]
+            L.c3o_sgssd_plane.restype = C.c_double
+            L.c3o_sgssd_plane.argtypes = [C.c_void_p, _dp, C.c_int, _dp, _dp, _dp]
             cls._lib = L
         return cls._lib
 
@@ -229,6 +231,13 @@ class Oracle:
         b = np.zeros(self.n2 + 1)
         self.L.c3o_sgssd(self.h, _p(U), _p(SGSVIS), _p(c), _p(b))
         return c, b
+
+    def sgssd_plane(self, U, j):
+        """SGSSD's plane J: (s1, s2, PA, BETA) with s1, s2 the A..E sums of the
+        FILTER and FILTER4 levels"""
+        s1, s2, pa = np.zeros(5), np.zeros(5), np.zeros(5)
+        beta = self.L.c3o_sgssd_plane(self.h, _p(U), int(j), _p(s1), _p(s2), _p(pa))
+        return s1, s2, pa, beta
 
     @classmethod
     def sd_coefficient(cls, s1, s2):
