@@ -2374,6 +2374,68 @@ void c3o_wallmodel_instant(c3o *o, const double *U) {
   o->taow = o->wsm[0];
 }
 
+/* ------------------------------------------------------- WALLMODEL_WERNER */
+/* compact3d.f:4815-4958 (WMODEL = 4): the Werner-Wengle power-law wall
+ * stress at both walls from the first cell's centre velocity, a linear
+ * (viscous) branch below VCASE = AB3*sqrt|WSM(1)|/11.81 (WSM(1) of the
+ * previous call), then averaged onto the faces.  The reference takes the
+ * I neighbour as IPA(K) and IMA(K) (:4849, :4925: the K index in IPA/IMA),
+ * which is reproduced; it is defined for N3-1 <= N1-1. */
+void c3o_wallmodel_werner(c3o *o, const double *U) {
+  const int N1M = o->n1m, N2M = o->n2m, N3M = o->n3m;
+  const double RE = o->re;
+  const double *DY = o->dy;
+  const double *W = U + 2 * NU;
+  const double B = 0.1428571429;
+  const double AB1 = 7.202125274, AB3 = 69.74057973, AB4 = 0.1376936317, AB5 = 1.75;
+  double *TWE = dalloc((size_t)N1M * N3M * 4);
+#define TE(i, k, l) TWE[((size_t)(l) - 1) * N1M * N3M + ((size_t)(k) - 1) * N1M + ((i) - 1)]
+  for (int wall = 0; wall < 2; wall++) {
+    const int j = wall == 0 ? 1 : N2M;
+    const double deltap = DY[j];
+    for (int k = 1; k <= N3M; k++) {
+      const int kp = o->kpa[k];
+      for (int i = 1; i <= N1M; i++) {
+        const int ip = o->ipa[k];
+        const double u1 = 0.5 * (U[IX(i, j, k)] + U[IX(ip, j, k)]);
+        const double u2 = 0.5 * (W[IX(i, j, k)] + W[IX(i, j, kp)]);
+        const double velocp = sqrt(u1 * u1 + u2 * u2);
+        const double temp = sqrt(fabs(o->wsm[0])) / 11.81;
+        const double vcase = AB3 * temp;
+        double tauw1 = 0.0;
+        if (velocp <= vcase) {
+          tauw1 = 2. * velocp / deltap / RE;
+        } else if (velocp > vcase) {
+          tauw1 = AB1 * pow(1.0 / deltap / RE, 1. + B);
+          tauw1 = tauw1 + AB4 * pow(1.0 / deltap / RE, B) * velocp;
+          tauw1 = pow(tauw1, AB5);
+        }
+        TE(i, k, 2 * wall + 1) = tauw1 * (u1 / velocp);
+        TE(i, k, 2 * wall + 2) = tauw1 * (u2 / velocp);
+      }
+    }
+  }
+  for (int k = 1; k <= N3M; k++) {
+    const int kp = o->kma[k];
+    for (int i = 1; i <= N1M; i++) {
+      const int ip = o->ima[k];
+      o->tauw[TAUX(i, k, 1)] = 0.5 * (TE(i, k, 1) + TE(ip, k, 1));
+      o->taub[TAUX(i, k, 1)] = 0.5 * (TE(i, k, 2) + TE(i, kp, 2));
+      o->tauw[TAUX(i, k, 2)] = 0.5 * (TE(i, k, 3) + TE(ip, k, 3));
+      o->taub[TAUX(i, k, 2)] = 0.5 * (TE(i, k, 4) + TE(i, kp, 4));
+    }
+  }
+  double w1 = 0.0;
+  for (int k = 1; k <= N3M; k++)
+    for (int i = 1; i <= N1M; i++) w1 = w1 + TE(i, k, 1);
+  o->wsm[0] = w1 / (double)(N1M * N3M);
+  o->wsm[1] = 0.0;
+  o->wsm[2] = 0.0;
+  o->taow = o->wsm[0];
+#undef TE
+  free(TWE);
+}
+
 /* ------------------------------------------------------------ FILTER(4) */
 /* FILTER (compact3d.f:5190-5228): FILV(1:N1,1:N3,n) */
 void c3o_filter(c3o *o, double *FILV, int n) {
@@ -2533,6 +2595,9 @@ void c3o_getup(c3o *o, double *U, double *P, double *HH, double *SGSVIS,
       c3o_wallss(o, U);
     } else if (o->wmodel == 1.0) {
       c3o_wallmodel_instant(o, U);
+      o->ifwall = 1;
+    } else if (o->wmodel == 4.0) {
+      c3o_wallmodel_werner(o, U);
       o->ifwall = 1;
     }
   }

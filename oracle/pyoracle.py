@@ -265,6 +265,9 @@ class Oracle:
     def wallmodel_instant(self, U):
         self.L.c3o_wallmodel_instant(self.h, _p(U))
 
+    def wallmodel_werner(self, U):
+        self.L.c3o_wallmodel_werner(self.h, _p(U))
+
     def filter(self, FILV, four=False):
         n = FILV.shape[2]
         (self.L.c3o_filter4 if four else self.L.c3o_filter)(self.h, _p(FILV), n)
@@ -492,6 +495,11 @@ class RefLib:
     def wallmodel_instant(self, U, SGSVIS):
         tw, tb = self.get_tau()
         self.L.wallmodel_instant_(_p(U), _p(tw), _p(tb), _p(SGSVIS))
+        self.set_tau(tw, tb)
+
+    def wallmodel_werner(self, U, SGSVIS, ntime=1):
+        tw, tb = self.get_tau()
+        self.L.wallmodel_werner_(_p(U), _p(tw), _p(tb), _p(SGSVIS), self._i(ntime))
         self.set_tau(tw, tb)
 
     def strain(self, U, j):

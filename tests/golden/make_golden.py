@@ -37,6 +37,7 @@ CASES = {
     "les": dict(re=5200.0, alx=2 * math.pi, aly=2.0, alz=math.pi, dt=5e-4, files=2.0, wmodel=1.0),
     "dyn": dict(re=5200.0, alx=2 * math.pi, aly=2.0, alz=math.pi, dt=5e-4, files=1.0, wmodel=1.0),
     "sd": dict(re=5200.0, alx=2 * math.pi, aly=2.0, alz=math.pi, dt=5e-4, files=3.0, wmodel=1.0),
+    "wer": dict(re=5200.0, alx=2 * math.pi, aly=2.0, alz=math.pi, dt=5e-4, files=2.0, wmodel=4.0),
 }
 
 
@@ -76,8 +77,11 @@ def run_case(name, cfg):
         s1, s2 = rq.standard_normal(5), rq.standard_normal(5)
         out["COEF_IN"] = np.concatenate([s1, s2])
         out["COEF_PA"] = ref.coefficient(s1, s2)
-    if cfg["files"] != 0.0 and cfg["wmodel"] == 1.0:
-        ref.wallmodel_instant(U, S)
+    if cfg["files"] != 0.0 and cfg["wmodel"] in (1.0, 4.0):
+        if cfg["wmodel"] == 1.0:
+            ref.wallmodel_instant(U, S)
+        else:
+            ref.wallmodel_werner(U, S)
         ref.set_ifwall(1)
         tw, tb = ref.get_tau()
         out["TAUW"], out["TAUB"] = tw, tb

@@ -5,7 +5,8 @@ Every phase the reference can run without FFTW is compared: BCOND, RHS1-3,
 GETUH1-3, RHSDP, UPCALC, DIVCHECK, CFL, SGSCS/STRAIN, SGSDY (the dynamic
 model, its per-plane coefficient included), SGSSD (the scale-dependent
 model) with COEFFICIENT and the DQRRT/CHHQR root selection,
-WALLMODEL_INSTANT, CTDMA11, TDMAP, FILTER, FILTER4.  The oracle follows the reference's operation
+WALLMODEL_INSTANT, WALLMODEL_WERNER (both walls), CTDMA11, TDMAP, FILTER,
+FILTER4.  The oracle follows the reference's operation
 order, so the expected agreement is exact; the bound allows 4 ulps."""
 import os
 
@@ -37,7 +38,7 @@ def _check(fx, key, arr, stride):
     assert es <= 1e-12, (key, es)
 
 
-@pytest.mark.parametrize("name", ["dns", "les", "dyn", "sd"])
+@pytest.mark.parametrize("name", ["dns", "les", "dyn", "sd", "wer"])
 def test_oracle_matches_reference_fixture(name):
     from pyoracle import Oracle
     fx = _load(name)
@@ -63,8 +64,11 @@ def test_oracle_matches_reference_fixture(name):
         for pa, beta in zip(fx["DQRRT_PA"], fx["DQRRT_BETA"]):
             assert Oracle.dqrrt(pa) == beta
         assert np.array_equal(Oracle.sd_coefficient(fx["COEF_IN"][:5], fx["COEF_IN"][5:]), fx["COEF_PA"])
-    if files != 0.0 and wmodel == 1.0:
-        o.wallmodel_instant(U)
+    if files != 0.0 and wmodel in (1.0, 4.0):
+        if wmodel == 1.0:
+            o.wallmodel_instant(U)
+        else:
+            o.wallmodel_werner(U)
         o.set_ifwall(1)
         tw, tb = o.get_tau()
         _check(fx, "TAUW", tw, stride)
