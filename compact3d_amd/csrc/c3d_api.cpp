@@ -521,7 +521,10 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   if (cf.precision != C3D_FP64 && cf.precision != C3D_MIXED) return bad("precision must be C3D_FP64 or C3D_MIXED");
   if (cf.files != 0.0 && cf.files != 1.0 && cf.files != 2.0 && cf.files != 3.0)
     return bad("FILES must be 0 (DNS), 1 (dynamic model), 2 (Smagorinsky) or 3 (scale-dependent dynamic model)");
-  if (cf.wmodel != 0.0 && cf.wmodel != 1.0) return bad("WMODEL must be 0 or 1");
+  if (cf.wmodel != 0.0 && cf.wmodel != 1.0 && cf.wmodel != 4.0)
+    return bad("WMODEL must be 0 (resolved walls), 1 (log law, bottom wall) or 4 (Werner-Wengle, both walls)");
+  // WALLMODEL_WERNER takes its I neighbour from IPA(K) (compact3d.f:4849)
+  if (cf.wmodel == 4.0 && cf.n3 > cf.n1) return bad("WMODEL = 4 needs N3 <= N1");
   if (cf.re <= 0 || cf.dt <= 0 || cf.alx <= 0 || cf.alz <= 0) return bad("RE, DT, ALX, ALZ must be > 0");
   if (hipSetDevice(cf.device) != hipSuccess) return bad("hipSetDevice failed");
   Ctx& c = h->c;
@@ -705,9 +708,11 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
     F.hhp[q] = h->hhbuf[1][q];
   }
   const size_t n2d = (size_t)g.nx * g.nz;
-  if (dalloc(h, &F.ubc_u, n2d) || dalloc(h, &F.ubc_w, n2d) || dalloc(h, &F.tauw, n2d) ||
-      dalloc(h, &F.taub, n2d))
+  if (dalloc(h, &F.ubc_u, n2d) || dalloc(h, &F.ubc_w, n2d) || dalloc(h, &F.tauw, 2 * n2d) ||
+      dalloc(h, &F.taub, 2 * n2d))
     return bad(h->err);
+  F.tauwe = nullptr;
+  if (cf.wmodel == 4.0 && dalloc(h, &F.tauwe, 4 * n2d)) return bad(h->err);
   if (dalloc(h, &F.spec, (size_t)g.nxhp * g.nz * g.n2m)) return bad(h->err);
   if (cf.files != 0.0) {
     double** les[] = {&F.nux, &F.nuz, &F.nue, &F.dvdx, &F.dwdx, &F.dudz, &F.dvdz};
@@ -1014,6 +1019,9 @@ extern "C" int c3d_sgs(c3d_handle* h) {
 static void wall_model_impl(c3d_handle* h) {
   if (h->cfg.wmodel == 1.0) {
     c3dk::wallmodel(h->c);
+    h->c.s.ifwall = 1;
+  } else if (h->cfg.wmodel == 4.0) {
+    c3dk::wallmodel_werner(h->c);
     h->c.s.ifwall = 1;
   } else {
     c3dk::wallss(h->c);
@@ -1748,14 +1756,18 @@ extern "C" int c3d_get_field(c3d_handle* h, int which, double* out) {
     case C3D_F_TAUW:
     case C3D_F_TAUB: {
       const Geo& g = h->c.g;
-      std::vector<double> t((size_t)g.nx * g.nz);
+      const size_t n2d = (size_t)g.nx * g.nz;
+      std::vector<double> t(2 * n2d);
       HIPCHK(hipMemcpyAsync(t.data(), which == C3D_F_TAUW ? F.tauw : F.taub, t.size() * 8, hipMemcpyDeviceToHost,
                             h->c.st));
       HIPCHK(hipStreamSynchronize(h->c.st));
       const int n1 = h->cfg.n1, n3 = h->cfg.n3;
-      for (size_t q = 0; q < (size_t)(n1 + 1) * (n3 + 1) * 2; q++) out[q] = 0.0;
-      for (int k = 0; k < g.nz; k++)
-        for (int i = 0; i < g.nx; i++) out[(size_t)(i + 1) + (size_t)(n1 + 1) * (k + 1)] = t[(size_t)k * g.nx + i];
+      const size_t lay = (size_t)(n1 + 1) * (n3 + 1);
+      for (size_t q = 0; q < 2 * lay; q++) out[q] = 0.0;
+      for (int l = 0; l < 2; l++)
+        for (int k = 0; k < g.nz; k++)
+          for (int i = 0; i < g.nx; i++)
+            out[l * lay + (size_t)(i + 1) + (size_t)(n1 + 1) * (k + 1)] = t[l * n2d + (size_t)k * g.nx + i];
       return 0;
     }
     default: return fail(h, "unknown field");

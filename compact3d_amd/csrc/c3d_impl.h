@@ -103,7 +103,10 @@ struct Fields {
   double *nux, *nuz, *nue, *dvdx, *dwdx, *dudz, *dvdz;
   double *div;
   double *ubc_u, *ubc_w;  // top ghost values U(I,N2,K,1), U(I,N2,K,3) from BCOND (NX*NZ)
-  double *tauw, *taub;    // bottom-wall stresses TAUW(I,K,1), TAUB(I,K,1) (NX*NZ)
+  // wall stresses TAUW(I,K,L), TAUB(I,K,L) at [(L-1)*NX*NZ + (K-1)*NX + I-1]:
+  // L = 1 bottom, 2 top (zero unless a two-wall model sets it)
+  double *tauw, *taub;
+  double *tauwe;          // WMODEL = 4: the cell-centre stresses TAUWE(I,K,1:4)
   double2 *spec;          // Poisson spectrum [jm][k][kx], NXH*NZ*N2M
   double *pib, *pgt;      // Poisson TDMAP factors per mode: 1/BET(J), GAM(J+1) [jm][kz][kx]
   double *sc;             // device scalars (SC_*)
@@ -201,6 +204,7 @@ void sgsdy(const Ctx& c);            // SGSDY, the dynamic model (FILES = 1)
 void sgssd(const Ctx& c);            // SGSSD, the scale-dependent dynamic model (FILES = 3)
 int sgsdy_tiles(const Geo& g);      // tiles per plane of its partial sums
 void wallmodel(const Ctx& c);
+void wallmodel_werner(const Ctx& c);  // WALLMODEL_WERNER (WMODEL = 4), both walls
 void wallss(const Ctx& c);
 void sgs_terms(const Ctx& c);
 void init_attrs();

@@ -595,7 +595,7 @@ __global__ __launch_bounds__(C3D_NT) void k_wallmodel(Ctx c) {
   if (t == 0) F.red[blockIdx.x * 4] = red[0];
 }
 
-__global__ __launch_bounds__(C3D_NT) void k_wallmodel_fin(Ctx c, int nblk) {
+__global__ __launch_bounds__(C3D_NT) void k_wallmodel_fin(Ctx c, int nblk, int zero23) {
   __shared__ double red[C3D_NT];
   const Fields F = c.d;
   const int t = threadIdx.x;
@@ -611,7 +611,83 @@ __global__ __launch_bounds__(C3D_NT) void k_wallmodel_fin(Ctx c, int nblk) {
     const double w = red[0] / (double)((long long)c.g.nx * c.g.nz);
     F.sc[SC_WSM1] = w;
     F.sc[SC_TAOW] = w;
+    if (zero23) {
+      F.sc[SC_WSM2] = 0.0;
+      F.sc[SC_WSM3] = 0.0;
+    }
   }
+}
+
+// WALLMODEL_WERNER (compact3d.f:4815-4958), WMODEL = 4: the Werner-Wengle
+// power law at both walls from the first cell's centre velocity
+// (|u| <= AB3 sqrt|WSM(1)|/11.81: the viscous branch 2|u|/DELTAP/RE), WSM(1)
+// being the previous call's.  k_werner: the cell-centre stresses TAUWE of
+// both walls (:4840-4909); the I neighbour is IPA(K), as the reference has
+// it (:4849).  k_werner_avg: the face stresses TAUW, TAUB of both walls
+// (neighbours IMA(K) and KMA(K), :4918-4931) and the block sums of
+// TAUWE(.,.,1) for WSM(1) (:4934-4951).
+__global__ __launch_bounds__(C3D_NT) void k_werner(Ctx c) {
+  const Geo g = c.g;
+  const Metrics M = c.m;
+  const Fields F = c.d;
+  const long long n2d = (long long)g.nx * g.nz;
+  const double RE = c.s.re;
+  const double B = 0.1428571429;
+  const double AB1 = 7.202125274, AB3 = 69.74057973, AB4 = 0.1376936317, AB5 = 1.75;
+  const double temp = sqrt(fabs(F.sc[SC_WSM1])) / 11.81;
+  const double vcase = AB3 * temp;
+  for (long long t = blockIdx.x * (long long)C3D_NT + threadIdx.x; t < 2 * n2d; t += (long long)gridDim.x * C3D_NT) {
+    const int wall = (int)(t / n2d);
+    const long long e = t - wall * n2d;
+    const int i = (int)(e % g.nx), k = (int)(e / g.nx);
+    const int j = wall == 0 ? 1 : g.n2m;
+    const double deltap = M.dy[j];
+    const int ip = (k + 1 < g.nx) ? k + 1 : 0;  // IPA(K), K = k+1
+    const int kp = wrapp(k, g.nz);
+    const double u1 = 0.5 * (F.u[gidx(g, i, j, k)] + F.u[gidx(g, ip, j, k)]);
+    const double u2 = 0.5 * (F.w[gidx(g, i, j, k)] + F.w[gidx(g, i, j, kp)]);
+    const double velocp = sqrt(u1 * u1 + u2 * u2);
+    double tauw1 = 0.0;
+    if (velocp <= vcase) {
+      tauw1 = 2. * velocp / deltap / RE;
+    } else if (velocp > vcase) {
+      tauw1 = AB1 * pow(1.0 / deltap / RE, 1. + B);
+      tauw1 = tauw1 + AB4 * pow(1.0 / deltap / RE, B) * velocp;
+      tauw1 = pow(tauw1, AB5);
+    }
+    F.tauwe[(2 * wall) * n2d + e] = tauw1 * (u1 / velocp);
+    F.tauwe[(2 * wall + 1) * n2d + e] = tauw1 * (u2 / velocp);
+  }
+}
+
+__global__ __launch_bounds__(C3D_NT) void k_werner_avg(Ctx c) {
+  __shared__ double red[C3D_NT];
+  const Geo g = c.g;
+  const Fields F = c.d;
+  const long long n2d = (long long)g.nx * g.nz;
+  const double* T1 = F.tauwe;
+  const double* T2 = F.tauwe + n2d;
+  const double* T3 = F.tauwe + 2 * n2d;
+  const double* T4 = F.tauwe + 3 * n2d;
+  double acc = 0.0;
+  for (long long e = blockIdx.x * (long long)C3D_NT + threadIdx.x; e < n2d; e += (long long)gridDim.x * C3D_NT) {
+    const int i = (int)(e % g.nx), k = (int)(e / g.nx);
+    const long long im = (long long)k * g.nx + (k > 0 ? k - 1 : g.nx - 1);  // (IMA(K), K)
+    const long long km = (long long)wrapm(k, g.nz) * g.nx + i;              // (I, KMA(K))
+    F.tauw[e] = 0.5 * (T1[e] + T1[im]);
+    F.taub[e] = 0.5 * (T2[e] + T2[km]);
+    F.tauw[n2d + e] = 0.5 * (T3[e] + T3[im]);
+    F.taub[n2d + e] = 0.5 * (T4[e] + T4[km]);
+    acc += T1[e];
+  }
+  const int t = threadIdx.x;
+  red[t] = acc;
+  __syncthreads();
+  for (int h = C3D_NT / 2; h > 0; h >>= 1) {
+    if (t < h) red[t] += red[t + h];
+    __syncthreads();
+  }
+  if (t == 0) F.red[blockIdx.x * 4] = red[0];
 }
 
 // WALLSS (compact3d.f:11077-11143), including its use of WST(.,.,1) in WSM(2)
@@ -1100,7 +1176,16 @@ void wallmodel(const Ctx& c) {
     hipLaunchKernelGGL(k_wallmodel, dim3(nb), dim3(C3D_NT), 0, c.st, c);
   }
   KScope ks_(c, KID_WALL);
-  hipLaunchKernelGGL(k_wallmodel_fin, dim3(1), dim3(C3D_NT), 0, c.st, c, nb);
+  hipLaunchKernelGGL(k_wallmodel_fin, dim3(1), dim3(C3D_NT), 0, c.st, c, nb, 0);
+}
+
+void wallmodel_werner(const Ctx& c) {
+  const long long n2d = (long long)c.g.nx * c.g.nz;
+  const int nb = std::min(1024, pgrid(n2d));
+  KScope ks_(c, KID_WALL);
+  hipLaunchKernelGGL(k_werner, dim3(std::min(2048, pgrid(2 * n2d))), dim3(C3D_NT), 0, c.st, c);
+  hipLaunchKernelGGL(k_werner_avg, dim3(nb), dim3(C3D_NT), 0, c.st, c);
+  hipLaunchKernelGGL(k_wallmodel_fin, dim3(1), dim3(C3D_NT), 0, c.st, c, nb, 1);
 }
 
 void wallss(const Ctx& c) {

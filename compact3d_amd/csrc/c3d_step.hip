@@ -541,7 +541,9 @@ __device__ __forceinline__ void yfuse_u(const Ctx& c, const T* tb, unsigned b, u
   const T rinv = (T)c.s.rinv, dt = (T)c.s.dt;
   const unsigned nx = g.nx;
   const T uN2 = su[b + g.n2 * nx], uW = su[b];
+  // bottom and top wall stresses (:6765-6768)
   const T bcw = -(T)F.tauw[t2] / (T)c.m.dy[1] * (T)c.s.ifwall;
+  const T bcu = -(T)F.tauw[(unsigned)(g.nx * g.nz) + t2] / (T)c.m.dy[g.n2m] * (T)c.s.ifwall * (T)(c.s.aly - 1.);
   const unsigned a0 = b + (unsigned)y.j0 * nx;  // row J0
   T us[CH + 2], w[CH + 1], ru[CH], hx[CH], hq[CH];
 #pragma unroll
@@ -568,7 +570,7 @@ __device__ __forceinline__ void yfuse_u(const Ctx& c, const T* tb, unsigned b, u
       const T v1 = (J == 1) ? (T)0 : w[q];
       const T h = hx[q] + (T)t[8] * (v2 * u2 - v1 * u1);
       hh[a0 + q * nx] = h;
-      const T r = r0 - (T)(3. / 2.) * h + (T)(1. / 2.) * hq[q] + (T)t[10] * bcw;
+      const T r = r0 - (T)(3. / 2.) * h + (T)(1. / 2.) * hq[q] + ((T)t[10] * bcw + (T)t[9] * bcu);
       v[q] = (T)t[YT_BET] * (r * dt);
     }
 }
@@ -628,6 +630,7 @@ __device__ __forceinline__ void yfuse_w(const Ctx& c, const T* tb, unsigned b, u
   const unsigned nx = g.nx;
   const T wN2 = sw[b + g.n2 * nx], wW = sw[b];
   const T bcw = -(T)F.taub[t2] / (T)c.m.dy[1] * (T)c.s.ifwall;
+  const T bcu = -(T)F.taub[(unsigned)(g.nx * g.nz) + t2] / (T)c.m.dy[g.n2m] * (T)c.s.ifwall * (T)(c.s.aly - 1.);
   const unsigned a0 = b + (unsigned)y.j0 * nx;
   T ws[CH + 2], s3[CH + 1], ru[CH], hx[CH], hq[CH];
 #pragma unroll
@@ -654,7 +657,7 @@ __device__ __forceinline__ void yfuse_w(const Ctx& c, const T* tb, unsigned b, u
       const T v1 = (J == 1) ? (T)0 : s3[q];
       const T h = hx[q] + (T)t[8] * (v2 * w2 - v1 * w1);
       hh[a0 + q * nx] = h;
-      const T r = r0 - (T)(3. / 2.) * h + (T)(1. / 2.) * hq[q] + (T)t[10] * bcw;
+      const T r = r0 - (T)(3. / 2.) * h + (T)(1. / 2.) * hq[q] + ((T)t[10] * bcw + (T)t[9] * bcu);
       v[q] = (T)t[YT_BET] * (r * dt);
     }
 }

@@ -147,13 +147,15 @@ def test_apply_op_vs_dense(op, direction):
     ch.close()
 
 
-def test_les_phases_match_oracle():
-    """Smagorinsky SGSCS, the log-law wall model and the SGS terms of RHS1-3
-    (FILES=2, WMODEL=1, uniform y grid as MESH makes it, compact3d.f:356-357)"""
+@pytest.mark.parametrize("wmodel", [1.0, 4.0])
+def test_les_phases_match_oracle(wmodel):
+    """Smagorinsky SGSCS, the wall model (WMODEL=1: the log law at the bottom
+    wall; 4: Werner-Wengle at both walls) and the SGS and wall-stress terms
+    of RHS1-3 (FILES=2, uniform y grid as MESH makes it, compact3d.f:356-357)"""
     from compact3d_amd import c3d
     from pyoracle import Oracle
     n1, n2, n3 = 33, 33, 33
-    cfg = config(n1, n2, n3, re=5200.0, dt=5e-4, files=2.0, wmodel=1.0)
+    cfg = config(n1, n2, n3, re=5200.0, dt=5e-4, files=2.0, wmodel=wmodel)
     orc = Oracle(**cfg)
     ch = c3d.Channel(**cfg)
     U, P, HH2 = inputs(orc, seed=7)
@@ -170,10 +172,15 @@ def test_les_phases_match_oracle():
     e = relerr(interior_u(So, n1, n2, n3), interior_u(ch.download()["SGSVIS"], n1, n2, n3))
     print(f"SGSCS {e:.3e}")
     assert e <= 1e-13
-    orc.wallmodel_instant(Uo)
+    if wmodel == 1.0:
+        orc.wallmodel_instant(Uo)
+    else:
+        orc.wallmodel_werner(Uo)
     orc.set_ifwall(1)
     ch.wall_model()
     tw, tb = orc.get_tau()
+    if wmodel == 4.0:
+        assert np.max(np.abs(tw[:, :, 1])) > 0 and np.max(np.abs(tb[:, :, 1])) > 0  # the top wall too
     e1 = relerr(tw, ch.field(c3d.F_TAUW))
     e2 = relerr(tb, ch.field(c3d.F_TAUB))
     ew = abs(orc.get_wsm()[0] - ch.scalar(c3d.S_WSM1)) / orc.get_wsm()[0]
@@ -197,7 +204,7 @@ def test_les_phases_match_oracle():
     ch.close()
 
 
-@pytest.mark.parametrize("wmodel", [1.0, 0.0])
+@pytest.mark.parametrize("wmodel", [1.0, 0.0, 4.0])
 def test_les_trajectory_10_steps(wmodel):
     from compact3d_amd import c3d
     from pyoracle import Oracle
