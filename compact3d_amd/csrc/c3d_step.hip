@@ -528,6 +528,22 @@ __global__ __launch_bounds__(512) void k_adi_y(Ctx c) {
 // all of its loads in flight at once; the expressions are those of
 // yterm_u/v/w.  With SGS terms it calls yterm_*.  Returns TDMAI's right-hand
 // side BET(J) * RUH(J) * DT.
+// the wall-stress term of RHS1/RHS3 (:6765-6768, :8096-8103):
+// (1-FJMU(J)) (-TAU(I,K,1)/DY(1)) IFWALL + (1-FJPA(J)) (-TAU(I,K,2)/DY(N2M))
+// IFWALL (ALY-1).  The flags are 0 except FJMU(1) and FJPA(N2M), so only the
+// rows J = 1 and J = N2M load a stress; t: the row's coefficients.
+template <typename T>
+__device__ __forceinline__ T wall_bc(const Ctx& c, const double* tau, unsigned t2, int J, const T* t) {
+  T bc = (T)0;
+  if (J == 1) bc = (T)t[10] * (-(T)tau[t2] / (T)c.m.dy[1] * (T)c.s.ifwall);
+  if (J == c.g.n2m)
+    bc = bc + (T)t[9] * (-(T)tau[(unsigned)(c.g.nx * c.g.nz) + t2] / (T)c.m.dy[c.g.n2m] * (T)c.s.ifwall *
+                         (T)(c.s.aly - 1.));
+  return bc;
+}
+
+// RUH, HH and HH2 are read once and HH written once per step: their loads
+// and stores carry the nontemporal hint (measured 1-1.5 % faster than plain)
 template <typename T, int CH>
 __device__ __forceinline__ void yfuse_u(const Ctx& c, const T* tb, unsigned b, unsigned t2, const YPos& y,
                                         T (&v)[CH]) {
@@ -541,9 +557,7 @@ __device__ __forceinline__ void yfuse_u(const Ctx& c, const T* tb, unsigned b, u
   const T rinv = (T)c.s.rinv, dt = (T)c.s.dt;
   const unsigned nx = g.nx;
   const T uN2 = su[b + g.n2 * nx], uW = su[b];
-  // bottom and top wall stresses (:6765-6768)
-  const T bcw = -(T)F.tauw[t2] / (T)c.m.dy[1] * (T)c.s.ifwall;
-  const T bcu = -(T)F.tauw[(unsigned)(g.nx * g.nz) + t2] / (T)c.m.dy[g.n2m] * (T)c.s.ifwall * (T)(c.s.aly - 1.);
+
   const unsigned a0 = b + (unsigned)y.j0 * nx;  // row J0
   T us[CH + 2], w[CH + 1], ru[CH], hx[CH], hq[CH];
 #pragma unroll
@@ -553,9 +567,9 @@ __device__ __forceinline__ void yfuse_u(const Ctx& c, const T* tb, unsigned b, u
 #pragma unroll
   for (int q = 0; q < CH; q++) {
     const bool in = q < y.len;
-    ru[q] = in ? ruh[a0 + q * nx] : (T)0;
-    hx[q] = in ? hh[a0 + q * nx] : (T)0;
-    hq[q] = in ? hp[a0 + q * nx] : (T)0;
+    ru[q] = in ? __builtin_nontemporal_load(&ruh[a0 + q * nx]) : (T)0;
+    hx[q] = in ? __builtin_nontemporal_load(&hh[a0 + q * nx]) : (T)0;
+    hq[q] = in ? __builtin_nontemporal_load(&hp[a0 + q * nx]) : (T)0;
   }
 #pragma unroll
   for (int q = 0; q < CH; q++)
@@ -569,8 +583,8 @@ __device__ __forceinline__ void yfuse_u(const Ctx& c, const T* tb, unsigned b, u
       const T v2 = (J == g.n2m) ? (T)0 : w[q + 1];
       const T v1 = (J == 1) ? (T)0 : w[q];
       const T h = hx[q] + (T)t[8] * (v2 * u2 - v1 * u1);
-      hh[a0 + q * nx] = h;
-      const T r = r0 - (T)(3. / 2.) * h + (T)(1. / 2.) * hq[q] + ((T)t[10] * bcw + (T)t[9] * bcu);
+      __builtin_nontemporal_store(h, &hh[a0 + q * nx]);
+      const T r = r0 - (T)(3. / 2.) * h + (T)(1. / 2.) * hq[q] + wall_bc<T>(c, F.tauw, t2, J, t);
       v[q] = (T)t[YT_BET] * (r * dt);
     }
 }
@@ -594,9 +608,9 @@ __device__ __forceinline__ void yfuse_v(const Ctx& c, const T* tb, unsigned b, c
 #pragma unroll
   for (int q = 0; q < CH; q++) {
     const bool in = q < y.len;
-    ru[q] = in ? ruh[a0 + q * nx] : (T)0;
-    hx[q] = in ? hh[a0 + q * nx] : (T)0;
-    hq[q] = in ? hp[a0 + q * nx] : (T)0;
+    ru[q] = in ? __builtin_nontemporal_load(&ruh[a0 + q * nx]) : (T)0;
+    hx[q] = in ? __builtin_nontemporal_load(&hh[a0 + q * nx]) : (T)0;
+    hq[q] = in ? __builtin_nontemporal_load(&hp[a0 + q * nx]) : (T)0;
   }
 #pragma unroll
   for (int q = 0; q < CH; q++)
@@ -610,7 +624,7 @@ __device__ __forceinline__ void yfuse_v(const Ctx& c, const T* tb, unsigned b, c
       const T v2 = (T)0.5 * (vp + v0);
       const T v1 = (T)0.5 * (v0 + vm);
       const T h = hx[q] + (T)t[4] * (v2 * v2 - v1 * v1);
-      hh[a0 + q * nx] = h;
+      __builtin_nontemporal_store(h, &hh[a0 + q * nx]);
       const T r = r0 - (T)(3. / 2.) * h + (T)(1. / 2.) * hq[q];
       v[q] = (T)t[YT_BET] * (r * dt);
     }
@@ -629,8 +643,7 @@ __device__ __forceinline__ void yfuse_w(const Ctx& c, const T* tb, unsigned b, u
   const T rinv = (T)c.s.rinv, dt = (T)c.s.dt;
   const unsigned nx = g.nx;
   const T wN2 = sw[b + g.n2 * nx], wW = sw[b];
-  const T bcw = -(T)F.taub[t2] / (T)c.m.dy[1] * (T)c.s.ifwall;
-  const T bcu = -(T)F.taub[(unsigned)(g.nx * g.nz) + t2] / (T)c.m.dy[g.n2m] * (T)c.s.ifwall * (T)(c.s.aly - 1.);
+
   const unsigned a0 = b + (unsigned)y.j0 * nx;
   T ws[CH + 2], s3[CH + 1], ru[CH], hx[CH], hq[CH];
 #pragma unroll
@@ -640,9 +653,9 @@ __device__ __forceinline__ void yfuse_w(const Ctx& c, const T* tb, unsigned b, u
 #pragma unroll
   for (int q = 0; q < CH; q++) {
     const bool in = q < y.len;
-    ru[q] = in ? ruh[a0 + q * nx] : (T)0;
-    hx[q] = in ? hh[a0 + q * nx] : (T)0;
-    hq[q] = in ? hp[a0 + q * nx] : (T)0;
+    ru[q] = in ? __builtin_nontemporal_load(&ruh[a0 + q * nx]) : (T)0;
+    hx[q] = in ? __builtin_nontemporal_load(&hh[a0 + q * nx]) : (T)0;
+    hq[q] = in ? __builtin_nontemporal_load(&hp[a0 + q * nx]) : (T)0;
   }
 #pragma unroll
   for (int q = 0; q < CH; q++)
@@ -656,8 +669,8 @@ __device__ __forceinline__ void yfuse_w(const Ctx& c, const T* tb, unsigned b, u
       const T v2 = (J == g.n2m) ? (T)0 : s3[q + 1];
       const T v1 = (J == 1) ? (T)0 : s3[q];
       const T h = hx[q] + (T)t[8] * (v2 * w2 - v1 * w1);
-      hh[a0 + q * nx] = h;
-      const T r = r0 - (T)(3. / 2.) * h + (T)(1. / 2.) * hq[q] + ((T)t[10] * bcw + (T)t[9] * bcu);
+      __builtin_nontemporal_store(h, &hh[a0 + q * nx]);
+      const T r = r0 - (T)(3. / 2.) * h + (T)(1. / 2.) * hq[q] + wall_bc<T>(c, F.taub, t2, J, t);
       v[q] = (T)t[YT_BET] * (r * dt);
     }
 }

@@ -62,11 +62,14 @@ __device__ __forceinline__ T yterm_u(const Ctx& c, const YIdx& x, int i, int j, 
     hh = (T)hd;
   }
   ((T*)F.hh[0])[x.b] = hh;
-  // wall stresses of the bottom and the top wall (:6765-6768); the top
-  // layer of TAUW is zero unless a two-wall model set it
-  const T bc_down = -(T)F.tauw[x.t2] / (T)M.dy[1] * (T)s.ifwall;
-  const T bc_up = -(T)F.tauw[(unsigned)(g.nx * g.nz) + x.t2] / (T)M.dy[g.n2m] * (T)s.ifwall * (T)(s.aly - 1.);
-  const T bc = (T)(1. - fjum) * bc_down + (T)(1. - fjup) * bc_up;
+  // wall stresses of the bottom and the top wall (:6765-6768): the flags
+  // make the term zero except at J = 1 and J = N2M; the top layer of TAUW is
+  // zero unless a two-wall model set it
+  T bc = (T)0;
+  if (j == 1) bc = (T)(1. - fjum) * (-(T)F.tauw[x.t2] / (T)M.dy[1] * (T)s.ifwall);
+  if (j == g.n2m)
+    bc = bc + (T)(1. - fjup) * (-(T)F.tauw[(unsigned)(g.nx * g.nz) + x.t2] / (T)M.dy[g.n2m] * (T)s.ifwall *
+                               (T)(s.aly - 1.));
   return ru - (T)(3. / 2.) * hh + (T)(1. / 2.) * ((const T*)F.hhp[0])[x.b] + bc;
 }
 
@@ -133,8 +136,10 @@ __device__ __forceinline__ T yterm_w(const Ctx& c, const YIdx& x, int i, int j, 
     hh = (T)hd;
   }
   ((T*)F.hh[2])[x.b] = hh;
-  const T bc_down = -(T)F.taub[x.t2] / (T)M.dy[1] * (T)s.ifwall;
-  const T bc_up = -(T)F.taub[(unsigned)(g.nx * g.nz) + x.t2] / (T)M.dy[g.n2m] * (T)s.ifwall * (T)(s.aly - 1.);
-  const T bc = (T)(1. - fjum) * bc_down + (T)(1. - fjup) * bc_up;
+  T bc = (T)0;
+  if (j == 1) bc = (T)(1. - fjum) * (-(T)F.taub[x.t2] / (T)M.dy[1] * (T)s.ifwall);
+  if (j == g.n2m)
+    bc = bc + (T)(1. - fjup) * (-(T)F.taub[(unsigned)(g.nx * g.nz) + x.t2] / (T)M.dy[g.n2m] * (T)s.ifwall *
+                               (T)(s.aly - 1.));
   return ru - (T)(3. / 2.) * hh + (T)(1. / 2.) * ((const T*)F.hhp[2])[x.b] + bc;
 }
