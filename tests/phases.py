@@ -1,0 +1,83 @@
+"""Phase-by-phase parity of one GETUP step (the GPU through the C-ABI
+against the oracle), shared by the GPU tests."""
+from cases import inputs, interior_p, interior_u, relerr
+
+
+def check_phases(cfg, seed=0):
+    """one GETUP phase by phase against the oracle: BCOND, RHS1-3 (RUH, HH),
+    GETUH1-3, RHSDP, GETDP, UPCALC, DIVCHECK, CFL, CHKMF"""
+    from compact3d_amd import c3d
+    from pyoracle import Oracle
+    n1, n2, n3 = cfg["n1"], cfg["n2"], cfg["n3"]
+    orc = Oracle(**cfg)
+    ch = c3d.Channel(**cfg)
+    U, P, HH2 = inputs(orc, seed=seed)
+    presg, presg3 = 0.003, 0.002
+    ch.upload(U, P, HH=HH2, presg=presg, presg3=presg3)
+    # oracle, following GETUP
+    Uo = U.copy(order="F")
+    So = orc.zeros_u(0)
+    orc.bcond(Uo, So)
+    ch.bcond()
+    st = ch.download()
+    err = relerr(Uo, st["U"])
+    print(f"BCOND {err:.3e}")
+    assert err == 0.0
+    HHo = orc.zeros_u()
+    RUH = [orc.zeros_u(0) for _ in range(3)]
+    for c in (1, 2, 3):
+        orc.rhs(c, Uo, P, HHo, HH2, RUH[c - 1], presg if c == 1 else presg3, So)
+    ch.rhs()
+    for c in (1, 2, 3):
+        j0 = 2 if c == 2 else 1
+        e1 = relerr(interior_u(RUH[c - 1], n1, n2, n3, j0=j0),
+                    interior_u(ch.field(c3d.F_RUH1 + c - 1), n1, n2, n3, j0=j0))
+        e2 = relerr(interior_u(HHo, n1, n2, n3, c - 1, j0=j0),
+                    interior_u(ch.field(c3d.F_HH1 + c - 1), n1, n2, n3, j0=j0))
+        print(f"RHS{c} {e1:.3e}  HH{c} {e2:.3e}")
+        assert e1 <= 1e-12 and e2 <= 1e-12
+    UHo = orc.zeros_u()
+    for c in (1, 2, 3):
+        orc.getuh(c, Uo, UHo, RUH[c - 1])
+    ch.getuh()
+    for c in (1, 2, 3):
+        j0 = 2 if c == 2 else 1
+        e = relerr(interior_u(UHo, n1, n2, n3, c - 1, j0=j0),
+                   interior_u(ch.field(c3d.F_UH1 + c - 1), n1, n2, n3, j0=j0))
+        print(f"GETUH{c} {e:.3e}")
+        assert e <= 1e-12
+    RDPo = orc.zeros_p()
+    DPo = orc.zeros_p()
+    orc.rhsdp(RDPo, UHo)
+    orc.getdp(DPo, RDPo)
+    ch.dpcalc()
+    e_rdp = relerr(interior_p(RDPo, n1, n2, n3), interior_p(ch.field(c3d.F_RDP), n1, n2, n3))
+    e_dp = relerr(interior_p(DPo, n1, n2, n3), interior_p(ch.field(c3d.F_DP), n1, n2, n3))
+    print(f"RHSDP {e_rdp:.3e}  GETDP {e_dp:.3e}")
+    assert e_rdp <= 1e-12 and e_dp <= 1e-11
+    Po = P.copy(order="F")
+    pg, pg3 = orc.upcalc(Uo, Po, UHo, DPo, presg, presg3)
+    ch.upcalc()
+    st = ch.download()
+    eu = [relerr(interior_u(Uo, n1, n2, n3, c, j0=(2 if c == 1 else 1)),
+                 interior_u(st["U"], n1, n2, n3, c, j0=(2 if c == 1 else 1))) for c in range(3)]
+    ep = relerr(interior_p(Po, n1, n2, n3), interior_p(st["P"], n1, n2, n3))
+    print("UPCALC U", ["%.3e" % x for x in eu], f"P {ep:.3e}", "PRESG", pg, st["PRESG"], pg3, st["PRESG3"])
+    assert max(eu) <= 1e-12 and ep <= 1e-11
+    # the ghost planes follow UBC
+    assert relerr(Uo, st["U"]) <= 1e-12
+    dmo, dmg = orc.divcheck(Uo), ch.divcheck()
+    cfo, cfg_ = orc.cfl(Uo), ch.cfl()
+    print("DIVCHECK", dmo, dmg, "CFL", cfo, cfg_)
+    assert abs(cfo - cfg_) <= 1e-12 * cfo
+    assert dmg <= 1e-10  # post-projection divergence is at roundoff
+    f1o, f3o = orc.chkmf(Uo)
+    f1g, f3g = ch.chkmf()
+    print("CHKMF", f1o, f1g, f3o, f3g)
+    # CHKMF's sums run over one plane (N2M*N3M and N1M*N2M terms): the
+    # oracle's serial sum carries up to (terms)*eps of rounding, which at
+    # 192x129x192 exceeds 1e-12
+    t1 = max(1e-12, (n2 - 1) * (n3 - 1) * 2.2e-16)
+    t3 = max(1e-12, (n1 - 1) * (n2 - 1) * 2.2e-16)
+    assert abs(f1o - f1g) <= t1 * abs(f1o) and abs(f3o - f3g) <= t3 * max(abs(f3o), 1e-300) + 1e-15
+    ch.close()

@@ -26,74 +26,9 @@ def _mk(cfg):
 
 @pytest.mark.parametrize("grid", [REF_GRID, (49, 41, 33)])
 def test_phases_match_oracle(grid):
+    from phases import check_phases
     n1, n2, n3 = grid
-    cfg = config(n1, n2, n3)
-    orc, ch, c3d = _mk(cfg)
-    U, P, HH2 = inputs(orc, seed=0)
-    presg, presg3 = 0.003, 0.002
-    ch.upload(U, P, HH=HH2, presg=presg, presg3=presg3)
-    # oracle, following GETUP
-    Uo = U.copy(order="F")
-    So = orc.zeros_u(0)
-    orc.bcond(Uo, So)
-    ch.bcond()
-    st = ch.download()
-    err = relerr(Uo, st["U"])
-    print(f"BCOND {err:.3e}")
-    assert err == 0.0
-    HHo = orc.zeros_u()
-    RUH = [orc.zeros_u(0) for _ in range(3)]
-    for c in (1, 2, 3):
-        orc.rhs(c, Uo, P, HHo, HH2, RUH[c - 1], presg if c == 1 else presg3, So)
-    ch.rhs()
-    for c in (1, 2, 3):
-        j0 = 2 if c == 2 else 1
-        e1 = relerr(interior_u(RUH[c - 1], n1, n2, n3, j0=j0),
-                    interior_u(ch.field(c3d.F_RUH1 + c - 1), n1, n2, n3, j0=j0))
-        e2 = relerr(interior_u(HHo, n1, n2, n3, c - 1, j0=j0),
-                    interior_u(ch.field(c3d.F_HH1 + c - 1), n1, n2, n3, j0=j0))
-        print(f"RHS{c} {e1:.3e}  HH{c} {e2:.3e}")
-        assert e1 <= 1e-12 and e2 <= 1e-12
-    UHo = orc.zeros_u()
-    for c in (1, 2, 3):
-        orc.getuh(c, Uo, UHo, RUH[c - 1])
-    ch.getuh()
-    for c in (1, 2, 3):
-        j0 = 2 if c == 2 else 1
-        e = relerr(interior_u(UHo, n1, n2, n3, c - 1, j0=j0),
-                   interior_u(ch.field(c3d.F_UH1 + c - 1), n1, n2, n3, j0=j0))
-        print(f"GETUH{c} {e:.3e}")
-        assert e <= 1e-12
-    RDPo = orc.zeros_p()
-    DPo = orc.zeros_p()
-    orc.rhsdp(RDPo, UHo)
-    orc.getdp(DPo, RDPo)
-    ch.dpcalc()
-    e_rdp = relerr(interior_p(RDPo, n1, n2, n3), interior_p(ch.field(c3d.F_RDP), n1, n2, n3))
-    e_dp = relerr(interior_p(DPo, n1, n2, n3), interior_p(ch.field(c3d.F_DP), n1, n2, n3))
-    print(f"RHSDP {e_rdp:.3e}  GETDP {e_dp:.3e}")
-    assert e_rdp <= 1e-12 and e_dp <= 1e-11
-    Po = P.copy(order="F")
-    pg, pg3 = orc.upcalc(Uo, Po, UHo, DPo, presg, presg3)
-    ch.upcalc()
-    st = ch.download()
-    eu = [relerr(interior_u(Uo, n1, n2, n3, c, j0=(2 if c == 1 else 1)),
-                 interior_u(st["U"], n1, n2, n3, c, j0=(2 if c == 1 else 1))) for c in range(3)]
-    ep = relerr(interior_p(Po, n1, n2, n3), interior_p(st["P"], n1, n2, n3))
-    print("UPCALC U", ["%.3e" % x for x in eu], f"P {ep:.3e}", "PRESG", pg, st["PRESG"], pg3, st["PRESG3"])
-    assert max(eu) <= 1e-12 and ep <= 1e-11
-    # the ghost planes follow UBC
-    assert relerr(Uo, st["U"]) <= 1e-12
-    dmo, dmg = orc.divcheck(Uo), ch.divcheck()
-    cfo, cfg_ = orc.cfl(Uo), ch.cfl()
-    print("DIVCHECK", dmo, dmg, "CFL", cfo, cfg_)
-    assert abs(cfo - cfg_) <= 1e-12 * cfo
-    assert dmg <= 1e-10  # post-projection divergence is at roundoff
-    f1o, f3o = orc.chkmf(Uo)
-    f1g, f3g = ch.chkmf()
-    print("CHKMF", f1o, f1g, f3o, f3g)
-    assert abs(f1o - f1g) <= 1e-12 * abs(f1o) and abs(f3o - f3g) <= 1e-12 * max(abs(f3o), 1e-300) + 1e-15
-    ch.close()
+    check_phases(config(n1, n2, n3), seed=0)
 
 
 def test_trajectory_10_steps():
