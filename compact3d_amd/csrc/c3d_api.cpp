@@ -1191,7 +1191,7 @@ static std::vector<LShape> cand_lz(int n, bool fp32) {
 static std::vector<YShape> cand_y(int m, int row, int ntmax) {
   std::vector<YShape> v;
   for (int ch : {6, 12})
-    for (int nt : {256, 512}) {
+    for (int nt : {256, 512, 1024}) {
       if (nt > ntmax) continue;
       const YShape y = make_yshape(m, nt, row, ch);
       if (y.L * y.C > ntmax || (m + y.C - 1) / y.C > 12) continue;
@@ -1278,9 +1278,9 @@ extern "C" int c3d_autotune(const c3d_config* cfg, int steps, int* shapes, doubl
     }
   };
   if (mixed)
-    knob(cand_y(g.n2m, 32, 512), [](Shapes& s, const YShape& y) { s.ys32 = y; });
+    knob(cand_y(g.n2m, 32, 1024), [](Shapes& s, const YShape& y) { s.ys32 = y; });
   else
-    knob(cand_y(g.n2m, 16, 512), [](Shapes& s, const YShape& y) { s.ys = y; });
+    knob(cand_y(g.n2m, 16, 1024), [](Shapes& s, const YShape& y) { s.ys = y; });
   if (mixed) knob(cand_lz(g.nz, true), [](Shapes& s, const LShape& z) { s.lz32 = z; });
   knob(cand_lz(g.nz, false), [](Shapes& s, const LShape& z) { s.lz = z; });
   knob(cand_lx(g.nx), [](Shapes& s, const LShape& x) { s.lx = x; });

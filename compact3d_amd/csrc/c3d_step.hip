@@ -473,9 +473,9 @@ __device__ __forceinline__ void ychain_bwd(T (&v)[N], MF m, const YPos& y, int C
     }
 }
 
-template <typename T>
-__global__ __launch_bounds__(512) void k_adi_y(Ctx c) {
-  __shared__ T xs[4][512];  // C*L <= 512 entries each
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_adi_y(Ctx c) {
+  __shared__ T xs[4][NT];  // C*L <= NT entries each
   const Geo g = c.g;
   const YShape ys = sizeof(T) == 4 ? c.o.ys32 : c.o.ys;
   const int comp = blockIdx.y;
@@ -675,9 +675,10 @@ __device__ __forceinline__ void yfuse_w(const Ctx& c, const T* tb, unsigned b, u
     }
 }
 
-template <typename T, bool SGS, int CH>
-__global__ __launch_bounds__(512) void k_rhsy_adiy(Ctx c, int staged) {
-  __shared__ T xs[4][512];  // C*L <= 512 entries each
+// NT: the block size bound, 512 or 1024 (tiles of L*C threads)
+template <typename T, bool SGS, int CH, int NT>
+__global__ __launch_bounds__(NT) void k_rhsy_adiy(Ctx c, int staged) {
+  __shared__ T xs[4][NT];  // C*L <= NT entries each
   extern __shared__ __align__(16) unsigned char smem[];
   const Geo g = c.g;
   const YShape ys = sizeof(T) == 4 ? c.o.ys32 : c.o.ys;
@@ -1569,24 +1570,39 @@ template <typename T>
 static void getuh_t(const Ctx& c, bool fused) {
   const YShape& ys = sizeof(T) == 4 ? c.o.ys32 : c.o.ys;
   const int ylx = (c.g.nx + ys.L - 1) / ys.L;
+  // tiles of more than 512 threads run the NT = 1024 instances (rows of
+  // 256 bytes: 64 FP32 or 32 FP64 columns)
+  const bool big = ys.L * ys.C > 512;
   if (fused) {
     const size_t tbytes = sizeof(T) * (size_t)(c.g.n2 + 2) * YTAB_W;
-    const int staged = tbytes <= 40 * 1024 ? 1 : 0;
+    // the coefficient table goes to LDS when it fits beside the chain arrays
+    // within 64 KB
+    const size_t xsb = 4 * sizeof(T) * (big ? 1024 : 512);
+    const int staged = tbytes + xsb <= 64 * 1024 && tbytes <= 40 * 1024 ? 1 : 0;
     // chunk length of the shape (u and w have the most points, N2M)
     const int ch = (c.g.n2m + ys.C - 1) / ys.C;
     const dim3 gr(ylx * c.g.nz, 3), th(ys.L * ys.C);
     const size_t lb = staged ? tbytes : 0;
     KScope ks_(c, KID_ADI_Y);
-#define YF_(SGS, CH) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rhsy_adiy<T, SGS, CH>), gr, th, lb, c.st, c, staged)
+#define YF_(SGS, CH, NT) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rhsy_adiy<T, SGS, CH, NT>), gr, th, lb, c.st, c, staged)
+#define YFN_(SGS, CH) \
+  if (big)            \
+    YF_(SGS, CH, 1024); \
+  else                \
+    YF_(SGS, CH, 512)
     if (c.s.files) {
-      if (ch <= 6) YF_(true, 6); else YF_(true, CHMAX);
+      if (ch <= 6) { YFN_(true, 6); } else { YFN_(true, CHMAX); }
     } else {
-      if (ch <= 6) YF_(false, 6); else YF_(false, CHMAX);
+      if (ch <= 6) { YFN_(false, 6); } else { YFN_(false, CHMAX); }
     }
+#undef YFN_
 #undef YF_
   } else {
     KScope ks_(c, KID_ADI_Y);
-    hipLaunchKernelGGL(k_adi_y<T>, dim3(ylx * c.g.nz, 3), dim3(ys.L * ys.C), 0, c.st, c);
+    if (big)
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adi_y<T, 1024>), dim3(ylx * c.g.nz, 3), dim3(ys.L * ys.C), 0, c.st, c);
+    else
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adi_y<T, 512>), dim3(ylx * c.g.nz, 3), dim3(ys.L * ys.C), 0, c.st, c);
   }
   const size_t lx = lds_line(c.o.lx, true, sizeof(T));
   // GETUH1/2: y, x, z;  GETUH3: y, z, x (compact3d.f:8508-8607)
