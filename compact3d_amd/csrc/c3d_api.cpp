@@ -1201,10 +1201,12 @@ static std::vector<YShape> cand_y(int m, int row, int ntmax) {
     }
   return v;
 }
+// LDS a workgroup can address on gfx950 (160 KB)
+static const size_t kLdsMax = 160 * 1024;
 static std::vector<int> cand_tl(int n, bool padded) {
   std::vector<int> v;
   for (int t = 1; t <= 16; t *= 2)
-    if (n * t <= 4096 && 16 * (size_t)n * (t + (padded ? 2 : 1)) <= 65536) v.push_back(t);
+    if (n * t <= 4096 && 16 * (size_t)n * (t + (padded ? 2 : 1)) <= kLdsMax) v.push_back(t);
   return v;
 }
 
@@ -1286,10 +1288,13 @@ extern "C" int c3d_autotune(const c3d_config* cfg, int steps, int* shapes, doubl
   knob(cand_lx(g.nx), [](Shapes& s, const LShape& x) { s.lx = x; });
   std::vector<YShape> yp;
   for (int ch : {6, 12})
-    for (int row : {8, 16}) {
-      const YShape y = make_yshape(g.n2m, 256, row, ch);
-      if (y.L * y.C <= 256 && (g.n2m + y.C - 1) / y.C <= 12) yp.push_back(y);
-    }
+    for (int nt : {256, 512})
+      for (int row : {8, 16}) {
+        const YShape y = make_yshape(g.n2m, nt, row, ch);
+        bool dup = false;
+        for (const YShape& w : yp) dup = dup || (w.L == y.L && w.C == y.C);
+        if (!dup && y.L * y.C <= nt && (g.n2m + y.C - 1) / y.C <= 12) yp.push_back(y);
+      }
   knob(yp, [](Shapes& s, const YShape& y) { s.yp = y; });
   knob(cand_tl(g.nx, true), [](Shapes& s, int t) { s.tlx = t; });
   knob(cand_tl(g.nz, false), [](Shapes& s, int t) { s.tlz = t; });

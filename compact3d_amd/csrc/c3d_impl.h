@@ -208,6 +208,7 @@ void wallmodel_werner(const Ctx& c);  // WALLMODEL_WERNER (WMODEL = 4), both wal
 void wallss(const Ctx& c);
 void sgs_terms(const Ctx& c);
 void init_attrs();
+void fft_attrs();  // LDS limits of the FFT kernels
 void filter(const Ctx& c, int width, const double* in, double* out, int nplanes, int ld1, int ld3);
 void apply_op(const Ctx& c, int op, int dir, const double* in, double* out);
 void zero(const Ctx& c, double* p, long long n);

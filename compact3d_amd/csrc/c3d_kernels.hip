@@ -451,7 +451,8 @@ static int zgrid(const Ctx& c) { return ((c.g.nx + c.o.tz.nl - 1) / c.o.tz.nl) *
 static int pgrid(long long n) { return (int)((n + C3D_NT - 1) / C3D_NT); }
 
 void init_attrs() {
-  // every kernel keeps its dynamic LDS within 64 KB; clear any stale error
+  fft_attrs();
+  // clear any stale error
   (void)hipGetLastError();
 }
 

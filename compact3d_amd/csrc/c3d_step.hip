@@ -1077,9 +1077,9 @@ __global__ __launch_bounds__(NT) void k_fft_z(Ctx c, int sign, void* spec) {
 // kx = kx0 + l by the partitioned recurrences.
 // Spectrum in TI, recurrences and result in TO (FP64 in both modes); chunks
 // of at most CH points.
-template <typename TI, typename TO, int CH>
-__global__ __launch_bounds__(256) void k_pois_y(Ctx c, const void* spec_in, void* spec_out) {
-  __shared__ TO xs[8][256];
+template <typename TI, typename TO, int CH, int NT>
+__global__ __launch_bounds__(NT) void k_pois_y(Ctx c, const void* spec_in, void* spec_out) {
+  __shared__ TO xs[8][NT];
   typedef cplx_t<TI> zi;
   typedef cplx_t<TO> zo;
   const Geo g = c.g;
@@ -1669,19 +1669,38 @@ static void getdp_fp64(const Ctx& c) {
   { KScope ks_(c, KID_FFT_X); FFTW_(k_fft_x, ex, nx_t, dim3(gx), dim3(nx_t), ldsx, c.st, c); }
   { KScope ks_(c, KID_FFT_Z); FFTW_(k_fft_z, ez, nz_t, dim3(gz), dim3(nz_t), ldsz, c.st, c, -1, sp); }
   {
+    // tiles of up to 256 or 512 threads (16 modes: spectrum rows of 256 bytes)
     KScope ks_(c, KID_POIS_Y);
-    if ((g.n2m + c.o.yp.C - 1) / c.o.yp.C <= 6)
-      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pois_y<T, T, 6>), dim3(gy), dim3(c.o.yp.L * c.o.yp.C), 0, c.st, c,
-                         (const void*)sp, sp);
-    else
-      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pois_y<T, T, CHMAX>), dim3(gy), dim3(c.o.yp.L * c.o.yp.C), 0, c.st,
-                         c, (const void*)sp, sp);
+    const int npt = c.o.yp.L * c.o.yp.C;
+    const bool six = (g.n2m + c.o.yp.C - 1) / c.o.yp.C <= 6;
+#define PY_(CH, NT) \
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pois_y<T, T, CH, NT>), dim3(gy), dim3(npt), 0, c.st, c, (const void*)sp, sp)
+    if (npt > 256) {
+      if (six) PY_(6, 512); else PY_(CHMAX, 512);
+    } else {
+      if (six) PY_(6, 256); else PY_(CHMAX, 256);
+    }
+#undef PY_
   }
   { KScope ks_(c, KID_FFT_Z); FFTW_(k_fft_z, ez, nz_t, dim3(gz), dim3(nz_t), ldsz, c.st, c, +1, sp); }
   { KScope ks_(c, KID_FFT_XINV); FFTW_(k_fft_xinv, ex, nx_t, dim3(gx), dim3(nx_t), ldsx, c.st, c, (const void*)sp); }
 #undef FFTW_
 }
 void getdp(const Ctx& c) { getdp_fp64(c); }
+
+// FFT tiles may use more than 64 KB of LDS (up to the 160 KB a workgroup
+// can address on gfx950)
+void fft_attrs() {
+  const int lim = 160 * 1024;
+#define FA_(K)                                                                                                   \
+  (void)hipFuncSetAttribute((const void*)&K<double, 4096, 512>, hipFuncAttributeMaxDynamicSharedMemorySize, lim); \
+  (void)hipFuncSetAttribute((const void*)&K<double, 2048, 512>, hipFuncAttributeMaxDynamicSharedMemorySize, lim); \
+  (void)hipFuncSetAttribute((const void*)&K<double, 1024, 256>, hipFuncAttributeMaxDynamicSharedMemorySize, lim)
+  FA_(k_fft_x);
+  FA_(k_fft_z);
+  FA_(k_fft_xinv);
+#undef FA_
+}
 
 void pois_table(const Ctx& c) {
   const long long nm = (long long)c.g.nxh * c.g.nz;
