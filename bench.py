@@ -363,6 +363,9 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=16)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched LES ensemble leg")
     ap.add_argument("--no-autotune", action="store_true", help="keep the rule-derived tile shapes")
+    ap.add_argument("--shapes", metavar="JSON",
+                    help="tile shapes of the main legs from a file ({'fp64': {...}, 'mixed': {...}}, the 'autotune' "
+                         "field of an earlier run) instead of autotuning them; for profiling the same kernels")
     ap.add_argument("--ens-members", type=int, default=48)
     ap.add_argument("--ens-steps", type=int, default=200)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -405,7 +408,14 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     tune = {}
-    if not args.no_autotune:
+    if args.shapes:
+        with open(args.shapes) as f:
+            given = json.load(f)
+        for name, prec in (("fp64", c3d.FP64), ("mixed", c3d.MIXED)):
+            c3d.autotune_set(given[name], **cfg, precision=prec)
+            tune[name] = {k: given[name][k] for k in c3d.SHAPE_KEYS}
+            tune[name]["from"] = args.shapes
+    elif not args.no_autotune:
         for name, prec in (("fp64", c3d.FP64), ("mixed", c3d.MIXED)):
             t = c3d.autotune(steps=4, **cfg, precision=prec)
             tune[name] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in t.items()}

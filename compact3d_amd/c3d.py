@@ -86,6 +86,7 @@ def lib():
         L.c3d_filter.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int]
         L.c3d_timing.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.c3d_autotune.argtypes = [C.POINTER(Config), C.c_int, C.POINTER(C.c_int), _dp]
+        L.c3d_autotune_set.argtypes = [C.POINTER(Config), C.POINTER(C.c_int)]
         L.c3d_autotune_reset.argtypes = []
         L.c3d_autotune_reset.restype = None
         L.c3d_ensemble_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]
@@ -166,6 +167,15 @@ def autotune(steps=4, **kw):
     out = dict(zip(SHAPE_KEYS, list(sh)))
     out.update(ms_rule=float(ms[0]), ms_tuned=float(ms[1]))
     return out
+
+
+def autotune_set(shapes, **kw):
+    """c3d_autotune_set: record the given shapes (a dict with SHAPE_KEYS, as
+    autotune() returns) for this configuration without timing"""
+    L = lib()
+    sh = (C.c_int * 12)(*[int(shapes[k]) for k in SHAPE_KEYS])
+    if L.c3d_autotune_set(C.byref(make_config(**kw)), sh) != 0:
+        raise C3DError("c3d_autotune_set: " + L.c3d_last_error(None).decode())
 
 
 def autotune_reset():

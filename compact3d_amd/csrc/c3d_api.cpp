@@ -1201,6 +1201,19 @@ static std::vector<YShape> cand_y(int m, int row, int ntmax) {
     }
   return v;
 }
+// Poisson y-solve tiles: <= 12-point chunks, 8 or 16 modes, <= 512 threads
+static std::vector<YShape> cand_yp(int m) {
+  std::vector<YShape> yp;
+  for (int ch : {6, 12})
+    for (int nt : {256, 512})
+      for (int row : {8, 16}) {
+        const YShape y = make_yshape(m, nt, row, ch);
+        bool dup = false;
+        for (const YShape& w : yp) dup = dup || (w.L == y.L && w.C == y.C);
+        if (!dup && y.L * y.C <= nt && (m + y.C - 1) / y.C <= 12) yp.push_back(y);
+      }
+  return yp;
+}
 // LDS a workgroup can address on gfx950 (160 KB)
 static const size_t kLdsMax = 160 * 1024;
 static std::vector<int> cand_tl(int n, bool padded) {
@@ -1286,16 +1299,7 @@ extern "C" int c3d_autotune(const c3d_config* cfg, int steps, int* shapes, doubl
   if (mixed) knob(cand_lz(g.nz, true), [](Shapes& s, const LShape& z) { s.lz32 = z; });
   knob(cand_lz(g.nz, false), [](Shapes& s, const LShape& z) { s.lz = z; });
   knob(cand_lx(g.nx), [](Shapes& s, const LShape& x) { s.lx = x; });
-  std::vector<YShape> yp;
-  for (int ch : {6, 12})
-    for (int nt : {256, 512})
-      for (int row : {8, 16}) {
-        const YShape y = make_yshape(g.n2m, nt, row, ch);
-        bool dup = false;
-        for (const YShape& w : yp) dup = dup || (w.L == y.L && w.C == y.C);
-        if (!dup && y.L * y.C <= nt && (g.n2m + y.C - 1) / y.C <= 12) yp.push_back(y);
-      }
-  knob(yp, [](Shapes& s, const YShape& y) { s.yp = y; });
+  knob(cand_yp(g.n2m), [](Shapes& s, const YShape& y) { s.yp = y; });
   knob(cand_tl(g.nx, true), [](Shapes& s, int t) { s.tlx = t; });
   knob(cand_tl(g.nz, false), [](Shapes& s, int t) { s.tlz = t; });
   c3d_destroy(h);
@@ -1316,6 +1320,66 @@ extern "C" int c3d_autotune(const c3d_config* cfg, int steps, int* shapes, doubl
     ms[0] = t0;
     ms[1] = tb;
   }
+  return 0;
+}
+
+// Registers given tile shapes (c3d_autotune's 12-value layout) for a
+// configuration, as if the autotuner had chosen them, after checking that
+// each is the rule-derived shape or one of the autotuner's candidates.
+extern "C" int c3d_autotune_set(const c3d_config* cfg, const int* v) {
+  g_create_err.clear();
+  if (!cfg || !v) return -1;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned.erase(tune_key(*cfg));
+  }
+  c3d_handle* h = nullptr;
+  if (c3d_create(cfg, &h)) return -1;
+  Shapes s = get_shapes(h);
+  const Geo g = h->c.g;
+  c3d_destroy(h);
+  bool ok = true;
+  auto lsh = [&](std::vector<LShape> c, const LShape& rule, int L, int seg, LShape& out) {
+    c.push_back(rule);
+    for (const LShape& x : c)
+      if (x.L == L && (seg <= 0 || x.seg == seg)) {
+        out = x;
+        return;
+      }
+    ok = false;
+  };
+  auto ysh = [&](std::vector<YShape> c, const YShape& rule, int L, int C, YShape& out) {
+    c.push_back(rule);
+    for (const YShape& x : c)
+      if (x.L == L && x.C == C) {
+        out = x;
+        return;
+      }
+    ok = false;
+  };
+  auto tl = [&](std::vector<int> c, int rule, int t, int& out) {
+    c.push_back(rule);
+    for (int x : c)
+      if (x == t) {
+        out = x;
+        return;
+      }
+    ok = false;
+  };
+  lsh(cand_lx(g.nx), s.lx, v[0], 0, s.lx);
+  lsh(cand_lz(g.nz, false), s.lz, v[1], 0, s.lz);
+  lsh(cand_lz(g.nz, true), s.lz32, v[2], v[3], s.lz32);
+  ysh(cand_y(g.n2m, 16, 1024), s.ys, v[4], v[5], s.ys);
+  ysh(cand_y(g.n2m, 32, 1024), s.ys32, v[6], v[7], s.ys32);
+  ysh(cand_yp(g.n2m), s.yp, v[8], v[9], s.yp);
+  tl(cand_tl(g.nx, true), s.tlx, v[10], s.tlx);
+  tl(cand_tl(g.nz, false), s.tlz, v[11], s.tlz);
+  if (!ok) {
+    g_create_err = "c3d_autotune_set: a shape outside the candidate sets of this configuration";
+    return -1;
+  }
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tuned[tune_key(*cfg)] = s;
   return 0;
 }
 

@@ -225,6 +225,11 @@ const char *c3d_kernel_name(int id);
  * columns and chunks, x- and z-FFT lines per tile; ms (2, may be NULL):
  * ms per step with the rule-derived and with the chosen shapes. */
 int c3d_autotune(const c3d_config *cfg, int steps, int *shapes, double *ms);
+/* records given shapes (c3d_autotune's 12-value layout, e.g. from an
+ * earlier c3d_autotune) for this configuration without timing anything;
+ * each must be the rule-derived shape or one of the autotuner's candidates,
+ * else the call fails (c3d_last_error(NULL)) and records nothing */
+int c3d_autotune_set(const c3d_config *cfg, const int *shapes);
 /* forgets every tuned shape set */
 void c3d_autotune_reset(void);
 

@@ -71,3 +71,23 @@ def test_autotune_mixed_keeps_divergence():
         b.close()
     finally:
         c3d.autotune_reset()
+
+
+def test_autotune_set_accepts_candidates_only():
+    """c3d_autotune_set records shapes the autotuner could have chosen and
+    refuses any other (no kernel ever runs an unchecked tile shape)"""
+    from compact3d_amd import c3d
+    cfg = config(65, 49, 65, cks=1.5)
+    try:
+        t = c3d.autotune(steps=2, **cfg, precision=c3d.FP64)
+        c3d.autotune_reset()
+        c3d.autotune_set(t, **cfg, precision=c3d.FP64)
+        ch = c3d.Channel(**cfg)
+        ch.close()
+        for key, val in (("y_cols_fp64", 7), ("x_lines", 3), ("fft_z_lines", 5)):
+            bad = dict(t)
+            bad[key] = val
+            with pytest.raises(c3d.C3DError):
+                c3d.autotune_set(bad, **cfg, precision=c3d.FP64)
+    finally:
+        c3d.autotune_reset()
