@@ -521,8 +521,9 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   if (cf.precision != C3D_FP64 && cf.precision != C3D_MIXED) return bad("precision must be C3D_FP64 or C3D_MIXED");
   if (cf.files != 0.0 && cf.files != 1.0 && cf.files != 2.0 && cf.files != 3.0)
     return bad("FILES must be 0 (DNS), 1 (dynamic model), 2 (Smagorinsky) or 3 (scale-dependent dynamic model)");
-  if (cf.wmodel != 0.0 && cf.wmodel != 1.0 && cf.wmodel != 4.0)
-    return bad("WMODEL must be 0 (resolved walls), 1 (log law, bottom wall) or 4 (Werner-Wengle, both walls)");
+  if (cf.wmodel != 0.0 && cf.wmodel != 1.0 && cf.wmodel != 4.0 && cf.wmodel != 6.0)
+    return bad("WMODEL must be 0 (resolved walls), 1 (log law, bottom wall), 4 (Werner-Wengle, both walls) "
+               "or 6 (log law, both walls)");
   // WALLMODEL_WERNER takes its I neighbour from IPA(K) (compact3d.f:4849)
   if (cf.wmodel == 4.0 && cf.n3 > cf.n1) return bad("WMODEL = 4 needs N3 <= N1");
   if (cf.re <= 0 || cf.dt <= 0 || cf.alx <= 0 || cf.alz <= 0) return bad("RE, DT, ALX, ALZ must be > 0");
@@ -1017,8 +1018,8 @@ extern "C" int c3d_sgs(c3d_handle* h) {
 // WALL_MODEL (compact3d.f:1981-1991): WMODEL=0 -> WALLSS, IFWALL=0;
 // WMODEL=1 (Y0=0) -> WALLMODEL_INSTANT, IFWALL=1
 static void wall_model_impl(c3d_handle* h) {
-  if (h->cfg.wmodel == 1.0) {
-    c3dk::wallmodel(h->c);
+  if (h->cfg.wmodel == 1.0 || h->cfg.wmodel == 6.0) {
+    c3dk::wallmodel(h->c, h->cfg.wmodel == 6.0);
     h->c.s.ifwall = 1;
   } else if (h->cfg.wmodel == 4.0) {
     c3dk::wallmodel_werner(h->c);

@@ -203,7 +203,7 @@ void sgscs(const Ctx& c);
 void sgsdy(const Ctx& c);            // SGSDY, the dynamic model (FILES = 1)
 void sgssd(const Ctx& c);            // SGSSD, the scale-dependent dynamic model (FILES = 3)
 int sgsdy_tiles(const Geo& g);      // tiles per plane of its partial sums
-void wallmodel(const Ctx& c);
+void wallmodel(const Ctx& c, bool both = false);  // WALLMODEL_INSTANT; both: also the top wall (WMODEL = 6)
 void wallmodel_werner(const Ctx& c);  // WALLMODEL_WERNER (WMODEL = 4), both walls
 void wallss(const Ctx& c);
 void sgs_terms(const Ctx& c);

@@ -556,18 +556,26 @@ __global__ __launch_bounds__(256) void k_sgssd_coef(Ctx c, int ntile, const doub
 // without a bound; here the loop also stops after 10000 iterations (the
 // damped iteration converges in tens).
 #define WM_MAXIT 10000
-__global__ __launch_bounds__(C3D_NT) void k_wallmodel(Ctx c) {
+// `top` (WMODEL = 6, this build's two-wall extension): the same iteration
+// at the top wall, on the third cell from it (J = N2M-2, wall distance
+// DY(N2M) + DY(N2M-1) + DY(N2M-2)/2), into TAUW(I,K,2), TAUB(I,K,2); its
+// plane sums are not kept (WSM(1) is the bottom wall's).
+__global__ __launch_bounds__(C3D_NT) void k_wallmodel(Ctx c, int top) {
   __shared__ double red[C3D_NT];
   const Geo g = c.g;
   const Metrics M = c.m;
   const Fields F = c.d;
   const long long n = (long long)g.nx * g.nz;
   const double RE = c.s.re, KAPA = 0.41, B = 5.2;
-  const double deltap = M.dy[1] + M.dy[2] + M.dy[3] / 2.0;
+  const int jw = top ? g.n2m - 2 : 3;
+  const double deltap = top ? M.dy[g.n2m] + M.dy[g.n2m - 1] + M.dy[g.n2m - 2] / 2.0
+                            : M.dy[1] + M.dy[2] + M.dy[3] / 2.0;
+  double* TW = F.tauw + (top ? n : 0);
+  double* TB = F.taub + (top ? n : 0);
   double acc = 0.0;
   for (long long t = blockIdx.x * (long long)C3D_NT + threadIdx.x; t < n; t += (long long)gridDim.x * C3D_NT) {
     const int i = t % g.nx, k = t / g.nx;
-    const double u3 = F.u[gidx(g, i, 3, k)], w3 = F.w[gidx(g, i, 3, k)];
+    const double u3 = F.u[gidx(g, i, jw, k)], w3 = F.w[gidx(g, i, jw, k)];
     const double u_ave = sqrt(u3 * u3 + w3 * w3);
     double ut1 = 1.0, ut2 = 0.000001, f1, f2, f3, lam = 1.0;
     int it = 0;
@@ -582,9 +590,10 @@ __global__ __launch_bounds__(C3D_NT) void k_wallmodel(Ctx c) {
     }
     const double tauw1 = ut1 * ut1;
     acc += tauw1;
-    F.tauw[t] = tauw1 * u3 / u_ave;
-    F.taub[t] = tauw1 * w3 / u_ave;
+    TW[t] = tauw1 * u3 / u_ave;
+    TB[t] = tauw1 * w3 / u_ave;
   }
+  if (top) return;
   const int t = threadIdx.x;
   red[t] = acc;
   __syncthreads();
@@ -1169,14 +1178,15 @@ void sgssd(const Ctx& c) {
   hipLaunchKernelGGL(k_sgsdy_clip, dim3(pgrid((long long)c.g.nx * c.g.n2m * c.g.nz)), dim3(C3D_NT), 0, c.st, c);
 }
 
-void wallmodel(const Ctx& c) {
+void wallmodel(const Ctx& c, bool both) {
   const int nb = std::min(1024, pgrid((long long)c.g.nx * c.g.nz));
   {
     KScope ks_(c, KID_WALL);
-    hipLaunchKernelGGL(k_wallmodel, dim3(nb), dim3(C3D_NT), 0, c.st, c);
+    hipLaunchKernelGGL(k_wallmodel, dim3(nb), dim3(C3D_NT), 0, c.st, c, 0);
   }
   KScope ks_(c, KID_WALL);
   hipLaunchKernelGGL(k_wallmodel_fin, dim3(1), dim3(C3D_NT), 0, c.st, c, nb, 0);
+  if (both) hipLaunchKernelGGL(k_wallmodel, dim3(nb), dim3(C3D_NT), 0, c.st, c, 1);
 }
 
 void wallmodel_werner(const Ctx& c) {

@@ -51,7 +51,9 @@ typedef struct c3d_config {
   double files;       /* SGS model switch FILES: 0 none (DNS), 1 dynamic (SGSDY), 2 Smagorinsky (SGSCS),
                          3 scale-dependent dynamic (SGSSD) */
   double wmodel;      /* WMODEL: 0 resolved walls, 1 instantaneous log-law wall model (bottom wall),
-                         4 Werner-Wengle wall model (both walls; needs N3 <= N1) */
+                         4 Werner-Wengle wall model (both walls; needs N3 <= N1),
+                         6 the log-law model at both walls (no counterpart in the
+                         reference, whose model 1 leaves the top wall unmodelled) */
   double cks;         /* tanh stretching of y (the reference hard-codes 2.5, :338) */
   const double *y;    /* optional Y(0:N2) to use instead of the tanh grid, or NULL */
   int precision;      /* enum c3d_precision */

@@ -2344,17 +2344,18 @@ void c3o_sgssd(c3o *o, const double *U, double *SGSVIS, double *smagc_out, doubl
 
 /* ------------------------------------------------------ WALLMODEL_INSTANT */
 /* compact3d.f:2319-2493 */
-void c3o_wallmodel_instant(c3o *o, const double *U) {
+/* the log-law Newton iteration of WALLMODEL_INSTANT on row j at the wall
+ * distance deltap, into layer `layer` of TAUW/TAUB; returns the sum of
+ * tau_w over the plane */
+static double instant_wall(c3o *o, const double *U, int j, double deltap, int layer) {
   const int N1M = o->n1m, N3M = o->n3m;
   const double RE = o->re;
-  const double *DY = o->dy;
   const double *W = U + 2 * NU;
   const double KAPA = 0.41, B = 5.2;
-  const double deltap = DY[1] + DY[2] + DY[3] / 2.0;
   double wsm1 = 0.0;
   for (int k = 1; k <= N3M; k++)
     for (int i = 1; i <= N1M; i++) {
-      double u3 = U[IX(i, 3, k)], w3 = W[IX(i, 3, k)];
+      double u3 = U[IX(i, j, k)], w3 = W[IX(i, j, k)];
       double u_ave = sqrt(u3 * u3 + w3 * w3);
       double ut1 = 1.0, ut2 = 0.000001, f1, f2, f3, lam = 1.0;
       while (fabs(ut2 - ut1) > 1.0E-6) {
@@ -2367,11 +2368,30 @@ void c3o_wallmodel_instant(c3o *o, const double *U) {
       }
       double tauw1 = ut1 * ut1;
       wsm1 = wsm1 + tauw1;
-      o->tauw[TAUX(i, k, 1)] = tauw1 * u3 / u_ave;
-      o->taub[TAUX(i, k, 1)] = tauw1 * w3 / u_ave;
+      o->tauw[TAUX(i, k, layer)] = tauw1 * u3 / u_ave;
+      o->taub[TAUX(i, k, layer)] = tauw1 * w3 / u_ave;
     }
-  o->wsm[0] = wsm1 / (double)(N1M * N3M);
+  return wsm1;
+}
+
+void c3o_wallmodel_instant(c3o *o, const double *U) {
+  const double *DY = o->dy;
+  const double wsm1 = instant_wall(o, U, 3, DY[1] + DY[2] + DY[3] / 2.0, 1);
+  o->wsm[0] = wsm1 / (double)(o->n1m * o->n3m);
   o->taow = o->wsm[0];
+}
+
+/* WMODEL = 6, an extension of this build (the reference's WALLMODEL_INSTANT
+ * models the bottom wall only, so a full channel's top wall has no wall
+ * stress): WALLMODEL_INSTANT, then the same iteration at the top wall on
+ * the third cell from it, J = N2M-2, at the distance DY(N2M) + DY(N2M-1) +
+ * DY(N2M-2)/2, into TAUW(I,K,2) and TAUB(I,K,2).  WSM(1) = TAOW stay the
+ * bottom wall's mean. */
+void c3o_wallmodel_instant2(c3o *o, const double *U) {
+  const double *DY = o->dy;
+  const int N2M = o->n2m;
+  c3o_wallmodel_instant(o, U);
+  (void)instant_wall(o, U, N2M - 2, DY[N2M] + DY[N2M - 1] + DY[N2M - 2] / 2.0, 2);
 }
 
 /* ------------------------------------------------------- WALLMODEL_WERNER */
@@ -2598,6 +2618,9 @@ void c3o_getup(c3o *o, double *U, double *P, double *HH, double *SGSVIS,
       o->ifwall = 1;
     } else if (o->wmodel == 4.0) {
       c3o_wallmodel_werner(o, U);
+      o->ifwall = 1;
+    } else if (o->wmodel == 6.0) {
+      c3o_wallmodel_instant2(o, U);
       o->ifwall = 1;
     }
   }

@@ -74,6 +74,8 @@ double c3o_dqrrt(const double *pa, int n);
 /* one plane of SGSSD: its two levels' sums s1, s2 (A..E), PA; returns BETA */
 double c3o_sgssd_plane(c3o *o, const double *U, int j, double *s1, double *s2, double *pa);
 void c3o_wallmodel_instant(c3o *o, const double *U);
+/* WMODEL = 6 (this build's extension): WALLMODEL_INSTANT at both walls */
+void c3o_wallmodel_instant2(c3o *o, const double *U);
 /* WALLMODEL_WERNER compact3d.f:4815-4958 (WMODEL = 4; needs N3 <= N1) */
 void c3o_wallmodel_werner(c3o *o, const double *U);
 void c3o_filter(c3o *o, double *FILV, int n);

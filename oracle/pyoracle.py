@@ -268,6 +268,9 @@ class Oracle:
     def wallmodel_werner(self, U):
         self.L.c3o_wallmodel_werner(self.h, _p(U))
 
+    def wallmodel_instant2(self, U):
+        self.L.c3o_wallmodel_instant2(self.h, _p(U))
+
     def filter(self, FILV, four=False):
         n = FILV.shape[2]
         (self.L.c3o_filter4 if four else self.L.c3o_filter)(self.h, _p(FILV), n)

@@ -82,10 +82,11 @@ def test_apply_op_vs_dense(op, direction):
     ch.close()
 
 
-@pytest.mark.parametrize("wmodel", [1.0, 4.0])
+@pytest.mark.parametrize("wmodel", [1.0, 4.0, 6.0])
 def test_les_phases_match_oracle(wmodel):
     """Smagorinsky SGSCS, the wall model (WMODEL=1: the log law at the bottom
-    wall; 4: Werner-Wengle at both walls) and the SGS and wall-stress terms
+    wall; 4: Werner-Wengle at both walls; 6: the log law at both walls) and
+    the SGS and wall-stress terms
     of RHS1-3 (FILES=2, uniform y grid as MESH makes it, compact3d.f:356-357)"""
     from compact3d_amd import c3d
     from pyoracle import Oracle
@@ -109,12 +110,14 @@ def test_les_phases_match_oracle(wmodel):
     assert e <= 1e-13
     if wmodel == 1.0:
         orc.wallmodel_instant(Uo)
+    elif wmodel == 6.0:
+        orc.wallmodel_instant2(Uo)
     else:
         orc.wallmodel_werner(Uo)
     orc.set_ifwall(1)
     ch.wall_model()
     tw, tb = orc.get_tau()
-    if wmodel == 4.0:
+    if wmodel != 1.0:
         assert np.max(np.abs(tw[:, :, 1])) > 0 and np.max(np.abs(tb[:, :, 1])) > 0  # the top wall too
     e1 = relerr(tw, ch.field(c3d.F_TAUW))
     e2 = relerr(tb, ch.field(c3d.F_TAUB))
@@ -139,7 +142,7 @@ def test_les_phases_match_oracle(wmodel):
     ch.close()
 
 
-@pytest.mark.parametrize("wmodel", [1.0, 0.0, 4.0])
+@pytest.mark.parametrize("wmodel", [1.0, 0.0, 4.0, 6.0])
 def test_les_trajectory_10_steps(wmodel):
     from compact3d_amd import c3d
     from pyoracle import Oracle

@@ -250,3 +250,32 @@ def test_dqrrt_selects_the_largest_real_part_in_0_2():
     for roots, beta in cases:
         pa = np.real(np.poly(roots))[1:]
         assert abs(Oracle.dqrrt(pa) - beta) <= 2e-5, (roots, Oracle.dqrrt(pa))
+
+
+def test_two_wall_log_law_is_the_mirror_image():
+    """WMODEL = 6 (WALLMODEL_INSTANT at both walls): the top-wall stress of
+    a field equals the bottom-wall stress of the field mirrored in y (u, w
+    at cell centres J <-> N2-J, v at faces J <-> N2+1-J with its sign
+    flipped), and the bottom wall is WALLMODEL_INSTANT's own result"""
+    from pyoracle import Oracle
+    from cases import config, inputs
+    n1, n2, n3 = 33, 33, 33
+    cfg = config(n1, n2, n3, re=5200.0, dt=5e-4, files=2.0, wmodel=6.0)
+    o = Oracle(**cfg)
+    U, _, _ = inputs(o, seed=3)
+    M = np.asfortranarray(U[:, ::-1, :, :])      # J -> N2 - J
+    M[:, 1:n2 + 1, :, 1] = -U[:, n2:0:-1, :, 1]  # faces J -> N2 + 1 - J
+    M[:, 0, :, 1] = 0.0
+    o.wallmodel_instant2(U)
+    tw, tb = o.get_tau()
+    o1 = Oracle(**config(n1, n2, n3, re=5200.0, dt=5e-4, files=2.0, wmodel=1.0))
+    o1.wallmodel_instant(M)
+    mw, mb = o1.get_tau()
+    sl = (slice(1, n1), slice(1, n3))
+    for a, b in ((tw[sl + (1,)], mw[sl + (0,)]), (tb[sl + (1,)], mb[sl + (0,)])):
+        assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(b))
+    o2 = Oracle(**config(n1, n2, n3, re=5200.0, dt=5e-4, files=2.0, wmodel=1.0))
+    o2.wallmodel_instant(U)
+    bw, bb = o2.get_tau()
+    assert np.array_equal(tw[:, :, 0], bw[:, :, 0]) and np.array_equal(tb[:, :, 0], bb[:, :, 0])
+    assert np.max(np.abs(tw[sl + (1,)])) > 0
