@@ -207,7 +207,7 @@ def served_leg(args):
 def ensemble_leg(args):
     """Batched uneven instances: wall-modelled Smagorinsky LES channels
     (Cs 0.1 with van Driest damping, log-law wall stress by Newton iteration
-    at the first off-wall points) with grids drawn by seed 3 from
+    at both walls, WMODEL=6) with grids drawn by seed 3 from
     {48x33x48, 64x49x64, 96x65x96, 128x65x128} and Re_tau from
     {1000, 2000, 5200}, mixed precision, advanced args.ens_steps steps by one
     captured-graph launch per step for all of them (c3d_ensemble), then the
@@ -265,7 +265,7 @@ def ensemble_leg(args):
     for r in reqs:
         key = "%dx%dx%d" % r.grid
         counts[key] = counts.get(key, 0) + 1
-    return dict(members=len(reqs), grids=counts, steps=n, precision="mixed", model="Smagorinsky + log-law wall model",
+    return dict(members=len(reqs), grids=counts, steps=n, precision="mixed", model="Smagorinsky + log-law wall model at both walls (WMODEL=6)",
                 autotune=tuned or None,
                 pts_per_step=pts, ensemble_ms_per_step=round(1000.0 * (t1 - t0) / n, 4),
                 ensemble_mpts_steps_s=round(thr_ens, 1), streams_mpts_steps_s=round(thr_str, 1),

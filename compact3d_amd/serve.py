@@ -45,7 +45,7 @@ class Request:
     steps: int
     seed: int = 0
     precision: int = 0   # 0 FP64, 1 mixed
-    les: bool = False    # Smagorinsky SGS (FILES=2) + log-law wall model (WMODEL=1)
+    les: bool = False    # Smagorinsky SGS (FILES=2) + the log-law wall model at both walls (WMODEL=6)
     dt: float = 5e-4
     cks: float = 1.5
     t_arrive: float = 0.0
@@ -57,7 +57,7 @@ class Request:
     def config(self):
         n1m, n2, n3m = self.grid
         return dict(n1=n1m + 1, n2=n2, n3=n3m + 1, re=self.re, alx=2 * PI, aly=2.0, alz=PI, dt=self.dt,
-                    files=2.0 if self.les else 0.0, wmodel=1.0 if self.les else 0.0, cks=self.cks)
+                    files=2.0 if self.les else 0.0, wmodel=6.0 if self.les else 0.0, cks=self.cks)
 
     @property
     def pts(self):
