@@ -52,6 +52,13 @@ struct c3o {
   double *tauw, *taub; /* (0:M1,0:M3,2) */
   double wsm[3];
   double taow;
+  /* WMODEL = 2 (IWMLES): the planes of COMMON /IWMLES/ (iwm_* below), its
+   * time step wm_dt, CFLMAX of /TSTEP/, and the number of points of the last
+   * call that took the excessive-linear-correction branch without an
+   * equilibrium stress of their own (see c3o_wallmodel_iwmles) */
+  double *iwm;
+  double wm_dt, cflmax;
+  int iwm_fixed;
   cycfac fx[3], fz[3]; /* b = 6, 10, 22 along x (n1m) and z (n3m) */
 };
 
@@ -239,6 +246,7 @@ c3o *c3o_new(int n1, int n2, int n3, double re, double alx, double aly,
   o->ubc = dalloc((size_t)(n1 + 1) * (n3 + 1) * 6);
   o->tauw = dalloc((size_t)(n1 + 1) * (n3 + 1) * 2);
   o->taub = dalloc((size_t)(n1 + 1) * (n3 + 1) * 2);
+  o->iwm = wmodel == 2.0 ? dalloc((size_t)IW_NPL * (n1 - 1) * (n3 - 1)) : NULL;
 
   /* MESH: y grid (:338-361) */
   double *Y = o->y, *DY = o->dy, *H = o->h;
@@ -367,7 +375,7 @@ void c3o_free(c3o *o) {
   if (!o) return;
   double *ps[] = {o->y,   o->dy,  o->h,   o->hm,  o->hc,   o->hp,  o->dym,
                   o->dyc, o->dyp, o->yc,  o->fjpa, o->fjmu, o->fjmv, o->pmj,
-                  o->pcj, o->ppj, o->ak1, o->ak3, o->ubc,  o->tauw, o->taub};
+                  o->pcj, o->ppj, o->ak1, o->ak3, o->ubc,  o->tauw, o->taub, o->iwm};
   for (size_t t = 0; t < sizeof(ps) / sizeof(ps[0]); t++) free(ps[t]);
   free(o->ipa);
   free(o->ima);
@@ -2394,6 +2402,357 @@ void c3o_wallmodel_instant2(c3o *o, const double *U) {
   (void)instant_wall(o, U, N2M - 2, DY[N2M] + DY[N2M - 1] + DY[N2M - 2] / 2.0, 2);
 }
 
+/* ------------------------------------------------ WALLMODEL_IWMLES_SMOOTH */
+/* compact3d.f:3386-4329 (WMODEL = 2 with Y0 = 0): the integral wall model of
+ * Yang et al. (2015) for a smooth wall, at the bottom wall.  Every
+ * IWM_SKIP-th step (NTIME multiple of 5) it time-filters the third cell's
+ * velocity and the first cell's pressure fluctuation, forms the integrated
+ * momentum balance of the wall layer (iwm_calc_lhs_smooth) and solves it per
+ * point for the friction velocities (utx, uty) by Newton's method with a
+ * finite-difference Jacobian (iwm_calc_wallstress_smooth); every step it
+ * imposes the stored stresses.  State: the 43 planes of COMMON /IWMLES/ in
+ * their COMMON order (IW_* in c3d_oracle.h) and wm_dt.
+ *
+ * Powers follow what the reference's compiler makes of them, measured on
+ * the reference build: x**2.0 and x**2 are x*x, x**3.0 is x*x*x, x**4.0
+ * and x**0.25 call pow.
+ *
+ * Three points differ from the reference's text, where it is undefined:
+ *  - iwm_wallstress_smooth stores the spanwise stress in TAUW(I,K,3) of a
+ *    (0:M1,0:M3,2) array (:3549); in COMMON /TAUXYZ/ that element is
+ *    TAUB(I,K,1), which is where it is stored here (and read back for
+ *    WSM(1), :3470).
+ *  - the excessive-linear-correction branch (:4139-4147) takes the
+ *    equilibrium stress equilutx/equiluty of the point, which the reference
+ *    computes only when the Newton iteration failed (:4065-4099); otherwise
+ *    it reads whatever value the local variable kept from an earlier point.
+ *    Here the point's equilibrium stress is computed when it was not
+ *    (o->iwm_fixed counts those points in the last call).
+ *  - the equilibrium iterations (:3616-3627, :4077-4088) have no iteration
+ *    limit in the reference; here they stop after IWM_EQ_MAXIT iterations. */
+#define IWM_SKIP 5
+#define IWM_EQ_MAXIT 10000
+#define IWP(p, i, k) o->iwm[((size_t)(p) * N3M + (size_t)(k) - 1) * N1M + (size_t)(i) - 1]
+
+/* the log-law friction velocity for the speed u_ave at the distance deltap
+ * (KAPA = 0.4, B = 5; :3616-3627, :4077-4088) */
+static double iwm_equil(double u_ave, double deltap, double re) {
+  const double KAPA = 0.4, B = 5.0;
+  double ut1 = 1.0, ut2 = 0.000001, f1, f2, f3, lam = 1.0;
+  int it = 0;
+  while (fabs(ut2 - ut1) > 1.0E-6 && it < IWM_EQ_MAXIT) {
+    ut1 = ut2;
+    f1 = u_ave / ut1 - log(ut1 * deltap * re) / KAPA - B;
+    f2 = -u_ave / (ut1 * ut1) - 1. / KAPA / ut1;
+    ut2 = ut1 - lam * f1 / f2;
+    f3 = u_ave / ut2 - log(ut2 * deltap * re) / KAPA - B;
+    if (fabs(f3) > fabs(f1)) lam = 0.5 * lam;
+    it++;
+  }
+  return ut1;
+}
+
+/* iwm_init_smooth (:3561-3714) */
+static void iwm_init(c3o *o, const double *U) {
+  const int N1M = o->n1m, N3M = o->n3m;
+  const double *DY = o->dy;
+  const double vonk = 0.4, REINV = 1.0 / o->re;
+  const double Dzp = DY[1] + DY[2] + DY[3] / 2.0;
+  double uinit = 0.0;
+  for (int k = 1; k <= N3M; k++)
+    for (int i = 1; i <= N1M; i++) uinit = uinit + U[IX(i, 3, k)];
+  uinit = uinit / (double)(N1M * N3M);
+  const double usinit = iwm_equil(uinit, Dzp, o->re);
+  for (size_t t = 0; t < (size_t)IW_NPL * N1M * N3M; t++) o->iwm[t] = 0.0;
+  for (int k = 1; k <= N3M; k++)
+    for (int i = 1; i <= N1M; i++) {
+      IWP(IW_UTX, i, k) = usinit;
+      IWP(IW_TAUWX, i, k) = usinit * usinit;
+      IWP(IW_FLT, i, k) = uinit;
+      IWP(IW_FLTM, i, k) = uinit;
+      IWP(IW_INTE + 0, i, k) = uinit * Dzp;
+      IWP(IW_INTE + 1, i, k) = uinit * uinit * Dzp;
+      IWP(IW_INTEM + 0, i, k) = uinit * Dzp;
+      IWP(IW_INTEM + 1, i, k) = uinit * uinit * Dzp;
+      IWP(IW_LHS, i, k) = -uinit * Dzp;
+      IWP(IW_LHS + 1, i, k) = -uinit * Dzp;
+      IWP(IW_DUDZT, i, k) = usinit / vonk / Dzp;
+      IWP(IW_FLTUS, i, k) = usinit;
+      IWP(IW_TR, i, k) = (o->cflmax * o->alx / N1M / uinit) / (Dzp / vonk / usinit);
+      IWP(IW_DZ, i, k) = Dzp;
+      IWP(IW_DELI, i, k) = fmin(11.0 * REINV / usinit, Dzp);
+      IWP(IW_DELV, i, k) = REINV / usinit;
+    }
+  o->wm_dt = IWM_SKIP * o->dt;
+}
+
+/* iwm_calc_lhs_smooth (:3718-3877) */
+static void iwm_lhs(c3o *o, const double *U, const double *P, double presg, double presg3) {
+  const int N1M = o->n1m, N3M = o->n3m;
+  const double *W = U + 2 * NU;
+  const double DX1 = o->dx1, DX3 = o->dx3, wm_dt = o->wm_dt;
+  double p_bar = 0.0;
+  for (int i = 1; i <= N1M; i++)
+    for (int k = 1; k <= N3M; k++) p_bar = p_bar + P[PX(i, 1, k)];
+  p_bar = p_bar / (double)N1M / (double)N3M;
+  for (int i = 1; i <= N1M; i++)
+    for (int k = 1; k <= N3M; k++) {
+      const double tr = IWP(IW_TR, i, k);
+      IWP(IW_FLTM, i, k) = IWP(IW_FLT, i, k);
+      IWP(IW_FLTM + 1, i, k) = IWP(IW_FLT + 1, i, k);
+      IWP(IW_FLT, i, k) = IWP(IW_FLT, i, k) * (1.0 - tr) + U[IX(i, 3, k)] * tr;
+      IWP(IW_FLT + 1, i, k) = IWP(IW_FLT + 1, i, k) * (1.0 - tr) + W[IX(i, 3, k)] * tr;
+      IWP(IW_FLTP, i, k) = IWP(IW_FLTP, i, k) * (1.0 - tr) + (P[PX(i, 1, k)] - p_bar) * tr;
+    }
+  for (int k = 1; k <= N3M; k++) {
+    const int kp = o->kpa[k], km = o->kma[k];
+    for (int i = 1; i <= N1M; i++) {
+      const int ip = o->ipa[i], im = o->ima[i];
+      const double Dz = IWP(IW_DZ, i, k);
+      IWP(IW_UNSDY, i, k) = (IWP(IW_INTE + 0, i, k) - IWP(IW_INTEM + 0, i, k)) / wm_dt;
+      IWP(IW_UNSDY + 1, i, k) = (IWP(IW_INTE + 2, i, k) - IWP(IW_INTEM + 2, i, k)) / wm_dt;
+      const double uux = (IWP(IW_INTE + 1, ip, k) - IWP(IW_INTE + 1, im, k)) * DX1 / 2.0;
+      const double uvy = (IWP(IW_INTE + 4, i, kp) - IWP(IW_INTE + 4, i, km)) * DX3 / 2.0;
+      const double uvx = (IWP(IW_INTE + 4, ip, k) - IWP(IW_INTE + 4, im, k)) * DX1 / 2.0;
+      const double vvy = (IWP(IW_INTE + 3, i, kp) - IWP(IW_INTE + 3, i, km)) * DX3 / 2.0;
+      const double ux = (IWP(IW_INTE + 0, ip, k) - IWP(IW_INTE + 0, im, k)) * DX1 / 2.0;
+      const double vy = (IWP(IW_INTE + 2, i, kp) - IWP(IW_INTE + 2, i, km)) * DX3 / 2.0;
+      const double cx = uux + uvy - IWP(IW_FLTM, i, k) * (ux + vy);
+      const double cy = uvx + vvy - IWP(IW_FLTM + 1, i, k) * (ux + vy);
+      const double px = (IWP(IW_FLTP, ip, k) - IWP(IW_FLTP, im, k)) * DX1 / 2.0 * Dz + presg * Dz;
+      const double py = (IWP(IW_FLTP, i, kp) - IWP(IW_FLTP, i, km)) * DX3 / 2.0 * Dz + presg3 * Dz;
+      IWP(IW_CONV, i, k) = cx;
+      IWP(IW_CONV + 1, i, k) = cy;
+      IWP(IW_PRSG, i, k) = px;
+      IWP(IW_PRSG + 1, i, k) = py;
+      IWP(IW_LHS, i, k) = -IWP(IW_INTE + 0, i, k) + wm_dt * (cx + px - IWP(IW_DIFF, i, k));
+      IWP(IW_LHS + 1, i, k) = -IWP(IW_INTE + 2, i, k) + wm_dt * (cy + py - IWP(IW_DIFF + 1, i, k));
+    }
+  }
+}
+
+/* iwm_slv_smooth (:3881-3926): the residuals of the integrated balance for
+ * the friction velocities (utx, uty) */
+static void iwm_slv(double lhsx, double lhsy, double Ux, double Uy, double Dz, double utx, double uty,
+                    double re, double *fx, double *fy) {
+  const double vonk = 0.4, reinv = 1.0 / re;
+  const double us = pow(pow(utx, 4.0) + pow(uty, 4.0), 0.25);
+  const double delv = reinv / us;
+  const double deli = fmin(11.0 * reinv / us, Dz);
+  const double Vel = sqrt(Ux * Ux + Uy * Uy);
+  const double lg = log(deli / Dz);
+  const double Ax = (Ux / us + Ux / Vel / vonk * lg - deli / delv * utx / us) / (1.0 - deli / Dz);
+  const double Ay = (Uy / us + Uy / Vel / vonk * lg - deli / delv * uty / us) / (1.0 - deli / Dz);
+  const double Cx = Ux / us - Ax;
+  const double Cy = Uy / us - Ay;
+  const double inteLu = 0.5 * utx * (deli * deli) / delv +
+                        us * Dz * (0.5 * Ax * (1.0 - (deli * deli) / (Dz * Dz)) + Cx * (1.0 - deli / Dz) -
+                                   1.0 / vonk * Ux / Vel * (1.0 - deli / Dz + deli / Dz * lg));
+  const double inteLv = 0.5 * uty * (deli * deli) / delv +
+                        us * Dz * (0.5 * Ay * (1.0 - (deli * deli) / (Dz * Dz)) + Cy * (1.0 - deli / Dz) -
+                                   1.0 / vonk * Uy / Vel * (1.0 - deli / Dz + deli / Dz * lg));
+  *fx = inteLu + lhsx;
+  *fy = inteLv + lhsy;
+}
+
+/* iwm_calc_wallstress_smooth (:3930-4279) at one point; returns 1 when the
+ * excessive-linear-correction branch found no equilibrium stress of the
+ * point's own */
+static int iwm_point(c3o *o, int i, int k) {
+  const int N1M = o->n1m, N3M = o->n3m;
+  const double *DY = o->dy;
+  const double vonk = 0.4, re = o->re, reinv = 1.0 / re;
+  const double deltap = DY[1] + DY[2] + DY[3] / 2.0;
+  const int MaxIter = 1500;
+  const double wm_tol = 0.000001, wm_eps = 0.000000001;
+  const double lx = IWP(IW_LHS, i, k), ly = IWP(IW_LHS + 1, i, k);
+  const double Ux = IWP(IW_FLT, i, k), Uy = IWP(IW_FLT + 1, i, k), Dz = IWP(IW_DZ, i, k);
+  double utx = IWP(IW_UTX, i, k), uty = IWP(IW_UTY, i, k);
+  double fx, fy, fxp, fyp;
+  iwm_slv(lx, ly, Ux, Uy, Dz, utx, uty, re, &fx, &fy);
+  int iter = 0, equil = 0, have_eq = 0, fixed = 0;
+  while (fmax(fabs(fx), fabs(fy)) > wm_tol) {
+    iwm_slv(lx, ly, Ux, Uy, Dz, utx + wm_eps, uty, re, &fxp, &fyp);
+    const double a11 = (fxp - fx) / wm_eps, a21 = (fyp - fy) / wm_eps;
+    iwm_slv(lx, ly, Ux, Uy, Dz, utx, uty + wm_eps, re, &fxp, &fyp);
+    const double a12 = (fxp - fx) / wm_eps, a22 = (fyp - fy) / wm_eps;
+    const double nx = utx - (a22 * fx - a12 * fy) / (a11 * a22 - a12 * a21);
+    const double ny = uty - (-a21 * fx + a11 * fy) / (a11 * a22 - a12 * a21);
+    utx = nx;
+    uty = ny;
+    iwm_slv(lx, ly, Ux, Uy, Dz, utx, uty, re, &fx, &fy);
+    iter = iter + 1;
+    if (iter > MaxIter) {
+      equil = 1;
+      break;
+    }
+  }
+  if (utx - 1.0 == utx || uty - 1.0 == uty) equil = 1;
+  double equilutx = 0.0, equiluty = 0.0;
+  if (equil) {
+    const double U_ave = sqrt(Ux * Ux + Uy * Uy);
+    const double ut = iwm_equil(U_ave, deltap, re);
+    const double TAUW1 = ut * ut;
+    equilutx = TAUW1 * Ux / U_ave;
+    equiluty = TAUW1 * Uy / U_ave;
+    utx = sqrt(fabs(equilutx));
+    uty = sqrt(fabs(equiluty));
+    have_eq = 1;
+  }
+  /* the Newton (or equilibrium) friction velocities: everything below uses
+   * these (wmputx, wmputy), also where the branch after the linear
+   * correction stores other values in the state (:4110-4147) */
+  const double putx = utx, puty = uty;
+  const double utaup = pow(pow(putx, 4.0) + pow(puty, 4.0), 0.25);
+  const double delv = reinv / utaup;
+  const double deli = fmin(11.0 * reinv / utaup, Dz);
+  const double Vel = sqrt(Ux * Ux + Uy * Uy);
+  const double lg = log(deli / Dz);
+  double Ax, Ay, Cx, Cy;
+  if (equil) {
+    Ax = Ay = Cx = Cy = 0.0;
+  } else {
+    Ax = (Ux / utaup + Ux / Vel / vonk * lg - deli / delv * putx / utaup) / (1.0 - deli / Dz);
+    Ay = (Uy / utaup + Uy / Vel / vonk * lg - deli / delv * puty / utaup) / (1.0 - deli / Dz);
+    Cx = Ux / utaup - Ax;
+    Cy = Uy / utaup - Ay;
+  }
+  if (fabs(Ax) > 1.0 || fabs(Ay) > 1.0) {
+    if (!have_eq) {
+      const double U_ave = sqrt(Ux * Ux + Uy * Uy);
+      const double ut = iwm_equil(U_ave, deltap, re);
+      const double TAUW1 = ut * ut;
+      equilutx = TAUW1 * Ux / U_ave;
+      equiluty = TAUW1 * Uy / U_ave;
+      fixed = 1;
+    }
+    equil = 1;
+    utx = sqrt(fabs(equilutx));
+    uty = sqrt(fabs(equiluty));
+    Ax = Ay = Cx = Cy = 0.0;
+  }
+  IWP(IW_UTX, i, k) = utx;  /* the state: after the branch */
+  IWP(IW_UTY, i, k) = uty;
+  IWP(IW_AX, i, k) = Ax;
+  IWP(IW_AY, i, k) = Ay;
+  IWP(IW_CX, i, k) = Cx;
+  IWP(IW_CY, i, k) = Cy;
+  for (int q = 0; q < 5; q++) IWP(IW_INTEM + q, i, k) = IWP(IW_INTE + q, i, k);
+  const double d2 = deli * deli, d3 = deli * deli * deli, z2 = Dz * Dz, z3 = Dz * Dz * Dz;
+  const double dv2 = delv * delv, us2 = utaup * utaup, V2 = Vel * Vel, k2 = vonk * vonk;
+  IWP(IW_INTE + 0, i, k) = 0.5 * putx * d2 / delv +
+                           utaup * Dz * (0.5 * Ax * (1.0 - d2 / z2) + Cx * (1.0 - deli / Dz) -
+                                         1.0 / vonk * Ux / Vel * (1.0 - deli / Dz + deli / Dz * lg));
+  IWP(IW_INTE + 2, i, k) = 0.5 * puty * d2 / delv +
+                           utaup * Dz * (0.5 * Ay * (1.0 - d2 / z2) + Cy * (1.0 - deli / Dz) -
+                                         1.0 / vonk * Uy / Vel * (1.0 - deli / Dz + deli / Dz * lg));
+  IWP(IW_INTE + 4, i, k) =
+      1.0 / 3.0 * putx * puty * d3 / dv2 +
+      us2 * Dz *
+          (-1.0 / vonk * (Ax * Uy / Vel + Ay * Ux / Vel) * (0.25 - 0.25 * d2 / z2 + 0.5 * d2 / Dz * lg) -
+           1.0 / vonk * (Cx * Uy / Vel + Cy * Ux / Vel) * (1.0 - deli / Dz + deli / Dz * lg) -
+           Ux * Uy / V2 / k2 * (deli / Dz - 2.0 + deli / Dz * ((lg - 1.0) * (lg - 1.0))) +
+           1.0 / 3.0 * Ax * Ay * (1.0 - d3 / z3) + 0.5 * (Ax * Cy + Ay * Cx) * (1.0 - d2 / z2) +
+           Cx * Cy * (1.0 - deli / Dz));
+  {
+    const double tx = Cx - 1.0 / vonk * Ux / Vel, sx = Cx - Ux / Vel / vonk + Ux / Vel / vonk * lg;
+    const double ty = Cy - 1.0 / vonk * Uy / Vel, sy = Cy - Uy / Vel / vonk + Uy / Vel / vonk * lg;
+    IWP(IW_INTE + 1, i, k) =
+        1.0 / 3.0 * (putx * putx) * d3 / dv2 +
+        us2 * Dz *
+            (-Ax * Ux / Vel / vonk * d2 / z2 * lg + Ax * (Cx - 1.0 / 2.0 / vonk * Ux / Vel) * (1.0 - d2 / z2) +
+             Ax * Ax / 3.0 * (1.0 - d3 / z3) + tx * tx - deli / Dz * (sx * sx) +
+             1.0 / k2 * (Ux * Ux) / V2 * (1.0 - deli / Dz));
+    IWP(IW_INTE + 3, i, k) =
+        1.0 / 3.0 * (puty * puty) * d3 / dv2 +
+        us2 * Dz *
+            (-Ay * Uy / Vel / vonk * d2 / z2 * lg + Ay * (Cy - 1.0 / 2.0 / vonk * Uy / Vel) * (1.0 - d2 / z2) +
+             Ay * Ay / 3.0 * (1.0 - d3 / z3) + ty * ty - deli / Dz * (sy * sy) +
+             1.0 / k2 * (Uy * Uy) / V2 * (1.0 - deli / Dz));
+  }
+  double tx, ty;
+  if (deli < Dz) {
+    tx = utaup / Dz * (Ax + 1.0 / vonk * Ux / Vel);
+    ty = utaup / Dz * (Ay + 1.0 / vonk * Uy / Vel);
+  } else {
+    tx = putx / delv;
+    ty = puty / delv;
+  }
+  const double bx = putx / delv, by = puty / delv;
+  IWP(IW_DUDZT, i, k) = tx;
+  IWP(IW_DUDZT + 1, i, k) = ty;
+  IWP(IW_DUDZB, i, k) = bx;
+  IWP(IW_DUDZB + 1, i, k) = by;
+  const double dVelzT = fabs(Ux / Vel * tx + Uy / Vel * ty);
+  if (deli < Dz) {
+    IWP(IW_DIFF, i, k) = (reinv + (vonk * Dz) * (vonk * Dz) * dVelzT) * tx - reinv * bx;
+    IWP(IW_DIFF + 1, i, k) = (reinv + (vonk * Dz) * (vonk * Dz) * dVelzT) * ty - reinv * by;
+  } else {
+    IWP(IW_DIFF, i, k) = 0.0;
+    IWP(IW_DIFF + 1, i, k) = 0.0;
+  }
+  if (equil) {
+    IWP(IW_TAUWX, i, k) = equilutx;
+    IWP(IW_TAUWY, i, k) = equiluty;
+  } else if (deli < Dz) {
+    IWP(IW_TAUWX, i, k) = putx * putx;
+    IWP(IW_TAUWY, i, k) = puty * puty;
+  } else {
+    IWP(IW_TAUWX, i, k) = reinv * Ux / Dz;
+    IWP(IW_TAUWY, i, k) = reinv * Uy / Dz;
+  }
+  const double tR = IWP(IW_TR, i, k);
+  IWP(IW_FLTUS, i, k) = IWP(IW_FLTUS, i, k) * (1.0 - tR) + utaup * tR;
+  double ntr = o->wm_dt / (Dz / IWP(IW_FLTUS, i, k) / vonk);
+  if (ntr > 1.0) ntr = 1.0;
+  IWP(IW_TR, i, k) = ntr;
+  return fixed;
+}
+
+void c3o_wallmodel_iwmles(c3o *o, const double *U, const double *P, double presg, double presg3, int ntime) {
+  const int N1M = o->n1m, N3M = o->n3m;
+  if (ntime == 1) iwm_init(o, U);
+  /* iwm_wallstress_smooth (:3489-3557) */
+  if (ntime % IWM_SKIP == 1)
+    o->wm_dt = o->dt;
+  else
+    o->wm_dt = o->wm_dt + o->dt;
+  o->iwm_fixed = 0;
+  if (ntime % IWM_SKIP == 0) {
+    iwm_lhs(o, U, P, presg, presg3);
+    for (int i = 1; i <= N1M; i++)
+      for (int k = 1; k <= N3M; k++) o->iwm_fixed += iwm_point(o, i, k);
+  }
+  for (int i = 1; i <= N1M; i++)
+    for (int k = 1; k <= N3M; k++) {
+      o->tauw[TAUX(i, k, 1)] = IWP(IW_TAUWX, i, k);
+      o->taub[TAUX(i, k, 1)] = IWP(IW_TAUWY, i, k);
+    }
+  /* WSM(1) = TAOW, the mean of |tau| (:3463-3476) */
+  double wsm1 = 0.0;
+  for (int k = 1; k <= N3M; k++)
+    for (int i = 1; i <= N1M; i++) {
+      const double a = o->tauw[TAUX(i, k, 1)], b = o->taub[TAUX(i, k, 1)];
+      wsm1 = wsm1 + sqrt(a * a + b * b);
+    }
+  o->wsm[0] = wsm1 / (double)(N1M * N3M);
+  o->taow = o->wsm[0];
+}
+
+void c3o_iwm_state(const c3o *o, double *planes, double *wm_dt, int *fixed) {
+  if (planes && o->iwm) memcpy(planes, o->iwm, sizeof(double) * IW_NPL * o->n1m * o->n3m);
+  if (wm_dt) *wm_dt = o->wm_dt;
+  if (fixed) *fixed = o->iwm_fixed;
+}
+
+void c3o_iwm_set(c3o *o, const double *planes, double wm_dt) {
+  if (o->iwm) memcpy(o->iwm, planes, sizeof(double) * IW_NPL * o->n1m * o->n3m);
+  o->wm_dt = wm_dt;
+}
+
+void c3o_set_cflmax(c3o *o, double cflmax) { o->cflmax = cflmax; }
+#undef IWP
+
 /* ------------------------------------------------------- WALLMODEL_WERNER */
 /* compact3d.f:4815-4958 (WMODEL = 4): the Werner-Wengle power-law wall
  * stress at both walls from the first cell's centre velocity, a linear
@@ -2599,7 +2958,6 @@ void c3o_plane_stats(c3o *o, const double *U, const double *P, double *out) {
 /* compact3d.f:879-960 */
 void c3o_getup(c3o *o, double *U, double *P, double *HH, double *SGSVIS,
                double *presg, double *presg3, int ntime) {
-  (void)ntime;
   const size_t nu = NU, np = (size_t)o->n1 * o->n2 * o->n3;
   double *UH = dalloc(3 * nu), *HH2 = dalloc(3 * nu), *RUH = dalloc(3 * nu);
   double *DP = dalloc(np), *RDP = dalloc(np);
@@ -2615,6 +2973,9 @@ void c3o_getup(c3o *o, double *U, double *P, double *HH, double *SGSVIS,
       c3o_wallss(o, U);
     } else if (o->wmodel == 1.0) {
       c3o_wallmodel_instant(o, U);
+      o->ifwall = 1;
+    } else if (o->wmodel == 2.0) {
+      c3o_wallmodel_iwmles(o, U, P, *presg, *presg3, ntime);
       o->ifwall = 1;
     } else if (o->wmodel == 4.0) {
       c3o_wallmodel_werner(o, U);

@@ -23,6 +23,12 @@ c3o *c3o_new(int n1, int n2, int n3, double re, double alx, double aly,
              double alz, double dt, double files, double wmodel, double cks);
 void c3o_free(c3o *o);
 void c3o_set_threads(int n);
+/* the planes of COMMON /IWMLES/ (compact3d.f:3399-3408) in their order */
+enum {
+  IW_UTX = 0, IW_UTY, IW_TAUWX, IW_TAUWY, IW_FLT, IW_FLTM = 6, IW_FLTP = 8, IW_INTE = 9, IW_INTEM = 14,
+  IW_UNSDY = 19, IW_CONV = 21, IW_PRSG = 23, IW_DIFF = 25, IW_LHS = 27, IW_DUDZT = 29, IW_DUDZB = 31,
+  IW_FLTUS = 33, IW_TR, IW_DZ, IW_Z0, IW_AX, IW_AY, IW_DELV, IW_DELI, IW_CX, IW_CY, IW_NPL
+};
 void c3o_set_dt(c3o *o, double dt);
 void c3o_set_ifwall(c3o *o, int ifwall);
 /* which: 0 Y(0:N2) 1 DY(0:N2) 2 H 3 HM 4 HC 5 HP 6 DYM 7 DYC 8 DYP (1..N2 stored at
@@ -76,6 +82,15 @@ double c3o_sgssd_plane(c3o *o, const double *U, int j, double *s1, double *s2, d
 void c3o_wallmodel_instant(c3o *o, const double *U);
 /* WMODEL = 6 (this build's extension): WALLMODEL_INSTANT at both walls */
 void c3o_wallmodel_instant2(c3o *o, const double *U);
+/* WALLMODEL_IWMLES_SMOOTH compact3d.f:3386-4329 (WMODEL = 2, Y0 = 0), called
+ * with GETUP's NTIME; P is (1:N1,1:N2,1:N3) */
+void c3o_wallmodel_iwmles(c3o *o, const double *U, const double *P, double presg, double presg3, int ntime);
+/* its state: the IW_NPL planes of COMMON /IWMLES/ (N1M x N3M each, I
+ * fastest), wm_dt, and the points of the last call whose excessive linear
+ * correction needed an equilibrium stress of their own (any may be NULL) */
+void c3o_iwm_state(const c3o *o, double *planes, double *wm_dt, int *fixed);
+void c3o_iwm_set(c3o *o, const double *planes, double wm_dt);
+void c3o_set_cflmax(c3o *o, double cflmax);
 /* WALLMODEL_WERNER compact3d.f:4815-4958 (WMODEL = 4; needs N3 <= N1) */
 void c3o_wallmodel_werner(c3o *o, const double *U);
 void c3o_filter(c3o *o, double *FILV, int n);

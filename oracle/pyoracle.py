@@ -23,6 +23,15 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libc3dref.so")
 
+# WMODEL = 2: the planes of COMMON /IWMLES/ (compact3d.f:3399-3408) in order
+IWM_PLANES = ("utx", "uty", "tauwx", "tauwy", "flt_u", "flt_w", "flt_um", "flt_wm", "flt_p",
+              "inte_lu", "inte_luu", "inte_lv", "inte_lvv", "inte_luv",
+              "intem_lu", "intem_luu", "intem_lv", "intem_lvv", "intem_luv",
+              "unsdy_x", "unsdy_y", "conv_x", "conv_y", "prsgrad_x", "prsgrad_y", "diff_x", "diff_y",
+              "lhs_x", "lhs_y", "dudzt_x", "dudzt_y", "dudzb_x", "dudzb_y",
+              "flt_us", "tr", "dz", "z0", "ax", "ay", "delv", "deli", "cx", "cy")
+IWM_NPL = len(IWM_PLANES)
+
 _dp = C.POINTER(C.c_double)
 
 
@@ -75,6 +84,10 @@ class Oracle:
             L.c3o_dqrrt.argtypes = [_dp, C.c_int]
             L.c3o_sgssd_plane.restype = C.c_double
             L.c3o_sgssd_plane.argtypes = [C.c_void_p, _dp, C.c_int, _dp, _dp, _dp]
+            L.c3o_wallmodel_iwmles.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_double, C.c_int]
+            L.c3o_iwm_state.argtypes = [C.c_void_p, _dp, _dp, C.POINTER(C.c_int)]
+            L.c3o_set_cflmax.argtypes = [C.c_void_p, C.c_double]
+            L.c3o_iwm_set.argtypes = [C.c_void_p, _dp, C.c_double]
             cls._lib = L
         return cls._lib
 
@@ -270,6 +283,26 @@ class Oracle:
 
     def wallmodel_instant2(self, U):
         self.L.c3o_wallmodel_instant2(self.h, _p(U))
+
+    def set_cflmax(self, v):
+        self.L.c3o_set_cflmax(self.h, float(v))
+
+    def wallmodel_iwmles(self, U, P, presg, presg3, ntime):
+        self.L.c3o_wallmodel_iwmles(self.h, _p(U), _p(P), float(presg), float(presg3), int(ntime))
+
+    def iwm_state(self):
+        """the 43 planes of COMMON /IWMLES/ as [plane, K-1, I-1], wm_dt, and
+        the points of the last call whose linear-correction branch needed an
+        equilibrium stress of their own"""
+        n1m, n3m = self.n1 - 1, self.n3 - 1
+        pl = np.zeros(IWM_NPL * n3m * n1m)
+        dt = C.c_double(0.0)
+        fx = C.c_int(0)
+        self.L.c3o_iwm_state(self.h, _p(pl), C.byref(dt), C.byref(fx))
+        return pl.reshape(IWM_NPL, n3m, n1m), dt.value, fx.value
+
+    def iwm_set(self, planes, wm_dt):
+        self.L.c3o_iwm_set(self.h, _p(np.ascontiguousarray(planes, dtype=np.float64).reshape(-1)), float(wm_dt))
 
     def filter(self, FILV, four=False):
         n = FILV.shape[2]
@@ -499,6 +532,34 @@ class RefLib:
         tw, tb = self.get_tau()
         self.L.wallmodel_instant_(_p(U), _p(tw), _p(tb), _p(SGSVIS))
         self.set_tau(tw, tb)
+
+    def set_cflmax(self, v):
+        self.L.refh_setcflmax_(C.byref(C.c_double(float(v))))
+
+    def iwmles(self, U, P, presg, presg3, ntime):
+        """WMODEL = 2: the reference's iwm_* routines under the harness's
+        driver (run in a scratch directory: iwm_monitor_smooth appends to
+        iwm_track.dat)"""
+        a, b = C.c_double(presg), C.c_double(presg3)
+        cwd = os.getcwd()
+        with tempfile.TemporaryDirectory() as td:
+            os.chdir(td)
+            try:
+                self.L.refh_iwmles_(_p(U), _p(P), C.byref(a), C.byref(b), self._i(ntime))
+            finally:
+                os.chdir(cwd)
+
+    def iwm_state(self):
+        """the 43 planes of COMMON /IWMLES/ as [plane, K-1, I-1] and wm_dt"""
+        n1m, n3m = self.n1 - 1, self.n3 - 1
+        out = np.zeros((n1m, n3m, IWM_NPL), order="F")
+        dt = C.c_double(0.0)
+        self.L.refh_getiwm_(_p(out), C.byref(dt))
+        return np.ascontiguousarray(out.transpose(2, 1, 0)), dt.value
+
+    def iwm_set(self, planes, wm_dt):
+        a = np.asfortranarray(np.asarray(planes, dtype=np.float64).transpose(2, 1, 0))
+        self.L.refh_setiwm_(_p(a), C.byref(C.c_double(float(wm_dt))))
 
     def wallmodel_werner(self, U, SGSVIS, ntime=1):
         tw, tb = self.get_tau()

@@ -171,3 +171,98 @@ C
       F(2)=FLOW3
       RETURN
       END
+
+C     WMODEL = 2 (IWMLES): CFLMAX of /TSTEP/, and the 43 planes of COMMON
+C     /IWMLES/ (compact3d.f:3399-3408, 43 arrays of (M1,M3) in sequence)
+C     with wm_dt of /IWMLES_PARA/ (ten INTEGERs, then wm_dt)
+      SUBROUTINE REFH_SETCFLMAX(X)
+      COMMON/TSTEP/NTST,DT,CFLMAX
+      CFLMAX=X
+      RETURN
+      END
+
+      SUBROUTINE REFH_GETIWM(OUT,XDT)
+      INCLUDE 'PARAM.H'
+      COMMON/DIM/N1,N2,N3,N1M,N2M,N3M
+      COMMON/IWMLES/WMS(M1,M3,43)
+      COMMON/IWMLES_PARA/IPRM(10),WMDT
+      INTEGER IPRM
+      REAL OUT(N1M,N3M,43)
+      DO 10 L=1,43
+      DO 10 K=1,N3M
+      DO 10 I=1,N1M
+ 10   OUT(I,K,L)=WMS(I,K,L)
+      XDT=WMDT
+      RETURN
+      END
+
+C     The two driver routines of WMODEL = 2, WALLMODEL_IWMLES_SMOOTH and
+C     iwm_wallstress_smooth (compact3d.f:3386-3557), store the spanwise
+C     stress as TAUW(I,K,3) of a (0:M1,0:M3,2) array, which flang rejects at
+C     compile time; they are restated here (that element is TAUB(I,K,1) in
+C     COMMON /TAUXYZ/) around the reference's own iwm_init_smooth,
+C     iwm_calc_lhs_smooth, iwm_calc_wallstress_smooth and iwm_monitor_smooth.
+      SUBROUTINE REFH_IWMLES(U,P,PRESG,PRESG3,NTIME)
+      INCLUDE 'PARAM.H'
+      COMMON/DIM/N1,N2,N3,N1M,N2M,N3M
+      COMMON/TSTEP/NTST,DT,CFLMAX
+      COMMON/TAUXYZ/TAUW(0:M1,0:M3,2),TAUB(0:M1,0:M3,2)
+      COMMON/WSMEM/WSM(3),WSMO(3)
+      COMMON/WALLTAO/TAOW
+      COMMON/IWMLES/WMS(M1,M3,43)
+      COMMON/IWMLES_PARA/IPRM(10),WMDT
+      INTEGER IPRM
+      REAL U(0:M1,0:M2,0:M3,3),P(M1,M2,M3)
+      REAL PRESG,PRESG3,T1
+      INTEGER NTIME
+C     iwm_dirx, iwm_diry, iwm_DN, iwm_Lu, iwm_Luu, iwm_Lv, iwm_Lvv,
+C     iwm_Luv, iwm_LN, iwm_ntime_skip
+      IPRM(1)=1
+      IPRM(2)=2
+      IPRM(3)=2
+      IPRM(4)=1
+      IPRM(5)=2
+      IPRM(6)=3
+      IPRM(7)=4
+      IPRM(8)=5
+      IPRM(9)=5
+      IPRM(10)=5
+      IF (NTIME.EQ.1) CALL iwm_init_smooth(U)
+      IF (MOD(NTIME,IPRM(10)).EQ.1) THEN
+        WMDT=DT
+      ELSE
+        WMDT=WMDT+DT
+      ENDIF
+      IF (MOD(NTIME,IPRM(10)).EQ.0) THEN
+        CALL iwm_calc_lhs_smooth(U,P,PRESG,PRESG3)
+        CALL iwm_calc_wallstress_smooth(U)
+        CALL iwm_monitor_smooth(NTIME)
+      ENDIF
+      DO 10 I=1,N1M
+      DO 10 K=1,N3M
+      TAUW(I,K,1)=WMS(I,K,3)
+ 10   TAUB(I,K,1)=WMS(I,K,4)
+      WSM(1)=0.0
+      DO 20 K=1,N3M
+      DO 20 I=1,N1M
+      T1=SQRT(TAUW(I,K,1)**2.+TAUB(I,K,1)**2.)
+ 20   WSM(1)=WSM(1)+T1
+      WSM(1)=WSM(1)/REAL(N1M*N3M)
+      TAOW=WSM(1)
+      RETURN
+      END
+
+      SUBROUTINE REFH_SETIWM(XIN,XDT)
+      INCLUDE 'PARAM.H'
+      COMMON/DIM/N1,N2,N3,N1M,N2M,N3M
+      COMMON/IWMLES/WMS(M1,M3,43)
+      COMMON/IWMLES_PARA/IPRM(10),WMDT
+      INTEGER IPRM
+      REAL XIN(N1M,N3M,43)
+      DO 10 L=1,43
+      DO 10 K=1,N3M
+      DO 10 I=1,N1M
+ 10   WMS(I,K,L)=XIN(I,K,L)
+      WMDT=XDT
+      RETURN
+      END

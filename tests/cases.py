@@ -71,3 +71,80 @@ def writeup_inputs(orc, seed=7):
     rng = np.random.default_rng(2000 + seed)
     tauw = rng.standard_normal((orc.n3 - 1, orc.n1 - 1)) * 1e-3 + 2.5e-3
     return U, P, tauw
+
+
+IWM_KAPPA, IWM_B = 0.4, 5.0
+
+
+def iwm_equil(u, dz, re):
+    """the log-law friction velocity u/ut = ln(ut dz re)/0.4 + 5 (bisection
+    on ln ut; independent of the models' Newton iterations)"""
+    u = np.asarray(u, dtype=np.float64)
+    lo, hi = np.full(u.shape, -30.0), np.full(u.shape, 5.0)
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        ut = np.exp(mid)
+        f = u / ut - np.log(ut * dz * re) / IWM_KAPPA - IWM_B
+        lo = np.where(f > 0, mid, lo)
+        hi = np.where(f > 0, hi, mid)
+    return np.exp(0.5 * (lo + hi))
+
+
+def iwm_integrals(utx, uty, ux, uy, dz, re):
+    """the wall-layer integrals of u and w (the residual terms of the
+    integral wall model's balance) for friction velocities utx, uty and the
+    velocity (ux, uy) at the height dz"""
+    k = IWM_KAPPA
+    us = (utx ** 4 + uty ** 4) ** 0.25
+    delv = 1.0 / re / us
+    deli = np.minimum(11.0 / re / us, dz)
+    vel = np.sqrt(ux * ux + uy * uy)
+    lg = np.log(deli / dz)
+    out = []
+    for ut, u in ((utx, ux), (uty, uy)):
+        a = (u / us + u / vel / k * lg - deli / delv * ut / us) / (1 - deli / dz)
+        c = u / us - a
+        out.append(0.5 * ut * deli ** 2 / delv + us * dz * (0.5 * a * (1 - deli ** 2 / dz ** 2) + c * (1 - deli / dz)
+                                                          - u / vel / k * (1 - deli / dz + deli / dz * lg)))
+    return out
+
+
+def iwm_inputs(n1, n3, dy, re, dt, seed=0):
+    """A state of the integral wall model (WMODEL = 2) consistent with the
+    log law, so that its Newton iteration converges with small linear
+    corrections: the velocity of the third cell from the wall (a log-law
+    speed with a seeded variation, spanwise component positive), friction
+    velocities near the equilibrium value along the velocity's direction, the wall-layer integrals that
+    go with them, and a time-filter weight of 0.2.  Returns U
+    ((0:N1,0:N2,0:N3,3) sized from dy, only J = 3 set), P (first cell set),
+    the 43 state planes [plane, K-1, I-1] and wm_dt four steps into a
+    five-step cycle."""
+    from pyoracle import IWM_NPL, IWM_PLANES
+    n1m, n3m, n2 = n1 - 1, n3 - 1, len(dy) - 1
+    rng = np.random.default_rng(seed)
+    dz = dy[1] + dy[2] + dy[3] / 2.0
+    ut0 = 0.045
+    u0 = ut0 * (np.log(dz * ut0 * re) / IWM_KAPPA + IWM_B)
+    x = np.arange(n1m)[None, :] * 2 * PI / n1m
+    z = np.arange(n3m)[:, None] * 2 * PI / n3m
+    ux = u0 * (1 + 0.08 * np.sin(x) * np.cos(2 * z) + 0.02 * rng.standard_normal((n3m, n1m)))
+    uy = u0 * (0.15 + 0.05 * np.cos(x + z) + 0.01 * rng.standard_normal((n3m, n1m)))
+    vel = np.sqrt(ux * ux + uy * uy)
+    ut = iwm_equil(vel, dz, re) * (1 + 0.01 * rng.standard_normal((n3m, n1m)))
+    utx, uty = ut * ux / vel, ut * uy / vel
+    il, iv = iwm_integrals(utx, uty, ux, uy, dz, re)
+    pl = np.zeros((IWM_NPL, n3m, n1m))
+    ix = {name: q for q, name in enumerate(IWM_PLANES)}
+    for name, v in (("utx", utx), ("uty", uty), ("tauwx", utx * utx), ("tauwy", uty * uty),
+                    ("flt_u", ux), ("flt_w", uy), ("flt_um", ux), ("flt_wm", uy),
+                    ("inte_lu", il), ("inte_lv", iv), ("inte_luu", ux * ux * dz), ("inte_lvv", uy * uy * dz),
+                    ("inte_luv", ux * uy * dz), ("intem_lu", il), ("intem_lv", iv), ("flt_us", ut),
+                    ("tr", np.full_like(ux, 0.2)), ("dz", np.full_like(ux, dz)), ("delv", 1.0 / re / ut),
+                    ("deli", np.minimum(11.0 / re / ut, dz))):
+        pl[ix[name]] = v
+    U = np.zeros((n1 + 1, n2 + 1, n3 + 1, 3), order="F")
+    U[1:n1, 3, 1:n3, 0] = (ux * (1 + 0.01 * rng.standard_normal(ux.shape))).T
+    U[1:n1, 3, 1:n3, 2] = (uy * (1 + 0.01 * rng.standard_normal(uy.shape))).T
+    P = np.zeros((n1, n2, n3), order="F")
+    P[:n1m, 0, :n3m] = 1e-3 * rng.standard_normal((n1m, n3m))
+    return U, P, pl, 4 * dt

@@ -13,7 +13,7 @@ it; the fixture's UPCALC input DP is the oracle's GETDP of the reference's RDP
 WRITEUP's Tecplot text is stored as checksums and sample lines
 (ref_writeup.json).
 
-Usage:  python tests/golden/make_golden.py      (build container only)
+Usage:  python tests/golden/make_golden.py [case ...]   (build container only)
 """
 import math
 import os
@@ -27,7 +27,7 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
-from cases import inputs  # noqa: E402
+from cases import inputs, iwm_inputs  # noqa: E402
 from pyoracle import Oracle, RefLib  # noqa: E402
 
 STRIDE = 13
@@ -146,6 +146,67 @@ def run_case(name, cfg):
     print(path, os.path.getsize(path), "bytes")
 
 
+IWM = dict(re=5200.0, alx=2 * math.pi, aly=2.0, alz=math.pi, dt=5e-4, files=2.0, wmodel=2.0)
+IWM_CFLMAX = 0.5
+
+
+def iwm_drive(lib, U0, P0, first, last):
+    """calls NTIME = first..last of the WMODEL = 2 wall model with seeded
+    variations of U and P; yields (NTIME, U, P) after each call"""
+    for nt in range(first, last + 1):
+        U = np.asfortranarray(U0 * (1 + 0.01 * math.sin(nt)))
+        P = np.asfortranarray(P0 * (1 + 0.1 * nt))
+        yield nt, U, P
+
+
+def run_iwm():
+    """WMODEL = 2 (IWMLES, smooth wall): NTIME = 1..4 from the reference's
+    own initialisation (the first compute step, NTIME = 5, reads an
+    undefined local of the reference there and is not stored), then
+    NTIME = 5..15 from a log-law-consistent state (tests/cases.py
+    iwm_inputs), which contains three compute steps."""
+    ref = RefLib(**IWM)
+    n1, n3 = ref.n1, ref.n3
+    orc = Oracle(n1, ref.n2, n3, cks=2.5, **IWM)
+    ref.set_cflmax(IWM_CFLMAX)
+    out = {}
+    U0, P0, _ = inputs(orc, seed=7)
+    for nt, U, P in iwm_drive(ref, U0, P0, 1, 4):
+        ref.iwmles(U, P, 0.003, -0.001, nt)
+    pl, wdt = ref.iwm_state()
+    tw, tb = ref.get_tau()
+    out["INIT_PLANES"], out["INIT_TAUW"], out["INIT_TAUB"] = pl, tw, tb
+    out["INIT_WSM"] = np.array([ref.get_wsm()[0], wdt])
+    dy = orc.mesh()["DY"]
+    U0, P0, pl0, wdt0 = iwm_inputs(n1, n3, dy, IWM["re"], IWM["dt"], seed=1)
+    ref.iwm_set(pl0, wdt0)
+    wsm = []
+    for nt, U, P in iwm_drive(ref, U0, P0, 5, 15):
+        ref.iwmles(U, P, 0.003, -0.001, nt)
+        wsm.append(ref.get_wsm()[0])
+        if nt % 5 == 0:
+            pl, wdt = ref.iwm_state()
+            out[f"PLANES_{nt}"] = pl
+            out[f"WDT_{nt}"] = np.array([wdt])
+    tw, tb = ref.get_tau()
+    out["TAUW"], out["TAUB"] = tw, tb
+    out["WSM"] = np.array(wsm)
+    store = {"config": np.array([n1, ref.n2, n3, IWM["re"], IWM["alx"], IWM["aly"], IWM["alz"], IWM["dt"],
+                                 IWM["files"], IWM["wmodel"], IWM_CFLMAX, STRIDE])}
+    for k, v in out.items():
+        if k.startswith("WSM") or k.startswith("INIT_WSM") or k.startswith("WDT"):
+            store[k] = np.asarray(v, dtype=np.float64)
+        elif "PLANES" in k:
+            store[k + "__sub"] = np.stack([sub(p) for p in v])
+            store[k + "__sum"] = np.stack([summ(p) for p in v])
+        else:
+            store[k + "__sub"] = sub(v)
+            store[k + "__sum"] = summ(v)
+    path = os.path.join(HERE, "ref_iwm.npz")
+    np.savez_compressed(path, **store)
+    print(path, os.path.getsize(path), "bytes")
+
+
 def run_writeup():
     """WRITEUP's FLOW and STRAIN files for the writeup_inputs() field:
     sha256 of each text, its line count and a few of its lines"""
@@ -172,6 +233,11 @@ def run_writeup():
 
 
 if __name__ == "__main__":
+    only = set(sys.argv[1:])  # optional case names: dns les ... writeup iwm
     for name, cfg in CASES.items():
-        run_case(name, cfg)
-    run_writeup()
+        if not only or name in only:
+            run_case(name, cfg)
+    if not only or "writeup" in only:
+        run_writeup()
+    if not only or "iwm" in only:
+        run_iwm()

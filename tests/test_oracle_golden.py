@@ -127,3 +127,54 @@ def test_strain_matches_reference_les():
     o = Oracle(n1, n2, n3, re, alx, aly, alz, dt, files, wmodel, 2.5)
     U, _, _ = inputs(o, seed=int(c[10]))
     _check(fx, "STRAIN3", o.strain(U, 3), int(c[13]))
+
+
+def test_oracle_iwmles_matches_reference():
+    """WMODEL = 2 (IWMLES, smooth wall): the oracle against the reference's
+    iwm_init_smooth / iwm_calc_lhs_smooth / iwm_calc_wallstress_smooth over
+    NTIME = 1..4 from its own initialisation and NTIME = 5..15 (three compute
+    steps) from a log-law-consistent state; every plane of its state, the
+    wall stresses and WSM(1)."""
+    import sys
+    sys.path.insert(0, HERE + "/golden")
+    from make_golden import IWM, IWM_CFLMAX, iwm_drive
+    from cases import iwm_inputs
+    from pyoracle import Oracle
+    fx = _load("iwm")
+    c = fx["config"]
+    stride = int(c[11])
+    o = Oracle(int(c[0]), int(c[1]), int(c[2]), cks=2.5, **IWM)
+    o.set_cflmax(IWM_CFLMAX)
+
+    def planes(key, pl):
+        for q in range(pl.shape[0]):
+            got = _sub(pl[q], 1)[::stride]
+            ref = fx[key + "__sub"][q]
+            assert np.array_equal(got, ref), (key, q)
+            s = np.array([pl[q].sum(), (pl[q] * pl[q]).sum(), np.abs(pl[q]).max()])
+            assert np.allclose(s, fx[key + "__sum"][q], rtol=1e-12, atol=0), (key, q)
+
+    U0, P0, _ = inputs(o, seed=7)
+    for nt, U, P in iwm_drive(o, U0, P0, 1, 4):
+        o.wallmodel_iwmles(U, P, 0.003, -0.001, nt)
+    pl, wdt, _ = o.iwm_state()
+    planes("INIT_PLANES", pl)
+    tw, tb = o.get_tau()
+    _check(fx, "INIT_TAUW", tw, stride)
+    _check(fx, "INIT_TAUB", tb, stride)
+    assert o.get_wsm()[0] == fx["INIT_WSM"][0] and wdt == fx["INIT_WSM"][1]
+    U0, P0, pl0, wdt0 = iwm_inputs(o.n1, o.n3, o.mesh()["DY"], IWM["re"], IWM["dt"], seed=1)
+    o.iwm_set(pl0, wdt0)
+    wsm = []
+    for nt, U, P in iwm_drive(o, U0, P0, 5, 15):
+        o.wallmodel_iwmles(U, P, 0.003, -0.001, nt)
+        wsm.append(o.get_wsm()[0])
+        pl, wdt, fixed = o.iwm_state()
+        assert fixed == 0
+        if nt % 5 == 0:
+            planes(f"PLANES_{nt}", pl)
+            assert wdt == fx[f"WDT_{nt}"][0]
+    assert np.array_equal(np.array(wsm), fx["WSM"])
+    tw, tb = o.get_tau()
+    _check(fx, "TAUW", tw, stride)
+    _check(fx, "TAUB", tb, stride)

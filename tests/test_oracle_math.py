@@ -279,3 +279,42 @@ def test_two_wall_log_law_is_the_mirror_image():
     bw, bb = o2.get_tau()
     assert np.array_equal(tw[:, :, 0], bw[:, :, 0]) and np.array_equal(tb[:, :, 0], bb[:, :, 0])
     assert np.max(np.abs(tw[sl + (1,)])) > 0
+
+
+def test_iwmles_equilibrium_branch():
+    """WMODEL = 2 from its own initialisation: the wall-layer integral starts
+    as plug flow (u * Dz), so the first compute step (NTIME = 5) needs linear
+    corrections |A| > 1 at every point and takes the equilibrium branch.  The
+    reference reads an undefined local there; the oracle computes the
+    point's log-law stress, checked here against an independent bisection of
+    u/ut = ln(ut Dz Re)/0.4 + 5 (the Newton iteration stops at a step of
+    1e-6 and returns the iterate before it: 1e-4 relative in tau)."""
+    import math
+    from cases import inputs, iwm_equil
+    from pyoracle import IWM_PLANES, Oracle
+    re = 5200.0
+    o = Oracle(24, 17, 20, re, 2 * math.pi, 2.0, math.pi, 5e-4, 2.0, 2.0, 2.5)
+    o.set_cflmax(0.5)
+    U0, P0, _ = inputs(o, seed=3)
+    for nt in range(1, 6):
+        o.wallmodel_iwmles(U0, P0, 0.003, -0.001, nt)
+    pl, wdt, fixed = o.iwm_state()
+    ix = {n: q for q, n in enumerate(IWM_PLANES)}
+    assert fixed == 23 * 19
+    assert wdt == 5e-4 + 5e-4 + 5e-4 + 5e-4 + 5e-4
+    ux, uy, dz = pl[ix["flt_u"]], pl[ix["flt_w"]], pl[ix["dz"]][0, 0]
+    dy = o.mesh()["DY"]
+    assert dz == dy[1] + dy[2] + dy[3] / 2.0
+    vel = np.sqrt(ux * ux + uy * uy)
+    ut = iwm_equil(vel, dz, re)
+    tx, ty = ut * ut * ux / vel, ut * ut * uy / vel
+    assert np.allclose(pl[ix["tauwx"]], tx, rtol=1e-4, atol=0)
+    assert np.allclose(pl[ix["tauwy"]], ty, rtol=1e-4, atol=1e-4 * np.abs(tx).max())
+    assert np.array_equal(pl[ix["utx"]], np.sqrt(np.abs(pl[ix["tauwx"]])))
+    for n in ("ax", "ay", "cx", "cy"):
+        assert not pl[ix[n]].any()
+    tw, tb = o.get_tau()
+    assert np.array_equal(tw[1:24, 1:20, 0], pl[ix["tauwx"]].T)
+    assert np.array_equal(tb[1:24, 1:20, 0], pl[ix["tauwy"]].T)
+    assert not tw[:, :, 1].any() and not tb[:, :, 1].any()
+    assert o.get_wsm()[0] == pytest.approx(np.sqrt(tx * tx + ty * ty).mean(), rel=2e-4)
