@@ -21,8 +21,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libc3d.so")
 
 FP64, MIXED = 0, 1
-F_RUH1, F_RUH2, F_RUH3, F_UH1, F_UH2, F_UH3, F_RDP, F_DP, F_HH1, F_HH2, F_HH3, F_TAUW, F_TAUB = range(1, 14)
-S_PRESG, S_PRESG3, S_DMPRESG, S_DMPRESG3, S_FLOW1, S_FLOW3, S_WSM1, S_WSM2, S_WSM3, S_REFINES, S_DIVMAX1 = range(11)
+F_RUH1, F_RUH2, F_RUH3, F_UH1, F_UH2, F_UH3, F_RDP, F_DP, F_HH1, F_HH2, F_HH3, F_TAUW, F_TAUB, F_IWM = range(1, 15)
+(S_PRESG, S_PRESG3, S_DMPRESG, S_DMPRESG3, S_FLOW1, S_FLOW3, S_WSM1, S_WSM2, S_WSM3, S_REFINES, S_DIVMAX1,
+ S_NTIME, S_CFLMAX, S_WMDT) = range(14)
+IWM_NPL = 43  # WMODEL = 2: planes of the integral wall model's state (COMMON /IWMLES/)
 NKID = 48  # C3D_NKID of include/c3d.h: entries of the per-kernel timing tables
 
 _dp = C.POINTER(C.c_double)
@@ -78,6 +80,7 @@ def lib():
         L.c3d_format_flow.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
         L.c3d_format_strain.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, C.c_double]
         L.c3d_get_field.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.c3d_set_field.argtypes = [C.c_void_p, C.c_int, _dp]
         L.c3d_get_scalar.argtypes = [C.c_void_p, C.c_int, _dp]
         L.c3d_set_scalar.argtypes = [C.c_void_p, C.c_int, C.c_double]
         L.c3d_set_dt.argtypes = [C.c_void_p, C.c_double]
@@ -238,10 +241,12 @@ class Channel:
         return dict(U=U, P=P, HH=HH, SGSVIS=S, PRESG=a.value, PRESG3=b.value)
 
     # ---- reference-named operations
-    def getup(self, ntime=1):
+    # ntime: GETUP's NTIME (the integral wall model, WMODEL = 2, counts its
+    # steps by it); 0 continues from the last step's
+    def getup(self, ntime=0):
         self._ck(self.L.c3d_getup(self.h, ntime), "getup")
 
-    def step(self, ntime=1, checks=True):
+    def step(self, ntime=0, checks=True):
         if checks:
             d, c = C.c_double(), C.c_double()
             self._ck(self.L.c3d_step(self.h, ntime, C.byref(d), C.byref(c)), "step")
@@ -315,10 +320,19 @@ class Channel:
             out = np.zeros(self.pshape(), order="F")
         elif which in (F_TAUW, F_TAUB):
             out = np.zeros((self.n1 + 1, self.n3 + 1, 2), order="F")
+        elif which == F_IWM:
+            out = np.zeros(IWM_NPL * (self.n3 - 1) * (self.n1 - 1))
+            self._ck(self.L.c3d_get_field(self.h, which, _p(out)), "get_field")
+            return out.reshape(IWM_NPL, self.n3 - 1, self.n1 - 1)
         else:
             out = np.zeros(self.ushape(), order="F")
         self._ck(self.L.c3d_get_field(self.h, which, _p(out)), "get_field")
         return out
+
+    def set_field(self, which, values):
+        """F_IWM only: [plane, K-1, I-1] as field(F_IWM) returns it"""
+        a = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        self._ck(self.L.c3d_set_field(self.h, which, _p(a)), "set_field")
 
     def scalar(self, which):
         v = C.c_double()

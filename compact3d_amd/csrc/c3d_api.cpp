@@ -256,6 +256,7 @@ static void build_mesh(c3d_handle* h) {
   s.dx1q = s.dx1 * s.dx1;
   s.dx3q = s.dx3 * s.dx3;
   s.vol = cf.alx * aly * cf.alz;
+  s.alx = cf.alx;
   s.ifwall = 0;
   s.files = cf.files != 0.0;
   s.wmodel = cf.wmodel;
@@ -452,6 +453,7 @@ long long c3d_footprint_bytes(const c3d_config* cfg) {
   if (cfg->files != 0.0) bytes += 7 * fsz * 8;
   if (cfg->files == 1.0) bytes += (2 * n2m * ((nx + 15) / 16) * ((nz + 15) / 16) + n2m + 3) * 8;
   if (cfg->files == 3.0) bytes += (10 * n2m * ((nx + 15) / 16) * ((nz + 15) / 16) + 2 * (n2m + 3)) * 8;
+  if (cfg->wmodel == 2.0) bytes += (long long)IW_NPL * nx * nz * 8;
   const long long nm = ((nx / 2 + 1 + 7) / 8 * 8) * nz * n2m;
   bytes += nm * 16 + nm * 16;  // spectrum + TDMAP factors
   bytes += 4 * std::max(n2m * nz, nx * n2m) * 8 + 128 * 8;  // flow-rate partial sums, max slots
@@ -521,9 +523,9 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   if (cf.precision != C3D_FP64 && cf.precision != C3D_MIXED) return bad("precision must be C3D_FP64 or C3D_MIXED");
   if (cf.files != 0.0 && cf.files != 1.0 && cf.files != 2.0 && cf.files != 3.0)
     return bad("FILES must be 0 (DNS), 1 (dynamic model), 2 (Smagorinsky) or 3 (scale-dependent dynamic model)");
-  if (cf.wmodel != 0.0 && cf.wmodel != 1.0 && cf.wmodel != 4.0 && cf.wmodel != 6.0)
-    return bad("WMODEL must be 0 (resolved walls), 1 (log law, bottom wall), 4 (Werner-Wengle, both walls) "
-               "or 6 (log law, both walls)");
+  if (cf.wmodel != 0.0 && cf.wmodel != 1.0 && cf.wmodel != 2.0 && cf.wmodel != 4.0 && cf.wmodel != 6.0)
+    return bad("WMODEL must be 0 (resolved walls), 1 (log law, bottom wall), 2 (integral wall model, bottom "
+               "wall), 4 (Werner-Wengle, both walls) or 6 (log law, both walls)");
   // WALLMODEL_WERNER takes its I neighbour from IPA(K) (compact3d.f:4849)
   if (cf.wmodel == 4.0 && cf.n3 > cf.n1) return bad("WMODEL = 4 needs N3 <= N1");
   if (cf.re <= 0 || cf.dt <= 0 || cf.alx <= 0 || cf.alz <= 0) return bad("RE, DT, ALX, ALZ must be > 0");
@@ -714,6 +716,8 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
     return bad(h->err);
   F.tauwe = nullptr;
   if (cf.wmodel == 4.0 && dalloc(h, &F.tauwe, 4 * n2d)) return bad(h->err);
+  F.iwm = nullptr;
+  if (cf.wmodel == 2.0 && dalloc(h, &F.iwm, (size_t)IW_NPL * n2d)) return bad(h->err);
   if (dalloc(h, &F.spec, (size_t)g.nxhp * g.nz * g.n2m)) return bad(h->err);
   if (cf.files != 0.0) {
     double** les[] = {&F.nux, &F.nuz, &F.nue, &F.dvdx, &F.dwdx, &F.dudz, &F.dvdz};
@@ -1015,10 +1019,15 @@ extern "C" int c3d_sgs(c3d_handle* h) {
   return launched(h);
 }
 
-// WALL_MODEL (compact3d.f:1981-1991): WMODEL=0 -> WALLSS, IFWALL=0;
-// WMODEL=1 (Y0=0) -> WALLMODEL_INSTANT, IFWALL=1
-static void wall_model_impl(c3d_handle* h) {
-  if (h->cfg.wmodel == 1.0 || h->cfg.wmodel == 6.0) {
+// WALL_MODEL (compact3d.f:1981-2013): WMODEL=0 -> WALLSS, IFWALL=0;
+// WMODEL=1 (Y0=0) -> WALLMODEL_INSTANT, 2 (Y0=0) -> WALLMODEL_IWMLES_SMOOTH,
+// 4 -> WALLMODEL_WERNER, IFWALL=1.  ntime: GETUP's NTIME (IWMLES counts
+// its steps by it); <= 0 continues from the last call's.
+static void wall_model_impl(c3d_handle* h, int ntime) {
+  if (h->cfg.wmodel == 2.0) {
+    c3dk::wallmodel_iwmles(h->c, ntime);
+    h->c.s.ifwall = 1;
+  } else if (h->cfg.wmodel == 1.0 || h->cfg.wmodel == 6.0) {
     c3dk::wallmodel(h->c, h->cfg.wmodel == 6.0);
     h->c.s.ifwall = 1;
   } else if (h->cfg.wmodel == 4.0) {
@@ -1032,7 +1041,7 @@ static void wall_model_impl(c3d_handle* h) {
 
 extern "C" int c3d_wall_model(c3d_handle* h) {
   if (!h) return -1;
-  wall_model_impl(h);
+  wall_model_impl(h, 0);
   return launched(h);
 }
 
@@ -1043,13 +1052,13 @@ extern "C" int c3d_wallss(c3d_handle* h) {
 }
 
 // the kernels of GETUP (no host synchronisation)
-static int step_impl(c3d_handle* h, bool chk) {
+static int step_impl(c3d_handle* h, bool chk, int ntime) {
   swap_hh(h);
   // GETUP (:925-929): SGS model, then the wall model
   if (h->cfg.files == 1.0) c3dk::sgsdy(h->c);
   if (h->cfg.files == 2.0) c3dk::sgscs(h->c);
   if (h->cfg.files == 3.0) c3dk::sgssd(h->c);
-  if (h->cfg.files != 0.0) wall_model_impl(h);
+  if (h->cfg.files != 0.0) wall_model_impl(h, ntime);
   c3dk::bcond(h->c);
   c3dk::rhs_xz(h->c);
   c3dk::getuh_fused(h->c);
@@ -1060,9 +1069,8 @@ static int step_impl(c3d_handle* h, bool chk) {
 }
 
 extern "C" int c3d_getup(c3d_handle* h, int ntime) {
-  (void)ntime;
   if (!h) return -1;
-  if (step_impl(h, false)) return -1;
+  if (step_impl(h, false, ntime)) return -1;
   return h->c.s.mixed ? refine_impl(h, nullptr) : 0;
 }
 
@@ -1082,7 +1090,7 @@ static void step_launch(c3d_handle* h) {
   if (h->cfg.files == 1.0) c3dk::sgsdy(h->c);
   if (h->cfg.files == 2.0) c3dk::sgscs(h->c);
   if (h->cfg.files == 3.0) c3dk::sgssd(h->c);
-  if (h->cfg.files != 0.0) wall_model_impl(h);
+  if (h->cfg.files != 0.0) wall_model_impl(h, 0);
   c3dk::bcond(h->c);
   c3dk::rhs_xz(h->c);
   c3dk::getuh_fused(h->c);
@@ -1572,12 +1580,11 @@ extern "C" int c3d_ensemble_sync(c3d_ensemble* e, double* divmax, double* cflm) 
 }
 
 extern "C" int c3d_step(c3d_handle* h, int ntime, double* divmax, double* cflm) {
-  (void)ntime;
   if (!h) return -1;
   // MAIN (:125) evaluates WALLSS before GETUP when WMODEL=0; only the SGS
   // model consumes it
   if (h->cfg.wmodel == 0.0 && h->cfg.files != 0.0) c3dk::wallss(h->c);
-  if (step_impl(h, true)) return -1;
+  if (step_impl(h, true, ntime)) return -1;
   // one read-back of DIVMAX and CFLM per step (mixed mode: the refinement
   // check uses the same DIVMAX)
   double dm = 0.0, cf = 0.0;
@@ -1840,6 +1847,13 @@ extern "C" int c3d_get_field(c3d_handle* h, int which, double* out) {
             out[l * lay + (size_t)(i + 1) + (size_t)(n1 + 1) * (k + 1)] = t[l * n2d + (size_t)k * g.nx + i];
       return 0;
     }
+    case C3D_F_IWM: {
+      if (!F.iwm) return fail(h, "the integral wall model's state exists only with WMODEL = 2");
+      const size_t nb = (size_t)IW_NPL * h->c.g.nx * h->c.g.nz * sizeof(double);
+      HIPCHK(hipMemcpyAsync(out, F.iwm, nb, hipMemcpyDeviceToHost, h->c.st));
+      HIPCHK(hipStreamSynchronize(h->c.st));
+      return 0;
+    }
     default: return fail(h, "unknown field");
   }
   if (h->c.s.mixed && which != C3D_F_DP && which != C3D_F_RDP) {
@@ -1856,22 +1870,39 @@ extern "C" int c3d_get_field(c3d_handle* h, int which, double* out) {
   return from_dev(h, out, src, lay);
 }
 
+extern "C" int c3d_set_field(c3d_handle* h, int which, const double* in) {
+  if (!h || !in) return -1;
+  if (which != C3D_F_IWM) return fail(h, "only C3D_F_IWM can be set");
+  if (!h->c.d.iwm) return fail(h, "the integral wall model's state exists only with WMODEL = 2");
+  const size_t nb = (size_t)IW_NPL * h->c.g.nx * h->c.g.nz * sizeof(double);
+  HIPCHK(hipMemcpyAsync(h->c.d.iwm, in, nb, hipMemcpyHostToDevice, h->c.st));
+  HIPCHK(hipStreamSynchronize(h->c.st));
+  return 0;
+}
+
+// c3d_get_scalar / c3d_set_scalar index -> device slot (-1: not a slot)
+static int scalar_slot(int which) {
+  static const int map[14] = {SC_PRESG, SC_PRESG3, SC_DMPRESG, SC_DMPRESG3, SC_FLOW1, SC_FLOW3, SC_WSM1,
+                              SC_WSM2,  SC_WSM3,   -1,         -1,          SC_NTIME, SC_CFLMAX, SC_WMDT};
+  return which >= 0 && which < 14 ? map[which] : -1;
+}
+
 extern "C" int c3d_get_scalar(c3d_handle* h, int which, double* out) {
   if (!h || !out) return -1;
   if (which == 9 || which == 10) {  // mixed mode diagnostics
     *out = which == 9 ? (double)h->refines : h->first_divmax;
     return 0;
   }
-  const int map[9] = {SC_PRESG, SC_PRESG3, SC_DMPRESG, SC_DMPRESG3, SC_FLOW1, SC_FLOW3, SC_WSM1, SC_WSM2, SC_WSM3};
-  if (which < 0 || which > 8) return fail(h, "unknown scalar");
-  return read_sc(h, map[which], out);
+  const int slot = scalar_slot(which);
+  if (slot < 0) return fail(h, "unknown scalar");
+  return read_sc(h, slot, out);
 }
 
 extern "C" int c3d_set_scalar(c3d_handle* h, int which, double v) {
   if (!h) return -1;
-  const int map[9] = {SC_PRESG, SC_PRESG3, SC_DMPRESG, SC_DMPRESG3, SC_FLOW1, SC_FLOW3, SC_WSM1, SC_WSM2, SC_WSM3};
-  if (which < 0 || which > 8) return fail(h, "unknown scalar");
-  HIPCHK(hipMemcpyAsync(h->c.d.sc + map[which], &v, sizeof(double), hipMemcpyHostToDevice, h->c.st));
+  const int slot = scalar_slot(which);
+  if (slot < 0) return fail(h, "unknown scalar");
+  HIPCHK(hipMemcpyAsync(h->c.d.sc + slot, &v, sizeof(double), hipMemcpyHostToDevice, h->c.st));
   HIPCHK(hipStreamSynchronize(h->c.st));
   return 0;
 }

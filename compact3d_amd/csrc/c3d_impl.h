@@ -13,7 +13,19 @@ enum {
   SC_DIVMAX,  // stored as uint64 bit pattern (atomicMax on non-negative doubles)
   SC_CFLM,    // same
   SC_CHK1, SC_CHK3,
-  SC_COUNT = 16
+  // WMODEL = 2: NTIME of the last wall-model call, wm_dt, CFLMAX, the
+  // initial mean velocity and friction velocity, the mean first-cell pressure
+  SC_NTIME, SC_WMDT, SC_CFLMAX, SC_UINIT, SC_USINIT, SC_PBAR,
+  SC_COUNT = 24
+};
+
+// WMODEL = 2 (the integral wall model): its state planes, NX*NZ doubles
+// each ([(K-1)*NX + I-1]), in the order of the reference's COMMON /IWMLES/
+// (compact3d.f:3399-3408); the five integrals are those of u, uu, w, ww, uw
+enum {
+  IW_UTX = 0, IW_UTY, IW_TAUWX, IW_TAUWY, IW_FLT, IW_FLTM = 6, IW_FLTP = 8, IW_INTE = 9, IW_INTEM = 14,
+  IW_UNSDY = 19, IW_CONV = 21, IW_PRSG = 23, IW_DIFF = 25, IW_LHS = 27, IW_DUDZT = 29, IW_DUDZB = 31,
+  IW_FLTUS = 33, IW_TR, IW_DZ, IW_Z0, IW_AX, IW_AY, IW_DELV, IW_DELI, IW_CX, IW_CY, IW_NPL
 };
 
 // metric arrays on the device, indexed by the reference's J (0..N2+1)
@@ -38,7 +50,7 @@ struct Metrics {
 };
 
 struct Scal {
-  double re, rinv, dt, dx1, dx1q, dx3, dx3q, aly, vol;
+  double re, rinv, dt, dx1, dx1q, dx3, dx3q, aly, vol, alx;
   int ifwall;
   int mixed;  // C3D_MIXED: FP32 intermediates (see Fields)
   int files;  // FILES != 0
@@ -107,6 +119,7 @@ struct Fields {
   // L = 1 bottom, 2 top (zero unless a two-wall model sets it)
   double *tauw, *taub;
   double *tauwe;          // WMODEL = 4: the cell-centre stresses TAUWE(I,K,1:4)
+  double *iwm;            // WMODEL = 2: IW_NPL state planes
   double2 *spec;          // Poisson spectrum [jm][k][kx], NXH*NZ*N2M
   double *pib, *pgt;      // Poisson TDMAP factors per mode: 1/BET(J), GAM(J+1) [jm][kz][kx]
   double *sc;             // device scalars (SC_*)
@@ -205,6 +218,9 @@ void sgssd(const Ctx& c);            // SGSSD, the scale-dependent dynamic model
 int sgsdy_tiles(const Geo& g);      // tiles per plane of its partial sums
 void wallmodel(const Ctx& c, bool both = false);  // WALLMODEL_INSTANT; both: also the top wall (WMODEL = 6)
 void wallmodel_werner(const Ctx& c);  // WALLMODEL_WERNER (WMODEL = 4), both walls
+// WALLMODEL_IWMLES_SMOOTH (WMODEL = 2); ntime > 0: this call's NTIME, else
+// the last call's + 1 (graph replays)
+void wallmodel_iwmles(const Ctx& c, int ntime);
 void wallss(const Ctx& c);
 void sgs_terms(const Ctx& c);
 void init_attrs();

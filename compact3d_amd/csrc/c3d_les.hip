@@ -627,6 +627,387 @@ __global__ __launch_bounds__(C3D_NT) void k_wallmodel_fin(Ctx c, int nblk, int z
   }
 }
 
+// WALLMODEL_IWMLES_SMOOTH (compact3d.f:3386-4329), WMODEL = 2: the integral
+// wall model of Yang et al. (2015) at the bottom wall.  Every fifth step
+// (NTIME a multiple of 5) the third cell's velocity and the first cell's
+// pressure fluctuation are time-filtered (k_iwm_filter), the integrated
+// momentum balance of the wall layer is formed from the integrals of the
+// last compute step and their x/z differences (k_iwm_lhs), and each point
+// solves it for its friction velocities (utx, uty) by Newton's method with a
+// finite-difference Jacobian, then stores the new integrals, the diffusion
+// term, the stresses and the filter weight (k_iwm_solve).  Every step the
+// stored stresses are imposed and WSM(1) = TAOW = mean |tau| (k_iwm_apply,
+// k_wallmodel_fin).  The kernels read NTIME from the device (k_iwm_begin
+// advances it), so a captured step replays correctly.  As in the
+// reference: x**2.0 and x**3.0 are products, x**4.0 and x**0.25 pow.
+// Where the reference is undefined: the spanwise stress goes to
+// TAUB(I,K,1), the element its TAUW(I,K,3) addresses through COMMON
+// /TAUXYZ/ (:3549); the excessive-linear-correction branch (:4139) uses the
+// point's own equilibrium stress, computed there when the Newton iteration
+// had not (the reference reads a stale local); the equilibrium iterations
+// stop after WM_MAXIT iterations.  No multiply-add contraction: the
+// arithmetic is the reference's operation by operation.
+#define IWM_SKIP 5
+
+__device__ double iwm_equil(double u_ave, double deltap, double re) {
+#pragma clang fp contract(off)
+  const double KAPA = 0.4, B = 5.0;
+  double ut1 = 1.0, ut2 = 0.000001, f1, f2, f3, lam = 1.0;
+  int it = 0;
+  while (fabs(ut2 - ut1) > 1.0E-6 && it < WM_MAXIT) {
+    ut1 = ut2;
+    f1 = u_ave / ut1 - log(ut1 * deltap * re) / KAPA - B;
+    f2 = -u_ave / (ut1 * ut1) - 1. / KAPA / ut1;
+    ut2 = ut1 - lam * f1 / f2;
+    f3 = u_ave / ut2 - log(ut2 * deltap * re) / KAPA - B;
+    if (fabs(f3) > fabs(f1)) lam = 0.5 * lam;
+    it++;
+  }
+  return ut1;
+}
+
+// iwm_slv_smooth (:3881-3926)
+__device__ void iwm_slv(double lhsx, double lhsy, double Ux, double Uy, double Dz, double utx, double uty,
+                        double re, double* fx, double* fy) {
+#pragma clang fp contract(off)
+  const double vonk = 0.4, reinv = 1.0 / re;
+  const double us = pow(pow(utx, 4.0) + pow(uty, 4.0), 0.25);
+  const double delv = reinv / us;
+  const double deli = fmin(11.0 * reinv / us, Dz);
+  const double Vel = sqrt(Ux * Ux + Uy * Uy);
+  const double lg = log(deli / Dz);
+  const double Ax = (Ux / us + Ux / Vel / vonk * lg - deli / delv * utx / us) / (1.0 - deli / Dz);
+  const double Ay = (Uy / us + Uy / Vel / vonk * lg - deli / delv * uty / us) / (1.0 - deli / Dz);
+  const double Cx = Ux / us - Ax;
+  const double Cy = Uy / us - Ay;
+  *fx = 0.5 * utx * (deli * deli) / delv +
+        us * Dz * (0.5 * Ax * (1.0 - (deli * deli) / (Dz * Dz)) + Cx * (1.0 - deli / Dz) -
+                   1.0 / vonk * Ux / Vel * (1.0 - deli / Dz + deli / Dz * lg)) +
+        lhsx;
+  *fy = 0.5 * uty * (deli * deli) / delv +
+        us * Dz * (0.5 * Ay * (1.0 - (deli * deli) / (Dz * Dz)) + Cy * (1.0 - deli / Dz) -
+                   1.0 / vonk * Uy / Vel * (1.0 - deli / Dz + deli / Dz * lg)) +
+        lhsy;
+}
+
+__device__ __forceinline__ double iwm_dzp(const Ctx& c) { return c.m.dy[1] + c.m.dy[2] + c.m.dy[3] / 2.0; }
+
+// NTIME and wm_dt (:3528-3533); at NTIME = 1 the mean third-cell velocity
+// and its friction velocity (iwm_init_smooth, :3598-3628); at compute steps
+// the mean first-cell pressure (:3786-3797).  One block.
+__global__ __launch_bounds__(C3D_NT) void k_iwm_begin(Ctx c, int ntime) {
+#pragma clang fp contract(off)
+  __shared__ double red[C3D_NT];
+  __shared__ int snt;
+  const Geo g = c.g;
+  const Fields F = c.d;
+  const int t = threadIdx.x;
+  if (t == 0) {
+    const int nt = ntime > 0 ? ntime : (int)F.sc[SC_NTIME] + 1;
+    F.sc[SC_NTIME] = (double)nt;
+    F.sc[SC_WMDT] = (nt % IWM_SKIP == 1) ? c.s.dt : F.sc[SC_WMDT] + c.s.dt;
+    snt = nt;
+  }
+  __syncthreads();
+  const int nt = snt;
+  const int n = g.nx * g.nz;
+  for (int pass = 0; pass < 2; pass++) {
+    if (pass == 0 && nt != 1) continue;
+    if (pass == 1 && nt % IWM_SKIP != 0) continue;
+    double a = 0.0;
+    for (int q = t; q < n; q += C3D_NT) {
+      const int i = q % g.nx, k = q / g.nx;
+      a += pass == 0 ? F.u[gidx(g, i, 3, k)] : F.p[gidx(g, i, 1, k)];
+    }
+    red[t] = a;
+    __syncthreads();
+    for (int h = C3D_NT / 2; h > 0; h >>= 1) {
+      if (t < h) red[t] += red[t + h];
+      __syncthreads();
+    }
+    if (t == 0) {
+      if (pass == 0) {
+        const double uinit = red[0] / (double)n;
+        F.sc[SC_UINIT] = uinit;
+        F.sc[SC_USINIT] = iwm_equil(uinit, iwm_dzp(c), c.s.re);
+      } else {
+        F.sc[SC_PBAR] = red[0] / (double)g.nx / (double)g.nz;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// iwm_init_smooth (:3629-3710), at NTIME = 1
+__global__ __launch_bounds__(C3D_NT) void k_iwm_init(Ctx c) {
+#pragma clang fp contract(off)
+  const Fields F = c.d;
+  if ((int)F.sc[SC_NTIME] != 1) return;
+  const unsigned n = (unsigned)(c.g.nx * c.g.nz);
+  const unsigned t = blockIdx.x * C3D_NT + threadIdx.x;
+  if (t >= n) return;
+  const double vonk = 0.4, REINV = 1.0 / c.s.re, Dzp = iwm_dzp(c);
+  const double uinit = F.sc[SC_UINIT], usinit = F.sc[SC_USINIT];
+  double* W = F.iwm;
+  for (int q = 0; q < IW_NPL; q++) W[q * n + t] = 0.0;
+  W[IW_UTX * n + t] = usinit;
+  W[IW_TAUWX * n + t] = usinit * usinit;
+  W[IW_FLT * n + t] = uinit;
+  W[IW_FLTM * n + t] = uinit;
+  W[(IW_INTE + 0) * n + t] = uinit * Dzp;
+  W[(IW_INTE + 1) * n + t] = uinit * uinit * Dzp;
+  W[(IW_INTEM + 0) * n + t] = uinit * Dzp;
+  W[(IW_INTEM + 1) * n + t] = uinit * uinit * Dzp;
+  W[IW_LHS * n + t] = -uinit * Dzp;
+  W[(IW_LHS + 1) * n + t] = -uinit * Dzp;
+  W[IW_DUDZT * n + t] = usinit / vonk / Dzp;
+  W[IW_FLTUS * n + t] = usinit;
+  W[IW_TR * n + t] = (F.sc[SC_CFLMAX] * c.s.alx / c.g.nx / uinit) / (Dzp / vonk / usinit);
+  W[IW_DZ * n + t] = Dzp;
+  W[IW_DELI * n + t] = fmin(11.0 * REINV / usinit, Dzp);
+  W[IW_DELV * n + t] = REINV / usinit;
+}
+
+// the temporal filter of iwm_calc_lhs_smooth (:3766-3817)
+__global__ __launch_bounds__(C3D_NT) void k_iwm_filter(Ctx c) {
+#pragma clang fp contract(off)
+  const Fields F = c.d;
+  if ((int)F.sc[SC_NTIME] % IWM_SKIP != 0) return;
+  const Geo g = c.g;
+  const unsigned n = (unsigned)(g.nx * g.nz);
+  const unsigned t = blockIdx.x * C3D_NT + threadIdx.x;
+  if (t >= n) return;
+  const int i = t % g.nx, k = t / g.nx;
+  double* W = F.iwm;
+  const double tr = W[IW_TR * n + t];
+  const double fu = W[IW_FLT * n + t], fw = W[(IW_FLT + 1) * n + t];
+  W[IW_FLTM * n + t] = fu;
+  W[(IW_FLTM + 1) * n + t] = fw;
+  W[IW_FLT * n + t] = fu * (1.0 - tr) + F.u[gidx(g, i, 3, k)] * tr;
+  W[(IW_FLT + 1) * n + t] = fw * (1.0 - tr) + F.w[gidx(g, i, 3, k)] * tr;
+  W[IW_FLTP * n + t] = W[IW_FLTP * n + t] * (1.0 - tr) + (F.p[gidx(g, i, 1, k)] - F.sc[SC_PBAR]) * tr;
+}
+
+// the balance of iwm_calc_lhs_smooth (:3821-3875): unsteady, convective
+// and pressure-gradient terms and the left-hand side
+__global__ __launch_bounds__(C3D_NT) void k_iwm_lhs(Ctx c) {
+#pragma clang fp contract(off)
+  const Fields F = c.d;
+  if ((int)F.sc[SC_NTIME] % IWM_SKIP != 0) return;
+  const Geo g = c.g;
+  const unsigned n = (unsigned)(g.nx * g.nz);
+  const unsigned t = blockIdx.x * C3D_NT + threadIdx.x;
+  if (t >= n) return;
+  const int i = t % g.nx, k = t / g.nx;
+  const unsigned ip = k * g.nx + (i + 1 < g.nx ? i + 1 : 0), im = k * g.nx + (i > 0 ? i - 1 : g.nx - 1);
+  const unsigned kp = (k + 1 < g.nz ? k + 1 : 0) * g.nx + i, km = (k > 0 ? k - 1 : g.nz - 1) * g.nx + i;
+  const double DX1 = c.s.dx1, DX3 = c.s.dx3, wm_dt = F.sc[SC_WMDT];
+  const double presg = F.sc[SC_PRESG], presg3 = F.sc[SC_PRESG3];
+  double* W = F.iwm;
+  const double* I0 = W + IW_INTE * n;  // integrals: u, uu, w, ww, uw
+  const double Dz = W[IW_DZ * n + t];
+  W[IW_UNSDY * n + t] = (I0[t] - W[IW_INTEM * n + t]) / wm_dt;
+  W[(IW_UNSDY + 1) * n + t] = (I0[2 * n + t] - W[(IW_INTEM + 2) * n + t]) / wm_dt;
+  const double uux = (I0[n + ip] - I0[n + im]) * DX1 / 2.0;
+  const double uvy = (I0[4 * n + kp] - I0[4 * n + km]) * DX3 / 2.0;
+  const double uvx = (I0[4 * n + ip] - I0[4 * n + im]) * DX1 / 2.0;
+  const double vvy = (I0[3 * n + kp] - I0[3 * n + km]) * DX3 / 2.0;
+  const double ux = (I0[ip] - I0[im]) * DX1 / 2.0;
+  const double vy = (I0[2 * n + kp] - I0[2 * n + km]) * DX3 / 2.0;
+  const double cx = uux + uvy - W[IW_FLTM * n + t] * (ux + vy);
+  const double cy = uvx + vvy - W[(IW_FLTM + 1) * n + t] * (ux + vy);
+  const double* FP = W + IW_FLTP * n;
+  const double px = (FP[ip] - FP[im]) * DX1 / 2.0 * Dz + presg * Dz;
+  const double py = (FP[kp] - FP[km]) * DX3 / 2.0 * Dz + presg3 * Dz;
+  W[IW_CONV * n + t] = cx;
+  W[(IW_CONV + 1) * n + t] = cy;
+  W[IW_PRSG * n + t] = px;
+  W[(IW_PRSG + 1) * n + t] = py;
+  W[IW_LHS * n + t] = -I0[t] + wm_dt * (cx + px - W[IW_DIFF * n + t]);
+  W[(IW_LHS + 1) * n + t] = -I0[2 * n + t] + wm_dt * (cy + py - W[(IW_DIFF + 1) * n + t]);
+}
+
+// iwm_calc_wallstress_smooth (:3930-4279) at one point per thread
+__global__ __launch_bounds__(C3D_NT) void k_iwm_solve(Ctx c) {
+#pragma clang fp contract(off)
+  const Fields F = c.d;
+  if ((int)F.sc[SC_NTIME] % IWM_SKIP != 0) return;
+  const unsigned n = (unsigned)(c.g.nx * c.g.nz);
+  const unsigned t = blockIdx.x * C3D_NT + threadIdx.x;
+  if (t >= n) return;
+  double* W = F.iwm;
+  const double vonk = 0.4, re = c.s.re, reinv = 1.0 / re;
+  const double deltap = iwm_dzp(c);
+  const int MaxIter = 1500;
+  const double wm_tol = 0.000001, wm_eps = 0.000000001;
+  const double lx = W[IW_LHS * n + t], ly = W[(IW_LHS + 1) * n + t];
+  const double Ux = W[IW_FLT * n + t], Uy = W[(IW_FLT + 1) * n + t], Dz = W[IW_DZ * n + t];
+  double utx = W[IW_UTX * n + t], uty = W[IW_UTY * n + t];
+  double fx, fy, fxp, fyp;
+  iwm_slv(lx, ly, Ux, Uy, Dz, utx, uty, re, &fx, &fy);
+  int iter = 0;
+  bool equil = false, have_eq = false;
+  while (fmax(fabs(fx), fabs(fy)) > wm_tol) {
+    iwm_slv(lx, ly, Ux, Uy, Dz, utx + wm_eps, uty, re, &fxp, &fyp);
+    const double a11 = (fxp - fx) / wm_eps, a21 = (fyp - fy) / wm_eps;
+    iwm_slv(lx, ly, Ux, Uy, Dz, utx, uty + wm_eps, re, &fxp, &fyp);
+    const double a12 = (fxp - fx) / wm_eps, a22 = (fyp - fy) / wm_eps;
+    const double nx = utx - (a22 * fx - a12 * fy) / (a11 * a22 - a12 * a21);
+    const double ny = uty - (-a21 * fx + a11 * fy) / (a11 * a22 - a12 * a21);
+    utx = nx;
+    uty = ny;
+    iwm_slv(lx, ly, Ux, Uy, Dz, utx, uty, re, &fx, &fy);
+    iter = iter + 1;
+    if (iter > MaxIter) {
+      equil = true;
+      break;
+    }
+  }
+  if (utx - 1.0 == utx || uty - 1.0 == uty) equil = true;
+  double equilutx = 0.0, equiluty = 0.0;
+  if (equil) {
+    const double U_ave = sqrt(Ux * Ux + Uy * Uy);
+    const double ut = iwm_equil(U_ave, deltap, re);
+    const double TAUW1 = ut * ut;
+    equilutx = TAUW1 * Ux / U_ave;
+    equiluty = TAUW1 * Uy / U_ave;
+    utx = sqrt(fabs(equilutx));
+    uty = sqrt(fabs(equiluty));
+    have_eq = true;
+  }
+  // the Newton (or equilibrium) friction velocities: everything below uses
+  // these, also where the correction branch stores other values (:4110-4147)
+  const double putx = utx, puty = uty;
+  const double utaup = pow(pow(putx, 4.0) + pow(puty, 4.0), 0.25);
+  const double delv = reinv / utaup;
+  const double deli = fmin(11.0 * reinv / utaup, Dz);
+  const double Vel = sqrt(Ux * Ux + Uy * Uy);
+  const double lg = log(deli / Dz);
+  double Ax = 0.0, Ay = 0.0, Cx = 0.0, Cy = 0.0;
+  if (!equil) {
+    Ax = (Ux / utaup + Ux / Vel / vonk * lg - deli / delv * putx / utaup) / (1.0 - deli / Dz);
+    Ay = (Uy / utaup + Uy / Vel / vonk * lg - deli / delv * puty / utaup) / (1.0 - deli / Dz);
+    Cx = Ux / utaup - Ax;
+    Cy = Uy / utaup - Ay;
+  }
+  if (fabs(Ax) > 1.0 || fabs(Ay) > 1.0) {
+    if (!have_eq) {
+      const double U_ave = sqrt(Ux * Ux + Uy * Uy);
+      const double ut = iwm_equil(U_ave, deltap, re);
+      const double TAUW1 = ut * ut;
+      equilutx = TAUW1 * Ux / U_ave;
+      equiluty = TAUW1 * Uy / U_ave;
+    }
+    equil = true;
+    utx = sqrt(fabs(equilutx));
+    uty = sqrt(fabs(equiluty));
+    Ax = Ay = Cx = Cy = 0.0;
+  }
+  W[IW_UTX * n + t] = utx;
+  W[IW_UTY * n + t] = uty;
+  W[IW_AX * n + t] = Ax;
+  W[IW_AY * n + t] = Ay;
+  W[IW_CX * n + t] = Cx;
+  W[IW_CY * n + t] = Cy;
+  for (int q = 0; q < 5; q++) W[(IW_INTEM + q) * n + t] = W[(IW_INTE + q) * n + t];
+  const double d2 = deli * deli, d3 = deli * deli * deli, z2 = Dz * Dz, z3 = Dz * Dz * Dz;
+  const double dv2 = delv * delv, us2 = utaup * utaup, V2 = Vel * Vel, k2 = vonk * vonk;
+  W[IW_INTE * n + t] = 0.5 * putx * d2 / delv +
+                       utaup * Dz * (0.5 * Ax * (1.0 - d2 / z2) + Cx * (1.0 - deli / Dz) -
+                                     1.0 / vonk * Ux / Vel * (1.0 - deli / Dz + deli / Dz * lg));
+  W[(IW_INTE + 2) * n + t] = 0.5 * puty * d2 / delv +
+                             utaup * Dz * (0.5 * Ay * (1.0 - d2 / z2) + Cy * (1.0 - deli / Dz) -
+                                           1.0 / vonk * Uy / Vel * (1.0 - deli / Dz + deli / Dz * lg));
+  W[(IW_INTE + 4) * n + t] =
+      1.0 / 3.0 * putx * puty * d3 / dv2 +
+      us2 * Dz *
+          (-1.0 / vonk * (Ax * Uy / Vel + Ay * Ux / Vel) * (0.25 - 0.25 * d2 / z2 + 0.5 * d2 / Dz * lg) -
+           1.0 / vonk * (Cx * Uy / Vel + Cy * Ux / Vel) * (1.0 - deli / Dz + deli / Dz * lg) -
+           Ux * Uy / V2 / k2 * (deli / Dz - 2.0 + deli / Dz * ((lg - 1.0) * (lg - 1.0))) +
+           1.0 / 3.0 * Ax * Ay * (1.0 - d3 / z3) + 0.5 * (Ax * Cy + Ay * Cx) * (1.0 - d2 / z2) +
+           Cx * Cy * (1.0 - deli / Dz));
+  {
+    const double tx = Cx - 1.0 / vonk * Ux / Vel, sx = Cx - Ux / Vel / vonk + Ux / Vel / vonk * lg;
+    const double ty = Cy - 1.0 / vonk * Uy / Vel, sy = Cy - Uy / Vel / vonk + Uy / Vel / vonk * lg;
+    W[(IW_INTE + 1) * n + t] =
+        1.0 / 3.0 * (putx * putx) * d3 / dv2 +
+        us2 * Dz *
+            (-Ax * Ux / Vel / vonk * d2 / z2 * lg + Ax * (Cx - 1.0 / 2.0 / vonk * Ux / Vel) * (1.0 - d2 / z2) +
+             Ax * Ax / 3.0 * (1.0 - d3 / z3) + tx * tx - deli / Dz * (sx * sx) +
+             1.0 / k2 * (Ux * Ux) / V2 * (1.0 - deli / Dz));
+    W[(IW_INTE + 3) * n + t] =
+        1.0 / 3.0 * (puty * puty) * d3 / dv2 +
+        us2 * Dz *
+            (-Ay * Uy / Vel / vonk * d2 / z2 * lg + Ay * (Cy - 1.0 / 2.0 / vonk * Uy / Vel) * (1.0 - d2 / z2) +
+             Ay * Ay / 3.0 * (1.0 - d3 / z3) + ty * ty - deli / Dz * (sy * sy) +
+             1.0 / k2 * (Uy * Uy) / V2 * (1.0 - deli / Dz));
+  }
+  double tx, ty;
+  if (deli < Dz) {
+    tx = utaup / Dz * (Ax + 1.0 / vonk * Ux / Vel);
+    ty = utaup / Dz * (Ay + 1.0 / vonk * Uy / Vel);
+  } else {
+    tx = putx / delv;
+    ty = puty / delv;
+  }
+  const double bx = putx / delv, by = puty / delv;
+  W[IW_DUDZT * n + t] = tx;
+  W[(IW_DUDZT + 1) * n + t] = ty;
+  W[IW_DUDZB * n + t] = bx;
+  W[(IW_DUDZB + 1) * n + t] = by;
+  const double dVelzT = fabs(Ux / Vel * tx + Uy / Vel * ty);
+  if (deli < Dz) {
+    W[IW_DIFF * n + t] = (reinv + (vonk * Dz) * (vonk * Dz) * dVelzT) * tx - reinv * bx;
+    W[(IW_DIFF + 1) * n + t] = (reinv + (vonk * Dz) * (vonk * Dz) * dVelzT) * ty - reinv * by;
+  } else {
+    W[IW_DIFF * n + t] = 0.0;
+    W[(IW_DIFF + 1) * n + t] = 0.0;
+  }
+  double twx, twy;
+  if (equil) {
+    twx = equilutx;
+    twy = equiluty;
+  } else if (deli < Dz) {
+    twx = putx * putx;
+    twy = puty * puty;
+  } else {
+    twx = reinv * Ux / Dz;
+    twy = reinv * Uy / Dz;
+  }
+  W[IW_TAUWX * n + t] = twx;
+  W[IW_TAUWY * n + t] = twy;
+  const double tR = W[IW_TR * n + t];
+  const double fus = W[IW_FLTUS * n + t] * (1.0 - tR) + utaup * tR;
+  W[IW_FLTUS * n + t] = fus;
+  double ntr = F.sc[SC_WMDT] / (Dz / fus / vonk);
+  if (ntr > 1.0) ntr = 1.0;
+  W[IW_TR * n + t] = ntr;
+}
+
+// iwm_wallstress_smooth's imposed stresses (:3541-3553) and the block sums
+// of |tau| for WSM(1) (:3463-3476); k_wallmodel_fin completes WSM(1)
+__global__ __launch_bounds__(C3D_NT) void k_iwm_apply(Ctx c) {
+#pragma clang fp contract(off)
+  __shared__ double red[C3D_NT];
+  const Fields F = c.d;
+  const unsigned n = (unsigned)(c.g.nx * c.g.nz);
+  double acc = 0.0;
+  for (unsigned t = blockIdx.x * C3D_NT + threadIdx.x; t < n; t += gridDim.x * C3D_NT) {
+    const double a = F.iwm[IW_TAUWX * n + t], b = F.iwm[IW_TAUWY * n + t];
+    F.tauw[t] = a;
+    F.taub[t] = b;
+    acc += sqrt(a * a + b * b);
+  }
+  const int t = threadIdx.x;
+  red[t] = acc;
+  __syncthreads();
+  for (int h = C3D_NT / 2; h > 0; h >>= 1) {
+    if (t < h) red[t] += red[t + h];
+    __syncthreads();
+  }
+  if (t == 0) F.red[blockIdx.x * 4] = red[0];
+}
+
 // WALLMODEL_WERNER (compact3d.f:4815-4958), WMODEL = 4: the Werner-Wengle
 // power law at both walls from the first cell's centre velocity
 // (|u| <= AB3 sqrt|WSM(1)|/11.81: the viscous branch 2|u|/DELTAP/RE), WSM(1)
@@ -1187,6 +1568,19 @@ void wallmodel(const Ctx& c, bool both) {
   KScope ks_(c, KID_WALL);
   hipLaunchKernelGGL(k_wallmodel_fin, dim3(1), dim3(C3D_NT), 0, c.st, c, nb, 0);
   if (both) hipLaunchKernelGGL(k_wallmodel, dim3(nb), dim3(C3D_NT), 0, c.st, c, 1);
+}
+
+void wallmodel_iwmles(const Ctx& c, int ntime) {
+  const long long n2d = (long long)c.g.nx * c.g.nz;
+  const int nb = std::min(1024, pgrid(n2d));
+  KScope ks_(c, KID_WALL);
+  hipLaunchKernelGGL(k_iwm_begin, dim3(1), dim3(C3D_NT), 0, c.st, c, ntime);
+  hipLaunchKernelGGL(k_iwm_init, dim3(pgrid(n2d)), dim3(C3D_NT), 0, c.st, c);
+  hipLaunchKernelGGL(k_iwm_filter, dim3(pgrid(n2d)), dim3(C3D_NT), 0, c.st, c);
+  hipLaunchKernelGGL(k_iwm_lhs, dim3(pgrid(n2d)), dim3(C3D_NT), 0, c.st, c);
+  hipLaunchKernelGGL(k_iwm_solve, dim3(pgrid(n2d)), dim3(C3D_NT), 0, c.st, c);
+  hipLaunchKernelGGL(k_iwm_apply, dim3(nb), dim3(C3D_NT), 0, c.st, c);
+  hipLaunchKernelGGL(k_wallmodel_fin, dim3(1), dim3(C3D_NT), 0, c.st, c, nb, 0);
 }
 
 void wallmodel_werner(const Ctx& c) {

@@ -89,6 +89,12 @@ module c3d_iface
       integer(c_int), value :: which
       real(c_double), intent(out) :: v
     end function
+    integer(c_int) function c3d_set_scalar(h, which, v) bind(C, name='c3d_set_scalar')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: which
+      real(c_double), value :: v
+    end function
     integer(c_int) function c3d_set_dt(h, dt) bind(C, name='c3d_set_dt')
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: h

@@ -40,6 +40,18 @@ subroutine C3D_BEGIN(N1, N2, N3, RE, ALX, ALY, ALZ, DT, FILES, WMODEL, CKS, PREC
   call c3d_check(rc, h3d, 'c3d_upload_state')
 end subroutine C3D_BEGIN
 
+! CFLMAX of /TSTEP/ (compact3d.f:247), which the integral wall model
+! (WMODEL = 2) uses for its first time-filter weight (:3700); call after
+! C3D_BEGIN
+subroutine C3D_SET_CFLMAX(CFLMAX)
+  use c3d_state
+  implicit none
+  real(8), intent(in) :: CFLMAX
+  integer(c_int) :: rc
+  rc = c3d_set_scalar(h3d, 12_c_int, CFLMAX)
+  call c3d_check(rc, h3d, 'c3d_set_scalar')
+end subroutine C3D_SET_CFLMAX
+
 subroutine C3D_SYNC(U, P, HH, SGSVIS, PRESG, PRESG3)
   use c3d_state
   implicit none

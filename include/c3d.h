@@ -51,6 +51,8 @@ typedef struct c3d_config {
   double files;       /* SGS model switch FILES: 0 none (DNS), 1 dynamic (SGSDY), 2 Smagorinsky (SGSCS),
                          3 scale-dependent dynamic (SGSSD) */
   double wmodel;      /* WMODEL: 0 resolved walls, 1 instantaneous log-law wall model (bottom wall),
+                         2 the integral wall model IWMLES for a smooth wall (bottom wall; set
+                           CFLMAX with c3d_set_scalar before the first step),
                          4 Werner-Wengle wall model (both walls; needs N3 <= N1),
                          6 the log-law model at both walls (no counterpart in the
                          reference, whose model 1 leaves the top wall unmodelled) */
@@ -125,7 +127,9 @@ int c3d_dpcalc(c3d_handle *h);    /* DPCALC = RHSDP + GETDP compact3d.f:1037-104
 int c3d_upcalc(c3d_handle *h);    /* UPCALC compact3d.f:1052-1238 */
 int c3d_sgs(c3d_handle *h);       /* SGSDY (FILES=1) compact3d.f:1250 / SGSCS (FILES=2) :1413 /
                                      SGSSD (FILES=3) :1502 */
-int c3d_wall_model(c3d_handle *h); /* WALL_MODEL -> WALLMODEL_INSTANT compact3d.f:1952, 2319 */
+int c3d_wall_model(c3d_handle *h); /* WALL_MODEL compact3d.f:1952-2013 -> WALLMODEL_INSTANT :2319,
+                                     WALLMODEL_IWMLES_SMOOTH :3386 (NTIME = the last call's + 1),
+                                     WALLMODEL_WERNER :4815 */
 int c3d_wallss(c3d_handle *h);    /* WALLSS compact3d.f:11077 (sets WSM(1:3)) */
 
 /* FILTER (width 2, compact3d.f:5190) / FILTER4 (width 4, :5481) of a host
@@ -179,15 +183,23 @@ enum c3d_field {
   C3D_F_UH1, C3D_F_UH2, C3D_F_UH3,
   C3D_F_RDP, C3D_F_DP,                    /* P-like */
   C3D_F_HH1, C3D_F_HH2, C3D_F_HH3,        /* current HH, U-like single component */
-  C3D_F_TAUW, C3D_F_TAUB                  /* (0:N1,0:N3,2) */
+  C3D_F_TAUW, C3D_F_TAUB,                 /* (0:N1,0:N3,2) */
+  C3D_F_IWM  /* WMODEL = 2: the 43 planes of the reference's COMMON /IWMLES/ (compact3d.f:3399-3408),
+                N1-1 x N3-1 each (I fastest), in that order */
 };
 /* copies one device field into a host array in its Fortran layout */
 int c3d_get_field(c3d_handle *h, int which, double *out);
 /* device scalars: 0 PRESG 1 PRESG3 2 DMPRESG 3 DMPRESG3 4 FLOW1 5 FLOW3
  * 6 WSM(1) 7 WSM(2) 8 WSM(3); get only (mixed mode): 9 refinement passes
- * run, 10 DIVMAX of the last step's first projection */
+ * run, 10 DIVMAX of the last step's first projection; WMODEL = 2: 11 NTIME
+ * of the last wall-model call (c3d_advance and c3d_wall_model continue
+ * from it), 12 CFLMAX (/TSTEP/, sets the first filter weight; default 0),
+ * 13 wm_dt */
 int c3d_get_scalar(c3d_handle *h, int which, double *out);
 int c3d_set_scalar(c3d_handle *h, int which, double value);
+/* the inverse of c3d_get_field, for C3D_F_IWM only (a saved or constructed
+ * state of the integral wall model) */
+int c3d_set_field(c3d_handle *h, int which, const double *in);
 int c3d_set_dt(c3d_handle *h, double dt);
 /* the grid MESH built (compact3d.f:302-443): Y(0:N2) and {DX1, DX3, ALY, VOL} */
 int c3d_get_mesh(c3d_handle *h, double *y, double *scal4);

@@ -1,0 +1,130 @@
+"""GPU parity of the integral wall model (WMODEL = 2, WALLMODEL_IWMLES_SMOOTH,
+compact3d.f:3386-4329) against the oracle, through the C-ABI.
+
+The oracle is bit-exact with the reference's iwm_* routines
+(tests/test_oracle_golden.py).  The device path differs from it only in
+rounding: tree sums for the plane means (initial velocity, first-cell
+pressure), the device's pow/log, and the per-point Newton iteration.  That iteration stops once
+its residuals (wall-layer integrals of about u*Dz ~ 0.1 here) are below
+1e-6 absolute, and its finite-difference Jacobian (step 1e-9) is nearly
+singular in the spanwise direction when uty starts at 0, so two roundings
+may stop at different iterates anywhere inside that band: the friction
+velocities and the integrals formed from them are determined to ~1e-5
+relative by the model itself.  The oracle shows it: inputs changed by 1 ulp
+move the integral of ww by up to 1.2e-6 of its magnitude at NTIME = 10
+(tests/test_oracle_math.py::test_iwmles_ulp_sensitivity).  Bound: 1e-5
+relative to each state plane's largest magnitude, the same for the wall
+stresses and WSM(1); observed 1.1e-6 (the integral of ww at NTIME = 10)."""
+import numpy as np
+import pytest
+
+from cases import config, inputs, interior_u, iwm_inputs, relerr
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _cmp_planes(a, b, tag):
+    worst = 0.0
+    for q in range(a.shape[0]):
+        s = max(np.max(np.abs(a[q])), 1e-300)
+        e = np.max(np.abs(a[q] - b[q])) / s
+        worst = max(worst, e)
+        assert np.all(np.isfinite(b[q])), (tag, q)
+        assert e <= TOL, (tag, q, e)
+    return worst
+
+
+def _setup(seed=7):
+    from compact3d_amd import c3d
+    from pyoracle import Oracle
+    cfg = config(33, 33, 33, re=5200.0, dt=5e-4, files=2.0, wmodel=2.0)
+    orc = Oracle(**cfg)
+    ch = c3d.Channel(**cfg)
+    orc.set_cflmax(0.5)
+    ch.set_scalar(c3d.S_CFLMAX, 0.5)
+    return cfg, orc, ch, c3d
+
+
+def _check_call(orc, ch, c3d, nt):
+    pl, wdt, fixed = orc.iwm_state()
+    got = ch.field(c3d.F_IWM)
+    worst = _cmp_planes(pl, got, nt)
+    tw, tb = orc.get_tau()
+    e1, e2 = relerr(tw, ch.field(c3d.F_TAUW)), relerr(tb, ch.field(c3d.F_TAUB))
+    w = orc.get_wsm()[0]
+    ew = abs(w - ch.scalar(c3d.S_WSM1)) / abs(w)
+    assert ch.scalar(c3d.S_NTIME) == nt
+    assert ch.scalar(c3d.S_WMDT) == wdt
+    print(f"NTIME {nt}: planes {worst:.2e} TAUW {e1:.2e} TAUB {e2:.2e} WSM1 {ew:.2e} fixed {fixed}", flush=True)
+    assert e1 <= TOL and e2 <= TOL and ew <= TOL
+    return fixed
+
+
+def test_iwmles_from_its_initialisation():
+    """NTIME = 1..11 from the model's own initialisation: the equilibrium
+    branch at the compute steps 5 and 10"""
+    cfg, orc, ch, c3d = _setup()
+    U, P, _ = inputs(orc, seed=7)
+    ch.upload(U, P, presg=0.003, presg3=-0.001)
+    for nt in range(1, 12):
+        print("IWMLES init case, NTIME", nt, flush=True)
+        orc.wallmodel_iwmles(U, P, 0.003, -0.001, nt)
+        ch.wall_model()
+        fixed = _check_call(orc, ch, c3d, nt)
+        if nt % 5 == 0:
+            assert fixed > 0
+    ch.close()
+
+
+def test_iwmles_newton_path():
+    """NTIME = 5..15 from a log-law-consistent state: the Newton iteration
+    converges with small linear corrections at every point"""
+    cfg, orc, ch, c3d = _setup()
+    U, P, pl0, wdt0 = iwm_inputs(33, 33, orc.mesh()["DY"], cfg["re"], cfg["dt"], seed=1)
+    ch.upload(U, P, presg=0.003, presg3=-0.001)
+    orc.iwm_set(pl0, wdt0)
+    ch.set_field(c3d.F_IWM, pl0)
+    ch.set_scalar(c3d.S_NTIME, 4)
+    ch.set_scalar(c3d.S_WMDT, wdt0)
+    for nt in range(5, 16):
+        print("IWMLES Newton case, NTIME", nt, flush=True)
+        orc.wallmodel_iwmles(U, P, 0.003, -0.001, nt)
+        ch.wall_model()
+        assert _check_call(orc, ch, c3d, nt) == 0
+    ax = ch.field(c3d.F_IWM)[37]
+    assert np.max(np.abs(ax)) > 0.1  # the linear corrections are in use
+    ch.close()
+
+
+def test_iwmles_trajectory_and_graph_replay():
+    """10 GETUP steps with WMODEL = 2 (compute steps 5 and 10) against the
+    oracle; the same 10 steps replayed from captured graphs, which advance
+    NTIME on the device, agree bit for bit with the explicit-NTIME steps"""
+    cfg, orc, ch, c3d = _setup()
+    n1, n2, n3 = 33, 33, 33
+    U, P, _ = inputs(orc, seed=9)
+    ch.upload(U, P)
+    HH = orc.zeros_u()
+    S = orc.zeros_u(0)
+    pg = pg3 = 0.0
+    Uo, Po = U.copy(order="F"), P.copy(order="F")
+    for n in range(10):
+        pg, pg3 = orc.getup(Uo, Po, HH, S, pg, pg3, n + 1)
+        ch.getup(n + 1)
+    st = ch.download()
+    eu = [relerr(interior_u(Uo, n1, n2, n3, c), interior_u(st["U"], n1, n2, n3, c)) for c in range(3)]
+    print("IWMLES 10 steps U", ["%.3e" % x for x in eu], flush=True)
+    assert max(eu) <= 1e-9
+    _cmp_planes(orc.iwm_state()[0], ch.field(c3d.F_IWM), "trajectory")
+    g = c3d.Channel(**cfg)
+    g.set_scalar(c3d.S_CFLMAX, 0.5)
+    g.upload(U, P)
+    g.advance(10, use_graph=True)
+    sg = g.download()
+    assert g.scalar(c3d.S_NTIME) == 10
+    assert np.array_equal(sg["U"], st["U"]) and np.array_equal(sg["P"], st["P"])
+    assert np.array_equal(g.field(c3d.F_IWM), ch.field(c3d.F_IWM))
+    g.close()
+    ch.close()

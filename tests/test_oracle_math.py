@@ -318,3 +318,43 @@ def test_iwmles_equilibrium_branch():
     assert np.array_equal(tb[1:24, 1:20, 0], pl[ix["tauwy"]].T)
     assert not tw[:, :, 1].any() and not tb[:, :, 1].any()
     assert o.get_wsm()[0] == pytest.approx(np.sqrt(tx * tx + ty * ty).mean(), rel=2e-4)
+
+
+def test_iwmles_ulp_sensitivity():
+    """WMODEL = 2 from its own initialisation: the per-point Newton iteration
+    stops at residuals below 1e-6 with a finite-difference Jacobian that is
+    nearly singular in the spanwise direction (uty starts at 0), so the
+    stopping iterate depends on rounding.  Inputs changed by about 1 ulp move
+    the state planes by 1e-11 at the first compute step and by up to ~1e-6
+    (the integral of ww) at the second: the scale of any two faithful
+    implementations' disagreement."""
+    import math
+    from cases import config, inputs
+    from pyoracle import Oracle
+
+    cfg = config(33, 33, 33, re=5200.0, dt=5e-4, files=2.0, wmodel=2.0)
+
+    def run(pert):
+        o = Oracle(**cfg)
+        o.set_cflmax(0.5)
+        U, P, _ = inputs(o, seed=7)
+        if pert:
+            rng = np.random.default_rng(1)
+            U = np.asfortranarray(U * (1 + 2e-16 * rng.standard_normal(U.shape)))
+            P = np.asfortranarray(P * (1 + 2e-16 * rng.standard_normal(P.shape)))
+        out = {}
+        for nt in range(1, 11):
+            o.wallmodel_iwmles(U, P, 0.003, -0.001, nt)
+            if nt in (5, 10):
+                out[nt] = o.iwm_state()[0].copy()
+        return out
+
+    a, b = run(False), run(True)
+
+    def worst(nt):
+        return max(np.max(np.abs(a[nt][q] - b[nt][q])) / max(np.max(np.abs(a[nt][q])), 1e-300)
+                   for q in range(a[nt].shape[0]))
+
+    assert worst(5) < 1e-9
+    assert 1e-8 < worst(10) < 1e-5
+    assert math.isfinite(worst(10))
