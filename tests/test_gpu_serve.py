@@ -72,10 +72,13 @@ def test_served_result_equals_alone():
 
 def test_ensemble_equals_alone():
     """members of different grids, precisions and models, advanced by one
-    graph launch per step, end bit for bit where each ends stepped alone"""
+    graph launch per step, end bit for bit where each ends stepped alone
+    (the integral wall model's member, WMODEL = 2, counts its steps on the
+    device: its compute step NTIME = 5 falls inside the ensemble's run)"""
     from compact3d_amd import c3d
     specs = [(_cfg(49, 41, 33), 0, 5), (_cfg(33, 33, 33, les=True, re=20000.0), 1, 6),
-             (_cfg(65, 33, 49), 1, 7), (_cfg(33, 33, 33, les=True, re=20000.0), 0, 8)]
+             (_cfg(65, 33, 49), 1, 7), (_cfg(33, 33, 33, les=True, re=20000.0), 0, 8),
+             (dict(_cfg(33, 33, 33, les=True, re=20000.0), wmodel=2.0), 1, 9)]
     ens_ch, alone = [], []
     for cfg, prec, seed in specs:
         pair = [c3d.Channel(**cfg, precision=prec) for _ in range(2)]
