@@ -48,10 +48,11 @@ ALG_BYTES = {
 ALG_BYTES_ADI_X_PER_STEP = 56
 # mixed mode: FP32 intermediates and state shadows, FP64 state, RDP, DP, DIV
 # (state read as FP32 shadows by the line kernels; the update reads and
-# writes the FP64 state and writes the shadows)
+# writes the FP64 state and writes the shadows; rhsdp_z adds the FP64 DIV of
+# the state)
 ALG_BYTES_MIXED = {
     "rhs_x": 44, "rhs_z": 56, "rhs_zw": 20, "rhs_x2": 12, "adi_y": 84, "adi_z": 24,
-    "rhsdp_x": 24, "rhsdp_z": 20, "fft_x": 16, "fft_z": 16, "pois_y": 24, "fft_xinv": 16,
+    "rhsdp_x": 16, "rhsdp_z": 28, "fft_x": 16, "fft_z": 16, "pois_y": 24, "fft_xinv": 16,
     "upd_x": 64, "upd_z": 92,
 }
 

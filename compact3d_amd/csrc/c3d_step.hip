@@ -831,7 +831,8 @@ __global__ __launch_bounds__(512) void k_adi_z(Ctx c, CompList cl) {
 // (the CBC term is zero: UBC(.,.,2,.) = 0, :978, :986).  Arithmetic and RDP
 // in FP64; UH stored as TS.  Mixed mode (TS = float): UH holds DELTA and
 // RDP = DIV(U)/DT + DIV(DELTA)/DT, with DIV(U) of the current state kept in
-// FP64 by the previous update.
+// FP64 by the previous update; k_rhsdp_z adds DIV(U)/DT (in the order
+// ((x and y terms) + DIV(U)/DT) + z term), so this kernel stages only UH.
 template <typename T, int SEG, typename TS>
 __global__ __launch_bounds__(256) void k_rhsdp_x(Ctx c) {
   XSETUP;
@@ -850,12 +851,7 @@ __global__ __launch_bounds__(256) void k_rhsdp_x(Ctx c) {
   X.pre((const TS*)F.uh[1], 0, pf);
   SQ e[q] = fjpa * XA(B, q);
   B = X.put(pf);
-  if (MIX) X.pre(F.div, 0, pf);
   SQ d[q] = (e[q] - fjmu * XA(B, q)) / dy / dt + d[q] / dt;
-  if (MIX) {
-    B = X.put(pf);
-    SQ d[q] = d[q] + XA(B, q) / dt;
-  }
   X.out(F.rdp, d);
 }
 
@@ -869,7 +865,13 @@ __global__ __launch_bounds__(512) void k_rhsdp_z(Ctx c) {
   SQ d[q] = (T)24. * dx3 * (nxt(a, q, rt) - a[q]);
   psolve<T, SEG>(d, G, L, l, gs, PZ.s22, bf.xf, bf.xb);
   double* rdp = c.d.rdp;
-  if (ok) SQ rdp[ZI(q)] = rdp[ZI(q)] + d[q] / dt;
+  if (sizeof(TS) == 4) {
+    // mixed mode: DIV(U)/DT of the current state (see k_rhsdp_x)
+    const double* dv = c.d.div;
+    if (ok) SQ rdp[ZI(q)] = (rdp[ZI(q)] + dv[ZI(q)] / dt) + d[q] / dt;
+  } else {
+    if (ok) SQ rdp[ZI(q)] = rdp[ZI(q)] + d[q] / dt;
+  }
 }
 
 // ------------------------------------------------------------------ GETDP
