@@ -94,9 +94,10 @@ def test_mixed_state_roundtrip():
     b.close()
 
 
-@pytest.mark.parametrize("wmodel", [1.0, 0.0])
+@pytest.mark.parametrize("wmodel", [1.0, 0.0, 2.0])
 def test_mixed_les_trajectory_10_steps(wmodel):
-    """wall-modelled / wall-resolved Smagorinsky LES: mixed against FP64"""
+    """wall-modelled (log law; the integral wall model, whose compute steps
+    are NTIME 5 and 10) / wall-resolved Smagorinsky LES: mixed against FP64"""
     from compact3d_amd import c3d
     from pyoracle import Oracle
     n1, n2, n3 = 49, 33, 33
