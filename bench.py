@@ -385,7 +385,7 @@ def main():
                                                               v["traffic_over_alg"]))
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; the multi-GPU decomposition is a design "
+        print(json.dumps({"error": "bench.py runs on one GPU; the multi-GPU decomposition is a design "
                                    "only.  Use --gpus 1."}))
         sys.exit(2)
 

@@ -556,7 +556,7 @@ __global__ __launch_bounds__(256) void k_sgssd_coef(Ctx c, int ntile, const doub
 // without a bound; here the loop also stops after 10000 iterations (the
 // damped iteration converges in tens).
 #define WM_MAXIT 10000
-// `top` (WMODEL = 6, this build's two-wall extension): the same iteration
+// `top` (WMODEL = 6, a two-wall extension with no reference counterpart): the same iteration
 // at the top wall, on the third cell from it (J = N2M-2, wall distance
 // DY(N2M) + DY(N2M-1) + DY(N2M-2)/2), into TAUW(I,K,2), TAUB(I,K,2); its
 // plane sums are not kept (WSM(1) is the bottom wall's).

@@ -2389,7 +2389,7 @@ void c3o_wallmodel_instant(c3o *o, const double *U) {
   o->taow = o->wsm[0];
 }
 
-/* WMODEL = 6, an extension of this build (the reference's WALLMODEL_INSTANT
+/* WMODEL = 6, an extension with no reference counterpart (the reference's WALLMODEL_INSTANT
  * models the bottom wall only, so a full channel's top wall has no wall
  * stress): WALLMODEL_INSTANT, then the same iteration at the top wall on
  * the third cell from it, J = N2M-2, at the distance DY(N2M) + DY(N2M-1) +

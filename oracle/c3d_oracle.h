@@ -80,7 +80,7 @@ double c3o_dqrrt(const double *pa, int n);
 /* one plane of SGSSD: its two levels' sums s1, s2 (A..E), PA; returns BETA */
 double c3o_sgssd_plane(c3o *o, const double *U, int j, double *s1, double *s2, double *pa);
 void c3o_wallmodel_instant(c3o *o, const double *U);
-/* WMODEL = 6 (this build's extension): WALLMODEL_INSTANT at both walls */
+/* WMODEL = 6 (an extension with no reference counterpart): WALLMODEL_INSTANT at both walls */
 void c3o_wallmodel_instant2(c3o *o, const double *U);
 /* WALLMODEL_IWMLES_SMOOTH compact3d.f:3386-4329 (WMODEL = 2, Y0 = 0), called
  * with GETUP's NTIME; P is (1:N1,1:N2,1:N3) */
