@@ -128,3 +128,18 @@ def test_iwmles_trajectory_and_graph_replay():
     assert np.array_equal(g.field(c3d.F_IWM), ch.field(c3d.F_IWM))
     g.close()
     ch.close()
+
+
+def test_iwm_field_errors():
+    """the state field exists only with WMODEL = 2, and only it can be set"""
+    from compact3d_amd import c3d
+    cfg, orc, ch, _ = _setup()
+    with pytest.raises(c3d.C3DError):
+        ch.set_field(c3d.F_TAUW, np.zeros(2 * 34 * 34))
+    other = c3d.Channel(**dict(cfg, wmodel=1.0))
+    with pytest.raises(c3d.C3DError):
+        other.field(c3d.F_IWM)
+    with pytest.raises(c3d.C3DError):
+        other.set_field(c3d.F_IWM, np.zeros(43 * 32 * 32))
+    other.close()
+    ch.close()
