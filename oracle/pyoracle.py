@@ -476,8 +476,46 @@ class RefLib:
         return c.value
 
     def wallss(self, U, P):
+        """the reference's WALLSS; returns WSM(1:3)"""
         t = C.c_double(0.0)
         self.L.wallss_(_p(U), _p(P), C.byref(t))
+        return self.get_wsm()[:3].copy()
+
+    def chkmf(self, U):
+        """the reference's CHKMF (run in a scratch directory: it writes
+        TIME, FLOW1, FLOW3 to unit 32); returns FLOW1, FLOW3 of COMMON
+        /FLOWRATE/"""
+        t = C.c_double(0.0)
+        cwd = os.getcwd()
+        with tempfile.TemporaryDirectory() as td:
+            os.chdir(td)
+            try:
+                self.L.chkmf_(_p(U), C.byref(t))
+            finally:
+                os.chdir(cwd)
+        f = np.zeros(2)
+        self.L.refh_getflow_(_p(f))
+        return float(f[0]), float(f[1])
+
+    def uhcalc(self, U, UH, P, HH, HH2, presg, presg3, SGSVIS, ntime=1):
+        """the reference's UHCALC (compact3d.f:1004-1036): RHS1-3 then GETUH1-3"""
+        SS = np.zeros_like(HH)
+        SS2 = np.zeros_like(HH)
+        a, b = C.c_double(presg), C.c_double(presg3)
+        self.L.uhcalc_(_p(U), _p(UH), _p(P), _p(HH), _p(HH2), _p(SS), _p(SS2), C.byref(a), C.byref(b),
+                       _p(SGSVIS), self._i(ntime))
+
+    def reset_commons(self):
+        """zeroes the COMMON state that routines read from earlier calls:
+        TAUW/TAUB (/TAUXYZ/), WSM and WSMO (/WSMEM/), TAOWO and the
+        integral wall model's planes and wm_dt, so each case starts from
+        the same state whatever ran before it in this process"""
+        tw, tb = self.get_tau()
+        self.set_tau(np.zeros_like(tw), np.zeros_like(tb))
+        self.set_wsm([0.0, 0.0, 0.0])
+        self.L.refh_setwsmo_(_p(np.zeros(3)), C.byref(C.c_double(0.0)))
+        self.iwm_set(np.zeros((IWM_NPL, self.n3 - 1, self.n1 - 1)), 0.0)
+        self.set_ifwall(0)
 
     def sgscs(self, U, SGSVIS, ntime=1):
         self.L.sgscs_(_p(U), _p(SGSVIS), self._i(ntime))

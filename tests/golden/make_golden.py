@@ -13,7 +13,11 @@ it; the fixture's UPCALC input DP is the oracle's GETDP of the reference's RDP
 WRITEUP's Tecplot text is stored as checksums and sample lines
 (ref_writeup.json).
 
-Usage:  python tests/golden/make_golden.py [case ...]   (build container only)
+Every case first zeroes the COMMON state that routines carry between calls
+(RefLib.reset_commons), so a fixture does not depend on which cases ran
+before it in the same process.
+
+Usage:  python tests/golden/make_golden.py [--out DIR] [case ...]   (build container only)
 """
 import math
 import os
@@ -41,8 +45,8 @@ CASES = {
 }
 
 
-def sub(a):
-    return np.ascontiguousarray(a, dtype=np.float64).reshape(-1, order="F")[::STRIDE].copy()
+def sub(a, stride=STRIDE):
+    return np.ascontiguousarray(a, dtype=np.float64).reshape(-1, order="F")[::stride].copy()
 
 
 def summ(a):
@@ -52,6 +56,7 @@ def summ(a):
 
 def run_case(name, cfg):
     ref = RefLib(**cfg)
+    ref.reset_commons()
     n1, n2, n3 = ref.n1, ref.n2, ref.n3
     orc = Oracle(n1, n2, n3, cks=2.5, **cfg)  # inputs only; outputs come from ref
     U, P, HH2 = inputs(orc, seed=7)
@@ -103,6 +108,7 @@ def run_case(name, cfg):
     RDP = np.zeros((n1, n2, n3), order="F")
     ref.rhsdp(RDP, UH, U)
     out["RDP"] = RDP
+    UHb = UH.copy(order="F")
     DP = np.zeros_like(RDP)
     orc.getdp(DP, RDP)
     Pn = P.copy(order="F")
@@ -112,6 +118,13 @@ def run_case(name, cfg):
     out["PRESG"] = np.array([pg, pg3])
     out["DIVMAX"] = np.array([ref.divcheck(U)])
     out["CFL"] = np.array([ref.cfl(U)])
+    # the diagnostics MAIN calls after the step: CHKMF (with its sums over
+    # U(1,J,K,1) and U(I,J,1,3)) and WALLSS (WSM(1:3))
+    out["CHKMF"] = np.array(ref.chkmf(U))
+    out["WALLSS"] = ref.wallss(U, Pn)
+    # DIVCHECK and CFL of the unprojected UH: an O(1) divergence
+    out["DIVMAX_UH"] = np.array([ref.divcheck(UHb)])
+    out["CFL_UH"] = np.array([ref.cfl(UHb)])
     # line solvers and filters
     rng = np.random.default_rng(99)
     d = rng.uniform(-1, 1, n1 - 1)
@@ -133,15 +146,15 @@ def run_case(name, cfg):
     out["FILTER4"] = F4
     store = {"config": np.array([n1, n2, n3, cfg["re"], cfg["alx"], cfg["aly"], cfg["alz"], cfg["dt"],
                                  cfg["files"], cfg["wmodel"], 7, presg, presg3, STRIDE])}
-    small = {"PRESG", "DIVMAX", "CFL", "WSM", "SMAGC", "CTDMA_RHS", "TDMAP_IN", "TDMAP", "FILTER_IN", "FILTER",
-             "FILTER4", "DQRRT_PA", "DQRRT_BETA", "COEF_IN", "COEF_PA"} | {k for k in out if k.startswith("CTDMA11")}
+    small = {"PRESG", "DIVMAX", "CFL", "WSM", "DIVMAX_UH", "CFL_UH", "CHKMF", "WALLSS", "SMAGC", "CTDMA_RHS",
+             "TDMAP_IN", "TDMAP", "FILTER_IN", "FILTER", "FILTER4", "DQRRT_PA", "DQRRT_BETA", "COEF_IN", "COEF_PA"} | {k for k in out if k.startswith("CTDMA11")}
     for k, v in out.items():
         if k in small:
             store[k] = np.asarray(v, dtype=np.float64)
         else:
             store[k + "__sub"] = sub(v)
             store[k + "__sum"] = summ(v)
-    path = os.path.join(HERE, f"ref_{name}.npz")
+    path = os.path.join(OUT, f"ref_{name}.npz")
     np.savez_compressed(path, **store)
     print(path, os.path.getsize(path), "bytes")
 
@@ -166,6 +179,7 @@ def run_iwm():
     NTIME = 5..15 from a log-law-consistent state (tests/cases.py
     iwm_inputs), which contains three compute steps."""
     ref = RefLib(**IWM)
+    ref.reset_commons()
     n1, n3 = ref.n1, ref.n3
     orc = Oracle(n1, ref.n2, n3, cks=2.5, **IWM)
     ref.set_cflmax(IWM_CFLMAX)
@@ -202,7 +216,87 @@ def run_iwm():
         else:
             store[k + "__sub"] = sub(v)
             store[k + "__sum"] = summ(v)
-    path = os.path.join(HERE, "ref_iwm.npz")
+    path = os.path.join(OUT, "ref_iwm.npz")
+    np.savez_compressed(path, **store)
+    print(path, os.path.getsize(path), "bytes")
+
+
+GETUP_CASES = {
+    "dns": CASES["dns"],
+    "les": CASES["les"],
+    "wer": CASES["wer"],
+    # FILES != 0 with WMODEL = 0: WALL_MODEL calls WALLSS, whose WSM feeds
+    # the next step's SGSCS
+    "lesw0": dict(re=5200.0, alx=2 * math.pi, aly=2.0, alz=math.pi, dt=5e-4, files=2.0, wmodel=0.0),
+}
+GETUP_STEPS = 3
+GETUP_STRIDE = 53
+
+
+def ref_getup(ref, orc, U, P, HH, S, presg, presg3, ntime, cfg):
+    """one GETUP (compact3d.f:879-960) from the reference's own units: the
+    prologue's clears and AB2 shift HH2 <- HH, HH <- 0 (:895-923), the SGS
+    model (:925-927), WALL_MODEL's dispatch (:1981-2008), BCOND, UHCALC
+    (RHS1-3 then GETUH1-3, :1004-1036), RHSDP, GETDP (the oracle's: FFTW
+    is absent here) and UPCALC.  U, P, HH, S are updated in place;
+    returns PRESG, PRESG3."""
+    n1, n2, n3 = ref.n1, ref.n2, ref.n3
+    HH2 = HH.copy(order="F")
+    HH[...] = 0.0
+    S[...] = 0.0
+    UH = np.zeros_like(U)
+    if cfg["files"] == 1.0:
+        ref.sgsdy(U, S, ntime)
+    elif cfg["files"] == 2.0:
+        ref.sgscs(U, S, ntime)
+    elif cfg["files"] == 3.0:
+        ref.sgssd(U, S, ntime)
+    if cfg["files"] != 0.0:
+        if cfg["wmodel"] == 0.0:
+            ref.set_ifwall(0)
+            ref.wallss(U, P)
+        elif cfg["wmodel"] == 1.0:
+            ref.wallmodel_instant(U, S)
+            ref.set_ifwall(1)
+        elif cfg["wmodel"] == 4.0:
+            ref.wallmodel_werner(U, S, ntime)
+            ref.set_ifwall(1)
+        else:
+            raise ValueError(cfg["wmodel"])
+    ref.bcond(U, S, ntime)
+    ref.uhcalc(U, UH, P, HH, HH2, presg, presg3, S, ntime)
+    RDP = np.zeros((n1, n2, n3), order="F")
+    ref.rhsdp(RDP, UH, U, ntime)
+    DP = np.zeros_like(RDP)
+    orc.getdp(DP, RDP)
+    return ref.upcalc(U, P, UH, DP, presg, presg3, ntime)
+
+
+def run_getup():
+    """GETUP_STEPS steps of GETUP composed from the reference's units
+    (ref_getup) per GETUP_CASES entry, from the seeded inputs with a random
+    previous HH; stores U, P, HH and SGSVIS after steps 1 and GETUP_STEPS,
+    PRESG/PRESG3 and WSM after each step"""
+    store = {"steps": np.array([GETUP_STEPS, GETUP_STRIDE])}
+    for name, cfg in GETUP_CASES.items():
+        ref = RefLib(**cfg)
+        ref.reset_commons()
+        orc = Oracle(ref.n1, ref.n2, ref.n3, cks=2.5, **cfg)
+        U, P, HH = inputs(orc, seed=11)
+        S = np.zeros(U.shape[:3], order="F")
+        presg, presg3 = 0.003, -0.001
+        if cfg["files"] != 0.0:
+            ref.set_wsm([2.5e-3, 0.0, 4.0e-4])
+        pg = []
+        for nt in range(1, GETUP_STEPS + 1):
+            presg, presg3 = ref_getup(ref, orc, U, P, HH, S, presg, presg3, nt, cfg)
+            pg.append([presg, presg3] + list(ref.get_wsm()[:3]))
+            if nt in (1, GETUP_STEPS):
+                for k, v in (("U", U), ("P", P), ("HH", HH), ("SGSVIS", S)):
+                    store[f"{name}_{k}{nt}__sub"] = sub(v, GETUP_STRIDE)
+                    store[f"{name}_{k}{nt}__sum"] = summ(v)
+        store[f"{name}_PRESG"] = np.array(pg)
+    path = os.path.join(OUT, "ref_getup.npz")
     np.savez_compressed(path, **store)
     print(path, os.path.getsize(path), "bytes")
 
@@ -215,6 +309,7 @@ def run_writeup():
     from cases import writeup_inputs
     cfg = CASES["dns"]
     ref = RefLib(**cfg)
+    ref.reset_commons()
     orc = Oracle(ref.n1, ref.n2, ref.n3, cks=2.5, **cfg)
     U, P, tauw = writeup_inputs(orc)
     tw, tb = ref.get_tau()
@@ -226,18 +321,30 @@ def run_writeup():
         lines = text.splitlines(keepends=True)
         out[name] = {"sha256": hashlib.sha256(text.encode()).hexdigest(), "lines": len(lines),
                      "head": lines[:4], "special": lines[2:2 + 16] if name == "flow" else []}
-    path = os.path.join(HERE, "ref_writeup.json")
+    path = os.path.join(OUT, "ref_writeup.json")
     with open(path, "w") as f:
         json.dump(out, f, indent=1)
     print(path, os.path.getsize(path), "bytes")
 
 
+OUT = HERE
+
 if __name__ == "__main__":
-    only = set(sys.argv[1:])  # optional case names: dns les ... writeup iwm
+    # optional case names: dns les ... writeup getup iwm; --out DIR writes the
+    # fixtures there instead of tests/golden
+    args = sys.argv[1:]
+    if "--out" in args:
+        i = args.index("--out")
+        OUT = args[i + 1]
+        del args[i:i + 2]
+        os.makedirs(OUT, exist_ok=True)
+    only = set(args)
     for name, cfg in CASES.items():
         if not only or name in only:
             run_case(name, cfg)
     if not only or "writeup" in only:
         run_writeup()
+    if not only or "getup" in only:
+        run_getup()
     if not only or "iwm" in only:
         run_iwm()

@@ -2,7 +2,9 @@
 subroutines (tests/golden/make_golden.py; compact3d.f compiled unmodified).
 
 Every phase the reference can run without FFTW is compared: BCOND, RHS1-3,
-GETUH1-3, RHSDP, UPCALC, DIVCHECK, CFL, SGSCS/STRAIN, SGSDY (the dynamic
+GETUH1-3, RHSDP, UPCALC, DIVCHECK and CFL (of the projected field and of the
+unprojected UH), CHKMF, WALLSS, GETUP's composition over three steps,
+SGSCS/STRAIN, SGSDY (the dynamic
 model, its per-plane coefficient included), SGSSD (the scale-dependent
 model) with COEFFICIENT and the DQRRT/CHHQR root selection,
 WALLMODEL_INSTANT, WALLMODEL_WERNER (both walls), CTDMA11, TDMAP, FILTER,
@@ -89,6 +91,7 @@ def test_oracle_matches_reference_fixture(name):
     RDP = o.zeros_p()
     o.rhsdp(RDP, UH)
     _check(fx, "RDP", RDP, stride)
+    UHb = UH.copy(order="F")
     DP = o.zeros_p()
     o.getdp(DP, RDP)
     Pn = P.copy(order="F")
@@ -98,6 +101,76 @@ def test_oracle_matches_reference_fixture(name):
     assert np.max(np.abs(np.array([pg, pg3]) - fx["PRESG"])) <= TOL * np.max(np.abs(fx["PRESG"]))
     assert o.divcheck(U) == pytest.approx(fx["DIVMAX"][0], rel=1e-12, abs=1e-15)
     assert o.cfl(U) == pytest.approx(fx["CFL"][0], rel=1e-15)
+    # CHKMF (its U(1,J,K,1) / U(I,J,1,3) sums) and WALLSS of the stepped field
+    f1, f3 = o.chkmf(U)
+    assert abs(f1 - fx["CHKMF"][0]) <= TOL * abs(fx["CHKMF"][0])
+    assert abs(f3 - fx["CHKMF"][1]) <= TOL * max(abs(fx["CHKMF"][1]), abs(fx["CHKMF"][0]))
+    o.wallss(U)
+    wsm = o.get_wsm()[:3]
+    assert np.max(np.abs(wsm - fx["WALLSS"])) <= TOL * np.max(np.abs(fx["WALLSS"])), (wsm, fx["WALLSS"])
+    # DIVCHECK and CFL of the unprojected UH, whose divergence is O(1)
+    assert fx["DIVMAX_UH"][0] > 1e-3
+    assert abs(o.divcheck(UHb) - fx["DIVMAX_UH"][0]) <= TOL * fx["DIVMAX_UH"][0]
+    assert abs(o.cfl(UHb) - fx["CFL_UH"][0]) <= TOL * fx["CFL_UH"][0]
+
+
+@pytest.mark.parametrize("name", ["dns", "les", "wer", "lesw0"])
+def test_oracle_getup_matches_reference_composition(name):
+    """c3o_getup over three steps against GETUP composed from the
+    reference's own units (make_golden.ref_getup): pins the step's
+    composition -- the AB2 shift HH2 <- HH of the previous step, the clears,
+    the model dispatch and the state carried between steps (WSM, TAUW/TAUB,
+    PRESG) -- which the per-phase fixtures do not"""
+    import sys
+    sys.path.insert(0, HERE + "/golden")
+    from make_golden import GETUP_CASES
+    from pyoracle import Oracle
+    fx = _load("getup")
+    nsteps, stride = (int(x) for x in fx["steps"])
+    cfg = GETUP_CASES[name]
+    o = Oracle(33, 33, 33, cks=2.5, **cfg)
+    U, P, HH = inputs(o, seed=11)
+    S = o.zeros_u(0)
+    presg, presg3 = 0.003, -0.001
+    if cfg["files"] != 0.0:
+        o.set_wsm([2.5e-3, 0.0, 4.0e-4])
+    for nt in range(1, nsteps + 1):
+        presg, presg3 = o.getup(U, P, HH, S, presg, presg3, nt)
+        ref = fx[f"{name}_PRESG"][nt - 1]
+        got = np.array([presg, presg3] + list(o.get_wsm()[:3]))
+        assert np.max(np.abs(got - ref)) <= 1e-14 * np.max(np.abs(ref)), (nt, got, ref)
+        if nt in (1, nsteps):
+            for k, v in (("U", U), ("P", P), ("HH", HH), ("SGSVIS", S)):
+                key = f"{name}_{k}{nt}"
+                r = fx[key + "__sub"]
+                err = np.max(np.abs(_sub(v, stride) - r)) / max(np.max(np.abs(r)), 1e-300)
+                assert err <= 4 * TOL, (key, err)
+                s = np.array([v.sum(), (v * v).sum(), np.abs(v).max()])
+                es = np.max(np.abs(s - fx[key + "__sum"]) / np.maximum(np.abs(fx[key + "__sum"]), 1e-300))
+                assert es <= 1e-12, (key, es)
+
+
+def test_getup_fixture_distinguishes_ab2_slip():
+    """the composition fixture fails a plausible slip: a GETUP that keeps
+    HH2 from the first step (no AB2 shift) differs from it after three
+    steps by far more than the tolerance"""
+    import sys
+    sys.path.insert(0, HERE + "/golden")
+    from make_golden import GETUP_CASES
+    from pyoracle import Oracle
+    fx = _load("getup")
+    nsteps, stride = (int(x) for x in fx["steps"])
+    o = Oracle(33, 33, 33, cks=2.5, **GETUP_CASES["dns"])
+    U, P, HH = inputs(o, seed=11)
+    HH0 = HH.copy(order="F")
+    S = o.zeros_u(0)
+    presg, presg3 = 0.003, -0.001
+    for nt in range(1, nsteps + 1):
+        HHk = HH0.copy(order="F")  # the slip: HH2 stays the initial history
+        presg, presg3 = o.getup(U, P, HHk, S, presg, presg3, nt)
+    r = fx[f"dns_U{nsteps}__sub"]
+    err = np.max(np.abs(_sub(U, stride) - r)) / np.max(np.abs(r))
+    assert err > 1e3 * TOL, err
 
 
 def test_line_solvers_and_filters_match_reference():
