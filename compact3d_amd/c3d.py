@@ -22,8 +22,10 @@ LIB_PATH = os.path.join(HERE, "libc3d.so")
 
 FP64, MIXED = 0, 1
 F_RUH1, F_RUH2, F_RUH3, F_UH1, F_UH2, F_UH3, F_RDP, F_DP, F_HH1, F_HH2, F_HH3, F_TAUW, F_TAUB, F_IWM = range(1, 15)
+F_DIV = 15
 (S_PRESG, S_PRESG3, S_DMPRESG, S_DMPRESG3, S_FLOW1, S_FLOW3, S_WSM1, S_WSM2, S_WSM3, S_REFINES, S_DIVMAX1,
- S_NTIME, S_CFLMAX, S_WMDT) = range(14)
+ S_NTIME, S_CFLMAX, S_WMDT, S_TAOW, S_UINIT, S_USINIT, S_PBAR) = range(18)
+ERR_NONFINITE = -3  # C3D_ERR_NONFINITE: the step ran, its DIVMAX or CFLM is NaN or infinite
 IWM_NPL = 43  # WMODEL = 2: planes of the integral wall model's state (COMMON /IWMLES/)
 NKID = 48  # C3D_NKID of include/c3d.h: entries of the per-kernel timing tables
 
@@ -34,12 +36,22 @@ class C3DError(RuntimeError):
     pass
 
 
+class C3DDiverged(C3DError):
+    """a step's DIVMAX or CFLM is NaN or infinite (C3D_ERR_NONFINITE);
+    .divmax and .cflm hold the values"""
+
+    def __init__(self, msg, divmax=float("nan"), cflm=float("nan")):
+        super().__init__(msg)
+        self.divmax, self.cflm = divmax, cflm
+
+
 class Config(C.Structure):
     _fields_ = [
         ("n1", C.c_int), ("n2", C.c_int), ("n3", C.c_int),
         ("re", C.c_double), ("alx", C.c_double), ("aly", C.c_double), ("alz", C.c_double),
         ("dt", C.c_double), ("files", C.c_double), ("wmodel", C.c_double), ("cks", C.c_double),
         ("y", _dp), ("precision", C.c_int), ("device", C.c_int), ("stream", C.c_void_p),
+        ("cflmax", C.c_double),
     ]
 
 
@@ -141,7 +153,7 @@ def format_strain(path, ntime, xc, zc, tauw, utau):
 
 
 def make_config(n1, n2, n3, re, alx, aly, alz, dt, files=0.0, wmodel=0.0, cks=2.5, y=None,
-                precision=FP64, device=0):
+                precision=FP64, device=0, cflmax=0.0):
     cfg = Config()
     cfg.n1, cfg.n2, cfg.n3 = n1, n2, n3
     cfg.re, cfg.alx, cfg.aly, cfg.alz, cfg.dt = re, alx, aly, alz, dt
@@ -151,6 +163,7 @@ def make_config(n1, n2, n3, re, alx, aly, alz, dt, files=0.0, wmodel=0.0, cks=2.
         cfg._y = np.ascontiguousarray(y, dtype=np.float64)
         cfg.y = cfg._y.ctypes.data_as(_dp)
     cfg.precision, cfg.device, cfg.stream = precision, device, None
+    cfg.cflmax = cflmax
     return cfg
 
 
@@ -193,9 +206,10 @@ class Channel:
     """One channel instance resident on a GPU."""
 
     def __init__(self, n1, n2, n3, re, alx, aly, alz, dt, files=0.0, wmodel=0.0, cks=2.5, y=None,
-                 precision=FP64, device=0):
+                 precision=FP64, device=0, cflmax=0.0):
         self.L = lib()
-        self.cfg = make_config(n1, n2, n3, re, alx, aly, alz, dt, files, wmodel, cks, y, precision, device)
+        self.cfg = make_config(n1, n2, n3, re, alx, aly, alz, dt, files, wmodel, cks, y, precision, device,
+                               cflmax)
         h = C.c_void_p()
         rc = self.L.c3d_create(C.byref(self.cfg), C.byref(h))
         if rc != 0:
@@ -203,6 +217,8 @@ class Channel:
         self.h = h
         self.n1, self.n2, self.n3 = n1, n2, n3
         self.files = float(files)
+        self.wmodel = float(wmodel)
+        self.precision = int(precision)
 
     def close(self):
         if getattr(self, "h", None):
@@ -246,10 +262,17 @@ class Channel:
     def getup(self, ntime=0):
         self._ck(self.L.c3d_getup(self.h, ntime), "getup")
 
+    def _ck_checks(self, rc, what, d, c):
+        if rc == ERR_NONFINITE:
+            raise C3DDiverged(f"{what}: " + self.L.c3d_last_error(self.h).decode(), d.value, c.value)
+        self._ck(rc, what)
+
     def step(self, ntime=0, checks=True):
+        """GETUP + DIVCHECK + CFL; (DIVMAX, CFLM).  Raises C3DDiverged when
+        either is NaN or infinite."""
         if checks:
             d, c = C.c_double(), C.c_double()
-            self._ck(self.L.c3d_step(self.h, ntime, C.byref(d), C.byref(c)), "step")
+            self._ck_checks(self.L.c3d_step(self.h, ntime, C.byref(d), C.byref(c)), "step", d, c)
             return d.value, c.value
         self._ck(self.L.c3d_step(self.h, ntime, None, None), "step")
         return None
@@ -259,9 +282,10 @@ class Channel:
         self._ck(self.L.c3d_advance(self.h, int(nsteps), 1 if use_graph else 0), "advance")
 
     def sync(self):
-        """wait for the queued steps; (DIVMAX, CFLM) of the last one"""
+        """wait for the queued steps; (DIVMAX, CFLM) of the last one.  Raises
+        C3DDiverged when either is NaN or infinite."""
         d, c = C.c_double(), C.c_double()
-        self._ck(self.L.c3d_sync(self.h, C.byref(d), C.byref(c)), "sync")
+        self._ck_checks(self.L.c3d_sync(self.h, C.byref(d), C.byref(c)), "sync", d, c)
         return d.value, c.value
 
     def bcond(self):
@@ -316,7 +340,7 @@ class Channel:
         return a.value, b.value
 
     def field(self, which):
-        if which in (F_RDP, F_DP):
+        if which in (F_RDP, F_DP, F_DIV):
             out = np.zeros(self.pshape(), order="F")
         elif which in (F_TAUW, F_TAUB):
             out = np.zeros((self.n1 + 1, self.n3 + 1, 2), order="F")
@@ -330,8 +354,12 @@ class Channel:
         return out
 
     def set_field(self, which, values):
-        """F_IWM only: [plane, K-1, I-1] as field(F_IWM) returns it"""
-        a = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        """F_IWM ([plane, K-1, I-1] as field(F_IWM) returns it), F_TAUW,
+        F_TAUB, F_DIV (the layouts field() returns)"""
+        if which == F_IWM:
+            a = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        else:
+            a = np.asfortranarray(values, dtype=np.float64)
         self._ck(self.L.c3d_set_field(self.h, which, _p(a)), "set_field")
 
     def scalar(self, which):
@@ -439,10 +467,15 @@ class Ensemble:
         self._ck(self.L.c3d_ensemble_advance(self.e, int(nsteps)), "ensemble_advance")
 
     def sync(self):
-        """wait; arrays of every member's (DIVMAX, CFLM) after its last step"""
+        """wait; arrays of every member's (DIVMAX, CFLM) after its last step.
+        Raises C3DDiverged (with the arrays) when a member's value is NaN
+        or infinite."""
         d = np.zeros(len(self.members))
         c = np.zeros(len(self.members))
-        self._ck(self.L.c3d_ensemble_sync(self.e, _p(d), _p(c)), "ensemble_sync")
+        rc = self.L.c3d_ensemble_sync(self.e, _p(d), _p(c))
+        if rc == ERR_NONFINITE:
+            raise C3DDiverged("ensemble_sync: " + self.L.c3d_ensemble_last_error(self.e).decode(), d, c)
+        self._ck(rc, "ensemble_sync")
         return d, c
 
     def close(self):

@@ -528,6 +528,10 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
                "wall), 4 (Werner-Wengle, both walls) or 6 (log law, both walls)");
   // WALLMODEL_WERNER takes its I neighbour from IPA(K) (compact3d.f:4849)
   if (cf.wmodel == 4.0 && cf.n3 > cf.n1) return bad("WMODEL = 4 needs N3 <= N1");
+  // the integral wall model's first time-filter weight is CFLMAX*ALX/N1M/UINIT
+  // over its viscous time scale (compact3d.f:3700): a zero CFLMAX would
+  // freeze its filtered inputs at their initial values
+  if (cf.wmodel == 2.0 && !(cf.cflmax > 0)) return bad("WMODEL = 2 needs cflmax > 0 (CFLMAX of /TSTEP/)");
   if (cf.re <= 0 || cf.dt <= 0 || cf.alx <= 0 || cf.alz <= 0) return bad("RE, DT, ALX, ALZ must be > 0");
   if (hipSetDevice(cf.device) != hipSuccess) return bad("hipSetDevice failed");
   Ctx& c = h->c;
@@ -740,6 +744,8 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
       return bad("Poisson factor set-up failed");
   }
   if (dalloc(h, &F.sc, SC_COUNT)) return bad(h->err);
+  if (hipMemcpyAsync(F.sc + SC_CFLMAX, &cf.cflmax, sizeof(double), hipMemcpyHostToDevice, c.st) != hipSuccess)
+    return bad("scalar upload failed");
   if (hipHostMalloc((void**)&h->hread, 2 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
     h->hread = nullptr;
     return bad("hipHostMalloc of the read-back slots failed");
@@ -971,6 +977,14 @@ static int read_checks(c3d_handle* h, double* divmax, double* cflm) {
   return 0;
 }
 
+// C3D_ERR_NONFINITE when a step's DIVMAX or CFLM is NaN or infinite: the
+// step ran, but the state has diverged
+static int check_finite(c3d_handle* h, double dm, double cf) {
+  if (std::isfinite(dm) && std::isfinite(cf)) return 0;
+  fail(h, "the state diverged: DIVMAX = " + std::to_string(dm) + ", CFLM = " + std::to_string(cf));
+  return C3D_ERR_NONFINITE;
+}
+
 // Mixed mode, after UPCALC, given the DIVMAX dm of the projection (FP64, on
 // the new field): above kRefineTol one FP64-residual refinement follows: DIV
 // is the residual of the Poisson solve, DP2 solves L DP2 = DIV/DT the same
@@ -979,7 +993,8 @@ static int read_checks(c3d_handle* h, double* divmax, double* cflm) {
 // final DIVMAX and CFLM (unchanged when no refinement ran).
 static int refine_from(c3d_handle* h, double dm, double* divmax, double* cflm) {
   h->first_divmax = dm;
-  if (!(dm <= kRefineTol)) {
+  // a NaN or infinite DIVMAX is reported, not refined
+  if (dm > kRefineTol && std::isfinite(dm)) {
     h->refines++;
     c3dk::div_rdp(h->c);
     c3dk::getdp(h->c);
@@ -1164,7 +1179,7 @@ extern "C" int c3d_sync(c3d_handle* h, double* divmax, double* cflm) {
   if (h->c.s.mixed && refine_from(h, dm, &dm, &cf)) return -1;
   if (divmax) *divmax = dm;
   if (cflm) *cflm = cf;
-  return 0;
+  return check_finite(h, dm, cf);
 }
 
 // ------------------------------------------------------------- autotuner
@@ -1567,16 +1582,25 @@ extern "C" int c3d_ensemble_advance(c3d_ensemble* e, int nsteps) {
 extern "C" int c3d_ensemble_sync(c3d_ensemble* e, double* divmax, double* cflm) {
   if (!e) return -1;
   EHIPCHK(hipStreamSynchronize(e->st));
+  // every member's values are returned; a diverged member gives
+  // C3D_ERR_NONFINITE naming the first one
+  int out = 0;
   for (size_t q = 0; q < e->m.size(); q++) {
     double dm = 0.0, cf = 0.0;
-    if (c3d_sync(e->m[q], &dm, &cf)) {
+    const int rc = c3d_sync(e->m[q], &dm, &cf);
+    if (divmax) divmax[q] = dm;
+    if (cflm) cflm[q] = cf;
+    if (rc == C3D_ERR_NONFINITE) {
+      if (out == 0) {
+        e->err = std::string("member ") + std::to_string(q) + ": " + e->m[q]->err;
+        out = C3D_ERR_NONFINITE;
+      }
+    } else if (rc) {
       e->err = std::string("member ") + std::to_string(q) + ": " + e->m[q]->err;
       return -1;
     }
-    if (divmax) divmax[q] = dm;
-    if (cflm) cflm[q] = cf;
   }
-  return 0;
+  return out;
 }
 
 extern "C" int c3d_step(c3d_handle* h, int ntime, double* divmax, double* cflm) {
@@ -1592,7 +1616,7 @@ extern "C" int c3d_step(c3d_handle* h, int ntime, double* divmax, double* cflm) 
   if (h->c.s.mixed && refine_from(h, dm, &dm, &cf)) return -1;
   if (divmax) *divmax = dm;
   if (cflm) *cflm = cf;
-  return 0;
+  return check_finite(h, dm, cf);
 }
 
 extern "C" int c3d_divcheck(c3d_handle* h, double* divmax) {
@@ -1827,6 +1851,7 @@ extern "C" int c3d_get_field(c3d_handle* h, int which, double* out) {
     case C3D_F_UH3: src = F.uh[2]; break;
     case C3D_F_RDP: src = F.rdp; lay = LAY_P; break;  // FP64 in both modes
     case C3D_F_DP: src = F.dp; lay = LAY_P; break;  // FP64 in both modes
+    case C3D_F_DIV: return from_dev(h, out, F.div, LAY_P);  // FP64 in both modes
     case C3D_F_HH1: src = F.hh[0]; break;
     case C3D_F_HH2: src = F.hh[1]; break;
     case C3D_F_HH3: src = F.hh[2]; break;
@@ -1872,19 +1897,43 @@ extern "C" int c3d_get_field(c3d_handle* h, int which, double* out) {
 
 extern "C" int c3d_set_field(c3d_handle* h, int which, const double* in) {
   if (!h || !in) return -1;
-  if (which != C3D_F_IWM) return fail(h, "only C3D_F_IWM can be set");
-  if (!h->c.d.iwm) return fail(h, "the integral wall model's state exists only with WMODEL = 2");
-  const size_t nb = (size_t)IW_NPL * h->c.g.nx * h->c.g.nz * sizeof(double);
-  HIPCHK(hipMemcpyAsync(h->c.d.iwm, in, nb, hipMemcpyHostToDevice, h->c.st));
-  HIPCHK(hipStreamSynchronize(h->c.st));
-  return 0;
+  Fields& F = h->c.d;
+  switch (which) {
+    case C3D_F_IWM: {
+      if (!F.iwm) return fail(h, "the integral wall model's state exists only with WMODEL = 2");
+      const size_t nb = (size_t)IW_NPL * h->c.g.nx * h->c.g.nz * sizeof(double);
+      HIPCHK(hipMemcpyAsync(F.iwm, in, nb, hipMemcpyHostToDevice, h->c.st));
+      HIPCHK(hipStreamSynchronize(h->c.st));
+      return 0;
+    }
+    case C3D_F_TAUW:
+    case C3D_F_TAUB: {
+      // the inverse of c3d_get_field's (0:N1,0:N3,2) layout
+      const Geo& g = h->c.g;
+      const size_t n2d = (size_t)g.nx * g.nz;
+      const int n1 = h->cfg.n1, n3 = h->cfg.n3;
+      const size_t lay = (size_t)(n1 + 1) * (n3 + 1);
+      std::vector<double> t(2 * n2d);
+      for (int l = 0; l < 2; l++)
+        for (int k = 0; k < g.nz; k++)
+          for (int i = 0; i < g.nx; i++)
+            t[l * n2d + (size_t)k * g.nx + i] = in[l * lay + (size_t)(i + 1) + (size_t)(n1 + 1) * (k + 1)];
+      HIPCHK(hipMemcpyAsync(which == C3D_F_TAUW ? F.tauw : F.taub, t.data(), t.size() * 8, hipMemcpyHostToDevice,
+                            h->c.st));
+      HIPCHK(hipStreamSynchronize(h->c.st));
+      return 0;
+    }
+    case C3D_F_DIV: return to_dev(h, F.div, in, LAY_P);
+    default: return fail(h, "only C3D_F_TAUW, C3D_F_TAUB, C3D_F_IWM and C3D_F_DIV can be set");
+  }
 }
 
 // c3d_get_scalar / c3d_set_scalar index -> device slot (-1: not a slot)
 static int scalar_slot(int which) {
-  static const int map[14] = {SC_PRESG, SC_PRESG3, SC_DMPRESG, SC_DMPRESG3, SC_FLOW1, SC_FLOW3, SC_WSM1,
-                              SC_WSM2,  SC_WSM3,   -1,         -1,          SC_NTIME, SC_CFLMAX, SC_WMDT};
-  return which >= 0 && which < 14 ? map[which] : -1;
+  static const int map[C3D_NSCALAR] = {SC_PRESG, SC_PRESG3, SC_DMPRESG, SC_DMPRESG3, SC_FLOW1,  SC_FLOW3,
+                                       SC_WSM1,  SC_WSM2,   SC_WSM3,    -1,          -1,        SC_NTIME,
+                                       SC_CFLMAX, SC_WMDT,  SC_TAOW,    SC_UINIT,    SC_USINIT, SC_PBAR};
+  return which >= 0 && which < C3D_NSCALAR ? map[which] : -1;
 }
 
 extern "C" int c3d_get_scalar(c3d_handle* h, int which, double* out) {

@@ -16,6 +16,7 @@ module c3d_iface
     type(c_ptr) :: y
     integer(c_int) :: precision, device
     type(c_ptr) :: stream
+    real(c_double) :: cflmax
   end type c3d_config
 
   interface

@@ -10,8 +10,13 @@
   bottom-wall shear stress fluctuation over UTAU**2), text as the
   reference's Fortran runtime writes it (write_result).
 * The build's own checkpoint: an .npz with U, P, HH (the AB2 history that
-  CONTINUE omits), SGSVIS, PRESG, PRESG3 and the grid, so that a resumed run
-  continues bit-for-bit.
+  CONTINUE omits), SGSVIS, PRESG, PRESG3, the grid and model switches, and
+  the state the wall and SGS models carry from step to step: WSM(1:3) and
+  TAOW (SGSCS's friction velocity and WALLMODEL_WERNER's viscous branch read
+  the previous step's), TAUW/TAUB, the integral wall model's 43 planes with
+  its NTIME, wm_dt, CFLMAX and initial means (WMODEL = 2), and in mixed
+  precision the FP64 divergence the next step starts from.  A run resumed
+  from it continues bit for bit.
 """
 from __future__ import annotations
 
@@ -57,18 +62,50 @@ def read_continue(path, n1, n2, n3):
     return U, P, float(presg), float(presg3)
 
 
+def _carried_scalars(channel):
+    from . import c3d
+    out = [c3d.S_WSM1, c3d.S_WSM2, c3d.S_WSM3, c3d.S_TAOW]
+    if channel.wmodel == 2.0:
+        out += [c3d.S_NTIME, c3d.S_WMDT, c3d.S_CFLMAX, c3d.S_UINIT, c3d.S_USINIT, c3d.S_PBAR]
+    return out
+
+
 def save_checkpoint(path, channel):
+    """the resident state of `channel` (a c3d.Channel) to an .npz"""
+    from . import c3d
     st = channel.download()
-    np.savez(path, U=st["U"], P=st["P"], HH=st["HH"], SGSVIS=st["SGSVIS"],
-             PRESG=st["PRESG"], PRESG3=st["PRESG3"], grid=np.array([channel.n1, channel.n2, channel.n3]))
+    keys = _carried_scalars(channel)
+    z = dict(U=st["U"], P=st["P"], HH=st["HH"], SGSVIS=st["SGSVIS"], PRESG=st["PRESG"], PRESG3=st["PRESG3"],
+             grid=np.array([channel.n1, channel.n2, channel.n3]),
+             model=np.array([channel.files, channel.wmodel, channel.precision], dtype=np.float64),
+             scalar_ids=np.array(keys), scalars=np.array([channel.scalar(k) for k in keys]),
+             TAUW=channel.field(c3d.F_TAUW), TAUB=channel.field(c3d.F_TAUB))
+    if channel.wmodel == 2.0:
+        z["IWM"] = channel.field(c3d.F_IWM)
+    if channel.precision == c3d.MIXED:
+        z["DIV"] = channel.field(c3d.F_DIV)
+    np.savez(path, **z)
 
 
 def load_checkpoint(path, channel):
+    """restores a save_checkpoint file into `channel`, which must have the
+    same grid, FILES, WMODEL and precision"""
+    from . import c3d
     z = np.load(path, allow_pickle=False)
     if tuple(z["grid"]) != (channel.n1, channel.n2, channel.n3):
         raise ValueError("checkpoint grid does not match the channel")
+    if tuple(z["model"]) != (channel.files, channel.wmodel, float(channel.precision)):
+        raise ValueError("checkpoint FILES, WMODEL or precision does not match the channel")
     channel.upload(np.asfortranarray(z["U"]), np.asfortranarray(z["P"]), HH=np.asfortranarray(z["HH"]),
                    SGSVIS=np.asfortranarray(z["SGSVIS"]), presg=float(z["PRESG"]), presg3=float(z["PRESG3"]))
+    for k, v in zip(z["scalar_ids"], z["scalars"]):
+        channel.set_scalar(int(k), float(v))
+    channel.set_field(c3d.F_TAUW, z["TAUW"])
+    channel.set_field(c3d.F_TAUB, z["TAUB"])
+    if "IWM" in z:
+        channel.set_field(c3d.F_IWM, z["IWM"])
+    if "DIV" in z:
+        channel.set_field(c3d.F_DIV, z["DIV"])
 
 
 def result_names(ntime):

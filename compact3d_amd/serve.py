@@ -5,14 +5,22 @@ A request asks for `steps` time steps of one channel (grid, Reynolds
 number, optional Smagorinsky + wall model, precision) from a seeded initial
 field.  The loop admits waiting requests while their HBM footprint
 (`c3d_footprint_bytes`) fits the budget and a slot is free, keeps every
-admitted state resident, and advances all live instances together: each turn
-queues up to `chunk` steps of every live instance on that instance's own HIP
-stream (`c3d_advance`, replaying captured graphs), then waits for each.
-Work of different instances therefore overlaps on the device, requests that
-arrive meanwhile are admitted at the next turn, and a finished instance
-frees its memory and slot at once.  Each instance's arithmetic does not
-depend on what else is resident, so a result is the same alone and in the
-mix.
+admitted state resident, and advances the live instances as a rolling
+pipeline: every live instance always has a chunk of up to `chunk` steps
+queued on its own HIP stream (`c3d_advance`, replaying captured graphs).
+The loop waits for the instance whose chunk was queued first, reads its
+DIVMAX and CFL, queues its next chunk at once (or retires it), and admits
+any request that has arrived and now fits before waiting for the next one.
+Work of different instances therefore overlaps on the device, a request
+that arrives while others run is admitted as soon as a slot and memory are
+free (at the next chunk boundary of any instance, not after a whole round),
+and a finished instance frees its memory and slot at once.  Each instance's
+arithmetic does not depend on what else is resident, so a result is the
+same alone and in the mix.
+
+Failure detection: an instance whose DIVMAX or CFLM comes back NaN or
+infinite (c3d_sync's C3D_ERR_NONFINITE) has diverged; only that request is
+marked failed (`Request.error`) and evicted, and the others continue.
 
 The reference runs one channel per process and has no counterpart; this is
 the ensemble/serving driver over the library's C-ABI.
@@ -21,6 +29,7 @@ from __future__ import annotations
 
 import math
 import time
+from collections import deque
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -53,6 +62,7 @@ class Request:
     t_admit: float = float("nan")
     t_done: float = float("nan")
     result: dict = field(default_factory=dict)
+    error: str = ""       # set when the instance diverged (NaN/Inf DIVMAX or CFL)
 
     def config(self):
         n1m, n2, n3m = self.grid
@@ -117,56 +127,78 @@ class Server:
         U, P = self.fields.get(req, ch.mesh())
         ch.upload(U, P)
         req.t_admit = now
-        return dict(req=req, ch=ch, left=req.steps, bytes=self.footprint(req))
+        return dict(req=req, ch=ch, left=req.steps, bytes=self.footprint(req), n=0)
+
+    def _queue(self, rec):
+        n = min(self.chunk, rec["left"])
+        rec["ch"].advance(n, self.use_graph)
+        rec["n"] = n
+
+    @staticmethod
+    def _wait(rec):
+        """waits for rec's queued chunk; (divmax, cflm, error message or "")"""
+        from .c3d import C3DDiverged
+        try:
+            divmax, cflm = rec["ch"].sync()
+        except C3DDiverged as e:
+            return e.divmax, e.cflm, str(e)
+        if not (math.isfinite(divmax) and math.isfinite(cflm)):
+            return divmax, cflm, f"diverged: DIVMAX = {divmax}, CFLM = {cflm}"
+        return divmax, cflm, ""
 
     def run(self, requests, t0=None):
         """Serve `requests` (t_arrive relative to the start) to completion;
-        returns the finished requests in completion order."""
+        returns the finished requests in completion order.  Requests whose
+        instance diverged are in `self.failed` instead."""
         pending = sorted(requests, key=lambda r: r.t_arrive)
         for r in pending:
             if self.footprint(r) > self.budget:
                 raise ValueError(f"request {r.rid} needs {self.footprint(r)} B, more than the budget")
-        waiting, live, done = [], [], []
+        waiting, live, done = [], deque(), []
+        self.failed = []
         used = 0
         start = self.clock() if t0 is None else t0
-        while pending or waiting or live:
+
+        def admit_ready():
+            nonlocal used
             now = self.clock() - start
             while pending and pending[0].t_arrive <= now:
                 waiting.append(pending.pop(0))
             # admission in arrival order while memory and slots allow
             while waiting and len(live) < self.max_live and used + self.footprint(waiting[0]) <= self.budget:
                 rec = self._admit(waiting.pop(0), self.clock() - start)
+                self._queue(rec)
                 live.append(rec)
                 used += rec["bytes"]
                 self.peak_bytes = max(self.peak_bytes, used)
+
+        while pending or waiting or live:
+            admit_ready()
             if not live:
                 if pending:
                     self.sleep(max(0.0, pending[0].t_arrive - (self.clock() - start)))
                 continue
-            # one turn: queue work on every live instance, then wait for each
-            for rec in live:
-                n = min(self.chunk, rec["left"])
-                rec["ch"].advance(n, self.use_graph)
-                rec["n"] = n
-            for rec in live:
-                divmax, cflm = rec["ch"].sync()
-                rec["left"] -= rec["n"]
-                rec["divmax"], rec["cflm"] = divmax, cflm
-            still = []
-            for rec in live:
-                if rec["left"] > 0:
-                    still.append(rec)
-                    continue
-                req = rec["req"]
-                req.t_done = self.clock() - start
-                req.result = dict(divmax=rec["divmax"], cflm=rec["cflm"])
+            rec = live.popleft()
+            divmax, cflm, err = self._wait(rec)
+            rec["left"] -= rec["n"]
+            rec["divmax"], rec["cflm"] = divmax, cflm
+            req = rec["req"]
+            if not err and rec["left"] > 0:
+                self._queue(rec)
+                live.append(rec)
+                continue
+            req.t_done = self.clock() - start
+            req.result = dict(divmax=divmax, cflm=cflm)
+            if err:
+                req.error = err
+                self.failed.append(req)
+            else:
                 if self.keep_results:
                     st = rec["ch"].download()
                     req.result.update(U=st["U"], P=st["P"], PRESG=st["PRESG"])
-                rec["ch"].close()
-                used -= rec["bytes"]
                 done.append(req)
-            live = still
+            rec["ch"].close()
+            used -= rec["bytes"]
         return done
 
 

@@ -10,6 +10,10 @@
  * Conventions
  *  - Status codes: 0 = success, < 0 = error; the message is available from
  *    c3d_last_error().  No exception or longjmp crosses the ABI.
+ *    C3D_ERR_NONFINITE (c3d_step, c3d_sync, c3d_ensemble_sync) says that
+ *    the step ran but its DIVMAX or CFLM is NaN or infinite: the state has
+ *    diverged (the reference has no status returns; its runs go on with
+ *    NaNs, compact3d.f:130-163).
  *  - Host arrays use the reference's Fortran layouts with M1=N1, M2=N2,
  *    M3=N3 (compact3d.f:55-60):
  *      U, HH  (0:N1, 0:N2, 0:N3, 3)   column-major, I fastest
@@ -28,6 +32,9 @@ extern "C" {
 #endif
 
 typedef struct c3d_handle c3d_handle;
+
+#define C3D_ERR (-1)
+#define C3D_ERR_NONFINITE (-3)
 
 enum c3d_precision {
   C3D_FP64 = 0,  /* every operator in FP64 */
@@ -51,8 +58,8 @@ typedef struct c3d_config {
   double files;       /* SGS model switch FILES: 0 none (DNS), 1 dynamic (SGSDY), 2 Smagorinsky (SGSCS),
                          3 scale-dependent dynamic (SGSSD) */
   double wmodel;      /* WMODEL: 0 resolved walls, 1 instantaneous log-law wall model (bottom wall),
-                         2 the integral wall model IWMLES for a smooth wall (bottom wall; set
-                           CFLMAX with c3d_set_scalar before the first step),
+                         2 the integral wall model IWMLES for a smooth wall (bottom wall; needs
+                           cflmax > 0),
                          4 Werner-Wengle wall model (both walls; needs N3 <= N1),
                          6 the log-law model at both walls (no counterpart in the
                          reference, whose model 1 leaves the top wall unmodelled) */
@@ -61,6 +68,8 @@ typedef struct c3d_config {
   int precision;      /* enum c3d_precision */
   int device;         /* HIP device ordinal */
   void *stream;       /* hipStream_t to run on, or NULL for the library's own */
+  double cflmax;      /* CFLMAX of /TSTEP/ (compact3d.f:247): WMODEL = 2 sets its first
+                         time-filter weight from it (:3700) and needs it > 0; unused otherwise */
 } c3d_config;
 
 /* SETUP/MESH/INDICES/INIWAVE/METRICPOISSON (compact3d.f:192-299, 302-494,
@@ -95,7 +104,9 @@ int c3d_step(c3d_handle *h, int ntime, double *divmax, double *cflm);
  * c3d_sync applies it to the last step. */
 int c3d_advance(c3d_handle *h, int nsteps, int use_graph);
 /* waits for the handle's stream and returns DIVMAX and CFLM of the last
- * step (mixed mode: after the refinement check); either may be NULL */
+ * step (mixed mode: after the refinement check); either may be NULL.
+ * C3D_ERR_NONFINITE when either is NaN or infinite (both are still
+ * returned). */
 int c3d_sync(c3d_handle *h, double *divmax, double *cflm);
 
 /* Ensembles (no counterpart in the reference, which runs one channel per
@@ -184,8 +195,10 @@ enum c3d_field {
   C3D_F_RDP, C3D_F_DP,                    /* P-like */
   C3D_F_HH1, C3D_F_HH2, C3D_F_HH3,        /* current HH, U-like single component */
   C3D_F_TAUW, C3D_F_TAUB,                 /* (0:N1,0:N3,2) */
-  C3D_F_IWM  /* WMODEL = 2: the 43 planes of the reference's COMMON /IWMLES/ (compact3d.f:3399-3408),
+  C3D_F_IWM, /* WMODEL = 2: the 43 planes of the reference's COMMON /IWMLES/ (compact3d.f:3399-3408),
                 N1-1 x N3-1 each (I fastest), in that order */
+  C3D_F_DIV  /* P-like: the divergence DIVCHECK forms (compact3d.f:9112) of the resident field (mixed
+                mode: the FP64 DIV the next step's RDP starts from) */
 };
 /* copies one device field into a host array in its Fortran layout */
 int c3d_get_field(c3d_handle *h, int which, double *out);
@@ -193,12 +206,19 @@ int c3d_get_field(c3d_handle *h, int which, double *out);
  * 6 WSM(1) 7 WSM(2) 8 WSM(3); get only (mixed mode): 9 refinement passes
  * run, 10 DIVMAX of the last step's first projection; WMODEL = 2: 11 NTIME
  * of the last wall-model call (c3d_advance and c3d_wall_model continue
- * from it), 12 CFLMAX (/TSTEP/, sets the first filter weight; default 0),
- * 13 wm_dt */
+ * from it), 12 CFLMAX (/TSTEP/, sets the first filter weight; the config's
+ * cflmax), 13 wm_dt; 14 TAOW (/WALLTAO/); WMODEL = 2: 15 the initial mean
+ * velocity, 16 the initial friction velocity, 17 the mean first-cell
+ * pressure of the last compute step.  Slots 11-17 with 6-8 and the fields
+ * C3D_F_TAUW, C3D_F_TAUB, C3D_F_IWM (and C3D_F_DIV in mixed mode) are the
+ * state a step carries beyond U, P, HH and PRESG: a checkpoint saves them. */
+#define C3D_NSCALAR 18
 int c3d_get_scalar(c3d_handle *h, int which, double *out);
 int c3d_set_scalar(c3d_handle *h, int which, double value);
-/* the inverse of c3d_get_field, for C3D_F_IWM only (a saved or constructed
- * state of the integral wall model) */
+/* the inverse of c3d_get_field for the carried state: C3D_F_TAUW and
+ * C3D_F_TAUB ((0:N1,0:N3,2), entries outside I = 1..N1-1, K = 1..N3-1
+ * ignored), C3D_F_IWM (a saved or constructed state of the integral wall
+ * model) and C3D_F_DIV (P-like) */
 int c3d_set_field(c3d_handle *h, int which, const double *in);
 int c3d_set_dt(c3d_handle *h, double dt);
 /* the grid MESH built (compact3d.f:302-443): Y(0:N2) and {DX1, DX3, ALY, VOL} */

@@ -1,7 +1,8 @@
 """The Fortran drop-in: the project's ISO_C_BINDING module and wrappers named
 GETUP/DIVCHECK/CFL/CHKMF/WALLSS/WRITEUP, driven by a MAIN-like loop reading a
 READUP CONTINUE file, give the same trajectory and the same WRITEUP file as
-the Python binding of the same library."""
+the Python binding of the same library; GETUP's host sync puts the stepped
+field into the caller's arrays."""
 import os
 import subprocess
 
@@ -33,17 +34,25 @@ def test_fortran_driver_matches_python(tmp_path, files, wmodel):
     write_continue(str(cont), U, P, 0.0, 0.0)
     nst = 6
     args = [exe, str(n1), str(n2), str(n3), repr(cfg["re"]), repr(cfg["alx"]), repr(cfg["aly"]),
-            repr(cfg["alz"]), repr(cfg["dt"]), repr(files), repr(wmodel), repr(cfg["cks"]), str(nst), "1", str(cont)]
+            repr(cfg["alz"]), repr(cfg["dt"]), repr(files), repr(wmodel), repr(cfg["cks"]), str(nst), "1", str(cont),
+            "0.5", "3"]
     (tmp_path / "RESULT").mkdir()
     out = subprocess.run(args, check=True, capture_output=True, text=True, timeout=120, cwd=str(tmp_path)).stdout
     print(out)
     lines = [ln for ln in out.splitlines() if ln.startswith("STEP=")]
     assert len(lines) == nst
     presg_f = float(lines[-1].split("DP=")[1].split()[0])
-    ch = c3d.Channel(**cfg)
+    ch = c3d.Channel(**cfg, cflmax=0.5)
     ch.upload(U, P)
+    # GETUP's host sync (every 3 steps): the caller's U holds the stepped field
+    syncs = {int(ln.split()[0].split("=")[1]): float(ln.split("U(5,5,5,1)=")[1])
+             for ln in out.splitlines() if ln.startswith("HOSTSYNC=")}
+    assert sorted(syncs) == [3, 6]
     for n in range(nst):
         ch.getup(n + 1)
+        if n + 1 in syncs:
+            u = ch.download()["U"][5, 5, 5, 0]
+            assert abs(syncs[n + 1] - u) <= 1e-14 * abs(u), (n + 1, syncs[n + 1], u)
     presg_p = ch.scalar(c3d.S_PRESG)
     u555 = ch.download()["U"][5, 5, 5, 0]
     u_f = float([ln for ln in out.splitlines() if ln.startswith("U(5,5,5,1)=")][0].split("=")[1])

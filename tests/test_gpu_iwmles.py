@@ -41,9 +41,9 @@ def _setup(seed=7):
     from pyoracle import Oracle
     cfg = config(33, 33, 33, re=5200.0, dt=5e-4, files=2.0, wmodel=2.0)
     orc = Oracle(**cfg)
-    ch = c3d.Channel(**cfg)
+    ch = c3d.Channel(**cfg, cflmax=0.5)
     orc.set_cflmax(0.5)
-    ch.set_scalar(c3d.S_CFLMAX, 0.5)
+    assert ch.scalar(c3d.S_CFLMAX) == 0.5
     return cfg, orc, ch, c3d
 
 
@@ -73,6 +73,9 @@ def test_iwmles_from_its_initialisation():
         orc.wallmodel_iwmles(U, P, 0.003, -0.001, nt)
         ch.wall_model()
         fixed = _check_call(orc, ch, c3d, nt)
+        if nt == 1:
+            # the time-filter weight, set from CFLMAX at the initialisation
+            assert np.min(np.abs(ch.field(c3d.F_IWM)[34])) > 0
         if nt % 5 == 0:
             assert fixed > 0
     ch.close()
@@ -118,8 +121,7 @@ def test_iwmles_trajectory_and_graph_replay():
     print("IWMLES 10 steps U", ["%.3e" % x for x in eu], flush=True)
     assert max(eu) <= 1e-9
     _cmp_planes(orc.iwm_state()[0], ch.field(c3d.F_IWM), "trajectory")
-    g = c3d.Channel(**cfg)
-    g.set_scalar(c3d.S_CFLMAX, 0.5)
+    g = c3d.Channel(**cfg, cflmax=0.5)
     g.upload(U, P)
     g.advance(10, use_graph=True)
     sg = g.download()
@@ -131,11 +133,14 @@ def test_iwmles_trajectory_and_graph_replay():
 
 
 def test_iwm_field_errors():
-    """the state field exists only with WMODEL = 2, and only it can be set"""
+    """the state field exists only with WMODEL = 2; only the carried state
+    can be set; WMODEL = 2 needs CFLMAX > 0 at creation"""
     from compact3d_amd import c3d
     cfg, orc, ch, _ = _setup()
     with pytest.raises(c3d.C3DError):
-        ch.set_field(c3d.F_TAUW, np.zeros(2 * 34 * 34))
+        ch.set_field(c3d.F_RUH1, np.zeros((34, 34, 34), order="F"))
+    with pytest.raises(c3d.C3DError, match="cflmax"):
+        c3d.Channel(**cfg)
     other = c3d.Channel(**dict(cfg, wmodel=1.0))
     with pytest.raises(c3d.C3DError):
         other.field(c3d.F_IWM)

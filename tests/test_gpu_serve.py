@@ -78,7 +78,7 @@ def test_ensemble_equals_alone():
     from compact3d_amd import c3d
     specs = [(_cfg(49, 41, 33), 0, 5), (_cfg(33, 33, 33, les=True, re=20000.0), 1, 6),
              (_cfg(65, 33, 49), 1, 7), (_cfg(33, 33, 33, les=True, re=20000.0), 0, 8),
-             (dict(_cfg(33, 33, 33, les=True, re=20000.0), wmodel=2.0), 1, 9)]
+             (dict(_cfg(33, 33, 33, les=True, re=20000.0), wmodel=2.0, cflmax=0.5), 1, 9)]
     ens_ch, alone = [], []
     for cfg, prec, seed in specs:
         pair = [c3d.Channel(**cfg, precision=prec) for _ in range(2)]

@@ -114,3 +114,67 @@ def test_stats_and_request_stream():
     s = stats(reqs, 10.0)
     assert s["p50_s"] == pytest.approx(0.5) and s["p99_s"] == pytest.approx(0.5)
     assert s["throughput_mpts_steps_s"] > 0
+
+
+class DivergingChannel(FakeChannel):
+    """a stand-in whose state diverges: sync() reports NaN from its second
+    chunk on"""
+
+    def sync(self):
+        return (float("nan"), float("nan")) if self.steps > 10 else (1e-14, 0.1)
+
+
+def test_diverged_request_is_failed_and_evicted():
+    FakeChannel.live = FakeChannel.peak = 0
+    clock = FakeClock()
+    log = []
+    bad_rid = 1
+
+    def make(r):
+        cls = DivergingChannel if r.rid == bad_rid else FakeChannel
+        return cls(r, clock, log)
+    reqs = [Request(rid=q, grid=(8, 9, 8), re=100.0, steps=40, t_arrive=0.0) for q in range(3)]
+    # the budget holds two instances: the third is admitted when the diverged one is evicted
+    srv = Server(2 * reqs[0].pts, max_live=8, chunk=10, make_channel=make, footprint=lambda r: r.pts,
+                 fields=NoFields(), clock=clock, sleep=clock.sleep)
+    done = srv.run(reqs)
+    assert sorted(r.rid for r in done) == [0, 2]
+    assert [r.rid for r in srv.failed] == [bad_rid]
+    bad = srv.failed[0]
+    assert "DIVMAX" in bad.error and bad.result["divmax"] != bad.result["divmax"]
+    ran = {}
+    for rid, n in log:
+        ran[rid] = ran.get(rid, 0) + n
+    assert ran[bad_rid] == 20 and ran[0] == 40 and ran[2] == 40
+    assert reqs[2].t_admit >= bad.t_done  # admitted into the freed memory
+    assert FakeChannel.live == 0
+
+
+def test_diverged_request_raised_by_the_binding():
+    """the binding's C3DDiverged (C3D_ERR_NONFINITE) is handled like a NaN"""
+    from compact3d_amd.c3d import C3DDiverged
+    FakeChannel.live = FakeChannel.peak = 0
+    clock = FakeClock()
+    log = []
+
+    class Raising(FakeChannel):
+        def sync(self):
+            raise C3DDiverged("sync: the state diverged", float("inf"), 1.0)
+    srv = Server(10 ** 9, max_live=8, chunk=10, make_channel=lambda r: (Raising if r.rid == 0 else FakeChannel)(
+        r, clock, log), footprint=lambda r: r.pts, fields=NoFields(), clock=clock, sleep=clock.sleep)
+    done = srv.run([Request(rid=q, grid=(8, 9, 8), re=100.0, steps=30, t_arrive=0.0) for q in range(2)])
+    assert [r.rid for r in done] == [1] and [r.rid for r in srv.failed] == [0]
+    assert srv.failed[0].result["divmax"] == float("inf")
+
+
+def test_arrival_admitted_at_the_next_chunk_boundary():
+    """a request arriving while instances run is admitted when the next
+    instance's chunk completes, before the other live instances' chunks"""
+    FakeChannel.live = FakeChannel.peak = 0
+    reqs = [Request(rid=q, grid=(8, 9, 8), re=100.0, steps=100, t_arrive=0.0) for q in range(3)]
+    late = Request(rid=9, grid=(8, 9, 8), re=100.0, steps=10, t_arrive=0.025)
+    srv, clock, log = _server(budget=10 ** 9, max_live=8, chunk=10)
+    srv.run(reqs + [late])
+    # first queue: 0, 1, 2 (clock 0.03); the late request joins right after
+    # the first completion, ahead of the re-queues of instances 1 and 2
+    assert log[:5] == [(0, 10), (1, 10), (2, 10), (0, 10), (9, 10)]
