@@ -1851,7 +1851,15 @@ extern "C" int c3d_get_field(c3d_handle* h, int which, double* out) {
     case C3D_F_UH3: src = F.uh[2]; break;
     case C3D_F_RDP: src = F.rdp; lay = LAY_P; break;  // FP64 in both modes
     case C3D_F_DP: src = F.dp; lay = LAY_P; break;  // FP64 in both modes
-    case C3D_F_DIV: return from_dev(h, out, F.div, LAY_P);  // FP64 in both modes
+    case C3D_F_DIV:
+      // mixed mode keeps the full DIV of the state (the next RDP starts from
+      // it); the FP64 step's update kernels keep only its x and y parts, so
+      // DIVCHECK forms it here
+      if (!h->c.s.mixed) {
+        c3dk::divcheck(h->c);
+        if (launched(h)) return -1;
+      }
+      return from_dev(h, out, F.div, LAY_P);
     case C3D_F_HH1: src = F.hh[0]; break;
     case C3D_F_HH2: src = F.hh[1]; break;
     case C3D_F_HH3: src = F.hh[2]; break;

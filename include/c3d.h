@@ -198,7 +198,8 @@ enum c3d_field {
   C3D_F_IWM, /* WMODEL = 2: the 43 planes of the reference's COMMON /IWMLES/ (compact3d.f:3399-3408),
                 N1-1 x N3-1 each (I fastest), in that order */
   C3D_F_DIV  /* P-like: the divergence DIVCHECK forms (compact3d.f:9112) of the resident field (mixed
-                mode: the FP64 DIV the next step's RDP starts from) */
+                mode: the FP64 DIV the last step's update formed, which the next step's RDP starts
+                from; FP64 mode: formed by DIVCHECK when read) */
 };
 /* copies one device field into a host array in its Fortran layout */
 int c3d_get_field(c3d_handle *h, int which, double *out);

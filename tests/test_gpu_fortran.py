@@ -45,7 +45,7 @@ def test_fortran_driver_matches_python(tmp_path, files, wmodel):
     ch = c3d.Channel(**cfg, cflmax=0.5)
     ch.upload(U, P)
     # GETUP's host sync (every 3 steps): the caller's U holds the stepped field
-    syncs = {int(ln.split()[0].split("=")[1]): float(ln.split("U(5,5,5,1)=")[1])
+    syncs = {int(ln.split("=")[1].split()[0]): float(ln.split("U(5,5,5,1)=")[1])
              for ln in out.splitlines() if ln.startswith("HOSTSYNC=")}
     assert sorted(syncs) == [3, 6]
     for n in range(nst):

@@ -38,6 +38,43 @@ def _pair(cfg, seed):
     return a, b, U, P
 
 
+def test_mixed_vs_oracle_10_steps():
+    """mixed precision against the FP64 oracle directly (the reference's
+    algorithm on the CPU): per-field max and L2 relative errors after 10
+    steps, and the mean-velocity profile"""
+    from pyoracle import Oracle
+    from compact3d_amd import c3d
+    n1, n2, n3 = 65, 65, 65
+    cfg = config(n1, n2, n3, cks=1.5)
+    orc = Oracle(**cfg)
+    U, P, _ = inputs(orc, seed=6)
+    ch = c3d.Channel(**cfg, precision=c3d.MIXED)
+    ch.upload(U, P)
+    Uo, Po = U.copy(order="F"), P.copy(order="F")
+    HH, S = orc.zeros_u(), orc.zeros_u(0)
+    pg = pg3 = 0.0
+    for n in range(10):
+        pg, pg3 = orc.getup(Uo, Po, HH, S, pg, pg3, n + 1)
+        d, _ = ch.step(n + 1)
+        assert d <= 1e-9
+    st = ch.download()
+    out = {}
+    for name, c, j0 in (("u", 0, 1), ("v", 1, 2), ("w", 2, 1)):
+        a, b = interior_u(Uo, n1, n2, n3, c, j0=j0), interior_u(st["U"], n1, n2, n3, c, j0=j0)
+        out[name] = (relerr(a, b), float(np.linalg.norm(a - b) / np.linalg.norm(a)))
+    a, b = interior_p(Po, n1, n2, n3), interior_p(st["P"], n1, n2, n3)
+    out["P"] = (relerr(a, b), float(np.linalg.norm(a - b) / np.linalg.norm(a)))
+    um_o = interior_u(Uo, n1, n2, n3, 0).mean(axis=(0, 2))
+    um_g = interior_u(st["U"], n1, n2, n3, 0).mean(axis=(0, 2))
+    em = float(np.max(np.abs(um_o - um_g)) / np.max(np.abs(um_o)))
+    print("mixed vs oracle after 10 steps, max / L2:", {k: "%.2e / %.2e" % v for k, v in out.items()},
+          "mean u %.2e" % em, "PRESG %.3e vs %.3e" % (st["PRESG"], pg))
+    assert max(out[k][0] for k in ("u", "v", "w")) <= 1e-5
+    assert out["P"][0] <= 1e-3
+    assert em <= 1e-5
+    ch.close()
+
+
 def test_mixed_trajectory_10_steps():
     n1, n2, n3 = 65, 65, 65
     cfg = config(n1, n2, n3, cks=1.5)
@@ -94,7 +131,7 @@ def test_mixed_state_roundtrip():
     b.close()
 
 
-@pytest.mark.parametrize("wmodel", [1.0, 0.0, 2.0])
+@pytest.mark.parametrize("wmodel", [1.0, 0.0, 2.0, 6.0])
 def test_mixed_les_trajectory_10_steps(wmodel):
     """wall-modelled (log law; the integral wall model, whose compute steps
     are NTIME 5 and 10) / wall-resolved Smagorinsky LES: mixed against FP64"""
@@ -106,7 +143,7 @@ def test_mixed_les_trajectory_10_steps(wmodel):
     U, P, _ = inputs(orc, seed=9)
     chs = []
     for prec in (c3d.FP64, c3d.MIXED):
-        ch = c3d.Channel(**cfg, precision=prec)
+        ch = c3d.Channel(**cfg, precision=prec, cflmax=0.5)
         ch.upload(U, P)
         ch.set_scalar(c3d.S_WSM1, 1e-3)
         ch.set_scalar(c3d.S_WSM3, 1e-4)
