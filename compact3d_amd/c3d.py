@@ -85,6 +85,7 @@ def lib():
         L.c3d_chkmf.argtypes = [C.c_void_p, _dp, _dp]
         L.c3d_flowrate.argtypes = [C.c_void_p, _dp, _dp]
         L.c3d_stats.argtypes = [C.c_void_p, _dp]
+        L.c3d_stats_noalloc.argtypes = [C.c_void_p, _dp]
         L.c3d_cellcentre.argtypes = [C.c_void_p, _dp]
         L.c3d_sgs_profiles.argtypes = [C.c_void_p, _dp, _dp]
         L.c3d_write_flow.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
@@ -411,6 +412,12 @@ class Channel:
         rows in the order of STAT_NAMES (J = 1..N2-1 along each row)"""
         out = np.zeros(20 * (self.n2 - 1))
         self._ck(self.L.c3d_stats(self.h, _p(out)), "stats")
+        return out.reshape(20, self.n2 - 1)
+
+    def stats_noalloc(self):
+        """stats() through c3d_stats_noalloc (no allocation; N2-1 <= 2048)"""
+        out = np.zeros(20 * (self.n2 - 1))
+        self._ck(self.L.c3d_stats_noalloc(self.h, _p(out)), "stats_noalloc")
         return out.reshape(20, self.n2 - 1)
 
     def sgs_profiles(self):

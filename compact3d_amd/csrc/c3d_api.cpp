@@ -63,6 +63,41 @@ static int fail(c3d_handle* h, const std::string& m) {
   return -1;
 }
 
+// ---------------------------------------------- the allocation-free scratch
+// c3d_stats_noalloc stages its 20 x (N2-1) plane statistics in this
+// file-scope device array instead of the handle's scratch field.  Every GPU
+// the library runs on holds its own copy (C3D_STATS_NOALLOC_MAX_PLANES * 20
+// doubles, 320 KiB) until the process ends.  In host code `g_scratch` names a
+// host-side stand-in, not a device address: only the per-device pointers
+// from hipGetSymbolAddress (g_scratch_ptr) are passed to kernels and copies.
+static __device__ __align__(256) double g_scratch[20 * C3D_STATS_NOALLOC_MAX_PLANES];
+static const int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices] = {};
+
+// once per device, with that device current (c3d_create calls it)
+static int scratch_init(int device, std::string& err) {
+  if (device < 0 || device >= kMaxDevices) {
+    err = "device ordinal out of range";
+    return -1;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return 0;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(g_scratch));
+  if (e != hipSuccess || !p) {
+    err = std::string("hipGetSymbolAddress of the stats scratch: ") + hipGetErrorString(e);
+    return -1;
+  }
+  g_scratch_ptr[device] = (double*)p;
+  return 0;
+}
+
+static double* scratch_of(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return (device >= 0 && device < kMaxDevices) ? g_scratch_ptr[device] : nullptr;
+}
+
 // ------------------------------------------------------------ line solves
 static LineSolve make_solve(double a, double b) {
   // a*x[p-1] + b*x[p] + a*x[p+1] = d[p]  ->  alpha, scale, warm-up (c3d_dev.h)
@@ -534,6 +569,10 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   if (cf.wmodel == 2.0 && !(cf.cflmax > 0)) return bad("WMODEL = 2 needs cflmax > 0 (CFLMAX of /TSTEP/)");
   if (cf.re <= 0 || cf.dt <= 0 || cf.alx <= 0 || cf.alz <= 0) return bad("RE, DT, ALX, ALZ must be > 0");
   if (hipSetDevice(cf.device) != hipSuccess) return bad("hipSetDevice failed");
+  {
+    std::string serr;
+    if (scratch_init(cf.device, serr)) return bad(serr);
+  }
   Ctx& c = h->c;
   c.kt = &h->kt;
   Geo& g = c.g;
@@ -1641,6 +1680,24 @@ extern "C" int c3d_chkmf(c3d_handle* h, double* flow1, double* flow3) {
   if (launched(h)) return -1;
   if (flow1 && read_sc(h, SC_CHK1, flow1)) return -1;
   if (flow3 && read_sc(h, SC_CHK3, flow3)) return -1;
+  return 0;
+}
+
+// the same statistics with the scratch in g_scratch: no allocation of any
+// kind; calls that overlap on one GPU share (and race on) that GPU's copy
+extern "C" int c3d_stats_noalloc(c3d_handle* h, double* out) {
+  if (!h) return -1;
+  if (!out) return fail(h, "c3d_stats_noalloc needs an output array of 20*(N2-1) values");
+  if (h->c.g.n2m > C3D_STATS_NOALLOC_MAX_PLANES)
+    return fail(h, "c3d_stats_noalloc: N2-1 = " + std::to_string(h->c.g.n2m) + " exceeds its scratch of " +
+                       std::to_string(C3D_STATS_NOALLOC_MAX_PLANES) + " planes (use c3d_stats)");
+  double* sp = scratch_of(h->cfg.device);
+  if (!sp) return fail(h, "c3d_stats_noalloc: no scratch address for this device");
+  const size_t n = 20 * (size_t)h->c.g.n2m;
+  c3dk::plane_stats(h->c, sp);
+  if (launched(h)) return -1;
+  HIPCHK(hipMemcpyAsync(out, sp, n * sizeof(double), hipMemcpyDeviceToHost, h->c.st));
+  HIPCHK(hipStreamSynchronize(h->c.st));
   return 0;
 }
 

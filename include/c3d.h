@@ -162,6 +162,15 @@ int c3d_flowrate(c3d_handle *h, double *flow1, double *flow3);
  * (mean vorticity) 7 VRMS(J,1..6) (uu vv ww uv uw vw) 13 PRMS 14 PV
  * 15 WRMS(J,1..3) 18 U2V 19 VW2; 20*(N2-1) values */
 int c3d_stats(c3d_handle *h, double *out);
+/* c3d_stats without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops).  Its kernel stages the statistics in a library-owned device array
+ * of C3D_STATS_NOALLOC_MAX_PLANES * 20 doubles per GPU (held until the
+ * process ends) instead of the handle's scratch field; grids with N2-1
+ * above that limit are refused (use c3d_stats).  Calls that overlap on one
+ * GPU share that array and race. */
+#define C3D_STATS_NOALLOC_MAX_PLANES 2048
+int c3d_stats_noalloc(c3d_handle *h, double *out);
 /* the dynamic models' per-plane profiles of the last SGS evaluation:
  * smagc[J-1] = SMAGC(J) (COMMON/SGSMEAN, FILES = 1 or 3) and beta[J-1] =
  * SGSSD's BETA(J) (the BETAZ(J) of BETA.plt before its 0.125 floor,
