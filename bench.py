@@ -63,8 +63,10 @@ def workload(grid):
                 dt=7.5e-4, files=0.0, wmodel=0.0, cks=1.8)
 
 
-def cpu_baseline(cfg, target_s, seed):
-    """FP64 oracle on the host cores: a bounded sample of the same workload."""
+def oracle_run(cfg, seed, target_s, max_steps):
+    """the FP64 oracle (OpenMP on the host cores) stepping the seeded field of
+    `cfg` (GETUP + DIVCHECK + CFL per step, as MAIN) until max_steps or
+    target_s seconds; returns (steps, seconds, threads, U, P)"""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from compact3d_amd.initfield import perturbed_poiseuille
     from pyoracle import Oracle
@@ -84,13 +86,214 @@ def cpu_baseline(cfg, target_s, seed):
         o.cfl(U)
         steps += 1
         el = time.perf_counter() - t0
-        if el >= target_s or steps >= 50:
+        if el >= target_s or steps >= max_steps:
             break
+    return steps, el, threads, U, P
+
+
+def cpu_baseline(cfg, target_s, seed):
+    """FP64 oracle on the host cores: a bounded sample of the same workload.
+    Returns the baseline record and the oracle's field after its steps."""
+    steps, el, threads, U, P = oracle_run(cfg, seed, target_s, 50)
     pts = (cfg["n1"] - 1) * (cfg["n2"] - 1) * (cfg["n3"] - 1)
-    return dict(value=pts * steps / el / 1e6, unit="Mpts·steps/s", cores=threads, kind="port",
-                sample=f"{steps} steps (GETUP+DIVCHECK+CFL) of the same {cfg['n1']-1}x{cfg['n2']}x{cfg['n3']-1} "
-                       f"FP64 workload by the C oracle (bit-exact restatement of the reference's "
-                       f"subroutines), OpenMP {threads} threads, {el:.1f} s")
+    rec = dict(value=pts * steps / el / 1e6, unit="Mpts·steps/s", cores=threads, kind="port",
+               sample=f"{steps} steps (GETUP+DIVCHECK+CFL) of the same {cfg['n1']-1}x{cfg['n2']}x{cfg['n3']-1} "
+                      f"FP64 workload by the C oracle (bit-exact restatement of the reference's "
+                      f"subroutines), OpenMP {threads} threads, {el:.1f} s")
+    return rec, steps, U, P
+
+
+def calibration(reps=20):
+    """the oracle against the reference's own subroutines (oracle/_ref, built
+    from compact3d.f) on the reference's PARAM.H grid 32x33x32, one thread
+    each (the reference library is built without OpenMP): seconds per pass
+    of BCOND, RHS1-3, GETUH1-3, RHSDP, UPCALC, DIVCHECK and CFL (GETDP needs
+    FFTW, which the reference build lacks; both sides take the oracle's DP),
+    and their ratio.  None when the reference library was not built."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from pyoracle import REF_SO, Oracle, RefLib
+    if not os.path.exists(REF_SO):
+        return None
+    from cases import inputs
+    cfg = dict(re=4200.0, alx=2 * math.pi, aly=2.0, alz=math.pi, dt=1e-3, files=0.0, wmodel=0.0)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    Oracle.set_threads(1)
+    try:
+        ref = RefLib(**cfg)
+        o = Oracle(ref.n1, ref.n2, ref.n3, cks=2.5, **cfg)
+        U0, P0, HH2 = inputs(o, seed=7)
+        S = o.zeros_u(0)
+        DP = o.zeros_p()
+        RDP0 = o.zeros_p()
+        UH0 = o.zeros_u()
+        RUH0 = [o.zeros_u(0) for _ in range(3)]
+        HH0 = o.zeros_u()
+        for c in (1, 2, 3):
+            o.rhs(c, U0, P0, HH0, HH2, RUH0[c - 1], 0.003, S)
+            o.getuh(c, U0, UH0, RUH0[c - 1])
+        o.rhsdp(RDP0, UH0)
+        o.getdp(DP, RDP0)
+
+        def one(lib, is_ref):
+            U, P = U0.copy(order="F"), P0.copy(order="F")
+            HH, UH, RDP = o.zeros_u(), o.zeros_u(), o.zeros_p()
+            RUH = [o.zeros_u(0) for _ in range(3)]
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                lib.bcond(U, S)
+                for c in (1, 2, 3):
+                    lib.rhs(c, U, P, HH, HH2, RUH[c - 1], 0.003, S)
+                for c in (1, 2, 3):
+                    if is_ref:
+                        lib.getuh(c, U, UH, RUH[c - 1], S)
+                    else:
+                        lib.getuh(c, U, UH, RUH[c - 1])
+                if is_ref:
+                    lib.rhsdp(RDP, UH, U)
+                else:
+                    lib.rhsdp(RDP, UH)
+                lib.upcalc(U, P, UH, DP, 0.003, -0.001)
+                lib.divcheck(U)
+                lib.cfl(U)
+            return (time.perf_counter() - t0) / reps
+
+        one(ref, True)
+        one(o, False)
+        t_ref, t_orc = one(ref, True), one(o, False)
+    finally:
+        Oracle.set_threads(threads)
+    return dict(grid="32x33x32", threads=1, ref_s_per_pass=round(t_ref, 6), oracle_s_per_pass=round(t_orc, 6),
+                ref_over_oracle=round(t_ref / t_orc, 3),
+                phases="BCOND RHS1-3 GETUH1-3 RHSDP UPCALC DIVCHECK CFL (GETDP excluded: FFTW absent)")
+
+
+def field_errors(ref_U, ref_P, got_U, got_P, n1, n2, n3):
+    """per-field max and L2 errors relative to the reference field, and the
+    mean-velocity (bulk u) error"""
+    import numpy as np
+    out = {}
+    for name, comp, j0 in (("u", 0, 1), ("v", 1, 2), ("w", 2, 1)):
+        a_ = ref_U[1:n1, j0:n2, 1:n3, comp]
+        b_ = got_U[1:n1, j0:n2, 1:n3, comp]
+        out[name] = dict(max=float(np.abs(a_ - b_).max() / np.abs(a_).max()),
+                         l2=float(np.linalg.norm(a_ - b_) / np.linalg.norm(a_)))
+    a_, b_ = ref_P[: n1 - 1, : n2 - 1, : n3 - 1], got_P[: n1 - 1, : n2 - 1, : n3 - 1]
+    out["p"] = dict(max=float(np.abs(a_ - b_).max() / np.abs(a_).max()),
+                    l2=float(np.linalg.norm(a_ - b_) / np.linalg.norm(a_)))
+    ua = ref_U[1:n1, 1:n2, 1:n3, 0].mean(axis=(0, 2))
+    ub = got_U[1:n1, 1:n2, 1:n3, 0].mean(axis=(0, 2))
+    out["mean_u_profile"] = float(np.abs(ua - ub).max() / np.abs(ua).max())
+    return out
+
+
+def accuracy_leg(c3d, cfg, U, P, nsteps):
+    """nsteps of the mixed and the FP64 path from the same field, DIVMAX read
+    back after every step: the largest DIVMAX of each and the mixed path's
+    per-field errors against the FP64 path at the end"""
+    chs = [c3d.Channel(**cfg, precision=p) for p in (c3d.FP64, c3d.MIXED)]
+    for ch in chs:
+        ch.upload(U, P)
+    dmax = [0.0, 0.0]
+    t0 = time.perf_counter()
+    for n in range(1, nsteps + 1):
+        for q, ch in enumerate(chs):
+            d, _ = ch.step(n)
+            dmax[q] = max(dmax[q], d) if d == d else float("nan")
+    el = time.perf_counter() - t0
+    s64, smx = chs[0].download(), chs[1].download()
+    errs = field_errors(s64["U"], s64["P"], smx["U"], smx["P"], cfg["n1"], cfg["n2"], cfg["n3"])
+    for ch in chs:
+        ch.close()
+    return dict(steps=nsteps, divmax_max_fp64=dmax[0], divmax_max_mixed=dmax[1],
+                mixed_divmax_le_1e9_every_step=bool(dmax[1] <= 1e-9), mixed_vs_fp64=errs,
+                mean_u_le_1e5=bool(errs["mean_u_profile"] <= 1e-5), seconds=round(el, 2))
+
+
+def configs_leg(c3d, args):
+    """BASELINE's two smaller configurations on the GPU (FP64 and mixed) and
+    the oracle on the host cores: 64x65x64 (Re_tau 180, tanh 1.5, dt 1e-3,
+    200 steps; the oracle runs all of them) and 192x129x192 (500 steps; the
+    oracle a bounded sample)"""
+    from compact3d_amd.initfield import perturbed_poiseuille
+    out = {}
+    for name, grid, steps, cpu_s, seed in (("c0", (64, 65, 64), 200, 60.0, 0), ("c1", (192, 129, 192), 500, 10.0, 1)):
+        n1m, n2, n3m = grid
+        cfg = dict(n1=n1m + 1, n2=n2, n3=n3m + 1, re=4200.0, alx=2 * math.pi, aly=2.0, alz=math.pi, dt=1e-3,
+                   files=0.0, wmodel=0.0, cks=1.5)
+        pts = n1m * (n2 - 1) * n3m
+        rec = {"grid": "%dx%dx%d" % grid, "steps": steps}
+        probe = c3d.Channel(**cfg)
+        m = probe.mesh()
+        probe.close()
+        U, P = perturbed_poiseuille(cfg["n1"], cfg["n2"], cfg["n3"], m["Y"], m["DX1"], m["DX3"], m["ALY"], seed=seed)
+        for pname, prec in (("fp64", c3d.FP64), ("mixed", c3d.MIXED)):
+            ch = c3d.Channel(**cfg, precision=prec)
+            ch.upload(U, P)
+            for n in range(1, 4):
+                ch.step(n)
+            t0 = time.perf_counter()
+            dm = 0.0
+            for n in range(4, 4 + steps):
+                d, _ = ch.step(n)
+                dm = max(dm, d)
+            el = time.perf_counter() - t0
+            ch.close()
+            rec[pname] = dict(mpts_steps_s=round(pts * steps / el / 1e6, 1), ms_per_step=round(1000 * el / steps, 4),
+                              divmax_max=dm)
+        if not args.no_cpu:
+            st, el, thr, _, _ = oracle_run(cfg, seed, cpu_s, steps)
+            rec["cpu_oracle"] = dict(mpts_steps_s=round(pts * st / el / 1e6, 2), steps=st, seconds=round(el, 1),
+                                     cores=thr)
+        out[name] = rec
+    return out
+
+
+def operator_leg(c3d):
+    """isolated operator benches on a seeded uniform[-1,1] FP64 field of
+    192x129x192: the compact operators D2, D1 and I (both staggerings) along
+    x and z (one periodic tridiagonal solve per line, c3d_apply_op's
+    stand-alone kernel), the Poisson solve (RHSDP + GETDP) and DIVCHECK/CFL,
+    kernel times from HIP events; GB/s counts one read and one write of the
+    field per operator"""
+    import numpy as np
+    cfg = dict(n1=193, n2=129, n3=193, re=4200.0, alx=2 * math.pi, aly=2.0, alz=math.pi, dt=1e-3,
+               files=0.0, wmodel=0.0, cks=1.5)
+    n1, n2, n3 = cfg["n1"], cfg["n2"], cfg["n3"]
+    pts = (n1 - 1) * (n2 - 1) * (n3 - 1)
+    rng = np.random.default_rng(1)
+    f = np.zeros((n1, n2, n3), order="F")
+    f[: n1 - 1, : n2 - 1, : n3 - 1] = rng.uniform(-1, 1, (n1 - 1, n2 - 1, n3 - 1))
+    ch = c3d.Channel(**cfg)
+    U = np.zeros((n1 + 1, n2 + 1, n3 + 1, 3), order="F")
+    U[1:n1, 0:n2 + 1, 1:n3, :] = rng.uniform(-1, 1, (n1 - 1, n2 + 1, n3 - 1, 3))
+    ch.upload(U, f)
+    names = {0: "D2", 1: "D1 centre->face", 2: "D1 face->centre", 3: "I centre->face", 4: "I face->centre"}
+    out = {}
+    for direction, dname in ((1, "x"), (3, "z")):
+        for op in range(5):
+            ch.apply_op(op, direction, f)
+            ch.timing(True, 64)
+            for _ in range(5):
+                ch.apply_op(op, direction, f)
+            kt = ch.kernel_times()
+            ch.timing(False)
+            ms, cnt = kt["apply_op"]
+            us = 1000.0 * ms / cnt
+            out["%s %s" % (names[op], dname)] = dict(avg_us=round(us, 2), gbs=round(16.0 * pts / us / 1e3, 1))
+    ch.getuh()
+    ch.timing(True, 256)
+    for _ in range(5):
+        ch.dpcalc()
+        ch.divcheck()
+        ch.cfl()
+    kt = ch.kernel_times()
+    ch.timing(False)
+    ch.close()
+    for k, (ms, cnt) in kt.items():
+        out[k] = dict(avg_us=round(1000.0 * ms / cnt, 2), launches=cnt)
+    return dict(grid="192x129x192", field="uniform[-1,1] FP64, seed 1", kernels=out)
+
 
 
 def run_leg(c3d, cfg, U, P, precision, args, pts, table):
@@ -165,13 +368,13 @@ def served_leg(args):
     """The serving metric: a seeded Poisson stream of channel requests served
     by the continuous-batching loop (compact3d_amd.serve), and the same
     requests all present at t=0 (the batched throughput it is compared to).
-    Step counts are scaled down from 100-2000 so the leg takes seconds; the
-    offered load stays above what the GPU completes, so the loop is busy."""
+    100-2000 steps per request; the offered load stays above what the GPU
+    completes, so the latencies include queueing."""
     import copy
     from compact3d_amd import c3d
     from compact3d_amd.serve import FieldCache, Server, poisson_requests, stats
     grids = [(64, 49, 64), (96, 65, 96), (128, 65, 128), (192, 129, 192), (256, 129, 256)]
-    reqs = poisson_requests(args.serve_requests, 20.0, grids, [180, 395, 1000], (100, 400), seed=4)
+    reqs = poisson_requests(args.serve_requests, 20.0, grids, [180, 395, 1000], (100, 2000), seed=4)
     if not args.no_autotune:  # tile shapes of every configuration in the stream, before any timing
         seen = set()
         for r in reqs:
@@ -196,11 +399,12 @@ def served_leg(args):
         st = stats(done, wall)
         st["peak_resident_gb"] = srv.peak_bytes / 1e9
         st["divmax_max"] = max(r.result["divmax"] for r in done)
+        st["failed"] = len(srv.failed)
         out[name] = st
     out["served_over_batched"] = out["served"]["throughput_mpts_steps_s"] / out["batched"]["throughput_mpts_steps_s"]
     out["stream"] = ("%d requests, Poisson arrivals 20/s (seed 4), grids 64x49x64..256x129x256, Re_tau "
-                     "{180, 395; 1000 wall-modelled LES}, all mixed precision, 100-400 steps, dt 5e-4, "
-                     "%d slots, up to 10 steps per turn, captured graphs on each instance's own stream" % (
+                     "{180, 395; 1000 wall-modelled LES}, all mixed precision, 100-2000 steps, dt 5e-4, "
+                     "%d slots, chunks of up to 10 steps, captured graphs on each instance's own stream" % (
                          len(reqs), args.serve_slots))
     return out
 
@@ -367,6 +571,10 @@ def main():
     ap.add_argument("--shapes", metavar="JSON",
                     help="tile shapes of the main legs from a file ({'fp64': {...}, 'mixed': {...}}, the 'autotune' "
                          "field of an earlier run) instead of autotuning them; for profiling the same kernels")
+    ap.add_argument("--acc-steps", type=int, default=1000,
+                    help="steps of the mixed-vs-FP64 accuracy leg (0 skips it)")
+    ap.add_argument("--no-configs", action="store_true", help="skip the 64x65x64 and 192x129x192 legs")
+    ap.add_argument("--no-operators", action="store_true", help="skip the isolated operator benches")
     ap.add_argument("--ens-members", type=int, default=48)
     ap.add_argument("--ens-steps", type=int, default=200)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -457,10 +665,33 @@ def main():
 
     served = None if args.no_serve else served_leg(args)
     ensemble = None if args.no_ensemble else ensemble_leg(args)
+    accuracy = None
+    if args.acc_steps > 0:
+        U, P = perturbed_poiseuille(cfg["n1"], cfg["n2"], cfg["n3"], m["Y"], m["DX1"], m["DX3"], m["ALY"],
+                                    seed=args.seed)
+        accuracy = accuracy_leg(c3d, cfg, U, P, args.acc_steps)
+        del U, P
+    configs = None if args.no_configs else configs_leg(c3d, args)
+    operators = None if args.no_operators else operator_leg(c3d)
 
     cpu = None
+    vs_oracle = None
     if not args.no_cpu:
-        cpu = cpu_baseline(cfg, args.cpu_seconds, args.seed)
+        cpu, ksteps, Uo, Po = cpu_baseline(cfg, args.cpu_seconds, args.seed)
+        cpu["calibration"] = calibration()
+        # the GPU paths after the oracle's steps, from the same field
+        U, P = perturbed_poiseuille(cfg["n1"], cfg["n2"], cfg["n3"], m["Y"], m["DX1"], m["DX3"], m["ALY"],
+                                    seed=args.seed)
+        vs_oracle = {"steps": ksteps}
+        for pname, prec in (("fp64", c3d.FP64), ("mixed", c3d.MIXED)):
+            ch = c3d.Channel(**cfg, precision=prec)
+            ch.upload(U, P)
+            for n in range(1, ksteps + 1):
+                ch.step(n)
+            st = ch.download()
+            ch.close()
+            vs_oracle[pname] = field_errors(Uo, Po, st["U"], st["P"], n1, n2, n3)
+        del U, P, Uo, Po
 
     out = {
         "metric": METRIC,
@@ -489,6 +720,10 @@ def main():
         "mixed_divmax_max": rmx["divmax_max"],
         "mixed_refinements": refines,
         "mixed_vs_fp64_err": errs,
+        "vs_oracle": vs_oracle,
+        "accuracy": accuracy,
+        "configs": configs,
+        "operators": operators,
         "autotune": tune or None,
         "served": served,
         "ensemble": ensemble,
