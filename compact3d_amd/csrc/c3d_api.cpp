@@ -1458,6 +1458,21 @@ extern "C" void c3d_autotune_reset(void) {
 // member's stream, join back), so the device overlaps the members' kernels
 // and the host issues one launch for all of them.  As for c3d_advance, two
 // graphs cover the two buffer states a step alternates between.
+// Members of one shape (grid, precision, models, tile shapes, buffer
+// state) form a batch: each kernel of their step is launched once with the
+// members along grid z, every block reading its member's context from a
+// device array (CtxArg<true>).  Members of the models the batched launchers
+// cover (FILES 0 or 2 with WMODEL 0, 1 or 6) are batched; the others run
+// their own step as a branch of the graph, as before.
+struct EnsBatch {
+  std::vector<int> mem;              // member indices
+  hipStream_t st = nullptr;          // capture branch stream (batches only)
+  hipEvent_t join = nullptr;
+  Ctx* dsnap[2] = {nullptr, nullptr};  // device context snapshots per graph parity
+  std::vector<Ctx> hsnap[2];         // their host copies, written during capture
+};
+static const int kEnsSnapMax = 8;    // context snapshots per step (the step needs 3-4)
+
 struct c3d_ensemble {
   std::vector<c3d_handle*> m;
   hipStream_t st = nullptr;
@@ -1465,6 +1480,7 @@ struct c3d_ensemble {
   std::vector<hipEvent_t> join;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   std::vector<int> gkey[2];
+  std::vector<EnsBatch> batches;     // the grouping the graphs were captured with
   std::string err;
 };
 
@@ -1474,12 +1490,23 @@ static std::vector<int> ens_key(const c3d_ensemble* e) {
   return k;
 }
 
+static void ens_batches_free(c3d_ensemble* e) {
+  for (EnsBatch& b : e->batches) {
+    for (int p = 0; p < 2; p++)
+      if (b.dsnap[p]) (void)hipFree(b.dsnap[p]);
+    if (b.join) (void)hipEventDestroy(b.join);
+    if (b.st) (void)hipStreamDestroy(b.st);
+  }
+  e->batches.clear();
+}
+
 static void ens_graphs_free(c3d_ensemble* e) {
   for (int p = 0; p < 2; p++) {
     if (e->gexec[p]) (void)hipGraphExecDestroy(e->gexec[p]);
     e->gexec[p] = nullptr;
     e->gkey[p].clear();
   }
+  ens_batches_free(e);
 }
 
 #define EHIPCHK(x)                                                                        \
@@ -1491,8 +1518,115 @@ static void ens_graphs_free(c3d_ensemble* e) {
     }                                                                                     \
   } while (0)
 
+// the batching key of a member ("" = not batchable): everything that fixes
+// its kernel instances and launch geometry, and its buffer state
+static std::string batch_key(const c3d_handle* h) {
+  const c3d_config& f = h->cfg;
+  const bool ok = (f.files == 0.0 || f.files == 2.0) && (f.wmodel == 0.0 || f.wmodel == 1.0 || f.wmodel == 6.0);
+  if (!ok) return "";
+  const Shapes sh = get_shapes(h);
+  const Ops& o = h->c.o;
+  std::string k = std::to_string(f.n1) + "x" + std::to_string(f.n2) + "x" + std::to_string(f.n3) + "/" +
+                  std::to_string(f.precision) + "/" + std::to_string(f.files) + "/" + std::to_string(f.wmodel) +
+                  "/" + std::to_string(buf_key(h)) + "/";
+  k.append((const char*)&sh, sizeof(sh));
+  const TileShape ts[4] = {o.tx, o.tz, o.tx4, o.tz4};
+  k.append((const char*)ts, sizeof(ts));
+  return k;
+}
+
+// the members grouped into batches (members of one key) and single members
+static int ens_group(c3d_ensemble* e) {
+  ens_batches_free(e);
+  std::map<std::string, int> at;
+  for (size_t q = 0; q < e->m.size(); q++) {
+    const std::string k = batch_key(e->m[q]);
+    if (!k.empty() && at.count(k)) {
+      e->batches[at[k]].mem.push_back((int)q);
+      continue;
+    }
+    if (!k.empty()) at[k] = (int)e->batches.size();
+    e->batches.emplace_back();
+    e->batches.back().mem.push_back((int)q);
+  }
+  for (EnsBatch& b : e->batches) {
+    if (b.mem.size() < 2) continue;
+    EHIPCHK(hipStreamCreateWithFlags(&b.st, hipStreamNonBlocking));
+    EHIPCHK(hipEventCreateWithFlags(&b.join, hipEventDisableTiming));
+    for (int p = 0; p < 2; p++) {
+      EHIPCHK(hipMalloc((void**)&b.dsnap[p], sizeof(Ctx) * kEnsSnapMax * b.mem.size()));
+      b.hsnap[p].clear();
+    }
+  }
+  return 0;
+}
+
+// the context of batch b for a batched launch in graph parity p: the
+// members' current host contexts as a device snapshot (a new one when they
+// changed since the last), with member 0's shapes, the batch's stream and
+// member 0's timer
+static int batch_ctx(c3d_ensemble* e, EnsBatch& b, int p, Ctx& out) {
+  const size_t n = b.mem.size();
+  std::vector<Ctx>& hs = b.hsnap[p];
+  const size_t nsnap = hs.size() / n;
+  bool same = nsnap > 0;
+  for (size_t q = 0; q < n && same; q++)
+    same = std::memcmp(&hs[(nsnap - 1) * n + q], &e->m[b.mem[q]]->c, sizeof(Ctx)) == 0;
+  size_t at = nsnap - 1;
+  if (!same) {
+    if (nsnap >= (size_t)kEnsSnapMax) {
+      e->err = "ensemble capture: too many context snapshots";
+      return -1;
+    }
+    for (size_t q = 0; q < n; q++) hs.push_back(e->m[b.mem[q]]->c);
+    at = nsnap;
+  }
+  out = e->m[b.mem[0]]->c;
+  out.bdev = b.dsnap[p] + at * n;
+  out.bn = (int)n;
+  out.st = b.st;
+  return 0;
+}
+
+// one MAIN-loop step of every member of batch b (step_launch's sequence,
+// each kernel launched once for the batch)
+static int step_launch_batch(c3d_ensemble* e, EnsBatch& b, int p) {
+  c3d_handle* h0 = e->m[b.mem[0]];
+  const c3d_config& f = h0->cfg;
+  Ctx c;
+  if (f.wmodel == 0.0 && f.files != 0.0) {
+    if (batch_ctx(e, b, p, c)) return -1;
+    c3dk::wallss(c);
+  }
+  for (int q : b.mem) swap_hh(e->m[q]);
+  if (f.files == 2.0) {
+    if (batch_ctx(e, b, p, c)) return -1;
+    c3dk::sgscs(c);
+  }
+  if (f.files != 0.0) {
+    if (batch_ctx(e, b, p, c)) return -1;
+    if (f.wmodel == 0.0)
+      c3dk::wallss(c);
+    else
+      c3dk::wallmodel(c, f.wmodel == 6.0);
+    for (int q : b.mem) e->m[q]->c.s.ifwall = f.wmodel == 0.0 ? 0 : 1;
+  }
+  if (batch_ctx(e, b, p, c)) return -1;
+  c3dk::bcond(c);
+  c3dk::rhs_xz(c);
+  c3dk::getuh_fused(c);
+  c3dk::rhsdp(c);
+  c3dk::getdp(c);
+  if (h0->c.s.mixed)
+    for (int q : b.mem) swap_v(e->m[q]);
+  if (batch_ctx(e, b, p, c)) return -1;
+  c3dk::upcalc(c, true);
+  return 0;
+}
+
 static int ens_capture(c3d_ensemble* e) {
   ens_graphs_free(e);
+  if (ens_group(e)) return -1;
   std::vector<bool> timing;
   for (c3d_handle* h : e->m) {
     timing.push_back(h->kt.on);
@@ -1508,23 +1642,41 @@ static int ens_capture(c3d_ensemble* e) {
       break;
     }
     hipError_t r = hipEventRecord(e->fork, e->st);
-    for (size_t q = 0; q < e->m.size() && r == hipSuccess; q++) {
-      c3d_handle* h = e->m[q];
-      r = hipStreamWaitEvent(h->c.st, e->fork, 0);
-      if (r != hipSuccess) break;
-      step_launch(h);
-      r = hipEventRecord(e->join[q], h->c.st);
-      if (r == hipSuccess) r = hipStreamWaitEvent(e->st, e->join[q], 0);
+    for (size_t q = 0; q < e->batches.size() && r == hipSuccess && rc == 0; q++) {
+      EnsBatch& b = e->batches[q];
+      if (b.mem.size() < 2) {
+        c3d_handle* h = e->m[b.mem[0]];
+        r = hipStreamWaitEvent(h->c.st, e->fork, 0);
+        if (r != hipSuccess) break;
+        step_launch(h);
+        r = hipEventRecord(e->join[b.mem[0]], h->c.st);
+      } else {
+        r = hipStreamWaitEvent(b.st, e->fork, 0);
+        if (r != hipSuccess) break;
+        rc = step_launch_batch(e, b, p);
+        if (r == hipSuccess) r = hipGetLastError();
+        if (r == hipSuccess) r = hipEventRecord(b.join, b.st);
+      }
+      if (r == hipSuccess) r = hipStreamWaitEvent(e->st, b.mem.size() < 2 ? e->join[b.mem[0]] : b.join, 0);
     }
     const hipError_t r2 = hipStreamEndCapture(e->st, &g);
     if (r == hipSuccess) r = r2;
-    if (r == hipSuccess) r = hipGraphInstantiate(&e->gexec[p], g, nullptr, nullptr, 0);
+    if (r == hipSuccess && rc == 0) r = hipGraphInstantiate(&e->gexec[p], g, nullptr, nullptr, 0);
     if (g) (void)hipGraphDestroy(g);
     if (r != hipSuccess) {
       e->err = std::string("ensemble capture: ") + hipGetErrorString(r);
       rc = -1;
     }
   }
+  // the graphs read the context snapshots written during capture
+  for (EnsBatch& b : e->batches)
+    for (int p = 0; p < 2 && rc == 0; p++)
+      if (b.dsnap[p] && !b.hsnap[p].empty() &&
+          hipMemcpy(b.dsnap[p], b.hsnap[p].data(), sizeof(Ctx) * b.hsnap[p].size(), hipMemcpyHostToDevice) !=
+              hipSuccess) {
+        e->err = "ensemble context upload failed";
+        rc = -1;
+      }
   for (size_t q = 0; q < e->m.size(); q++) e->m[q]->kt.on = timing[q];
   if (rc) ens_graphs_free(e);
   return rc;

@@ -171,7 +171,55 @@ struct Ctx {
   Fields d;
   hipStream_t st;
   KTimer* kt;
+  // batched launches (ensembles of same-shape members): when bdev is set,
+  // the launchers run every kernel once for bn members, blockIdx.z selecting
+  // the member's Ctx in the device array bdev (this Ctx supplies the shapes,
+  // stream and timer)
+  const Ctx* bdev = nullptr;
+  int bn = 0;
 };
+
+// kernel argument carrying the context: by value (one channel) or as a
+// device array indexed by blockIdx.z (a batch of same-shape channels)
+template <bool B>
+struct CtxArg;
+template <>
+struct CtxArg<false> {
+  Ctx c;
+};
+template <>
+struct CtxArg<true> {
+  const Ctx* cs;
+};
+__device__ __forceinline__ const Ctx& ctx_get(const CtxArg<false>& a) { return a.c; }
+__device__ __forceinline__ const Ctx& ctx_get(const CtxArg<true>& a) { return a.cs[blockIdx.z]; }
+
+#define C3D_TARGS(...) __VA_ARGS__
+// launches kernel template K<TA..., B> for the context C (one channel, or
+// C.bn channels along grid z)
+#define C3D_LAUNCH(K, TA, GRID, BLOCK, LDS, C, ...)                                                      \
+  do {                                                                                                   \
+    if ((C).bdev) {                                                                                      \
+      dim3 g_ = (GRID);                                                                                  \
+      g_.z = (unsigned)(C).bn;                                                                           \
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(K<C3D_TARGS TA, true>), g_, BLOCK, LDS, (C).st,                 \
+                         CtxArg<true>{(C).bdev}, ##__VA_ARGS__);                                         \
+    } else {                                                                                             \
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(K<C3D_TARGS TA, false>), GRID, BLOCK, LDS, (C).st,              \
+                         CtxArg<false>{(C)}, ##__VA_ARGS__);                                             \
+    }                                                                                                    \
+  } while (0)
+// the same for kernels templated on the context argument only
+#define C3D_LAUNCH0(K, GRID, BLOCK, LDS, C, ...)                                                         \
+  do {                                                                                                   \
+    if ((C).bdev) {                                                                                      \
+      dim3 g_ = (GRID);                                                                                  \
+      g_.z = (unsigned)(C).bn;                                                                           \
+      hipLaunchKernelGGL(K<true>, g_, BLOCK, LDS, (C).st, CtxArg<true>{(C).bdev}, ##__VA_ARGS__);       \
+    } else {                                                                                             \
+      hipLaunchKernelGGL(K<false>, GRID, BLOCK, LDS, (C).st, CtxArg<false>{(C)}, ##__VA_ARGS__);         \
+    }                                                                                                    \
+  } while (0)
 
 // brackets a launch with timing events when the recorder is on
 struct KScope {

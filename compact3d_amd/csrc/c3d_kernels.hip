@@ -21,7 +21,9 @@
 #define RED_BLOCKS 1024
 
 // ------------------------------------------------------------------ BCOND
-__global__ void k_bcond(Ctx c) {
+template <bool CB_>
+__global__ void k_bcond(CtxArg<CB_> ca_) {
+  const Ctx c = ctx_get(ca_);
   const Geo g = c.g;
   const long long n = (long long)g.nx * g.nz;
   long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -173,7 +175,9 @@ __global__ __launch_bounds__(256) void k_plane_stats(Ctx c, double* out) {
 }
 
 // ghost planes after the update (:1130-1135, :1147-1152, :1210-1215)
-__global__ void k_ghosts(Ctx c) {
+template <bool CB_>
+__global__ void k_ghosts(CtxArg<CB_> ca_) {
+  const Ctx c = ctx_get(ca_);
   const Geo g = c.g;
   const long long n = (long long)g.nx * g.nz;
   long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -471,7 +475,7 @@ void zero(const Ctx& c, double* p, long long n) {
 }
 
 void bcond(const Ctx& c) {
-  { KScope ks_(c, KID_BCOND); hipLaunchKernelGGL(k_bcond, dim3(pgrid((long long)c.g.nx * c.g.nz)), dim3(C3D_NT), 0, c.st, c); }
+  { KScope ks_(c, KID_BCOND); C3D_LAUNCH0(k_bcond, dim3(pgrid((long long)c.g.nx * c.g.nz)), dim3(C3D_NT), 0, c); }
 }
 
 void rhs_xz(const Ctx& c) {
@@ -542,7 +546,7 @@ void cellcentre(const Ctx& c, int q, double* out) {
 
 void ghosts(const Ctx& c) {
   KScope ks_(c, KID_GHOSTS);
-  hipLaunchKernelGGL(k_ghosts, dim3(pgrid((long long)c.g.nx * c.g.nz)), dim3(C3D_NT), 0, c.st, c);
+  C3D_LAUNCH0(k_ghosts, dim3(pgrid((long long)c.g.nx * c.g.nz)), dim3(C3D_NT), 0, c);
 }
 
 void divcheck(const Ctx& c) {

@@ -6,6 +6,7 @@
 #include <algorithm>
 
 #include "c3d_impl.h"
+#include "c3d_lines.h"
 
 // ------------------------------------------------------------ STRAIN
 // STRAIN (compact3d.f:5558-5689) at one cell centre: the six components
@@ -94,7 +95,9 @@ __device__ __forceinline__ double sdots(const double s[6]) {
 // SGSCS (compact3d.f:1413-1488) with STRAIN: |S| = sqrt(2 S:S),
 // nu_t = max((0.1 (1-exp(-y+/26)))^2 (DY/(DX1 DX3))^(2/3) |S|, -1/RE),
 // y+ = (Y(J)+DY(J)/2) u_tau RE, u_tau = (WSM1^2 + WSM3^2)^(1/4) (previous step).
-__global__ void k_sgscs(Ctx c) {
+template <bool CB_>
+__global__ void k_sgscs(CtxArg<CB_> ca_) {
+  const Ctx c = ctx_get(ca_);
   const Geo g = c.g;
   const long long tot = (long long)g.nx * g.n2m * g.nz;
   long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -560,7 +563,9 @@ __global__ __launch_bounds__(256) void k_sgssd_coef(Ctx c, int ntile, const doub
 // at the top wall, on the third cell from it (J = N2M-2, wall distance
 // DY(N2M) + DY(N2M-1) + DY(N2M-2)/2), into TAUW(I,K,2), TAUB(I,K,2); its
 // plane sums are not kept (WSM(1) is the bottom wall's).
-__global__ __launch_bounds__(C3D_NT) void k_wallmodel(Ctx c, int top) {
+template <bool CB_>
+__global__ __launch_bounds__(C3D_NT) void k_wallmodel(CtxArg<CB_> ca_, int top) {
+  const Ctx c = ctx_get(ca_);
   __shared__ double red[C3D_NT];
   const Geo g = c.g;
   const Metrics M = c.m;
@@ -604,7 +609,9 @@ __global__ __launch_bounds__(C3D_NT) void k_wallmodel(Ctx c, int top) {
   if (t == 0) F.red[blockIdx.x * 4] = red[0];
 }
 
-__global__ __launch_bounds__(C3D_NT) void k_wallmodel_fin(Ctx c, int nblk, int zero23) {
+template <bool CB_>
+__global__ __launch_bounds__(C3D_NT) void k_wallmodel_fin(CtxArg<CB_> ca_, int nblk, int zero23) {
+  const Ctx c = ctx_get(ca_);
   __shared__ double red[C3D_NT];
   const Fields F = c.d;
   const int t = threadIdx.x;
@@ -1082,7 +1089,9 @@ __global__ __launch_bounds__(C3D_NT) void k_werner_avg(Ctx c) {
 
 // WALLSS (compact3d.f:11077-11143), including its use of WST(.,.,1) in WSM(2)
 // and WSM(3) (:11130-11132)
-__global__ __launch_bounds__(C3D_NT) void k_wallss(Ctx c) {
+template <bool CB_>
+__global__ __launch_bounds__(C3D_NT) void k_wallss(CtxArg<CB_> ca_) {
+  const Ctx c = ctx_get(ca_);
   __shared__ double red[3][C3D_NT];
   const Geo g = c.g;
   const Metrics M = c.m;
@@ -1121,7 +1130,9 @@ __global__ __launch_bounds__(C3D_NT) void k_wallss(Ctx c) {
   if (t < 3) F.red[blockIdx.x * 4 + t] = red[t][0];
 }
 
-__global__ __launch_bounds__(C3D_NT) void k_wallss_fin(Ctx c, int nblk) {
+template <bool CB_>
+__global__ __launch_bounds__(C3D_NT) void k_wallss_fin(CtxArg<CB_> ca_, int nblk) {
+  const Ctx c = ctx_get(ca_);
   __shared__ double red[3][C3D_NT];
   const Fields F = c.d;
   const int t = threadIdx.x;
@@ -1150,342 +1161,199 @@ __global__ __launch_bounds__(C3D_NT) void k_wallss_fin(Ctx c, int nblk) {
 //   dvdx = D1x(v), dwdx = D1x(w) (centre -> x-face)   UU12, VV12, WW13, X11
 //   dudz = D1z(u), dvdz = D1z(v) (centre -> z-face)   X33 of :6519-6524, UU31,
 //                                                     VV23, VV32
-#define SG_TILE_LDS 4
-template <typename TH>
-__global__ __launch_bounds__(C3D_NT) void k_sgs_x(Ctx c) {
-  extern __shared__ double sm[];
-  const Geo g = c.g;
-  const TileShape ts = c.o.tx;
-  const int n = ts.n, L = ts.nl, LS = ts.ls, nL = n * L;
-  double* Wt = sm;
-  double* Kt = sm + n * LS;
-  const int nlines = g.n2m * g.nz;
-  const int ln0 = blockIdx.x * L;
-  const Scal s = c.s;
+// All four kernels run on the register line engine (c3d_lines.h) in FP64,
+// whatever the storage type TH of HH (FP32 in mixed mode); each HH update
+// is rounded to TH after every term, as the reference's sequence of
+// separate updates does.
+
+// x-lines (j = 1..N2M): nux, dvdx, dwdx, and D(SGS*(DU/DX))/DX of RHS1
+// (:6445-6466): HH1 -= 2 D1x(SGSVIS * D1x(u))
+template <typename TH, int SEG, bool CB_>
+__global__ __launch_bounds__(256) void k_sgs_x(CtxArg<CB_> ca_) {
+  const Ctx c = ctx_get(ca_);
+  typedef double T;
+  XSETUP;
   const Fields F = c.d;
-#define SXDEC                             \
-  const int l = e / n, p = e % n;         \
-  const int ln = ln0 + l;                 \
-  const bool ok = ln < nlines;            \
-  const int j = 1 + (ok ? ln % g.n2m : 0); \
-  const long long b = gidx(g, 0, j, ok ? ln / g.n2m : 0); \
-  const int pm = wrapm(p, n), pp = wrapp(p, n);
-  // nux = Ix(SGSVIS)
-  EL_LOOP(nL) SXDEC;
-  (void)pp;
-  Wt[p * LS + l] = ok ? 4. * (F.sgs[b + p] + F.sgs[b + pm]) : 0.0;
-  EL_END;
-  __syncthreads();
-  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s6);
-  EL_LOOP(nL) SXDEC;
-  (void)pm; (void)pp; (void)j;
-  if (ok) F.nux[b + p] = Wt[p * LS + l];
-  EL_END;
-  __syncthreads();
-  // dvdx = D1x(v)
-  EL_LOOP(nL) SXDEC;
-  (void)pp; (void)j;
-  Wt[p * LS + l] = ok ? 24. * s.dx1 * (F.v[b + p] - F.v[b + pm]) : 0.0;
-  EL_END;
-  __syncthreads();
-  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
-  EL_LOOP(nL) SXDEC;
-  (void)pm; (void)pp; (void)j;
-  if (ok) F.dvdx[b + p] = Wt[p * LS + l];
-  EL_END;
-  __syncthreads();
-  // dwdx = D1x(w)
-  EL_LOOP(nL) SXDEC;
-  (void)pp; (void)j;
-  Wt[p * LS + l] = ok ? 24. * s.dx1 * (F.w[b + p] - F.w[b + pm]) : 0.0;
-  EL_END;
-  __syncthreads();
-  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
-  EL_LOOP(nL) SXDEC;
-  (void)pm; (void)pp; (void)j;
-  if (ok) F.dwdx[b + p] = Wt[p * LS + l];
-  EL_END;
-  __syncthreads();
-  // D(SGS*(DU/DX))/DX (:6445-6466): HH1 -= 2 D1x(SGSVIS * D1x(u))
-  EL_LOOP(nL) SXDEC;
-  (void)pm; (void)j;
-  Wt[p * LS + l] = ok ? 24. * s.dx1 * (F.u[b + pp] - F.u[b + p]) : 0.0;
-  EL_END;
-  __syncthreads();
-  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
-  EL_LOOP(nL) SXDEC;
-  (void)pp; (void)j;
-  Kt[p * LS + l] = ok ? F.sgs[b + p] * Wt[p * LS + l] : 0.0;
-  EL_END;
-  __syncthreads();
-  EL_LOOP(nL) SXDEC;
-  (void)pp; (void)j; (void)b;
-  Wt[p * LS + l] = ok ? 24. * s.dx1 * (Kt[p * LS + l] - Kt[pm * LS + l]) : 0.0;
-  EL_END;
-  __syncthreads();
-  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
-  EL_LOOP(nL) SXDEC;
-  (void)pm; (void)pp; (void)j;
-  if (ok) ((TH*)F.hh[0])[b + p] = (TH)((double)((TH*)F.hh[0])[b + p] - 2. * Wt[p * LS + l]);
-  EL_END;
-#undef SXDEC
+  const PSolve s6 = c.o.px.s6, s22 = c.o.px.s22;
+  const T dx1 = c.s.dx1;
+  T d[SEG], sv[SEG], pf[SEG];
+  X.pre(F.sgs, 0, pf);
+  const T* B = X.put(pf);  // SGSVIS
+  X.pre(F.v, 0, pf);
+  SQ sv[q] = XA(B, q);
+  SQ d[q] = 4. * (XA(B, q) + XA(B, q - 1));
+  psolve<T, SEG>(d, G, L, l, gs, s6, bf.xf, bf.xb);
+  X.out(F.nux, d);
+  B = X.put(pf);  // v
+  X.pre(F.w, 0, pf);
+  SQ d[q] = 24. * dx1 * (XA(B, q) - XA(B, q - 1));
+  psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
+  X.out(F.dvdx, d);
+  B = X.put(pf);  // w
+  X.pre(F.u, 0, pf);
+  SQ d[q] = 24. * dx1 * (XA(B, q) - XA(B, q - 1));
+  psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
+  X.out(F.dwdx, d);
+  B = X.put(pf);  // u
+  X.pre((const TH*)F.hh[0], 0, pf);
+  SQ d[q] = 24. * dx1 * (XA(B, q + 1) - XA(B, q));
+  psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
+  SQ d[q] = sv[q] * d[q];
+  const T lf = xch_left(d[SEG - 1], G, L, l, gs, bf.xl);
+  SQR d[q] = 24. * dx1 * (d[q] - prv(d, q, lf));
+  psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
+  B = X.put(pf);  // HH1
+  SQ d[q] = XA(B, q) - 2. * d[q];
+  X.out((TH*)F.hh[0], d);
 }
 
-template <typename TH>
-__global__ __launch_bounds__(C3D_NT) void k_sgs_z(Ctx c) {
-  extern __shared__ double sm[];
-  const Geo g = c.g;
-  const TileShape ts = c.o.tz4;
-  const int n = ts.n, L = ts.nl, LS = ts.ls, nL = n * L;
-  double* Wt = sm;
-  double* Et = sm + n * LS;      // nue
-  double* Dt = sm + 2 * n * LS;  // dudz
-  double* Kt = sm + 3 * n * LS;
-  const int nit = (g.nx + L - 1) / L;
-  const int it = blockIdx.x % nit;
-  const int j = 1 + blockIdx.x / nit;
-  const Scal s = c.s;
+// z-lines (j = 1..N2M): nuz, nue, dudz, dvdz, and
+//   D(SGS*(DW/DZ))/DZ of RHS3 (:7919-7940): HH3 -= 2 D1z(SGSVIS * D1z(w))
+//   D(SGS*(DU/DZ))/DZ of RHS1 (:6491-6540): HH1 -= D1z(dudz * nue)
+//   D(SGS*(DW/DX))/DZ of RHS1 (:6713-6764): HH1 -= D1z(dwdx * nue)
+template <typename TH, int SEG, bool CB_>
+__global__ __launch_bounds__(512) void k_sgs_z(CtxArg<CB_> ca_) {
+  const Ctx c = ctx_get(ca_);
+  typedef double T;
+  ZSETUP;
   const Fields F = c.d;
-  const long long pl = g.plane;
-#define SZDEC                               \
-  const int l = e % L, p = e / L;           \
-  const int i = it * L + l;                 \
-  const bool ok = i < g.nx;                 \
-  const long long b = gidx(g, ok ? i : 0, j, 0); \
-  const int pm = wrapm(p, n), pp = wrapp(p, n);
+  const PSolve s6 = c.o.pz.s6, s22 = c.o.pz.s22;
+  const T dx3 = c.s.dx3;
+  T a[SEG], d[SEG], sv[SEG], e[SEG], dz[SEG], lf, rt;
   // nuz = Iz(SGSVIS)
-  EL_LOOP(nL) SZDEC;
-  (void)pp;
-  Wt[p * LS + l] = ok ? 4. * (F.sgs[b + pm * pl] + F.sgs[b + p * pl]) : 0.0;
-  EL_END;
-  __syncthreads();
-  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s6);
-  EL_LOOP(nL) SZDEC;
-  (void)pm; (void)pp;
-  if (ok) F.nuz[b + p * pl] = Wt[p * LS + l];
-  EL_END;
-  __syncthreads();
+  zload(F.sgs, b, pl, p0, bm, bp, sv, lf, rt);
+  SQ d[q] = 4. * (prv(sv, q, lf) + sv[q]);
+  zload(F.nux, b, pl, p0, bm, bp, a, lf, rt);
+  psolve<T, SEG>(d, G, L, l, gs, s6, bf.xf, bf.xb);
+  if (ok) SQ F.nuz[ZI(q)] = d[q];
   // nue = Iz(nux)
-  EL_LOOP(nL) SZDEC;
-  (void)pp;
-  Et[p * LS + l] = ok ? 4. * (F.nux[b + p * pl] + F.nux[b + pm * pl]) : 0.0;
-  EL_END;
-  __syncthreads();
-  tile_solve(Et, n, L, LS, ts.g, ts.seg, c.o.s6);
-  EL_LOOP(nL) SZDEC;
-  (void)pm; (void)pp;
-  if (ok) F.nue[b + p * pl] = Et[p * LS + l];
-  EL_END;
+  SQ e[q] = 4. * (a[q] + prv(a, q, lf));
+  zload(F.u, b, pl, p0, bm, bp, a, lf, rt);
+  psolve<T, SEG>(e, G, L, l, gs, s6, bf.xf, bf.xb);
+  if (ok) SQ F.nue[ZI(q)] = e[q];
   // dudz = D1z(u)
-  EL_LOOP(nL) SZDEC;
-  (void)pp;
-  Dt[p * LS + l] = ok ? 24. * s.dx3 * (F.u[b + p * pl] - F.u[b + pm * pl]) : 0.0;
-  EL_END;
-  __syncthreads();
-  tile_solve(Dt, n, L, LS, ts.g, ts.seg, c.o.s22);
-  EL_LOOP(nL) SZDEC;
-  (void)pm; (void)pp;
-  if (ok) F.dudz[b + p * pl] = Dt[p * LS + l];
-  EL_END;
+  SQ dz[q] = 24. * dx3 * (a[q] - prv(a, q, lf));
+  zload(F.v, b, pl, p0, bm, bp, a, lf, rt);
+  psolve<T, SEG>(dz, G, L, l, gs, s22, bf.xf, bf.xb);
+  if (ok) SQ F.dudz[ZI(q)] = dz[q];
   // dvdz = D1z(v)
-  EL_LOOP(nL) SZDEC;
-  (void)pp;
-  Wt[p * LS + l] = ok ? 24. * s.dx3 * (F.v[b + p * pl] - F.v[b + pm * pl]) : 0.0;
-  EL_END;
-  __syncthreads();
-  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
-  EL_LOOP(nL) SZDEC;
-  (void)pm; (void)pp;
-  if (ok) F.dvdz[b + p * pl] = Wt[p * LS + l];
-  EL_END;
-  __syncthreads();
-  // D(SGS*(DW/DZ))/DZ (:7919-7940): HH3 -= 2 D1z(SGSVIS * D1z(w))
-  EL_LOOP(nL) SZDEC;
-  (void)pm;
-  Wt[p * LS + l] = ok ? 24. * s.dx3 * (F.w[b + pp * pl] - F.w[b + p * pl]) : 0.0;
-  EL_END;
-  __syncthreads();
-  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
-  EL_LOOP(nL) SZDEC;
-  (void)pm; (void)pp;
-  Kt[p * LS + l] = ok ? Wt[p * LS + l] * F.sgs[b + p * pl] : 0.0;
-  EL_END;
-  __syncthreads();
-  EL_LOOP(nL) SZDEC;
-  (void)pp; (void)b;
-  Wt[p * LS + l] = ok ? 24. * s.dx3 * (Kt[p * LS + l] - Kt[pm * LS + l]) : 0.0;
-  EL_END;
-  __syncthreads();
-  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
-  EL_LOOP(nL) SZDEC;
-  (void)pm; (void)pp;
-  if (ok) ((TH*)F.hh[2])[b + p * pl] = (TH)((double)((TH*)F.hh[2])[b + p * pl] - 2. * Wt[p * LS + l]);
-  EL_END;
-  __syncthreads();
-  // D(SGS*(DU/DZ))/DZ (:6491-6540): HH1 -= D1z(dudz * nue)
-  EL_LOOP(nL) SZDEC;
-  (void)pm; (void)b;
-  Wt[p * LS + l] = ok ? 24. * s.dx3 * (Dt[pp * LS + l] * Et[pp * LS + l] - Dt[p * LS + l] * Et[p * LS + l]) : 0.0;
-  EL_END;
-  __syncthreads();
-  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
-  EL_LOOP(nL) SZDEC;
-  (void)pm; (void)pp;
-  if (ok) ((TH*)F.hh[0])[b + p * pl] = (TH)((double)((TH*)F.hh[0])[b + p * pl] - Wt[p * LS + l]);
-  EL_END;
-  __syncthreads();
-  // D(SGS*(DW/DX))/DZ (:6713-6764): HH1 -= D1z(dwdx * nue)
-  EL_LOOP(nL) SZDEC;
-  (void)pm;
-  Wt[p * LS + l] = ok ? 24. * s.dx3 * (F.dwdx[b + pp * pl] * Et[pp * LS + l] - F.dwdx[b + p * pl] * Et[p * LS + l])
-                      : 0.0;
-  EL_END;
-  __syncthreads();
-  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
-  EL_LOOP(nL) SZDEC;
-  (void)pm; (void)pp;
-  if (ok) ((TH*)F.hh[0])[b + p * pl] = (TH)((double)((TH*)F.hh[0])[b + p * pl] - Wt[p * LS + l]);
-  EL_END;
-#undef SZDEC
+  SQ d[q] = 24. * dx3 * (a[q] - prv(a, q, lf));
+  zload(F.w, b, pl, p0, bm, bp, a, lf, rt);
+  psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
+  if (ok) SQ F.dvdz[ZI(q)] = d[q];
+  // HH3 -= 2 D1z(SGSVIS * D1z(w))
+  SQ d[q] = 24. * dx3 * (nxt(a, q, rt) - a[q]);
+  psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
+  SQ d[q] = d[q] * sv[q];
+  lf = xch_left(d[SEG - 1], G, L, l, gs, bf.xl);
+  SQR d[q] = 24. * dx3 * (d[q] - prv(d, q, lf));
+  SQ a[q] = (T)((const TH*)F.hh[2])[ZI(q)];
+  psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
+  if (ok) SQ((TH*)F.hh[2])[ZI(q)] = (TH)(a[q] - 2. * d[q]);
+  // HH1 -= D1z(dudz * nue)
+  SQ d[q] = dz[q] * e[q];
+  rt = xch_right(d[0], G, L, l, gs, bf.xr);
+  SQ d[q] = 24. * dx3 * (nxt(d, q, rt) - d[q]);
+  SQ a[q] = (T)((const TH*)F.hh[0])[ZI(q)];
+  psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
+  SQ a[q] = (T)(TH)(a[q] - d[q]);
+  // HH1 -= D1z(dwdx * nue)
+  SQ d[q] = F.dwdx[ZI(q)] * e[q];
+  rt = xch_right(d[0], G, L, l, gs, bf.xr);
+  SQ d[q] = 24. * dx3 * (nxt(d, q, rt) - d[q]);
+  psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
+  if (ok) SQ((TH*)F.hh[0])[ZI(q)] = (TH)(a[q] - d[q]);
 }
 
 // z-lines at J >= 2: the two RHS2 terms that need nuz at J-1 and J
 //   D(SGS*(DW/DY))/DZ (:7177-7226), D(SGS*(DV/DZ))/DZ (:7245-7304)
-template <typename TH>
-__global__ __launch_bounds__(C3D_NT) void k_sgs_z2(Ctx c) {
-  extern __shared__ double sm[];
-  const Geo g = c.g;
-  const TileShape ts = c.o.tz4;
-  const int n = ts.n, L = ts.nl, LS = ts.ls, nL = n * L;
-  double* Wt = sm;
-  double* Yt = sm + n * LS;  // nuz interpolated to the v face J
-  double* Kt = sm + 2 * n * LS;
-  const int nit = (g.nx + L - 1) / L;
-  const int it = blockIdx.x % nit;
-  const int j = 2 + blockIdx.x / nit;
-  const Scal s = c.s;
+template <typename TH, int SEG, bool CB_>
+__global__ __launch_bounds__(512) void k_sgs_z2(CtxArg<CB_> ca_) {
+  const Ctx c = ctx_get(ca_);
+  typedef double T;
+  ZSETUP_J0(2);
   const Metrics M = c.m;
   const Fields F = c.d;
-  const long long pl = g.plane;
-#define SZDEC                               \
-  const int l = e % L, p = e / L;           \
-  const int i = it * L + l;                 \
-  const bool ok = i < g.nx;                 \
-  const long long b = gidx(g, ok ? i : 0, j, 0); \
-  const int pp = wrapp(p, n);
-  const double hinv = 1. / M.h[j], dyj = M.dy[j] / 2., dyjm = M.dy[j - 1] / 2.;
-  EL_LOOP(nL) SZDEC;
-  (void)pp;
-  double yn = 0.0, gw = 0.0;
-  if (ok) {
-    yn = hinv * (dyj * F.nuz[b - g.nx + p * pl] + dyjm * F.nuz[b + p * pl]);
-    gw = hinv * (F.w[b + p * pl] - F.w[b - g.nx + p * pl]);
+  const PSolve s22 = c.o.pz.s22;
+  const T dx3 = c.s.dx3;
+  const T hinv = 1. / M.h[j], dyj = M.dy[j] / 2., dyjm = M.dy[j - 1] / 2.;
+  T d[SEG], yn[SEG], a[SEG], rt;
+  SQ {
+    yn[q] = hinv * (dyj * F.nuz[ZI(q) - g.nx] + dyjm * F.nuz[ZI(q)]);
+    d[q] = yn[q] * (hinv * (F.w[ZI(q)] - F.w[ZI(q) - g.nx]));
   }
-  Yt[p * LS + l] = yn;
-  Kt[p * LS + l] = yn * gw;
-  EL_END;
-  __syncthreads();
-  EL_LOOP(nL) SZDEC;
-  (void)b;
-  Wt[p * LS + l] = ok ? 24. * s.dx3 * (Kt[pp * LS + l] - Kt[p * LS + l]) : 0.0;
-  EL_END;
-  __syncthreads();
-  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
-  EL_LOOP(nL) SZDEC;
-  (void)pp;
-  if (ok) ((TH*)F.hh[1])[b + p * pl] = (TH)((double)((TH*)F.hh[1])[b + p * pl] - Wt[p * LS + l]);
-  EL_END;
-  __syncthreads();
-  EL_LOOP(nL) SZDEC;
-  Wt[p * LS + l] = ok ? 24. * s.dx3 * (Yt[pp * LS + l] * F.dvdz[b + pp * pl] - Yt[p * LS + l] * F.dvdz[b + p * pl])
-                      : 0.0;
-  EL_END;
-  __syncthreads();
-  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
-  EL_LOOP(nL) SZDEC;
-  (void)pp;
-  if (ok) ((TH*)F.hh[1])[b + p * pl] = (TH)((double)((TH*)F.hh[1])[b + p * pl] - Wt[p * LS + l]);
-  EL_END;
-#undef SZDEC
+  rt = xch_right(d[0], G, L, l, gs, bf.xr);
+  SQ d[q] = 24. * dx3 * (nxt(d, q, rt) - d[q]);
+  SQ a[q] = (T)((const TH*)F.hh[1])[ZI(q)];
+  psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
+  SQ a[q] = (T)(TH)(a[q] - d[q]);
+  SQ d[q] = yn[q] * F.dvdz[ZI(q)];
+  rt = xch_right(d[0], G, L, l, gs, bf.xr);
+  SQ d[q] = 24. * dx3 * (nxt(d, q, rt) - d[q]);
+  psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
+  if (ok) SQ((TH*)F.hh[1])[ZI(q)] = (TH)(a[q] - d[q]);
 }
 
 // x-lines: D(SGS*(DU/DY))/DX and D(SGS*(DV/DX))/DX of RHS2 (:7104-7158,
 // :7323-7384, J >= 2), D(SGS*(DW/DX))/DX and D(SGS*(DU/DZ))/DX of RHS3
 // (:7797-7847, :7972-8024)
-template <typename TH>
-__global__ __launch_bounds__(C3D_NT) void k_sgs_x2(Ctx c) {
-  extern __shared__ double sm[];
-  const Geo g = c.g;
-  const TileShape ts = c.o.tx4;
-  const int n = ts.n, L = ts.nl, LS = ts.ls, nL = n * L;
-  double* Wt = sm;
-  double* Yt = sm + n * LS;
-  double* Kt = sm + 2 * n * LS;
-  const int nlines = g.n2m * g.nz;
-  const int ln0 = blockIdx.x * L;
-  const Scal s = c.s;
+template <typename TH, int SEG, bool CB_>
+__global__ __launch_bounds__(256) void k_sgs_x2(CtxArg<CB_> ca_) {
+  const Ctx c = ctx_get(ca_);
+  typedef double T;
+  XSETUP;
   const Metrics M = c.m;
   const Fields F = c.d;
-#define SXDEC                              \
-  const int l = e / n, p = e % n;          \
-  const int ln = ln0 + l;                  \
-  const bool ok = ln < nlines;             \
-  const int j = 1 + (ok ? ln % g.n2m : 0); \
-  const long long b = gidx(g, 0, j, ok ? ln / g.n2m : 0); \
-  const int pp = wrapp(p, n);
-  EL_LOOP(nL) SXDEC;
-  (void)pp;
-  double yn = 0.0, gu = 0.0;
-  if (ok && j >= 2) {
-    const double hinv = 1. / M.h[j];
-    yn = hinv * (M.dy[j - 1] / 2. * F.nux[b + p] + M.dy[j] / 2. * F.nux[b - g.nx + p]);
-    gu = hinv * (F.u[b + p] - F.u[b - g.nx + p]);
-  }
-  Yt[p * LS + l] = yn;
-  Kt[p * LS + l] = yn * gu;
-  EL_END;
-  __syncthreads();
-  EL_LOOP(nL) SXDEC;
-  (void)b;
-  Wt[p * LS + l] = ok ? 24. * s.dx1 * (Kt[pp * LS + l] - Kt[p * LS + l]) : 0.0;
-  EL_END;
-  __syncthreads();
-  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
-  EL_LOOP(nL) SXDEC;
-  (void)pp;
-  if (ok && j >= 2) ((TH*)F.hh[1])[b + p] = (TH)((double)((TH*)F.hh[1])[b + p] - Wt[p * LS + l]);
-  EL_END;
-  __syncthreads();
-  EL_LOOP(nL) SXDEC;
-  Wt[p * LS + l] = ok ? 24. * s.dx1 * (Yt[pp * LS + l] * F.dvdx[b + pp] - Yt[p * LS + l] * F.dvdx[b + p]) : 0.0;
-  EL_END;
-  __syncthreads();
-  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
-  EL_LOOP(nL) SXDEC;
-  (void)pp;
-  if (ok && j >= 2) ((TH*)F.hh[1])[b + p] = (TH)((double)((TH*)F.hh[1])[b + p] - Wt[p * LS + l]);
-  EL_END;
-  __syncthreads();
-  EL_LOOP(nL) SXDEC;
-  Wt[p * LS + l] = ok ? 24. * s.dx1 * (F.nue[b + pp] * F.dwdx[b + pp] - F.nue[b + p] * F.dwdx[b + p]) : 0.0;
-  EL_END;
-  __syncthreads();
-  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
-  EL_LOOP(nL) SXDEC;
-  (void)pp;
-  if (ok) ((TH*)F.hh[2])[b + p] = (TH)((double)((TH*)F.hh[2])[b + p] - Wt[p * LS + l]);
-  EL_END;
-  __syncthreads();
-  EL_LOOP(nL) SXDEC;
-  Wt[p * LS + l] = ok ? 24. * s.dx1 * (F.nue[b + pp] * F.dudz[b + pp] - F.nue[b + p] * F.dudz[b + p]) : 0.0;
-  EL_END;
-  __syncthreads();
-  tile_solve(Wt, n, L, LS, ts.g, ts.seg, c.o.s22);
-  EL_LOOP(nL) SXDEC;
-  (void)pp;
-  if (ok) ((TH*)F.hh[2])[b + p] = (TH)((double)((TH*)F.hh[2])[b + p] - Wt[p * LS + l]);
-  EL_END;
-#undef SXDEC
+  const PSolve s22 = c.o.px.s22;
+  const T dx1 = c.s.dx1;
+  const bool j2 = ok && j >= 2;
+  const T hinv = 1. / M.h[j];
+  const T dyjm = M.dy[j - 1] / 2., dyj = M.dy[j] / 2.;
+  T d[SEG], yn[SEG], e[SEG], pf[SEG], rt;
+  X.pre(F.nux, 0, pf);
+  const T* B = X.put(pf);  // nux(J)
+  X.pre(F.nux, -1, pf);
+  SQ yn[q] = dyjm * XA(B, q);
+  B = X.put(pf);  // nux(J-1)
+  X.pre(F.u, 0, pf);
+  SQ yn[q] = j2 ? hinv * (yn[q] + dyj * XA(B, q)) : 0.;
+  B = X.put(pf);  // u(J)
+  X.pre(F.u, -1, pf);
+  SQ d[q] = XA(B, q);
+  B = X.put(pf);  // u(J-1)
+  X.pre(F.dvdx, 0, pf);
+  SQ d[q] = j2 ? yn[q] * (hinv * (d[q] - XA(B, q))) : 0.;
+  rt = xch_right(d[0], G, L, l, gs, bf.xr);
+  SQ d[q] = 24. * dx1 * (nxt(d, q, rt) - d[q]);
+  psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
+  B = X.put(pf);  // dvdx
+  X.pre((const TH*)F.hh[1], 0, pf);
+  SQ e[q] = yn[q] * XA(B, q);
+  rt = xch_right(e[0], G, L, l, gs, bf.xr);
+  SQ e[q] = 24. * dx1 * (nxt(e, q, rt) - e[q]);
+  psolve<T, SEG>(e, G, L, l, gs, s22, bf.xf, bf.xb);
+  B = X.put(pf);  // HH2
+  X.pre(F.nue, 0, pf);
+  SQ d[q] = (T)(TH)((T)(TH)(XA(B, q) - d[q]) - e[q]);
+  X.out((TH*)F.hh[1], d, 2);
+  B = X.put(pf);  // nue
+  X.pre(F.dwdx, 0, pf);
+  SQ yn[q] = XA(B, q);
+  B = X.put(pf);  // dwdx
+  X.pre(F.dudz, 0, pf);
+  SQ d[q] = yn[q] * XA(B, q);
+  rt = xch_right(d[0], G, L, l, gs, bf.xr);
+  SQ d[q] = 24. * dx1 * (nxt(d, q, rt) - d[q]);
+  psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
+  B = X.put(pf);  // dudz
+  X.pre((const TH*)F.hh[2], 0, pf);
+  SQ e[q] = yn[q] * XA(B, q);
+  rt = xch_right(e[0], G, L, l, gs, bf.xr);
+  SQ e[q] = 24. * dx1 * (nxt(e, q, rt) - e[q]);
+  psolve<T, SEG>(e, G, L, l, gs, s22, bf.xf, bf.xb);
+  B = X.put(pf);  // HH3
+  SQ d[q] = (T)(TH)((T)(TH)(XA(B, q) - d[q]) - e[q]);
+  X.out((TH*)F.hh[2], d);
 }
 
 // ------------------------------------------------------- test filters
@@ -1528,7 +1396,7 @@ static size_t lds(const TileShape& t, int ntiles) { return sizeof(double) * (siz
 
 void sgscs(const Ctx& c) {
   KScope ks_(c, KID_SGS);
-  hipLaunchKernelGGL(k_sgscs, dim3(pgrid((long long)c.g.nx * c.g.n2m * c.g.nz)), dim3(C3D_NT), 0, c.st, c);
+  C3D_LAUNCH0(k_sgscs, dim3(pgrid((long long)c.g.nx * c.g.n2m * c.g.nz)), dim3(C3D_NT), 0, c);
 }
 
 int sgsdy_tiles(const Geo& g) {
@@ -1563,11 +1431,11 @@ void wallmodel(const Ctx& c, bool both) {
   const int nb = std::min(1024, pgrid((long long)c.g.nx * c.g.nz));
   {
     KScope ks_(c, KID_WALL);
-    hipLaunchKernelGGL(k_wallmodel, dim3(nb), dim3(C3D_NT), 0, c.st, c, 0);
+    C3D_LAUNCH0(k_wallmodel, dim3(nb), dim3(C3D_NT), 0, c, 0);
   }
   KScope ks_(c, KID_WALL);
-  hipLaunchKernelGGL(k_wallmodel_fin, dim3(1), dim3(C3D_NT), 0, c.st, c, nb, 0);
-  if (both) hipLaunchKernelGGL(k_wallmodel, dim3(nb), dim3(C3D_NT), 0, c.st, c, 1);
+  C3D_LAUNCH0(k_wallmodel_fin, dim3(1), dim3(C3D_NT), 0, c, nb, 0);
+  if (both) C3D_LAUNCH0(k_wallmodel, dim3(nb), dim3(C3D_NT), 0, c, 1);
 }
 
 void wallmodel_iwmles(const Ctx& c, int ntime) {
@@ -1580,7 +1448,7 @@ void wallmodel_iwmles(const Ctx& c, int ntime) {
   hipLaunchKernelGGL(k_iwm_lhs, dim3(pgrid(n2d)), dim3(C3D_NT), 0, c.st, c);
   hipLaunchKernelGGL(k_iwm_solve, dim3(pgrid(n2d)), dim3(C3D_NT), 0, c.st, c);
   hipLaunchKernelGGL(k_iwm_apply, dim3(nb), dim3(C3D_NT), 0, c.st, c);
-  hipLaunchKernelGGL(k_wallmodel_fin, dim3(1), dim3(C3D_NT), 0, c.st, c, nb, 0);
+  C3D_LAUNCH0(k_wallmodel_fin, dim3(1), dim3(C3D_NT), 0, c, nb, 0);
 }
 
 void wallmodel_werner(const Ctx& c) {
@@ -1589,37 +1457,53 @@ void wallmodel_werner(const Ctx& c) {
   KScope ks_(c, KID_WALL);
   hipLaunchKernelGGL(k_werner, dim3(std::min(2048, pgrid(2 * n2d))), dim3(C3D_NT), 0, c.st, c);
   hipLaunchKernelGGL(k_werner_avg, dim3(nb), dim3(C3D_NT), 0, c.st, c);
-  hipLaunchKernelGGL(k_wallmodel_fin, dim3(1), dim3(C3D_NT), 0, c.st, c, nb, 1);
+  C3D_LAUNCH0(k_wallmodel_fin, dim3(1), dim3(C3D_NT), 0, c, nb, 1);
 }
 
 void wallss(const Ctx& c) {
   const int nb = std::min(1024, pgrid((long long)c.g.nx * c.g.nz));
   {
     KScope ks_(c, KID_WALLSS);
-    hipLaunchKernelGGL(k_wallss, dim3(nb), dim3(C3D_NT), 0, c.st, c);
+    C3D_LAUNCH0(k_wallss, dim3(nb), dim3(C3D_NT), 0, c);
   }
   KScope ks_(c, KID_WALLSS);
-  hipLaunchKernelGGL(k_wallss_fin, dim3(1), dim3(C3D_NT), 0, c.st, c, nb);
+  C3D_LAUNCH0(k_wallss_fin, dim3(1), dim3(C3D_NT), 0, c, nb);
 }
 
 // TH: storage type of HH (FP32 in mixed mode); the SGS terms are formed in
 // FP64 from the FP64 state and added to HH in FP64 arithmetic
 template <typename TH>
 static void sgs_terms_t(const Ctx& c) {
+  // the FP64 line shapes (x staged, z direct), in both precisions
+  const LShape &sx = c.o.lx, &sz = c.o.lz;
+  const size_t lx = sizeof(double) * (2 * (size_t)sx.n * (sx.L + 1) + 4 * (size_t)sx.G * sx.L);
+  const size_t lz = sizeof(double) * 4 * (size_t)sz.G * sz.L;
+  const int nxb = (c.g.n2m * c.g.nz + sx.L - 1) / sx.L;
+  const int nit = (c.g.nx + sz.L - 1) / sz.L;
+  const dim3 tx(sx.L * sx.G), tz(sz.L * sz.G);
+#define SEG_(seg, ...)         \
+  if ((seg) == 8) {            \
+    constexpr int S_ = 8;      \
+    __VA_ARGS__;               \
+  } else {                     \
+    constexpr int S_ = 2;      \
+    __VA_ARGS__;               \
+  }
   {
     KScope ks_(c, KID_SGS_X);
-    hipLaunchKernelGGL(k_sgs_x<TH>, dim3(xgrid(c, c.o.tx)), dim3(C3D_NT), lds(c.o.tx, 2), c.st, c);
+    SEG_(sx.seg, C3D_LAUNCH(k_sgs_x, (TH, S_), dim3(nxb), tx, lx, c));
   }
   {
     KScope ks_(c, KID_SGS_Z);
-    hipLaunchKernelGGL(k_sgs_z<TH>, dim3(zgrid(c, c.o.tz4, 1)), dim3(C3D_NT), lds(c.o.tz4, 4), c.st, c);
+    SEG_(sz.seg, C3D_LAUNCH(k_sgs_z, (TH, S_), dim3(nit * c.g.n2m), tz, lz, c));
   }
   {
     KScope ks_(c, KID_SGS_Z2);
-    hipLaunchKernelGGL(k_sgs_z2<TH>, dim3(zgrid(c, c.o.tz4, 2)), dim3(C3D_NT), lds(c.o.tz4, 3), c.st, c);
+    SEG_(sz.seg, C3D_LAUNCH(k_sgs_z2, (TH, S_), dim3(nit * (c.g.n2m - 1)), tz, lz, c));
   }
   KScope ks_(c, KID_SGS_X2);
-  hipLaunchKernelGGL(k_sgs_x2<TH>, dim3(xgrid(c, c.o.tx4)), dim3(C3D_NT), lds(c.o.tx4, 3), c.st, c);
+  SEG_(sx.seg, C3D_LAUNCH(k_sgs_x2, (TH, S_), dim3(nxb), tx, lx, c));
+#undef SEG_
 }
 void sgs_terms(const Ctx& c) {
   if (c.s.mixed)

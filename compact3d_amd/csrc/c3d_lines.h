@@ -1,0 +1,210 @@
+// c3d_lines.h -- the register line engine's tile layouts (c3d_dev.h has
+// the periodic solve psolve): x-line staging through LDS, z-line tiles read
+// directly from HBM, and the helpers the line kernels share.
+//
+// z-lines: a tile is L consecutive i at one j; thread (l, g) keeps points k
+// in [g*seg, (g+1)*seg) of column i = i0+l in registers and reads/writes HBM
+// directly (16 FP64 lanes = one 128-byte row piece).  x-lines: a tile is L
+// consecutive (j,k) rows; rows are staged through LDS with coalesced row
+// loads, then each thread takes its segment.
+#pragma once
+#include "c3d_impl.h"
+
+// ------------------------------------------------------------ helpers
+template <typename T, int SEG>
+__device__ __forceinline__ T prv(const T (&v)[SEG], const int q, const T left) {
+  return q == 0 ? left : v[q > 0 ? q - 1 : 0];
+}
+template <typename T, int SEG>
+__device__ __forceinline__ T nxt(const T (&v)[SEG], const int q, const T right) {
+  return (q + 1 < SEG) ? v[q + 1 < SEG ? q + 1 : SEG - 1] : right;
+}
+
+#define SQ _Pragma("unroll") for (int q = 0; q < SEG; q++)
+#define SQR _Pragma("unroll") for (int q = SEG - 1; q >= 0; q--)
+
+// LDS carve-up of a line kernel: two staging tiles (x-lines only) and four
+// exchange arrays of G*L entries
+template <typename T>
+struct LBuf {
+  T *st0, *st1, *xf, *xb, *xl, *xr;
+};
+template <typename T>
+__device__ __forceinline__ LBuf<T> lbuf(unsigned char* sm, const LShape& sh, bool staged) {
+  LBuf<T> b;
+  const int tile = staged ? sh.n * (sh.L + 1) : 0;
+  b.st0 = (T*)sm;
+  b.st1 = b.st0 + tile;
+  b.xf = b.st1 + tile;
+  b.xb = b.xf + sh.G * sh.L;
+  b.xl = b.xb + sh.G * sh.L;
+  b.xr = b.xl + sh.G * sh.L;
+  return b;
+}
+
+// x-line staging.  Lines ln = ln0 + l (l < L) are rows (j, k) with
+// j = 1 + ln % N2M, k = ln / N2M.  A tile holds n*L = SEG*blockDim values, so
+// each thread moves exactly SEG of them: element e = t + m*blockDim is row
+// e / n, point e % n.  The host makes blockDim a multiple of n (L a multiple
+// of SEG), so a thread's elements share the point p = t % n and lie in rows
+// t/n + m*(blockDim/n): their LDS indices are li0 + m*lstep.  pre() issues
+// the global loads of a tile into registers (so they can be in flight while
+// the previous tile is used); put() writes them to LDS.  Two LDS tiles
+// alternate and every put()/out() starts with a barrier, so a tile stays
+// readable until the second staging operation after the one that filled it
+// begins.  Offsets are 32-bit (fields hold fewer than 2^31 elements).
+template <typename T, int SEG>
+struct XStage {
+  T *st0, *st1;
+  int sb;
+  int nx, LS;
+  int l, p0, pm, pp;   // this thread's line, segment start, and periodic neighbours
+  unsigned go[SEG];    // element offset of element m (row j)
+  unsigned vm;         // bit m set: element m exists (its line is in the grid)
+  int li0, lstep;      // LDS index of element m: li0 + m*lstep
+  unsigned j1;         // bit m set: element m lies in row j = 1
+  __device__ __forceinline__ T* next() {
+    sb ^= 1;
+    return sb ? st1 : st0;
+  }
+  template <typename S>
+  __device__ __forceinline__ void pre(const S* F, int joff, T (&r)[SEG]) const {
+    const unsigned off = (unsigned)(joff * nx);  // row j+joff exists for every j (ghost planes)
+    SQ r[q] = ((vm >> q) & 1u) ? (T)F[go[q] + off] : (T)0;
+  }
+  __device__ __forceinline__ const T* put(const T (&r)[SEG]) {
+    T* B = next();
+    __syncthreads();
+    SQ B[li0 + q * lstep] = r[q];
+    __syncthreads();
+    return B;
+  }
+  template <typename S>
+  __device__ __forceinline__ const T* in(const S* F, int joff) {
+    T r[SEG];
+    pre(F, joff, r);
+    return put(r);
+  }
+  // element at segment offset q in [-1, SEG] of this thread's line
+  __device__ __forceinline__ T at(const T* B, int q) const {
+    const int p = q < 0 ? pm : (q >= SEG ? pp : p0 + q);
+    return B[p * LS + l];
+  }
+  __device__ __forceinline__ bool keep(int q, int jmin) const {
+    return ((vm >> q) & 1u) && (jmin <= 1 || !((j1 >> q) & 1u));
+  }
+  // the same tile to two arrays (a field and its FP32 shadow)
+  template <typename S1, typename S2>
+  __device__ __forceinline__ void out2(S1* F1, S2* F2, const T (&v)[SEG], int jmin = 1) {
+    T* B = next();
+    __syncthreads();
+    SQ B[(p0 + q) * LS + l] = v[q];
+    __syncthreads();
+    SQ if (keep(q, jmin)) {
+      const T x = B[li0 + q * lstep];
+      F1[go[q]] = (S1)x;
+      F2[go[q]] = (S2)x;
+    }
+  }
+  // rows with j < jmin are left untouched
+  template <typename S>
+  __device__ __forceinline__ void out(S* F, const T (&v)[SEG], int jmin = 1) {
+    T* B = next();
+    __syncthreads();
+    SQ B[(p0 + q) * LS + l] = v[q];
+    __syncthreads();
+    SQ if (keep(q, jmin)) F[go[q]] = (S)B[li0 + q * lstep];
+  }
+};
+
+// Tile order across the 8 XCDs: the dispatcher hands workgroup b to XCD
+// b % 8, and each XCD has its own L2.  Tile t = xcd_tile(b, nb) gives XCD x
+// the contiguous tile range [x*per + min(x, rem), ...), so the z tiles that
+// share neighbour planes (j-1 and j+1, nit tiles apart) run on one L2.  A
+// bijection of [0, nb) for any nb.  (x tiles hold 8 consecutive j of one k
+// and keep the plain order, which measured faster for them.)
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  const int per = nb >> 3, rem = nb & 7, x = b & 7, r = b >> 3;
+  return x * per + min(x, rem) + r;
+}
+
+template <typename T, int SEG>
+__device__ __forceinline__ void xstage(XStage<T, SEG>& x, const LBuf<T>& b, const Geo& g, const LShape& sh,
+                                       int l, int p0) {
+  x.st0 = b.st0;
+  x.st1 = b.st1;
+  x.sb = 1;
+  x.nx = g.nx;
+  x.LS = sh.L + 1;
+  x.l = l;
+  x.p0 = p0;
+  x.pm = p0 == 0 ? sh.n - 1 : p0 - 1;
+  x.pp = p0 + SEG == sh.n ? 0 : p0 + SEG;
+  const int ln0 = blockIdx.x * sh.L;
+  const int nlines = g.n2m * g.nz;
+  const int nt = blockDim.x;
+  const int t = threadIdx.x;
+  const int p = t % sh.n, ll0 = t / sh.n;
+  x.lstep = nt / sh.n;
+  x.li0 = p * x.LS + ll0;
+  x.j1 = 0u;
+  x.vm = 0u;
+  SQ {
+    const int ll = ll0 + q * x.lstep;
+    const int ln = ln0 + ll;
+    const int jj = 1 + ln % g.n2m;
+    const bool v = ll < sh.L && ln < nlines;
+    x.go[q] = v ? (unsigned)gidx(g, p, jj, ln / g.n2m) : 0u;
+    if (v) x.vm |= 1u << q;
+    if (jj == 1) x.j1 |= 1u << q;
+  }
+}
+
+#define XSETUP                                                        \
+  extern __shared__ __align__(16) unsigned char smem[];               \
+  const Geo g = c.g;                                                  \
+  const LShape sh = c.o.lx;                                           \
+  const int L = sh.L, G = sh.G;                                       \
+  const LBuf<T> bf = lbuf<T>(smem, sh, true);                         \
+  const int l = threadIdx.x % L, gs = threadIdx.x / L, p0 = gs * SEG; \
+  XStage<T, SEG> X;                                                   \
+  xstage(X, bf, g, sh, l, p0);                                        \
+  const int ln = blockIdx.x * L + l;                                  \
+  const bool ok = ln < g.n2m * g.nz;                                  \
+  const int j = 1 + (ok ? ln % g.n2m : 0);                            \
+  (void)j;
+#define XA(B, q) X.at(B, q)
+
+// z-lines: a tile is L consecutive i at one j (up to 512 threads); column
+// i = i0 + l, k = p
+#define ZSETUP ZSETUP_J0(1)
+// the same for tiles whose first plane is J0 (tile / nit = J - J0)
+#define ZSETUP_J0(J0)                                                 \
+  extern __shared__ __align__(16) unsigned char smem[];               \
+  const Geo g = c.g;                                                  \
+  const LShape sh = sizeof(T) == 4 ? c.o.lz32 : c.o.lz;             \
+  const int n = sh.n, L = sh.L, G = sh.G;                             \
+  const LBuf<T> bf = lbuf<T>(smem, sh, false);                        \
+  const int l = threadIdx.x % L, gs = threadIdx.x / L, p0 = gs * SEG; \
+  const int nit = (g.nx + L - 1) / L;                                 \
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);                   \
+  const int j = (J0) + tile / nit;                                    \
+  const int i = (tile % nit) * L + l;                                 \
+  const bool ok = i < g.nx;                                           \
+  const unsigned pl = (unsigned)g.plane;                              \
+  const unsigned b = (unsigned)gidx(g, ok ? i : 0, j, 0);             \
+  const unsigned bm = b + (unsigned)(p0 == 0 ? n - 1 : p0 - 1) * pl;  \
+  const unsigned bp = b + (unsigned)(p0 + SEG == n ? 0 : p0 + SEG) * pl;
+// element offsets are 32-bit unsigned (fields hold fewer than 2^31
+// elements), which lets loads use a uniform base plus a 32-bit lane offset
+#define ZI(q) (b + (unsigned)(p0 + (q)) * pl)
+#define PZ (sizeof(T) == 4 ? c.o.pz32 : c.o.pz)
+
+template <typename T, int SEG, typename S>
+__device__ __forceinline__ void zload(const S* F, unsigned b, unsigned pl, int p0, unsigned bm, unsigned bp,
+                                      T (&v)[SEG], T& left, T& right) {
+  SQ v[q] = (T)F[b + (unsigned)(p0 + q) * pl];
+  left = (T)F[bm];
+  right = (T)F[bp];
+}
+

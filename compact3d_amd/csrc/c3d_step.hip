@@ -24,203 +24,8 @@
 #include <math.h>
 
 #include "c3d_impl.h"
+#include "c3d_lines.h"
 #include "c3d_yterms.h"
-
-// ------------------------------------------------------------ helpers
-template <typename T, int SEG>
-__device__ __forceinline__ T prv(const T (&v)[SEG], const int q, const T left) {
-  return q == 0 ? left : v[q > 0 ? q - 1 : 0];
-}
-template <typename T, int SEG>
-__device__ __forceinline__ T nxt(const T (&v)[SEG], const int q, const T right) {
-  return (q + 1 < SEG) ? v[q + 1 < SEG ? q + 1 : SEG - 1] : right;
-}
-
-#define SQ _Pragma("unroll") for (int q = 0; q < SEG; q++)
-#define SQR _Pragma("unroll") for (int q = SEG - 1; q >= 0; q--)
-
-// LDS carve-up of a line kernel: two staging tiles (x-lines only) and four
-// exchange arrays of G*L entries
-template <typename T>
-struct LBuf {
-  T *st0, *st1, *xf, *xb, *xl, *xr;
-};
-template <typename T>
-__device__ __forceinline__ LBuf<T> lbuf(unsigned char* sm, const LShape& sh, bool staged) {
-  LBuf<T> b;
-  const int tile = staged ? sh.n * (sh.L + 1) : 0;
-  b.st0 = (T*)sm;
-  b.st1 = b.st0 + tile;
-  b.xf = b.st1 + tile;
-  b.xb = b.xf + sh.G * sh.L;
-  b.xl = b.xb + sh.G * sh.L;
-  b.xr = b.xl + sh.G * sh.L;
-  return b;
-}
-
-// x-line staging.  Lines ln = ln0 + l (l < L) are rows (j, k) with
-// j = 1 + ln % N2M, k = ln / N2M.  A tile holds n*L = SEG*blockDim values, so
-// each thread moves exactly SEG of them: element e = t + m*blockDim is row
-// e / n, point e % n.  The host makes blockDim a multiple of n (L a multiple
-// of SEG), so a thread's elements share the point p = t % n and lie in rows
-// t/n + m*(blockDim/n): their LDS indices are li0 + m*lstep.  pre() issues
-// the global loads of a tile into registers (so they can be in flight while
-// the previous tile is used); put() writes them to LDS.  Two LDS tiles
-// alternate and every put()/out() starts with a barrier, so a tile stays
-// readable until the second staging operation after the one that filled it
-// begins.  Offsets are 32-bit (fields hold fewer than 2^31 elements).
-template <typename T, int SEG>
-struct XStage {
-  T *st0, *st1;
-  int sb;
-  int nx, LS;
-  int l, p0, pm, pp;   // this thread's line, segment start, and periodic neighbours
-  unsigned go[SEG];    // element offset of element m (row j)
-  unsigned vm;         // bit m set: element m exists (its line is in the grid)
-  int li0, lstep;      // LDS index of element m: li0 + m*lstep
-  unsigned j1;         // bit m set: element m lies in row j = 1
-  __device__ __forceinline__ T* next() {
-    sb ^= 1;
-    return sb ? st1 : st0;
-  }
-  template <typename S>
-  __device__ __forceinline__ void pre(const S* F, int joff, T (&r)[SEG]) const {
-    const unsigned off = (unsigned)(joff * nx);  // row j+joff exists for every j (ghost planes)
-    SQ r[q] = ((vm >> q) & 1u) ? (T)F[go[q] + off] : (T)0;
-  }
-  __device__ __forceinline__ const T* put(const T (&r)[SEG]) {
-    T* B = next();
-    __syncthreads();
-    SQ B[li0 + q * lstep] = r[q];
-    __syncthreads();
-    return B;
-  }
-  template <typename S>
-  __device__ __forceinline__ const T* in(const S* F, int joff) {
-    T r[SEG];
-    pre(F, joff, r);
-    return put(r);
-  }
-  // element at segment offset q in [-1, SEG] of this thread's line
-  __device__ __forceinline__ T at(const T* B, int q) const {
-    const int p = q < 0 ? pm : (q >= SEG ? pp : p0 + q);
-    return B[p * LS + l];
-  }
-  __device__ __forceinline__ bool keep(int q, int jmin) const {
-    return ((vm >> q) & 1u) && (jmin <= 1 || !((j1 >> q) & 1u));
-  }
-  // the same tile to two arrays (a field and its FP32 shadow)
-  template <typename S1, typename S2>
-  __device__ __forceinline__ void out2(S1* F1, S2* F2, const T (&v)[SEG], int jmin = 1) {
-    T* B = next();
-    __syncthreads();
-    SQ B[(p0 + q) * LS + l] = v[q];
-    __syncthreads();
-    SQ if (keep(q, jmin)) {
-      const T x = B[li0 + q * lstep];
-      F1[go[q]] = (S1)x;
-      F2[go[q]] = (S2)x;
-    }
-  }
-  // rows with j < jmin are left untouched
-  template <typename S>
-  __device__ __forceinline__ void out(S* F, const T (&v)[SEG], int jmin = 1) {
-    T* B = next();
-    __syncthreads();
-    SQ B[(p0 + q) * LS + l] = v[q];
-    __syncthreads();
-    SQ if (keep(q, jmin)) F[go[q]] = (S)B[li0 + q * lstep];
-  }
-};
-
-// Tile order across the 8 XCDs: the dispatcher hands workgroup b to XCD
-// b % 8, and each XCD has its own L2.  Tile t = xcd_tile(b, nb) gives XCD x
-// the contiguous tile range [x*per + min(x, rem), ...), so the z tiles that
-// share neighbour planes (j-1 and j+1, nit tiles apart) run on one L2.  A
-// bijection of [0, nb) for any nb.  (x tiles hold 8 consecutive j of one k
-// and keep the plain order, which measured faster for them.)
-__device__ __forceinline__ int xcd_tile(int b, int nb) {
-  const int per = nb >> 3, rem = nb & 7, x = b & 7, r = b >> 3;
-  return x * per + min(x, rem) + r;
-}
-
-template <typename T, int SEG>
-__device__ __forceinline__ void xstage(XStage<T, SEG>& x, const LBuf<T>& b, const Geo& g, const LShape& sh,
-                                       int l, int p0) {
-  x.st0 = b.st0;
-  x.st1 = b.st1;
-  x.sb = 1;
-  x.nx = g.nx;
-  x.LS = sh.L + 1;
-  x.l = l;
-  x.p0 = p0;
-  x.pm = p0 == 0 ? sh.n - 1 : p0 - 1;
-  x.pp = p0 + SEG == sh.n ? 0 : p0 + SEG;
-  const int ln0 = blockIdx.x * sh.L;
-  const int nlines = g.n2m * g.nz;
-  const int nt = blockDim.x;
-  const int t = threadIdx.x;
-  const int p = t % sh.n, ll0 = t / sh.n;
-  x.lstep = nt / sh.n;
-  x.li0 = p * x.LS + ll0;
-  x.j1 = 0u;
-  x.vm = 0u;
-  SQ {
-    const int ll = ll0 + q * x.lstep;
-    const int ln = ln0 + ll;
-    const int jj = 1 + ln % g.n2m;
-    const bool v = ll < sh.L && ln < nlines;
-    x.go[q] = v ? (unsigned)gidx(g, p, jj, ln / g.n2m) : 0u;
-    if (v) x.vm |= 1u << q;
-    if (jj == 1) x.j1 |= 1u << q;
-  }
-}
-
-#define XSETUP                                                        \
-  extern __shared__ __align__(16) unsigned char smem[];               \
-  const Geo g = c.g;                                                  \
-  const LShape sh = c.o.lx;                                           \
-  const int L = sh.L, G = sh.G;                                       \
-  const LBuf<T> bf = lbuf<T>(smem, sh, true);                         \
-  const int l = threadIdx.x % L, gs = threadIdx.x / L, p0 = gs * SEG; \
-  XStage<T, SEG> X;                                                   \
-  xstage(X, bf, g, sh, l, p0);                                        \
-  const int ln = blockIdx.x * L + l;                                  \
-  const bool ok = ln < g.n2m * g.nz;                                  \
-  const int j = 1 + (ok ? ln % g.n2m : 0);                            \
-  (void)j;
-#define XA(B, q) X.at(B, q)
-
-// z-lines: a tile is L consecutive i at one j (up to 512 threads); column
-// i = i0 + l, k = p
-#define ZSETUP                                                        \
-  extern __shared__ __align__(16) unsigned char smem[];               \
-  const Geo g = c.g;                                                  \
-  const LShape sh = sizeof(T) == 4 ? c.o.lz32 : c.o.lz;             \
-  const int n = sh.n, L = sh.L, G = sh.G;                             \
-  const LBuf<T> bf = lbuf<T>(smem, sh, false);                        \
-  const int l = threadIdx.x % L, gs = threadIdx.x / L, p0 = gs * SEG; \
-  const int nit = (g.nx + L - 1) / L;                                 \
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);                   \
-  const int j = 1 + tile / nit;                                       \
-  const int i = (tile % nit) * L + l;                                 \
-  const bool ok = i < g.nx;                                           \
-  const unsigned pl = (unsigned)g.plane;                              \
-  const unsigned b = (unsigned)gidx(g, ok ? i : 0, j, 0);             \
-  const unsigned bm = b + (unsigned)(p0 == 0 ? n - 1 : p0 - 1) * pl;  \
-  const unsigned bp = b + (unsigned)(p0 + SEG == n ? 0 : p0 + SEG) * pl;
-// element offsets are 32-bit unsigned (fields hold fewer than 2^31
-// elements), which lets loads use a uniform base plus a 32-bit lane offset
-#define ZI(q) (b + (unsigned)(p0 + (q)) * pl)
-#define PZ (sizeof(T) == 4 ? c.o.pz32 : c.o.pz)
-
-template <typename T, int SEG, typename S>
-__device__ __forceinline__ void zload(const S* F, unsigned b, unsigned pl, int p0, unsigned bm, unsigned bp,
-                                      T (&v)[SEG], T& left, T& right) {
-  SQ v[q] = (T)F[b + (unsigned)(p0 + q) * pl];
-  left = (T)F[bm];
-  right = (T)F[bp];
-}
 
 // ------------------------------------------------------------ RHS x-pass
 // per x-line (j,k), j = 1..N2M:
@@ -231,8 +36,9 @@ __device__ __forceinline__ void zload(const S* F, unsigned b, unsigned pl, int p
 //   HH1   = D1x((Ix u)^2)                                  (:6295-6334)
 //   RUH3  = (1/RE) D2x w                                   (:7483-7498)
 //   WW1   = Ix w                 (stored)                  (:6380-6394, :7718-7732)
-template <typename T, int SEG>
-__global__ __launch_bounds__(256) void k_rhs_x(Ctx c) {
+template <typename T, int SEG, bool CB_>
+__global__ __launch_bounds__(256) void k_rhs_x(CtxArg<CB_> ca_) {
+  const Ctx c = ctx_get(ca_);
   XSETUP;
   const Scal s = c.s;
   const Metrics M = c.m;
@@ -300,8 +106,9 @@ __global__ __launch_bounds__(256) void k_rhs_x(Ctx c) {
 //   UW    = Iz(u) * WW1 (stored);  HH1 += D1z(UW)           (:6360-6433)
 //   RUH2 += (1/RE) D2z v                          (j>=2)   (:6907-6923)
 //   VV3   = Iz v (stored);  HH2 += D1z(VV3*WW2)   (j>=2)   (:7023-7084)
-template <typename T, int SEG>
-__global__ __launch_bounds__(512) void k_rhs_z(Ctx c) {
+template <typename T, int SEG, bool CB_>
+__global__ __launch_bounds__(512) void k_rhs_z(CtxArg<CB_> ca_) {
+  const Ctx c = ctx_get(ca_);
   ZSETUP;
   const Scal s = c.s;
   const Metrics M = c.m;
@@ -358,8 +165,9 @@ __global__ __launch_bounds__(512) void k_rhs_z(Ctx c) {
 
 // z-lines, second kernel: RUH3 += (1/RE) D2z w - D1z(P) - PRESG3
 // (:7510-7550) and HH3 = D1z((Iz w)^2) (:7633-7672)
-template <typename T, int SEG>
-__global__ __launch_bounds__(512) void k_rhs_zw(Ctx c) {
+template <typename T, int SEG, bool CB_>
+__global__ __launch_bounds__(512) void k_rhs_zw(CtxArg<CB_> ca_) {
+  const Ctx c = ctx_get(ca_);
   ZSETUP;
   const Scal s = c.s;
   const Fields F = c.d;
@@ -394,8 +202,9 @@ __global__ __launch_bounds__(512) void k_rhs_zw(Ctx c) {
 
 // -------------------------------------------------------- RHS x-pass 2
 // HH3 += D1x(UU3*WW1) face -> centre (:7734-7769)
-template <typename T, int SEG>
-__global__ __launch_bounds__(256) void k_rhs_x2(Ctx c) {
+template <typename T, int SEG, bool CB_>
+__global__ __launch_bounds__(256) void k_rhs_x2(CtxArg<CB_> ca_) {
+  const Ctx c = ctx_get(ca_);
   XSETUP;
   const T dx1 = (T)c.s.dx1;
   T d[SEG], pf[SEG];
@@ -676,8 +485,9 @@ __device__ __forceinline__ void yfuse_w(const Ctx& c, const T* tb, unsigned b, u
 }
 
 // NT: the block size bound, 512 or 1024 (tiles of L*C threads)
-template <typename T, bool SGS, int CH, int NT>
-__global__ __launch_bounds__(NT) void k_rhsy_adiy(Ctx c, int staged) {
+template <typename T, bool SGS, int CH, int NT, bool CB_>
+__global__ __launch_bounds__(NT) void k_rhsy_adiy(CtxArg<CB_> ca_, int staged) {
+  const Ctx c = ctx_get(ca_);
   __shared__ T xs[4][NT];  // C*L <= NT entries each
   extern __shared__ __align__(16) unsigned char smem[];
   const Geo g = c.g;
@@ -754,8 +564,9 @@ struct CompList {
   int fin[3];
 };
 
-template <typename T, int SEG>
-__global__ __launch_bounds__(256) void k_adi_x(Ctx c, CompList cl) {
+template <typename T, int SEG, bool CB_>
+__global__ __launch_bounds__(256) void k_adi_x(CtxArg<CB_> ca_, CompList cl) {
+  const Ctx c = ctx_get(ca_);
   XSETUP;
   const int sel = blockIdx.y;
   const int comp = sel == 0 ? cl.comp[0] : sel == 1 ? cl.comp[1] : cl.comp[2];
@@ -789,8 +600,9 @@ __global__ __launch_bounds__(256) void k_adi_x(Ctx c, CompList cl) {
   }
 }
 
-template <typename T, int SEG>
-__global__ __launch_bounds__(512) void k_adi_z(Ctx c, CompList cl) {
+template <typename T, int SEG, bool CB_>
+__global__ __launch_bounds__(512) void k_adi_z(CtxArg<CB_> ca_, CompList cl) {
+  const Ctx c = ctx_get(ca_);
   ZSETUP;
   const int sel = blockIdx.y;
   const int comp = sel == 0 ? cl.comp[0] : sel == 1 ? cl.comp[1] : cl.comp[2];
@@ -833,8 +645,9 @@ __global__ __launch_bounds__(512) void k_adi_z(Ctx c, CompList cl) {
 // RDP = DIV(U)/DT + DIV(DELTA)/DT, with DIV(U) of the current state kept in
 // FP64 by the previous update; k_rhsdp_z adds DIV(U)/DT (in the order
 // ((x and y terms) + DIV(U)/DT) + z term), so this kernel stages only UH.
-template <typename T, int SEG, typename TS>
-__global__ __launch_bounds__(256) void k_rhsdp_x(Ctx c) {
+template <typename T, int SEG, typename TS, bool CB_>
+__global__ __launch_bounds__(256) void k_rhsdp_x(CtxArg<CB_> ca_) {
+  const Ctx c = ctx_get(ca_);
   XSETUP;
   constexpr bool MIX = sizeof(TS) == 4;
   const Fields F = c.d;
@@ -856,8 +669,9 @@ __global__ __launch_bounds__(256) void k_rhsdp_x(Ctx c) {
 }
 
 // z-lines: RDP += D1z(UH3)/DT (:8196-8211)
-template <typename T, int SEG, typename TS>
-__global__ __launch_bounds__(512) void k_rhsdp_z(Ctx c) {
+template <typename T, int SEG, typename TS, bool CB_>
+__global__ __launch_bounds__(512) void k_rhsdp_z(CtxArg<CB_> ca_) {
+  const Ctx c = ctx_get(ca_);
   ZSETUP;
   const T dx3 = (T)c.s.dx3, dt = (T)c.s.dt;
   T a[SEG], d[SEG], lf, rt;
@@ -996,8 +810,9 @@ __device__ __forceinline__ void fft_lines(cplx_t<T>* a, int n, int nl, int ls, i
 // stride TL+1 in LDS so that the row-wise fill and drain hit distinct banks.
 // The FFT kernels copy their twiddle table (n complex values) into LDS
 // behind the tile, so the stages read twiddles from LDS.
-template <typename T, int NE, int NT>
-__global__ __launch_bounds__(NT) void k_fft_x(Ctx c) {
+template <typename T, int NE, int NT, bool CB_>
+__global__ __launch_bounds__(NT) void k_fft_x(CtxArg<CB_> ca_) {
+  const Ctx c = ctx_get(ca_);
   static_assert(sizeof(T) == 8, "the twiddle tables are FP64");
   extern __shared__ __align__(16) unsigned char smem[];
   typedef cplx_t<T> zc;
@@ -1036,8 +851,10 @@ __global__ __launch_bounds__(NT) void k_fft_x(Ctx c) {
 }
 
 // z-DFT of the half spectrum, columns (jm, kx); sign -1 forward, +1 backward
-template <typename T, int NE, int NT>
-__global__ __launch_bounds__(NT) void k_fft_z(Ctx c, int sign, void* spec) {
+template <typename T, int NE, int NT, bool CB_>
+__global__ __launch_bounds__(NT) void k_fft_z(CtxArg<CB_> ca_, int sign) {
+  const Ctx c = ctx_get(ca_);
+  void* spec = (void*)c.d.spec;
   static_assert(sizeof(T) == 8, "the twiddle tables are FP64");
   extern __shared__ __align__(16) unsigned char smem[];
   typedef cplx_t<T> zc;
@@ -1079,8 +896,11 @@ __global__ __launch_bounds__(NT) void k_fft_z(Ctx c, int sign, void* spec) {
 // kx = kx0 + l by the partitioned recurrences.
 // Spectrum in TI, recurrences and result in TO (FP64 in both modes); chunks
 // of at most CH points.
-template <typename TI, typename TO, int CH, int NT>
-__global__ __launch_bounds__(NT) void k_pois_y(Ctx c, const void* spec_in, void* spec_out) {
+template <typename TI, typename TO, int CH, int NT, bool CB_>
+__global__ __launch_bounds__(NT) void k_pois_y(CtxArg<CB_> ca_) {
+  const Ctx c = ctx_get(ca_);
+  const void* spec_in = (const void*)c.d.spec;
+  void* spec_out = (void*)c.d.spec;
   __shared__ TO xs[8][NT];
   typedef cplx_t<TI> zi;
   typedef cplx_t<TO> zo;
@@ -1173,8 +993,10 @@ __global__ void k_pois_table(Ctx c) {
 // part / (N1M*N3M) into DP (:9429-9433); two rows per complex transform:
 // Z = A + iB with the imaginary parts of A and B at kx = 0 and NX/2 dropped
 // (they do not reach the real part of the reference's transform)
-template <typename T, int NE, int NT>
-__global__ __launch_bounds__(NT) void k_fft_xinv(Ctx c, const void* spec) {
+template <typename T, int NE, int NT, bool CB_>
+__global__ __launch_bounds__(NT) void k_fft_xinv(CtxArg<CB_> ca_) {
+  const Ctx c = ctx_get(ca_);
+  const void* spec = (const void*)c.d.spec;
   static_assert(sizeof(T) == 8, "the twiddle tables are FP64");
   extern __shared__ __align__(16) unsigned char smem[];
   typedef cplx_t<T> zc;
@@ -1230,8 +1052,9 @@ __global__ __launch_bounds__(NT) void k_fft_xinv(Ctx c, const void* spec) {
 // shadows are written beside the state.
 // `refine` (mixed mode): the correction pass of the FP64-residual
 // refinement, U -= DT*grad(DP) with no DELTA and no DMPRESG.
-template <typename T, int SEG, typename TS>
-__global__ __launch_bounds__(256) void k_upd_x(Ctx c, int chk, int refine) {
+template <typename T, int SEG, typename TS, bool CB_>
+__global__ __launch_bounds__(256) void k_upd_x(CtxArg<CB_> ca_, int chk, int refine) {
+  const Ctx c = ctx_get(ca_);
   XSETUP;
   constexpr bool MIX = sizeof(TS) == 4;
   const Scal s = c.s;
@@ -1310,8 +1133,9 @@ __global__ __launch_bounds__(256) void k_upd_x(Ctx c, int chk, int refine) {
 // (:1218-1230); with `chk`: DIV += D1z(U3), DIVMAX, and the CFL maximum of
 // the new field (:9112-9128, :9150-9162).  Mixed mode also stores the full
 // DIV (the next RDP starts from it) and the FP32 shadows of W and P.
-template <typename T, int SEG, typename TS>
-__global__ __launch_bounds__(512) void k_upd_z(Ctx c, int chk, int refine) {
+template <typename T, int SEG, typename TS, bool CB_>
+__global__ __launch_bounds__(512) void k_upd_z(CtxArg<CB_> ca_, int chk, int refine) {
+  const Ctx c = ctx_get(ca_);
   ZSETUP;
   constexpr bool MIX = sizeof(TS) == 4;
   const Scal s = c.s;
@@ -1443,7 +1267,9 @@ __device__ __forceinline__ void region_sums(const Fields& F, const int (&reg)[NR
 // the two flow sums.  nu / nw: partial counts of the u and w regions; with
 // `flows` the flow rates FLOW1/FLOW3 of U are formed too (FP64 mode, where
 // the ADI sweeps read U).
-__global__ __launch_bounds__(FIN_THREADS) void k_flow_fin(Ctx c, int nu, int nw, int flows) {
+template <bool CB_>
+__global__ __launch_bounds__(FIN_THREADS) void k_flow_fin(CtxArg<CB_> ca_, int nu, int nw, int flows) {
+  const Ctx c = ctx_get(ca_);
   const Fields F = c.d;
   const Scal s = c.s;
   const int reg[4] = {FP_DU, FP_DW, FP_U, FP_W};
@@ -1468,7 +1294,9 @@ __global__ __launch_bounds__(FIN_THREADS) void k_flow_fin(Ctx c, int nu, int nw,
 // are cleared again for the next k_upd_z; with `flows` (mixed mode, where the
 // update kernels read the old U) also the flow rates FLOW1/FLOW3 from nu /
 // nw partial sums
-__global__ __launch_bounds__(FIN_THREADS) void k_max_fin(Ctx c, int nu, int nw, int flows) {
+template <bool CB_>
+__global__ __launch_bounds__(FIN_THREADS) void k_max_fin(CtxArg<CB_> ca_, int nu, int nw, int flows) {
+  const Ctx c = ctx_get(ca_);
   const Fields F = c.d;
   if (flows) {
     const int reg[2] = {FP_U, FP_W};
@@ -1547,19 +1375,19 @@ static const LShape& zshape(const Ctx& c) {
     const size_t lz_ = lds_line(z_, false, sizeof(T));                                                \
     const dim3 gr_(zblocks(c, z_), GY), th_(z_.L * z_.G);                                             \
     if (sizeof(T) == 4) {                                                                             \
-      ZSEG32(z_.seg, { KScope ks_(c, KID); hipLaunchKernelGGL(HIP_KERNEL_NAME(KERNEL<float, S_>), gr_, th_, lz_, c.st, c, ##__VA_ARGS__); }); \
+      ZSEG32(z_.seg, { KScope ks_(c, KID); C3D_LAUNCH(KERNEL, (float, S_), gr_, th_, lz_, c, ##__VA_ARGS__); }); \
     } else {                                                                                          \
-      ZSEG(z_.seg, { KScope ks_(c, KID); hipLaunchKernelGGL(HIP_KERNEL_NAME(KERNEL<double, S_>), gr_, th_, lz_, c.st, c, ##__VA_ARGS__); }); \
+      ZSEG(z_.seg, { KScope ks_(c, KID); C3D_LAUNCH(KERNEL, (double, S_), gr_, th_, lz_, c, ##__VA_ARGS__); }); \
     }                                                                                                 \
   }
 
 template <typename T>
 static void rhs_lines(const Ctx& c) {
   const size_t lx = lds_line(c.o.lx, true, sizeof(T));
-  XSEG(c.o.lx.seg, { KScope ks_(c, KID_RHS_X); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rhs_x<T, S_>), dim3(xblocks(c)), xthreads(c), lx, c.st, c); });
+  XSEG(c.o.lx.seg, { KScope ks_(c, KID_RHS_X); C3D_LAUNCH(k_rhs_x, (T, S_), dim3(xblocks(c)), xthreads(c), lx, c); });
   ZLAUNCH(T, KID_RHS_Z, k_rhs_z, 1);
   ZLAUNCH(T, KID_RHS_ZW, k_rhs_zw, 1);
-  XSEG(c.o.lx.seg, { KScope ks_(c, KID_RHS_X2); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rhs_x2<T, S_>), dim3(xblocks(c)), xthreads(c), lx, c.st, c); });
+  XSEG(c.o.lx.seg, { KScope ks_(c, KID_RHS_X2); C3D_LAUNCH(k_rhs_x2, (T, S_), dim3(xblocks(c)), xthreads(c), lx, c); });
 }
 void rhs_lines(const Ctx& c) {
   if (c.s.mixed)
@@ -1586,7 +1414,7 @@ static void getuh_t(const Ctx& c, bool fused) {
     const dim3 gr(ylx * c.g.nz, 3), th(ys.L * ys.C);
     const size_t lb = staged ? tbytes : 0;
     KScope ks_(c, KID_ADI_Y);
-#define YF_(SGS, CH, NT) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rhsy_adiy<T, SGS, CH, NT>), gr, th, lb, c.st, c, staged)
+#define YF_(SGS, CH, NT) C3D_LAUNCH(k_rhsy_adiy, (T, SGS, CH, NT), gr, th, lb, c, staged)
 #define YFN_(SGS, CH) \
   if (big)            \
     YF_(SGS, CH, 1024); \
@@ -1611,9 +1439,9 @@ static void getuh_t(const Ctx& c, bool fused) {
   const CompList x12{2, {0, 1, 0}, {0, 0, 0}};
   const CompList z123{3, {0, 1, 2}, {1, 1, 0}};
   const CompList x3{1, {2, 0, 0}, {1, 0, 0}};
-  XSEG(c.o.lx.seg, { KScope ks_(c, KID_ADI_X); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adi_x<T, S_>), dim3(xblocks(c), 2), xthreads(c), lx, c.st, c, x12); });
+  XSEG(c.o.lx.seg, { KScope ks_(c, KID_ADI_X); C3D_LAUNCH(k_adi_x, (T, S_), dim3(xblocks(c), 2), xthreads(c), lx, c, x12); });
   ZLAUNCH(T, KID_ADI_Z, k_adi_z, 3, z123);
-  XSEG(c.o.lx.seg, { KScope ks_(c, KID_ADI_X); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adi_x<T, S_>), dim3(xblocks(c), 1), xthreads(c), lx, c.st, c, x3); });
+  XSEG(c.o.lx.seg, { KScope ks_(c, KID_ADI_X); C3D_LAUNCH(k_adi_x, (T, S_), dim3(xblocks(c), 1), xthreads(c), lx, c, x3); });
 }
 void getuh(const Ctx& c) {
   if (c.s.mixed)
@@ -1632,8 +1460,8 @@ void getuh_fused(const Ctx& c) {
 template <typename TS>
 static void rhsdp_t(const Ctx& c) {
   const size_t lx = lds_line(c.o.lx, true, sizeof(double)), lz = lds_line(c.o.lz, false, sizeof(double));
-  XSEG(c.o.lx.seg, { KScope ks_(c, KID_RHSDP_X); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rhsdp_x<double, S_, TS>), dim3(xblocks(c)), xthreads(c), lx, c.st, c); });
-  ZSEG(c.o.lz.seg, { KScope ks_(c, KID_RHSDP_Z); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rhsdp_z<double, S_, TS>), dim3(zblocks(c)), zthreads(c), lz, c.st, c); });
+  XSEG(c.o.lx.seg, { KScope ks_(c, KID_RHSDP_X); C3D_LAUNCH(k_rhsdp_x, (double, S_, TS), dim3(xblocks(c)), xthreads(c), lx, c); });
+  ZSEG(c.o.lz.seg, { KScope ks_(c, KID_RHSDP_Z); C3D_LAUNCH(k_rhsdp_z, (double, S_, TS), dim3(zblocks(c)), zthreads(c), lz, c); });
 }
 void rhsdp(const Ctx& c) {
   if (c.s.mixed)
@@ -1654,7 +1482,6 @@ static void getdp_fp64(const Ctx& c) {
   const int gx = (g.n2m * g.nz + 2 * f.tlx - 1) / (2 * f.tlx);
   const int gz = ((g.nxh + f.tlz - 1) / f.tlz) * g.n2m;
   const int gy = ((g.nxh + c.o.yp.L - 1) / c.o.yp.L) * g.nz;
-  void* sp = (void*)c.d.spec;
   // tile sizes (complex values per block): up to 1024 on 256 threads, up
   // to 2048 or 4096 on 512 (the autotuner picks the lines per tile)
   const int ex = g.nx * f.tlx, ez = g.nz * f.tlz;
@@ -1662,21 +1489,21 @@ static void getdp_fp64(const Ctx& c) {
 #define FFTW_(K, e, nt, ...)                                                            \
   do {                                                                                  \
     if ((e) > 2048)                                                                     \
-      hipLaunchKernelGGL(HIP_KERNEL_NAME(K<T, 4096, 512>), __VA_ARGS__);               \
+      C3D_LAUNCH(K, (T, 4096, 512), __VA_ARGS__);                                       \
     else if ((e) > 1024)                                                                \
-      hipLaunchKernelGGL(HIP_KERNEL_NAME(K<T, 2048, 512>), __VA_ARGS__);               \
+      C3D_LAUNCH(K, (T, 2048, 512), __VA_ARGS__);                                       \
     else                                                                                \
-      hipLaunchKernelGGL(HIP_KERNEL_NAME(K<T, 1024, 256>), __VA_ARGS__);               \
+      C3D_LAUNCH(K, (T, 1024, 256), __VA_ARGS__);                                       \
   } while (0)
-  { KScope ks_(c, KID_FFT_X); FFTW_(k_fft_x, ex, nx_t, dim3(gx), dim3(nx_t), ldsx, c.st, c); }
-  { KScope ks_(c, KID_FFT_Z); FFTW_(k_fft_z, ez, nz_t, dim3(gz), dim3(nz_t), ldsz, c.st, c, -1, sp); }
+  { KScope ks_(c, KID_FFT_X); FFTW_(k_fft_x, ex, nx_t, dim3(gx), dim3(nx_t), ldsx, c); }
+  { KScope ks_(c, KID_FFT_Z); FFTW_(k_fft_z, ez, nz_t, dim3(gz), dim3(nz_t), ldsz, c, -1); }
   {
     // tiles of up to 256 or 512 threads (16 modes: spectrum rows of 256 bytes)
     KScope ks_(c, KID_POIS_Y);
     const int npt = c.o.yp.L * c.o.yp.C;
     const bool six = (g.n2m + c.o.yp.C - 1) / c.o.yp.C <= 6;
 #define PY_(CH, NT) \
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pois_y<T, T, CH, NT>), dim3(gy), dim3(npt), 0, c.st, c, (const void*)sp, sp)
+  C3D_LAUNCH(k_pois_y, (T, T, CH, NT), dim3(gy), dim3(npt), 0, c)
     if (npt > 256) {
       if (six) PY_(6, 512); else PY_(CHMAX, 512);
     } else {
@@ -1684,8 +1511,8 @@ static void getdp_fp64(const Ctx& c) {
     }
 #undef PY_
   }
-  { KScope ks_(c, KID_FFT_Z); FFTW_(k_fft_z, ez, nz_t, dim3(gz), dim3(nz_t), ldsz, c.st, c, +1, sp); }
-  { KScope ks_(c, KID_FFT_XINV); FFTW_(k_fft_xinv, ex, nx_t, dim3(gx), dim3(nx_t), ldsx, c.st, c, (const void*)sp); }
+  { KScope ks_(c, KID_FFT_Z); FFTW_(k_fft_z, ez, nz_t, dim3(gz), dim3(nz_t), ldsz, c, +1); }
+  { KScope ks_(c, KID_FFT_XINV); FFTW_(k_fft_xinv, ex, nx_t, dim3(gx), dim3(nx_t), ldsx, c); }
 #undef FFTW_
 }
 void getdp(const Ctx& c) { getdp_fp64(c); }
@@ -1695,9 +1522,12 @@ void getdp(const Ctx& c) { getdp_fp64(c); }
 void fft_attrs() {
   const int lim = 160 * 1024;
 #define FA_(K)                                                                                                   \
-  (void)hipFuncSetAttribute((const void*)&K<double, 4096, 512>, hipFuncAttributeMaxDynamicSharedMemorySize, lim); \
-  (void)hipFuncSetAttribute((const void*)&K<double, 2048, 512>, hipFuncAttributeMaxDynamicSharedMemorySize, lim); \
-  (void)hipFuncSetAttribute((const void*)&K<double, 1024, 256>, hipFuncAttributeMaxDynamicSharedMemorySize, lim)
+  (void)hipFuncSetAttribute((const void*)&K<double, 4096, 512, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim); \
+  (void)hipFuncSetAttribute((const void*)&K<double, 2048, 512, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim); \
+  (void)hipFuncSetAttribute((const void*)&K<double, 1024, 256, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim); \
+  (void)hipFuncSetAttribute((const void*)&K<double, 4096, 512, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim); \
+  (void)hipFuncSetAttribute((const void*)&K<double, 2048, 512, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim); \
+  (void)hipFuncSetAttribute((const void*)&K<double, 1024, 256, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim)
   FA_(k_fft_x);
   FA_(k_fft_z);
   FA_(k_fft_xinv);
@@ -1712,15 +1542,15 @@ void pois_table(const Ctx& c) {
 template <typename TS>
 static void upd_x_t(const Ctx& c, bool chk, int refine) {
   const size_t lx = lds_line(c.o.lx, true, sizeof(double));
-  XSEG(c.o.lx.seg, { KScope ks_(c, KID_UPD_X); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_upd_x<double, S_, TS>), dim3(xblocks(c)), xthreads(c), lx, c.st, c, chk ? 1 : 0, refine); });
+  XSEG(c.o.lx.seg, { KScope ks_(c, KID_UPD_X); C3D_LAUNCH(k_upd_x, (double, S_, TS), dim3(xblocks(c)), xthreads(c), lx, c, chk ? 1 : 0, refine); });
 }
 template <typename TS>
 static void upd_z_t(const Ctx& c, bool chk, int refine) {
   const size_t lz = lds_line(c.o.lz, false, sizeof(double));
-  ZSEG(c.o.lz.seg, { KScope ks_(c, KID_UPD_Z); hipLaunchKernelGGL(HIP_KERNEL_NAME(k_upd_z<double, S_, TS>), dim3(zblocks(c)), zthreads(c), lz, c.st, c, chk ? 1 : 0, refine); });
+  ZSEG(c.o.lz.seg, { KScope ks_(c, KID_UPD_Z); C3D_LAUNCH(k_upd_z, (double, S_, TS), dim3(zblocks(c)), zthreads(c), lz, c, chk ? 1 : 0, refine); });
   if (chk) {
     KScope ks_(c, KID_FLOW_FIN);
-    hipLaunchKernelGGL(k_max_fin, dim3(1), dim3(FIN_THREADS), 0, c.st, c, xblocks(c), zblocks(c), c.s.mixed ? 1 : 0);
+    C3D_LAUNCH0(k_max_fin, dim3(1), dim3(FIN_THREADS), 0, c, xblocks(c), zblocks(c), c.s.mixed ? 1 : 0);
   }
 }
 // DMPRESG/DMPRESG3 (and PRESG += ...) from the partial sums of the last
@@ -1729,7 +1559,7 @@ static void upd_z_t(const Ctx& c, bool chk, int refine) {
 void flow_fin(const Ctx& c) {
   const int nu = c.s.mixed ? zblocks(c, c.o.lz32) : zblocks(c, c.o.lz);
   KScope ks_(c, KID_FLOW_FIN);
-  hipLaunchKernelGGL(k_flow_fin, dim3(1), dim3(FIN_THREADS), 0, c.st, c, nu, xblocks(c), c.s.mixed ? 0 : 1);
+  C3D_LAUNCH0(k_flow_fin, dim3(1), dim3(FIN_THREADS), 0, c, nu, xblocks(c), c.s.mixed ? 0 : 1);
 }
 void upd_x(const Ctx& c, bool chk, bool refine) {
   if (c.s.mixed)

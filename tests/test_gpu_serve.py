@@ -105,3 +105,50 @@ def test_ensemble_equals_alone():
     ens_ch[0].step(8)
     for ch in ens_ch + alone:
         ch.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batched_ensemble_equals_alone(precision):
+    """members of the same shape run batched (one launch per kernel for all
+    of them, each block reading its member's context): three wall-modelled
+    LES channels of one grid at different Reynolds numbers (the log law at
+    both walls and at the bottom wall), two DNS channels of another grid,
+    one resolved-wall LES pair (WMODEL = 0, WALLSS feeds SGSCS) and an
+    integral-wall-model member that runs alone; every member ends bit for
+    bit where it ends stepped alone"""
+    from compact3d_amd import c3d
+    specs = [(dict(_cfg(49, 33, 33, les=True, re=20000.0), wmodel=6.0), 1),
+             (dict(_cfg(49, 33, 33, les=True, re=40000.0), wmodel=6.0), 2),
+             (dict(_cfg(49, 33, 33, les=True, re=90000.0), wmodel=6.0), 3),
+             (_cfg(33, 41, 33), 4), (_cfg(33, 41, 33, re=8000.0), 5),
+             (dict(_cfg(33, 33, 33, les=True, re=20000.0), wmodel=0.0), 6),
+             (dict(_cfg(33, 33, 33, les=True, re=30000.0), wmodel=0.0), 7),
+             (dict(_cfg(33, 33, 33, les=True, re=20000.0), wmodel=2.0, cflmax=0.5), 8)]
+    ens_ch, alone = [], []
+    for cfg, seed in specs:
+        pair = [c3d.Channel(**cfg, precision=precision) for _ in range(2)]
+        U, P = _field(pair[0], cfg, seed)
+        for ch in pair:
+            ch.upload(U, P)
+            if cfg["files"]:
+                ch.set_scalar(c3d.S_WSM1, 1e-3)
+                ch.set_scalar(c3d.S_WSM3, 1e-4)
+        ens_ch.append(pair[0])
+        alone.append(pair[1])
+    print("batched ensemble, precision", precision, flush=True)
+    ens = c3d.Ensemble(ens_ch)
+    ens.advance(5)
+    ens.advance(4)
+    dm, cf = ens.sync()
+    ens.close()
+    for q, ch in enumerate(alone):
+        for n in range(9):
+            d, c = ch.step(n + 1)
+        a, b = ch.download(), ens_ch[q].download()
+        print("member", q, "divmax ensemble %.2e alone %.2e" % (dm[q], d), flush=True)
+        for key in ("U", "P", "HH", "SGSVIS"):
+            assert np.array_equal(a[key], b[key]), f"member {q}: {key} differs"
+        assert a["PRESG"] == b["PRESG"] and a["PRESG3"] == b["PRESG3"]
+        assert dm[q] == d and cf[q] == c
+    for ch in ens_ch + alone:
+        ch.close()
