@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -466,13 +467,20 @@ static int upload_dt_terms(c3d_handle* h) {
 
 template <typename T>
 static int dalloc(c3d_handle* h, T** p, size_t n) {
+  // Successive allocations start at different 256-byte phases (0..63 x 256
+  // bytes, stepping by 37): fields of equal size would otherwise all begin
+  // at the same large alignment, and the same element of every field (or of
+  // every channel of an ensemble) would fall on the same HBM channel when
+  // they are streamed together.
+  static std::atomic<unsigned> seq{0};
+  const size_t pad = 256 * (size_t)((seq.fetch_add(1) * 37u) % 64u);
   void* q = nullptr;
-  HIPCHK(hipMalloc(&q, n * sizeof(T) + 64));
+  HIPCHK(hipMalloc(&q, n * sizeof(T) + 64 + pad));
   // on the handle's stream: it is non-blocking, so the null stream would not
   // order the fill before the kernels and copies that follow on it
-  HIPCHK(hipMemsetAsync(q, 0, n * sizeof(T) + 64, h->c.st));
+  HIPCHK(hipMemsetAsync(q, 0, n * sizeof(T) + 64 + pad, h->c.st));
   h->allocs.push_back(q);
-  *p = (T*)q;
+  *p = (T*)((char*)q + pad);
   return 0;
 }
 

@@ -12,44 +12,68 @@
 // STRAIN (compact3d.f:5558-5689) at one cell centre: the six components
 // S11 S12 S13 S22 S23 S33 from second-order differences (wall rows by the
 // FJPA/FJMU flags, WMODEL != 0 doubling the y differences at J = 1, N2M).
-__device__ __forceinline__ void strain_at(const Ctx& c, int i, int j, int k, double sr[6]) {
-  const Geo& g = c.g;
-  const Scal& s = c.s;
+// The factors that depend on J only are formed once per plane (StrainJ),
+// each as the reference's expression evaluates it left to right, so the
+// per-point arithmetic is the same.
+struct StrainJ {
+  double up, um;    // FJPA(J)*1/H(J+1), FJMU(J)*1/H(J)
+  double cp, cm;    // wall weights: (1-FJPA)*0.5, (1-FJMU)*0.5 (WMODEL = 0); 1-FJPA, 1-FJMU otherwise
+  double yf, rdy;   // YFAC/DY(J), 1/DY(J)
+  double dyj, dyjp, dyjm;  // DY(J)/2, DY(J+1)/2, DY(J-1)/2
+  bool wm0;
+};
+__device__ __forceinline__ StrainJ strain_j(const Ctx& c, int j) {
   const Metrics& M = c.m;
+  const double* DY = M.dy;
+  const double* H = M.h;
+  const int jp = j + 1, jm = j - 1;
+  const int fjum = (int)M.fjmu[j], fjup = (int)M.fjpa[j];
+  StrainJ f;
+  f.wm0 = (c.s.wmodel == 0.0);
+  f.up = fjup * 1.0 / H[jp];
+  f.um = fjum * 1.0 / H[j];
+  f.cp = f.wm0 ? (1. - fjup) * 0.5 : (1. - fjup);
+  f.cm = f.wm0 ? (1. - fjum) * 0.5 : (1. - fjum);
+  const double yfac = (!f.wm0 && (j == 1 || j == c.g.n2m)) ? 2.0 : 1.0;
+  f.yf = yfac / DY[j];
+  f.rdy = 1.0 / DY[j];
+  f.dyj = DY[j] / 2.0;
+  f.dyjp = DY[jp] / 2.0;
+  f.dyjm = DY[jm] / 2.0;
+  return f;
+}
+__device__ __forceinline__ void strain_core(const Ctx& c, const StrainJ& f, int i, int j, int k, double sr[6]) {
+  const Geo& g = c.g;
   const double* U = c.d.u;
   const double* V = c.d.v;
   const double* W = c.d.w;
   const int ip = wrapp(i, g.nx), im = wrapm(i, g.nx), kp = wrapp(k, g.nz), km = wrapm(k, g.nz);
   const int jp = j + 1, jm = j - 1;
-  const double DX1 = s.dx1, DX3 = s.dx3;
-  const double* DY = M.dy;
-  const double* H = M.h;
-  const int fjum = (int)M.fjmu[j], fjup = (int)M.fjpa[j];
-  const bool wm0 = (s.wmodel == 0.0);
-  auto u_ = [&](int ii, int jj, int kk) { return U[gidx(g, ii, jj, kk)]; };
-  auto v_ = [&](int ii, int jj, int kk) { return V[gidx(g, ii, jj, kk)]; };
-  auto w_ = [&](int ii, int jj, int kk) { return W[gidx(g, ii, jj, kk)]; };
+  const double DX1 = c.s.dx1, DX3 = c.s.dx3;
+  // 32-bit element offsets (fields hold fewer than 2^31 elements)
+  const unsigned nx = (unsigned)g.nx, pl = (unsigned)g.plane;
+  auto o_ = [&](int ii, int jj, int kk) { return (unsigned)kk * pl + (unsigned)jj * nx + (unsigned)ii; };
+  auto u_ = [&](int ii, int jj, int kk) { return U[o_(ii, jj, kk)]; };
+  auto v_ = [&](int ii, int jj, int kk) { return V[o_(ii, jj, kk)]; };
+  auto w_ = [&](int ii, int jj, int kk) { return W[o_(ii, jj, kk)]; };
   const int N2 = g.n2;
   double u2 = u_(ip, j, k), u1 = u_(i, j, k);
   sr[0] = DX1 * (u2 - u1);
   double ujp = 0.5 * (u_(i, jp, k) + u_(ip, jp, k));
   double ujc = 0.5 * (u_(i, j, k) + u_(ip, j, k));
   double ujm = 0.5 * (u_(i, jm, k) + u_(ip, jm, k));
-  if (wm0) {
-    u2 = fjup * 1.0 / H[jp] * (DY[j] / 2.0 * ujp + DY[jp] / 2.0 * ujc) +
-         (1. - fjup) * 0.5 * (u_(i, N2, k) + u_(ip, N2, k));
-    u1 = fjum * 1.0 / H[j] * (DY[j] / 2.0 * ujm + DY[jm] / 2.0 * ujc) +
-         (1. - fjum) * 0.5 * (u_(i, 0, k) + u_(ip, 0, k));
+  if (f.wm0) {
+    u2 = f.up * (f.dyj * ujp + f.dyjp * ujc) + f.cp * (u_(i, N2, k) + u_(ip, N2, k));
+    u1 = f.um * (f.dyj * ujm + f.dyjm * ujc) + f.cm * (u_(i, 0, k) + u_(ip, 0, k));
   } else {
-    u2 = fjup * 1.0 / H[jp] * (DY[j] / 2.0 * ujp + DY[jp] / 2.0 * ujc) + (1. - fjup) * ujc;
-    u1 = fjum * 1.0 / H[j] * (DY[j] / 2.0 * ujm + DY[jm] / 2.0 * ujc) + (1. - fjum) * ujc;
+    u2 = f.up * (f.dyj * ujp + f.dyjp * ujc) + f.cp * ujc;
+    u1 = f.um * (f.dyj * ujm + f.dyjm * ujc) + f.cm * ujc;
   }
   double vip = 0.5 * (v_(ip, j, k) + v_(ip, jp, k));
   double vic = 0.5 * (v_(i, j, k) + v_(i, jp, k));
   double vim = 0.5 * (v_(im, j, k) + v_(im, jp, k));
   double v2 = 0.5 * (vip + vic), v1 = 0.5 * (vim + vic);
-  const double yfac = (!wm0 && (j == 1 || j == g.n2m)) ? 2.0 : 1.0;
-  sr[1] = 0.5 * (DX1 * (v2 - v1) + yfac / DY[j] * (u2 - u1));
+  sr[1] = 0.5 * (DX1 * (v2 - v1) + f.yf * (u2 - u1));
   double ukp = 0.5 * (u_(i, j, kp) + u_(ip, j, kp));
   double ukc = 0.5 * (u_(i, j, k) + u_(ip, j, k));
   double ukm = 0.5 * (u_(i, j, km) + u_(ip, j, km));
@@ -62,28 +86,29 @@ __device__ __forceinline__ void strain_at(const Ctx& c, int i, int j, int k, dou
   sr[2] = 0.5 * (DX1 * (w2 - w1) + DX3 * (u2 - u1));
   v2 = v_(i, jp, k);
   v1 = v_(i, j, k);
-  sr[3] = 1.0 / DY[j] * (v2 - v1);
+  sr[3] = f.rdy * (v2 - v1);
   double wjp = 0.5 * (w_(i, jp, k) + w_(i, jp, kp));
   double wjc = 0.5 * (w_(i, j, k) + w_(i, j, kp));
   double wjm = 0.5 * (w_(i, jm, k) + w_(i, jm, kp));
-  if (wm0) {
-    w2 = fjup * 1.0 / H[jp] * (DY[j] / 2.0 * wjp + DY[jp] / 2.0 * wjc) +
-         (1. - fjup) * 0.5 * (w_(i, N2, k) + w_(i, N2, kp));
-    w1 = fjum * 1.0 / H[j] * (DY[j] / 2.0 * wjm + DY[jm] / 2.0 * wjc) +
-         (1. - fjum) * 0.5 * (w_(i, 0, k) + w_(i, 0, kp));
+  if (f.wm0) {
+    w2 = f.up * (f.dyj * wjp + f.dyjp * wjc) + f.cp * (w_(i, N2, k) + w_(i, N2, kp));
+    w1 = f.um * (f.dyj * wjm + f.dyjm * wjc) + f.cm * (w_(i, 0, k) + w_(i, 0, kp));
   } else {
-    w2 = fjup * 1.0 / H[jp] * (DY[j] / 2.0 * wjp + DY[jp] / 2.0 * wjc) + (1. - fjup) * wjc;
-    w1 = fjum * 1.0 / H[j] * (DY[j] / 2.0 * wjm + DY[jm] / 2.0 * wjc) + (1. - fjum) * wjc;
+    w2 = f.up * (f.dyj * wjp + f.dyjp * wjc) + f.cp * wjc;
+    w1 = f.um * (f.dyj * wjm + f.dyjm * wjc) + f.cm * wjc;
   }
   double vkp = 0.5 * (v_(i, j, kp) + v_(i, jp, kp));
   double vkc = 0.5 * (v_(i, j, k) + v_(i, jp, k));
   double vkm = 0.5 * (v_(i, j, km) + v_(i, jp, km));
   v2 = 0.5 * (vkp + vkc);
   v1 = 0.5 * (vkm + vkc);
-  sr[4] = 0.5 * (yfac / DY[j] * (w2 - w1) + DX3 * (v2 - v1));
+  sr[4] = 0.5 * (f.yf * (w2 - w1) + DX3 * (v2 - v1));
   w2 = w_(i, j, kp);
   w1 = w_(i, j, k);
   sr[5] = DX3 * (w2 - w1);
+}
+__device__ __forceinline__ void strain_at(const Ctx& c, int i, int j, int k, double sr[6]) {
+  strain_core(c, strain_j(c, j), i, j, k, sr);
 }
 
 // 2 S:S with the off-diagonal components counted twice, in STRAIN's order
@@ -95,24 +120,25 @@ __device__ __forceinline__ double sdots(const double s[6]) {
 // SGSCS (compact3d.f:1413-1488) with STRAIN: |S| = sqrt(2 S:S),
 // nu_t = max((0.1 (1-exp(-y+/26)))^2 (DY/(DX1 DX3))^(2/3) |S|, -1/RE),
 // y+ = (Y(J)+DY(J)/2) u_tau RE, u_tau = (WSM1^2 + WSM3^2)^(1/4) (previous step).
+// One block per (tile of 64 i x 4*kpt k, plane J): the plane's factors
+// (STRAIN's and the model constant times the filter width) once per
+// thread, then kpt points per thread (8 on large grids, fewer when the grid
+// would have fewer than ~2048 blocks).
+#define SG_TI 64
 template <bool CB_>
-__global__ void k_sgscs(CtxArg<CB_> ca_) {
+__global__ __launch_bounds__(256) void k_sgscs(CtxArg<CB_> ca_, int kpt) {
   const Ctx c = ctx_get(ca_);
   const Geo g = c.g;
-  const long long tot = (long long)g.nx * g.n2m * g.nz;
-  long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (t >= tot) return;
-  const int i = t % g.nx;
-  const int j = 1 + (int)((t / g.nx) % g.n2m);
-  const int k = (int)(t / ((long long)g.nx * g.n2m));
   const Scal s = c.s;
   const Metrics M = c.m;
   const Fields F = c.d;
+  const int ntx = (g.nx + SG_TI - 1) / SG_TI;
+  const int ti = blockIdx.x % ntx, tk = blockIdx.x / ntx;
+  const int j = 1 + blockIdx.y;
+  const int i = ti * SG_TI + (threadIdx.x % SG_TI);
+  if (i >= g.nx) return;
   const double* DY = M.dy;
-  double sr[6];
-  strain_at(c, i, j, k, sr);
-  double sumsr = sdots(sr);
-  double abssr = sqrt(2. * sumsr);
+  const StrainJ fj = strain_j(c, j);
   const double w1s = F.sc[SC_WSM1], w3s = F.sc[SC_WSM3];
   const double utau = sqrt(sqrt(w1s * w1s + w3s * w3s));
   double cs = 0.1;
@@ -120,8 +146,19 @@ __global__ void k_sgscs(CtxArg<CB_> ca_) {
   cs = cs * (1 - exp(-yplus / 26.0));
   const double smagc = cs * cs;
   const double dxyz = 1.0 / s.dx1 * DY[j] / s.dx3;
-  const double turbv = smagc * pow(dxyz, 2. / 3.) * abssr;
-  F.sgs[gidx(g, i, j, k)] = fmax(turbv, -s.rinv);
+  const double coef = smagc * pow(dxyz, 2. / 3.);
+  const int k0 = tk * 4 * kpt + threadIdx.x / SG_TI;
+#pragma unroll 2
+  for (int m = 0; m < kpt; m++) {
+    const int k = k0 + 4 * m;
+    if (k >= g.nz) break;
+    double sr[6];
+    strain_core(c, fj, i, j, k, sr);
+    double sumsr = sdots(sr);
+    double abssr = sqrt(2. * sumsr);
+    const double turbv = coef * abssr;
+    F.sgs[(unsigned)k * (unsigned)g.plane + (unsigned)j * (unsigned)g.nx + (unsigned)i] = fmax(turbv, -s.rinv);
+  }
 }
 
 // ------------------------------------------------------------ SGSDY
@@ -1396,7 +1433,10 @@ static size_t lds(const TileShape& t, int ntiles) { return sizeof(double) * (siz
 
 void sgscs(const Ctx& c) {
   KScope ks_(c, KID_SGS);
-  C3D_LAUNCH0(k_sgscs, dim3(pgrid((long long)c.g.nx * c.g.n2m * c.g.nz)), dim3(C3D_NT), 0, c);
+  const int ntx = (c.g.nx + SG_TI - 1) / SG_TI;
+  int kpt = 8;
+  while (kpt > 1 && (long long)ntx * ((c.g.nz + 4 * kpt - 1) / (4 * kpt)) * c.g.n2m < 2048) kpt /= 2;
+  C3D_LAUNCH0(k_sgscs, dim3(ntx * ((c.g.nz + 4 * kpt - 1) / (4 * kpt)), c.g.n2m), dim3(256), 0, c, kpt);
 }
 
 int sgsdy_tiles(const Geo& g) {

@@ -5,7 +5,10 @@
 //   RHS2 :6840-6863, :7386-7389
 //   RHS3 :7457-7463, :7594-7605, :7858-7908, :8042-8094, :8096-8103
 // On entry RUH and HH hold the x and z parts; yterm_c() stores the final HH
-// and returns the final RUH (it does not store RUH).
+// and returns the final RUH (it does not store RUH).  Reciprocals and
+// products of J-only metrics come from the per-J vectors the host formed
+// with the same expressions (Metrics: cu1, cu2, dyh, rdy, rh), so each term
+// evaluates as the reference's does without a division per point.
 #pragma once
 #include "c3d_impl.h"
 
@@ -48,17 +51,17 @@ __device__ __forceinline__ T yterm_u(const Ctx& c, const YIdx& x, int i, int j, 
     const int jp = j + 1, jm = j - 1;
     const int im = wrapm(i, g.nx);
     const double n0 = F.nux[x.b], np1 = (j < g.n2m) ? F.nux[x.bp] : 0.0, nm1 = (j > 1) ? F.nux[x.bm] : 0.0;
-    const double sgsj2 = fjup * 1. / M.h[jp] * (M.dy[j] / 2. * np1 + M.dy[jp] / 2. * n0) +
+    const double sgsj2 = M.cu2[j] * (M.dyh[j] * np1 + M.dyh[jp] * n0) +
                          (1. - fjup) * 0.5 * (F.sgs[x.bN2] + F.sgs[gidx(g, im, g.n2, k)]);
-    const double sgsj1 = fjum * 1. / M.h[j] * (M.dy[jm] / 2. * n0 + M.dy[j] / 2. * nm1) +
+    const double sgsj1 = M.cu1[j] * (M.dyh[jm] * n0 + M.dyh[j] * nm1) +
                          (1. - fjum) * 0.5 * (F.sgs[x.b0] + F.sgs[gidx(g, im, 0, k)]);
     const double wfu = 1. - s.ifwall * (1. - fjup) * (s.aly - 1.);
     const double wfd = 1. - s.ifwall * (1. - fjum);
     const double uu22 = (F.u[x.bp] - F.u[x.b]) / M.h[jp];
     const double uu11 = (F.u[x.b] - F.u[x.bm]) / M.h[j];
-    double hd = (double)hh - 1. / M.dy[j] * (wfu * sgsj2 * uu22 - wfd * sgsj1 * uu11);
+    double hd = (double)hh - M.rdy[j] * (wfu * sgsj2 * uu22 - wfd * sgsj1 * uu11);
     const double dvp = (j < g.n2m) ? F.dvdx[x.bp] : 0.0;
-    hd = hd - 1.0 / M.dy[j] * (wfu * sgsj2 * dvp - wfd * sgsj1 * F.dvdx[x.b]);
+    hd = hd - M.rdy[j] * (wfu * sgsj2 * dvp - wfd * sgsj1 * F.dvdx[x.b]);
     hh = (T)hd;
   }
   ((T*)F.hh[0])[x.b] = hh;
@@ -90,9 +93,9 @@ __device__ __forceinline__ T yterm_v(const Ctx& c, const YIdx& x, int i, int j, 
   T hh = ((const T*)F.hh[1])[x.b] + (T)M.rh[j] * (v2 * v2 - v1 * v1);
   if (s.files) {
     const int jm = j - 1;
-    const double g2 = 1. / M.dy[j] * (F.v[x.bp] - F.v[x.b]);
-    const double g1 = 1. / M.dy[jm] * (F.v[x.b] - F.v[x.bm]);
-    const double vis22 = 1.0 / M.h[j] * (F.sgs[x.b] * g2 - F.sgs[x.bm] * g1);
+    const double g2 = M.rdy[j] * (F.v[x.bp] - F.v[x.b]);
+    const double g1 = M.rdy[jm] * (F.v[x.b] - F.v[x.bm]);
+    const double vis22 = M.rh[j] * (F.sgs[x.b] * g2 - F.sgs[x.bm] * g1);
     hh = (T)((double)hh - 2. * vis22);
   }
   ((T*)F.hh[1])[x.b] = hh;
@@ -122,17 +125,17 @@ __device__ __forceinline__ T yterm_w(const Ctx& c, const YIdx& x, int i, int j, 
     const int jp = j + 1, jm = j - 1;
     const int km = wrapm(k, g.nz);
     const double z0 = F.nuz[x.b], zp1 = (j < g.n2m) ? F.nuz[x.bp] : 0.0, zm1 = (j > 1) ? F.nuz[x.bm] : 0.0;
-    const double sgsj1 = fjum * 1. / M.h[j] * (M.dy[jm] / 2. * z0 + M.dy[j] / 2. * zm1) +
+    const double sgsj1 = M.cu1[j] * (M.dyh[jm] * z0 + M.dyh[j] * zm1) +
                          (1. - fjum) * 0.5 * (F.sgs[x.b0] + F.sgs[gidx(g, i, 0, km)]);
-    const double sgsj2 = fjup * 1.0 / M.h[jp] * (M.dy[j] / 2. * zp1 + M.dy[jp] / 2. * z0) +
+    const double sgsj2 = M.cu2[j] * (M.dyh[j] * zp1 + M.dyh[jp] * z0) +
                          (1. - fjup) * 0.5 * (F.sgs[x.bN2] + F.sgs[gidx(g, i, g.n2, km)]);
     const double wfu = 1. - s.ifwall * (1. - fjup) * (s.aly - 1.);
     const double wfd = 1. - s.ifwall * (1. - fjum);
-    const double ww22 = 1. / M.h[jp] * (F.w[x.bp] - F.w[x.b]);
-    const double ww11 = 1. / M.h[j] * (F.w[x.b] - F.w[x.bm]);
-    double hd = (double)hh - 1. / M.dy[j] * (wfu * sgsj2 * ww22 - wfd * sgsj1 * ww11);
+    const double ww22 = M.rh[jp] * (F.w[x.bp] - F.w[x.b]);
+    const double ww11 = M.rh[j] * (F.w[x.b] - F.w[x.bm]);
+    double hd = (double)hh - M.rdy[j] * (wfu * sgsj2 * ww22 - wfd * sgsj1 * ww11);
     const double dzp = (j < g.n2m) ? F.dvdz[x.bp] : 0.0;
-    hd = hd - 1.0 / M.dy[j] * (wfu * sgsj2 * dzp - wfd * sgsj1 * F.dvdz[x.b]);
+    hd = hd - M.rdy[j] * (wfu * sgsj2 * dzp - wfd * sgsj1 * F.dvdz[x.b]);
     hh = (T)hd;
   }
   ((T*)F.hh[2])[x.b] = hh;
