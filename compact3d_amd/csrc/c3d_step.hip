@@ -806,6 +806,291 @@ __device__ __forceinline__ void fft_lines(cplx_t<T>* a, int n, int nl, int ls, i
   }
 }
 
+// ------------------------------------------- two-pass register FFT
+// For n = N1*N2 in {64 = 8*8, 128 = 8*16, 256 = 16*16} a line is split as
+// p = a + N1*b: pass A (thread a of the line) takes the N2 points
+// x[a + N1 b] from HBM (or the caller) into registers, transforms over b,
+// multiplies by W_n^(a k2), and writes Y[a][k2] to LDS; after one barrier
+// pass B (thread c) reads Y[.][k2] for its k2 = c, c + N1, ... and
+// transforms over a, giving X[k2 + N2 k1].  One LDS exchange per transform
+// (the Stockham kernels above make a round trip per radix stage).
+
+// DFT of 4 values in place (x[m] at v[s*m], natural order out)
+template <typename T>
+__device__ __forceinline__ void rdft4(cplx_t<T>& v0, cplx_t<T>& v1, cplx_t<T>& v2, cplx_t<T>& v3, int sign) {
+  const cplx_t<T> a0 = {v0.x + v2.x, v0.y + v2.y};
+  const cplx_t<T> a1 = {v0.x - v2.x, v0.y - v2.y};
+  const cplx_t<T> a2 = {v1.x + v3.x, v1.y + v3.y};
+  const cplx_t<T> a3 = {v1.x - v3.x, v1.y - v3.y};
+  const cplx_t<T> a3r = (sign < 0) ? cplx_t<T>{a3.y, -a3.x} : cplx_t<T>{-a3.y, a3.x};
+  v0 = {a0.x + a2.x, a0.y + a2.y};
+  v2 = {a0.x - a2.x, a0.y - a2.y};
+  v1 = {a1.x + a3r.x, a1.y + a3r.y};
+  v3 = {a1.x - a3r.x, a1.y - a3r.y};
+}
+// exp(sign 2 pi i k / 16), k = 0..15
+template <typename T>
+__device__ __forceinline__ cplx_t<T> w16(int k, int sign) {
+  const T c[5] = {(T)1.0, (T)0.92387953251128675613, (T)0.70710678118654752440, (T)0.38268343236508977173,
+                  (T)0.0};
+  // cos and sin of k*pi/8 from the first quadrant
+  const int q = k & 3, h = (k >> 2) & 3;
+  T cr = c[q], si = c[4 - q];
+  T x, y;
+  if (h == 0) { x = cr; y = si; }
+  else if (h == 1) { x = -si; y = cr; }
+  else if (h == 2) { x = -cr; y = -si; }
+  else { x = si; y = -cr; }
+  return {x, sign < 0 ? -y : y};
+}
+// in-register DFT of N = 8 or 16 values, natural order in and out
+template <typename T, int N>
+__device__ __forceinline__ void rdft(cplx_t<T> (&v)[N], int sign) {
+  static_assert(N == 8 || N == 16, "register DFT of 8 or 16 points");
+  constexpr int R = N / 4;  // n = r + R m: R DFT4s over m, then R-point DFTs over r
+  cplx_t<T> y[N];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    cplx_t<T> t0 = v[r], t1 = v[r + R], t2 = v[r + 2 * R], t3 = v[r + 3 * R];
+    rdft4(t0, t1, t2, t3, sign);
+    y[r * 4 + 0] = t0;
+    y[r * 4 + 1] = t1;
+    y[r * 4 + 2] = t2;
+    y[r * 4 + 3] = t3;
+  }
+  // twiddles W_N^(r s) = W_16^(r s 16/N)
+#pragma unroll
+  for (int r = 1; r < R; r++)
+#pragma unroll
+    for (int sx = 1; sx < 4; sx++) y[r * 4 + sx] = cmul(y[r * 4 + sx], w16<T>((r * sx * (16 / N)) & 15, sign));
+  // X[s + 4 t] = sum_r y[r][s] W_R^(r t)
+#pragma unroll
+  for (int sx = 0; sx < 4; sx++) {
+    if (R == 4) {
+      cplx_t<T> t0 = y[sx], t1 = y[4 + sx], t2 = y[8 + sx], t3 = y[12 + sx];
+      rdft4(t0, t1, t2, t3, sign);
+      v[sx] = t0;
+      v[sx + 4] = t1;
+      v[sx + 8] = t2;
+      v[sx + 12] = t3;
+    } else {
+      const cplx_t<T> a = y[sx], b = y[4 + sx];
+      v[sx] = {a.x + b.x, a.y + b.y};
+      v[sx + 4] = {a.x - b.x, a.y - b.y};
+    }
+  }
+}
+
+template <int N>
+struct RFFT;
+template <>
+struct RFFT<64> {
+  static constexpr int N1 = 8, N2 = 8;
+};
+template <>
+struct RFFT<128> {
+  static constexpr int N1 = 8, N2 = 16;
+};
+template <>
+struct RFFT<256> {
+  static constexpr int N1 = 16, N2 = 16;
+};
+
+// LDS slot of point p of line l.  Column layout (z-lines, the line index l
+// fastest across a wavefront): p*(L+1) + l.  Row layout (x-lines, the
+// point index fastest): l*(N + N/N1) + p + p/N1, one pad slot per N1
+// points so that pass B's lanes (stride N1 points) fall on distinct banks.
+template <int N, bool ROW>
+__device__ __forceinline__ int rslot(int p, int l) {
+  constexpr int N1 = RFFT<N>::N1, L = 256 / N1;
+  return ROW ? l * (N + N / N1) + p + p / N1 : p * (L + 1) + l;
+}
+template <int N, bool ROW>
+constexpr size_t rfft_lds() {
+  return sizeof(double) * 2 * (size_t)(256 / RFFT<N>::N1) * (size_t)(N + N / RFFT<N>::N1 + 1) +
+         sizeof(double) * 2 * N;
+}
+
+// pass A over the values v[b] = x[a + N1 b] of this thread: transform,
+// twiddle, store Y[a][k2] at the slot of point a + N1 k2
+template <typename T, int N, bool ROW>
+__device__ __forceinline__ void rfft_a(cplx_t<T> (&v)[RFFT<N>::N2], int a, int l, cplx_t<T>* A,
+                                       const cplx_t<T>* TW, int sign) {
+  constexpr int N1 = RFFT<N>::N1, N2 = RFFT<N>::N2;
+  rdft<T, N2>(v, sign);
+#pragma unroll
+  for (int k2 = 1; k2 < N2; k2++) v[k2] = cmul(v[k2], TW[a * k2]);
+#pragma unroll
+  for (int k2 = 0; k2 < N2; k2++) A[rslot<N, ROW>(a + N1 * k2, l)] = v[k2];
+}
+// pass B for k2 = c + N1 m (m < N2/N1): w[m][k1] = X[k2 + N2 k1]
+template <typename T, int N, bool ROW>
+__device__ __forceinline__ void rfft_b(cplx_t<T> (&w)[RFFT<N>::N2 / RFFT<N>::N1][RFFT<N>::N1], int c, int l,
+                                       const cplx_t<T>* A, int sign) {
+  constexpr int N1 = RFFT<N>::N1, N2 = RFFT<N>::N2, M = N2 / N1;
+#pragma unroll
+  for (int m = 0; m < M; m++) {
+    const int k2 = c + N1 * m;
+#pragma unroll
+    for (int a = 0; a < N1; a++) w[m][a] = A[rslot<N, ROW>(a + N1 * k2, l)];
+    rdft<T, N1>(w[m], sign);
+  }
+}
+
+// z-DFT of the half spectrum (as k_fft_z) with the register FFT: L columns
+// kx per block, N1 threads per column, the N2 values of pass A read from
+// HBM and the results of pass B written to HBM directly
+template <typename T, int N, bool CB_>
+__global__ __launch_bounds__(256) void k_rfft_z(CtxArg<CB_> ca_, int sign) {
+  const Ctx c = ctx_get(ca_);
+  constexpr int N1 = RFFT<N>::N1, N2 = RFFT<N>::N2, M = N2 / N1;
+  constexpr int L = 256 / N1;
+  extern __shared__ __align__(16) unsigned char smem[];
+  typedef cplx_t<T> zc;
+  const Geo g = c.g;
+  zc* A = (zc*)smem;
+  zc* TW = A + (size_t)L * (N + N / N1 + 1);
+  {
+    const zc* twg = (const zc*)(sign < 0 ? c.f.twzf : c.f.twzb);
+    for (int e = threadIdx.x; e < N; e += 256) TW[e] = twg[e];
+  }
+  const int nit = (g.nxh + L - 1) / L;
+  const int it = blockIdx.x % nit;
+  const int jm = blockIdx.x / nit;
+  const int l = threadIdx.x % L, a = threadIdx.x / L;
+  const int kx = it * L + l;
+  const bool ok = kx < g.nxh;
+  zc* S = (zc*)c.d.spec + (long long)jm * g.nz * g.nxhp + (ok ? kx : 0);
+  const unsigned rs = (unsigned)g.nxhp;
+  zc v[N2];
+#pragma unroll
+  for (int b = 0; b < N2; b++) v[b] = S[(unsigned)(a + N1 * b) * rs];
+  __syncthreads();  // the twiddle table
+  rfft_a<T, N, false>(v, a, l, A, TW, sign);
+  __syncthreads();
+  zc w[M][N1];
+  rfft_b<T, N, false>(w, a, l, A, sign);
+  if (ok)
+#pragma unroll
+    for (int m = 0; m < M; m++)
+#pragma unroll
+      for (int k1 = 0; k1 < N1; k1++) S[(unsigned)(a + N1 * m + N2 * k1) * rs] = w[m][k1];
+}
+
+// forward real x-DFT of RDP rows (as k_fft_x) with the register FFT: L
+// complex lines (2L rows) per block; pass A reads x[a + N1 b] of both rows
+// straight from HBM, the post-processing pairs X[kx] with X[n-kx] through
+// LDS
+template <typename T, int N, bool CB_>
+__global__ __launch_bounds__(256) void k_rfft_x(CtxArg<CB_> ca_) {
+  const Ctx c = ctx_get(ca_);
+  constexpr int N1 = RFFT<N>::N1, N2 = RFFT<N>::N2, M = N2 / N1;
+  constexpr int L = 256 / N1;
+  extern __shared__ __align__(16) unsigned char smem[];
+  typedef cplx_t<T> zc;
+  const Geo g = c.g;
+  zc* A = (zc*)smem;
+  zc* TW = A + (size_t)L * (N + N / N1 + 1);
+  for (int e = threadIdx.x; e < N; e += 256) TW[e] = ((const zc*)c.f.twxf)[e];
+  const int nlines = g.n2m * g.nz;
+  const int ln0 = blockIdx.x * 2 * L;
+  // thread (a, l): a fastest, so a wavefront reads 16 or 8 consecutive points
+  const int a = threadIdx.x % N1, l = threadIdx.x / N1;
+  const int la = ln0 + l, lb = ln0 + L + l;
+  const T* src = (const T*)c.d.rdp;
+  const unsigned oa = la < nlines ? (unsigned)gidx(g, 0, 1 + la % g.n2m, la / g.n2m) : 0u;
+  const unsigned ob = lb < nlines ? (unsigned)gidx(g, 0, 1 + lb % g.n2m, lb / g.n2m) : 0u;
+  zc v[N2];
+#pragma unroll
+  for (int b = 0; b < N2; b++) {
+    const unsigned p = (unsigned)(a + N1 * b);
+    v[b] = {la < nlines ? src[oa + p] : (T)0, lb < nlines ? src[ob + p] : (T)0};
+  }
+  __syncthreads();  // the twiddle table
+  rfft_a<T, N, true>(v, a, l, A, TW, -1);
+  __syncthreads();
+  zc w[M][N1];
+  rfft_b<T, N, true>(w, a, l, A, -1);
+  __syncthreads();  // every pass-B read is done before the results overwrite A
+#pragma unroll
+  for (int m = 0; m < M; m++)
+#pragma unroll
+    for (int k1 = 0; k1 < N1; k1++) A[rslot<N, true>(a + N1 * m + N2 * k1, l)] = w[m][k1];
+  __syncthreads();
+  const int nxh = g.nxh;
+  zc* S = (zc*)c.d.spec;
+  for (int e = threadIdx.x; e < L * nxh; e += 256) {
+    const int ll = e / nxh, kk = e - ll * nxh;
+    const int ea = ln0 + ll, eb = ln0 + L + ll;
+    const zc z1 = A[rslot<N, true>(kk, ll)];
+    const zc z2 = A[rslot<N, true>(kk == 0 ? 0 : N - kk, ll)];
+    const zc va = {(T)0.5 * (z1.x + z2.x), (T)0.5 * (z1.y - z2.y)};
+    const zc vb = {(T)0.5 * (z1.y + z2.y), (T)-0.5 * (z1.x - z2.x)};
+    if (ea < nlines) S[((long long)(ea % g.n2m) * g.nz + ea / g.n2m) * g.nxhp + kk] = va;
+    if (eb < nlines) S[((long long)(eb % g.n2m) * g.nz + eb / g.n2m) * g.nxhp + kk] = vb;
+  }
+}
+
+// inverse x-DFT (as k_fft_xinv) with the register FFT: pass A forms its
+// inputs (Hermitian fill, two rows per complex line) from the half spectrum
+// in HBM, pass B writes the real and imaginary parts / (N1M N3M) to DP
+template <typename T, int N, bool CB_>
+__global__ __launch_bounds__(256) void k_rfft_xinv(CtxArg<CB_> ca_) {
+  const Ctx c = ctx_get(ca_);
+  constexpr int N1 = RFFT<N>::N1, N2 = RFFT<N>::N2, M = N2 / N1;
+  constexpr int L = 256 / N1;
+  extern __shared__ __align__(16) unsigned char smem[];
+  typedef cplx_t<T> zc;
+  const Geo g = c.g;
+  zc* A = (zc*)smem;
+  zc* TW = A + (size_t)L * (N + N / N1 + 1);
+  for (int e = threadIdx.x; e < N; e += 256) TW[e] = ((const zc*)c.f.twxb)[e];
+  const int nlines = g.n2m * g.nz;
+  const int ln0 = blockIdx.x * 2 * L;
+  const int a = threadIdx.x % N1, l = threadIdx.x / N1;
+  const int la = ln0 + l, lb = ln0 + L + l;
+  const zc* S = (const zc*)c.d.spec;
+  const int nxh = g.nxh;
+  const long long sa = la < nlines ? ((long long)(la % g.n2m) * g.nz + la / g.n2m) * g.nxhp : -1;
+  const long long sb = lb < nlines ? ((long long)(lb % g.n2m) * g.nz + lb / g.n2m) * g.nxhp : -1;
+  zc v[N2];
+#pragma unroll
+  for (int b = 0; b < N2; b++) {
+    const int kx = a + N1 * b;
+    const bool lo = kx < nxh;
+    const int kk = lo ? kx : N - kx;
+    zc xa = sa >= 0 ? S[sa + kk] : zc{(T)0, (T)0};
+    zc xb = sb >= 0 ? S[sb + kk] : zc{(T)0, (T)0};
+    if (kx == 0 || 2 * kx == N) {
+      xa.y = (T)0;
+      xb.y = (T)0;
+    }
+    if (!lo) {
+      xa.y = -xa.y;
+      xb.y = -xb.y;
+    }
+    v[b] = {xa.x - xb.y, xa.y + xb.x};
+  }
+  __syncthreads();  // the twiddle table
+  rfft_a<T, N, true>(v, a, l, A, TW, +1);
+  __syncthreads();
+  zc w[M][N1];
+  rfft_b<T, N, true>(w, a, l, A, +1);
+  const T norm = (T)(1.0 / ((double)g.nx * (double)g.nz));
+  double* dp = c.d.dp;
+  const unsigned oa = la < nlines ? (unsigned)gidx(g, 0, 1 + la % g.n2m, la / g.n2m) : 0u;
+  const unsigned ob = lb < nlines ? (unsigned)gidx(g, 0, 1 + lb % g.n2m, lb / g.n2m) : 0u;
+#pragma unroll
+  for (int m = 0; m < M; m++)
+#pragma unroll
+    for (int k1 = 0; k1 < N1; k1++) {
+      const unsigned p = (unsigned)(a + N1 * m + N2 * k1);
+      const zc z = w[m][k1];
+      if (la < nlines) dp[oa + p] = z.x * norm;
+      if (lb < nlines) dp[ob + p] = z.y * norm;
+    }
+}
+
 // forward x-DFT of RDP rows (J=1..N2M, K), 2*TL rows per block; line
 // stride TL+1 in LDS so that the row-wise fill and drain hit distinct banks.
 // The FFT kernels copy their twiddle table (n complex values) into LDS
@@ -1495,8 +1780,36 @@ static void getdp_fp64(const Ctx& c) {
     else                                                                                \
       C3D_LAUNCH(K, (T, 1024, 256), __VA_ARGS__);                                       \
   } while (0)
-  { KScope ks_(c, KID_FFT_X); FFTW_(k_fft_x, ex, nx_t, dim3(gx), dim3(nx_t), ldsx, c); }
-  { KScope ks_(c, KID_FFT_Z); FFTW_(k_fft_z, ez, nz_t, dim3(gz), dim3(nz_t), ldsz, c, -1); }
+  // the register FFT for line lengths 64, 128, 256 (256 threads, L lines or
+  // columns per block)
+  const int rx = (g.nx == 64 || g.nx == 128 || g.nx == 256) ? g.nx : 0;
+  const int rz = (g.nz == 64 || g.nz == 128 || g.nz == 256) ? g.nz : 0;
+  const int lx = rx ? 256 / (rx == 256 ? 16 : 8) : 0, lzr = rz ? 256 / (rz == 256 ? 16 : 8) : 0;
+  const int gxr = rx ? (g.n2m * g.nz + 2 * lx - 1) / (2 * lx) : 0;
+  const int gzr = rz ? ((g.nxh + lzr - 1) / lzr) * g.n2m : 0;
+#define RF_(K, n, GR, ...)                                                                                  \
+  do {                                                                                                      \
+    if ((n) == 256)                                                                                         \
+      C3D_LAUNCH(K, (T, 256), dim3(GR), dim3(256), (rfft_lds<256, true>()), c, ##__VA_ARGS__);            \
+    else if ((n) == 128)                                                                                    \
+      C3D_LAUNCH(K, (T, 128), dim3(GR), dim3(256), (rfft_lds<128, true>()), c, ##__VA_ARGS__);            \
+    else                                                                                                    \
+      C3D_LAUNCH(K, (T, 64), dim3(GR), dim3(256), (rfft_lds<64, true>()), c, ##__VA_ARGS__);              \
+  } while (0)
+  {
+    KScope ks_(c, KID_FFT_X);
+    if (rx)
+      RF_(k_rfft_x, rx, gxr);
+    else
+      FFTW_(k_fft_x, ex, nx_t, dim3(gx), dim3(nx_t), ldsx, c);
+  }
+  {
+    KScope ks_(c, KID_FFT_Z);
+    if (rz)
+      RF_(k_rfft_z, rz, gzr, -1);
+    else
+      FFTW_(k_fft_z, ez, nz_t, dim3(gz), dim3(nz_t), ldsz, c, -1);
+  }
   {
     // tiles of up to 256 or 512 threads (16 modes: spectrum rows of 256 bytes)
     KScope ks_(c, KID_POIS_Y);
@@ -1511,8 +1824,21 @@ static void getdp_fp64(const Ctx& c) {
     }
 #undef PY_
   }
-  { KScope ks_(c, KID_FFT_Z); FFTW_(k_fft_z, ez, nz_t, dim3(gz), dim3(nz_t), ldsz, c, +1); }
-  { KScope ks_(c, KID_FFT_XINV); FFTW_(k_fft_xinv, ex, nx_t, dim3(gx), dim3(nx_t), ldsx, c); }
+  {
+    KScope ks_(c, KID_FFT_Z);
+    if (rz)
+      RF_(k_rfft_z, rz, gzr, +1);
+    else
+      FFTW_(k_fft_z, ez, nz_t, dim3(gz), dim3(nz_t), ldsz, c, +1);
+  }
+  {
+    KScope ks_(c, KID_FFT_XINV);
+    if (rx)
+      RF_(k_rfft_xinv, rx, gxr);
+    else
+      FFTW_(k_fft_xinv, ex, nx_t, dim3(gx), dim3(nx_t), ldsx, c);
+  }
+#undef RF_
 #undef FFTW_
 }
 void getdp(const Ctx& c) { getdp_fp64(c); }
@@ -1532,6 +1858,17 @@ void fft_attrs() {
   FA_(k_fft_z);
   FA_(k_fft_xinv);
 #undef FA_
+#define FR_(K)                                                                                          \
+  (void)hipFuncSetAttribute((const void*)&K<double, 256, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim); \
+  (void)hipFuncSetAttribute((const void*)&K<double, 128, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim); \
+  (void)hipFuncSetAttribute((const void*)&K<double, 64, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);  \
+  (void)hipFuncSetAttribute((const void*)&K<double, 256, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);  \
+  (void)hipFuncSetAttribute((const void*)&K<double, 128, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);  \
+  (void)hipFuncSetAttribute((const void*)&K<double, 64, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim)
+  FR_(k_rfft_x);
+  FR_(k_rfft_z);
+  FR_(k_rfft_xinv);
+#undef FR_
 }
 
 void pois_table(const Ctx& c) {
