@@ -1414,6 +1414,94 @@ __global__ __launch_bounds__(256) void k_upd_x(CtxArg<CB_> ca_, int chk, int ref
   X.out(F.div, d);
 }
 
+// Mixed mode splits the x-line update: k_upd_xu updates U (x-lines) and
+// stores D1x of the new U in DIV; k_upd_v updates V along y-columns and
+// adds (V(J+1) - V(J))/DY to DIV, each new V(J) formed once and carried to
+// the next J (the x-line form re-reads the J+1 rows of DELTA_v, V and DP).
+// Same expressions and operation order as k_upd_x.
+template <int SEG, bool CB_>
+__global__ __launch_bounds__(256) void k_upd_xu(CtxArg<CB_> ca_, int refine) {
+  const Ctx c = ctx_get(ca_);
+  typedef double T;
+  XSETUP;
+  const Scal s = c.s;
+  const Metrics M = c.m;
+  const Fields F = c.d;
+  const T dx1 = (T)s.dx1, dt = (T)s.dt;
+  const T dmp = refine ? (T)0 : (T)F.sc[SC_DMPRESG];
+  const T ku = refine ? (T)0 : (T)1;
+  T d[SEG], un[SEG], pf[SEG], pg[SEG];
+  X.pre(F.dp, 0, pf);
+  const T* B = X.put(pf);
+  X.pre((const float*)F.uh[0], 0, pf);
+  X.pre(F.u, 0, pg);
+  SQ d[q] = (T)24. * dx1 * (-XA(B, q - 1) + XA(B, q));
+  psolve<T, SEG>(d, G, L, l, gs, c.o.px.s22, bf.xf, bf.xb);
+  B = X.put(pf);
+  SQ un[q] = ku * XA(B, q) - dt * d[q] - dt * dmp;
+  // the flow rate of U before the update (FLOW1, :1081)
+  double so = 0.0;
+  B = X.put(pg);
+  SQ {
+    const T uo = XA(B, q);
+    so += (double)uo;
+    un[q] = uo + un[q];
+  }
+  X.out2(F.u, F.uf, un);
+  block_sum2(so * (ok ? M.dy[j] : 0.0), 0.0, F.flowp + FP_U * F.flowr + blockIdx.x, nullptr);
+  const T rt = xch_right(un[0], G, L, l, gs, bf.xr);
+  SQ d[q] = (T)24. * dx1 * (nxt(un, q, rt) - un[q]);
+  psolve<T, SEG>(d, G, L, l, gs, c.o.px.s22, bf.xf, bf.xb);
+  X.out(F.div, d);
+}
+
+// y-columns: thread (i, k, chunk) walks J = J0..J1 of one column (i fastest
+// across threads: rows are coalesced); V(J) = VOLD(J) + (DELTA_v(J) -
+// DT*(DP(J)-DP(J-1))/H(J)) for J >= 2 (V(1) is the wall's and left to the
+// ghost pass), DIV(J) = (V(J+1) - V(J))/DY(J) + DIV(J) with V(N2) = 0
+template <bool CB_>
+__global__ __launch_bounds__(256) void k_upd_v(CtxArg<CB_> ca_, int refine, int ch) {
+  const Ctx c = ctx_get(ca_);
+  const Geo g = c.g;
+  const Metrics M = c.m;
+  const Fields F = c.d;
+  const double dt = c.s.dt;
+  const double ku = refine ? 0.0 : 1.0;
+  const unsigned ncol = (unsigned)g.nx * (unsigned)g.nz;
+  const unsigned t = blockIdx.x * 256u + threadIdx.x;
+  const unsigned nch = (unsigned)((g.n2m + ch - 1) / ch);
+  if (t >= ncol * nch) return;
+  const unsigned col = t % ncol, cc = t / ncol;
+  const unsigned i = col % (unsigned)g.nx, k = col / (unsigned)g.nx;
+  const int J0 = 1 + (int)cc * ch, J1 = min(J0 + ch - 1, g.n2m);
+  const unsigned nx = (unsigned)g.nx, base = k * (unsigned)g.plane + i;
+  const float* dvr = (const float*)F.uh[1];
+  const double* vo = F.vold;
+  // V(J) of the new field from DP(J-1), DP(J)
+  auto vnew = [&](int J, double dpj, double dpm) -> double {
+    const double d = ku * (double)dvr[base + (unsigned)J * nx] - dt * (dpj - dpm) / M.h[J];
+    return vo[base + (unsigned)J * nx] + d;
+  };
+  double dpm = F.dp[base + (unsigned)(J0 - 1) * nx];
+  double dpj = F.dp[base + (unsigned)J0 * nx];
+  double vj = J0 >= 2 ? vnew(J0, dpj, dpm) : 0.0;
+  for (int J = J0; J <= J1; J++) {
+    const unsigned a = base + (unsigned)J * nx;
+    double vp = 0.0, dpp = 0.0;
+    if (J + 1 <= g.n2m) {
+      dpp = F.dp[a + nx];
+      vp = vnew(J + 1, dpp, dpj);
+    }
+    if (J >= 2) {
+      F.v[a] = vj;
+      F.vf[a] = (float)vj;
+    }
+    F.div[a] = (vp - vj) / M.dy[J] + F.div[a];
+    vj = vp;
+    dpj = dpp;
+  }
+}
+
 // z-lines: U3 = UH3 - DT*D1z(DP) - DT*DMPRESG3 (:1193-1208); P += DP
 // (:1218-1230); with `chk`: DIV += D1z(U3), DIVMAX, and the CFL maximum of
 // the new field (:9112-9128, :9150-9162).  Mixed mode also stores the full
@@ -1899,10 +1987,17 @@ void flow_fin(const Ctx& c) {
   C3D_LAUNCH0(k_flow_fin, dim3(1), dim3(FIN_THREADS), 0, c, nu, xblocks(c), c.s.mixed ? 0 : 1);
 }
 void upd_x(const Ctx& c, bool chk, bool refine) {
-  if (c.s.mixed)
-    upd_x_t<float>(c, chk, refine ? 1 : 0);
-  else
+  if (c.s.mixed) {
+    // U on x-lines, then V on y-columns (chunks of up to 16 J)
+    const size_t lx = lds_line(c.o.lx, true, sizeof(double));
+    XSEG(c.o.lx.seg, { KScope ks_(c, KID_UPD_X); C3D_LAUNCH(k_upd_xu, (S_), dim3(xblocks(c)), xthreads(c), lx, c, refine ? 1 : 0); });
+    const int ch = 16, nch = (c.g.n2m + ch - 1) / ch;
+    const long long nt = (long long)c.g.nx * c.g.nz * nch;
+    KScope ks_(c, KID_UPD_V);
+    C3D_LAUNCH0(k_upd_v, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c, refine ? 1 : 0, ch);
+  } else {
     upd_x_t<double>(c, chk, 0);
+  }
 }
 void upd_z(const Ctx& c, bool chk, bool refine) {
   if (c.s.mixed)
