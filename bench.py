@@ -53,7 +53,10 @@ ALG_BYTES_ADI_X_PER_STEP = 56
 ALG_BYTES_MIXED = {
     "rhs_x": 44, "rhs_z": 56, "rhs_zw": 20, "rhs_x2": 12, "adi_y": 84, "adi_z": 24,
     "rhsdp_x": 16, "rhsdp_z": 28, "fft_x": 16, "fft_z": 16, "pois_y": 24, "fft_xinv": 16,
-    "upd_x": 64, "upd_z": 92,
+    # the update: U on x-lines (DP, DELTA_u, U in; U, its shadow, D1x U out),
+    # V on y-columns (V, DELTA_v, DP, DIV in; V, its shadow, DIV out), then
+    # W and P on z-lines with DIV, DIVMAX and CFL
+    "upd_x": 40, "upd_v": 48, "upd_z": 92,
 }
 
 
@@ -481,7 +484,8 @@ def ensemble_leg(args):
 KERNEL_KEYS = {"k_rhsy_adiy": "adi_y", "k_rhs_x": "rhs_x", "k_rhs_x2": "rhs_x2", "k_rhs_z": "rhs_z",
                "k_rhs_zw": "rhs_zw", "k_adi_x": "adi_x", "k_adi_z": "adi_z", "k_rhsdp_x": "rhsdp_x",
                "k_rhsdp_z": "rhsdp_z", "k_fft_x": "fft_x", "k_fft_z": "fft_z", "k_pois_y": "pois_y",
-               "k_fft_xinv": "fft_xinv", "k_upd_x": "upd_x", "k_upd_z": "upd_z"}
+               "k_fft_xinv": "fft_xinv", "k_rfft_x": "fft_x", "k_rfft_z": "fft_z", "k_rfft_xinv": "fft_xinv",
+               "k_upd_x": "upd_x", "k_upd_xu": "upd_x", "k_upd_v": "upd_v", "k_upd_z": "upd_z"}
 # adi_x runs twice per step; its per-launch algorithmic bytes are the mean
 ADI_X_ALG = {"fp64": ALG_BYTES_ADI_X_PER_STEP / 2.0, "mixed": 12.0}
 
