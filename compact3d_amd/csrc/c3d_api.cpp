@@ -617,9 +617,10 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   // its arithmetic), 16 columns (128-byte rows), up to 512 threads
   o.ys = make_yshape(g.n2m, 512, 16, 6);
   o.ys32 = make_yshape(g.n2m, 512, 32);
-  // Poisson: 8 modes of 16 bytes per row and chunks of 6 points keep its
-  // three FP64 chunk arrays small; 12-point chunks beyond N2-1 = 1536
-  o.yp = g.n2m <= 6 * 256 ? make_yshape(g.n2m, 256, 8, 6) : make_yshape(g.n2m, 256, 16);
+  // Poisson: 16 columns (8 modes of 16 bytes per row at the two kz of a
+  // pair) and chunks of 6 points keep its three FP64 chunk arrays small;
+  // 12-point chunks beyond N2-1 = 1536 (at least 2 columns up to 3072)
+  o.yp = g.n2m <= 6 * 256 ? make_yshape(g.n2m, 512, 16, 6) : make_yshape(g.n2m, 512, 16);
   o.s6 = make_solve(1.0, 6.0);
   o.s10 = make_solve(1.0, 10.0);
   o.s22 = make_solve(1.0, 22.0);
@@ -784,7 +785,7 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   }
   c3dk::init_attrs();
   {
-    const size_t nmj = (size_t)g.nxhp * g.nz * g.n2m;
+    const size_t nmj = (size_t)g.nxhp * (g.nz / 2 + 1) * g.n2m;
     if (dalloc(h, &F.pib, nmj) || dalloc(h, &F.pgt, nmj)) return bad(h->err);
     c3dk::pois_table(c);
     if (hipStreamSynchronize(c.st) != hipSuccess || hipGetLastError() != hipSuccess)
@@ -1272,16 +1273,18 @@ static std::vector<YShape> cand_y(int m, int row, int ntmax) {
     }
   return v;
 }
-// Poisson y-solve tiles: <= 12-point chunks, 8 or 16 modes, <= 512 threads
+// Poisson y-solve tiles: an even number of columns (modes x the two kz of
+// a pair), <= 12-point chunks in <= 512 threads or 6-point chunks in 1024
 static std::vector<YShape> cand_yp(int m) {
   std::vector<YShape> yp;
   for (int ch : {6, 12})
-    for (int nt : {256, 512})
-      for (int row : {8, 16}) {
+    for (int nt : {512, 1024})
+      for (int row : {16, 32}) {
+        if (ch > 6 && nt > 512) continue;
         const YShape y = make_yshape(m, nt, row, ch);
         bool dup = false;
         for (const YShape& w : yp) dup = dup || (w.L == y.L && w.C == y.C);
-        if (!dup && y.L * y.C <= nt && (m + y.C - 1) / y.C <= 12) yp.push_back(y);
+        if (!dup && y.L % 2 == 0 && y.L * y.C <= nt && (m + y.C - 1) / y.C <= ch) yp.push_back(y);
       }
   return yp;
 }

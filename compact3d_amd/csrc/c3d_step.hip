@@ -282,6 +282,78 @@ __device__ __forceinline__ void ychain_bwd(T (&v)[N], MF m, const YPos& y, int C
     }
 }
 
+// the same two recurrences on two arrays with one multiplier sequence (the
+// real and imaginary parts of a Poisson mode): one exchange, one product per
+// chunk, and the two cross-chunk chains interleaved
+template <typename T, int N, typename MF>
+__device__ __forceinline__ void ychain_fwd2(T (&v)[N], T (&w)[N], MF m, const YPos& y, int L, T* xa, T* xb,
+                                            T* xm) {
+  T r = (T)0, s = (T)0, P = (T)1;
+#pragma unroll
+  for (int q = 0; q < N; q++)
+    if (q < y.len) {
+      const T mq = m(q);
+      r = v[q] + mq * r;
+      s = w[q] + mq * s;
+      v[q] = r;
+      w[q] = s;
+      P *= mq;
+    }
+  xa[y.c * L + y.l] = r;
+  xb[y.c * L + y.l] = s;
+  xm[y.c * L + y.l] = P;
+  __syncthreads();
+  T Y = (T)0, Z = (T)0;
+  for (int cc = 0; cc < y.c; cc++) {
+    const T mc = xm[cc * L + y.l];
+    Y = xa[cc * L + y.l] + mc * Y;
+    Z = xb[cc * L + y.l] + mc * Z;
+  }
+#pragma unroll
+  for (int q = 0; q < N; q++)
+    if (q < y.len) {
+      const T mq = m(q);
+      Y *= mq;
+      Z *= mq;
+      v[q] += Y;
+      w[q] += Z;
+    }
+}
+template <typename T, int N, typename MF>
+__device__ __forceinline__ void ychain_bwd2(T (&v)[N], T (&w)[N], MF m, const YPos& y, int C, int L, T* xa,
+                                            T* xb, T* xm) {
+  T r = (T)0, s = (T)0, P = (T)1;
+#pragma unroll
+  for (int q = N - 1; q >= 0; q--)
+    if (q < y.len) {
+      const T mq = m(q);
+      r = v[q] + mq * r;
+      s = w[q] + mq * s;
+      v[q] = r;
+      w[q] = s;
+      P *= mq;
+    }
+  xa[y.c * L + y.l] = r;
+  xb[y.c * L + y.l] = s;
+  xm[y.c * L + y.l] = P;
+  __syncthreads();
+  T Y = (T)0, Z = (T)0;
+  for (int cc = C - 1; cc > y.c; cc--) {
+    const T mc = xm[cc * L + y.l];
+    Y = xa[cc * L + y.l] + mc * Y;
+    Z = xb[cc * L + y.l] + mc * Z;
+  }
+#pragma unroll
+  for (int q = N - 1; q >= 0; q--)
+    if (q < y.len) {
+      const T mq = m(q);
+      Y *= mq;
+      Z *= mq;
+      v[q] += Y;
+      w[q] += Z;
+    }
+}
+
 template <typename T, int NT>
 __global__ __launch_bounds__(NT) void k_adi_y(Ctx c) {
   __shared__ T xs[4][NT];  // C*L <= NT entries each
@@ -1177,8 +1249,11 @@ __global__ __launch_bounds__(NT) void k_fft_z(CtxArg<CB_> ca_, int sign) {
 // identity with zero RHS; imaginary parts of the self-conjugate modes
 // (kx in {0, NX/2}) x (kz in {0, NZ/2}) are zero.  The elimination factors
 // depend on the mode only: IBET(J) = 1/BET(J) and GAM(J+1) are tabulated at
-// set-up (k_pois_table); here each thread (l, c) solves chunk c of mode
-// kx = kx0 + l by the partitioned recurrences.
+// set-up (k_pois_table) for kz = 0..NZ/2, since AK3 is even in kz and the
+// factors of kz and NZ-kz are the same numbers.  A block solves L/2 modes kx
+// at the two kz of a pair (kz, NZ-kz) (pair 0: kz = 0 and NZ/2), so the lanes
+// of the two halves read each factor row in the same load; thread (l, c)
+// solves chunk c of column l by the partitioned recurrences.
 // Spectrum in TI, recurrences and result in TO (FP64 in both modes); chunks
 // of at most CH points.
 template <typename TI, typename TO, int CH, int NT, bool CB_>
@@ -1186,12 +1261,12 @@ __global__ __launch_bounds__(NT) void k_pois_y(CtxArg<CB_> ca_) {
   const Ctx c = ctx_get(ca_);
   const void* spec_in = (const void*)c.d.spec;
   void* spec_out = (void*)c.d.spec;
-  __shared__ TO xs[8][NT];
+  __shared__ TO xs[6][NT];
   typedef cplx_t<TI> zi;
   typedef cplx_t<TO> zo;
   const Geo g = c.g;
   const YShape ys = c.o.yp;
-  const int L = ys.L, C = ys.C;
+  const int L = ys.L, C = ys.C, LH = L >> 1;  // L columns: LH modes x 2 kz
   const int t = threadIdx.x, l = t % L;
   YPos y;
   y.l = l;
@@ -1202,14 +1277,19 @@ __global__ __launch_bounds__(NT) void k_pois_y(CtxArg<CB_> ca_) {
   // XCD-aware tile order: the FP64 factor rows of 8 modes are half a
   // 128-byte line, shared with the neighbouring tile, which then runs on the
   // same L2
-  const int nit = (g.nxh + L - 1) / L;
+  const int nit = (g.nxh + LH - 1) / LH;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int it = tile % nit;
-  const int kz = tile / nit;
-  const int kx = it * L + l;
+  const int pz = tile / nit;  // the pair (pz, NZ - pz); pair 0 is (0, NZ/2)
+  const int half = l >= LH;
+  const int kz = half ? (pz == 0 ? g.nz / 2 : g.nz - pz) : pz;
+  const int kk = (half && pz == 0) ? g.nz / 2 : pz;  // factor row
+  const int kx = it * LH + l - half * LH;
   const bool ok = kx < g.nxh;
   const long long jstr = (long long)g.nz * g.nxhp;
+  const long long tstr = (long long)(g.nz / 2 + 1) * g.nxhp;
   const long long b = (long long)kz * g.nxhp + (ok ? kx : 0);
+  const long long bt = (long long)kk * g.nxhp + (ok ? kx : 0);
   const zi* Si = (const zi*)spec_in;
   zo* So = (zo*)spec_out;
   const double* IB = c.d.pib;
@@ -1217,29 +1297,22 @@ __global__ __launch_bounds__(NT) void k_pois_y(CtxArg<CB_> ca_) {
   const double* pmj = c.m.pmj;
   const bool selfconj = (kx == 0 || 2 * kx == g.nx) && (kz == 0 || 2 * kz == g.nz);
   const bool pin = (kx == 0 && kz == 0);
-  TO vr[CH], vi[CH], mm[CH];
+  // every load of the chunk (spectrum, both factor rows) is issued first
+  TO vr[CH], vi[CH], mm[CH], gt[CH];
 #pragma unroll
   for (int q = 0; q < CH; q++)
     if (q < y.len) {
       const int jm = y.j0 + q;
       const long long a = jm * jstr + b;
       const zi r = Si[a];
-      const TO ib = (TO)IB[a];
+      const TO ib = (TO)IB[jm * tstr + bt];
+      gt[q] = (jm == g.n2m - 1) ? (TO)0 : -(TO)GT[jm * tstr + bt];
       vr[q] = (pin && jm == 0) ? (TO)0 : (TO)r.x * ib;
       vi[q] = selfconj ? (TO)0 : (TO)r.y * ib;
       mm[q] = (jm == 0) ? (TO)0 : -(TO)pmj[jm + 1] * ib;
     }
-  auto mf = [&](int q) { return mm[q]; };
-  ychain_fwd(vr, mf, y, L, xs[0], xs[1]);
-  ychain_fwd(vi, mf, y, L, xs[2], xs[3]);
-#pragma unroll
-  for (int q = 0; q < CH; q++)
-    if (q < y.len) {
-      const int jm = y.j0 + q;
-      mm[q] = (jm == g.n2m - 1) ? (TO)0 : -(TO)GT[jm * jstr + b];
-    }
-  ychain_bwd(vr, mf, y, C, L, xs[4], xs[5]);
-  ychain_bwd(vi, mf, y, C, L, xs[6], xs[7]);
+  ychain_fwd2(vr, vi, [&](int q) { return mm[q]; }, y, L, xs[0], xs[1], xs[2]);
+  ychain_bwd2(vr, vi, [&](int q) { return gt[q]; }, y, C, L, xs[3], xs[4], xs[5]);
   if (ok)
 #pragma unroll
     for (int q = 0; q < CH; q++)
@@ -1247,20 +1320,20 @@ __global__ __launch_bounds__(NT) void k_pois_y(CtxArg<CB_> ca_) {
 }
 
 // set-up: per mode, BET(J) and GAM(J) of TDMAP on (PMJ, PCJ - AK1 - AK3, PPJ)
-// with the pinned first row of mode (0,0); IB = 1/BET(J), GT(J) = GAM(J+1)
+// with the pinned first row of mode (0,0); IB = 1/BET(J), GT(J) = GAM(J+1),
+// for kz = 0..NZ/2 (AK3 is even in kz), rows of the spectrum's pitch
 __global__ void k_pois_table(Ctx c) {
   const Geo g = c.g;
-  const long long nm = (long long)g.nxh * g.nz;
+  const long long nm = (long long)g.nxh * (g.nz / 2 + 1);
   const long long tm = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (tm >= nm) return;
-  const int kx = tm % g.nxh, kz = tm / g.nxh;
-  const long long t = (long long)kz * g.nxhp + kx;  // the spectrum's layout
+  const int kx = tm % g.nxh, kk = tm / g.nxh;
+  const long long t = (long long)kk * g.nxhp + kx;
   const Metrics M = c.m;
-  const int kk = (kz <= g.nz / 2) ? kz : g.nz - kz;  // AK3 is even in kz
   const double ak = M.ak3[kk + 1];
   const double a1 = M.ak1[kx + 1];
-  const bool pin = (kx == 0 && kz == 0);
-  const long long jstr = (long long)g.nz * g.nxhp;
+  const bool pin = (kx == 0 && kk == 0);
+  const long long jstr = (long long)(g.nz / 2 + 1) * g.nxhp;
   double bet = pin ? 1.0 : (M.pcj[1] - ak - a1);
   c.d.pib[t] = 1.0 / bet;
   for (int jm = 1; jm < g.n2m; jm++) {
@@ -1854,7 +1927,7 @@ static void getdp_fp64(const Ctx& c) {
   const size_t ldsz = sizeof(cplx_t<T>) * (size_t)g.nz * (f.tlz + 1);
   const int gx = (g.n2m * g.nz + 2 * f.tlx - 1) / (2 * f.tlx);
   const int gz = ((g.nxh + f.tlz - 1) / f.tlz) * g.n2m;
-  const int gy = ((g.nxh + c.o.yp.L - 1) / c.o.yp.L) * g.nz;
+  const int gy = ((g.nxh + c.o.yp.L / 2 - 1) / (c.o.yp.L / 2)) * (g.nz / 2);
   // tile sizes (complex values per block): up to 1024 on 256 threads, up
   // to 2048 or 4096 on 512 (the autotuner picks the lines per tile)
   const int ex = g.nx * f.tlx, ez = g.nz * f.tlz;
@@ -1899,13 +1972,15 @@ static void getdp_fp64(const Ctx& c) {
       FFTW_(k_fft_z, ez, nz_t, dim3(gz), dim3(nz_t), ldsz, c, -1);
   }
   {
-    // tiles of up to 256 or 512 threads (16 modes: spectrum rows of 256 bytes)
+    // tiles of up to 512 threads, or 1024 with 6-point chunks
     KScope ks_(c, KID_POIS_Y);
     const int npt = c.o.yp.L * c.o.yp.C;
     const bool six = (g.n2m + c.o.yp.C - 1) / c.o.yp.C <= 6;
 #define PY_(CH, NT) \
   C3D_LAUNCH(k_pois_y, (T, T, CH, NT), dim3(gy), dim3(npt), 0, c)
-    if (npt > 256) {
+    if (npt > 512)
+      PY_(6, 1024);
+    else if (npt > 256) {
       if (six) PY_(6, 512); else PY_(CHMAX, 512);
     } else {
       if (six) PY_(6, 256); else PY_(CHMAX, 256);
@@ -1960,7 +2035,7 @@ void fft_attrs() {
 }
 
 void pois_table(const Ctx& c) {
-  const long long nm = (long long)c.g.nxh * c.g.nz;
+  const long long nm = (long long)c.g.nxh * (c.g.nz / 2 + 1);
   hipLaunchKernelGGL(k_pois_table, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, c.st, c);
 }
 

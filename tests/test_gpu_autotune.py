@@ -22,7 +22,7 @@ def test_autotune_then_parity_fp64():
         assert tuned["x_lines"] % 8 == 0 and tuned["x_lines"] * (n1 - 1) // 8 <= 256
         assert tuned["z_cols_fp64"] * (n3 - 1) // 8 <= 512
         assert tuned["y_cols_fp64"] * tuned["y_chunks_fp64"] <= 1024
-        assert tuned["pois_cols"] * tuned["pois_chunks"] <= 512
+        assert tuned["pois_cols"] % 2 == 0 and tuned["pois_cols"] * tuned["pois_chunks"] <= 1024
         orc = Oracle(**cfg)
         U, P, HH = inputs(orc, seed=4)
         ch = c3d.Channel(**cfg)
