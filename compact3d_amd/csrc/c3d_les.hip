@@ -1232,14 +1232,15 @@ __global__ __launch_bounds__(256) void k_sgs_x(CtxArg<CB_> ca_) {
   psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
   X.out(F.dwdx, d);
   B = X.put(pf);  // u
-  X.pre((const TH*)F.hh[0], 0, pf);
+  TH ph[SEG];
+  X.pre_raw((const TH*)F.hh[0], 0, ph);
   SQ d[q] = 24. * dx1 * (XA(B, q + 1) - XA(B, q));
   psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
   SQ d[q] = sv[q] * d[q];
   const T lf = xch_left(d[SEG - 1], G, L, l, gs, bf.xl);
   SQR d[q] = 24. * dx1 * (d[q] - prv(d, q, lf));
   psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
-  B = X.put(pf);  // HH1
+  B = X.put(ph);  // HH1
   SQ d[q] = XA(B, q) - 2. * d[q];
   X.out((TH*)F.hh[0], d);
 }
@@ -1364,12 +1365,13 @@ __global__ __launch_bounds__(256) void k_sgs_x2(CtxArg<CB_> ca_) {
   SQ d[q] = 24. * dx1 * (nxt(d, q, rt) - d[q]);
   psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
   B = X.put(pf);  // dvdx
-  X.pre((const TH*)F.hh[1], 0, pf);
+  TH ph[SEG];
+  X.pre_raw((const TH*)F.hh[1], 0, ph);
   SQ e[q] = yn[q] * XA(B, q);
   rt = xch_right(e[0], G, L, l, gs, bf.xr);
   SQ e[q] = 24. * dx1 * (nxt(e, q, rt) - e[q]);
   psolve<T, SEG>(e, G, L, l, gs, s22, bf.xf, bf.xb);
-  B = X.put(pf);  // HH2
+  B = X.put(ph);  // HH2
   X.pre(F.nue, 0, pf);
   SQ d[q] = (T)(TH)((T)(TH)(XA(B, q) - d[q]) - e[q]);
   X.out((TH*)F.hh[1], d, 2);
@@ -1383,12 +1385,12 @@ __global__ __launch_bounds__(256) void k_sgs_x2(CtxArg<CB_> ca_) {
   SQ d[q] = 24. * dx1 * (nxt(d, q, rt) - d[q]);
   psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
   B = X.put(pf);  // dudz
-  X.pre((const TH*)F.hh[2], 0, pf);
+  X.pre_raw((const TH*)F.hh[2], 0, ph);
   SQ e[q] = yn[q] * XA(B, q);
   rt = xch_right(e[0], G, L, l, gs, bf.xr);
   SQ e[q] = 24. * dx1 * (nxt(e, q, rt) - e[q]);
   psolve<T, SEG>(e, G, L, l, gs, s22, bf.xf, bf.xb);
-  B = X.put(pf);  // HH3
+  B = X.put(ph);  // HH3
   SQ d[q] = (T)(TH)((T)(TH)(XA(B, q) - d[q]) - e[q]);
   X.out((TH*)F.hh[2], d);
 }

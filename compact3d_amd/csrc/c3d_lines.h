@@ -53,6 +53,13 @@ __device__ __forceinline__ LBuf<T> lbuf(unsigned char* sm, const LShape& sh, boo
 // alternate and every put()/out() starts with a barrier, so a tile stays
 // readable until the second staging operation after the one that filled it
 // begins.  Offsets are 32-bit (fields hold fewer than 2^31 elements).
+// 4-byte tiles use a paired layout instead: thread t moves SEG/2 pairs of
+// neighbouring points (2*(t % (n/2)), +1) of rows t/(n/2) + m*(2*blockDim/n),
+// one 8-byte load or store per pair, so a wave's load carries as many bytes
+// as an FP64 one (single 4-byte loads halve the bytes in flight per
+// instruction, which measured ~20 % slower in the FP32 line kernels).  A
+// 4-byte tile (T = float) is always staged paired; an FP64 tile takes a
+// paired 4-byte field through pre_raw() and put().
 template <typename T, int SEG>
 struct XStage {
   T *st0, *st1;
@@ -63,19 +70,83 @@ struct XStage {
   unsigned vm;         // bit m set: element m exists (its line is in the grid)
   int li0, lstep;      // LDS index of element m: li0 + m*lstep
   unsigned j1;         // bit m set: element m lies in row j = 1
+  // the paired layout: pair m holds points 2*pq, 2*pq+1 of a row
+  unsigned go2[SEG / 2];  // element offset of pair m's first point
+  unsigned vm2, j12;      // as vm and j1, per pair
+  int li2, lstep2;        // LDS index of pair m's first point: li2 + m*lstep2 (second: + LS)
   __device__ __forceinline__ T* next() {
     sb ^= 1;
     return sb ? st1 : st0;
   }
+  // pair m of r holds r[2m], r[2m+1]
+  template <typename S>
+  __device__ __forceinline__ void pre_pairs(const S* F, unsigned off, S (&r)[SEG]) const {
+#pragma unroll
+    for (int m = 0; m < SEG / 2; m++) {
+      S x0 = (S)0, x1 = (S)0;
+      if ((vm2 >> m) & 1u) {
+        if constexpr (sizeof(S) == 4) {
+          // the offset is summed in 32 bits first: off wraps for joff < 0
+          const float2 v = *(const float2*)(F + (go2[m] + off));
+          x0 = (S)v.x;
+          x1 = (S)v.y;
+        } else {
+          x0 = F[go2[m] + off];
+          x1 = F[go2[m] + off + 1];
+        }
+      }
+      r[2 * m] = x0;
+      r[2 * m + 1] = x1;
+    }
+  }
+  template <typename S>
+  __device__ __forceinline__ void put_pairs(T* B, const S (&r)[SEG]) const {
+#pragma unroll
+    for (int m = 0; m < SEG / 2; m++) {
+      B[li2 + m * lstep2] = (T)r[2 * m];
+      B[li2 + LS + m * lstep2] = (T)r[2 * m + 1];
+    }
+  }
   template <typename S>
   __device__ __forceinline__ void pre(const S* F, int joff, T (&r)[SEG]) const {
     const unsigned off = (unsigned)(joff * nx);  // row j+joff exists for every j (ghost planes)
-    SQ r[q] = ((vm >> q) & 1u) ? (T)F[go[q] + off] : (T)0;
+    if constexpr (sizeof(T) == 4) {
+      S t[SEG];
+      pre_pairs(F, off, t);
+      SQ r[q] = (T)t[q];
+    } else {
+      SQ r[q] = ((vm >> q) & 1u) ? (T)F[go[q] + off] : (T)0;
+    }
   }
   __device__ __forceinline__ const T* put(const T (&r)[SEG]) {
     T* B = next();
     __syncthreads();
-    SQ B[li0 + q * lstep] = r[q];
+    if constexpr (sizeof(T) == 4)
+      put_pairs(B, r);
+    else
+      SQ B[li0 + q * lstep] = r[q];
+    __syncthreads();
+    return B;
+  }
+  // a tile of another storage type: prefetched as stored (pre_raw), so the
+  // loads stay in flight until put() converts them on the way into LDS;
+  // 4-byte fields in the paired layout
+  template <typename S>
+  __device__ __forceinline__ void pre_raw(const S* F, int joff, S (&r)[SEG]) const {
+    const unsigned off = (unsigned)(joff * nx);
+    if constexpr (sizeof(S) == 4 || sizeof(T) == 4)
+      pre_pairs(F, off, r);
+    else
+      SQ r[q] = ((vm >> q) & 1u) ? F[go[q] + off] : (S)0;
+  }
+  template <typename S>
+  __device__ __forceinline__ const T* put(const S (&r)[SEG]) {
+    T* B = next();
+    __syncthreads();
+    if constexpr (sizeof(S) == 4 || sizeof(T) == 4)
+      put_pairs(B, r);
+    else
+      SQ B[li0 + q * lstep] = (T)r[q];
     __syncthreads();
     return B;
   }
@@ -93,6 +164,23 @@ struct XStage {
   __device__ __forceinline__ bool keep(int q, int jmin) const {
     return ((vm >> q) & 1u) && (jmin <= 1 || !((j1 >> q) & 1u));
   }
+  __device__ __forceinline__ bool keep2(int m, int jmin) const {
+    return ((vm2 >> m) & 1u) && (jmin <= 1 || !((j12 >> m) & 1u));
+  }
+  template <typename S>
+  __device__ __forceinline__ void out_pairs(S* F, const T* B, int jmin) const {
+#pragma unroll
+    for (int m = 0; m < SEG / 2; m++)
+      if (keep2(m, jmin)) {
+        const T x0 = B[li2 + m * lstep2], x1 = B[li2 + LS + m * lstep2];
+        if constexpr (sizeof(S) == 4) {
+          *(float2*)(F + go2[m]) = float2{(float)x0, (float)x1};
+        } else {
+          F[go2[m]] = (S)x0;
+          F[go2[m] + 1] = (S)x1;
+        }
+      }
+  }
   // the same tile to two arrays (a field and its FP32 shadow)
   template <typename S1, typename S2>
   __device__ __forceinline__ void out2(S1* F1, S2* F2, const T (&v)[SEG], int jmin = 1) {
@@ -100,10 +188,15 @@ struct XStage {
     __syncthreads();
     SQ B[(p0 + q) * LS + l] = v[q];
     __syncthreads();
-    SQ if (keep(q, jmin)) {
-      const T x = B[li0 + q * lstep];
-      F1[go[q]] = (S1)x;
-      F2[go[q]] = (S2)x;
+    if constexpr (sizeof(T) == 4) {
+      out_pairs(F1, B, jmin);
+      out_pairs(F2, B, jmin);
+    } else {
+      SQ if (keep(q, jmin)) {
+        const T x = B[li0 + q * lstep];
+        F1[go[q]] = (S1)x;
+        F2[go[q]] = (S2)x;
+      }
     }
   }
   // rows with j < jmin are left untouched
@@ -113,7 +206,10 @@ struct XStage {
     __syncthreads();
     SQ B[(p0 + q) * LS + l] = v[q];
     __syncthreads();
-    SQ if (keep(q, jmin)) F[go[q]] = (S)B[li0 + q * lstep];
+    if constexpr (sizeof(T) == 4)
+      out_pairs(F, B, jmin);
+    else
+      SQ if (keep(q, jmin)) F[go[q]] = (S)B[li0 + q * lstep];
   }
 };
 
@@ -157,6 +253,23 @@ __device__ __forceinline__ void xstage(XStage<T, SEG>& x, const LBuf<T>& b, cons
     x.go[q] = v ? (unsigned)gidx(g, p, jj, ln / g.n2m) : 0u;
     if (v) x.vm |= 1u << q;
     if (jj == 1) x.j1 |= 1u << q;
+  }
+  // the paired layout (n even: the grid's N1-1 is)
+  const int nh = sh.n >> 1;
+  const int pq = t % nh, lq = t / nh;
+  x.lstep2 = nt / nh;
+  x.li2 = 2 * pq * x.LS + lq;
+  x.j12 = 0u;
+  x.vm2 = 0u;
+#pragma unroll
+  for (int m = 0; m < SEG / 2; m++) {
+    const int ll = lq + m * x.lstep2;
+    const int ln = ln0 + ll;
+    const int jj = 1 + ln % g.n2m;
+    const bool v = ll < sh.L && ln < nlines;
+    x.go2[m] = v ? (unsigned)gidx(g, 2 * pq, jj, ln / g.n2m) : 0u;
+    if (v) x.vm2 |= 1u << m;
+    if (jj == 1) x.j12 |= 1u << m;
   }
 }
 

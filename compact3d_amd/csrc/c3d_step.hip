@@ -725,15 +725,16 @@ __global__ __launch_bounds__(256) void k_rhsdp_x(CtxArg<CB_> ca_) {
   const Fields F = c.d;
   const Metrics M = c.m;
   const T dx1 = (T)c.s.dx1, dt = (T)c.s.dt;
-  T d[SEG], e[SEG], pf[SEG];
-  X.pre((const TS*)F.uh[0], 0, pf);
+  T d[SEG], e[SEG];
+  TS pf[SEG];
+  X.pre_raw((const TS*)F.uh[0], 0, pf);
   const T* B = X.put(pf);
-  X.pre((const TS*)F.uh[1], 1, pf);
+  X.pre_raw((const TS*)F.uh[1], 1, pf);
   SQ d[q] = (T)24. * dx1 * (XA(B, q + 1) - XA(B, q));
   psolve<T, SEG>(d, G, L, l, gs, c.o.px.s22, bf.xf, bf.xb);
   const T fjpa = (T)M.fjpa[j], fjmu = (T)M.fjmu[j], dy = (T)M.dy[j];
   B = X.put(pf);
-  X.pre((const TS*)F.uh[1], 0, pf);
+  X.pre_raw((const TS*)F.uh[1], 0, pf);
   SQ e[q] = fjpa * XA(B, q);
   B = X.put(pf);
   SQ d[q] = (e[q] - fjmu * XA(B, q)) / dy / dt + d[q] / dt;
@@ -1504,13 +1505,14 @@ __global__ __launch_bounds__(256) void k_upd_xu(CtxArg<CB_> ca_, int refine) {
   const T dmp = refine ? (T)0 : (T)F.sc[SC_DMPRESG];
   const T ku = refine ? (T)0 : (T)1;
   T d[SEG], un[SEG], pf[SEG], pg[SEG];
+  float ph[SEG];
   X.pre(F.dp, 0, pf);
   const T* B = X.put(pf);
-  X.pre((const float*)F.uh[0], 0, pf);
+  X.pre_raw((const float*)F.uh[0], 0, ph);
   X.pre(F.u, 0, pg);
   SQ d[q] = (T)24. * dx1 * (-XA(B, q - 1) + XA(B, q));
   psolve<T, SEG>(d, G, L, l, gs, c.o.px.s22, bf.xf, bf.xb);
-  B = X.put(pf);
+  B = X.put(ph);
   SQ un[q] = ku * XA(B, q) - dt * d[q] - dt * dmp;
   // the flow rate of U before the update (FLOW1, :1081)
   double so = 0.0;
