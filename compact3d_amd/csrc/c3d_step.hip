@@ -500,7 +500,7 @@ __device__ __forceinline__ void yfuse_v(const Ctx& c, const T* tb, unsigned b, c
       const T vm = vs[q], v0 = vs[q + 1], vp = vs[q + 2];
       const T* t = tb + J * YTAB_W;
       const T viscos = rinv * ((T)t[0] * vp - (T)t[1] * v0 + (T)t[2] * vm);
-      const T pressg2 = (ps[q + 1] - ps[q]) / (T)t[3];
+      const T pressg2 = (ps[q + 1] - ps[q]) * (T)t[4];  // t[4] = 1/H(J)
       const T r0 = ru[q] + (-pressg2 + viscos);
       const T v2 = (T)0.5 * (vp + v0);
       const T v1 = (T)0.5 * (v0 + vm);
@@ -732,12 +732,14 @@ __global__ __launch_bounds__(256) void k_rhsdp_x(CtxArg<CB_> ca_) {
   X.pre_raw((const TS*)F.uh[1], 1, pf);
   SQ d[q] = (T)24. * dx1 * (XA(B, q + 1) - XA(B, q));
   psolve<T, SEG>(d, G, L, l, gs, c.o.px.s22, bf.xf, bf.xb);
-  const T fjpa = (T)M.fjpa[j], fjmu = (T)M.fjmu[j], dy = (T)M.dy[j];
+  const T fjpa = (T)M.fjpa[j], fjmu = (T)M.fjmu[j];
   B = X.put(pf);
   X.pre_raw((const TS*)F.uh[1], 0, pf);
   SQ e[q] = fjpa * XA(B, q);
   B = X.put(pf);
-  SQ d[q] = (e[q] - fjmu * XA(B, q)) / dy / dt + d[q] / dt;
+  // reciprocals instead of three divisions per point
+  const T rdy = (T)M.rdy[j], rdt = (T)1. / dt;
+  SQ d[q] = ((e[q] - fjmu * XA(B, q)) * rdy) * rdt + d[q] * rdt;
   X.out(F.rdp, d);
 }
 
@@ -755,9 +757,11 @@ __global__ __launch_bounds__(512) void k_rhsdp_z(CtxArg<CB_> ca_) {
   if (sizeof(TS) == 4) {
     // mixed mode: DIV(U)/DT of the current state (see k_rhsdp_x)
     const double* dv = c.d.div;
-    if (ok) SQ rdp[ZI(q)] = (rdp[ZI(q)] + dv[ZI(q)] / dt) + d[q] / dt;
+    const T rdt = (T)1. / dt;
+    if (ok) SQ rdp[ZI(q)] = (rdp[ZI(q)] + dv[ZI(q)] * rdt) + d[q] * rdt;
   } else {
-    if (ok) SQ rdp[ZI(q)] = rdp[ZI(q)] + d[q] / dt;
+    const T rdt = (T)1. / dt;
+    if (ok) SQ rdp[ZI(q)] = rdp[ZI(q)] + d[q] * rdt;
   }
 }
 
@@ -1454,11 +1458,11 @@ __global__ __launch_bounds__(256) void k_upd_x(CtxArg<CB_> ca_, int chk, int ref
     const double dy = ok ? M.dy[j] : 0.0;
     block_sum2(so * dy, 0.0, F.flowp + FP_U * F.flowr + blockIdx.x, nullptr);
   }
-  const T hj = (T)M.h[j];
+  const T rhj = (T)M.rh[j];  // 1/H(J)
   const T* B1 = X.put(pf);
   X.pre(F.dp, -1, pf);
   B = X.put(pf);
-  SQ d[q] = (j >= 2) ? ku * XA(B1, q) - dt * (dpr[q] - XA(B, q)) / hj : (T)0;
+  SQ d[q] = (j >= 2) ? ku * XA(B1, q) - dt * (dpr[q] - XA(B, q)) * rhj : (T)0;
   if (chk) X.pre((const TS*)F.uh[1], 1, pf);
   if (MIX) {
     B = X.put(pg);
@@ -1470,17 +1474,17 @@ __global__ __launch_bounds__(256) void k_upd_x(CtxArg<CB_> ca_, int chk, int ref
   }
   if (!chk) return;
   // new V at J+1 (zero at J+1 = N2: the wall value); d = (V(J+1) - V(J))/DY
-  const T hjp = (T)M.h[j + 1], dy = (T)M.dy[j];
+  const T rhjp = (T)M.rh[j + 1], rdy = (T)M.rdy[j];
   B1 = X.put(pf);
   X.pre(F.dp, 1, pf);
   B = X.put(pf);
   T e[SEG];
-  SQ e[q] = ku * XA(B1, q) - dt * (XA(B, q) - dpr[q]) / hjp;
+  SQ e[q] = ku * XA(B1, q) - dt * (XA(B, q) - dpr[q]) * rhjp;
   if (MIX) {
     B = X.put(pg);
     SQ e[q] = XA(B, q) + e[q];
   }
-  SQ d[q] = (((j + 1 <= g.n2m) ? e[q] : (T)0) - d[q]) / dy;
+  SQ d[q] = (((j + 1 <= g.n2m) ? e[q] : (T)0) - d[q]) * rdy;
   const T rt = xch_right(un[0], G, L, l, gs, bf.xr);
   SQ dpr[q] = (T)24. * dx1 * (nxt(un, q, rt) - un[q]);
   psolve<T, SEG>(dpr, G, L, l, gs, c.o.px.s22, bf.xf, bf.xb);
@@ -1554,7 +1558,7 @@ __global__ __launch_bounds__(256) void k_upd_v(CtxArg<CB_> ca_, int refine, int 
   const double* vo = F.vold;
   // V(J) of the new field from DP(J-1), DP(J)
   auto vnew = [&](int J, double dpj, double dpm) -> double {
-    const double d = ku * (double)dvr[base + (unsigned)J * nx] - dt * (dpj - dpm) / M.h[J];
+    const double d = ku * (double)dvr[base + (unsigned)J * nx] - dt * (dpj - dpm) * M.rh[J];
     return vo[base + (unsigned)J * nx] + d;
   };
   double dpm = F.dp[base + (unsigned)(J0 - 1) * nx];
@@ -1571,7 +1575,7 @@ __global__ __launch_bounds__(256) void k_upd_v(CtxArg<CB_> ca_, int refine, int 
       F.v[a] = vj;
       F.vf[a] = (float)vj;
     }
-    F.div[a] = (vp - vj) / M.dy[J] + F.div[a];
+    F.div[a] = (vp - vj) * M.rdy[J] + F.div[a];
     vj = vp;
     dpj = dpp;
   }
@@ -1624,7 +1628,7 @@ __global__ __launch_bounds__(512) void k_upd_z(CtxArg<CB_> ca_, int chk, int ref
   psolve<T, SEG>(d, G, L, l, gs, PZ.s22, bf.xf, bf.xb);
   double dmax = 0.0, cmax = 0.0;
   if (ok) {
-    const double dyj = M.dy[j];
+    const double hdy = 0.5 / M.dy[j];
     const unsigned bi = (unsigned)gidx(g, i + 1 == g.nx ? 0 : i + 1, j, 0);
     SQ {
       const unsigned a = ZI(q);
@@ -1633,7 +1637,7 @@ __global__ __launch_bounds__(512) void k_upd_z(CtxArg<CB_> ca_, int chk, int ref
       dmax = (dv != dv || dmax != dmax) ? NAN : fmax(dmax, fabs(dv));
       const double wq = (double)wn[q], wq1 = (double)nxt(wn, q, rt);
       const double cfll = fabs(F.u[a] + F.u[bi + (unsigned)(p0 + q) * pl]) * 0.5 * s.dx1 +
-                          fabs(F.v[a] + F.v[a + g.nx]) * 0.5 / dyj + fabs(wq + wq1) * 0.5 * s.dx3;
+                          fabs(F.v[a] + F.v[a + g.nx]) * hdy + fabs(wq + wq1) * 0.5 * s.dx3;
       cmax = (cfll != cfll || cmax != cmax) ? NAN : fmax(cmax, cfll);
     }
   }

@@ -86,7 +86,7 @@ __device__ __forceinline__ T yterm_v(const Ctx& c, const YIdx& x, int i, int j, 
   const T* sp = (const T*)F.sp;
   const T v0 = sv[x.b], vp = sv[x.bp], vm = sv[x.bm];
   const T viscos = (T)s.rinv * ((T)M.dyp[j] * vp - (T)M.dyc[j] * v0 + (T)M.dym[j] * vm);
-  const T pressg2 = (sp[x.b] - sp[x.bm]) / (T)M.h[j];
+  const T pressg2 = (sp[x.b] - sp[x.bm]) * (T)M.rh[j];
   const T ru = ((const T*)F.ruh[1])[x.b] + (-pressg2 + viscos);
   const T v2 = (T)0.5 * (vp + v0);
   const T v1 = (T)0.5 * (v0 + vm);
