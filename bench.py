@@ -54,10 +54,10 @@ ALG_BYTES_MIXED = {
     "rhs_x": 44, "rhs_z": 56, "rhs_zw": 20, "rhs_x2": 12, "adi_y": 84, "adi_z": 24,
     "rhsdp_x": 16, "rhsdp_z": 28, "fft_x": 16, "fft_z": 16, "pois_y": 24, "fft_xinv": 16,
     # the update: U on x-lines (DP, DELTA_u, U in; U, its shadow, D1x U out),
-    # V on y-columns (V, DELTA_v, DP, DIV in; V, its shadow, DIV out), then
-    # W and P on z-lines (DP, DELTA_w, W, P, DIV, U, V in; W, P, their
-    # shadows, DIV out) with DIVMAX and CFL
-    "upd_x": 40, "upd_v": 48, "upd_z": 84,
+    # V on y-columns (V, DELTA_v, DP in; V, its shadow out), then W and P on
+    # z-lines (DP, DELTA_w, W, P, DIV, U, V in; W, P, their shadows, DIV out)
+    # with the y and z parts of DIV, DIVMAX and CFL
+    "upd_x": 40, "upd_v": 32, "upd_z": 84,
 }
 
 

@@ -1407,71 +1407,40 @@ __global__ __launch_bounds__(NT) void k_fft_xinv(CtxArg<CB_> ca_) {
 }
 
 // ----------------------------------------------------------------- UPCALC
-// x-lines: U1 = UH1 - DT*D1x(DP) - DT*DMPRESG (:1113-1128); U2 = UH2 -
-// DT*(DP(J)-DP(J-1))/H(J) (:1139-1145); with `chk`, the x part of the
+// x-lines (FP64 mode): U1 = UH1 - DT*D1x(DP) - DT*DMPRESG (:1113-1128); U2 =
+// UH2 - DT*(DP(J)-DP(J-1))/H(J) (:1139-1145); with `chk`, the x part of the
 // divergence of the new field, (V(J+1)-V(J))/DY + D1x(U) (:9066-9102).
-// Arithmetic in T (FP64); DP and UH stored as TS.  Mixed mode (TS = float):
-// UH holds DELTA, the new state is U + (DELTA - DT*...), and the FP32
-// shadows are written beside the state.
-// `refine` (mixed mode): the correction pass of the FP64-residual
-// refinement, U -= DT*grad(DP) with no DELTA and no DMPRESG.
-template <typename T, int SEG, typename TS, bool CB_>
-__global__ __launch_bounds__(256) void k_upd_x(CtxArg<CB_> ca_, int chk, int refine) {
+// Mixed mode updates U and V with k_upd_xu and k_upd_v below.
+template <typename T, int SEG, bool CB_>
+__global__ __launch_bounds__(256) void k_upd_x(CtxArg<CB_> ca_, int chk) {
   const Ctx c = ctx_get(ca_);
   XSETUP;
-  constexpr bool MIX = sizeof(TS) == 4;
   const Scal s = c.s;
   const Metrics M = c.m;
   const Fields F = c.d;
   const T dx1 = (T)s.dx1, dt = (T)s.dt;
-  const T dmp = refine ? (T)0 : (T)F.sc[SC_DMPRESG];
-  const T ku = refine ? (T)0 : (T)1;  // weight of UH (DELTA)
-  // pf: next UH/DP tile in flight; pg: next FP64 state tile (mixed mode)
-  T dpr[SEG], d[SEG], un[SEG], pf[SEG], pg[SEG];
+  const T dmp = (T)F.sc[SC_DMPRESG];
+  // pf: the next UH/DP tile in flight
+  T dpr[SEG], d[SEG], un[SEG], pf[SEG];
   X.pre(F.dp, 0, pf);
   const T* B = X.put(pf);
-  X.pre((const TS*)F.uh[0], 0, pf);
-  if (MIX) X.pre(F.u, 0, pg);
+  X.pre((const T*)F.uh[0], 0, pf);
   SQ {
     dpr[q] = XA(B, q);
     d[q] = (T)24. * dx1 * (-XA(B, q - 1) + XA(B, q));
   }
   psolve<T, SEG>(d, G, L, l, gs, c.o.px.s22, bf.xf, bf.xb);
   B = X.put(pf);
-  SQ un[q] = ku * XA(B, q) - dt * d[q] - dt * dmp;
-  X.pre((const TS*)F.uh[1], 0, pf);
-  // mixed mode: the flow rate of U before the update (FLOW1, :1081)
-  double so = 0.0;
-  if (MIX) {
-    B = X.put(pg);
-    SQ {
-      const T uo = XA(B, q);
-      so += (double)uo;
-      un[q] = uo + un[q];
-    }
-    X.pre(F.vold, 0, pg);
-    X.out2(F.u, F.uf, un);
-  } else {
-    X.out(F.u, un);
-  }
-  if (MIX) {
-    const double dy = ok ? M.dy[j] : 0.0;
-    block_sum2(so * dy, 0.0, F.flowp + FP_U * F.flowr + blockIdx.x, nullptr);
-  }
+  SQ un[q] = XA(B, q) - dt * d[q] - dt * dmp;
+  X.pre((const T*)F.uh[1], 0, pf);
+  X.out(F.u, un);
   const T rhj = (T)M.rh[j];  // 1/H(J)
   const T* B1 = X.put(pf);
   X.pre(F.dp, -1, pf);
   B = X.put(pf);
-  SQ d[q] = (j >= 2) ? ku * XA(B1, q) - dt * (dpr[q] - XA(B, q)) * rhj : (T)0;
-  if (chk) X.pre((const TS*)F.uh[1], 1, pf);
-  if (MIX) {
-    B = X.put(pg);
-    SQ d[q] = (j >= 2) ? XA(B, q) + d[q] : (T)0;
-    if (chk) X.pre(F.vold, 1, pg);
-    X.out2(F.v, F.vf, d, 2);  // V(I,1,K) is not written by UPCALC
-  } else {
-    X.out(F.v, d, 2);
-  }
+  SQ d[q] = (j >= 2) ? XA(B1, q) - dt * (dpr[q] - XA(B, q)) * rhj : (T)0;
+  if (chk) X.pre((const T*)F.uh[1], 1, pf);
+  X.out(F.v, d, 2);  // V(I,1,K) is not written by UPCALC
   if (!chk) return;
   // new V at J+1 (zero at J+1 = N2: the wall value); d = (V(J+1) - V(J))/DY
   const T rhjp = (T)M.rh[j + 1], rdy = (T)M.rdy[j];
@@ -1479,11 +1448,7 @@ __global__ __launch_bounds__(256) void k_upd_x(CtxArg<CB_> ca_, int chk, int ref
   X.pre(F.dp, 1, pf);
   B = X.put(pf);
   T e[SEG];
-  SQ e[q] = ku * XA(B1, q) - dt * (XA(B, q) - dpr[q]) * rhjp;
-  if (MIX) {
-    B = X.put(pg);
-    SQ e[q] = XA(B, q) + e[q];
-  }
+  SQ e[q] = XA(B1, q) - dt * (XA(B, q) - dpr[q]) * rhjp;
   SQ d[q] = (((j + 1 <= g.n2m) ? e[q] : (T)0) - d[q]) * rdy;
   const T rt = xch_right(un[0], G, L, l, gs, bf.xr);
   SQ dpr[q] = (T)24. * dx1 * (nxt(un, q, rt) - un[q]);
@@ -1493,10 +1458,10 @@ __global__ __launch_bounds__(256) void k_upd_x(CtxArg<CB_> ca_, int chk, int ref
 }
 
 // Mixed mode splits the x-line update: k_upd_xu updates U (x-lines) and
-// stores D1x of the new U in DIV; k_upd_v updates V along y-columns and
-// adds (V(J+1) - V(J))/DY to DIV, each new V(J) formed once and carried to
-// the next J (the x-line form re-reads the J+1 rows of DELTA_v, V and DP).
-// Same expressions and operation order as k_upd_x.
+// stores D1x of the new U in DIV; k_upd_v updates V along y-columns (DP(J-1)
+// carried to the next J); k_upd_z adds (V(J+1) - V(J))/DY, from the V(J),
+// V(J+1) its CFL term reads anyway, and D1z of the new W.  Same expressions
+// and operation order as k_upd_x.
 template <int SEG, bool CB_>
 __global__ __launch_bounds__(256) void k_upd_xu(CtxArg<CB_> ca_, int refine) {
   const Ctx c = ctx_get(ca_);
@@ -1537,7 +1502,7 @@ __global__ __launch_bounds__(256) void k_upd_xu(CtxArg<CB_> ca_, int refine) {
 // y-columns: thread (i, k, chunk) walks J = J0..J1 of one column (i fastest
 // across threads: rows are coalesced); V(J) = VOLD(J) + (DELTA_v(J) -
 // DT*(DP(J)-DP(J-1))/H(J)) for J >= 2 (V(1) is the wall's and left to the
-// ghost pass), DIV(J) = (V(J+1) - V(J))/DY(J) + DIV(J) with V(N2) = 0
+// ghost pass)
 template <bool CB_>
 __global__ __launch_bounds__(256) void k_upd_v(CtxArg<CB_> ca_, int refine, int ch) {
   const Ctx c = ctx_get(ca_);
@@ -1562,22 +1527,15 @@ __global__ __launch_bounds__(256) void k_upd_v(CtxArg<CB_> ca_, int refine, int 
     return vo[base + (unsigned)J * nx] + d;
   };
   double dpm = F.dp[base + (unsigned)(J0 - 1) * nx];
-  double dpj = F.dp[base + (unsigned)J0 * nx];
-  double vj = J0 >= 2 ? vnew(J0, dpj, dpm) : 0.0;
   for (int J = J0; J <= J1; J++) {
     const unsigned a = base + (unsigned)J * nx;
-    double vp = 0.0, dpp = 0.0;
-    if (J + 1 <= g.n2m) {
-      dpp = F.dp[a + nx];
-      vp = vnew(J + 1, dpp, dpj);
-    }
+    const double dpj = F.dp[a];
     if (J >= 2) {
-      F.v[a] = vj;
-      F.vf[a] = (float)vj;
+      const double v = vnew(J, dpj, dpm);
+      F.v[a] = v;
+      F.vf[a] = (float)v;
     }
-    F.div[a] = (vp - vj) * M.rdy[J] + F.div[a];
-    vj = vp;
-    dpj = dpp;
+    dpm = dpj;
   }
 }
 
@@ -1630,10 +1588,19 @@ __global__ __launch_bounds__(512) void k_upd_z(CtxArg<CB_> ca_, int chk, int ref
   if (ok) {
     const double hdy = 0.5 / M.dy[j];
     const unsigned bi = (unsigned)gidx(g, i + 1 == g.nx ? 0 : i + 1, j, 0);
+    const double rdyj = M.rdy[j];
     SQ {
       const unsigned a = ZI(q);
-      const double dv = F.div[a] + (double)d[q];
-      if (MIX) F.div[a] = dv;
+      double dv;
+      if (MIX) {
+        // DIV of the new field: D1x U (k_upd_xu) + (V(J+1) - V(J))/DY (the
+        // wall values V(1) = V(N2) = 0) + D1z W
+        const double vj = j >= 2 ? F.v[a] : 0.0, vp = j + 1 <= g.n2m ? F.v[a + g.nx] : 0.0;
+        dv = ((vp - vj) * rdyj + F.div[a]) + (double)d[q];
+        F.div[a] = dv;
+      } else {
+        dv = F.div[a] + (double)d[q];
+      }
       dmax = (dv != dv || dmax != dmax) ? NAN : fmax(dmax, fabs(dv));
       const double wq = (double)wn[q], wq1 = (double)nxt(wn, q, rt);
       const double cfll = fabs(F.u[a] + F.u[bi + (unsigned)(p0 + q) * pl]) * 0.5 * s.dx1 +
@@ -2045,10 +2012,9 @@ void pois_table(const Ctx& c) {
   hipLaunchKernelGGL(k_pois_table, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, c.st, c);
 }
 
-template <typename TS>
-static void upd_x_t(const Ctx& c, bool chk, int refine) {
+static void upd_x_fp64(const Ctx& c, bool chk) {
   const size_t lx = lds_line(c.o.lx, true, sizeof(double));
-  XSEG(c.o.lx.seg, { KScope ks_(c, KID_UPD_X); C3D_LAUNCH(k_upd_x, (double, S_, TS), dim3(xblocks(c)), xthreads(c), lx, c, chk ? 1 : 0, refine); });
+  XSEG(c.o.lx.seg, { KScope ks_(c, KID_UPD_X); C3D_LAUNCH(k_upd_x, (double, S_), dim3(xblocks(c)), xthreads(c), lx, c, chk ? 1 : 0); });
 }
 template <typename TS>
 static void upd_z_t(const Ctx& c, bool chk, int refine) {
@@ -2077,7 +2043,7 @@ void upd_x(const Ctx& c, bool chk, bool refine) {
     KScope ks_(c, KID_UPD_V);
     C3D_LAUNCH0(k_upd_v, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c, refine ? 1 : 0, ch);
   } else {
-    upd_x_t<double>(c, chk, 0);
+    upd_x_fp64(c, chk);
   }
 }
 void upd_z(const Ctx& c, bool chk, bool refine) {
