@@ -1559,15 +1559,16 @@ __global__ __launch_bounds__(512) void k_upd_z(CtxArg<CB_> ca_, int chk, int ref
   SQ d[q] = (T)24. * dx3 * (-prv(dpr, q, lf) + dpr[q]);
   psolve<T, SEG>(d, G, L, l, gs, PZ.s22, bf.xf, bf.xb);
   SQ wn[q] = ku * (T)((const TS*)F.uh[2])[ZI(q)] - dt * d[q] - dt * dmp3;
+  // mixed mode: the flow rate of W before the update (FLOW3, :1168), summed
+  // over the block with the maxima at the end
+  double so = 0.0;
   if (MIX) {
-    // the flow rate of W before the update (FLOW3, :1168)
-    double so = 0.0;
     SQ {
       const T wo = (T)F.w[ZI(q)];
       so += (double)wo;
       wn[q] = wo + wn[q];
     }
-    block_sum2(ok ? so * M.dy[j] : 0.0, 0.0, F.flowp + FP_W * F.flowr + blockIdx.x, nullptr);
+    so = ok ? so * M.dy[j] : 0.0;
   }
   if (ok) {
     SQ {
@@ -1580,7 +1581,10 @@ __global__ __launch_bounds__(512) void k_upd_z(CtxArg<CB_> ca_, int chk, int ref
       }
     }
   }
-  if (!chk) return;
+  if (!chk) {
+    if (MIX) block_sum2(so, 0.0, F.flowp + FP_W * F.flowr + blockIdx.x, nullptr);
+    return;
+  }
   rt = xch_right(wn[0], G, L, l, gs, bf.xr);
   SQ d[q] = (T)24. * dx3 * (nxt(wn, q, rt) - wn[q]);
   psolve<T, SEG>(d, G, L, l, gs, PZ.s22, bf.xf, bf.xb);
@@ -1591,11 +1595,12 @@ __global__ __launch_bounds__(512) void k_upd_z(CtxArg<CB_> ca_, int chk, int ref
     const double rdyj = M.rdy[j];
     SQ {
       const unsigned a = ZI(q);
+      const double v0 = F.v[a], v1 = F.v[a + g.nx];
       double dv;
       if (MIX) {
         // DIV of the new field: D1x U (k_upd_xu) + (V(J+1) - V(J))/DY (the
         // wall values V(1) = V(N2) = 0) + D1z W
-        const double vj = j >= 2 ? F.v[a] : 0.0, vp = j + 1 <= g.n2m ? F.v[a + g.nx] : 0.0;
+        const double vj = j >= 2 ? v0 : 0.0, vp = j + 1 <= g.n2m ? v1 : 0.0;
         dv = ((vp - vj) * rdyj + F.div[a]) + (double)d[q];
         F.div[a] = dv;
       } else {
@@ -1604,19 +1609,23 @@ __global__ __launch_bounds__(512) void k_upd_z(CtxArg<CB_> ca_, int chk, int ref
       dmax = (dv != dv || dmax != dmax) ? NAN : fmax(dmax, fabs(dv));
       const double wq = (double)wn[q], wq1 = (double)nxt(wn, q, rt);
       const double cfll = fabs(F.u[a] + F.u[bi + (unsigned)(p0 + q) * pl]) * 0.5 * s.dx1 +
-                          fabs(F.v[a] + F.v[a + g.nx]) * hdy + fabs(wq + wq1) * 0.5 * s.dx3;
+                          fabs(v0 + v1) * hdy + fabs(wq + wq1) * 0.5 * s.dx3;
       cmax = (cfll != cfll || cmax != cmax) ? NAN : fmax(cmax, cfll);
     }
   }
   // block maxima into 64 spread slots (bit patterns of non-negative
-  // doubles order like the values); k_max_fin reduces the slots
-  __shared__ double red[2][8];
+  // doubles order like the values); k_max_fin reduces the slots.  Mixed
+  // mode: the block's flow-rate sum in block_sum2's order (wavefront sums,
+  // then the wavefronts in index order)
+  __shared__ double red[3][8];
   dmax = wave_max(dmax);
   cmax = wave_max(cmax);
+  if (MIX) so = wave_sum(so);
   const int wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
   if ((threadIdx.x & 63) == 0) {
     red[0][wv] = dmax;
     red[1][wv] = cmax;
+    if (MIX) red[2][wv] = so;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -1626,6 +1635,11 @@ __global__ __launch_bounds__(512) void k_upd_z(CtxArg<CB_> ca_, int chk, int ref
     }
     atomic_max_pos(&F.mxs[blockIdx.x & 63], dmax);
     atomic_max_pos(&F.mxs[64 + (blockIdx.x & 63)], cmax);
+    if (MIX) {
+      double sa = 0.0;
+      for (int q = 0; q < nw; q++) sa += red[2][q];
+      F.flowp[FP_W * F.flowr + blockIdx.x] = sa;
+    }
   }
 }
 
