@@ -39,9 +39,10 @@ ALG_BYTES = {
     # read once, written once); adi_y is the ADI y-sweep with the RHS y terms
     # fused in (per component: state, VV1/VV3 or P, RUH, HH, HH2 in; HH, UH out);
     # the spectral kernels move (NX/2+1)/NX ~ 1/2 a complex per cell, pois_y
-    # also its two per-mode factor tables
+    # also its two per-mode factor tables, stored for kz <= NZ/2 (1/4 of a
+    # double pair per cell)
     "rhs_x": 88, "rhs_z": 112, "rhs_zw": 40, "rhs_x2": 24, "adi_y": 168, "adi_z": 64,
-    "rhsdp_x": 24, "rhsdp_z": 24, "fft_x": 16, "fft_z": 16, "pois_y": 24, "fft_xinv": 16,
+    "rhsdp_x": 24, "rhsdp_z": 24, "fft_x": 16, "fft_z": 16, "pois_y": 20, "fft_xinv": 16,
     "upd_x": 48, "upd_z": 64,
 }
 # adi_x runs twice per step: (u,v) read+write 32 B, then w with U added 24 B
@@ -52,7 +53,7 @@ ALG_BYTES_ADI_X_PER_STEP = 56
 # the state)
 ALG_BYTES_MIXED = {
     "rhs_x": 44, "rhs_z": 56, "rhs_zw": 20, "rhs_x2": 12, "adi_y": 84, "adi_z": 24,
-    "rhsdp_x": 16, "rhsdp_z": 28, "fft_x": 16, "fft_z": 16, "pois_y": 24, "fft_xinv": 16,
+    "rhsdp_x": 16, "rhsdp_z": 28, "fft_x": 16, "fft_z": 16, "pois_y": 20, "fft_xinv": 16,
     # the update: U on x-lines (DP, DELTA_u, U in; U, its shadow, D1x U out),
     # V on y-columns (V, DELTA_v, DP in; V, its shadow out), then W and P on
     # z-lines (DP, DELTA_w, W, P, DIV, U, V in; W, P, their shadows, DIV out)
