@@ -493,12 +493,12 @@ long long c3d_footprint_bytes(const c3d_config* cfg) {
   // FP64: state 4, DP, RDP, SGSVIS, DIV, 2 scratch (+ second V in mixed
   // mode); intermediates 10 + HH 6; FP32 shadows 4 in mixed mode; LES 7
   long long bytes = (10 + (mixed ? 1 : 0)) * fsz * 8 + 16 * fsz * es + (mixed ? 4 * fsz * 4 : 0);
-  if (cfg->files != 0.0) bytes += 7 * fsz * 8;
+  if (cfg->files != 0.0) bytes += 7 * fsz * es + (mixed ? fsz * 4 : 0);
   if (cfg->files == 1.0) bytes += (2 * n2m * ((nx + 15) / 16) * ((nz + 15) / 16) + n2m + 3) * 8;
   if (cfg->files == 3.0) bytes += (10 * n2m * ((nx + 15) / 16) * ((nz + 15) / 16) + 2 * (n2m + 3)) * 8;
   if (cfg->wmodel == 2.0) bytes += (long long)IW_NPL * nx * nz * 8;
-  const long long nm = ((nx / 2 + 1 + 7) / 8 * 8) * nz * n2m;
-  bytes += nm * 16 + nm * 16;  // spectrum + TDMAP factors
+  const long long nxhp = (nx / 2 + 1 + 7) / 8 * 8;
+  bytes += nxhp * nz * n2m * 16 + nxhp * (nz / 2 + 1) * n2m * 16;  // spectrum + TDMAP factors
   bytes += 4 * std::max(n2m * nz, nx * n2m) * 8 + 128 * 8;  // flow-rate partial sums, max slots
   return bytes + 4 * nx * nz * 8 + (1 << 20);
 }
@@ -771,10 +771,16 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   F.iwm = nullptr;
   if (cf.wmodel == 2.0 && dalloc(h, &F.iwm, (size_t)IW_NPL * n2d)) return bad(h->err);
   if (dalloc(h, &F.spec, (size_t)g.nxhp * g.nz * g.n2m)) return bad(h->err);
+  F.sgsf = nullptr;
+  F.ssgs = F.sgs;
   if (cf.files != 0.0) {
     double** les[] = {&F.nux, &F.nuz, &F.nue, &F.dvdx, &F.dwdx, &F.dudz, &F.dvdz};
     for (auto pp : les)
-      if (dalloc(h, pp, fs)) return bad(h->err);
+      if (dalloc(h, pp, fsi)) return bad(h->err);
+    if (mixed) {
+      if (dalloc(h, &F.sgsf, fs)) return bad(h->err);
+      F.ssgs = F.sgsf;
+    }
   }
   F.dyp = F.smagc = F.sdbeta = nullptr;
   if (cf.files == 1.0 || cf.files == 3.0) {
@@ -918,6 +924,7 @@ extern "C" int c3d_upload_state(c3d_handle* h, const double* U, const double* P,
     c3dk::to_f32(h->c, F.v, F.vf, fsz);
     c3dk::to_f32(h->c, F.w, F.wf, fsz);
     c3dk::to_f32(h->c, F.p, F.pf, fsz);
+    if (F.sgsf) c3dk::to_f32(h->c, F.sgs, F.sgsf, fsz);
     c3dk::divcheck(h->c);
     HIPCHK(hipStreamSynchronize(h->c.st));
     HIPCHK(hipGetLastError());

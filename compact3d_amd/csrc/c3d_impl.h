@@ -109,9 +109,12 @@ struct Fields {
   double *vv1, *ww1, *uw, *vv3;
   double *rdp, *dp;
   double *sgs;
+  float *sgsf;        // FP32 shadow of SGSVIS (mixed mode, FILES != 0)
+  const void *ssgs;   // SGSVIS as read by the SGS line kernels and y terms
   // LES intermediates (allocated when FILES != 0): nu_t interpolated to
   // x-faces (nux), z-faces (nuz), x-z edges (nue), and the compact
-  // derivatives dv/dx, dw/dx (at x-faces), du/dz, dv/dz (at z-faces)
+  // derivatives dv/dx, dw/dx (at x-faces), du/dz, dv/dz (at z-faces); in
+  // the compute type of the line kernels (FP32 in mixed mode)
   double *nux, *nuz, *nue, *dvdx, *dwdx, *dudz, *dvdz;
   double *div;
   double *ubc_u, *ubc_w;  // top ghost values U(I,N2,K,1), U(I,N2,K,3) from BCOND (NX*NZ)

@@ -53,6 +53,10 @@ __global__ void k_bcond(CtxArg<CB_> ca_) {
   if (c.s.files) {
     F.sgs[b0] = 0.0;
     F.sgs[bn] = F.sgs[bm] * (2.0 - aly);
+    if (F.sgsf) {
+      F.sgsf[b0] = 0.f;
+      F.sgsf[bn] = (float)F.sgs[bn];
+    }
   }
 }
 

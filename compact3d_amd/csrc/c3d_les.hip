@@ -42,14 +42,18 @@ __device__ __forceinline__ StrainJ strain_j(const Ctx& c, int j) {
   f.dyjm = DY[jm] / 2.0;
   return f;
 }
-__device__ __forceinline__ void strain_core(const Ctx& c, const StrainJ& f, int i, int j, int k, double sr[6]) {
+// T = float (mixed mode): the FP32 shadows of the state, FP32 arithmetic
+template <typename T>
+__device__ __forceinline__ void strain_core(const Ctx& c, const StrainJ& f, int i, int j, int k, T sr[6]) {
   const Geo& g = c.g;
-  const double* U = c.d.u;
-  const double* V = c.d.v;
-  const double* W = c.d.w;
+  const T* U = sizeof(T) == 4 ? (const T*)c.d.su : (const T*)c.d.u;
+  const T* V = sizeof(T) == 4 ? (const T*)c.d.sv : (const T*)c.d.v;
+  const T* W = sizeof(T) == 4 ? (const T*)c.d.sw : (const T*)c.d.w;
   const int ip = wrapp(i, g.nx), im = wrapm(i, g.nx), kp = wrapp(k, g.nz), km = wrapm(k, g.nz);
   const int jp = j + 1, jm = j - 1;
-  const double DX1 = c.s.dx1, DX3 = c.s.dx3;
+  const T DX1 = (T)c.s.dx1, DX3 = (T)c.s.dx3;
+  const T up = (T)f.up, um = (T)f.um, cp = (T)f.cp, cm = (T)f.cm, yf = (T)f.yf, rdy = (T)f.rdy;
+  const T dyj = (T)f.dyj, dyjp = (T)f.dyjp, dyjm = (T)f.dyjm, h = (T)0.5;
   // 32-bit element offsets (fields hold fewer than 2^31 elements)
   const unsigned nx = (unsigned)g.nx, pl = (unsigned)g.plane;
   auto o_ = [&](int ii, int jj, int kk) { return (unsigned)kk * pl + (unsigned)jj * nx + (unsigned)ii; };
@@ -57,63 +61,64 @@ __device__ __forceinline__ void strain_core(const Ctx& c, const StrainJ& f, int 
   auto v_ = [&](int ii, int jj, int kk) { return V[o_(ii, jj, kk)]; };
   auto w_ = [&](int ii, int jj, int kk) { return W[o_(ii, jj, kk)]; };
   const int N2 = g.n2;
-  double u2 = u_(ip, j, k), u1 = u_(i, j, k);
+  T u2 = u_(ip, j, k), u1 = u_(i, j, k);
   sr[0] = DX1 * (u2 - u1);
-  double ujp = 0.5 * (u_(i, jp, k) + u_(ip, jp, k));
-  double ujc = 0.5 * (u_(i, j, k) + u_(ip, j, k));
-  double ujm = 0.5 * (u_(i, jm, k) + u_(ip, jm, k));
+  T ujp = h * (u_(i, jp, k) + u_(ip, jp, k));
+  T ujc = h * (u_(i, j, k) + u_(ip, j, k));
+  T ujm = h * (u_(i, jm, k) + u_(ip, jm, k));
   if (f.wm0) {
-    u2 = f.up * (f.dyj * ujp + f.dyjp * ujc) + f.cp * (u_(i, N2, k) + u_(ip, N2, k));
-    u1 = f.um * (f.dyj * ujm + f.dyjm * ujc) + f.cm * (u_(i, 0, k) + u_(ip, 0, k));
+    u2 = up * (dyj * ujp + dyjp * ujc) + cp * (u_(i, N2, k) + u_(ip, N2, k));
+    u1 = um * (dyj * ujm + dyjm * ujc) + cm * (u_(i, 0, k) + u_(ip, 0, k));
   } else {
-    u2 = f.up * (f.dyj * ujp + f.dyjp * ujc) + f.cp * ujc;
-    u1 = f.um * (f.dyj * ujm + f.dyjm * ujc) + f.cm * ujc;
+    u2 = up * (dyj * ujp + dyjp * ujc) + cp * ujc;
+    u1 = um * (dyj * ujm + dyjm * ujc) + cm * ujc;
   }
-  double vip = 0.5 * (v_(ip, j, k) + v_(ip, jp, k));
-  double vic = 0.5 * (v_(i, j, k) + v_(i, jp, k));
-  double vim = 0.5 * (v_(im, j, k) + v_(im, jp, k));
-  double v2 = 0.5 * (vip + vic), v1 = 0.5 * (vim + vic);
-  sr[1] = 0.5 * (DX1 * (v2 - v1) + f.yf * (u2 - u1));
-  double ukp = 0.5 * (u_(i, j, kp) + u_(ip, j, kp));
-  double ukc = 0.5 * (u_(i, j, k) + u_(ip, j, k));
-  double ukm = 0.5 * (u_(i, j, km) + u_(ip, j, km));
-  u2 = 0.5 * (ukp + ukc);
-  u1 = 0.5 * (ukm + ukc);
-  double wip = 0.5 * (w_(ip, j, k) + w_(ip, j, kp));
-  double wic = 0.5 * (w_(i, j, k) + w_(i, j, kp));
-  double wim = 0.5 * (w_(im, j, k) + w_(im, j, kp));
-  double w2 = 0.5 * (wip + wic), w1 = 0.5 * (wim + wic);
-  sr[2] = 0.5 * (DX1 * (w2 - w1) + DX3 * (u2 - u1));
+  T vip = h * (v_(ip, j, k) + v_(ip, jp, k));
+  T vic = h * (v_(i, j, k) + v_(i, jp, k));
+  T vim = h * (v_(im, j, k) + v_(im, jp, k));
+  T v2 = h * (vip + vic), v1 = h * (vim + vic);
+  sr[1] = h * (DX1 * (v2 - v1) + yf * (u2 - u1));
+  T ukp = h * (u_(i, j, kp) + u_(ip, j, kp));
+  T ukc = h * (u_(i, j, k) + u_(ip, j, k));
+  T ukm = h * (u_(i, j, km) + u_(ip, j, km));
+  u2 = h * (ukp + ukc);
+  u1 = h * (ukm + ukc);
+  T wip = h * (w_(ip, j, k) + w_(ip, j, kp));
+  T wic = h * (w_(i, j, k) + w_(i, j, kp));
+  T wim = h * (w_(im, j, k) + w_(im, j, kp));
+  T w2 = h * (wip + wic), w1 = h * (wim + wic);
+  sr[2] = h * (DX1 * (w2 - w1) + DX3 * (u2 - u1));
   v2 = v_(i, jp, k);
   v1 = v_(i, j, k);
-  sr[3] = f.rdy * (v2 - v1);
-  double wjp = 0.5 * (w_(i, jp, k) + w_(i, jp, kp));
-  double wjc = 0.5 * (w_(i, j, k) + w_(i, j, kp));
-  double wjm = 0.5 * (w_(i, jm, k) + w_(i, jm, kp));
+  sr[3] = rdy * (v2 - v1);
+  T wjp = h * (w_(i, jp, k) + w_(i, jp, kp));
+  T wjc = h * (w_(i, j, k) + w_(i, j, kp));
+  T wjm = h * (w_(i, jm, k) + w_(i, jm, kp));
   if (f.wm0) {
-    w2 = f.up * (f.dyj * wjp + f.dyjp * wjc) + f.cp * (w_(i, N2, k) + w_(i, N2, kp));
-    w1 = f.um * (f.dyj * wjm + f.dyjm * wjc) + f.cm * (w_(i, 0, k) + w_(i, 0, kp));
+    w2 = up * (dyj * wjp + dyjp * wjc) + cp * (w_(i, N2, k) + w_(i, N2, kp));
+    w1 = um * (dyj * wjm + dyjm * wjc) + cm * (w_(i, 0, k) + w_(i, 0, kp));
   } else {
-    w2 = f.up * (f.dyj * wjp + f.dyjp * wjc) + f.cp * wjc;
-    w1 = f.um * (f.dyj * wjm + f.dyjm * wjc) + f.cm * wjc;
+    w2 = up * (dyj * wjp + dyjp * wjc) + cp * wjc;
+    w1 = um * (dyj * wjm + dyjm * wjc) + cm * wjc;
   }
-  double vkp = 0.5 * (v_(i, j, kp) + v_(i, jp, kp));
-  double vkc = 0.5 * (v_(i, j, k) + v_(i, jp, k));
-  double vkm = 0.5 * (v_(i, j, km) + v_(i, jp, km));
-  v2 = 0.5 * (vkp + vkc);
-  v1 = 0.5 * (vkm + vkc);
-  sr[4] = 0.5 * (f.yf * (w2 - w1) + DX3 * (v2 - v1));
+  T vkp = h * (v_(i, j, kp) + v_(i, jp, kp));
+  T vkc = h * (v_(i, j, k) + v_(i, jp, k));
+  T vkm = h * (v_(i, j, km) + v_(i, jp, km));
+  v2 = h * (vkp + vkc);
+  v1 = h * (vkm + vkc);
+  sr[4] = h * (yf * (w2 - w1) + DX3 * (v2 - v1));
   w2 = w_(i, j, kp);
   w1 = w_(i, j, k);
   sr[5] = DX3 * (w2 - w1);
 }
 __device__ __forceinline__ void strain_at(const Ctx& c, int i, int j, int k, double sr[6]) {
-  strain_core(c, strain_j(c, j), i, j, k, sr);
+  strain_core<double>(c, strain_j(c, j), i, j, k, sr);
 }
 
 // 2 S:S with the off-diagonal components counted twice, in STRAIN's order
-__device__ __forceinline__ double sdots(const double s[6]) {
-  return 2. * (s[1] * s[1]) + s[0] * s[0] + 2. * (s[2] * s[2]) + s[3] * s[3] + 2. * (s[4] * s[4]) + s[5] * s[5];
+template <typename T>
+__device__ __forceinline__ T sdots(const T s[6]) {
+  return (T)2 * (s[1] * s[1]) + s[0] * s[0] + (T)2 * (s[2] * s[2]) + s[3] * s[3] + (T)2 * (s[4] * s[4]) + s[5] * s[5];
 }
 
 // ------------------------------------------------------------ SGSCS
@@ -125,7 +130,9 @@ __device__ __forceinline__ double sdots(const double s[6]) {
 // thread, then kpt points per thread (8 on large grids, fewer when the grid
 // would have fewer than ~2048 blocks).
 #define SG_TI 64
-template <bool CB_>
+// T = float (mixed mode): STRAIN from the FP32 shadows in FP32; SGSVIS is
+// stored in FP64 and as its FP32 shadow
+template <typename T, bool CB_>
 __global__ __launch_bounds__(256) void k_sgscs(CtxArg<CB_> ca_, int kpt) {
   const Ctx c = ctx_get(ca_);
   const Geo g = c.g;
@@ -146,18 +153,22 @@ __global__ __launch_bounds__(256) void k_sgscs(CtxArg<CB_> ca_, int kpt) {
   cs = cs * (1 - exp(-yplus / 26.0));
   const double smagc = cs * cs;
   const double dxyz = 1.0 / s.dx1 * DY[j] / s.dx3;
-  const double coef = smagc * pow(dxyz, 2. / 3.);
+  const T coef = (T)(smagc * pow(dxyz, 2. / 3.));
+  const T nmin = (T)(-s.rinv);
   const int k0 = tk * 4 * kpt + threadIdx.x / SG_TI;
 #pragma unroll 2
   for (int m = 0; m < kpt; m++) {
     const int k = k0 + 4 * m;
     if (k >= g.nz) break;
-    double sr[6];
-    strain_core(c, fj, i, j, k, sr);
-    double sumsr = sdots(sr);
-    double abssr = sqrt(2. * sumsr);
-    const double turbv = coef * abssr;
-    F.sgs[(unsigned)k * (unsigned)g.plane + (unsigned)j * (unsigned)g.nx + (unsigned)i] = fmax(turbv, -s.rinv);
+    T sr[6];
+    strain_core<T>(c, fj, i, j, k, sr);
+    const T sumsr = sdots(sr);
+    const T abssr = sqrt((T)2 * sumsr);
+    const T turbv = coef * abssr;
+    const T nu = fmax(turbv, nmin);
+    const unsigned a = (unsigned)k * (unsigned)g.plane + (unsigned)j * (unsigned)g.nx + (unsigned)i;
+    F.sgs[a] = (double)nu;
+    if (sizeof(T) == 4) F.sgsf[a] = (float)nu;
   }
 }
 
@@ -299,7 +310,9 @@ __global__ void k_sgsdy_clip(Ctx c) {
   const double dxyz = 1.0 / c.s.dx1 * c.m.dy[j] / c.s.dx3;
   const long long a = gidx(g, i, j, k);
   const double turbv = c.d.smagc[j] * pow(dxyz, 2. / 3.) * c.d.sgs[a];
-  c.d.sgs[a] = fmax(turbv, -c.s.rinv);
+  const double nu = fmax(turbv, -c.s.rinv);
+  c.d.sgs[a] = nu;
+  if (c.d.sgsf) c.d.sgsf[a] = (float)nu;
 }
 
 // ------------------------------------------------------------ SGSSD
@@ -1198,201 +1211,205 @@ __global__ __launch_bounds__(C3D_NT) void k_wallss_fin(CtxArg<CB_> ca_, int nblk
 //   dvdx = D1x(v), dwdx = D1x(w) (centre -> x-face)   UU12, VV12, WW13, X11
 //   dudz = D1z(u), dvdz = D1z(v) (centre -> z-face)   X33 of :6519-6524, UU31,
 //                                                     VV23, VV32
-// All four kernels run on the register line engine (c3d_lines.h) in FP64,
-// whatever the storage type TH of HH (FP32 in mixed mode); each HH update
-// is rounded to TH after every term, as the reference's sequence of
-// separate updates does.
+// All four kernels run on the register line engine (c3d_lines.h) in the
+// compute type T of the step's line kernels: FP64, or FP32 in mixed mode,
+// where they read the FP32 shadows of the state and of SGSVIS and keep the
+// intermediates in FP32 (HH is FP32 there too).  Each HH update is a
+// separate rounded store of a term, as the reference's sequence of updates.
 
 // x-lines (j = 1..N2M): nux, dvdx, dwdx, and D(SGS*(DU/DX))/DX of RHS1
 // (:6445-6466): HH1 -= 2 D1x(SGSVIS * D1x(u))
-template <typename TH, int SEG, bool CB_>
+template <typename T, int SEG, bool CB_>
 __global__ __launch_bounds__(256) void k_sgs_x(CtxArg<CB_> ca_) {
   const Ctx c = ctx_get(ca_);
-  typedef double T;
   XSETUP;
   const Fields F = c.d;
   const PSolve s6 = c.o.px.s6, s22 = c.o.px.s22;
-  const T dx1 = c.s.dx1;
+  const T dx1 = (T)c.s.dx1;
   T d[SEG], sv[SEG], pf[SEG];
-  X.pre(F.sgs, 0, pf);
+  X.pre((const T*)F.ssgs, 0, pf);
   const T* B = X.put(pf);  // SGSVIS
-  X.pre(F.v, 0, pf);
+  X.pre((const T*)F.sv, 0, pf);
   SQ sv[q] = XA(B, q);
-  SQ d[q] = 4. * (XA(B, q) + XA(B, q - 1));
+  SQ d[q] = (T)4. * (XA(B, q) + XA(B, q - 1));
   psolve<T, SEG>(d, G, L, l, gs, s6, bf.xf, bf.xb);
-  X.out(F.nux, d);
+  X.out((T*)F.nux, d);
   B = X.put(pf);  // v
-  X.pre(F.w, 0, pf);
-  SQ d[q] = 24. * dx1 * (XA(B, q) - XA(B, q - 1));
+  X.pre((const T*)F.sw, 0, pf);
+  SQ d[q] = (T)24. * dx1 * (XA(B, q) - XA(B, q - 1));
   psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
-  X.out(F.dvdx, d);
+  X.out((T*)F.dvdx, d);
   B = X.put(pf);  // w
-  X.pre(F.u, 0, pf);
-  SQ d[q] = 24. * dx1 * (XA(B, q) - XA(B, q - 1));
+  X.pre((const T*)F.su, 0, pf);
+  SQ d[q] = (T)24. * dx1 * (XA(B, q) - XA(B, q - 1));
   psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
-  X.out(F.dwdx, d);
+  X.out((T*)F.dwdx, d);
   B = X.put(pf);  // u
-  TH ph[SEG];
-  X.pre_raw((const TH*)F.hh[0], 0, ph);
-  SQ d[q] = 24. * dx1 * (XA(B, q + 1) - XA(B, q));
+  X.pre((const T*)F.hh[0], 0, pf);
+  SQ d[q] = (T)24. * dx1 * (XA(B, q + 1) - XA(B, q));
   psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
   SQ d[q] = sv[q] * d[q];
   const T lf = xch_left(d[SEG - 1], G, L, l, gs, bf.xl);
-  SQR d[q] = 24. * dx1 * (d[q] - prv(d, q, lf));
+  SQR d[q] = (T)24. * dx1 * (d[q] - prv(d, q, lf));
   psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
-  B = X.put(ph);  // HH1
-  SQ d[q] = XA(B, q) - 2. * d[q];
-  X.out((TH*)F.hh[0], d);
+  B = X.put(pf);  // HH1
+  SQ d[q] = XA(B, q) - (T)2. * d[q];
+  X.out((T*)F.hh[0], d);
 }
 
 // z-lines (j = 1..N2M): nuz, nue, dudz, dvdz, and
 //   D(SGS*(DW/DZ))/DZ of RHS3 (:7919-7940): HH3 -= 2 D1z(SGSVIS * D1z(w))
 //   D(SGS*(DU/DZ))/DZ of RHS1 (:6491-6540): HH1 -= D1z(dudz * nue)
 //   D(SGS*(DW/DX))/DZ of RHS1 (:6713-6764): HH1 -= D1z(dwdx * nue)
-template <typename TH, int SEG, bool CB_>
+template <typename T, int SEG, bool CB_>
 __global__ __launch_bounds__(512) void k_sgs_z(CtxArg<CB_> ca_) {
   const Ctx c = ctx_get(ca_);
-  typedef double T;
   ZSETUP;
   const Fields F = c.d;
-  const PSolve s6 = c.o.pz.s6, s22 = c.o.pz.s22;
-  const T dx3 = c.s.dx3;
+  const PSolve s6 = PZ.s6, s22 = PZ.s22;
+  const T dx3 = (T)c.s.dx3;
+  T* nuz = (T*)F.nuz;
+  T* nue = (T*)F.nue;
+  T* dudz = (T*)F.dudz;
+  T* dvdz = (T*)F.dvdz;
+  T* hh0 = (T*)F.hh[0];
+  T* hh2 = (T*)F.hh[2];
   T a[SEG], d[SEG], sv[SEG], e[SEG], dz[SEG], lf, rt;
   // nuz = Iz(SGSVIS)
-  zload(F.sgs, b, pl, p0, bm, bp, sv, lf, rt);
-  SQ d[q] = 4. * (prv(sv, q, lf) + sv[q]);
-  zload(F.nux, b, pl, p0, bm, bp, a, lf, rt);
+  zload((const T*)F.ssgs, b, pl, p0, bm, bp, sv, lf, rt);
+  SQ d[q] = (T)4. * (prv(sv, q, lf) + sv[q]);
+  zload((const T*)F.nux, b, pl, p0, bm, bp, a, lf, rt);
   psolve<T, SEG>(d, G, L, l, gs, s6, bf.xf, bf.xb);
-  if (ok) SQ F.nuz[ZI(q)] = d[q];
+  if (ok) SQ nuz[ZI(q)] = d[q];
   // nue = Iz(nux)
-  SQ e[q] = 4. * (a[q] + prv(a, q, lf));
-  zload(F.u, b, pl, p0, bm, bp, a, lf, rt);
+  SQ e[q] = (T)4. * (a[q] + prv(a, q, lf));
+  zload((const T*)F.su, b, pl, p0, bm, bp, a, lf, rt);
   psolve<T, SEG>(e, G, L, l, gs, s6, bf.xf, bf.xb);
-  if (ok) SQ F.nue[ZI(q)] = e[q];
+  if (ok) SQ nue[ZI(q)] = e[q];
   // dudz = D1z(u)
-  SQ dz[q] = 24. * dx3 * (a[q] - prv(a, q, lf));
-  zload(F.v, b, pl, p0, bm, bp, a, lf, rt);
+  SQ dz[q] = (T)24. * dx3 * (a[q] - prv(a, q, lf));
+  zload((const T*)F.sv, b, pl, p0, bm, bp, a, lf, rt);
   psolve<T, SEG>(dz, G, L, l, gs, s22, bf.xf, bf.xb);
-  if (ok) SQ F.dudz[ZI(q)] = dz[q];
+  if (ok) SQ dudz[ZI(q)] = dz[q];
   // dvdz = D1z(v)
-  SQ d[q] = 24. * dx3 * (a[q] - prv(a, q, lf));
-  zload(F.w, b, pl, p0, bm, bp, a, lf, rt);
+  SQ d[q] = (T)24. * dx3 * (a[q] - prv(a, q, lf));
+  zload((const T*)F.sw, b, pl, p0, bm, bp, a, lf, rt);
   psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
-  if (ok) SQ F.dvdz[ZI(q)] = d[q];
+  if (ok) SQ dvdz[ZI(q)] = d[q];
   // HH3 -= 2 D1z(SGSVIS * D1z(w))
-  SQ d[q] = 24. * dx3 * (nxt(a, q, rt) - a[q]);
+  SQ d[q] = (T)24. * dx3 * (nxt(a, q, rt) - a[q]);
   psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
   SQ d[q] = d[q] * sv[q];
   lf = xch_left(d[SEG - 1], G, L, l, gs, bf.xl);
-  SQR d[q] = 24. * dx3 * (d[q] - prv(d, q, lf));
-  SQ a[q] = (T)((const TH*)F.hh[2])[ZI(q)];
+  SQR d[q] = (T)24. * dx3 * (d[q] - prv(d, q, lf));
+  SQ a[q] = hh2[ZI(q)];
   psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
-  if (ok) SQ((TH*)F.hh[2])[ZI(q)] = (TH)(a[q] - 2. * d[q]);
+  if (ok) SQ hh2[ZI(q)] = a[q] - (T)2. * d[q];
   // HH1 -= D1z(dudz * nue)
   SQ d[q] = dz[q] * e[q];
   rt = xch_right(d[0], G, L, l, gs, bf.xr);
-  SQ d[q] = 24. * dx3 * (nxt(d, q, rt) - d[q]);
-  SQ a[q] = (T)((const TH*)F.hh[0])[ZI(q)];
+  SQ d[q] = (T)24. * dx3 * (nxt(d, q, rt) - d[q]);
+  SQ a[q] = hh0[ZI(q)];
   psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
-  SQ a[q] = (T)(TH)(a[q] - d[q]);
+  SQ a[q] = a[q] - d[q];
   // HH1 -= D1z(dwdx * nue)
-  SQ d[q] = F.dwdx[ZI(q)] * e[q];
+  SQ d[q] = ((const T*)F.dwdx)[ZI(q)] * e[q];
   rt = xch_right(d[0], G, L, l, gs, bf.xr);
-  SQ d[q] = 24. * dx3 * (nxt(d, q, rt) - d[q]);
+  SQ d[q] = (T)24. * dx3 * (nxt(d, q, rt) - d[q]);
   psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
-  if (ok) SQ((TH*)F.hh[0])[ZI(q)] = (TH)(a[q] - d[q]);
+  if (ok) SQ hh0[ZI(q)] = a[q] - d[q];
 }
 
 // z-lines at J >= 2: the two RHS2 terms that need nuz at J-1 and J
 //   D(SGS*(DW/DY))/DZ (:7177-7226), D(SGS*(DV/DZ))/DZ (:7245-7304)
-template <typename TH, int SEG, bool CB_>
+template <typename T, int SEG, bool CB_>
 __global__ __launch_bounds__(512) void k_sgs_z2(CtxArg<CB_> ca_) {
   const Ctx c = ctx_get(ca_);
-  typedef double T;
   ZSETUP_J0(2);
   const Metrics M = c.m;
   const Fields F = c.d;
-  const PSolve s22 = c.o.pz.s22;
-  const T dx3 = c.s.dx3;
-  const T hinv = 1. / M.h[j], dyj = M.dy[j] / 2., dyjm = M.dy[j - 1] / 2.;
+  const PSolve s22 = PZ.s22;
+  const T dx3 = (T)c.s.dx3;
+  const T hinv = (T)(1. / M.h[j]), dyj = (T)(M.dy[j] / 2.), dyjm = (T)(M.dy[j - 1] / 2.);
+  const T* nuz = (const T*)F.nuz;
+  const T* sw = (const T*)F.sw;
+  T* hh1 = (T*)F.hh[1];
   T d[SEG], yn[SEG], a[SEG], rt;
   SQ {
-    yn[q] = hinv * (dyj * F.nuz[ZI(q) - g.nx] + dyjm * F.nuz[ZI(q)]);
-    d[q] = yn[q] * (hinv * (F.w[ZI(q)] - F.w[ZI(q) - g.nx]));
+    yn[q] = hinv * (dyj * nuz[ZI(q) - g.nx] + dyjm * nuz[ZI(q)]);
+    d[q] = yn[q] * (hinv * (sw[ZI(q)] - sw[ZI(q) - g.nx]));
   }
   rt = xch_right(d[0], G, L, l, gs, bf.xr);
-  SQ d[q] = 24. * dx3 * (nxt(d, q, rt) - d[q]);
-  SQ a[q] = (T)((const TH*)F.hh[1])[ZI(q)];
+  SQ d[q] = (T)24. * dx3 * (nxt(d, q, rt) - d[q]);
+  SQ a[q] = hh1[ZI(q)];
   psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
-  SQ a[q] = (T)(TH)(a[q] - d[q]);
-  SQ d[q] = yn[q] * F.dvdz[ZI(q)];
+  SQ a[q] = a[q] - d[q];
+  SQ d[q] = yn[q] * ((const T*)F.dvdz)[ZI(q)];
   rt = xch_right(d[0], G, L, l, gs, bf.xr);
-  SQ d[q] = 24. * dx3 * (nxt(d, q, rt) - d[q]);
+  SQ d[q] = (T)24. * dx3 * (nxt(d, q, rt) - d[q]);
   psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
-  if (ok) SQ((TH*)F.hh[1])[ZI(q)] = (TH)(a[q] - d[q]);
+  if (ok) SQ hh1[ZI(q)] = a[q] - d[q];
 }
 
 // x-lines: D(SGS*(DU/DY))/DX and D(SGS*(DV/DX))/DX of RHS2 (:7104-7158,
 // :7323-7384, J >= 2), D(SGS*(DW/DX))/DX and D(SGS*(DU/DZ))/DX of RHS3
 // (:7797-7847, :7972-8024)
-template <typename TH, int SEG, bool CB_>
+template <typename T, int SEG, bool CB_>
 __global__ __launch_bounds__(256) void k_sgs_x2(CtxArg<CB_> ca_) {
   const Ctx c = ctx_get(ca_);
-  typedef double T;
   XSETUP;
   const Metrics M = c.m;
   const Fields F = c.d;
   const PSolve s22 = c.o.px.s22;
-  const T dx1 = c.s.dx1;
+  const T dx1 = (T)c.s.dx1;
   const bool j2 = ok && j >= 2;
-  const T hinv = 1. / M.h[j];
-  const T dyjm = M.dy[j - 1] / 2., dyj = M.dy[j] / 2.;
+  const T hinv = (T)(1. / M.h[j]);
+  const T dyjm = (T)(M.dy[j - 1] / 2.), dyj = (T)(M.dy[j] / 2.);
   T d[SEG], yn[SEG], e[SEG], pf[SEG], rt;
-  X.pre(F.nux, 0, pf);
+  X.pre((const T*)F.nux, 0, pf);
   const T* B = X.put(pf);  // nux(J)
-  X.pre(F.nux, -1, pf);
+  X.pre((const T*)F.nux, -1, pf);
   SQ yn[q] = dyjm * XA(B, q);
   B = X.put(pf);  // nux(J-1)
-  X.pre(F.u, 0, pf);
-  SQ yn[q] = j2 ? hinv * (yn[q] + dyj * XA(B, q)) : 0.;
+  X.pre((const T*)F.su, 0, pf);
+  SQ yn[q] = j2 ? hinv * (yn[q] + dyj * XA(B, q)) : (T)0;
   B = X.put(pf);  // u(J)
-  X.pre(F.u, -1, pf);
+  X.pre((const T*)F.su, -1, pf);
   SQ d[q] = XA(B, q);
   B = X.put(pf);  // u(J-1)
-  X.pre(F.dvdx, 0, pf);
-  SQ d[q] = j2 ? yn[q] * (hinv * (d[q] - XA(B, q))) : 0.;
+  X.pre((const T*)F.dvdx, 0, pf);
+  SQ d[q] = j2 ? yn[q] * (hinv * (d[q] - XA(B, q))) : (T)0;
   rt = xch_right(d[0], G, L, l, gs, bf.xr);
-  SQ d[q] = 24. * dx1 * (nxt(d, q, rt) - d[q]);
+  SQ d[q] = (T)24. * dx1 * (nxt(d, q, rt) - d[q]);
   psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
   B = X.put(pf);  // dvdx
-  TH ph[SEG];
-  X.pre_raw((const TH*)F.hh[1], 0, ph);
+  X.pre((const T*)F.hh[1], 0, pf);
   SQ e[q] = yn[q] * XA(B, q);
   rt = xch_right(e[0], G, L, l, gs, bf.xr);
-  SQ e[q] = 24. * dx1 * (nxt(e, q, rt) - e[q]);
+  SQ e[q] = (T)24. * dx1 * (nxt(e, q, rt) - e[q]);
   psolve<T, SEG>(e, G, L, l, gs, s22, bf.xf, bf.xb);
-  B = X.put(ph);  // HH2
-  X.pre(F.nue, 0, pf);
-  SQ d[q] = (T)(TH)((T)(TH)(XA(B, q) - d[q]) - e[q]);
-  X.out((TH*)F.hh[1], d, 2);
+  B = X.put(pf);  // HH2
+  X.pre((const T*)F.nue, 0, pf);
+  SQ d[q] = (XA(B, q) - d[q]) - e[q];
+  X.out((T*)F.hh[1], d, 2);
   B = X.put(pf);  // nue
-  X.pre(F.dwdx, 0, pf);
+  X.pre((const T*)F.dwdx, 0, pf);
   SQ yn[q] = XA(B, q);
   B = X.put(pf);  // dwdx
-  X.pre(F.dudz, 0, pf);
+  X.pre((const T*)F.dudz, 0, pf);
   SQ d[q] = yn[q] * XA(B, q);
   rt = xch_right(d[0], G, L, l, gs, bf.xr);
-  SQ d[q] = 24. * dx1 * (nxt(d, q, rt) - d[q]);
+  SQ d[q] = (T)24. * dx1 * (nxt(d, q, rt) - d[q]);
   psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
   B = X.put(pf);  // dudz
-  X.pre_raw((const TH*)F.hh[2], 0, ph);
+  X.pre((const T*)F.hh[2], 0, pf);
   SQ e[q] = yn[q] * XA(B, q);
   rt = xch_right(e[0], G, L, l, gs, bf.xr);
-  SQ e[q] = 24. * dx1 * (nxt(e, q, rt) - e[q]);
+  SQ e[q] = (T)24. * dx1 * (nxt(e, q, rt) - e[q]);
   psolve<T, SEG>(e, G, L, l, gs, s22, bf.xf, bf.xb);
-  B = X.put(ph);  // HH3
-  SQ d[q] = (T)(TH)((T)(TH)(XA(B, q) - d[q]) - e[q]);
-  X.out((TH*)F.hh[2], d);
+  B = X.put(pf);  // HH3
+  SQ d[q] = (XA(B, q) - d[q]) - e[q];
+  X.out((T*)F.hh[2], d);
 }
 
 // ------------------------------------------------------- test filters
@@ -1438,7 +1455,11 @@ void sgscs(const Ctx& c) {
   const int ntx = (c.g.nx + SG_TI - 1) / SG_TI;
   int kpt = 8;
   while (kpt > 1 && (long long)ntx * ((c.g.nz + 4 * kpt - 1) / (4 * kpt)) * c.g.n2m < 2048) kpt /= 2;
-  C3D_LAUNCH0(k_sgscs, dim3(ntx * ((c.g.nz + 4 * kpt - 1) / (4 * kpt)), c.g.n2m), dim3(256), 0, c, kpt);
+  const dim3 gr(ntx * ((c.g.nz + 4 * kpt - 1) / (4 * kpt)), c.g.n2m);
+  if (c.s.mixed)
+    C3D_LAUNCH(k_sgscs, (float), gr, dim3(256), 0, c, kpt);
+  else
+    C3D_LAUNCH(k_sgscs, (double), gr, dim3(256), 0, c, kpt);
 }
 
 int sgsdy_tiles(const Geo& g) {
@@ -1514,16 +1535,18 @@ void wallss(const Ctx& c) {
 
 // TH: storage type of HH (FP32 in mixed mode); the SGS terms are formed in
 // FP64 from the FP64 state and added to HH in FP64 arithmetic
-template <typename TH>
+// the SGS line kernels in the compute type T of the step's line kernels:
+// x-lines on the x shape (4-byte fields staged in pairs), z-lines on the
+// z shape of T (FP32 tiles: 32 columns, 16-point segments)
+template <typename T>
 static void sgs_terms_t(const Ctx& c) {
-  // the FP64 line shapes (x staged, z direct), in both precisions
-  const LShape &sx = c.o.lx, &sz = c.o.lz;
-  const size_t lx = sizeof(double) * (2 * (size_t)sx.n * (sx.L + 1) + 4 * (size_t)sx.G * sx.L);
-  const size_t lz = sizeof(double) * 4 * (size_t)sz.G * sz.L;
+  const LShape &sx = c.o.lx, &sz = sizeof(T) == 4 ? c.o.lz32 : c.o.lz;
+  const size_t lx = sizeof(T) * (2 * (size_t)sx.n * (sx.L + 1) + 4 * (size_t)sx.G * sx.L);
+  const size_t lz = sizeof(T) * 4 * (size_t)sz.G * sz.L;
   const int nxb = (c.g.n2m * c.g.nz + sx.L - 1) / sx.L;
   const int nit = (c.g.nx + sz.L - 1) / sz.L;
   const dim3 tx(sx.L * sx.G), tz(sz.L * sz.G);
-#define SEG_(seg, ...)         \
+#define SEGX_(seg, ...)        \
   if ((seg) == 8) {            \
     constexpr int S_ = 8;      \
     __VA_ARGS__;               \
@@ -1531,21 +1554,33 @@ static void sgs_terms_t(const Ctx& c) {
     constexpr int S_ = 2;      \
     __VA_ARGS__;               \
   }
+#define SEGZ_(seg, ...)                       \
+  if (sizeof(T) == 4 && (seg) == 16) {        \
+    constexpr int S_ = sizeof(T) == 4 ? 16 : 8; \
+    __VA_ARGS__;                              \
+  } else if ((seg) == 8) {                    \
+    constexpr int S_ = 8;                     \
+    __VA_ARGS__;                              \
+  } else {                                    \
+    constexpr int S_ = 2;                     \
+    __VA_ARGS__;                              \
+  }
   {
     KScope ks_(c, KID_SGS_X);
-    SEG_(sx.seg, C3D_LAUNCH(k_sgs_x, (TH, S_), dim3(nxb), tx, lx, c));
+    SEGX_(sx.seg, C3D_LAUNCH(k_sgs_x, (T, S_), dim3(nxb), tx, lx, c));
   }
   {
     KScope ks_(c, KID_SGS_Z);
-    SEG_(sz.seg, C3D_LAUNCH(k_sgs_z, (TH, S_), dim3(nit * c.g.n2m), tz, lz, c));
+    SEGZ_(sz.seg, C3D_LAUNCH(k_sgs_z, (T, S_), dim3(nit * c.g.n2m), tz, lz, c));
   }
   {
     KScope ks_(c, KID_SGS_Z2);
-    SEG_(sz.seg, C3D_LAUNCH(k_sgs_z2, (TH, S_), dim3(nit * (c.g.n2m - 1)), tz, lz, c));
+    SEGZ_(sz.seg, C3D_LAUNCH(k_sgs_z2, (T, S_), dim3(nit * (c.g.n2m - 1)), tz, lz, c));
   }
   KScope ks_(c, KID_SGS_X2);
-  SEG_(sx.seg, C3D_LAUNCH(k_sgs_x2, (TH, S_), dim3(nxb), tx, lx, c));
-#undef SEG_
+  SEGX_(sx.seg, C3D_LAUNCH(k_sgs_x2, (T, S_), dim3(nxb), tx, lx, c));
+#undef SEGX_
+#undef SEGZ_
 }
 void sgs_terms(const Ctx& c) {
   if (c.s.mixed)

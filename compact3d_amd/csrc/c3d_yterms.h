@@ -8,7 +8,9 @@
 // and returns the final RUH (it does not store RUH).  Reciprocals and
 // products of J-only metrics come from the per-J vectors the host formed
 // with the same expressions (Metrics: cu1, cu2, dyh, rdy, rh), so each term
-// evaluates as the reference's does without a division per point.
+// evaluates as the reference's does without a division per point.  The SGS
+// terms use the compute type T too: in mixed mode the FP32 shadows of the
+// state and of SGSVIS and the FP32 SGS intermediates.
 #pragma once
 #include "c3d_impl.h"
 
@@ -50,19 +52,22 @@ __device__ __forceinline__ T yterm_u(const Ctx& c, const YIdx& x, int i, int j, 
     // J=1..N2M enters only multiplied by a zero wall flag and is taken as 0
     const int jp = j + 1, jm = j - 1;
     const int im = wrapm(i, g.nx);
-    const double n0 = F.nux[x.b], np1 = (j < g.n2m) ? F.nux[x.bp] : 0.0, nm1 = (j > 1) ? F.nux[x.bm] : 0.0;
-    const double sgsj2 = M.cu2[j] * (M.dyh[j] * np1 + M.dyh[jp] * n0) +
-                         (1. - fjup) * 0.5 * (F.sgs[x.bN2] + F.sgs[gidx(g, im, g.n2, k)]);
-    const double sgsj1 = M.cu1[j] * (M.dyh[jm] * n0 + M.dyh[j] * nm1) +
-                         (1. - fjum) * 0.5 * (F.sgs[x.b0] + F.sgs[gidx(g, im, 0, k)]);
-    const double wfu = 1. - s.ifwall * (1. - fjup) * (s.aly - 1.);
-    const double wfd = 1. - s.ifwall * (1. - fjum);
-    const double uu22 = (F.u[x.bp] - F.u[x.b]) / M.h[jp];
-    const double uu11 = (F.u[x.b] - F.u[x.bm]) / M.h[j];
-    double hd = (double)hh - M.rdy[j] * (wfu * sgsj2 * uu22 - wfd * sgsj1 * uu11);
-    const double dvp = (j < g.n2m) ? F.dvdx[x.bp] : 0.0;
-    hd = hd - M.rdy[j] * (wfu * sgsj2 * dvp - wfd * sgsj1 * F.dvdx[x.b]);
-    hh = (T)hd;
+    const T* nux = (const T*)F.nux;
+    const T* sg = (const T*)F.ssgs;
+    const T* dvdx = (const T*)F.dvdx;
+    const T n0 = nux[x.b], np1 = (j < g.n2m) ? nux[x.bp] : (T)0, nm1 = (j > 1) ? nux[x.bm] : (T)0;
+    const T sgsj2 = (T)M.cu2[j] * ((T)M.dyh[j] * np1 + (T)M.dyh[jp] * n0) +
+                    (T)(1. - fjup) * (T)0.5 * (sg[x.bN2] + sg[gidx(g, im, g.n2, k)]);
+    const T sgsj1 = (T)M.cu1[j] * ((T)M.dyh[jm] * n0 + (T)M.dyh[j] * nm1) +
+                    (T)(1. - fjum) * (T)0.5 * (sg[x.b0] + sg[gidx(g, im, 0, k)]);
+    const T wfu = (T)(1. - s.ifwall * (1. - fjup) * (s.aly - 1.));
+    const T wfd = (T)(1. - s.ifwall * (1. - fjum));
+    const T uu22 = (up - u0) / (T)M.h[jp];
+    const T uu11 = (u0 - um) / (T)M.h[j];
+    const T rdy = (T)M.rdy[j];
+    T hd = hh - rdy * (wfu * sgsj2 * uu22 - wfd * sgsj1 * uu11);
+    const T dvp = (j < g.n2m) ? dvdx[x.bp] : (T)0;
+    hh = hd - rdy * (wfu * sgsj2 * dvp - wfd * sgsj1 * dvdx[x.b]);
   }
   ((T*)F.hh[0])[x.b] = hh;
   // wall stresses of the bottom and the top wall (:6765-6768): the flags
@@ -93,10 +98,11 @@ __device__ __forceinline__ T yterm_v(const Ctx& c, const YIdx& x, int i, int j, 
   T hh = ((const T*)F.hh[1])[x.b] + (T)M.rh[j] * (v2 * v2 - v1 * v1);
   if (s.files) {
     const int jm = j - 1;
-    const double g2 = M.rdy[j] * (F.v[x.bp] - F.v[x.b]);
-    const double g1 = M.rdy[jm] * (F.v[x.b] - F.v[x.bm]);
-    const double vis22 = M.rh[j] * (F.sgs[x.b] * g2 - F.sgs[x.bm] * g1);
-    hh = (T)((double)hh - 2. * vis22);
+    const T* sg = (const T*)F.ssgs;
+    const T g2 = (T)M.rdy[j] * (vp - v0);
+    const T g1 = (T)M.rdy[jm] * (v0 - vm);
+    const T vis22 = (T)M.rh[j] * (sg[x.b] * g2 - sg[x.bm] * g1);
+    hh = hh - (T)2. * vis22;
   }
   ((T*)F.hh[1])[x.b] = hh;
   return ru - (T)(3. / 2.) * hh + (T)(1. / 2.) * ((const T*)F.hhp[1])[x.b];
@@ -124,19 +130,22 @@ __device__ __forceinline__ T yterm_w(const Ctx& c, const YIdx& x, int i, int j, 
     // D(SGS*(DW/DY))/DY and D(SGS*(DV/DZ))/DY (:7858-7908, :8042-8094)
     const int jp = j + 1, jm = j - 1;
     const int km = wrapm(k, g.nz);
-    const double z0 = F.nuz[x.b], zp1 = (j < g.n2m) ? F.nuz[x.bp] : 0.0, zm1 = (j > 1) ? F.nuz[x.bm] : 0.0;
-    const double sgsj1 = M.cu1[j] * (M.dyh[jm] * z0 + M.dyh[j] * zm1) +
-                         (1. - fjum) * 0.5 * (F.sgs[x.b0] + F.sgs[gidx(g, i, 0, km)]);
-    const double sgsj2 = M.cu2[j] * (M.dyh[j] * zp1 + M.dyh[jp] * z0) +
-                         (1. - fjup) * 0.5 * (F.sgs[x.bN2] + F.sgs[gidx(g, i, g.n2, km)]);
-    const double wfu = 1. - s.ifwall * (1. - fjup) * (s.aly - 1.);
-    const double wfd = 1. - s.ifwall * (1. - fjum);
-    const double ww22 = M.rh[jp] * (F.w[x.bp] - F.w[x.b]);
-    const double ww11 = M.rh[j] * (F.w[x.b] - F.w[x.bm]);
-    double hd = (double)hh - M.rdy[j] * (wfu * sgsj2 * ww22 - wfd * sgsj1 * ww11);
-    const double dzp = (j < g.n2m) ? F.dvdz[x.bp] : 0.0;
-    hd = hd - M.rdy[j] * (wfu * sgsj2 * dzp - wfd * sgsj1 * F.dvdz[x.b]);
-    hh = (T)hd;
+    const T* nuz = (const T*)F.nuz;
+    const T* sg = (const T*)F.ssgs;
+    const T* dvdz = (const T*)F.dvdz;
+    const T z0 = nuz[x.b], zp1 = (j < g.n2m) ? nuz[x.bp] : (T)0, zm1 = (j > 1) ? nuz[x.bm] : (T)0;
+    const T sgsj1 = (T)M.cu1[j] * ((T)M.dyh[jm] * z0 + (T)M.dyh[j] * zm1) +
+                    (T)(1. - fjum) * (T)0.5 * (sg[x.b0] + sg[gidx(g, i, 0, km)]);
+    const T sgsj2 = (T)M.cu2[j] * ((T)M.dyh[j] * zp1 + (T)M.dyh[jp] * z0) +
+                    (T)(1. - fjup) * (T)0.5 * (sg[x.bN2] + sg[gidx(g, i, g.n2, km)]);
+    const T wfu = (T)(1. - s.ifwall * (1. - fjup) * (s.aly - 1.));
+    const T wfd = (T)(1. - s.ifwall * (1. - fjum));
+    const T ww22 = (T)M.rh[jp] * (wp - w0);
+    const T ww11 = (T)M.rh[j] * (w0 - wm);
+    const T rdy = (T)M.rdy[j];
+    T hd = hh - rdy * (wfu * sgsj2 * ww22 - wfd * sgsj1 * ww11);
+    const T dzp = (j < g.n2m) ? dvdz[x.bp] : (T)0;
+    hh = hd - rdy * (wfu * sgsj2 * dzp - wfd * sgsj1 * dvdz[x.b]);
   }
   ((T*)F.hh[2])[x.b] = hh;
   T bc = (T)0;
