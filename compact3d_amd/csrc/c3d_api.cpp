@@ -431,6 +431,8 @@ static void build_ytab(c3d_handle* h, const std::vector<double>& ych, std::vecto
         t[2] = h->DYM[J];
         t[3] = h->H[J];
         t[4] = 1. / h->H[J];
+        t[5] = 1. / h->DY[J];  // the SGS term's 1/DY(J), 1/DY(J-1)
+        t[6] = J >= 2 ? 1. / h->DY[J - 1] : 0.0;
       } else {
         t[0] = h->HP[J];
         t[1] = h->HC[J];
@@ -443,6 +445,8 @@ static void build_ytab(c3d_handle* h, const std::vector<double>& ych, std::vecto
         t[8] = 1. / h->DY[J];
         t[9] = 1. - h->FJPA[J];
         t[10] = 1. - h->FJMU[J];
+        t[14] = 1. / h->H[J];  // the SGS terms' 1/H(J), 1/H(J+1)
+        t[15] = 1. / h->H[J + 1];
       }
       t[YT_BET] = BET[J];
       t[YT_M] = (J == ns) ? 0.0 : -A[J] * BET[J];

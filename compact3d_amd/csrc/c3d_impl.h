@@ -38,8 +38,9 @@ struct Metrics {
   const double *cu2, *cu1, *dyh, *rdy, *rh;
   // per-component, per-J coefficients of the fused RHS-y / ADI-y sweep,
   // YTAB_W doubles per J (index comp*(N2+2)*YTAB_W + J*YTAB_W + slot):
-  //   u, w: HP HC HM CU2 CU1 DYH(J-1) DYH(J) DYH(J+1) RDY 1-FJPA 1-FJMU
-  //   v:    DYP DYC DYM H RH
+  //   u, w: HP HC HM CU2 CU1 DYH(J-1) DYH(J) DYH(J+1) RDY 1-FJPA 1-FJMU,
+  //         slot 14 1/H(J), 15 1/H(J+1)
+  //   v:    DYP DYC DYM H RH 1/DY(J) 1/DY(J-1)
   //   all:  slot 11 BET, 12 -A*BET (0 at the first row), 13 -GAM (0 at N2M)
   const double* ytab;
   const float* ytab32;  // ytab rounded to FP32, for the FP32 sweep

@@ -424,10 +424,14 @@ __device__ __forceinline__ T wall_bc(const Ctx& c, const double* tau, unsigned t
 }
 
 // RUH, HH and HH2 are read once and HH written once per step: their loads
-// and stores carry the nontemporal hint (measured 1-1.5 % faster than plain)
-template <typename T, int CH>
+// and stores carry the nontemporal hint (measured 1-1.5 % faster than plain).
+// With SGS (FILES != 0) the SGS y terms of c3d_yterms.h join HH before it is
+// stored, their operands loaded with the chunk's (SGSVIS at the wall rows
+// only where the chunk holds J = 1 or N2M, the only rows whose weight is
+// not zero); im: the column i-1 (periodic) of SGSVIS's wall rows.
+template <typename T, int CH, bool SGS = false>
 __device__ __forceinline__ void yfuse_u(const Ctx& c, const T* tb, unsigned b, unsigned t2, const YPos& y,
-                                        T (&v)[CH]) {
+                                        T (&v)[CH], int di = 0) {
   const Geo& g = c.g;
   const Fields& F = c.d;
   const T* __restrict__ su = (const T*)F.su;
@@ -452,6 +456,21 @@ __device__ __forceinline__ void yfuse_u(const Ctx& c, const T* tb, unsigned b, u
     hx[q] = in ? __builtin_nontemporal_load(&hh[a0 + q * nx]) : (T)0;
     hq[q] = in ? __builtin_nontemporal_load(&hp[a0 + q * nx]) : (T)0;
   }
+  // SGS operands: nux at J0-1..J0+len, dv/dx at J0..J0+len, and the sums
+  // of SGSVIS at (I,0,K) + (I-1,0,K) and (I,N2,K) + (I-1,N2,K)
+  T nu[SGS ? CH + 2 : 1], dv[SGS ? CH + 1 : 1], sgb = (T)0, sgt = (T)0;
+  if constexpr (SGS) {
+    const T* __restrict__ nux = (const T*)F.nux;
+    const T* __restrict__ dvdx = (const T*)F.dvdx;
+    const T* __restrict__ sg = (const T*)F.ssgs;
+#pragma unroll
+    for (int q = 0; q < CH + 2; q++) nu[q] = (q < y.len + 2) ? nux[a0 - nx + q * nx] : (T)0;
+#pragma unroll
+    for (int q = 0; q < CH + 1; q++) dv[q] = (q < y.len + 1) ? dvdx[a0 + q * nx] : (T)0;
+    const unsigned bim = b + di;
+    if (y.j0 == 1) sgb = sg[b] + sg[bim];
+    if (y.j0 + y.len - 1 == g.n2m) sgt = sg[b + g.n2 * nx] + sg[bim + g.n2 * nx];
+  }
 #pragma unroll
   for (int q = 0; q < CH; q++)
     if (q < y.len) {
@@ -463,14 +482,27 @@ __device__ __forceinline__ void yfuse_u(const Ctx& c, const T* tb, unsigned b, u
       const T u1 = (T)t[4] * ((T)t[5] * u0 + (T)t[6] * um) + (T)t[10] * uW;
       const T v2 = (J == g.n2m) ? (T)0 : w[q + 1];
       const T v1 = (J == 1) ? (T)0 : w[q];
-      const T h = hx[q] + (T)t[8] * (v2 * u2 - v1 * u1);
+      T h = hx[q] + (T)t[8] * (v2 * u2 - v1 * u1);
+      if constexpr (SGS) {
+        // D(SGS*(DU/DY))/DY and D(SGS*(DV/DX))/DY (:6551-6680), as yterm_u
+        const T n0 = nu[q + 1], np1 = (J < g.n2m) ? nu[q + 2] : (T)0, nm1 = (J > 1) ? nu[q] : (T)0;
+        const T sgsj2 = (T)t[3] * ((T)t[6] * np1 + (T)t[7] * n0) + (T)t[9] * (T)0.5 * sgt;
+        const T sgsj1 = (T)t[4] * ((T)t[5] * n0 + (T)t[6] * nm1) + (T)t[10] * (T)0.5 * sgb;
+        const T wfu = (T)1 - (T)c.s.ifwall * (T)t[9] * (T)(c.s.aly - 1.);
+        const T wfd = (T)1 - (T)c.s.ifwall * (T)t[10];
+        const T uu22 = (up - u0) * (T)t[15];
+        const T uu11 = (u0 - um) * (T)t[14];
+        const T hd = h - (T)t[8] * (wfu * sgsj2 * uu22 - wfd * sgsj1 * uu11);
+        const T dvp = (J < g.n2m) ? dv[q + 1] : (T)0;
+        h = hd - (T)t[8] * (wfu * sgsj2 * dvp - wfd * sgsj1 * dv[q]);
+      }
       __builtin_nontemporal_store(h, &hh[a0 + q * nx]);
       const T r = r0 - (T)(3. / 2.) * h + (T)(1. / 2.) * hq[q] + wall_bc<T>(c, F.tauw, t2, J, t);
       v[q] = (T)t[YT_BET] * (r * dt);
     }
 }
 
-template <typename T, int CH>
+template <typename T, int CH, bool SGS = false>
 __device__ __forceinline__ void yfuse_v(const Ctx& c, const T* tb, unsigned b, const YPos& y, T (&v)[CH]) {
   const Fields& F = c.d;
   const T* __restrict__ sv = (const T*)F.sv;
@@ -493,6 +525,13 @@ __device__ __forceinline__ void yfuse_v(const Ctx& c, const T* tb, unsigned b, c
     hx[q] = in ? __builtin_nontemporal_load(&hh[a0 + q * nx]) : (T)0;
     hq[q] = in ? __builtin_nontemporal_load(&hp[a0 + q * nx]) : (T)0;
   }
+  // SGS operand: SGSVIS at J0-1..J0+len-1
+  T sgv[SGS ? CH + 1 : 1];
+  if constexpr (SGS) {
+    const T* __restrict__ sg = (const T*)c.d.ssgs;
+#pragma unroll
+    for (int q = 0; q < CH + 1; q++) sgv[q] = (q < y.len + 1) ? sg[a0 - nx + q * nx] : (T)0;
+  }
 #pragma unroll
   for (int q = 0; q < CH; q++)
     if (q < y.len) {
@@ -504,16 +543,23 @@ __device__ __forceinline__ void yfuse_v(const Ctx& c, const T* tb, unsigned b, c
       const T r0 = ru[q] + (-pressg2 + viscos);
       const T v2 = (T)0.5 * (vp + v0);
       const T v1 = (T)0.5 * (v0 + vm);
-      const T h = hx[q] + (T)t[4] * (v2 * v2 - v1 * v1);
+      T h = hx[q] + (T)t[4] * (v2 * v2 - v1 * v1);
+      if constexpr (SGS) {
+        // 2 D(SGS*(DV/DY))/DY (as yterm_v); t[5] = 1/DY(J), t[6] = 1/DY(J-1)
+        const T g2 = (T)t[5] * (vp - v0);
+        const T g1 = (T)t[6] * (v0 - vm);
+        const T vis22 = (T)t[4] * (sgv[q + 1] * g2 - sgv[q] * g1);
+        h = h - (T)2. * vis22;
+      }
       __builtin_nontemporal_store(h, &hh[a0 + q * nx]);
       const T r = r0 - (T)(3. / 2.) * h + (T)(1. / 2.) * hq[q];
       v[q] = (T)t[YT_BET] * (r * dt);
     }
 }
 
-template <typename T, int CH>
+template <typename T, int CH, bool SGS = false>
 __device__ __forceinline__ void yfuse_w(const Ctx& c, const T* tb, unsigned b, unsigned t2, const YPos& y,
-                                        T (&v)[CH]) {
+                                        T (&v)[CH], int dk = 0) {
   const Geo& g = c.g;
   const Fields& F = c.d;
   const T* __restrict__ sw = (const T*)F.sw;
@@ -538,6 +584,21 @@ __device__ __forceinline__ void yfuse_w(const Ctx& c, const T* tb, unsigned b, u
     hx[q] = in ? __builtin_nontemporal_load(&hh[a0 + q * nx]) : (T)0;
     hq[q] = in ? __builtin_nontemporal_load(&hp[a0 + q * nx]) : (T)0;
   }
+  // SGS operands: nuz at J0-1..J0+len, dv/dz at J0..J0+len, and the sums
+  // of SGSVIS at (I,0,K) + (I,0,K-1) and (I,N2,K) + (I,N2,K-1)
+  T nz[SGS ? CH + 2 : 1], dz[SGS ? CH + 1 : 1], sgb = (T)0, sgt = (T)0;
+  if constexpr (SGS) {
+    const T* __restrict__ nuz = (const T*)F.nuz;
+    const T* __restrict__ dvdz = (const T*)F.dvdz;
+    const T* __restrict__ sg = (const T*)F.ssgs;
+#pragma unroll
+    for (int q = 0; q < CH + 2; q++) nz[q] = (q < y.len + 2) ? nuz[a0 - nx + q * nx] : (T)0;
+#pragma unroll
+    for (int q = 0; q < CH + 1; q++) dz[q] = (q < y.len + 1) ? dvdz[a0 + q * nx] : (T)0;
+    const unsigned bkm = b + dk;
+    if (y.j0 == 1) sgb = sg[b] + sg[bkm];
+    if (y.j0 + y.len - 1 == g.n2m) sgt = sg[b + g.n2 * nx] + sg[bkm + g.n2 * nx];
+  }
 #pragma unroll
   for (int q = 0; q < CH; q++)
     if (q < y.len) {
@@ -549,7 +610,21 @@ __device__ __forceinline__ void yfuse_w(const Ctx& c, const T* tb, unsigned b, u
       const T w1 = (T)t[4] * ((T)t[6] * wm + (T)t[5] * w0) + (T)t[10] * wW;
       const T v2 = (J == g.n2m) ? (T)0 : s3[q + 1];
       const T v1 = (J == 1) ? (T)0 : s3[q];
-      const T h = hx[q] + (T)t[8] * (v2 * w2 - v1 * w1);
+      T h = hx[q] + (T)t[8] * (v2 * w2 - v1 * w1);
+      if constexpr (SGS) {
+        // D(SGS*(DW/DY))/DY and D(SGS*(DV/DZ))/DY (:7858-7908, :8042-8094),
+        // as yterm_w
+        const T z0 = nz[q + 1], zp1 = (J < g.n2m) ? nz[q + 2] : (T)0, zm1 = (J > 1) ? nz[q] : (T)0;
+        const T sgsj1 = (T)t[4] * ((T)t[5] * z0 + (T)t[6] * zm1) + (T)t[10] * (T)0.5 * sgb;
+        const T sgsj2 = (T)t[3] * ((T)t[6] * zp1 + (T)t[7] * z0) + (T)t[9] * (T)0.5 * sgt;
+        const T wfu = (T)1 - (T)c.s.ifwall * (T)t[9] * (T)(c.s.aly - 1.);
+        const T wfd = (T)1 - (T)c.s.ifwall * (T)t[10];
+        const T ww22 = (T)t[15] * (wp - w0);
+        const T ww11 = (T)t[14] * (w0 - wm);
+        const T hd = h - (T)t[8] * (wfu * sgsj2 * ww22 - wfd * sgsj1 * ww11);
+        const T dzp = (J < g.n2m) ? dz[q + 1] : (T)0;
+        h = hd - (T)t[8] * (wfu * sgsj2 * dzp - wfd * sgsj1 * dz[q]);
+      }
       __builtin_nontemporal_store(h, &hh[a0 + q * nx]);
       const T r = r0 - (T)(3. / 2.) * h + (T)(1. / 2.) * hq[q] + wall_bc<T>(c, F.taub, t2, J, t);
       v[q] = (T)t[YT_BET] * (r * dt);
@@ -600,22 +675,12 @@ __global__ __launch_bounds__(NT) void k_rhsy_adiy(CtxArg<CB_> ca_, int staged) {
 #pragma unroll
   for (int q = 0; q < CH; q++) v[q] = (T)0;
   if (ok && y.len > 0) {
-    if constexpr (SGS) {
-#pragma unroll
-      for (int q = 0; q < CH; q++)
-        if (q < y.len) {
-          const int J = y.j0 + q;
-          const YIdx x = yidx(g, i, J, k);
-          const T r = comp == 0 ? yterm_u<T>(c, x, i, J, k)
-                                : comp == 1 ? yterm_v<T>(c, x, i, J, k) : yterm_w<T>(c, x, i, J, k);
-          v[q] = (T)tb[J * YTAB_W + YT_BET] * (r * dt);
-        }
-    } else if (comp == 0) {
-      yfuse_u<T, CH>(c, tb, b, t2, y, v);
+    if (comp == 0) {
+      yfuse_u<T, CH, SGS>(c, tb, b, t2, y, v, (i == 0 ? g.nx - 1 : i - 1) - i);
     } else if (comp == 1) {
-      yfuse_v<T, CH>(c, tb, b, y, v);
+      yfuse_v<T, CH, SGS>(c, tb, b, y, v);
     } else {
-      yfuse_w<T, CH>(c, tb, b, t2, y, v);
+      yfuse_w<T, CH, SGS>(c, tb, b, t2, y, v, ((k == 0 ? g.nz - 1 : k - 1) - k) * (int)g.plane);
     }
   }
   ychain_fwd(v, [&](int q) { return (T)tb[(y.j0 + q) * YTAB_W + YT_M]; }, y, L, xs[0], xs[1]);
