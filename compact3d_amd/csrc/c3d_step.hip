@@ -1619,9 +1619,20 @@ __global__ __launch_bounds__(512) void k_upd_z(CtxArg<CB_> ca_, int chk, int ref
   const T dx3 = (T)s.dx3, dt = (T)s.dt;
   const T dmp3 = refine ? (T)0 : (T)F.sc[SC_DMPRESG3];
   const T ku = refine ? (T)0 : (T)1;
-  T dpr[SEG], d[SEG], wn[SEG], lf, rt;
-  zload(F.dp, b, pl, p0, bm, bp, dpr, lf, rt);
-  SQ d[q] = (T)24. * dx3 * (-prv(dpr, q, lf) + dpr[q]);
+  T d[SEG], wn[SEG], lf, rt;
+  {
+    // P += DP first, so that DP's segment is dead during the solve
+    T dpr[SEG];
+    zload(F.dp, b, pl, p0, bm, bp, dpr, lf, rt);
+    SQ d[q] = (T)24. * dx3 * (-prv(dpr, q, lf) + dpr[q]);
+    if (ok) {
+      SQ {
+        const double pn = F.p[ZI(q)] + dpr[q];
+        F.p[ZI(q)] = pn;
+        if (MIX) F.pf[ZI(q)] = (float)pn;
+      }
+    }
+  }
   psolve<T, SEG>(d, G, L, l, gs, PZ.s22, bf.xf, bf.xb);
   SQ wn[q] = ku * (T)((const TS*)F.uh[2])[ZI(q)] - dt * d[q] - dt * dmp3;
   // mixed mode: the flow rate of W before the update (FLOW3, :1168), summed
@@ -1638,12 +1649,7 @@ __global__ __launch_bounds__(512) void k_upd_z(CtxArg<CB_> ca_, int chk, int ref
   if (ok) {
     SQ {
       F.w[ZI(q)] = wn[q];
-      const double pn = F.p[ZI(q)] + dpr[q];
-      F.p[ZI(q)] = pn;
-      if (MIX) {
-        F.wf[ZI(q)] = (float)wn[q];
-        F.pf[ZI(q)] = (float)pn;
-      }
+      if (MIX) F.wf[ZI(q)] = (float)wn[q];
     }
   }
   if (!chk) {
