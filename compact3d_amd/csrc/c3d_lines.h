@@ -59,8 +59,10 @@ __device__ __forceinline__ LBuf<T> lbuf(unsigned char* sm, const LShape& sh, boo
 // as an FP64 one (single 4-byte loads halve the bytes in flight per
 // instruction, which measured ~20 % slower in the FP32 line kernels).  A
 // 4-byte tile (T = float) is always staged paired; an FP64 tile takes a
-// paired 4-byte field through pre_raw() and put().
-template <typename T, int SEG>
+// paired 4-byte field through pre_raw() and put(), and with PAIR stages
+// every field paired (FP64 pairs as 16-byte loads and stores), which keeps
+// one index set live instead of two.
+template <typename T, int SEG, bool PAIR = (sizeof(T) == 4)>
 struct XStage {
   T *st0, *st1;
   int sb;
@@ -85,14 +87,15 @@ struct XStage {
     for (int m = 0; m < SEG / 2; m++) {
       S x0 = (S)0, x1 = (S)0;
       if ((vm2 >> m) & 1u) {
+        // the offset is summed in 32 bits first: off wraps for joff < 0
         if constexpr (sizeof(S) == 4) {
-          // the offset is summed in 32 bits first: off wraps for joff < 0
           const float2 v = *(const float2*)(F + (go2[m] + off));
           x0 = (S)v.x;
           x1 = (S)v.y;
         } else {
-          x0 = F[go2[m] + off];
-          x1 = F[go2[m] + off + 1];
+          const double2 v = *(const double2*)(F + (go2[m] + off));
+          x0 = (S)v.x;
+          x1 = (S)v.y;
         }
       }
       r[2 * m] = x0;
@@ -110,7 +113,7 @@ struct XStage {
   template <typename S>
   __device__ __forceinline__ void pre(const S* F, int joff, T (&r)[SEG]) const {
     const unsigned off = (unsigned)(joff * nx);  // row j+joff exists for every j (ghost planes)
-    if constexpr (sizeof(T) == 4) {
+    if constexpr (PAIR) {
       S t[SEG];
       pre_pairs(F, off, t);
       SQ r[q] = (T)t[q];
@@ -121,7 +124,7 @@ struct XStage {
   __device__ __forceinline__ const T* put(const T (&r)[SEG]) {
     T* B = next();
     __syncthreads();
-    if constexpr (sizeof(T) == 4)
+    if constexpr (PAIR)
       put_pairs(B, r);
     else
       SQ B[li0 + q * lstep] = r[q];
@@ -134,7 +137,7 @@ struct XStage {
   template <typename S>
   __device__ __forceinline__ void pre_raw(const S* F, int joff, S (&r)[SEG]) const {
     const unsigned off = (unsigned)(joff * nx);
-    if constexpr (sizeof(S) == 4 || sizeof(T) == 4)
+    if constexpr (sizeof(S) == 4 || PAIR)
       pre_pairs(F, off, r);
     else
       SQ r[q] = ((vm >> q) & 1u) ? F[go[q] + off] : (S)0;
@@ -143,7 +146,7 @@ struct XStage {
   __device__ __forceinline__ const T* put(const S (&r)[SEG]) {
     T* B = next();
     __syncthreads();
-    if constexpr (sizeof(S) == 4 || sizeof(T) == 4)
+    if constexpr (sizeof(S) == 4 || PAIR)
       put_pairs(B, r);
     else
       SQ B[li0 + q * lstep] = (T)r[q];
@@ -173,12 +176,10 @@ struct XStage {
     for (int m = 0; m < SEG / 2; m++)
       if (keep2(m, jmin)) {
         const T x0 = B[li2 + m * lstep2], x1 = B[li2 + LS + m * lstep2];
-        if constexpr (sizeof(S) == 4) {
+        if constexpr (sizeof(S) == 4)
           *(float2*)(F + go2[m]) = float2{(float)x0, (float)x1};
-        } else {
-          F[go2[m]] = (S)x0;
-          F[go2[m] + 1] = (S)x1;
-        }
+        else
+          *(double2*)(F + go2[m]) = double2{(double)x0, (double)x1};
       }
   }
   // the same tile to two arrays (a field and its FP32 shadow)
@@ -188,7 +189,7 @@ struct XStage {
     __syncthreads();
     SQ B[(p0 + q) * LS + l] = v[q];
     __syncthreads();
-    if constexpr (sizeof(T) == 4) {
+    if constexpr (PAIR) {
       out_pairs(F1, B, jmin);
       out_pairs(F2, B, jmin);
     } else {
@@ -206,7 +207,7 @@ struct XStage {
     __syncthreads();
     SQ B[(p0 + q) * LS + l] = v[q];
     __syncthreads();
-    if constexpr (sizeof(T) == 4)
+    if constexpr (PAIR)
       out_pairs(F, B, jmin);
     else
       SQ if (keep(q, jmin)) F[go[q]] = (S)B[li0 + q * lstep];
@@ -224,8 +225,8 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
   return x * per + min(x, rem) + r;
 }
 
-template <typename T, int SEG>
-__device__ __forceinline__ void xstage(XStage<T, SEG>& x, const LBuf<T>& b, const Geo& g, const LShape& sh,
+template <typename T, int SEG, bool PAIR>
+__device__ __forceinline__ void xstage(XStage<T, SEG, PAIR>& x, const LBuf<T>& b, const Geo& g, const LShape& sh,
                                        int l, int p0) {
   x.st0 = b.st0;
   x.st1 = b.st1;
@@ -273,14 +274,16 @@ __device__ __forceinline__ void xstage(XStage<T, SEG>& x, const LBuf<T>& b, cons
   }
 }
 
-#define XSETUP                                                        \
+#define XSETUP XSETUP_P(sizeof(T) == 4)
+// the same with every field staged in the paired layout when PAIR_
+#define XSETUP_P(PAIR_)                                               \
   extern __shared__ __align__(16) unsigned char smem[];               \
   const Geo g = c.g;                                                  \
   const LShape sh = c.o.lx;                                           \
   const int L = sh.L, G = sh.G;                                       \
   const LBuf<T> bf = lbuf<T>(smem, sh, true);                         \
   const int l = threadIdx.x % L, gs = threadIdx.x / L, p0 = gs * SEG; \
-  XStage<T, SEG> X;                                                   \
+  XStage<T, SEG, (PAIR_)> X;                                          \
   xstage(X, bf, g, sh, l, p0);                                        \
   const int ln = blockIdx.x * L + l;                                  \
   const bool ok = ln < g.n2m * g.nz;                                  \

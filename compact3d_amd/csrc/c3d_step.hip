@@ -785,7 +785,7 @@ __global__ __launch_bounds__(512) void k_adi_z(CtxArg<CB_> ca_, CompList cl) {
 template <typename T, int SEG, typename TS, bool CB_>
 __global__ __launch_bounds__(256) void k_rhsdp_x(CtxArg<CB_> ca_) {
   const Ctx c = ctx_get(ca_);
-  XSETUP;
+  XSETUP_P(sizeof(TS) == 4);  // mixed: FP32 operands and the FP64 RDP, all paired
   constexpr bool MIX = sizeof(TS) == 4;
   const Fields F = c.d;
   const Metrics M = c.m;
@@ -1531,7 +1531,7 @@ template <int SEG, bool CB_>
 __global__ __launch_bounds__(256) void k_upd_xu(CtxArg<CB_> ca_, int refine) {
   const Ctx c = ctx_get(ca_);
   typedef double T;
-  XSETUP;
+  XSETUP_P(true);  // FP64 and FP32 fields, all paired
   const Scal s = c.s;
   const Metrics M = c.m;
   const Fields F = c.d;
@@ -1543,9 +1543,9 @@ __global__ __launch_bounds__(256) void k_upd_xu(CtxArg<CB_> ca_, int refine) {
   X.pre(F.dp, 0, pf);
   const T* B = X.put(pf);
   X.pre_raw((const float*)F.uh[0], 0, ph);
-  X.pre(F.u, 0, pg);
   SQ d[q] = (T)24. * dx1 * (-XA(B, q - 1) + XA(B, q));
   psolve<T, SEG>(d, G, L, l, gs, c.o.px.s22, bf.xf, bf.xb);
+  X.pre(F.u, 0, pg);  // after the solve: the old U tile is not live across it
   B = X.put(ph);
   SQ un[q] = ku * XA(B, q) - dt * d[q] - dt * dmp;
   // the flow rate of U before the update (FLOW1, :1081)
