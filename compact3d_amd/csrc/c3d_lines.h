@@ -274,7 +274,7 @@ __device__ __forceinline__ void xstage(XStage<T, SEG, PAIR>& x, const LBuf<T>& b
   }
 }
 
-#define XSETUP XSETUP_P(sizeof(T) == 4)
+#define XSETUP XSETUP_P(true)
 // the same with every field staged in the paired layout when PAIR_
 #define XSETUP_P(PAIR_)                                               \
   extern __shared__ __align__(16) unsigned char smem[];               \

@@ -785,7 +785,7 @@ __global__ __launch_bounds__(512) void k_adi_z(CtxArg<CB_> ca_, CompList cl) {
 template <typename T, int SEG, typename TS, bool CB_>
 __global__ __launch_bounds__(256) void k_rhsdp_x(CtxArg<CB_> ca_) {
   const Ctx c = ctx_get(ca_);
-  XSETUP_P(sizeof(TS) == 4);  // mixed: FP32 operands and the FP64 RDP, all paired
+  XSETUP;
   constexpr bool MIX = sizeof(TS) == 4;
   const Fields F = c.d;
   const Metrics M = c.m;
