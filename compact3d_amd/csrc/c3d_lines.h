@@ -44,37 +44,29 @@ __device__ __forceinline__ LBuf<T> lbuf(unsigned char* sm, const LShape& sh, boo
 
 // x-line staging.  Lines ln = ln0 + l (l < L) are rows (j, k) with
 // j = 1 + ln % N2M, k = ln / N2M.  A tile holds n*L = SEG*blockDim values, so
-// each thread moves exactly SEG of them: element e = t + m*blockDim is row
-// e / n, point e % n.  The host makes blockDim a multiple of n (L a multiple
-// of SEG), so a thread's elements share the point p = t % n and lie in rows
-// t/n + m*(blockDim/n): their LDS indices are li0 + m*lstep.  pre() issues
-// the global loads of a tile into registers (so they can be in flight while
-// the previous tile is used); put() writes them to LDS.  Two LDS tiles
+// each thread moves exactly SEG of them, as SEG/2 pairs of neighbouring
+// points: pair m of thread t holds points 2*(t % (n/2)) and the next one of
+// tile row t/(n/2) + m*(2*blockDim/n) (the host makes blockDim a multiple of
+// n and n is even), one 8-byte (FP32) or 16-byte (FP64) load or store per
+// pair.  A wave's load then carries at least 512 bytes: single 4-byte loads
+// halved the bytes in flight per instruction and measured ~20 % slower in
+// the FP32 line kernels, and 16-byte FP64 pairs beat single 8-byte loads by
+// 8-13 %.  pre() issues the global loads of a tile into registers (so they
+// can be in flight while the previous tile is used); put() writes them to
+// LDS, converting to the tile's type; pre_raw() keeps a field's own type in
+// the registers, so that the conversion waits for put().  Two LDS tiles
 // alternate and every put()/out() starts with a barrier, so a tile stays
 // readable until the second staging operation after the one that filled it
 // begins.  Offsets are 32-bit (fields hold fewer than 2^31 elements).
-// 4-byte tiles use a paired layout instead: thread t moves SEG/2 pairs of
-// neighbouring points (2*(t % (n/2)), +1) of rows t/(n/2) + m*(2*blockDim/n),
-// one 8-byte load or store per pair, so a wave's load carries as many bytes
-// as an FP64 one (single 4-byte loads halve the bytes in flight per
-// instruction, which measured ~20 % slower in the FP32 line kernels).  A
-// 4-byte tile (T = float) is always staged paired; an FP64 tile takes a
-// paired 4-byte field through pre_raw() and put(), and with PAIR stages
-// every field paired (FP64 pairs as 16-byte loads and stores), which keeps
-// one index set live instead of two.
-template <typename T, int SEG, bool PAIR = (sizeof(T) == 4)>
+template <typename T, int SEG>
 struct XStage {
   T *st0, *st1;
   int sb;
   int nx, LS;
-  int l, p0, pm, pp;   // this thread's line, segment start, and periodic neighbours
-  unsigned go[SEG];    // element offset of element m (row j)
-  unsigned vm;         // bit m set: element m exists (its line is in the grid)
-  int li0, lstep;      // LDS index of element m: li0 + m*lstep
-  unsigned j1;         // bit m set: element m lies in row j = 1
-  // the paired layout: pair m holds points 2*pq, 2*pq+1 of a row
-  unsigned go2[SEG / 2];  // element offset of pair m's first point
-  unsigned vm2, j12;      // as vm and j1, per pair
+  int l, p0, pm, pp;      // this thread's line, segment start, and periodic neighbours
+  unsigned go2[SEG / 2];  // element offset of pair m's first point (row j)
+  unsigned vm2;           // bit m set: pair m exists (its line is in the grid)
+  unsigned j12;           // bit m set: pair m lies in row j = 1
   int li2, lstep2;        // LDS index of pair m's first point: li2 + m*lstep2 (second: + LS)
   __device__ __forceinline__ T* next() {
     sb ^= 1;
@@ -83,6 +75,7 @@ struct XStage {
   // pair m of r holds r[2m], r[2m+1]
   template <typename S>
   __device__ __forceinline__ void pre_pairs(const S* F, unsigned off, S (&r)[SEG]) const {
+    static_assert(sizeof(S) == 4 || sizeof(S) == 8, "4- or 8-byte fields");
 #pragma unroll
     for (int m = 0; m < SEG / 2; m++) {
       S x0 = (S)0, x1 = (S)0;
@@ -103,69 +96,32 @@ struct XStage {
     }
   }
   template <typename S>
-  __device__ __forceinline__ void put_pairs(T* B, const S (&r)[SEG]) const {
-#pragma unroll
-    for (int m = 0; m < SEG / 2; m++) {
-      B[li2 + m * lstep2] = (T)r[2 * m];
-      B[li2 + LS + m * lstep2] = (T)r[2 * m + 1];
-    }
-  }
-  template <typename S>
   __device__ __forceinline__ void pre(const S* F, int joff, T (&r)[SEG]) const {
     const unsigned off = (unsigned)(joff * nx);  // row j+joff exists for every j (ghost planes)
-    if constexpr (PAIR) {
-      S t[SEG];
-      pre_pairs(F, off, t);
-      SQ r[q] = (T)t[q];
-    } else {
-      SQ r[q] = ((vm >> q) & 1u) ? (T)F[go[q] + off] : (T)0;
-    }
+    S t[SEG];
+    pre_pairs(F, off, t);
+    SQ r[q] = (T)t[q];
   }
-  __device__ __forceinline__ const T* put(const T (&r)[SEG]) {
-    T* B = next();
-    __syncthreads();
-    if constexpr (PAIR)
-      put_pairs(B, r);
-    else
-      SQ B[li0 + q * lstep] = r[q];
-    __syncthreads();
-    return B;
-  }
-  // a tile of another storage type: prefetched as stored (pre_raw), so the
-  // loads stay in flight until put() converts them on the way into LDS;
-  // 4-byte fields in the paired layout
   template <typename S>
   __device__ __forceinline__ void pre_raw(const S* F, int joff, S (&r)[SEG]) const {
-    const unsigned off = (unsigned)(joff * nx);
-    if constexpr (sizeof(S) == 4 || PAIR)
-      pre_pairs(F, off, r);
-    else
-      SQ r[q] = ((vm >> q) & 1u) ? F[go[q] + off] : (S)0;
+    pre_pairs(F, (unsigned)(joff * nx), r);
   }
   template <typename S>
   __device__ __forceinline__ const T* put(const S (&r)[SEG]) {
     T* B = next();
     __syncthreads();
-    if constexpr (sizeof(S) == 4 || PAIR)
-      put_pairs(B, r);
-    else
-      SQ B[li0 + q * lstep] = (T)r[q];
+#pragma unroll
+    for (int m = 0; m < SEG / 2; m++) {
+      B[li2 + m * lstep2] = (T)r[2 * m];
+      B[li2 + LS + m * lstep2] = (T)r[2 * m + 1];
+    }
     __syncthreads();
     return B;
-  }
-  template <typename S>
-  __device__ __forceinline__ const T* in(const S* F, int joff) {
-    T r[SEG];
-    pre(F, joff, r);
-    return put(r);
   }
   // element at segment offset q in [-1, SEG] of this thread's line
   __device__ __forceinline__ T at(const T* B, int q) const {
     const int p = q < 0 ? pm : (q >= SEG ? pp : p0 + q);
     return B[p * LS + l];
-  }
-  __device__ __forceinline__ bool keep(int q, int jmin) const {
-    return ((vm >> q) & 1u) && (jmin <= 1 || !((j1 >> q) & 1u));
   }
   __device__ __forceinline__ bool keep2(int m, int jmin) const {
     return ((vm2 >> m) & 1u) && (jmin <= 1 || !((j12 >> m) & 1u));
@@ -182,35 +138,24 @@ struct XStage {
           *(double2*)(F + go2[m]) = double2{(double)x0, (double)x1};
       }
   }
-  // the same tile to two arrays (a field and its FP32 shadow)
+  // the same tile to two arrays (a field and its FP32 shadow); rows with
+  // j < jmin are left untouched
   template <typename S1, typename S2>
   __device__ __forceinline__ void out2(S1* F1, S2* F2, const T (&v)[SEG], int jmin = 1) {
     T* B = next();
     __syncthreads();
     SQ B[(p0 + q) * LS + l] = v[q];
     __syncthreads();
-    if constexpr (PAIR) {
-      out_pairs(F1, B, jmin);
-      out_pairs(F2, B, jmin);
-    } else {
-      SQ if (keep(q, jmin)) {
-        const T x = B[li0 + q * lstep];
-        F1[go[q]] = (S1)x;
-        F2[go[q]] = (S2)x;
-      }
-    }
+    out_pairs(F1, B, jmin);
+    out_pairs(F2, B, jmin);
   }
-  // rows with j < jmin are left untouched
   template <typename S>
   __device__ __forceinline__ void out(S* F, const T (&v)[SEG], int jmin = 1) {
     T* B = next();
     __syncthreads();
     SQ B[(p0 + q) * LS + l] = v[q];
     __syncthreads();
-    if constexpr (PAIR)
-      out_pairs(F, B, jmin);
-    else
-      SQ if (keep(q, jmin)) F[go[q]] = (S)B[li0 + q * lstep];
+    out_pairs(F, B, jmin);
   }
 };
 
@@ -225,8 +170,8 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
   return x * per + min(x, rem) + r;
 }
 
-template <typename T, int SEG, bool PAIR>
-__device__ __forceinline__ void xstage(XStage<T, SEG, PAIR>& x, const LBuf<T>& b, const Geo& g, const LShape& sh,
+template <typename T, int SEG>
+__device__ __forceinline__ void xstage(XStage<T, SEG>& x, const LBuf<T>& b, const Geo& g, const LShape& sh,
                                        int l, int p0) {
   x.st0 = b.st0;
   x.st1 = b.st1;
@@ -241,21 +186,6 @@ __device__ __forceinline__ void xstage(XStage<T, SEG, PAIR>& x, const LBuf<T>& b
   const int nlines = g.n2m * g.nz;
   const int nt = blockDim.x;
   const int t = threadIdx.x;
-  const int p = t % sh.n, ll0 = t / sh.n;
-  x.lstep = nt / sh.n;
-  x.li0 = p * x.LS + ll0;
-  x.j1 = 0u;
-  x.vm = 0u;
-  SQ {
-    const int ll = ll0 + q * x.lstep;
-    const int ln = ln0 + ll;
-    const int jj = 1 + ln % g.n2m;
-    const bool v = ll < sh.L && ln < nlines;
-    x.go[q] = v ? (unsigned)gidx(g, p, jj, ln / g.n2m) : 0u;
-    if (v) x.vm |= 1u << q;
-    if (jj == 1) x.j1 |= 1u << q;
-  }
-  // the paired layout (n even: the grid's N1-1 is)
   const int nh = sh.n >> 1;
   const int pq = t % nh, lq = t / nh;
   x.lstep2 = nt / nh;
@@ -274,16 +204,14 @@ __device__ __forceinline__ void xstage(XStage<T, SEG, PAIR>& x, const LBuf<T>& b
   }
 }
 
-#define XSETUP XSETUP_P(true)
-// the same with every field staged in the paired layout when PAIR_
-#define XSETUP_P(PAIR_)                                               \
+#define XSETUP                                                        \
   extern __shared__ __align__(16) unsigned char smem[];               \
   const Geo g = c.g;                                                  \
   const LShape sh = c.o.lx;                                           \
   const int L = sh.L, G = sh.G;                                       \
   const LBuf<T> bf = lbuf<T>(smem, sh, true);                         \
   const int l = threadIdx.x % L, gs = threadIdx.x / L, p0 = gs * SEG; \
-  XStage<T, SEG, (PAIR_)> X;                                          \
+  XStage<T, SEG> X;                                                   \
   xstage(X, bf, g, sh, l, p0);                                        \
   const int ln = blockIdx.x * L + l;                                  \
   const bool ok = ln < g.n2m * g.nz;                                  \

@@ -1531,7 +1531,7 @@ template <int SEG, bool CB_>
 __global__ __launch_bounds__(256) void k_upd_xu(CtxArg<CB_> ca_, int refine) {
   const Ctx c = ctx_get(ca_);
   typedef double T;
-  XSETUP_P(true);  // FP64 and FP32 fields, all paired
+  XSETUP;
   const Scal s = c.s;
   const Metrics M = c.m;
   const Fields F = c.d;
