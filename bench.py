@@ -628,7 +628,7 @@ def main():
             given = json.load(f)
         for name, prec in (("fp64", c3d.FP64), ("mixed", c3d.MIXED)):
             c3d.autotune_set(given[name], **cfg, precision=prec)
-            tune[name] = {k: given[name][k] for k in c3d.SHAPE_KEYS}
+            tune[name] = {k: given[name].get(k, c3d.SHAPE_DEFAULTS.get(k)) for k in c3d.SHAPE_KEYS}
             tune[name]["from"] = args.shapes
     elif not args.no_autotune:
         for name, prec in (("fp64", c3d.FP64), ("mixed", c3d.MIXED)):

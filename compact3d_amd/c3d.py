@@ -168,8 +168,13 @@ def make_config(n1, n2, n3, re, alx, aly, alz, dt, files=0.0, wmodel=0.0, cks=2.
     return cfg
 
 
+# c3d_autotune's shape layout (C3D_NSHAPES values); y_cols_fp32 counts the
+# FP32 y tile's threads per chunk row, each holding y_pair_fp32 columns
 SHAPE_KEYS = ("x_lines", "z_cols_fp64", "z_cols_fp32", "z_seg_fp32", "y_cols_fp64", "y_chunks_fp64",
-              "y_cols_fp32", "y_chunks_fp32", "pois_cols", "pois_chunks", "fft_x_lines", "fft_z_lines")
+              "y_cols_fp32", "y_chunks_fp32", "pois_cols", "pois_chunks", "fft_x_lines", "fft_z_lines",
+              "y_pair_fp32")
+# keys that shape sets written before a key existed may lack, and their value
+SHAPE_DEFAULTS = {"y_pair_fp32": 1}
 
 
 def autotune(steps=4, **kw):
@@ -177,7 +182,7 @@ def autotune(steps=4, **kw):
     record the fastest for handles created afterwards.  Returns the chosen
     shapes and ms per step before ('ms_rule') and after ('ms_tuned')."""
     L = lib()
-    sh = (C.c_int * 12)()
+    sh = (C.c_int * len(SHAPE_KEYS))()
     ms = np.zeros(2)
     if L.c3d_autotune(C.byref(make_config(**kw)), int(steps), sh, _p(ms)) != 0:
         raise C3DError("c3d_autotune: " + L.c3d_last_error(None).decode())
@@ -190,7 +195,7 @@ def autotune_set(shapes, **kw):
     """c3d_autotune_set: record the given shapes (a dict with SHAPE_KEYS, as
     autotune() returns) for this configuration without timing"""
     L = lib()
-    sh = (C.c_int * 12)(*[int(shapes[k]) for k in SHAPE_KEYS])
+    sh = (C.c_int * len(SHAPE_KEYS))(*[int(shapes.get(k, SHAPE_DEFAULTS.get(k))) for k in SHAPE_KEYS])
     if L.c3d_autotune_set(C.byref(make_config(**kw)), sh) != 0:
         raise C3DError("c3d_autotune_set: " + L.c3d_last_error(None).decode())
 

@@ -203,6 +203,14 @@ static YShape make_yshape(int m, int nt, int row, int ch = 12) {
   return y;
 }
 
+// FP32 y tiles of column pairs: chunks of at most 6 points, L threads (a
+// multiple of 16, i.e. 32 columns = 128-byte rows, where possible)
+static YShape make_yshape_pair(int m, int nt) {
+  YShape y = make_yshape(m, nt, 16, 6);
+  y.P = 2;
+  return y;
+}
+
 static TileShape make_tile(int n, int ntiles) {
   TileShape t;
   t.n = n;
@@ -620,7 +628,9 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   // FP64 y-sweep: 6-point chunks (every operand of a chunk is loaded before
   // its arithmetic), 16 columns (128-byte rows), up to 512 threads
   o.ys = make_yshape(g.n2m, 512, 16, 6);
-  o.ys32 = make_yshape(g.n2m, 512, 32);
+  // FP32 y-sweep: the same tiles over column pairs (8-byte loads, 32
+  // columns per 16 threads: 128-byte rows)
+  o.ys32 = make_yshape_pair(g.n2m, 512);
   // Poisson: 16 columns (8 modes of 16 bytes per row at the two kz of a
   // pair) and chunks of 6 points keep its three FP64 chunk arrays small;
   // 12-point chunks beyond N2-1 = 1536 (at least 2 columns up to 3072)
@@ -1271,17 +1281,23 @@ static std::vector<LShape> cand_lz(int n, bool fp32) {
   }
   return v;
 }
-static std::vector<YShape> cand_y(int m, int row, int ntmax) {
+// pair: also the column-pair shapes (FP32)
+static std::vector<YShape> cand_y(int m, int row, int ntmax, bool pair = false) {
   std::vector<YShape> v;
+  auto add = [&](const YShape& y, int chmax) {
+    if (y.L * y.C > ntmax || (m + y.C - 1) / y.C > chmax) return;
+    bool dup = false;
+    for (const YShape& w : v) dup = dup || (w.L == y.L && w.C == y.C && w.P == y.P);
+    if (!dup) v.push_back(y);
+  };
   for (int ch : {6, 12})
     for (int nt : {256, 512, 1024}) {
       if (nt > ntmax) continue;
-      const YShape y = make_yshape(m, nt, row, ch);
-      if (y.L * y.C > ntmax || (m + y.C - 1) / y.C > 12) continue;
-      bool dup = false;
-      for (const YShape& w : v) dup = dup || (w.L == y.L && w.C == y.C);
-      if (!dup) v.push_back(y);
+      add(make_yshape(m, nt, row, ch), 12);
     }
+  if (pair)
+    for (int nt : {512, 1024})
+      if (nt <= ntmax) add(make_yshape_pair(m, nt), 6);
   return v;
 }
 // Poisson y-solve tiles: an even number of columns (modes x the two kz of
@@ -1378,7 +1394,7 @@ extern "C" int c3d_autotune(const c3d_config* cfg, int steps, int* shapes, doubl
     }
   };
   if (mixed)
-    knob(cand_y(g.n2m, 32, 1024), [](Shapes& s, const YShape& y) { s.ys32 = y; });
+    knob(cand_y(g.n2m, 32, 1024, true), [](Shapes& s, const YShape& y) { s.ys32 = y; });
   else
     knob(cand_y(g.n2m, 16, 1024), [](Shapes& s, const YShape& y) { s.ys = y; });
   if (mixed) knob(cand_lz(g.nz, true), [](Shapes& s, const LShape& z) { s.lz32 = z; });
@@ -1397,9 +1413,10 @@ extern "C" int c3d_autotune(const c3d_config* cfg, int steps, int* shapes, doubl
     g_tuned[tune_key(*cfg)] = best;
   }
   if (shapes) {
-    const int v[12] = {best.lx.L, best.lz.L, best.lz32.L, best.lz32.seg, best.ys.L, best.ys.C,
-                       best.ys32.L, best.ys32.C, best.yp.L, best.yp.C, best.tlx, best.tlz};
-    for (int q = 0; q < 12; q++) shapes[q] = v[q];
+    const int v[C3D_NSHAPES] = {best.lx.L, best.lz.L, best.lz32.L, best.lz32.seg, best.ys.L, best.ys.C,
+                                best.ys32.L, best.ys32.C, best.yp.L, best.yp.C, best.tlx, best.tlz,
+                                best.ys32.P};
+    for (int q = 0; q < C3D_NSHAPES; q++) shapes[q] = v[q];
   }
   if (ms) {
     ms[0] = t0;
@@ -1433,10 +1450,10 @@ extern "C" int c3d_autotune_set(const c3d_config* cfg, const int* v) {
       }
     ok = false;
   };
-  auto ysh = [&](std::vector<YShape> c, const YShape& rule, int L, int C, YShape& out) {
+  auto ysh = [&](std::vector<YShape> c, const YShape& rule, int L, int C, int P, YShape& out) {
     c.push_back(rule);
     for (const YShape& x : c)
-      if (x.L == L && x.C == C) {
+      if (x.L == L && x.C == C && x.P == P) {
         out = x;
         return;
       }
@@ -1454,9 +1471,9 @@ extern "C" int c3d_autotune_set(const c3d_config* cfg, const int* v) {
   lsh(cand_lx(g.nx), s.lx, v[0], 0, s.lx);
   lsh(cand_lz(g.nz, false), s.lz, v[1], 0, s.lz);
   lsh(cand_lz(g.nz, true), s.lz32, v[2], v[3], s.lz32);
-  ysh(cand_y(g.n2m, 16, 1024), s.ys, v[4], v[5], s.ys);
-  ysh(cand_y(g.n2m, 32, 1024), s.ys32, v[6], v[7], s.ys32);
-  ysh(cand_yp(g.n2m), s.yp, v[8], v[9], s.yp);
+  ysh(cand_y(g.n2m, 16, 1024), s.ys, v[4], v[5], 1, s.ys);
+  ysh(cand_y(g.n2m, 32, 1024, true), s.ys32, v[6], v[7], v[12], s.ys32);
+  ysh(cand_yp(g.n2m), s.yp, v[8], v[9], 1, s.yp);
   tl(cand_tl(g.nx, true), s.tlx, v[10], s.tlx);
   tl(cand_tl(g.nz, false), s.tlz, v[11], s.tlz);
   if (!ok) {

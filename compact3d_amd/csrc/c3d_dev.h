@@ -379,7 +379,10 @@ __device__ __forceinline__ void block_sum2(double a, double b, double* oa, doubl
   }
 }
 
-// y-line tile: L columns x C chunks, thread (l, c) = l + L*c
+// y-line tile: L threads x C chunks, thread (l, c) = l + L*c; each thread
+// holds P neighbouring columns (P = 2 only for FP32 sweeps), so a tile
+// covers L*P columns
 struct YShape {
   int L, C;
+  int P = 1;
 };

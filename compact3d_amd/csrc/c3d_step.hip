@@ -228,15 +228,63 @@ struct YPos {
   int l, c, j0, len;
 };
 
+// Column values of the y-sweeps.  A thread holds one column (V = T) or, in
+// FP32, two neighbouring columns (V = f2v, one 8-byte load or store per
+// pair of points: a wave's 4-byte loads carry half the bytes of its 8-byte
+// ones).  The J-only coefficients stay scalars (S) and multiply both.
+typedef float f2v __attribute__((ext_vector_type(2)));
+template <typename V>
+struct YCol {
+  typedef V S;
+  static constexpr int P = 1;
+};
+template <>
+struct YCol<f2v> {
+  typedef float S;
+  static constexpr int P = 2;
+};
+template <typename V>
+__device__ __forceinline__ V vsplat(typename YCol<V>::S x) {
+  return (V)x;
+}
+// element idx (and idx+1 for a pair) of a field of the scalar type
+template <typename V>
+__device__ __forceinline__ V vld(const typename YCol<V>::S* p, unsigned idx) {
+  return *(const V*)(p + idx);
+}
+template <typename V>
+__device__ __forceinline__ V vld_nt(const typename YCol<V>::S* p, unsigned idx) {
+  return __builtin_nontemporal_load((const V*)(p + idx));
+}
+template <typename V>
+__device__ __forceinline__ void vst(typename YCol<V>::S* p, unsigned idx, V x) {
+  *(V*)(p + idx) = x;
+}
+template <typename V>
+__device__ __forceinline__ void vst_nt(typename YCol<V>::S* p, unsigned idx, V x) {
+  __builtin_nontemporal_store(x, (V*)(p + idx));
+}
+// FP64 plane values (wall stresses) of the thread's columns
+template <typename V>
+__device__ __forceinline__ V vld_d(const double* p, unsigned idx) {
+  typedef typename YCol<V>::S S;
+  if constexpr (YCol<V>::P == 2)
+    return V{(S)p[idx], (S)p[idx + 1]};
+  else
+    return (V)p[idx];
+}
+
 // chunk-parallel linear recurrence v[q] <- v[q] + m(q) * v[q-1] (forward):
 // each chunk runs it from zero, the chunks' end values and multiplier
 // products are combined across the block through LDS, and the incoming
 // value Y reaches point q of the chunk as Y * m(0) * ... * m(q).  m(q) is
 // evaluated twice, so a multiplier held in LDS or recomputed costs no
-// registers across the barrier.
-template <typename T, int N, typename MF>
-__device__ __forceinline__ void ychain_fwd(T (&v)[N], MF m, const YPos& y, int L, T* xa, T* xm) {
-  T r = (T)0, P = (T)1;
+// registers across the barrier.  V: the column value type (one column, or
+// a pair sharing the multipliers).
+template <typename V, typename T, int N, typename MF>
+__device__ __forceinline__ void ychain_fwd(V (&v)[N], MF m, const YPos& y, int L, V* xa, T* xm) {
+  V r = vsplat<V>((T)0);
+  T P = (T)1;
 #pragma unroll
   for (int q = 0; q < N; q++)
     if (q < y.len) {
@@ -248,7 +296,7 @@ __device__ __forceinline__ void ychain_fwd(T (&v)[N], MF m, const YPos& y, int L
   xa[y.c * L + y.l] = r;
   xm[y.c * L + y.l] = P;
   __syncthreads();
-  T Y = (T)0;
+  V Y = vsplat<V>((T)0);
   for (int cc = 0; cc < y.c; cc++) Y = xa[cc * L + y.l] + xm[cc * L + y.l] * Y;
 #pragma unroll
   for (int q = 0; q < N; q++)
@@ -258,9 +306,10 @@ __device__ __forceinline__ void ychain_fwd(T (&v)[N], MF m, const YPos& y, int L
     }
 }
 // backward: v[q] <- v[q] + m(q) * v[q+1]
-template <typename T, int N, typename MF>
-__device__ __forceinline__ void ychain_bwd(T (&v)[N], MF m, const YPos& y, int C, int L, T* xa, T* xm) {
-  T r = (T)0, P = (T)1;
+template <typename V, typename T, int N, typename MF>
+__device__ __forceinline__ void ychain_bwd(V (&v)[N], MF m, const YPos& y, int C, int L, V* xa, T* xm) {
+  V r = vsplat<V>((T)0);
+  T P = (T)1;
 #pragma unroll
   for (int q = N - 1; q >= 0; q--)
     if (q < y.len) {
@@ -272,7 +321,7 @@ __device__ __forceinline__ void ychain_bwd(T (&v)[N], MF m, const YPos& y, int C
   xa[y.c * L + y.l] = r;
   xm[y.c * L + y.l] = P;
   __syncthreads();
-  T Z = (T)0;
+  V Z = vsplat<V>((T)0);
   for (int cc = C - 1; cc > y.c; cc--) Z = xa[cc * L + y.l] + xm[cc * L + y.l] * Z;
 #pragma unroll
   for (int q = N - 1; q >= 0; q--)
@@ -354,11 +403,11 @@ __device__ __forceinline__ void ychain_bwd2(T (&v)[N], T (&w)[N], MF m, const YP
     }
 }
 
+// ys: the tile shape (one column per thread)
 template <typename T, int NT>
-__global__ __launch_bounds__(NT) void k_adi_y(Ctx c) {
+__global__ __launch_bounds__(NT) void k_adi_y(Ctx c, YShape ys) {
   __shared__ T xs[4][NT];  // C*L <= NT entries each
   const Geo g = c.g;
-  const YShape ys = sizeof(T) == 4 ? c.o.ys32 : c.o.ys;
   const int comp = blockIdx.y;
   const int ns = (comp == 1) ? 2 : 1;
   const int L = ys.L, C = ys.C;
@@ -413,14 +462,23 @@ __global__ __launch_bounds__(NT) void k_adi_y(Ctx c) {
 // (1-FJMU(J)) (-TAU(I,K,1)/DY(1)) IFWALL + (1-FJPA(J)) (-TAU(I,K,2)/DY(N2M))
 // IFWALL (ALY-1).  The flags are 0 except FJMU(1) and FJPA(N2M), so only the
 // rows J = 1 and J = N2M load a stress; t: the row's coefficients.
-template <typename T>
-__device__ __forceinline__ T wall_bc(const Ctx& c, const double* tau, unsigned t2, int J, const T* t) {
-  T bc = (T)0;
-  if (J == 1) bc = (T)t[10] * (-(T)tau[t2] / (T)c.m.dy[1] * (T)c.s.ifwall);
+template <typename V, typename T>
+__device__ __forceinline__ V wall_bc(const Ctx& c, const double* tau, unsigned t2, int J, const T* t) {
+  V bc = vsplat<V>((T)0);
+  if (J == 1) bc = (T)t[10] * (-vld_d<V>(tau, t2) / (T)c.m.dy[1] * (T)c.s.ifwall);
   if (J == c.g.n2m)
-    bc = bc + (T)t[9] * (-(T)tau[(unsigned)(c.g.nx * c.g.nz) + t2] / (T)c.m.dy[c.g.n2m] * (T)c.s.ifwall *
+    bc = bc + (T)t[9] * (-vld_d<V>(tau, (unsigned)(c.g.nx * c.g.nz) + t2) / (T)c.m.dy[c.g.n2m] * (T)c.s.ifwall *
                          (T)(c.s.aly - 1.));
   return bc;
+}
+// the value at column I-1 of each of the thread's columns (di: the offset
+// from column I to its periodic left neighbour)
+template <typename V>
+__device__ __forceinline__ V vld_im(const typename YCol<V>::S* p, unsigned idx, int di) {
+  if constexpr (YCol<V>::P == 2)
+    return V{p[idx + di], p[idx]};
+  else
+    return p[idx + di];
 }
 
 // RUH, HH and HH2 are read once and HH written once per step: their loads
@@ -428,10 +486,12 @@ __device__ __forceinline__ T wall_bc(const Ctx& c, const double* tau, unsigned t
 // With SGS (FILES != 0) the SGS y terms of c3d_yterms.h join HH before it is
 // stored, their operands loaded with the chunk's (SGSVIS at the wall rows
 // only where the chunk holds J = 1 or N2M, the only rows whose weight is
-// not zero); im: the column i-1 (periodic) of SGSVIS's wall rows.
-template <typename T, int CH, bool SGS = false>
+// not zero); di: the offset of column I-1 (periodic) for SGSVIS's wall rows.
+// V: one column (T) or a pair of FP32 columns (f2v), with the same
+// arithmetic per column.
+template <typename T, typename V, int CH, bool SGS = false>
 __device__ __forceinline__ void yfuse_u(const Ctx& c, const T* tb, unsigned b, unsigned t2, const YPos& y,
-                                        T (&v)[CH], int di = 0) {
+                                        V (&v)[CH], int di = 0) {
   const Geo& g = c.g;
   const Fields& F = c.d;
   const T* __restrict__ su = (const T*)F.su;
@@ -441,69 +501,69 @@ __device__ __forceinline__ void yfuse_u(const Ctx& c, const T* tb, unsigned b, u
   const T* __restrict__ hp = (const T*)F.hhp[0];
   const T rinv = (T)c.s.rinv, dt = (T)c.s.dt;
   const unsigned nx = g.nx;
-  const T uN2 = su[b + g.n2 * nx], uW = su[b];
+  const V zero = vsplat<V>((T)0);
+  const V uN2 = vld<V>(su, b + g.n2 * nx), uW = vld<V>(su, b);
 
   const unsigned a0 = b + (unsigned)y.j0 * nx;  // row J0
-  T us[CH + 2], w[CH + 1], ru[CH], hx[CH], hq[CH];
+  V us[CH + 2], w[CH + 1], ru[CH], hx[CH], hq[CH];
 #pragma unroll
-  for (int q = 0; q < CH + 2; q++) us[q] = (q < y.len + 2) ? su[a0 - nx + q * nx] : (T)0;
+  for (int q = 0; q < CH + 2; q++) us[q] = (q < y.len + 2) ? vld<V>(su, a0 - nx + q * nx) : zero;
 #pragma unroll
-  for (int q = 0; q < CH + 1; q++) w[q] = (q < y.len + 1) ? vv1[a0 + q * nx] : (T)0;
+  for (int q = 0; q < CH + 1; q++) w[q] = (q < y.len + 1) ? vld<V>(vv1, a0 + q * nx) : zero;
 #pragma unroll
   for (int q = 0; q < CH; q++) {
     const bool in = q < y.len;
-    ru[q] = in ? __builtin_nontemporal_load(&ruh[a0 + q * nx]) : (T)0;
-    hx[q] = in ? __builtin_nontemporal_load(&hh[a0 + q * nx]) : (T)0;
-    hq[q] = in ? __builtin_nontemporal_load(&hp[a0 + q * nx]) : (T)0;
+    ru[q] = in ? vld_nt<V>(ruh, a0 + q * nx) : zero;
+    hx[q] = in ? vld_nt<V>(hh, a0 + q * nx) : zero;
+    hq[q] = in ? vld_nt<V>(hp, a0 + q * nx) : zero;
   }
   // SGS operands: nux at J0-1..J0+len, dv/dx at J0..J0+len, and the sums
   // of SGSVIS at (I,0,K) + (I-1,0,K) and (I,N2,K) + (I-1,N2,K)
-  T nu[SGS ? CH + 2 : 1], dv[SGS ? CH + 1 : 1], sgb = (T)0, sgt = (T)0;
+  V nu[SGS ? CH + 2 : 1], dv[SGS ? CH + 1 : 1], sgb = zero, sgt = zero;
   if constexpr (SGS) {
     const T* __restrict__ nux = (const T*)F.nux;
     const T* __restrict__ dvdx = (const T*)F.dvdx;
     const T* __restrict__ sg = (const T*)F.ssgs;
 #pragma unroll
-    for (int q = 0; q < CH + 2; q++) nu[q] = (q < y.len + 2) ? nux[a0 - nx + q * nx] : (T)0;
+    for (int q = 0; q < CH + 2; q++) nu[q] = (q < y.len + 2) ? vld<V>(nux, a0 - nx + q * nx) : zero;
 #pragma unroll
-    for (int q = 0; q < CH + 1; q++) dv[q] = (q < y.len + 1) ? dvdx[a0 + q * nx] : (T)0;
-    const unsigned bim = b + di;
-    if (y.j0 == 1) sgb = sg[b] + sg[bim];
-    if (y.j0 + y.len - 1 == g.n2m) sgt = sg[b + g.n2 * nx] + sg[bim + g.n2 * nx];
+    for (int q = 0; q < CH + 1; q++) dv[q] = (q < y.len + 1) ? vld<V>(dvdx, a0 + q * nx) : zero;
+    if (y.j0 == 1) sgb = vld<V>(sg, b) + vld_im<V>(sg, b, di);
+    if (y.j0 + y.len - 1 == g.n2m) sgt = vld<V>(sg, b + g.n2 * nx) + vld_im<V>(sg, b + g.n2 * nx, di);
   }
 #pragma unroll
   for (int q = 0; q < CH; q++)
     if (q < y.len) {
       const int J = y.j0 + q;
-      const T um = us[q], u0 = us[q + 1], up = us[q + 2];
+      const V um = us[q], u0 = us[q + 1], up = us[q + 2];
       const T* t = tb + J * YTAB_W;
-      const T r0 = ru[q] + rinv * ((T)t[0] * up - (T)t[1] * u0 + (T)t[2] * um);
-      const T u2 = (T)t[3] * ((T)t[6] * up + (T)t[7] * u0) + (T)t[9] * uN2;
-      const T u1 = (T)t[4] * ((T)t[5] * u0 + (T)t[6] * um) + (T)t[10] * uW;
-      const T v2 = (J == g.n2m) ? (T)0 : w[q + 1];
-      const T v1 = (J == 1) ? (T)0 : w[q];
-      T h = hx[q] + (T)t[8] * (v2 * u2 - v1 * u1);
+      const V r0 = ru[q] + rinv * ((T)t[0] * up - (T)t[1] * u0 + (T)t[2] * um);
+      const V u2 = (T)t[3] * ((T)t[6] * up + (T)t[7] * u0) + (T)t[9] * uN2;
+      const V u1 = (T)t[4] * ((T)t[5] * u0 + (T)t[6] * um) + (T)t[10] * uW;
+      const V v2 = (J == g.n2m) ? zero : w[q + 1];
+      const V v1 = (J == 1) ? zero : w[q];
+      V h = hx[q] + (T)t[8] * (v2 * u2 - v1 * u1);
       if constexpr (SGS) {
         // D(SGS*(DU/DY))/DY and D(SGS*(DV/DX))/DY (:6551-6680), as yterm_u
-        const T n0 = nu[q + 1], np1 = (J < g.n2m) ? nu[q + 2] : (T)0, nm1 = (J > 1) ? nu[q] : (T)0;
-        const T sgsj2 = (T)t[3] * ((T)t[6] * np1 + (T)t[7] * n0) + (T)t[9] * (T)0.5 * sgt;
-        const T sgsj1 = (T)t[4] * ((T)t[5] * n0 + (T)t[6] * nm1) + (T)t[10] * (T)0.5 * sgb;
+        const V n0 = nu[q + 1], np1 = (J < g.n2m) ? nu[q + 2] : zero, nm1 = (J > 1) ? nu[q] : zero;
+        const V sgsj2 = (T)t[3] * ((T)t[6] * np1 + (T)t[7] * n0) + (T)t[9] * (T)0.5 * sgt;
+        const V sgsj1 = (T)t[4] * ((T)t[5] * n0 + (T)t[6] * nm1) + (T)t[10] * (T)0.5 * sgb;
         const T wfu = (T)1 - (T)c.s.ifwall * (T)t[9] * (T)(c.s.aly - 1.);
         const T wfd = (T)1 - (T)c.s.ifwall * (T)t[10];
-        const T uu22 = (up - u0) * (T)t[15];
-        const T uu11 = (u0 - um) * (T)t[14];
-        const T hd = h - (T)t[8] * (wfu * sgsj2 * uu22 - wfd * sgsj1 * uu11);
-        const T dvp = (J < g.n2m) ? dv[q + 1] : (T)0;
+        const V uu22 = (up - u0) * (T)t[15];
+        const V uu11 = (u0 - um) * (T)t[14];
+        const V hd = h - (T)t[8] * (wfu * sgsj2 * uu22 - wfd * sgsj1 * uu11);
+        const V dvp = (J < g.n2m) ? dv[q + 1] : zero;
         h = hd - (T)t[8] * (wfu * sgsj2 * dvp - wfd * sgsj1 * dv[q]);
       }
-      __builtin_nontemporal_store(h, &hh[a0 + q * nx]);
-      const T r = r0 - (T)(3. / 2.) * h + (T)(1. / 2.) * hq[q] + wall_bc<T>(c, F.tauw, t2, J, t);
+      vst_nt<V>(hh, a0 + q * nx, h);
+      const V r = r0 - (T)(3. / 2.) * h + (T)(1. / 2.) * hq[q] + wall_bc<V>(c, F.tauw, t2, J, t);
       v[q] = (T)t[YT_BET] * (r * dt);
     }
 }
 
-template <typename T, int CH, bool SGS = false>
-__device__ __forceinline__ void yfuse_v(const Ctx& c, const T* tb, unsigned b, const YPos& y, T (&v)[CH]) {
+template <typename T, typename V, int CH, bool SGS = false>
+__device__ __forceinline__ void yfuse_v(const Ctx& c, const T* tb, unsigned b, const YPos& y, V (&v)[CH]) {
   const Fields& F = c.d;
   const T* __restrict__ sv = (const T*)F.sv;
   const T* __restrict__ sp = (const T*)F.sp;
@@ -512,54 +572,55 @@ __device__ __forceinline__ void yfuse_v(const Ctx& c, const T* tb, unsigned b, c
   const T* __restrict__ hp = (const T*)F.hhp[1];
   const T rinv = (T)c.s.rinv, dt = (T)c.s.dt;
   const unsigned nx = c.g.nx;
+  const V zero = vsplat<V>((T)0);
   const unsigned a0 = b + (unsigned)y.j0 * nx;
-  T vs[CH + 2], ps[CH + 1], ru[CH], hx[CH], hq[CH];
+  V vs[CH + 2], ps[CH + 1], ru[CH], hx[CH], hq[CH];
 #pragma unroll
-  for (int q = 0; q < CH + 2; q++) vs[q] = (q < y.len + 2) ? sv[a0 - nx + q * nx] : (T)0;
+  for (int q = 0; q < CH + 2; q++) vs[q] = (q < y.len + 2) ? vld<V>(sv, a0 - nx + q * nx) : zero;
 #pragma unroll
-  for (int q = 0; q < CH + 1; q++) ps[q] = (q < y.len + 1) ? sp[a0 - nx + q * nx] : (T)0;
+  for (int q = 0; q < CH + 1; q++) ps[q] = (q < y.len + 1) ? vld<V>(sp, a0 - nx + q * nx) : zero;
 #pragma unroll
   for (int q = 0; q < CH; q++) {
     const bool in = q < y.len;
-    ru[q] = in ? __builtin_nontemporal_load(&ruh[a0 + q * nx]) : (T)0;
-    hx[q] = in ? __builtin_nontemporal_load(&hh[a0 + q * nx]) : (T)0;
-    hq[q] = in ? __builtin_nontemporal_load(&hp[a0 + q * nx]) : (T)0;
+    ru[q] = in ? vld_nt<V>(ruh, a0 + q * nx) : zero;
+    hx[q] = in ? vld_nt<V>(hh, a0 + q * nx) : zero;
+    hq[q] = in ? vld_nt<V>(hp, a0 + q * nx) : zero;
   }
   // SGS operand: SGSVIS at J0-1..J0+len-1
-  T sgv[SGS ? CH + 1 : 1];
+  V sgv[SGS ? CH + 1 : 1];
   if constexpr (SGS) {
     const T* __restrict__ sg = (const T*)c.d.ssgs;
 #pragma unroll
-    for (int q = 0; q < CH + 1; q++) sgv[q] = (q < y.len + 1) ? sg[a0 - nx + q * nx] : (T)0;
+    for (int q = 0; q < CH + 1; q++) sgv[q] = (q < y.len + 1) ? vld<V>(sg, a0 - nx + q * nx) : zero;
   }
 #pragma unroll
   for (int q = 0; q < CH; q++)
     if (q < y.len) {
       const int J = y.j0 + q;
-      const T vm = vs[q], v0 = vs[q + 1], vp = vs[q + 2];
+      const V vm = vs[q], v0 = vs[q + 1], vp = vs[q + 2];
       const T* t = tb + J * YTAB_W;
-      const T viscos = rinv * ((T)t[0] * vp - (T)t[1] * v0 + (T)t[2] * vm);
-      const T pressg2 = (ps[q + 1] - ps[q]) * (T)t[4];  // t[4] = 1/H(J)
-      const T r0 = ru[q] + (-pressg2 + viscos);
-      const T v2 = (T)0.5 * (vp + v0);
-      const T v1 = (T)0.5 * (v0 + vm);
-      T h = hx[q] + (T)t[4] * (v2 * v2 - v1 * v1);
+      const V viscos = rinv * ((T)t[0] * vp - (T)t[1] * v0 + (T)t[2] * vm);
+      const V pressg2 = (ps[q + 1] - ps[q]) * (T)t[4];  // t[4] = 1/H(J)
+      const V r0 = ru[q] + (-pressg2 + viscos);
+      const V v2 = (T)0.5 * (vp + v0);
+      const V v1 = (T)0.5 * (v0 + vm);
+      V h = hx[q] + (T)t[4] * (v2 * v2 - v1 * v1);
       if constexpr (SGS) {
         // 2 D(SGS*(DV/DY))/DY (as yterm_v); t[5] = 1/DY(J), t[6] = 1/DY(J-1)
-        const T g2 = (T)t[5] * (vp - v0);
-        const T g1 = (T)t[6] * (v0 - vm);
-        const T vis22 = (T)t[4] * (sgv[q + 1] * g2 - sgv[q] * g1);
+        const V g2 = (T)t[5] * (vp - v0);
+        const V g1 = (T)t[6] * (v0 - vm);
+        const V vis22 = (T)t[4] * (sgv[q + 1] * g2 - sgv[q] * g1);
         h = h - (T)2. * vis22;
       }
-      __builtin_nontemporal_store(h, &hh[a0 + q * nx]);
-      const T r = r0 - (T)(3. / 2.) * h + (T)(1. / 2.) * hq[q];
+      vst_nt<V>(hh, a0 + q * nx, h);
+      const V r = r0 - (T)(3. / 2.) * h + (T)(1. / 2.) * hq[q];
       v[q] = (T)t[YT_BET] * (r * dt);
     }
 }
 
-template <typename T, int CH, bool SGS = false>
+template <typename T, typename V, int CH, bool SGS = false>
 __device__ __forceinline__ void yfuse_w(const Ctx& c, const T* tb, unsigned b, unsigned t2, const YPos& y,
-                                        T (&v)[CH], int dk = 0) {
+                                        V (&v)[CH], int dk = 0) {
   const Geo& g = c.g;
   const Fields& F = c.d;
   const T* __restrict__ sw = (const T*)F.sw;
@@ -569,73 +630,78 @@ __device__ __forceinline__ void yfuse_w(const Ctx& c, const T* tb, unsigned b, u
   const T* __restrict__ hp = (const T*)F.hhp[2];
   const T rinv = (T)c.s.rinv, dt = (T)c.s.dt;
   const unsigned nx = g.nx;
-  const T wN2 = sw[b + g.n2 * nx], wW = sw[b];
+  const V zero = vsplat<V>((T)0);
+  const V wN2 = vld<V>(sw, b + g.n2 * nx), wW = vld<V>(sw, b);
 
   const unsigned a0 = b + (unsigned)y.j0 * nx;
-  T ws[CH + 2], s3[CH + 1], ru[CH], hx[CH], hq[CH];
+  V ws[CH + 2], s3[CH + 1], ru[CH], hx[CH], hq[CH];
 #pragma unroll
-  for (int q = 0; q < CH + 2; q++) ws[q] = (q < y.len + 2) ? sw[a0 - nx + q * nx] : (T)0;
+  for (int q = 0; q < CH + 2; q++) ws[q] = (q < y.len + 2) ? vld<V>(sw, a0 - nx + q * nx) : zero;
 #pragma unroll
-  for (int q = 0; q < CH + 1; q++) s3[q] = (q < y.len + 1) ? vv3[a0 + q * nx] : (T)0;
+  for (int q = 0; q < CH + 1; q++) s3[q] = (q < y.len + 1) ? vld<V>(vv3, a0 + q * nx) : zero;
 #pragma unroll
   for (int q = 0; q < CH; q++) {
     const bool in = q < y.len;
-    ru[q] = in ? __builtin_nontemporal_load(&ruh[a0 + q * nx]) : (T)0;
-    hx[q] = in ? __builtin_nontemporal_load(&hh[a0 + q * nx]) : (T)0;
-    hq[q] = in ? __builtin_nontemporal_load(&hp[a0 + q * nx]) : (T)0;
+    ru[q] = in ? vld_nt<V>(ruh, a0 + q * nx) : zero;
+    hx[q] = in ? vld_nt<V>(hh, a0 + q * nx) : zero;
+    hq[q] = in ? vld_nt<V>(hp, a0 + q * nx) : zero;
   }
   // SGS operands: nuz at J0-1..J0+len, dv/dz at J0..J0+len, and the sums
   // of SGSVIS at (I,0,K) + (I,0,K-1) and (I,N2,K) + (I,N2,K-1)
-  T nz[SGS ? CH + 2 : 1], dz[SGS ? CH + 1 : 1], sgb = (T)0, sgt = (T)0;
+  V nz[SGS ? CH + 2 : 1], dz[SGS ? CH + 1 : 1], sgb = zero, sgt = zero;
   if constexpr (SGS) {
     const T* __restrict__ nuz = (const T*)F.nuz;
     const T* __restrict__ dvdz = (const T*)F.dvdz;
     const T* __restrict__ sg = (const T*)F.ssgs;
 #pragma unroll
-    for (int q = 0; q < CH + 2; q++) nz[q] = (q < y.len + 2) ? nuz[a0 - nx + q * nx] : (T)0;
+    for (int q = 0; q < CH + 2; q++) nz[q] = (q < y.len + 2) ? vld<V>(nuz, a0 - nx + q * nx) : zero;
 #pragma unroll
-    for (int q = 0; q < CH + 1; q++) dz[q] = (q < y.len + 1) ? dvdz[a0 + q * nx] : (T)0;
+    for (int q = 0; q < CH + 1; q++) dz[q] = (q < y.len + 1) ? vld<V>(dvdz, a0 + q * nx) : zero;
     const unsigned bkm = b + dk;
-    if (y.j0 == 1) sgb = sg[b] + sg[bkm];
-    if (y.j0 + y.len - 1 == g.n2m) sgt = sg[b + g.n2 * nx] + sg[bkm + g.n2 * nx];
+    if (y.j0 == 1) sgb = vld<V>(sg, b) + vld<V>(sg, bkm);
+    if (y.j0 + y.len - 1 == g.n2m) sgt = vld<V>(sg, b + g.n2 * nx) + vld<V>(sg, bkm + g.n2 * nx);
   }
 #pragma unroll
   for (int q = 0; q < CH; q++)
     if (q < y.len) {
       const int J = y.j0 + q;
-      const T wm = ws[q], w0 = ws[q + 1], wp = ws[q + 2];
+      const V wm = ws[q], w0 = ws[q + 1], wp = ws[q + 2];
       const T* t = tb + J * YTAB_W;
-      const T r0 = ru[q] + rinv * ((T)t[0] * wp - (T)t[1] * w0 + (T)t[2] * wm);
-      const T w2 = (T)t[3] * ((T)t[7] * w0 + (T)t[6] * wp) + (T)t[9] * wN2;
-      const T w1 = (T)t[4] * ((T)t[6] * wm + (T)t[5] * w0) + (T)t[10] * wW;
-      const T v2 = (J == g.n2m) ? (T)0 : s3[q + 1];
-      const T v1 = (J == 1) ? (T)0 : s3[q];
-      T h = hx[q] + (T)t[8] * (v2 * w2 - v1 * w1);
+      const V r0 = ru[q] + rinv * ((T)t[0] * wp - (T)t[1] * w0 + (T)t[2] * wm);
+      const V w2 = (T)t[3] * ((T)t[7] * w0 + (T)t[6] * wp) + (T)t[9] * wN2;
+      const V w1 = (T)t[4] * ((T)t[6] * wm + (T)t[5] * w0) + (T)t[10] * wW;
+      const V v2 = (J == g.n2m) ? zero : s3[q + 1];
+      const V v1 = (J == 1) ? zero : s3[q];
+      V h = hx[q] + (T)t[8] * (v2 * w2 - v1 * w1);
       if constexpr (SGS) {
         // D(SGS*(DW/DY))/DY and D(SGS*(DV/DZ))/DY (:7858-7908, :8042-8094),
         // as yterm_w
-        const T z0 = nz[q + 1], zp1 = (J < g.n2m) ? nz[q + 2] : (T)0, zm1 = (J > 1) ? nz[q] : (T)0;
-        const T sgsj1 = (T)t[4] * ((T)t[5] * z0 + (T)t[6] * zm1) + (T)t[10] * (T)0.5 * sgb;
-        const T sgsj2 = (T)t[3] * ((T)t[6] * zp1 + (T)t[7] * z0) + (T)t[9] * (T)0.5 * sgt;
+        const V z0 = nz[q + 1], zp1 = (J < g.n2m) ? nz[q + 2] : zero, zm1 = (J > 1) ? nz[q] : zero;
+        const V sgsj1 = (T)t[4] * ((T)t[5] * z0 + (T)t[6] * zm1) + (T)t[10] * (T)0.5 * sgb;
+        const V sgsj2 = (T)t[3] * ((T)t[6] * zp1 + (T)t[7] * z0) + (T)t[9] * (T)0.5 * sgt;
         const T wfu = (T)1 - (T)c.s.ifwall * (T)t[9] * (T)(c.s.aly - 1.);
         const T wfd = (T)1 - (T)c.s.ifwall * (T)t[10];
-        const T ww22 = (T)t[15] * (wp - w0);
-        const T ww11 = (T)t[14] * (w0 - wm);
-        const T hd = h - (T)t[8] * (wfu * sgsj2 * ww22 - wfd * sgsj1 * ww11);
-        const T dzp = (J < g.n2m) ? dz[q + 1] : (T)0;
+        const V ww22 = (T)t[15] * (wp - w0);
+        const V ww11 = (T)t[14] * (w0 - wm);
+        const V hd = h - (T)t[8] * (wfu * sgsj2 * ww22 - wfd * sgsj1 * ww11);
+        const V dzp = (J < g.n2m) ? dz[q + 1] : zero;
         h = hd - (T)t[8] * (wfu * sgsj2 * dzp - wfd * sgsj1 * dz[q]);
       }
-      __builtin_nontemporal_store(h, &hh[a0 + q * nx]);
-      const T r = r0 - (T)(3. / 2.) * h + (T)(1. / 2.) * hq[q] + wall_bc<T>(c, F.taub, t2, J, t);
+      vst_nt<V>(hh, a0 + q * nx, h);
+      const V r = r0 - (T)(3. / 2.) * h + (T)(1. / 2.) * hq[q] + wall_bc<V>(c, F.taub, t2, J, t);
       v[q] = (T)t[YT_BET] * (r * dt);
     }
 }
 
-// NT: the block size bound, 512 or 1024 (tiles of L*C threads)
-template <typename T, bool SGS, int CH, int NT, bool CB_>
+// NT: the block size bound, 512 or 1024 (tiles of L*C threads); V: T, or
+// f2v for FP32 tiles of column pairs (YShape.P = 2: L threads cover 2L
+// columns)
+template <typename T, typename V, bool SGS, int CH, int NT, bool CB_>
 __global__ __launch_bounds__(NT) void k_rhsy_adiy(CtxArg<CB_> ca_, int staged) {
+  constexpr int PR = YCol<V>::P;
   const Ctx c = ctx_get(ca_);
-  __shared__ T xs[4][NT];  // C*L <= NT entries each
+  __shared__ V xs[2][NT];  // C*L <= NT entries each
+  __shared__ T xp[2][NT];
   extern __shared__ __align__(16) unsigned char smem[];
   const Geo g = c.g;
   const YShape ys = sizeof(T) == 4 ? c.o.ys32 : c.o.ys;
@@ -650,11 +716,11 @@ __global__ __launch_bounds__(NT) void k_rhsy_adiy(CtxArg<CB_> ca_, int staged) {
   const int ch = (m + C - 1) / C;
   y.j0 = ns + y.c * ch;
   y.len = max(0, min(ch, g.n2m + 1 - y.j0));
-  const int nit = (g.nx + L - 1) / L;
+  const int nit = (g.nx + L * PR - 1) / (L * PR);
   const int it = blockIdx.x % nit;
   const int k = blockIdx.x / nit;
-  const int i = it * L + l;
-  const bool ok = i < g.nx;
+  const int i = (it * L + l) * PR;
+  const bool ok = i < g.nx;  // NX is even, so a pair is whole
   const Fields F = c.d;
   const Metrics M = c.m;
   T* X = (T*)(comp == 0 ? F.uh[0] : comp == 1 ? F.uh[1] : F.uh[2]);
@@ -671,24 +737,24 @@ __global__ __launch_bounds__(NT) void k_rhsy_adiy(CtxArg<CB_> ca_, int staged) {
     __syncthreads();
     tb = st;
   }
-  T v[CH];
+  V v[CH];
 #pragma unroll
-  for (int q = 0; q < CH; q++) v[q] = (T)0;
+  for (int q = 0; q < CH; q++) v[q] = vsplat<V>((T)0);
   if (ok && y.len > 0) {
     if (comp == 0) {
-      yfuse_u<T, CH, SGS>(c, tb, b, t2, y, v, (i == 0 ? g.nx - 1 : i - 1) - i);
+      yfuse_u<T, V, CH, SGS>(c, tb, b, t2, y, v, (i == 0 ? g.nx - 1 : i - 1) - i);
     } else if (comp == 1) {
-      yfuse_v<T, CH, SGS>(c, tb, b, y, v);
+      yfuse_v<T, V, CH, SGS>(c, tb, b, y, v);
     } else {
-      yfuse_w<T, CH, SGS>(c, tb, b, t2, y, v, ((k == 0 ? g.nz - 1 : k - 1) - k) * (int)g.plane);
+      yfuse_w<T, V, CH, SGS>(c, tb, b, t2, y, v, ((k == 0 ? g.nz - 1 : k - 1) - k) * (int)g.plane);
     }
   }
-  ychain_fwd(v, [&](int q) { return (T)tb[(y.j0 + q) * YTAB_W + YT_M]; }, y, L, xs[0], xs[1]);
-  ychain_bwd(v, [&](int q) { return (T)tb[(y.j0 + q) * YTAB_W + YT_G]; }, y, C, L, xs[2], xs[3]);
+  ychain_fwd(v, [&](int q) { return (T)tb[(y.j0 + q) * YTAB_W + YT_M]; }, y, L, xs[0], xp[0]);
+  ychain_bwd(v, [&](int q) { return (T)tb[(y.j0 + q) * YTAB_W + YT_G]; }, y, C, L, xs[1], xp[1]);
   if (ok)
 #pragma unroll
     for (int q = 0; q < CH; q++)
-      if (q < y.len) X[b + (unsigned)(y.j0 + q) * g.nx] = v[q];
+      if (q < y.len) vst<V>(X, b + (unsigned)(y.j0 + q) * g.nx, v[q]);
 }
 
 // --------------------------------------------------- GETUH x / z sweeps
@@ -1903,7 +1969,10 @@ void rhs_lines(const Ctx& c) {
 template <typename T>
 static void getuh_t(const Ctx& c, bool fused) {
   const YShape& ys = sizeof(T) == 4 ? c.o.ys32 : c.o.ys;
-  const int ylx = (c.g.nx + ys.L - 1) / ys.L;
+  // FP32 tiles of column pairs (P = 2: L threads cover 2L columns) exist
+  // with chunks of at most 6 points only (the shape candidates ensure it)
+  const bool pair = sizeof(T) == 4 && ys.P == 2;
+  const int ylx = (c.g.nx + ys.L * ys.P - 1) / (ys.L * ys.P);
   // tiles of more than 512 threads run the NT = 1024 instances (rows of
   // 256 bytes: 64 FP32 or 32 FP64 columns)
   const bool big = ys.L * ys.C > 512;
@@ -1911,32 +1980,44 @@ static void getuh_t(const Ctx& c, bool fused) {
     const size_t tbytes = sizeof(T) * (size_t)(c.g.n2 + 2) * YTAB_W;
     // the coefficient table goes to LDS when it fits beside the chain arrays
     // within 64 KB
-    const size_t xsb = 4 * sizeof(T) * (big ? 1024 : 512);
+    const size_t xsb = 2 * (sizeof(T) * ys.P + sizeof(T)) * (big ? 1024 : 512);
     const int staged = tbytes + xsb <= 64 * 1024 && tbytes <= 40 * 1024 ? 1 : 0;
     // chunk length of the shape (u and w have the most points, N2M)
     const int ch = (c.g.n2m + ys.C - 1) / ys.C;
     const dim3 gr(ylx * c.g.nz, 3), th(ys.L * ys.C);
     const size_t lb = staged ? tbytes : 0;
     KScope ks_(c, KID_ADI_Y);
-#define YF_(SGS, CH, NT) C3D_LAUNCH(k_rhsy_adiy, (T, SGS, CH, NT), gr, th, lb, c, staged)
-#define YFN_(SGS, CH) \
-  if (big)            \
-    YF_(SGS, CH, 1024); \
-  else                \
-    YF_(SGS, CH, 512)
-    if (c.s.files) {
-      if (ch <= 6) { YFN_(true, 6); } else { YFN_(true, CHMAX); }
+#define YF_(V, SGS, CH, NT) C3D_LAUNCH(k_rhsy_adiy, (T, V, SGS, CH, NT), gr, th, lb, c, staged)
+#define YFN_(V, SGS, CH) \
+  if (big)               \
+    YF_(V, SGS, CH, 1024); \
+  else                   \
+    YF_(V, SGS, CH, 512)
+    bool done = false;
+    if constexpr (sizeof(T) == 4) {
+      if (pair) {
+        if (c.s.files) { YFN_(f2v, true, 6); } else { YFN_(f2v, false, 6); }
+        done = true;
+      }
+    }
+    if (done) {
+    } else if (c.s.files) {
+      if (ch <= 6) { YFN_(T, true, 6); } else { YFN_(T, true, CHMAX); }
     } else {
-      if (ch <= 6) { YFN_(false, 6); } else { YFN_(false, CHMAX); }
+      if (ch <= 6) { YFN_(T, false, 6); } else { YFN_(T, false, CHMAX); }
     }
 #undef YFN_
 #undef YF_
   } else {
+    // the stand-alone sweep holds one column per thread: a pair shape runs
+    // as L columns of the same chunks
     KScope ks_(c, KID_ADI_Y);
+    const YShape y1{ys.L, ys.C, 1};
+    const dim3 gr((c.g.nx + y1.L - 1) / y1.L * c.g.nz, 3), th(y1.L * y1.C);
     if (big)
-      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adi_y<T, 1024>), dim3(ylx * c.g.nz, 3), dim3(ys.L * ys.C), 0, c.st, c);
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adi_y<T, 1024>), gr, th, 0, c.st, c, y1);
     else
-      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adi_y<T, 512>), dim3(ylx * c.g.nz, 3), dim3(ys.L * ys.C), 0, c.st, c);
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adi_y<T, 512>), gr, th, 0, c.st, c, y1);
   }
   const size_t lx = lds_line(c.o.lx, true, sizeof(T));
   // GETUH1/2: y, x, z;  GETUH3: y, z, x (compact3d.f:8508-8607)

@@ -263,13 +263,17 @@ const char *c3d_kernel_name(int id);
  * the rule-derived shapes, and records the fastest set in a process-wide
  * table keyed by (N1, N2, N3, precision, FILES != 0).  Handles created
  * afterwards with that key use it; existing handles keep theirs.  Shapes
- * change only the rounding order of the line solves.  shapes (12 ints, may
- * be NULL): x lines per tile, FP64 z columns, FP32 z columns, FP32 z
- * segment, FP64 y columns and chunks, FP32 y columns and chunks, Poisson y
- * columns and chunks, x- and z-FFT lines per tile; ms (2, may be NULL):
- * ms per step with the rule-derived and with the chosen shapes. */
+ * change only the rounding order of the line solves.  shapes
+ * (C3D_NSHAPES ints, may be NULL): x lines per tile, FP64 z columns, FP32 z
+ * columns, FP32 z segment, FP64 y columns and chunks, FP32 y threads per
+ * chunk row and chunks, Poisson y columns and chunks, x- and z-FFT lines
+ * per tile, FP32 y columns per thread (1, or 2: neighbouring column pairs
+ * moved as 8-byte values, so a tile covers twice its threads per row in
+ * columns); ms (2, may be NULL): ms per step with the rule-derived and with
+ * the chosen shapes. */
+#define C3D_NSHAPES 13
 int c3d_autotune(const c3d_config *cfg, int steps, int *shapes, double *ms);
-/* records given shapes (c3d_autotune's 12-value layout, e.g. from an
+/* records given shapes (c3d_autotune's C3D_NSHAPES-value layout, e.g. from an
  * earlier c3d_autotune) for this configuration without timing anything;
  * each must be the rule-derived shape or one of the autotuner's candidates,
  * else the call fails (c3d_last_error(NULL)) and records nothing */

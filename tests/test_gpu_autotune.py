@@ -91,3 +91,47 @@ def test_autotune_set_accepts_candidates_only():
                 c3d.autotune_set(bad, **cfg, precision=c3d.FP64)
     finally:
         c3d.autotune_reset()
+
+
+@pytest.mark.parametrize("files,wmodel", [(0.0, 0.0), (2.0, 1.0)])
+def test_fp32_y_column_pairs_equal_single_columns(files, wmodel):
+    """The FP32 y-sweep over neighbouring column pairs (8-byte loads and
+    stores, y_pair_fp32 = 2) runs each column's arithmetic exactly as the
+    one-column sweep of the same chunks: after 3 mixed steps (DNS, and LES
+    with its SGS y terms) and after the stand-alone GETUH phase, every field
+    is bit-identical.  Grid 48x97x32: NX = 48 leaves the second half of a
+    64-column pair tile empty."""
+    import numpy as np
+    from compact3d_amd import c3d
+    from pyoracle import Oracle
+    n1, n2, n3 = 49, 97, 33
+    cfg = config(n1, n2, n3, cks=1.5, files=files, wmodel=wmodel)
+    try:
+        t = c3d.autotune(steps=1, **cfg, precision=c3d.MIXED)
+        orc = Oracle(**config(n1, n2, n3, cks=1.5))
+        U, P, HH = inputs(orc, seed=5)
+        out = {}
+        # N2M = 96: C = 16 chunks of 6 points; 32 threads per chunk row
+        for pair in (1, 2):
+            s = dict(t)
+            s.update(y_cols_fp32=32, y_chunks_fp32=16, y_pair_fp32=pair)
+            c3d.autotune_set(s, **cfg, precision=c3d.MIXED)
+            ch = c3d.Channel(**cfg, precision=c3d.MIXED)
+            ch.upload(U, P, HH)
+            for n in range(3):
+                d, _ = ch.step(n + 1)
+                assert d <= 1e-9
+            st = ch.download()
+            ch.bcond()
+            ch.rhs()
+            ch.getuh()
+            out[pair] = (st["U"], st["P"], [ch.field(f) for f in (c3d.F_UH1, c3d.F_UH2, c3d.F_UH3,
+                                                                    c3d.F_HH1, c3d.F_HH2, c3d.F_HH3)])
+            ch.close()
+        (u1, p1, f1), (u2, p2, f2) = out[1], out[2]
+        print("pair vs single: max |dU| %.3e, |dP| %.3e" % (np.max(np.abs(u1 - u2)), np.max(np.abs(p1 - p2))))
+        assert np.array_equal(u1, u2) and np.array_equal(p1, p2)
+        for a, b in zip(f1, f2):
+            assert np.array_equal(a, b)
+    finally:
+        c3d.autotune_reset()
