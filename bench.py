@@ -49,11 +49,14 @@ ALG_BYTES = {
 ALG_BYTES_ADI_X_PER_STEP = 56
 # mixed mode: FP32 intermediates and state shadows, FP64 state, RDP, DP, DIV
 # (state read as FP32 shadows by the line kernels; the update reads and
-# writes the FP64 state and writes the shadows; rhsdp_z adds the FP64 DIV of
-# the state)
+# writes the FP64 state and writes the shadows).  The x and z ADI sweeps run
+# z (u, v: DELTA in and out), then x (u, v, w: DELTA in and out; u's also
+# writes RDP = D1x(DELTA_u)/DT in FP64), then z for w, which also reads RDP,
+# DELTA_v and the FP64 DIV of the state and writes the final RDP: RHSDP has
+# no kernel of its own
 ALG_BYTES_MIXED = {
-    "rhs_x": 44, "rhs_z": 56, "rhs_zw": 20, "rhs_x2": 12, "adi_y": 84, "adi_z": 24,
-    "rhsdp_x": 16, "rhsdp_z": 28, "fft_x": 16, "fft_z": 16, "pois_y": 20, "fft_xinv": 16,
+    "rhs_x": 44, "rhs_z": 56, "rhs_zw": 20, "rhs_x2": 12, "adi_y": 84, "adi_z": 16,
+    "adi_xr": 32, "adi_zr": 36, "fft_x": 16, "fft_z": 16, "pois_y": 20, "fft_xinv": 16,
     # the update: U on x-lines (DP, DELTA_u, U in; U, its shadow, D1x U out),
     # V on y-columns (V, DELTA_v, DP in; V, its shadow out), then W and P on
     # z-lines (DP, DELTA_w, W, P, DIV, U, V in; W, P, their shadows, DIV out)
@@ -487,9 +490,11 @@ KERNEL_KEYS = {"k_rhsy_adiy": "adi_y", "k_rhs_x": "rhs_x", "k_rhs_x2": "rhs_x2",
                "k_rhs_zw": "rhs_zw", "k_adi_x": "adi_x", "k_adi_z": "adi_z", "k_rhsdp_x": "rhsdp_x",
                "k_rhsdp_z": "rhsdp_z", "k_fft_x": "fft_x", "k_fft_z": "fft_z", "k_pois_y": "pois_y",
                "k_fft_xinv": "fft_xinv", "k_rfft_x": "fft_x", "k_rfft_z": "fft_z", "k_rfft_xinv": "fft_xinv",
-               "k_upd_x": "upd_x", "k_upd_xu": "upd_x", "k_upd_v": "upd_v", "k_upd_z": "upd_z"}
-# adi_x runs twice per step; its per-launch algorithmic bytes are the mean
-ADI_X_ALG = {"fp64": ALG_BYTES_ADI_X_PER_STEP / 2.0, "mixed": 12.0}
+               "k_upd_x": "upd_x", "k_upd_xu": "upd_x", "k_upd_v": "upd_v", "k_upd_z": "upd_z",
+               "k_adi_xr": "adi_xr", "k_adi_zr": "adi_zr"}
+# adi_x runs twice per FP64 step; its per-launch algorithmic bytes are the
+# mean (the mixed step has no adi_x: its x-sweeps are adi_xr)
+ADI_X_ALG = {"fp64": ALG_BYTES_ADI_X_PER_STEP / 2.0}
 
 
 def summarize_counters(rd_csv, wr_csv, grid, out_json):

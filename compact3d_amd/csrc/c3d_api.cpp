@@ -2265,7 +2265,7 @@ static const char* kid_names[C3D_NKID] = {
     "bcond", "rhs_x", "rhs_z", "rhs_x2", "rhs_y", "adi_y", "adi_x", "adi_z", "rhsdp_x", "rhsdp_z",
     "fft_x", "fft_z", "pois_y", "fft_xinv", "flow_red", "flow_fin", "upd_x", "ghosts", "upd_z",
     "div_x", "div_z", "cfl", "chkmf", "zero", "apply_op", "sgscs", "wallmodel", "sgs_x", "sgs_z",
-    "sgs_z2", "sgs_x2", "wallss", "filter", "rhs_zw", "upd_v"};
+    "sgs_z2", "sgs_x2", "wallss", "filter", "rhs_zw", "upd_v", "adi_xr", "adi_zr"};
 static_assert(KID_COUNT <= C3D_NKID, "kernel id table too small");
 
 extern "C" const char* c3d_kernel_name(int id) {

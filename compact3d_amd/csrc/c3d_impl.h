@@ -152,7 +152,7 @@ enum {
   KID_RHSDP_X, KID_RHSDP_Z, KID_FFT_X, KID_FFT_Z, KID_POIS_Y, KID_FFT_XINV, KID_FLOW_RED,
   KID_FLOW_FIN, KID_UPD_X, KID_GHOSTS, KID_UPD_Z, KID_DIV_X, KID_DIV_Z, KID_CFL, KID_CHKMF,
   KID_ZERO, KID_APPLY, KID_SGS, KID_WALL, KID_SGS_X, KID_SGS_Z, KID_SGS_Z2, KID_SGS_X2,
-  KID_WALLSS, KID_FILTER, KID_RHS_ZW, KID_UPD_V, KID_COUNT
+  KID_WALLSS, KID_FILTER, KID_RHS_ZW, KID_UPD_V, KID_ADI_XR, KID_ADI_ZR, KID_COUNT
 };
 
 // host-side recorder: when `on`, every launch is bracketed by two events on

@@ -6,6 +6,8 @@
 //   k_rhs_y (c3d_kernels.hip)    pointwise y terms, AB2, wall BC
 //   k_adi_y, k_adi_x, k_adi_z    GETUH1/2/3 (TDMAI, CTDMA1J/3I)      :8218-8627
 //   k_rhsdp_x, k_rhsdp_z         RHSDP                               :8117-8214
+//   (mixed precision: k_adi_y, k_adi_z, then k_adi_xr and k_adi_zr,
+//    the last sweeps, which also form RHSDP)
 //   k_fft_x, k_fft_z, k_pois_y,  GETDP: real-to-complex x-DFT (two  :9271-9441
 //     k_fft_z, k_fft_xinv          rows per complex transform), z-DFT, one
 //                                  partitioned Thomas solve per mode (FP64),
@@ -838,6 +840,96 @@ __global__ __launch_bounds__(512) void k_adi_z(CtxArg<CB_> ca_, CompList cl) {
   if (fin && comp == 0) {
     const double dy = c.m.dy[j];
     block_sum2(sd * dy, su * dy, F.flowp + FP_DU * F.flowr + blockIdx.x, F.flowp + FP_U * F.flowr + blockIdx.x);
+  }
+}
+
+// Mixed mode (FP32 sweeps): the x and z ADI factors act on different indices
+// with coefficients that depend on that index only, so they commute, and
+// the sweeps run in the order y, z, x for u and v and y, x, z for w.  The
+// last sweep of u and of w then holds the final increments DELTA in
+// registers, and RHSDP (:8117-8214) is formed there instead of in two passes
+// of its own: the x-sweep forms RDP = D1x(DELTA_u)/DT, the z-sweep of w adds
+// the y term, DIV(U)/DT of the current state and D1z(DELTA_w)/DT, in FP64
+// arithmetic from the FP32 increments.  The flow-rate sums of UPCALC come
+// from the same sweeps (u: x-blocks, w: z-blocks).
+//
+// x-lines, the last sweep of u and v and the first of w (blockIdx.y: u, v,
+// w).  The FP64 D1x of u reuses the block's LDS: the two FP32 staging tiles
+// as one FP64 tile and the four FP32 exchange arrays as two FP64 ones.
+template <int SEG, bool CB_>
+__global__ __launch_bounds__(256) void k_adi_xr(CtxArg<CB_> ca_) {
+  typedef float T;
+  const Ctx c = ctx_get(ca_);
+  XSETUP;
+  const int comp = blockIdx.y;
+  const Fields F = c.d;
+  T* UH = (T*)(comp == 0 ? F.uh[0] : comp == 1 ? F.uh[1] : F.uh[2]);
+  T d[SEG], pf[SEG];
+  X.pre((const T*)UH, 0, pf);
+  const T* B = X.put(pf);
+  SQ d[q] = XA(B, q - 1) + (T)10. * XA(B, q) + XA(B, q + 1);
+  psolve<T, SEG>(d, G, L, l, gs, c.o.px.adi, bf.xf, bf.xb);
+  X.out(UH, d, comp == 1 ? 2 : 1);  // V rows J=1 are not part of the system
+  if (comp != 0) return;  // uniform per block
+  // u: the flow-rate term of UPCALC (:1079-1090), sum DELTA*DY
+  double sd = 0.0;
+  SQ sd += (double)d[q];
+  block_sum2(sd * (ok ? c.m.dy[j] : 0.0), 0.0, F.flowp + FP_DU * F.flowr + blockIdx.x, nullptr);
+  // RDP = D1x(DELTA_u)/DT, face -> centre (:8138-8186)
+  double* const x8f = (double*)bf.xf;  // G*L doubles over xf, xb
+  double* const x8b = (double*)bf.xl;  // G*L doubles over xl, xr
+  double e[SEG];
+  __syncthreads();  // the FP32 exchange arrays are read no more
+  const double rt = xch_right((double)d[0], G, L, l, gs, x8b);
+  SQ e[q] = 24. * c.s.dx1 * ((double)nxt(d, q, (T)rt) - (double)d[q]);
+  psolve<double, SEG>(e, G, L, l, gs, c.o.px.s22, x8f, x8b);
+  const double rdt = 1. / c.s.dt;
+  SQ e[q] = e[q] * rdt;
+  XStage<double, SEG> X8;
+  const LBuf<double> b8{(double*)bf.st0, (double*)bf.st0, x8f, x8b, nullptr, nullptr};
+  xstage(X8, b8, g, sh, l, p0);
+  X8.out(F.rdp, e);
+}
+
+// z-lines: the last sweep of w, then RDP += ((FJPA DELTA_v(J+1) -
+// FJMU DELTA_v(J))/DY)/DT + DIV(U)/DT + D1z(DELTA_w)/DT (:8138-8211; the
+// CBC term is zero, UBC(.,.,2,.) = 0)
+template <int SEG, bool CB_>
+__global__ __launch_bounds__(512) void k_adi_zr(CtxArg<CB_> ca_) {
+  typedef float T;
+  const Ctx c = ctx_get(ca_);
+  ZSETUP;
+  const Fields F = c.d;
+  const Metrics M = c.m;
+  T* UH = (T*)F.uh[2];
+  T a[SEG], d[SEG], lf, rt;
+  zload((const T*)UH, b, pl, p0, bm, bp, a, lf, rt);
+  SQ d[q] = prv(a, q, lf) + (T)10. * a[q] + nxt(a, q, rt);
+  psolve<T, SEG>(d, G, L, l, gs, PZ.adi, bf.xf, bf.xb);
+  double sd = 0.0;
+  if (ok) SQ {
+      sd += (double)d[q];
+      UH[ZI(q)] = d[q];
+    }
+  // w: the flow-rate term of UPCALC (:1160-1172), sum DELTA*DY
+  block_sum2(sd * M.dy[j], 0.0, F.flowp + FP_DW * F.flowr + blockIdx.x, nullptr);
+  double* const x8f = (double*)bf.xf;
+  double* const x8b = (double*)bf.xl;
+  double e[SEG];
+  __syncthreads();  // the FP32 exchange arrays are read no more
+  const double r8 = xch_right((double)d[0], G, L, l, gs, x8b);
+  SQ e[q] = 24. * c.s.dx3 * ((double)nxt(d, q, (T)r8) - (double)d[q]);
+  psolve<double, SEG>(e, G, L, l, gs, PZ.s22, x8f, x8b);
+  if (!ok) return;
+  const T* dv = (const T*)F.uh[1];
+  double* rdp = F.rdp;
+  const double* dvu = F.div;
+  const double rdt = 1. / c.s.dt, rdy = M.rdy[j], fjpa = M.fjpa[j], fjmu = M.fjmu[j];
+  const unsigned nx = g.nx;
+  SQ {
+    const unsigned a0 = ZI(q);
+    const double ty = ((fjpa * (double)dv[a0 + nx] - fjmu * (double)dv[a0]) * rdy) * rdt;
+    rdp[a0] = ((rdp[a0] + ty) + dvu[a0] * rdt) + e[q] * rdt;
   }
 }
 
@@ -2020,13 +2112,24 @@ static void getuh_t(const Ctx& c, bool fused) {
       hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adi_y<T, 512>), gr, th, 0, c.st, c, y1);
   }
   const size_t lx = lds_line(c.o.lx, true, sizeof(T));
-  // GETUH1/2: y, x, z;  GETUH3: y, z, x (compact3d.f:8508-8607)
-  const CompList x12{2, {0, 1, 0}, {0, 0, 0}};
-  const CompList z123{3, {0, 1, 2}, {1, 1, 0}};
-  const CompList x3{1, {2, 0, 0}, {1, 0, 0}};
-  XSEG(c.o.lx.seg, { KScope ks_(c, KID_ADI_X); C3D_LAUNCH(k_adi_x, (T, S_), dim3(xblocks(c), 2), xthreads(c), lx, c, x12); });
-  ZLAUNCH(T, KID_ADI_Z, k_adi_z, 3, z123);
-  XSEG(c.o.lx.seg, { KScope ks_(c, KID_ADI_X); C3D_LAUNCH(k_adi_x, (T, S_), dim3(xblocks(c), 1), xthreads(c), lx, c, x3); });
+  if constexpr (sizeof(T) == 4) {
+    // mixed mode: u and v sweep y, z, x and w y, x, z; the last sweeps form
+    // RDP and the flow-rate sums (k_adi_xr, k_adi_zr)
+    const CompList z12{2, {0, 1, 0}, {0, 0, 0}};
+    ZLAUNCH(T, KID_ADI_Z, k_adi_z, 2, z12);
+    XSEG(c.o.lx.seg, { KScope ks_(c, KID_ADI_XR); C3D_LAUNCH(k_adi_xr, (S_), dim3(xblocks(c), 3), xthreads(c), lx, c); });
+    const LShape& z_ = c.o.lz32;
+    const size_t lz_ = lds_line(z_, false, sizeof(T));
+    ZSEG32(z_.seg, { KScope ks_(c, KID_ADI_ZR); C3D_LAUNCH(k_adi_zr, (S_), dim3(zblocks(c, z_)), dim3(z_.L * z_.G), lz_, c); });
+  } else {
+    // GETUH1/2: y, x, z;  GETUH3: y, z, x (compact3d.f:8508-8607)
+    const CompList x12{2, {0, 1, 0}, {0, 0, 0}};
+    const CompList z123{3, {0, 1, 2}, {1, 1, 0}};
+    const CompList x3{1, {2, 0, 0}, {1, 0, 0}};
+    XSEG(c.o.lx.seg, { KScope ks_(c, KID_ADI_X); C3D_LAUNCH(k_adi_x, (T, S_), dim3(xblocks(c), 2), xthreads(c), lx, c, x12); });
+    ZLAUNCH(T, KID_ADI_Z, k_adi_z, 3, z123);
+    XSEG(c.o.lx.seg, { KScope ks_(c, KID_ADI_X); C3D_LAUNCH(k_adi_x, (T, S_), dim3(xblocks(c), 1), xthreads(c), lx, c, x3); });
+  }
 }
 void getuh(const Ctx& c) {
   if (c.s.mixed)
@@ -2048,11 +2151,9 @@ static void rhsdp_t(const Ctx& c) {
   XSEG(c.o.lx.seg, { KScope ks_(c, KID_RHSDP_X); C3D_LAUNCH(k_rhsdp_x, (double, S_, TS), dim3(xblocks(c)), xthreads(c), lx, c); });
   ZSEG(c.o.lz.seg, { KScope ks_(c, KID_RHSDP_Z); C3D_LAUNCH(k_rhsdp_z, (double, S_, TS), dim3(zblocks(c)), zthreads(c), lz, c); });
 }
+// mixed mode: the last ADI sweeps formed RDP already (k_adi_xr, k_adi_zr)
 void rhsdp(const Ctx& c) {
-  if (c.s.mixed)
-    rhsdp_t<float>(c);
-  else
-    rhsdp_t<double>(c);
+  if (!c.s.mixed) rhsdp_t<double>(c);
 }
 
 // GETDP in FP64 in both modes: rounding RDP's transform or the solution to
@@ -2195,9 +2296,12 @@ static void upd_z_t(const Ctx& c, bool chk, int refine) {
 // ADI sweeps: u in the z-sweep (blocks of the compute type's z shape), w in
 // the final x-sweep
 void flow_fin(const Ctx& c) {
-  const int nu = c.s.mixed ? zblocks(c, c.o.lz32) : zblocks(c, c.o.lz);
+  // FP64: u from the z-sweep, w from the x-sweep; mixed: u from the x-sweep,
+  // w from the z-sweep (FP32 z tiles)
+  const int nu = c.s.mixed ? xblocks(c) : zblocks(c, c.o.lz);
+  const int nw = c.s.mixed ? zblocks(c, c.o.lz32) : xblocks(c);
   KScope ks_(c, KID_FLOW_FIN);
-  C3D_LAUNCH0(k_flow_fin, dim3(1), dim3(FIN_THREADS), 0, c, nu, xblocks(c), c.s.mixed ? 0 : 1);
+  C3D_LAUNCH0(k_flow_fin, dim3(1), dim3(FIN_THREADS), 0, c, nu, nw, c.s.mixed ? 0 : 1);
 }
 void upd_x(const Ctx& c, bool chk, bool refine) {
   if (c.s.mixed) {

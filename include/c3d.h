@@ -133,8 +133,11 @@ const char *c3d_ensemble_last_error(const c3d_ensemble *e);
 int c3d_bcond(c3d_handle *h);     /* BCOND  compact3d.f:963-1001 */
 int c3d_rhs(c3d_handle *h);       /* RHS1/RHS2/RHS3 compact3d.f:6063, 6784, 7399 (after the
                                      GETUP prologue :895-923 and BCOND) */
-int c3d_getuh(c3d_handle *h);     /* GETUH1/2/3 compact3d.f:8218, 8344, 8478 */
-int c3d_dpcalc(c3d_handle *h);    /* DPCALC = RHSDP + GETDP compact3d.f:1037-1049 */
+int c3d_getuh(c3d_handle *h);     /* GETUH1/2/3 compact3d.f:8218, 8344, 8478; in
+                                     mixed precision also RHSDP (:8117), formed by
+                                     the last ADI sweeps */
+int c3d_dpcalc(c3d_handle *h);    /* DPCALC = RHSDP + GETDP compact3d.f:1037-1049
+                                     (mixed precision: GETDP only) */
 int c3d_upcalc(c3d_handle *h);    /* UPCALC compact3d.f:1052-1238 */
 int c3d_sgs(c3d_handle *h);       /* SGSDY (FILES=1) compact3d.f:1250 / SGSCS (FILES=2) :1413 /
                                      SGSSD (FILES=3) :1502 */

@@ -119,6 +119,36 @@ def test_mixed_rhs_phase():
     b.close()
 
 
+def test_mixed_getuh_forms_rdp():
+    """Mixed mode forms RHSDP (compact3d.f:8117-8214) in its last ADI sweeps
+    (u and v sweep y, z, x; w sweeps y, x, z): after GETUH, RDP equals the
+    oracle's RHSDP, which is linear in UH, of the FP32 increments DELTA the
+    sweeps stored, plus DIV(U)/DT of the resident state, to 1e-12 of its
+    largest value (FP64 arithmetic from the same FP32 increments)"""
+    n1, n2, n3 = 49, 41, 33
+    cfg = config(n1, n2, n3)
+    from pyoracle import Oracle
+    from compact3d_amd import c3d
+    orc = Oracle(**cfg)
+    U, P, HH = inputs(orc, seed=11)
+    ch = c3d.Channel(**cfg, precision=c3d.MIXED)
+    ch.upload(U, P, HH)
+    ch.bcond()
+    ch.rhs()
+    ch.getuh()
+    D = orc.zeros_u()
+    for c in range(3):
+        D[:, :, :, c] = ch.field(c3d.F_UH1 + c)
+    rdp_d = orc.zeros_p()
+    orc.rhsdp(rdp_d, D)
+    ref = rdp_d + ch.field(c3d.F_DIV) / cfg["dt"]
+    got = ch.field(c3d.F_RDP)
+    e = relerr(interior_p(ref, n1, n2, n3), interior_p(got, n1, n2, n3))
+    print("mixed RDP vs oracle RHSDP(DELTA) + DIV/DT: %.2e" % e)
+    assert e <= 1e-12
+    ch.close()
+
+
 def test_mixed_state_roundtrip():
     """the FP64 state survives upload/download in mixed mode bit for bit"""
     n1, n2, n3 = 33, 33, 33
