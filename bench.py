@@ -57,11 +57,11 @@ ALG_BYTES_ADI_X_PER_STEP = 56
 ALG_BYTES_MIXED = {
     "rhs_x": 44, "rhs_z": 56, "rhs_zw": 20, "rhs_x2": 12, "adi_y": 84, "adi_z": 16,
     "adi_xr": 32, "adi_zr": 36, "fft_x": 16, "fft_z": 16, "pois_y": 20, "fft_xinv": 16,
-    # the update: U on x-lines (DP, DELTA_u, U in; U, its shadow, D1x U out),
-    # V on y-columns (V, DELTA_v, DP in; V, its shadow out), then W and P on
-    # z-lines (DP, DELTA_w, W, P, DIV, U, V in; W, P, their shadows, DIV out)
-    # with the y and z parts of DIV, DIVMAX and CFL
-    "upd_x": 40, "upd_v": 32, "upd_z": 84,
+    # the update: U and V on x-lines (DP, DELTA_u, U, DELTA_v, V in; U, V,
+    # their shadows, D1x U out), then W and P on z-lines (DP, DELTA_w, W, P,
+    # D1x U, U, V in; W, P, their shadows, the FP32 DIV out) with the y and z
+    # parts of DIV, DIVMAX and CFL
+    "upd_x": 64, "upd_z": 80,
 }
 
 

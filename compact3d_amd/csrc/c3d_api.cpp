@@ -503,8 +503,9 @@ long long c3d_footprint_bytes(const c3d_config* cfg) {
   const bool mixed = cfg->precision == C3D_MIXED;
   const long long es = mixed ? 4 : 8;
   // FP64: state 4, DP, RDP, SGSVIS, DIV, 2 scratch (+ second V in mixed
-  // mode); intermediates 10 + HH 6; FP32 shadows 4 in mixed mode; LES 7
-  long long bytes = (10 + (mixed ? 1 : 0)) * fsz * 8 + 16 * fsz * es + (mixed ? 4 * fsz * 4 : 0);
+  // mode); intermediates 10 + HH 6; FP32 shadows 4 and DIV in mixed mode;
+  // LES 7
+  long long bytes = (10 + (mixed ? 1 : 0)) * fsz * 8 + 16 * fsz * es + (mixed ? 5 * fsz * 4 : 0);
   if (cfg->files != 0.0) bytes += 7 * fsz * es + (mixed ? fsz * 4 : 0);
   if (cfg->files == 1.0) bytes += (2 * n2m * ((nx + 15) / 16) * ((nz + 15) / 16) + n2m + 3) * 8;
   if (cfg->files == 3.0) bytes += (10 * n2m * ((nx + 15) / 16) * ((nz + 15) / 16) + 2 * (n2m + 3)) * 8;
@@ -755,7 +756,7 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   if (mixed) {
     h->vbuf[0] = F.v;
     if (dalloc(h, &h->vbuf[1], fs)) return bad(h->err);
-    float** f32[] = {&F.uf, &F.vf, &F.wf, &F.pf};
+    float** f32[] = {&F.uf, &F.vf, &F.wf, &F.pf, &F.divf};
     for (auto pp : f32)
       if (dalloc(h, pp, fs)) return bad(h->err);
     F.su = F.uf;
@@ -940,6 +941,7 @@ extern "C" int c3d_upload_state(c3d_handle* h, const double* U, const double* P,
     c3dk::to_f32(h->c, F.p, F.pf, fsz);
     if (F.sgsf) c3dk::to_f32(h->c, F.sgs, F.sgsf, fsz);
     c3dk::divcheck(h->c);
+    c3dk::to_f32(h->c, F.div, F.divf, fsz);
     HIPCHK(hipStreamSynchronize(h->c.st));
     HIPCHK(hipGetLastError());
   }
@@ -2106,8 +2108,11 @@ extern "C" int c3d_get_field(c3d_handle* h, int which, double* out) {
       if (!h->c.s.mixed) {
         c3dk::divcheck(h->c);
         if (launched(h)) return -1;
+        return from_dev(h, out, F.div, LAY_P);
       }
-      return from_dev(h, out, F.div, LAY_P);
+      c3dk::to_f64(h->c, F.divf, h->dtmp, h->c.g.fsz);
+      if (launched(h)) return -1;
+      return from_dev(h, out, h->dtmp, LAY_P);
     case C3D_F_HH1: src = F.hh[0]; break;
     case C3D_F_HH2: src = F.hh[1]; break;
     case C3D_F_HH3: src = F.hh[2]; break;
@@ -2179,7 +2184,13 @@ extern "C" int c3d_set_field(c3d_handle* h, int which, const double* in) {
       HIPCHK(hipStreamSynchronize(h->c.st));
       return 0;
     }
-    case C3D_F_DIV: return to_dev(h, F.div, in, LAY_P);
+    case C3D_F_DIV:
+      if (to_dev(h, F.div, in, LAY_P)) return -1;
+      if (h->c.s.mixed) {
+        c3dk::to_f32(h->c, F.div, F.divf, h->c.g.fsz);
+        if (launched(h)) return -1;
+      }
+      return 0;
     default: return fail(h, "only C3D_F_TAUW, C3D_F_TAUB, C3D_F_IWM and C3D_F_DIV can be set");
   }
 }

@@ -99,6 +99,11 @@ struct FFTPlan {
 struct Fields {
   double *u, *v, *w, *p;
   float *uf, *vf, *wf, *pf;  // FP32 shadows (mixed mode)
+  // mixed mode: DIV of the state, which the next RDP starts from, in FP32
+  // (after the FP64 projection its values are at the rounding level); the
+  // update passes the FP64 D1x U of the new field to k_upd_z in RDP's
+  // buffer, which GETDP has consumed by then
+  float* divf;
   // mixed mode double-buffers V: the update reads V(J+1) of the old field
   // while other blocks write the new one
   const double* vold;

@@ -923,13 +923,13 @@ __global__ __launch_bounds__(512) void k_adi_zr(CtxArg<CB_> ca_) {
   if (!ok) return;
   const T* dv = (const T*)F.uh[1];
   double* rdp = F.rdp;
-  const double* dvu = F.div;
+  const float* dvu = F.divf;
   const double rdt = 1. / c.s.dt, rdy = M.rdy[j], fjpa = M.fjpa[j], fjmu = M.fjmu[j];
   const unsigned nx = g.nx;
   SQ {
     const unsigned a0 = ZI(q);
     const double ty = ((fjpa * (double)dv[a0 + nx] - fjmu * (double)dv[a0]) * rdy) * rdt;
-    rdp[a0] = ((rdp[a0] + ty) + dvu[a0] * rdt) + e[q] * rdt;
+    rdp[a0] = ((rdp[a0] + ty) + (double)dvu[a0] * rdt) + e[q] * rdt;
   }
 }
 
@@ -1633,7 +1633,7 @@ __global__ __launch_bounds__(NT) void k_fft_xinv(CtxArg<CB_> ca_) {
 // x-lines (FP64 mode): U1 = UH1 - DT*D1x(DP) - DT*DMPRESG (:1113-1128); U2 =
 // UH2 - DT*(DP(J)-DP(J-1))/H(J) (:1139-1145); with `chk`, the x part of the
 // divergence of the new field, (V(J+1)-V(J))/DY + D1x(U) (:9066-9102).
-// Mixed mode updates U and V with k_upd_xu and k_upd_v below.
+// Mixed mode updates U and V with k_upd_xu below.
 template <typename T, int SEG, bool CB_>
 __global__ __launch_bounds__(256) void k_upd_x(CtxArg<CB_> ca_, int chk) {
   const Ctx c = ctx_get(ca_);
@@ -1680,11 +1680,10 @@ __global__ __launch_bounds__(256) void k_upd_x(CtxArg<CB_> ca_, int chk) {
   X.out(F.div, d);
 }
 
-// Mixed mode splits the x-line update: k_upd_xu updates U (x-lines) and
-// stores D1x of the new U in DIV; k_upd_v updates V along y-columns (DP(J-1)
-// carried to the next J); k_upd_z adds (V(J+1) - V(J))/DY, from the V(J),
-// V(J+1) its CFL term reads anyway, and D1z of the new W.  Same expressions
-// and operation order as k_upd_x.
+// Mixed mode: k_upd_xu updates U on x-lines and stores D1x of the new U
+// (FP64, in RDP's buffer), then updates V pointwise; k_upd_z adds
+// (V(J+1) - V(J))/DY, from the V(J), V(J+1) its CFL term reads anyway, and
+// D1z of the new W.  Same expressions and operation order as k_upd_x.
 template <int SEG, bool CB_>
 __global__ __launch_bounds__(256) void k_upd_xu(CtxArg<CB_> ca_, int refine) {
   const Ctx c = ctx_get(ca_);
@@ -1719,46 +1718,28 @@ __global__ __launch_bounds__(256) void k_upd_xu(CtxArg<CB_> ca_, int refine) {
   const T rt = xch_right(un[0], G, L, l, gs, bf.xr);
   SQ d[q] = (T)24. * dx1 * (nxt(un, q, rt) - un[q]);
   psolve<T, SEG>(d, G, L, l, gs, c.o.px.s22, bf.xf, bf.xb);
-  X.out(F.div, d);
-}
-
-// y-columns: thread (i, k, chunk) walks J = J0..J1 of one column (i fastest
-// across threads: rows are coalesced); V(J) = VOLD(J) + (DELTA_v(J) -
-// DT*(DP(J)-DP(J-1))/H(J)) for J >= 2 (V(1) is the wall's and left to the
-// ghost pass)
-template <bool CB_>
-__global__ __launch_bounds__(256) void k_upd_v(CtxArg<CB_> ca_, int refine, int ch) {
-  const Ctx c = ctx_get(ca_);
-  const Geo g = c.g;
-  const Metrics M = c.m;
-  const Fields F = c.d;
-  const double dt = c.s.dt;
-  const double ku = refine ? 0.0 : 1.0;
-  const unsigned ncol = (unsigned)g.nx * (unsigned)g.nz;
-  const unsigned t = blockIdx.x * 256u + threadIdx.x;
-  const unsigned nch = (unsigned)((g.n2m + ch - 1) / ch);
-  if (t >= ncol * nch) return;
-  const unsigned col = t % ncol, cc = t / ncol;
-  const unsigned i = col % (unsigned)g.nx, k = col / (unsigned)g.nx;
-  const int J0 = 1 + (int)cc * ch, J1 = min(J0 + ch - 1, g.n2m);
-  const unsigned nx = (unsigned)g.nx, base = k * (unsigned)g.plane + i;
-  const float* dvr = (const float*)F.uh[1];
-  const double* vo = F.vold;
-  // V(J) of the new field from DP(J-1), DP(J)
-  auto vnew = [&](int J, double dpj, double dpm) -> double {
-    const double d = ku * (double)dvr[base + (unsigned)J * nx] - dt * (dpj - dpm) * M.rh[J];
-    return vo[base + (unsigned)J * nx] + d;
-  };
-  double dpm = F.dp[base + (unsigned)(J0 - 1) * nx];
-  for (int J = J0; J <= J1; J++) {
-    const unsigned a = base + (unsigned)J * nx;
-    const double dpj = F.dp[a];
-    if (J >= 2) {
-      const double v = vnew(J, dpj, dpm);
-      F.v[a] = v;
-      F.vf[a] = (float)v;
+  X.out(F.rdp, d);  // D1x U of the new field, for k_upd_z
+  // V (J >= 2; V(1) is the wall's, left to the ghost pass), pointwise in the
+  // staging's pair layout: V = VOLD + DELTA_v - DT*(DP(J) - DP(J-1))/H(J)
+  // (:1139-1145).  The block's DP rows come back from L2 (it read them
+  // above), so only VOLD and DELTA_v are new reads.
+  double vo[SEG], dj[SEG], dm[SEG];
+  float dv[SEG];
+  X.pre_pairs(F.vold, 0u, vo);
+  X.pre_pairs((const float*)F.uh[1], 0u, dv);
+  X.pre_pairs(F.dp, 0u, dj);
+  X.pre_pairs(F.dp, (unsigned)(-g.nx), dm);  // row J-1 (the offset wraps in 32 bits)
+  const int lq = threadIdx.x / (sh.n >> 1);
+#pragma unroll
+  for (int m = 0; m < SEG / 2; m++) {
+    const int jm = 1 + (blockIdx.x * L + lq + m * X.lstep2) % g.n2m;
+    if (((X.vm2 >> m) & 1u) && jm >= 2) {
+      const double rh = M.rh[jm];
+      const double v0 = vo[2 * m] + (ku * (double)dv[2 * m] - dt * (dj[2 * m] - dm[2 * m]) * rh);
+      const double v1 = vo[2 * m + 1] + (ku * (double)dv[2 * m + 1] - dt * (dj[2 * m + 1] - dm[2 * m + 1]) * rh);
+      *(double2*)(F.v + X.go2[m]) = double2{v0, v1};
+      *(float2*)(F.vf + X.go2[m]) = float2{(float)v0, (float)v1};
     }
-    dpm = dpj;
   }
 }
 
@@ -1827,11 +1808,12 @@ __global__ __launch_bounds__(512) void k_upd_z(CtxArg<CB_> ca_, int chk, int ref
       const double v0 = F.v[a], v1 = F.v[a + g.nx];
       double dv;
       if (MIX) {
-        // DIV of the new field: D1x U (k_upd_xu) + (V(J+1) - V(J))/DY (the
-        // wall values V(1) = V(N2) = 0) + D1z W
+        // DIV of the new field: D1x U (k_upd_xu, in RDP's buffer) +
+        // (V(J+1) - V(J))/DY (the wall values V(1) = V(N2) = 0) + D1z W,
+        // kept in FP32 for the next RDP
         const double vj = j >= 2 ? v0 : 0.0, vp = j + 1 <= g.n2m ? v1 : 0.0;
-        dv = ((vp - vj) * rdyj + F.div[a]) + (double)d[q];
-        F.div[a] = dv;
+        dv = ((vp - vj) * rdyj + F.rdp[a]) + (double)d[q];
+        F.divf[a] = (float)dv;
       } else {
         dv = F.div[a] + (double)d[q];
       }
@@ -2305,13 +2287,9 @@ void flow_fin(const Ctx& c) {
 }
 void upd_x(const Ctx& c, bool chk, bool refine) {
   if (c.s.mixed) {
-    // U on x-lines, then V on y-columns (chunks of up to 16 J)
+    // U on x-lines, then V pointwise in the same blocks
     const size_t lx = lds_line(c.o.lx, true, sizeof(double));
     XSEG(c.o.lx.seg, { KScope ks_(c, KID_UPD_X); C3D_LAUNCH(k_upd_xu, (S_), dim3(xblocks(c)), xthreads(c), lx, c, refine ? 1 : 0); });
-    const int ch = 16, nch = (c.g.n2m + ch - 1) / ch;
-    const long long nt = (long long)c.g.nx * c.g.nz * nch;
-    KScope ks_(c, KID_UPD_V);
-    C3D_LAUNCH0(k_upd_v, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c, refine ? 1 : 0, ch);
   } else {
     upd_x_fp64(c, chk);
   }
@@ -2329,7 +2307,7 @@ __global__ void k_div_rdp(Ctx c) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= g.nx) return;
   const long long b = gidx(g, i, 1 + blockIdx.y, blockIdx.z);
-  c.d.rdp[b] = c.d.div[b] / c.s.dt;
+  c.d.rdp[b] = (c.s.mixed ? (double)c.d.divf[b] : c.d.div[b]) / c.s.dt;
 }
 void div_rdp(const Ctx& c) {
   const int bx = c.g.nx < 256 ? c.g.nx : 256;
