@@ -138,12 +138,17 @@ static PSolve make_psolve(double a, double b, int seg) {
   return p;
 }
 
+// x-line tiles: the staging's pair index arithmetic needs the block, L*G =
+// (2L/seg)*(n/2) threads, to be a multiple of n/2, i.e. L a multiple of
+// seg/2; L is kept a multiple of seg where L*G <= 256 allows it, and lines
+// of 512 points (G = 64) take 4 lines per 256-thread tile
+static int xline_step(int seg, int G) { return seg * G <= 256 ? seg : std::max(1, seg / 2); }
+
 // line-engine tile: seg = 8 when it divides n (else 2, the other compiled
 // length), G = n/seg segments, L lines per tile.  x-lines: L*G <= 256 with L
-// a multiple of seg (so the block size is a multiple of n, which the
-// staging's index arithmetic relies on), 2 LDS row tiles of n*(L+1) plus
-// the exchange arrays within 64 KB.  z-lines: L*G <= 512 and L a multiple of
-// 16 (128-byte rows) up to 64.
+// a multiple of xline_step, 2 LDS row tiles of n*(L+1) plus the exchange
+// arrays within 64 KB (so n <= 512).  z-lines: L*G <= 512 and L a multiple
+// of 16 (128-byte rows) up to 64.
 static bool make_lshape(int n, bool staged, size_t es, LShape& sh) {
   const int seg = (n % 8 == 0) ? 8 : 2;
   if (n % seg) return false;
@@ -152,9 +157,10 @@ static bool make_lshape(int n, bool staged, size_t es, LShape& sh) {
   if (G > tmax) return false;
   int L = std::min(64, tmax / G);
   if (staged) {
-    L -= L % seg;
-    while (L > seg && es * (2 * (size_t)n * (L + 1) + 4 * (size_t)G * L) > 65536) L -= seg;
-    if (L < seg) return false;
+    const int step = xline_step(seg, G);
+    L -= L % step;
+    while (L > step && es * (2 * (size_t)n * (L + 1) + 4 * (size_t)G * L) > 65536) L -= step;
+    if (L < step || es * (2 * (size_t)n * (L + 1) + 4 * (size_t)G * L) > 65536) return false;
   } else if (L >= 16) {
     L -= L % 16;
   }
@@ -624,7 +630,7 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   build_mesh(h);
   Ops& o = c.o;
   if (!make_lshape(g.nx, true, sizeof(double), o.lx) || !make_lshape(g.nz, false, sizeof(double), o.lz))
-    return bad("N1-1 (N3-1) too long for the line tiles: at most 256 (4096) points");
+    return bad("N1-1 (N3-1) too long for the line tiles: at most 512 (4096) points");
   if (!make_lshape32(g.nz, o.lz32)) return bad("N3-1 too long for the FP32 line tiles");
   // FP64 y-sweep: 6-point chunks (every operand of a chunk is loaded before
   // its arithmetic), 16 columns (128-byte rows), up to 512 threads
@@ -1071,7 +1077,6 @@ static int refine_from(c3d_handle* h, double dm, double* divmax, double* cflm) {
     c3dk::getdp(h->c);
     swap_v(h);
     c3dk::upd_x(h->c, true, true);
-    c3dk::ghosts(h->c);
     c3dk::upd_z(h->c, true, true);
     if (read_checks(h, &dm, cflm)) return -1;
   }
@@ -1255,7 +1260,7 @@ extern "C" int c3d_sync(c3d_handle* h, double* divmax, double* cflm) {
 
 // ------------------------------------------------------------- autotuner
 // Candidate shapes per knob, each within what its kernels assume: x tiles a
-// multiple of the segment with L*G <= 256 and two FP64 staging tiles in
+// multiple of xline_step with L*G <= 256 and two FP64 staging tiles in
 // 64 KB; z tiles L*G <= 512 with a compiled segment (FP64 8/2, FP32
 // 16/8/2); y tiles of <= 12-point chunks and <= 512 threads (Poisson
 // <= 256); FFT tiles of <= 4096 complex values and <= 64 KB of LDS (the
@@ -1263,8 +1268,8 @@ extern "C" int c3d_sync(c3d_handle* h, double* divmax, double* cflm) {
 // values.
 static std::vector<LShape> cand_lx(int n) {
   std::vector<LShape> v;
-  const int seg = (n % 8 == 0) ? 8 : 2, G = n / seg;
-  for (int L = seg; L * G <= 256 && L <= 64; L += seg)
+  const int seg = (n % 8 == 0) ? 8 : 2, G = n / seg, step = xline_step(seg, G);
+  for (int L = step; L * G <= 256 && L <= 64; L += step)
     if (8 * (2 * (size_t)n * (L + 1) + 4 * (size_t)G * L) <= 65536) v.push_back(LShape{n, L, G, seg});
   return v;
 }

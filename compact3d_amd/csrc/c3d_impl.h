@@ -262,7 +262,6 @@ void flow_fin(const Ctx& c);  // DMPRESG/DMPRESG3 from the ADI sweeps' partial s
 void upd_x(const Ctx& c, bool chk, bool refine = false);
 void upd_z(const Ctx& c, bool chk, bool refine = false);
 void div_rdp(const Ctx& c);
-void ghosts(const Ctx& c);
 void pois_table(const Ctx& c);
 void divcheck(const Ctx& c);
 void cfl(const Ctx& c);

@@ -3,7 +3,6 @@
 //
 //   k_bcond                 BCOND                                 :963-1001
 //   k_rhs_y                 RHS1/2/3 pointwise y parts, AB2, wall BC :6063-8114
-//   k_ghosts                ghost planes after the update           :1130-1215
 //   k_div_x/k_div_z, k_cfl  DIVCHECK and CFL on the resident field  :9049-9169
 //   k_chkmf                 CHKMF                                   :9463-9503
 //   k_apply_op              one compact operator (test hook)
@@ -176,32 +175,6 @@ __global__ __launch_bounds__(256) void k_plane_stats(Ctx c, double* out) {
   }
   const double nxz = 1.0 / (double)npl;
   if (t < ST_NQ) out[(long long)t * g.n2m + (j - 1)] = red[t][0] * nxz;
-}
-
-// ghost planes after the update (:1130-1135, :1147-1152, :1210-1215)
-template <bool CB_>
-__global__ void k_ghosts(CtxArg<CB_> ca_) {
-  const Ctx c = ctx_get(ca_);
-  const Geo g = c.g;
-  const long long n = (long long)g.nx * g.nz;
-  long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const int i = t % g.nx, k = t / g.nx;
-  const Fields F = c.d;
-  F.u[gidx(g, i, 0, k)] = 0.0;
-  F.u[gidx(g, i, g.n2, k)] = F.ubc_u[t];
-  F.v[gidx(g, i, 1, k)] = 0.0;
-  F.v[gidx(g, i, g.n2, k)] = 0.0;
-  F.w[gidx(g, i, 0, k)] = 0.0;
-  F.w[gidx(g, i, g.n2, k)] = F.ubc_w[t];
-  if (c.s.mixed) {
-    F.uf[gidx(g, i, 0, k)] = 0.f;
-    F.uf[gidx(g, i, g.n2, k)] = (float)F.ubc_u[t];
-    F.vf[gidx(g, i, 1, k)] = 0.f;
-    F.vf[gidx(g, i, g.n2, k)] = 0.f;
-    F.wf[gidx(g, i, 0, k)] = 0.f;
-    F.wf[gidx(g, i, g.n2, k)] = (float)F.ubc_w[t];
-  }
 }
 
 // ------------------------------------------------ DIVCHECK / CFL (alone)
@@ -504,8 +477,7 @@ void upcalc(const Ctx& c, bool chk) {
   // mixed mode always forms DIV of the new field: the next RDP starts from it
   chk = chk || c.s.mixed;
   flow_fin(c);
-  upd_x(c, chk);
-  ghosts(c);
+  upd_x(c, chk);  // also the ghost planes
   upd_z(c, chk);
 }
 
@@ -546,11 +518,6 @@ void cellcentre(const Ctx& c, int q, double* out) {
   KScope ks_(c, KID_CHKMF);
   hipLaunchKernelGGL(k_cellcentre, dim3(pgrid((long long)c.g.nx * c.g.n2m * c.g.nz)), dim3(C3D_NT), 0, c.st, c,
                      q, out);
-}
-
-void ghosts(const Ctx& c) {
-  KScope ks_(c, KID_GHOSTS);
-  C3D_LAUNCH0(k_ghosts, dim3(pgrid((long long)c.g.nx * c.g.nz)), dim3(C3D_NT), 0, c);
 }
 
 void divcheck(const Ctx& c) {

@@ -47,7 +47,7 @@ __device__ __forceinline__ LBuf<T> lbuf(unsigned char* sm, const LShape& sh, boo
 // each thread moves exactly SEG of them, as SEG/2 pairs of neighbouring
 // points: pair m of thread t holds points 2*(t % (n/2)) and the next one of
 // tile row t/(n/2) + m*(2*blockDim/n) (the host makes blockDim a multiple of
-// n and n is even), one 8-byte (FP32) or 16-byte (FP64) load or store per
+// n/2 and n is even), one 8-byte (FP32) or 16-byte (FP64) load or store per
 // pair.  A wave's load then carries at least 512 bytes: single 4-byte loads
 // halved the bytes in flight per instruction and measured ~20 % slower in
 // the FP32 line kernels, and 16-byte FP64 pairs beat single 8-byte loads by

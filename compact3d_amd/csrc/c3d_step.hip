@@ -12,8 +12,8 @@
 //     k_fft_z, k_fft_xinv          rows per complex transform), z-DFT, one
 //                                  partitioned Thomas solve per mode (FP64),
 //                                  inverse z- and x-DFTs (FP64)
-//   k_flow_red/fin, k_upd_x,     UPCALC (+ DIVCHECK and CFL fused)   :1052-1238
-//     k_ghosts, k_upd_z
+//   k_flow_fin, k_upd_x (and     UPCALC (+ DIVCHECK and CFL fused)   :1052-1238
+//     the ghost planes), k_upd_z
 //
 // Thread layouts.  z-lines: a tile is L consecutive i at one j; thread
 // (l, g) keeps points k in [g*seg, (g+1)*seg) of column i = i0+l in
@@ -1630,6 +1630,50 @@ __global__ __launch_bounds__(NT) void k_fft_xinv(CtxArg<CB_> ca_) {
 }
 
 // ----------------------------------------------------------------- UPCALC
+// The ghost planes after the update (:1130-1135, :1147-1152, :1210-1215),
+// written by the x-line update blocks that hold the rows J = 1 and J = N2M,
+// in the staging's pair layout: U(0) = W(0) = V(1) = V(N2) = 0, U(N2) and
+// W(N2) from BCOND's UBC, and their FP32 shadows in mixed mode.
+template <int SEG>
+__device__ __forceinline__ void upd_ghosts(const Ctx& c, const XStage<double, SEG>& X, int L, int n) {
+  const Geo& g = c.g;
+  const Fields& F = c.d;
+  const unsigned nx = (unsigned)g.nx;
+  const int nh = n >> 1, pq = threadIdx.x % nh, lq = threadIdx.x / nh;
+  const bool mix = c.s.mixed != 0;
+#pragma unroll
+  for (int m = 0; m < SEG / 2; m++) {
+    if (!((X.vm2 >> m) & 1u)) continue;
+    const int ln = blockIdx.x * L + lq + m * X.lstep2;
+    const int jj = 1 + ln % g.n2m;
+    const unsigned o = X.go2[m];
+    if (jj == 1) {
+      const double2 z{0.0, 0.0};
+      *(double2*)(F.u + (o - nx)) = z;
+      *(double2*)(F.w + (o - nx)) = z;
+      *(double2*)(F.v + o) = z;
+      if (mix) {
+        const float2 zf{0.f, 0.f};
+        *(float2*)(F.uf + (o - nx)) = zf;
+        *(float2*)(F.wf + (o - nx)) = zf;
+        *(float2*)(F.vf + o) = zf;
+      }
+    }
+    if (jj == g.n2m) {
+      const unsigned t2 = (unsigned)(ln / g.n2m) * nx + 2u * (unsigned)pq;
+      const double2 ub{F.ubc_u[t2], F.ubc_u[t2 + 1]}, wb{F.ubc_w[t2], F.ubc_w[t2 + 1]};
+      *(double2*)(F.u + (o + nx)) = ub;
+      *(double2*)(F.w + (o + nx)) = wb;
+      *(double2*)(F.v + (o + nx)) = double2{0.0, 0.0};
+      if (mix) {
+        *(float2*)(F.uf + (o + nx)) = float2{(float)ub.x, (float)ub.y};
+        *(float2*)(F.wf + (o + nx)) = float2{(float)wb.x, (float)wb.y};
+        *(float2*)(F.vf + (o + nx)) = float2{0.f, 0.f};
+      }
+    }
+  }
+}
+
 // x-lines (FP64 mode): U1 = UH1 - DT*D1x(DP) - DT*DMPRESG (:1113-1128); U2 =
 // UH2 - DT*(DP(J)-DP(J-1))/H(J) (:1139-1145); with `chk`, the x part of the
 // divergence of the new field, (V(J+1)-V(J))/DY + D1x(U) (:9066-9102).
@@ -1664,6 +1708,7 @@ __global__ __launch_bounds__(256) void k_upd_x(CtxArg<CB_> ca_, int chk) {
   SQ d[q] = (j >= 2) ? XA(B1, q) - dt * (dpr[q] - XA(B, q)) * rhj : (T)0;
   if (chk) X.pre((const T*)F.uh[1], 1, pf);
   X.out(F.v, d, 2);  // V(I,1,K) is not written by UPCALC
+  upd_ghosts<SEG>(c, X, L, sh.n);
   if (!chk) return;
   // new V at J+1 (zero at J+1 = N2: the wall value); d = (V(J+1) - V(J))/DY
   const T rhjp = (T)M.rh[j + 1], rdy = (T)M.rdy[j];
@@ -1741,6 +1786,7 @@ __global__ __launch_bounds__(256) void k_upd_xu(CtxArg<CB_> ca_, int refine) {
       *(float2*)(F.vf + X.go2[m]) = float2{(float)v0, (float)v1};
     }
   }
+  upd_ghosts<SEG>(c, X, L, sh.n);
 }
 
 // z-lines: U3 = UH3 - DT*D1z(DP) - DT*DMPRESG3 (:1193-1208); P += DP
