@@ -125,11 +125,118 @@ __device__ __forceinline__ T sdots(const T s[6]) {
 // SGSCS (compact3d.f:1413-1488) with STRAIN: |S| = sqrt(2 S:S),
 // nu_t = max((0.1 (1-exp(-y+/26)))^2 (DY/(DX1 DX3))^(2/3) |S|, -1/RE),
 // y+ = (Y(J)+DY(J)/2) u_tau RE, u_tau = (WSM1^2 + WSM3^2)^(1/4) (previous step).
-// One block per (tile of 64 i x 4*kpt k, plane J): the plane's factors
-// (STRAIN's and the model constant times the filter width) once per
-// thread, then kpt points per thread (8 on large grids, fewer when the grid
-// would have fewer than ~2048 blocks).
+// One block per (tile of 64 i x 4 runs of kpt consecutive k, plane J): the
+// plane's factors (STRAIN's and the model constant times the filter width)
+// once per thread; each thread walks its run of k with the stencil values
+// of the planes K-1, K and K+1 in registers (SPlane), so every value of a
+// plane is loaded once instead of once per point that reads it.  The
+// arithmetic of each point is strain_core's, term for term.
 #define SG_TI 64
+// the values of one k-plane that STRAIN reads at (I, J) (as plane K or K+1;
+// plane K-1 uses the U and V entries at (I, J) and (I, J+1) only)
+template <typename T>
+struct SPlane {
+  T u_i, u_ip, u_ijp, u_ipjp, u_ijm, u_ipjm, u_in2, u_ipn2, u_i0, u_ip0;
+  T v_ip, v_ipjp, v_i, v_ijp, v_im, v_imjp;
+  T w_ip, w_i, w_im, w_ijp, w_ijm, w_in2, w_i0;
+};
+template <typename T>
+__device__ __forceinline__ SPlane<T> splane(const Ctx& c, bool wm0, int i, int ip, int im, int j, int k) {
+  const Geo& g = c.g;
+  const T* U = sizeof(T) == 4 ? (const T*)c.d.su : (const T*)c.d.u;
+  const T* V = sizeof(T) == 4 ? (const T*)c.d.sv : (const T*)c.d.v;
+  const T* W = sizeof(T) == 4 ? (const T*)c.d.sw : (const T*)c.d.w;
+  const unsigned nx = (unsigned)g.nx;
+  const unsigned r0 = (unsigned)k * (unsigned)g.plane, rj = r0 + (unsigned)j * nx;
+  const unsigned rjp = rj + nx, rjm = rj - nx, rn2 = r0 + (unsigned)g.n2 * nx;
+  SPlane<T> p;
+  p.u_i = U[rj + i];
+  p.u_ip = U[rj + ip];
+  p.u_ijp = U[rjp + i];
+  p.u_ipjp = U[rjp + ip];
+  p.u_ijm = U[rjm + i];
+  p.u_ipjm = U[rjm + ip];
+  p.v_ip = V[rj + ip];
+  p.v_ipjp = V[rjp + ip];
+  p.v_i = V[rj + i];
+  p.v_ijp = V[rjp + i];
+  p.v_im = V[rj + im];
+  p.v_imjp = V[rjp + im];
+  p.w_ip = W[rj + ip];
+  p.w_i = W[rj + i];
+  p.w_im = W[rj + im];
+  p.w_ijp = W[rjp + i];
+  p.w_ijm = W[rjm + i];
+  if (wm0) {
+    p.u_in2 = U[rn2 + i];
+    p.u_ipn2 = U[rn2 + ip];
+    p.u_i0 = U[r0 + i];
+    p.u_ip0 = U[r0 + ip];
+    p.w_in2 = W[rn2 + i];
+    p.w_i0 = W[r0 + i];
+  } else {
+    p.u_in2 = p.u_ipn2 = p.u_i0 = p.u_ip0 = p.w_in2 = p.w_i0 = (T)0;
+  }
+  return p;
+}
+// strain_core's six components from the planes K-1 (pm), K (p0), K+1 (pp)
+template <typename T>
+__device__ __forceinline__ void strain_win(const Ctx& c, const StrainJ& f, const SPlane<T>& pm, const SPlane<T>& p0,
+                                           const SPlane<T>& pp, T sr[6]) {
+  const T DX1 = (T)c.s.dx1, DX3 = (T)c.s.dx3;
+  const T up = (T)f.up, um = (T)f.um, cp = (T)f.cp, cm = (T)f.cm, yf = (T)f.yf, rdy = (T)f.rdy;
+  const T dyj = (T)f.dyj, dyjp = (T)f.dyjp, dyjm = (T)f.dyjm, h = (T)0.5;
+  T u2 = p0.u_ip, u1 = p0.u_i;
+  sr[0] = DX1 * (u2 - u1);
+  T ujp = h * (p0.u_ijp + p0.u_ipjp);
+  T ujc = h * (p0.u_i + p0.u_ip);
+  T ujm = h * (p0.u_ijm + p0.u_ipjm);
+  if (f.wm0) {
+    u2 = up * (dyj * ujp + dyjp * ujc) + cp * (p0.u_in2 + p0.u_ipn2);
+    u1 = um * (dyj * ujm + dyjm * ujc) + cm * (p0.u_i0 + p0.u_ip0);
+  } else {
+    u2 = up * (dyj * ujp + dyjp * ujc) + cp * ujc;
+    u1 = um * (dyj * ujm + dyjm * ujc) + cm * ujc;
+  }
+  T vip = h * (p0.v_ip + p0.v_ipjp);
+  T vic = h * (p0.v_i + p0.v_ijp);
+  T vim = h * (p0.v_im + p0.v_imjp);
+  T v2 = h * (vip + vic), v1 = h * (vim + vic);
+  sr[1] = h * (DX1 * (v2 - v1) + yf * (u2 - u1));
+  T ukp = h * (pp.u_i + pp.u_ip);
+  T ukc = h * (p0.u_i + p0.u_ip);
+  T ukm = h * (pm.u_i + pm.u_ip);
+  u2 = h * (ukp + ukc);
+  u1 = h * (ukm + ukc);
+  T wip = h * (p0.w_ip + pp.w_ip);
+  T wic = h * (p0.w_i + pp.w_i);
+  T wim = h * (p0.w_im + pp.w_im);
+  T w2 = h * (wip + wic), w1 = h * (wim + wic);
+  sr[2] = h * (DX1 * (w2 - w1) + DX3 * (u2 - u1));
+  v2 = p0.v_ijp;
+  v1 = p0.v_i;
+  sr[3] = rdy * (v2 - v1);
+  T wjp = h * (p0.w_ijp + pp.w_ijp);
+  T wjc = h * (p0.w_i + pp.w_i);
+  T wjm = h * (p0.w_ijm + pp.w_ijm);
+  if (f.wm0) {
+    w2 = up * (dyj * wjp + dyjp * wjc) + cp * (p0.w_in2 + pp.w_in2);
+    w1 = um * (dyj * wjm + dyjm * wjc) + cm * (p0.w_i0 + pp.w_i0);
+  } else {
+    w2 = up * (dyj * wjp + dyjp * wjc) + cp * wjc;
+    w1 = um * (dyj * wjm + dyjm * wjc) + cm * wjc;
+  }
+  T vkp = h * (pp.v_i + pp.v_ijp);
+  T vkc = h * (p0.v_i + p0.v_ijp);
+  T vkm = h * (pm.v_i + pm.v_ijp);
+  v2 = h * (vkp + vkc);
+  v1 = h * (vkm + vkc);
+  sr[4] = h * (yf * (w2 - w1) + DX3 * (v2 - v1));
+  w2 = pp.w_i;
+  w1 = p0.w_i;
+  sr[5] = DX3 * (w2 - w1);
+}
+
 // T = float (mixed mode): STRAIN from the FP32 shadows in FP32; SGSVIS is
 // stored in FP64 and as its FP32 shadow
 template <typename T, bool CB_>
@@ -155,13 +262,16 @@ __global__ __launch_bounds__(256) void k_sgscs(CtxArg<CB_> ca_, int kpt) {
   const double dxyz = 1.0 / s.dx1 * DY[j] / s.dx3;
   const T coef = (T)(smagc * pow(dxyz, 2. / 3.));
   const T nmin = (T)(-s.rinv);
-  const int k0 = tk * 4 * kpt + threadIdx.x / SG_TI;
-#pragma unroll 2
-  for (int m = 0; m < kpt; m++) {
-    const int k = k0 + 4 * m;
-    if (k >= g.nz) break;
+  const int kb = (tk * 4 + threadIdx.x / SG_TI) * kpt;
+  if (kb >= g.nz) return;
+  const int ke = min(kb + kpt, g.nz);
+  const int ip = wrapp(i, g.nx), im = wrapm(i, g.nx);
+  SPlane<T> pm = splane<T>(c, fj.wm0, i, ip, im, j, wrapm(kb, g.nz));
+  SPlane<T> p0 = splane<T>(c, fj.wm0, i, ip, im, j, kb);
+  for (int k = kb; k < ke; k++) {
+    const SPlane<T> pp = splane<T>(c, fj.wm0, i, ip, im, j, wrapp(k, g.nz));
     T sr[6];
-    strain_core<T>(c, fj, i, j, k, sr);
+    strain_win<T>(c, fj, pm, p0, pp, sr);
     const T sumsr = sdots(sr);
     const T abssr = sqrt((T)2 * sumsr);
     const T turbv = coef * abssr;
@@ -169,6 +279,8 @@ __global__ __launch_bounds__(256) void k_sgscs(CtxArg<CB_> ca_, int kpt) {
     const unsigned a = (unsigned)k * (unsigned)g.plane + (unsigned)j * (unsigned)g.nx + (unsigned)i;
     F.sgs[a] = (double)nu;
     if (sizeof(T) == 4) F.sgsf[a] = (float)nu;
+    pm = p0;
+    p0 = pp;
   }
 }
 
@@ -1453,7 +1565,9 @@ static size_t lds(const TileShape& t, int ntiles) { return sizeof(double) * (siz
 void sgscs(const Ctx& c) {
   KScope ks_(c, KID_SGS);
   const int ntx = (c.g.nx + SG_TI - 1) / SG_TI;
-  int kpt = 8;
+  // runs of up to 16 consecutive k (the window loads kpt + 2 planes for
+  // kpt points), shorter when the grid would have fewer than ~2048 blocks
+  int kpt = 16;
   while (kpt > 1 && (long long)ntx * ((c.g.nz + 4 * kpt - 1) / (4 * kpt)) * c.g.n2m < 2048) kpt /= 2;
   const dim3 gr(ntx * ((c.g.nz + 4 * kpt - 1) / (4 * kpt)), c.g.n2m);
   if (c.s.mixed)
