@@ -1323,51 +1323,16 @@ __global__ __launch_bounds__(C3D_NT) void k_wallss_fin(CtxArg<CB_> ca_, int nblk
 //   dvdx = D1x(v), dwdx = D1x(w) (centre -> x-face)   UU12, VV12, WW13, X11
 //   dudz = D1z(u), dvdz = D1z(v) (centre -> z-face)   X33 of :6519-6524, UU31,
 //                                                     VV23, VV32
-// All four kernels run on the register line engine (c3d_lines.h) in the
+// These kernels (and the x-line part inside k_rhs_x) run on the register
+// line engine (c3d_lines.h) in the
 // compute type T of the step's line kernels: FP64, or FP32 in mixed mode,
 // where they read the FP32 shadows of the state and of SGSVIS and keep the
 // intermediates in FP32 (HH is FP32 there too).  Each HH update is a
 // separate rounded store of a term, as the reference's sequence of updates.
 
 // x-lines (j = 1..N2M): nux, dvdx, dwdx, and D(SGS*(DU/DX))/DX of RHS1
-// (:6445-6466): HH1 -= 2 D1x(SGSVIS * D1x(u))
-template <typename T, int SEG, bool CB_>
-__global__ __launch_bounds__(256) void k_sgs_x(CtxArg<CB_> ca_) {
-  const Ctx c = ctx_get(ca_);
-  XSETUP;
-  const Fields F = c.d;
-  const PSolve s6 = c.o.px.s6, s22 = c.o.px.s22;
-  const T dx1 = (T)c.s.dx1;
-  T d[SEG], sv[SEG], pf[SEG];
-  X.pre((const T*)F.ssgs, 0, pf);
-  const T* B = X.put(pf);  // SGSVIS
-  X.pre((const T*)F.sv, 0, pf);
-  SQ sv[q] = XA(B, q);
-  SQ d[q] = (T)4. * (XA(B, q) + XA(B, q - 1));
-  psolve<T, SEG>(d, G, L, l, gs, s6, bf.xf, bf.xb);
-  X.out((T*)F.nux, d);
-  B = X.put(pf);  // v
-  X.pre((const T*)F.sw, 0, pf);
-  SQ d[q] = (T)24. * dx1 * (XA(B, q) - XA(B, q - 1));
-  psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
-  X.out((T*)F.dvdx, d);
-  B = X.put(pf);  // w
-  X.pre((const T*)F.su, 0, pf);
-  SQ d[q] = (T)24. * dx1 * (XA(B, q) - XA(B, q - 1));
-  psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
-  X.out((T*)F.dwdx, d);
-  B = X.put(pf);  // u
-  X.pre((const T*)F.hh[0], 0, pf);
-  SQ d[q] = (T)24. * dx1 * (XA(B, q + 1) - XA(B, q));
-  psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
-  SQ d[q] = sv[q] * d[q];
-  const T lf = xch_left(d[SEG - 1], G, L, l, gs, bf.xl);
-  SQR d[q] = (T)24. * dx1 * (d[q] - prv(d, q, lf));
-  psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
-  B = X.put(pf);  // HH1
-  SQ d[q] = XA(B, q) - (T)2. * d[q];
-  X.out((T*)F.hh[0], d);
-}
+// (:6445-6466) run inside the RHS x-pass (k_rhs_x, c3d_step.hip), which
+// stages u, v and w on the same lines.
 
 // z-lines (j = 1..N2M): nuz, nue, dudz, dvdz, and
 //   D(SGS*(DW/DZ))/DZ of RHS3 (:7919-7940): HH3 -= 2 D1z(SGSVIS * D1z(w))
@@ -1678,10 +1643,6 @@ static void sgs_terms_t(const Ctx& c) {
   } else {                                    \
     constexpr int S_ = 2;                     \
     __VA_ARGS__;                              \
-  }
-  {
-    KScope ks_(c, KID_SGS_X);
-    SEGX_(sx.seg, C3D_LAUNCH(k_sgs_x, (T, S_), dim3(nxb), tx, lx, c));
   }
   {
     KScope ks_(c, KID_SGS_Z);

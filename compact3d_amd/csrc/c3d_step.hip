@@ -38,7 +38,11 @@
 //   HH1   = D1x((Ix u)^2)                                  (:6295-6334)
 //   RUH3  = (1/RE) D2x w                                   (:7483-7498)
 //   WW1   = Ix w                 (stored)                  (:6380-6394, :7718-7732)
-template <typename T, int SEG, bool CB_>
+// With SGS (FILES != 0) the x-line parts of the SGS terms join, from the
+// same staged tiles: nux = Ix(SGSVIS), dvdx = D1x(v), dwdx = D1x(w) (the
+// intermediates of the other SGS line solves) and HH1 -= 2 D1x(SGSVIS *
+// D1x(u)) (:6445-6466), which HH1 takes before it is stored.
+template <typename T, int SEG, bool SGS, bool CB_>
 __global__ __launch_bounds__(256) void k_rhs_x(CtxArg<CB_> ca_) {
   const Ctx c = ctx_get(ca_);
   XSETUP;
@@ -49,17 +53,35 @@ __global__ __launch_bounds__(256) void k_rhs_x(CtxArg<CB_> ca_) {
   const T dx1 = (T)s.dx1, dx1q = (T)s.dx1q, rinv = (T)s.rinv;
   const bool j2 = j >= 2;
   T d[SEG], acc[SEG], vv[SEG], pf[SEG];
-  // v: RUH2, VV1
-  X.pre((const T*)F.sv, 0, pf);
-  const T* B = X.put(pf);
+  T sv[SGS ? SEG : 1], e[SGS ? SEG : 1];  // SGSVIS of the segment; the SGS solves
+  const T* B;
+  if constexpr (SGS) {
+    // SGSVIS: nux = Ix(SGSVIS), centre -> x-face
+    X.pre((const T*)F.ssgs, 0, pf);
+    B = X.put(pf);
+    X.pre((const T*)F.sv, 0, pf);
+    SQ sv[q] = XA(B, q);
+    SQ d[q] = (T)4. * (XA(B, q) + XA(B, q - 1));
+    psolve<T, SEG>(d, G, L, l, gs, s6, bf.xf, bf.xb);
+    X.out((T*)F.nux, d);
+  } else {
+    X.pre((const T*)F.sv, 0, pf);
+  }
+  // v: RUH2, VV1 (and dvdx)
+  B = X.put(pf);
   X.pre((const T*)F.su, -1, pf);  // next tile in flight during the v work
   SQ d[q] = (T)12. * dx1q * (XA(B, q - 1) - (T)2. * XA(B, q) + XA(B, q + 1));
   psolve<T, SEG>(d, G, L, l, gs, s10, bf.xf, bf.xb);
   SQ d[q] = j2 ? rinv * d[q] : (T)0;
   X.out((T*)F.ruh[1], d);
   SQ vv[q] = (T)4. * (XA(B, q - 1) + XA(B, q));
+  if constexpr (SGS) SQ e[q] = (T)24. * dx1 * (XA(B, q) - XA(B, q - 1));
   psolve<T, SEG>(vv, G, L, l, gs, s6, bf.xf, bf.xb);
   X.out((T*)F.vv1, vv);
+  if constexpr (SGS) {
+    psolve<T, SEG>(e, G, L, l, gs, s22, bf.xf, bf.xb);
+    X.out((T*)F.dvdx, e);
+  }
   // UU2 = (DY(J)/2 U(J-1) + DY(J-1)/2 U(J)) / H(J)  (:6966); HH2 = D1x(UU2 VV1)
   const T hj = (T)(1. / M.h[j]), dyj = (T)(M.dy[j] / 2.), dyjm = (T)(M.dy[j - 1] / 2.);
   B = X.put(pf);
@@ -77,11 +99,21 @@ __global__ __launch_bounds__(256) void k_rhs_x(CtxArg<CB_> ca_) {
   SQ acc[q] = (T)12. * dx1q * (XA(B, q - 1) - (T)2. * XA(B, q) + XA(B, q + 1));
   psolve<T, SEG>(acc, G, L, l, gs, s10, bf.xf, bf.xb);
   SQ d[q] = (T)4. * (XA(B, q) + XA(B, q + 1));
+  if constexpr (SGS) SQ e[q] = (T)24. * dx1 * (XA(B, q + 1) - XA(B, q));
   psolve<T, SEG>(d, G, L, l, gs, s6, bf.xf, bf.xb);
   SQ d[q] = d[q] * d[q];
   const T lf = xch_left(d[SEG - 1], G, L, l, gs, bf.xl);
   SQR d[q] = (T)24. * dx1 * (d[q] - prv(d, q, lf));
   psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
+  if constexpr (SGS) {
+    // HH1 -= 2 D1x(SGSVIS * D1x(u)), u face -> centre -> face
+    psolve<T, SEG>(e, G, L, l, gs, s22, bf.xf, bf.xb);
+    SQ e[q] = sv[q] * e[q];
+    const T le = xch_left(e[SEG - 1], G, L, l, gs, bf.xl);
+    SQR e[q] = (T)24. * dx1 * (e[q] - prv(e, q, le));
+    psolve<T, SEG>(e, G, L, l, gs, s22, bf.xf, bf.xb);
+    SQ d[q] = d[q] - (T)2. * e[q];
+  }
   X.out((T*)F.hh[0], d);
   // P: RUH1 = (1/RE) D2x u - D1x P - PRESG
   B = X.put(pf);
@@ -91,15 +123,20 @@ __global__ __launch_bounds__(256) void k_rhs_x(CtxArg<CB_> ca_) {
   const T presg = (T)F.sc[SC_PRESG];
   SQ d[q] = rinv * acc[q] - d[q] - presg;
   X.out((T*)F.ruh[0], d);
-  // w: RUH3, WW1
+  // w: RUH3, WW1 (and dwdx)
   B = X.put(pf);
   SQ d[q] = (T)12. * dx1q * (XA(B, q - 1) - (T)2. * XA(B, q) + XA(B, q + 1));
   psolve<T, SEG>(d, G, L, l, gs, s10, bf.xf, bf.xb);
   SQ d[q] = rinv * d[q];
   X.out((T*)F.ruh[2], d);
   SQ d[q] = (T)4. * (XA(B, q - 1) + XA(B, q));
+  if constexpr (SGS) SQ e[q] = (T)24. * dx1 * (XA(B, q) - XA(B, q - 1));
   psolve<T, SEG>(d, G, L, l, gs, s6, bf.xf, bf.xb);
   X.out((T*)F.ww1, d);
+  if constexpr (SGS) {
+    psolve<T, SEG>(e, G, L, l, gs, s22, bf.xf, bf.xb);
+    X.out((T*)F.dwdx, e);
+  }
 }
 
 // ------------------------------------------------------------ RHS z-pass
@@ -2216,7 +2253,12 @@ static const LShape& zshape(const Ctx& c) {
 template <typename T>
 static void rhs_lines(const Ctx& c) {
   const size_t lx = lds_line(c.o.lx, true, sizeof(T));
-  XSEG(c.o.lx.seg, { KScope ks_(c, KID_RHS_X); C3D_LAUNCH(k_rhs_x, (T, S_), dim3(xblocks(c)), xthreads(c), lx, c); });
+  // with SGS the x-line SGS terms run in the same kernel (k_rhs_x)
+  if (c.s.files) {
+    XSEG(c.o.lx.seg, { KScope ks_(c, KID_RHS_X); C3D_LAUNCH(k_rhs_x, (T, S_, true), dim3(xblocks(c)), xthreads(c), lx, c); });
+  } else {
+    XSEG(c.o.lx.seg, { KScope ks_(c, KID_RHS_X); C3D_LAUNCH(k_rhs_x, (T, S_, false), dim3(xblocks(c)), xthreads(c), lx, c); });
+  }
   ZLAUNCH(T, KID_RHS_Z, k_rhs_z, 1);
   ZLAUNCH(T, KID_RHS_ZW, k_rhs_zw, 1);
   XSEG(c.o.lx.seg, { KScope ks_(c, KID_RHS_X2); C3D_LAUNCH(k_rhs_x2, (T, S_), dim3(xblocks(c)), xthreads(c), lx, c); });
