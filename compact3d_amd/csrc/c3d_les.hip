@@ -1430,7 +1430,9 @@ __global__ __launch_bounds__(512) void k_sgs_z2(CtxArg<CB_> ca_) {
 
 // x-lines: D(SGS*(DU/DY))/DX and D(SGS*(DV/DX))/DX of RHS2 (:7104-7158,
 // :7323-7384, J >= 2), D(SGS*(DW/DX))/DX and D(SGS*(DU/DZ))/DX of RHS3
-// (:7797-7847, :7972-8024)
+// (:7797-7847, :7972-8024), after the RHS x-pass 2's HH3 += D1x(UW)
+// (:7734-7769, k_rhs_x2 without SGS), which joins here: HH3 is read and
+// written once for both.
 template <typename T, int SEG, bool CB_>
 __global__ __launch_bounds__(256) void k_sgs_x2(CtxArg<CB_> ca_) {
   const Ctx c = ctx_get(ca_);
@@ -1442,9 +1444,14 @@ __global__ __launch_bounds__(256) void k_sgs_x2(CtxArg<CB_> ca_) {
   const bool j2 = ok && j >= 2;
   const T hinv = (T)(1. / M.h[j]);
   const T dyjm = (T)(M.dy[j - 1] / 2.), dyj = (T)(M.dy[j] / 2.);
-  T d[SEG], yn[SEG], e[SEG], pf[SEG], rt;
+  T d[SEG], yn[SEG], e[SEG], ux[SEG], pf[SEG], rt;
+  // D1x(UW), face -> centre, as k_rhs_x2
+  X.pre((const T*)F.uw, 0, pf);
+  const T* B = X.put(pf);  // UW
   X.pre((const T*)F.nux, 0, pf);
-  const T* B = X.put(pf);  // nux(J)
+  SQ ux[q] = (T)24. * dx1 * (-XA(B, q) + XA(B, q + 1));
+  psolve<T, SEG>(ux, G, L, l, gs, s22, bf.xf, bf.xb);
+  B = X.put(pf);  // nux(J)
   X.pre((const T*)F.nux, -1, pf);
   SQ yn[q] = dyjm * XA(B, q);
   B = X.put(pf);  // nux(J-1)
@@ -1485,7 +1492,7 @@ __global__ __launch_bounds__(256) void k_sgs_x2(CtxArg<CB_> ca_) {
   SQ e[q] = (T)24. * dx1 * (nxt(e, q, rt) - e[q]);
   psolve<T, SEG>(e, G, L, l, gs, s22, bf.xf, bf.xb);
   B = X.put(pf);  // HH3
-  SQ d[q] = (XA(B, q) - d[q]) - e[q];
+  SQ d[q] = ((XA(B, q) + ux[q]) - d[q]) - e[q];
   X.out((T*)F.hh[2], d);
 }
 

@@ -2261,7 +2261,10 @@ static void rhs_lines(const Ctx& c) {
   }
   ZLAUNCH(T, KID_RHS_Z, k_rhs_z, 1);
   ZLAUNCH(T, KID_RHS_ZW, k_rhs_zw, 1);
-  XSEG(c.o.lx.seg, { KScope ks_(c, KID_RHS_X2); C3D_LAUNCH(k_rhs_x2, (T, S_), dim3(xblocks(c)), xthreads(c), lx, c); });
+  // with SGS, HH3 += D1x(UW) joins the last SGS x-pass (k_sgs_x2)
+  if (!c.s.files) {
+    XSEG(c.o.lx.seg, { KScope ks_(c, KID_RHS_X2); C3D_LAUNCH(k_rhs_x2, (T, S_), dim3(xblocks(c)), xthreads(c), lx, c); });
+  }
 }
 void rhs_lines(const Ctx& c) {
   if (c.s.mixed)
