@@ -152,3 +152,48 @@ def test_batched_ensemble_equals_alone(precision):
         assert dm[q] == d and cf[q] == c
     for ch in ens_ch + alone:
         ch.close()
+
+
+def test_batched_ensemble_config3_grids_equal_alone():
+    """the batched ensemble on the grid set of the 48-member LES benchmark
+    (48x33x48, 64x49x64, 96x65x96, 128x65x128, two members each at
+    different Re_tau from {1000, 2000, 5200}; Smagorinsky with the log law
+    at both walls, mixed precision, dt 5e-4): every member ends bit for bit
+    where it ends stepped alone, with its autotuned shapes"""
+    from compact3d_amd import c3d
+    from compact3d_amd.serve import re_from_re_tau
+    grids = [(49, 33, 49), (65, 49, 65), (97, 65, 97), (129, 65, 129)]
+    specs = []
+    for q, (n1, n2, n3) in enumerate(grids):
+        for r, rt in enumerate((1000, 2000, 5200)[q % 2:q % 2 + 2]):
+            cfg = dict(_cfg(n1, n2, n3, les=True, re=re_from_re_tau(rt)), wmodel=6.0, dt=5e-4, cks=1.8)
+            specs.append((cfg, 10 * q + r))
+    try:
+        for n1, n2, n3 in grids:
+            c3d.autotune(steps=2, **dict(_cfg(n1, n2, n3, les=True), wmodel=6.0, dt=5e-4, cks=1.8),
+                         precision=c3d.MIXED)
+        ens_ch, alone = [], []
+        for cfg, seed in specs:
+            pair = [c3d.Channel(**cfg, precision=c3d.MIXED) for _ in range(2)]
+            U, P = _field(pair[0], cfg, seed)
+            for ch in pair:
+                ch.upload(U, P)
+            ens_ch.append(pair[0])
+            alone.append(pair[1])
+        ens = c3d.Ensemble(ens_ch)
+        ens.advance(3)
+        ens.advance(4)
+        dm, cf = ens.sync()
+        ens.close()
+        for q, ch in enumerate(alone):
+            for n in range(7):
+                d, c = ch.step(n + 1)
+            a, b = ch.download(), ens_ch[q].download()
+            print("member", q, specs[q][0]["n1"] - 1, "divmax ensemble %.2e alone %.2e" % (dm[q], d), flush=True)
+            for key in ("U", "P", "HH", "SGSVIS"):
+                assert np.array_equal(a[key], b[key]), f"member {q}: {key} differs"
+            assert dm[q] == d and cf[q] == c and d <= 1e-9
+        for ch in ens_ch + alone:
+            ch.close()
+    finally:
+        c3d.autotune_reset()
