@@ -41,7 +41,7 @@ ALG_BYTES = {
     # the spectral kernels move (NX/2+1)/NX ~ 1/2 a complex per cell, pois_y
     # also its two per-mode factor tables, stored for kz <= NZ/2 (1/4 of a
     # double pair per cell)
-    "rhs_x": 88, "rhs_z": 112, "rhs_zw": 40, "rhs_x2": 24, "adi_y": 168, "adi_z": 64,
+    "rhs_x": 88, "rhs_z": 144, "rhs_x2": 24, "adi_y": 168, "adi_z": 64,
     "rhsdp_x": 24, "rhsdp_z": 24, "fft_x": 16, "fft_z": 16, "pois_y": 20, "fft_xinv": 16,
     "upd_x": 48, "upd_z": 64,
 }
@@ -55,7 +55,7 @@ ALG_BYTES_ADI_X_PER_STEP = 56
 # DELTA_v and the FP64 DIV of the state and writes the final RDP: RHSDP has
 # no kernel of its own
 ALG_BYTES_MIXED = {
-    "rhs_x": 44, "rhs_z": 56, "rhs_zw": 20, "rhs_x2": 12, "adi_y": 84, "adi_z": 16,
+    "rhs_x": 44, "rhs_z": 72, "rhs_x2": 12, "adi_y": 84, "adi_z": 16,
     "adi_xr": 32, "adi_zr": 36, "fft_x": 16, "fft_z": 16, "pois_y": 20, "fft_xinv": 16,
     # the update: U and V on x-lines (DP, DELTA_u, U, DELTA_v, V in; U, V,
     # their shadows, D1x U out), then W and P on z-lines (DP, DELTA_w, W, P,
@@ -487,7 +487,7 @@ def ensemble_leg(args):
 
 # kernel base names of the step -> bench keys
 KERNEL_KEYS = {"k_rhsy_adiy": "adi_y", "k_rhs_x": "rhs_x", "k_rhs_x2": "rhs_x2", "k_rhs_z": "rhs_z",
-               "k_rhs_zw": "rhs_zw", "k_adi_x": "adi_x", "k_adi_z": "adi_z", "k_rhsdp_x": "rhsdp_x",
+               "k_adi_x": "adi_x", "k_adi_z": "adi_z", "k_rhsdp_x": "rhsdp_x",
                "k_rhsdp_z": "rhsdp_z", "k_fft_x": "fft_x", "k_fft_z": "fft_z", "k_pois_y": "pois_y",
                "k_fft_xinv": "fft_xinv", "k_rfft_x": "fft_x", "k_rfft_z": "fft_z", "k_rfft_xinv": "fft_xinv",
                "k_upd_x": "upd_x", "k_upd_xu": "upd_x", "k_upd_v": "upd_v", "k_upd_z": "upd_z",

@@ -140,11 +140,13 @@ __global__ __launch_bounds__(256) void k_rhs_x(CtxArg<CB_> ca_) {
 }
 
 // ------------------------------------------------------------ RHS z-pass
-// per z-line (i,j), first kernel (u and v):
+// per z-line (i,j), u, v, then w and P:
 //   RUH1 += (1/RE) D2z u                                    (:6170-6185)
 //   UW    = Iz(u) * WW1 (stored);  HH1 += D1z(UW)           (:6360-6433)
 //   RUH2 += (1/RE) D2z v                          (j>=2)   (:6907-6923)
 //   VV3   = Iz v (stored);  HH2 += D1z(VV3*WW2)   (j>=2)   (:7023-7084)
+//   RUH3 += (1/RE) D2z w - D1z(P) - PRESG3                  (:7510-7550)
+//   HH3   = D1z((Iz w)^2)                                   (:7633-7672)
 template <typename T, int SEG, bool CB_>
 __global__ __launch_bounds__(512) void k_rhs_z(CtxArg<CB_> ca_) {
   const Ctx c = ctx_get(ca_);
@@ -200,42 +202,32 @@ __global__ __launch_bounds__(512) void k_rhs_z(CtxArg<CB_> ca_) {
     psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
     if (ok) SQ hh1[ZI(q)] = pr[q] + d[q];
   }
-}
-
-// z-lines, second kernel: RUH3 += (1/RE) D2z w - D1z(P) - PRESG3
-// (:7510-7550) and HH3 = D1z((Iz w)^2) (:7633-7672)
-template <typename T, int SEG, bool CB_>
-__global__ __launch_bounds__(512) void k_rhs_zw(CtxArg<CB_> ca_) {
-  const Ctx c = ctx_get(ca_);
-  ZSETUP;
-  const Scal s = c.s;
-  const Fields F = c.d;
-  const PSolve s6 = PZ.s6, s10 = PZ.s10, s22 = PZ.s22;
-  T* ruh2 = (T*)F.ruh[2];
-  T* hh2 = (T*)F.hh[2];
-  const T dx3 = (T)s.dx3, dx3q = (T)s.dx3q, rinv = (T)s.rinv;
-  T a[SEG], d[SEG], e[SEG], r[SEG], lf, rt;
-  // w, P and the RUH3 operand are all loaded before the first solve
-  zload((const T*)F.sw, b, pl, p0, bm, bp, a, lf, rt);
-  const T* sp = (const T*)F.sp;
-  const T pv0 = sp[bm];
-  SQ d[q] = sp[ZI(q)];
-  SQ r[q] = ruh2[ZI(q)];
-  // w, P: RUH3
-  SQ e[q] = (T)12. * dx3q * (prv(a, q, lf) - (T)2. * a[q] + nxt(a, q, rt));
-  psolve<T, SEG>(e, G, L, l, gs, s10, bf.xf, bf.xb);
-  SQR d[q] = (T)24. * dx3 * (-(q == 0 ? pv0 : d[q > 0 ? q - 1 : 0]) + d[q]);
-  psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
-  const T presg3 = (T)F.sc[SC_PRESG3];
-  if (ok) SQ ruh2[ZI(q)] = r[q] + rinv * e[q] - d[q] - presg3;
-  // HH3 = D1z((Iz w)^2), w face -> centre -> face
-  SQ d[q] = (T)4. * (a[q] + nxt(a, q, rt));
-  psolve<T, SEG>(d, G, L, l, gs, s6, bf.xf, bf.xb);
-  SQ d[q] = d[q] * d[q];
-  lf = xch_left(d[SEG - 1], G, L, l, gs, bf.xl);
-  SQR d[q] = (T)24. * dx3 * (d[q] - prv(d, q, lf));
-  psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
-  if (ok) SQ hh2[ZI(q)] = d[q];
+  // w and P (a phase of its own, so its registers are those the u and v
+  // phases used): RUH3 += (1/RE) D2z w - D1z(P) - PRESG3 (:7510-7550),
+  // HH3 = D1z((Iz w)^2) (:7633-7672); w, P and the RUH3 operand are all
+  // loaded before the first solve
+  {
+    T r[SEG];
+    zload(sw, b, pl, p0, bm, bp, a, lf, rt);
+    const T* sp = (const T*)F.sp;
+    const T pv0 = sp[bm];
+    SQ d[q] = sp[ZI(q)];
+    SQ r[q] = ruh2[ZI(q)];
+    SQ e[q] = (T)12. * dx3q * (prv(a, q, lf) - (T)2. * a[q] + nxt(a, q, rt));
+    psolve<T, SEG>(e, G, L, l, gs, s10, bf.xf, bf.xb);
+    SQR d[q] = (T)24. * dx3 * (-(q == 0 ? pv0 : d[q > 0 ? q - 1 : 0]) + d[q]);
+    psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
+    const T presg3 = (T)F.sc[SC_PRESG3];
+    if (ok) SQ ruh2[ZI(q)] = r[q] + rinv * e[q] - d[q] - presg3;
+    // HH3 = D1z((Iz w)^2), w face -> centre -> face
+    SQ d[q] = (T)4. * (a[q] + nxt(a, q, rt));
+    psolve<T, SEG>(d, G, L, l, gs, s6, bf.xf, bf.xb);
+    SQ d[q] = d[q] * d[q];
+    lf = xch_left(d[SEG - 1], G, L, l, gs, bf.xl);
+    SQR d[q] = (T)24. * dx3 * (d[q] - prv(d, q, lf));
+    psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
+    if (ok) SQ hh2[ZI(q)] = d[q];
+  }
 }
 
 
@@ -2260,7 +2252,6 @@ static void rhs_lines(const Ctx& c) {
     XSEG(c.o.lx.seg, { KScope ks_(c, KID_RHS_X); C3D_LAUNCH(k_rhs_x, (T, S_, false), dim3(xblocks(c)), xthreads(c), lx, c); });
   }
   ZLAUNCH(T, KID_RHS_Z, k_rhs_z, 1);
-  ZLAUNCH(T, KID_RHS_ZW, k_rhs_zw, 1);
   // with SGS, HH3 += D1x(UW) joins the last SGS x-pass (k_sgs_x2)
   if (!c.s.files) {
     XSEG(c.o.lx.seg, { KScope ks_(c, KID_RHS_X2); C3D_LAUNCH(k_rhs_x2, (T, S_), dim3(xblocks(c)), xthreads(c), lx, c); });
