@@ -1338,6 +1338,7 @@ __global__ __launch_bounds__(C3D_NT) void k_wallss_fin(CtxArg<CB_> ca_, int nblk
 //   D(SGS*(DW/DZ))/DZ of RHS3 (:7919-7940): HH3 -= 2 D1z(SGSVIS * D1z(w))
 //   D(SGS*(DU/DZ))/DZ of RHS1 (:6491-6540): HH1 -= D1z(dudz * nue)
 //   D(SGS*(DW/DX))/DZ of RHS1 (:6713-6764): HH1 -= D1z(dwdx * nue)
+// and, at J >= 2, the RHS2 terms D(SGS*(DW/DY))/DZ and D(SGS*(DV/DZ))/DZ
 template <typename T, int SEG, bool CB_>
 __global__ __launch_bounds__(512) void k_sgs_z(CtxArg<CB_> ca_) {
   const Ctx c = ctx_get(ca_);
@@ -1395,37 +1396,37 @@ __global__ __launch_bounds__(512) void k_sgs_z(CtxArg<CB_> ca_) {
   SQ d[q] = (T)24. * dx3 * (nxt(d, q, rt) - d[q]);
   psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
   if (ok) SQ hh0[ZI(q)] = a[q] - d[q];
-}
-
-// z-lines at J >= 2: the two RHS2 terms that need nuz at J-1 and J
-//   D(SGS*(DW/DY))/DZ (:7177-7226), D(SGS*(DV/DZ))/DZ (:7245-7304)
-template <typename T, int SEG, bool CB_>
-__global__ __launch_bounds__(512) void k_sgs_z2(CtxArg<CB_> ca_) {
-  const Ctx c = ctx_get(ca_);
-  ZSETUP_J0(2);
-  const Metrics M = c.m;
-  const Fields F = c.d;
-  const PSolve s22 = PZ.s22;
-  const T dx3 = (T)c.s.dx3;
-  const T hinv = (T)(1. / M.h[j]), dyj = (T)(M.dy[j] / 2.), dyjm = (T)(M.dy[j - 1] / 2.);
-  const T* nuz = (const T*)F.nuz;
-  const T* sw = (const T*)F.sw;
-  T* hh1 = (T*)F.hh[1];
-  T d[SEG], yn[SEG], a[SEG], rt;
-  SQ {
-    yn[q] = hinv * (dyj * nuz[ZI(q) - g.nx] + dyjm * nuz[ZI(q)]);
-    d[q] = yn[q] * (hinv * (sw[ZI(q)] - sw[ZI(q) - g.nx]));
+  if (j < 2) return;  // uniform per block
+  // J >= 2: the two RHS2 terms that need nuz at J-1 and J,
+  //   D(SGS*(DW/DY))/DZ (:7177-7226), D(SGS*(DV/DZ))/DZ (:7245-7304).
+  // nuz(J-1) = Iz(SGSVIS(J-1)) is formed here by the operations the tile at
+  // J-1 uses for its nuz (the same values, bit for bit); nuz(J), dvdz(J)
+  // and w come back from L2 (this block wrote or read them above).
+  {
+    const Metrics M = c.m;
+    const T hinv = (T)(1. / M.h[j]), dyj = (T)(M.dy[j] / 2.), dyjm = (T)(M.dy[j - 1] / 2.);
+    const T* sw = (const T*)F.sw;
+    T* hh1 = (T*)F.hh[1];
+    const unsigned nx = (unsigned)g.nx;
+    T yn[SEG];
+    zload((const T*)F.ssgs, b - nx, pl, p0, bm - nx, bp - nx, sv, lf, rt);
+    SQ yn[q] = (T)4. * (prv(sv, q, lf) + sv[q]);
+    psolve<T, SEG>(yn, G, L, l, gs, s6, bf.xf, bf.xb);
+    SQ {
+      yn[q] = hinv * (dyj * yn[q] + dyjm * nuz[ZI(q)]);
+      d[q] = yn[q] * (hinv * (sw[ZI(q)] - sw[ZI(q) - nx]));
+    }
+    rt = xch_right(d[0], G, L, l, gs, bf.xr);
+    SQ d[q] = (T)24. * dx3 * (nxt(d, q, rt) - d[q]);
+    SQ a[q] = hh1[ZI(q)];
+    psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
+    SQ a[q] = a[q] - d[q];
+    SQ d[q] = yn[q] * dvdz[ZI(q)];
+    rt = xch_right(d[0], G, L, l, gs, bf.xr);
+    SQ d[q] = (T)24. * dx3 * (nxt(d, q, rt) - d[q]);
+    psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
+    if (ok) SQ hh1[ZI(q)] = a[q] - d[q];
   }
-  rt = xch_right(d[0], G, L, l, gs, bf.xr);
-  SQ d[q] = (T)24. * dx3 * (nxt(d, q, rt) - d[q]);
-  SQ a[q] = hh1[ZI(q)];
-  psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
-  SQ a[q] = a[q] - d[q];
-  SQ d[q] = yn[q] * ((const T*)F.dvdz)[ZI(q)];
-  rt = xch_right(d[0], G, L, l, gs, bf.xr);
-  SQ d[q] = (T)24. * dx3 * (nxt(d, q, rt) - d[q]);
-  psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
-  if (ok) SQ hh1[ZI(q)] = a[q] - d[q];
 }
 
 // x-lines: D(SGS*(DU/DY))/DX and D(SGS*(DV/DX))/DX of RHS2 (:7104-7158,
@@ -1654,10 +1655,6 @@ static void sgs_terms_t(const Ctx& c) {
   {
     KScope ks_(c, KID_SGS_Z);
     SEGZ_(sz.seg, C3D_LAUNCH(k_sgs_z, (T, S_), dim3(nit * c.g.n2m), tz, lz, c));
-  }
-  {
-    KScope ks_(c, KID_SGS_Z2);
-    SEGZ_(sz.seg, C3D_LAUNCH(k_sgs_z2, (T, S_), dim3(nit * (c.g.n2m - 1)), tz, lz, c));
   }
   KScope ks_(c, KID_SGS_X2);
   SEGX_(sx.seg, C3D_LAUNCH(k_sgs_x2, (T, S_), dim3(nxb), tx, lx, c));
