@@ -485,6 +485,65 @@ def ensemble_leg(args):
                 divmax_max=float(max(max(dm), max(dm2))), finite=bool(np.all(np.isfinite(dm)) and np.all(np.isfinite(cf))))
 
 
+def les_leg(c3d, args, given=None):
+    """the wall-modelled LES channel (Smagorinsky SGSCS, the log law at both
+    walls, WMODEL=6) of the served stream's Re_tau 1000 requests on the
+    headline grid 256x193x256 (dt 5e-4), FP64 and mixed: ms per step and
+    Mpts.steps/s over args.les_steps steps of c3d_step (DIVMAX and CFL read
+    back every step) and the per-kernel table of a short event-timed pass.
+    Tile shapes from `given` (the --shapes file's les_fp64 / les_mixed sets)
+    or the autotuner."""
+    import torch
+    from compact3d_amd.initfield import perturbed_poiseuille
+    from compact3d_amd.serve import re_from_re_tau
+    cfg = dict(n1=257, n2=193, n3=257, re=re_from_re_tau(1000), alx=2 * math.pi, aly=2.0, alz=math.pi,
+               dt=5e-4, files=2.0, wmodel=6.0, cks=1.8)
+    pts = 256 * 192 * 256
+    out = dict(grid="256x193x256", re_tau=1000, model="Smagorinsky + log-law wall model at both walls (WMODEL=6)")
+    for name, prec in (("fp64", c3d.FP64), ("mixed", c3d.MIXED)):
+        key = "les_" + name
+        if given and key in given:
+            c3d.autotune_set(given[key], **cfg, precision=prec)
+            shapes = {k: given[key].get(k, c3d.SHAPE_DEFAULTS.get(k)) for k in c3d.SHAPE_KEYS}
+        elif not args.no_autotune:
+            t = c3d.autotune(steps=3, **cfg, precision=prec)
+            shapes = {k: t[k] for k in c3d.SHAPE_KEYS}
+        else:
+            shapes = None
+        ch = c3d.Channel(**cfg, precision=prec)
+        m = ch.mesh()
+        U, P = perturbed_poiseuille(cfg["n1"], cfg["n2"], cfg["n3"], m["Y"], m["DX1"], m["DX3"], m["ALY"], seed=2)
+        ch.upload(U, P)
+        del U, P
+        n = 0
+        for _ in range(3):
+            n += 1
+            ch.step(n)
+        kernels = {}
+        if not args.no_timing:
+            ch.timing(True, 4096)
+            for _ in range(5):
+                n += 1
+                ch.step(n)
+            for k, (ms, cnt) in ch.kernel_times().items():
+                kernels[k] = dict(avg_us=round(1000.0 * ms / cnt, 2), launches=cnt)
+            ch.timing(False)
+        torch.cuda.synchronize()
+        dm = 0.0
+        t0 = time.perf_counter()
+        for _ in range(args.les_steps):
+            n += 1
+            d, _ = ch.step(n)
+            dm = max(dm, d) if d == d else float("nan")
+        torch.cuda.synchronize()
+        dt_s = (time.perf_counter() - t0) / args.les_steps
+        ch.close()
+        out[name] = dict(mpts_steps_s=round(pts / dt_s / 1e6, 1), ms_per_step=round(1000.0 * dt_s, 4),
+                         divmax_max=dm, kernels=kernels, shapes=shapes)
+    out["mixed_over_fp64"] = round(out["mixed"]["mpts_steps_s"] / out["fp64"]["mpts_steps_s"], 3)
+    return out
+
+
 # kernel base names of the step -> bench keys
 KERNEL_KEYS = {"k_rhsy_adiy": "adi_y", "k_rhs_x": "rhs_x", "k_rhs_x2": "rhs_x2", "k_rhs_z": "rhs_z",
                "k_adi_x": "adi_x", "k_adi_z": "adi_z", "k_rhsdp_x": "rhsdp_x",
@@ -586,6 +645,8 @@ def main():
                     help="steps of the mixed-vs-FP64 accuracy leg (0 skips it)")
     ap.add_argument("--no-configs", action="store_true", help="skip the 64x65x64 and 192x129x192 legs")
     ap.add_argument("--no-operators", action="store_true", help="skip the isolated operator benches")
+    ap.add_argument("--les-steps", type=int, default=20,
+                    help="steps of the 256x193x256 wall-modelled LES leg per precision (0 skips it)")
     ap.add_argument("--ens-members", type=int, default=48)
     ap.add_argument("--ens-steps", type=int, default=200)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -628,6 +689,7 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     tune = {}
+    given = None
     if args.shapes:
         with open(args.shapes) as f:
             given = json.load(f)
@@ -674,6 +736,7 @@ def main():
     roof64 = roofline(r64["kt"], ALG_BYTES, pts, args.grid, "fp64")
     step_frac = 112.0 * pts / r64["dt_s"] / (HBM_PEAK_GBS * 1e9)
 
+    les = les_leg(c3d, args, given) if args.les_steps > 0 else None
     served = None if args.no_serve else served_leg(args)
     ensemble = None if args.no_ensemble else ensemble_leg(args)
     accuracy = None
@@ -736,6 +799,7 @@ def main():
         "configs": configs,
         "operators": operators,
         "autotune": tune or None,
+        "les": les,
         "served": served,
         "ensemble": ensemble,
         "cfl_after": cflm * cfg["dt"],
