@@ -1105,30 +1105,26 @@ __global__ __launch_bounds__(512) void k_adi_zr(CtxArg<CB_> ca_) {
 }
 
 // ------------------------------------------------------------------ RHSDP
-// x-lines: RDP = (FJUP*UH2(J+1) - FJUM*UH2(J))/DY/DT + D1x(UH1)/DT  (:8138-8186)
-// (the CBC term is zero: UBC(.,.,2,.) = 0, :978, :986).  Arithmetic and RDP
-// in FP64; UH stored as TS.  Mixed mode (TS = float): UH holds DELTA and
-// RDP = DIV(U)/DT + DIV(DELTA)/DT, with DIV(U) of the current state kept in
-// FP64 by the previous update; k_rhsdp_z adds DIV(U)/DT (in the order
-// ((x and y terms) + DIV(U)/DT) + z term), so this kernel stages only UH.
-template <typename T, int SEG, typename TS, bool CB_>
+// FP64 mode (mixed mode forms RDP in its last ADI sweeps, k_adi_xr and
+// k_adi_zr).  x-lines: RDP = (FJUP*UH2(J+1) - FJUM*UH2(J))/DY/DT +
+// D1x(UH1)/DT (:8138-8186; the CBC term is zero: UBC(.,.,2,.) = 0, :978,
+// :986).
+template <typename T, int SEG, bool CB_>
 __global__ __launch_bounds__(256) void k_rhsdp_x(CtxArg<CB_> ca_) {
   const Ctx c = ctx_get(ca_);
   XSETUP;
-  constexpr bool MIX = sizeof(TS) == 4;
   const Fields F = c.d;
   const Metrics M = c.m;
   const T dx1 = (T)c.s.dx1, dt = (T)c.s.dt;
-  T d[SEG], e[SEG];
-  TS pf[SEG];
-  X.pre_raw((const TS*)F.uh[0], 0, pf);
+  T d[SEG], e[SEG], pf[SEG];
+  X.pre_raw((const T*)F.uh[0], 0, pf);
   const T* B = X.put(pf);
-  X.pre_raw((const TS*)F.uh[1], 1, pf);
+  X.pre_raw((const T*)F.uh[1], 1, pf);
   SQ d[q] = (T)24. * dx1 * (XA(B, q + 1) - XA(B, q));
   psolve<T, SEG>(d, G, L, l, gs, c.o.px.s22, bf.xf, bf.xb);
   const T fjpa = (T)M.fjpa[j], fjmu = (T)M.fjmu[j];
   B = X.put(pf);
-  X.pre_raw((const TS*)F.uh[1], 0, pf);
+  X.pre_raw((const T*)F.uh[1], 0, pf);
   SQ e[q] = fjpa * XA(B, q);
   B = X.put(pf);
   // reciprocals instead of three divisions per point
@@ -1138,25 +1134,18 @@ __global__ __launch_bounds__(256) void k_rhsdp_x(CtxArg<CB_> ca_) {
 }
 
 // z-lines: RDP += D1z(UH3)/DT (:8196-8211)
-template <typename T, int SEG, typename TS, bool CB_>
+template <typename T, int SEG, bool CB_>
 __global__ __launch_bounds__(512) void k_rhsdp_z(CtxArg<CB_> ca_) {
   const Ctx c = ctx_get(ca_);
   ZSETUP;
   const T dx3 = (T)c.s.dx3, dt = (T)c.s.dt;
   T a[SEG], d[SEG], lf, rt;
-  zload((const TS*)c.d.uh[2], b, pl, p0, bm, bp, a, lf, rt);
+  zload((const T*)c.d.uh[2], b, pl, p0, bm, bp, a, lf, rt);
   SQ d[q] = (T)24. * dx3 * (nxt(a, q, rt) - a[q]);
   psolve<T, SEG>(d, G, L, l, gs, PZ.s22, bf.xf, bf.xb);
   double* rdp = c.d.rdp;
-  if (sizeof(TS) == 4) {
-    // mixed mode: DIV(U)/DT of the current state (see k_rhsdp_x)
-    const double* dv = c.d.div;
-    const T rdt = (T)1. / dt;
-    if (ok) SQ rdp[ZI(q)] = (rdp[ZI(q)] + dv[ZI(q)] * rdt) + d[q] * rdt;
-  } else {
-    const T rdt = (T)1. / dt;
-    if (ok) SQ rdp[ZI(q)] = rdp[ZI(q)] + d[q] * rdt;
-  }
+  const T rdt = (T)1. / dt;
+  if (ok) SQ rdp[ZI(q)] = rdp[ZI(q)] + d[q] * rdt;
 }
 
 // ------------------------------------------------------------------ GETDP
@@ -2350,16 +2339,13 @@ void getuh_fused(const Ctx& c) {
     getuh_t<double>(c, true);
 }
 
-// FP64 arithmetic; UH (DELTA) stored as TS
-template <typename TS>
-static void rhsdp_t(const Ctx& c) {
-  const size_t lx = lds_line(c.o.lx, true, sizeof(double)), lz = lds_line(c.o.lz, false, sizeof(double));
-  XSEG(c.o.lx.seg, { KScope ks_(c, KID_RHSDP_X); C3D_LAUNCH(k_rhsdp_x, (double, S_, TS), dim3(xblocks(c)), xthreads(c), lx, c); });
-  ZSEG(c.o.lz.seg, { KScope ks_(c, KID_RHSDP_Z); C3D_LAUNCH(k_rhsdp_z, (double, S_, TS), dim3(zblocks(c)), zthreads(c), lz, c); });
-}
-// mixed mode: the last ADI sweeps formed RDP already (k_adi_xr, k_adi_zr)
+// FP64 mode; mixed mode: the last ADI sweeps formed RDP already (k_adi_xr,
+// k_adi_zr)
 void rhsdp(const Ctx& c) {
-  if (!c.s.mixed) rhsdp_t<double>(c);
+  if (c.s.mixed) return;
+  const size_t lx = lds_line(c.o.lx, true, sizeof(double)), lz = lds_line(c.o.lz, false, sizeof(double));
+  XSEG(c.o.lx.seg, { KScope ks_(c, KID_RHSDP_X); C3D_LAUNCH(k_rhsdp_x, (double, S_), dim3(xblocks(c)), xthreads(c), lx, c); });
+  ZSEG(c.o.lz.seg, { KScope ks_(c, KID_RHSDP_Z); C3D_LAUNCH(k_rhsdp_z, (double, S_), dim3(zblocks(c)), zthreads(c), lz, c); });
 }
 
 // GETDP in FP64 in both modes: rounding RDP's transform or the solution to
