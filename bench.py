@@ -222,7 +222,9 @@ def configs_leg(c3d, args):
     """BASELINE's two smaller configurations on the GPU (FP64 and mixed) and
     the oracle on the host cores: 64x65x64 (Re_tau 180, tanh 1.5, dt 1e-3,
     200 steps; the oracle runs all of them) and 192x129x192 (500 steps; the
-    oracle a bounded sample)"""
+    oracle a bounded sample).  The GPU steps run as c3d_step calls and again
+    as captured step graphs replayed one step per call (graph_*), both with
+    DIVMAX and CFL read back every step."""
     from compact3d_amd.initfield import perturbed_poiseuille
     out = {}
     for name, grid, steps, cpu_s, seed in (("c0", (64, 65, 64), 200, 60.0, 0), ("c1", (192, 129, 192), 500, 10.0, 1)):
@@ -246,9 +248,22 @@ def configs_leg(c3d, args):
                 d, _ = ch.step(n)
                 dm = max(dm, d)
             el = time.perf_counter() - t0
+            # the same steps replayed as the captured step graph, one step
+            # per call with its DIVMAX and CFL read back (c3d_advance(1) +
+            # c3d_sync): one launch per step instead of the step's kernels
+            ch.advance(2, use_graph=True)
+            ch.sync()
+            t0 = time.perf_counter()
+            dmg = 0.0
+            for _ in range(steps):
+                ch.advance(1, use_graph=True)
+                d, _ = ch.sync()
+                dmg = max(dmg, d)
+            elg = time.perf_counter() - t0
             ch.close()
             rec[pname] = dict(mpts_steps_s=round(pts * steps / el / 1e6, 1), ms_per_step=round(1000 * el / steps, 4),
-                              divmax_max=dm)
+                              divmax_max=dm, graph_mpts_steps_s=round(pts * steps / elg / 1e6, 1),
+                              graph_ms_per_step=round(1000 * elg / steps, 4), graph_divmax_max=dmg)
         if not args.no_cpu:
             st, el, thr, _, _ = oracle_run(cfg, seed, cpu_s, steps)
             rec["cpu_oracle"] = dict(mpts_steps_s=round(pts * st / el / 1e6, 2), steps=st, seconds=round(el, 1),
