@@ -144,6 +144,15 @@ static PSolve make_psolve(double a, double b, int seg) {
 // of 512 points (G = 64) take 4 lines per 256-thread tile
 static int xline_step(int seg, int G) { return seg * G <= 256 ? seg : std::max(1, seg / 2); }
 
+// row stride of the stored spectrum (complex values): the smallest odd
+// multiple of 16 (256 bytes) that holds n columns, so that rows written or
+// read together do not sit a power of two apart
+static int spec_pitch(int n) {
+  int p = (n + 15) / 16;
+  if (p % 2 == 0) p++;
+  return 16 * p;
+}
+
 // line-engine tile: seg = 8 when it divides n (else 2, the other compiled
 // length), G = n/seg segments, L lines per tile.  x-lines: L*G <= 256 with L
 // a multiple of xline_step, 2 LDS row tiles of n*(L+1) plus the exchange
@@ -516,8 +525,8 @@ long long c3d_footprint_bytes(const c3d_config* cfg) {
   if (cfg->files == 1.0) bytes += (2 * n2m * ((nx + 15) / 16) * ((nz + 15) / 16) + n2m + 3) * 8;
   if (cfg->files == 3.0) bytes += (10 * n2m * ((nx + 15) / 16) * ((nz + 15) / 16) + 2 * (n2m + 3)) * 8;
   if (cfg->wmodel == 2.0) bytes += (long long)IW_NPL * nx * nz * 8;
-  const long long nxhp = (nx / 2 + 1 + 7) / 8 * 8;
-  bytes += nxhp * nz * n2m * 16 + nxhp * (nz / 2 + 1) * n2m * 16;  // spectrum + TDMAP factors
+  const long long nxhp = (nx / 2 + 1 + 15) / 16 * 16, nxsp = spec_pitch(nx / 2);
+  bytes += nxsp * nz * n2m * 16 + nxhp * (nz / 2 + 1) * n2m * 16;  // spectrum + TDMAP factors
   bytes += 4 * std::max(n2m * nz, nx * n2m) * 8 + 128 * 8;  // flow-rate partial sums, max slots
   return bytes + 4 * nx * nz * 8 + (1 << 20);
 }
@@ -610,6 +619,8 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   g.nj = cf.n2 + 1;
   g.nxh = g.nx / 2 + 1;
   g.nxhp = (g.nxh + 15) / 16 * 16;
+  g.nxs = g.nx / 2;
+  g.nxsp = spec_pitch(g.nxs);
   g.plane = (long long)g.nj * g.nx;
   g.fsz = g.plane * g.nz;
   if (g.nx % 2 || g.nz % 2) return bad("N1-1 and N3-1 must be even");
@@ -791,7 +802,7 @@ extern "C" int c3d_create(const c3d_config* cfg, c3d_handle** out) {
   if (cf.wmodel == 4.0 && dalloc(h, &F.tauwe, 4 * n2d)) return bad(h->err);
   F.iwm = nullptr;
   if (cf.wmodel == 2.0 && dalloc(h, &F.iwm, (size_t)IW_NPL * n2d)) return bad(h->err);
-  if (dalloc(h, &F.spec, (size_t)g.nxhp * g.nz * g.n2m)) return bad(h->err);
+  if (dalloc(h, &F.spec, (size_t)g.nxsp * g.nz * g.n2m)) return bad(h->err);
   F.sgsf = nullptr;
   F.ssgs = F.sgs;
   if (cf.files != 0.0) {

@@ -14,8 +14,12 @@
 #endif
 
 struct Geo {
-  int nx, nz, n2, n2m, nj, nxh;  // nxh = NX/2+1 (half spectrum)
-  int nxhp;                      // spectrum row stride: nxh rounded up to 16 (128-byte rows of the complex spectrum and of the FP64 factor tables)
+  int nx, nz, n2, n2m, nj, nxh;  // nxh = NX/2+1 (the x-modes kx = 0..NX/2)
+  int nxhp;                      // row stride of the Poisson factor tables: nxh rounded up to 16
+  // the stored half spectrum: nxs = NX/2 columns, column 0 packing the two
+  // real x-modes X(0) + i X(NX/2); row stride nxsp >= nxs, an odd multiple
+  // of 16 complex values (256 bytes)
+  int nxs, nxsp;
   long long plane;               // NJ*NX
   long long fsz;                 // NZ*NJ*NX
 };

@@ -129,7 +129,7 @@ struct Fields {
   double *tauw, *taub;
   double *tauwe;          // WMODEL = 4: the cell-centre stresses TAUWE(I,K,1:4)
   double *iwm;            // WMODEL = 2: IW_NPL state planes
-  double2 *spec;          // Poisson spectrum [jm][k][kx], NXH*NZ*N2M
+  double2 *spec;          // Poisson spectrum [jm][kz][kx], rows of nxsp: kx < NX/2, column 0 = X(0) + i X(NX/2)
   double *pib, *pgt;      // Poisson TDMAP factors per mode: 1/BET(J), GAM(J+1) [jm][kz][kx]
   double *sc;             // device scalars (SC_*)
   double *red;            // reduction partials

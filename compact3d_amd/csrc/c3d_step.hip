@@ -1152,8 +1152,11 @@ __global__ __launch_bounds__(512) void k_rhsdp_z(CtxArg<CB_> ca_) {
 // The plane DFT of :9325-9346 is factored as a real x-DFT (two rows packed
 // into one complex transform: z = a + i b, A[k] = (Z[k] + conj Z[n-k])/2,
 // B[k] = (Z[k] - conj Z[n-k])/(2i)), keeping kx = 0..NX/2, then complex
-// z-DFTs of that half spectrum.  Spectrum layout [jm][kz][kx], rows of
-// NXHP >= NX/2+1 (padded to 128 bytes).
+// z-DFTs of that half spectrum.  The x-modes 0 and NX/2 of a real row are
+// real, so they share one stored column: X(0) + i X(NX/2) at kx = 0, and the
+// spectrum has NX/2 columns (the tiles of the z-DFT and the Poisson solve
+// divide them evenly).  Spectrum layout [jm][kz][kx], rows of NXSP >= NX/2
+// (an odd multiple of 256 bytes).
 template <typename T, int R>
 __device__ __forceinline__ void dft_r(cplx_t<T> (&v)[R], int sign) {
   if (R == 2) {
@@ -1414,14 +1417,14 @@ __global__ __launch_bounds__(256) void k_rfft_z(CtxArg<CB_> ca_, int sign) {
     const zc* twg = (const zc*)(sign < 0 ? c.f.twzf : c.f.twzb);
     for (int e = threadIdx.x; e < N; e += 256) TW[e] = twg[e];
   }
-  const int nit = (g.nxh + L - 1) / L;
+  const int nit = (g.nxs + L - 1) / L;
   const int it = blockIdx.x % nit;
   const int jm = blockIdx.x / nit;
   const int l = threadIdx.x % L, a = threadIdx.x / L;
   const int kx = it * L + l;
-  const bool ok = kx < g.nxh;
-  zc* S = (zc*)c.d.spec + (long long)jm * g.nz * g.nxhp + (ok ? kx : 0);
-  const unsigned rs = (unsigned)g.nxhp;
+  const bool ok = kx < g.nxs;
+  zc* S = (zc*)c.d.spec + (long long)jm * g.nz * g.nxsp + (ok ? kx : 0);
+  const unsigned rs = (unsigned)g.nxsp;
   zc v[N2];
 #pragma unroll
   for (int b = 0; b < N2; b++) v[b] = S[(unsigned)(a + N1 * b) * rs];
@@ -1477,17 +1480,19 @@ __global__ __launch_bounds__(256) void k_rfft_x(CtxArg<CB_> ca_) {
 #pragma unroll
     for (int k1 = 0; k1 < N1; k1++) A[rslot<N, true>(a + N1 * m + N2 * k1, l)] = w[m][k1];
   __syncthreads();
-  const int nxh = g.nxh;
+  const int nxs = g.nxs;
   zc* S = (zc*)c.d.spec;
-  for (int e = threadIdx.x; e < L * nxh; e += 256) {
-    const int ll = e / nxh, kk = e - ll * nxh;
+  for (int e = threadIdx.x; e < L * nxs; e += 256) {
+    const int ll = e / nxs, kk = e - ll * nxs;
     const int ea = ln0 + ll, eb = ln0 + L + ll;
     const zc z1 = A[rslot<N, true>(kk, ll)];
-    const zc z2 = A[rslot<N, true>(kk == 0 ? 0 : N - kk, ll)];
-    const zc va = {(T)0.5 * (z1.x + z2.x), (T)0.5 * (z1.y - z2.y)};
-    const zc vb = {(T)0.5 * (z1.y + z2.y), (T)-0.5 * (z1.x - z2.x)};
-    if (ea < nlines) S[((long long)(ea % g.n2m) * g.nz + ea / g.n2m) * g.nxhp + kk] = va;
-    if (eb < nlines) S[((long long)(eb % g.n2m) * g.nz + eb / g.n2m) * g.nxhp + kk] = vb;
+    const zc z2 = A[rslot<N, true>(kk == 0 ? N / 2 : N - kk, ll)];
+    // column 0: the real modes 0 and N/2 of each row, packed (Z(0) and Z(N/2)
+    // hold X_a + i X_b there, all four real)
+    const zc va = kk == 0 ? zc{z1.x, z2.x} : zc{(T)0.5 * (z1.x + z2.x), (T)0.5 * (z1.y - z2.y)};
+    const zc vb = kk == 0 ? zc{z1.y, z2.y} : zc{(T)0.5 * (z1.y + z2.y), (T)-0.5 * (z1.x - z2.x)};
+    if (ea < nlines) S[((long long)(ea % g.n2m) * g.nz + ea / g.n2m) * g.nxsp + kk] = va;
+    if (eb < nlines) S[((long long)(eb % g.n2m) * g.nz + eb / g.n2m) * g.nxsp + kk] = vb;
   }
 }
 
@@ -1510,22 +1515,20 @@ __global__ __launch_bounds__(256) void k_rfft_xinv(CtxArg<CB_> ca_) {
   const int a = threadIdx.x % N1, l = threadIdx.x / N1;
   const int la = ln0 + l, lb = ln0 + L + l;
   const zc* S = (const zc*)c.d.spec;
-  const int nxh = g.nxh;
-  const long long sa = la < nlines ? ((long long)(la % g.n2m) * g.nz + la / g.n2m) * g.nxhp : -1;
-  const long long sb = lb < nlines ? ((long long)(lb % g.n2m) * g.nz + lb / g.n2m) * g.nxhp : -1;
+  const long long sa = la < nlines ? ((long long)(la % g.n2m) * g.nz + la / g.n2m) * g.nxsp : -1;
+  const long long sb = lb < nlines ? ((long long)(lb % g.n2m) * g.nz + lb / g.n2m) * g.nxsp : -1;
   zc v[N2];
 #pragma unroll
   for (int b = 0; b < N2; b++) {
     const int kx = a + N1 * b;
-    const bool lo = kx < nxh;
-    const int kk = lo ? kx : N - kx;
+    const bool rl = kx == 0 || 2 * kx == N;  // the real modes, packed in column 0
+    const int kk = rl ? 0 : (2 * kx < N ? kx : N - kx);
     zc xa = sa >= 0 ? S[sa + kk] : zc{(T)0, (T)0};
     zc xb = sb >= 0 ? S[sb + kk] : zc{(T)0, (T)0};
-    if (kx == 0 || 2 * kx == N) {
-      xa.y = (T)0;
-      xb.y = (T)0;
-    }
-    if (!lo) {
+    if (rl) {
+      xa = {kx == 0 ? xa.x : xa.y, (T)0};
+      xb = {kx == 0 ? xb.x : xb.y, (T)0};
+    } else if (2 * kx > N) {
       xa.y = -xa.y;
       xb.y = -xb.y;
     }
@@ -1579,19 +1582,20 @@ __global__ __launch_bounds__(NT) void k_fft_x(CtxArg<CB_> ca_) {
   }
   __syncthreads();
   fft_lines<T, NE, NT>(A, n, NL, LS, c.f.codex, c.f.nradx, TW, -1);
-  const int nxh = g.nxh;
+  const int nxs = g.nxs;
   zc* S = (zc*)c.d.spec;
 #pragma unroll 2
-  for (int e = threadIdx.x; e < NL * nxh; e += NT) {
-    const int l = e / nxh, kx = e - l * nxh;
+  for (int e = threadIdx.x; e < NL * nxs; e += NT) {
+    const int l = e / nxs, kx = e - l * nxs;
     const int la = ln0 + l, lb = ln0 + NL + l;
     const zc z1 = A[kx * LS + l];
-    const zc z2 = A[(kx == 0 ? 0 : n - kx) * LS + l];
-    // A = (z1 + conj z2)/2, B = (z1 - conj z2)/(2i)
-    const zc va = {(T)0.5 * (z1.x + z2.x), (T)0.5 * (z1.y - z2.y)};
-    const zc vb = {(T)0.5 * (z1.y + z2.y), (T)-0.5 * (z1.x - z2.x)};
-    if (la < nlines) S[((long long)(la % g.n2m) * g.nz + la / g.n2m) * g.nxhp + kx] = va;
-    if (lb < nlines) S[((long long)(lb % g.n2m) * g.nz + lb / g.n2m) * g.nxhp + kx] = vb;
+    const zc z2 = A[(kx == 0 ? n / 2 : n - kx) * LS + l];
+    // A = (z1 + conj z2)/2, B = (z1 - conj z2)/(2i); column 0 packs the real
+    // modes 0 and n/2 of each row: A(0) + i A(n/2) and B(0) + i B(n/2)
+    const zc va = kx == 0 ? zc{z1.x, z2.x} : zc{(T)0.5 * (z1.x + z2.x), (T)0.5 * (z1.y - z2.y)};
+    const zc vb = kx == 0 ? zc{z1.y, z2.y} : zc{(T)0.5 * (z1.y + z2.y), (T)-0.5 * (z1.x - z2.x)};
+    if (la < nlines) S[((long long)(la % g.n2m) * g.nz + la / g.n2m) * g.nxsp + kx] = va;
+    if (lb < nlines) S[((long long)(lb % g.n2m) * g.nz + lb / g.n2m) * g.nxsp + kx] = vb;
   }
 }
 
@@ -1611,16 +1615,16 @@ __global__ __launch_bounds__(NT) void k_fft_z(CtxArg<CB_> ca_, int sign) {
     const zc* twg = (const zc*)(sign < 0 ? c.f.twzf : c.f.twzb);
     for (int e = threadIdx.x; e < n; e += NT) TW[e] = twg[e];
   }
-  const int nit = (g.nxh + L - 1) / L;
+  const int nit = (g.nxs + L - 1) / L;
   const int it = blockIdx.x % nit;
   const int jm = blockIdx.x / nit;
-  zc* S = (zc*)spec + (long long)jm * g.nz * g.nxhp;
+  zc* S = (zc*)spec + (long long)jm * g.nz * g.nxsp;
   const int tot = n * L;
 #pragma unroll 4
   for (int e = threadIdx.x; e < tot; e += NT) {
     const int l = e % L, p = e / L;
     const int kx = it * L + l;
-    A[p * LS + l] = kx < g.nxh ? S[(long long)p * g.nxhp + kx] : zc{(T)0, (T)0};
+    A[p * LS + l] = kx < g.nxs ? S[(long long)p * g.nxsp + kx] : zc{(T)0, (T)0};
   }
   __syncthreads();
   fft_lines<T, NE, NT>(A, n, L, LS, c.f.codez, c.f.nradz, TW, sign);
@@ -1628,7 +1632,7 @@ __global__ __launch_bounds__(NT) void k_fft_z(CtxArg<CB_> ca_, int sign) {
   for (int e = threadIdx.x; e < tot; e += NT) {
     const int l = e % L, p = e / L;
     const int kx = it * L + l;
-    if (kx < g.nxh) S[(long long)p * g.nxhp + kx] = A[p * LS + l];
+    if (kx < g.nxs) S[(long long)p * g.nxsp + kx] = A[p * LS + l];
   }
 }
 
@@ -1637,19 +1641,38 @@ __global__ __launch_bounds__(NT) void k_fft_z(CtxArg<CB_> ca_, int sign) {
 // identity with zero RHS; imaginary parts of the self-conjugate modes
 // (kx in {0, NX/2}) x (kz in {0, NZ/2}) are zero.  The elimination factors
 // depend on the mode only: IBET(J) = 1/BET(J) and GAM(J+1) are tabulated at
-// set-up (k_pois_table) for kz = 0..NZ/2, since AK3 is even in kz and the
-// factors of kz and NZ-kz are the same numbers.  A block solves L/2 modes kx
-// at the two kz of a pair (kz, NZ-kz) (pair 0: kz = 0 and NZ/2), so the lanes
-// of the two halves read each factor row in the same load; thread (l, c)
-// solves chunk c of column l by the partitioned recurrences.
+// set-up (k_pois_table) for kx = 0..NX/2 and kz = 0..NZ/2, since AK3 is even
+// in kz and the factors of kz and NZ-kz are the same numbers.  A block solves
+// L/2 stored columns kx at the two kz of a pair (kz, NZ-kz) (pair 0: kz = 0
+// and NZ/2), so the lanes of the two halves read each factor row in the same
+// load; thread (l, c) solves chunk c of column l by the partitioned
+// recurrences.
+// The stored column 0 holds Z = A + iB: A and B are the z-spectra of the
+// real x-modes 0 and NX/2, so A(-kz) = conj A(kz), likewise B.  In the tile
+// of column 0 the two threads of a chunk (halves 0 and 1) swap their values:
+// for pairs pz > 0 half 0 solves A(pz) = (Z(pz) + conj Z(NZ-pz))/2 and half 1
+// B(NZ-pz) = (Z(NZ-pz) - conj Z(pz))/(2i) with the kx = NX/2 factors; they
+// swap again to store Z(pz) = A(pz) + i conj B(NZ-pz) and Z(NZ-pz) =
+// conj A(pz) + i B(NZ-pz).  In pair 0 the four modes are real (their
+// imaginary parts are the zeros above): the tile of column 0 solves A(0) and
+// A(NZ/2) in the real parts, and B(0) and B(NZ/2) are solved with the kx =
+// NX/2 factors either (sep = 1) by block 0 of the launch, an extra block that
+// streams next to nothing and is dispatched first, which stores the
+// imaginary parts, or (sep = 0) by that tile in a second pass over the
+// imaginary parts, with the kx = NX/2 multipliers staged in LDS (dynamic,
+// 4 N2M TO) and multipliers 0 in its other threads, which leaves their
+// values unchanged.  The extra block lengthens the launch by its own time
+// when the tiles fill whole rounds of resident blocks, the second pass by
+// its time in any case; the host picks.
 // Spectrum in TI, recurrences and result in TO (FP64 in both modes); chunks
-// of at most CH points.
+// of at most CH points.  Grid: sep + the tiles.
 template <typename TI, typename TO, int CH, int NT, bool CB_>
-__global__ __launch_bounds__(NT) void k_pois_y(CtxArg<CB_> ca_) {
+__global__ __launch_bounds__(NT, 4) void k_pois_y(CtxArg<CB_> ca_, int sep) {
   const Ctx c = ctx_get(ca_);
   const void* spec_in = (const void*)c.d.spec;
   void* spec_out = (void*)c.d.spec;
   __shared__ TO xs[6][NT];
+  extern __shared__ __align__(16) unsigned char smem[];
   typedef cplx_t<TI> zi;
   typedef cplx_t<TO> zo;
   const Geo g = c.g;
@@ -1662,49 +1685,142 @@ __global__ __launch_bounds__(NT) void k_pois_y(CtxArg<CB_> ca_) {
   const int ch = (g.n2m + C - 1) / C;
   y.j0 = y.c * ch;  // jm index (J - 1)
   y.len = max(0, min(ch, g.n2m - y.j0));
-  // XCD-aware tile order: the FP64 factor rows of 8 modes are half a
-  // 128-byte line, shared with the neighbouring tile, which then runs on the
-  // same L2
-  const int nit = (g.nxh + LH - 1) / LH;
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  const int it = tile % nit;
-  const int pz = tile / nit;  // the pair (pz, NZ - pz); pair 0 is (0, NZ/2)
+  // block 0 solves B(0) and B(NZ/2) in column 0 at kz = 0 and NZ/2 (threads
+  // l = 0 and LH, the kx = NX/2 factors; its other threads load the same
+  // addresses and store nothing); blocks 1.. are the tiles, in XCD-aware
+  // order: the FP64 factor rows of 8 modes are half a 128-byte line, shared
+  // with the neighbouring tile, which then runs on the same L2
+  const bool nyq = sep && blockIdx.x == 0;
+  const int nit = (g.nxs + LH - 1) / LH;
+  const int tile = nyq ? 0 : xcd_tile(blockIdx.x - sep, gridDim.x - sep);
+  const int it = nyq ? -1 : tile % nit;
+  const int pz = nyq ? 0 : tile / nit;  // the pair (pz, NZ - pz); pair 0 is (0, NZ/2)
   const int half = l >= LH;
   const int kz = half ? (pz == 0 ? g.nz / 2 : g.nz - pz) : pz;
   const int kk = (half && pz == 0) ? g.nz / 2 : pz;  // factor row
-  const int kx = it * LH + l - half * LH;
-  const bool ok = kx < g.nxh;
-  const long long jstr = (long long)g.nz * g.nxhp;
+  const int kx = nyq ? ((l == 0 || l == LH) ? 0 : g.nxs) : it * LH + l - half * LH;
+  const bool ok = kx < g.nxs;
+  const bool c0 = it == 0;          // the tile of column 0 (block-uniform)
+  const bool pk = c0 && kx == 0;    // a thread of column 0
+  const bool dual = !sep && c0 && pz == 0;  // the second pass (block-uniform)
+  // factor column: kx, or NX/2 in block 0 and for the B half of column 0 in
+  // pairs pz > 0
+  const int fx = (nyq || (pk && half && pz != 0)) ? g.nxh - 1 : (ok ? kx : 0);
+  const long long jstr = (long long)g.nz * g.nxsp;
   const long long tstr = (long long)(g.nz / 2 + 1) * g.nxhp;
-  const long long b = (long long)kz * g.nxhp + (ok ? kx : 0);
-  const long long bt = (long long)kk * g.nxhp + (ok ? kx : 0);
+  const long long b = (long long)kz * g.nxsp + (ok ? kx : 0);
+  const long long bt = (long long)kk * g.nxhp + fx;
   const zi* Si = (const zi*)spec_in;
   zo* So = (zo*)spec_out;
   const double* IB = c.d.pib;
   const double* GT = c.d.pgt;
   const double* pmj = c.m.pmj;
-  const bool selfconj = (kx == 0 || 2 * kx == g.nx) && (kz == 0 || 2 * kz == g.nz);
-  const bool pin = (kx == 0 && kz == 0);
+  const bool pin = pk && kz == 0;  // the real part of mode (0,0)
+  // the second pass: the kx = NX/2 multipliers of the factor rows 0 and NZ/2
+  // (nm and ng, 2 x N2M each, row r = half)
+  TO* nm = (TO*)smem;
+  TO* ng = nm + 2 * g.n2m;
+  if (dual) {
+    for (int e = t; e < 2 * g.n2m; e += NT) {
+      const int r = e / g.n2m, jm = e - r * g.n2m;
+      const long long a = jm * tstr + (long long)(r ? g.nz / 2 : 0) * g.nxhp + (g.nxh - 1);
+      nm[e] = jm == 0 ? (TO)0 : -(TO)pmj[jm + 1] * (TO)IB[a];
+      ng[e] = jm == g.n2m - 1 ? (TO)0 : -(TO)GT[a];
+    }
+    __syncthreads();
+  }
   // every load of the chunk (spectrum, both factor rows) is issued first
   TO vr[CH], vi[CH], mm[CH], gt[CH];
 #pragma unroll
   for (int q = 0; q < CH; q++)
     if (q < y.len) {
       const int jm = y.j0 + q;
-      const long long a = jm * jstr + b;
-      const zi r = Si[a];
+      const zi r = Si[jm * jstr + b];
       const TO ib = (TO)IB[jm * tstr + bt];
       gt[q] = (jm == g.n2m - 1) ? (TO)0 : -(TO)GT[jm * tstr + bt];
-      vr[q] = (pin && jm == 0) ? (TO)0 : (TO)r.x * ib;
-      vi[q] = selfconj ? (TO)0 : (TO)r.y * ib;
-      mm[q] = (jm == 0) ? (TO)0 : -(TO)pmj[jm + 1] * ib;
+      if (c0) {
+        // scaled once column 0 is split (mm holds IBET until then)
+        vr[q] = (TO)r.x;
+        vi[q] = (TO)r.y;
+        mm[q] = ib;
+      } else {
+        vr[q] = (TO)r.x * ib;
+        vi[q] = (TO)r.y * ib;
+        mm[q] = (jm == 0) ? (TO)0 : -(TO)pmj[jm + 1] * ib;
+      }
     }
+  if (c0) {
+#pragma unroll
+    for (int q = 0; q < CH; q++)
+      if (q < y.len) {
+        const int jm = y.j0 + q;
+        // the other half's thread of this chunk and column (same wavefront)
+        const TO pr = __shfl_xor(vr[q], LH, 64), pi = __shfl_xor(vi[q], LH, 64);
+        TO ar = vr[q], ai = vi[q];
+        if (pk && pz != 0) {
+          if (!half) {
+            ar = (TO)0.5 * (vr[q] + pr);  // A(pz)
+            ai = (TO)0.5 * (vi[q] - pi);
+          } else {
+            ar = (TO)0.5 * (vi[q] + pi);  // B(NZ - pz)
+            ai = (TO)0.5 * (pr - vr[q]);
+          }
+        }
+        const TO ib = mm[q];
+        vr[q] = (pin && jm == 0) ? (TO)0 : ar * ib;
+        vi[q] = ai * ib;
+        mm[q] = (jm == 0) ? (TO)0 : -(TO)pmj[jm + 1] * ib;
+      }
+  }
   ychain_fwd2(vr, vi, [&](int q) { return mm[q]; }, y, L, xs[0], xs[1], xs[2]);
   ychain_bwd2(vr, vi, [&](int q) { return gt[q]; }, y, C, L, xs[3], xs[4], xs[5]);
+  if (dual) {
+    // B(0), B(NZ/2): the imaginary parts of column 0 again, from the
+    // spectrum, with the kx = NX/2 factors.  Every thread is past the last
+    // barrier of the first pass, so the exchange arrays are free.
+#pragma unroll
+    for (int q = 0; q < CH; q++)
+      if (q < y.len && pk) {
+        const int jm = y.j0 + q;
+        vi[q] = (TO)Si[jm * jstr + b].y * (TO)IB[jm * tstr + (long long)kk * g.nxhp + (g.nxh - 1)];
+      }
+    const TO* mr = nm + half * g.n2m + y.j0;
+    const TO* gr = ng + half * g.n2m + y.j0;
+    ychain_fwd(vi, [&](int q) { return pk ? mr[q] : (TO)0; }, y, L, xs[0], xs[2]);
+    ychain_bwd(vi, [&](int q) { return pk ? gr[q] : (TO)0; }, y, C, L, xs[3], xs[5]);
+  }
+  if (c0) {
+#pragma unroll
+    for (int q = 0; q < CH; q++)
+      if (q < y.len) {
+        const TO pr = __shfl_xor(vr[q], LH, 64), pi = __shfl_xor(vi[q], LH, 64);
+        if (pk && pz != 0) {
+          const TO sr = vr[q], si = vi[q];
+          if (!half) {
+            vr[q] = sr + pi;  // A(pz) + i conj B(NZ - pz)
+            vi[q] = si + pr;
+          } else {
+            vr[q] = pr - si;  // conj A(pz) + i B(NZ - pz)
+            vi[q] = sr - pi;
+          }
+        }
+      }
+  }
+  // with sep, pair 0's column 0: the real parts here, the imaginary parts in
+  // block 0
+  const bool sx = !nyq, sy = !(sep && pk && pz == 0);
   if (ok)
 #pragma unroll
     for (int q = 0; q < CH; q++)
-      if (q < y.len) So[(y.j0 + q) * jstr + b] = zo{vr[q], vi[q]};
+      if (q < y.len) {
+        zo* o = So + (y.j0 + q) * jstr + b;
+        if (sx && sy)
+          *o = zo{vr[q], vi[q]};
+        else if (sx)
+          o->x = vr[q];
+        else
+          o->y = vi[q];
+      }
 }
 
 // set-up: per mode, BET(J) and GAM(J) of TDMAP on (PMJ, PCJ - AK1 - AK3, PPJ)
@@ -1737,8 +1853,9 @@ __global__ void k_pois_table(Ctx c) {
 
 // inverse x-DFT from the half spectrum (Hermitian fill, :9419-9425), real
 // part / (N1M*N3M) into DP (:9429-9433); two rows per complex transform:
-// Z = A + iB with the imaginary parts of A and B at kx = 0 and NX/2 dropped
-// (they do not reach the real part of the reference's transform)
+// Z = A + iB, with A and B at kx = 0 and NX/2 the real and imaginary parts of
+// the stored column 0 (their own imaginary parts do not reach the real part
+// of the reference's transform)
 template <typename T, int NE, int NT, bool CB_>
 __global__ __launch_bounds__(NT) void k_fft_xinv(CtxArg<CB_> ca_) {
   const Ctx c = ctx_get(ca_);
@@ -1754,22 +1871,20 @@ __global__ __launch_bounds__(NT) void k_fft_xinv(CtxArg<CB_> ca_) {
   const int nlines = g.n2m * g.nz;
   const int ln0 = blockIdx.x * 2 * NL;
   const zc* S = (const zc*)spec;
-  const int nxh = g.nxh;
 #pragma unroll 4
   for (int e = threadIdx.x; e < NL * n; e += NT) {
     const int l = e / n, kx = e - l * n;
     const int la = ln0 + l, lb = ln0 + NL + l;
-    const long long sa = la < nlines ? ((long long)(la % g.n2m) * g.nz + la / g.n2m) * g.nxhp : -1;
-    const long long sb = lb < nlines ? ((long long)(lb % g.n2m) * g.nz + lb / g.n2m) * g.nxhp : -1;
-    const bool lo = kx < nxh;
-    const int kk = lo ? kx : n - kx;
+    const long long sa = la < nlines ? ((long long)(la % g.n2m) * g.nz + la / g.n2m) * g.nxsp : -1;
+    const long long sb = lb < nlines ? ((long long)(lb % g.n2m) * g.nz + lb / g.n2m) * g.nxsp : -1;
+    const bool rl = kx == 0 || 2 * kx == n;  // the real modes, packed in column 0
+    const int kk = rl ? 0 : (2 * kx < n ? kx : n - kx);
     zc xa = sa >= 0 ? S[sa + kk] : zc{(T)0, (T)0};
     zc xb = sb >= 0 ? S[sb + kk] : zc{(T)0, (T)0};
-    if (kx == 0 || 2 * kx == n) {
-      xa.y = (T)0;
-      xb.y = (T)0;
-    }
-    if (!lo) {
+    if (rl) {
+      xa = {kx == 0 ? xa.x : xa.y, (T)0};
+      xb = {kx == 0 ? xb.x : xb.y, (T)0};
+    } else if (2 * kx > n) {
       xa.y = -xa.y;
       xb.y = -xb.y;
     }
@@ -2350,6 +2465,13 @@ void rhsdp(const Ctx& c) {
 
 // GETDP in FP64 in both modes: rounding RDP's transform or the solution to
 // FP32 would be amplified ~1/DY^2 by the projection's divergence
+// k_pois_y's dynamic LDS in its second-pass form: pair 0's kx = NX/2
+// multipliers
+static size_t pois_lds(const Geo& g) { return sizeof(double) * 4 * (size_t)g.n2m; }
+// resident blocks of the k_pois_y instantiations (6, 1024), (6, 512),
+// (CHMAX, 512), (6, 256), (CHMAX, 256) on the whole GPU, from fft_attrs()
+static int g_pois_slots[5];
+
 static void getdp_fp64(const Ctx& c) {
   typedef double T;
   const Geo& g = c.g;
@@ -2358,8 +2480,8 @@ static void getdp_fp64(const Ctx& c) {
   const size_t ldsx = sizeof(cplx_t<T>) * (size_t)g.nx * (f.tlx + 2);
   const size_t ldsz = sizeof(cplx_t<T>) * (size_t)g.nz * (f.tlz + 1);
   const int gx = (g.n2m * g.nz + 2 * f.tlx - 1) / (2 * f.tlx);
-  const int gz = ((g.nxh + f.tlz - 1) / f.tlz) * g.n2m;
-  const int gy = ((g.nxh + c.o.yp.L / 2 - 1) / (c.o.yp.L / 2)) * (g.nz / 2);
+  const int gz = ((g.nxs + f.tlz - 1) / f.tlz) * g.n2m;
+  const int gy = ((g.nxs + c.o.yp.L / 2 - 1) / (c.o.yp.L / 2)) * (g.nz / 2);
   // tile sizes (complex values per block): up to 1024 on 256 threads, up
   // to 2048 or 4096 on 512 (the autotuner picks the lines per tile)
   const int ex = g.nx * f.tlx, ez = g.nz * f.tlz;
@@ -2379,7 +2501,7 @@ static void getdp_fp64(const Ctx& c) {
   const int rz = (g.nz == 64 || g.nz == 128 || g.nz == 256) ? g.nz : 0;
   const int lx = rx ? 256 / (rx == 256 ? 16 : 8) : 0, lzr = rz ? 256 / (rz == 256 ? 16 : 8) : 0;
   const int gxr = rx ? (g.n2m * g.nz + 2 * lx - 1) / (2 * lx) : 0;
-  const int gzr = rz ? ((g.nxh + lzr - 1) / lzr) * g.n2m : 0;
+  const int gzr = rz ? ((g.nxs + lzr - 1) / lzr) * g.n2m : 0;
 #define RF_(K, n, GR, ...)                                                                                  \
   do {                                                                                                      \
     if ((n) == 256)                                                                                         \
@@ -2408,14 +2530,20 @@ static void getdp_fp64(const Ctx& c) {
     KScope ks_(c, KID_POIS_Y);
     const int npt = c.o.yp.L * c.o.yp.C;
     const bool six = (g.n2m + c.o.yp.C - 1) / c.o.yp.C <= 6;
-#define PY_(CH, NT) \
-  C3D_LAUNCH(k_pois_y, (T, T, CH, NT), dim3(gy), dim3(npt), 0, c)
+    // the extra block for pair 0's B modes, unless the tiles fill whole
+    // rounds of resident blocks (then the second pass)
+    const long long tiles = (long long)gy * (c.bdev ? c.bn : 1);
+#define PY_(CH, NT, S)                                                                             \
+  do {                                                                                             \
+    const int sep = (g_pois_slots[S] > 0 && tiles % g_pois_slots[S] == 0) ? 0 : 1;               \
+    C3D_LAUNCH(k_pois_y, (T, T, CH, NT), dim3(gy + sep), dim3(npt), sep ? 0 : pois_lds(g), c, sep); \
+  } while (0)
     if (npt > 512)
-      PY_(6, 1024);
+      PY_(6, 1024, 0);
     else if (npt > 256) {
-      if (six) PY_(6, 512); else PY_(CHMAX, 512);
+      if (six) PY_(6, 512, 1); else PY_(CHMAX, 512, 2);
     } else {
-      if (six) PY_(6, 256); else PY_(CHMAX, 256);
+      if (six) PY_(6, 256, 3); else PY_(CHMAX, 256, 4);
     }
 #undef PY_
   }
@@ -2464,6 +2592,29 @@ void fft_attrs() {
   FR_(k_rfft_z);
   FR_(k_rfft_xinv);
 #undef FR_
+  // k_pois_y: its second-pass LDS (4 N2M doubles) beside the static exchange
+  // arrays (6 NT doubles), and its resident blocks per GPU
+  int dev = 0, ncu = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+#define PA_(CH, NT, S)                                                                                \
+  do {                                                                                                \
+    (void)hipFuncSetAttribute((const void*)&k_pois_y<double, double, CH, NT, false>,                 \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lim - 48 * (NT));           \
+    (void)hipFuncSetAttribute((const void*)&k_pois_y<double, double, CH, NT, true>,                  \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lim - 48 * (NT));           \
+    int nb = 0;                                                                                       \
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)&k_pois_y<double, double, CH, NT, false>, \
+                                                     NT, 0) != hipSuccess)                            \
+      nb = 0;                                                                                         \
+    g_pois_slots[S] = nb * ncu;                                                                       \
+  } while (0)
+  PA_(6, 1024, 0);
+  PA_(6, 512, 1);
+  PA_(CHMAX, 512, 2);
+  PA_(6, 256, 3);
+  PA_(CHMAX, 256, 4);
+#undef PA_
 }
 
 void pois_table(const Ctx& c) {
