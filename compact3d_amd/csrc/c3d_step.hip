@@ -1962,46 +1962,77 @@ __global__ __launch_bounds__(256) void k_upd_x(CtxArg<CB_> ca_, int chk) {
   const Fields F = c.d;
   const T dx1 = (T)s.dx1, dt = (T)s.dt;
   const T dmp = (T)F.sc[SC_DMPRESG];
-  // pf: the next UH/DP tile in flight
-  T dpr[SEG], d[SEG], un[SEG], pf[SEG];
+  // V(J) and V(J+1) read DP at J-1, J, J+1 and UH2 at J, J+1: rows J-1 and
+  // J+1 of tile line l are lines l-1 and l+1 of the same tiles (in one
+  // k-plane wherever they are used), the row before line 0 (top halo) sits
+  // in ht, the row after line L-1 (bottom halo) in the tiles' pad slots
+  // p*LS + L.  So DP and UH2 are read from HBM once, plus a row or two in L.
+  __shared__ double ht[512];
+  const int nh = sh.n >> 1, pq = threadIdx.x % nh, lq = threadIdx.x / nh;
+  const int nlines = g.n2m * g.nz, ln0 = blockIdx.x * L;
+  const int lnl = min(ln0 + L, nlines) - 1;  // the tile's last line
+  const int j0 = 1 + ln0 % g.n2m, jl = 1 + lnl % g.n2m;
+  const bool htop = lq == 0 && j0 >= 2, hbot = chk && lq == 0 && ln0 + L <= nlines && jl + 1 <= g.n2m;
+  const unsigned otop = (unsigned)gidx(g, 2 * pq, j0 - 1, ln0 / g.n2m);
+  const unsigned obot = (unsigned)gidx(g, 2 * pq, jl + 1, lnl / g.n2m);
+  T d[SEG], un[SEG], pf[SEG], vj[SEG], vd[SEG];
+  double2 h0 = {0.0, 0.0}, h1 = {0.0, 0.0};
   X.pre(F.dp, 0, pf);
-  const T* B = X.put(pf);
-  X.pre((const T*)F.uh[0], 0, pf);
-  SQ {
-    dpr[q] = XA(B, q);
-    d[q] = (T)24. * dx1 * (-XA(B, q - 1) + XA(B, q));
+  if (htop) h0 = *(const double2*)(F.dp + otop);
+  if (hbot) h1 = *(const double2*)(F.dp + obot);
+  const T* A = X.put(pf);  // DP
+  if (htop) {
+    ht[2 * pq] = h0.x;
+    ht[2 * pq + 1] = h0.y;
   }
-  psolve<T, SEG>(d, G, L, l, gs, c.o.px.s22, bf.xf, bf.xb);
-  B = X.put(pf);
-  SQ un[q] = XA(B, q) - dt * d[q] - dt * dmp;
+  if (hbot) {
+    T* Aw = (T*)A;
+    Aw[2 * pq * X.LS + L] = h1.x;
+    Aw[(2 * pq + 1) * X.LS + L] = h1.y;
+  }
   X.pre((const T*)F.uh[1], 0, pf);
-  X.out(F.u, un);
+  if (hbot) h1 = *(const double2*)((const T*)F.uh[1] + obot);
+  SQ d[q] = (T)24. * dx1 * (-XA(A, q - 1) + XA(A, q));
+  psolve<T, SEG>(d, G, L, l, gs, c.o.px.s22, bf.xf, bf.xb);  // its barriers publish the halos
+  const T* B = X.put(pf);  // UH2
+  if (hbot) {
+    T* Bw = (T*)B;
+    Bw[2 * pq * X.LS + L] = h1.x;
+    Bw[(2 * pq + 1) * X.LS + L] = h1.y;
+  }
+  if (chk) __syncthreads();
+  X.pre((const T*)F.uh[0], 0, pf);
+  // V(J) = UH2(J) - DT*(DP(J) - DP(J-1))/H(J) (:1139-1145); with chk also
+  // (V(J+1) - V(J))/DY, V(N2) = 0
   const T rhj = (T)M.rh[j];  // 1/H(J)
-  const T* B1 = X.put(pf);
-  X.pre(F.dp, -1, pf);
-  B = X.put(pf);
-  SQ d[q] = (j >= 2) ? XA(B1, q) - dt * (dpr[q] - XA(B, q)) * rhj : (T)0;
-  if (chk) X.pre((const T*)F.uh[1], 1, pf);
-  X.out(F.v, d, 2);  // V(I,1,K) is not written by UPCALC
+  SQ {
+    const int p = p0 + q;
+    const T dm = l == 0 ? ht[p] : A[p * X.LS + l - 1];
+    vj[q] = (j >= 2) ? B[p * X.LS + l] - dt * (A[p * X.LS + l] - dm) * rhj : (T)0;
+  }
+  if (chk) {
+    const T rhjp = (T)M.rh[j + 1], rdy = (T)M.rdy[j];
+    SQ {
+      const int p = p0 + q;
+      const T e = B[p * X.LS + l + 1] - dt * (A[p * X.LS + l + 1] - A[p * X.LS + l]) * rhjp;
+      vd[q] = (((j + 1 <= g.n2m) ? e : (T)0) - vj[q]) * rdy;
+    }
+  }
+  X.out(F.v, vj, 2);  // V(I,1,K) is not written by UPCALC
+  B = X.put(pf);      // UH1
+  SQ un[q] = XA(B, q) - dt * d[q] - dt * dmp;
+  X.out(F.u, un);
   upd_ghosts<SEG>(c, X, L, sh.n);
   if (!chk) return;
-  // new V at J+1 (zero at J+1 = N2: the wall value); d = (V(J+1) - V(J))/DY
-  const T rhjp = (T)M.rh[j + 1], rdy = (T)M.rdy[j];
-  B1 = X.put(pf);
-  X.pre(F.dp, 1, pf);
-  B = X.put(pf);
-  T e[SEG];
-  SQ e[q] = XA(B1, q) - dt * (XA(B, q) - dpr[q]) * rhjp;
-  SQ d[q] = (((j + 1 <= g.n2m) ? e[q] : (T)0) - d[q]) * rdy;
   const T rt = xch_right(un[0], G, L, l, gs, bf.xr);
-  SQ dpr[q] = (T)24. * dx1 * (nxt(un, q, rt) - un[q]);
-  psolve<T, SEG>(dpr, G, L, l, gs, c.o.px.s22, bf.xf, bf.xb);
-  SQ d[q] = d[q] + dpr[q];
+  SQ d[q] = (T)24. * dx1 * (nxt(un, q, rt) - un[q]);
+  psolve<T, SEG>(d, G, L, l, gs, c.o.px.s22, bf.xf, bf.xb);
+  SQ d[q] = vd[q] + d[q];
   X.out(F.div, d);
 }
 
-// Mixed mode: k_upd_xu updates U on x-lines and stores D1x of the new U
-// (FP64, in RDP's buffer), then updates V pointwise; k_upd_z adds
+// Mixed mode: k_upd_xu updates V pointwise from its DP tile, then U on
+// x-lines, and stores D1x of the new U (FP64, in RDP's buffer); k_upd_z adds
 // (V(J+1) - V(J))/DY, from the V(J), V(J+1) its CFL term reads anyway, and
 // D1z of the new W.  Same expressions and operation order as k_upd_x.
 template <int SEG, bool CB_>
@@ -2017,9 +2048,43 @@ __global__ __launch_bounds__(256) void k_upd_xu(CtxArg<CB_> ca_, int refine) {
   const T ku = refine ? (T)0 : (T)1;
   T d[SEG], un[SEG], pf[SEG], pg[SEG];
   float ph[SEG];
+  // V (J >= 2; V(1) is the wall's, left to the ghost pass), pointwise in the
+  // staging's pair layout: V = VOLD + DELTA_v - DT*(DP(J) - DP(J-1))/H(J)
+  // (:1139-1145), with DP(J) and DP(J-1) from the DP tile in LDS: row J-1 of
+  // tile line ll >= 1 is line ll-1 (when J >= 2 both lie in one k-plane),
+  // and the row before line 0 (the halo) goes to the tile's pad slots
+  // p*LS + L.  So DP is read from HBM once, plus one row in L.
+  const int nh = sh.n >> 1, pq = threadIdx.x % nh, lq = threadIdx.x / nh;
+  const int jh = 1 + (blockIdx.x * L) % g.n2m;  // J of line 0
+  const bool hl = lq == 0 && jh >= 2 && blockIdx.x * L < g.n2m * g.nz;
+  double2 hv = {0.0, 0.0};
+  if (hl) hv = *(const double2*)(F.dp + (X.go2[0] - (unsigned)g.nx));
   X.pre(F.dp, 0, pf);
+  double vo[SEG];
+  float dv[SEG];
+  X.pre_pairs(F.vold, 0u, vo);
+  X.pre_pairs((const float*)F.uh[1], 0u, dv);
   const T* B = X.put(pf);
+  if (hl) {
+    T* Bw = (T*)B;
+    Bw[2 * pq * X.LS + L] = hv.x;
+    Bw[(2 * pq + 1) * X.LS + L] = hv.y;
+  }
+  __syncthreads();
   X.pre_raw((const float*)F.uh[0], 0, ph);
+#pragma unroll
+  for (int m = 0; m < SEG / 2; m++) {
+    const int ll = lq + m * X.lstep2;
+    const int jm = 1 + (blockIdx.x * L + ll) % g.n2m;
+    if (((X.vm2 >> m) & 1u) && jm >= 2) {
+      const int ij = X.li2 + m * X.lstep2, im = ll == 0 ? 2 * pq * X.LS + L : ij - 1;
+      const double rh = M.rh[jm];
+      const double v0 = vo[2 * m] + (ku * (double)dv[2 * m] - dt * (B[ij] - B[im]) * rh);
+      const double v1 = vo[2 * m + 1] + (ku * (double)dv[2 * m + 1] - dt * (B[ij + X.LS] - B[im + X.LS]) * rh);
+      *(double2*)(F.v + X.go2[m]) = double2{v0, v1};
+      *(float2*)(F.vf + X.go2[m]) = float2{(float)v0, (float)v1};
+    }
+  }
   SQ d[q] = (T)24. * dx1 * (-XA(B, q - 1) + XA(B, q));
   psolve<T, SEG>(d, G, L, l, gs, c.o.px.s22, bf.xf, bf.xb);
   X.pre(F.u, 0, pg);  // after the solve: the old U tile is not live across it
@@ -2039,28 +2104,6 @@ __global__ __launch_bounds__(256) void k_upd_xu(CtxArg<CB_> ca_, int refine) {
   SQ d[q] = (T)24. * dx1 * (nxt(un, q, rt) - un[q]);
   psolve<T, SEG>(d, G, L, l, gs, c.o.px.s22, bf.xf, bf.xb);
   X.out(F.rdp, d);  // D1x U of the new field, for k_upd_z
-  // V (J >= 2; V(1) is the wall's, left to the ghost pass), pointwise in the
-  // staging's pair layout: V = VOLD + DELTA_v - DT*(DP(J) - DP(J-1))/H(J)
-  // (:1139-1145).  The block's DP rows come back from L2 (it read them
-  // above), so only VOLD and DELTA_v are new reads.
-  double vo[SEG], dj[SEG], dm[SEG];
-  float dv[SEG];
-  X.pre_pairs(F.vold, 0u, vo);
-  X.pre_pairs((const float*)F.uh[1], 0u, dv);
-  X.pre_pairs(F.dp, 0u, dj);
-  X.pre_pairs(F.dp, (unsigned)(-g.nx), dm);  // row J-1 (the offset wraps in 32 bits)
-  const int lq = threadIdx.x / (sh.n >> 1);
-#pragma unroll
-  for (int m = 0; m < SEG / 2; m++) {
-    const int jm = 1 + (blockIdx.x * L + lq + m * X.lstep2) % g.n2m;
-    if (((X.vm2 >> m) & 1u) && jm >= 2) {
-      const double rh = M.rh[jm];
-      const double v0 = vo[2 * m] + (ku * (double)dv[2 * m] - dt * (dj[2 * m] - dm[2 * m]) * rh);
-      const double v1 = vo[2 * m + 1] + (ku * (double)dv[2 * m + 1] - dt * (dj[2 * m + 1] - dm[2 * m + 1]) * rh);
-      *(double2*)(F.v + X.go2[m]) = double2{v0, v1};
-      *(float2*)(F.vf + X.go2[m]) = float2{(float)v0, (float)v1};
-    }
-  }
   upd_ghosts<SEG>(c, X, L, sh.n);
 }
 
