@@ -52,11 +52,11 @@ ALG_BYTES_ADI_X_PER_STEP = 56
 # writes the FP64 state and writes the shadows).  The x and z ADI sweeps run
 # z (u, v: DELTA in and out), then x (u, v, w: DELTA in and out; u's also
 # writes RDP = D1x(DELTA_u)/DT in FP64), then z for w, which also reads RDP,
-# DELTA_v and the FP64 DIV of the state and writes the final RDP: RHSDP has
-# no kernel of its own
+# DELTA_v and the DIV of the state (carried in FP32) and writes the final
+# RDP: RHSDP has no kernel of its own
 ALG_BYTES_MIXED = {
     "rhs_x": 44, "rhs_z": 72, "rhs_x2": 12, "adi_y": 84, "adi_z": 16,
-    "adi_xr": 32, "adi_zr": 36, "fft_x": 16, "fft_z": 16, "pois_y": 20, "fft_xinv": 16,
+    "adi_xr": 32, "adi_zr": 32, "fft_x": 16, "fft_z": 16, "pois_y": 20, "fft_xinv": 16,
     # the update: U and V on x-lines (DP, DELTA_u, U, DELTA_v, V in; U, V,
     # their shadows, D1x U out), then W and P on z-lines (DP, DELTA_w, W, P,
     # D1x U, U, V in; W, P, their shadows, the FP32 DIV out) with the y and z
