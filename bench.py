@@ -416,7 +416,8 @@ def served_leg(args):
         if name == "batched":
             for r in stream:
                 r.t_arrive = 0.0
-        srv = Server(budget, max_live=args.serve_slots, chunk=10, use_graph=True, fields=fields)
+        srv = Server(budget, max_live=args.serve_slots, chunk=10, use_graph=True, fields=fields,
+                     policy=args.serve_policy, max_active=args.serve_active)
         done = srv.run(stream)
         wall = max(r.t_done for r in done) - min(r.t_arrive for r in done)
         st = stats(done, wall)
@@ -427,8 +428,11 @@ def served_leg(args):
     out["served_over_batched"] = out["served"]["throughput_mpts_steps_s"] / out["batched"]["throughput_mpts_steps_s"]
     out["stream"] = ("%d requests, Poisson arrivals 20/s (seed 4), grids 64x49x64..256x129x256, Re_tau "
                      "{180, 395; 1000 wall-modelled LES}, all mixed precision, 100-2000 steps, dt 5e-4, "
-                     "%d slots, chunks of up to 10 steps, captured graphs on each instance's own stream" % (
-                         len(reqs), args.serve_slots))
+                     "%d slots, chunks of up to 10 steps, captured graphs on each instance's own stream, "
+                     "policy %s" % (len(reqs), args.serve_slots, args.serve_policy))
+    out["policy"] = args.serve_policy
+    if args.serve_policy == "srpt":
+        out["max_active"] = args.serve_active
     return out
 
 
@@ -651,6 +655,10 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--serve-slots", type=int, default=16)
+    ap.add_argument("--serve-policy", default="srpt", choices=["srpt", "fifo"],
+                    help="serving loop policy: shortest remaining work first, or arrival order")
+    ap.add_argument("--serve-active", type=int, default=8,
+                    help="srpt: instances with a chunk queued at a time")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched LES ensemble leg")
     ap.add_argument("--no-autotune", action="store_true", help="keep the rule-derived tile shapes")
     ap.add_argument("--shapes", metavar="JSON",

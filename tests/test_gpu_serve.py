@@ -45,7 +45,8 @@ def test_advance_and_graphs_equal_stepping(precision):
         ch.close()
 
 
-def test_served_result_equals_alone():
+@pytest.mark.parametrize("policy", ["fifo", "srpt"])
+def test_served_result_equals_alone(policy):
     from compact3d_amd import c3d
     from compact3d_amd.serve import Request, Server
     reqs = [Request(rid=0, grid=(48, 33, 48), re=4200.0, steps=23, seed=1, precision=1, dt=1e-3),
@@ -53,7 +54,7 @@ def test_served_result_equals_alone():
             Request(rid=2, grid=(32, 33, 32), re=20000.0, steps=17, seed=3, precision=0, les=True, dt=5e-4,
                     t_arrive=0.002),
             Request(rid=3, grid=(48, 33, 48), re=4200.0, steps=31, seed=1, precision=0, dt=1e-3, t_arrive=0.003)]
-    srv = Server(budget_bytes=8 << 30, max_live=8, chunk=6, keep_results=True)
+    srv = Server(budget_bytes=8 << 30, max_live=8, chunk=6, keep_results=True, policy=policy, max_active=2)
     done = srv.run(reqs)
     assert len(done) == 4
     for r in done:

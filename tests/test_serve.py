@@ -178,3 +178,78 @@ def test_arrival_admitted_at_the_next_chunk_boundary():
     # first queue: 0, 1, 2 (clock 0.03); the late request joins right after
     # the first completion, ahead of the re-queues of instances 1 and 2
     assert log[:5] == [(0, 10), (1, 10), (2, 10), (0, 10), (9, 10)]
+
+
+def _server_policy(budget, max_live, chunk=10, **kw):
+    clock = FakeClock()
+    log = []
+    srv = Server(budget, max_live=max_live, chunk=chunk, make_channel=lambda r: FakeChannel(r, clock, log),
+                 footprint=lambda r: r.pts, fields=NoFields(), clock=clock, sleep=clock.sleep, **kw)
+    return srv, clock, log
+
+
+def test_srpt_serves_the_least_work_first():
+    FakeChannel.live = FakeChannel.peak = 0
+    steps = [40, 10, 30, 20]
+    reqs = [Request(rid=q, grid=(8, 9, 8), re=100.0, steps=s, t_arrive=0.0) for q, s in enumerate(steps)]
+    srv, clock, log = _server_policy(10 ** 9, 8, policy="srpt", max_active=1)
+    done = srv.run(reqs)
+    assert [r.rid for r in done] == [1, 3, 2, 0]
+    # one instance at a time, each to the end of its steps
+    assert [rid for rid, n in log] == [1, 3, 3, 2, 2, 2, 0, 0, 0, 0]
+    assert all(n == 10 for rid, n in log) and FakeChannel.live == 0
+
+
+def test_srpt_weighs_cells_times_steps():
+    """a request of a bigger grid counts its cells: 10 steps of 4x the cells
+    rank after 30 steps of the small grid"""
+    FakeChannel.live = FakeChannel.peak = 0
+    small = Request(rid=0, grid=(8, 9, 8), re=100.0, steps=30, t_arrive=0.0)
+    big = Request(rid=1, grid=(16, 9, 16), re=100.0, steps=10, t_arrive=0.0)
+    srv, clock, log = _server_policy(10 ** 9, 8, policy="srpt", max_active=1)
+    done = srv.run([big, small])
+    assert [r.rid for r in done] == [0, 1]
+
+
+def test_srpt_late_short_request_overtakes_a_long_one():
+    FakeChannel.live = FakeChannel.peak = 0
+    long = Request(rid=0, grid=(8, 9, 8), re=100.0, steps=200, t_arrive=0.0)
+    late = Request(rid=1, grid=(8, 9, 8), re=100.0, steps=10, t_arrive=0.05)
+    srv, clock, log = _server_policy(10 ** 9, 8, policy="srpt", max_active=1)
+    done = srv.run([long, late])
+    assert [r.rid for r in done] == [1, 0]
+    # admitted at the next chunk boundary and run before the long one's next chunk
+    assert late.t_done - late.t_arrive <= 0.02 + 1e-12
+    i = log.index((1, 10))
+    assert all(rid == 0 for rid, n in log[:i]) and all(rid == 0 for rid, n in log[i + 1:])
+    assert sum(n for rid, n in log if rid == 0) == 200
+
+
+def test_srpt_mean_latency_not_above_fifo():
+    """on a seeded stream of mixed sizes, shortest remaining work first does
+    not raise the mean latency over arrival order with every instance
+    sharing the (stand-in, serial) device; every request runs all its steps"""
+    import random
+    rng = random.Random(7)
+    base = [Request(rid=q, grid=rng.choice([(8, 9, 8), (16, 9, 16)]), re=100.0, steps=rng.randrange(10, 200, 10),
+                    t_arrive=0.01 * q) for q in range(12)]
+    lat = {}
+    for policy in ("fifo", "srpt"):
+        FakeChannel.live = FakeChannel.peak = 0
+        reqs = [Request(rid=r.rid, grid=r.grid, re=r.re, steps=r.steps, t_arrive=r.t_arrive) for r in base]
+        srv, clock, log = _server_policy(10 ** 9, 16, policy=policy, max_active=1)
+        done = srv.run(reqs)
+        assert len(done) == 12 and FakeChannel.live == 0
+        ran = {}
+        for rid, n in log:
+            ran[rid] = ran.get(rid, 0) + n
+        assert all(ran[r.rid] == r.steps for r in reqs)
+        lat[policy] = sum(r.t_done - r.t_arrive for r in done) / len(done)
+    assert lat["srpt"] <= lat["fifo"]
+
+
+def test_unknown_policy_is_refused():
+    with pytest.raises(ValueError):
+        Server(10, policy="lifo")
+    with pytest.raises(ValueError):
+        Server(10, policy="srpt", max_active=0)
