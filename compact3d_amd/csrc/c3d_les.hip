@@ -1358,22 +1358,22 @@ __global__ __launch_bounds__(512) void k_sgs_z(CtxArg<CB_> ca_) {
   SQ d[q] = (T)4. * (prv(sv, q, lf) + sv[q]);
   zload((const T*)F.nux, b, pl, p0, bm, bp, a, lf, rt);
   psolve<T, SEG>(d, G, L, l, gs, s6, bf.xf, bf.xb);
-  if (ok) SQ nuz[ZI(q)] = d[q];
+  if (ok) SQ st_nt(&nuz[ZI(q)], (T)(d[q]));
   // nue = Iz(nux)
   SQ e[q] = (T)4. * (a[q] + prv(a, q, lf));
   zload((const T*)F.su, b, pl, p0, bm, bp, a, lf, rt);
   psolve<T, SEG>(e, G, L, l, gs, s6, bf.xf, bf.xb);
-  if (ok) SQ nue[ZI(q)] = e[q];
+  if (ok) SQ st_nt(&nue[ZI(q)], (T)(e[q]));
   // dudz = D1z(u)
   SQ dz[q] = (T)24. * dx3 * (a[q] - prv(a, q, lf));
   zload((const T*)F.sv, b, pl, p0, bm, bp, a, lf, rt);
   psolve<T, SEG>(dz, G, L, l, gs, s22, bf.xf, bf.xb);
-  if (ok) SQ dudz[ZI(q)] = dz[q];
+  if (ok) SQ st_nt(&dudz[ZI(q)], (T)(dz[q]));
   // dvdz = D1z(v)
   SQ d[q] = (T)24. * dx3 * (a[q] - prv(a, q, lf));
   zload((const T*)F.sw, b, pl, p0, bm, bp, a, lf, rt);
   psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
-  if (ok) SQ dvdz[ZI(q)] = d[q];
+  if (ok) SQ st_nt(&dvdz[ZI(q)], (T)(d[q]));
   // HH3 -= 2 D1z(SGSVIS * D1z(w))
   SQ d[q] = (T)24. * dx3 * (nxt(a, q, rt) - a[q]);
   psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
@@ -1382,7 +1382,7 @@ __global__ __launch_bounds__(512) void k_sgs_z(CtxArg<CB_> ca_) {
   SQR d[q] = (T)24. * dx3 * (d[q] - prv(d, q, lf));
   SQ a[q] = hh2[ZI(q)];
   psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
-  if (ok) SQ hh2[ZI(q)] = a[q] - (T)2. * d[q];
+  if (ok) SQ st_nt(&hh2[ZI(q)], (T)(a[q] - (T)2. * d[q]));
   // HH1 -= D1z(dudz * nue)
   SQ d[q] = dz[q] * e[q];
   rt = xch_right(d[0], G, L, l, gs, bf.xr);
@@ -1395,7 +1395,7 @@ __global__ __launch_bounds__(512) void k_sgs_z(CtxArg<CB_> ca_) {
   rt = xch_right(d[0], G, L, l, gs, bf.xr);
   SQ d[q] = (T)24. * dx3 * (nxt(d, q, rt) - d[q]);
   psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
-  if (ok) SQ hh0[ZI(q)] = a[q] - d[q];
+  if (ok) SQ st_nt(&hh0[ZI(q)], (T)(a[q] - d[q]));
   if (j < 2) return;  // uniform per block
   // J >= 2: the two RHS2 terms that need nuz at J-1 and J,
   //   D(SGS*(DW/DY))/DZ (:7177-7226), D(SGS*(DV/DZ))/DZ (:7245-7304).
@@ -1425,7 +1425,7 @@ __global__ __launch_bounds__(512) void k_sgs_z(CtxArg<CB_> ca_) {
     rt = xch_right(d[0], G, L, l, gs, bf.xr);
     SQ d[q] = (T)24. * dx3 * (nxt(d, q, rt) - d[q]);
     psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
-    if (ok) SQ hh1[ZI(q)] = a[q] - d[q];
+    if (ok) SQ st_nt(&hh1[ZI(q)], (T)(a[q] - d[q]));
   }
 }
 

@@ -132,10 +132,14 @@ struct XStage {
     for (int m = 0; m < SEG / 2; m++)
       if (keep2(m, jmin)) {
         const T x0 = B[li2 + m * lstep2], x1 = B[li2 + LS + m * lstep2];
+        // pair stores with the nontemporal hint (st_nt); the builtin takes
+        // clang vector types, not HIP's float2/double2 structs
+        typedef float nf2 __attribute__((ext_vector_type(2)));
+        typedef double nd2 __attribute__((ext_vector_type(2)));
         if constexpr (sizeof(S) == 4)
-          *(float2*)(F + go2[m]) = float2{(float)x0, (float)x1};
+          st_nt((nf2*)(F + go2[m]), nf2{(float)x0, (float)x1});
         else
-          *(double2*)(F + go2[m]) = double2{(double)x0, (double)x1};
+          st_nt((nd2*)(F + go2[m]), nd2{(double)x0, (double)x1});
       }
   }
   // the same tile to two arrays (a field and its FP32 shadow); rows with

@@ -170,37 +170,37 @@ __global__ __launch_bounds__(512) void k_rhs_z(CtxArg<CB_> ca_) {
   SQ d[q] = (T)12. * dx3q * (prv(a, q, lf) - (T)2. * a[q] + nxt(a, q, rt));
   SQ pr[q] = ruh0[ZI(q)];
   psolve<T, SEG>(d, G, L, l, gs, s10, bf.xf, bf.xb);
-  if (ok) SQ ruh0[ZI(q)] = pr[q] + rinv * d[q];
+  if (ok) SQ st_nt(&ruh0[ZI(q)], (T)(pr[q] + rinv * d[q]));
   SQ d[q] = (T)4. * (prv(a, q, lf) + a[q]);
   SQ e[q] = ((const T*)F.ww1)[ZI(q)];
   psolve<T, SEG>(d, G, L, l, gs, s6, bf.xf, bf.xb);
   SQ d[q] = d[q] * e[q];  // UW
-  if (ok) SQ((T*)F.uw)[ZI(q)] = d[q];
+  if (ok) SQ st_nt(&((T*)F.uw)[ZI(q)], (T)(d[q]));
   SQ pr[q] = hh0[ZI(q)];
   rt = xch_right(d[0], G, L, l, gs, bf.xr);
   SQ d[q] = (T)24. * dx3 * (-d[q] + nxt(d, q, rt));
   zload((const T*)F.sv, b, pl, p0, bm, bp, a, lf, rt);
   psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
-  if (ok) SQ hh0[ZI(q)] = pr[q] + d[q];
+  if (ok) SQ st_nt(&hh0[ZI(q)], (T)(pr[q] + d[q]));
   // v
   if (j >= 2) {
     SQ d[q] = (T)12. * dx3q * (prv(a, q, lf) - (T)2. * a[q] + nxt(a, q, rt));
     SQ pr[q] = ruh1[ZI(q)];
     psolve<T, SEG>(d, G, L, l, gs, s10, bf.xf, bf.xb);
-    if (ok) SQ ruh1[ZI(q)] = pr[q] + rinv * d[q];
+    if (ok) SQ st_nt(&ruh1[ZI(q)], (T)(pr[q] + rinv * d[q]));
   }
   SQ d[q] = (T)4. * (prv(a, q, lf) + a[q]);
   const T hj = (T)(1. / M.h[j]), dyj = (T)(M.dy[j] / 2.), dyjm = (T)(M.dy[j - 1] / 2.);
   if (j >= 2) SQ e[q] = hj * (dyj * sw[ZI(q) - g.nx] + dyjm * sw[ZI(q)]);  // WW2 (:7028)
   psolve<T, SEG>(d, G, L, l, gs, s6, bf.xf, bf.xb);
-  if (ok) SQ((T*)F.vv3)[ZI(q)] = d[q];
+  if (ok) SQ st_nt(&((T*)F.vv3)[ZI(q)], (T)(d[q]));
   if (j >= 2) {
     SQ d[q] = d[q] * e[q];
     SQ pr[q] = hh1[ZI(q)];
     rt = xch_right(d[0], G, L, l, gs, bf.xr);
     SQ d[q] = (T)24. * dx3 * (-d[q] + nxt(d, q, rt));
     psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
-    if (ok) SQ hh1[ZI(q)] = pr[q] + d[q];
+    if (ok) SQ st_nt(&hh1[ZI(q)], (T)(pr[q] + d[q]));
   }
   // w and P (a phase of its own, so its registers are those the u and v
   // phases used): RUH3 += (1/RE) D2z w - D1z(P) - PRESG3 (:7510-7550),
@@ -218,7 +218,7 @@ __global__ __launch_bounds__(512) void k_rhs_z(CtxArg<CB_> ca_) {
     SQR d[q] = (T)24. * dx3 * (-(q == 0 ? pv0 : d[q > 0 ? q - 1 : 0]) + d[q]);
     psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
     const T presg3 = (T)F.sc[SC_PRESG3];
-    if (ok) SQ ruh2[ZI(q)] = r[q] + rinv * e[q] - d[q] - presg3;
+    if (ok) SQ st_nt(&ruh2[ZI(q)], (T)(r[q] + rinv * e[q] - d[q] - presg3));
     // HH3 = D1z((Iz w)^2), w face -> centre -> face
     SQ d[q] = (T)4. * (a[q] + nxt(a, q, rt));
     psolve<T, SEG>(d, G, L, l, gs, s6, bf.xf, bf.xb);
@@ -226,7 +226,7 @@ __global__ __launch_bounds__(512) void k_rhs_z(CtxArg<CB_> ca_) {
     lf = xch_left(d[SEG - 1], G, L, l, gs, bf.xl);
     SQR d[q] = (T)24. * dx3 * (d[q] - prv(d, q, lf));
     psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
-    if (ok) SQ hh2[ZI(q)] = d[q];
+    if (ok) SQ st_nt(&hh2[ZI(q)], (T)(d[q]));
   }
 }
 
@@ -999,12 +999,12 @@ __global__ __launch_bounds__(512) void k_adi_z(CtxArg<CB_> ca_, CompList cl) {
         const T uq = (T)U[ZI(q)];
         sd += (double)d[q];
         su += (double)uq;
-        UH[ZI(q)] = uq + d[q];
+        st_nt(&UH[ZI(q)], (T)(uq + d[q]));
       }
     } else {
       SQ {
         sd += (double)d[q];
-        UH[ZI(q)] = d[q];
+        st_nt(&UH[ZI(q)], (T)(d[q]));
       }
     }
   }
@@ -1080,7 +1080,7 @@ __global__ __launch_bounds__(512) void k_adi_zr(CtxArg<CB_> ca_) {
   double sd = 0.0;
   if (ok) SQ {
       sd += (double)d[q];
-      UH[ZI(q)] = d[q];
+      st_nt(&UH[ZI(q)], (T)(d[q]));
     }
   // w: the flow-rate term of UPCALC (:1160-1172), sum DELTA*DY
   block_sum2(sd * M.dy[j], 0.0, F.flowp + FP_DW * F.flowr + blockIdx.x, nullptr);
@@ -1145,7 +1145,7 @@ __global__ __launch_bounds__(512) void k_rhsdp_z(CtxArg<CB_> ca_) {
   psolve<T, SEG>(d, G, L, l, gs, PZ.s22, bf.xf, bf.xb);
   double* rdp = c.d.rdp;
   const T rdt = (T)1. / dt;
-  if (ok) SQ rdp[ZI(q)] = rdp[ZI(q)] + d[q] * rdt;
+  if (ok) SQ st_nt(&rdp[ZI(q)], rdp[ZI(q)] + d[q] * rdt);
 }
 
 // ------------------------------------------------------------------ GETDP
@@ -2131,8 +2131,8 @@ __global__ __launch_bounds__(512) void k_upd_z(CtxArg<CB_> ca_, int chk, int ref
     if (ok) {
       SQ {
         const double pn = F.p[ZI(q)] + dpr[q];
-        F.p[ZI(q)] = pn;
-        if (MIX) F.pf[ZI(q)] = (float)pn;
+        st_nt(&F.p[ZI(q)], pn);
+        if (MIX) st_nt(&F.pf[ZI(q)], (float)pn);
       }
     }
   }
@@ -2151,8 +2151,8 @@ __global__ __launch_bounds__(512) void k_upd_z(CtxArg<CB_> ca_, int chk, int ref
   }
   if (ok) {
     SQ {
-      F.w[ZI(q)] = wn[q];
-      if (MIX) F.wf[ZI(q)] = (float)wn[q];
+      st_nt(&F.w[ZI(q)], (double)wn[q]);
+      if (MIX) st_nt(&F.wf[ZI(q)], (float)wn[q]);
     }
   }
   if (!chk) {
@@ -2177,7 +2177,7 @@ __global__ __launch_bounds__(512) void k_upd_z(CtxArg<CB_> ca_, int chk, int ref
         // kept in FP32 for the next RDP
         const double vj = j >= 2 ? v0 : 0.0, vp = j + 1 <= g.n2m ? v1 : 0.0;
         dv = ((vp - vj) * rdyj + F.rdp[a]) + (double)d[q];
-        F.divf[a] = (float)dv;
+        st_nt(&F.divf[a], (float)dv);
       } else {
         dv = F.div[a] + (double)d[q];
       }
