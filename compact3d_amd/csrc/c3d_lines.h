@@ -58,7 +58,10 @@ __device__ __forceinline__ LBuf<T> lbuf(unsigned char* sm, const LShape& sh, boo
 // alternate and every put()/out() starts with a barrier, so a tile stays
 // readable until the second staging operation after the one that filled it
 // begins.  Offsets are 32-bit (fields hold fewer than 2^31 elements).
-template <typename T, int SEG>
+// NTL: the loads carry the nontemporal hint, for kernels whose fields no
+// later kernel reads soon again (measured: the x-line updates 9-13 us
+// faster; the RHS x-pass slower, since the z-pass re-reads its fields).
+template <typename T, int SEG, bool NTL = false>
 struct XStage {
   T *st0, *st1;
   int sb;
@@ -81,12 +84,16 @@ struct XStage {
       S x0 = (S)0, x1 = (S)0;
       if ((vm2 >> m) & 1u) {
         // the offset is summed in 32 bits first: off wraps for joff < 0
+        typedef float nf2 __attribute__((ext_vector_type(2)));
+        typedef double nd2 __attribute__((ext_vector_type(2)));
         if constexpr (sizeof(S) == 4) {
-          const float2 v = *(const float2*)(F + (go2[m] + off));
+          const nf2* a = (const nf2*)(F + (go2[m] + off));
+          const nf2 v = NTL ? __builtin_nontemporal_load(a) : *a;
           x0 = (S)v.x;
           x1 = (S)v.y;
         } else {
-          const double2 v = *(const double2*)(F + (go2[m] + off));
+          const nd2* a = (const nd2*)(F + (go2[m] + off));
+          const nd2 v = NTL ? __builtin_nontemporal_load(a) : *a;
           x0 = (S)v.x;
           x1 = (S)v.y;
         }
@@ -174,8 +181,8 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
   return x * per + min(x, rem) + r;
 }
 
-template <typename T, int SEG>
-__device__ __forceinline__ void xstage(XStage<T, SEG>& x, const LBuf<T>& b, const Geo& g, const LShape& sh,
+template <typename T, int SEG, bool NTL>
+__device__ __forceinline__ void xstage(XStage<T, SEG, NTL>& x, const LBuf<T>& b, const Geo& g, const LShape& sh,
                                        int l, int p0) {
   x.st0 = b.st0;
   x.st1 = b.st1;
@@ -208,14 +215,17 @@ __device__ __forceinline__ void xstage(XStage<T, SEG>& x, const LBuf<T>& b, cons
   }
 }
 
-#define XSETUP                                                        \
+#define XSETUP XSETUP_NTL(false)
+// the same with nontemporal staging loads (XStage's NTL)
+#define XSETUP_NT XSETUP_NTL(true)
+#define XSETUP_NTL(NTL_)                                              \
   extern __shared__ __align__(16) unsigned char smem[];               \
   const Geo g = c.g;                                                  \
   const LShape sh = c.o.lx;                                           \
   const int L = sh.L, G = sh.G;                                       \
   const LBuf<T> bf = lbuf<T>(smem, sh, true);                         \
   const int l = threadIdx.x % L, gs = threadIdx.x / L, p0 = gs * SEG; \
-  XStage<T, SEG> X;                                                   \
+  XStage<T, SEG, NTL_> X;                                             \
   xstage(X, bf, g, sh, l, p0);                                        \
   const int ln = blockIdx.x * L + l;                                  \
   const bool ok = ln < g.n2m * g.nz;                                  \
@@ -252,6 +262,15 @@ template <typename T, int SEG, typename S>
 __device__ __forceinline__ void zload(const S* F, unsigned b, unsigned pl, int p0, unsigned bm, unsigned bp,
                                       T (&v)[SEG], T& left, T& right) {
   SQ v[q] = (T)F[b + (unsigned)(p0 + q) * pl];
+  left = (T)F[bm];
+  right = (T)F[bp];
+}
+// the same with the segment's own points loaded nontemporally (ld_nt); the
+// two neighbour points, which the adjacent segments also read, plainly
+template <typename T, int SEG, typename S>
+__device__ __forceinline__ void zload_nt(const S* F, unsigned b, unsigned pl, int p0, unsigned bm, unsigned bp,
+                                         T (&v)[SEG], T& left, T& right) {
+  SQ v[q] = (T)ld_nt(F + (b + (unsigned)(p0 + q) * pl));
   left = (T)F[bm];
   right = (T)F[bp];
 }

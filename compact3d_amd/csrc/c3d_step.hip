@@ -163,20 +163,21 @@ __global__ __launch_bounds__(512) void k_rhs_z(CtxArg<CB_> ca_) {
   T* hh1 = (T*)F.hh[1];
   T* hh2 = (T*)F.hh[2];
   const T dx3 = (T)s.dx3, dx3q = (T)s.dx3q, rinv = (T)s.rinv;
-  // each read-modify-write operand is loaded one solve ahead of its use
+  // each read-modify-write operand is loaded one solve ahead of its use,
+  // nontemporally (ld_nt), like P and WW1: read once here
   T a[SEG], d[SEG], e[SEG], pr[SEG], lf, rt;
   // u
   zload((const T*)F.su, b, pl, p0, bm, bp, a, lf, rt);
   SQ d[q] = (T)12. * dx3q * (prv(a, q, lf) - (T)2. * a[q] + nxt(a, q, rt));
-  SQ pr[q] = ruh0[ZI(q)];
+  SQ pr[q] = ld_nt(&ruh0[ZI(q)]);
   psolve<T, SEG>(d, G, L, l, gs, s10, bf.xf, bf.xb);
   if (ok) SQ st_nt(&ruh0[ZI(q)], (T)(pr[q] + rinv * d[q]));
   SQ d[q] = (T)4. * (prv(a, q, lf) + a[q]);
-  SQ e[q] = ((const T*)F.ww1)[ZI(q)];
+  SQ e[q] = ld_nt(&((const T*)F.ww1)[ZI(q)]);
   psolve<T, SEG>(d, G, L, l, gs, s6, bf.xf, bf.xb);
   SQ d[q] = d[q] * e[q];  // UW
   if (ok) SQ st_nt(&((T*)F.uw)[ZI(q)], (T)(d[q]));
-  SQ pr[q] = hh0[ZI(q)];
+  SQ pr[q] = ld_nt(&hh0[ZI(q)]);
   rt = xch_right(d[0], G, L, l, gs, bf.xr);
   SQ d[q] = (T)24. * dx3 * (-d[q] + nxt(d, q, rt));
   zload((const T*)F.sv, b, pl, p0, bm, bp, a, lf, rt);
@@ -185,7 +186,7 @@ __global__ __launch_bounds__(512) void k_rhs_z(CtxArg<CB_> ca_) {
   // v
   if (j >= 2) {
     SQ d[q] = (T)12. * dx3q * (prv(a, q, lf) - (T)2. * a[q] + nxt(a, q, rt));
-    SQ pr[q] = ruh1[ZI(q)];
+    SQ pr[q] = ld_nt(&ruh1[ZI(q)]);
     psolve<T, SEG>(d, G, L, l, gs, s10, bf.xf, bf.xb);
     if (ok) SQ st_nt(&ruh1[ZI(q)], (T)(pr[q] + rinv * d[q]));
   }
@@ -196,7 +197,7 @@ __global__ __launch_bounds__(512) void k_rhs_z(CtxArg<CB_> ca_) {
   if (ok) SQ st_nt(&((T*)F.vv3)[ZI(q)], (T)(d[q]));
   if (j >= 2) {
     SQ d[q] = d[q] * e[q];
-    SQ pr[q] = hh1[ZI(q)];
+    SQ pr[q] = ld_nt(&hh1[ZI(q)]);
     rt = xch_right(d[0], G, L, l, gs, bf.xr);
     SQ d[q] = (T)24. * dx3 * (-d[q] + nxt(d, q, rt));
     psolve<T, SEG>(d, G, L, l, gs, s22, bf.xf, bf.xb);
@@ -211,8 +212,8 @@ __global__ __launch_bounds__(512) void k_rhs_z(CtxArg<CB_> ca_) {
     zload(sw, b, pl, p0, bm, bp, a, lf, rt);
     const T* sp = (const T*)F.sp;
     const T pv0 = sp[bm];
-    SQ d[q] = sp[ZI(q)];
-    SQ r[q] = ruh2[ZI(q)];
+    SQ d[q] = ld_nt(&sp[ZI(q)]);
+    SQ r[q] = ld_nt(&ruh2[ZI(q)]);
     SQ e[q] = (T)12. * dx3q * (prv(a, q, lf) - (T)2. * a[q] + nxt(a, q, rt));
     psolve<T, SEG>(e, G, L, l, gs, s10, bf.xf, bf.xb);
     SQR d[q] = (T)24. * dx3 * (-(q == 0 ? pv0 : d[q > 0 ? q - 1 : 0]) + d[q]);
@@ -236,7 +237,7 @@ __global__ __launch_bounds__(512) void k_rhs_z(CtxArg<CB_> ca_) {
 template <typename T, int SEG, bool CB_>
 __global__ __launch_bounds__(256) void k_rhs_x2(CtxArg<CB_> ca_) {
   const Ctx c = ctx_get(ca_);
-  XSETUP;
+  XSETUP_NT;
   const T dx1 = (T)c.s.dx1;
   T d[SEG], pf[SEG];
   X.pre((const T*)c.d.uw, 0, pf);
@@ -1031,7 +1032,7 @@ template <int SEG, bool CB_>
 __global__ __launch_bounds__(256) void k_adi_xr(CtxArg<CB_> ca_) {
   typedef float T;
   const Ctx c = ctx_get(ca_);
-  XSETUP;
+  XSETUP_NT;
   const int comp = blockIdx.y;
   const Fields F = c.d;
   T* UH = (T*)(comp == 0 ? F.uh[0] : comp == 1 ? F.uh[1] : F.uh[2]);
@@ -1909,8 +1910,8 @@ __global__ __launch_bounds__(NT) void k_fft_xinv(CtxArg<CB_> ca_) {
 // written by the x-line update blocks that hold the rows J = 1 and J = N2M,
 // in the staging's pair layout: U(0) = W(0) = V(1) = V(N2) = 0, U(N2) and
 // W(N2) from BCOND's UBC, and their FP32 shadows in mixed mode.
-template <int SEG>
-__device__ __forceinline__ void upd_ghosts(const Ctx& c, const XStage<double, SEG>& X, int L, int n) {
+template <int SEG, bool NTL>
+__device__ __forceinline__ void upd_ghosts(const Ctx& c, const XStage<double, SEG, NTL>& X, int L, int n) {
   const Geo& g = c.g;
   const Fields& F = c.d;
   const unsigned nx = (unsigned)g.nx;
@@ -1956,7 +1957,7 @@ __device__ __forceinline__ void upd_ghosts(const Ctx& c, const XStage<double, SE
 template <typename T, int SEG, bool CB_>
 __global__ __launch_bounds__(256) void k_upd_x(CtxArg<CB_> ca_, int chk) {
   const Ctx c = ctx_get(ca_);
-  XSETUP;
+  XSETUP_NT;
   const Scal s = c.s;
   const Metrics M = c.m;
   const Fields F = c.d;
@@ -2039,7 +2040,7 @@ template <int SEG, bool CB_>
 __global__ __launch_bounds__(256) void k_upd_xu(CtxArg<CB_> ca_, int refine) {
   const Ctx c = ctx_get(ca_);
   typedef double T;
-  XSETUP;
+  XSETUP_NT;
   const Scal s = c.s;
   const Metrics M = c.m;
   const Fields F = c.d;
@@ -2126,24 +2127,24 @@ __global__ __launch_bounds__(512) void k_upd_z(CtxArg<CB_> ca_, int chk, int ref
   {
     // P += DP first, so that DP's segment is dead during the solve
     T dpr[SEG];
-    zload(F.dp, b, pl, p0, bm, bp, dpr, lf, rt);
+    zload_nt(F.dp, b, pl, p0, bm, bp, dpr, lf, rt);
     SQ d[q] = (T)24. * dx3 * (-prv(dpr, q, lf) + dpr[q]);
     if (ok) {
       SQ {
-        const double pn = F.p[ZI(q)] + dpr[q];
+        const double pn = ld_nt(&F.p[ZI(q)]) + dpr[q];
         st_nt(&F.p[ZI(q)], pn);
         if (MIX) st_nt(&F.pf[ZI(q)], (float)pn);
       }
     }
   }
   psolve<T, SEG>(d, G, L, l, gs, PZ.s22, bf.xf, bf.xb);
-  SQ wn[q] = ku * (T)((const TS*)F.uh[2])[ZI(q)] - dt * d[q] - dt * dmp3;
+  SQ wn[q] = ku * (T)ld_nt(&((const TS*)F.uh[2])[ZI(q)]) - dt * d[q] - dt * dmp3;
   // mixed mode: the flow rate of W before the update (FLOW3, :1168), summed
   // over the block with the maxima at the end
   double so = 0.0;
   if (MIX) {
     SQ {
-      const T wo = (T)F.w[ZI(q)];
+      const T wo = (T)ld_nt(&F.w[ZI(q)]);
       so += (double)wo;
       wn[q] = wo + wn[q];
     }
@@ -2176,7 +2177,7 @@ __global__ __launch_bounds__(512) void k_upd_z(CtxArg<CB_> ca_, int chk, int ref
         // (V(J+1) - V(J))/DY (the wall values V(1) = V(N2) = 0) + D1z W,
         // kept in FP32 for the next RDP
         const double vj = j >= 2 ? v0 : 0.0, vp = j + 1 <= g.n2m ? v1 : 0.0;
-        dv = ((vp - vj) * rdyj + F.rdp[a]) + (double)d[q];
+        dv = ((vp - vj) * rdyj + ld_nt(&F.rdp[a])) + (double)d[q];
         st_nt(&F.divf[a], (float)dv);
       } else {
         dv = F.div[a] + (double)d[q];
