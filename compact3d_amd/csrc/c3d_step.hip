@@ -284,6 +284,9 @@ template <typename V>
 __device__ __forceinline__ V vld(const typename YCol<V>::S* p, unsigned idx) {
   return *(const V*)(p + idx);
 }
+// nontemporal: the sweep's read-once streams (RUH, HH, HH2) and its last
+// reads of the step's state shadows and VV1/VV3 (measured: the z-sweep
+// after it 41 -> 35 us mixed)
 template <typename V>
 __device__ __forceinline__ V vld_nt(const typename YCol<V>::S* p, unsigned idx) {
   return __builtin_nontemporal_load((const V*)(p + idx));
@@ -539,9 +542,9 @@ __device__ __forceinline__ void yfuse_u(const Ctx& c, const T* tb, unsigned b, u
   const unsigned a0 = b + (unsigned)y.j0 * nx;  // row J0
   V us[CH + 2], w[CH + 1], ru[CH], hx[CH], hq[CH];
 #pragma unroll
-  for (int q = 0; q < CH + 2; q++) us[q] = (q < y.len + 2) ? vld<V>(su, a0 - nx + q * nx) : zero;
+  for (int q = 0; q < CH + 2; q++) us[q] = (q < y.len + 2) ? vld_nt<V>(su, a0 - nx + q * nx) : zero;
 #pragma unroll
-  for (int q = 0; q < CH + 1; q++) w[q] = (q < y.len + 1) ? vld<V>(vv1, a0 + q * nx) : zero;
+  for (int q = 0; q < CH + 1; q++) w[q] = (q < y.len + 1) ? vld_nt<V>(vv1, a0 + q * nx) : zero;
 #pragma unroll
   for (int q = 0; q < CH; q++) {
     const bool in = q < y.len;
@@ -680,9 +683,9 @@ __device__ __forceinline__ void yfuse_v(const Ctx& c, const T* tb, unsigned b, c
   const unsigned a0 = b + (unsigned)y.j0 * nx;
   V vs[CH + 2], ps[CH + 1], ru[CH], hx[CH], hq[CH];
 #pragma unroll
-  for (int q = 0; q < CH + 2; q++) vs[q] = (q < y.len + 2) ? vld<V>(sv, a0 - nx + q * nx) : zero;
+  for (int q = 0; q < CH + 2; q++) vs[q] = (q < y.len + 2) ? vld_nt<V>(sv, a0 - nx + q * nx) : zero;
 #pragma unroll
-  for (int q = 0; q < CH + 1; q++) ps[q] = (q < y.len + 1) ? vld<V>(sp, a0 - nx + q * nx) : zero;
+  for (int q = 0; q < CH + 1; q++) ps[q] = (q < y.len + 1) ? vld_nt<V>(sp, a0 - nx + q * nx) : zero;
 #pragma unroll
   for (int q = 0; q < CH; q++) {
     const bool in = q < y.len;
@@ -740,9 +743,9 @@ __device__ __forceinline__ void yfuse_w(const Ctx& c, const T* tb, unsigned b, u
   const unsigned a0 = b + (unsigned)y.j0 * nx;
   V ws[CH + 2], s3[CH + 1], ru[CH], hx[CH], hq[CH];
 #pragma unroll
-  for (int q = 0; q < CH + 2; q++) ws[q] = (q < y.len + 2) ? vld<V>(sw, a0 - nx + q * nx) : zero;
+  for (int q = 0; q < CH + 2; q++) ws[q] = (q < y.len + 2) ? vld_nt<V>(sw, a0 - nx + q * nx) : zero;
 #pragma unroll
-  for (int q = 0; q < CH + 1; q++) s3[q] = (q < y.len + 1) ? vld<V>(vv3, a0 + q * nx) : zero;
+  for (int q = 0; q < CH + 1; q++) s3[q] = (q < y.len + 1) ? vld_nt<V>(vv3, a0 + q * nx) : zero;
 #pragma unroll
   for (int q = 0; q < CH; q++) {
     const bool in = q < y.len;
@@ -1668,7 +1671,7 @@ __global__ __launch_bounds__(NT) void k_fft_z(CtxArg<CB_> ca_, int sign) {
 // Spectrum in TI, recurrences and result in TO (FP64 in both modes); chunks
 // of at most CH points.  Grid: sep + the tiles.
 template <typename TI, typename TO, int CH, int NT, bool CB_>
-__global__ __launch_bounds__(NT, 4) void k_pois_y(CtxArg<CB_> ca_, int sep) {
+__global__ __launch_bounds__(NT, CH <= 6 ? 6 : 4) void k_pois_y(CtxArg<CB_> ca_, int sep) {
   const Ctx c = ctx_get(ca_);
   const void* spec_in = (const void*)c.d.spec;
   void* spec_out = (void*)c.d.spec;
