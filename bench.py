@@ -219,7 +219,7 @@ def accuracy_leg(c3d, cfg, U, P, nsteps):
 
 
 def configs_leg(c3d, args):
-    """BASELINE's two smaller configurations on the GPU (FP64 and mixed) and
+    """The two smaller benchmark channels on the GPU (FP64 and mixed) and
     the oracle on the host cores: 64x65x64 (Re_tau 180, tanh 1.5, dt 1e-3,
     200 steps; the oracle runs all of them) and 192x129x192 (500 steps; the
     oracle a bounded sample).  The GPU steps run as c3d_step calls and again

@@ -1,7 +1,7 @@
 """Host-side check of the x-line tile layout the line kernels stage through
 (c3d_lines.h, XStage/xstage): for every candidate tile of the autotuner
 (c3d_api.cpp cand_lx, the rule xline_step) at the line lengths of the
-BASELINE grids and at 512, the staging's pair layout puts every point of
+benchmarked grids (48 to 256) and at 512, the staging's pair layout puts every point of
 every valid line of a tile in exactly one LDS slot, at slot p*(L+1) + l, its
 global offsets for rows J-1, J, J+1 stay inside the field, and every point a
 thread's segment reads (its SEG points and one neighbour each side) is
