@@ -31,6 +31,17 @@ def test_phases_match_oracle(grid):
     check_phases(config(n1, n2, n3), seed=0)
 
 
+def test_phases_match_oracle_pois_second_pass():
+    """GETDP with pair 0's kx = NX/2 modes solved in the y-solve's second
+    pass: at 256x33x384 the default Poisson tile (64 columns, 8 chunks) makes
+    4 x 192 = 768 tiles, on this GPU (256 CUs, three 512-thread blocks of the
+    6-point form per CU) exactly one round of resident blocks, so the launch
+    takes the second pass instead of the extra block that the other grids of
+    these tests use"""
+    from phases import check_phases
+    check_phases(config(257, 33, 385), seed=3)
+
+
 @pytest.mark.parametrize("precision", [0, 1])
 def test_fused_div_matches_divcheck_operator(precision):
     """the DIV and DIVMAX c3d_step forms inside its update kernels"""
