@@ -1284,7 +1284,10 @@ static std::vector<LShape> cand_lx(int n) {
     if (8 * (2 * (size_t)n * (L + 1) + 4 * (size_t)G * L) <= 65536) v.push_back(LShape{n, L, G, seg});
   return v;
 }
-static std::vector<LShape> cand_lz(int n, bool fp32) {
+// wide: the FP64 z tiles of mixed mode, where only k_upd_z runs them; its
+// 128 VGPRs also fit 1024-thread tiles (8-point segments, 32 or 64
+// columns: 256-byte FP64 and 128-byte FP32 rows)
+static std::vector<LShape> cand_lz(int n, bool fp32, bool wide = false) {
   std::vector<LShape> v;
   const int segs64[] = {8, 2}, segs32[] = {16, 8, 2};
   const int* segs = fp32 ? segs32 : segs64;
@@ -1295,7 +1298,7 @@ static std::vector<LShape> cand_lz(int n, bool fp32) {
     if (n % seg) continue;
     const int G = n / seg;
     for (int L : {16, 32, 64})
-      if (L * G <= 512) v.push_back(LShape{n, L, G, seg});
+      if (L * G <= 512 || (wide && !fp32 && seg == 8 && L * G <= 1024)) v.push_back(LShape{n, L, G, seg});
   }
   return v;
 }
@@ -1416,7 +1419,7 @@ extern "C" int c3d_autotune(const c3d_config* cfg, int steps, int* shapes, doubl
   else
     knob(cand_y(g.n2m, 16, 1024), [](Shapes& s, const YShape& y) { s.ys = y; });
   if (mixed) knob(cand_lz(g.nz, true), [](Shapes& s, const LShape& z) { s.lz32 = z; });
-  knob(cand_lz(g.nz, false), [](Shapes& s, const LShape& z) { s.lz = z; });
+  knob(cand_lz(g.nz, false, mixed), [](Shapes& s, const LShape& z) { s.lz = z; });
   knob(cand_lx(g.nx), [](Shapes& s, const LShape& x) { s.lx = x; });
   knob(cand_yp(g.n2m), [](Shapes& s, const YShape& y) { s.yp = y; });
   knob(cand_tl(g.nx, true), [](Shapes& s, int t) { s.tlx = t; });
@@ -1487,7 +1490,7 @@ extern "C" int c3d_autotune_set(const c3d_config* cfg, const int* v) {
     ok = false;
   };
   lsh(cand_lx(g.nx), s.lx, v[0], 0, s.lx);
-  lsh(cand_lz(g.nz, false), s.lz, v[1], 0, s.lz);
+  lsh(cand_lz(g.nz, false, cfg->precision == C3D_MIXED), s.lz, v[1], 0, s.lz);
   lsh(cand_lz(g.nz, true), s.lz32, v[2], v[3], s.lz32);
   ysh(cand_y(g.n2m, 16, 1024), s.ys, v[4], v[5], 1, s.ys);
   ysh(cand_y(g.n2m, 32, 1024, true), s.ys32, v[6], v[7], v[12], s.ys32);

@@ -2115,8 +2115,8 @@ __global__ __launch_bounds__(256) void k_upd_xu(CtxArg<CB_> ca_, int refine) {
 // (:1218-1230); with `chk`: DIV += D1z(U3), DIVMAX, and the CFL maximum of
 // the new field (:9112-9128, :9150-9162).  Mixed mode also stores the full
 // DIV (the next RDP starts from it) and the FP32 shadows of W and P.
-template <typename T, int SEG, typename TS, bool CB_>
-__global__ __launch_bounds__(512) void k_upd_z(CtxArg<CB_> ca_, int chk, int refine) {
+template <typename T, int SEG, typename TS, int NT, bool CB_>
+__global__ __launch_bounds__(NT) void k_upd_z(CtxArg<CB_> ca_, int chk, int refine) {
   const Ctx c = ctx_get(ca_);
   ZSETUP;
   constexpr bool MIX = sizeof(TS) == 4;
@@ -2196,7 +2196,7 @@ __global__ __launch_bounds__(512) void k_upd_z(CtxArg<CB_> ca_, int chk, int ref
   // doubles order like the values); k_max_fin reduces the slots.  Mixed
   // mode: the block's flow-rate sum in block_sum2's order (wavefront sums,
   // then the wavefronts in index order)
-  __shared__ double red[3][8];
+  __shared__ double red[3][NT / 64];
   dmax = wave_max(dmax);
   cmax = wave_max(cmax);
   if (MIX) so = wave_sum(so);
@@ -2676,7 +2676,19 @@ static void upd_x_fp64(const Ctx& c, bool chk) {
 template <typename TS>
 static void upd_z_t(const Ctx& c, bool chk, int refine) {
   const size_t lz = lds_line(c.o.lz, false, sizeof(double));
-  ZSEG(c.o.lz.seg, { KScope ks_(c, KID_UPD_Z); C3D_LAUNCH(k_upd_z, (double, S_, TS), dim3(zblocks(c)), zthreads(c), lz, c, chk ? 1 : 0, refine); });
+  // tiles of more than 512 threads (mixed mode only, 8-point segments: the
+  // candidate sets offer no other) run the NT = 1024 form
+  bool done = false;
+  if constexpr (sizeof(TS) == 4) {
+    if (c.o.lz.L * c.o.lz.G > 512) {
+      KScope ks_(c, KID_UPD_Z);
+      C3D_LAUNCH(k_upd_z, (double, 8, TS, 1024), dim3(zblocks(c)), zthreads(c), lz, c, chk ? 1 : 0, refine);
+      done = true;
+    }
+  }
+  if (!done) {
+    ZSEG(c.o.lz.seg, { KScope ks_(c, KID_UPD_Z); C3D_LAUNCH(k_upd_z, (double, S_, TS, 512), dim3(zblocks(c)), zthreads(c), lz, c, chk ? 1 : 0, refine); });
+  }
   if (chk) {
     KScope ks_(c, KID_FLOW_FIN);
     C3D_LAUNCH0(k_max_fin, dim3(1), dim3(FIN_THREADS), 0, c, xblocks(c), zblocks(c), c.s.mixed ? 1 : 0);
