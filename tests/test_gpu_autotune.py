@@ -135,3 +135,52 @@ def test_fp32_y_column_pairs_equal_single_columns(files, wmodel):
             assert np.array_equal(a, b)
     finally:
         c3d.autotune_reset()
+
+
+@pytest.mark.parametrize("files,wmodel", [(0.0, 0.0), (2.0, 1.0)])
+def test_mixed_wide_z_update_tiles_equal_narrow(files, wmodel):
+    """Mixed mode's z-line update with 32-column tiles of 1024 threads
+    (z_cols_fp64 = 32 at NZ = 256: 32 x 32 lines of 8-point segments) gives
+    the fields of the 16-column, 512-thread tiles.  Each point's arithmetic
+    is the same in both; only the per-block flow-rate partial sums change
+    their order, so after 3 steps the fields agree to 1e-12 relative and
+    DIVMAX stays <= 1e-9.  NX = 48 leaves half of the last 32-column tile of
+    an x row empty."""
+    import numpy as np
+    from compact3d_amd import c3d
+    from pyoracle import Oracle
+    n1, n2, n3 = 49, 33, 257
+    cfg = config(n1, n2, n3, cks=1.5, files=files, wmodel=wmodel)
+    try:
+        t = c3d.autotune(steps=1, **cfg, precision=c3d.MIXED)
+        orc = Oracle(**config(n1, n2, n3, cks=1.5))
+        U, P, HH = inputs(orc, seed=6)
+        # 32 x 32 = 1024 threads: a candidate of mixed mode only, where the
+        # FP64 z shape serves the update alone
+        s = dict(t)
+        s.update(z_cols_fp64=16)
+        c3d.autotune_set(s, **cfg, precision=c3d.FP64)
+        s.update(z_cols_fp64=32)
+        with pytest.raises(c3d.C3DError):
+            c3d.autotune_set(s, **cfg, precision=c3d.FP64)
+        out = {}
+        for cols in (16, 32):
+            s = dict(t)
+            s.update(z_cols_fp64=cols)
+            c3d.autotune_set(s, **cfg, precision=c3d.MIXED)
+            ch = c3d.Channel(**cfg, precision=c3d.MIXED)
+            ch.upload(U, P, HH)
+            dm = []
+            for n in range(3):
+                d, _ = ch.step(n + 1)
+                assert d <= 1e-9
+                dm.append(d)
+            st = ch.download()
+            out[cols] = (st["U"], st["P"], dm)
+            ch.close()
+        (u1, p1, d1), (u2, p2, d2) = out[16], out[32]
+        eu, ep = relerr(u1, u2), relerr(p1, p2)
+        print("32 vs 16 columns: U %.2e, P %.2e, DIVMAX %s / %s" % (eu, ep, d1, d2))
+        assert eu <= 1e-12 and ep <= 1e-12
+    finally:
+        c3d.autotune_reset()
