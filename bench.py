@@ -657,8 +657,9 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=16)
     ap.add_argument("--serve-policy", default="srpt", choices=["srpt", "fifo"],
                     help="serving loop policy: shortest remaining work first, or arrival order")
-    ap.add_argument("--serve-active", type=int, default=8,
-                    help="srpt: instances with a chunk queued at a time")
+    ap.add_argument("--serve-active", type=int, default=6,
+                    help="srpt: instances with a chunk queued at a time (6: the lowest p50 and mean "
+                         "latency of 4..12 at full throughput on the 24-request stream)")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched LES ensemble leg")
     ap.add_argument("--no-autotune", action="store_true", help="keep the rule-derived tile shapes")
     ap.add_argument("--shapes", metavar="JSON",
